@@ -1,0 +1,269 @@
+"""Python access to the MI355X encoder pipeline (libbc250_hip.so) and its
+host C library (libbc250_host.so), through the C ABI of include/gpu_compute.h
+and include/bc250_stages.h.
+
+The libraries are built in-tree (``make -C bc250-vaapi_amd``); importing this
+module without them raises, there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB = os.path.join(_HERE, "libbc250_hip.so")
+HOST_LIB = os.path.join(_HERE, "libbc250_host.so")
+
+GPU_IMAGE_NV12 = 0
+GPU_IMAGE_P010 = 1
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is not built: run `make -C {_HERE}` (the HIP extension is required; there is no CPU path)")
+    return C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+hip = _load(HIP_LIB)
+host = _load(HOST_LIB)
+
+
+class GpuImage(C.Structure):
+    """gpu_image_t (include/gpu_compute.h)."""
+    _fields_ = [("y_plane", C.c_void_p), ("uv_plane", C.c_void_p), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("coded_width", C.c_uint32), ("coded_height", C.c_uint32),
+                ("y_pitch", C.c_uint32), ("uv_pitch", C.c_uint32), ("current_layout", C.c_int),
+                ("format", C.c_int)]
+
+
+class GpuMemory(C.Structure):
+    _fields_ = [("memory", C.c_void_p), ("size", C.c_size_t), ("mapped_ptr", C.c_void_p)]
+
+
+class GpuContext(C.Structure):
+    _fields_ = [("hip", C.c_void_p), ("device", C.c_int), ("frames_submitted", C.c_uint32),
+                ("frame_width", C.c_uint32), ("frame_height", C.c_uint32)]
+
+
+class EncodeParams(C.Structure):
+    _fields_ = [("search_range", C.c_int), ("lambda8", C.c_int), ("deblock_idc", C.c_int),
+                ("me_8x8", C.c_int), ("readback", C.c_int)]
+
+
+class SeqCfg(C.Structure):
+    _fields_ = [("qp", C.c_int), ("gop", C.c_int), ("num_slices", C.c_int), ("pipelined", C.c_int)]
+
+
+MV_DTYPE = np.dtype([("mvx", "<i4"), ("mvy", "<i4"), ("sad", "<u4"), ("pad", "<u4")])
+
+_P = C.c_void_p
+_I = C.c_int
+_CTX = C.POINTER(GpuContext)
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_sig(hip, "bc250_gpu_init", _I, [_CTX])
+_sig(hip, "bc250_gpu_destroy", None, [_CTX])
+_sig(hip, "gpu_compute_create_image", _I, [_CTX, _I, _I, _I, C.POINTER(GpuImage), C.POINTER(GpuMemory)])
+_sig(hip, "gpu_compute_destroy_image", None, [_CTX, GpuImage, GpuMemory])
+_sig(hip, "gpu_compute_upload_nv12", _I, [_CTX, C.POINTER(GpuImage), GpuMemory, _P, _I, _P, _I, _I, _I])
+_sig(hip, "gpu_compute_download_nv12", _I, [_CTX, C.POINTER(GpuImage), GpuMemory, _P, _I, _P, _I, _I, _I])
+_sig(hip, "gpu_compute_begin_picture", _I, [_CTX, GpuImage])
+_sig(hip, "gpu_compute_dispatch_encode_ext", _I, [_CTX, GpuImage, _I, _I, _I, _I, _I, _I, _P])
+_sig(hip, "gpu_compute_dispatch_me_only", _I, [_CTX, GpuImage, _I, _I])
+_sig(hip, "gpu_compute_end_picture", _I, [_CTX])
+_sig(hip, "gpu_compute_submitted_slot", _I, [_CTX])
+_sig(hip, "gpu_compute_sync_slot", _I, [_CTX, _I])
+for _n in ("quant", "dc", "pred_mode", "mv", "nz", "mv8"):
+    _sig(hip, f"gpu_compute_get_{_n}_staging_data_slot", _I, [_CTX, _I, C.POINTER(_P), C.POINTER(C.c_size_t)])
+_sig(hip, "gpu_compute_set_encode_params", _I, [_CTX, C.POINTER(EncodeParams)])
+_sig(hip, "gpu_compute_download_recon", _I, [_CTX, _P, _I, _P, _I])
+_sig(hip, "gpu_compute_get_stage_times", _I, [_CTX, _I, C.POINTER(C.c_float)])
+_sig(hip, "gpu_compute_take_error_flags", _I, [_CTX])
+_sig(hip, "gpu_compute_get_last_latency_ms", C.c_double, [_CTX])
+
+_U = C.c_uint
+_sig(hip, "bc250_launch_me_fullsearch", _I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P, _P])
+_sig(hip, "bc250_launch_me_refine", _I, [_P, _I, _P, _I, _I, _I, _P, _P, _P, _P, _I, _P])
+_sig(hip, "bc250_launch_p_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P,
+                                       _P, _P, _P])
+_sig(hip, "bc250_launch_i_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _U,
+                                       _P])
+_sig(hip, "bc250_launch_deblock", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _U, _P])
+_sig(hip, "bc250_launch_zero_i32", _I, [_P, _I, _P])
+_sig(hip, "bc250_launch_pad_edges", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P])
+
+_sig(host, "bc250_gen_create", _P, [C.c_uint64, _I, _I, _I])
+_sig(host, "bc250_gen_destroy", None, [_P])
+_sig(host, "bc250_gen_frame", _I, [_P, _I, _P, _I, _P, _I])
+_sig(host, "bc250_gen_offset_qpel", None, [_P, _I, C.POINTER(_I), C.POINTER(_I)])
+_sig(host, "bc250_encode_sequence", _I, [_CTX, C.POINTER(GpuImage), _I, _I, _I, C.POINTER(SeqCfg),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double)])
+_sig(host, "h264_intra16_luma_dc_transform", None, [_P, _I, _P, _P])
+_sig(host, "h264_chroma_dc_levels", None, [_P, _I, _P])
+_sig(host, "h264_frame_dc_levels", None, [_P, _I, _I, _I, _P, _P])
+_sig(host, "h264_chroma_qp", _I, [_I])
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------- frames
+
+class FrameGen:
+    """Seeded synthetic NV12 frames (host/framegen.c)."""
+
+    def __init__(self, seed, width, height, nrect):
+        self.w, self.h = width, height
+        self._g = host.bc250_gen_create(seed, width, height, nrect)
+        if not self._g:
+            raise ValueError("bad generator arguments")
+
+    def frame(self, f):
+        y = np.empty((self.h, self.w), np.uint8)
+        uv = np.empty(((self.h + 1) // 2, 2 * ((self.w + 1) // 2)), np.uint8)
+        host.bc250_gen_frame(self._g, f, _ptr(y), y.strides[0], _ptr(uv), uv.strides[0])
+        return y, uv
+
+    def offset_qpel(self, f):
+        ox, oy = C.c_int(), C.c_int()
+        host.bc250_gen_offset_qpel(self._g, f, C.byref(ox), C.byref(oy))
+        return ox.value, oy.value
+
+    def close(self):
+        if self._g:
+            host.bc250_gen_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        self.close()
+
+
+def pad_coded(y, uv):
+    """Visible NV12 planes -> coded-size planes (16-aligned) with the last
+    row/column replicated, which is what an upload leaves on the device."""
+    h, w = y.shape
+    wc, hc = (w + 15) // 16 * 16, (h + 15) // 16 * 16
+    yc = np.pad(y, ((0, hc - h), (0, wc - w)), mode="edge")
+    u = uv[:, 0::2]
+    v = uv[:, 1::2]
+    u = np.pad(u, ((0, hc // 2 - u.shape[0]), (0, wc // 2 - u.shape[1])), mode="edge")
+    v = np.pad(v, ((0, hc // 2 - v.shape[0]), (0, wc // 2 - v.shape[1])), mode="edge")
+    uvc = np.empty((hc // 2, wc), np.uint8)
+    uvc[:, 0::2] = u
+    uvc[:, 1::2] = v
+    return np.ascontiguousarray(yc), uvc
+
+
+# ---------------------------------------------------------------- encoder
+
+class Encoder:
+    """One gpu_compute context: upload frames, encode, read staging back."""
+
+    def __init__(self, width, height, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1):
+        self.w, self.h = width, height
+        self.wc, self.hc = (width + 15) // 16 * 16, (height + 15) // 16 * 16
+        self.nmb = (self.wc // 16) * (self.hc // 16)
+        self.ctx = GpuContext()
+        if hip.bc250_gpu_init(C.byref(self.ctx)) != 0:
+            raise RuntimeError("bc250_gpu_init failed (no HIP device?)")
+        self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback)
+        self.images = []
+
+    def set_params(self, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1):
+        p = EncodeParams(search_range, lambda8, deblock_idc, me_8x8, readback)
+        if hip.gpu_compute_set_encode_params(C.byref(self.ctx), C.byref(p)) != 0:
+            raise ValueError("invalid encode parameters")
+
+    def new_image(self):
+        img, mem = GpuImage(), GpuMemory()
+        if hip.gpu_compute_create_image(C.byref(self.ctx), self.w, self.h, GPU_IMAGE_NV12, C.byref(img),
+                                        C.byref(mem)) != 0:
+            raise RuntimeError("create_image failed")
+        self.images.append((img, mem))
+        return img, mem
+
+    def upload(self, img_mem, y, uv):
+        img, mem = img_mem
+        y = np.ascontiguousarray(y)
+        uv = np.ascontiguousarray(uv)
+        if hip.gpu_compute_upload_nv12(C.byref(self.ctx), C.byref(img), mem, _ptr(y), y.strides[0], _ptr(uv),
+                                       uv.strides[0], self.w, self.h) != 0:
+            raise RuntimeError("upload failed")
+
+    def encode(self, img_mem, qp, is_intra, num_slices=1, cpu_mvs=None):
+        """Submit one frame and wait for it; returns the slot."""
+        img, _ = img_mem
+        c = C.byref(self.ctx)
+        if hip.gpu_compute_begin_picture(c, img) != 0:
+            raise RuntimeError("begin_picture failed")
+        mvp = None
+        if cpu_mvs is not None:
+            cpu_mvs = np.ascontiguousarray(cpu_mvs, dtype=MV_DTYPE)
+            mvp = _ptr(cpu_mvs)
+        if hip.gpu_compute_dispatch_encode_ext(c, img, self.w, self.h, qp, int(is_intra), num_slices, 0, mvp) != 0:
+            raise RuntimeError("dispatch_encode_ext failed")
+        if hip.gpu_compute_end_picture(c) != 0:
+            raise RuntimeError("end_picture failed")
+        slot = hip.gpu_compute_submitted_slot(c)
+        if hip.gpu_compute_sync_slot(c, slot) != 0:
+            raise RuntimeError("sync_slot failed (error flags %d)" % hip.gpu_compute_take_error_flags(c))
+        return slot
+
+    def staging(self, slot, name):
+        d, n = C.c_void_p(), C.c_size_t()
+        f = getattr(hip, f"gpu_compute_get_{name}_staging_data_slot")
+        if f(C.byref(self.ctx), slot, C.byref(d), C.byref(n)) != 0:
+            raise RuntimeError(f"no {name} staging data in slot {slot}")
+        dt = {"quant": np.int16, "dc": np.int32, "pred_mode": np.uint32, "nz": np.uint32, "mv": MV_DTYPE,
+              "mv8": MV_DTYPE}[name]
+        buf = (C.c_uint8 * n.value).from_address(d.value)
+        return np.frombuffer(buf, dtype=dt).copy()
+
+    def results(self, slot):
+        return {k: self.staging(slot, k) for k in ("quant", "dc", "pred_mode", "mv", "nz", "mv8")}
+
+    def recon(self):
+        y = np.empty((self.h, self.w), np.uint8)
+        uv = np.empty(((self.h + 1) // 2, 2 * ((self.w + 1) // 2)), np.uint8)
+        if hip.gpu_compute_download_recon(C.byref(self.ctx), _ptr(y), y.strides[0], _ptr(uv), uv.strides[0]) != 0:
+            raise RuntimeError("download_recon failed")
+        return y, uv
+
+    def stage_times(self, slot):
+        ms = (C.c_float * 5)()
+        if hip.gpu_compute_get_stage_times(C.byref(self.ctx), slot, ms) != 0:
+            raise RuntimeError("stage times unavailable")
+        return list(ms)
+
+    def encode_sequence(self, images, qp, gop=0, num_slices=1, pipelined=1):
+        n = len(images)
+        arr = (GpuImage * n)(*[im for im, _ in images])
+        cfg = SeqCfg(qp, gop, num_slices, pipelined)
+        fms = (C.c_double * n)()
+        tot = C.c_double()
+        rc = host.bc250_encode_sequence(C.byref(self.ctx), arr, n, self.w, self.h, C.byref(cfg), fms, C.byref(tot))
+        if rc != 0:
+            raise RuntimeError("encode_sequence failed")
+        return np.array(fms[:], dtype=np.float64), tot.value
+
+    def close(self):
+        if self.ctx.hip:
+            for img, mem in self.images:
+                hip.gpu_compute_destroy_image(C.byref(self.ctx), img, mem)
+            self.images = []
+            hip.bc250_gpu_destroy(C.byref(self.ctx))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,708 @@
+// gpu_compute.hip - HIP implementation of the gpu_compute_* C ABI
+// (include/gpu_compute.h), the drop-in for the reference's Vulkan
+// orchestration (approach1-compute-encoder/src/gpu_compute.c).
+//
+// One context = one HIP stream (default priority) + per-resolution device
+// buffers + two staging slots of pinned host memory.  dispatch_encode_ext
+// enqueues the stages of a frame on the stream; end_picture "submits" the
+// slot (records its completion event) and toggles the slot, as the
+// reference's vkQueueSubmit + slot flip (gpu_compute.c:3523-3619);
+// sync_slot waits on that event (vkWaitForFences, gpu_compute.c:3640-3712).
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "bc250_dev.h"
+#include "bc250_stages.h"
+#include "gpu_compute.h"
+
+#define HIP_OK(x) ((x) == hipSuccess)
+
+namespace {
+
+constexpr int NSLOT = 2;
+constexpr int NEV = 5;  // start, after ME, after encode, after deblock, after copies
+constexpr unsigned kSpinLimit = 1u << 22;
+
+struct Plane2 {
+    uint8_t *y = nullptr, *uv = nullptr;
+    int py = 0, puv = 0;
+    void *base = nullptr;
+};
+
+struct Slot {
+    int16_t *levels = nullptr;
+    int32_t *dc = nullptr;
+    uint32_t *modes = nullptr, *nz = nullptr;
+    gpu_mv_t *mvs = nullptr, *mv8 = nullptr;
+    int *err = nullptr;  // 2 ints: intra wait error, deblock wait error
+    hipEvent_t ev[NEV] = {};
+    bool pending = false, recorded = false, have_data = false, was_intra = false;
+};
+
+}  // namespace
+
+struct bc250_hip_state {
+    hipStream_t stream = nullptr;
+    // resolution-dependent state
+    int w = 0, h = 0, wc = 0, hc = 0, mbw = 0, mbh = 0, nmb = 0;
+    Plane2 recon[2];
+    int ref_idx = 0;
+    bool has_recon = false;
+    int16_t *levels = nullptr;
+    int32_t *dc = nullptr;
+    uint32_t *nz = nullptr, *modes = nullptr;
+    uint32_t *mvs = nullptr, *mv8 = nullptr, *imv16 = nullptr, *imv8 = nullptr;
+    int *sync_i = nullptr, *sync_d = nullptr;
+    Slot slot[NSLOT];
+    int cur = 0, submitted = 0;
+    double last_latency_ms = 0.0;
+    gpu_encode_params_t params{64, 1, 0, 1, 1};
+    int err_flags = 0;
+    // map_surface mirror
+    void *map_host = nullptr;
+    size_t map_size = 0;
+    gpu_image_t map_image{};
+};
+
+static int align_up(int v, int a) { return (v + a - 1) / a * a; }
+
+// ------------------------------------------------------------ tiny kernels
+
+__global__ void zero_i32_kernel(int *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+extern "C" int bc250_launch_zero_i32(int *p, int n, void *stream)
+{
+    if (n <= 0) return 0;
+    zero_i32_kernel<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(p, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// One thread per coded-area byte outside the visible region of a plane.
+__global__ void pad_plane_kernel(uint8_t *p, int pitch, int w, int h, int wc, int hc, int bpp)
+{
+    const int xb = blockIdx.x * blockDim.x + threadIdx.x;  // byte column
+    const int y = blockIdx.y;
+    if (xb >= wc || y >= hc) return;
+    if (xb < w && y < h) return;
+    const int sx = xb < w ? xb : (w - bpp) + (xb % bpp);
+    const int sy = y < h ? y : h - 1;
+    p[(size_t)y * pitch + xb] = p[(size_t)sy * pitch + sx];
+}
+
+extern "C" int bc250_launch_pad_edges(uint8_t *y, uint8_t *uv, int py, int puv, int w, int h, int wc, int hc,
+                                      void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (w == wc && h == hc) return 0;
+    // columns first within visible rows, then whole rows: a single pass reads
+    // a source that is either visible or itself padded on an earlier row, so
+    // run rows in two launches: visible rows, then the rows below.
+    dim3 b(256);
+    const int cwb = 2 * ((w + 1) / 2), ch = (h + 1) / 2;  // visible chroma bytes per row / rows
+    if (w < wc) {
+        pad_plane_kernel<<<dim3((wc + 255) / 256, h), b, 0, st>>>(y, py, w, h, wc, h, 1);
+        if (cwb < wc) pad_plane_kernel<<<dim3((wc + 255) / 256, ch), b, 0, st>>>(uv, puv, cwb, ch, wc, ch, 2);
+    }
+    if (h < hc) {
+        pad_plane_kernel<<<dim3((wc + 255) / 256, hc), b, 0, st>>>(y, py, wc, h, wc, hc, 1);
+        if (ch < hc / 2)
+            pad_plane_kernel<<<dim3((wc + 255) / 256, hc / 2), b, 0, st>>>(uv, puv, wc, ch, wc, hc / 2, 2);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ------------------------------------------------------------ buffers
+
+static void free_res(bc250_hip_state *s)
+{
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (auto &r : s->recon) {
+        if (r.base) hipFree(r.base);
+        r = Plane2{};
+    }
+    void *dev[] = {s->levels, s->dc, s->nz, s->modes, s->mvs, s->mv8, s->imv16, s->imv8, s->sync_i, s->sync_d};
+    for (void *p : dev)
+        if (p) hipFree(p);
+    s->levels = nullptr; s->dc = nullptr; s->nz = nullptr; s->modes = nullptr; s->mvs = nullptr;
+    s->mv8 = nullptr; s->imv16 = nullptr; s->imv8 = nullptr; s->sync_i = nullptr; s->sync_d = nullptr;
+    for (auto &sl : s->slot) {
+        void *host[] = {sl.levels, sl.dc, sl.modes, sl.nz, sl.mvs, sl.mv8, sl.err};
+        for (void *p : host)
+            if (p) hipHostFree(p);
+        sl.levels = nullptr; sl.dc = nullptr; sl.modes = nullptr; sl.nz = nullptr; sl.mvs = nullptr;
+        sl.mv8 = nullptr; sl.err = nullptr;
+        sl.pending = sl.have_data = false;
+    }
+    s->w = s->h = s->wc = s->hc = s->mbw = s->mbh = s->nmb = 0;
+    s->has_recon = false;
+}
+
+static int alloc_plane2(Plane2 *p, int wc, int hc)
+{
+    p->py = align_up(wc, 256);
+    p->puv = p->py;
+    size_t ysz = (size_t)p->py * hc, uvsz = (size_t)p->puv * (hc / 2);
+    if (!HIP_OK(hipMalloc(&p->base, ysz + uvsz))) return -1;
+    p->y = (uint8_t *)p->base;
+    p->uv = (uint8_t *)p->base + ysz;
+    return 0;
+}
+
+// Lazily (re)allocates everything for a resolution, like
+// allocate_encoding_buffers() at gpu_compute.c:233-532.
+static int ensure_res(bc250_hip_state *s, int w, int h)
+{
+    if (s->w == w && s->h == h && s->levels) return 0;
+    free_res(s);
+    const int wc = align_up(w, 16), hc = align_up(h, 16);
+    const int mbw = wc / 16, mbh = hc / 16, nmb = mbw * mbh;
+    bool ok = alloc_plane2(&s->recon[0], wc, hc) == 0 && alloc_plane2(&s->recon[1], wc, hc) == 0;
+    ok = ok && HIP_OK(hipMalloc(&s->levels, (size_t)nmb * 384 * 2)) && HIP_OK(hipMalloc(&s->dc, (size_t)nmb * 24 * 4)) &&
+         HIP_OK(hipMalloc(&s->nz, (size_t)nmb * 24 * 4)) && HIP_OK(hipMalloc(&s->modes, (size_t)nmb * 4)) &&
+         HIP_OK(hipMalloc(&s->mvs, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->mv8, (size_t)nmb * 64)) &&
+         HIP_OK(hipMalloc(&s->imv16, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->imv8, (size_t)nmb * 64)) &&
+         HIP_OK(hipMalloc(&s->sync_i, (size_t)(mbh + 2) * 4)) && HIP_OK(hipMalloc(&s->sync_d, (size_t)(mbh + 2) * 4));
+    for (auto &sl : s->slot) {
+        ok = ok && HIP_OK(hipHostMalloc((void **)&sl.levels, (size_t)nmb * 384 * 2, hipHostMallocDefault)) &&
+             HIP_OK(hipHostMalloc((void **)&sl.dc, (size_t)nmb * 24 * 4, hipHostMallocDefault)) &&
+             HIP_OK(hipHostMalloc((void **)&sl.modes, (size_t)nmb * 4, hipHostMallocDefault)) &&
+             HIP_OK(hipHostMalloc((void **)&sl.nz, (size_t)nmb * 24 * 4, hipHostMallocDefault)) &&
+             HIP_OK(hipHostMalloc((void **)&sl.mvs, (size_t)nmb * 16, hipHostMallocDefault)) &&
+             HIP_OK(hipHostMalloc((void **)&sl.mv8, (size_t)nmb * 64, hipHostMallocDefault)) &&
+             HIP_OK(hipHostMalloc((void **)&sl.err, 2 * sizeof(int), hipHostMallocDefault));
+    }
+    if (!ok) {
+        fprintf(stderr, "[bc250-hip] buffer allocation failed for %dx%d\n", w, h);
+        free_res(s);
+        return -1;
+    }
+    s->w = w; s->h = h; s->wc = wc; s->hc = hc; s->mbw = mbw; s->mbh = mbh; s->nmb = nmb;
+    s->has_recon = false;
+    s->ref_idx = 0;
+    return 0;
+}
+
+static bc250_hip_state *st_of(gpu_context_t *ctx) { return ctx ? ctx->hip : nullptr; }
+
+// ------------------------------------------------------------ lifecycle
+
+extern "C" int bc250_gpu_init(bc250_gpu_context_t *ctx)
+{
+    if (!ctx) return -1;
+    if (ctx->hip) return 0;
+    int n = 0;
+    if (!HIP_OK(hipGetDeviceCount(&n)) || n < 1) {
+        fprintf(stderr, "[bc250-hip] no HIP device\n");
+        return -1;
+    }
+    if (!HIP_OK(hipSetDevice(ctx->device))) return -1;
+    bc250_hip_state *s = new bc250_hip_state();
+    if (!HIP_OK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking))) {
+        delete s;
+        return -1;
+    }
+    for (auto &sl : s->slot)
+        for (auto &e : sl.ev)
+            if (!HIP_OK(hipEventCreate(&e))) {
+                hipStreamDestroy(s->stream);
+                delete s;
+                return -1;
+            }
+    ctx->hip = s;
+    ctx->frames_submitted = 0;
+    return 0;
+}
+
+extern "C" void bc250_gpu_destroy(bc250_gpu_context_t *ctx)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s) return;
+    free_res(s);
+    if (s->map_host) hipHostFree(s->map_host);
+    for (auto &sl : s->slot)
+        for (auto &e : sl.ev)
+            if (e) hipEventDestroy(e);
+    hipStreamDestroy(s->stream);
+    delete s;
+    ctx->hip = nullptr;
+}
+
+extern "C" void gpu_compute_report_cu_topology(bc250_gpu_context_t *ctx)
+{
+    hipDeviceProp_t p;
+    if (!ctx || !HIP_OK(hipGetDeviceProperties(&p, ctx->device))) return;
+    fprintf(stderr, "[bc250-hip] %s (%s): %d CUs, %d KB LDS/block, %.1f GB HBM\n", p.name, p.gcnArchName,
+            p.multiProcessorCount, (int)(p.sharedMemPerBlock / 1024), p.totalGlobalMem / 1073741824.0);
+}
+
+extern "C" int gpu_compute_init(gpu_context_t *ctx) { return bc250_gpu_init(ctx); }
+extern "C" void gpu_compute_terminate(gpu_context_t *ctx) { bc250_gpu_destroy(ctx); }
+
+// ------------------------------------------------------------ surfaces
+
+extern "C" int gpu_compute_create_image(gpu_context_t *ctx, int width, int height, int format, gpu_image_t *image,
+                                        gpu_memory_t *memory)
+{
+    if (!ctx || !ctx->hip || !image || !memory || width <= 0 || height <= 0) return -1;
+    const int bpp = format == GPU_IMAGE_P010 ? 2 : 1;
+    const int wc = align_up(width, 16), hc = align_up(height, 16);
+    const int pitch = align_up(wc * bpp, 256);
+    const size_t ysz = (size_t)pitch * hc, total = ysz + (size_t)pitch * (hc / 2);
+    void *base = nullptr;
+    if (!HIP_OK(hipMalloc(&base, total))) return -1;
+    memset(image, 0, sizeof(*image));
+    image->y_plane = base;
+    image->uv_plane = (uint8_t *)base + ysz;
+    image->width = (uint32_t)width;
+    image->height = (uint32_t)height;
+    image->coded_width = (uint32_t)wc;
+    image->coded_height = (uint32_t)hc;
+    image->y_pitch = image->uv_pitch = (uint32_t)pitch;
+    image->format = format;
+    memory->memory = base;
+    memory->size = total;
+    memory->mapped_ptr = nullptr;
+    return 0;
+}
+
+extern "C" void gpu_compute_destroy_image(gpu_context_t *ctx, gpu_image_t image, gpu_memory_t memory)
+{
+    (void)image;
+    if (ctx && ctx->hip) hipStreamSynchronize(ctx->hip->stream);
+    if (memory.memory) hipFree(memory.memory);
+}
+
+extern "C" int gpu_compute_get_nv12_layout(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory,
+                                           gpu_nv12_layout_t *layout)
+{
+    if (!ctx || !image || !layout || !memory.memory) return -1;
+    layout->y_pitch = image->y_pitch;
+    layout->y_offset = 0;
+    layout->uv_pitch = image->uv_pitch;
+    layout->uv_offset = (uint64_t)((uint8_t *)image->uv_plane - (uint8_t *)image->y_plane);
+    layout->total_size = memory.size;
+    return 0;
+}
+
+extern "C" uint8_t *gpu_compute_map_surface(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory,
+                                            gpu_nv12_layout_t *layout, bool *unmap)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !image || !memory.memory) return nullptr;
+    if (s->map_size < memory.size) {
+        if (s->map_host) hipHostFree(s->map_host);
+        s->map_host = nullptr;
+        s->map_size = 0;
+        if (!HIP_OK(hipHostMalloc(&s->map_host, memory.size, hipHostMallocDefault))) return nullptr;
+        s->map_size = memory.size;
+    }
+    if (!HIP_OK(hipMemcpyAsync(s->map_host, memory.memory, memory.size, hipMemcpyDeviceToHost, s->stream)) ||
+        !HIP_OK(hipStreamSynchronize(s->stream)))
+        return nullptr;
+    s->map_image = *image;
+    if (layout) gpu_compute_get_nv12_layout(ctx, image, memory, layout);
+    if (unmap) *unmap = true;
+    return (uint8_t *)s->map_host;
+}
+
+extern "C" void gpu_compute_unmap_surface(gpu_context_t *ctx, gpu_memory_t memory, bool unmap)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !unmap || !s->map_host || !memory.memory) return;
+    hipMemcpyAsync(memory.memory, s->map_host, memory.size, hipMemcpyHostToDevice, s->stream);
+    const gpu_image_t &im = s->map_image;
+    bc250_launch_pad_edges((uint8_t *)im.y_plane, (uint8_t *)im.uv_plane, im.y_pitch, im.uv_pitch, im.width,
+                           im.height, im.coded_width, im.coded_height, s->stream);
+    hipStreamSynchronize(s->stream);
+}
+
+extern "C" int gpu_compute_upload_nv12(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory,
+                                       const uint8_t *y_plane, int y_pitch, const uint8_t *uv_plane, int uv_pitch,
+                                       int width, int height)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !image || !memory.memory || !y_plane || !uv_plane) return -1;
+    const int bpp = image->format == GPU_IMAGE_P010 ? 2 : 1;
+    if (width <= 0 || height <= 0 || (uint32_t)width > image->width || (uint32_t)height > image->height) return -1;
+    bool ok = HIP_OK(hipMemcpy2DAsync(image->y_plane, image->y_pitch, y_plane, y_pitch, (size_t)width * bpp, height,
+                                      hipMemcpyHostToDevice, s->stream)) &&
+              HIP_OK(hipMemcpy2DAsync(image->uv_plane, image->uv_pitch, uv_plane, uv_pitch,
+                                      (size_t)((width + 1) / 2) * 2 * bpp, (height + 1) / 2, hipMemcpyHostToDevice,
+                                      s->stream));
+    if (ok && bpp == 1)
+        ok = bc250_launch_pad_edges((uint8_t *)image->y_plane, (uint8_t *)image->uv_plane, image->y_pitch,
+                                    image->uv_pitch, width, height, image->coded_width, image->coded_height,
+                                    s->stream) == 0;
+    ok = ok && HIP_OK(hipStreamSynchronize(s->stream));
+    return ok ? 0 : -1;
+}
+
+// P010 keeps 10-bit samples in the high bits of 16-bit words (reference
+// gpu_compute.h:409-420); the shift happens on the host copy.
+extern "C" int gpu_compute_upload_p010(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory,
+                                       const uint16_t *y_plane, int y_pitch, const uint16_t *uv_plane, int uv_pitch,
+                                       int width, int height)
+{
+    if (!image || image->format != GPU_IMAGE_P010 || !y_plane || !uv_plane || width <= 0 || height <= 0) return -1;
+    const int cw = (width + 1) / 2, ch = (height + 1) / 2;
+    uint16_t *ty = (uint16_t *)malloc((size_t)width * height * 2);
+    uint16_t *tuv = (uint16_t *)malloc((size_t)cw * 2 * ch * 2);
+    if (!ty || !tuv) {
+        free(ty);
+        free(tuv);
+        return -1;
+    }
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++)
+            ty[(size_t)y * width + x] = (uint16_t)(((const uint16_t *)((const uint8_t *)y_plane + (size_t)y * y_pitch))[x] << 6);
+    for (int y = 0; y < ch; y++)
+        for (int x = 0; x < 2 * cw; x++)
+            tuv[(size_t)y * 2 * cw + x] = (uint16_t)(((const uint16_t *)((const uint8_t *)uv_plane + (size_t)y * uv_pitch))[x] << 6);
+    int r = gpu_compute_upload_nv12(ctx, image, memory, (const uint8_t *)ty, width * 2, (const uint8_t *)tuv, cw * 4,
+                                    width, height);
+    free(ty);
+    free(tuv);
+    return r;
+}
+
+extern "C" int gpu_compute_video_proc(gpu_context_t *, gpu_image_t *, const int *, gpu_image_t *, const int *)
+{
+    return -1;  // VideoProc entrypoint: not part of the encoder compute path
+}
+
+extern "C" void gpu_compute_copy_from_wc(void *dst, const void *src, size_t n) { memcpy(dst, src, n); }
+
+extern "C" int gpu_compute_download_nv12(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory,
+                                         uint8_t *y_plane, int y_pitch, uint8_t *uv_plane, int uv_pitch, int width,
+                                         int height)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !image || !memory.memory || !y_plane || !uv_plane || width <= 0 || height <= 0) return -1;
+    const int bpp = image->format == GPU_IMAGE_P010 ? 2 : 1;
+    bool ok = HIP_OK(hipMemcpy2DAsync(y_plane, y_pitch, image->y_plane, image->y_pitch, (size_t)width * bpp, height,
+                                      hipMemcpyDeviceToHost, s->stream)) &&
+              HIP_OK(hipMemcpy2DAsync(uv_plane, uv_pitch, image->uv_plane, image->uv_pitch,
+                                      (size_t)((width + 1) / 2) * 2 * bpp, (height + 1) / 2, hipMemcpyDeviceToHost,
+                                      s->stream)) &&
+              HIP_OK(hipStreamSynchronize(s->stream));
+    return ok ? 0 : -1;
+}
+
+extern "C" int gpu_compute_export_nv12_dmabuf(gpu_context_t *, gpu_memory_t, int *) { return -1; }
+extern "C" int gpu_compute_import_dmabuf_image(gpu_context_t *, int, int, int, int, uint32_t, uint32_t,
+                                               gpu_image_t *, gpu_memory_t *)
+{
+    return -1;
+}
+// No foreign writers to order against: callers proceed on the -1, exactly as
+// with the reference when the capability is missing.
+extern "C" int gpu_compute_wait_for_image_ready(gpu_context_t *, gpu_memory_t) { return -1; }
+extern "C" int gpu_compute_wait_for_image_ready_host(gpu_context_t *, gpu_memory_t) { return -1; }
+
+extern "C" int gpu_compute_cpu_read_safe(gpu_context_t *ctx)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s) return 0;
+    return hipStreamQuery(s->stream) == hipSuccess ? 1 : 0;
+}
+
+// ------------------------------------------------------------ encode
+
+extern "C" int gpu_compute_begin_picture(gpu_context_t *ctx, gpu_image_t render_target)
+{
+    (void)render_target;
+    bc250_hip_state *s = st_of(ctx);
+    if (!s) return -1;
+    Slot &sl = s->slot[s->cur];
+    if (sl.pending) {
+        // the slot is about to be reused: its previous frame must be done
+        if (!HIP_OK(hipEventSynchronize(sl.ev[NEV - 1]))) return -1;
+        sl.pending = false;
+    }
+    return 0;
+}
+
+static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &ref)
+{
+    const gpu_encode_params_t &p = s->params;
+    int r = bc250_launch_me_fullsearch((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc,
+                                       p.search_range, p.lambda8, s->imv16, s->imv8, s->stream);
+    if (r == 0)
+        r = bc250_launch_me_refine((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc, s->imv16,
+                                   s->imv8, s->mvs, s->mv8, p.me_8x8, s->stream);
+    return r;
+}
+
+extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t rt, int width, int height, int qp,
+                                               int is_intra, int num_slices, int me_mode, const gpu_mv_t *cpu_mvs)
+{
+    (void)me_mode;
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !rt.y_plane || qp < 0 || qp > 51) return -1;
+    if (num_slices < 1) num_slices = 1;
+    if (rt.format != GPU_IMAGE_NV12) return -1;  // 8-bit pipeline
+    if (ensure_res(s, width, height) != 0) return -1;
+    if (rt.coded_width != (uint32_t)s->wc || rt.coded_height != (uint32_t)s->hc) return -1;
+    Slot &sl = s->slot[s->cur];
+    hipStream_t st = s->stream;
+    const gpu_encode_params_t &p = s->params;
+
+    Plane2 cur_in;
+    cur_in.y = (uint8_t *)rt.y_plane;
+    cur_in.uv = (uint8_t *)rt.uv_plane;
+    cur_in.py = rt.y_pitch;
+    cur_in.puv = rt.uv_pitch;
+    const Plane2 &ref = s->has_recon ? s->recon[s->ref_idx] : cur_in;
+    const int out_idx = s->has_recon ? (s->ref_idx ^ 1) : 0;
+    Plane2 &out = s->recon[out_idx];
+
+    int r = 0;
+    hipEventRecord(sl.ev[0], st);
+    if (!is_intra) {
+        if (cpu_mvs) {
+            r = HIP_OK(hipMemcpyAsync(s->mvs, cpu_mvs, (size_t)s->nmb * 16, hipMemcpyHostToDevice, st)) ? 0 : -1;
+        } else {
+            r = enqueue_me(s, rt, ref);
+        }
+    }
+    hipEventRecord(sl.ev[1], st);
+    if (r == 0) {
+        if (is_intra) {
+            r = bc250_launch_zero_i32(s->sync_i, s->mbh + 2, st);
+            if (r == 0)
+                r = bc250_launch_i_encode(cur_in.y, cur_in.uv, cur_in.py, cur_in.puv, out.y, out.uv, out.py, out.puv,
+                                          s->wc, s->hc, qp, num_slices, s->levels, s->dc, s->nz, s->modes, s->sync_i,
+                                          kSpinLimit, st);
+            if (r == 0) r = bc250_launch_zero_i32((int *)s->mvs, s->nmb * 4, st);
+            if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, s->nmb * 16, st);
+        } else {
+            r = bc250_launch_p_encode(cur_in.y, cur_in.uv, cur_in.py, cur_in.puv, ref.y, ref.uv, ref.py, ref.puv,
+                                      out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, s->mvs, s->levels, s->dc,
+                                      s->nz, s->modes, st);
+        }
+    }
+    hipEventRecord(sl.ev[2], st);
+    if (r == 0 && p.deblock_idc != 1) {
+        r = bc250_launch_zero_i32(s->sync_d, s->mbh + 2, st);
+        if (r == 0)
+            r = bc250_launch_deblock(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->nz, s->mvs,
+                                     num_slices, p.deblock_idc, s->sync_d, kSpinLimit, st);
+    }
+    hipEventRecord(sl.ev[3], st);
+    if (r == 0 && p.readback) {
+        const size_t n = (size_t)s->nmb;
+        bool ok = HIP_OK(hipMemcpyAsync(sl.levels, s->levels, n * 384 * 2, hipMemcpyDeviceToHost, st)) &&
+                  HIP_OK(hipMemcpyAsync(sl.dc, s->dc, n * 24 * 4, hipMemcpyDeviceToHost, st)) &&
+                  HIP_OK(hipMemcpyAsync(sl.modes, s->modes, n * 4, hipMemcpyDeviceToHost, st)) &&
+                  HIP_OK(hipMemcpyAsync(sl.mvs, s->mvs, n * 16, hipMemcpyDeviceToHost, st)) &&
+                  HIP_OK(hipMemcpyAsync(sl.nz, s->nz, n * 24 * 4, hipMemcpyDeviceToHost, st)) &&
+                  HIP_OK(hipMemcpyAsync(sl.mv8, s->mv8, n * 64, hipMemcpyDeviceToHost, st));
+        r = ok ? 0 : -1;
+    }
+    if (r == 0) {
+        // wait-error flags of the two wavefront kernels (sync[1])
+        bool ok = HIP_OK(hipMemcpyAsync(&sl.err[0], s->sync_i + 1, sizeof(int), hipMemcpyDeviceToHost, st)) &&
+                  HIP_OK(hipMemcpyAsync(&sl.err[1], s->sync_d + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        r = ok ? 0 : -1;
+    }
+    sl.have_data = r == 0 && p.readback;
+    sl.was_intra = is_intra != 0;
+    sl.recorded = false;
+    if (r != 0) {
+        fprintf(stderr, "[bc250-hip] dispatch_encode failed (%dx%d qp %d intra %d)\n", width, height, qp, is_intra);
+        return -1;
+    }
+    s->ref_idx = out_idx;
+    s->has_recon = true;
+    return 0;
+}
+
+extern "C" int gpu_compute_dispatch_encode(gpu_context_t *ctx, gpu_image_t rt, int width, int height, int qp,
+                                           int is_intra, int num_slices)
+{
+    return gpu_compute_dispatch_encode_ext(ctx, rt, width, height, qp, is_intra, num_slices, 0, nullptr);
+}
+
+// ME alone (gpu_compute.c:3432-3521): MVs land in the current slot's staging.
+extern "C" int gpu_compute_dispatch_me_only(gpu_context_t *ctx, gpu_image_t rt, int width, int height)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !rt.y_plane || rt.format != GPU_IMAGE_NV12) return -1;
+    if (ensure_res(s, width, height) != 0) return -1;
+    Slot &sl = s->slot[s->cur];
+    Plane2 cur_in;
+    cur_in.y = (uint8_t *)rt.y_plane;
+    cur_in.py = rt.y_pitch;
+    const Plane2 &ref = s->has_recon ? s->recon[s->ref_idx] : cur_in;
+    hipEventRecord(sl.ev[0], s->stream);
+    int r = enqueue_me(s, rt, ref);
+    hipEventRecord(sl.ev[1], s->stream);
+    hipEventRecord(sl.ev[2], s->stream);
+    hipEventRecord(sl.ev[3], s->stream);
+    if (r == 0 && s->params.readback) {
+        bool ok = HIP_OK(hipMemcpyAsync(sl.mvs, s->mvs, (size_t)s->nmb * 16, hipMemcpyDeviceToHost, s->stream)) &&
+                  HIP_OK(hipMemcpyAsync(sl.mv8, s->mv8, (size_t)s->nmb * 64, hipMemcpyDeviceToHost, s->stream));
+        r = ok ? 0 : -1;
+    }
+    if (sl.err) sl.err[0] = sl.err[1] = 0;
+    sl.have_data = false;
+    sl.recorded = false;
+    return r;
+}
+
+extern "C" double gpu_compute_get_last_latency_ms(const gpu_context_t *ctx)
+{
+    return ctx && ctx->hip ? ctx->hip->last_latency_ms : 0.0;
+}
+
+extern "C" int gpu_compute_end_picture(gpu_context_t *ctx)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s) return -1;
+    Slot &sl = s->slot[s->cur];
+    if (!HIP_OK(hipEventRecord(sl.ev[NEV - 1], s->stream))) return -1;
+    sl.pending = true;
+    sl.recorded = true;
+    s->submitted = s->cur;
+    s->cur ^= 1;
+    ctx->frames_submitted++;
+    return 0;
+}
+
+extern "C" int gpu_compute_submitted_slot(gpu_context_t *ctx)
+{
+    bc250_hip_state *s = st_of(ctx);
+    return s ? s->submitted : 0;
+}
+
+extern "C" int gpu_compute_sync_slot(gpu_context_t *ctx, int slot)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || slot < 0 || slot >= NSLOT) return -1;
+    Slot &sl = s->slot[slot];
+    if (!sl.recorded) return -1;
+    if (!HIP_OK(hipEventSynchronize(sl.ev[NEV - 1]))) {
+        sl.have_data = false;
+        return -1;
+    }
+    sl.pending = false;
+    float ms = 0.f;
+    if (HIP_OK(hipEventElapsedTime(&ms, sl.ev[0], sl.ev[NEV - 1]))) s->last_latency_ms = ms;
+    if (sl.err && (sl.err[0] | sl.err[1])) {
+        s->err_flags |= (sl.err[0] ? 1 : 0) | (sl.err[1] ? 2 : 0);
+        fprintf(stderr, "[bc250-hip] wavefront dependency wait timed out (slot %d)\n", slot);
+        sl.have_data = false;
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" int gpu_compute_sync(gpu_context_t *ctx)
+{
+    return gpu_compute_sync_slot(ctx, gpu_compute_submitted_slot(ctx));
+}
+
+static int staging(gpu_context_t *ctx, int slot, int which, void **data, size_t *size)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !data || slot < 0 || slot >= NSLOT) return -1;
+    Slot &sl = s->slot[slot];
+    const size_t n = (size_t)s->nmb;
+    bool mv_ok = which == 3 || which == 5;  // MVs are valid after dispatch_me_only too
+    if (!sl.levels || (!sl.have_data && !mv_ok)) {
+        *data = nullptr;
+        if (size) *size = 0;
+        return -1;
+    }
+    switch (which) {
+    case 0: *data = sl.levels; if (size) *size = n * 384 * 2; break;
+    case 1: *data = sl.dc; if (size) *size = n * 24 * 4; break;
+    case 2: *data = sl.modes; if (size) *size = n * 4; break;
+    case 3: *data = sl.mvs; if (size) *size = n * 16; break;
+    case 4: *data = sl.nz; if (size) *size = n * 24 * 4; break;
+    default: *data = sl.mv8; if (size) *size = n * 64; break;
+    }
+    return 0;
+}
+
+extern "C" int gpu_compute_get_quant_staging_data_slot(gpu_context_t *c, int slot, void **d, size_t *z) { return staging(c, slot, 0, d, z); }
+extern "C" int gpu_compute_get_dc_staging_data_slot(gpu_context_t *c, int slot, void **d, size_t *z) { return staging(c, slot, 1, d, z); }
+extern "C" int gpu_compute_get_pred_mode_staging_data_slot(gpu_context_t *c, int slot, void **d, size_t *z) { return staging(c, slot, 2, d, z); }
+extern "C" int gpu_compute_get_mv_staging_data_slot(gpu_context_t *c, int slot, void **d, size_t *z) { return staging(c, slot, 3, d, z); }
+extern "C" int gpu_compute_get_nz_staging_data_slot(gpu_context_t *c, int slot, void **d, size_t *z) { return staging(c, slot, 4, d, z); }
+extern "C" int gpu_compute_get_mv8_staging_data_slot(gpu_context_t *c, int slot, void **d, size_t *z) { return staging(c, slot, 5, d, z); }
+
+extern "C" int gpu_compute_get_quant_staging_data(gpu_context_t *c, void **d, size_t *z) { return staging(c, gpu_compute_submitted_slot(c), 0, d, z); }
+extern "C" int gpu_compute_get_dc_staging_data(gpu_context_t *c, void **d, size_t *z) { return staging(c, gpu_compute_submitted_slot(c), 1, d, z); }
+extern "C" int gpu_compute_get_pred_mode_staging_data(gpu_context_t *c, void **d, size_t *z) { return staging(c, gpu_compute_submitted_slot(c), 2, d, z); }
+extern "C" int gpu_compute_get_mv_staging_data(gpu_context_t *c, void **d, size_t *z) { return staging(c, gpu_compute_submitted_slot(c), 3, d, z); }
+extern "C" int gpu_compute_get_nz_staging_data(gpu_context_t *c, void **d, size_t *z) { return staging(c, gpu_compute_submitted_slot(c), 4, d, z); }
+// The reference's generic staging getter returns the levels of the last frame.
+extern "C" int gpu_compute_get_staging_data(gpu_context_t *c, void **d, size_t *z) { return staging(c, gpu_compute_submitted_slot(c), 0, d, z); }
+extern "C" int gpu_compute_release_staging_data(gpu_context_t *ctx) { return ctx ? 0 : -1; }
+
+extern "C" void bc250_debug_dump_nv12_frame(const uint8_t *, int, const uint8_t *, int, int, int) {}
+extern "C" void gpu_compute_debug_dump_recon(gpu_context_t *, int, int) {}
+extern "C" void gpu_compute_debug_dump_real_input(gpu_context_t *, gpu_image_t *, gpu_memory_t, int, int) {}
+
+// ------------------------------------------------------------ extensions
+
+extern "C" int gpu_compute_set_encode_params(gpu_context_t *ctx, const gpu_encode_params_t *p)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !p) return -1;
+    if (p->search_range != 16 && p->search_range != 32 && p->search_range != 64) return -1;
+    if (p->lambda8 < 0 || p->lambda8 > 15 || p->deblock_idc < 0 || p->deblock_idc > 2) return -1;
+    s->params = *p;
+    s->params.me_8x8 = p->me_8x8 ? 1 : 0;
+    s->params.readback = p->readback ? 1 : 0;
+    return 0;
+}
+
+extern "C" int gpu_compute_get_encode_params(gpu_context_t *ctx, gpu_encode_params_t *p)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !p) return -1;
+    *p = s->params;
+    return 0;
+}
+
+extern "C" int gpu_compute_download_recon(gpu_context_t *ctx, uint8_t *y_plane, int y_pitch, uint8_t *uv_plane,
+                                          int uv_pitch)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !s->has_recon || !y_plane || !uv_plane) return -1;
+    const Plane2 &r = s->recon[s->ref_idx];
+    bool ok = HIP_OK(hipMemcpy2DAsync(y_plane, y_pitch, r.y, r.py, s->w, s->h, hipMemcpyDeviceToHost, s->stream)) &&
+              HIP_OK(hipMemcpy2DAsync(uv_plane, uv_pitch, r.uv, r.puv, (size_t)((s->w + 1) / 2) * 2, (s->h + 1) / 2,
+                                      hipMemcpyDeviceToHost, s->stream)) &&
+              HIP_OK(hipStreamSynchronize(s->stream));
+    return ok ? 0 : -1;
+}
+
+extern "C" int gpu_compute_get_stage_times(gpu_context_t *ctx, int slot, float ms[5])
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !ms || slot < 0 || slot >= NSLOT || !s->slot[slot].recorded) return -1;
+    Slot &sl = s->slot[slot];
+    if (!HIP_OK(hipEventSynchronize(sl.ev[NEV - 1]))) return -1;
+    for (int i = 0; i < 4; i++)
+        if (!HIP_OK(hipEventElapsedTime(&ms[i], sl.ev[i], sl.ev[i + 1]))) return -1;
+    return HIP_OK(hipEventElapsedTime(&ms[4], sl.ev[0], sl.ev[NEV - 1])) ? 0 : -1;
+}
+
+extern "C" int gpu_compute_take_error_flags(gpu_context_t *ctx)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s) return -1;
+    int f = s->err_flags;
+    s->err_flags = 0;
+    return f;
+}

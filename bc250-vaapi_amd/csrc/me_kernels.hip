@@ -1,0 +1,289 @@
+// me_kernels.hip - +-R integer full-search motion estimation and the
+// half/quarter-pel refinement, hand-written for gfx950.
+//
+// Integer search (me_fullsearch_kernel): one workgroup per macroblock.  The
+// reference window of the MB (edge-replicated, (16+2R+7) x (20+2R) bytes) is
+// staged in LDS; the 16x16 current block is wave-uniform and lives in SGPRs.
+// Every lane owns a tile of 4 (dx) x 8 (dy) candidates: for each reference
+// row it reads 5 aligned dwords from LDS, builds the byte-shifted copies with
+// v_alignbyte_b32, and feeds v_sad_u8 with four 8x8-quadrant accumulators
+// per candidate (128 VGPRs), so the 16x16 cost and the four 8x8 costs come
+// from one pass.  Accumulators start at lambda8*|mv|_1, which makes
+//   cost8 = SAD8 + lambda8*|mv|_1      cost16 = SAD16 + 4*lambda8*|mv|_1.
+// The argmin key is cost*2^15 + raster index of (dx,dy) (ties -> lowest
+// raster index), reduced per lane, across the wave with DPP shuffles and
+// across waves through LDS.
+#include "bc250_dev.h"
+#include "bc250_stages.h"
+
+namespace {
+
+constexpr int ME_THREADS = 192;
+
+template <int R>
+struct FsGeom {
+    static constexpr int NC = 2 * R + 1;          // candidates per axis
+    static constexpr int NG = (NC + 3) / 4;       // dx groups of 4
+    static constexpr int NGY = (NC + 7) / 8;      // dy groups of 8
+    static constexpr int NT = NG * NGY;           // lane tasks per MB
+    static constexpr int NR = NGY * 8 + 15;       // window rows
+    static constexpr int WD = NG + 4;             // window dwords per row
+    static constexpr int PITCH = WD | 1;          // odd dword pitch
+    static constexpr int SLOTS = (NT + ME_THREADS - 1) / ME_THREADS;
+    static_assert(R % 4 == 0, "range must be a multiple of 4");
+    static_assert(NC * NC <= 32768, "raster index must fit 15 bits");
+};
+
+}  // namespace
+
+// out16[m] = {dx, dy, cost, sad}; out8[4m+q] likewise for quadrant q.
+template <int R>
+__global__ __launch_bounds__(ME_THREADS) void me_fullsearch_kernel(
+    const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref, int rpitch,
+    int wc, int hc, int mbw, int nmb, int lambda8, uint4 *__restrict__ out16,
+    uint4 *__restrict__ out8)
+{
+    using Gm = FsGeom<R>;
+    __shared__ uint32_t win[Gm::NR * Gm::PITCH];
+    __shared__ uint32_t red[3][5];
+
+    const int mb = xcd_item(blockIdx.x, nmb);
+    if (mb >= nmb) return;
+    const int mbx = mb % mbw, mby = mb / mbw;
+    const int x0 = mbx * 16, y0 = mby * 16;
+    const int tid = threadIdx.x;
+
+    // ---- stage the edge-replicated reference window in LDS
+    const int wx0 = x0 - R, wy0 = y0 - R;
+    for (int i = tid; i < Gm::NR * Gm::WD; i += ME_THREADS) {
+        int r = i / Gm::WD, d = i - r * Gm::WD;
+        int gy = clampi(wy0 + r, 0, hc - 1);
+        int gx = wx0 + 4 * d;
+        const uint8_t *row = ref + (size_t)gy * rpitch;
+        uint32_t v;
+        if (gx >= 0 && gx + 3 < wc) {
+            v = *(const uint32_t *)(row + gx);
+        } else {
+            v = (uint32_t)row[clampi(gx, 0, wc - 1)] | ((uint32_t)row[clampi(gx + 1, 0, wc - 1)] << 8) |
+                ((uint32_t)row[clampi(gx + 2, 0, wc - 1)] << 16) |
+                ((uint32_t)row[clampi(gx + 3, 0, wc - 1)] << 24);
+        }
+        win[r * Gm::PITCH + d] = v;
+    }
+
+    // ---- current block: uniform loads (scalar unit)
+    uint32_t cw[16][4];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t *p = (const uint32_t *)(cur + (size_t)(y0 + r) * cpitch + x0);
+#pragma unroll
+        for (int c = 0; c < 4; c++) cw[r][c] = __builtin_amdgcn_readfirstlane(p[c]);
+    }
+    __syncthreads();
+
+    uint32_t best16 = 0xffffffffu, bq[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+
+#pragma unroll 1
+    for (int slot = 0; slot < Gm::SLOTS; slot++) {
+        const int t = tid + slot * ME_THREADS;
+        if (t < Gm::NT) {
+            const int dxg = t % Gm::NG, dyg = t / Gm::NG;
+            const int dx0 = -R + 4 * dxg, dy0 = -R + 8 * dyg;
+            uint32_t acc[8][4][4];
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    int dx = dx0 + k, dy = dy0 + j;
+                    uint32_t init = (uint32_t)(lambda8 * (iabs(dx) + iabs(dy)));
+                    if (dx > R || dy > R) init += 1u << 20;  // outside the range: never wins
+#pragma unroll
+                    for (int q = 0; q < 4; q++) acc[j][k][q] = init;
+                }
+            const uint32_t *wrow = win + (dyg * 8) * Gm::PITCH + dxg;
+#pragma unroll
+            for (int r = 0; r < 23; r++) {
+                uint32_t d[5];
+#pragma unroll
+                for (int i = 0; i < 5; i++) d[i] = wrow[r * Gm::PITCH + i];
+                uint32_t s[4][4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    s[0][c] = d[c];
+                    s[1][c] = __builtin_amdgcn_alignbyte(d[c + 1], d[c], 1);
+                    s[2][c] = __builtin_amdgcn_alignbyte(d[c + 1], d[c], 2);
+                    s[3][c] = __builtin_amdgcn_alignbyte(d[c + 1], d[c], 3);
+                }
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int cr = r - j;
+                    if (cr < 0 || cr > 15) continue;
+                    const int qr = (cr >> 3) << 1;
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++)
+                            acc[j][k][qr + (c >> 1)] =
+                                __builtin_amdgcn_sad_u8(s[k][c], cw[cr][c], acc[j][k][qr + (c >> 1)]);
+                }
+            }
+            // per-lane argmin with a 5-bit local index (j-major = raster order)
+            uint32_t l16 = 0xffffffffu, lq[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t li = (uint32_t)(j * 4 + k);
+                    uint32_t s16 = acc[j][k][0] + acc[j][k][1] + acc[j][k][2] + acc[j][k][3];
+                    l16 = min(l16, (s16 << 5) | li);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) lq[q] = min(lq[q], (acc[j][k][q] << 5) | li);
+                }
+            // promote to global keys: cost*2^15 + raster index
+            auto gkey = [&](uint32_t lk) -> uint32_t {
+                uint32_t li = lk & 31u, cost = lk >> 5;
+                int dx = dx0 + (int)(li & 3u), dy = dy0 + (int)(li >> 2);
+                uint32_t idx = (uint32_t)((dy + R) * Gm::NC + (dx + R));
+                return (cost << 15) | idx;
+            };
+            best16 = min(best16, gkey(l16));
+#pragma unroll
+            for (int q = 0; q < 4; q++) bq[q] = min(bq[q], gkey(lq[q]));
+        }
+    }
+
+    // ---- reduce across the wave, then across the 3 waves
+    best16 = wave_min_u32(best16);
+#pragma unroll
+    for (int q = 0; q < 4; q++) bq[q] = wave_min_u32(bq[q]);
+    const int wave = tid >> 6, lane = tid & 63;
+    if (lane == 0) {
+        red[wave][0] = best16;
+#pragma unroll
+        for (int q = 0; q < 4; q++) red[wave][1 + q] = bq[q];
+    }
+    __syncthreads();
+    if (tid < 5) {
+        uint32_t k = min(min(red[0][tid], red[1][tid]), red[2][tid]);
+        uint32_t idx = k & 0x7fffu, cost = k >> 15;
+        int dx = (int)(idx % Gm::NC) - R, dy = (int)(idx / Gm::NC) - R;
+        uint32_t l1 = (uint32_t)(iabs(dx) + iabs(dy));
+        uint32_t sad = cost - (tid == 0 ? 4u : 1u) * (uint32_t)lambda8 * l1;
+        uint4 o = make_uint4((uint32_t)dx, (uint32_t)dy, cost, sad);
+        if (tid == 0) out16[mb] = o;
+        else out8[4 * mb + tid - 1] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sub-pel refinement: 16x16 then each 8x8 quadrant, one wave per MB.  For a
+// block with integer winner (dx,dy): SAD at the integer position, then the 8
+// half-pel candidates (2,0),(-2,0),(0,2),(0,-2),(2,2),(2,-2),(-2,2),(-2,-2)
+// and the 8 quarter-pel ones around the half winner, strict < in that order.
+// The interpolation window (block + 4 on every side) is staged in LDS with
+// replicated edges, then every lane interpolates its pixels per candidate.
+__global__ __launch_bounds__(64) void me_refine_kernel(
+    const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref, int rpitch,
+    int wc, int hc, int mbw, int nmb, const uint4 *__restrict__ in16, const uint4 *__restrict__ in8,
+    uint4 *__restrict__ mv16, uint4 *__restrict__ mv8, int do8)
+{
+    __shared__ uint8_t w[24 * 24];
+    __shared__ uint8_t cb[16 * 16];
+    const int mb = xcd_item(blockIdx.x, nmb);
+    if (mb >= nmb) return;
+    const int lane = threadIdx.x;
+    const int mbx = mb % mbw, mby = mb / mbw;
+    const int X0 = mbx * 16, Y0 = mby * 16;
+    const int HO[8][2] = {{2, 0}, {-2, 0}, {0, 2}, {0, -2}, {2, 2}, {2, -2}, {-2, 2}, {-2, -2}};
+    const int QO[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+
+    for (int i = lane; i < 256; i += 64) cb[i] = cur[(size_t)(Y0 + (i >> 4)) * cpitch + X0 + (i & 15)];
+
+    const int njobs = do8 ? 5 : 1;
+    for (int job = 0; job < njobs; job++) {
+        const int bw = job == 0 ? 16 : 8;
+        const int bx0 = job == 0 ? 0 : ((job - 1) & 1) * 8, by0 = job == 0 ? 0 : ((job - 1) >> 1) * 8;
+        const uint4 iv = job == 0 ? in16[mb] : in8[4 * mb + job - 1];
+        const int dx = (int)iv.x, dy = (int)iv.y;
+        const int ww = bw + 8;
+        __syncthreads();
+        for (int i = lane; i < ww * ww; i += 64) {
+            int yy = i / ww, xx = i - yy * ww;
+            int gx = clampi(X0 + bx0 + dx - 4 + xx, 0, wc - 1);
+            int gy = clampi(Y0 + by0 + dy - 4 + yy, 0, hc - 1);
+            w[yy * 24 + xx] = ref[(size_t)gy * rpitch + gx];
+        }
+        __syncthreads();
+        const int npx = bw * bw;
+        // SAD of this block at quarter-pel offset (ox, oy) from the integer winner
+        auto sad_at = [&](int ox, int oy) -> uint32_t {
+            uint32_t s = 0;
+            for (int p = lane; p < npx; p += 64) {
+                int px = p % bw, pyy = p / bw;
+                int qx = 4 * px + ox, qy = 4 * pyy + oy;  // window-relative quarter pel, origin at +4
+                int ix = (qx >> 2) + 4, iy = (qy >> 2) + 4;
+                int v = qpel_win(w, 24, ix, iy, qx & 3, qy & 3);
+                s += (uint32_t)iabs((int)cb[(by0 + pyy) * 16 + bx0 + px] - v);
+            }
+            return wave_sum_u32(s);
+        };
+        uint32_t best = sad_at(0, 0);
+        int hx = 0, hy = 0;
+        for (int c = 0; c < 8; c++) {
+            uint32_t s = sad_at(HO[c][0], HO[c][1]);
+            if (s < best) { best = s; hx = HO[c][0]; hy = HO[c][1]; }
+        }
+        int qx = 0, qy = 0;
+        for (int c = 0; c < 8; c++) {
+            uint32_t s = sad_at(hx + QO[c][0], hy + QO[c][1]);
+            if (s < best) { best = s; qx = QO[c][0]; qy = QO[c][1]; }
+        }
+        if (lane == 0) {
+            uint4 o = make_uint4((uint32_t)(4 * dx + hx + qx), (uint32_t)(4 * dy + hy + qy), best, 0u);
+            if (job == 0) mv16[mb] = o;
+            else mv8[4 * mb + job - 1] = o;
+        }
+    }
+}
+
+// Host-side launchers (declared in bc250_stages.h).
+extern "C" int bc250_launch_me_fullsearch(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch,
+                                          int wc, int hc, int range, int lambda8, uint32_t *out16_,
+                                          uint32_t *out8_, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    uint4 *out16 = (uint4 *)out16_, *out8 = (uint4 *)out8_;
+    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || (cpitch & 3) || lambda8 < 0 || lambda8 > 15) return -1;
+    const int mbw = wc / 16, nmb = mbw * (hc / 16);
+    const int grid = ((nmb + 7) / 8) * 8;
+    switch (range) {
+    case 16:
+        me_fullsearch_kernel<16><<<grid, ME_THREADS, 0, st>>>(cur, cpitch, ref, rpitch, wc, hc, mbw, nmb,
+                                                             lambda8, out16, out8);
+        break;
+    case 32:
+        me_fullsearch_kernel<32><<<grid, ME_THREADS, 0, st>>>(cur, cpitch, ref, rpitch, wc, hc, mbw, nmb,
+                                                             lambda8, out16, out8);
+        break;
+    case 64:
+        me_fullsearch_kernel<64><<<grid, ME_THREADS, 0, st>>>(cur, cpitch, ref, rpitch, wc, hc, mbw, nmb,
+                                                             lambda8, out16, out8);
+        break;
+    default:
+        return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_launch_me_refine(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
+                                      int hc, const uint32_t *in16, const uint32_t *in8, uint32_t *mv16,
+                                      uint32_t *mv8, int do8, void *stream)
+{
+    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15)) return -1;
+    const int mbw = wc / 16, nmb = mbw * (hc / 16);
+    const int grid = ((nmb + 7) / 8) * 8;
+    me_refine_kernel<<<grid, 64, 0, (hipStream_t)stream>>>(cur, cpitch, ref, rpitch, wc, hc, mbw, nmb,
+                                                            (const uint4 *)in16, (const uint4 *)in8,
+                                                            (uint4 *)mv16, (uint4 *)mv8, do8);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -1,0 +1,64 @@
+/*
+ * bc250_host.h - host-side C of the MI355X encoder pipeline: the seeded
+ * synthetic frame generator, the H.264 host DC stages that the entropy
+ * coder runs on the GPU's `dc` readback, and the frame-sequence driver
+ * built on the gpu_compute_* ABI.
+ */
+#ifndef BC250_HOST_H
+#define BC250_HOST_H
+
+#include <stdint.h>
+
+#include "gpu_compute.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- seeded synthetic frames --------------------------------------------
+ * luma = 128 + sum of 8 sinusoids (periods 8..128 px, sum of |amplitude| =
+ * 40) translated per frame by a global quarter-pel motion uniform in
+ * [-4,4] px, then `nrect` rectangles (32..128 px) of uniform-noise texture
+ * moving by a random +-8 px per frame, then Gaussian noise sigma 2, clamped
+ * to [16,235]; chroma = 2x2 average of luma re-centred at 128 +-20.  Frame
+ * f depends only on (seed, f): frames may be generated in any order and on
+ * any number of threads. */
+typedef struct bc250_gen bc250_gen_t;
+bc250_gen_t *bc250_gen_create(uint64_t seed, int width, int height, int nrect);
+void bc250_gen_destroy(bc250_gen_t *g);
+/* NV12 into host planes (uv interleaved Cb/Cr). Returns 0. */
+int bc250_gen_frame(bc250_gen_t *g, int frame, uint8_t *y, int y_pitch, uint8_t *uv, int uv_pitch);
+/* Global offset of frame f in quarter pels (for tests). */
+void bc250_gen_offset_qpel(bc250_gen_t *g, int frame, int *ox, int *oy);
+
+/* ---- H.264 host DC stages (reference encoder_h264.c:653-797) ------------ */
+int h264_chroma_qp(int qpy);
+/* Reference encoder_h264.c:775: transmitted I16x16 luma DC levels in the
+ * natural row/column order (Hadamard, threshold-5 quantiser, transpose). */
+void h264_intra16_luma_dc_transform(const int dc_in[4][4], int qp, int dc_out[16], int dc_out_pretranspose[16]);
+/* Transmitted chroma DC levels of one component (2x2 Hadamard + quantiser
+ * at QPc), raster TL, TR, BL, BR. */
+void h264_chroma_dc_levels(const int dc_in[4], int qpc, int out[4]);
+/* Frame-level DC levels from the `dc` readback: luma [nmb*16] (I16x16
+ * frames only, zero otherwise) and chroma [nmb*8] (Cb 4 then Cr 4). */
+void h264_frame_dc_levels(const int32_t *dc, int nmb, int qp, int is_intra, int16_t *luma_dc, int16_t *chroma_dc);
+
+/* ---- sequence driver ------------------------------------------------------ */
+typedef struct {
+    int qp;          /* CQP */
+    int gop;         /* I-frame period (0: only the first frame is I) */
+    int num_slices;
+    int pipelined;   /* submit frame n+1 before syncing frame n */
+} bc250_seq_cfg_t;
+
+/* Encodes frames[0..n) (device images already holding the pictures) as
+ * I P P P ... through begin_picture / dispatch_encode_ext / end_picture /
+ * sync_slot.  frame_ms[i] (optional) receives submit-to-sync latency of
+ * frame i (CLOCK_MONOTONIC), *total_ms the wall time of the whole run. */
+int bc250_encode_sequence(gpu_context_t *ctx, const gpu_image_t *frames, int n, int width, int height,
+                          const bc250_seq_cfg_t *cfg, double *frame_ms, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,71 @@
+/*
+ * bc250_stages.h - per-stage entry points of the gfx950 encoder pipeline.
+ *
+ * Each launcher enqueues one stage on `stream` (a hipStream_t, NULL for the
+ * null stream) over device buffers in the layouts of gpu_compute.h and
+ * returns 0, or -1 when the launch is rejected.  They replace the
+ * reference's per-shader dispatches recorded by
+ * gpu_compute_dispatch_encode_ext (approach1-compute-encoder/src/
+ * gpu_compute.c:3078-3290); gpu_compute.hip calls them in that order, and
+ * the parity tests call them one at a time.
+ *
+ * MV records are 4 x uint32 per block: {mvx, mvy, sad, pad} (quarter pel)
+ * for refined vectors, {dx, dy, cost, sad} (whole pel) for the integer
+ * search.  `*16` arrays hold one record per MB, `*8` arrays four per MB
+ * (8x8 quadrants TL, TR, BL, BR).
+ */
+#ifndef BC250_STAGES_H
+#define BC250_STAGES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* +-range integer full search, range in {16, 32, 64}, lambda8 in 0..15.
+ * Replaces the motion_estimation.comp dispatch (gpu_compute.c:3093-3104),
+ * integer stage (shaders/motion_estimation.comp:278-374). */
+int bc250_launch_me_fullsearch(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
+                               int range, int lambda8, uint32_t *out16, uint32_t *out8, void *stream);
+
+/* Half- then quarter-pel refinement of the integer winners
+ * (shaders/motion_estimation.comp:376-469). do8 = also the 8x8 quadrants. */
+int bc250_launch_me_refine(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
+                           const uint32_t *in16, const uint32_t *in8, uint32_t *mv16, uint32_t *mv8, int do8,
+                           void *stream);
+
+/* P16x16 encode: MC, residual, 4x4 transform, inter quant, reconstruction.
+ * Replaces residual_predict -> dct_transform -> quantize -> reconstruct
+ * (gpu_compute.c:3109-3190). `o*` must not alias `r*`. */
+int bc250_launch_p_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, const uint8_t *ry,
+                          const uint8_t *ruv, int rpy, int rpuv, uint8_t *oy, uint8_t *ouv, int opy, int opuv,
+                          int wc, int hc, int qp, const uint32_t *mvs, int16_t *levels, int32_t *dc, uint32_t *nz,
+                          uint32_t *modes, void *stream);
+
+/* I16x16 wavefront in one persistent launch; replaces the 187 per-diagonal
+ * intra_wavefront.comp dispatches (gpu_compute.c:3191-3255).  `sync` holds
+ * 2 + coded_height/16 ints and must be zero; sync[1] != 0 afterwards means
+ * a dependency wait gave up after spin_limit polls. */
+int bc250_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, uint8_t *oy, uint8_t *ouv,
+                          int opy, int opuv, int wc, int hc, int qp, int num_slices, int16_t *levels, int32_t *dc,
+                          uint32_t *nz, uint32_t *modes, int *sync, unsigned spin_limit, void *stream);
+
+/* H.264 8.7 deblocking in place (luma + chroma), one persistent launch;
+ * replaces the two deblock_filter.comp passes (gpu_compute.c:3257-3290).
+ * `sync` as for bc250_launch_i_encode. */
+int bc250_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                         const uint32_t *nz, const uint32_t *mvs, int num_slices, int idc, int *sync,
+                         unsigned spin_limit, void *stream);
+
+/* Zero `n` ints with a fill kernel (no memset node, so capturable). */
+int bc250_launch_zero_i32(int *p, int n, void *stream);
+
+/* Replicate the last visible row / column of a plane pair into the coded
+ * padding (w x h visible, wc x hc coded). */
+int bc250_launch_pad_edges(uint8_t *y, uint8_t *uv, int py, int puv, int w, int h, int wc, int hc, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

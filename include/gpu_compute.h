@@ -1,0 +1,201 @@
+/*
+ * gpu_compute.h - C ABI of the MI355X (gfx950) encoder compute pipeline.
+ *
+ * Drop-in for the reference's Vulkan orchestration layer
+ * (approach1-compute-encoder/src/gpu_compute.h).  Every declaration keeps
+ * the reference name, argument meaning, return convention (0 / -1) and the
+ * double-buffered staging contract; the citation on each names the
+ * reference declaration it replaces.  Entry points marked EXTENSION have no
+ * reference counterpart.  No HIP or torch type appears here: device
+ * memory is carried as plain pointers, so the host encoders compile with a
+ * plain C compiler and link against libbc250_hip.so.
+ *
+ * Surface layout: an image is one device allocation holding the luma plane
+ * (coded size: width and height rounded up to 16) followed by the
+ * interleaved Cb/Cr plane (coded width x coded height/2).  Rows are padded
+ * to a pitch that is a multiple of 256 bytes.  Uploads replicate the last
+ * visible row/column into the coded padding.
+ */
+#ifndef BC250_GPU_COMPUTE_H
+#define BC250_GPU_COMPUTE_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reference gpu_compute.h:23-40 (gpu_image_t, passed by value) */
+typedef struct {
+    void *y_plane;          /* device pointer, luma, coded size */
+    void *uv_plane;         /* device pointer, interleaved Cb/Cr */
+    uint32_t width;         /* visible size */
+    uint32_t height;
+    uint32_t coded_width;   /* 16-aligned */
+    uint32_t coded_height;
+    uint32_t y_pitch;       /* bytes */
+    uint32_t uv_pitch;
+    int current_layout;     /* kept for the reference's struct shape; unused */
+    int format;             /* GPU_IMAGE_NV12 / GPU_IMAGE_P010 */
+} gpu_image_t;
+
+/* reference gpu_compute.h:42-46 */
+typedef struct {
+    void *memory;           /* base of the device allocation */
+    size_t size;
+    void *mapped_ptr;       /* host staging while mapped, else NULL */
+} gpu_memory_t;
+
+/* reference gpu_compute.h:62-68 */
+typedef struct {
+    uint32_t y_pitch;
+    uint64_t y_offset;
+    uint32_t uv_pitch;
+    uint64_t uv_offset;
+    uint64_t total_size;
+} gpu_nv12_layout_t;
+
+struct bc250_hip_state;
+
+/* reference gpu_compute.h:70-352 (bc250_gpu_context_t, caller-allocated).
+ * The HIP device state lives behind `hip`; the caller zero-initialises the
+ * struct and calls bc250_gpu_init(). */
+typedef struct bc250_gpu_context {
+    struct bc250_hip_state *hip;
+    int device;
+    uint32_t frames_submitted;
+    uint32_t frame_width;
+    uint32_t frame_height;
+} bc250_gpu_context_t;
+
+typedef bc250_gpu_context_t gpu_context_t;
+
+/* reference gpu_compute.h:571-576: per-MB motion vector in quarter-pel
+ * units plus the SAD of the final (sub-pel) position. */
+typedef struct {
+    int32_t mvx, mvy;
+    uint32_t sad;
+    uint32_t _pad;
+} gpu_mv_t;
+
+#define GPU_IMAGE_NV12 0 /* reference gpu_compute.h:371-372 */
+#define GPU_IMAGE_P010 1
+
+/* ---- lifecycle: reference gpu_compute.h:357-377 -------------------------- */
+int bc250_gpu_init(bc250_gpu_context_t *ctx);
+void bc250_gpu_destroy(bc250_gpu_context_t *ctx);
+void gpu_compute_report_cu_topology(bc250_gpu_context_t *ctx);
+int gpu_compute_init(gpu_context_t *ctx);
+void gpu_compute_terminate(gpu_context_t *ctx);
+
+/* ---- surfaces: reference gpu_compute.h:380-463 ---------------------------- */
+int gpu_compute_create_image(gpu_context_t *ctx, int width, int height, int format, gpu_image_t *image,
+                             gpu_memory_t *memory);
+void gpu_compute_destroy_image(gpu_context_t *ctx, gpu_image_t image, gpu_memory_t memory);
+int gpu_compute_get_nv12_layout(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory,
+                                gpu_nv12_layout_t *layout);
+/* Device memory is not host-visible: mapping hands out a pinned host mirror
+ * (with the same layout) and unmap uploads it. */
+uint8_t *gpu_compute_map_surface(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory,
+                                 gpu_nv12_layout_t *layout, bool *unmap);
+void gpu_compute_unmap_surface(gpu_context_t *ctx, gpu_memory_t memory, bool unmap);
+int gpu_compute_upload_nv12(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory, const uint8_t *y_plane,
+                            int y_pitch, const uint8_t *uv_plane, int uv_pitch, int width, int height);
+int gpu_compute_upload_p010(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory, const uint16_t *y_plane,
+                            int y_pitch, const uint16_t *uv_plane, int uv_pitch, int width, int height);
+int gpu_compute_video_proc(gpu_context_t *ctx, gpu_image_t *src, const int src_rect[4], gpu_image_t *dst,
+                           const int dst_rect[4]);
+void gpu_compute_copy_from_wc(void *dst, const void *src, size_t n);
+int gpu_compute_download_nv12(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory, uint8_t *y_plane,
+                              int y_pitch, uint8_t *uv_plane, int uv_pitch, int width, int height);
+int gpu_compute_export_nv12_dmabuf(gpu_context_t *ctx, gpu_memory_t memory, int *out_fd);
+int gpu_compute_import_dmabuf_image(gpu_context_t *ctx, int dma_buf_fd, int width, int height, int format,
+                                    uint32_t stride, uint32_t offset, gpu_image_t *image, gpu_memory_t *memory);
+int gpu_compute_wait_for_image_ready(gpu_context_t *ctx, gpu_memory_t memory);
+int gpu_compute_wait_for_image_ready_host(gpu_context_t *ctx, gpu_memory_t memory);
+int gpu_compute_cpu_read_safe(gpu_context_t *ctx);
+
+/* ---- encode pipeline: reference gpu_compute.h:582-612 ---------------------
+ * dispatch_encode_ext records, for the current slot, the stages of one frame
+ * in the reference's order: motion estimation (skipped when cpu_mvs is
+ * given; those MVs are uploaded instead), then the I16x16 wavefront (intra)
+ * or the P16x16 encode (inter), then deblocking, then the copies of levels,
+ * dc, pred modes, MVs and nz masks into the slot's host staging.  The
+ * reference picture is this context's previous reconstruction (the render
+ * target itself for the first frame).  me_mode is accepted for signature
+ * compatibility; ME is the +-R full search set by
+ * gpu_compute_set_encode_params(). */
+int gpu_compute_begin_picture(gpu_context_t *ctx, gpu_image_t render_target);
+int gpu_compute_dispatch_encode(gpu_context_t *ctx, gpu_image_t render_target, int width, int height, int qp,
+                                int is_intra, int num_slices);
+int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t render_target, int width, int height, int qp,
+                                    int is_intra, int num_slices, int me_mode, const gpu_mv_t *cpu_mvs);
+int gpu_compute_dispatch_me_only(gpu_context_t *ctx, gpu_image_t render_target, int width, int height);
+double gpu_compute_get_last_latency_ms(const gpu_context_t *ctx);
+int gpu_compute_end_picture(gpu_context_t *ctx);
+int gpu_compute_sync(gpu_context_t *ctx);
+int gpu_compute_submitted_slot(gpu_context_t *ctx);
+int gpu_compute_sync_slot(gpu_context_t *ctx, int slot);
+
+/* ---- readback: reference gpu_compute.h:613-648 ----------------------------
+ * Pointers into pinned host buffers owned by ctx, valid until the slot is
+ * reused.  Layouts: levels int16 [nmb*24*16]; dc int32 [nmb*24]; pred modes
+ * uint32 [nmb]; mvs gpu_mv_t [nmb]; nz uint32 [nmb*24]. */
+int gpu_compute_get_quant_staging_data_slot(gpu_context_t *ctx, int slot, void **data, size_t *size);
+int gpu_compute_get_dc_staging_data_slot(gpu_context_t *ctx, int slot, void **data, size_t *size);
+int gpu_compute_get_pred_mode_staging_data_slot(gpu_context_t *ctx, int slot, void **data, size_t *size);
+int gpu_compute_get_mv_staging_data_slot(gpu_context_t *ctx, int slot, void **data, size_t *size);
+int gpu_compute_get_nz_staging_data_slot(gpu_context_t *ctx, int slot, void **data, size_t *size);
+int gpu_compute_get_staging_data(gpu_context_t *ctx, void **data, size_t *size);
+int gpu_compute_release_staging_data(gpu_context_t *ctx);
+int gpu_compute_get_quant_staging_data(gpu_context_t *ctx, void **data, size_t *size);
+int gpu_compute_get_dc_staging_data(gpu_context_t *ctx, void **data, size_t *size);
+int gpu_compute_get_pred_mode_staging_data(gpu_context_t *ctx, void **data, size_t *size);
+int gpu_compute_get_mv_staging_data(gpu_context_t *ctx, void **data, size_t *size);
+int gpu_compute_get_nz_staging_data(gpu_context_t *ctx, void **data, size_t *size);
+
+/* ---- debug hooks: reference gpu_compute.h:556-667 (no-ops here) ---------- */
+void bc250_debug_dump_nv12_frame(const uint8_t *y_plane, int y_pitch, const uint8_t *uv_plane, int uv_pitch,
+                                 int width, int height);
+void gpu_compute_debug_dump_recon(gpu_context_t *ctx, int width, int height);
+void gpu_compute_debug_dump_real_input(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory, int width,
+                                       int height);
+
+/* ---- EXTENSIONS (no reference counterpart) -------------------------------- */
+
+/* Encoder knobs the reference fixes in code (its push constants,
+ * gpu_compute.c:2982-2983) or in environment variables. */
+typedef struct {
+    int search_range;  /* +-R integer full search: 16, 32 or 64 (default 64) */
+    int lambda8;       /* 0..15: cost8 = SAD8 + lambda8*|mv|, cost16 = SAD16 + 4*lambda8*|mv| (default 1) */
+    int deblock_idc;   /* disable_deblocking_filter_idc: 0 on, 1 off, 2 not across slices (default 0) */
+    int me_8x8;        /* also refine the four 8x8 quadrant MVs (default 1) */
+    int readback;      /* copy results into host staging at end_picture (default 1) */
+} gpu_encode_params_t;
+
+int gpu_compute_set_encode_params(gpu_context_t *ctx, const gpu_encode_params_t *params);
+int gpu_compute_get_encode_params(gpu_context_t *ctx, gpu_encode_params_t *params);
+
+/* The four 8x8 quadrant MVs of each MB (TL, TR, BL, BR), gpu_mv_t [nmb*4]. */
+int gpu_compute_get_mv8_staging_data_slot(gpu_context_t *ctx, int slot, void **data, size_t *size);
+
+/* Synchronously copies the newest reconstruction (deblocked; the reference
+ * picture of the next frame) into host planes of the visible size. */
+int gpu_compute_download_recon(gpu_context_t *ctx, uint8_t *y_plane, int y_pitch, uint8_t *uv_plane,
+                               int uv_pitch);
+
+/* GPU time of the stages of the frame in `slot` (ms): [0] ME, [1] intra or
+ * inter encode, [2] deblock, [3] staging copies, [4] total. */
+int gpu_compute_get_stage_times(gpu_context_t *ctx, int slot, float ms[5]);
+
+/* Nonzero when a wavefront kernel gave up waiting on a dependency (its
+ * results are then invalid); cleared by the call. */
+int gpu_compute_take_error_flags(gpu_context_t *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BC250_GPU_COMPUTE_H */

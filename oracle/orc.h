@@ -1,0 +1,104 @@
+/*
+ * orc.h - CPU oracle for the encoder compute pipeline (TEST INFRASTRUCTURE).
+ *
+ * This library is the checker, never the product: only tests/, the smoke()
+ * entry and bench.py's cpu_baseline leg may load it. It restates, in plain
+ * C, the semantics of the reference's compute stages (cited per function in
+ * the .c files) on the coded-size frame layout used by the HIP path:
+ *
+ *   luma   : wc x hc bytes, wc = 16*mb_w, hc = 16*mb_h, row pitch `py`
+ *   chroma : NV12 interleaved Cb/Cr, wc bytes x hc/2 rows, pitch `puv`
+ *   levels : int16 [num_mbs*24*16], block b of MB m at (m*24+b)*16,
+ *            blocks 0..15 luma raster (by*4+bx), 16..19 Cb, 20..23 Cr,
+ *            position p = row*4+col inside a block
+ *   dc     : int32 [num_mbs*24]   pre-quant coefficient 0 of each block
+ *   nz     : uint32 [num_mbs*24]  bit p set iff levels[p] != 0
+ *   modes  : uint32 [num_mbs]     bits 0-1 I16x16 mode, bits 2-3 chroma mode
+ *   mvs    : orc_mv_t [num_mbs]   quarter-pel MV + SAD (16 bytes each)
+ */
+#ifndef ORC_H
+#define ORC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { int32_t mvx, mvy; uint32_t sad, pad; } orc_mv_t;
+
+/* Integer full-search result for one block: MV in whole pels + its cost. */
+typedef struct { int32_t dx, dy; uint32_t cost, sad; } orc_imv_t;
+
+int  orc_chroma_qp(int qpy);
+
+/* --- 4x4 transform / quant primitives (H.264 8.5) ------------------------ */
+void orc_fwd4x4(const int32_t in[16], int32_t out[16]);
+void orc_inv4x4(const int32_t in[16], int32_t out[16]);
+int  orc_quant_ac(int c, int pos, int qp, int is_intra);
+int  orc_dequant_ac(int level, int pos, int qp);
+void orc_luma_dc_hadamard_fwd(const int32_t in[16], int32_t out[16]);
+void orc_luma_dc_hadamard_inv(const int32_t in[16], int32_t out[16]);
+void orc_chroma_dc_hadamard(const int32_t in[4], int32_t out[4]);
+int  orc_quant_dc_luma(int v, int qp);
+int  orc_quant_dc_chroma(int v, int qpc);
+int  orc_dequant_dc_luma(int f, int qp);
+int  orc_dequant_dc_chroma(int f, int qpc);
+void orc_intra16_luma_dc_levels(const int32_t dc_in[16], int qp, int32_t out[16]);
+
+/* --- interpolation (H.264 8.4.2.2) --------------------------------------- */
+int  orc_luma_qpel(const uint8_t *ref, int py, int wc, int hc, int xq, int yq);
+int  orc_chroma_8th(const uint8_t *uv, int puv, int wcc, int hcc, int comp,
+                    int x8, int y8);
+void orc_pred_luma_block(const uint8_t *ref, int py, int wc, int hc,
+                         int x0, int y0, int w, int h, int mvx, int mvy,
+                         uint8_t *dst, int dpitch);
+
+/* --- motion estimation ----------------------------------------------------- */
+void orc_me_fullsearch(const uint8_t *cur, const uint8_t *ref, int py,
+                       int wc, int hc, int range, int lambda8,
+                       orc_imv_t *int16, orc_imv_t *int8);
+void orc_me_fullsearch_mb(const uint8_t *cur, const uint8_t *ref, int py,
+                          int wc, int hc, int range, int lambda8, int mbx,
+                          int mby, orc_imv_t *int16, orc_imv_t *int8);
+void orc_me_refine(const uint8_t *cur, const uint8_t *ref, int py, int wc,
+                   int hc, const orc_imv_t *int16, const orc_imv_t *int8,
+                   orc_mv_t *mv16, orc_mv_t *mv8);
+
+/* --- macroblock encode ----------------------------------------------------- */
+void orc_h264_encode_p(const uint8_t *cy, const uint8_t *cuv,
+                       const uint8_t *ry, const uint8_t *ruv, int py, int puv,
+                       int wc, int hc, int qp, const orc_mv_t *mvs,
+                       int16_t *levels, int32_t *dc, uint32_t *nz,
+                       uint32_t *modes, uint8_t *oy, uint8_t *ouv);
+void orc_h264_encode_i(const uint8_t *cy, const uint8_t *cuv, int py, int puv,
+                       int wc, int hc, int qp, int num_slices,
+                       int16_t *levels, int32_t *dc, uint32_t *nz,
+                       uint32_t *modes, uint8_t *oy, uint8_t *ouv);
+int  orc_slice_of(int mb, int total_mbs, int num_slices);
+void orc_pred16x16(int mode, const int top[16], const int left[16], int corner, int ta, int la,
+                   uint8_t out[256]);
+void orc_pred_chroma_dc(const int top[8], const int left[8], int ta, int la, uint8_t out[64]);
+
+/* --- deblocking (H.264 8.7) ------------------------------------------------ */
+void orc_h264_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc,
+                      int qp, int is_intra, const uint32_t *nz,
+                      const orc_mv_t *mvs, int num_slices, int disable_idc);
+void orc_h264_filter_line_luma(uint8_t *q0, int step, int bs, int qp);
+void orc_h264_filter_line_chroma(uint8_t *q0, int step, int bs, int qpc);
+int  orc_h264_alpha(int index_a);
+int  orc_h264_beta(int index_b);
+int  orc_h264_tc0(int index_a, int bs);
+
+/* --- whole frame (stage order of the reference's dispatch) ----------------- */
+void orc_h264_encode_frame(const uint8_t *cy, const uint8_t *cuv,
+                           const uint8_t *ry, const uint8_t *ruv, int py,
+                           int puv, int wc, int hc, int qp, int is_intra,
+                           int num_slices, int range, int lambda8,
+                           int deblock_idc, orc_mv_t *mvs, orc_mv_t *mv8,
+                           int16_t *levels, int32_t *dc, uint32_t *nz,
+                           uint32_t *modes, uint8_t *oy, uint8_t *ouv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,470 @@
+"""Pins the CPU oracle (oracle/orc_h264.c) before it is trusted as the
+checker of the HIP path.
+
+1. Against the reference's own H.264 decoder functions, compiled from its
+   sources into oracle/_ref/libref_h264.so (h264_mc.c, h264_pred.c,
+   h264_recon.c, h264_dec_tables.c): quarter-pel / eighth-pel MC at every
+   fractional position including picture edges, Intra16x16 and chroma DC
+   prediction, and a full decode==recon check -- the oracle's levels and
+   modes are decoded with the reference's dequant / IDCT / DC transforms and
+   must rebuild exactly the oracle's reconstruction, I and P frames.
+2. Closed forms: the forward core transform equals C X C^T; the MF / V
+   tables satisfy MF*V*16 ~ 2^21 * {1, 0.64, 0.8}; inverse of a DC-only
+   block; brute-force full search on tiny inputs.
+3. The reference's regression fixture for the Intra16x16 DC transpose
+   (tests/test_encode.c:60-96 of the reference).
+4. Deblocking: tables equal the reference decoder's, hand-computed lines for
+   each filter branch, and picture-level invariants.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import orc
+
+REF = orc.REF_H264
+need_ref = pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built (reference tree absent)")
+
+_ref = None
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        lib = C.CDLL(REF)
+        P, I = C.c_void_p, C.c_int
+        lib.h264d_mc_fetch_luma.restype = P
+        lib.h264d_mc_fetch_luma.argtypes = [P, C.POINTER(I), P, I, I, I, I, I, I, I]
+        lib.h264d_mc_fetch_chroma.restype = P
+        lib.h264d_mc_fetch_chroma.argtypes = [P, C.POINTER(I), P, I, I, I, I, I, I, I]
+        lib.h264d_mc_luma.argtypes = [P, I, P, I, I, I, I, I]
+        lib.h264d_mc_chroma.argtypes = [P, I, P, I, I, I, I, I]
+        lib.h264d_pred16x16.argtypes = [P, I, I, P, P, C.c_uint8, C.c_bool, C.c_bool]
+        lib.h264d_pred_chroma.argtypes = [P, I, I, P, P, C.c_uint8, C.c_bool, C.c_bool]
+        lib.h264d_dequant_build.argtypes = [P, I, P, P]
+        lib.h264d_idct4_add.argtypes = [P, I, P, P, I, C.c_bool]
+        lib.h264d_luma_dc_transform.argtypes = [P, I, C.c_int32]
+        lib.h264d_chroma_dc_transform.argtypes = [P, I, C.c_int32]
+        _ref = lib
+    return _ref
+
+
+def p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def rand_plane(rs, h, w):
+    return rs.integers(0, 256, (h, w)).astype(np.uint8)
+
+
+# ------------------------------------------------------------ interpolation
+
+@need_ref
+def test_luma_qpel_matches_reference_mc():
+    lib = ref()
+    rs = np.random.default_rng(1)
+    plane = rand_plane(rs, 40, 48)
+    buf = np.zeros(64 * 64, np.uint8)
+    out = np.zeros((16, 16), np.uint8)
+    for trial in range(60):
+        w = int(rs.choice([4, 8, 16]))
+        h = int(rs.choice([4, 8, 16]))
+        x, y = int(rs.integers(-24, 60)), int(rs.integers(-24, 50))  # includes far outside
+        for frac in range(16):
+            fx, fy = frac & 3, frac >> 2
+            ss = C.c_int()
+            src = lib.h264d_mc_fetch_luma(p(buf), C.byref(ss), p(plane), plane.strides[0], 48, 40, x, y, w, h)
+            lib.h264d_mc_luma(p(out), 16, src, ss.value, w, h, fx, fy)
+            exp = out[:h, :w]
+            got = np.array([[orc.luma_qpel(plane, (x + i) * 4 + fx, (y + j) * 4 + fy) for i in range(w)]
+                            for j in range(h)], np.uint8)
+            np.testing.assert_array_equal(got, exp, err_msg=f"x={x} y={y} frac={frac}")
+
+
+@need_ref
+def test_chroma_eighth_pel_matches_reference_mc():
+    lib = ref()
+    rs = np.random.default_rng(2)
+    uv = rand_plane(rs, 20, 48)  # NV12 interleaved, 24 samples per component
+    buf = np.zeros(32 * 32, np.uint8)
+    out = np.zeros((8, 8), np.uint8)
+    for comp in range(2):
+        plane = np.ascontiguousarray(uv[:, comp::2])
+        for trial in range(40):
+            x, y = int(rs.integers(-12, 30)), int(rs.integers(-12, 26))
+            for fx in range(8):
+                for fy in range(8):
+                    ss = C.c_int()
+                    src = lib.h264d_mc_fetch_chroma(p(buf), C.byref(ss), p(plane), plane.strides[0], 24, 20, x, y,
+                                                    8, 8)
+                    lib.h264d_mc_chroma(p(out), 8, src, ss.value, 8, 8, fx, fy)
+                    got = np.array([[orc.lib.orc_chroma_8th(p(uv), uv.strides[0], 24, 20, comp,
+                                                            (x + i) * 8 + fx, (y + j) * 8 + fy)
+                                     for i in range(8)] for j in range(8)], np.uint8)
+                    np.testing.assert_array_equal(got, out)
+
+
+# ------------------------------------------------------------ intra prediction
+
+@need_ref
+def test_pred16x16_and_chroma_dc_match_reference():
+    lib = ref()
+    rs = np.random.default_rng(3)
+    out_r = np.zeros((16, 16), np.uint8)
+    out_o = np.zeros(256, np.uint8)
+    for trial in range(200):
+        top = rs.integers(0, 256, 16).astype(np.int32)
+        left = rs.integers(0, 256, 16).astype(np.int32)
+        corner = int(rs.integers(0, 256))
+        ta, la = bool(rs.integers(0, 2)), bool(rs.integers(0, 2))
+        if trial < 4:
+            ta, la = bool(trial & 1), bool(trial & 2)
+        t8, l8 = top.astype(np.uint8), left.astype(np.uint8)
+        for mode in range(4):
+            if (mode == 0 and not ta) or (mode == 1 and not la) or (mode == 3 and not (ta and la)):
+                continue
+            lib.h264d_pred16x16(p(out_r), 16, mode, p(t8), p(l8), corner, ta, la)
+            orc.lib.orc_pred16x16(mode, p(top), p(left), corner, int(ta), int(la), p(out_o))
+            np.testing.assert_array_equal(out_o.reshape(16, 16), out_r, err_msg=f"mode {mode} ta={ta} la={la}")
+        # chroma DC (mode 0 of h264d_pred_chroma)
+        co = np.zeros(64, np.uint8)
+        cr = np.zeros((8, 8), np.uint8)
+        lib.h264d_pred_chroma(p(cr), 8, 0, p(t8[:8].copy()), p(l8[:8].copy()), corner, ta, la)
+        orc.lib.orc_pred_chroma_dc(p(top[:8].copy()), p(left[:8].copy()), int(ta), int(la), p(co))
+        np.testing.assert_array_equal(co.reshape(8, 8), cr)
+
+
+# ------------------------------------------------------------ decode == recon
+
+class Dequant(C.Structure):
+    _fields_ = [("d4", (C.c_int32 * 16) * 6), ("d8", (C.c_int32 * 64) * 6), ("qp", C.c_int), ("valid", C.c_int)]
+
+
+def _dequant(lib, qp):
+    dq = Dequant()
+    flat4 = np.full((6, 16), 16, np.uint8)
+    flat8 = np.full((6, 64), 16, np.uint8)
+    lib.h264d_dequant_build(C.byref(dq), qp, p(flat4), p(flat8))
+    return np.array(dq.d4[0][:], np.int32)
+
+
+def _decode_mb_residual(lib, dst_y, dst_uv, mbx, mby, levels, qp, intra, dc_levels_luma):
+    """Add the residual of one MB to the prediction already in dst, the way
+    the reference decoder does (h264_recon_mb.c)."""
+    qpc = orc.lib.orc_chroma_qp(qp)
+    dq, dqc = _dequant(lib, qp), _dequant(lib, qpc)
+    lv = levels.reshape(24, 16)
+    if intra:
+        dcl = np.ascontiguousarray(dc_levels_luma, np.int16)
+        lib.h264d_luma_dc_transform(p(dcl), qp, int(dq[0]))
+    for b in range(16):
+        blk = lv[b].astype(np.int16).copy()
+        if intra:
+            blk[0] = dcl[b]
+        y0, x0 = mby * 16 + (b >> 2) * 4, mbx * 16 + (b & 3) * 4
+        sub = np.ascontiguousarray(dst_y[y0:y0 + 4, x0:x0 + 4])
+        lib.h264d_idct4_add(p(sub), 4, p(blk), p(dq), qp, intra)
+        dst_y[y0:y0 + 4, x0:x0 + 4] = sub
+    dcs = np.zeros(8, np.int32)
+    return dcs
+
+
+def _decode_chroma(lib, dst_uv, mbx, mby, levels, dc_in, qp):
+    qpc = orc.lib.orc_chroma_qp(qp)
+    dqc = _dequant(lib, qpc)
+    lv = levels.reshape(24, 16)
+    for comp in range(2):
+        raw = np.array([int(dc_in[16 + comp * 4 + s]) for s in range(4)], np.int32)
+        # transmitted chroma DC levels (host stage): 2x2 Hadamard + quantiser
+        h = np.array([raw[0] + raw[1] + raw[2] + raw[3], raw[0] - raw[1] + raw[2] - raw[3],
+                      raw[0] + raw[1] - raw[2] - raw[3], raw[0] - raw[1] - raw[2] + raw[3]])
+        cl = np.array([orc.lib.orc_quant_dc_chroma(int(v), qpc) for v in h], np.int16)
+        lib.h264d_chroma_dc_transform(p(cl), qpc, int(dqc[0]))
+        plane = np.ascontiguousarray(dst_uv[:, comp::2])
+        for s in range(4):
+            blk = lv[16 + comp * 4 + s].astype(np.int16).copy()
+            blk[0] = cl[s]
+            y0, x0 = mby * 8 + (s >> 1) * 4, mbx * 8 + (s & 1) * 4
+            sub = np.ascontiguousarray(plane[y0:y0 + 4, x0:x0 + 4])
+            lib.h264d_idct4_add(p(sub), 4, p(blk), p(dqc), qpc, True)
+            plane[y0:y0 + 4, x0:x0 + 4] = sub
+        dst_uv[:, comp::2] = plane
+
+
+def _frames(w, h, n, seed=4):
+    rs = np.random.default_rng(seed)
+    # smooth content + noise + one moving square, enough for real residuals
+    yy, xx = np.mgrid[0:h, 0:w]
+    out = []
+    for f in range(n):
+        y = 128 + 40 * np.sin((xx + 2.3 * f) / 7.0) * np.cos((yy - 1.7 * f) / 9.0)
+        y = y + rs.normal(0, 3, (h, w))
+        y[10 + f:30 + f, 20 + 2 * f:44 + 2 * f] = rs.integers(16, 236, (20, 24))
+        y = np.clip(np.rint(y), 0, 255).astype(np.uint8)
+        u = np.clip(128 + (y[0::2, 0::2].astype(int) - 128) // 4, 0, 255)
+        v = np.clip(128 - (y[1::2, 1::2].astype(int) - 128) // 5, 0, 255)
+        uv = np.empty((h // 2, w), np.uint8)
+        uv[:, 0::2], uv[:, 1::2] = u, v
+        out.append((y, uv))
+    return out
+
+
+@need_ref
+@pytest.mark.parametrize("qp", [4, 22, 27, 32, 37, 51])
+def test_decode_equals_recon_intra(qp):
+    lib = ref()
+    (cy, cuv), = _frames(64, 48, 1)
+    for ns in (1, 2):
+        e = orc.encode_i(cy, cuv, qp, ns)
+        mbw, mbh = 4, 3
+        dy = np.zeros_like(cy)
+        duv = np.zeros_like(cuv)
+        for m in range(mbw * mbh):
+            mbx, mby = m % mbw, m // mbw
+            sl = orc.lib.orc_slice_of(m, mbw * mbh, ns)
+            ta = mby > 0 and orc.lib.orc_slice_of(m - mbw, mbw * mbh, ns) == sl
+            la = mbx > 0 and orc.lib.orc_slice_of(m - 1, mbw * mbh, ns) == sl
+            y0, x0 = mby * 16, mbx * 16
+            top = dy[y0 - 1, x0:x0 + 16].copy() if mby > 0 else np.full(16, 128, np.uint8)
+            left = dy[y0:y0 + 16, x0 - 1].copy() if mbx > 0 else np.full(16, 128, np.uint8)
+            corner = int(dy[y0 - 1, x0 - 1]) if (mbx > 0 and mby > 0) else 128
+            pred = np.zeros((16, 16), np.uint8)
+            lib.h264d_pred16x16(p(pred), 16, int(e["modes"][m]), p(top), p(left), corner, ta, la)
+            dy[y0:y0 + 16, x0:x0 + 16] = pred
+            dc_in = e["dc"][m * 24:m * 24 + 16]
+            nat = np.zeros(16, np.int32)
+            orc.lib.orc_intra16_luma_dc_levels(p(np.ascontiguousarray(dc_in, np.int32)), qp, p(nat))
+            _decode_mb_residual(lib, dy, duv, mbx, mby, e["levels"][m * 384:(m + 1) * 384], qp, True, nat)
+            for comp in range(2):
+                t = duv[mby * 8 - 1, mbx * 16 + comp:mbx * 16 + 16:2].copy() if ta else np.full(8, 128, np.uint8)
+                l = duv[mby * 8:mby * 8 + 8, mbx * 16 - 2 + comp].copy() if la else np.full(8, 128, np.uint8)
+                cp = np.zeros((8, 8), np.uint8)
+                lib.h264d_pred_chroma(p(cp), 8, 0, p(np.ascontiguousarray(t)), p(np.ascontiguousarray(l)), 128, ta,
+                                      la)
+                duv[mby * 8:mby * 8 + 8, mbx * 16 + comp:mbx * 16 + 16:2] = cp
+            _decode_chroma(lib, duv, mbx, mby, e["levels"][m * 384:(m + 1) * 384], e["dc"][m * 24:(m + 1) * 24], qp)
+        np.testing.assert_array_equal(dy, e["y"], err_msg=f"luma qp={qp} ns={ns}")
+        np.testing.assert_array_equal(duv, e["uv"], err_msg=f"chroma qp={qp} ns={ns}")
+
+
+@need_ref
+@pytest.mark.parametrize("qp", [0, 18, 27, 36, 51])
+def test_decode_equals_recon_inter(qp):
+    lib = ref()
+    (ry, ruv), (cy, cuv) = _frames(64, 48, 2)
+    rs = np.random.default_rng(qp)
+    nmb = 12
+    mvs = np.zeros(nmb, orc.MV)
+    mvs["mvx"] = rs.integers(-70, 70, nmb)
+    mvs["mvy"] = rs.integers(-70, 70, nmb)
+    e = orc.encode_p(cy, cuv, ry, ruv, qp, mvs)
+    dy = np.zeros_like(cy)
+    duv = np.zeros_like(cuv)
+    buf = np.zeros(64 * 64, np.uint8)
+    for m in range(nmb):
+        mbx, mby = m % 4, m // 4
+        mx, my = int(mvs["mvx"][m]), int(mvs["mvy"][m])
+        ss = C.c_int()
+        src = lib.h264d_mc_fetch_luma(p(buf), C.byref(ss), p(ry), 64, 64, 48, mbx * 16 + (mx >> 2),
+                                      mby * 16 + (my >> 2), 16, 16)
+        pred = np.zeros((16, 16), np.uint8)
+        lib.h264d_mc_luma(p(pred), 16, src, ss.value, 16, 16, mx & 3, my & 3)
+        dy[mby * 16:mby * 16 + 16, mbx * 16:mbx * 16 + 16] = pred
+        for comp in range(2):
+            plane = np.ascontiguousarray(ruv[:, comp::2])
+            src = lib.h264d_mc_fetch_chroma(p(buf), C.byref(ss), p(plane), 32, 32, 24, mbx * 8 + (mx >> 3),
+                                            mby * 8 + (my >> 3), 8, 8)
+            cp = np.zeros((8, 8), np.uint8)
+            lib.h264d_mc_chroma(p(cp), 8, src, ss.value, 8, 8, mx & 7, my & 7)
+            duv[mby * 8:mby * 8 + 8, mbx * 16 + comp:mbx * 16 + 16:2] = cp
+        _decode_mb_residual(lib, dy, duv, mbx, mby, e["levels"][m * 384:(m + 1) * 384], qp, False, None)
+        _decode_chroma(lib, duv, mbx, mby, e["levels"][m * 384:(m + 1) * 384], e["dc"][m * 24:(m + 1) * 24], qp)
+    np.testing.assert_array_equal(dy, e["y"])
+    np.testing.assert_array_equal(duv, e["uv"])
+
+
+# ------------------------------------------------------------ closed forms
+
+def test_forward_transform_is_matrix_product():
+    Cm = np.array([[1, 1, 1, 1], [2, 1, -1, -2], [1, -1, -1, 1], [1, -2, 2, -1]])
+    rs = np.random.default_rng(5)
+    for _ in range(200):
+        x = rs.integers(-255, 256, (4, 4))
+        np.testing.assert_array_equal(orc.fwd4x4(x).reshape(4, 4), Cm @ x @ Cm.T)
+
+
+def test_inverse_of_dc_only_block():
+    for v in range(-2000, 2001, 37):
+        d = np.zeros(16, np.int32)
+        d[0] = v
+        assert (orc.inv4x4(d) == (v + 32) >> 6).all()
+
+
+def test_mf_v_design_products():
+    # MF * V * 16 is 2^21 times the squared norm ratio of the position class
+    targets = {0: 1.0, 1: 0.64, 2: 0.8}
+    pos_of_class = {0: 0, 1: 5, 2: 1}
+    for qp in range(6):
+        for cls, pos in pos_of_class.items():
+            mf = None
+            # recover MF from the quantiser: level for |c| = 2^15 at qp%6, per = 0 with f removed
+            mf = orc.lib.orc_quant_ac(1 << 15, pos, qp, 0)  # (2^15*MF + 2^15/6) >> 15 = MF
+            v16 = orc.lib.orc_dequant_ac(1 << 4, pos, qp)   # per = 0: (16 * 16V + 8) >> 4 = 16V
+            assert abs(mf * v16 / 2 ** 21 - targets[cls]) < 2e-3, (qp, cls, mf, v16)
+
+
+def test_dequant_shift_doubles_every_six_qp():
+    for qp in range(24, 46):
+        for pos in (0, 1, 5):
+            assert orc.lib.orc_dequant_ac(3, pos, qp + 6) == 2 * orc.lib.orc_dequant_ac(3, pos, qp)
+
+
+def test_chroma_qp_table_matches_reference():
+    lib = C.CDLL(REF) if os.path.exists(REF) else None
+    if lib is None:
+        pytest.skip("oracle/_ref not built")
+    t = (C.c_uint8 * 52).in_dll(lib, "h264d_chroma_qp")
+    assert [orc.lib.orc_chroma_qp(q) for q in range(52)] == list(t)
+
+
+def test_intra16_dc_transpose_fixture():
+    """Reference fixture tests/test_encode.c:60-96: levels must come out as
+    the transpose of the raw Hadamard order, and the raw order must be
+    asymmetric for the check to mean anything."""
+    dc_flat = np.array([16, 28, 40, 52, 64, 76, 88, 100, 112, 124, 136, 148, 160, 172, 184, 196], np.int32)
+    raw = np.zeros(16, np.int32)
+    orc.lib.orc_luma_dc_hadamard_fwd(p(dc_flat), p(raw))
+    pre = np.array([orc.lib.orc_quant_dc_luma(int(v), 26) for v in raw]).reshape(4, 4)
+    assert (pre != pre.T).any()
+    out = np.zeros(16, np.int32)
+    orc.lib.orc_intra16_luma_dc_levels(p(dc_flat), 26, p(out))
+    np.testing.assert_array_equal(out.reshape(4, 4), pre.T)
+
+
+def test_host_dc_stage_matches_oracle(pkg):
+    """The product's host DC stage (host/h264_host.c) against the oracle."""
+    rs = np.random.default_rng(6)
+    for qp in range(52):
+        dc_in = rs.integers(-4000, 4000, 16).astype(np.int32)
+        a = np.zeros(16, np.int32)
+        b = np.zeros(16, np.int32)
+        pre = np.zeros(16, np.int32)
+        pkg.host.h264_intra16_luma_dc_transform(p(dc_in), qp, p(a), p(pre))
+        orc.lib.orc_intra16_luma_dc_levels(p(dc_in), qp, p(b))
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(a.reshape(4, 4), pre.reshape(4, 4).T)
+        c4 = rs.integers(-3000, 3000, 4).astype(np.int32)
+        o = np.zeros(4, np.int32)
+        qpc = orc.lib.orc_chroma_qp(qp)
+        pkg.host.h264_chroma_dc_levels(p(c4), qpc, p(o))
+        h = [c4[0] + c4[1] + c4[2] + c4[3], c4[0] - c4[1] + c4[2] - c4[3], c4[0] + c4[1] - c4[2] - c4[3],
+             c4[0] - c4[1] - c4[2] + c4[3]]
+        assert list(o) == [orc.lib.orc_quant_dc_chroma(int(v), qpc) for v in h]
+        assert pkg.host.h264_chroma_qp(qp) == qpc
+
+
+def test_fullsearch_brute_force_tiny():
+    rs = np.random.default_rng(7)
+    for trial in range(6):
+        ref_p = rs.integers(0, 256, (32, 48)).astype(np.uint8)
+        cur = np.roll(ref_p, (int(rs.integers(-3, 4)), int(rs.integers(-3, 4))), axis=(0, 1))
+        cur = np.clip(cur.astype(int) + rs.integers(-3, 4, cur.shape), 0, 255).astype(np.uint8)
+        R, lam = 8, int(trial % 3)
+        i16, i8 = orc.me_fullsearch(cur, ref_p, R, lam)
+        pad = np.pad(ref_p.astype(int), R, mode="edge")
+        for m in range(6):
+            y0, x0 = (m // 3) * 16, (m % 3) * 16
+            d = np.arange(-R, R + 1)
+            l1 = np.abs(d)[:, None] + np.abs(d)[None, :]
+            sw = np.lib.stride_tricks.sliding_window_view(pad[y0:y0 + 16 + 2 * R, x0:x0 + 16 + 2 * R], (16, 16))
+            blk = cur[y0:y0 + 16, x0:x0 + 16].astype(int)
+            c16 = np.abs(sw - blk).sum(axis=(2, 3)) + 4 * lam * l1
+            k = int(np.argmin(c16))
+            assert (i16[m]["dy"], i16[m]["dx"]) == (k // (2 * R + 1) - R, k % (2 * R + 1) - R)
+            assert i16[m]["cost"] == c16.flat[k]
+            for q in range(4):
+                qy, qx = (q >> 1) * 8, (q & 1) * 8
+                c8 = np.abs(sw[:, :, qy:qy + 8, qx:qx + 8] - blk[qy:qy + 8, qx:qx + 8]).sum(axis=(2, 3)) + lam * l1
+                k8 = int(np.argmin(c8))
+                assert (i8[4 * m + q]["dy"], i8[4 * m + q]["dx"]) == (k8 // (2 * R + 1) - R, k8 % (2 * R + 1) - R)
+
+
+def test_refine_never_worse_than_integer():
+    rs = np.random.default_rng(8)
+    (ry, _), (cy, _) = _frames(64, 48, 2, seed=9)
+    i16, i8 = orc.me_fullsearch(cy, ry, 8, 1)
+    m16, m8 = orc.me_refine(cy, ry, i16, i8)
+    assert (m16["sad"] <= i16["sad"]).all()
+    assert (m8["sad"] <= i8["sad"]).all()
+    assert (np.abs(m16["mvx"] - 4 * i16["dx"]) <= 3).all()
+    # reported SAD is the SAD at the reported quarter-pel vector
+    for m in range(12):
+        x0, y0 = (m % 4) * 16, (m // 4) * 16
+        pred = np.zeros((16, 16), np.uint8)
+        orc.lib.orc_pred_luma_block(p(ry), 64, 64, 48, x0, y0, 16, 16, int(m16["mvx"][m]), int(m16["mvy"][m]),
+                                    p(pred), 16)
+        assert np.abs(pred.astype(int) - cy[y0:y0 + 16, x0:x0 + 16]).sum() == m16["sad"][m]
+
+
+# ------------------------------------------------------------ deblocking
+
+@need_ref
+def test_deblock_tables_match_reference():
+    lib = C.CDLL(REF)
+    alpha = list((C.c_uint8 * 52).in_dll(lib, "h264d_alpha"))
+    beta = list((C.c_uint8 * 52).in_dll(lib, "h264d_beta"))
+    tc0 = np.array(((C.c_uint8 * 3) * 52).in_dll(lib, "h264d_tc0")).reshape(52, 3)
+    assert [orc.lib.orc_h264_alpha(i) for i in range(52)] == alpha
+    assert [orc.lib.orc_h264_beta(i) for i in range(52)] == beta
+    mine = np.array([[orc.lib.orc_h264_tc0(i, b) for b in (1, 2, 3)] for i in range(52)])
+    np.testing.assert_array_equal(mine, tc0)
+
+
+def _line(vals, fn, bs, qp):
+    a = np.array(vals, np.uint8)
+    fn(p(a[4:]) if len(a) == 8 else p(a[2:]), 1, bs, qp)
+    return list(a)
+
+
+def test_deblock_line_kats():
+    L, Cf = orc.lib.orc_h264_filter_line_luma, orc.lib.orc_h264_filter_line_chroma
+    L.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    Cf.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    # hand-computed from H.264 8.7.2.3 / 8.7.2.4 (alpha/beta/tc0 from Tables 8-16, 8-17)
+    assert _line([60, 60, 60, 60, 70, 70, 70, 70], L, 4, 40) == [60, 61, 63, 64, 66, 68, 69, 70]
+    assert _line([50, 52, 54, 56, 64, 66, 68, 70], L, 2, 30) == [50, 52, 55, 59, 61, 65, 68, 70]
+    assert _line([54, 56, 64, 66], Cf, 2, 30) == [54, 58, 62, 66]
+    assert _line([54, 56, 64, 66], Cf, 4, 30) == [54, 58, 63, 66]
+    # |p0-q0| >= alpha: untouched;  qp below 16: alpha = 0, never filtered
+    assert _line([10, 10, 10, 10, 200, 200, 200, 200], L, 4, 40) == [10, 10, 10, 10, 200, 200, 200, 200]
+    assert _line([0, 0, 0, 0, 255, 255, 255, 255], L, 4, 51) == [0, 0, 0, 0, 255, 255, 255, 255]
+    assert _line([60, 60, 60, 60, 62, 62, 62, 62], L, 4, 15) == [60, 60, 60, 60, 62, 62, 62, 62]
+
+
+def test_deblock_picture_invariants():
+    rs = np.random.default_rng(10)
+    y = rs.integers(0, 256, (48, 64)).astype(np.uint8)
+    uv = rs.integers(0, 256, (24, 64)).astype(np.uint8)
+    nmb = 12
+    nz0 = np.zeros(nmb * 24, np.uint32)
+    mv0 = np.zeros(nmb, orc.MV)
+    # P picture, no coefficients, equal MVs: every bS is 0
+    a, b = orc.deblock(y, uv, 40, 0, nz0, mv0, 1, 0)
+    assert (a == y).all() and (b == uv).all()
+    # disable_deblocking_filter_idc = 1
+    a, b = orc.deblock(y, uv, 40, 1, nz0, mv0, 1, 1)
+    assert (a == y).all() and (b == uv).all()
+    # flat picture stays flat
+    f = np.full((48, 64), 99, np.uint8)
+    a, _ = orc.deblock(f, np.full((24, 64), 128, np.uint8), 51, 1, nz0, mv0, 1, 0)
+    assert (a == 99).all()
+    # bands of constant value per MB row: only horizontal MB edges carry a
+    # step.  With one slice per MB row, idc 2 filters none of them, idc 0 all.
+    bands = np.repeat(np.array([100, 110, 100], np.uint8), 16)[:, None].repeat(64, axis=1)
+    flat_uv = np.full((24, 64), 128, np.uint8)
+    a, _ = orc.deblock(bands, flat_uv, 40, 1, nz0, mv0, 3, 2)
+    assert (a == bands).all()
+    a, _ = orc.deblock(bands, flat_uv, 40, 1, nz0, mv0, 3, 0)
+    assert (a[13:19] != bands[13:19]).any(axis=1).all() and (a[:12] == bands[:12]).all()
