@@ -51,6 +51,11 @@ class EncodeParams(C.Structure):
                 ("me_8x8", C.c_int), ("readback", C.c_int)]
 
 
+class StageStats(C.Structure):
+    _fields_ = [("sum_ms", C.c_double * 5), ("total_ms", C.c_double), ("frames", C.c_int), ("p_frames", C.c_int),
+                ("me_int_ms_p", C.c_double)]
+
+
 class SeqCfg(C.Structure):
     _fields_ = [("qp", C.c_int), ("gop", C.c_int), ("num_slices", C.c_int), ("pipelined", C.c_int)]
 
@@ -87,6 +92,7 @@ _sig(hip, "gpu_compute_set_encode_params", _I, [_CTX, C.POINTER(EncodeParams)])
 _sig(hip, "gpu_compute_download_recon", _I, [_CTX, _P, _I, _P, _I])
 _sig(hip, "gpu_compute_get_stage_times", _I, [_CTX, _I, C.POINTER(C.c_float)])
 _sig(hip, "gpu_compute_take_error_flags", _I, [_CTX])
+_sig(hip, "gpu_compute_get_stage_stats", _I, [_CTX, C.POINTER(StageStats), _I])
 _sig(hip, "gpu_compute_get_last_latency_ms", C.c_double, [_CTX])
 
 _U = C.c_uint
@@ -239,10 +245,20 @@ class Encoder:
         return y, uv
 
     def stage_times(self, slot):
-        ms = (C.c_float * 5)()
+        ms = (C.c_float * 6)()
         if hip.gpu_compute_get_stage_times(C.byref(self.ctx), slot, ms) != 0:
             raise RuntimeError("stage times unavailable")
         return list(ms)
+
+    def stage_stats(self, reset=False):
+        st = StageStats()
+        if hip.gpu_compute_get_stage_stats(C.byref(self.ctx), C.byref(st), int(reset)) != 0:
+            raise RuntimeError("stage stats unavailable")
+        return {"sum_ms": list(st.sum_ms), "total_ms": st.total_ms, "frames": st.frames, "p_frames": st.p_frames,
+                "me_int_ms_p": st.me_int_ms_p}
+
+    def error_flags(self):
+        return hip.gpu_compute_take_error_flags(C.byref(self.ctx))
 
     def encode_sequence(self, images, qp, gop=0, num_slices=1, pipelined=1):
         n = len(images)

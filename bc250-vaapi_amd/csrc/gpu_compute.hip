@@ -23,7 +23,8 @@
 namespace {
 
 constexpr int NSLOT = 2;
-constexpr int NEV = 5;  // start, after ME, after encode, after deblock, after copies
+constexpr int NEV = 6;  // start, after integer ME, after refine, after encode, after deblock, after copies
+constexpr int NSTAGE = NEV - 1;
 constexpr unsigned kSpinLimit = 1u << 22;
 
 struct Plane2 {
@@ -61,6 +62,11 @@ struct bc250_hip_state {
     double last_latency_ms = 0.0;
     gpu_encode_params_t params{64, 1, 0, 1, 1};
     int err_flags = 0;
+    // per-stage GPU time accumulated at sync_slot: [stage] sums, frames seen
+    double stage_sum[NSTAGE] = {0, 0, 0, 0, 0};
+    double total_sum = 0;
+    int stage_frames = 0, stage_p_frames = 0;
+    double me_int_sum_p = 0;
     // map_surface mirror
     void *map_host = nullptr;
     size_t map_size = 0;
@@ -429,11 +435,12 @@ extern "C" int gpu_compute_begin_picture(gpu_context_t *ctx, gpu_image_t render_
     return 0;
 }
 
-static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &ref)
+static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &ref, hipEvent_t mid)
 {
     const gpu_encode_params_t &p = s->params;
     int r = bc250_launch_me_fullsearch((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc,
                                        p.search_range, p.lambda8, s->imv16, s->imv8, s->stream);
+    hipEventRecord(mid, s->stream);
     if (r == 0)
         r = bc250_launch_me_refine((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc, s->imv16,
                                    s->imv8, s->mvs, s->mv8, p.me_8x8, s->stream);
@@ -465,14 +472,14 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
 
     int r = 0;
     hipEventRecord(sl.ev[0], st);
-    if (!is_intra) {
-        if (cpu_mvs) {
+    if (!is_intra && !cpu_mvs) {
+        r = enqueue_me(s, rt, ref, sl.ev[1]);
+    } else {
+        hipEventRecord(sl.ev[1], st);
+        if (!is_intra)
             r = HIP_OK(hipMemcpyAsync(s->mvs, cpu_mvs, (size_t)s->nmb * 16, hipMemcpyHostToDevice, st)) ? 0 : -1;
-        } else {
-            r = enqueue_me(s, rt, ref);
-        }
     }
-    hipEventRecord(sl.ev[1], st);
+    hipEventRecord(sl.ev[2], st);
     if (r == 0) {
         if (is_intra) {
             r = bc250_launch_zero_i32(s->sync_i, s->mbh + 2, st);
@@ -488,14 +495,14 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
                                       s->nz, s->modes, st);
         }
     }
-    hipEventRecord(sl.ev[2], st);
+    hipEventRecord(sl.ev[3], st);
     if (r == 0 && p.deblock_idc != 1) {
         r = bc250_launch_zero_i32(s->sync_d, s->mbh + 2, st);
         if (r == 0)
             r = bc250_launch_deblock(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->nz, s->mvs,
                                      num_slices, p.deblock_idc, s->sync_d, kSpinLimit, st);
     }
-    hipEventRecord(sl.ev[3], st);
+    hipEventRecord(sl.ev[4], st);
     if (r == 0 && p.readback) {
         const size_t n = (size_t)s->nmb;
         bool ok = HIP_OK(hipMemcpyAsync(sl.levels, s->levels, n * 384 * 2, hipMemcpyDeviceToHost, st)) &&
@@ -542,10 +549,8 @@ extern "C" int gpu_compute_dispatch_me_only(gpu_context_t *ctx, gpu_image_t rt, 
     cur_in.py = rt.y_pitch;
     const Plane2 &ref = s->has_recon ? s->recon[s->ref_idx] : cur_in;
     hipEventRecord(sl.ev[0], s->stream);
-    int r = enqueue_me(s, rt, ref);
-    hipEventRecord(sl.ev[1], s->stream);
-    hipEventRecord(sl.ev[2], s->stream);
-    hipEventRecord(sl.ev[3], s->stream);
+    int r = enqueue_me(s, rt, ref, sl.ev[1]);
+    for (int i = 2; i < NEV - 1; i++) hipEventRecord(sl.ev[i], s->stream);
     if (r == 0 && s->params.readback) {
         bool ok = HIP_OK(hipMemcpyAsync(sl.mvs, s->mvs, (size_t)s->nmb * 16, hipMemcpyDeviceToHost, s->stream)) &&
                   HIP_OK(hipMemcpyAsync(sl.mv8, s->mv8, (size_t)s->nmb * 64, hipMemcpyDeviceToHost, s->stream));
@@ -594,7 +599,19 @@ extern "C" int gpu_compute_sync_slot(gpu_context_t *ctx, int slot)
     }
     sl.pending = false;
     float ms = 0.f;
-    if (HIP_OK(hipEventElapsedTime(&ms, sl.ev[0], sl.ev[NEV - 1]))) s->last_latency_ms = ms;
+    if (HIP_OK(hipEventElapsedTime(&ms, sl.ev[0], sl.ev[NEV - 1]))) {
+        s->last_latency_ms = ms;
+        s->total_sum += ms;
+        for (int i = 0; i < NSTAGE; i++) {
+            float t = 0.f;
+            if (HIP_OK(hipEventElapsedTime(&t, sl.ev[i], sl.ev[i + 1]))) {
+                s->stage_sum[i] += t;
+                if (i == 0 && !sl.was_intra) s->me_int_sum_p += t;
+            }
+        }
+        s->stage_frames++;
+        if (!sl.was_intra) s->stage_p_frames++;
+    }
     if (sl.err && (sl.err[0] | sl.err[1])) {
         s->err_flags |= (sl.err[0] ? 1 : 0) | (sl.err[1] ? 2 : 0);
         fprintf(stderr, "[bc250-hip] wavefront dependency wait timed out (slot %d)\n", slot);
@@ -687,15 +704,32 @@ extern "C" int gpu_compute_download_recon(gpu_context_t *ctx, uint8_t *y_plane, 
     return ok ? 0 : -1;
 }
 
-extern "C" int gpu_compute_get_stage_times(gpu_context_t *ctx, int slot, float ms[5])
+extern "C" int gpu_compute_get_stage_times(gpu_context_t *ctx, int slot, float ms[6])
 {
     bc250_hip_state *s = st_of(ctx);
     if (!s || !ms || slot < 0 || slot >= NSLOT || !s->slot[slot].recorded) return -1;
     Slot &sl = s->slot[slot];
     if (!HIP_OK(hipEventSynchronize(sl.ev[NEV - 1]))) return -1;
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < NSTAGE; i++)
         if (!HIP_OK(hipEventElapsedTime(&ms[i], sl.ev[i], sl.ev[i + 1]))) return -1;
-    return HIP_OK(hipEventElapsedTime(&ms[4], sl.ev[0], sl.ev[NEV - 1])) ? 0 : -1;
+    return HIP_OK(hipEventElapsedTime(&ms[NSTAGE], sl.ev[0], sl.ev[NEV - 1])) ? 0 : -1;
+}
+
+extern "C" int gpu_compute_get_stage_stats(gpu_context_t *ctx, gpu_stage_stats_t *st, int reset)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !st) return -1;
+    for (int i = 0; i < NSTAGE; i++) st->sum_ms[i] = s->stage_sum[i];
+    st->total_ms = s->total_sum;
+    st->frames = s->stage_frames;
+    st->p_frames = s->stage_p_frames;
+    st->me_int_ms_p = s->me_int_sum_p;
+    if (reset) {
+        for (int i = 0; i < NSTAGE; i++) s->stage_sum[i] = 0;
+        s->total_sum = s->me_int_sum_p = 0;
+        s->stage_frames = s->stage_p_frames = 0;
+    }
+    return 0;
 }
 
 extern "C" int gpu_compute_take_error_flags(gpu_context_t *ctx)

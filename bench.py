@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Benchmark of the encoder compute pipeline on one MI355X.
+
+One step = one 1920x1080 NV12 frame through the whole GPU pipeline (+-64
+integer full search on 16x16 and 8x8 blocks, quarter-pel refinement, P16x16
+or I16x16 encode, deblocking), frames already resident in HBM.  The timed
+region is an IPPP sequence of --steps frames driven by the host C sequence
+driver (host/h264_host.c) through the gpu_compute_* ABI, pipelined over the
+two staging slots.
+
+Prints ONE JSON line.  Besides the contract fields it carries the per-stage
+GPU times from HIP events recorded on the pipeline's own stream during the
+timed region, the roofline of the dominant kernel (the integer full search),
+and a CPU baseline: the oracle timed on a bounded sample of the same
+workload on this host.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "1080p HEVC fps on 1 GPU, plain vs learned-ME at ≤1% BD-rate; p99 frame ms"
+W, H = 1920, 1080
+HBM_PEAK_GBS = 8000.0
+# 256 CUs x 4 SIMD16 x 2.4 GHz x 4 byte-absdiffs per v_sad_u8 lane-op
+SAD_PEAK_T = 256 * 64 * 2.4e9 * 4 / 1e12
+
+
+def load_pkg():
+    name = "bc250_vaapi_amd"
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "bc250-vaapi_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sad_peak(pkg, torch):
+    """Sustained v_sad_u8 rate (T byte-absdiffs/s) from the calibration kernel."""
+    import ctypes as C
+    blocks, iters = 256 * 8, 64
+    out = torch.zeros(blocks * 256, dtype=torch.int32, device="cuda")
+    f = pkg.hip.bc250_calib_sad_peak
+    f.restype, f.argtypes = C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    f(out.data_ptr(), blocks, iters, None)  # warm
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    f(out.data_ptr(), blocks, iters, None)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    return blocks * 256 * iters * 128 * 4 / (ms * 1e-3) / 1e12
+
+
+def cpu_baseline(frames_coded, qp, rng, lam, budget_s):
+    """Oracle (single thread) on a bounded sample: the integer search of a
+    spread sample of macroblocks, scaled to the frame, plus sub-pel refine,
+    P encode and deblock of a whole frame."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    (ry, ruv), (cy, cuv) = frames_coded[0], frames_coded[1]
+    hc, wc = cy.shape
+    mbw, nmb = wc // 16, (wc // 16) * (hc // 16)
+    t = time.perf_counter()
+    orc.me_fullsearch_mb(cy, ry, rng, lam, mbw // 2, (hc // 16) // 2)
+    per_mb = time.perf_counter() - t
+    n = int(max(8, min(nmb, budget_s / max(per_mb, 1e-6))))
+    sample = np.linspace(0, nmb - 1, n).astype(int)
+    i16 = np.zeros(nmb, orc.IMV)
+    i8 = np.zeros(nmb * 4, orc.IMV)
+    t = time.perf_counter()
+    for m in sample:
+        a, b = orc.me_fullsearch_mb(cy, ry, rng, lam, m % mbw, m // mbw)
+        i16[m], i8[4 * m:4 * m + 4] = a, b
+    t_int = (time.perf_counter() - t) / n * nmb
+    t = time.perf_counter()
+    mv16, _ = orc.me_refine(cy, ry, i16, i8)
+    t_ref = time.perf_counter() - t
+    t = time.perf_counter()
+    e = orc.encode_p(cy, cuv, ry, ruv, qp, mv16)
+    orc.deblock(e["y"], e["uv"], qp, 0, e["nz"], mv16, 1, 0)
+    t_rest = time.perf_counter() - t
+    frame_s = t_int + t_ref + t_rest
+    return {"value": round(1.0 / frame_s, 5), "unit": "fps", "cores": 1, "kind": "port",
+            "sample": f"oracle, 1 thread: +-{rng} integer search of {n} of {nmb} MBs of one 1080p P frame "
+                      f"scaled to the frame ({t_int:.1f} s), plus sub-pel refine, P encode and deblock of the whole "
+                      f"frame ({t_ref + t_rest:.1f} s)",
+            "frame_s": round(frame_s, 2)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--qp", type=int, default=27)
+    ap.add_argument("--range", type=int, default=64)
+    ap.add_argument("--lambda8", type=int, default=1)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle integer search")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the pipeline shards by session (one GPU per group of sessions); "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    pkg = load_pkg()
+    n = args.warmup + args.steps
+    log(f"generating {n} frames {W}x{H}")
+    g = pkg.FrameGen(1080, W, H, 12)
+    frames = [g.frame(i) for i in range(n)]
+    enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, me_8x8=1, readback=0)
+    images = []
+    for y, uv in frames:
+        im = enc.new_image()
+        enc.upload(im, y, uv)
+        images.append(im)
+    log("warmup")
+    enc.encode_sequence(images[:args.warmup], args.qp, gop=0, pipelined=1)
+    enc.stage_stats(reset=True)
+
+    log(f"timed: {args.steps} frames IPPP")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    frame_ms, _ = enc.encode_sequence(images[args.warmup:], args.qp, gop=0, pipelined=1)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = enc.stage_stats(reset=True)
+    errs = enc.error_flags()
+
+    wc, hc = (W + 15) // 16 * 16, (H + 15) // 16 * 16
+    nmb = (wc // 16) * (hc // 16)
+    fps = args.steps / wall
+    p_frames = max(st["p_frames"], 1)
+    me_ms = st["me_int_ms_p"] / p_frames  # average integer-search kernel time per launch
+    stage_names = ["me_integer", "me_subpel", "encode", "deblock", "staging"]
+    stage_ms = {k: round(st["sum_ms"][i] / max(st["frames"], 1), 4) for i, k in enumerate(stage_names)}
+    cand = (2 * args.range + 1) ** 2
+    sad_ops = cand * nmb * 256                 # byte-absdiffs per launch
+    hbm_bytes = 2 * wc * hc + nmb * 16 * 5     # cur + ref luma read, 5 MV records per MB written
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "me_fullsearch_1080p_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    peak_meas = sad_peak(pkg, torch)
+    res = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "fps",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain full-search pipeline, 1920x1080 NV12, {args.steps} frames IPPP, CQP {args.qp}, "
+                        f"+-{args.range} integer full search on 16x16 and 8x8 + quarter-pel refine, "
+                        "P16x16 / I16x16 encode with 4x4 integer transform and quant, H.264 8.7 deblock "
+                        "(the reference's gpu_compute pipeline, H.264 syntax)",
+            "width": W, "height": H, "frames": args.steps, "qp": args.qp, "search_range": args.range,
+            "lambda8": args.lambda8, "seed": 1080, "rectangles": 12,
+        },
+        "p50_frame_ms": round(float(np.percentile(frame_ms, 50)), 4),
+        "p99_frame_ms": round(float(np.percentile(frame_ms, 99)), 4),
+        "mpixel_per_s": round(fps * W * H / 1e6, 1),
+        "stage_ms": stage_ms,
+        "wavefront_wait_errors": errs,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "me_fullsearch_kernel<64>",
+            "achieved": round(hbm_bytes / (me_ms * 1e-3) / 1e9, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(hbm_bytes / (me_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
+            "traffic": traffic,
+            "kernel_ms": round(me_ms, 4),
+            "algorithmic_bytes": hbm_bytes,
+        },
+        "roofline_valu": {
+            "bound": "valu (v_sad_u8)",
+            "achieved": round(sad_ops / (me_ms * 1e-3) / 1e12, 3),
+            "peak": round(SAD_PEAK_T, 1),
+            "peak_measured": round(peak_meas, 1),
+            "unit": "T byte-absdiff/s",
+            "frac": round(sad_ops / (me_ms * 1e-3) / 1e12 / SAD_PEAK_T, 4),
+            "algorithmic_ops": sad_ops,
+        },
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle)")
+        coded = [pkg.pad_coded(*frames[args.warmup]), pkg.pad_coded(*frames[args.warmup + 1])]
+        res["cpu_baseline"] = cpu_baseline(coded, args.qp, args.range, args.lambda8, args.cpu_budget)
+    else:
+        res["cpu_baseline"] = None
+    enc.close()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

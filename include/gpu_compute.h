@@ -186,9 +186,21 @@ int gpu_compute_get_mv8_staging_data_slot(gpu_context_t *ctx, int slot, void **d
 int gpu_compute_download_recon(gpu_context_t *ctx, uint8_t *y_plane, int y_pitch, uint8_t *uv_plane,
                                int uv_pitch);
 
-/* GPU time of the stages of the frame in `slot` (ms): [0] ME, [1] intra or
- * inter encode, [2] deblock, [3] staging copies, [4] total. */
-int gpu_compute_get_stage_times(gpu_context_t *ctx, int slot, float ms[5]);
+/* GPU time of the stages of the frame in `slot` (ms), from HIP events on
+ * the context's stream: [0] integer search, [1] sub-pel refine, [2] intra
+ * or inter encode, [3] deblock, [4] staging copies, [5] total. */
+int gpu_compute_get_stage_times(gpu_context_t *ctx, int slot, float ms[6]);
+
+/* Stage times summed over every frame synced since the last reset (the
+ * reference's BC250_PERF_STATS timestamps, gpu_compute.c:3684-3708). */
+typedef struct {
+    double sum_ms[5];     /* stages as in gpu_compute_get_stage_times */
+    double total_ms;
+    int frames;
+    int p_frames;
+    double me_int_ms_p;   /* integer-search time summed over P frames */
+} gpu_stage_stats_t;
+int gpu_compute_get_stage_stats(gpu_context_t *ctx, gpu_stage_stats_t *stats, int reset);
 
 /* Nonzero when a wavefront kernel gave up waiting on a dependency (its
  * results are then invalid); cleared by the call. */
