@@ -104,6 +104,8 @@ _sig(hip, "bc250_launch_i_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _I, _I, 
                                        _P])
 _sig(hip, "bc250_launch_deblock", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _U, _P])
 _sig(hip, "bc250_launch_zero_i32", _I, [_P, _I, _P])
+_sig(hip, "bc250_deblock_work_bytes", C.c_size_t, [_I, _I])
+_sig(hip, "bc250_intra_work_bytes", C.c_size_t, [_I, _I])
 _sig(hip, "bc250_launch_pad_edges", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P])
 
 _sig(host, "bc250_gen_create", _P, [C.c_uint64, _I, _I, _I])

@@ -258,6 +258,23 @@ BC_DEV void wait_at_least(const int *p, int want, int *err, unsigned limit)
     }
 }
 
+// Orders one wave's LDS traffic around a hand-off: LDS executes a wave's
+// instructions in order, the clobber stops the compiler from moving them.
+BC_DEV void wave_lds_sync()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// System-coherent (sc0 sc1) accesses for hand-offs between workgroups that
+// may run on different XCDs: they bypass the non-coherent caches, so no L2
+// writeback / invalidate is needed around them.
+BC_DEV uint32_t ld_sys(const uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); }
+BC_DEV void st_sys(uint32_t *p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); }
+
+// Waits until every store this wave issued has completed.
+BC_DEV void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // Slice of MB `mb` for start(s) = floor(s*total/ns).
 BC_DEV int slice_of(int mb, int total, int ns)
 {

@@ -174,7 +174,8 @@ static int ensure_res(bc250_hip_state *s, int w, int h)
          HIP_OK(hipMalloc(&s->nz, (size_t)nmb * 24 * 4)) && HIP_OK(hipMalloc(&s->modes, (size_t)nmb * 4)) &&
          HIP_OK(hipMalloc(&s->mvs, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->mv8, (size_t)nmb * 64)) &&
          HIP_OK(hipMalloc(&s->imv16, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->imv8, (size_t)nmb * 64)) &&
-         HIP_OK(hipMalloc(&s->sync_i, (size_t)(mbh + 2) * 4)) && HIP_OK(hipMalloc(&s->sync_d, (size_t)(mbh + 2) * 4));
+         HIP_OK(hipMalloc(&s->sync_i, bc250_intra_work_bytes(wc, hc))) &&
+         HIP_OK(hipMalloc(&s->sync_d, bc250_deblock_work_bytes(wc, hc)));
     for (auto &sl : s->slot) {
         ok = ok && HIP_OK(hipHostMalloc((void **)&sl.levels, (size_t)nmb * 384 * 2, hipHostMallocDefault)) &&
              HIP_OK(hipHostMalloc((void **)&sl.dc, (size_t)nmb * 24 * 4, hipHostMallocDefault)) &&
@@ -497,7 +498,7 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     }
     hipEventRecord(sl.ev[3], st);
     if (r == 0 && p.deblock_idc != 1) {
-        r = bc250_launch_zero_i32(s->sync_d, s->mbh + 2, st);
+        r = bc250_launch_zero_i32(s->sync_d, 2 * s->mbh + 2, st);
         if (r == 0)
             r = bc250_launch_deblock(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->nz, s->mvs,
                                      num_slices, p.deblock_idc, s->sync_d, kSpinLimit, st);

@@ -1,20 +1,29 @@
 // h264_deblock.hip - H.264 8.7 deblocking (luma + chroma, bS 1..4 incl. the
-// strong filter) as ONE persistent wavefront kernel for gfx950.
+// strong filter) for gfx950: a fully parallel boundary-strength kernel, then
+// ONE persistent wavefront kernel that filters in the standard's order.
 //
 // The standard filters macroblocks in raster order, each MB's vertical
 // edges then its horizontal edges, on samples already modified by earlier
-// MBs.  MB (x,y) reads/writes samples that MB (x+1,y-1) modified last (its
-// left edge touches the right columns of MB (x,y-1)), so the exact order
-// is reproduced by a wavefront where row y runs two MBs behind row y-1.
-// Each workgroup is one wave that takes a row from an atomic ticket and keeps
-// a sliding tile of the previous and current MB (+4 rows above) in LDS:
-//   luma tile  [20][32]: rows Y0-4..Y0+15, cols X0-16..X0+15
-//   chroma     [2][10][16]: rows V0-2..V0+7, cols U0-8..U0+7
-// Step x filters MB x in LDS, writes back MB x-1 (now final for this row)
-// and the rows of MB (x,y-1) its top edge changed, then publishes
-// progress[y] = x (MBs 0..x-1 final).  Row y+1 starts MB x once
-// progress[y] >= x+1, i.e. after MB x+1 of row y has filtered its left edge.
-// Waits are bounded and report on *err.
+// MBs.  MB (x,y) reads samples that MB (x+1,y-1) modified last (its left
+// edge touches the right columns of MB (x,y-1)), so the exact order is a
+// wavefront in which row y runs two MBs behind row y-1.  Luma and chroma
+// never read each other, so they run as two independent wavefronts.
+//
+// Work layout: a workgroup is 8 waves for 4 MB rows (row group from an
+// atomic ticket): waves 0..3 filter the luma of rows 4g..4g+3, waves 4..7
+// their chroma.  A wave keeps a sliding LDS tile (previous MB, current MB,
+// rows above).  Per MB it makes two passes: one lane per line through the
+// four vertical edges in registers, then one lane per column through the
+// four horizontal edges.  When MB x-1 is final for its row (after MB x
+// filtered its left edge) the wave
+//   - hands the bottom rows that the next row's top edge reads and partly
+//     rewrites (luma 12..15, chroma 6..7) to the wave below: through an LDS
+//     ring inside the workgroup, or a system-coherent record in global memory
+//     to the next workgroup, published after an s_waitcnt,
+//   - stores the other rows to the frame; the wave below stores the rows it
+//     rewrote (luma 13..15, chroma 7), so each frame byte has one writer.
+// No cache writeback or invalidate is issued.  Waits are bounded and report
+// on *err.
 #include "bc250_dev.h"
 #include "bc250_stages.h"
 
@@ -35,203 +44,362 @@ __constant__ uint8_t kTc0[52][3] = {
     {4, 5, 7}, {4, 5, 8}, {4, 6, 9}, {5, 7, 10}, {6, 8, 11}, {6, 8, 13}, {7, 10, 14}, {8, 11, 16},
     {9, 12, 18}, {10, 13, 20}, {11, 15, 23}, {13, 17, 25}};
 
-constexpr int LP = 32;  // luma tile pitch
-constexpr int CP = 16;  // chroma tile pitch
+constexpr int ROWS = 4;        // MB rows per workgroup
+constexpr int LP = 32;         // luma tile pitch: cols 0..15 MB x-1, 16..31 MB x
+constexpr int CP = 16;         // chroma tile pitch: cols 0..7 MB x-1, 8..15 MB x
+constexpr int RING = 16;       // LDS hand-off slots per producing wave
+constexpr int LHAND = 16;      // luma hand-off dwords: rows 12..15, 16 bytes each
+constexpr int CHAND = 8;       // chroma hand-off dwords: Cb rows 6..7, Cr rows 6..7 (8 bytes each)
+constexpr int GHAND = LHAND + CHAND;
 
-__device__ __forceinline__ void luma_line(uint8_t *q, int d, int bs, int qp)
+// v[0..7] = p3 p2 p1 p0 q0 q1 q2 q3; filters in place.
+__device__ __forceinline__ void luma_filter(int *v, int bs, int alpha, int beta, int tc0)
 {
-    const int alpha = kAlpha[qp], beta = kBeta[qp];
-    int p0 = q[-d], p1 = q[-2 * d], p2 = q[-3 * d], p3 = q[-4 * d];
-    int q0 = q[0], q1 = q[d], q2 = q[2 * d], q3 = q[3 * d];
+    const int p3 = v[0], p2 = v[1], p1 = v[2], p0 = v[3], q0 = v[4], q1 = v[5], q2 = v[6], q3 = v[7];
     if (!(iabs(p0 - q0) < alpha && iabs(p1 - p0) < beta && iabs(q1 - q0) < beta)) return;
-    int ap = iabs(p2 - p0), aq = iabs(q2 - q0);
+    const int ap = iabs(p2 - p0), aq = iabs(q2 - q0);
     if (bs < 4) {
-        int tc0 = kTc0[qp][bs - 1];
-        int tc = tc0 + (ap < beta) + (aq < beta);
-        int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
-        q[-d] = (uint8_t)clip255(p0 + delta);
-        q[0] = (uint8_t)clip255(q0 - delta);
-        if (ap < beta) q[-2 * d] = (uint8_t)(p1 + clampi((p2 + ((p0 + q0 + 1) >> 1) - (p1 << 1)) >> 1, -tc0, tc0));
-        if (aq < beta) q[d] = (uint8_t)(q1 + clampi((q2 + ((p0 + q0 + 1) >> 1) - (q1 << 1)) >> 1, -tc0, tc0));
+        const int tc = tc0 + (ap < beta) + (aq < beta);
+        const int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+        v[3] = clip255(p0 + delta);
+        v[4] = clip255(q0 - delta);
+        if (ap < beta) v[2] = p1 + clampi((p2 + ((p0 + q0 + 1) >> 1) - (p1 << 1)) >> 1, -tc0, tc0);
+        if (aq < beta) v[5] = q1 + clampi((q2 + ((p0 + q0 + 1) >> 1) - (q1 << 1)) >> 1, -tc0, tc0);
     } else {
         const bool narrow = iabs(p0 - q0) < ((alpha >> 2) + 2);
         if (ap < beta && narrow) {
-            q[-d] = (uint8_t)((p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3);
-            q[-2 * d] = (uint8_t)((p2 + p1 + p0 + q0 + 2) >> 2);
-            q[-3 * d] = (uint8_t)((2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3);
+            v[3] = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3;
+            v[2] = (p2 + p1 + p0 + q0 + 2) >> 2;
+            v[1] = (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3;
         } else {
-            q[-d] = (uint8_t)((2 * p1 + p0 + q1 + 2) >> 2);
+            v[3] = (2 * p1 + p0 + q1 + 2) >> 2;
         }
         if (aq < beta && narrow) {
-            q[0] = (uint8_t)((p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3);
-            q[d] = (uint8_t)((p0 + q0 + q1 + q2 + 2) >> 2);
-            q[2 * d] = (uint8_t)((2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3);
+            v[4] = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3;
+            v[5] = (p0 + q0 + q1 + q2 + 2) >> 2;
+            v[6] = (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3;
         } else {
-            q[0] = (uint8_t)((2 * q1 + q0 + p1 + 2) >> 2);
+            v[4] = (2 * q1 + q0 + p1 + 2) >> 2;
         }
     }
 }
 
-__device__ __forceinline__ void chroma_line(uint8_t *q, int d, int bs, int qpc)
+// v[0..3] = p1 p0 q0 q1 (chromaEdgeFlag = 1).
+__device__ __forceinline__ void chroma_filter(int *v, int bs, int alpha, int beta, int tc0)
 {
-    const int alpha = kAlpha[qpc], beta = kBeta[qpc];
-    int p0 = q[-d], p1 = q[-2 * d], q0 = q[0], q1 = q[d];
+    const int p1 = v[0], p0 = v[1], q0 = v[2], q1 = v[3];
     if (!(iabs(p0 - q0) < alpha && iabs(p1 - p0) < beta && iabs(q1 - q0) < beta)) return;
     if (bs < 4) {
-        int tc = kTc0[qpc][bs - 1] + 1;
-        int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
-        q[-d] = (uint8_t)clip255(p0 + delta);
-        q[0] = (uint8_t)clip255(q0 - delta);
+        const int tc = tc0 + 1;
+        const int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+        v[1] = clip255(p0 + delta);
+        v[2] = clip255(q0 - delta);
     } else {
-        q[-d] = (uint8_t)((2 * p1 + p0 + q1 + 2) >> 2);
-        q[0] = (uint8_t)((2 * q1 + q0 + p1 + 2) >> 2);
+        v[1] = (2 * p1 + p0 + q1 + 2) >> 2;
+        v[2] = (2 * q1 + q0 + p1 + 2) >> 2;
     }
 }
+
+// bS words: per direction two words, edges 0..1 then 2..3, 4 bits per (edge, segment).
+__device__ __forceinline__ int bs_get(uint32_t lo, uint32_t hi, int e, int i)
+{
+    return ((e < 2 ? lo : hi) >> (4 * ((e & 1) * 4 + i))) & 15;
+}
+
+struct alignas(16) LumaLds {
+    uint8_t T[20 * LP];  // rows 0..3 above, 4..19 body
+};
+struct alignas(16) ChromaLds {
+    uint8_t T[2][10 * CP];  // [Cb, Cr] rows 0..1 above, 2..9 body
+};
 }  // namespace
 
-__global__ __launch_bounds__(64) void h264_deblock_kernel(
-    uint8_t *__restrict__ y_, uint8_t *__restrict__ uv, int py, int puv, int mbw, int mbh, int qp,
-    int is_intra, const uint32_t *__restrict__ nz, const uint4 *__restrict__ mvs, int num_slices, int idc,
-    int *sync /* [0] ticket, [1] err, [2..] progress */, unsigned spin_limit)
+// ---------------------------------------------------------------------------
+// Boundary strengths (H.264 8.7.2.1), every MB in parallel.  bs[m] = {vertical
+// edges 0..1, vertical edges 2..3, horizontal edges 0..1, horizontal edges
+// 2..3}, 4 bits per (edge e, segment i) at bit 4*(4*(e&1)+i) of its word.
+// Edges that are not filtered (picture border, slice border with idc 2) get 0.
+__global__ void h264_bs_kernel(int mbw, int mbh, int is_intra, const uint32_t *__restrict__ nz,
+                               const uint4 *__restrict__ mvs, int num_slices, int idc, uint4 *__restrict__ bs)
 {
-    __shared__ uint8_t L[20 * LP];
-    __shared__ uint8_t C[2][10 * CP];
-    __shared__ int bsv[2][4][4];  // [dir][edge][segment]
-    __shared__ int s_row;
+    const int m = blockIdx.x * 8 + (threadIdx.x >> 5);
+    const int j = threadIdx.x & 31;
+    const int nmb = mbw * mbh;
+    if (m >= nmb) return;
+    const int x = m % mbw, y = m / mbw;
+    const int dir = j >> 4, e = (j >> 2) & 3, i = j & 3;
+    int b = 0;
+    bool exists = true;
+    if (e == 0) {
+        const int nb = dir == 0 ? m - 1 : m - mbw;
+        exists = dir == 0 ? x > 0 : y > 0;
+        if (exists && idc == 2 && slice_of(nb, nmb, num_slices) != slice_of(m, nmb, num_slices)) exists = false;
+    }
+    if (exists) {
+        if (is_intra) {
+            b = e == 0 ? 4 : 3;
+        } else {
+            const int pm = e ? m : (dir == 0 ? m - 1 : m - mbw);
+            const int bq = dir == 0 ? i * 4 + e : e * 4 + i;
+            const int bp = dir == 0 ? (e ? i * 4 + e - 1 : i * 4 + 3) : (e ? (e - 1) * 4 + i : 12 + i);
+            if (nz[pm * 24 + bp] || nz[m * 24 + bq]) {
+                b = 2;
+            } else {
+                const uint4 a = mvs[pm], c = mvs[m];
+                if (iabs((int)a.x - (int)c.x) >= 4 || iabs((int)a.y - (int)c.y) >= 4) b = 1;
+            }
+        }
+    }
+    // pack 8 nibbles per word with a reduction over the 8 lanes of (dir, e>>1)
+    uint32_t word = (uint32_t)b << (4 * ((e & 1) * 4 + i));
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) word |= __shfl_xor(word, o, 64);
+    uint32_t *out = (uint32_t *)&bs[m];
+    if ((j & 7) == 0) out[j >> 3] = word;
+}
 
-    const int lane = threadIdx.x;
-    int *ticket = sync, *err = sync + 1, *progress = sync + 2;
-    if (lane == 0) s_row = atomicAdd(ticket, 1);
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void h264_deblock_kernel(
+    uint8_t *__restrict__ y_, uint8_t *__restrict__ uv, int py, int puv, int mbw, int mbh, int qp,
+    const uint4 *__restrict__ bsw, int *sync /* [0] ticket, [1] err, [2..2+mbh) luma, [..+mbh) chroma */,
+    uint32_t *ghand /* [mbh][mbw][GHAND] */, unsigned spin_limit)
+{
+    __shared__ LumaLds ll[ROWS];
+    __shared__ ChromaLds cl[ROWS];
+    __shared__ uint32_t lring[ROWS - 1][RING][LHAND];
+    __shared__ uint32_t cring[ROWS - 1][RING][CHAND];
+    __shared__ int prog[2][ROWS], cons[2][ROWS];
+    __shared__ int s_group;
+
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int chroma = wv >= ROWS, r4 = wv & (ROWS - 1);
+    int *err = sync + 1, *gprog = sync + 2 + (chroma ? mbh : 0);
+    if (tid == 0) s_group = atomicAdd(sync, 1);
+    if (tid < 2 * ROWS) { prog[tid >> 2][tid & 3] = 0; cons[tid >> 2][tid & 3] = 0; }
     __syncthreads();
-    const int yrow = s_row;
+    const int yrow = s_group * ROWS + r4;
     if (yrow >= mbh) return;
-    const int total = mbw * mbh;
-    const int qpc = chroma_qp(qp);
+    volatile int *vprog = prog[chroma], *vcons = cons[chroma];
+    const bool last_row = yrow == mbh - 1;
+    const bool lds_below = r4 < ROWS - 1 && !last_row;
+    const bool lds_above = r4 > 0;
+    const int qpx = chroma ? chroma_qp(qp) : qp;
+    const int alpha = kAlpha[qpx], beta = kBeta[qpx];
     const int Y0 = yrow * 16, V0 = yrow * 8;
+    const int nhand = chroma ? CHAND : LHAND;
+    uint32_t *gh = ghand + (chroma ? LHAND : 0);
+    uint8_t *LT = ll[r4].T;
+    uint8_t(*CT)[10 * CP] = cl[r4].T;
+
+    // one-MB-ahead prefetch: body (luma: 1 dword per lane; chroma: lanes 0..31) and bS words
+    uint32_t pre = 0;
+    uint4 pbs = make_uint4(0, 0, 0, 0);
+    auto prefetch = [&](int x) {
+        if (!chroma) pre = *(const uint32_t *)(y_ + (size_t)(Y0 + (lane >> 2)) * py + x * 16 + (lane & 3) * 4);
+        else if (lane < 32) pre = *(const uint32_t *)(uv + (size_t)(V0 + (lane >> 2)) * puv + x * 16 + (lane & 3) * 4);
+        pbs = bsw[yrow * mbw + x];
+    };
+    prefetch(0);
 
     for (int x = 0; x <= mbw; x++) {
-        const int X0 = x * 16, U0 = x * 8;
+        const int X0 = x * 16;
+        const uint4 bs = pbs;
         if (x < mbw) {
-            const int m = yrow * mbw + x;
+            // ---- body of MB x into the tile
+            if (!chroma) {
+                *(uint32_t *)&LT[(4 + (lane >> 2)) * LP + 16 + (lane & 3) * 4] = pre;
+            } else if (lane < 32) {
+                const int r = lane >> 2, c = (lane & 3) * 2;
+                CT[0][(2 + r) * CP + 8 + c] = (uint8_t)pre;
+                CT[1][(2 + r) * CP + 8 + c] = (uint8_t)(pre >> 8);
+                CT[0][(2 + r) * CP + 9 + c] = (uint8_t)(pre >> 16);
+                CT[1][(2 + r) * CP + 9 + c] = (uint8_t)(pre >> 24);
+            }
+            // ---- rows above: MB (x, y-1) must be final; take its hand-off
             if (yrow > 0) {
-                if (lane == 0) wait_at_least(&progress[yrow - 1], x + 1, err, spin_limit);
-                __syncthreads();
-                __threadfence();
-            }
-            const int sl = slice_of(m, total, num_slices);
-            const bool skip_left = x == 0 || (idc == 2 && slice_of(m - 1, total, num_slices) != sl);
-            const bool skip_top = yrow == 0 || (idc == 2 && slice_of(m - mbw, total, num_slices) != sl);
-
-            // ---- load MB x body (+ 4 rows / 2 chroma rows above)
-            {
-                const int r = lane >> 2, c = (lane & 3) * 4;
-                *(uint32_t *)&L[(4 + r) * LP + 16 + c] = *(const uint32_t *)(y_ + (size_t)(Y0 + r) * py + X0 + c);
-                if (lane < 16 && yrow > 0)
-                    *(uint32_t *)&L[r * LP + 16 + c] = *(const uint32_t *)(y_ + (size_t)(Y0 - 4 + r) * py + X0 + c);
-                if (lane < 40) {
-                    // chroma: 8 body rows + 2 rows above, 4 dwords (2 Cb/Cr pairs each) per row
-                    const int cr = lane >> 2, cc = (lane & 3) * 2;  // tile row 0..9, chroma col 0..7 step 2
-                    if (cr >= 2 || yrow > 0) {
-                        uint32_t v = *(const uint32_t *)(uv + (size_t)(V0 - 2 + cr) * puv + 2 * (U0 + cc));
-                        C[0][cr * CP + 8 + cc] = (uint8_t)v;
-                        C[1][cr * CP + 8 + cc] = (uint8_t)(v >> 8);
-                        C[0][cr * CP + 9 + cc] = (uint8_t)(v >> 16);
-                        C[1][cr * CP + 9 + cc] = (uint8_t)(v >> 24);
+                uint32_t h = 0;
+                if (lds_above) {
+                    unsigned n = 0;
+                    while (vprog[r4 - 1] < x + 1) {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++n > spin_limit) { atomicOr(err, 1); break; }
                     }
-                }
-            }
-            // ---- boundary strengths (H.264 8.7.2.1)
-            if (lane < 32) {
-                const int dir = lane >> 4, e = (lane >> 2) & 3, i = lane & 3;
-                int bs = 0;
-                if (is_intra) {
-                    bs = e == 0 ? 4 : 3;
+                    wave_lds_sync();
+                    if (lane < nhand) h = chroma ? cring[r4 - 1][x % RING][lane] : lring[r4 - 1][x % RING][lane];
+                    wave_lds_sync();
+                    if (lane == 0) vcons[r4] = x + 1;
                 } else {
-                    const int pm = e ? m : (dir == 0 ? m - 1 : m - mbw);
-                    const int bq = dir == 0 ? i * 4 + e : e * 4 + i;
-                    const int bp = dir == 0 ? (e ? i * 4 + e - 1 : i * 4 + 3) : (e ? (e - 1) * 4 + i : 12 + i);
-                    const bool edge_exists = e != 0 || (dir == 0 ? x > 0 : yrow > 0);
-                    if (edge_exists) {
-                        if (nz[pm * 24 + bp] || nz[m * 24 + bq]) bs = 2;
-                        else {
-                            uint4 a = mvs[pm], b = mvs[m];
-                            if (iabs((int)a.x - (int)b.x) >= 4 || iabs((int)a.y - (int)b.y) >= 4) bs = 1;
+                    if (lane == 0) {
+                        unsigned n = 0;
+                        while ((int)ld_sys((const uint32_t *)&gprog[yrow - 1]) < x + 1) {
+                            __builtin_amdgcn_s_sleep(1);
+                            if (++n > spin_limit) { atomicOr(err, 1); break; }
                         }
                     }
+                    __builtin_amdgcn_wave_barrier();
+                    asm volatile("" ::: "memory");
+                    if (lane < nhand) h = ld_sys(gh + ((size_t)(yrow - 1) * mbw + x) * GHAND + lane);
                 }
-                bsv[dir][e][i] = bs;
+                if (!chroma) {
+                    if (lane < LHAND) *(uint32_t *)&LT[(lane >> 2) * LP + 16 + (lane & 3) * 4] = h;
+                } else if (lane < CHAND) {
+                    const int comp = lane >> 2, r = (lane >> 1) & 1, c = (lane & 1) * 4;
+                    *(uint32_t *)&CT[comp][r * CP + 8 + c] = h;
+                }
             }
-            __syncthreads();
+            if (x + 1 < mbw) prefetch(x + 1);
+            wave_lds_sync();
 
-            // ---- vertical edges, left to right
-            for (int e = 0; e < 4; e++) {
-                if (!(e == 0 && skip_left)) {
-                    if (lane < 16) {
-                        int bs = bsv[0][e][lane >> 2];
-                        if (bs) luma_line(&L[(4 + lane) * LP + 16 + 4 * e], 1, bs, qp);
-                    } else if (lane < 32 && (e == 0 || e == 2)) {
-                        const int comp = (lane - 16) >> 3, k = (lane - 16) & 7;
-                        int bs = bsv[0][e][k >> 1];
-                        if (bs) chroma_line(&C[comp][(2 + k) * CP + 8 + 2 * e], 1, bs, qpc);
+            if (!chroma) {
+                // ---- vertical edges: lane = line, tile cols 12..31 in registers
+                if (lane < 16) {
+                    uint8_t *row = &LT[(4 + lane) * LP + 12];
+                    int v[20];
+#pragma unroll
+                    for (int d = 0; d < 5; d++) {
+                        const uint32_t wd = *(const uint32_t *)(row + 4 * d);
+#pragma unroll
+                        for (int k = 0; k < 4; k++) v[4 * d + k] = (wd >> (8 * k)) & 255;
                     }
-                }
-                __syncthreads();
-            }
-            // ---- horizontal edges, top to bottom
-            for (int e = 0; e < 4; e++) {
-                if (!(e == 0 && skip_top)) {
-                    if (lane < 16) {
-                        int bs = bsv[1][e][lane >> 2];
-                        if (bs) luma_line(&L[(4 + 4 * e) * LP + 16 + lane], LP, bs, qp);
-                    } else if (lane < 32 && (e == 0 || e == 2)) {
-                        const int comp = (lane - 16) >> 3, k = (lane - 16) & 7;
-                        int bs = bsv[1][e][k >> 1];
-                        if (bs) chroma_line(&C[comp][(2 + 2 * e) * CP + 8 + k], CP, bs, qpc);
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int b = bs_get(bs.x, bs.y, e, lane >> 2);
+                        if (b) luma_filter(&v[4 * e], b, alpha, beta, b < 4 ? kTc0[qpx][b - 1] : 0);
                     }
+#pragma unroll
+                    for (int d = 0; d < 5; d++)
+                        *(uint32_t *)(row + 4 * d) = (uint32_t)v[4 * d] | ((uint32_t)v[4 * d + 1] << 8) |
+                                                     ((uint32_t)v[4 * d + 2] << 16) | ((uint32_t)v[4 * d + 3] << 24);
                 }
-                __syncthreads();
+                wave_lds_sync();
+                // ---- horizontal edges: lane = column, tile rows 0..19 in registers
+                if (lane < 16) {
+                    uint8_t *col = &LT[16 + lane];
+                    int v[20];
+#pragma unroll
+                    for (int k = 0; k < 20; k++) v[k] = col[k * LP];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int b = bs_get(bs.z, bs.w, e, lane >> 2);
+                        if (b) luma_filter(&v[4 * e], b, alpha, beta, b < 4 ? kTc0[qpx][b - 1] : 0);
+                    }
+#pragma unroll
+                    for (int k = 1; k < 19; k++) col[k * LP] = (uint8_t)v[k];
+                }
+            } else {
+                // ---- chroma: lane = (component, line), tile cols 4..15
+                if (lane < 16) {
+                    const int comp = lane >> 3, k = lane & 7;
+                    uint8_t *row = &CT[comp][(2 + k) * CP + 4];
+                    int v[12];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const uint32_t wd = *(const uint32_t *)(row + 4 * d);
+#pragma unroll
+                        for (int q = 0; q < 4; q++) v[4 * d + q] = (wd >> (8 * q)) & 255;
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; e += 2) {
+                        const int b = bs_get(bs.x, bs.y, e, k >> 1);
+                        if (b) chroma_filter(&v[2 + 2 * e], b, alpha, beta, b < 4 ? kTc0[qpx][b - 1] : 0);
+                    }
+#pragma unroll
+                    for (int d = 0; d < 3; d++)
+                        *(uint32_t *)(row + 4 * d) = (uint32_t)v[4 * d] | ((uint32_t)v[4 * d + 1] << 8) |
+                                                     ((uint32_t)v[4 * d + 2] << 16) | ((uint32_t)v[4 * d + 3] << 24);
+                }
+                wave_lds_sync();
+                if (lane < 16) {
+                    const int comp = lane >> 3, k = lane & 7;
+                    uint8_t *col = &CT[comp][8 + k];
+                    int v[10];
+#pragma unroll
+                    for (int q = 0; q < 10; q++) v[q] = col[q * CP];
+#pragma unroll
+                    for (int e = 0; e < 4; e += 2) {
+                        const int b = bs_get(bs.z, bs.w, e, k >> 1);
+                        if (b) chroma_filter(&v[2 * e], b, alpha, beta, b < 4 ? kTc0[qpx][b - 1] : 0);
+                    }
+#pragma unroll
+                    for (int q = 1; q < 9; q++) col[q * CP] = (uint8_t)v[q];
+                }
             }
-            // ---- rows of MB (x,y-1) changed by the top edge
-            if (!skip_top && lane < 12) {
+            wave_lds_sync();
+        }
+        // ---- MB x-1 final for this row: hand its bottom rows down first
+        if (x > 0 && !last_row) {
+            uint32_t h = 0;
+            if (!chroma) {
+                if (lane < LHAND) h = *(const uint32_t *)&LT[(16 + (lane >> 2)) * LP + (lane & 3) * 4];
+            } else if (lane < CHAND) {
+                const int comp = lane >> 2, r = (lane >> 1) & 1, c = (lane & 1) * 4;
+                h = *(const uint32_t *)&CT[comp][(8 + r) * CP + c];
+            }
+            if (lds_below) {
+                unsigned n = 0;
+                while (vcons[r4 + 1] < x - RING) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++n > spin_limit) { atomicOr(err, 2); break; }
+                }
+                if (lane < nhand) {
+                    if (chroma) cring[r4][(x - 1) % RING][lane] = h;
+                    else lring[r4][(x - 1) % RING][lane] = h;
+                }
+                wave_lds_sync();
+                if (lane == 0) vprog[r4] = x;
+            } else {
+                if (lane < nhand) st_sys(gh + ((size_t)yrow * mbw + x - 1) * GHAND + lane, h);
+                wait_stores();
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) st_sys((uint32_t *)&gprog[yrow], (uint32_t)x);
+            }
+        }
+        // ---- frame stores: MB x-1 (rows this wave owns) and the rewritten rows of MB (x, y-1)
+        if (!chroma) {
+            if (x > 0) {
+                const int r = lane >> 2, c = (lane & 3) * 4;
+                if (r < (last_row ? 16 : 13))
+                    *(uint32_t *)(y_ + (size_t)(Y0 + r) * py + X0 - 16 + c) = *(const uint32_t *)&LT[(4 + r) * LP + c];
+            }
+            if (x < mbw && yrow > 0 && lane < 12) {
                 const int r = 1 + (lane >> 2), c = (lane & 3) * 4;
-                *(uint32_t *)(y_ + (size_t)(Y0 - 4 + r) * py + X0 + c) = *(const uint32_t *)&L[r * LP + 16 + c];
+                *(uint32_t *)(y_ + (size_t)(Y0 - 4 + r) * py + X0 + c) = *(const uint32_t *)&LT[r * LP + 16 + c];
             }
-            if (!skip_top && lane >= 16 && lane < 20) {
-                const int cc = (lane - 16) * 2;
-                uint32_t v = (uint32_t)C[0][CP + 8 + cc] | ((uint32_t)C[1][CP + 8 + cc] << 8) |
-                             ((uint32_t)C[0][CP + 9 + cc] << 16) | ((uint32_t)C[1][CP + 9 + cc] << 24);
-                *(uint32_t *)(uv + (size_t)(V0 - 1) * puv + 2 * (U0 + cc)) = v;
+        } else {
+            if (x > 0 && lane < 32) {
+                const int r = lane >> 2, c = (lane & 3) * 2;
+                if (r < (last_row ? 8 : 7)) {
+                    const uint32_t v = (uint32_t)CT[0][(2 + r) * CP + c] | ((uint32_t)CT[1][(2 + r) * CP + c] << 8) |
+                                       ((uint32_t)CT[0][(2 + r) * CP + 1 + c] << 16) |
+                                       ((uint32_t)CT[1][(2 + r) * CP + 1 + c] << 24);
+                    *(uint32_t *)(uv + (size_t)(V0 + r) * puv + X0 - 16 + 2 * c) = v;
+                }
             }
-        }
-        // ---- write back MB x-1 (final for this row)
-        if (x > 0) {
-            const int r = lane >> 2, c = (lane & 3) * 4;
-            *(uint32_t *)(y_ + (size_t)(Y0 + r) * py + X0 - 16 + c) = *(const uint32_t *)&L[(4 + r) * LP + c];
-            if (lane < 32) {
-                const int cr = lane >> 2, cc = (lane & 3) * 2;
-                uint32_t v = (uint32_t)C[0][(2 + cr) * CP + cc] | ((uint32_t)C[1][(2 + cr) * CP + cc] << 8) |
-                             ((uint32_t)C[0][(2 + cr) * CP + 1 + cc] << 16) |
-                             ((uint32_t)C[1][(2 + cr) * CP + 1 + cc] << 24);
-                *(uint32_t *)(uv + (size_t)(V0 + cr) * puv + 2 * (U0 - 8 + cc)) = v;
+            if (x < mbw && yrow > 0 && lane >= 32 && lane < 36) {
+                const int c = (lane - 32) * 2;
+                const uint32_t v = (uint32_t)CT[0][CP + 8 + c] | ((uint32_t)CT[1][CP + 8 + c] << 8) |
+                                   ((uint32_t)CT[0][CP + 9 + c] << 16) | ((uint32_t)CT[1][CP + 9 + c] << 24);
+                *(uint32_t *)(uv + (size_t)(V0 - 1) * puv + X0 + 2 * c) = v;
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         // ---- slide: MB x becomes the left neighbour
-        {
+        if (!chroma) {
             const int r = lane >> 2, c = (lane & 3) * 4;
-            *(uint32_t *)&L[(4 + r) * LP + c] = *(const uint32_t *)&L[(4 + r) * LP + 16 + c];
-            if (lane < 40) {
-                const int cr = lane >> 2, cc = (lane & 3) * 2;
-                C[0][cr * CP + cc] = C[0][cr * CP + 8 + cc];
-                C[0][cr * CP + cc + 1] = C[0][cr * CP + 9 + cc];
-                C[1][cr * CP + cc] = C[1][cr * CP + 8 + cc];
-                C[1][cr * CP + cc + 1] = C[1][cr * CP + 9 + cc];
-            }
+            *(uint32_t *)&LT[(4 + r) * LP + c] = *(const uint32_t *)&LT[(4 + r) * LP + 16 + c];
+        } else if (lane < 20) {
+            const int cr = lane >> 1, comp = lane & 1;
+            *(uint2 *)&CT[comp][cr * CP] = *(const uint2 *)&CT[comp][cr * CP + 8];
         }
-        __threadfence();
-        __syncthreads();
-        if (lane == 0) flag_store_release(&progress[yrow], x);
+        wave_lds_sync();
     }
+}
+
+extern "C" size_t bc250_deblock_work_bytes(int wc, int hc)
+{
+    const int mbw = wc / 16, mbh = hc / 16;
+    const size_t flags = ((size_t)(2 * mbh + 2) * sizeof(int) + 255) & ~(size_t)255;
+    return flags + (size_t)mbh * mbw * GHAND * 4 + (size_t)mbh * mbw * 16;
 }
 
 extern "C" int bc250_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp,
@@ -242,9 +410,14 @@ extern "C" int bc250_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, in
         num_slices < 1 || idc < 0 || idc > 2)
         return -1;
     if (idc == 1) return 0;
-    const int mbw = wc / 16, mbh = hc / 16;
-    h264_deblock_kernel<<<mbh, 64, 0, (hipStream_t)stream>>>(y, uv, py, puv, mbw, mbh, qp, is_intra, nz,
-                                                              (const uint4 *)mvs, num_slices, idc, sync,
-                                                              spin_limit);
+    hipStream_t st = (hipStream_t)stream;
+    const int mbw = wc / 16, mbh = hc / 16, nmb = mbw * mbh;
+    // workspace: flags | hand-off records | bS words
+    const size_t flags = ((size_t)(2 * mbh + 2) * sizeof(int) + 255) & ~(size_t)255;
+    uint32_t *ghand = (uint32_t *)((uint8_t *)sync + flags);
+    uint4 *bsw = (uint4 *)((uint8_t *)ghand + (size_t)nmb * GHAND * 4);
+    h264_bs_kernel<<<(nmb + 7) / 8, 256, 0, st>>>(mbw, mbh, is_intra, nz, (const uint4 *)mvs, num_slices, idc, bsw);
+    const int groups = (mbh + ROWS - 1) / ROWS;
+    h264_deblock_kernel<<<groups, 128 * ROWS, 0, st>>>(y, uv, py, puv, mbw, mbh, qp, bsw, sync, ghand, spin_limit);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

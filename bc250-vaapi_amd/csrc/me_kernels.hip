@@ -180,15 +180,56 @@ __global__ __launch_bounds__(ME_THREADS) void me_fullsearch_kernel(
 // block with integer winner (dx,dy): SAD at the integer position, then the 8
 // half-pel candidates (2,0),(-2,0),(0,2),(0,-2),(2,2),(2,-2),(-2,2),(-2,-2)
 // and the 8 quarter-pel ones around the half winner, strict < in that order.
-// The interpolation window (block + 4 on every side) is staged in LDS with
-// replicated edges, then every lane interpolates its pixels per candidate.
+// Per block the reference window (block + 4 on every side, replicated
+// edges) is staged in LDS and its H.264 half-sample planes are built once:
+// b (horizontal), h (vertical) and j (centre, from the unrounded horizontal
+// intermediates), each over the (bw+2)^2 positions a +-3/4-pel candidate can
+// touch.  A candidate's sample is then one or two LDS reads and an average.
+namespace {
+constexpr int RW = 24;       // window pitch (16 + 8)
+constexpr int RP = 18;       // plane pitch (16 + 2)
+
+struct RefineLds {
+    uint8_t win[RW * RW];
+    uint8_t cb[16 * 16];
+    int16_t hr[23 * RP];     // unrounded horizontal 6-tap at window rows 1..23, cols 3..20
+    uint8_t B[RP * RP], Hh[RP * RP], J[RP * RP];
+};
+
+// Sample at window integer position (xi, yi) and fraction (fx, fy), from
+// the planes (whose origin is window position (3,3)).
+__device__ __forceinline__ int sample_at(const RefineLds &L, int xi, int yi, int fx, int fy)
+{
+    const int px = xi - 3, py_ = yi - 3;
+    const int c = fy * 4 + fx;
+    const int g = L.win[yi * RW + xi];
+    switch (c) {
+    case 0: return g;
+    case 1: return (g + L.B[py_ * RP + px] + 1) >> 1;
+    case 2: return L.B[py_ * RP + px];
+    case 3: return (L.win[yi * RW + xi + 1] + L.B[py_ * RP + px] + 1) >> 1;
+    case 4: return (g + L.Hh[py_ * RP + px] + 1) >> 1;
+    case 5: return (L.B[py_ * RP + px] + L.Hh[py_ * RP + px] + 1) >> 1;
+    case 6: return (L.B[py_ * RP + px] + L.J[py_ * RP + px] + 1) >> 1;
+    case 7: return (L.B[py_ * RP + px] + L.Hh[py_ * RP + px + 1] + 1) >> 1;
+    case 8: return L.Hh[py_ * RP + px];
+    case 9: return (L.Hh[py_ * RP + px] + L.J[py_ * RP + px] + 1) >> 1;
+    case 10: return L.J[py_ * RP + px];
+    case 11: return (L.J[py_ * RP + px] + L.Hh[py_ * RP + px + 1] + 1) >> 1;
+    case 12: return (L.win[(yi + 1) * RW + xi] + L.Hh[py_ * RP + px] + 1) >> 1;
+    case 13: return (L.Hh[py_ * RP + px] + L.B[(py_ + 1) * RP + px] + 1) >> 1;
+    case 14: return (L.J[py_ * RP + px] + L.B[(py_ + 1) * RP + px] + 1) >> 1;
+    default: return (L.Hh[py_ * RP + px + 1] + L.B[(py_ + 1) * RP + px] + 1) >> 1;
+    }
+}
+}  // namespace
+
 __global__ __launch_bounds__(64) void me_refine_kernel(
     const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref, int rpitch,
     int wc, int hc, int mbw, int nmb, const uint4 *__restrict__ in16, const uint4 *__restrict__ in8,
     uint4 *__restrict__ mv16, uint4 *__restrict__ mv8, int do8)
 {
-    __shared__ uint8_t w[24 * 24];
-    __shared__ uint8_t cb[16 * 16];
+    __shared__ RefineLds L;
     const int mb = xcd_item(blockIdx.x, nmb);
     if (mb >= nmb) return;
     const int lane = threadIdx.x;
@@ -197,7 +238,10 @@ __global__ __launch_bounds__(64) void me_refine_kernel(
     const int HO[8][2] = {{2, 0}, {-2, 0}, {0, 2}, {0, -2}, {2, 2}, {2, -2}, {-2, 2}, {-2, -2}};
     const int QO[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
 
-    for (int i = lane; i < 256; i += 64) cb[i] = cur[(size_t)(Y0 + (i >> 4)) * cpitch + X0 + (i & 15)];
+    for (int i = lane; i < 64; i += 64) {
+        const uint32_t v = *(const uint32_t *)(cur + (size_t)(Y0 + (i >> 2)) * cpitch + X0 + (i & 3) * 4);
+        *(uint32_t *)&L.cb[(i >> 2) * 16 + (i & 3) * 4] = v;
+    }
 
     const int njobs = do8 ? 5 : 1;
     for (int job = 0; job < njobs; job++) {
@@ -205,39 +249,64 @@ __global__ __launch_bounds__(64) void me_refine_kernel(
         const int bx0 = job == 0 ? 0 : ((job - 1) & 1) * 8, by0 = job == 0 ? 0 : ((job - 1) >> 1) * 8;
         const uint4 iv = job == 0 ? in16[mb] : in8[4 * mb + job - 1];
         const int dx = (int)iv.x, dy = (int)iv.y;
-        const int ww = bw + 8;
-        __syncthreads();
+        const int ww = bw + 8, np = bw + 2;
+        wave_lds_sync();
+        // window with replicated edges
         for (int i = lane; i < ww * ww; i += 64) {
-            int yy = i / ww, xx = i - yy * ww;
-            int gx = clampi(X0 + bx0 + dx - 4 + xx, 0, wc - 1);
-            int gy = clampi(Y0 + by0 + dy - 4 + yy, 0, hc - 1);
-            w[yy * 24 + xx] = ref[(size_t)gy * rpitch + gx];
+            const int yy = i / ww, xx = i - yy * ww;
+            const int gx = clampi(X0 + bx0 + dx - 4 + xx, 0, wc - 1);
+            const int gy = clampi(Y0 + by0 + dy - 4 + yy, 0, hc - 1);
+            L.win[yy * RW + xx] = ref[(size_t)gy * rpitch + gx];
         }
-        __syncthreads();
+        wave_lds_sync();
+        // unrounded horizontal intermediates at rows 1..ww-1, cols 3..3+np-1
+        for (int i = lane; i < (ww - 1) * np; i += 64) {
+            const int r = i / np, k = i - r * np;
+            const uint8_t *p = &L.win[(1 + r) * RW + 3 + k];
+            L.hr[r * RP + k] = (int16_t)tap6(p[-2], p[-1], p[0], p[1], p[2], p[3]);
+        }
+        wave_lds_sync();
+        for (int i = lane; i < np * np; i += 64) {
+            const int r = i / np, k = i - r * np;
+            // b at window (3+k, 3+r) is hr row (3+r)-1
+            L.B[r * RP + k] = (uint8_t)clip255((L.hr[(2 + r) * RP + k] + 16) >> 5);
+            const uint8_t *q = &L.win[(3 + r) * RW + 3 + k];
+            L.Hh[r * RP + k] = (uint8_t)clip255((tap6(q[-2 * RW], q[-RW], q[0], q[RW], q[2 * RW], q[3 * RW]) + 16) >> 5);
+            const int16_t *t = &L.hr[r * RP + k];  // rows (3+r)-2 .. (3+r)+3 are hr rows r .. r+5
+            const int j1 = tap6(t[0], t[RP], t[2 * RP], t[3 * RP], t[4 * RP], t[5 * RP]);
+            L.J[r * RP + k] = (uint8_t)clip255((j1 + 512) >> 10);
+        }
+        wave_lds_sync();
         const int npx = bw * bw;
-        // SAD of this block at quarter-pel offset (ox, oy) from the integer winner
         auto sad_at = [&](int ox, int oy) -> uint32_t {
             uint32_t s = 0;
+            const int fx = ox & 3, fy = oy & 3;
             for (int p = lane; p < npx; p += 64) {
-                int px = p % bw, pyy = p / bw;
-                int qx = 4 * px + ox, qy = 4 * pyy + oy;  // window-relative quarter pel, origin at +4
-                int ix = (qx >> 2) + 4, iy = (qy >> 2) + 4;
-                int v = qpel_win(w, 24, ix, iy, qx & 3, qy & 3);
-                s += (uint32_t)iabs((int)cb[(by0 + pyy) * 16 + bx0 + px] - v);
+                const int px = p % bw, pyy = p / bw;
+                const int xi = 4 + px + (ox >> 2), yi = 4 + pyy + (oy >> 2);
+                s += (uint32_t)iabs((int)L.cb[(by0 + pyy) * 16 + bx0 + px] - sample_at(L, xi, yi, fx, fy));
             }
-            return wave_sum_u32(s);
+            return s;
         };
-        uint32_t best = sad_at(0, 0);
+        uint32_t best = wave_sum_u32(sad_at(0, 0));
+        uint32_t hs[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) hs[c] = sad_at(HO[c][0], HO[c][1]);
+#pragma unroll
+        for (int c = 0; c < 8; c++) hs[c] = wave_sum_u32(hs[c]);
         int hx = 0, hy = 0;
-        for (int c = 0; c < 8; c++) {
-            uint32_t s = sad_at(HO[c][0], HO[c][1]);
-            if (s < best) { best = s; hx = HO[c][0]; hy = HO[c][1]; }
-        }
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            if (hs[c] < best) { best = hs[c]; hx = HO[c][0]; hy = HO[c][1]; }
+        uint32_t qs[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) qs[c] = sad_at(hx + QO[c][0], hy + QO[c][1]);
+#pragma unroll
+        for (int c = 0; c < 8; c++) qs[c] = wave_sum_u32(qs[c]);
         int qx = 0, qy = 0;
-        for (int c = 0; c < 8; c++) {
-            uint32_t s = sad_at(hx + QO[c][0], hy + QO[c][1]);
-            if (s < best) { best = s; qx = QO[c][0]; qy = QO[c][1]; }
-        }
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            if (qs[c] < best) { best = qs[c]; qx = QO[c][0]; qy = QO[c][1]; }
         if (lane == 0) {
             uint4 o = make_uint4((uint32_t)(4 * dx + hx + qx), (uint32_t)(4 * dy + hy + qy), best, 0u);
             if (job == 0) mv16[mb] = o;

@@ -17,6 +17,7 @@
 #ifndef BC250_STAGES_H
 #define BC250_STAGES_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -44,16 +45,22 @@ int bc250_launch_p_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cp
                           uint32_t *modes, void *stream);
 
 /* I16x16 wavefront in one persistent launch; replaces the 187 per-diagonal
- * intra_wavefront.comp dispatches (gpu_compute.c:3191-3255).  `sync` holds
- * 2 + coded_height/16 ints and must be zero; sync[1] != 0 afterwards means
- * a dependency wait gave up after spin_limit polls. */
+ * intra_wavefront.comp dispatches (gpu_compute.c:3191-3255).  `sync` is a
+ * workspace of bc250_intra_work_bytes() bytes whose first
+ * 2 + coded_height/16 ints must be zero; sync[1] != 0 afterwards means a
+ * dependency wait gave up after spin_limit polls. */
+size_t bc250_intra_work_bytes(int wc, int hc);
 int bc250_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, uint8_t *oy, uint8_t *ouv,
                           int opy, int opuv, int wc, int hc, int qp, int num_slices, int16_t *levels, int32_t *dc,
                           uint32_t *nz, uint32_t *modes, int *sync, unsigned spin_limit, void *stream);
 
-/* H.264 8.7 deblocking in place (luma + chroma), one persistent launch;
- * replaces the two deblock_filter.comp passes (gpu_compute.c:3257-3290).
- * `sync` as for bc250_launch_i_encode. */
+/* H.264 8.7 deblocking in place (luma + chroma): a boundary-strength launch
+ * and one persistent wavefront launch; replaces the two deblock_filter.comp
+ * passes (gpu_compute.c:3257-3290).  `sync` is a workspace of
+ * bc250_deblock_work_bytes() bytes whose first 2 + 2*coded_height/16 ints
+ * must be zero; sync[1] != 0 afterwards means a dependency wait gave up
+ * after spin_limit polls. */
+size_t bc250_deblock_work_bytes(int wc, int hc);
 int bc250_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
                          const uint32_t *nz, const uint32_t *mvs, int num_slices, int idc, int *sync,
                          unsigned spin_limit, void *stream);

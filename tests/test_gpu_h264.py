@@ -145,7 +145,8 @@ def test_i_encode_stage(pkg, qp, ns):
     dc = torch.zeros(nmb * 24, dtype=torch.int32, device="cuda")
     nz = torch.zeros(nmb * 24, dtype=torch.int32, device="cuda")
     md = torch.zeros(nmb, dtype=torch.int32, device="cuda")
-    sync = torch.zeros(hc // 16 + 2, dtype=torch.int32, device="cuda")
+    nwork = pkg.hip.bc250_intra_work_bytes(wc, hc)
+    sync = torch.zeros((nwork + 3) // 4, dtype=torch.int32, device="cuda")
     P = lambda t: C.c_void_p(t.data_ptr())
     rc = pkg.hip.bc250_launch_i_encode(P(dcy), P(dcuv), wc, wc, P(oy), P(ouv), wc, wc, wc, hc, qp, ns, P(lv), P(dc),
                                        P(nz), P(md), P(sync), 1 << 22, None)
@@ -180,16 +181,61 @@ def test_deblock_stage(pkg, intra, qp, ns, idc):
     ey, euv = orc.deblock(y, uv, qp, intra, nz, mvs, ns, idc)
     dy, duv = dev(torch, y), dev(torch, uv)
     dnz, dmv = dev(torch, nz.view(np.int32)), dev(torch, mvs.view(np.int32))
-    sync = torch.zeros(hc // 16 + 2, dtype=torch.int32, device="cuda")
+    nwork = pkg.hip.bc250_deblock_work_bytes(wc, hc)
+    sync = torch.zeros((nwork + 3) // 4, dtype=torch.int32, device="cuda")
     P = lambda t: C.c_void_p(t.data_ptr())
     rc = pkg.hip.bc250_launch_deblock(P(dy), P(duv), wc, wc, wc, hc, qp, intra, P(dnz), P(dmv), ns, idc, P(sync),
                                       1 << 22, None)
     assert rc == 0
     torch.cuda.synchronize()
     assert host(sync)[1] == 0, "wavefront wait gave up"
+    # boundary strengths left in the workspace (after the flags and hand-off records)
+    mbw, mbh = wc // 16, hc // 16
+    flags = ((2 * mbh + 2) * 4 + 255) // 256 * 256
+    off = flags + nmb * 24 * 4
+    words = host(sync).view(np.uint8)[off:off + nmb * 16].view(np.uint32).reshape(nmb, 4)
+    np.testing.assert_array_equal(words, expected_bs_words(mbw, mbh, intra, nz, mvs, ns, idc))
     assert (ey != y).any()  # the case exercises the filter
     np.testing.assert_array_equal(host(dy), ey)
     np.testing.assert_array_equal(host(duv), euv)
+
+
+def expected_bs_words(mbw, mbh, intra, nz, mvs, ns, idc):
+    """H.264 8.7.2.1 strengths, 4 bits per (edge, segment): words vertical
+    edges 0-1, vertical 2-3, horizontal 0-1, horizontal 2-3; 0 where an edge
+    is not filtered."""
+    nmb = mbw * mbh
+    out = np.zeros((nmb, 4), np.uint32)
+    sl = [orc.lib.orc_slice_of(m, nmb, ns) for m in range(nmb)]
+    for m in range(nmb):
+        x, y = m % mbw, m // mbw
+        for d in range(2):
+            w = [0, 0]
+            for e in range(4):
+                for i in range(4):
+                    if e == 0:
+                        nb = m - 1 if d == 0 else m - mbw
+                        exists = (x > 0) if d == 0 else (y > 0)
+                        if exists and idc == 2 and sl[nb] != sl[m]:
+                            exists = False
+                        if not exists:
+                            continue
+                    if intra:
+                        b = 4 if e == 0 else 3
+                    else:
+                        pm = m if e else (m - 1 if d == 0 else m - mbw)
+                        bq = i * 4 + e if d == 0 else e * 4 + i
+                        bp = (i * 4 + e - 1 if e else i * 4 + 3) if d == 0 else ((e - 1) * 4 + i if e else 12 + i)
+                        if nz[pm * 24 + bp] or nz[m * 24 + bq]:
+                            b = 2
+                        elif abs(int(mvs[pm]["mvx"]) - int(mvs[m]["mvx"])) >= 4 or \
+                                abs(int(mvs[pm]["mvy"]) - int(mvs[m]["mvy"])) >= 4:
+                            b = 1
+                        else:
+                            b = 0
+                    w[e >> 1] |= b << (4 * ((e & 1) * 4 + i))
+            out[m, 2 * d:2 * d + 2] = w
+    return out
 
 
 # ------------------------------------------------------------------ pipeline
