@@ -271,6 +271,10 @@ BC_DEV void wave_lds_sync()
 // writeback / invalidate is needed around them.
 BC_DEV uint32_t ld_sys(const uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); }
 BC_DEV void st_sys(uint32_t *p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); }
+// 64-bit forms: an aligned 8-byte access is single-copy atomic, so a record
+// {tag, value} written by one store is seen whole or not at all.
+BC_DEV uint64_t ld_sys64(const uint64_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); }
+BC_DEV void st_sys64(uint64_t *p, uint64_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); }
 
 // Waits until every store this wave issued has completed.
 BC_DEV void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }

@@ -498,7 +498,7 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     }
     hipEventRecord(sl.ev[3], st);
     if (r == 0 && p.deblock_idc != 1) {
-        r = bc250_launch_zero_i32(s->sync_d, 2 * s->mbh + 2, st);
+        r = bc250_launch_zero_i32(s->sync_d, 2, st);
         if (r == 0)
             r = bc250_launch_deblock(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->nz, s->mvs,
                                      num_slices, p.deblock_idc, s->sync_d, kSpinLimit, st);

@@ -9,21 +9,23 @@
 // wavefront in which row y runs two MBs behind row y-1.  Luma and chroma
 // never read each other, so they run as two independent wavefronts.
 //
-// Work layout: a workgroup is 8 waves for 4 MB rows (row group from an
-// atomic ticket): waves 0..3 filter the luma of rows 4g..4g+3, waves 4..7
-// their chroma.  A wave keeps a sliding LDS tile (previous MB, current MB,
+// Work layout: a workgroup is 2*ROWS waves for ROWS MB rows (row group from
+// an atomic ticket), one wave per SIMD: the first ROWS waves filter the luma
+// of rows ROWS*g.., the others their chroma.  A wave keeps a sliding LDS tile (previous MB, current MB,
 // rows above).  Per MB it makes two passes: one lane per line through the
 // four vertical edges in registers, then one lane per column through the
 // four horizontal edges.  When MB x-1 is final for its row (after MB x
 // filtered its left edge) the wave
 //   - hands the bottom rows that the next row's top edge reads and partly
 //     rewrites (luma 12..15, chroma 6..7) to the wave below: through an LDS
-//     ring inside the workgroup, or a system-coherent record in global memory
-//     to the next workgroup, published after an s_waitcnt,
+//     ring inside the workgroup, or to the next workgroup as system-coherent
+//     64-bit records {tag = x + 1, 4 samples} that the reader polls until
+//     every tag matches, so the writer never waits for its stores,
 //   - stores the other rows to the frame; the wave below stores the rows it
 //     rewrote (luma 13..15, chroma 7), so each frame byte has one writer.
-// No cache writeback or invalidate is issued.  Waits are bounded and report
-// on *err.
+// The records are cleared by the boundary-strength kernel before each
+// frame.  No cache writeback or invalidate is issued.  Waits are bounded and
+// report on *err.
 #include "bc250_dev.h"
 #include "bc250_stages.h"
 
@@ -44,7 +46,7 @@ __constant__ uint8_t kTc0[52][3] = {
     {4, 5, 7}, {4, 5, 8}, {4, 6, 9}, {5, 7, 10}, {6, 8, 11}, {6, 8, 13}, {7, 10, 14}, {8, 11, 16},
     {9, 12, 18}, {10, 13, 20}, {11, 15, 23}, {13, 17, 25}};
 
-constexpr int ROWS = 4;        // MB rows per workgroup
+constexpr int ROWS = 2;        // MB rows per workgroup
 constexpr int LP = 32;         // luma tile pitch: cols 0..15 MB x-1, 16..31 MB x
 constexpr int CP = 16;         // chroma tile pitch: cols 0..7 MB x-1, 8..15 MB x
 constexpr int RING = 16;       // LDS hand-off slots per producing wave
@@ -52,52 +54,62 @@ constexpr int LHAND = 16;      // luma hand-off dwords: rows 12..15, 16 bytes ea
 constexpr int CHAND = 8;       // chroma hand-off dwords: Cb rows 6..7, Cr rows 6..7 (8 bytes each)
 constexpr int GHAND = LHAND + CHAND;
 
-// v[0..7] = p3 p2 p1 p0 q0 q1 q2 q3; filters in place.
-__device__ __forceinline__ void luma_filter(int *v, int bs, int alpha, int beta, int tc0)
+// One luma line across an edge (8.7.2.3 / 8.7.2.4): v[0..7] = p3 p2 p1 p0
+// q0 q1 q2 q3, filtered in place.  Written without branches: every candidate
+// output is computed, conditions are combined with '&', and each '?:' picks
+// between two ready values, so it lowers to v_cndmask.  (A branchy form makes
+// the compiler copy the whole line at every join.)  can4 = false promises bS
+// < 4 (a compile-time constant at every call), which drops the strong form.
+__device__ __forceinline__ void luma_filter(int *v, int bs, int alpha, int beta, int tc0, bool can4)
 {
     const int p3 = v[0], p2 = v[1], p1 = v[2], p0 = v[3], q0 = v[4], q1 = v[5], q2 = v[6], q3 = v[7];
-    if (!(iabs(p0 - q0) < alpha && iabs(p1 - p0) < beta && iabs(q1 - q0) < beta)) return;
-    const int ap = iabs(p2 - p0), aq = iabs(q2 - q0);
-    if (bs < 4) {
-        const int tc = tc0 + (ap < beta) + (aq < beta);
-        const int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
-        v[3] = clip255(p0 + delta);
-        v[4] = clip255(q0 - delta);
-        if (ap < beta) v[2] = p1 + clampi((p2 + ((p0 + q0 + 1) >> 1) - (p1 << 1)) >> 1, -tc0, tc0);
-        if (aq < beta) v[5] = q1 + clampi((q2 + ((p0 + q0 + 1) >> 1) - (q1 << 1)) >> 1, -tc0, tc0);
-    } else {
-        const bool narrow = iabs(p0 - q0) < ((alpha >> 2) + 2);
-        if (ap < beta && narrow) {
-            v[3] = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3;
-            v[2] = (p2 + p1 + p0 + q0 + 2) >> 2;
-            v[1] = (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3;
-        } else {
-            v[3] = (2 * p1 + p0 + q1 + 2) >> 2;
-        }
-        if (aq < beta && narrow) {
-            v[4] = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3;
-            v[5] = (p0 + q0 + q1 + q2 + 2) >> 2;
-            v[6] = (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3;
-        } else {
-            v[4] = (2 * q1 + q0 + p1 + 2) >> 2;
-        }
-    }
+    const int d0 = iabs(p0 - q0);
+    const bool f = (bs != 0) & (d0 < alpha) & (iabs(p1 - p0) < beta) & (iabs(q1 - q0) < beta);
+    const bool apb = iabs(p2 - p0) < beta, aqb = iabs(q2 - q0) < beta;
+    const bool st = can4 && bs >= 4;
+    // bS < 4
+    const int tc = tc0 + (int)apb + (int)aqb;
+    const int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+    const int avg = (p0 + q0 + 1) >> 1;
+    const int w_p0 = clip255(p0 + delta), w_q0 = clip255(q0 - delta);
+    const int c_p1 = p1 + clampi((p2 + avg - (p1 << 1)) >> 1, -tc0, tc0);
+    const int c_q1 = q1 + clampi((q2 + avg - (q1 << 1)) >> 1, -tc0, tc0);
+    // bS == 4
+    const bool narrow = d0 < ((alpha >> 2) + 2);
+    const bool sp = apb & narrow, sq = aqb & narrow;
+    const int sp_p0 = (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3, wp_p0 = (2 * p1 + p0 + q1 + 2) >> 2;
+    const int sp_p1 = (p2 + p1 + p0 + q0 + 2) >> 2;
+    const int sp_p2 = (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3;
+    const int sq_q0 = (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3, wq_q0 = (2 * q1 + q0 + p1 + 2) >> 2;
+    const int sq_q1 = (p0 + q0 + q1 + q2 + 2) >> 2;
+    const int sq_q2 = (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3;
+    const int s_p0 = sp ? sp_p0 : wp_p0, s_q0 = sq ? sq_q0 : wq_q0;
+    const int n_p1 = apb ? c_p1 : p1, n_q1 = aqb ? c_q1 : q1;
+    const int o_p0 = st ? s_p0 : w_p0, o_q0 = st ? s_q0 : w_q0;
+    const int o_p1 = st ? (sp ? sp_p1 : p1) : n_p1, o_q1 = st ? (sq ? sq_q1 : q1) : n_q1;
+    const int o_p2 = (st & sp) ? sp_p2 : p2, o_q2 = (st & sq) ? sq_q2 : q2;
+    v[1] = f ? o_p2 : p2;
+    v[2] = f ? o_p1 : p1;
+    v[3] = f ? o_p0 : p0;
+    v[4] = f ? o_q0 : q0;
+    v[5] = f ? o_q1 : q1;
+    v[6] = f ? o_q2 : q2;
 }
 
-// v[0..3] = p1 p0 q0 q1 (chromaEdgeFlag = 1).
-__device__ __forceinline__ void chroma_filter(int *v, int bs, int alpha, int beta, int tc0)
+// One chroma line across an edge (chromaEdgeFlag = 1): v[0..3] = p1 p0 q0
+// q1, filtered in place, branch-free like luma_filter (same can4).
+__device__ __forceinline__ void chroma_filter(int *v, int bs, int alpha, int beta, int tc0, bool can4)
 {
     const int p1 = v[0], p0 = v[1], q0 = v[2], q1 = v[3];
-    if (!(iabs(p0 - q0) < alpha && iabs(p1 - p0) < beta && iabs(q1 - q0) < beta)) return;
-    if (bs < 4) {
-        const int tc = tc0 + 1;
-        const int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
-        v[1] = clip255(p0 + delta);
-        v[2] = clip255(q0 - delta);
-    } else {
-        v[1] = (2 * p1 + p0 + q1 + 2) >> 2;
-        v[2] = (2 * q1 + q0 + p1 + 2) >> 2;
-    }
+    const bool f = (bs != 0) & (iabs(p0 - q0) < alpha) & (iabs(p1 - p0) < beta) & (iabs(q1 - q0) < beta);
+    const int tc = tc0 + 1;
+    const int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+    const bool st = can4 && bs >= 4;
+    const int s_p0 = (2 * p1 + p0 + q1 + 2) >> 2, s_q0 = (2 * q1 + q0 + p1 + 2) >> 2;
+    const int w_p0 = clip255(p0 + delta), w_q0 = clip255(q0 - delta);
+    const int o_p0 = st ? s_p0 : w_p0, o_q0 = st ? s_q0 : w_q0;
+    v[1] = f ? o_p0 : p0;
+    v[2] = f ? o_q0 : q0;
 }
 
 // bS words: per direction two words, edges 0..1 then 2..3, 4 bits per (edge, segment).
@@ -120,13 +132,15 @@ struct alignas(16) ChromaLds {
 // 2..3}, 4 bits per (edge e, segment i) at bit 4*(4*(e&1)+i) of its word.
 // Edges that are not filtered (picture border, slice border with idc 2) get 0.
 __global__ void h264_bs_kernel(int mbw, int mbh, int is_intra, const uint32_t *__restrict__ nz,
-                               const uint4 *__restrict__ mvs, int num_slices, int idc, uint4 *__restrict__ bs)
+                               const uint4 *__restrict__ mvs, int num_slices, int idc, uint4 *__restrict__ bs,
+                               uint64_t *__restrict__ ghand)
 {
     const int m = blockIdx.x * 8 + (threadIdx.x >> 5);
     const int j = threadIdx.x & 31;
     const int nmb = mbw * mbh;
     if (m >= nmb) return;
     const int x = m % mbw, y = m / mbw;
+    if (j < GHAND) ghand[(size_t)m * GHAND + j] = 0;  // no valid tag
     const int dir = j >> 4, e = (j >> 2) & 3, i = j & 3;
     int b = 0;
     bool exists = true;
@@ -159,10 +173,13 @@ __global__ void h264_bs_kernel(int mbw, int mbh, int is_intra, const uint32_t *_
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(512) void h264_deblock_kernel(
+// INTRA: bS is 4 on MB edges (edge 0) and 3 inside; otherwise bS <= 2, so
+// each edge's filter form is known at compile time.
+template <bool INTRA>
+__global__ __launch_bounds__(128 * ROWS) void h264_deblock_kernel(
     uint8_t *__restrict__ y_, uint8_t *__restrict__ uv, int py, int puv, int mbw, int mbh, int qp,
-    const uint4 *__restrict__ bsw, int *sync /* [0] ticket, [1] err, [2..2+mbh) luma, [..+mbh) chroma */,
-    uint32_t *ghand /* [mbh][mbw][GHAND] */, unsigned spin_limit)
+    const uint4 *__restrict__ bsw, int *sync /* [0] ticket, [1] err */,
+    uint64_t *ghand /* [mbh][mbw][GHAND] {tag, value} */, unsigned spin_limit)
 {
     __shared__ LumaLds ll[ROWS];
     __shared__ ChromaLds cl[ROWS];
@@ -173,9 +190,9 @@ __global__ __launch_bounds__(512) void h264_deblock_kernel(
 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int chroma = wv >= ROWS, r4 = wv & (ROWS - 1);
-    int *err = sync + 1, *gprog = sync + 2 + (chroma ? mbh : 0);
+    int *err = sync + 1;
     if (tid == 0) s_group = atomicAdd(sync, 1);
-    if (tid < 2 * ROWS) { prog[tid >> 2][tid & 3] = 0; cons[tid >> 2][tid & 3] = 0; }
+    if (tid < ROWS) prog[0][tid] = prog[1][tid] = cons[0][tid] = cons[1][tid] = 0;
     __syncthreads();
     const int yrow = s_group * ROWS + r4;
     if (yrow >= mbh) return;
@@ -185,9 +202,13 @@ __global__ __launch_bounds__(512) void h264_deblock_kernel(
     const bool lds_above = r4 > 0;
     const int qpx = chroma ? chroma_qp(qp) : qp;
     const int alpha = kAlpha[qpx], beta = kBeta[qpx];
+    // tC0 for bS 1..3 in registers: a per-lane table lookup would be a vector
+    // memory load, and its wait would also wait for this wave's frame stores.
+    const int tc1 = kTc0[qpx][0], tc2 = kTc0[qpx][1], tc3 = kTc0[qpx][2];
+    auto tc0_of = [&](int b) { return (b == 1 ? tc1 : 0) | (b == 2 ? tc2 : 0) | (b == 3 ? tc3 : 0); };
     const int Y0 = yrow * 16, V0 = yrow * 8;
     const int nhand = chroma ? CHAND : LHAND;
-    uint32_t *gh = ghand + (chroma ? LHAND : 0);
+    uint64_t *gh = ghand + (chroma ? LHAND : 0);
     uint8_t *LT = ll[r4].T;
     uint8_t(*CT)[10 * CP] = cl[r4].T;
 
@@ -229,16 +250,24 @@ __global__ __launch_bounds__(512) void h264_deblock_kernel(
                     wave_lds_sync();
                     if (lane == 0) vcons[r4] = x + 1;
                 } else {
-                    if (lane == 0) {
-                        unsigned n = 0;
-                        while ((int)ld_sys((const uint32_t *)&gprog[yrow - 1]) < x + 1) {
-                            __builtin_amdgcn_s_sleep(1);
-                            if (++n > spin_limit) { atomicOr(err, 1); break; }
+                    const uint64_t *src = gh + ((size_t)(yrow - 1) * mbw + x) * GHAND + lane;
+                    const uint32_t tag = (uint32_t)x + 1;
+                    unsigned n = 0;
+                    uint64_t q = 0;
+                    for (;;) {
+                        bool ok = true;
+                        if (lane < nhand) {
+                            q = ld_sys64(src);
+                            ok = (uint32_t)(q >> 32) == tag;
+                        }
+                        if (__all(ok)) break;
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++n > spin_limit) {
+                            if (lane == 0) atomicOr(err, 1);
+                            break;
                         }
                     }
-                    __builtin_amdgcn_wave_barrier();
-                    asm volatile("" ::: "memory");
-                    if (lane < nhand) h = ld_sys(gh + ((size_t)(yrow - 1) * mbw + x) * GHAND + lane);
+                    h = (uint32_t)q;
                 }
                 if (!chroma) {
                     if (lane < LHAND) *(uint32_t *)&LT[(lane >> 2) * LP + 16 + (lane & 3) * 4] = h;
@@ -264,7 +293,7 @@ __global__ __launch_bounds__(512) void h264_deblock_kernel(
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
                         const int b = bs_get(bs.x, bs.y, e, lane >> 2);
-                        if (b) luma_filter(&v[4 * e], b, alpha, beta, b < 4 ? kTc0[qpx][b - 1] : 0);
+                        if (__any(b != 0)) luma_filter(&v[4 * e], b, alpha, beta, tc0_of(b), INTRA && e == 0);
                     }
 #pragma unroll
                     for (int d = 0; d < 5; d++)
@@ -281,7 +310,7 @@ __global__ __launch_bounds__(512) void h264_deblock_kernel(
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
                         const int b = bs_get(bs.z, bs.w, e, lane >> 2);
-                        if (b) luma_filter(&v[4 * e], b, alpha, beta, b < 4 ? kTc0[qpx][b - 1] : 0);
+                        if (__any(b != 0)) luma_filter(&v[4 * e], b, alpha, beta, tc0_of(b), INTRA && e == 0);
                     }
 #pragma unroll
                     for (int k = 1; k < 19; k++) col[k * LP] = (uint8_t)v[k];
@@ -301,7 +330,7 @@ __global__ __launch_bounds__(512) void h264_deblock_kernel(
 #pragma unroll
                     for (int e = 0; e < 4; e += 2) {
                         const int b = bs_get(bs.x, bs.y, e, k >> 1);
-                        if (b) chroma_filter(&v[2 + 2 * e], b, alpha, beta, b < 4 ? kTc0[qpx][b - 1] : 0);
+                        chroma_filter(&v[2 + 2 * e], b, alpha, beta, tc0_of(b), INTRA && e == 0);
                     }
 #pragma unroll
                     for (int d = 0; d < 3; d++)
@@ -318,7 +347,7 @@ __global__ __launch_bounds__(512) void h264_deblock_kernel(
 #pragma unroll
                     for (int e = 0; e < 4; e += 2) {
                         const int b = bs_get(bs.z, bs.w, e, k >> 1);
-                        if (b) chroma_filter(&v[2 * e], b, alpha, beta, b < 4 ? kTc0[qpx][b - 1] : 0);
+                        chroma_filter(&v[2 * e], b, alpha, beta, tc0_of(b), INTRA && e == 0);
                     }
 #pragma unroll
                     for (int q = 1; q < 9; q++) col[q * CP] = (uint8_t)v[q];
@@ -347,11 +376,8 @@ __global__ __launch_bounds__(512) void h264_deblock_kernel(
                 }
                 wave_lds_sync();
                 if (lane == 0) vprog[r4] = x;
-            } else {
-                if (lane < nhand) st_sys(gh + ((size_t)yrow * mbw + x - 1) * GHAND + lane, h);
-                wait_stores();
-                __builtin_amdgcn_wave_barrier();
-                if (lane == 0) st_sys((uint32_t *)&gprog[yrow], (uint32_t)x);
+            } else if (lane < nhand) {
+                st_sys64(gh + ((size_t)yrow * mbw + x - 1) * GHAND + lane, ((uint64_t)x << 32) | h);
             }
         }
         // ---- frame stores: MB x-1 (rows this wave owns) and the rewritten rows of MB (x, y-1)
@@ -397,9 +423,8 @@ __global__ __launch_bounds__(512) void h264_deblock_kernel(
 
 extern "C" size_t bc250_deblock_work_bytes(int wc, int hc)
 {
-    const int mbw = wc / 16, mbh = hc / 16;
-    const size_t flags = ((size_t)(2 * mbh + 2) * sizeof(int) + 255) & ~(size_t)255;
-    return flags + (size_t)mbh * mbw * GHAND * 4 + (size_t)mbh * mbw * 16;
+    const size_t nmb = (size_t)(wc / 16) * (hc / 16);
+    return 256 + nmb * GHAND * 8 + nmb * 16;
 }
 
 extern "C" int bc250_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp,
@@ -412,12 +437,17 @@ extern "C" int bc250_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, in
     if (idc == 1) return 0;
     hipStream_t st = (hipStream_t)stream;
     const int mbw = wc / 16, mbh = hc / 16, nmb = mbw * mbh;
-    // workspace: flags | hand-off records | bS words
-    const size_t flags = ((size_t)(2 * mbh + 2) * sizeof(int) + 255) & ~(size_t)255;
-    uint32_t *ghand = (uint32_t *)((uint8_t *)sync + flags);
-    uint4 *bsw = (uint4 *)((uint8_t *)ghand + (size_t)nmb * GHAND * 4);
-    h264_bs_kernel<<<(nmb + 7) / 8, 256, 0, st>>>(mbw, mbh, is_intra, nz, (const uint4 *)mvs, num_slices, idc, bsw);
+    // workspace: 256 B of flags | hand-off records | bS words
+    uint64_t *ghand = (uint64_t *)((uint8_t *)sync + 256);
+    uint4 *bsw = (uint4 *)(ghand + (size_t)nmb * GHAND);
+    h264_bs_kernel<<<(nmb + 7) / 8, 256, 0, st>>>(mbw, mbh, is_intra, nz, (const uint4 *)mvs, num_slices, idc, bsw,
+                                                  ghand);
     const int groups = (mbh + ROWS - 1) / ROWS;
-    h264_deblock_kernel<<<groups, 128 * ROWS, 0, st>>>(y, uv, py, puv, mbw, mbh, qp, bsw, sync, ghand, spin_limit);
+    if (is_intra)
+        h264_deblock_kernel<true><<<groups, 128 * ROWS, 0, st>>>(y, uv, py, puv, mbw, mbh, qp, bsw, sync, ghand,
+                                                                 spin_limit);
+    else
+        h264_deblock_kernel<false><<<groups, 128 * ROWS, 0, st>>>(y, uv, py, puv, mbw, mbh, qp, bsw, sync, ghand,
+                                                                  spin_limit);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -62,6 +62,21 @@ def sad_peak(pkg, torch):
     return blocks * 256 * iters * 128 * 4 / (ms * 1e-3) / 1e12
 
 
+def copy_calibration(pkg, torch, nbytes=64 << 20, reps=3):
+    """A plain copy of a known size, launched so that a counter run over this
+    process (FETCH_SIZE / WRITE_SIZE) has a known-bytes kernel to scale by."""
+    import ctypes as C
+    src = torch.full((nbytes,), 7, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    f = pkg.hip.bc250_calib_copy
+    f.restype, f.argtypes = C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+    for _ in range(reps):
+        if f(src.data_ptr(), dst.data_ptr(), nbytes, None) != 0:
+            raise RuntimeError("calibration copy failed")
+    torch.cuda.synchronize()
+    return nbytes
+
+
 def cpu_baseline(frames_coded, qp, rng, lam, budget_s):
     """Oracle (single thread) on a bounded sample: the integer search of a
     spread sample of macroblocks, scaled to the frame, plus sub-pel refine,
@@ -155,6 +170,7 @@ def main():
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     peak_meas = sad_peak(pkg, torch)
+    copy_calibration(pkg, torch)
     res = {
         "metric": METRIC,
         "value": round(fps, 2),

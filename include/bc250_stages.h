@@ -57,9 +57,9 @@ int bc250_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cp
 /* H.264 8.7 deblocking in place (luma + chroma): a boundary-strength launch
  * and one persistent wavefront launch; replaces the two deblock_filter.comp
  * passes (gpu_compute.c:3257-3290).  `sync` is a workspace of
- * bc250_deblock_work_bytes() bytes whose first 2 + 2*coded_height/16 ints
- * must be zero; sync[1] != 0 afterwards means a dependency wait gave up
- * after spin_limit polls. */
+ * bc250_deblock_work_bytes() bytes whose first 2 ints must be zero (the
+ * rest is cleared by the launch itself); sync[1] != 0 afterwards means a
+ * dependency wait gave up after spin_limit polls. */
 size_t bc250_deblock_work_bytes(int wc, int hc);
 int bc250_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
                          const uint32_t *nz, const uint32_t *mvs, int num_slices, int idc, int *sync,

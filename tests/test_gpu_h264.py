@@ -191,8 +191,7 @@ def test_deblock_stage(pkg, intra, qp, ns, idc):
     assert host(sync)[1] == 0, "wavefront wait gave up"
     # boundary strengths left in the workspace (after the flags and hand-off records)
     mbw, mbh = wc // 16, hc // 16
-    flags = ((2 * mbh + 2) * 4 + 255) // 256 * 256
-    off = flags + nmb * 24 * 4
+    off = 256 + nmb * 24 * 8
     words = host(sync).view(np.uint8)[off:off + nmb * 16].view(np.uint32).reshape(nmb, 4)
     np.testing.assert_array_equal(words, expected_bs_words(mbw, mbh, intra, nz, mvs, ns, idc))
     assert (ey != y).any()  # the case exercises the filter
