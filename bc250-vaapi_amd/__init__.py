@@ -41,9 +41,17 @@ class GpuMemory(C.Structure):
     _fields_ = [("memory", C.c_void_p), ("size", C.c_size_t), ("mapped_ptr", C.c_void_p)]
 
 
+class Nv12Layout(C.Structure):
+    """gpu_nv12_layout_t (include/gpu_compute.h)."""
+    _fields_ = [("y_pitch", C.c_uint32), ("y_offset", C.c_uint64), ("uv_pitch", C.c_uint32),
+                ("uv_offset", C.c_uint64), ("total_size", C.c_uint64)]
+
+
 class GpuContext(C.Structure):
     _fields_ = [("hip", C.c_void_p), ("device", C.c_int), ("frames_submitted", C.c_uint32),
-                ("frame_width", C.c_uint32), ("frame_height", C.c_uint32)]
+                ("frame_width", C.c_uint32), ("frame_height", C.c_uint32),
+                ("recon_image", GpuImage), ("recon_memory", GpuMemory),
+                ("has_recon_frame", C.c_bool), ("any_surface_exported", C.c_bool)]
 
 
 class EncodeParams(C.Structure):
@@ -94,6 +102,12 @@ _sig(hip, "gpu_compute_get_stage_times", _I, [_CTX, _I, C.POINTER(C.c_float)])
 _sig(hip, "gpu_compute_take_error_flags", _I, [_CTX])
 _sig(hip, "gpu_compute_get_stage_stats", _I, [_CTX, C.POINTER(StageStats), _I])
 _sig(hip, "gpu_compute_get_last_latency_ms", C.c_double, [_CTX])
+_sig(hip, "gpu_compute_cpu_read_safe", _I, [_CTX])
+_sig(hip, "gpu_compute_wait_for_image_ready_host", _I, [_CTX, GpuMemory])
+_sig(hip, "gpu_compute_get_nv12_layout", _I, [_CTX, C.POINTER(GpuImage), GpuMemory, C.POINTER(Nv12Layout)])
+_sig(hip, "gpu_compute_map_surface", _P, [_CTX, C.POINTER(GpuImage), GpuMemory, C.POINTER(Nv12Layout),
+                                          C.POINTER(C.c_bool)])
+_sig(hip, "gpu_compute_unmap_surface", None, [_CTX, GpuMemory, C.c_bool])
 
 _U = C.c_uint
 _sig(hip, "bc250_launch_me_fullsearch", _I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P, _P])
@@ -244,6 +258,33 @@ class Encoder:
         uv = np.empty(((self.h + 1) // 2, 2 * ((self.w + 1) // 2)), np.uint8)
         if hip.gpu_compute_download_recon(C.byref(self.ctx), _ptr(y), y.strides[0], _ptr(uv), uv.strides[0]) != 0:
             raise RuntimeError("download_recon failed")
+        return y, uv
+
+    def map_recon(self):
+        """Reads the newest reconstruction the way the reference's CPU ME route
+        does (encoder_h264.c:2325-2378): cpu_read_safe, then map_surface on
+        ctx.recon_image / ctx.recon_memory, then unmap.  Returns coded-size
+        (y, uv) copies."""
+        import numpy as np
+        c = C.byref(self.ctx)
+        if hip.gpu_compute_cpu_read_safe(c) != 1:
+            raise RuntimeError("GPU work still outstanding")
+        mem = self.ctx.recon_memory
+        if hip.gpu_compute_wait_for_image_ready_host(c, mem) != 0:
+            raise RuntimeError("recon surface not ready")
+        lay, unmap = Nv12Layout(), C.c_bool(False)
+        base = hip.gpu_compute_map_surface(c, C.byref(self.ctx.recon_image), mem, C.byref(lay), C.byref(unmap))
+        if not base:
+            raise RuntimeError("map_surface failed")
+        try:
+            im = self.ctx.recon_image
+            raw = np.ctypeslib.as_array((C.c_uint8 * lay.total_size).from_address(base))
+            y = raw[lay.y_offset:lay.y_offset + lay.y_pitch * im.coded_height].reshape(
+                im.coded_height, lay.y_pitch)[:, :im.coded_width].copy()
+            uv = raw[lay.uv_offset:lay.uv_offset + lay.uv_pitch * (im.coded_height // 2)].reshape(
+                im.coded_height // 2, lay.uv_pitch)[:, :im.coded_width].copy()
+        finally:
+            hip.gpu_compute_unmap_surface(c, mem, unmap)
         return y, uv
 
     def stage_times(self, slot):

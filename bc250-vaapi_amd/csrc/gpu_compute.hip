@@ -10,6 +10,7 @@
 // sync_slot waits on that event (vkWaitForFences, gpu_compute.c:3640-3712).
 #include <hip/hip_runtime.h>
 
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -74,6 +75,11 @@ struct bc250_hip_state {
 };
 
 static int align_up(int v, int a) { return (v + a - 1) / a * a; }
+
+// The caller allocates the context; its layout is part of the ABI.
+static_assert(sizeof(gpu_image_t) == 48 && sizeof(gpu_memory_t) == 24, "ABI struct size");
+static_assert(offsetof(bc250_gpu_context_t, recon_image) == 24 && sizeof(bc250_gpu_context_t) == 104,
+              "ABI context layout");
 
 // ------------------------------------------------------------ tiny kernels
 
@@ -198,6 +204,26 @@ static int ensure_res(bc250_hip_state *s, int w, int h)
 
 static bc250_hip_state *st_of(gpu_context_t *ctx) { return ctx ? ctx->hip : nullptr; }
 
+// Describes the newest reconstruction in ctx->recon_image / recon_memory, at
+// its coded size so that a map/unmap round trip pads nothing.
+static void publish_recon(gpu_context_t *ctx, bc250_hip_state *s)
+{
+    const Plane2 &r = s->recon[s->ref_idx];
+    gpu_image_t &im = ctx->recon_image;
+    im.y_plane = r.y;
+    im.uv_plane = r.uv;
+    im.width = im.coded_width = (uint32_t)s->wc;
+    im.height = im.coded_height = (uint32_t)s->hc;
+    im.y_pitch = (uint32_t)r.py;
+    im.uv_pitch = (uint32_t)r.puv;
+    im.current_layout = 0;
+    im.format = GPU_IMAGE_NV12;
+    ctx->recon_memory.memory = r.base;
+    ctx->recon_memory.size = (size_t)r.py * s->hc + (size_t)r.puv * (s->hc / 2);
+    ctx->recon_memory.mapped_ptr = nullptr;
+    ctx->has_recon_frame = true;
+}
+
 // ------------------------------------------------------------ lifecycle
 
 extern "C" int bc250_gpu_init(bc250_gpu_context_t *ctx)
@@ -224,6 +250,8 @@ extern "C" int bc250_gpu_init(bc250_gpu_context_t *ctx)
             }
     ctx->hip = s;
     ctx->frames_submitted = 0;
+    ctx->has_recon_frame = false;
+    ctx->any_surface_exported = false;
     return 0;
 }
 
@@ -239,6 +267,9 @@ extern "C" void bc250_gpu_destroy(bc250_gpu_context_t *ctx)
     hipStreamDestroy(s->stream);
     delete s;
     ctx->hip = nullptr;
+    ctx->recon_image = gpu_image_t{};
+    ctx->recon_memory = gpu_memory_t{};
+    ctx->has_recon_frame = false;
 }
 
 extern "C" void gpu_compute_report_cu_topology(bc250_gpu_context_t *ctx)
@@ -408,10 +439,16 @@ extern "C" int gpu_compute_import_dmabuf_image(gpu_context_t *, int, int, int, i
 {
     return -1;
 }
-// No foreign writers to order against: callers proceed on the -1, exactly as
-// with the reference when the capability is missing.
+// Surfaces here have no foreign writers (no dma-buf import) and uploads
+// complete before they return.  The GPU-side wait is therefore the
+// reference's "capability missing" no-op (-1: proceed); the host-side wait
+// reports the surface quiescent (0), which is what lets the CPU ME route read
+// it (own GPU work is gated separately by gpu_compute_cpu_read_safe).
 extern "C" int gpu_compute_wait_for_image_ready(gpu_context_t *, gpu_memory_t) { return -1; }
-extern "C" int gpu_compute_wait_for_image_ready_host(gpu_context_t *, gpu_memory_t) { return -1; }
+extern "C" int gpu_compute_wait_for_image_ready_host(gpu_context_t *ctx, gpu_memory_t memory)
+{
+    return st_of(ctx) && memory.memory ? 0 : -1;
+}
 
 extern "C" int gpu_compute_cpu_read_safe(gpu_context_t *ctx)
 {
@@ -457,6 +494,11 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     if (num_slices < 1) num_slices = 1;
     if (rt.format != GPU_IMAGE_NV12) return -1;  // 8-bit pipeline
     if (ensure_res(s, width, height) != 0) return -1;
+    if (!s->has_recon) {  // new resolution: no reconstruction yet
+        ctx->recon_image = gpu_image_t{};
+        ctx->recon_memory = gpu_memory_t{};
+        ctx->has_recon_frame = false;
+    }
     if (rt.coded_width != (uint32_t)s->wc || rt.coded_height != (uint32_t)s->hc) return -1;
     Slot &sl = s->slot[s->cur];
     hipStream_t st = s->stream;
@@ -529,6 +571,7 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     }
     s->ref_idx = out_idx;
     s->has_recon = true;
+    publish_recon(ctx, s);
     return 0;
 }
 
@@ -544,6 +587,11 @@ extern "C" int gpu_compute_dispatch_me_only(gpu_context_t *ctx, gpu_image_t rt, 
     bc250_hip_state *s = st_of(ctx);
     if (!s || !rt.y_plane || rt.format != GPU_IMAGE_NV12) return -1;
     if (ensure_res(s, width, height) != 0) return -1;
+    if (!s->has_recon) {  // new resolution: no reconstruction yet
+        ctx->recon_image = gpu_image_t{};
+        ctx->recon_memory = gpu_memory_t{};
+        ctx->has_recon_frame = false;
+    }
     Slot &sl = s->slot[s->cur];
     Plane2 cur_in;
     cur_in.y = (uint8_t *)rt.y_plane;

@@ -68,6 +68,17 @@ typedef struct bc250_gpu_context {
     uint32_t frames_submitted;
     uint32_t frame_width;
     uint32_t frame_height;
+    /* reference gpu_compute.h:241-242: the newest reconstruction (the next
+     * frame's reference picture), valid after the first dispatch.  Its
+     * width/height are the coded size; read it with map_surface after
+     * gpu_compute_cpu_read_safe() reports 1 (encoder_h264.c:2325-2340). */
+    gpu_image_t recon_image;
+    gpu_memory_t recon_memory;
+    /* reference gpu_compute.h:243 and :333, read by encoder_h264.c:2315,2334:
+     * a reconstruction exists; a surface was exported as dma-buf (never
+     * here, export is unsupported). */
+    bool has_recon_frame;
+    bool any_surface_exported;
 } bc250_gpu_context_t;
 
 typedef bc250_gpu_context_t gpu_context_t;

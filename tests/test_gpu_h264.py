@@ -266,6 +266,9 @@ def _run_sequence(pkg, w, h, n, qp, ns, rng, lam, seed, nrect, idc=0, gop=0):
             np.testing.assert_array_equal(gy, exp["y"][:h, :w], err_msg=f"frame {i} recon y")
             np.testing.assert_array_equal(guv, exp["uv"][:(h + 1) // 2, :2 * ((w + 1) // 2)],
                                           err_msg=f"frame {i} recon uv")
+            my, muv = enc.map_recon()  # ctx.recon_image, as the CPU ME route reads it
+            np.testing.assert_array_equal(my, exp["y"], err_msg=f"frame {i} mapped recon y")
+            np.testing.assert_array_equal(muv, exp["uv"], err_msg=f"frame {i} mapped recon uv")
             ref_y, ref_uv = exp["y"], exp["uv"]
             say(f"  frame {i} ok ({'I' if intra else 'P'})")
     finally:
