@@ -98,6 +98,30 @@ void orc_h264_encode_frame(const uint8_t *cy, const uint8_t *cuv,
                            int16_t *levels, int32_t *dc, uint32_t *nz,
                            uint32_t *modes, uint8_t *oy, uint8_t *ouv);
 
+/* --- HEVC stages (orc_hevc.c) --------------------------------------------- */
+int  orc_hevc_chroma_qp(int qp_luma);
+void orc_hevc_quant(const int32_t *v, int n, int shift, int q, int round_q12, int16_t *out);
+void orc_hevc_dequant(const int16_t *c, int n, int q, int bd_shift, int16_t *out);
+void orc_hevc_tq4(const int16_t res[16], int qp, int use_dst, int round_q12, int bd, int16_t out[16]);
+void orc_hevc_dqit4(const int16_t coef[16], int qp, int use_dst, int bd, int16_t out[16]);
+void orc_hevc_tq8(const int16_t res[64], int qp, int bd, int round_q12, int16_t out[64]);
+void orc_hevc_dqit8(const int16_t coef[64], int qp, int bd, int16_t out[64]);
+void orc_hevc_mc_uni(const void *ref, int stride, int w_pic, int h_pic, int x, int y, int w, int h,
+                     int mvx, int mvy, int chroma, int bd, void *dst, int dst_stride);
+void orc_hevc_predict4(const void *plane, int bd, int stride, int width, int height, int x0, int y0,
+                       int mode, int is_luma, int y_min, int32_t pred[16]);
+int  orc_hevc_choose_mode(const void *src, const void *recon, int bd, int stride, int width, int height,
+                          int x0, int y0, int y_min, int32_t pred[16]);
+int  orc_hevc_slice_y_min(int ctu_row, int ctu_rows, int ns);
+void orc_hevc_encode_i(const void *sy, const void *scb, const void *scr, void *ry, void *rcb, void *rcr,
+                       int bd, int wc, int hc, int qp, int num_slices, int round_q12, uint8_t *modes,
+                       int16_t *coef);
+void orc_hevc_encode_p(const void *sy, const void *scb, const void *scr, const void *fy, const void *fcb,
+                       const void *fcr, void *ry, void *rcb, void *rcr, int bd, int wc, int hc, int qp,
+                       int round_q12, const int32_t *mv, int16_t *coef);
+uint32_t orc_hevc_refine8(const void *cur, const void *ref, int bd, int wc, int hc, int x0, int y0,
+                          int *mvx, int *mvy);
+
 #ifdef __cplusplus
 }
 #endif

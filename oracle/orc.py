@@ -167,3 +167,114 @@ def encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lambda8=1
                               int(is_intra), num_slices, rng, lambda8, idc, p(o["mvs"]), p(o["mv8"]),
                               p(o["levels"]), p(o["dc"]), p(o["nz"]), p(o["modes"]), p(o["y"]), p(o["uv"]))
     return o
+
+
+# ------------------------------------------------------------------- HEVC
+REF_HEVC = os.path.join(_HERE, "_ref", "libref_hevc.so")
+
+_sig("orc_hevc_chroma_qp", _I, [_I])
+_sig("orc_hevc_quant", None, [_P, _I, _I, _I, _I, _P])
+_sig("orc_hevc_dequant", None, [_P, _I, _I, _I, _P])
+_sig("orc_hevc_tq4", None, [_P, _I, _I, _I, _I, _P])
+_sig("orc_hevc_dqit4", None, [_P, _I, _I, _I, _P])
+_sig("orc_hevc_tq8", None, [_P, _I, _I, _I, _P])
+_sig("orc_hevc_dqit8", None, [_P, _I, _I, _P])
+_sig("orc_hevc_mc_uni", None, [_P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _I])
+_sig("orc_hevc_predict4", None, [_P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P])
+_sig("orc_hevc_choose_mode", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P])
+_sig("orc_hevc_slice_y_min", _I, [_I, _I, _I])
+_sig("orc_hevc_encode_i", None, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P])
+_sig("orc_hevc_encode_p", None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P])
+_sig("orc_hevc_refine8", C.c_uint32, [_P, _P, _I, _I, _I, _I, _I, C.POINTER(_I), C.POINTER(_I)])
+
+HEVC_ROUND_INTER, HEVC_ROUND_INTRA_P, HEVC_ROUND_INTRA_I, HEVC_ROUND_NEAREST = 2, 3, 4, 6
+
+
+def _pix(bd):
+    return np.uint8 if bd == 8 else np.uint16
+
+
+def hevc_tq4(res, qp, use_dst, round_q12, bd=8):
+    r, o = np.ascontiguousarray(res, np.int16).reshape(16), np.zeros(16, np.int16)
+    lib.orc_hevc_tq4(p(r), qp, use_dst, round_q12, bd, p(o))
+    return o
+
+
+def hevc_dqit4(coef, qp, use_dst, bd=8):
+    c, o = np.ascontiguousarray(coef, np.int16).reshape(16), np.zeros(16, np.int16)
+    lib.orc_hevc_dqit4(p(c), qp, use_dst, bd, p(o))
+    return o
+
+
+def hevc_tq8(res, qp, bd, round_q12):
+    r, o = np.ascontiguousarray(res, np.int16).reshape(64), np.zeros(64, np.int16)
+    lib.orc_hevc_tq8(p(r), qp, bd, round_q12, p(o))
+    return o
+
+
+def hevc_dqit8(coef, qp, bd):
+    c, o = np.ascontiguousarray(coef, np.int16).reshape(64), np.zeros(64, np.int16)
+    lib.orc_hevc_dqit8(p(c), qp, bd, p(o))
+    return o
+
+
+def hevc_mc_uni(ref, x, y, w, h, mvx, mvy, chroma, bd=8):
+    ref = np.ascontiguousarray(ref, _pix(bd))
+    out = np.zeros((h, w), _pix(bd))
+    lib.orc_hevc_mc_uni(p(ref), ref.shape[1], ref.shape[1], ref.shape[0], x, y, w, h, mvx, mvy, chroma, bd,
+                        p(out), w)
+    return out
+
+
+def hevc_predict4(plane, x0, y0, mode, is_luma, y_min, bd=8):
+    plane = np.ascontiguousarray(plane, _pix(bd))
+    o = np.zeros(16, np.int32)
+    lib.orc_hevc_predict4(p(plane), bd, plane.shape[1], plane.shape[1], plane.shape[0], x0, y0, mode, is_luma,
+                          y_min, p(o))
+    return o
+
+
+def hevc_choose_mode(src, recon, x0, y0, y_min, bd=8):
+    src = np.ascontiguousarray(src, _pix(bd))
+    recon = np.ascontiguousarray(recon, _pix(bd))
+    o = np.zeros(16, np.int32)
+    m = lib.orc_hevc_choose_mode(p(src), p(recon), bd, src.shape[1], src.shape[1], src.shape[0], x0, y0, y_min,
+                                 p(o))
+    return m, o
+
+
+def split_uv(uv):
+    """Interleaved NV12 chroma -> separate (Cb, Cr) planes."""
+    return np.ascontiguousarray(uv[:, 0::2]), np.ascontiguousarray(uv[:, 1::2])
+
+
+def hevc_encode_i(sy, scb, scr, qp, num_slices=1, round_q12=HEVC_ROUND_INTRA_I, bd=8):
+    hc, wc = sy.shape
+    t = (wc // 16) * (hc // 16)
+    sy, scb, scr = (np.ascontiguousarray(a, _pix(bd)) for a in (sy, scb, scr))
+    ry, rcb, rcr = np.zeros_like(sy), np.zeros_like(scb), np.zeros_like(scr)
+    modes, coef = np.zeros(t * 16, np.uint8), np.zeros(t * 384, np.int16)
+    lib.orc_hevc_encode_i(p(sy), p(scb), p(scr), p(ry), p(rcb), p(rcr), bd, wc, hc, qp, num_slices, round_q12,
+                          p(modes), p(coef))
+    return dict(modes=modes, coef=coef, y=ry, cb=rcb, cr=rcr)
+
+
+def hevc_encode_p(sy, scb, scr, fy, fcb, fcr, qp, mv, round_q12=HEVC_ROUND_INTER, bd=8):
+    """mv: int32 [nctu*4][2] quarter-sample vectors, CUs in z-order per CTU."""
+    hc, wc = sy.shape
+    t = (wc // 16) * (hc // 16)
+    a = [np.ascontiguousarray(x, _pix(bd)) for x in (sy, scb, scr, fy, fcb, fcr)]
+    ry, rcb, rcr = np.zeros_like(a[0]), np.zeros_like(a[1]), np.zeros_like(a[2])
+    coef = np.zeros(t * 384, np.int16)
+    mv = np.ascontiguousarray(mv, np.int32).reshape(t * 4, 2)
+    lib.orc_hevc_encode_p(p(a[0]), p(a[1]), p(a[2]), p(a[3]), p(a[4]), p(a[5]), p(ry), p(rcb), p(rcr), bd, wc, hc,
+                          qp, round_q12, p(mv), p(coef))
+    return dict(coef=coef, y=ry, cb=rcb, cr=rcr)
+
+
+def hevc_refine8(cur, ref, x0, y0, mvx, mvy, bd=8):
+    cur = np.ascontiguousarray(cur, _pix(bd))
+    ref = np.ascontiguousarray(ref, _pix(bd))
+    mx, my = C.c_int(mvx), C.c_int(mvy)
+    sad = lib.orc_hevc_refine8(p(cur), p(ref), bd, cur.shape[1], cur.shape[0], x0, y0, C.byref(mx), C.byref(my))
+    return mx.value, my.value, sad

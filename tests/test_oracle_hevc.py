@@ -1,0 +1,388 @@
+"""Pins the HEVC half of the CPU oracle (oracle/orc_hevc.c) before it checks
+the HIP kernels.
+
+1. Against the reference's own functions, compiled from its sources into
+   oracle/_ref/libref_hevc.so (hevc_intra.c, hevc_mc.c, hevc_dec_tables.c):
+   4x4 DST-VII / DCT-II and 8x8 DCT-II with quantisation and
+   dequantisation + inverse, at 8 and 10 bits, every QP and rounding;
+   uni-prediction MC (8-tap luma, 4-tap chroma) at every fraction,
+   including blocks hanging over the picture edge; 4x4 intra prediction and
+   the SAD mode decision at every position class of the CTU16 / CU8 / PU4
+   z-scan, slice tops included.
+2. Whole I and P pictures: the oracle's frame coding against the same
+   reference functions driven in the reference's coding order
+   (hevc_enc_template.c: encode_ctu -> intra_trial; predict_cu ->
+   inter_residual with the 8x8 transform).
+3. Closed forms: DC-only round trips, the transforms' matrix products, the
+   quantiser as the inverse of the dequantiser, refinement never worse than
+   its start.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import orc
+
+need_ref = pytest.mark.skipif(not os.path.exists(orc.REF_HEVC), reason="oracle/_ref not built (reference tree absent)")
+_ref = None
+P, I = C.c_void_p, C.c_int
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        # lazy binding: the decoder file also references decoder-only
+        # functions that these entry points never call
+        lib = C.CDLL(orc.REF_HEVC, mode=os.RTLD_LAZY)
+        for n in ("hevc_transform_quant_4x4", "hevc_transform_quant_4x4_10"):
+            getattr(lib, n).argtypes = [P, I, I, I, P]
+        for n in ("hevc_dequant_itransform_4x4", "hevc_dequant_itransform_4x4_10"):
+            getattr(lib, n).argtypes = [P, I, I, P]
+        lib.hevc_transform_quant_8x8.argtypes = [P, I, I, I, P]
+        lib.hevc_dequant_itransform_8x8.argtypes = [P, I, I, P]
+        for n in ("hevc_mc_uni_8", "hevc_mc_uni_10"):
+            getattr(lib, n).argtypes = [P, I, I, I, I, I, I, I, I, I, I, P, I]
+        for n in ("hevc_predict_4x4", "hevc_predict_4x4_10"):
+            getattr(lib, n).argtypes = [P, I, I, I, I, I, I, I, I, P]
+        for n in ("hevc_choose_luma_mode", "hevc_choose_luma_mode_10"):
+            getattr(lib, n).argtypes = [I, P, P, I, I, I, I, I, P]
+            getattr(lib, n).restype = I
+        lib.hevc_chroma_qp_from_luma.argtypes = [I]
+        lib.hevc_chroma_qp_from_luma.restype = I
+        _ref = lib
+    return _ref
+
+
+def p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def pix(bd):
+    return np.uint8 if bd == 8 else np.uint16
+
+
+# ------------------------------------------------------------ transforms
+
+@need_ref
+@pytest.mark.parametrize("bd", [8, 10])
+def test_tq4_and_dqit4_match_reference(bd):
+    L = ref()
+    rs = np.random.default_rng(40 + bd)
+    vmax = (1 << bd) - 1
+    tq = L.hevc_transform_quant_4x4 if bd == 8 else L.hevc_transform_quant_4x4_10
+    dq = L.hevc_dequant_itransform_4x4 if bd == 8 else L.hevc_dequant_itransform_4x4_10
+    qps = range(0, 52) if bd == 8 else range(0, 64)
+    for qp in qps:
+        for dst in (0, 1):
+            for rnd in (2, 3, 4, 6):
+                for k in range(6):
+                    amp = [vmax, 40, 4, vmax][k % 4]
+                    res = rs.integers(-amp, amp + 1, 16).astype(np.int16)
+                    if k == 5:
+                        res[:] = rs.integers(-vmax, vmax + 1)  # constant block
+                    want = np.zeros(16, np.int16)
+                    tq(p(res), qp, dst, rnd, p(want))
+                    got = orc.hevc_tq4(res, qp, dst, rnd, bd)
+                    assert np.array_equal(got, want), (bd, qp, dst, rnd, res)
+                    back = np.zeros(16, np.int16)
+                    dq(p(want), qp, dst, p(back))
+                    assert np.array_equal(orc.hevc_dqit4(want, qp, dst, bd), back), (bd, qp, dst)
+    # levels far outside what a residual produces still dequantise the same
+    for qp in (0, 30, 51):
+        lv = rs.integers(-32768, 32768, 16).astype(np.int16)
+        back = np.zeros(16, np.int16)
+        dq(p(lv), qp, 1, p(back))
+        assert np.array_equal(orc.hevc_dqit4(lv, qp, 1, bd), back)
+
+
+@need_ref
+@pytest.mark.parametrize("bd", [8, 10])
+def test_tq8_and_dqit8_match_reference(bd):
+    L = ref()
+    rs = np.random.default_rng(80 + bd)
+    vmax = (1 << bd) - 1
+    for qp in range(0, 64, 1 if bd == 10 else 2):
+        for rnd in (2, 6):
+            for amp in (vmax, 30, 3):
+                res = rs.integers(-amp, amp + 1, 64).astype(np.int16)
+                want = np.zeros(64, np.int16)
+                L.hevc_transform_quant_8x8(p(res), qp, bd, rnd, p(want))
+                assert np.array_equal(orc.hevc_tq8(res, qp, bd, rnd), want), (bd, qp, rnd, amp)
+                back = np.zeros(64, np.int16)
+                L.hevc_dequant_itransform_8x8(p(want), qp, bd, p(back))
+                assert np.array_equal(orc.hevc_dqit8(want, qp, bd), back), (bd, qp)
+
+
+@need_ref
+def test_chroma_qp_matches_reference():
+    for q in range(-3, 60):
+        assert orc.lib.orc_hevc_chroma_qp(q) == ref().hevc_chroma_qp_from_luma(q)
+
+
+def test_dc_only_round_trip_is_exact():
+    """A constant residual v: forward (1, 8) shifts give coefficient 128*v at
+    (0,0) and nothing else; dequant + inverse at a QP fine enough returns v."""
+    for v in (-200, -17, -1, 1, 5, 99, 255):
+        res = np.full(16, v, np.int16)
+        lv = orc.hevc_tq4(res, 4, 0, 6)            # step 16*45 >> ... fine at QP 4
+        assert np.count_nonzero(lv[1:]) == 0
+        assert np.array_equal(orc.hevc_dqit4(lv, 4, 0), res)
+        lv8 = orc.hevc_tq8(np.full(64, v, np.int16), 4, 8, 6)
+        assert np.count_nonzero(lv8[1:]) == 0
+        assert np.array_equal(orc.hevc_dqit8(lv8, 4, 8), np.full(64, v, np.int16))
+
+
+def test_inverse_transform_is_matrix_product():
+    """Dequantised coefficients through the inverse: (M^T C M) with the two
+    rounding shifts of 8.6.4.2, for both 4x4 kinds and 8x8."""
+    dct4 = np.array([[64, 64, 64, 64], [83, 36, -36, -83], [64, -64, -64, 64], [36, -83, 83, -36]])
+    dst4 = np.array([[29, 55, 74, 84], [74, 74, 0, -74], [84, -29, -74, 55], [55, -84, 74, -29]])
+    n8 = np.arange(8)
+    # 8-point DCT-II from its definition: round(64*sqrt2*cos((2n+1)k*pi/16)) with the standard's tweaks
+    dct8 = np.array([[64] * 8, [89, 75, 50, 18, -18, -50, -75, -89], [83, 36, -36, -83, -83, -36, 36, 83],
+                     [75, -18, -89, -50, 50, 89, 18, -75], [64, -64, -64, 64, 64, -64, -64, 64],
+                     [50, -89, 18, 75, -75, -18, 89, -50], [36, -83, 83, -36, -36, 83, -83, 36],
+                     [18, -50, 75, -89, 89, -75, 50, -18]])
+    ideal = np.round(64 * np.sqrt(2) * np.cos((2 * n8[None, :] + 1) * n8[:, None] * np.pi / 16))
+    ideal[0] = 64
+    assert np.abs(dct8 - ideal).max() <= 2  # same basis as the real DCT
+    rs = np.random.default_rng(7)
+    for M, n, kind in ((dct4, 4, 0), (dst4, 4, 1), (dct8, 8, 2)):
+        for _ in range(50):
+            # 8.6.3 scaling with the flat list, levelScale[q % 6] = {40,45,51,57,64,72}
+            lv = rs.integers(-20, 21, n * n).astype(np.int16)
+            q = 4
+            ls = (40, 45, 51, 57, 64, 72)[q % 6]
+            bds = 5 if n == 4 else 6
+            d = np.clip((lv.astype(np.int64) * (ls << (q // 6 + 4)) + (1 << (bds - 1))) >> bds, -32768, 32767)
+            D = d.reshape(n, n)
+            g = np.clip((M.T @ D + 64) >> 7, -32768, 32767)
+            want = np.clip((g @ M + 2048) >> 12, -32768, 32767).reshape(-1)
+            got = orc.hevc_dqit4(lv, q, kind) if n == 4 else orc.hevc_dqit8(lv, q, 8)
+            assert np.array_equal(got, want)
+
+
+def test_quantiser_inverts_dequantiser():
+    """With rounding to nearest, quantising a dequantised level gives the
+    level back (the quantiser is the algebraic inverse of 8.6.3)."""
+    rs = np.random.default_rng(9)
+    for q in range(0, 52):
+        lv = rs.integers(-300, 301, 64).astype(np.int16)
+        d = np.zeros(64, np.int16)
+        orc.lib.orc_hevc_dequant(p(lv), 64, q, 5, p(d))
+        ok = np.abs(d.astype(int)) < 32767
+        back = np.zeros(64, np.int16)
+        orc.lib.orc_hevc_quant(p(d.astype(np.int32)), 64, 5, q, 6, p(back))
+        assert np.array_equal(back[ok], lv[ok]), q
+
+
+# ------------------------------------------------------------ MC
+
+@need_ref
+@pytest.mark.parametrize("bd", [8, 10])
+def test_mc_uni_matches_reference(bd):
+    L = ref()
+    f = L.hevc_mc_uni_8 if bd == 8 else L.hevc_mc_uni_10
+    rs = np.random.default_rng(3 + bd)
+    plane = rs.integers(0, 1 << bd, (40, 56)).astype(pix(bd))
+    for chroma in (0, 1):
+        nfrac = 8 if chroma else 4
+        for fy in range(nfrac):
+            for fx in range(nfrac):
+                for (x, y, w, h) in ((8, 8, 8, 8), (0, 0, 4, 4), (52, 36, 4, 4), (-9, 30, 8, 8), (20, -6, 16, 8)):
+                    mvx = int(rs.integers(-6, 7)) * nfrac + fx
+                    mvy = int(rs.integers(-6, 7)) * nfrac + fy
+                    want = np.zeros((h, w), pix(bd))
+                    f(p(plane), plane.shape[1], plane.shape[1], plane.shape[0], x, y, w, h, mvx, mvy, chroma,
+                      p(want), w)
+                    got = orc.hevc_mc_uni(plane, x, y, w, h, mvx, mvy, chroma, bd)
+                    assert np.array_equal(got, want), (bd, chroma, fx, fy, x, y, w, h)
+
+
+# ------------------------------------------------------------ intra
+
+def _positions(w, h):
+    out = []
+    for y0 in range(0, h, 4):
+        for x0 in range(0, w, 4):
+            out.append((x0, y0))
+    return out
+
+
+@need_ref
+@pytest.mark.parametrize("bd", [8, 10])
+def test_predict4_matches_reference(bd):
+    L = ref()
+    f = L.hevc_predict_4x4 if bd == 8 else L.hevc_predict_4x4_10
+    rs = np.random.default_rng(11 + bd)
+    h, w = 48, 64
+    plane = rs.integers(0, 1 << bd, (h, w)).astype(pix(bd))
+    for is_luma in (1, 0):
+        for y_min in (0, 16, 32):
+            for (x0, y0) in _positions(w, h):
+                for mode in (0, 1, 10, 26):
+                    want = np.zeros(16, pix(bd))
+                    f(p(plane), w, w, h, x0, y0, mode, is_luma, y_min, p(want))
+                    got = orc.hevc_predict4(plane, x0, y0, mode, is_luma, y_min, bd)
+                    assert np.array_equal(got, want.astype(np.int32)), (bd, is_luma, y_min, x0, y0, mode)
+
+
+@need_ref
+@pytest.mark.parametrize("bd", [8, 10])
+def test_choose_mode_matches_reference(bd):
+    L = ref()
+    f = L.hevc_choose_luma_mode if bd == 8 else L.hevc_choose_luma_mode_10
+    rs = np.random.default_rng(21 + bd)
+    h, w = 48, 64
+    for trial in range(3):
+        base = rs.integers(0, 1 << bd, (h, w))
+        recon = np.clip(base + rs.integers(-8, 9, (h, w)), 0, (1 << bd) - 1).astype(pix(bd))
+        src = np.clip(base + rs.integers(-20, 21, (h, w)), 0, (1 << bd) - 1).astype(pix(bd))
+        for (x0, y0) in _positions(w, h):
+            want = np.zeros(16, pix(bd))
+            m = f(16 * trial, p(src), p(recon), w, w, h, x0, y0, p(want))
+            gm, gp = orc.hevc_choose_mode(src, recon, x0, y0, 16 * trial, bd)
+            assert gm == m and np.array_equal(gp, want.astype(np.int32)), (bd, x0, y0)
+
+
+# ------------------------------------------------------------ whole pictures
+
+def _frame(rs, w, h, bd=8):
+    """Smooth content with texture, so the mode decision and the quantiser
+    both see varied input."""
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = 128 + 60 * np.sin(xx / 7.0) * np.cos(yy / 5.0)
+    y = np.clip(base + rs.normal(0, 6, (h, w)), 0, 255).astype(int)
+    cb = np.clip(128 + 30 * np.sin(xx[:h // 2, :w // 2] / 4.0) + rs.normal(0, 3, (h // 2, w // 2)), 0, 255).astype(int)
+    cr = np.clip(128 - 25 * np.cos(yy[:h // 2, :w // 2] / 3.0) + rs.normal(0, 3, (h // 2, w // 2)), 0, 255).astype(int)
+    sc = 1 if bd == 8 else 4
+    return [(a * sc).astype(pix(bd)) for a in (y, cb, cr)]
+
+
+def _ref_encode_i(sy, scb, scr, qp, ns, rnd, bd):
+    """encode_ctu's I branch with intra_trial, driven through the reference's
+    compiled functions in its coding order."""
+    L = ref()
+    choose = L.hevc_choose_luma_mode if bd == 8 else L.hevc_choose_luma_mode_10
+    pred4 = L.hevc_predict_4x4 if bd == 8 else L.hevc_predict_4x4_10
+    tq = L.hevc_transform_quant_4x4 if bd == 8 else L.hevc_transform_quant_4x4_10
+    dq = L.hevc_dequant_itransform_4x4 if bd == 8 else L.hevc_dequant_itransform_4x4_10
+    qoff = 0 if bd == 8 else 12
+    hc, wc = sy.shape
+    rows, cols = hc // 16, wc // 16
+    ry, rcb, rcr = np.zeros_like(sy), np.zeros_like(scb), np.zeros_like(scr)
+    modes, coef = np.zeros(rows * cols * 16, np.uint8), np.zeros(rows * cols * 384, np.int16)
+    vmax = (1 << bd) - 1
+    cqp = L.hevc_chroma_qp_from_luma(qp) + qoff
+    off = ((0, 0), (8, 0), (0, 8), (8, 8))
+    for s in range(ns):
+        r0, r1 = rows * s // ns, rows * (s + 1) // ns
+        y_min = r0 * 16
+        for ty in range(r0, r1):
+            for tx in range(cols):
+                t = ty * cols + tx
+                for cu in range(4):
+                    cx, cy = tx * 16 + off[cu][0], ty * 16 + off[cu][1]
+                    for pu in range(4):
+                        px, py = cx + off[pu][0] // 2, cy + off[pu][1] // 2
+                        pred = np.zeros(16, pix(bd))
+                        m = choose(y_min, p(sy), p(ry), wc, wc, hc, px, py, p(pred))
+                        modes[t * 16 + cu * 4 + pu] = m
+                        res = (sy[py:py + 4, px:px + 4].astype(np.int32).reshape(16) - pred).astype(np.int16)
+                        lv = np.zeros(16, np.int16)
+                        tq(p(res), qp + qoff, 1, rnd, p(lv))
+                        coef[t * 384 + (cu * 4 + pu) * 16:][:16] = lv
+                        rr = np.zeros(16, np.int16)
+                        if lv.any():
+                            dq(p(lv), qp + qoff, 1, p(rr))
+                        ry[py:py + 4, px:px + 4] = np.clip(pred.astype(int) + rr, 0, vmax).reshape(4, 4)
+                    ux, uy = cx // 2, cy // 2
+                    for comp, (s_, r_) in enumerate(((scb, rcb), (scr, rcr))):
+                        pred = np.zeros(16, pix(bd))
+                        pred4(p(r_), wc // 2, wc // 2, hc // 2, ux, uy, 1, 0, y_min // 2, p(pred))
+                        res = (s_[uy:uy + 4, ux:ux + 4].astype(np.int32).reshape(16) - pred).astype(np.int16)
+                        lv = np.zeros(16, np.int16)
+                        tq(p(res), cqp, 0, rnd, p(lv))
+                        coef[t * 384 + 256 + comp * 64 + cu * 16:][:16] = lv
+                        rr = np.zeros(16, np.int16)
+                        if lv.any():
+                            dq(p(lv), cqp, 0, p(rr))
+                        r_[uy:uy + 4, ux:ux + 4] = np.clip(pred.astype(int) + rr, 0, vmax).reshape(4, 4)
+    return dict(modes=modes, coef=coef, y=ry, cb=rcb, cr=rcr)
+
+
+@need_ref
+@pytest.mark.parametrize("bd,qp,ns", [(8, 22, 1), (8, 37, 2), (10, 27, 3)])
+def test_encode_i_matches_reference_order(bd, qp, ns):
+    rs = np.random.default_rng(qp)
+    sy, scb, scr = _frame(rs, 64, 48, bd)
+    want = _ref_encode_i(sy, scb, scr, qp, ns, orc.HEVC_ROUND_INTRA_I, bd)
+    got = orc.hevc_encode_i(sy, scb, scr, qp, ns, orc.HEVC_ROUND_INTRA_I, bd)
+    for k in ("modes", "coef", "y", "cb", "cr"):
+        assert np.array_equal(got[k], want[k]), k
+    assert len(set(got["modes"].tolist())) > 1  # the decision is exercised
+
+
+@need_ref
+@pytest.mark.parametrize("bd,qp", [(8, 27), (10, 32)])
+def test_encode_p_matches_reference_order(bd, qp):
+    L = ref()
+    mc = L.hevc_mc_uni_8 if bd == 8 else L.hevc_mc_uni_10
+    tq = L.hevc_transform_quant_4x4 if bd == 8 else L.hevc_transform_quant_4x4_10
+    dq = L.hevc_dequant_itransform_4x4 if bd == 8 else L.hevc_dequant_itransform_4x4_10
+    rs = np.random.default_rng(5)
+    sy, scb, scr = _frame(rs, 64, 48, bd)
+    fy, fcb, fcr = _frame(np.random.default_rng(6), 64, 48, bd)
+    hc, wc = sy.shape
+    t = (wc // 16) * (hc // 16)
+    mv = rs.integers(-40, 41, (t * 4, 2)).astype(np.int32)
+    got = orc.hevc_encode_p(sy, scb, scr, fy, fcb, fcr, qp, mv, orc.HEVC_ROUND_INTER, bd)
+    qoff = 0 if bd == 8 else 12
+    cqp = L.hevc_chroma_qp_from_luma(qp) + qoff
+    vmax = (1 << bd) - 1
+    off = ((0, 0), (8, 0), (0, 8), (8, 8))
+    for ti in range(t):
+        for cu in range(4):
+            cx, cy = (ti % (wc // 16)) * 16 + off[cu][0], (ti // (wc // 16)) * 16 + off[cu][1]
+            mx, my = (int(v) for v in mv[ti * 4 + cu])
+            pred = np.zeros(64, pix(bd))
+            mc(p(fy), wc, wc, hc, cx, cy, 8, 8, mx, my, 0, p(pred), 8)
+            res = (sy[cy:cy + 8, cx:cx + 8].astype(np.int32).reshape(64) - pred).astype(np.int16)
+            lv = np.zeros(64, np.int16)
+            L.hevc_transform_quant_8x8(p(res), qp + qoff, bd, orc.HEVC_ROUND_INTER, p(lv))
+            assert np.array_equal(got["coef"][ti * 384 + cu * 64:][:64], lv)
+            rr = np.zeros(64, np.int16)
+            if lv.any():
+                L.hevc_dequant_itransform_8x8(p(lv), qp + qoff, bd, p(rr))
+            assert np.array_equal(got["y"][cy:cy + 8, cx:cx + 8].reshape(64), np.clip(pred.astype(int) + rr, 0, vmax))
+            for comp, (s_, f_, r_) in enumerate(((scb, fcb, "cb"), (scr, fcr, "cr"))):
+                cp = np.zeros(16, pix(bd))
+                mc(p(f_), wc // 2, wc // 2, hc // 2, cx // 2, cy // 2, 4, 4, mx, my, 1, p(cp), 4)
+                cres = (s_[cy // 2:cy // 2 + 4, cx // 2:cx // 2 + 4].astype(np.int32).reshape(16) - cp).astype(np.int16)
+                clv = np.zeros(16, np.int16)
+                tq(p(cres), cqp, 0, orc.HEVC_ROUND_INTER, p(clv))
+                assert np.array_equal(got["coef"][ti * 384 + 256 + comp * 64 + cu * 16:][:16], clv)
+                crr = np.zeros(16, np.int16)
+                if clv.any():
+                    dq(p(clv), cqp, 0, p(crr))
+                assert np.array_equal(got[r_][cy // 2:cy // 2 + 4, cx // 2:cx // 2 + 4].reshape(16),
+                                      np.clip(cp.astype(int) + crr, 0, vmax))
+
+
+def test_refine8_never_worse_and_consistent():
+    rs = np.random.default_rng(12)
+    h, w = 48, 64
+    ref_ = np.clip(128 + 50 * np.sin(np.arange(w)[None, :] / 3.0 + np.arange(h)[:, None] / 4.0), 0, 255).astype(np.uint8)
+    cur = np.roll(ref_, (1, 2), axis=(0, 1))
+    for (x0, y0) in ((8, 8), (24, 16), (40, 32)):
+        for (dx, dy) in ((0, 0), (-2, -1), (1, 1)):
+            mx, my, sad = orc.hevc_refine8(cur, ref_, x0, y0, dx * 4, dy * 4)
+            start = np.abs(cur[y0:y0 + 8, x0:x0 + 8].astype(int) -
+                           orc.hevc_mc_uni(ref_, x0, y0, 8, 8, dx * 4, dy * 4, 0).astype(int)).sum()
+            assert sad <= start
+            again = np.abs(cur[y0:y0 + 8, x0:x0 + 8].astype(int) -
+                           orc.hevc_mc_uni(ref_, x0, y0, 8, 8, mx, my, 0).astype(int)).sum()
+            assert again == sad
+            assert abs(mx - dx * 4) <= 3 and abs(my - dy * 4) <= 3
