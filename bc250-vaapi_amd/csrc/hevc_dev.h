@@ -1,0 +1,81 @@
+// hevc_dev.h - device helpers for the HEVC stages (8-bit samples).
+//
+// Arithmetic follows H.265 and the reference encoder's chosen forward
+// shifts (enc/src/hevc_intra.c):
+//   4x4 forward  (1, 8)        inverse (7, 12)      dequant bdShift 5
+//   8x8 forward  (2, 9)        inverse (7, 12)      dequant bdShift 6
+// with int16 saturation after every pass, quantisation
+//   level = min(32767, floor(((|v| << shift) + off) / (ls << (q/6 + 4))))
+// and the flat-list dequantiser of 8.6.3.
+#pragma once
+#include "bc250_dev.h"
+
+namespace hevc {
+
+// 8.6.4.2 transMatrix rows for nTbS 4 (DCT-II and the DST-VII of intra
+// luma 4x4) and 8, and 8.5.3.3.3 interpolation filters: the standard's
+// constants.
+static __constant__ int8_t kDct4[16] = {64, 64, 64, 64, 83, 36, -36, -83, 64, -64, -64, 64, 36, -83, 83, -36};
+static __constant__ int8_t kDst4[16] = {29, 55, 74, 84, 74, 74, 0, -74, 84, -29, -74, 55, 55, -84, 74, -29};
+static __constant__ int8_t kDct8[64] = {64, 64,  64,  64,  64,  64,  64,  64,  89, 75,  50,  18,  -18, -50, -75, -89,
+                                 83, 36,  -36, -83, -83, -36, 36,  83,  75, -18, -89, -50, 50,  89,  18,  -75,
+                                 64, -64, -64, 64,  64,  -64, -64, 64,  50, -89, 18,  75,  -75, -18, 89,  -50,
+                                 36, -83, 83,  -36, -36, 83,  -83, 36,  18, -50, 75,  -89, 89,  -75, 50,  -18};
+static __constant__ int8_t kQpel[4][8] = {
+    {0, 0, 0, 0, 0, 0, 0, 0}, {-1, 4, -10, 58, 17, -5, 1, 0}, {-1, 4, -11, 40, 40, -11, 4, -1}, {0, 1, -5, 17, 58, -10, 4, -1}};
+static __constant__ int8_t kEpel[8][4] = {{0, 0, 0, 0},    {-2, 58, 10, -2}, {-4, 54, 16, -2}, {-6, 46, 28, -4},
+                                   {-4, 36, 36, -4}, {-4, 28, 46, -6}, {-2, 16, 54, -4}, {-2, 10, 58, -2}};
+
+BC_DEV int sat16(int v) { return v > 32767 ? 32767 : (v < -32768 ? -32768 : v); }
+
+BC_DEV int level_scale(int r)
+{
+    return r == 0 ? 40 : r == 1 ? 45 : r == 2 ? 51 : r == 3 ? 57 : r == 4 ? 64 : 72;
+}
+
+BC_DEV int chroma_qp(int qp)
+{
+    const int q = clampi(qp, 0, 57);
+    if (q < 30) return q;
+    if (q > 43) return q - 6;
+    const int mid[14] = {29, 30, 31, 32, 33, 33, 34, 34, 35, 35, 36, 36, 37, 37};  // Table 8-10
+    return mid[q - 30];
+}
+
+// Per-QP quantiser constants (uniform across a launch).
+struct Quant {
+    int p, shift;
+    uint32_t d, off, m;  // divisor, rounding offset, ceil(2^32 / d)
+};
+
+BC_DEV Quant make_quant(int q, int shift, int round_q12)
+{
+    Quant Q;
+    Q.p = q / 6 + 4;
+    Q.shift = shift;
+    Q.d = (uint32_t)level_scale(q % 6);
+    Q.off = (uint32_t)((((uint64_t)Q.d << Q.p) * (uint64_t)round_q12) / 12u);
+    Q.m = (uint32_t)(((1ull << 32) + Q.d - 1) / Q.d);
+    return Q;
+}
+
+// floor(t / d) as a multiply-high: exact because t < 2^21 and m*d - 2^32 < d.
+BC_DEV int quant(int v, const Quant &Q)
+{
+    const uint32_t mag = (uint32_t)(v < 0 ? -v : v);
+    const uint32_t t = ((mag << Q.shift) + Q.off) >> Q.p;
+    uint32_t l = __umulhi(t, Q.m);
+    l = l > 32767u ? 32767u : l;
+    return v < 0 ? -(int)l : (int)l;
+}
+
+// 8.6.3 flat-list scaling, kept in 32 bits by splitting on a >= bdShift.
+BC_DEV int dequant(int c, int q, int bds)
+{
+    const int a = q / 6 + 4;
+    const int prod = c * level_scale(q % 6);
+    const int v = a >= bds ? prod * (1 << (a - bds)) : (prod + (1 << (bds - a - 1))) >> (bds - a);
+    return sat16(v);
+}
+
+}  // namespace hevc

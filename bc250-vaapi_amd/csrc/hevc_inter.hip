@@ -1,0 +1,399 @@
+// hevc_inter.hip - HEVC transform / quant and motion-compensation stages for
+// gfx950: batched forms of the reference's per-block functions, the
+// quarter-sample refinement of the full-search 8x8 vectors, and the fused
+// P-picture CU coding (MC, 8x8 luma / 4x4 chroma DCT, quantisation,
+// dequantisation, inverse, reconstruction), one wave per 8x8 CU.
+//
+// Transforms run as one lane per coefficient: each pass is a dot product of a
+// matrix row with a column held in LDS, then the rounding shift and int16
+// saturation the reference applies after every pass.
+#include "bc250_hevc.h"
+#include "hevc_dev.h"
+
+using namespace hevc;
+
+namespace {
+
+// Bit-depth dependent shifts of the reference's transforms.
+struct Depth {
+    int s1_4, qsh4, s1_8, qsh8, s2inv, maxv, qmax4;
+};
+BC_DEV Depth depth_of(int bd)
+{
+    Depth d;
+    d.s1_4 = bd == 8 ? 1 : 3;    // 4x4 forward first pass
+    d.qsh4 = bd + 2 - 5;         // 4x4 bdShift = BitDepth + log2(4) - 5
+    d.s1_8 = bd - 6;             // 8x8 forward first pass
+    d.qsh8 = bd + 3 - 5;         // 8x8 bdShift
+    d.s2inv = 20 - bd;           // inverse second pass
+    d.maxv = (1 << bd) - 1;
+    d.qmax4 = bd == 8 ? 51 : 63;  // forward 4x4 QP clamp
+    return d;
+}
+
+// One n x n forward or inverse pass for the coefficient this lane owns:
+// lane (i, j) -> sat16((sum_k M[i][k] * X[k][j] + rnd) >> s)  (forward rows of
+// M; `transposed` uses M[k][i] instead).  X row-major in LDS.
+template <int N>
+BC_DEV int pass(const int8_t *M, bool transposed, const int *X, int i, int j, int s)
+{
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) acc += (int)(transposed ? M[k * N + i] : M[i * N + k]) * X[k * N + j];
+    return sat16((acc + (1 << (s - 1))) >> s);
+}
+
+// Second pass of a separable transform applied along rows: lane (i, j) ->
+// sat16((sum_k M[j][k] * T[i][k] + rnd) >> s), transposed as above.
+template <int N>
+BC_DEV int pass_rows(const int8_t *M, bool transposed, const int *T, int i, int j, int s)
+{
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) acc += (int)(transposed ? M[k * N + j] : M[j * N + k]) * T[i * N + k];
+    return sat16((acc + (1 << (s - 1))) >> s);
+}
+
+// HEVC 8-tap luma / 4-tap chroma uni-prediction of one sample from an int
+// window W (pitch P) whose (wx, wy) is the sample's integer position, with
+// the reference's intermediate int16 stores; returns the clipped sample.
+template <bool CHROMA>
+BC_DEV int mc_sample(const int *W, int P, int wx, int wy, int fx, int fy, int bd)
+{
+    constexpr int NT = CHROMA ? 4 : 8, BACK = CHROMA ? 1 : 3;
+    const int8_t *fh = CHROMA ? kEpel[fx] : kQpel[fx];
+    const int8_t *fv = CHROMA ? kEpel[fy] : kQpel[fy];
+    const int down = bd - 8;
+    if (!fx && !fy) return W[wy * P + wx];
+    int p14;
+    if (!fy) {
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < NT; k++) s += fh[k] * W[wy * P + wx + k - BACK];
+        p14 = (int16_t)(s >> down);
+    } else if (!fx) {
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < NT; k++) s += fv[k] * W[(wy + k - BACK) * P + wx];
+        p14 = (int16_t)(s >> down);
+    } else {
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < NT; k++) {
+            int t = 0;
+#pragma unroll
+            for (int j = 0; j < NT; j++) t += fh[j] * W[(wy + k - BACK) * P + wx + j - BACK];
+            s += fv[k] * (int)(int16_t)(t >> down);
+        }
+        p14 = (int16_t)(s >> 6);
+    }
+    return clampi((p14 + (1 << (13 - bd))) >> (14 - bd), 0, (1 << bd) - 1);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batched 4x4 / 8x8 transform + quant and dequant + inverse: one wave per
+// four 4x4 blocks (lane = block*16 + coefficient) or per 8x8 block.
+__global__ __launch_bounds__(256) void hevc_tq4_kernel(const int16_t *__restrict__ res, int n, int q, int use_dst,
+                                                       int round_q12, int bd, int16_t *__restrict__ out)
+{
+    __shared__ int X[4][64], T[4][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = (blockIdx.x * 4 + wv) * 4 + (lane >> 4), k = lane & 15;
+    const int i = k >> 2, j = k & 3, base = (lane >> 4) * 16;
+    const Depth D = depth_of(bd);
+    const int8_t *M = use_dst ? kDst4 : kDct4;
+    X[wv][lane] = b < n ? res[(size_t)b * 16 + k] : 0;
+    __syncthreads();
+    T[wv][lane] = pass<4>(M, false, &X[wv][base], i, j, D.s1_4);
+    __syncthreads();
+    const int v = pass_rows<4>(M, false, &T[wv][base], i, j, 8);
+    const Quant Q = make_quant(clampi(q, 0, D.qmax4), D.qsh4, round_q12);
+    if (b < n) out[(size_t)b * 16 + k] = (int16_t)quant(v, Q);
+}
+
+__global__ __launch_bounds__(256) void hevc_dqit4_kernel(const int16_t *__restrict__ coef, int n, int q, int use_dst,
+                                                         int bd, int16_t *__restrict__ out)
+{
+    __shared__ int X[4][64], T[4][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = (blockIdx.x * 4 + wv) * 4 + (lane >> 4), k = lane & 15;
+    const int i = k >> 2, j = k & 3, base = (lane >> 4) * 16;
+    const Depth D = depth_of(bd);
+    const int8_t *M = use_dst ? kDst4 : kDct4;
+    const int qq = bd == 8 ? q : clampi(q, 0, 63);
+    X[wv][lane] = b < n ? dequant(coef[(size_t)b * 16 + k], qq, D.qsh4) : 0;
+    __syncthreads();
+    T[wv][lane] = pass<4>(M, true, &X[wv][base], i, j, 7);
+    __syncthreads();
+    const int v = pass_rows<4>(M, true, &T[wv][base], i, j, D.s2inv);
+    if (b < n) out[(size_t)b * 16 + k] = (int16_t)v;
+}
+
+__global__ __launch_bounds__(256) void hevc_tq8_kernel(const int16_t *__restrict__ res, int n, int q, int bd,
+                                                       int round_q12, int16_t *__restrict__ out)
+{
+    __shared__ int X[4][64], T[4][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + wv, i = lane >> 3, j = lane & 7;
+    const Depth D = depth_of(bd);
+    X[wv][lane] = b < n ? res[(size_t)b * 64 + lane] : 0;
+    __syncthreads();
+    T[wv][lane] = pass<8>(kDct8, false, X[wv], i, j, D.s1_8);
+    __syncthreads();
+    const int v = pass_rows<8>(kDct8, false, T[wv], i, j, 9);
+    const Quant Q = make_quant(clampi(q, 0, 63), D.qsh8, round_q12);
+    if (b < n) out[(size_t)b * 64 + lane] = (int16_t)quant(v, Q);
+}
+
+__global__ __launch_bounds__(256) void hevc_dqit8_kernel(const int16_t *__restrict__ coef, int n, int q, int bd,
+                                                         int16_t *__restrict__ out)
+{
+    __shared__ int X[4][64], T[4][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + wv, i = lane >> 3, j = lane & 7;
+    const Depth D = depth_of(bd);
+    X[wv][lane] = b < n ? dequant(coef[(size_t)b * 64 + lane], clampi(q, 0, 63), D.qsh8) : 0;
+    __syncthreads();
+    T[wv][lane] = pass<8>(kDct8, true, X[wv], i, j, 7);
+    __syncthreads();
+    const int v = pass_rows<8>(kDct8, true, T[wv], i, j, D.s2inv);
+    if (b < n) out[(size_t)b * 64 + lane] = (int16_t)v;
+}
+
+// Batched uni-prediction: one wave per block (w*h <= 256 samples, each lane
+// up to 4 of them); the block's reference window (w+7) x (h+7) in LDS.
+template <typename Pix>
+__global__ __launch_bounds__(64) void hevc_mc_batch_kernel(const Pix *__restrict__ ref, int stride, int w_pic,
+                                                          int h_pic, const int4 *__restrict__ blk, int n, int w,
+                                                          int h, int chroma, int bd, Pix *__restrict__ dst)
+{
+    __shared__ int W[23 * 23];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    if (b >= n) return;
+    const int4 d = blk[b];
+    const int steps = chroma ? 3 : 2, back = chroma ? 1 : 3;
+    const int ix = d.x + (d.z >> steps), iy = d.y + (d.w >> steps);
+    const int fx = d.z & ((1 << steps) - 1), fy = d.w & ((1 << steps) - 1);
+    const int P = w + 7;
+    for (int t = lane; t < P * (h + 7); t += 64) {
+        const int r = t / P, c = t - r * P;
+        W[t] = ref[(size_t)clampi(iy - back + r, 0, h_pic - 1) * stride + clampi(ix - back + c, 0, w_pic - 1)];
+    }
+    __syncthreads();
+    for (int t = lane; t < w * h; t += 64) {
+        const int r = t / w, c = t - r * w;
+        const int v = chroma ? mc_sample<true>(W, P, c + back, r + back, fx, fy, bd)
+                             : mc_sample<false>(W, P, c + back, r + back, fx, fy, bd);
+        dst[(size_t)b * w * h + t] = (Pix)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Quarter-sample refinement, one wave per 8x8 CU, 4 CUs (one CTU) per
+// workgroup.  Every candidate lies within 3 quarter samples of the integer
+// vector, so all taps fall in a 16x16 window at (x0+dx-4, y0+dy-4).
+__global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__restrict__ cur, int cpitch,
+                                                           const uint8_t *__restrict__ ref, int rpitch, int wc, int hc,
+                                                           int nctu, const uint4 *__restrict__ imv8,
+                                                           uint4 *__restrict__ out)
+{
+    __shared__ int win[4][16 * 16];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ctu = xcd_item(blockIdx.x, nctu);
+    if (ctu >= nctu) return;
+    const int cu = ctu * 4 + wv, mbw = wc / 16;
+    const int x0 = (ctu % mbw) * 16 + (wv & 1) * 8, y0 = (ctu / mbw) * 16 + (wv >> 1) * 8;
+    const uint4 iv = imv8[cu];
+    const int dx = (int)iv.x, dy = (int)iv.y;
+    const int ox = x0 + dx - 4, oy = y0 + dy - 4;
+    int *W = win[wv];
+    for (int t = lane; t < 256; t += 64)
+        W[t] = ref[(size_t)clampi(oy + (t >> 4), 0, hc - 1) * rpitch + clampi(ox + (t & 15), 0, wc - 1)];
+    const int pr = lane >> 3, pc = lane & 7;
+    const int cv = cur[(size_t)(y0 + pr) * cpitch + x0 + pc];
+    wave_lds_sync();
+    auto sad_of = [&](int mx, int my) {
+        const int wx = pc + (mx >> 2) - dx + 4, wy = pr + (my >> 2) - dy + 4;
+        const int pv = mc_sample<false>(W, 16, wx, wy, mx & 3, my & 3, 8);
+        return wave_sum_u32((uint32_t)iabs(cv - pv));
+    };
+    const int hx[8] = {2, -2, 0, 0, 2, 2, -2, -2}, hy[8] = {0, 0, 2, -2, 2, -2, 2, -2};
+    int bx = dx * 4, by = dy * 4;
+    uint32_t best = sad_of(bx, by);
+#pragma unroll 1
+    for (int stage = 1; stage <= 2; stage++) {
+        const int cx = bx, cy = by;
+#pragma unroll 1
+        for (int k = 0; k < 8; k++) {
+            const int mx = cx + (stage == 1 ? hx[k] : hx[k] / 2), my = cy + (stage == 1 ? hy[k] : hy[k] / 2);
+            const uint32_t s = sad_of(mx, my);
+            if (s < best) {
+                best = s;
+                bx = mx;
+                by = my;
+            }
+        }
+    }
+    if (lane == 0) out[cu] = make_uint4((uint32_t)bx, (uint32_t)by, best, 0u);
+}
+
+// ---------------------------------------------------------------------------
+// P-picture CU coding, one wave per 8x8 CU (lane = luma sample / coefficient;
+// lanes 0..31 also carry the Cb and Cr 4x4 blocks), 4 CUs per workgroup.
+struct PcuLds {
+    int wl[15 * 15];    // luma reference window at (x0 + mvx>>2 - 3, y0 + mvy>>2 - 3)
+    int wcr[2][7 * 7];  // Cb / Cr windows at (x0/2 + mvx>>3 - 1, ...)
+    int a[64], b[64];   // luma residual -> pass 1 -> dequantised -> pass 1 (inverse)
+    int ca[32], cb[32];
+};
+
+__global__ __launch_bounds__(256) void hevc_p_encode_kernel(
+    const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, const uint8_t *__restrict__ ry,
+    const uint8_t *__restrict__ ruv, int rpy, int rpuv, uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy,
+    int opuv, int wc, int hc, int nctu, int qp, int round_q12, const uint4 *__restrict__ mvs,
+    int16_t *__restrict__ coef)
+{
+    __shared__ PcuLds Ls[4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ctu = xcd_item(blockIdx.x, nctu);
+    if (ctu >= nctu) return;
+    PcuLds &L = Ls[wv];
+    const int cu = ctu * 4 + wv, mbw = wc / 16;
+    const int x0 = (ctu % mbw) * 16 + (wv & 1) * 8, y0 = (ctu / mbw) * 16 + (wv >> 1) * 8;
+    const int ux0 = x0 >> 1, uy0 = y0 >> 1, wcc = wc >> 1, hcc = hc >> 1;
+    const uint4 mv = mvs[cu];
+    const int mvx = (int)mv.x, mvy = (int)mv.y;
+    const int lx = x0 + (mvx >> 2) - 3, ly = y0 + (mvy >> 2) - 3;
+    const int cx = ux0 + (mvx >> 3) - 1, cyy = uy0 + (mvy >> 3) - 1;
+    for (int t = lane; t < 225; t += 64)
+        L.wl[t] = ry[(size_t)clampi(ly + t / 15, 0, hc - 1) * rpy + clampi(lx + t % 15, 0, wc - 1)];
+    for (int t = lane; t < 98; t += 64) {
+        const int comp = t >= 49, k = t - comp * 49;
+        L.wcr[comp][k] = ruv[(size_t)clampi(cyy + k / 7, 0, hcc - 1) * rpuv + 2 * clampi(cx + k % 7, 0, wcc - 1) + comp];
+    }
+    const int pr = lane >> 3, pc = lane & 7;
+    const int cur = cy[(size_t)(y0 + pr) * cpy + x0 + pc];
+    const bool cl = lane < 32;
+    const int comp = (lane >> 4) & 1, ck = lane & 15, cr_ = ck >> 2, cc_ = ck & 3;
+    const int ccur = cl ? cuv[(size_t)(uy0 + cr_) * cpuv + 2 * (ux0 + cc_) + comp] : 0;
+    wave_lds_sync();
+
+    // prediction and residual
+    const int pred = mc_sample<false>(L.wl, 15, pc + 3, pr + 3, mvx & 3, mvy & 3, 8);
+    const int cpred = cl ? mc_sample<true>(L.wcr[comp], 7, cc_ + 1, cr_ + 1, mvx & 7, mvy & 7, 8) : 0;
+    L.a[lane] = cur - pred;
+    if (cl) L.ca[lane] = ccur - cpred;
+    wave_lds_sync();
+
+    // forward: columns, then rows
+    L.b[lane] = pass<8>(kDct8, false, L.a, pr, pc, 2);
+    if (cl) L.cb[lane] = pass<4>(kDct4, false, &L.ca[comp * 16], cr_, cc_, 1);
+    wave_lds_sync();
+    const int q = clampi(qp, 0, 63), cq = hevc::chroma_qp(qp);
+    const int lv = quant(pass_rows<8>(kDct8, false, L.b, pr, pc, 9), make_quant(q, 6, round_q12));
+    const int clv = cl ? quant(pass_rows<4>(kDct4, false, &L.cb[comp * 16], cr_, cc_, 8), make_quant(cq, 5, round_q12))
+                       : 0;
+    int16_t *co = coef + (size_t)ctu * 384;
+    co[wv * 64 + lane] = (int16_t)lv;
+    if (cl) co[256 + comp * 64 + wv * 16 + ck] = (int16_t)clv;
+
+    // dequantise, inverse: columns, then rows
+    L.a[lane] = dequant(lv, q, 6);
+    if (cl) L.ca[lane] = dequant(clv, cq, 5);
+    wave_lds_sync();
+    L.b[lane] = pass<8>(kDct8, true, L.a, pr, pc, 7);
+    if (cl) L.cb[lane] = pass<4>(kDct4, true, &L.ca[comp * 16], cr_, cc_, 7);
+    wave_lds_sync();
+    const int r = pass_rows<8>(kDct8, true, L.b, pr, pc, 12);
+    oy[(size_t)(y0 + pr) * opy + x0 + pc] = (uint8_t)clampi(pred + r, 0, 255);
+    if (cl) {
+        const int crr = pass_rows<4>(kDct4, true, &L.cb[comp * 16], cr_, cc_, 12);
+        ouv[(size_t)(uy0 + cr_) * opuv + 2 * (ux0 + cc_) + comp] = (uint8_t)clampi(cpred + crr, 0, 255);
+    }
+}
+
+// ---------------------------------------------------------------------------
+static int launch_ok() { return hipGetLastError() == hipSuccess ? 0 : -1; }
+static bool bd_ok(int bd) { return bd == 8 || bd == 10; }
+
+extern "C" int bc250_hevc_tq4_batch(const int16_t *res, int n, int qp, int use_dst, int round_q12, int bd,
+                                    int16_t *out, void *stream)
+{
+    if (n < 0 || !bd_ok(bd) || round_q12 < 0 || round_q12 > 12) return -1;
+    if (n == 0) return 0;
+    hevc_tq4_kernel<<<(n + 15) / 16, 256, 0, (hipStream_t)stream>>>(res, n, qp, use_dst, round_q12, bd, out);
+    return launch_ok();
+}
+
+extern "C" int bc250_hevc_dqit4_batch(const int16_t *coef, int n, int qp, int use_dst, int bd, int16_t *out,
+                                      void *stream)
+{
+    if (n < 0 || !bd_ok(bd) || qp < 0 || qp > 63) return -1;
+    if (n == 0) return 0;
+    hevc_dqit4_kernel<<<(n + 15) / 16, 256, 0, (hipStream_t)stream>>>(coef, n, qp, use_dst, bd, out);
+    return launch_ok();
+}
+
+extern "C" int bc250_hevc_tq8_batch(const int16_t *res, int n, int qp, int bd, int round_q12, int16_t *out,
+                                    void *stream)
+{
+    if (n < 0 || !bd_ok(bd) || round_q12 < 0 || round_q12 > 12) return -1;
+    if (n == 0) return 0;
+    hevc_tq8_kernel<<<(n + 3) / 4, 256, 0, (hipStream_t)stream>>>(res, n, qp, bd, round_q12, out);
+    return launch_ok();
+}
+
+extern "C" int bc250_hevc_dqit8_batch(const int16_t *coef, int n, int qp, int bd, int16_t *out, void *stream)
+{
+    if (n < 0 || !bd_ok(bd)) return -1;
+    if (n == 0) return 0;
+    hevc_dqit8_kernel<<<(n + 3) / 4, 256, 0, (hipStream_t)stream>>>(coef, n, qp, bd, out);
+    return launch_ok();
+}
+
+extern "C" int bc250_hevc_mc_uni_batch(const void *ref, int stride, int w_pic, int h_pic, const int32_t *blk, int n,
+                                       int w, int h, int chroma, int bd, void *dst, void *stream)
+{
+    if (n < 0 || w < 1 || h < 1 || w > 16 || h > 16 || w_pic < 1 || h_pic < 1 || stride < w_pic || !bd_ok(bd))
+        return -1;
+    if (n == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (bd == 8)
+        hevc_mc_batch_kernel<uint8_t><<<n, 64, 0, st>>>((const uint8_t *)ref, stride, w_pic, h_pic,
+                                                        (const int4 *)blk, n, w, h, chroma ? 1 : 0, 8,
+                                                        (uint8_t *)dst);
+    else
+        hevc_mc_batch_kernel<uint16_t><<<n, 64, 0, st>>>((const uint16_t *)ref, stride, w_pic, h_pic,
+                                                         (const int4 *)blk, n, w, h, chroma ? 1 : 0, 10,
+                                                         (uint16_t *)dst);
+    return launch_ok();
+}
+
+static int grid_xcd(int items) { return ((items + 7) / 8) * 8; }
+
+extern "C" int bc250_hevc_launch_refine8(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
+                                         int hc, const uint32_t *imv8, uint32_t *out, void *stream)
+{
+    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || cpitch < wc || rpitch < wc) return -1;
+    const int nctu = (wc / 16) * (hc / 16);
+    hevc_refine8_kernel<<<grid_xcd(nctu), 256, 0, (hipStream_t)stream>>>(cur, cpitch, ref, rpitch, wc, hc, nctu,
+                                                                         (const uint4 *)imv8, (uint4 *)out);
+    return launch_ok();
+}
+
+extern "C" int bc250_hevc_launch_p_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv,
+                                          const uint8_t *ry, const uint8_t *ruv, int rpy, int rpuv, uint8_t *oy,
+                                          uint8_t *ouv, int opy, int opuv, int wc, int hc, int qp, int round_q12,
+                                          const uint32_t *mvs, int16_t *coef, void *stream)
+{
+    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || round_q12 < 0 || round_q12 > 12 ||
+        cpy < wc || cpuv < wc || rpy < wc || rpuv < wc || opy < wc || opuv < wc)
+        return -1;
+    const int nctu = (wc / 16) * (hc / 16);
+    hevc_p_encode_kernel<<<grid_xcd(nctu), 256, 0, (hipStream_t)stream>>>(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy,
+                                                                          ouv, opy, opuv, wc, hc, nctu, qp, round_q12,
+                                                                          (const uint4 *)mvs, coef);
+    return launch_ok();
+}

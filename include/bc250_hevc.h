@@ -1,0 +1,88 @@
+/*
+ * bc250_hevc.h - HEVC stages of the encoder compute pipeline on gfx950.
+ *
+ * The reference codes HEVC on the CPU (enc/src/hevc_enc_template.c with
+ * enc/src/hevc_intra.c and the decoder's enc/src/hevc_mc_template.c); it has
+ * no GPU interface for it.  These entry points move those per-block
+ * functions onto the GPU as batched launches with the same argument meaning
+ * and results, plus frame-level launches that code a whole picture:
+ *
+ *   per-block functions (batched)            reference function replaced
+ *   bc250_hevc_tq4_batch                     hevc_transform_quant_4x4[_10]  (hevc_intra.h:132,155)
+ *   bc250_hevc_dqit4_batch                   hevc_dequant_itransform_4x4[_10] (hevc_intra.h:139,157)
+ *   bc250_hevc_tq8_batch                     hevc_transform_quant_8x8        (hevc_intra.h:162)
+ *   bc250_hevc_dqit8_batch                   hevc_dequant_itransform_8x8     (hevc_intra.h:164)
+ *   bc250_hevc_mc_uni_batch                  hevc_mc_uni_8 / hevc_mc_uni_10  (hevc_inter.h:26-31)
+ *
+ *   picture-level launches
+ *   bc250_hevc_launch_refine8                quarter-sample refinement of the +-R full-search
+ *                                            8x8 vectors with the HEVC interpolation
+ *   bc250_hevc_launch_p_encode               predict_cu + inter_residual with the 8x8 luma
+ *                                            transform (hevc_enc_template.c:669-680,817-933)
+ *   bc250_hevc_intra_work_bytes /
+ *   bc250_hevc_launch_i_encode               encode_ctu's I branch + intra_trial
+ *                                            (hevc_enc_template.c:502-594,1409-1427)
+ *
+ * Picture layout: coded size (16-aligned), 8-bit, NV12 (luma plane, then
+ * interleaved Cb/Cr), the same surfaces as the H.264 path.  CTUs are 16x16
+ * in raster order, each four 8x8 CUs in z-order (CU index = ctu*4 + z).
+ * Coefficient output per CTU, int16 [384]:
+ *   I picture: 16 luma PUs (z-order) x 16, then 4 Cb CUs x 16, then 4 Cr x 16
+ *   P picture: 4 luma CUs x 64 (8x8 row-major), then 4 Cb x 16, then 4 Cr x 16
+ * Each block is row-major; levels as the reference's quantiser returns them.
+ * All launchers return 0, or -1 on an invalid argument or launch failure.
+ */
+#ifndef BC250_HEVC_H
+#define BC250_HEVC_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* n blocks of 16 (or 64) int16 each; qp is Qp' (QP + 12 at ten bits),
+ * bd 8 or 10; round_q12 the rounding offset in twelfths of a step. */
+int bc250_hevc_tq4_batch(const int16_t *res, int n, int qp, int use_dst, int round_q12, int bd, int16_t *out,
+                         void *stream);
+int bc250_hevc_dqit4_batch(const int16_t *coef, int n, int qp, int use_dst, int bd, int16_t *out, void *stream);
+int bc250_hevc_tq8_batch(const int16_t *res, int n, int qp, int bd, int round_q12, int16_t *out, void *stream);
+int bc250_hevc_dqit8_batch(const int16_t *coef, int n, int qp, int bd, int16_t *out, void *stream);
+
+/* n blocks of w x h (w, h <= 16) from one plane (uint8 when bd == 8, else
+ * uint16; stride in samples): blk[4*i .. 4*i+3] = {x, y, mvx, mvy}, vectors
+ * in quarter (luma) or eighth (chroma) samples.  dst: n * w * h samples. */
+int bc250_hevc_mc_uni_batch(const void *ref, int stride, int w_pic, int h_pic, const int32_t *blk, int n, int w,
+                            int h, int chroma, int bd, void *dst, void *stream);
+
+/* Quarter-sample refinement of every 8x8 CU: imv8[4*cu] = {dx, dy, cost,
+ * sad} integer vectors (bc250_launch_me_fullsearch's 8x8 output); out[4*cu]
+ * = {mvx, mvy (quarter samples), sad, 0}.  Centre, then 8 half-sample
+ * candidates (2,0) (-2,0) (0,2) (0,-2) (2,2) (2,-2) (-2,2) (-2,-2), then the
+ * same offsets halved around the half winner, strictly lower SAD wins. */
+int bc250_hevc_launch_refine8(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
+                              const uint32_t *imv8, uint32_t *out, void *stream);
+
+/* P picture: every CU inter with mvs[4*cu] = {mvx, mvy, ...}; luma 8x8 DCT,
+ * chroma 4x4 DCT at QpC, rounding round_q12 (2 in the reference's default
+ * dead-zone mode).  Writes coef and the reconstruction (o*). */
+int bc250_hevc_launch_p_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, const uint8_t *ry,
+                               const uint8_t *ruv, int rpy, int rpuv, uint8_t *oy, uint8_t *ouv, int opy, int opuv,
+                               int wc, int hc, int qp, int round_q12, const uint32_t *mvs, int16_t *coef,
+                               void *stream);
+
+/* I picture: CTU16 -> 4 CU8 -> 4 PU4 intra (Planar/DC/H/V by SAD, DST),
+ * chroma DC per CU (DCT, QpC), slices as CTU-row ranges
+ * [floor(rows*s/ns), floor(rows*(s+1)/ns)).  One persistent wavefront
+ * launch; `sync` is a workspace of bc250_hevc_intra_work_bytes() bytes whose
+ * first 2 ints must be zero; sync[1] != 0 afterwards means a dependency wait
+ * gave up after spin_limit polls.  modes: uint8 [nctu*16] (0 Planar, 1 DC,
+ * 10 H, 26 V), PU z-order. */
+size_t bc250_hevc_intra_work_bytes(int wc, int hc);
+int bc250_hevc_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, uint8_t *oy, uint8_t *ouv,
+                               int opy, int opuv, int wc, int hc, int qp, int num_slices, int round_q12,
+                               uint8_t *modes, int16_t *coef, int *sync, unsigned spin_limit, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,172 @@
+"""GPU parity of the HEVC stages (include/bc250_hevc.h) against the CPU
+oracle (oracle/orc_hevc.c, itself pinned to the reference's compiled HEVC
+functions).  Every comparison is bit-exact."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def P(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_tq4_dqit4_batch(pkg, bd):
+    torch = _torch()
+    rs = np.random.default_rng(100 + bd)
+    vmax = (1 << bd) - 1
+    n = 999
+    res = rs.integers(-vmax, vmax + 1, (n, 16)).astype(np.int16)
+    res[::3] //= 37  # small residuals too
+    res[5] = 0
+    for qp in ([0, 13, 22, 27, 37, 51] if bd == 8 else [0, 12, 34, 39, 49, 63]):
+        for dst in (0, 1):
+            for rnd in (2, 4, 6):
+                say(f"tq4 bd={bd} qp={qp} dst={dst} rnd={rnd}")
+                dres, dout = dev(torch, res), torch.zeros((n, 16), dtype=torch.int16, device="cuda")
+                assert pkg.hip.bc250_hevc_tq4_batch(P(dres), n, qp, dst, rnd, bd, P(dout), None) == 0
+                dback = torch.zeros_like(dout)
+                assert pkg.hip.bc250_hevc_dqit4_batch(P(dout), n, qp, dst, bd, P(dback), None) == 0
+                torch.cuda.synchronize()
+                lv, back = host(dout), host(dback)
+                for b in range(0, n, 7):
+                    want = orc.hevc_tq4(res[b], qp, dst, rnd, bd)
+                    assert np.array_equal(lv[b], want), (b, qp)
+                    assert np.array_equal(back[b], orc.hevc_dqit4(want, qp, dst, bd)), (b, qp)
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_tq8_dqit8_batch(pkg, bd):
+    torch = _torch()
+    rs = np.random.default_rng(200 + bd)
+    vmax = (1 << bd) - 1
+    n = 301
+    res = rs.integers(-vmax, vmax + 1, (n, 64)).astype(np.int16)
+    res[1::2] //= 29
+    for qp in (0, 22, 27, 37, 51, 63):
+        say(f"tq8 bd={bd} qp={qp}")
+        dres, dout = dev(torch, res), torch.zeros((n, 64), dtype=torch.int16, device="cuda")
+        assert pkg.hip.bc250_hevc_tq8_batch(P(dres), n, qp, bd, 2, P(dout), None) == 0
+        dback = torch.zeros_like(dout)
+        assert pkg.hip.bc250_hevc_dqit8_batch(P(dout), n, qp, bd, P(dback), None) == 0
+        torch.cuda.synchronize()
+        lv, back = host(dout), host(dback)
+        for b in range(0, n, 5):
+            want = orc.hevc_tq8(res[b], qp, bd, 2)
+            assert np.array_equal(lv[b], want), (b, qp)
+            assert np.array_equal(back[b], orc.hevc_dqit8(want, qp, bd)), (b, qp)
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_mc_uni_batch(pkg, bd):
+    torch = _torch()
+    rs = np.random.default_rng(300 + bd)
+    pix = np.uint8 if bd == 8 else np.uint16
+    plane = rs.integers(0, 1 << bd, (72, 88)).astype(pix)
+    dplane = dev(torch, plane.view(np.int8 if bd == 8 else np.int16))
+    for chroma in (0, 1):
+        nfrac = 8 if chroma else 4
+        for (w, h) in ((4, 4), (8, 8), (16, 16), (8, 4)):
+            say(f"mc bd={bd} chroma={chroma} {w}x{h}")
+            n = 200
+            blk = np.stack([rs.integers(-12, 88, n), rs.integers(-12, 72, n),
+                            rs.integers(-10 * nfrac, 10 * nfrac, n), rs.integers(-10 * nfrac, 10 * nfrac, n)],
+                           1).astype(np.int32)
+            dblk = dev(torch, blk)
+            dout = torch.zeros(n * w * h * (1 if bd == 8 else 2), dtype=torch.uint8, device="cuda")
+            assert pkg.hip.bc250_hevc_mc_uni_batch(P(dplane), 88, 88, 72, P(dblk), n, w, h, chroma, bd, P(dout),
+                                                   None) == 0
+            torch.cuda.synchronize()
+            got = host(dout).view(pix).reshape(n, h, w)
+            for i in range(n):
+                want = orc.hevc_mc_uni(plane, int(blk[i, 0]), int(blk[i, 1]), w, h, int(blk[i, 2]), int(blk[i, 3]),
+                                       chroma, bd)
+                assert np.array_equal(got[i], want), (i, blk[i])
+
+
+def _nv12(y, cb, cr):
+    uv = np.empty((cb.shape[0], cb.shape[1] * 2), np.uint8)
+    uv[:, 0::2], uv[:, 1::2] = cb, cr
+    return y, uv
+
+
+def _frames(pkg, seed, w, h, n, nrect=3):
+    g = pkg.FrameGen(seed, w, h, nrect)
+    out = [pkg.pad_coded(*g.frame(i)) for i in range(n)]
+    g.close()
+    return out
+
+
+@pytest.mark.parametrize("w,h,rng", [(128, 96, 16), (320, 176, 32)])
+def test_refine8_stage(pkg, w, h, rng):
+    torch = _torch()
+    (cy, cuv), (ry, ruv) = _frames(pkg, 77, w, h, 2)
+    cy, ry = ry, cy  # search frame 0 in frame 1's reconstruction stand-in
+    hc, wc = cy.shape
+    _, i8 = orc.me_fullsearch(cy, ry, rng, 1)
+    dcy, dry = dev(torch, cy), dev(torch, ry)
+    di8 = dev(torch, i8.view(np.int32))
+    dout = torch.zeros(i8.shape[0] * 4, dtype=torch.int32, device="cuda")
+    assert pkg.hip.bc250_hevc_launch_refine8(P(dcy), wc, P(dry), wc, wc, hc, P(di8), P(dout), None) == 0
+    torch.cuda.synchronize()
+    got = host(dout).reshape(-1, 4)
+    for cu in range(i8.shape[0]):
+        ctu, z = cu // 4, cu % 4
+        x0 = (ctu % (wc // 16)) * 16 + (z & 1) * 8
+        y0 = (ctu // (wc // 16)) * 16 + (z >> 1) * 8
+        mx, my, sad = orc.hevc_refine8(cy, ry, x0, y0, int(i8[cu]["dx"]) * 4, int(i8[cu]["dy"]) * 4)
+        assert (got[cu, 0], got[cu, 1], got[cu, 2]) == (mx, my, sad), cu
+
+
+@pytest.mark.parametrize("qp", [0, 22, 32, 45, 51])
+def test_p_encode_stage(pkg, qp):
+    torch = _torch()
+    w, h = 192, 112
+    (cy, cuv), (ry, ruv) = _frames(pkg, 91 + qp, w, h, 2, 4)
+    hc, wc = cy.shape
+    t = (wc // 16) * (hc // 16)
+    rs = np.random.default_rng(qp)
+    mv = rs.integers(-70, 71, (t * 4, 2)).astype(np.int32)
+    mv[::5] = 0
+    cb, cr = orc.split_uv(cuv)
+    fb, fr = orc.split_uv(ruv)
+    want = orc.hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, orc.HEVC_ROUND_INTER)
+    mv4 = np.zeros((t * 4, 4), np.int32)
+    mv4[:, :2] = mv
+    d = [dev(torch, a) for a in (cy, cuv, ry, ruv, mv4)]
+    oy, ouv = torch.zeros_like(d[0]), torch.zeros_like(d[1])
+    coef = torch.zeros(t * 384, dtype=torch.int16, device="cuda")
+    assert pkg.hip.bc250_hevc_launch_p_encode(P(d[0]), P(d[1]), wc, wc, P(d[2]), P(d[3]), wc, wc, P(oy), P(ouv), wc,
+                                              wc, wc, hc, qp, orc.HEVC_ROUND_INTER, P(d[4]), P(coef), None) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(host(coef), want["coef"])
+    assert np.array_equal(host(oy), want["y"])
+    gb, gr = orc.split_uv(host(ouv))
+    assert np.array_equal(gb, want["cb"]) and np.array_equal(gr, want["cr"])
+    assert (want["coef"] != 0).any()
