@@ -1,0 +1,373 @@
+// hevc_intra.hip - HEVC I-picture coding on gfx950 as ONE persistent
+// wavefront launch: encode_ctu's I branch with intra_trial
+// (enc/src/hevc_enc_template.c:502-594,1409-1427) for every CTU.
+//
+// Coding shape (fixed, as in the reference): CTU16 -> 4 CU8 -> 4 PU4 in
+// z-order.  Each luma PU takes the SAD-best of Planar / DC / H / V (strict
+// '<' in that order) built from reconstructed neighbours with z-scan
+// availability and the substitution scan of the reference's gather
+// (hevc_intra_template.c:28-112), then DST-VII, quantisation, dequantisation,
+// inverse and reconstruction, so the next PU sees it.  Chroma is one 4x4 DC
+// block per CU and component (DCT-II, QpC).
+//
+// Work layout: one wave per CTU row, 4 rows per workgroup (row group from an
+// atomic ticket, so a row's producer always started before it).  Per CTU the
+// wave runs 16 steps, one per luma PU: lanes 0..15 hold the PU's samples,
+// lanes 16..31 / 32..47 the Cb / Cr block of the CU on its first step.  A
+// CTU needs the bottom rows of CTUs x-1, x, x+1 of the row above (corner,
+// top, top-right); each finished CTU hands its bottom luma / Cb / Cr row (32
+// bytes) to the wave below through an LDS ring, or to the next workgroup as
+// system-coherent 64-bit records {tag = x + 1, 4 samples} that the reader
+// polls until every tag matches (cleared before the launch).  Rows that
+// start a slice read nothing above and wait for nothing.  Waits are bounded
+// and report on *err.
+#include "bc250_hevc.h"
+#include "hevc_dev.h"
+
+using namespace hevc;
+
+namespace {
+constexpr int WAVES = 4;
+constexpr int RING = 16;
+constexpr int REC = 8;   // dwords per CTU record: luma row 15 (16 B), Cb row 7 (8 B), Cr row 7 (8 B)
+constexpr int LP = 20;   // luma tile pitch: col 3 = left neighbour column, cols 4..19 = CTU
+constexpr int CPT = 12;  // chroma tile pitch: col 3 = left neighbour, cols 4..11 = CTU
+
+// LDS bytes of one wave, addressed by offset so a lane can pick its plane.
+constexpr int O_TY = 0;                 // luma tile 16 x LP
+constexpr int O_TC = O_TY + 16 * LP;    // Cb, Cr tiles 8 x CPT each
+constexpr int O_AY = O_TC + 2 * 8 * CPT;  // above rows, luma: 4 CTU slots x 16
+constexpr int O_AC = O_AY + 4 * 16;     // above rows, Cb / Cr: 2 x 4 slots x 8
+constexpr int O_SRC = O_AC + 2 * 4 * 8;  // source CTU: luma 256, Cb 64, Cr 64
+constexpr int WAVE_BYTES = O_SRC + 384;
+
+struct alignas(16) WaveLds {
+    uint8_t b[WAVE_BYTES];
+    int res[64], tmp[64];
+};
+
+// z-scan rank for CTU16/CU8/PU4 (luma) or CTU8/CU4 (chroma), CTUs raster
+// (hevc_intra.c:75-130).
+BC_DEV int zrank(int x, int y, int wctu, bool luma)
+{
+    const int sh = luma ? 4 : 3, hs = sh - 1;
+    const int rx = x & ((1 << sh) - 1), ry = y & ((1 << sh) - 1);
+    int r = (((y >> sh) * wctu + (x >> sh)) << 2) + ((ry >> hs) << 1) + (rx >> hs);
+    if (luma) r = (r << 2) + (((ry & 7) >> 2) << 1) + ((rx & 7) >> 2);
+    return r;
+}
+
+BC_DEV bool zavail(int nx, int ny, int w, int h, int wctu, bool luma, int y_min, int cur)
+{
+    if (nx < 0 || ny < y_min || nx >= w || ny >= h) return false;
+    return zrank(nx, ny, wctu, luma) < cur;
+}
+
+BC_DEV int slice_first_row(int row, int rows, int ns)
+{
+    ns = ns < 1 ? 1 : (ns > rows ? rows : ns);
+    int r0 = 0;
+    for (int s = 0; s < ns; s++) {
+        const int b = (int)(((long long)rows * s) / ns);
+        if (row >= b) r0 = b;
+    }
+    return r0;
+}
+}  // namespace
+
+__global__ void hevc_clear_records_kernel(uint64_t *rec, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rec[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void hevc_i_encode_kernel(
+    const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, uint8_t *__restrict__ oy,
+    uint8_t *__restrict__ ouv, int opy, int opuv, int wc, int hc, int qp, int num_slices, int round_q12,
+    uint8_t *__restrict__ modes, int16_t *__restrict__ coef, int *sync, uint64_t *ghand, unsigned spin_limit)
+{
+    __shared__ WaveLds wl[WAVES];
+    __shared__ uint32_t ring[WAVES - 1][RING][REC];
+    __shared__ int prog[WAVES], cons[WAVES];
+    __shared__ int mat[32];  // DST-VII rows, then DCT-II rows
+    __shared__ int s_group;
+
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    int *err = sync + 1;
+    if (tid == 0) s_group = atomicAdd(sync, 1);
+    if (tid < WAVES) prog[tid] = cons[tid] = 0;
+    if (tid < 16) mat[tid] = kDst4[tid];
+    else if (tid < 32) mat[tid] = kDct4[tid - 16];
+    __syncthreads();
+    const int wtu = wc >> 4, htu = hc >> 4;
+    const int yrow = s_group * WAVES + wv;
+    if (yrow >= htu) return;
+    volatile int *vprog = prog, *vcons = cons;
+    uint8_t *B = wl[wv].b;
+    int *RES = wl[wv].res, *TMP = wl[wv].tmp;
+
+    const int r0 = slice_first_row(yrow, htu, num_slices);
+    const bool need_above = yrow > r0;
+    const bool give_below = yrow + 1 < htu && slice_first_row(yrow + 1, htu, num_slices) != yrow + 1;
+    const bool lds_above = wv > 0, lds_below = wv < WAVES - 1;
+    const int y_min = r0 * 16;
+
+    // lane roles: group 0 luma PU samples, 1 Cb, 2 Cr (3: idle copy of Cr)
+    const int grp = lane >> 4, k16 = lane & 15, pi = k16 >> 2, pj = k16 & 3;
+    const bool luma = grp == 0;
+    const int plane = grp > 2 ? 2 : grp;
+    const int pw = luma ? wc : wc >> 1, ph = luma ? hc : hc >> 1;
+    const int pwctu = wtu, cs = luma ? 16 : 8;
+    const int pymin = luma ? y_min : y_min >> 1;
+    const int tile = luma ? O_TY : O_TC + (plane - 1) * 8 * CPT, tp = luma ? LP : CPT;
+    const int above = luma ? O_AY : O_AC + (plane - 1) * 32;
+    const int srco = luma ? O_SRC : O_SRC + 256 + (plane - 1) * 64;
+    const int mbase = luma ? 0 : 16;  // DST for luma, DCT for chroma
+    const int cqp = hevc::chroma_qp(qp);
+    const Quant QL = make_quant(clampi(qp, 0, 51), 5, round_q12), QC = make_quant(clampi(cqp, 0, 51), 5, round_q12);
+    const int q = luma ? qp : cqp;
+
+    // source prefetch: luma one dword per lane, chroma one dword per lane 0..31
+    const int Y0r = yrow * 16;
+    uint32_t pre_y = 0, pre_c = 0;
+    auto prefetch = [&](int x) {
+        pre_y = *(const uint32_t *)(cy + (size_t)(Y0r + (lane >> 2)) * cpy + x * 16 + (lane & 3) * 4);
+        if (lane < 32) pre_c = *(const uint32_t *)(cuv + (size_t)(yrow * 8 + (lane >> 2)) * cpuv + x * 16 + (lane & 3) * 4);
+    };
+    prefetch(0);
+
+    auto sample = [&](int gx, int gy, int ctx0, int cty0) -> int {
+        if (gy < cty0) return B[above + (((gx / cs) & 3) * cs) + (gx & (cs - 1))];
+        return B[tile + (gy - cty0) * tp + 4 + gx - ctx0];
+    };
+    auto take_record = [&](int j, uint32_t v, int d) {
+        // dword d of record j: 0..3 luma bytes 4d.., 4..5 Cb, 6..7 Cr
+        if (d < 4) *(uint32_t *)&B[O_AY + (j & 3) * 16 + d * 4] = v;
+        else if (d < 6) *(uint32_t *)&B[O_AC + (j & 3) * 8 + (d - 4) * 4] = v;
+        else *(uint32_t *)&B[O_AC + 32 + (j & 3) * 8 + (d - 6) * 4] = v;
+    };
+    auto fetch_above = [&](int j) {  // record j of the row above into its slot
+        uint32_t v = 0;
+        if (lds_above) {
+            if (lane < REC) v = ring[wv - 1][j % RING][lane];
+        } else {
+            const uint64_t *src = ghand + ((size_t)(yrow - 1) * wtu + j) * REC + lane;
+            unsigned n = 0;
+            uint64_t qv = 0;
+            for (;;) {
+                bool ok = true;
+                if (lane < REC) {
+                    qv = ld_sys64(src);
+                    ok = (uint32_t)(qv >> 32) == (uint32_t)j + 1;
+                }
+                if (__all(ok)) break;
+                __builtin_amdgcn_s_sleep(1);
+                if (++n > spin_limit) {
+                    if (lane == 0) atomicOr(err, 1);
+                    break;
+                }
+            }
+            v = (uint32_t)qv;
+        }
+        if (lane < REC) take_record(j, v, lane);
+    };
+
+    for (int x = 0; x < wtu; x++) {
+        const int X0 = x * 16;
+        // ---- records of the row above up to CTU x+1
+        if (need_above) {
+            const int need = x + 2 < wtu ? x + 2 : wtu;
+            if (lds_above) {
+                unsigned n = 0;
+                while (vprog[wv - 1] < need) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++n > spin_limit) { atomicOr(err, 1); break; }
+                }
+                wave_lds_sync();
+            }
+            if (x == 0) fetch_above(0);
+            if (need > x + 1) fetch_above(x + 1);
+            wave_lds_sync();
+            if (lds_above && lane == 0) vcons[wv] = need;
+        }
+        // ---- source of CTU x into LDS; prefetch CTU x+1
+        *(uint32_t *)&B[O_SRC + lane * 4] = pre_y;
+        if (lane < 32) {
+            const int r = lane >> 2, c2 = (lane & 3) * 2;
+            B[O_SRC + 256 + r * 8 + c2] = (uint8_t)pre_c;
+            B[O_SRC + 320 + r * 8 + c2] = (uint8_t)(pre_c >> 8);
+            B[O_SRC + 256 + r * 8 + c2 + 1] = (uint8_t)(pre_c >> 16);
+            B[O_SRC + 320 + r * 8 + c2 + 1] = (uint8_t)(pre_c >> 24);
+        }
+        if (x + 1 < wtu) prefetch(x + 1);
+        wave_lds_sync();
+
+        const int ctx0 = luma ? X0 : X0 >> 1, cty0 = luma ? Y0r : Y0r >> 1;
+        for (int k = 0; k < 16; k++) {
+            const int cu = k >> 2, pu = k & 3;
+            const bool active = luma || (grp < 3 && pu == 0);
+            const int bx = luma ? X0 + (cu & 1) * 8 + (pu & 1) * 4 : ctx0 + (cu & 1) * 4;
+            const int by = luma ? Y0r + (cu >> 1) * 8 + (pu >> 1) * 4 : cty0 + (cu >> 1) * 4;
+            // ---- neighbours with the substitution scan
+            const int cur = zrank(bx, by, pwctu, luma);
+            const bool aL = zavail(bx - 1, by, pw, ph, pwctu, luma, pymin, cur);
+            const bool aT = zavail(bx, by - 1, pw, ph, pwctu, luma, pymin, cur);
+            const bool aC = zavail(bx - 1, by - 1, pw, ph, pwctu, luma, pymin, cur);
+            const bool aTR = zavail(bx + 4, by - 1, pw, ph, pwctu, luma, pymin, cur);
+            const bool aBL = zavail(bx - 1, by + 4, pw, ph, pwctu, luma, pymin, cur);
+            int v[10];
+            bool a[10];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                a[t] = aL;
+                v[t] = aL ? sample(bx - 1, by + 3 - t, ctx0, cty0) : 0;
+                a[5 + t] = aT;
+                v[5 + t] = aT ? sample(bx + t, by - 1, ctx0, cty0) : 0;
+            }
+            a[4] = aC;
+            v[4] = aC ? sample(bx - 1, by - 1, ctx0, cty0) : 0;
+            a[9] = aTR;
+            v[9] = aTR ? sample(bx + 4, by - 1, ctx0, cty0) : 0;
+            // first available in scan order fills what precedes it; later gaps copy the previous
+            int firstv = 128;
+            bool any = false;
+#pragma unroll
+            for (int t = 9; t >= 0; t--)
+                if (a[t]) { firstv = v[t]; any = true; }
+            int prev = firstv;
+#pragma unroll
+            for (int t = 0; t < 10; t++) {
+                v[t] = a[t] ? v[t] : prev;
+                prev = v[t];
+            }
+            (void)any;
+            const int L0 = v[3], L1 = v[2], L2 = v[1], L3 = v[0], Cc = v[4];
+            const int T0 = v[5], T1 = v[6], T2 = v[7], T3 = v[8], T4 = v[9];
+            const int L4 = aBL ? sample(bx - 1, by + 4, ctx0, cty0) : L3;
+            const int Li = pi == 0 ? L0 : pi == 1 ? L1 : pi == 2 ? L2 : L3;
+            const int Tj = pj == 0 ? T0 : pj == 1 ? T1 : pj == 2 ? T2 : T3;
+            // ---- the four predictions of this lane's sample
+            const int p_pl = ((3 - pj) * Li + (pj + 1) * T4 + (3 - pi) * Tj + (pi + 1) * L4 + 4) >> 3;
+            const int dc = (L0 + L1 + L2 + L3 + T0 + T1 + T2 + T3 + 4) >> 3;
+            int p_dc = dc;
+            if (luma) {
+                if (pi == 0 && pj == 0) p_dc = (L0 + 2 * dc + T0 + 2) >> 2;
+                else if (pi == 0) p_dc = (Tj + 3 * dc + 2) >> 2;
+                else if (pj == 0) p_dc = (Li + 3 * dc + 2) >> 2;
+            }
+            const int p_h = (luma && pi == 0) ? clampi(L0 + ((Tj - Cc) >> 1), 0, 255) : Li;
+            const int p_v = (luma && pj == 0) ? clampi(T0 + ((Li - Cc) >> 1), 0, 255) : Tj;
+            const int s = B[srco + (by - cty0 + pi) * cs + (bx - ctx0 + pj)];
+            // ---- SAD decision over the 16 lanes of the group (luma only)
+            uint32_t sp = (uint32_t)iabs(s - p_pl), sd = (uint32_t)iabs(s - p_dc);
+            uint32_t sh = (uint32_t)iabs(s - p_h), sv = (uint32_t)iabs(s - p_v);
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) {
+                sp += __shfl_xor(sp, o, 16);
+                sd += __shfl_xor(sd, o, 16);
+                sh += __shfl_xor(sh, o, 16);
+                sv += __shfl_xor(sv, o, 16);
+            }
+            int mode = 0, pred = p_pl;
+            uint32_t best = sp;
+            if (sd < best) { best = sd; mode = 1; pred = p_dc; }
+            if (sh < best) { best = sh; mode = 10; pred = p_h; }
+            if (sv < best) { best = sv; mode = 26; pred = p_v; }
+            if (!luma) { mode = 1; pred = dc; }
+            // ---- forward transform, quantisation
+            const int g16 = (lane >> 4) * 16;
+            RES[lane] = s - pred;
+            wave_lds_sync();
+            int acc = 0;
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc += mat[mbase + pi * 4 + r] * RES[g16 + r * 4 + pj];
+            TMP[lane] = sat16((acc + 1) >> 1);
+            wave_lds_sync();
+            acc = 0;
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc += mat[mbase + pj * 4 + c] * TMP[g16 + pi * 4 + c];
+            const int lv = quant(sat16((acc + 128) >> 8), luma ? QL : QC);
+            // ---- dequantisation, inverse, reconstruction
+            RES[lane] = dequant(lv, q, 5);
+            wave_lds_sync();
+            acc = 0;
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc += mat[mbase + t * 4 + pi] * RES[g16 + t * 4 + pj];
+            TMP[lane] = sat16((acc + 64) >> 7);
+            wave_lds_sync();
+            acc = 0;
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc += mat[mbase + t * 4 + pj] * TMP[g16 + pi * 4 + t];
+            const int rec = clampi(pred + sat16((acc + 2048) >> 12), 0, 255);
+            if (active) {
+                B[tile + (by - cty0 + pi) * tp + 4 + (bx - ctx0 + pj)] = (uint8_t)rec;
+                const size_t cb = (size_t)(yrow * wtu + x) * 384;
+                if (luma) coef[cb + k * 16 + k16] = (int16_t)lv;
+                else coef[cb + 256 + (plane - 1) * 64 + cu * 16 + k16] = (int16_t)lv;
+                if (lane == 0) modes[(size_t)(yrow * wtu + x) * 16 + k] = (uint8_t)mode;
+            }
+            wave_lds_sync();
+        }
+        // ---- hand the CTU's bottom rows down
+        if (give_below) {
+            uint32_t h = 0;
+            if (lane < 4) h = *(const uint32_t *)&B[O_TY + 15 * LP + 4 + lane * 4];
+            else if (lane < 6) h = *(const uint32_t *)&B[O_TC + 7 * CPT + 4 + (lane - 4) * 4];
+            else if (lane < 8) h = *(const uint32_t *)&B[O_TC + 8 * CPT + 7 * CPT + 4 + (lane - 6) * 4];
+            if (lds_below) {
+                unsigned n = 0;
+                while (vcons[wv + 1] < x + 1 - RING + 1) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++n > spin_limit) { atomicOr(err, 2); break; }
+                }
+                if (lane < REC) ring[wv][x % RING][lane] = h;
+                wave_lds_sync();
+                if (lane == 0) vprog[wv] = x + 1;
+            } else if (lane < REC) {
+                st_sys64(ghand + ((size_t)yrow * wtu + x) * REC + lane, ((uint64_t)(x + 1) << 32) | h);
+            }
+        }
+        // ---- the CTU's reconstruction to the frame
+        *(uint32_t *)(oy + (size_t)(Y0r + (lane >> 2)) * opy + X0 + (lane & 3) * 4) =
+            *(const uint32_t *)&B[O_TY + (lane >> 2) * LP + 4 + (lane & 3) * 4];
+        if (lane < 32) {
+            const int r = lane >> 2, c2 = (lane & 3) * 2;
+            const uint8_t *tb = &B[O_TC + r * CPT + 4 + c2], *tr = &B[O_TC + 8 * CPT + r * CPT + 4 + c2];
+            *(uint32_t *)(ouv + (size_t)(yrow * 8 + r) * opuv + X0 + (lane & 3) * 4) =
+                (uint32_t)tb[0] | ((uint32_t)tr[0] << 8) | ((uint32_t)tb[1] << 16) | ((uint32_t)tr[1] << 24);
+        }
+        wave_lds_sync();
+        // ---- slide: this CTU's right column becomes the left neighbour column
+        if (lane < 16) B[O_TY + lane * LP + 3] = B[O_TY + lane * LP + 19];
+        else if (lane < 32) {
+            const int c = (lane - 16) >> 3, r = lane & 7;
+            B[O_TC + c * 8 * CPT + r * CPT + 3] = B[O_TC + c * 8 * CPT + r * CPT + 11];
+        }
+        wave_lds_sync();
+    }
+}
+
+extern "C" size_t bc250_hevc_intra_work_bytes(int wc, int hc)
+{
+    const size_t nctu = (size_t)(wc / 16) * (hc / 16);
+    return 256 + nctu * REC * 8;
+}
+
+extern "C" int bc250_hevc_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, uint8_t *oy,
+                                          uint8_t *ouv, int opy, int opuv, int wc, int hc, int qp, int num_slices,
+                                          int round_q12, uint8_t *modes, int16_t *coef, int *sync,
+                                          unsigned spin_limit, void *stream)
+{
+    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || num_slices < 1 || round_q12 < 0 ||
+        round_q12 > 12 || (cpy & 3) || (cpuv & 3) || (opy & 3) || (opuv & 3) || cpy < wc || cpuv < wc || opy < wc ||
+        opuv < wc)
+        return -1;
+    hipStream_t st = (hipStream_t)stream;
+    const int htu = hc / 16, nrec = (wc / 16) * htu * REC;
+    uint64_t *ghand = (uint64_t *)((uint8_t *)sync + 256);
+    hevc_clear_records_kernel<<<(nrec + 255) / 256, 256, 0, st>>>(ghand, nrec);
+    hevc_i_encode_kernel<<<(htu + WAVES - 1) / WAVES, 64 * WAVES, 0, st>>>(cy, cuv, cpy, cpuv, oy, ouv, opy, opuv, wc,
+                                                                            hc, qp, num_slices, round_q12, modes, coef,
+                                                                            sync, ghand, spin_limit);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

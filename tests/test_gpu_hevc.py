@@ -170,3 +170,30 @@ def test_p_encode_stage(pkg, qp):
     gb, gr = orc.split_uv(host(ouv))
     assert np.array_equal(gb, want["cb"]) and np.array_equal(gr, want["cr"])
     assert (want["coef"] != 0).any()
+
+
+@pytest.mark.parametrize("w,h,qp,ns", [(64, 48, 22, 1), (128, 96, 37, 2), (200, 120, 27, 3), (320, 240, 32, 4)])
+def test_i_encode_stage(pkg, w, h, qp, ns):
+    torch = _torch()
+    say(f"hevc I {w}x{h} qp={qp} slices={ns}")
+    (cy, cuv), = _frames(pkg, 5 + qp, w, h, 1, 3)
+    hc, wc = cy.shape
+    t = (wc // 16) * (hc // 16)
+    cb, cr = orc.split_uv(cuv)
+    want = orc.hevc_encode_i(cy, cb, cr, qp, ns, orc.HEVC_ROUND_INTRA_I)
+    d = [dev(torch, a) for a in (cy, cuv)]
+    oy, ouv = torch.zeros_like(d[0]), torch.zeros_like(d[1])
+    modes = torch.zeros(t * 16, dtype=torch.uint8, device="cuda")
+    coef = torch.zeros(t * 384, dtype=torch.int16, device="cuda")
+    nwork = pkg.hip.bc250_hevc_intra_work_bytes(wc, hc)
+    sync = torch.zeros((nwork + 3) // 4, dtype=torch.int32, device="cuda")
+    assert pkg.hip.bc250_hevc_launch_i_encode(P(d[0]), P(d[1]), wc, wc, P(oy), P(ouv), wc, wc, wc, hc, qp, ns,
+                                              orc.HEVC_ROUND_INTRA_I, P(modes), P(coef), P(sync), 1 << 22,
+                                              None) == 0
+    torch.cuda.synchronize()
+    assert host(sync)[1] == 0, "wavefront wait gave up"
+    assert np.array_equal(host(modes), want["modes"])
+    assert np.array_equal(host(coef), want["coef"])
+    assert np.array_equal(host(oy), want["y"])
+    gb, gr = orc.split_uv(host(ouv))
+    assert np.array_equal(gb, want["cb"]) and np.array_equal(gr, want["cr"])
