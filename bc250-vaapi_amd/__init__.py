@@ -15,6 +15,7 @@ HIP_LIB = os.path.join(_HERE, "libbc250_hip.so")
 HOST_LIB = os.path.join(_HERE, "libbc250_host.so")
 
 GPU_IMAGE_NV12 = 0
+GPU_CODEC_H264, GPU_CODEC_HEVC = 0, 1
 GPU_IMAGE_P010 = 1
 
 
@@ -56,7 +57,7 @@ class GpuContext(C.Structure):
 
 class EncodeParams(C.Structure):
     _fields_ = [("search_range", C.c_int), ("lambda8", C.c_int), ("deblock_idc", C.c_int),
-                ("me_8x8", C.c_int), ("readback", C.c_int)]
+                ("me_8x8", C.c_int), ("readback", C.c_int), ("codec", C.c_int)]
 
 
 class StageStats(C.Structure):
@@ -94,7 +95,7 @@ _sig(hip, "gpu_compute_dispatch_me_only", _I, [_CTX, GpuImage, _I, _I])
 _sig(hip, "gpu_compute_end_picture", _I, [_CTX])
 _sig(hip, "gpu_compute_submitted_slot", _I, [_CTX])
 _sig(hip, "gpu_compute_sync_slot", _I, [_CTX, _I])
-for _n in ("quant", "dc", "pred_mode", "mv", "nz", "mv8"):
+for _n in ("quant", "dc", "pred_mode", "mv", "nz", "mv8", "hevc_mode"):
     _sig(hip, f"gpu_compute_get_{_n}_staging_data_slot", _I, [_CTX, _I, C.POINTER(_P), C.POINTER(C.c_size_t)])
 _sig(hip, "gpu_compute_set_encode_params", _I, [_CTX, C.POINTER(EncodeParams)])
 _sig(hip, "gpu_compute_download_recon", _I, [_CTX, _P, _I, _P, _I])
@@ -202,18 +203,20 @@ def pad_coded(y, uv):
 class Encoder:
     """One gpu_compute context: upload frames, encode, read staging back."""
 
-    def __init__(self, width, height, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1):
+    def __init__(self, width, height, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0):
         self.w, self.h = width, height
         self.wc, self.hc = (width + 15) // 16 * 16, (height + 15) // 16 * 16
         self.nmb = (self.wc // 16) * (self.hc // 16)
         self.ctx = GpuContext()
         if hip.bc250_gpu_init(C.byref(self.ctx)) != 0:
             raise RuntimeError("bc250_gpu_init failed (no HIP device?)")
-        self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback)
+        self.codec = codec
+        self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback, codec)
         self.images = []
 
-    def set_params(self, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1):
-        p = EncodeParams(search_range, lambda8, deblock_idc, me_8x8, readback)
+    def set_params(self, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0):
+        """codec: GPU_CODEC_H264 (0) or GPU_CODEC_HEVC (1)."""
+        p = EncodeParams(search_range, lambda8, deblock_idc, me_8x8, readback, codec)
         if hip.gpu_compute_set_encode_params(C.byref(self.ctx), C.byref(p)) != 0:
             raise ValueError("invalid encode parameters")
 
@@ -258,11 +261,13 @@ class Encoder:
         if f(C.byref(self.ctx), slot, C.byref(d), C.byref(n)) != 0:
             raise RuntimeError(f"no {name} staging data in slot {slot}")
         dt = {"quant": np.int16, "dc": np.int32, "pred_mode": np.uint32, "nz": np.uint32, "mv": MV_DTYPE,
-              "mv8": MV_DTYPE}[name]
+              "mv8": MV_DTYPE, "hevc_mode": np.uint8}[name]
         buf = (C.c_uint8 * n.value).from_address(d.value)
         return np.frombuffer(buf, dtype=dt).copy()
 
     def results(self, slot):
+        if self.codec == GPU_CODEC_HEVC:
+            return {k: self.staging(slot, k) for k in ("quant", "mv8", "hevc_mode")}
         return {k: self.staging(slot, k) for k in ("quant", "dc", "pred_mode", "mv", "nz", "mv8")}
 
     def recon(self):

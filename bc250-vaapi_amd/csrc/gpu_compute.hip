@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include "bc250_dev.h"
+#include "bc250_hevc.h"
 #include "bc250_stages.h"
 #include "gpu_compute.h"
 
@@ -39,6 +40,7 @@ struct Slot {
     int32_t *dc = nullptr;
     uint32_t *modes = nullptr, *nz = nullptr;
     gpu_mv_t *mvs = nullptr, *mv8 = nullptr;
+    uint8_t *hmodes = nullptr;  // HEVC PU modes
     int *err = nullptr;  // 2 ints: intra wait error, deblock wait error
     hipEvent_t ev[NEV] = {};
     bool pending = false, recorded = false, have_data = false, was_intra = false;
@@ -57,11 +59,12 @@ struct bc250_hip_state {
     int32_t *dc = nullptr;
     uint32_t *nz = nullptr, *modes = nullptr;
     uint32_t *mvs = nullptr, *mv8 = nullptr, *imv16 = nullptr, *imv8 = nullptr;
-    int *sync_i = nullptr, *sync_d = nullptr;
+    int *sync_i = nullptr, *sync_d = nullptr, *sync_h = nullptr;
+    uint8_t *hmodes = nullptr;
     Slot slot[NSLOT];
     int cur = 0, submitted = 0;
     double last_latency_ms = 0.0;
-    gpu_encode_params_t params{64, 1, 0, 1, 1};
+    gpu_encode_params_t params{64, 1, 0, 1, 1, GPU_CODEC_H264};
     int err_flags = 0;
     // per-stage GPU time accumulated at sync_slot: [stage] sums, frames seen
     double stage_sum[NSTAGE] = {0, 0, 0, 0, 0};
@@ -139,17 +142,19 @@ static void free_res(bc250_hip_state *s)
         if (r.base) hipFree(r.base);
         r = Plane2{};
     }
-    void *dev[] = {s->levels, s->dc, s->nz, s->modes, s->mvs, s->mv8, s->imv16, s->imv8, s->sync_i, s->sync_d};
+    void *dev[] = {s->levels, s->dc,     s->nz,     s->modes,  s->mvs,   s->mv8,
+                   s->imv16,  s->imv8, s->sync_i, s->sync_d, s->sync_h, s->hmodes};
     for (void *p : dev)
         if (p) hipFree(p);
     s->levels = nullptr; s->dc = nullptr; s->nz = nullptr; s->modes = nullptr; s->mvs = nullptr;
     s->mv8 = nullptr; s->imv16 = nullptr; s->imv8 = nullptr; s->sync_i = nullptr; s->sync_d = nullptr;
+    s->sync_h = nullptr; s->hmodes = nullptr;
     for (auto &sl : s->slot) {
-        void *host[] = {sl.levels, sl.dc, sl.modes, sl.nz, sl.mvs, sl.mv8, sl.err};
+        void *host[] = {sl.levels, sl.dc, sl.modes, sl.nz, sl.mvs, sl.mv8, sl.hmodes, sl.err};
         for (void *p : host)
             if (p) hipHostFree(p);
         sl.levels = nullptr; sl.dc = nullptr; sl.modes = nullptr; sl.nz = nullptr; sl.mvs = nullptr;
-        sl.mv8 = nullptr; sl.err = nullptr;
+        sl.mv8 = nullptr; sl.hmodes = nullptr; sl.err = nullptr;
         sl.pending = sl.have_data = false;
     }
     s->w = s->h = s->wc = s->hc = s->mbw = s->mbh = s->nmb = 0;
@@ -181,7 +186,9 @@ static int ensure_res(bc250_hip_state *s, int w, int h)
          HIP_OK(hipMalloc(&s->mvs, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->mv8, (size_t)nmb * 64)) &&
          HIP_OK(hipMalloc(&s->imv16, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->imv8, (size_t)nmb * 64)) &&
          HIP_OK(hipMalloc(&s->sync_i, bc250_intra_work_bytes(wc, hc))) &&
-         HIP_OK(hipMalloc(&s->sync_d, bc250_deblock_work_bytes(wc, hc)));
+         HIP_OK(hipMalloc(&s->sync_d, bc250_deblock_work_bytes(wc, hc))) &&
+         HIP_OK(hipMalloc(&s->sync_h, bc250_hevc_intra_work_bytes(wc, hc))) &&
+         HIP_OK(hipMalloc(&s->hmodes, (size_t)nmb * 16));
     for (auto &sl : s->slot) {
         ok = ok && HIP_OK(hipHostMalloc((void **)&sl.levels, (size_t)nmb * 384 * 2, hipHostMallocDefault)) &&
              HIP_OK(hipHostMalloc((void **)&sl.dc, (size_t)nmb * 24 * 4, hipHostMallocDefault)) &&
@@ -189,6 +196,7 @@ static int ensure_res(bc250_hip_state *s, int w, int h)
              HIP_OK(hipHostMalloc((void **)&sl.nz, (size_t)nmb * 24 * 4, hipHostMallocDefault)) &&
              HIP_OK(hipHostMalloc((void **)&sl.mvs, (size_t)nmb * 16, hipHostMallocDefault)) &&
              HIP_OK(hipHostMalloc((void **)&sl.mv8, (size_t)nmb * 64, hipHostMallocDefault)) &&
+             HIP_OK(hipHostMalloc((void **)&sl.hmodes, (size_t)nmb * 16, hipHostMallocDefault)) &&
              HIP_OK(hipHostMalloc((void **)&sl.err, 2 * sizeof(int), hipHostMallocDefault));
     }
     if (!ok) {
@@ -485,6 +493,54 @@ static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &r
     return r;
 }
 
+// HEVC stages of one frame (same events: start, integer search, refine,
+// encode, "deblock" (none), copies).  I: CTU wavefront; P: full search,
+// quarter-sample 8x8 refine with the HEVC interpolation, CU coding.
+static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const Plane2 &ref, Plane2 &out, int qp,
+                        int is_intra, int num_slices)
+{
+    hipStream_t st = s->stream;
+    const gpu_encode_params_t &p = s->params;
+    const size_t n = (size_t)s->nmb;
+    int r = 0;
+    hipEventRecord(sl.ev[0], st);
+    if (!is_intra)
+        r = bc250_launch_me_fullsearch(cur.y, cur.py, ref.y, ref.py, s->wc, s->hc, p.search_range, p.lambda8,
+                                       s->imv16, s->imv8, st);
+    hipEventRecord(sl.ev[1], st);
+    if (r == 0 && !is_intra) r = bc250_hevc_launch_refine8(cur.y, cur.py, ref.y, ref.py, s->wc, s->hc, s->imv8, s->mv8, st);
+    hipEventRecord(sl.ev[2], st);
+    if (r == 0) {
+        if (is_intra) {
+            r = bc250_launch_zero_i32(s->sync_h, 2, st);
+            if (r == 0)
+                r = bc250_hevc_launch_i_encode(cur.y, cur.uv, cur.py, cur.puv, out.y, out.uv, out.py, out.puv, s->wc,
+                                               s->hc, qp, num_slices, 4 /* rounding of intra CUs in an I picture */,
+                                               s->hmodes, (int16_t *)s->levels, s->sync_h, kSpinLimit, st);
+            if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, (int)n * 16, st);
+        } else {
+            r = bc250_hevc_launch_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv, out.y,
+                                           out.uv, out.py, out.puv, s->wc, s->hc, qp, 2 /* inter rounding */, s->mv8,
+                                           (int16_t *)s->levels, st);
+            if (r == 0) r = bc250_launch_zero_i32((int *)s->hmodes, (int)n * 4, st);
+        }
+    }
+    hipEventRecord(sl.ev[3], st);
+    hipEventRecord(sl.ev[4], st);
+    if (r == 0 && p.readback) {
+        bool ok = HIP_OK(hipMemcpyAsync(sl.levels, s->levels, n * 384 * 2, hipMemcpyDeviceToHost, st)) &&
+                  HIP_OK(hipMemcpyAsync(sl.hmodes, s->hmodes, n * 16, hipMemcpyDeviceToHost, st)) &&
+                  HIP_OK(hipMemcpyAsync(sl.mv8, s->mv8, n * 64, hipMemcpyDeviceToHost, st));
+        r = ok ? 0 : -1;
+    }
+    if (r == 0) {
+        bool ok = HIP_OK(hipMemcpyAsync(&sl.err[0], s->sync_h + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        sl.err[1] = 0;
+        r = ok ? 0 : -1;
+    }
+    return r;
+}
+
 extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t rt, int width, int height, int qp,
                                                int is_intra, int num_slices, int me_mode, const gpu_mv_t *cpu_mvs)
 {
@@ -512,6 +568,24 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     const Plane2 &ref = s->has_recon ? s->recon[s->ref_idx] : cur_in;
     const int out_idx = s->has_recon ? (s->ref_idx ^ 1) : 0;
     Plane2 &out = s->recon[out_idx];
+
+    if (p.codec == GPU_CODEC_HEVC) {
+        if (cpu_mvs) {
+            fprintf(stderr, "[bc250-hip] cpu_mvs is an H.264 route; HEVC searches on the GPU\n");
+            return -1;
+        }
+        if (enqueue_hevc(s, sl, cur_in, ref, out, qp, is_intra, num_slices) != 0) {
+            fprintf(stderr, "[bc250-hip] HEVC dispatch failed (%dx%d qp %d intra %d)\n", width, height, qp, is_intra);
+            return -1;
+        }
+        sl.have_data = p.readback != 0;
+        sl.was_intra = is_intra != 0;
+        sl.recorded = false;
+        s->ref_idx = out_idx;
+        s->has_recon = true;
+        publish_recon(ctx, s);
+        return 0;
+    }
 
     int r = 0;
     hipEventRecord(sl.ev[0], st);
@@ -693,11 +767,16 @@ static int staging(gpu_context_t *ctx, int slot, int which, void **data, size_t 
     case 2: *data = sl.modes; if (size) *size = n * 4; break;
     case 3: *data = sl.mvs; if (size) *size = n * 16; break;
     case 4: *data = sl.nz; if (size) *size = n * 24 * 4; break;
-    default: *data = sl.mv8; if (size) *size = n * 64; break;
+    case 5: *data = sl.mv8; if (size) *size = n * 64; break;
+    default: *data = sl.hmodes; if (size) *size = n * 16; break;
     }
     return 0;
 }
 
+extern "C" int gpu_compute_get_hevc_mode_staging_data_slot(gpu_context_t *c, int slot, void **d, size_t *z)
+{
+    return staging(c, slot, 6, d, z);
+}
 extern "C" int gpu_compute_get_quant_staging_data_slot(gpu_context_t *c, int slot, void **d, size_t *z) { return staging(c, slot, 0, d, z); }
 extern "C" int gpu_compute_get_dc_staging_data_slot(gpu_context_t *c, int slot, void **d, size_t *z) { return staging(c, slot, 1, d, z); }
 extern "C" int gpu_compute_get_pred_mode_staging_data_slot(gpu_context_t *c, int slot, void **d, size_t *z) { return staging(c, slot, 2, d, z); }
@@ -726,6 +805,7 @@ extern "C" int gpu_compute_set_encode_params(gpu_context_t *ctx, const gpu_encod
     if (!s || !p) return -1;
     if (p->search_range != 16 && p->search_range != 32 && p->search_range != 64) return -1;
     if (p->lambda8 < 0 || p->lambda8 > 15 || p->deblock_idc < 0 || p->deblock_idc > 2) return -1;
+    if (p->codec != GPU_CODEC_H264 && p->codec != GPU_CODEC_HEVC) return -1;
     s->params = *p;
     s->params.me_8x8 = p->me_8x8 ? 1 : 0;
     s->params.readback = p->readback ? 1 : 0;

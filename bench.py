@@ -1,12 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark of the encoder compute pipeline on one MI355X.
 
-One step = one 1920x1080 NV12 frame through the whole GPU pipeline (+-64
-integer full search on 16x16 and 8x8 blocks, quarter-pel refinement, P16x16
-or I16x16 encode, deblocking), frames already resident in HBM.  The timed
-region is an IPPP sequence of --steps frames driven by the host C sequence
-driver (host/h264_host.c) through the gpu_compute_* ABI, pipelined over the
-two staging slots.
+One step = one 1920x1080 NV12 frame through the whole GPU pipeline, frames
+already resident in HBM.  The headline (--codec hevc, the default) is the
+HEVC Main pipeline: +-64 integer full search on 16x16 and 8x8 blocks,
+quarter-sample refinement of every 8x8 CU with the HEVC interpolation, CU
+coding (8x8 luma / 4x4 chroma DCT, quant, recon), I picture as the CTU
+intra wavefront.  The H.264 pipeline (the reference's gpu_compute stages,
+with deblocking) runs on the same frames and is reported beside it.  The
+timed region is an IPPP sequence of --steps frames driven by the host C
+sequence driver (host/h264_host.c) through the gpu_compute_* ABI, pipelined
+over the two staging slots.
 
 Prints ONE JSON line.  Besides the contract fields it carries the per-stage
 GPU times from HIP events recorded on the pipeline's own stream during the
@@ -77,10 +81,11 @@ def copy_calibration(pkg, torch, nbytes=64 << 20, reps=3):
     return nbytes
 
 
-def cpu_baseline(frames_coded, qp, rng, lam, budget_s):
+def cpu_baseline(frames_coded, qp, rng, lam, budget_s, codec):
     """Oracle (single thread) on a bounded sample: the integer search of a
-    spread sample of macroblocks, scaled to the frame, plus sub-pel refine,
-    P encode and deblock of a whole frame."""
+    spread sample of macroblocks, scaled to the frame, plus the rest of a
+    P frame on the whole frame (HEVC: quarter-sample refine of every 8x8 CU
+    and CU coding; H.264: sub-pel refine, P encode and deblock)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     (ry, ruv), (cy, cuv) = frames_coded[0], frames_coded[1]
@@ -99,18 +104,55 @@ def cpu_baseline(frames_coded, qp, rng, lam, budget_s):
         i16[m], i8[4 * m:4 * m + 4] = a, b
     t_int = (time.perf_counter() - t) / n * nmb
     t = time.perf_counter()
-    mv16, _ = orc.me_refine(cy, ry, i16, i8)
-    t_ref = time.perf_counter() - t
-    t = time.perf_counter()
-    e = orc.encode_p(cy, cuv, ry, ruv, qp, mv16)
-    orc.deblock(e["y"], e["uv"], qp, 0, e["nz"], mv16, 1, 0)
+    if codec == "hevc":
+        mv = np.zeros((nmb * 4, 2), np.int32)
+        for cu in range(nmb * 4):
+            ctu, z = cu // 4, cu % 4
+            x0, y0 = (ctu % mbw) * 16 + (z & 1) * 8, (ctu // mbw) * 16 + (z >> 1) * 8
+            mx, my, _ = orc.hevc_refine8(cy, ry, x0, y0, int(i8[cu]["dx"]) * 4, int(i8[cu]["dy"]) * 4)
+            mv[cu] = (mx, my)
+        cb, cr = orc.split_uv(cuv)
+        fb, fr = orc.split_uv(ruv)
+        orc.hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, orc.HEVC_ROUND_INTER)
+        rest = "quarter-sample refine of every 8x8 CU and HEVC CU coding of the whole frame"
+    else:
+        mv16, _ = orc.me_refine(cy, ry, i16, i8)
+        e = orc.encode_p(cy, cuv, ry, ruv, qp, mv16)
+        orc.deblock(e["y"], e["uv"], qp, 0, e["nz"], mv16, 1, 0)
+        rest = "sub-pel refine, P encode and deblock of the whole frame"
     t_rest = time.perf_counter() - t
-    frame_s = t_int + t_ref + t_rest
+    frame_s = t_int + t_rest
     return {"value": round(1.0 / frame_s, 5), "unit": "fps", "cores": 1, "kind": "port",
             "sample": f"oracle, 1 thread: +-{rng} integer search of {n} of {nmb} MBs of one 1080p P frame "
-                      f"scaled to the frame ({t_int:.1f} s), plus sub-pel refine, P encode and deblock of the whole "
-                      f"frame ({t_ref + t_rest:.1f} s)",
+                      f"scaled to the frame ({t_int:.1f} s), plus {rest} ({t_rest:.1f} s)",
             "frame_s": round(frame_s, 2)}
+
+
+def run_pipeline(pkg, torch, frames, codec, args):
+    """Uploads the frames to a fresh context, runs the warm-up and the timed
+    IPPP sequence; returns fps, per-frame latencies and the stage statistics
+    from the pipeline's own HIP events."""
+    enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, me_8x8=1, readback=0,
+                      codec=pkg.GPU_CODEC_HEVC if codec == "hevc" else pkg.GPU_CODEC_H264)
+    try:
+        images = []
+        for y, uv in frames:
+            im = enc.new_image()
+            enc.upload(im, y, uv)
+            images.append(im)
+        log(f"{codec}: warmup")
+        enc.encode_sequence(images[:args.warmup], args.qp, gop=0, pipelined=1)
+        enc.stage_stats(reset=True)
+        log(f"{codec}: timed {args.steps} frames IPPP")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        frame_ms, _ = enc.encode_sequence(images[args.warmup:], args.qp, gop=0, pipelined=1)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return dict(wall=wall, fps=args.steps / wall, frame_ms=frame_ms, stats=enc.stage_stats(reset=True),
+                    errs=enc.error_flags())
+    finally:
+        enc.close()
 
 
 def main():
@@ -121,6 +163,9 @@ def main():
     ap.add_argument("--qp", type=int, default=27)
     ap.add_argument("--range", type=int, default=64)
     ap.add_argument("--lambda8", type=int, default=1)
+    ap.add_argument("--codec", choices=("hevc", "h264"), default="hevc",
+                    help="pipeline the headline value measures (the other one is reported beside it)")
+    ap.add_argument("--single", action="store_true", help="skip the second codec's run")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle integer search")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -135,32 +180,23 @@ def main():
     log(f"generating {n} frames {W}x{H}")
     g = pkg.FrameGen(1080, W, H, 12)
     frames = [g.frame(i) for i in range(n)]
-    enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, me_8x8=1, readback=0)
-    images = []
-    for y, uv in frames:
-        im = enc.new_image()
-        enc.upload(im, y, uv)
-        images.append(im)
-    log("warmup")
-    enc.encode_sequence(images[:args.warmup], args.qp, gop=0, pipelined=1)
-    enc.stage_stats(reset=True)
-
-    log(f"timed: {args.steps} frames IPPP")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    frame_ms, _ = enc.encode_sequence(images[args.warmup:], args.qp, gop=0, pipelined=1)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    st = enc.stage_stats(reset=True)
-    errs = enc.error_flags()
+    g.close()
+    other = "h264" if args.codec == "hevc" else "hevc"
+    runs = {args.codec: run_pipeline(pkg, torch, frames, args.codec, args)}
+    if not args.single:
+        runs[other] = run_pipeline(pkg, torch, frames, other, args)
+    head = runs[args.codec]
+    st = head["stats"]
 
     wc, hc = (W + 15) // 16 * 16, (H + 15) // 16 * 16
     nmb = (wc // 16) * (hc // 16)
-    fps = args.steps / wall
     p_frames = max(st["p_frames"], 1)
     me_ms = st["me_int_ms_p"] / p_frames  # average integer-search kernel time per launch
     stage_names = ["me_integer", "me_subpel", "encode", "deblock", "staging"]
-    stage_ms = {k: round(st["sum_ms"][i] / max(st["frames"], 1), 4) for i, k in enumerate(stage_names)}
+
+    def stage_ms(stats):
+        return {k: round(stats["sum_ms"][i] / max(stats["frames"], 1), 4) for i, k in enumerate(stage_names)}
+
     cand = (2 * args.range + 1) ** 2
     sad_ops = cand * nmb * 256                 # byte-absdiffs per launch
     hbm_bytes = 2 * wc * hc + nmb * 16 * 5     # cur + ref luma read, 5 MV records per MB written
@@ -171,32 +207,38 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
     peak_meas = sad_peak(pkg, torch)
     copy_calibration(pkg, torch)
+    if args.codec == "hevc":
+        desc = ("HEVC Main 8-bit pipeline: I picture as CTU16/CU8/PU4 intra (Planar/DC/H/V by SAD, 4x4 DST), "
+                "P pictures as 8x8 inter CUs with +-{r} integer full search (16x16 and 8x8 costs) refined to "
+                "quarter samples with the HEVC 8-tap filter, 8x8 luma / 4x4 chroma DCT, quant, recon; "
+                "no in-loop filter (as the reference's HEVC encoder)").format(r=args.range)
+    else:
+        desc = ("H.264 pipeline (the reference's gpu_compute stages): +-{r} integer full search on 16x16 and 8x8 "
+                "+ quarter-pel refine, P16x16 / I16x16 encode with 4x4 transform and quant, "
+                "H.264 8.7 deblock").format(r=args.range)
     res = {
         "metric": METRIC,
-        "value": round(fps, 2),
+        "value": round(head["fps"], 2),
         "unit": "fps",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "ms_per_step": round(head["wall"] * 1e3 / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
         "config": {
-            "workload": f"plain full-search pipeline, 1920x1080 NV12, {args.steps} frames IPPP, CQP {args.qp}, "
-                        f"+-{args.range} integer full search on 16x16 and 8x8 + quarter-pel refine, "
-                        "P16x16 / I16x16 encode with 4x4 integer transform and quant, H.264 8.7 deblock "
-                        "(the reference's gpu_compute pipeline, H.264 syntax)",
-            "width": W, "height": H, "frames": args.steps, "qp": args.qp, "search_range": args.range,
-            "lambda8": args.lambda8, "seed": 1080, "rectangles": 12,
+            "workload": f"1920x1080 NV12, {args.steps} frames IPPP, CQP {args.qp}: " + desc,
+            "codec": args.codec, "width": W, "height": H, "frames": args.steps, "qp": args.qp,
+            "search_range": args.range, "lambda8": args.lambda8, "seed": 1080, "rectangles": 12,
         },
-        "p50_frame_ms": round(float(np.percentile(frame_ms, 50)), 4),
-        "p99_frame_ms": round(float(np.percentile(frame_ms, 99)), 4),
-        "mpixel_per_s": round(fps * W * H / 1e6, 1),
-        "stage_ms": stage_ms,
-        "wavefront_wait_errors": errs,
+        "p50_frame_ms": round(float(np.percentile(head["frame_ms"], 50)), 4),
+        "p99_frame_ms": round(float(np.percentile(head["frame_ms"], 99)), 4),
+        "mpixel_per_s": round(head["fps"] * W * H / 1e6, 1),
+        "stage_ms": stage_ms(st),
+        "wavefront_wait_errors": head["errs"],
         "roofline": {
             "bound": "hbm",
             "kernel": "me_fullsearch_kernel<64>",
@@ -218,13 +260,17 @@ def main():
             "algorithmic_ops": sad_ops,
         },
     }
+    if other in runs:
+        o = runs[other]
+        res[f"{other}_pipeline"] = {"fps": round(o["fps"], 2), "ms_per_step": round(o["wall"] * 1e3 / args.steps, 4),
+                                    "p99_frame_ms": round(float(np.percentile(o["frame_ms"], 99)), 4),
+                                    "stage_ms": stage_ms(o["stats"]), "wavefront_wait_errors": o["errs"]}
     if not args.no_cpu:
         log("cpu baseline (oracle)")
         coded = [pkg.pad_coded(*frames[args.warmup]), pkg.pad_coded(*frames[args.warmup + 1])]
-        res["cpu_baseline"] = cpu_baseline(coded, args.qp, args.range, args.lambda8, args.cpu_budget)
+        res["cpu_baseline"] = cpu_baseline(coded, args.qp, args.range, args.lambda8, args.cpu_budget, args.codec)
     else:
         res["cpu_baseline"] = None
-    enc.close()
     print(json.dumps(res))
 
 

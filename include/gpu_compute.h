@@ -184,13 +184,30 @@ typedef struct {
     int deblock_idc;   /* disable_deblocking_filter_idc: 0 on, 1 off, 2 not across slices (default 0) */
     int me_8x8;        /* also refine the four 8x8 quadrant MVs (default 1) */
     int readback;      /* copy results into host staging at end_picture (default 1) */
+    int codec;         /* GPU_CODEC_H264 (default) or GPU_CODEC_HEVC, see below */
 } gpu_encode_params_t;
+
+/* codec = GPU_CODEC_HEVC runs the HEVC stages of include/bc250_hevc.h
+ * instead: I pictures as CTU16 / CU8 / PU4 intra (the reference's
+ * intra_trial), P pictures as 8x8 inter CUs with the +-R full-search 8x8
+ * vectors refined to quarter samples by the HEVC interpolation, 8x8 luma and
+ * 4x4 chroma DCT; no in-loop filter (the reference's HEVC encoder signals
+ * pps_deblocking_filter_disabled_flag = 1, encoder_h265.c:330).  Staging:
+ * quant = int16 [nctu*384] coefficients in the bc250_hevc.h layout, mv8 =
+ * the CU vectors (quarter samples), HEVC modes via
+ * gpu_compute_get_hevc_mode_staging_data_slot. */
+#define GPU_CODEC_H264 0
+#define GPU_CODEC_HEVC 1
 
 int gpu_compute_set_encode_params(gpu_context_t *ctx, const gpu_encode_params_t *params);
 int gpu_compute_get_encode_params(gpu_context_t *ctx, gpu_encode_params_t *params);
 
 /* The four 8x8 quadrant MVs of each MB (TL, TR, BL, BR), gpu_mv_t [nmb*4]. */
 int gpu_compute_get_mv8_staging_data_slot(gpu_context_t *ctx, int slot, void **data, size_t *size);
+
+/* HEVC I-picture PU modes, uint8 [nctu*16] in PU z-order (0 Planar, 1 DC,
+ * 10 H, 26 V); zero after a P picture. */
+int gpu_compute_get_hevc_mode_staging_data_slot(gpu_context_t *ctx, int slot, void **data, size_t *size);
 
 /* Synchronously copies the newest reconstruction (deblocked; the reference
  * picture of the next frame) into host planes of the visible size. */

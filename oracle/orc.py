@@ -278,3 +278,33 @@ def hevc_refine8(cur, ref, x0, y0, mvx, mvy, bd=8):
     mx, my = C.c_int(mvx), C.c_int(mvy)
     sad = lib.orc_hevc_refine8(p(cur), p(ref), bd, cur.shape[1], cur.shape[0], x0, y0, C.byref(mx), C.byref(my))
     return mx.value, my.value, sad
+
+
+def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lambda8=1):
+    """One picture of the HEVC pipeline in its stage order: I -> CTU intra
+    coding (rounding 4); P -> +-rng full search on 8x8 blocks, quarter-sample
+    refine with the HEVC interpolation, 8x8 inter CU coding (rounding 2).
+    Planes are coded-size NV12 (cuv / ruv interleaved); returns coef, modes
+    (uint8 [nctu*16]), mv8 (MV [nctu*4]) and the recon as NV12 (y, uv)."""
+    hc, wc = cy.shape
+    t = (wc // 16) * (hc // 16)
+    cb, cr = split_uv(cuv)
+    mv8 = np.zeros(t * 4, MV)
+    if is_intra:
+        o = hevc_encode_i(cy, cb, cr, qp, num_slices, HEVC_ROUND_INTRA_I)
+        modes = o["modes"]
+    else:
+        _, i8 = me_fullsearch(cy, ry, rng, lambda8)
+        mv = np.zeros((t * 4, 2), np.int32)
+        for cu in range(t * 4):
+            ctu, z = cu // 4, cu % 4
+            x0, y0 = (ctu % (wc // 16)) * 16 + (z & 1) * 8, (ctu // (wc // 16)) * 16 + (z >> 1) * 8
+            mx, my, sad = hevc_refine8(cy, ry, x0, y0, int(i8[cu]["dx"]) * 4, int(i8[cu]["dy"]) * 4)
+            mv[cu] = (mx, my)
+            mv8[cu] = (mx, my, sad, 0)
+        fb, fr = split_uv(ruv)
+        o = hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, HEVC_ROUND_INTER)
+        modes = np.zeros(t * 16, np.uint8)
+    uv = np.empty((hc // 2, wc), np.uint8)
+    uv[:, 0::2], uv[:, 1::2] = o["cb"], o["cr"]
+    return dict(coef=o["coef"], modes=modes, mv8=mv8, y=o["y"], uv=uv)

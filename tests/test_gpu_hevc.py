@@ -197,3 +197,48 @@ def test_i_encode_stage(pkg, w, h, qp, ns):
     assert np.array_equal(host(oy), want["y"])
     gb, gr = orc.split_uv(host(ouv))
     assert np.array_equal(gb, want["cb"]) and np.array_equal(gr, want["cr"])
+
+
+def _hevc_sequence(pkg, w, h, n, qp, ns, rng, lam, seed, nrect, gop=0):
+    say(f"hevc pipeline {w}x{h} frames={n} qp={qp} slices={ns} R={rng}")
+    fr = _frames(pkg, seed, w, h, n, nrect)
+    g = pkg.FrameGen(seed, w, h, nrect)
+    enc = pkg.Encoder(w, h, search_range=rng, lambda8=lam, codec=pkg.GPU_CODEC_HEVC)
+    ref = None
+    try:
+        for i in range(n):
+            im = enc.new_image()
+            enc.upload(im, *g.frame(i))
+            intra = i == 0 or (gop and i % gop == 0)
+            slot = enc.encode(im, qp, intra, ns)
+            got = enc.results(slot)
+            cy, cuv = fr[i]
+            ry, ruv = (cy, cuv) if ref is None else ref
+            exp = orc.hevc_encode_frame(cy, cuv, ry, ruv, qp, intra, ns, rng, lam)
+            assert np.array_equal(got["mv8"], exp["mv8"]), f"frame {i} mv8"
+            assert np.array_equal(got["hevc_mode"], exp["modes"]), f"frame {i} modes"
+            assert np.array_equal(got["quant"], exp["coef"]), f"frame {i} coef"
+            my, muv = enc.map_recon()
+            assert np.array_equal(my, exp["y"]) and np.array_equal(muv, exp["uv"]), f"frame {i} recon"
+            assert enc.error_flags() == 0
+            ref = (exp["y"], exp["uv"])
+            say(f"  frame {i} ok ({'I' if intra else 'P'})")
+    finally:
+        enc.close()
+        g.close()
+
+
+def test_hevc_pipeline_small_ipppp(pkg):
+    _torch()
+    _hevc_sequence(pkg, 128, 96, 5, 27, 1, 16, 1, seed=31, nrect=3)
+
+
+def test_hevc_pipeline_odd_size_slices_gop(pkg):
+    _torch()
+    _hevc_sequence(pkg, 200, 120, 5, 37, 3, 32, 2, seed=32, nrect=2, gop=3)
+
+
+def test_hevc_pipeline_640x480_cqp32_4slices(pkg):
+    """The 640x480 workload's shape: CQP 32, 4 slices, +-32 full search."""
+    _torch()
+    _hevc_sequence(pkg, 640, 480, 3, 32, 4, 32, 1, seed=2024, nrect=6)
