@@ -30,15 +30,17 @@ namespace {
 constexpr int WAVES = 4;
 constexpr int RING = 16;
 constexpr int REC = 8;   // dwords per CTU record: luma row 15 (16 B), Cb row 7 (8 B), Cr row 7 (8 B)
-constexpr int LP = 20;   // luma tile pitch: col 3 = left neighbour column, cols 4..19 = CTU
-constexpr int CPT = 12;  // chroma tile pitch: col 3 = left neighbour, cols 4..11 = CTU
+// Tiles: row 0 = the row above (col 3 corner, then the CTU's width, then 4
+// top-right samples), rows 1.. = the CTU with col 3 the left neighbour column.
+constexpr int LP = 24;   // luma tile pitch (cols 3..23), 17 rows
+constexpr int CPT = 16;  // chroma tile pitch (cols 3..15), 9 rows
 
 // LDS bytes of one wave, addressed by offset so a lane can pick its plane.
-constexpr int O_TY = 0;                 // luma tile 16 x LP
-constexpr int O_TC = O_TY + 16 * LP;    // Cb, Cr tiles 8 x CPT each
-constexpr int O_AY = O_TC + 2 * 8 * CPT;  // above rows, luma: 4 CTU slots x 16
-constexpr int O_AC = O_AY + 4 * 16;     // above rows, Cb / Cr: 2 x 4 slots x 8
-constexpr int O_SRC = O_AC + 2 * 4 * 8;  // source CTU: luma 256, Cb 64, Cr 64
+constexpr int O_TY = 0;                    // luma tile 17 x LP
+constexpr int O_TC = O_TY + 17 * LP;       // Cb, Cr tiles 9 x CPT each
+constexpr int O_AY = O_TC + 2 * 9 * CPT;   // records of the row above, luma: 4 CTU slots x 16
+constexpr int O_AC = O_AY + 4 * 16;        // records of the row above, Cb / Cr: 2 x 4 slots x 8
+constexpr int O_SRC = O_AC + 2 * 4 * 8;    // source CTU: luma 256, Cb 64, Cr 64
 constexpr int WAVE_BYTES = O_SRC + 384;
 
 struct alignas(16) WaveLds {
@@ -46,21 +48,24 @@ struct alignas(16) WaveLds {
     int res[64], tmp[64];
 };
 
-// z-scan rank for CTU16/CU8/PU4 (luma) or CTU8/CU4 (chroma), CTUs raster
-// (hevc_intra.c:75-130).
-BC_DEV int zrank(int x, int y, int wctu, bool luma)
+// z-scan order of the 4x4 blocks inside a CTU (hevc_intra.c:75-130): the
+// Morton code of the block coordinates (CTU16: 2 bits each, CTU8: 1 bit).
+BC_DEV int morton4(int rx, int ry)
 {
-    const int sh = luma ? 4 : 3, hs = sh - 1;
-    const int rx = x & ((1 << sh) - 1), ry = y & ((1 << sh) - 1);
-    int r = (((y >> sh) * wctu + (x >> sh)) << 2) + ((ry >> hs) << 1) + (rx >> hs);
-    if (luma) r = (r << 2) + (((ry & 7) >> 2) << 1) + ((rx & 7) >> 2);
-    return r;
+    const int bx = rx >> 2, by = ry >> 2;
+    return ((by >> 1) << 3) | ((bx >> 1) << 2) | ((by & 1) << 1) | (bx & 1);
 }
 
-BC_DEV bool zavail(int nx, int ny, int w, int h, int wctu, bool luma, int y_min, int cur)
+// Sum over each row of 16 lanes, result in every lane of the row: DPP
+// quad_perm [1,0,3,2] and [2,3,0,1], then half-row mirror and row mirror (all
+// gfx9 DPP forms), so no LDS round trip.
+BC_DEV uint32_t row16_sum(uint32_t v)
 {
-    if (nx < 0 || ny < y_min || nx >= w || ny >= h) return false;
-    return zrank(nx, ny, wctu, luma) < cur;
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);
+    return v;
 }
 
 BC_DEV int slice_first_row(int row, int rows, int ns)
@@ -89,15 +94,12 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
     __shared__ WaveLds wl[WAVES];
     __shared__ uint32_t ring[WAVES - 1][RING][REC];
     __shared__ int prog[WAVES], cons[WAVES];
-    __shared__ int mat[32];  // DST-VII rows, then DCT-II rows
     __shared__ int s_group;
 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     int *err = sync + 1;
     if (tid == 0) s_group = atomicAdd(sync, 1);
     if (tid < WAVES) prog[tid] = cons[tid] = 0;
-    if (tid < 16) mat[tid] = kDst4[tid];
-    else if (tid < 32) mat[tid] = kDct4[tid - 16];
     __syncthreads();
     const int wtu = wc >> 4, htu = hc >> 4;
     const int yrow = s_group * WAVES + wv;
@@ -110,19 +112,25 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
     const bool need_above = yrow > r0;
     const bool give_below = yrow + 1 < htu && slice_first_row(yrow + 1, htu, num_slices) != yrow + 1;
     const bool lds_above = wv > 0, lds_below = wv < WAVES - 1;
-    const int y_min = r0 * 16;
 
     // lane roles: group 0 luma PU samples, 1 Cb, 2 Cr (3: idle copy of Cr)
     const int grp = lane >> 4, k16 = lane & 15, pi = k16 >> 2, pj = k16 & 3;
     const bool luma = grp == 0;
     const int plane = grp > 2 ? 2 : grp;
-    const int pw = luma ? wc : wc >> 1, ph = luma ? hc : hc >> 1;
-    const int pwctu = wtu, cs = luma ? 16 : 8;
-    const int pymin = luma ? y_min : y_min >> 1;
-    const int tile = luma ? O_TY : O_TC + (plane - 1) * 8 * CPT, tp = luma ? LP : CPT;
-    const int above = luma ? O_AY : O_AC + (plane - 1) * 32;
+    const int cs = luma ? 16 : 8;
+    const int tile = luma ? O_TY : O_TC + (plane - 1) * 9 * CPT, tp = luma ? LP : CPT;
     const int srco = luma ? O_SRC : O_SRC + 256 + (plane - 1) * 64;
-    const int mbase = luma ? 0 : 16;  // DST for luma, DCT for chroma
+    // the matrix entries this lane multiplies by, kept in registers:
+    // row pi / row pj (forward passes), column pi / column pj (inverse passes)
+    int mri[4], mrj[4], mci[4], mcj[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int8_t *M = luma ? kDst4 : kDct4;
+        mri[t] = M[pi * 4 + t];
+        mrj[t] = M[pj * 4 + t];
+        mci[t] = M[t * 4 + pi];
+        mcj[t] = M[t * 4 + pj];
+    }
     const int cqp = hevc::chroma_qp(qp);
     const Quant QL = make_quant(clampi(qp, 0, 51), 5, round_q12), QC = make_quant(clampi(cqp, 0, 51), 5, round_q12);
     const int q = luma ? qp : cqp;
@@ -136,10 +144,8 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
     };
     prefetch(0);
 
-    auto sample = [&](int gx, int gy, int ctx0, int cty0) -> int {
-        if (gy < cty0) return B[above + (((gx / cs) & 3) * cs) + (gx & (cs - 1))];
-        return B[tile + (gy - cty0) * tp + 4 + gx - ctx0];
-    };
+    // CTU-local sample (rx in -1..cs+3, ry in -1..cs-1) of this lane's plane
+    auto sample = [&](int rx, int ry) -> int { return B[tile + (ry + 1) * tp + 4 + rx]; };
     auto take_record = [&](int j, uint32_t v, int d) {
         // dword d of record j: 0..3 luma bytes 4d.., 4..5 Cb, 6..7 Cr
         if (d < 4) *(uint32_t *)&B[O_AY + (j & 3) * 16 + d * 4] = v;
@@ -189,6 +195,18 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             if (need > x + 1) fetch_above(x + 1);
             wave_lds_sync();
             if (lds_above && lane == 0) vcons[wv] = need;
+            // tile row 0: corner (record x-1), top (record x), top-right (record x+1)
+            if (lane < 21) {
+                const int j = lane == 0 ? x - 1 : (lane <= 16 ? x : x + 1);
+                const int o = lane == 0 ? 15 : (lane <= 16 ? lane - 1 : lane - 17);
+                B[O_TY + 3 + lane] = B[O_AY + (j & 3) * 16 + o];
+            } else if (lane < 47) {
+                const int c = (lane - 21) / 13, t = (lane - 21) % 13;
+                const int j = t == 0 ? x - 1 : (t <= 8 ? x : x + 1);
+                const int o = t == 0 ? 7 : (t <= 8 ? t - 1 : t - 9);
+                B[O_TC + c * 9 * CPT + 3 + t] = B[O_AC + c * 32 + (j & 3) * 8 + o];
+            }
+            wave_lds_sync();
         }
         // ---- source of CTU x into LDS; prefetch CTU x+1
         *(uint32_t *)&B[O_SRC + lane * 4] = pre_y;
@@ -208,42 +226,43 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             const bool active = luma || (grp < 3 && pu == 0);
             const int bx = luma ? X0 + (cu & 1) * 8 + (pu & 1) * 4 : ctx0 + (cu & 1) * 4;
             const int by = luma ? Y0r + (cu >> 1) * 8 + (pu >> 1) * 4 : cty0 + (cu >> 1) * 4;
+            // ---- availability (z-scan inside the CTU, slices and edges outside)
+            const int rx = bx - ctx0, ry = by - cty0, mz = morton4(rx, ry);
+            const bool leftC = x > 0, aboveC = need_above, aboveR = need_above && x + 1 < wtu;
+            const bool aL = rx > 0 || leftC;
+            const bool aT = ry > 0 || aboveC;
+            const bool aC = rx > 0 ? (ry > 0 || aboveC) : (ry > 0 ? leftC : (leftC && aboveC));
+            const bool aTR = ry == 0 ? (rx + 4 < cs ? aboveC : aboveR) : (rx + 4 < cs && morton4(rx + 4, ry - 4) < mz);
+            const bool aBL = ry + 4 < cs && (rx == 0 ? leftC : morton4(rx - 4, ry + 4) < mz);
             // ---- neighbours with the substitution scan
-            const int cur = zrank(bx, by, pwctu, luma);
-            const bool aL = zavail(bx - 1, by, pw, ph, pwctu, luma, pymin, cur);
-            const bool aT = zavail(bx, by - 1, pw, ph, pwctu, luma, pymin, cur);
-            const bool aC = zavail(bx - 1, by - 1, pw, ph, pwctu, luma, pymin, cur);
-            const bool aTR = zavail(bx + 4, by - 1, pw, ph, pwctu, luma, pymin, cur);
-            const bool aBL = zavail(bx - 1, by + 4, pw, ph, pwctu, luma, pymin, cur);
             int v[10];
             bool a[10];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 a[t] = aL;
-                v[t] = aL ? sample(bx - 1, by + 3 - t, ctx0, cty0) : 0;
+                v[t] = sample(rx - 1, ry + 3 - t);
                 a[5 + t] = aT;
-                v[5 + t] = aT ? sample(bx + t, by - 1, ctx0, cty0) : 0;
+                v[5 + t] = sample(rx + t, ry - 1);
             }
             a[4] = aC;
-            v[4] = aC ? sample(bx - 1, by - 1, ctx0, cty0) : 0;
+            v[4] = sample(rx - 1, ry - 1);
             a[9] = aTR;
-            v[9] = aTR ? sample(bx + 4, by - 1, ctx0, cty0) : 0;
+            v[9] = sample(rx + 4, ry - 1);
+            const int vbl = sample(rx - 1, ry + 4 < cs ? ry + 4 : ry + 3);
             // first available in scan order fills what precedes it; later gaps copy the previous
             int firstv = 128;
-            bool any = false;
 #pragma unroll
             for (int t = 9; t >= 0; t--)
-                if (a[t]) { firstv = v[t]; any = true; }
+                if (a[t]) firstv = v[t];
             int prev = firstv;
 #pragma unroll
             for (int t = 0; t < 10; t++) {
                 v[t] = a[t] ? v[t] : prev;
                 prev = v[t];
             }
-            (void)any;
             const int L0 = v[3], L1 = v[2], L2 = v[1], L3 = v[0], Cc = v[4];
             const int T0 = v[5], T1 = v[6], T2 = v[7], T3 = v[8], T4 = v[9];
-            const int L4 = aBL ? sample(bx - 1, by + 4, ctx0, cty0) : L3;
+            const int L4 = aBL ? vbl : L3;
             const int Li = pi == 0 ? L0 : pi == 1 ? L1 : pi == 2 ? L2 : L3;
             const int Tj = pj == 0 ? T0 : pj == 1 ? T1 : pj == 2 ? T2 : T3;
             // ---- the four predictions of this lane's sample
@@ -261,13 +280,10 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             // ---- SAD decision over the 16 lanes of the group (luma only)
             uint32_t sp = (uint32_t)iabs(s - p_pl), sd = (uint32_t)iabs(s - p_dc);
             uint32_t sh = (uint32_t)iabs(s - p_h), sv = (uint32_t)iabs(s - p_v);
-#pragma unroll
-            for (int o = 8; o > 0; o >>= 1) {
-                sp += __shfl_xor(sp, o, 16);
-                sd += __shfl_xor(sd, o, 16);
-                sh += __shfl_xor(sh, o, 16);
-                sv += __shfl_xor(sv, o, 16);
-            }
+            sp = row16_sum(sp);
+            sd = row16_sum(sd);
+            sh = row16_sum(sh);
+            sv = row16_sum(sv);
             int mode = 0, pred = p_pl;
             uint32_t best = sp;
             if (sd < best) { best = sd; mode = 1; pred = p_dc; }
@@ -280,27 +296,27 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             wave_lds_sync();
             int acc = 0;
 #pragma unroll
-            for (int r = 0; r < 4; r++) acc += mat[mbase + pi * 4 + r] * RES[g16 + r * 4 + pj];
+            for (int r = 0; r < 4; r++) acc += mri[r] * RES[g16 + r * 4 + pj];
             TMP[lane] = sat16((acc + 1) >> 1);
             wave_lds_sync();
             acc = 0;
 #pragma unroll
-            for (int c = 0; c < 4; c++) acc += mat[mbase + pj * 4 + c] * TMP[g16 + pi * 4 + c];
+            for (int c = 0; c < 4; c++) acc += mrj[c] * TMP[g16 + pi * 4 + c];
             const int lv = quant(sat16((acc + 128) >> 8), luma ? QL : QC);
             // ---- dequantisation, inverse, reconstruction
             RES[lane] = dequant(lv, q, 5);
             wave_lds_sync();
             acc = 0;
 #pragma unroll
-            for (int t = 0; t < 4; t++) acc += mat[mbase + t * 4 + pi] * RES[g16 + t * 4 + pj];
+            for (int t = 0; t < 4; t++) acc += mci[t] * RES[g16 + t * 4 + pj];
             TMP[lane] = sat16((acc + 64) >> 7);
             wave_lds_sync();
             acc = 0;
 #pragma unroll
-            for (int t = 0; t < 4; t++) acc += mat[mbase + t * 4 + pj] * TMP[g16 + pi * 4 + t];
+            for (int t = 0; t < 4; t++) acc += mcj[t] * TMP[g16 + pi * 4 + t];
             const int rec = clampi(pred + sat16((acc + 2048) >> 12), 0, 255);
             if (active) {
-                B[tile + (by - cty0 + pi) * tp + 4 + (bx - ctx0 + pj)] = (uint8_t)rec;
+                B[tile + (ry + pi + 1) * tp + 4 + (rx + pj)] = (uint8_t)rec;
                 const size_t cb = (size_t)(yrow * wtu + x) * 384;
                 if (luma) coef[cb + k * 16 + k16] = (int16_t)lv;
                 else coef[cb + 256 + (plane - 1) * 64 + cu * 16 + k16] = (int16_t)lv;
@@ -311,9 +327,9 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
         // ---- hand the CTU's bottom rows down
         if (give_below) {
             uint32_t h = 0;
-            if (lane < 4) h = *(const uint32_t *)&B[O_TY + 15 * LP + 4 + lane * 4];
-            else if (lane < 6) h = *(const uint32_t *)&B[O_TC + 7 * CPT + 4 + (lane - 4) * 4];
-            else if (lane < 8) h = *(const uint32_t *)&B[O_TC + 8 * CPT + 7 * CPT + 4 + (lane - 6) * 4];
+            if (lane < 4) h = *(const uint32_t *)&B[O_TY + 16 * LP + 4 + lane * 4];
+            else if (lane < 6) h = *(const uint32_t *)&B[O_TC + 8 * CPT + 4 + (lane - 4) * 4];
+            else if (lane < 8) h = *(const uint32_t *)&B[O_TC + 9 * CPT + 8 * CPT + 4 + (lane - 6) * 4];
             if (lds_below) {
                 unsigned n = 0;
                 while (vcons[wv + 1] < x + 1 - RING + 1) {
@@ -329,19 +345,19 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
         }
         // ---- the CTU's reconstruction to the frame
         *(uint32_t *)(oy + (size_t)(Y0r + (lane >> 2)) * opy + X0 + (lane & 3) * 4) =
-            *(const uint32_t *)&B[O_TY + (lane >> 2) * LP + 4 + (lane & 3) * 4];
+            *(const uint32_t *)&B[O_TY + ((lane >> 2) + 1) * LP + 4 + (lane & 3) * 4];
         if (lane < 32) {
             const int r = lane >> 2, c2 = (lane & 3) * 2;
-            const uint8_t *tb = &B[O_TC + r * CPT + 4 + c2], *tr = &B[O_TC + 8 * CPT + r * CPT + 4 + c2];
+            const uint8_t *tb = &B[O_TC + (r + 1) * CPT + 4 + c2], *tr = &B[O_TC + 9 * CPT + (r + 1) * CPT + 4 + c2];
             *(uint32_t *)(ouv + (size_t)(yrow * 8 + r) * opuv + X0 + (lane & 3) * 4) =
                 (uint32_t)tb[0] | ((uint32_t)tr[0] << 8) | ((uint32_t)tb[1] << 16) | ((uint32_t)tr[1] << 24);
         }
         wave_lds_sync();
         // ---- slide: this CTU's right column becomes the left neighbour column
-        if (lane < 16) B[O_TY + lane * LP + 3] = B[O_TY + lane * LP + 19];
+        if (lane < 16) B[O_TY + (lane + 1) * LP + 3] = B[O_TY + (lane + 1) * LP + 19];
         else if (lane < 32) {
-            const int c = (lane - 16) >> 3, r = lane & 7;
-            B[O_TC + c * 8 * CPT + r * CPT + 3] = B[O_TC + c * 8 * CPT + r * CPT + 11];
+            const int c = (lane - 16) >> 3, r = (lane & 7) + 1;
+            B[O_TC + c * 9 * CPT + r * CPT + 3] = B[O_TC + c * 9 * CPT + r * CPT + 11];
         }
         wave_lds_sync();
     }
