@@ -127,6 +127,8 @@ _sig(hip, "bc250_hevc_tq4_batch", _I, [_P, _I, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_dqit4_batch", _I, [_P, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_tq8_batch", _I, [_P, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_dqit8_batch", _I, [_P, _I, _I, _I, _P, _P])
+_sig(hip, "bc250_hevc_tqn_batch", _I, [_P, _I, _I, _I, _I, _I, _P, _P])
+_sig(hip, "bc250_hevc_dqitn_batch", _I, [_P, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_mc_uni_batch", _I, [_P, _I, _I, _I, _P, _I, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_launch_refine8", _I, [_P, _I, _P, _I, _I, _I, _P, _P, _P])
 _sig(hip, "bc250_hevc_intra_work_bytes", C.c_size_t, [_I, _I])

@@ -162,6 +162,61 @@ __global__ __launch_bounds__(256) void hevc_dqit8_kernel(const int16_t *__restri
     if (b < n) out[(size_t)b * 64 + lane] = (int16_t)v;
 }
 
+// N x N DCT-II (N = 16 or 32) transform + quant and dequant + inverse, one
+// 256-thread workgroup per block.  The N-point matrix is built in LDS: row k
+// is row k*32/N of the 32-point one, entries are the standard's magnitudes of
+// 64*sqrt(2)*cos(pi*b/64) (b = (2n+1)k mod 128 folded into 0..32) with the
+// cosine's sign.  Shifts generalise the 4x4 / 8x8 ones: forward (log2n + bd -
+// 9, log2n + 6), quant / dequant bdShift bd + log2n - 5, inverse (7, 20 - bd).
+BC_DEV void build_dct_matrix(int *M, int N)
+{
+    const int oddset[16] = {90, 90, 88, 85, 82, 78, 73, 67, 61, 54, 46, 38, 31, 22, 13, 4};
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        const int k = e / N, n = e - k * N, kk = k * (32 / N);
+        int a = ((2 * n + 1) * kk) & 127, sign = 1;
+        if (a > 64) a = 128 - a;
+        if (a > 32) { a = 64 - a; sign = -1; }
+        int mag;
+        if (a & 1) mag = oddset[a >> 1];
+        else if (a & 2) mag = ((const int[8]){90, 87, 80, 70, 57, 43, 25, 9})[a >> 2];
+        else if (a & 4) mag = ((const int[4]){89, 75, 50, 18})[a >> 3];
+        else if (a & 8) mag = a == 8 ? 83 : 36;
+        else mag = 64;
+        M[e] = kk == 0 ? 64 : sign * mag;
+    }
+}
+
+template <bool INVERSE>
+__global__ __launch_bounds__(256) void hevc_nxn_kernel(const int16_t *__restrict__ in, int n, int log2n, int q, int bd,
+                                                       int round_q12, int16_t *__restrict__ out)
+{
+    __shared__ int M[1024], X[1024], T[1024];
+    const int N = 1 << log2n, N2 = N * N, b = blockIdx.x;
+    build_dct_matrix(M, N);
+    const int bds = bd + log2n - 5;
+    for (int e = threadIdx.x; e < N2; e += 256) {
+        const int v = in[(size_t)b * N2 + e];
+        X[e] = INVERSE ? dequant(v, clampi(q, 0, 63), bds) : v;
+    }
+    __syncthreads();
+    const int s1 = INVERSE ? 7 : log2n + bd - 9, s2 = INVERSE ? 20 - bd : log2n + 6;
+    for (int e = threadIdx.x; e < N2; e += 256) {  // columns: T[i][c] = sum_r M'[i][r] X[r][c]
+        const int i = e >> log2n, c = e & (N - 1);
+        int acc = 0;
+        for (int r = 0; r < N; r++) acc += (INVERSE ? M[r * N + i] : M[i * N + r]) * X[r * N + c];
+        T[e] = sat16((acc + (1 << (s1 - 1))) >> s1);
+    }
+    __syncthreads();
+    const Quant Q = make_quant(clampi(q, 0, bd == 8 ? 51 : 63), bds, round_q12);
+    for (int e = threadIdx.x; e < N2; e += 256) {  // rows: Y[i][j] = sum_c M'[j][c] T[i][c]
+        const int i = e >> log2n, j = e & (N - 1);
+        int acc = 0;
+        for (int c = 0; c < N; c++) acc += (INVERSE ? M[c * N + j] : M[j * N + c]) * T[i * N + c];
+        const int v = sat16((acc + (1 << (s2 - 1))) >> s2);
+        out[(size_t)b * N2 + e] = (int16_t)(INVERSE ? v : quant(v, Q));
+    }
+}
+
 // Batched uni-prediction: one wave per block (w*h <= 256 samples, each lane
 // up to 4 of them); the block's reference window (w+7) x (h+7) in LDS.
 template <typename Pix>
@@ -368,6 +423,24 @@ extern "C" int bc250_hevc_mc_uni_batch(const void *ref, int stride, int w_pic, i
         hevc_mc_batch_kernel<uint16_t><<<n, 64, 0, st>>>((const uint16_t *)ref, stride, w_pic, h_pic,
                                                          (const int4 *)blk, n, w, h, chroma ? 1 : 0, 10,
                                                          (uint16_t *)dst);
+    return launch_ok();
+}
+
+extern "C" int bc250_hevc_tqn_batch(const int16_t *res, int n, int log2n, int qp, int bd, int round_q12,
+                                    int16_t *out, void *stream)
+{
+    if (n < 0 || log2n < 2 || log2n > 5 || !bd_ok(bd) || round_q12 < 0 || round_q12 > 12) return -1;
+    if (n == 0) return 0;
+    hevc_nxn_kernel<false><<<n, 256, 0, (hipStream_t)stream>>>(res, n, log2n, qp, bd, round_q12, out);
+    return launch_ok();
+}
+
+extern "C" int bc250_hevc_dqitn_batch(const int16_t *coef, int n, int log2n, int qp, int bd, int16_t *out,
+                                      void *stream)
+{
+    if (n < 0 || log2n < 2 || log2n > 5 || !bd_ok(bd)) return -1;
+    if (n == 0) return 0;
+    hevc_nxn_kernel<true><<<n, 256, 0, (hipStream_t)stream>>>(coef, n, log2n, qp, bd, 0, out);
     return launch_ok();
 }
 

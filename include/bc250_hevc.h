@@ -48,6 +48,16 @@ int bc250_hevc_dqit4_batch(const int16_t *coef, int n, int qp, int use_dst, int 
 int bc250_hevc_tq8_batch(const int16_t *res, int n, int qp, int bd, int round_q12, int16_t *out, void *stream);
 int bc250_hevc_dqit8_batch(const int16_t *coef, int n, int qp, int bd, int16_t *out, void *stream);
 
+/* n blocks of N x N (N = 1 << log2n, 4..32) DCT-II: forward + quant and
+ * dequant + inverse.  The inverse is the decoder's (hevcd_transform,
+ * hevc_transform.c:274, with 8.6.2 flat-list scaling); the forward is the
+ * encoder's choice and generalises the reference's 4x4 / 8x8 shifts:
+ * (log2n + bd - 9, log2n + 6), quantiser shift bd + log2n - 5.  The
+ * reference encoder codes no TU above 8x8; these serve 16x16 / 32x32 TUs. */
+int bc250_hevc_tqn_batch(const int16_t *res, int n, int log2n, int qp, int bd, int round_q12, int16_t *out,
+                         void *stream);
+int bc250_hevc_dqitn_batch(const int16_t *coef, int n, int log2n, int qp, int bd, int16_t *out, void *stream);
+
 /* n blocks of w x h (w, h <= 16) from one plane (uint8 when bd == 8, else
  * uint16; stride in samples): blk[4*i .. 4*i+3] = {x, y, mvx, mvy}, vectors
  * in quarter (luma) or eighth (chroma) samples.  dst: n * w * h samples. */

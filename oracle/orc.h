@@ -119,6 +119,9 @@ void orc_hevc_encode_i(const void *sy, const void *scb, const void *scr, void *r
 void orc_hevc_encode_p(const void *sy, const void *scb, const void *scr, const void *fy, const void *fcb,
                        const void *fcr, void *ry, void *rcb, void *rcr, int bd, int wc, int hc, int qp,
                        int round_q12, const int32_t *mv, int16_t *coef);
+void orc_hevc_dct_matrix(int N, int *M);
+void orc_hevc_tqn(const int16_t *res, int log2n, int qp, int bd, int round_q12, int16_t *out);
+void orc_hevc_dqitn(const int16_t *coef, int log2n, int qp, int bd, int16_t *out);
 uint32_t orc_hevc_refine8(const void *cur, const void *ref, int bd, int wc, int hc, int x0, int y0,
                           int *mvx, int *mvy);
 

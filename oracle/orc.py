@@ -308,3 +308,27 @@ def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lamb
     uv = np.empty((hc // 2, wc), np.uint8)
     uv[:, 0::2], uv[:, 1::2] = o["cb"], o["cr"]
     return dict(coef=o["coef"], modes=modes, mv8=mv8, y=o["y"], uv=uv)
+
+_sig("orc_hevc_dct_matrix", None, [_I, _P])
+_sig("orc_hevc_tqn", None, [_P, _I, _I, _I, _I, _P])
+_sig("orc_hevc_dqitn", None, [_P, _I, _I, _I, _P])
+
+
+def hevc_dct_matrix(n):
+    m = np.zeros(n * n, np.int32)
+    lib.orc_hevc_dct_matrix(n, p(m))
+    return m.reshape(n, n)
+
+
+def hevc_tqn(res, log2n, qp, bd, round_q12):
+    n2 = 1 << (2 * log2n)
+    r, o = np.ascontiguousarray(res, np.int16).reshape(n2), np.zeros(n2, np.int16)
+    lib.orc_hevc_tqn(p(r), log2n, qp, bd, round_q12, p(o))
+    return o
+
+
+def hevc_dqitn(coef, log2n, qp, bd):
+    n2 = 1 << (2 * log2n)
+    c, o = np.ascontiguousarray(coef, np.int16).reshape(n2), np.zeros(n2, np.int16)
+    lib.orc_hevc_dqitn(p(c), log2n, qp, bd, p(o))
+    return o

@@ -43,7 +43,7 @@ static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v
  * horizontal frequency (or spatial row/column for the inverse). */
 static void transform2d(const int32_t *in, int n, const int *M, int inverse, int s1, int s2, int32_t *out)
 {
-    int32_t t[64];
+    int32_t t[32 * 32];
     for (int i = 0; i < n; i++)
         for (int c = 0; c < n; c++) {
             int64_t s = 0;
@@ -139,6 +139,61 @@ void orc_hevc_dqit8(const int16_t coef[64], int qp, int bd, int16_t out[64])
     for (int i = 0; i < 64; i++) x[i] = d[i];
     transform2d(x, 8, &kDct8[0][0], 1, 7, 20 - bd, y);
     for (int i = 0; i < 64; i++) out[i] = (int16_t)y[i];
+}
+
+/* N-point DCT-II of H.265 8.6.4.2 for N = 4..32: row k of the N-point
+ * matrix is row k*32/N of the 32-point one, whose entries are the standard's
+ * integer magnitudes of 64*sqrt(2)*cos(pi*b/64) (64 for the first row) with
+ * the cosine's sign, b = ((2n+1)k mod 128) folded into 0..32. */
+void orc_hevc_dct_matrix(int N, int *M)
+{
+    static const int odd32[16] = {90, 90, 88, 85, 82, 78, 73, 67, 61, 54, 46, 38, 31, 22, 13, 4};
+    static const int odd16[8] = {90, 87, 80, 70, 57, 43, 25, 9};
+    static const int odd8[4] = {89, 75, 50, 18};
+    static const int odd4[2] = {83, 36};
+    int mag[33];
+    for (int b = 0; b <= 32; b++) {
+        if (b & 1) mag[b] = odd32[b >> 1];
+        else if (b & 2) mag[b] = odd16[b >> 2];
+        else if (b & 4) mag[b] = odd8[b >> 3];
+        else if (b & 8) mag[b] = odd4[b >> 4];
+        else mag[b] = 64;  /* b = 0, 16, 32 (32 only as cos(pi/2), never reached) */
+    }
+    for (int k = 0; k < N; k++)
+        for (int n = 0; n < N; n++) {
+            const int kk = k * (32 / N);
+            int a = ((2 * n + 1) * kk) % 128, sign = 1;
+            if (a > 64) a = 128 - a;
+            if (a > 32) { a = 64 - a; sign = -1; }
+            M[k * N + n] = kk == 0 ? 64 : sign * mag[a];
+        }
+}
+
+/* N x N DCT-II (log2n 2..5) with the shifts the reference uses at 4 and 8
+ * generalised: forward (log2n + bd - 9, log2n + 6), quant shift bd + log2n -
+ * 5, dequant bdShift bd + log2n - 5, inverse (7, 20 - bd). */
+void orc_hevc_tqn(const int16_t *res, int log2n, int qp, int bd, int round_q12, int16_t *out)
+{
+    const int N = 1 << log2n, n2 = N * N;
+    static int M[32 * 32];
+    orc_hevc_dct_matrix(N, M);
+    int32_t x[1024], y[1024];
+    for (int i = 0; i < n2; i++) x[i] = res[i];
+    transform2d(x, N, M, 0, log2n + bd - 9, log2n + 6, y);
+    orc_hevc_quant(y, n2, bd + log2n - 5, clampi(qp, 0, bd == 8 ? 51 : 63), round_q12, out);
+}
+
+void orc_hevc_dqitn(const int16_t *coef, int log2n, int qp, int bd, int16_t *out)
+{
+    const int N = 1 << log2n, n2 = N * N;
+    static int M[32 * 32];
+    orc_hevc_dct_matrix(N, M);
+    int16_t d[1024];
+    orc_hevc_dequant(coef, n2, clampi(qp, 0, 63), bd + log2n - 5, d);
+    int32_t x[1024], y[1024];
+    for (int i = 0; i < n2; i++) x[i] = d[i];
+    transform2d(x, N, M, 1, 7, 20 - bd, y);
+    for (int i = 0; i < n2; i++) out[i] = (int16_t)y[i];
 }
 
 /* ---------------------------------------------------- motion compensation */

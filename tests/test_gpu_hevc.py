@@ -242,3 +242,26 @@ def test_hevc_pipeline_640x480_cqp32_4slices(pkg):
     """The 640x480 workload's shape: CQP 32, 4 slices, +-32 full search."""
     _torch()
     _hevc_sequence(pkg, 640, 480, 3, 32, 4, 32, 1, seed=2024, nrect=6)
+
+
+@pytest.mark.parametrize("log2n,bd", [(4, 8), (5, 8), (4, 10), (5, 10), (3, 8), (2, 8)])
+def test_tqn_dqitn_batch(pkg, log2n, bd):
+    torch = _torch()
+    n2 = 1 << (2 * log2n)
+    rs = np.random.default_rng(log2n * 100 + bd)
+    vmax = (1 << bd) - 1
+    n = 97
+    res = rs.integers(-vmax, vmax + 1, (n, n2)).astype(np.int16)
+    res[1::3] //= 17
+    for qp in ([0, 22, 27, 37, 51] if bd == 8 else [12, 39, 63]):
+        say(f"tqn log2n={log2n} bd={bd} qp={qp}")
+        dres, dout = dev(torch, res), torch.zeros((n, n2), dtype=torch.int16, device="cuda")
+        assert pkg.hip.bc250_hevc_tqn_batch(P(dres), n, log2n, qp, bd, 2, P(dout), None) == 0
+        dback = torch.zeros_like(dout)
+        assert pkg.hip.bc250_hevc_dqitn_batch(P(dout), n, log2n, qp, bd, P(dback), None) == 0
+        torch.cuda.synchronize()
+        lv, back = host(dout), host(dback)
+        for b in range(0, n, 4):
+            want = orc.hevc_tqn(res[b], log2n, qp, bd, 2)
+            assert np.array_equal(lv[b], want), (b, qp)
+            assert np.array_equal(back[b], orc.hevc_dqitn(want, log2n, qp, bd)), (b, qp)

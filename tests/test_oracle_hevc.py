@@ -386,3 +386,71 @@ def test_refine8_never_worse_and_consistent():
                            orc.hevc_mc_uni(ref_, x0, y0, 8, 8, mx, my, 0).astype(int)).sum()
             assert again == sad
             assert abs(mx - dx * 4) <= 3 and abs(my - dy * 4) <= 3
+
+
+# ------------------------------------------------------------ 16x16 / 32x32
+
+@need_ref
+def test_dct_matrices_match_reference_table():
+    """The N-point matrices generated from the standard's 32-point magnitudes
+    equal the reference decoder's hevcd_dct rows k*32/N (and the 4/8-point
+    ones the encoder uses)."""
+    table = np.frombuffer((C.c_int8 * 1024).in_dll(ref(), "hevcd_dct"), np.int8).reshape(32, 32).astype(int)
+    for n in (4, 8, 16, 32):
+        m = orc.hevc_dct_matrix(n)
+        assert np.array_equal(m, table[::32 // n, :n]), n
+    assert np.array_equal(orc.hevc_dct_matrix(4), [[64, 64, 64, 64], [83, 36, -36, -83], [64, -64, -64, 64],
+                                                   [36, -83, 83, -36]])
+    # and they are the DCT-II basis: within 2 of 64*sqrt(2)*cos(pi*(2n+1)k/(2N))
+    for n in (16, 32):
+        k, x = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+        ideal = np.round(64 * np.sqrt(2) * np.cos(np.pi * (2 * x + 1) * k / (2 * n)))
+        ideal[0] = 64
+        assert np.abs(orc.hevc_dct_matrix(n) - ideal).max() <= 2
+
+
+@need_ref
+@pytest.mark.parametrize("bd", [8, 10])
+@pytest.mark.parametrize("log2n", [4, 5])
+def test_inverse_nxn_matches_reference_decoder(bd, log2n):
+    L = ref()
+    L.hevcd_transform.argtypes = [P, I, C.c_bool, I]
+    n2 = 1 << (2 * log2n)
+    rs = np.random.default_rng(log2n * 10 + bd)
+    for qp in (4, 22, 27, 37, 51, 60):
+        for dens in (0.05, 0.3, 1.0):
+            lv = (rs.integers(-60, 61, n2) * (rs.random(n2) < dens)).astype(np.int16)
+            lv[rs.integers(0, n2)] = 0
+            d = np.zeros(n2, np.int16)
+            orc.lib.orc_hevc_dequant(p(lv), n2, qp, bd + log2n - 5, p(d))
+            want = d.copy()
+            L.hevcd_transform(p(want), log2n, False, bd)
+            assert np.array_equal(orc.hevc_dqitn(lv, log2n, qp, bd), want), (qp, dens)
+
+
+@pytest.mark.parametrize("log2n", [2, 3, 4, 5])
+def test_forward_nxn_round_trips(log2n):
+    """DC-only residuals come back exactly; textured residuals within a
+    quantisation step's error at a fine QP; the 4x4 and 8x8 forms equal the
+    dedicated ones the reference's encoder uses."""
+    n2 = 1 << (2 * log2n)
+    rs = np.random.default_rng(log2n)
+    for v in (-200, -3, 1, 77, 255):
+        res = np.full(n2, v, np.int16)
+        lv = orc.hevc_tqn(res, log2n, 4, 8, 6)
+        assert np.count_nonzero(lv[1:]) == 0
+        assert np.array_equal(orc.hevc_dqitn(lv, log2n, 4, 8), res)
+    # the integer bases are near-orthogonal, not exact: an impulse through the
+    # 32-point pair comes back 2% high, so the bound grows with the size
+    tol = {2: 3, 3: 3, 4: 3, 5: 6}[log2n]
+    for _ in range(10):
+        res = rs.integers(-255, 256, n2).astype(np.int16)
+        back = orc.hevc_dqitn(orc.hevc_tqn(res, log2n, 0, 8, 6), log2n, 0, 8)
+        assert np.abs(back.astype(int) - res).max() <= tol
+        if log2n == 3:
+            assert np.array_equal(orc.hevc_tqn(res, 3, 27, 8, 2), orc.hevc_tq8(res, 27, 8, 2))
+            assert np.array_equal(orc.hevc_dqitn(orc.hevc_tq8(res, 27, 8, 2), 3, 27, 8),
+                                  orc.hevc_dqit8(orc.hevc_tq8(res, 27, 8, 2), 27, 8))
+        if log2n == 2:
+            r4 = np.clip(res, -255, 255)
+            assert np.array_equal(orc.hevc_tqn(r4, 2, 27, 8, 4), orc.hevc_tq4(r4, 27, 0, 4, 8))
