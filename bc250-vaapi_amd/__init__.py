@@ -122,6 +122,8 @@ _sig(hip, "bc250_launch_zero_i32", _I, [_P, _I, _P])
 _sig(hip, "bc250_deblock_work_bytes", C.c_size_t, [_I, _I])
 _sig(hip, "bc250_intra_work_bytes", C.c_size_t, [_I, _I])
 _sig(hip, "bc250_launch_pad_edges", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P])
+_sig(hip, "bc250_h264_pred4x4_batch", _I, [_P, _P, _I, _P, _P])
+_sig(hip, "bc250_h264_pred8x8_batch", _I, [_P, _P, _I, _P, _P])
 # HEVC stages (include/bc250_hevc.h)
 _sig(hip, "bc250_hevc_tq4_batch", _I, [_P, _I, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_dqit4_batch", _I, [_P, _I, _I, _I, _I, _P, _P])

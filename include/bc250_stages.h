@@ -65,6 +65,19 @@ int bc250_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int h
                          const uint32_t *nz, const uint32_t *mvs, int num_slices, int idc, int *sync,
                          unsigned spin_limit, void *stream);
 
+/* Intra 4x4 / 8x8 luma prediction (H.264 8.3.1.2, 8.3.2.2 with the 8x8
+ * reference filter), batched over n blocks; the per-block functions they
+ * replace are the reference decoder's h264d_pred4x4 / h264d_pred8x8_luma
+ * (h264_pred.h:58-65).  The reference encoder codes only I16x16; these serve
+ * Intra4x4 / Intra8x8 macroblocks.  Inputs per block:
+ *   4x4: nbr[16*b] = p[-1,0..3], p[-1,-1], p[0..7,-1] (3 bytes unused)
+ *   8x8: nbr[32*b] = p[-1,0..7], p[-1,-1], p[0..15,-1] (7 bytes unused)
+ *   ctl[b] = mode (0..8: V, H, DC, DDL, DDR, VR, HD, VL, HU) | top<<4 |
+ *            left<<5 | corner<<6 | top-right<<7 (availability)
+ * out: 16 (64) samples per block, row-major. */
+int bc250_h264_pred4x4_batch(const uint8_t *nbr, const uint8_t *ctl, int n, uint8_t *out, void *stream);
+int bc250_h264_pred8x8_batch(const uint8_t *nbr, const uint8_t *ctl, int n, uint8_t *out, void *stream);
+
 /* Zero `n` ints with a fill kernel (no memset node, so capturable). */
 int bc250_launch_zero_i32(int *p, int n, void *stream);
 

@@ -89,6 +89,12 @@ int  orc_h264_alpha(int index_a);
 int  orc_h264_beta(int index_b);
 int  orc_h264_tc0(int index_a, int bs);
 
+/* --- intra 4x4 / 8x8 luma prediction (H.264 8.3.1.2, 8.3.2.2) ---------------- */
+void orc_h264_pred4x4(const uint8_t top8[8], const uint8_t left4[4], int corner, int mode, int avail_top,
+                      int avail_left, uint8_t out[16]);
+void orc_h264_pred8x8(const uint8_t top16[16], const uint8_t left8[8], int corner, int mode, int avail_top,
+                      int avail_left, int avail_corner, int avail_top_right, uint8_t out[64]);
+
 /* --- whole frame (stage order of the reference's dispatch) ----------------- */
 void orc_h264_encode_frame(const uint8_t *cy, const uint8_t *cuv,
                            const uint8_t *ry, const uint8_t *ruv, int py,

@@ -332,3 +332,24 @@ def hevc_dqitn(coef, log2n, qp, bd):
     c, o = np.ascontiguousarray(coef, np.int16).reshape(n2), np.zeros(n2, np.int16)
     lib.orc_hevc_dqitn(p(c), log2n, qp, bd, p(o))
     return o
+
+_sig("orc_h264_pred4x4", None, [_P, _P, _I, _I, _I, _I, _P])
+_sig("orc_h264_pred8x8", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P])
+
+
+def h264_pred4x4(top8, left4, corner, mode, at, al):
+    t, l = np.ascontiguousarray(top8, np.uint8), np.ascontiguousarray(left4, np.uint8)
+    o = np.zeros(16, np.uint8)
+    lib.orc_h264_pred4x4(p(t), p(l), int(corner), mode, int(at), int(al), p(o))
+    return o
+
+
+def h264_pred8x8(top16, left8, corner, mode, at, al, ac, atr):
+    t, l = np.ascontiguousarray(top16, np.uint8), np.ascontiguousarray(left8, np.uint8)
+    o = np.zeros(64, np.uint8)
+    lib.orc_h264_pred8x8(p(t), p(l), int(corner), mode, int(at), int(al), int(ac), int(atr), p(o))
+    return o
+
+
+# modes and the neighbours each needs (a decoder never uses a mode without them)
+H264_I4_NEEDS = {0: "t", 1: "l", 2: "", 3: "t", 4: "tlc", 5: "tlc", 6: "tlc", 7: "t", 8: "l"}

@@ -886,3 +886,110 @@ void orc_h264_encode_frame(const uint8_t *cy, const uint8_t *cuv, const uint8_t 
     }
     orc_h264_deblock(oy, ouv, py, puv, wc, hc, qp, is_intra, nz, mvs, ns, idc);
 }
+
+/* ------------------------------------------------------------------------
+ * Intra 4x4 and 8x8 luma prediction (H.264 8.3.1.2 and 8.3.2.2), restated
+ * from the standard's equations; the reference decoder's own forms are
+ * enc/src/h264_pred.c:45-150 and :321-470.  Neighbours in natural order:
+ * left[y] = p[-1,y], corner = p[-1,-1], top[x] = p[x,-1] (x < 8 for 4x4
+ * including the top-right run, x < 16 for 8x8).  Modes 0..8 = V, H, DC,
+ * diagonal down-left, diagonal down-right, vertical-right, horizontal-down,
+ * vertical-left, horizontal-up.
+ * ---------------------------------------------------------------------- */
+static int f3(int a, int b, int c) { return (a + 2 * b + c + 2) >> 2; }
+static int f2(int a, int b) { return (a + b + 1) >> 1; }
+
+/* One sample of the nine directional / DC modes for an n x n block
+ * (n = 4 or 8) from (already filtered, for 8x8) references; T(i) = p[i,-1]
+ * for i >= -1 (T(-1) = corner), L(i) = p[-1,i] for i >= -1. */
+static int dir_sample(int n, int mode, int x, int y, const int *top, const int *left, int corner, int dc)
+{
+#define T(i) ((i) < 0 ? corner : top[i])
+#define L(i) ((i) < 0 ? corner : left[i])
+    const int last = 2 * n - 1;   /* index of the last top sample */
+    switch (mode) {
+    case 0: return top[x];
+    case 1: return left[y];
+    case 2: return dc;
+    case 3:
+        if (x == n - 1 && y == n - 1) return (top[last - 1] + 3 * top[last] + 2) >> 2;
+        return f3(top[x + y], top[x + y + 1], top[x + y + 2]);
+    case 4:
+        if (x > y) return f3(T(x - y - 2), T(x - y - 1), top[x - y]);
+        if (x < y) return f3(L(y - x - 2), L(y - x - 1), left[y - x]);
+        return f3(top[0], corner, left[0]);
+    case 5: {
+        const int z = 2 * x - y;
+        if (z >= 0 && !(z & 1)) return f2(T(x - (y >> 1) - 1), top[x - (y >> 1)]);
+        if (z > 0) return f3(T(x - (y >> 1) - 2), T(x - (y >> 1) - 1), top[x - (y >> 1)]);
+        if (z == -1) return f3(left[0], corner, top[0]);
+        return f3(L(y - 2 * x - 1), L(y - 2 * x - 2), L(y - 2 * x - 3));
+    }
+    case 6: {
+        const int z = 2 * y - x;
+        if (z >= 0 && !(z & 1)) return f2(L(y - (x >> 1) - 1), left[y - (x >> 1)]);
+        if (z > 0) return f3(L(y - (x >> 1) - 2), L(y - (x >> 1) - 1), left[y - (x >> 1)]);
+        if (z == -1) return f3(left[0], corner, top[0]);
+        return f3(T(x - 2 * y - 1), T(x - 2 * y - 2), T(x - 2 * y - 3));
+    }
+    case 7:
+        if (!(y & 1)) return f2(top[x + (y >> 1)], top[x + (y >> 1) + 1]);
+        return f3(top[x + (y >> 1)], top[x + (y >> 1) + 1], top[x + (y >> 1) + 2]);
+    case 8: {
+        const int z = x + 2 * y, lim = 2 * n - 3;
+        if (z < lim && !(z & 1)) return f2(left[y + (x >> 1)], left[y + (x >> 1) + 1]);
+        if (z < lim) return f3(left[y + (x >> 1)], left[y + (x >> 1) + 1], left[y + (x >> 1) + 2]);
+        if (z == lim) return (left[n - 2] + 3 * left[n - 1] + 2) >> 2;
+        return left[n - 1];
+    }
+    default: return 128;
+    }
+#undef T
+#undef L
+}
+
+void orc_h264_pred4x4(const uint8_t top8[8], const uint8_t left4[4], int corner, int mode, int avail_top,
+                      int avail_left, uint8_t out[16])
+{
+    int t[8], l[4];
+    for (int i = 0; i < 8; i++) t[i] = top8[i];
+    for (int i = 0; i < 4; i++) l[i] = left4[i];
+    int dc;
+    if (avail_top && avail_left) dc = (t[0] + t[1] + t[2] + t[3] + l[0] + l[1] + l[2] + l[3] + 4) >> 3;
+    else if (avail_left) dc = (l[0] + l[1] + l[2] + l[3] + 2) >> 2;
+    else if (avail_top) dc = (t[0] + t[1] + t[2] + t[3] + 2) >> 2;
+    else dc = 128;
+    for (int y = 0; y < 4; y++)
+        for (int x = 0; x < 4; x++) out[y * 4 + x] = (uint8_t)dir_sample(4, mode, x, y, t, l, corner, dc);
+}
+
+/* 8.3.2.2.1 reference filtering, then the modes on the filtered samples. */
+void orc_h264_pred8x8(const uint8_t top16[16], const uint8_t left8[8], int corner, int mode, int avail_top,
+                      int avail_left, int avail_corner, int avail_top_right, uint8_t out[64])
+{
+    int t[16], l[8], ft[16], fl[8], fc = corner;
+    for (int i = 0; i < 16; i++) t[i] = (i >= 8 && avail_top && !avail_top_right) ? top16[7] : top16[i];
+    for (int i = 0; i < 8; i++) l[i] = left8[i];
+    for (int i = 0; i < 16; i++) ft[i] = t[i];
+    for (int i = 0; i < 8; i++) fl[i] = l[i];
+    if (avail_top) {
+        ft[0] = avail_corner ? f3(corner, t[0], t[1]) : (3 * t[0] + t[1] + 2) >> 2;
+        for (int i = 1; i < 15; i++) ft[i] = f3(t[i - 1], t[i], t[i + 1]);
+        ft[15] = (t[14] + 3 * t[15] + 2) >> 2;
+    }
+    if (avail_left) {
+        fl[0] = avail_corner ? f3(corner, l[0], l[1]) : (3 * l[0] + l[1] + 2) >> 2;
+        for (int i = 1; i < 7; i++) fl[i] = f3(l[i - 1], l[i], l[i + 1]);
+        fl[7] = (l[6] + 3 * l[7] + 2) >> 2;
+    }
+    if (avail_corner && avail_top && avail_left) fc = f3(t[0], corner, l[0]);
+    int dc;
+    int st = 0, sl = 0;
+    for (int i = 0; i < 8; i++) { st += ft[i]; sl += fl[i]; }
+    if (avail_top && avail_left) dc = (st + sl + 8) >> 4;
+    else if (avail_top) dc = (st + 4) >> 3;
+    else if (avail_left) dc = (sl + 4) >> 3;
+    else dc = 128;
+    for (int y = 0; y < 8; y++)
+        for (int x = 0; x < 8; x++) out[y * 8 + x] = (uint8_t)dir_sample(8, mode, x, y, ft, fl, fc, dc);
+}

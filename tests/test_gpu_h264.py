@@ -393,3 +393,41 @@ def test_1080p_integer_search_costs(pkg):
         costs = costs + 4 * lam * (np.abs(d)[:, None] + np.abs(d)[None, :])
         k = np.argmin(costs)  # first in raster order on ties
         assert (g16["dy"][m], g16["dx"][m]) == (k // (2 * rng + 1) - rng, k % (2 * rng + 1) - rng)
+
+
+# ------------------------------------------------------------------ intra 4x4 / 8x8
+
+@pytest.mark.parametrize("size", [4, 8])
+def test_intra_nxn_pred_batch(pkg, size):
+    torch = _torch()
+    say(f"intra {size}x{size} prediction batch")
+    rs = np.random.default_rng(size)
+    reps = []
+    for mode, needs in orc.H264_I4_NEEDS.items():
+        for at in (0, 1):
+            for al in (0, 1):
+                for ac in (0, 1):
+                    for atr in (0, 1):
+                        if ("t" in needs and not at) or ("l" in needs and not al) or ("c" in needs and not ac):
+                            continue
+                        reps += [(mode, at, al, ac, atr)] * 12
+    n = len(reps)
+    stride = 16 if size == 4 else 32
+    nbr = rs.integers(0, 256, (n, stride)).astype(np.uint8)
+    ctl = np.array([m | (at << 4) | (al << 5) | (ac << 6) | (atr << 7) for (m, at, al, ac, atr) in reps], np.uint8)
+    if size == 4:
+        for i, (m, at, al, ac, atr) in enumerate(reps):
+            if not atr:
+                nbr[i, 9:13] = nbr[i, 8]  # missing top-right run = p[3,-1] (the caller's substitution at 4x4)
+    dn, dc = dev(torch, nbr), dev(torch, ctl)
+    dout = torch.zeros(n * size * size, dtype=torch.uint8, device="cuda")
+    f = pkg.hip.bc250_h264_pred4x4_batch if size == 4 else pkg.hip.bc250_h264_pred8x8_batch
+    assert f(C.c_void_p(dn.data_ptr()), C.c_void_p(dc.data_ptr()), n, C.c_void_p(dout.data_ptr()), None) == 0
+    torch.cuda.synchronize()
+    got = host(dout).reshape(n, size * size)
+    for i, (m, at, al, ac, atr) in enumerate(reps):
+        if size == 4:
+            want = orc.h264_pred4x4(nbr[i, 5:13], nbr[i, 0:4], nbr[i, 4], m, at, al)
+        else:
+            want = orc.h264_pred8x8(nbr[i, 9:25], nbr[i, 0:8], nbr[i, 8], m, at, al, ac, atr)
+        assert np.array_equal(got[i], want), (i, m, at, al, ac, atr)

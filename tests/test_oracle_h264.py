@@ -468,3 +468,65 @@ def test_deblock_picture_invariants():
     assert (a == bands).all()
     a, _ = orc.deblock(bands, flat_uv, 40, 1, nz0, mv0, 3, 0)
     assert (a[13:19] != bands[13:19]).any(axis=1).all() and (a[:12] == bands[:12]).all()
+
+
+# ------------------------------------------------------------ intra 4x4 / 8x8
+
+def _avail_cases(needs):
+    out = []
+    for at in (0, 1):
+        for al in (0, 1):
+            for ac in (0, 1):
+                for atr in (0, 1):
+                    if ("t" in needs and not at) or ("l" in needs and not al) or ("c" in needs and not ac):
+                        continue
+                    if ac and not (at and al) and "c" in needs:
+                        continue
+                    out.append((at, al, ac, atr))
+    return out
+
+
+@need_ref
+def test_intra4x4_matches_reference():
+    L = ref()
+    L.h264d_pred4x4.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_uint8, C.c_bool,
+                                C.c_bool]
+    rs = np.random.default_rng(44)
+    for mode, needs in orc.H264_I4_NEEDS.items():
+        for (at, al, ac, atr) in _avail_cases(needs):
+            for _ in range(25):
+                t = rs.integers(0, 256, 8).astype(np.uint8)
+                l = rs.integers(0, 256, 4).astype(np.uint8)
+                if not atr:
+                    t[4:] = t[3]  # the decoder pads a missing top-right run with p[3,-1]
+                c = int(rs.integers(0, 256))
+                want = np.zeros(16, np.uint8)
+                L.h264d_pred4x4(p(want), 4, mode, p(t), p(l), c, bool(at), bool(al))
+                assert np.array_equal(orc.h264_pred4x4(t, l, c, mode, at, al), want), (mode, at, al)
+
+
+@need_ref
+def test_intra8x8_matches_reference():
+    L = ref()
+    L.h264d_pred8x8_luma.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_uint8, C.c_bool,
+                                     C.c_bool, C.c_bool, C.c_bool]
+    rs = np.random.default_rng(88)
+    for mode, needs in orc.H264_I4_NEEDS.items():
+        for (at, al, ac, atr) in _avail_cases(needs):
+            for _ in range(25):
+                t = rs.integers(0, 256, 16).astype(np.uint8)
+                l = rs.integers(0, 256, 8).astype(np.uint8)
+                c = int(rs.integers(0, 256))
+                want = np.zeros(64, np.uint8)
+                L.h264d_pred8x8_luma(p(want), 8, mode, p(t), p(l), c, bool(at), bool(al), bool(ac), bool(atr))
+                got = orc.h264_pred8x8(t, l, c, mode, at, al, ac, atr)
+                assert np.array_equal(got, want), (mode, at, al, ac, atr)
+
+
+def test_intra_flat_neighbours_predict_flat():
+    """Every mode maps constant neighbours to the same constant (each sample
+    is a normalised average of neighbours)."""
+    for v in (0, 17, 200, 255):
+        for mode in range(9):
+            assert (orc.h264_pred4x4(np.full(8, v), np.full(4, v), v, mode, 1, 1) == v).all()
+            assert (orc.h264_pred8x8(np.full(16, v), np.full(8, v), v, mode, 1, 1, 1, 1) == v).all()
