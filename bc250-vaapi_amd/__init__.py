@@ -124,6 +124,8 @@ _sig(hip, "bc250_intra_work_bytes", C.c_size_t, [_I, _I])
 _sig(hip, "bc250_launch_pad_edges", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P])
 _sig(hip, "bc250_h264_pred4x4_batch", _I, [_P, _P, _I, _P, _P])
 _sig(hip, "bc250_h264_pred8x8_batch", _I, [_P, _P, _I, _P, _P])
+_sig(hip, "bc250_h264_tq8x8_batch", _I, [_P, _I, _I, _I, _P, _P])
+_sig(hip, "bc250_h264_dqit8x8_batch", _I, [_P, _I, _I, _P, _P])
 # HEVC stages (include/bc250_hevc.h)
 _sig(hip, "bc250_hevc_tq4_batch", _I, [_P, _I, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_dqit4_batch", _I, [_P, _I, _I, _I, _I, _P, _P])
@@ -138,6 +140,7 @@ _sig(hip, "bc250_hevc_launch_i_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _I,
                                             _P])
 _sig(hip, "bc250_hevc_launch_p_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P,
                                             _P, _P])
+_sig(hip, "bc250_hevc_launch_deblock", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P])
 
 _sig(host, "bc250_gen_create", _P, [C.c_uint64, _I, _I, _I])
 _sig(host, "bc250_gen_destroy", None, [_P])

@@ -1,0 +1,232 @@
+// hevc_deblock.hip - H.265 8.7.2 deblocking for gfx950, for the pipeline's
+// coding shape (8x8 CUs, one PU each, one reference picture).
+//
+// HEVC filters edges on the 8x8 grid and changes at most three samples on
+// either side, so within one direction every edge segment is independent:
+// two flat launches, all vertical edges, then all horizontal edges on that
+// result.  One thread per 4-line segment: a luma segment of one 8x8-grid
+// edge, or a chroma segment (Cb and Cr together, NV12 interleaved) of one
+// edge of the 8x8 chroma grid (chroma is filtered at bS 2 only).  bS: 2
+// intra; 1 if either CU has coded luma or the vectors differ by >= 4
+// quarter samples; else 0 (the reference decoder's strength(),
+// enc/src/hevc_filter_template.c:221-290).  Picture borders and slice
+// borders (CTU-row slices, no filtering across slices) are left alone.
+#include "bc250_hevc.h"
+#include "hevc_dev.h"
+
+using namespace hevc;
+
+namespace {
+__constant__ uint8_t kBetaT[52] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  6,  7,
+                                   8,  9,  10, 11, 12, 13, 14, 15, 16, 17, 18, 20, 22, 24, 26, 28, 30, 32,
+                                   34, 36, 38, 40, 42, 44, 46, 48, 50, 52, 54, 56, 58, 60, 62, 64};
+__constant__ uint8_t kTcT[54] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,  0,  1,  1,  1,  1,  1,  1,  1,  1, 1,
+                                 2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 5, 5, 6, 6, 7, 8, 9, 10, 11, 13, 14, 16, 18, 20, 22, 24};
+
+BC_DEV int cu_at(int x, int y, int wctu)
+{
+    return ((y >> 4) * wctu + (x >> 4)) * 4 + ((y >> 3) & 1) * 2 + ((x >> 3) & 1);
+}
+
+BC_DEV int strength(int intra, const uint8_t *cbf, const uint4 *mv, int cp, int cq)
+{
+    if (intra) return 2;
+    if (cbf[cp] | cbf[cq]) return 1;
+    const uint4 a = mv[cp], b = mv[cq];
+    return (iabs((int)a.x - (int)b.x) >= 4 || iabs((int)a.y - (int)b.y) >= 4) ? 1 : 0;
+}
+
+BC_DEV int slice_first_row(int row, int rows, int ns)
+{
+    ns = ns < 1 ? 1 : (ns > rows ? rows : ns);
+    int r0 = 0;
+    for (int s = 0; s < ns; s++) {
+        const int b = (int)(((long long)rows * s) / ns);
+        if (row >= b) r0 = b;
+    }
+    return r0;
+}
+
+// One luma segment: v[k][0..7] = p3 p2 p1 p0 q0 q1 q2 q3 of line k (8.7.2.5.3,
+// 8.7.2.5.6, 8.7.2.5.7).
+BC_DEV void luma_segment(int v[4][8], int beta, int tc)
+{
+    const int dp0 = iabs(v[0][1] - 2 * v[0][2] + v[0][3]), dp3 = iabs(v[3][1] - 2 * v[3][2] + v[3][3]);
+    const int dq0 = iabs(v[0][6] - 2 * v[0][5] + v[0][4]), dq3 = iabs(v[3][6] - 2 * v[3][5] + v[3][4]);
+    if (dp0 + dq0 + dp3 + dq3 >= beta) return;
+    bool strong = true;
+#pragma unroll
+    for (int k = 0; k < 4; k += 3) {
+        const int dd = k ? dp3 + dq3 : dp0 + dq0;
+        strong = strong && 2 * dd < (beta >> 2) && iabs(v[k][0] - v[k][3]) + iabs(v[k][4] - v[k][7]) < (beta >> 3) &&
+                 iabs(v[k][3] - v[k][4]) < ((5 * tc + 1) >> 1);
+    }
+    const bool dEp = (dp0 + dp3) < ((beta + (beta >> 1)) >> 3), dEq = (dq0 + dq3) < ((beta + (beta >> 1)) >> 3);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int p3 = v[k][0], p2 = v[k][1], p1 = v[k][2], p0 = v[k][3];
+        const int q0 = v[k][4], q1 = v[k][5], q2 = v[k][6], q3 = v[k][7];
+        if (strong) {
+            const int t2 = 2 * tc;
+            v[k][3] = clampi((p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3, p0 - t2, p0 + t2);
+            v[k][2] = clampi((p2 + p1 + p0 + q0 + 2) >> 2, p1 - t2, p1 + t2);
+            v[k][1] = clampi((2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3, p2 - t2, p2 + t2);
+            v[k][4] = clampi((p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3, q0 - t2, q0 + t2);
+            v[k][5] = clampi((p0 + q0 + q1 + q2 + 2) >> 2, q1 - t2, q1 + t2);
+            v[k][6] = clampi((p0 + q0 + q1 + 3 * q2 + 2 * q3 + 4) >> 3, q2 - t2, q2 + t2);
+        } else {
+            int delta = (9 * (q0 - p0) - 3 * (q1 - p1) + 8) >> 4;
+            if (iabs(delta) >= tc * 10) continue;
+            delta = clampi(delta, -tc, tc);
+            v[k][3] = clip255(p0 + delta);
+            v[k][4] = clip255(q0 - delta);
+            const int th = tc >> 1;
+            if (dEp) v[k][2] = clip255(p1 + clampi((((p2 + p0 + 1) >> 1) - p1 + delta) >> 1, -th, th));
+            if (dEq) v[k][5] = clip255(q1 + clampi((((q2 + q0 + 1) >> 1) - q1 - delta) >> 1, -th, th));
+        }
+    }
+}
+
+BC_DEV void chroma_line(int &p1, int &p0, int &q0, int &q1, int tc)
+{
+    const int delta = clampi((((q0 - p0) * 4) + p1 - q1 + 4) >> 3, -tc, tc);
+    p0 = clip255(p0 + delta);
+    q0 = clip255(q0 - delta);
+}
+}  // namespace
+
+// VERT: vertical edges (filtered along x); otherwise horizontal edges.
+template <bool VERT>
+__global__ __launch_bounds__(256) void hevc_deblock_kernel(uint8_t *__restrict__ y, uint8_t *__restrict__ uv, int py,
+                                                           int puv, int wc, int hc, int qp, int intra,
+                                                           const uint8_t *__restrict__ cbf,
+                                                           const uint4 *__restrict__ mv, int num_slices, int nl,
+                                                           int nc)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int wctu = wc >> 4, hctu = hc >> 4;
+    if (t < nl) {
+        // luma: edges at 8, 16, .. (VERT: columns), segments of 4 lines along the edge
+        const int nseg = (VERT ? hc : wc) >> 2;
+        const int e = 8 * (t / nseg + 1), s = 4 * (t % nseg);
+        if (!VERT && (e & 15) == 0 && slice_first_row(e >> 4, hctu, num_slices) == (e >> 4)) return;
+        const int x = VERT ? e : s, yy = VERT ? s : e;
+        const int cq = cu_at(x, yy, wctu), cp = cu_at(VERT ? x - 1 : x, VERT ? yy : yy - 1, wctu);
+        const int bs = strength(intra, cbf, mv, cp, cq);
+        if (!bs) return;
+        const int beta = kBetaT[clampi(qp, 0, 51)], tc = kTcT[clampi(qp + 2 * (bs - 1), 0, 53)];
+        int v[4][8];
+        if (VERT) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t *row = (const uint32_t *)(y + (size_t)(yy + k) * py + x - 4);
+                const uint32_t a = row[0], b = row[1];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    v[k][i] = (a >> (8 * i)) & 255;
+                    v[k][4 + i] = (b >> (8 * i)) & 255;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const uint32_t r = *(const uint32_t *)(y + (size_t)(yy - 4 + i) * py + x);
+#pragma unroll
+                for (int k = 0; k < 4; k++) v[k][i] = (r >> (8 * k)) & 255;
+            }
+        }
+        luma_segment(v, beta, tc);
+        if (VERT) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                uint32_t *row = (uint32_t *)(y + (size_t)(yy + k) * py + x - 4);
+                row[0] = (uint32_t)v[k][0] | ((uint32_t)v[k][1] << 8) | ((uint32_t)v[k][2] << 16) | ((uint32_t)v[k][3] << 24);
+                row[1] = (uint32_t)v[k][4] | ((uint32_t)v[k][5] << 8) | ((uint32_t)v[k][6] << 16) | ((uint32_t)v[k][7] << 24);
+            }
+        } else {
+#pragma unroll
+            for (int i = 1; i < 7; i++)
+                *(uint32_t *)(y + (size_t)(yy - 4 + i) * py + x) = (uint32_t)v[0][i] | ((uint32_t)v[1][i] << 8) |
+                                                                    ((uint32_t)v[2][i] << 16) |
+                                                                    ((uint32_t)v[3][i] << 24);
+        }
+        return;
+    }
+    const int c = t - nl;
+    if (c >= nc) return;
+    // chroma: edges at chroma 8, 16, .. (luma multiples of 16), 4-line segments, bS 2 only
+    const int nseg = (VERT ? hc : wc) >> 3;
+    const int e = 8 * (c / nseg + 1), s = 4 * (c % nseg);
+    if (!VERT && (e & 7) == 0 && slice_first_row(e >> 3, hctu, num_slices) == (e >> 3)) return;
+    const int cx = VERT ? e : s, cy = VERT ? s : e;
+    const int lx = 2 * cx, ly = 2 * cy;
+    const int cq = cu_at(lx, ly, wctu), cp = cu_at(VERT ? lx - 1 : lx, VERT ? ly : ly - 1, wctu);
+    if (strength(intra, cbf, mv, cp, cq) != 2) return;
+    const int tc = kTcT[clampi(hevc::chroma_qp(qp) + 2, 0, 53)];
+    if (VERT) {
+        // per line: bytes Cb/Cr of chroma columns cx-2 .. cx+1 (8 bytes)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint32_t *row = (uint32_t *)(uv + (size_t)(cy + k) * puv + 2 * cx - 4);
+            uint32_t a = row[0], b = row[1];
+#pragma unroll
+            for (int comp = 0; comp < 2; comp++) {
+                int p1 = (a >> (8 * comp)) & 255, p0 = (a >> (16 + 8 * comp)) & 255;
+                int q0 = (b >> (8 * comp)) & 255, q1 = (b >> (16 + 8 * comp)) & 255;
+                chroma_line(p1, p0, q0, q1, tc);
+                a = (a & ~(0xffu << (16 + 8 * comp))) | ((uint32_t)p0 << (16 + 8 * comp));
+                b = (b & ~(0xffu << (8 * comp))) | ((uint32_t)q0 << (8 * comp));
+            }
+            row[0] = a;
+            row[1] = b;
+        }
+    } else {
+        // 4 chroma columns (8 interleaved bytes) across rows cy-2 .. cy+1
+        uint32_t r[4][2];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t *row = (const uint32_t *)(uv + (size_t)(cy - 2 + i) * puv + 2 * cx);
+            r[i][0] = row[0];
+            r[i][1] = row[1];
+        }
+#pragma unroll
+        for (int byte = 0; byte < 8; byte++) {
+            const int w = byte >> 2, sh = 8 * (byte & 3);
+            int p1 = (r[0][w] >> sh) & 255, p0 = (r[1][w] >> sh) & 255, q0 = (r[2][w] >> sh) & 255,
+                q1 = (r[3][w] >> sh) & 255;
+            chroma_line(p1, p0, q0, q1, tc);
+            r[1][w] = (r[1][w] & ~(0xffu << sh)) | ((uint32_t)p0 << sh);
+            r[2][w] = (r[2][w] & ~(0xffu << sh)) | ((uint32_t)q0 << sh);
+        }
+#pragma unroll
+        for (int i = 1; i < 3; i++) {
+            uint32_t *row = (uint32_t *)(uv + (size_t)(cy - 2 + i) * puv + 2 * cx);
+            row[0] = r[i][0];
+            row[1] = r[i][1];
+        }
+    }
+}
+
+extern "C" int bc250_hevc_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp,
+                                         int is_intra, const uint8_t *cbf, const uint32_t *mvs, int num_slices,
+                                         void *stream)
+{
+    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || (py & 3) || (puv & 3) || py < wc ||
+        puv < wc || num_slices < 1)
+        return -1;
+    hipStream_t st = (hipStream_t)stream;
+    // vertical pass
+    int nl = (wc / 8 - 1) * (hc / 4), nc = (wc / 16 - 1) * (hc / 8);
+    int n = nl + nc;
+    if (n > 0)
+        hevc_deblock_kernel<true><<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf,
+                                                                   (const uint4 *)mvs, num_slices, nl, nc);
+    // horizontal pass
+    nl = (hc / 8 - 1) * (wc / 4);
+    nc = (hc / 16 - 1) * (wc / 8);
+    n = nl + nc;
+    if (n > 0)
+        hevc_deblock_kernel<false><<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf,
+                                                                    (const uint4 *)mvs, num_slices, nl, nc);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

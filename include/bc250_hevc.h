@@ -92,6 +92,20 @@ int bc250_hevc_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, i
                                int opy, int opuv, int wc, int hc, int qp, int num_slices, int round_q12,
                                uint8_t *modes, int16_t *coef, int *sync, unsigned spin_limit, void *stream);
 
+/* H.265 8.7.2 deblocking of a reconstructed picture in place (wc, hc
+ * multiples of 16; pitches multiples of 4) for the pipeline's coding shape:
+ * 8x8 CUs in CTU-raster, z-order (cu = ctu*4 + (y/8&1)*2 + (x/8&1)), one
+ * PU each.  Vertical edges of the 8x8 grid first, then horizontal edges;
+ * picture and slice borders (CTU-row slices as in i_encode) are not
+ * filtered.  bS 2 when is_intra, else 1 if cbf[cp] | cbf[cq] or the vectors
+ * mvs[4*cu] = {mvx, mvy, ...} differ by >= 4 quarter samples, else 0
+ * (approach1-compute-encoder/src/hevc_filter_template.c:221-290, the
+ * reference decoder); chroma on the 8x8 chroma grid at bS 2.  The reference
+ * encoder's PPS disables this filter (src/encoder_h265.c:330), so the
+ * pipeline leaves it off by default. */
+int bc250_hevc_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                              const uint8_t *cbf, const uint32_t *mvs, int num_slices, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

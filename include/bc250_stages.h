@@ -78,6 +78,16 @@ int bc250_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int h
 int bc250_h264_pred4x4_batch(const uint8_t *nbr, const uint8_t *ctl, int n, uint8_t *out, void *stream);
 int bc250_h264_pred8x8_batch(const uint8_t *nbr, const uint8_t *ctl, int n, uint8_t *out, void *stream);
 
+/* 8x8 transform (High profile), batched over n blocks of 64 int16:
+ * residual -> levels (forward Y = C X C^T and a quantiser that is the exact
+ * inverse of the decoder's scaling, rounding round_q12/12 of a step: 4 for
+ * intra, 2 for inter as the 4x4 path's f) and levels -> residual (8.5.13.1
+ * flat-scaling dequant + 8.5.13.2 inverse, (x+32)>>6), the decoder's
+ * scale8 + h264d_idct8_add (h264_recon.c:87-93,192-210).  The reference
+ * encoder codes only 4x4 transforms; these serve transform_size_8x8 MBs. */
+int bc250_h264_tq8x8_batch(const int16_t *res, int n, int qp, int round_q12, int16_t *out, void *stream);
+int bc250_h264_dqit8x8_batch(const int16_t *lv, int n, int qp, int16_t *out, void *stream);
+
 /* Zero `n` ints with a fill kernel (no memset node, so capturable). */
 int bc250_launch_zero_i32(int *p, int n, void *stream);
 

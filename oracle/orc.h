@@ -95,6 +95,16 @@ void orc_h264_pred4x4(const uint8_t top8[8], const uint8_t left4[4], int corner,
 void orc_h264_pred8x8(const uint8_t top16[16], const uint8_t left8[8], int corner, int mode, int avail_top,
                       int avail_left, int avail_corner, int avail_top_right, uint8_t out[64]);
 
+/* --- 8x8 transform (H.264 8.5.13; forward + quantiser inverting it) ------- */
+int  orc_h264_class8(int i, int j);
+int  orc_h264_v8(int qp_rem, int cls);
+void orc_h264_fwd8x8(const int32_t in[64], int64_t out[64]);
+void orc_h264_quant8x8(const int64_t y[64], int qp, int round_q12, int16_t out[64]);
+void orc_h264_dequant8x8(const int16_t c[64], int qp, int32_t d[64]);
+void orc_h264_inv8x8(const int32_t d[64], int32_t out[64]);
+void orc_h264_tq8x8(const int16_t res[64], int qp, int round_q12, int16_t out[64]);
+void orc_h264_dqit8x8(const int16_t lv[64], int qp, int16_t out[64]);
+
 /* --- whole frame (stage order of the reference's dispatch) ----------------- */
 void orc_h264_encode_frame(const uint8_t *cy, const uint8_t *cuv,
                            const uint8_t *ry, const uint8_t *ruv, int py,
@@ -128,6 +138,11 @@ void orc_hevc_encode_p(const void *sy, const void *scb, const void *scr, const v
 void orc_hevc_dct_matrix(int N, int *M);
 void orc_hevc_tqn(const int16_t *res, int log2n, int qp, int bd, int round_q12, int16_t *out);
 void orc_hevc_dqitn(const int16_t *coef, int log2n, int qp, int bd, int16_t *out);
+int  orc_hevc_beta(int q);
+int  orc_hevc_tc(int q);
+int  orc_hevc_bs(int is_intra, const uint8_t *cbf, const int32_t *mv4, int cp, int cq);
+void orc_hevc_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                      const uint8_t *cbf, const int32_t *mv4, int num_slices);
 uint32_t orc_hevc_refine8(const void *cur, const void *ref, int bd, int wc, int hc, int x0, int y0,
                           int *mvx, int *mvy);
 

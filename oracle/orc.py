@@ -353,3 +353,36 @@ def h264_pred8x8(top16, left8, corner, mode, at, al, ac, atr):
 
 # modes and the neighbours each needs (a decoder never uses a mode without them)
 H264_I4_NEEDS = {0: "t", 1: "l", 2: "", 3: "t", 4: "tlc", 5: "tlc", 6: "tlc", 7: "t", 8: "l"}
+
+_sig("orc_h264_class8", _I, [_I, _I])
+_sig("orc_h264_v8", _I, [_I, _I])
+_sig("orc_h264_tq8x8", None, [_P, _I, _I, _P])
+_sig("orc_h264_dqit8x8", None, [_P, _I, _P])
+_sig("orc_h264_dequant8x8", None, [_P, _I, _P])
+
+
+def h264_tq8x8(res, qp, round_q12):
+    r, o = np.ascontiguousarray(res, np.int16).reshape(64), np.zeros(64, np.int16)
+    lib.orc_h264_tq8x8(p(r), qp, round_q12, p(o))
+    return o
+
+
+def h264_dqit8x8(lv, qp):
+    c, o = np.ascontiguousarray(lv, np.int16).reshape(64), np.zeros(64, np.int16)
+    lib.orc_h264_dqit8x8(p(c), qp, p(o))
+    return o
+
+_sig("orc_hevc_beta", _I, [_I])
+_sig("orc_hevc_tc", _I, [_I])
+_sig("orc_hevc_deblock", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I])
+
+
+def hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices=1):
+    """Returns deblocked copies of the NV12 planes; cbf uint8 [ncu], mv4 int32 [ncu][4]."""
+    y = np.ascontiguousarray(y, np.uint8).copy()
+    uv = np.ascontiguousarray(uv, np.uint8).copy()
+    hc, wc = y.shape
+    cbf = np.ascontiguousarray(cbf, np.uint8)
+    mv4 = np.ascontiguousarray(mv4, np.int32).reshape(-1, 4)
+    lib.orc_hevc_deblock(p(y), p(uv), wc, wc, wc, hc, qp, int(is_intra), p(cbf), p(mv4), num_slices)
+    return y, uv

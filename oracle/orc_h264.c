@@ -993,3 +993,125 @@ void orc_h264_pred8x8(const uint8_t top16[16], const uint8_t left8[8], int corne
     for (int y = 0; y < 8; y++)
         for (int x = 0; x < 8; x++) out[y * 8 + x] = (uint8_t)dir_sample(8, mode, x, y, ft, fl, fc, dc);
 }
+
+/* ------------------------------------------------------------------------
+ * 8x8 transform (High profile).  The reference encoder codes only 4x4
+ * blocks; this is the decoder's side of 8.5.13 (normative) with an encoder
+ * forward transform and quantiser chosen to invert it:
+ *   forward   Y = C X C^T, C the 8x8 integer basis below (exact, no rounding)
+ *   quant     level = floor((|Y| * 2^14 + off) / step), step(i,j) =
+ *             n_i n_j v8[qp%6][class] 2^(qp/6) with n = diag(C C^T) and off
+ *             = round_q12/12 of a step: the algebraic inverse of the scaling
+ *             below followed by the inverse transform
+ *   dequant   8.5.13.1 with a flat weight scale (16): d = (c * 16 * v8 <<
+ *             qp/6) >> 6, rounding below qp 36
+ *   inverse   8.5.13.2 butterflies, rows then columns, (x + 32) >> 6
+ * Reference decoder: enc/src/h264_recon.c:87-93 (scale8), :150-210.
+ * ---------------------------------------------------------------------- */
+static const int kC8[8][8] = {
+    {8, 8, 8, 8, 8, 8, 8, 8},         {12, 10, 6, 3, -3, -6, -10, -12}, {8, 4, -4, -8, -8, -4, 4, 8},
+    {10, -3, -12, -6, 6, 12, 3, -10}, {8, -8, -8, 8, 8, -8, -8, 8},     {6, -12, 3, 10, -10, -3, 12, -6},
+    {4, -8, 8, -4, -4, 8, -8, 4},     {3, -6, 10, -12, 12, -10, 6, -3}};
+static const int kN8[8] = {512, 578, 320, 578, 512, 578, 320, 578};
+/* normAdjust8x8 (8.5.9) v[qp%6][class] */
+static const int kV8[6][6] = {{20, 18, 32, 19, 25, 24}, {22, 19, 35, 21, 28, 26}, {26, 23, 42, 24, 33, 31},
+                              {28, 25, 45, 26, 35, 33}, {32, 28, 51, 30, 40, 38}, {36, 32, 58, 34, 46, 43}};
+
+/* 8.5.9 position class of raster position (i row, j column) of an 8x8 block */
+int orc_h264_class8(int i, int j)
+{
+    if (i % 4 == 0 && j % 4 == 0) return 0;
+    if (i % 2 == 1 && j % 2 == 1) return 1;
+    if (i % 4 == 2 && j % 4 == 2) return 2;
+    if ((i % 4 == 0 && j % 2 == 1) || (i % 2 == 1 && j % 4 == 0)) return 3;
+    if ((i % 4 == 0 && j % 4 == 2) || (i % 4 == 2 && j % 4 == 0)) return 4;
+    return 5;
+}
+
+int orc_h264_v8(int qp_rem, int cls) { return kV8[qp_rem][cls]; }
+
+void orc_h264_fwd8x8(const int32_t in[64], int64_t out[64])
+{
+    int64_t t[64];
+    for (int i = 0; i < 8; i++)
+        for (int c = 0; c < 8; c++) {
+            int64_t s = 0;
+            for (int r = 0; r < 8; r++) s += (int64_t)kC8[i][r] * in[r * 8 + c];
+            t[i * 8 + c] = s;
+        }
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) {
+            int64_t s = 0;
+            for (int c = 0; c < 8; c++) s += (int64_t)kC8[j][c] * t[i * 8 + c];
+            out[i * 8 + j] = s;
+        }
+}
+
+void orc_h264_quant8x8(const int64_t y[64], int qp, int round_q12, int16_t out[64])
+{
+    for (int p = 0; p < 64; p++) {
+        const int i = p >> 3, j = p & 7;
+        const uint64_t step = (uint64_t)kN8[i] * kN8[j] * kV8[qp % 6][orc_h264_class8(i, j)] << (qp / 6);
+        const uint64_t off = step * (uint64_t)round_q12 / 12;
+        const uint64_t mag = (uint64_t)(y[p] < 0 ? -y[p] : y[p]);
+        uint64_t l = ((mag << 14) + off) / step;
+        if (l > 32767) l = 32767;
+        out[p] = (int16_t)(y[p] < 0 ? -(int64_t)l : (int64_t)l);
+    }
+}
+
+void orc_h264_dequant8x8(const int16_t c[64], int qp, int32_t d[64])
+{
+    const int per = qp / 6;
+    for (int p = 0; p < 64; p++) {
+        const int32_t ls = 16 * kV8[qp % 6][orc_h264_class8(p >> 3, p & 7)];
+        d[p] = per >= 6 ? (c[p] * ls) * (1 << (per - 6)) : (c[p] * ls + (1 << (5 - per))) >> (6 - per);
+    }
+}
+
+/* 8.5.13.2 one-dimensional inverse over v[0..7] at stride s, in place */
+static void inv8_1d(int32_t *v, int s)
+{
+    const int32_t d0 = v[0], d1 = v[s], d2 = v[2 * s], d3 = v[3 * s], d4 = v[4 * s], d5 = v[5 * s],
+                  d6 = v[6 * s], d7 = v[7 * s];
+    const int32_t e0 = d0 + d4, e1 = -d3 + d5 - d7 - (d7 >> 1), e2 = d0 - d4, e3 = d1 + d7 - d3 - (d3 >> 1);
+    const int32_t e4 = (d2 >> 1) - d6, e5 = -d1 + d7 + d5 + (d5 >> 1), e6 = d2 + (d6 >> 1),
+                  e7 = d3 + d5 + d1 + (d1 >> 1);
+    const int32_t f0 = e0 + e6, f1 = e1 + (e7 >> 2), f2 = e2 + e4, f3 = e3 + (e5 >> 2);
+    const int32_t f4 = e2 - e4, f5 = (e3 >> 2) - e5, f6 = e0 - e6, f7 = e7 - (e1 >> 2);
+    v[0] = f0 + f7;
+    v[s] = f2 + f5;
+    v[2 * s] = f4 + f3;
+    v[3 * s] = f6 + f1;
+    v[4 * s] = f6 - f1;
+    v[5 * s] = f4 - f3;
+    v[6 * s] = f2 - f5;
+    v[7 * s] = f0 - f7;
+}
+
+void orc_h264_inv8x8(const int32_t d[64], int32_t out[64])
+{
+    int32_t t[64];
+    for (int p = 0; p < 64; p++) t[p] = d[p];
+    for (int i = 0; i < 8; i++) inv8_1d(t + i * 8, 1);
+    for (int i = 0; i < 8; i++) inv8_1d(t + i, 8);
+    for (int p = 0; p < 64; p++) out[p] = (t[p] + 32) >> 6;
+}
+
+/* residual -> levels, and levels -> reconstructed residual */
+void orc_h264_tq8x8(const int16_t res[64], int qp, int round_q12, int16_t out[64])
+{
+    int32_t x[64];
+    int64_t y[64];
+    for (int p = 0; p < 64; p++) x[p] = res[p];
+    orc_h264_fwd8x8(x, y);
+    orc_h264_quant8x8(y, qp, round_q12, out);
+}
+
+void orc_h264_dqit8x8(const int16_t lv[64], int qp, int16_t out[64])
+{
+    int32_t d[64], r[64];
+    orc_h264_dequant8x8(lv, qp, d);
+    orc_h264_inv8x8(d, r);
+    for (int p = 0; p < 64; p++) out[p] = (int16_t)r[p];
+}

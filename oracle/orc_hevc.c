@@ -533,3 +533,125 @@ uint32_t orc_hevc_refine8(const void *cur, const void *ref, int bd, int wc, int 
     *mvy = by;
     return best;
 }
+
+/* ------------------------------------------------------------------------
+ * Deblocking (H.265 8.7.2) for the fixed coding shape of this pipeline: 8x8
+ * CUs, each one PU and (P) one 8x8 TU or (I) 4x4 TUs, one reference picture,
+ * uni-prediction.  Edges on the 8x8 luma grid; all vertical edges of the
+ * picture first, then the horizontal ones on that result; picture borders
+ * and slice borders (CTU-row slices, loop filter not across slices) are not
+ * filtered.  bS: 2 intra, 1 if either CU has coded luma or the vectors
+ * differ by >= 4 quarter samples, else 0 (the reference decoder's
+ * strength(), hevc_filter_template.c:221-290).  Chroma edges on the 8x8
+ * chroma grid, bS 2 only.  Tables: Table 8-12 (hevcd_beta / hevcd_tc).
+ * ---------------------------------------------------------------------- */
+static const int kBeta[52] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  6,  7,
+                              8,  9,  10, 11, 12, 13, 14, 15, 16, 17, 18, 20, 22, 24, 26, 28, 30, 32,
+                              34, 36, 38, 40, 42, 44, 46, 48, 50, 52, 54, 56, 58, 60, 62, 64};
+static const int kTc[54] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,  0,  1,  1,  1,  1,  1,  1,  1,  1, 1,
+                            2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 5, 5, 6, 6, 7, 8, 9, 10, 11, 13, 14, 16, 18, 20, 22, 24};
+
+int orc_hevc_beta(int q) { return kBeta[clampi(q, 0, 51)]; }
+int orc_hevc_tc(int q) { return kTc[clampi(q, 0, 53)]; }
+
+/* Luma: 4 lines across one edge; px(k, i) addresses sample i on side p
+ * (i = 0 nearest the edge) of line k, qx(k, i) the q side. */
+static void luma_segment(uint8_t *base, int along, int across, int beta, int tc)
+{
+#define PX(k, i) base[(k) * along - ((i) + 1) * across]
+#define QX(k, i) base[(k) * along + (i) * across]
+    const int dp0 = abs(PX(0, 2) - 2 * PX(0, 1) + PX(0, 0)), dp3 = abs(PX(3, 2) - 2 * PX(3, 1) + PX(3, 0));
+    const int dq0 = abs(QX(0, 2) - 2 * QX(0, 1) + QX(0, 0)), dq3 = abs(QX(3, 2) - 2 * QX(3, 1) + QX(3, 0));
+    const int d = dp0 + dq0 + dp3 + dq3;
+    if (d >= beta) return;
+    int strong = 1;
+    for (int k = 0; k < 4; k += 3) {
+        const int dd = k ? dp3 + dq3 : dp0 + dq0;
+        if (!(2 * dd < (beta >> 2) && abs(PX(k, 3) - PX(k, 0)) + abs(QX(k, 0) - QX(k, 3)) < (beta >> 3) &&
+              abs(PX(k, 0) - QX(k, 0)) < ((5 * tc + 1) >> 1)))
+            strong = 0;
+    }
+    const int dEp = (dp0 + dp3) < ((beta + (beta >> 1)) >> 3), dEq = (dq0 + dq3) < ((beta + (beta >> 1)) >> 3);
+    for (int k = 0; k < 4; k++) {
+        const int p0 = PX(k, 0), p1 = PX(k, 1), p2 = PX(k, 2), p3 = PX(k, 3);
+        const int q0 = QX(k, 0), q1 = QX(k, 1), q2 = QX(k, 2), q3 = QX(k, 3);
+        if (strong) {
+            PX(k, 0) = (uint8_t)clampi((p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3, p0 - 2 * tc, p0 + 2 * tc);
+            PX(k, 1) = (uint8_t)clampi((p2 + p1 + p0 + q0 + 2) >> 2, p1 - 2 * tc, p1 + 2 * tc);
+            PX(k, 2) = (uint8_t)clampi((2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3, p2 - 2 * tc, p2 + 2 * tc);
+            QX(k, 0) = (uint8_t)clampi((p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3, q0 - 2 * tc, q0 + 2 * tc);
+            QX(k, 1) = (uint8_t)clampi((p0 + q0 + q1 + q2 + 2) >> 2, q1 - 2 * tc, q1 + 2 * tc);
+            QX(k, 2) = (uint8_t)clampi((p0 + q0 + q1 + 3 * q2 + 2 * q3 + 4) >> 3, q2 - 2 * tc, q2 + 2 * tc);
+        } else {
+            int delta = (9 * (q0 - p0) - 3 * (q1 - p1) + 8) >> 4;
+            if (abs(delta) >= tc * 10) continue;
+            delta = clampi(delta, -tc, tc);
+            PX(k, 0) = (uint8_t)clampi(p0 + delta, 0, 255);
+            QX(k, 0) = (uint8_t)clampi(q0 - delta, 0, 255);
+            if (dEp) PX(k, 1) = (uint8_t)clampi(p1 + clampi((((p2 + p0 + 1) >> 1) - p1 + delta) >> 1, -(tc >> 1), tc >> 1), 0, 255);
+            if (dEq) QX(k, 1) = (uint8_t)clampi(q1 + clampi((((q2 + q0 + 1) >> 1) - q1 - delta) >> 1, -(tc >> 1), tc >> 1), 0, 255);
+        }
+    }
+#undef PX
+#undef QX
+}
+
+static void chroma_segment(uint8_t *base, int along, int across, int tc)
+{
+    for (int k = 0; k < 4; k++) {
+        uint8_t *q0 = base + k * along, *p0 = q0 - across;
+        const int P1 = p0[-across], P0 = *p0, Q0 = *q0, Q1 = q0[across];
+        const int delta = clampi((((Q0 - P0) * 4) + P1 - Q1 + 4) >> 3, -tc, tc);
+        *p0 = (uint8_t)clampi(P0 + delta, 0, 255);
+        *q0 = (uint8_t)clampi(Q0 - delta, 0, 255);
+    }
+}
+
+/* cu index of luma sample (x, y): CTU raster, CU z-order */
+static int cu_at(int x, int y, int wctu) { return ((y >> 4) * wctu + (x >> 4)) * 4 + ((y >> 3) & 1) * 2 + ((x >> 3) & 1); }
+
+int orc_hevc_bs(int is_intra, const uint8_t *cbf, const int32_t *mv4, int cp, int cq)
+{
+    if (is_intra) return 2;
+    if (cbf[cp] || cbf[cq]) return 1;
+    const int32_t *a = mv4 + 4 * cp, *b = mv4 + 4 * cq;
+    return (abs(a[0] - b[0]) >= 4 || abs(a[1] - b[1]) >= 4) ? 1 : 0;
+}
+
+/* y: luma (pitch py), uv: interleaved Cb/Cr (pitch puv); mv4: {mvx, mvy, ., .}
+ * per CU; cbf: per CU. */
+void orc_hevc_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                      const uint8_t *cbf, const int32_t *mv4, int num_slices)
+{
+    const int wctu = wc / 16, hctu = hc / 16;
+    const int beta = kBeta[clampi(qp, 0, 51)];
+    const int qpc = orc_hevc_chroma_qp(qp);
+    for (int dir = 0; dir < 2; dir++) {       /* 0: vertical edges, 1: horizontal edges */
+        for (int e = 8; e < (dir ? hc : wc); e += 8) {
+            if (dir && (e & 15) == 0 && orc_hevc_slice_y_min(e >> 4, hctu, num_slices) == e) continue;
+            for (int s = 0; s < (dir ? wc : hc); s += 4) {   /* 4-sample segments along the edge */
+                const int x = dir ? s : e, yy = dir ? e : s;
+                const int cq = cu_at(x, yy, wctu), cp = cu_at(dir ? x : x - 1, dir ? yy - 1 : yy, wctu);
+                const int bs = orc_hevc_bs(is_intra, cbf, mv4, cp, cq);
+                if (!bs) continue;
+                const int tc = kTc[clampi(qp + 2 * (bs - 1), 0, 53)];
+                uint8_t *b = y + (size_t)yy * py + x;
+                luma_segment(b, dir ? 1 : py, dir ? py : 1, beta, tc);
+            }
+            /* chroma: edges on the 8x8 chroma grid (every 16 luma), bS 2 only */
+            if ((e & 15) == 0) {
+                const int ctc = kTc[clampi(qpc + 2, 0, 53)];
+                for (int s = 0; s < (dir ? wc : hc) / 2; s += 4) {
+                    const int cx = dir ? s : e / 2, cy = dir ? e / 2 : s;
+                    const int lx = 2 * cx, ly = 2 * cy;
+                    const int cq = cu_at(lx, ly, wctu), cp = cu_at(dir ? lx : lx - 1, dir ? ly - 1 : ly, wctu);
+                    if (orc_hevc_bs(is_intra, cbf, mv4, cp, cq) != 2) continue;
+                    for (int comp = 0; comp < 2; comp++) {
+                        uint8_t *b = uv + (size_t)cy * puv + 2 * cx + comp;
+                        chroma_segment(b, dir ? 2 : puv, dir ? puv : 2, ctc);
+                    }
+                }
+            }
+        }
+    }
+}

@@ -431,3 +431,25 @@ def test_intra_nxn_pred_batch(pkg, size):
         else:
             want = orc.h264_pred8x8(nbr[i, 9:25], nbr[i, 0:8], nbr[i, 8], m, at, al, ac, atr)
         assert np.array_equal(got[i], want), (i, m, at, al, ac, atr)
+
+
+@pytest.mark.parametrize("qp", [0, 12, 27, 36, 51])
+def test_t8x8_batch(pkg, qp):
+    torch = _torch()
+    say(f"8x8 transform batch qp={qp}")
+    rs = np.random.default_rng(qp + 3)
+    n = 257
+    res = rs.integers(-255, 256, (n, 64)).astype(np.int16)
+    res[::2] //= 13
+    for rnd in (2, 4):
+        dres, dlv = dev(torch, res), torch.zeros((n, 64), dtype=torch.int16, device="cuda")
+        P = lambda t: C.c_void_p(t.data_ptr())
+        assert pkg.hip.bc250_h264_tq8x8_batch(P(dres), n, qp, rnd, P(dlv), None) == 0
+        dback = torch.zeros_like(dlv)
+        assert pkg.hip.bc250_h264_dqit8x8_batch(P(dlv), n, qp, P(dback), None) == 0
+        torch.cuda.synchronize()
+        lv, back = host(dlv), host(dback)
+        for b in range(0, n, 3):
+            want = orc.h264_tq8x8(res[b], qp, rnd)
+            assert np.array_equal(lv[b], want), (b, rnd)
+            assert np.array_equal(back[b], orc.h264_dqit8x8(want, qp)), b

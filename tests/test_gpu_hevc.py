@@ -265,3 +265,44 @@ def test_tqn_dqitn_batch(pkg, log2n, bd):
             want = orc.hevc_tqn(res[b], log2n, qp, bd, 2)
             assert np.array_equal(lv[b], want), (b, qp)
             assert np.array_equal(back[b], orc.hevc_dqitn(want, log2n, qp, bd)), (b, qp)
+
+
+def _blocky(rs, h, w, step):
+    """Smooth ramps with per-8x8 offsets (block edges the filter acts on) and
+    light noise; plane values stay inside 0..255."""
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = 100 + (xx * 40) // max(w, 1) + (yy * 30) // max(h, 1)
+    off = rs.integers(-step, step + 1, (h // 8 + 1, w // 8 + 1))
+    v = base + off[yy // 8, xx // 8] + rs.integers(-1, 2, (h, w))
+    return np.clip(v, 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("w,h,qp,intra,ns", [(64, 48, 37, 1, 1), (64, 48, 30, 0, 1), (192, 112, 45, 1, 2),
+                                             (192, 112, 27, 0, 3), (320, 240, 51, 0, 4), (320, 240, 22, 1, 1)])
+def test_deblock(pkg, w, h, qp, intra, ns):
+    torch = _torch()
+    say(f"hevc deblock {w}x{h} qp={qp} intra={intra} slices={ns}")
+    rs = np.random.default_rng(w * 7 + qp)
+    y = _blocky(rs, h, w, 12)
+    uv = _blocky(rs, h // 2, w, 10)
+    ncu = (w // 8) * (h // 8)
+    cbf = (rs.random(ncu) < 0.3).astype(np.uint8)
+    mv4 = np.zeros((ncu, 4), np.int32)
+    mv4[:, :2] = rs.integers(-6, 7, (ncu, 2))
+    mv4[::3, :2] = 0
+    wy, wuv = orc.hevc_deblock(y, uv, qp, intra, cbf, mv4, ns)
+    assert not np.array_equal(wy, y)
+    pitch = w + 64          # a pitch wider than the picture; the margin must stay untouched
+    gy = np.full((h, pitch), 7, np.uint8)
+    guv = np.full((h // 2, pitch), 9, np.uint8)
+    gy[:, :w], guv[:, :w] = y, uv
+    d = [dev(torch, a) for a in (gy, guv, cbf, mv4)]
+    assert pkg.hip.bc250_hevc_launch_deblock(P(d[0]), P(d[1]), pitch, pitch, w, h, qp, intra, P(d[2]), P(d[3]), ns,
+                                             None) == 0
+    torch.cuda.synchronize()
+    oy, ouv = host(d[0]), host(d[1])
+    assert np.array_equal(oy[:, :w], wy)
+    assert np.array_equal(ouv[:, :w], wuv)
+    assert (oy[:, w:] == 7).all() and (ouv[:, w:] == 9).all()
+    assert pkg.hip.bc250_hevc_launch_deblock(P(d[0]), P(d[1]), pitch, pitch, w + 8, h, qp, intra, P(d[2]), P(d[3]),
+                                             ns, None) == -1

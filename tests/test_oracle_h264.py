@@ -530,3 +530,61 @@ def test_intra_flat_neighbours_predict_flat():
         for mode in range(9):
             assert (orc.h264_pred4x4(np.full(8, v), np.full(4, v), v, mode, 1, 1) == v).all()
             assert (orc.h264_pred8x8(np.full(16, v), np.full(8, v), v, mode, 1, 1, 1, 1) == v).all()
+
+
+# ------------------------------------------------------------ 8x8 transform
+
+@need_ref
+def test_8x8_tables_match_reference():
+    L = ref()
+    scan = np.frombuffer((C.c_uint8 * 16).in_dll(L, "h264d_dequant8_init_scan"), np.uint8)
+    v8 = np.frombuffer((C.c_uint8 * 36).in_dll(L, "h264d_dequant8_init"), np.uint8).reshape(6, 6)
+    for i in range(8):
+        for j in range(8):
+            assert orc.lib.orc_h264_class8(i, j) == scan[(i % 4) * 4 + j % 4]
+    for m in range(6):
+        for c in range(6):
+            assert orc.lib.orc_h264_v8(m, c) == v8[m, c]
+
+
+class _Dq(C.Structure):
+    _fields_ = [("d4", C.c_int32 * 96), ("d8", C.c_int32 * 384), ("qp", C.c_int), ("valid", C.c_int)]
+
+
+@need_ref
+@pytest.mark.parametrize("qp", [0, 10, 18, 27, 35, 36, 41, 51])
+def test_8x8_decode_equals_recon(qp):
+    """The oracle's 8x8 levels decoded by the reference decoder (flat scaling
+    lists, h264d_dequant_build + h264d_idct8_add) rebuild exactly the
+    oracle's reconstruction."""
+    L = ref()
+    flat4 = (C.c_uint8 * 96)(*([16] * 96))
+    flat8 = (C.c_uint8 * 384)(*([16] * 384))
+    dq = _Dq()
+    L.h264d_dequant_build(C.byref(dq), qp, flat4, flat8)
+    L.h264d_idct8_add.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+    rs = np.random.default_rng(qp + 8)
+    for _ in range(40):
+        pred = rs.integers(40, 216, 64).astype(np.uint8)
+        res = rs.integers(-40, 41, 64).astype(np.int16)
+        lv = orc.h264_tq8x8(res, qp, 4)
+        rec = np.clip(pred.astype(int) + orc.h264_dqit8x8(lv, qp), 0, 255).astype(np.uint8)
+        dst = pred.copy()
+        blk = lv.copy()
+        L.h264d_idct8_add(p(dst), 8, p(blk), C.addressof(dq.d8), qp)
+        assert np.array_equal(dst, rec), qp
+
+
+def test_8x8_quantiser_inverts_the_decoder():
+    """With rounding to nearest at QP 0 the forward + quantiser + decoder
+    chain returns the residual within a quantisation step (the scale of the
+    quantiser matches the decoder's), and a DC-only block comes back exactly."""
+    rs = np.random.default_rng(5)
+    for _ in range(30):
+        res = rs.integers(-255, 256, 64).astype(np.int16)
+        back = orc.h264_dqit8x8(orc.h264_tq8x8(res, 0, 6), 0)
+        assert np.abs(back.astype(int) - res).max() <= 2
+    for v in (-100, -7, 3, 50, 255):
+        lv = orc.h264_tq8x8(np.full(64, v, np.int16), 0, 6)
+        assert np.count_nonzero(lv[1:]) == 0
+        assert np.array_equal(orc.h264_dqit8x8(lv, 0), np.full(64, v, np.int16))

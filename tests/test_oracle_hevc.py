@@ -454,3 +454,103 @@ def test_forward_nxn_round_trips(log2n):
         if log2n == 2:
             r4 = np.clip(res, -255, 255)
             assert np.array_equal(orc.hevc_tqn(r4, 2, 27, 8, 4), orc.hevc_tq4(r4, 27, 0, 4, 8))
+
+
+# ------------------------------------------------------------ deblocking
+
+@need_ref
+def test_deblock_tables_match_reference():
+    beta = np.frombuffer((C.c_uint8 * 52).in_dll(ref(), "hevcd_beta"), np.uint8)
+    tc = np.frombuffer((C.c_uint8 * 54).in_dll(ref(), "hevcd_tc"), np.uint8)
+    assert [orc.lib.orc_hevc_beta(q) for q in range(52)] == beta.tolist()
+    assert [orc.lib.orc_hevc_tc(q) for q in range(54)] == tc.tolist()
+
+
+def _two_columns(left, right, w=32, h=16):
+    y = np.full((h, w), right, np.uint8)
+    y[:, :8] = left
+    uv = np.full((h // 2, w), 128, np.uint8)
+    return y, uv
+
+
+def test_deblock_line_kats():
+    """Hand-worked 8.7.2.5 cases on the vertical edge x = 8 of a 32x16
+    picture whose other edges are flat (so nothing else moves)."""
+    ncu = 2 * 4
+    mv = np.zeros((ncu, 4), np.int32)
+    # normal filter, bS 1 (coded luma), QP 37: beta 36, tc 4; p = 60 flat, q = 70 flat:
+    # delta = (9*10 - 3*10 + 8) >> 4 = 4; p1/q1 move by (+-4 >> 1) = +-2 (|| <= tc/2)
+    y, uv = _two_columns(60, 70)
+    cbf = np.ones(ncu, np.uint8)
+    oy, _ = orc.hevc_deblock(y, uv, 37, 0, cbf, mv)
+    assert oy[:, 4:12].tolist() == [[60, 60, 62, 64, 66, 68, 70, 70]] * 16
+    # strong filter, bS 2 (intra), QP 40: beta 42, tc 7; p = 50, q = 56:
+    # p0' = (50+100+100+112+56+4)>>3 = 52, p1' = 208>>2 = 52, p2' = 410>>3 = 51,
+    # q0' = 434>>3 = 54, q1' = 220>>2 = 55, q2' = 446>>3 = 55
+    y, uv = _two_columns(50, 56)
+    oy, ouv = orc.hevc_deblock(y, uv, 40, 1, np.zeros(ncu, np.uint8), mv)
+    assert oy[:, 4:12].tolist() == [[50, 51, 52, 52, 54, 55, 55, 56]] * 16
+    # chroma, bS 2, QP 30: QpC 29, tc = tc'[31] = 3; p = 40, q = 60 on the chroma
+    # edge at chroma x = 8 (luma 16): delta = clip(+-3, (80 - 20 + 4) >> 3) = 3
+    y = np.full((16, 32), 90, np.uint8)
+    uv = np.full((8, 32), 60, np.uint8)
+    uv[:, :16] = 40                     # chroma columns 0..7 (both components) = 40
+    _, ouv = orc.hevc_deblock(y, uv, 30, 1, np.zeros(ncu, np.uint8), mv)
+    assert ouv[:, 12:20].tolist() == [[40, 40, 43, 43, 57, 57, 60, 60]] * 8
+
+
+def test_deblock_invariants():
+    rs = np.random.default_rng(3)
+    h, w = 48, 64
+    ncu = (w // 8) * (h // 8)
+    flat = np.full((h, w), 99, np.uint8), np.full((h // 2, w), 120, np.uint8)
+    oy, ouv = orc.hevc_deblock(*flat, 30, 1, np.ones(ncu, np.uint8), np.zeros((ncu, 4), np.int32))
+    assert (oy == 99).all() and (ouv == 120).all()
+    # blocky but smooth inside the blocks: a ramp plus a random offset per 8x8
+    # (luma) / 4x4 (chroma) block, so the edges carry steps the filter acts on
+    yy, xx = np.mgrid[0:h, 0:w]
+    y = (100 + xx // 2 + yy // 3 + np.kron(rs.integers(-9, 10, (h // 8, w // 8)), np.ones((8, 8), int)))
+    y = y.astype(np.uint8)
+    uv = (128 + np.kron(rs.integers(-6, 7, (h // 8, w // 8)), np.ones((4, 8), int))).astype(np.uint8)
+    oy, ouv = orc.hevc_deblock(y, uv, 30, 0, np.zeros(ncu, np.uint8), np.zeros((ncu, 4), np.int32))
+    assert np.array_equal(oy, y) and np.array_equal(ouv, uv)  # bS 0 everywhere
+    # Symmetry.  The strong luma filter is symmetric in p and q, so a
+    # mirrored picture gives the mirrored result.  The normal luma and the
+    # chroma filters round with "+c >> s", which is not antisymmetric; they
+    # are symmetric under mirror plus sample negation (v -> 255 - v).  Each
+    # picture below forces one kind of filter on every edge.
+    def mirror_lr(a, b):
+        bb = b.reshape(b.shape[0], -1, 2)[:, ::-1, :].reshape(b.shape)
+        return np.ascontiguousarray(a[:, ::-1]), np.ascontiguousarray(bb)
+
+    def check(y, uv, qp, intra, cbf, negate):
+        mv = np.zeros((ncu, 4), np.int32)
+        oy, ouv = orc.hevc_deblock(y, uv, qp, intra, cbf, mv)
+        assert not np.array_equal(oy, y)
+        f = (lambda a: 255 - a) if negate else (lambda a: a)
+        for flip in ("lr", "ud"):
+            if flip == "lr":
+                my, muv = mirror_lr(f(y), f(uv))
+                ey, euv = mirror_lr(f(oy), f(ouv))
+            else:
+                my, muv = np.ascontiguousarray(f(y)[::-1]), np.ascontiguousarray(f(uv)[::-1])
+                ey, euv = f(oy)[::-1], f(ouv)[::-1]
+            gy, guv = orc.hevc_deblock(my, muv, qp, intra, cbf, mv)
+            assert np.array_equal(gy, ey), flip
+            if negate:
+                assert np.array_equal(guv, euv), flip
+        return oy, ouv
+
+    # strong only: gentle ramp, steps of at most 4 between blocks, intra QP 40
+    small = np.kron(rs.integers(-2, 3, (h // 8, w // 8)), np.ones((8, 8), int))
+    ys = (100 + xx // 4 + yy // 4 + small).astype(np.uint8)
+    check(ys, uv, 40, 1, np.zeros(ncu, np.uint8), negate=False)
+    # normal only: checkerboard of +-15 per block (every step 30 >= the strong
+    # limit (5*tc+1)/2 = 10 at tc 4), coded luma (bS 1), QP 37; chroma edges
+    # need bS 2, so the chroma check is an intra picture with the same steps
+    board = 15 * (2 * ((np.arange(h // 8)[:, None] + np.arange(w // 8)[None, :]) % 2) - 1)
+    yn = (120 + xx // 4 + np.kron(board, np.ones((8, 8), int))).astype(np.uint8)
+    check(yn, uv, 37, 0, np.ones(ncu, np.uint8), negate=True)
+    cb = (128 + np.kron(board // 3, np.ones((4, 8), int))).astype(np.uint8)
+    _, ouv = check(yn, cb, 37, 1, np.zeros(ncu, np.uint8), negate=True)
+    assert not np.array_equal(ouv, cb)
