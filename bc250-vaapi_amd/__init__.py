@@ -141,6 +141,7 @@ _sig(hip, "bc250_hevc_launch_i_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _I,
 _sig(hip, "bc250_hevc_launch_p_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P,
                                             _P, _P])
 _sig(hip, "bc250_hevc_launch_deblock", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P])
+_sig(hip, "bc250_hevc_cu_cbf", _I, [_P, _I, _P, _P])
 
 _sig(host, "bc250_gen_create", _P, [C.c_uint64, _I, _I, _I])
 _sig(host, "bc250_gen_destroy", None, [_P])

@@ -207,6 +207,26 @@ __global__ __launch_bounds__(256) void hevc_deblock_kernel(uint8_t *__restrict__
     }
 }
 
+// cbf[cu] = any of the CU's 64 luma levels (P layout: CTU t at coef[384 t],
+// CU z at +64 z) is non-zero; one 64-lane wave per CU, one level per lane.
+__global__ __launch_bounds__(256) void hevc_cu_cbf_kernel(const int16_t *__restrict__ coef, int ncu,
+                                                          uint8_t *__restrict__ cbf)
+{
+    const int cu = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (cu >= ncu) return;
+    const int v = coef[(size_t)(cu >> 2) * 384 + (cu & 3) * 64 + lane];
+    const bool any = __any(v != 0);
+    if (lane == 0) cbf[cu] = any ? 1 : 0;
+}
+
+extern "C" int bc250_hevc_cu_cbf(const int16_t *coef, int nctu, uint8_t *cbf, void *stream)
+{
+    if (nctu < 0) return -1;
+    if (nctu == 0) return 0;
+    hevc_cu_cbf_kernel<<<nctu, 256, 0, (hipStream_t)stream>>>(coef, nctu * 4, cbf);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 extern "C" int bc250_hevc_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp,
                                          int is_intra, const uint8_t *cbf, const uint32_t *mvs, int num_slices,
                                          void *stream)

@@ -84,8 +84,8 @@ def copy_calibration(pkg, torch, nbytes=64 << 20, reps=3):
 def cpu_baseline(frames_coded, qp, rng, lam, budget_s, codec):
     """Oracle (single thread) on a bounded sample: the integer search of a
     spread sample of macroblocks, scaled to the frame, plus the rest of a
-    P frame on the whole frame (HEVC: quarter-sample refine of every 8x8 CU
-    and CU coding; H.264: sub-pel refine, P encode and deblock)."""
+    P frame on the whole frame (HEVC: quarter-sample refine of every 8x8 CU,
+    CU coding and deblocking; H.264: sub-pel refine, P encode and deblock)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     (ry, ruv), (cy, cuv) = frames_coded[0], frames_coded[1]
@@ -113,8 +113,13 @@ def cpu_baseline(frames_coded, qp, rng, lam, budget_s, codec):
             mv[cu] = (mx, my)
         cb, cr = orc.split_uv(cuv)
         fb, fr = orc.split_uv(ruv)
-        orc.hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, orc.HEVC_ROUND_INTER)
-        rest = "quarter-sample refine of every 8x8 CU and HEVC CU coding of the whole frame"
+        o = orc.hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, orc.HEVC_ROUND_INTER)
+        mv4 = np.zeros((nmb * 4, 4), np.int32)
+        mv4[:, :2] = mv
+        uv = np.empty_like(cuv)
+        uv[:, 0::2], uv[:, 1::2] = o["cb"], o["cr"]
+        orc.hevc_deblock(o["y"], uv, qp, 0, orc.hevc_cu_cbf(o["coef"]), mv4, 1)
+        rest = "quarter-sample refine of every 8x8 CU, HEVC CU coding and deblocking of the whole frame"
     else:
         mv16, _ = orc.me_refine(cy, ry, i16, i8)
         e = orc.encode_p(cy, cuv, ry, ruv, qp, mv16)

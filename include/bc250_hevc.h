@@ -92,6 +92,11 @@ int bc250_hevc_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, i
                                int opy, int opuv, int wc, int hc, int qp, int num_slices, int round_q12,
                                uint8_t *modes, int16_t *coef, int *sync, unsigned spin_limit, void *stream);
 
+/* Per-CU coded-luma flags of a P picture's levels (bc250_hevc_launch_p_encode
+ * layout): cbf[4*t + z] = 1 when any of CU z's 64 luma levels is non-zero,
+ * the bS 1 condition of the deblocking filter. */
+int bc250_hevc_cu_cbf(const int16_t *coef, int nctu, uint8_t *cbf, void *stream);
+
 /* H.265 8.7.2 deblocking of a reconstructed picture in place (wc, hc
  * multiples of 16; pitches multiples of 4) for the pipeline's coding shape:
  * 8x8 CUs in CTU-raster, z-order (cu = ctu*4 + (y/8&1)*2 + (x/8&1)), one

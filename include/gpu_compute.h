@@ -191,8 +191,10 @@ typedef struct {
  * instead: I pictures as CTU16 / CU8 / PU4 intra (the reference's
  * intra_trial), P pictures as 8x8 inter CUs with the +-R full-search 8x8
  * vectors refined to quarter samples by the HEVC interpolation, 8x8 luma and
- * 4x4 chroma DCT; no in-loop filter (the reference's HEVC encoder signals
- * pps_deblocking_filter_disabled_flag = 1, encoder_h265.c:330).  Staging:
+ * 4x4 chroma DCT, then the 8.7.2 deblocking filter unless deblock_idc is 1
+ * (slices never filter across their borders, so 0 and 2 agree).  deblock_idc
+ * 1 is the reference's HEVC encoder, whose PPS sets
+ * pps_deblocking_filter_disabled_flag (encoder_h265.c:330).  Staging:
  * quant = int16 [nctu*384] coefficients in the bc250_hevc.h layout, mv8 =
  * the CU vectors (quarter samples), HEVC modes via
  * gpu_compute_get_hevc_mode_staging_data_slot. */

@@ -171,6 +171,7 @@ def encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lambda8=1
 
 # ------------------------------------------------------------------- HEVC
 REF_HEVC = os.path.join(_HERE, "_ref", "libref_hevc.so")
+REF_HEVC_DEBLOCK = os.path.join(_HERE, "_ref", "libref_hevc_deblock.so")
 
 _sig("orc_hevc_chroma_qp", _I, [_I])
 _sig("orc_hevc_quant", None, [_P, _I, _I, _I, _I, _P])
@@ -280,12 +281,14 @@ def hevc_refine8(cur, ref, x0, y0, mvx, mvy, bd=8):
     return mx.value, my.value, sad
 
 
-def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lambda8=1):
+def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lambda8=1, deblock=True):
     """One picture of the HEVC pipeline in its stage order: I -> CTU intra
     coding (rounding 4); P -> +-rng full search on 8x8 blocks, quarter-sample
-    refine with the HEVC interpolation, 8x8 inter CU coding (rounding 2).
-    Planes are coded-size NV12 (cuv / ruv interleaved); returns coef, modes
-    (uint8 [nctu*16]), mv8 (MV [nctu*4]) and the recon as NV12 (y, uv)."""
+    refine with the HEVC interpolation, 8x8 inter CU coding (rounding 2);
+    then, with deblock, the 8.7.2 filter (bS from intra / coded luma / vector
+    difference).  Planes are coded-size NV12 (cuv / ruv interleaved); returns
+    coef, modes (uint8 [nctu*16]), mv8 (MV [nctu*4]) and the recon as NV12
+    (y, uv)."""
     hc, wc = cy.shape
     t = (wc // 16) * (hc // 16)
     cb, cr = split_uv(cuv)
@@ -307,7 +310,20 @@ def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lamb
         modes = np.zeros(t * 16, np.uint8)
     uv = np.empty((hc // 2, wc), np.uint8)
     uv[:, 0::2], uv[:, 1::2] = o["cb"], o["cr"]
-    return dict(coef=o["coef"], modes=modes, mv8=mv8, y=o["y"], uv=uv)
+    y = o["y"]
+    if deblock:
+        cbf = hevc_cu_cbf(o["coef"]) if not is_intra else np.zeros(t * 4, np.uint8)
+        mv4 = np.zeros((t * 4, 4), np.int32)
+        mv4[:, 0], mv4[:, 1] = mv8["mvx"], mv8["mvy"]
+        y, uv = hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices)
+    return dict(coef=o["coef"], modes=modes, mv8=mv8, y=y, uv=uv)
+
+
+def hevc_cu_cbf(coef):
+    """Per-CU coded-luma flag of a P picture's levels (CU z of CTU t holds
+    coef[384 t + 64 z : +64])."""
+    c = np.asarray(coef, np.int16).reshape(-1, 384)[:, :256].reshape(-1, 64)
+    return (c != 0).any(axis=1).astype(np.uint8)
 
 _sig("orc_hevc_dct_matrix", None, [_I, _P])
 _sig("orc_hevc_tqn", None, [_P, _I, _I, _I, _I, _P])

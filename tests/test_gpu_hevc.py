@@ -199,11 +199,12 @@ def test_i_encode_stage(pkg, w, h, qp, ns):
     assert np.array_equal(gb, want["cb"]) and np.array_equal(gr, want["cr"])
 
 
-def _hevc_sequence(pkg, w, h, n, qp, ns, rng, lam, seed, nrect, gop=0):
-    say(f"hevc pipeline {w}x{h} frames={n} qp={qp} slices={ns} R={rng}")
+def _hevc_sequence(pkg, w, h, n, qp, ns, rng, lam, seed, nrect, gop=0, deblock=True):
+    say(f"hevc pipeline {w}x{h} frames={n} qp={qp} slices={ns} R={rng} deblock={deblock}")
     fr = _frames(pkg, seed, w, h, n, nrect)
     g = pkg.FrameGen(seed, w, h, nrect)
-    enc = pkg.Encoder(w, h, search_range=rng, lambda8=lam, codec=pkg.GPU_CODEC_HEVC)
+    enc = pkg.Encoder(w, h, search_range=rng, lambda8=lam, deblock_idc=0 if deblock else 1,
+                      codec=pkg.GPU_CODEC_HEVC)
     ref = None
     try:
         for i in range(n):
@@ -214,7 +215,7 @@ def _hevc_sequence(pkg, w, h, n, qp, ns, rng, lam, seed, nrect, gop=0):
             got = enc.results(slot)
             cy, cuv = fr[i]
             ry, ruv = (cy, cuv) if ref is None else ref
-            exp = orc.hevc_encode_frame(cy, cuv, ry, ruv, qp, intra, ns, rng, lam)
+            exp = orc.hevc_encode_frame(cy, cuv, ry, ruv, qp, intra, ns, rng, lam, deblock)
             assert np.array_equal(got["mv8"], exp["mv8"]), f"frame {i} mv8"
             assert np.array_equal(got["hevc_mode"], exp["modes"]), f"frame {i} modes"
             assert np.array_equal(got["quant"], exp["coef"]), f"frame {i} coef"
@@ -231,6 +232,12 @@ def _hevc_sequence(pkg, w, h, n, qp, ns, rng, lam, seed, nrect, gop=0):
 def test_hevc_pipeline_small_ipppp(pkg):
     _torch()
     _hevc_sequence(pkg, 128, 96, 5, 27, 1, 16, 1, seed=31, nrect=3)
+
+
+def test_hevc_pipeline_no_deblock(pkg):
+    """deblock_idc 1: the reference encoder's setting (filter disabled in the PPS)."""
+    _torch()
+    _hevc_sequence(pkg, 128, 96, 4, 32, 2, 16, 1, seed=33, nrect=3, deblock=False)
 
 
 def test_hevc_pipeline_odd_size_slices_gop(pkg):
@@ -306,3 +313,19 @@ def test_deblock(pkg, w, h, qp, intra, ns):
     assert (oy[:, w:] == 7).all() and (ouv[:, w:] == 9).all()
     assert pkg.hip.bc250_hevc_launch_deblock(P(d[0]), P(d[1]), pitch, pitch, w + 8, h, qp, intra, P(d[2]), P(d[3]),
                                              ns, None) == -1
+
+
+def test_cu_cbf(pkg):
+    torch = _torch()
+    rs = np.random.default_rng(4)
+    nctu = 37
+    coef = np.zeros((nctu, 384), np.int16)
+    for cu in range(nctu * 4):
+        if rs.random() < 0.5:
+            coef[cu // 4, (cu % 4) * 64 + rs.integers(0, 64)] = rs.integers(1, 9) * (1 if rs.random() < .5 else -1)
+    coef[:, 256:] = 5          # chroma levels never count
+    dc, dcbf = dev(torch, coef), torch.full((nctu * 4,), 7, dtype=torch.uint8, device="cuda")
+    assert pkg.hip.bc250_hevc_cu_cbf(P(dc), nctu, P(dcbf), None) == 0
+    torch.cuda.synchronize()
+    want = orc.hevc_cu_cbf(coef)
+    assert np.array_equal(host(dcbf), want) and 0 < want.sum() < nctu * 4
