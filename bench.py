@@ -6,7 +6,7 @@ already resident in HBM.  The headline (--codec hevc, the default) is the
 HEVC Main pipeline: +-64 integer full search on 16x16 and 8x8 blocks,
 quarter-sample refinement of every 8x8 CU with the HEVC interpolation, CU
 coding (8x8 luma / 4x4 chroma DCT, quant, recon), I picture as the CTU
-intra wavefront.  The H.264 pipeline (the reference's gpu_compute stages,
+intra wavefront, then 8.7.2 deblocking.  The H.264 pipeline (the reference's gpu_compute stages,
 with deblocking) runs on the same frames and is reported beside it.  The
 timed region is an IPPP sequence of --steps frames driven by the host C
 sequence driver (host/h264_host.c) through the gpu_compute_* ABI, pipelined
@@ -215,8 +215,8 @@ def main():
     if args.codec == "hevc":
         desc = ("HEVC Main 8-bit pipeline: I picture as CTU16/CU8/PU4 intra (Planar/DC/H/V by SAD, 4x4 DST), "
                 "P pictures as 8x8 inter CUs with +-{r} integer full search (16x16 and 8x8 costs) refined to "
-                "quarter samples with the HEVC 8-tap filter, 8x8 luma / 4x4 chroma DCT, quant, recon; "
-                "no in-loop filter (as the reference's HEVC encoder)").format(r=args.range)
+                "quarter samples with the HEVC 8-tap filter, 8x8 luma / 4x4 chroma DCT, quant, recon, "
+                "8.7.2 deblocking").format(r=args.range)
     else:
         desc = ("H.264 pipeline (the reference's gpu_compute stages): +-{r} integer full search on 16x16 and 8x8 "
                 "+ quarter-pel refine, P16x16 / I16x16 encode with 4x4 transform and quant, "
