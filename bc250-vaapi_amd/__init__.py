@@ -134,13 +134,13 @@ _sig(hip, "bc250_hevc_dqit8_batch", _I, [_P, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_tqn_batch", _I, [_P, _I, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_dqitn_batch", _I, [_P, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_mc_uni_batch", _I, [_P, _I, _I, _I, _P, _I, _I, _I, _I, _I, _P, _P])
-_sig(hip, "bc250_hevc_launch_refine8", _I, [_P, _I, _P, _I, _I, _I, _P, _P, _P])
+_sig(hip, "bc250_hevc_launch_refine8", _I, [_P, _I, _P, _I, _I, _I, _P, _P, _I, _P])
 _sig(hip, "bc250_hevc_intra_work_bytes", C.c_size_t, [_I, _I])
 _sig(hip, "bc250_hevc_launch_i_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _U,
                                             _P])
 _sig(hip, "bc250_hevc_launch_p_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P,
-                                            _P, _P])
-_sig(hip, "bc250_hevc_launch_deblock", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P])
+                                            _P, _I, _P])
+_sig(hip, "bc250_hevc_launch_deblock", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P])
 _sig(hip, "bc250_hevc_cu_cbf", _I, [_P, _I, _P, _P])
 
 _sig(host, "bc250_gen_create", _P, [C.c_uint64, _I, _I, _I])

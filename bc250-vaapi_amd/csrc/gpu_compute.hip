@@ -508,7 +508,8 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
         r = bc250_launch_me_fullsearch(cur.y, cur.py, ref.y, ref.py, s->wc, s->hc, p.search_range, p.lambda8,
                                        s->imv16, s->imv8, st);
     hipEventRecord(sl.ev[1], st);
-    if (r == 0 && !is_intra) r = bc250_hevc_launch_refine8(cur.y, cur.py, ref.y, ref.py, s->wc, s->hc, s->imv8, s->mv8, st);
+    if (r == 0 && !is_intra)
+        r = bc250_hevc_launch_refine8(cur.y, cur.py, ref.y, ref.py, s->wc, s->hc, s->imv8, s->mv8, 8, st);
     hipEventRecord(sl.ev[2], st);
     if (r == 0) {
         if (is_intra) {
@@ -521,7 +522,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
         } else {
             r = bc250_hevc_launch_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv, out.y,
                                            out.uv, out.py, out.puv, s->wc, s->hc, qp, 2 /* inter rounding */, s->mv8,
-                                           (int16_t *)s->levels, st);
+                                           (int16_t *)s->levels, 8, st);
             if (r == 0) r = bc250_launch_zero_i32((int *)s->hmodes, (int)n * 4, st);
         }
     }
@@ -532,7 +533,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
         if (!is_intra) r = bc250_hevc_cu_cbf((const int16_t *)s->levels, (int)n, s->hcbf, st);
         if (r == 0)
             r = bc250_hevc_launch_deblock(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->hcbf,
-                                          s->mv8, num_slices, st);
+                                          s->mv8, num_slices, 8, st);
     }
     hipEventRecord(sl.ev[4], st);
     if (r == 0 && p.readback) {

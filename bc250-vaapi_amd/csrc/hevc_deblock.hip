@@ -1,5 +1,6 @@
 // hevc_deblock.hip - H.265 8.7.2 deblocking for gfx950, for the pipeline's
-// coding shape (8x8 CUs, one PU each, one reference picture).
+// coding shape (8x8 CUs, one PU each, one reference picture), 8-bit NV12 or
+// 10-bit P010 (the sample in the high 10 bits of each 16-bit word).
 //
 // HEVC filters edges on the 8x8 grid and changes at most three samples on
 // either side, so within one direction every edge segment is independent:
@@ -48,8 +49,8 @@ BC_DEV int slice_first_row(int row, int rows, int ns)
 }
 
 // One luma segment: v[k][0..7] = p3 p2 p1 p0 q0 q1 q2 q3 of line k (8.7.2.5.3,
-// 8.7.2.5.6, 8.7.2.5.7).
-BC_DEV void luma_segment(int v[4][8], int beta, int tc)
+// 8.7.2.5.6, 8.7.2.5.7); beta and tc already scaled to the bit depth.
+BC_DEV void luma_segment(int v[4][8], int beta, int tc, int maxv)
 {
     const int dp0 = iabs(v[0][1] - 2 * v[0][2] + v[0][3]), dp3 = iabs(v[3][1] - 2 * v[3][2] + v[3][3]);
     const int dq0 = iabs(v[0][6] - 2 * v[0][5] + v[0][4]), dq3 = iabs(v[3][6] - 2 * v[3][5] + v[3][4]);
@@ -78,31 +79,62 @@ BC_DEV void luma_segment(int v[4][8], int beta, int tc)
             int delta = (9 * (q0 - p0) - 3 * (q1 - p1) + 8) >> 4;
             if (iabs(delta) >= tc * 10) continue;
             delta = clampi(delta, -tc, tc);
-            v[k][3] = clip255(p0 + delta);
-            v[k][4] = clip255(q0 - delta);
+            v[k][3] = clampi(p0 + delta, 0, maxv);
+            v[k][4] = clampi(q0 - delta, 0, maxv);
             const int th = tc >> 1;
-            if (dEp) v[k][2] = clip255(p1 + clampi((((p2 + p0 + 1) >> 1) - p1 + delta) >> 1, -th, th));
-            if (dEq) v[k][5] = clip255(q1 + clampi((((q2 + q0 + 1) >> 1) - q1 - delta) >> 1, -th, th));
+            if (dEp) v[k][2] = clampi(p1 + clampi((((p2 + p0 + 1) >> 1) - p1 + delta) >> 1, -th, th), 0, maxv);
+            if (dEq) v[k][5] = clampi(q1 + clampi((((q2 + q0 + 1) >> 1) - q1 - delta) >> 1, -th, th), 0, maxv);
         }
     }
 }
 
-BC_DEV void chroma_line(int &p1, int &p0, int &q0, int &q1, int tc)
+BC_DEV void chroma_line(int p1, int &p0, int &q0, int q1, int tc, int maxv)
 {
     const int delta = clampi((((q0 - p0) * 4) + p1 - q1 + 4) >> 3, -tc, tc);
-    p0 = clip255(p0 + delta);
-    q0 = clip255(q0 - delta);
+    p0 = clampi(p0 + delta, 0, maxv);
+    q0 = clampi(q0 - delta, 0, maxv);
+}
+
+// Four consecutive samples of a row: one dword (8-bit) or two (P010, where
+// the sample is word >> 6).
+template <typename Pix>
+BC_DEV void ld4(const uint8_t *row, int i, int v[4])
+{
+    if constexpr (sizeof(Pix) == 1) {
+        const uint32_t w = *(const uint32_t *)(row + i);
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = (w >> (8 * k)) & 255;
+    } else {
+        const uint2 w = *(const uint2 *)(row + 2 * i);
+        v[0] = (w.x & 0xffffu) >> 6;
+        v[1] = w.x >> 22;
+        v[2] = (w.y & 0xffffu) >> 6;
+        v[3] = w.y >> 22;
+    }
+}
+
+template <typename Pix>
+BC_DEV void st4(uint8_t *row, int i, const int v[4])
+{
+    if constexpr (sizeof(Pix) == 1) {
+        *(uint32_t *)(row + i) = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
+    } else {
+        *(uint2 *)(row + 2 * i) = make_uint2(((uint32_t)v[0] << 6) | ((uint32_t)v[1] << 22),
+                                             ((uint32_t)v[2] << 6) | ((uint32_t)v[3] << 22));
+    }
 }
 }  // namespace
 
 // VERT: vertical edges (filtered along x); otherwise horizontal edges.
-template <bool VERT>
+// Pitches in bytes; Pix uint8_t (8-bit) or uint16_t (P010, BD 10).
+template <typename Pix, int BD, bool VERT>
 __global__ __launch_bounds__(256) void hevc_deblock_kernel(uint8_t *__restrict__ y, uint8_t *__restrict__ uv, int py,
                                                            int puv, int wc, int hc, int qp, int intra,
                                                            const uint8_t *__restrict__ cbf,
                                                            const uint4 *__restrict__ mv, int num_slices, int nl,
                                                            int nc)
 {
+    constexpr int MAXV = (1 << BD) - 1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int wctu = wc >> 4, hctu = hc >> 4;
     if (t < nl) {
@@ -114,41 +146,38 @@ __global__ __launch_bounds__(256) void hevc_deblock_kernel(uint8_t *__restrict__
         const int cq = cu_at(x, yy, wctu), cp = cu_at(VERT ? x - 1 : x, VERT ? yy : yy - 1, wctu);
         const int bs = strength(intra, cbf, mv, cp, cq);
         if (!bs) return;
-        const int beta = kBetaT[clampi(qp, 0, 51)], tc = kTcT[clampi(qp + 2 * (bs - 1), 0, 53)];
+        const int beta = kBetaT[clampi(qp, 0, 51)] << (BD - 8), tc = kTcT[clampi(qp + 2 * (bs - 1), 0, 53)] << (BD - 8);
         int v[4][8];
         if (VERT) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const uint32_t *row = (const uint32_t *)(y + (size_t)(yy + k) * py + x - 4);
-                const uint32_t a = row[0], b = row[1];
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    v[k][i] = (a >> (8 * i)) & 255;
-                    v[k][4 + i] = (b >> (8 * i)) & 255;
-                }
+                const uint8_t *row = y + (size_t)(yy + k) * py;
+                ld4<Pix>(row, x - 4, &v[k][0]);
+                ld4<Pix>(row, x, &v[k][4]);
             }
         } else {
 #pragma unroll
             for (int i = 0; i < 8; i++) {
-                const uint32_t r = *(const uint32_t *)(y + (size_t)(yy - 4 + i) * py + x);
+                int r[4];
+                ld4<Pix>(y + (size_t)(yy - 4 + i) * py, x, r);
 #pragma unroll
-                for (int k = 0; k < 4; k++) v[k][i] = (r >> (8 * k)) & 255;
+                for (int k = 0; k < 4; k++) v[k][i] = r[k];
             }
         }
-        luma_segment(v, beta, tc);
+        luma_segment(v, beta, tc, MAXV);
         if (VERT) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                uint32_t *row = (uint32_t *)(y + (size_t)(yy + k) * py + x - 4);
-                row[0] = (uint32_t)v[k][0] | ((uint32_t)v[k][1] << 8) | ((uint32_t)v[k][2] << 16) | ((uint32_t)v[k][3] << 24);
-                row[1] = (uint32_t)v[k][4] | ((uint32_t)v[k][5] << 8) | ((uint32_t)v[k][6] << 16) | ((uint32_t)v[k][7] << 24);
+                uint8_t *row = y + (size_t)(yy + k) * py;
+                st4<Pix>(row, x - 4, &v[k][0]);
+                st4<Pix>(row, x, &v[k][4]);
             }
         } else {
 #pragma unroll
-            for (int i = 1; i < 7; i++)
-                *(uint32_t *)(y + (size_t)(yy - 4 + i) * py + x) = (uint32_t)v[0][i] | ((uint32_t)v[1][i] << 8) |
-                                                                    ((uint32_t)v[2][i] << 16) |
-                                                                    ((uint32_t)v[3][i] << 24);
+            for (int i = 1; i < 7; i++) {
+                const int r[4] = {v[0][i], v[1][i], v[2][i], v[3][i]};
+                st4<Pix>(y + (size_t)(yy - 4 + i) * py, x, r);
+            }
         }
         return;
     }
@@ -162,47 +191,36 @@ __global__ __launch_bounds__(256) void hevc_deblock_kernel(uint8_t *__restrict__
     const int lx = 2 * cx, ly = 2 * cy;
     const int cq = cu_at(lx, ly, wctu), cp = cu_at(VERT ? lx - 1 : lx, VERT ? ly : ly - 1, wctu);
     if (strength(intra, cbf, mv, cp, cq) != 2) return;
-    const int tc = kTcT[clampi(hevc::chroma_qp(qp) + 2, 0, 53)];
+    const int tc = kTcT[clampi(hevc::chroma_qp(qp) + 2, 0, 53)] << (BD - 8);
     if (VERT) {
-        // per line: bytes Cb/Cr of chroma columns cx-2 .. cx+1 (8 bytes)
+        // per line: interleaved Cb/Cr of chroma columns cx-2 .. cx+1 (8 samples)
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            uint32_t *row = (uint32_t *)(uv + (size_t)(cy + k) * puv + 2 * cx - 4);
-            uint32_t a = row[0], b = row[1];
+            uint8_t *row = uv + (size_t)(cy + k) * puv;
+            int a[4], b[4];
+            ld4<Pix>(row, 2 * cx - 4, a);
+            ld4<Pix>(row, 2 * cx, b);
 #pragma unroll
-            for (int comp = 0; comp < 2; comp++) {
-                int p1 = (a >> (8 * comp)) & 255, p0 = (a >> (16 + 8 * comp)) & 255;
-                int q0 = (b >> (8 * comp)) & 255, q1 = (b >> (16 + 8 * comp)) & 255;
-                chroma_line(p1, p0, q0, q1, tc);
-                a = (a & ~(0xffu << (16 + 8 * comp))) | ((uint32_t)p0 << (16 + 8 * comp));
-                b = (b & ~(0xffu << (8 * comp))) | ((uint32_t)q0 << (8 * comp));
-            }
-            row[0] = a;
-            row[1] = b;
+            for (int comp = 0; comp < 2; comp++) chroma_line(a[comp], a[2 + comp], b[comp], b[2 + comp], tc, MAXV);
+            st4<Pix>(row, 2 * cx - 4, a);
+            st4<Pix>(row, 2 * cx, b);
         }
     } else {
-        // 4 chroma columns (8 interleaved bytes) across rows cy-2 .. cy+1
-        uint32_t r[4][2];
+        // 4 chroma columns (8 interleaved samples) across rows cy-2 .. cy+1
+        int r[4][8];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const uint32_t *row = (const uint32_t *)(uv + (size_t)(cy - 2 + i) * puv + 2 * cx);
-            r[i][0] = row[0];
-            r[i][1] = row[1];
+            const uint8_t *row = uv + (size_t)(cy - 2 + i) * puv;
+            ld4<Pix>(row, 2 * cx, &r[i][0]);
+            ld4<Pix>(row, 2 * cx + 4, &r[i][4]);
         }
 #pragma unroll
-        for (int byte = 0; byte < 8; byte++) {
-            const int w = byte >> 2, sh = 8 * (byte & 3);
-            int p1 = (r[0][w] >> sh) & 255, p0 = (r[1][w] >> sh) & 255, q0 = (r[2][w] >> sh) & 255,
-                q1 = (r[3][w] >> sh) & 255;
-            chroma_line(p1, p0, q0, q1, tc);
-            r[1][w] = (r[1][w] & ~(0xffu << sh)) | ((uint32_t)p0 << sh);
-            r[2][w] = (r[2][w] & ~(0xffu << sh)) | ((uint32_t)q0 << sh);
-        }
+        for (int k = 0; k < 8; k++) chroma_line(r[0][k], r[1][k], r[2][k], r[3][k], tc, MAXV);
 #pragma unroll
         for (int i = 1; i < 3; i++) {
-            uint32_t *row = (uint32_t *)(uv + (size_t)(cy - 2 + i) * puv + 2 * cx);
-            row[0] = r[i][0];
-            row[1] = r[i][1];
+            uint8_t *row = uv + (size_t)(cy - 2 + i) * puv;
+            st4<Pix>(row, 2 * cx, &r[i][0]);
+            st4<Pix>(row, 2 * cx + 4, &r[i][4]);
         }
     }
 }
@@ -227,26 +245,39 @@ extern "C" int bc250_hevc_cu_cbf(const int16_t *coef, int nctu, uint8_t *cbf, vo
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-extern "C" int bc250_hevc_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp,
-                                         int is_intra, const uint8_t *cbf, const uint32_t *mvs, int num_slices,
-                                         void *stream)
+template <typename Pix, int BD>
+static void launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                           const uint8_t *cbf, const uint4 *mvs, int num_slices, hipStream_t st)
 {
-    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || (py & 3) || (puv & 3) || py < wc ||
-        puv < wc || num_slices < 1)
-        return -1;
-    hipStream_t st = (hipStream_t)stream;
     // vertical pass
     int nl = (wc / 8 - 1) * (hc / 4), nc = (wc / 16 - 1) * (hc / 8);
     int n = nl + nc;
     if (n > 0)
-        hevc_deblock_kernel<true><<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf,
-                                                                   (const uint4 *)mvs, num_slices, nl, nc);
+        hevc_deblock_kernel<Pix, BD, true>
+            <<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, num_slices, nl, nc);
     // horizontal pass
     nl = (hc / 8 - 1) * (wc / 4);
     nc = (hc / 16 - 1) * (wc / 8);
     n = nl + nc;
     if (n > 0)
-        hevc_deblock_kernel<false><<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf,
-                                                                    (const uint4 *)mvs, num_slices, nl, nc);
+        hevc_deblock_kernel<Pix, BD, false>
+            <<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, num_slices, nl, nc);
+}
+
+extern "C" int bc250_hevc_launch_deblock(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                                         const uint8_t *cbf, const uint32_t *mvs, int num_slices, int bd,
+                                         void *stream)
+{
+    const int bps = bd == 8 ? 1 : 2;
+    if ((bd != 8 && bd != 10) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || (py & 7) ||
+        (puv & 7) || py < wc * bps || puv < wc * bps || num_slices < 1)
+        return -1;
+    hipStream_t st = (hipStream_t)stream;
+    if (bd == 8)
+        launch_deblock<uint8_t, 8>((uint8_t *)y, (uint8_t *)uv, py, puv, wc, hc, qp, is_intra, cbf,
+                                   (const uint4 *)mvs, num_slices, st);
+    else
+        launch_deblock<uint16_t, 10>((uint8_t *)y, (uint8_t *)uv, py, puv, wc, hc, qp, is_intra, cbf,
+                                     (const uint4 *)mvs, num_slices, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

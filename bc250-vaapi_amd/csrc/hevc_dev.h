@@ -1,9 +1,10 @@
-// hevc_dev.h - device helpers for the HEVC stages (8-bit samples).
+// hevc_dev.h - device helpers for the HEVC stages.
 //
 // Arithmetic follows H.265 and the reference encoder's chosen forward
-// shifts (enc/src/hevc_intra.c):
-//   4x4 forward  (1, 8)        inverse (7, 12)      dequant bdShift 5
-//   8x8 forward  (2, 9)        inverse (7, 12)      dequant bdShift 6
+// shifts (enc/src/hevc_intra.c), at bit depth bd:
+//   4x4 forward  (1, 8) at 8 bits, (3, 8) at 10   inverse (7, 20 - bd)
+//   8x8 forward  (bd - 6, 9)                      inverse (7, 20 - bd)
+//   quant / dequant bdShift bd + log2(n) - 5, QP' = QP + 6 (bd - 8)
 // with int16 saturation after every pass, quantisation
 //   level = min(32767, floor(((|v| << shift) + off) / (ls << (q/6 + 4))))
 // and the flat-list dequantiser of 8.6.3.
@@ -76,6 +77,22 @@ BC_DEV int dequant(int c, int q, int bds)
     const int prod = c * level_scale(q % 6);
     const int v = a >= bds ? prod * (1 << (a - bds)) : (prod + (1 << (bds - a - 1))) >> (bds - a);
     return sat16(v);
+}
+
+// One sample of an 8-bit plane or of a P010 plane (the 10-bit sample in the
+// high bits of a 16-bit word); row is a byte pointer to the start of a row.
+template <typename Pix>
+BC_DEV int ldpix(const uint8_t *row, int x)
+{
+    if constexpr (sizeof(Pix) == 1) return row[x];
+    else return ((const uint16_t *)row)[x] >> 6;
+}
+
+template <typename Pix>
+BC_DEV void stpix(uint8_t *row, int x, int v)
+{
+    if constexpr (sizeof(Pix) == 1) row[x] = (uint8_t)v;
+    else ((uint16_t *)row)[x] = (uint16_t)(v << 6);
 }
 
 }  // namespace hevc

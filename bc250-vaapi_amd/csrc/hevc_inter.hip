@@ -248,7 +248,9 @@ __global__ __launch_bounds__(64) void hevc_mc_batch_kernel(const Pix *__restrict
 // ---------------------------------------------------------------------------
 // Quarter-sample refinement, one wave per 8x8 CU, 4 CUs (one CTU) per
 // workgroup.  Every candidate lies within 3 quarter samples of the integer
-// vector, so all taps fall in a 16x16 window at (x0+dx-4, y0+dy-4).
+// vector, so all taps fall in a 16x16 window at (x0+dx-4, y0+dy-4).  Pitches
+// in bytes; Pix uint8_t (8-bit) or uint16_t (P010, BD 10).
+template <typename Pix, int BD>
 __global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__restrict__ cur, int cpitch,
                                                            const uint8_t *__restrict__ ref, int rpitch, int wc, int hc,
                                                            int nctu, const uint4 *__restrict__ imv8,
@@ -265,13 +267,13 @@ __global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__rest
     const int ox = x0 + dx - 4, oy = y0 + dy - 4;
     int *W = win[wv];
     for (int t = lane; t < 256; t += 64)
-        W[t] = ref[(size_t)clampi(oy + (t >> 4), 0, hc - 1) * rpitch + clampi(ox + (t & 15), 0, wc - 1)];
+        W[t] = ldpix<Pix>(ref + (size_t)clampi(oy + (t >> 4), 0, hc - 1) * rpitch, clampi(ox + (t & 15), 0, wc - 1));
     const int pr = lane >> 3, pc = lane & 7;
-    const int cv = cur[(size_t)(y0 + pr) * cpitch + x0 + pc];
+    const int cv = ldpix<Pix>(cur + (size_t)(y0 + pr) * cpitch, x0 + pc);
     wave_lds_sync();
     auto sad_of = [&](int mx, int my) {
         const int wx = pc + (mx >> 2) - dx + 4, wy = pr + (my >> 2) - dy + 4;
-        const int pv = mc_sample<false>(W, 16, wx, wy, mx & 3, my & 3, 8);
+        const int pv = mc_sample<false>(W, 16, wx, wy, mx & 3, my & 3, BD);
         return wave_sum_u32((uint32_t)iabs(cv - pv));
     };
     const int hx[8] = {2, -2, 0, 0, 2, 2, -2, -2}, hy[8] = {0, 0, 2, -2, 2, -2, 2, -2};
@@ -304,12 +306,15 @@ struct PcuLds {
     int ca[32], cb[32];
 };
 
+template <typename Pix, int BD>
 __global__ __launch_bounds__(256) void hevc_p_encode_kernel(
     const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, const uint8_t *__restrict__ ry,
     const uint8_t *__restrict__ ruv, int rpy, int rpuv, uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy,
     int opuv, int wc, int hc, int nctu, int qp, int round_q12, const uint4 *__restrict__ mvs,
     int16_t *__restrict__ coef)
 {
+    constexpr int MAXV = (1 << BD) - 1, QOFF = 6 * (BD - 8);
+    constexpr int S1_8 = BD - 6, QSH8 = BD - 2, S1_4 = BD == 8 ? 1 : 3, QSH4 = BD - 3, S2INV = 20 - BD;
     __shared__ PcuLds Ls[4];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ctu = xcd_item(blockIdx.x, nctu);
@@ -323,49 +328,51 @@ __global__ __launch_bounds__(256) void hevc_p_encode_kernel(
     const int lx = x0 + (mvx >> 2) - 3, ly = y0 + (mvy >> 2) - 3;
     const int cx = ux0 + (mvx >> 3) - 1, cyy = uy0 + (mvy >> 3) - 1;
     for (int t = lane; t < 225; t += 64)
-        L.wl[t] = ry[(size_t)clampi(ly + t / 15, 0, hc - 1) * rpy + clampi(lx + t % 15, 0, wc - 1)];
+        L.wl[t] = ldpix<Pix>(ry + (size_t)clampi(ly + t / 15, 0, hc - 1) * rpy, clampi(lx + t % 15, 0, wc - 1));
     for (int t = lane; t < 98; t += 64) {
         const int comp = t >= 49, k = t - comp * 49;
-        L.wcr[comp][k] = ruv[(size_t)clampi(cyy + k / 7, 0, hcc - 1) * rpuv + 2 * clampi(cx + k % 7, 0, wcc - 1) + comp];
+        L.wcr[comp][k] =
+            ldpix<Pix>(ruv + (size_t)clampi(cyy + k / 7, 0, hcc - 1) * rpuv, 2 * clampi(cx + k % 7, 0, wcc - 1) + comp);
     }
     const int pr = lane >> 3, pc = lane & 7;
-    const int cur = cy[(size_t)(y0 + pr) * cpy + x0 + pc];
+    const int cur = ldpix<Pix>(cy + (size_t)(y0 + pr) * cpy, x0 + pc);
     const bool cl = lane < 32;
     const int comp = (lane >> 4) & 1, ck = lane & 15, cr_ = ck >> 2, cc_ = ck & 3;
-    const int ccur = cl ? cuv[(size_t)(uy0 + cr_) * cpuv + 2 * (ux0 + cc_) + comp] : 0;
+    const int ccur = cl ? ldpix<Pix>(cuv + (size_t)(uy0 + cr_) * cpuv, 2 * (ux0 + cc_) + comp) : 0;
     wave_lds_sync();
 
     // prediction and residual
-    const int pred = mc_sample<false>(L.wl, 15, pc + 3, pr + 3, mvx & 3, mvy & 3, 8);
-    const int cpred = cl ? mc_sample<true>(L.wcr[comp], 7, cc_ + 1, cr_ + 1, mvx & 7, mvy & 7, 8) : 0;
+    const int pred = mc_sample<false>(L.wl, 15, pc + 3, pr + 3, mvx & 3, mvy & 3, BD);
+    const int cpred = cl ? mc_sample<true>(L.wcr[comp], 7, cc_ + 1, cr_ + 1, mvx & 7, mvy & 7, BD) : 0;
     L.a[lane] = cur - pred;
     if (cl) L.ca[lane] = ccur - cpred;
     wave_lds_sync();
 
     // forward: columns, then rows
-    L.b[lane] = pass<8>(kDct8, false, L.a, pr, pc, 2);
-    if (cl) L.cb[lane] = pass<4>(kDct4, false, &L.ca[comp * 16], cr_, cc_, 1);
+    L.b[lane] = pass<8>(kDct8, false, L.a, pr, pc, S1_8);
+    if (cl) L.cb[lane] = pass<4>(kDct4, false, &L.ca[comp * 16], cr_, cc_, S1_4);
     wave_lds_sync();
-    const int q = clampi(qp, 0, 63), cq = hevc::chroma_qp(qp);
-    const int lv = quant(pass_rows<8>(kDct8, false, L.b, pr, pc, 9), make_quant(q, 6, round_q12));
-    const int clv = cl ? quant(pass_rows<4>(kDct4, false, &L.cb[comp * 16], cr_, cc_, 8), make_quant(cq, 5, round_q12))
+    const int q = clampi(qp + QOFF, 0, 63), cq = hevc::chroma_qp(qp) + QOFF;
+    const int lv = quant(pass_rows<8>(kDct8, false, L.b, pr, pc, 9), make_quant(q, QSH8, round_q12));
+    const int clv = cl ? quant(pass_rows<4>(kDct4, false, &L.cb[comp * 16], cr_, cc_, 8),
+                               make_quant(cq, QSH4, round_q12))
                        : 0;
     int16_t *co = coef + (size_t)ctu * 384;
     co[wv * 64 + lane] = (int16_t)lv;
     if (cl) co[256 + comp * 64 + wv * 16 + ck] = (int16_t)clv;
 
     // dequantise, inverse: columns, then rows
-    L.a[lane] = dequant(lv, q, 6);
-    if (cl) L.ca[lane] = dequant(clv, cq, 5);
+    L.a[lane] = dequant(lv, q, QSH8);
+    if (cl) L.ca[lane] = dequant(clv, cq, QSH4);
     wave_lds_sync();
     L.b[lane] = pass<8>(kDct8, true, L.a, pr, pc, 7);
     if (cl) L.cb[lane] = pass<4>(kDct4, true, &L.ca[comp * 16], cr_, cc_, 7);
     wave_lds_sync();
-    const int r = pass_rows<8>(kDct8, true, L.b, pr, pc, 12);
-    oy[(size_t)(y0 + pr) * opy + x0 + pc] = (uint8_t)clampi(pred + r, 0, 255);
+    const int r = pass_rows<8>(kDct8, true, L.b, pr, pc, S2INV);
+    stpix<Pix>(oy + (size_t)(y0 + pr) * opy, x0 + pc, clampi(pred + r, 0, MAXV));
     if (cl) {
-        const int crr = pass_rows<4>(kDct4, true, &L.cb[comp * 16], cr_, cc_, 12);
-        ouv[(size_t)(uy0 + cr_) * opuv + 2 * (ux0 + cc_) + comp] = (uint8_t)clampi(cpred + crr, 0, 255);
+        const int crr = pass_rows<4>(kDct4, true, &L.cb[comp * 16], cr_, cc_, S2INV);
+        stpix<Pix>(ouv + (size_t)(uy0 + cr_) * opuv, 2 * (ux0 + cc_) + comp, clampi(cpred + crr, 0, MAXV));
     }
 }
 
@@ -446,27 +453,45 @@ extern "C" int bc250_hevc_dqitn_batch(const int16_t *coef, int n, int log2n, int
 
 static int grid_xcd(int items) { return ((items + 7) / 8) * 8; }
 
-extern "C" int bc250_hevc_launch_refine8(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
-                                         int hc, const uint32_t *imv8, uint32_t *out, void *stream)
+extern "C" int bc250_hevc_launch_refine8(const void *cur, int cpitch, const void *ref, int rpitch, int wc, int hc,
+                                         const uint32_t *imv8, uint32_t *out, int bd, void *stream)
 {
-    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || cpitch < wc || rpitch < wc) return -1;
+    const int bps = bd == 8 ? 1 : 2;
+    if (!bd_ok(bd) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || cpitch < wc * bps || rpitch < wc * bps ||
+        (cpitch % bps) || (rpitch % bps))
+        return -1;
     const int nctu = (wc / 16) * (hc / 16);
-    hevc_refine8_kernel<<<grid_xcd(nctu), 256, 0, (hipStream_t)stream>>>(cur, cpitch, ref, rpitch, wc, hc, nctu,
-                                                                         (const uint4 *)imv8, (uint4 *)out);
+    hipStream_t st = (hipStream_t)stream;
+    if (bd == 8)
+        hevc_refine8_kernel<uint8_t, 8><<<grid_xcd(nctu), 256, 0, st>>>(
+            (const uint8_t *)cur, cpitch, (const uint8_t *)ref, rpitch, wc, hc, nctu, (const uint4 *)imv8, (uint4 *)out);
+    else
+        hevc_refine8_kernel<uint16_t, 10><<<grid_xcd(nctu), 256, 0, st>>>(
+            (const uint8_t *)cur, cpitch, (const uint8_t *)ref, rpitch, wc, hc, nctu, (const uint4 *)imv8, (uint4 *)out);
     return launch_ok();
 }
 
-extern "C" int bc250_hevc_launch_p_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv,
-                                          const uint8_t *ry, const uint8_t *ruv, int rpy, int rpuv, uint8_t *oy,
-                                          uint8_t *ouv, int opy, int opuv, int wc, int hc, int qp, int round_q12,
-                                          const uint32_t *mvs, int16_t *coef, void *stream)
+extern "C" int bc250_hevc_launch_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
+                                          const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy, int opuv,
+                                          int wc, int hc, int qp, int round_q12, const uint32_t *mvs, int16_t *coef,
+                                          int bd, void *stream)
 {
-    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || round_q12 < 0 || round_q12 > 12 ||
-        cpy < wc || cpuv < wc || rpy < wc || rpuv < wc || opy < wc || opuv < wc)
+    const int b = bd == 8 ? 1 : 2, mw = wc * b;
+    if (!bd_ok(bd) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || round_q12 < 0 ||
+        round_q12 > 12 || cpy < mw || cpuv < mw || rpy < mw || rpuv < mw || opy < mw || opuv < mw ||
+        ((cpy | cpuv | rpy | rpuv | opy | opuv) % b))
         return -1;
     const int nctu = (wc / 16) * (hc / 16);
-    hevc_p_encode_kernel<<<grid_xcd(nctu), 256, 0, (hipStream_t)stream>>>(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy,
-                                                                          ouv, opy, opuv, wc, hc, nctu, qp, round_q12,
-                                                                          (const uint4 *)mvs, coef);
+    hipStream_t st = (hipStream_t)stream;
+    auto *C = (const uint8_t *)cy, *CU = (const uint8_t *)cuv, *R = (const uint8_t *)ry, *RU = (const uint8_t *)ruv;
+    auto *O = (uint8_t *)oy, *OU = (uint8_t *)ouv;
+    if (bd == 8)
+        hevc_p_encode_kernel<uint8_t, 8><<<grid_xcd(nctu), 256, 0, st>>>(C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy,
+                                                                         opuv, wc, hc, nctu, qp, round_q12,
+                                                                         (const uint4 *)mvs, coef);
+    else
+        hevc_p_encode_kernel<uint16_t, 10><<<grid_xcd(nctu), 256, 0, st>>>(C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU,
+                                                                           opy, opuv, wc, hc, nctu, qp, round_q12,
+                                                                           (const uint4 *)mvs, coef);
     return launch_ok();
 }

@@ -64,21 +64,25 @@ int bc250_hevc_dqitn_batch(const int16_t *coef, int n, int log2n, int qp, int bd
 int bc250_hevc_mc_uni_batch(const void *ref, int stride, int w_pic, int h_pic, const int32_t *blk, int n, int w,
                             int h, int chroma, int bd, void *dst, void *stream);
 
+/* Frame-stage entry points.  Planes are 8-bit NV12 (bd 8) or P010 (bd 10:
+ * the sample in the high 10 bits of each 16-bit word); pitches in bytes. */
+
 /* Quarter-sample refinement of every 8x8 CU: imv8[4*cu] = {dx, dy, cost,
  * sad} integer vectors (bc250_launch_me_fullsearch's 8x8 output); out[4*cu]
  * = {mvx, mvy (quarter samples), sad, 0}.  Centre, then 8 half-sample
  * candidates (2,0) (-2,0) (0,2) (0,-2) (2,2) (2,-2) (-2,2) (-2,-2), then the
- * same offsets halved around the half winner, strictly lower SAD wins. */
-int bc250_hevc_launch_refine8(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
-                              const uint32_t *imv8, uint32_t *out, void *stream);
+ * same offsets halved around the half winner, strictly lower SAD wins; every
+ * candidate interpolated with the HEVC 8-tap filter at the bit depth. */
+int bc250_hevc_launch_refine8(const void *cur, int cpitch, const void *ref, int rpitch, int wc, int hc,
+                              const uint32_t *imv8, uint32_t *out, int bd, void *stream);
 
 /* P picture: every CU inter with mvs[4*cu] = {mvx, mvy, ...}; luma 8x8 DCT,
  * chroma 4x4 DCT at QpC, rounding round_q12 (2 in the reference's default
- * dead-zone mode).  Writes coef and the reconstruction (o*). */
-int bc250_hevc_launch_p_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, const uint8_t *ry,
-                               const uint8_t *ruv, int rpy, int rpuv, uint8_t *oy, uint8_t *ouv, int opy, int opuv,
-                               int wc, int hc, int qp, int round_q12, const uint32_t *mvs, int16_t *coef,
-                               void *stream);
+ * dead-zone mode), QP' = qp + 6 (bd - 8).  Writes coef and the
+ * reconstruction (o*) (inter_residual, hevc_enc_template.c:817-933). */
+int bc250_hevc_launch_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry, const void *ruv,
+                               int rpy, int rpuv, void *oy, void *ouv, int opy, int opuv, int wc, int hc, int qp,
+                               int round_q12, const uint32_t *mvs, int16_t *coef, int bd, void *stream);
 
 /* I picture: CTU16 -> 4 CU8 -> 4 PU4 intra (Planar/DC/H/V by SAD, DST),
  * chroma DC per CU (DCT, QpC), slices as CTU-row ranges
@@ -98,18 +102,20 @@ int bc250_hevc_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, i
 int bc250_hevc_cu_cbf(const int16_t *coef, int nctu, uint8_t *cbf, void *stream);
 
 /* H.265 8.7.2 deblocking of a reconstructed picture in place (wc, hc
- * multiples of 16; pitches multiples of 4) for the pipeline's coding shape:
- * 8x8 CUs in CTU-raster, z-order (cu = ctu*4 + (y/8&1)*2 + (x/8&1)), one
- * PU each.  Vertical edges of the 8x8 grid first, then horizontal edges;
- * picture and slice borders (CTU-row slices as in i_encode) are not
- * filtered.  bS 2 when is_intra, else 1 if cbf[cp] | cbf[cq] or the vectors
- * mvs[4*cu] = {mvx, mvy, ...} differ by >= 4 quarter samples, else 0
- * (approach1-compute-encoder/src/hevc_filter_template.c:221-290, the
- * reference decoder); chroma on the 8x8 chroma grid at bS 2.  The reference
- * encoder's PPS disables this filter (src/encoder_h265.c:330), so the
- * pipeline leaves it off by default. */
-int bc250_hevc_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
-                              const uint8_t *cbf, const uint32_t *mvs, int num_slices, void *stream);
+ * multiples of 16; pitches in bytes, multiples of 8): bd 8 on NV12 planes,
+ * bd 10 on P010 planes (the sample in the high 10 bits of each 16-bit word),
+ * for the pipeline's coding shape: 8x8 CUs in CTU-raster, z-order (cu =
+ * ctu*4 + (y/8&1)*2 + (x/8&1)), one PU each.  Vertical edges of the 8x8 grid
+ * first, then horizontal edges; picture and slice borders (CTU-row slices as
+ * in i_encode) are not filtered.  bS 2 when is_intra, else 1 if cbf[cp] |
+ * cbf[cq] or the vectors mvs[4*cu] = {mvx, mvy, ...} differ by >= 4 quarter
+ * samples, else 0 (approach1-compute-encoder/src/hevc_filter_template.c:
+ * 221-290, the reference decoder); chroma on the 8x8 chroma grid at bS 2;
+ * beta and tC scaled by 2^(bd-8).  The reference encoder's PPS disables this
+ * filter (src/encoder_h265.c:330), so the pipeline applies it only when
+ * asked. */
+int bc250_hevc_launch_deblock(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                              const uint8_t *cbf, const uint32_t *mvs, int num_slices, int bd, void *stream);
 
 #ifdef __cplusplus
 }

@@ -143,6 +143,8 @@ int  orc_hevc_tc(int q);
 int  orc_hevc_bs(int is_intra, const uint8_t *cbf, const int32_t *mv4, int cp, int cq);
 void orc_hevc_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
                       const uint8_t *cbf, const int32_t *mv4, int num_slices);
+void orc_hevc_deblock_bd(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                         const uint8_t *cbf, const int32_t *mv4, int num_slices, int bd);
 uint32_t orc_hevc_refine8(const void *cur, const void *ref, int bd, int wc, int hc, int x0, int y0,
                           int *mvx, int *mvy);
 

@@ -281,41 +281,44 @@ def hevc_refine8(cur, ref, x0, y0, mvx, mvy, bd=8):
     return mx.value, my.value, sad
 
 
-def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lambda8=1, deblock=True):
+def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lambda8=1, deblock=True, bd=8):
     """One picture of the HEVC pipeline in its stage order: I -> CTU intra
-    coding (rounding 4); P -> +-rng full search on 8x8 blocks, quarter-sample
-    refine with the HEVC interpolation, 8x8 inter CU coding (rounding 2);
-    then, with deblock, the 8.7.2 filter (bS from intra / coded luma / vector
-    difference).  Planes are coded-size NV12 (cuv / ruv interleaved); returns
-    coef, modes (uint8 [nctu*16]), mv8 (MV [nctu*4]) and the recon as NV12
-    (y, uv)."""
+    coding (rounding 4); P -> +-rng full search on 8x8 blocks (at 10 bits on
+    the top 8 bits of each sample, v >> 2), quarter-sample refine with the
+    HEVC interpolation at the full bit depth, 8x8 inter CU coding (rounding
+    2); then, with deblock, the 8.7.2 filter (bS from intra / coded luma /
+    vector difference).  Planes are coded-size NV12 (cuv / ruv interleaved;
+    uint8 at bd 8, else uint16 samples); returns coef, modes (uint8
+    [nctu*16]), mv8 (MV [nctu*4]) and the recon as NV12 (y, uv)."""
     hc, wc = cy.shape
     t = (wc // 16) * (hc // 16)
+    pix = _pix(bd)
     cb, cr = split_uv(cuv)
     mv8 = np.zeros(t * 4, MV)
     if is_intra:
-        o = hevc_encode_i(cy, cb, cr, qp, num_slices, HEVC_ROUND_INTRA_I)
+        o = hevc_encode_i(cy, cb, cr, qp, num_slices, HEVC_ROUND_INTRA_I, bd)
         modes = o["modes"]
     else:
-        _, i8 = me_fullsearch(cy, ry, rng, lambda8)
+        s8 = (lambda a: np.asarray(a, np.uint8)) if bd == 8 else (lambda a: (np.asarray(a) >> (bd - 8)).astype(np.uint8))
+        _, i8 = me_fullsearch(s8(cy), s8(ry), rng, lambda8)
         mv = np.zeros((t * 4, 2), np.int32)
         for cu in range(t * 4):
             ctu, z = cu // 4, cu % 4
             x0, y0 = (ctu % (wc // 16)) * 16 + (z & 1) * 8, (ctu // (wc // 16)) * 16 + (z >> 1) * 8
-            mx, my, sad = hevc_refine8(cy, ry, x0, y0, int(i8[cu]["dx"]) * 4, int(i8[cu]["dy"]) * 4)
+            mx, my, sad = hevc_refine8(cy, ry, x0, y0, int(i8[cu]["dx"]) * 4, int(i8[cu]["dy"]) * 4, bd)
             mv[cu] = (mx, my)
             mv8[cu] = (mx, my, sad, 0)
         fb, fr = split_uv(ruv)
-        o = hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, HEVC_ROUND_INTER)
+        o = hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, HEVC_ROUND_INTER, bd)
         modes = np.zeros(t * 16, np.uint8)
-    uv = np.empty((hc // 2, wc), np.uint8)
+    uv = np.empty((hc // 2, wc), pix)
     uv[:, 0::2], uv[:, 1::2] = o["cb"], o["cr"]
     y = o["y"]
     if deblock:
         cbf = hevc_cu_cbf(o["coef"]) if not is_intra else np.zeros(t * 4, np.uint8)
         mv4 = np.zeros((t * 4, 4), np.int32)
         mv4[:, 0], mv4[:, 1] = mv8["mvx"], mv8["mvy"]
-        y, uv = hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices)
+        y, uv = hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices, bd)
     return dict(coef=o["coef"], modes=modes, mv8=mv8, y=y, uv=uv)
 
 
@@ -393,12 +396,18 @@ _sig("orc_hevc_tc", _I, [_I])
 _sig("orc_hevc_deblock", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I])
 
 
-def hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices=1):
-    """Returns deblocked copies of the NV12 planes; cbf uint8 [ncu], mv4 int32 [ncu][4]."""
-    y = np.ascontiguousarray(y, np.uint8).copy()
-    uv = np.ascontiguousarray(uv, np.uint8).copy()
+_sig("orc_hevc_deblock_bd", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I])
+
+
+def hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices=1, bd=8):
+    """Returns deblocked copies of the NV12 planes (uint8 at bd 8, else
+    uint16 holding the sample in the low bits); cbf uint8 [ncu], mv4 int32
+    [ncu][4]."""
+    t = np.uint8 if bd == 8 else np.uint16
+    y = np.ascontiguousarray(y, t).copy()
+    uv = np.ascontiguousarray(uv, t).copy()
     hc, wc = y.shape
     cbf = np.ascontiguousarray(cbf, np.uint8)
     mv4 = np.ascontiguousarray(mv4, np.int32).reshape(-1, 4)
-    lib.orc_hevc_deblock(p(y), p(uv), wc, wc, wc, hc, qp, int(is_intra), p(cbf), p(mv4), num_slices)
+    lib.orc_hevc_deblock_bd(p(y), p(uv), wc, wc, wc, hc, qp, int(is_intra), p(cbf), p(mv4), num_slices, bd)
     return y, uv

@@ -16,6 +16,7 @@
  * Planes are separate (Y, Cb, Cr), samples uint8 (8-bit) or uint16 (10-bit),
  * in coded size (width and height multiples of 16), row stride in samples.
  */
+#include <stddef.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -554,12 +555,24 @@ static const int kTc[54] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,  
 int orc_hevc_beta(int q) { return kBeta[clampi(q, 0, 51)]; }
 int orc_hevc_tc(int q) { return kTc[clampi(q, 0, 53)]; }
 
-/* Luma: 4 lines across one edge; px(k, i) addresses sample i on side p
- * (i = 0 nearest the edge) of line k, qx(k, i) the q side. */
-static void luma_segment(uint8_t *base, int along, int across, int beta, int tc)
+/* Sample access by linear index into a uint8 (bd 8) or uint16 (bd > 8) plane. */
+static int gs(const void *p, int bd, ptrdiff_t i) { return bd == 8 ? ((const uint8_t *)p)[i] : ((const uint16_t *)p)[i]; }
+static void ps(void *p, int bd, ptrdiff_t i, int v)
 {
-#define PX(k, i) base[(k) * along - ((i) + 1) * across]
-#define QX(k, i) base[(k) * along + (i) * across]
+    if (bd == 8) ((uint8_t *)p)[i] = (uint8_t)v;
+    else ((uint16_t *)p)[i] = (uint16_t)v;
+}
+
+/* Luma: 4 lines across one edge whose q0 sample of line 0 is at index q;
+ * PX(k, i) is sample i on side p (i = 0 nearest the edge) of line k, QX(k, i)
+ * the q side.  beta and tc are already scaled to the bit depth. */
+static void luma_segment(void *pl, int bd, ptrdiff_t q, ptrdiff_t along, ptrdiff_t across, int beta, int tc)
+{
+    const int maxv = (1 << bd) - 1;
+#define PX(k, i) gs(pl, bd, q + (k) * along - ((i) + 1) * across)
+#define QX(k, i) gs(pl, bd, q + (k) * along + (i) * across)
+#define SET_P(k, i, v) ps(pl, bd, q + (k) * along - ((i) + 1) * across, (v))
+#define SET_Q(k, i, v) ps(pl, bd, q + (k) * along + (i) * across, (v))
     const int dp0 = abs(PX(0, 2) - 2 * PX(0, 1) + PX(0, 0)), dp3 = abs(PX(3, 2) - 2 * PX(3, 1) + PX(3, 0));
     const int dq0 = abs(QX(0, 2) - 2 * QX(0, 1) + QX(0, 0)), dq3 = abs(QX(3, 2) - 2 * QX(3, 1) + QX(3, 0));
     const int d = dp0 + dq0 + dp3 + dq3;
@@ -576,34 +589,37 @@ static void luma_segment(uint8_t *base, int along, int across, int beta, int tc)
         const int p0 = PX(k, 0), p1 = PX(k, 1), p2 = PX(k, 2), p3 = PX(k, 3);
         const int q0 = QX(k, 0), q1 = QX(k, 1), q2 = QX(k, 2), q3 = QX(k, 3);
         if (strong) {
-            PX(k, 0) = (uint8_t)clampi((p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3, p0 - 2 * tc, p0 + 2 * tc);
-            PX(k, 1) = (uint8_t)clampi((p2 + p1 + p0 + q0 + 2) >> 2, p1 - 2 * tc, p1 + 2 * tc);
-            PX(k, 2) = (uint8_t)clampi((2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3, p2 - 2 * tc, p2 + 2 * tc);
-            QX(k, 0) = (uint8_t)clampi((p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3, q0 - 2 * tc, q0 + 2 * tc);
-            QX(k, 1) = (uint8_t)clampi((p0 + q0 + q1 + q2 + 2) >> 2, q1 - 2 * tc, q1 + 2 * tc);
-            QX(k, 2) = (uint8_t)clampi((p0 + q0 + q1 + 3 * q2 + 2 * q3 + 4) >> 3, q2 - 2 * tc, q2 + 2 * tc);
+            SET_P(k, 0, clampi((p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3, p0 - 2 * tc, p0 + 2 * tc));
+            SET_P(k, 1, clampi((p2 + p1 + p0 + q0 + 2) >> 2, p1 - 2 * tc, p1 + 2 * tc));
+            SET_P(k, 2, clampi((2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3, p2 - 2 * tc, p2 + 2 * tc));
+            SET_Q(k, 0, clampi((p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3, q0 - 2 * tc, q0 + 2 * tc));
+            SET_Q(k, 1, clampi((p0 + q0 + q1 + q2 + 2) >> 2, q1 - 2 * tc, q1 + 2 * tc));
+            SET_Q(k, 2, clampi((p0 + q0 + q1 + 3 * q2 + 2 * q3 + 4) >> 3, q2 - 2 * tc, q2 + 2 * tc));
         } else {
             int delta = (9 * (q0 - p0) - 3 * (q1 - p1) + 8) >> 4;
             if (abs(delta) >= tc * 10) continue;
             delta = clampi(delta, -tc, tc);
-            PX(k, 0) = (uint8_t)clampi(p0 + delta, 0, 255);
-            QX(k, 0) = (uint8_t)clampi(q0 - delta, 0, 255);
-            if (dEp) PX(k, 1) = (uint8_t)clampi(p1 + clampi((((p2 + p0 + 1) >> 1) - p1 + delta) >> 1, -(tc >> 1), tc >> 1), 0, 255);
-            if (dEq) QX(k, 1) = (uint8_t)clampi(q1 + clampi((((q2 + q0 + 1) >> 1) - q1 - delta) >> 1, -(tc >> 1), tc >> 1), 0, 255);
+            SET_P(k, 0, clampi(p0 + delta, 0, maxv));
+            SET_Q(k, 0, clampi(q0 - delta, 0, maxv));
+            if (dEp) SET_P(k, 1, clampi(p1 + clampi((((p2 + p0 + 1) >> 1) - p1 + delta) >> 1, -(tc >> 1), tc >> 1), 0, maxv));
+            if (dEq) SET_Q(k, 1, clampi(q1 + clampi((((q2 + q0 + 1) >> 1) - q1 - delta) >> 1, -(tc >> 1), tc >> 1), 0, maxv));
         }
     }
 #undef PX
 #undef QX
+#undef SET_P
+#undef SET_Q
 }
 
-static void chroma_segment(uint8_t *base, int along, int across, int tc)
+static void chroma_segment(void *pl, int bd, ptrdiff_t q, ptrdiff_t along, ptrdiff_t across, int tc)
 {
+    const int maxv = (1 << bd) - 1;
     for (int k = 0; k < 4; k++) {
-        uint8_t *q0 = base + k * along, *p0 = q0 - across;
-        const int P1 = p0[-across], P0 = *p0, Q0 = *q0, Q1 = q0[across];
+        const ptrdiff_t iq = q + k * along, ip = iq - across;
+        const int P1 = gs(pl, bd, ip - across), P0 = gs(pl, bd, ip), Q0 = gs(pl, bd, iq), Q1 = gs(pl, bd, iq + across);
         const int delta = clampi((((Q0 - P0) * 4) + P1 - Q1 + 4) >> 3, -tc, tc);
-        *p0 = (uint8_t)clampi(P0 + delta, 0, 255);
-        *q0 = (uint8_t)clampi(Q0 - delta, 0, 255);
+        ps(pl, bd, ip, clampi(P0 + delta, 0, maxv));
+        ps(pl, bd, iq, clampi(Q0 - delta, 0, maxv));
     }
 }
 
@@ -618,13 +634,15 @@ int orc_hevc_bs(int is_intra, const uint8_t *cbf, const int32_t *mv4, int cp, in
     return (abs(a[0] - b[0]) >= 4 || abs(a[1] - b[1]) >= 4) ? 1 : 0;
 }
 
-/* y: luma (pitch py), uv: interleaved Cb/Cr (pitch puv); mv4: {mvx, mvy, ., .}
- * per CU; cbf: per CU. */
-void orc_hevc_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
-                      const uint8_t *cbf, const int32_t *mv4, int num_slices)
+/* y: luma (pitch py samples), uv: interleaved Cb/Cr (pitch puv samples),
+ * uint8 at bd 8, else uint16 with the sample in the low bits; mv4: {mvx,
+ * mvy, ., .} per CU; cbf: per CU.  beta and tC scale by 2^(bd-8) (8.7.2.5.3,
+ * 8.7.2.5.5). */
+void orc_hevc_deblock_bd(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                         const uint8_t *cbf, const int32_t *mv4, int num_slices, int bd)
 {
     const int wctu = wc / 16, hctu = hc / 16;
-    const int beta = kBeta[clampi(qp, 0, 51)];
+    const int beta = kBeta[clampi(qp, 0, 51)] << (bd - 8);
     const int qpc = orc_hevc_chroma_qp(qp);
     for (int dir = 0; dir < 2; dir++) {       /* 0: vertical edges, 1: horizontal edges */
         for (int e = 8; e < (dir ? hc : wc); e += 8) {
@@ -634,24 +652,27 @@ void orc_hevc_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, 
                 const int cq = cu_at(x, yy, wctu), cp = cu_at(dir ? x : x - 1, dir ? yy - 1 : yy, wctu);
                 const int bs = orc_hevc_bs(is_intra, cbf, mv4, cp, cq);
                 if (!bs) continue;
-                const int tc = kTc[clampi(qp + 2 * (bs - 1), 0, 53)];
-                uint8_t *b = y + (size_t)yy * py + x;
-                luma_segment(b, dir ? 1 : py, dir ? py : 1, beta, tc);
+                const int tc = kTc[clampi(qp + 2 * (bs - 1), 0, 53)] << (bd - 8);
+                luma_segment(y, bd, (ptrdiff_t)yy * py + x, dir ? 1 : py, dir ? py : 1, beta, tc);
             }
             /* chroma: edges on the 8x8 chroma grid (every 16 luma), bS 2 only */
             if ((e & 15) == 0) {
-                const int ctc = kTc[clampi(qpc + 2, 0, 53)];
+                const int ctc = kTc[clampi(qpc + 2, 0, 53)] << (bd - 8);
                 for (int s = 0; s < (dir ? wc : hc) / 2; s += 4) {
                     const int cx = dir ? s : e / 2, cy = dir ? e / 2 : s;
                     const int lx = 2 * cx, ly = 2 * cy;
                     const int cq = cu_at(lx, ly, wctu), cp = cu_at(dir ? lx : lx - 1, dir ? ly - 1 : ly, wctu);
                     if (orc_hevc_bs(is_intra, cbf, mv4, cp, cq) != 2) continue;
-                    for (int comp = 0; comp < 2; comp++) {
-                        uint8_t *b = uv + (size_t)cy * puv + 2 * cx + comp;
-                        chroma_segment(b, dir ? 2 : puv, dir ? puv : 2, ctc);
-                    }
+                    for (int comp = 0; comp < 2; comp++)
+                        chroma_segment(uv, bd, (ptrdiff_t)cy * puv + 2 * cx + comp, dir ? 2 : puv, dir ? puv : 2, ctc);
                 }
             }
         }
     }
+}
+
+void orc_hevc_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                      const uint8_t *cbf, const int32_t *mv4, int num_slices)
+{
+    orc_hevc_deblock_bd(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mv4, num_slices, 8);
 }

@@ -2,7 +2,8 @@
  * the reference decoder's own HEVC deblocking filter (hevcd_filter_stage,
  * approach1-compute-encoder/src/hevc_filter.c:507, over
  * hevc_filter_template.c:299-412) on a picture of the pipeline's coding
- * shape, so the oracle's orc_hevc_deblock can be checked against it.
+ * shape, at 8 or 10 bits, so the oracle's orc_hevc_deblock_bd can be
+ * checked against it.
  *
  * Built only into _ref/ from the reference tree (oracle/Makefile, target
  * ref): it includes the reference decoder's hevc_dec_internal.h and fills
@@ -21,9 +22,10 @@
 
 #include "hevc_dec_internal.h"
 
-int ref_hevc_deblock(uint8_t *y, uint8_t *uv, int wc, int hc, int qp, int is_intra, const uint8_t *cbf,
-                     const int32_t *mv4, int num_slices)
+int ref_hevc_deblock(void *y, void *uv, int wc, int hc, int qp, int is_intra, const uint8_t *cbf,
+                     const int32_t *mv4, int num_slices, int bd)
 {
+    const size_t bps = bd > 8 ? 2 : 1;  /* bytes per sample; planes hold the sample in the low bits */
     const int wctu = wc / 16, hctu = hc / 16, w4 = wc / 4, h4 = hc / 4, w8 = wc / 8, h8 = hc / 8;
     hevc_sps_t *sps = calloc(1, sizeof *sps);
     hevc_pps_t *pps = calloc(1, sizeof *pps);
@@ -33,13 +35,13 @@ int ref_hevc_deblock(uint8_t *y, uint8_t *uv, int wc, int hc, int qp, int is_int
     int8_t *qpm = malloc((size_t)w8 * h8);
     int32_t *slice_of = malloc(sizeof(int32_t) * wctu * hctu);
     hevcd_slice_filter_t *sf = calloc((size_t)(num_slices > 0 ? num_slices : 1), sizeof *sf);
-    uint8_t *cb = malloc((size_t)(wc / 2) * (hc / 2)), *cr = malloc((size_t)(wc / 2) * (hc / 2));
+    uint8_t *cb = malloc((size_t)(wc / 2) * (hc / 2) * bps), *cr = malloc((size_t)(wc / 2) * (hc / 2) * bps);
     int rc = -1;
     if (!sps || !pps || !d || !mvf || !cbf4 || !edges || !qpm || !slice_of || !sf || !cb || !cr) goto out;
 
     sps->width = wc;
     sps->height = hc;
-    sps->bit_depth_luma = sps->bit_depth_chroma = 8;
+    sps->bit_depth_luma = sps->bit_depth_chroma = bd;
     sps->log2_min_cb = 3;
     sps->log2_ctb = 4;
     sps->ctb_size = 16;
@@ -78,10 +80,10 @@ int ref_hevc_deblock(uint8_t *y, uint8_t *uv, int wc, int hc, int qp, int is_int
             }
         }
     for (int j = 0; j < hc / 2; j++)
-        for (int i = 0; i < wc / 2; i++) {
-            cb[j * (wc / 2) + i] = uv[(size_t)j * wc + 2 * i];
-            cr[j * (wc / 2) + i] = uv[(size_t)j * wc + 2 * i + 1];
-        }
+        for (int i = 0; i < wc / 2; i++)
+            for (int c = 0; c < 2; c++)
+                memcpy((c ? cr : cb) + ((size_t)j * (wc / 2) + i) * bps,
+                       (uint8_t *)uv + ((size_t)j * wc + 2 * i + c) * bps, bps);
 
     d->sps = sps;
     d->pps = pps;
@@ -111,10 +113,10 @@ int ref_hevc_deblock(uint8_t *y, uint8_t *uv, int wc, int hc, int qp, int is_int
     for (int r = 0; r < hctu; r++) hevcd_filter_stage(d, 1, r);
 
     for (int j = 0; j < hc / 2; j++)
-        for (int i = 0; i < wc / 2; i++) {
-            uv[(size_t)j * wc + 2 * i] = cb[j * (wc / 2) + i];
-            uv[(size_t)j * wc + 2 * i + 1] = cr[j * (wc / 2) + i];
-        }
+        for (int i = 0; i < wc / 2; i++)
+            for (int c = 0; c < 2; c++)
+                memcpy((uint8_t *)uv + ((size_t)j * wc + 2 * i + c) * bps,
+                       (c ? cr : cb) + ((size_t)j * (wc / 2) + i) * bps, bps);
     rc = 0;
 out:
     free(sps); free(pps); free(d); free(mvf); free(cbf4); free(edges); free(qpm); free(slice_of); free(sf);

@@ -123,30 +123,54 @@ def _frames(pkg, seed, w, h, n, nrect=3):
     return out
 
 
-@pytest.mark.parametrize("w,h,rng", [(128, 96, 16), (320, 176, 32)])
-def test_refine8_stage(pkg, w, h, rng):
+def _ten(a, rs):
+    """10-bit samples from 8-bit ones as the 4K P010 workload makes them: 4 v +
+    uniform [0, 3]."""
+    return np.asarray(a, np.uint16) * 4 + rs.integers(0, 4, np.shape(a)).astype(np.uint16)
+
+
+def _p010(torch, a):
+    """Device copy of a plane of 10-bit samples in P010 form (sample << 6)."""
+    return dev(torch, (np.asarray(a, np.uint16) << 6).view(np.int16))
+
+
+def _unp010(t):
+    v = host(t).view(np.uint16)
+    assert not (v & 63).any()
+    return v >> 6
+
+
+@pytest.mark.parametrize("w,h,rng,bd", [(128, 96, 16, 8), (320, 176, 32, 8), (128, 96, 16, 10), (256, 160, 32, 10)])
+def test_refine8_stage(pkg, w, h, rng, bd):
     torch = _torch()
+    say(f"hevc refine8 {w}x{h} R={rng} bd={bd}")
     (cy, cuv), (ry, ruv) = _frames(pkg, 77, w, h, 2)
     cy, ry = ry, cy  # search frame 0 in frame 1's reconstruction stand-in
     hc, wc = cy.shape
     _, i8 = orc.me_fullsearch(cy, ry, rng, 1)
-    dcy, dry = dev(torch, cy), dev(torch, ry)
+    if bd == 10:
+        rs = np.random.default_rng(5)
+        cy, ry = _ten(cy, rs), _ten(ry, rs)
+        dcy, dry, pitch = _p010(torch, cy), _p010(torch, ry), wc * 2
+    else:
+        dcy, dry, pitch = dev(torch, cy), dev(torch, ry), wc
     di8 = dev(torch, i8.view(np.int32))
     dout = torch.zeros(i8.shape[0] * 4, dtype=torch.int32, device="cuda")
-    assert pkg.hip.bc250_hevc_launch_refine8(P(dcy), wc, P(dry), wc, wc, hc, P(di8), P(dout), None) == 0
+    assert pkg.hip.bc250_hevc_launch_refine8(P(dcy), pitch, P(dry), pitch, wc, hc, P(di8), P(dout), bd, None) == 0
     torch.cuda.synchronize()
     got = host(dout).reshape(-1, 4)
     for cu in range(i8.shape[0]):
         ctu, z = cu // 4, cu % 4
         x0 = (ctu % (wc // 16)) * 16 + (z & 1) * 8
         y0 = (ctu // (wc // 16)) * 16 + (z >> 1) * 8
-        mx, my, sad = orc.hevc_refine8(cy, ry, x0, y0, int(i8[cu]["dx"]) * 4, int(i8[cu]["dy"]) * 4)
+        mx, my, sad = orc.hevc_refine8(cy, ry, x0, y0, int(i8[cu]["dx"]) * 4, int(i8[cu]["dy"]) * 4, bd)
         assert (got[cu, 0], got[cu, 1], got[cu, 2]) == (mx, my, sad), cu
 
 
-@pytest.mark.parametrize("qp", [0, 22, 32, 45, 51])
-def test_p_encode_stage(pkg, qp):
+@pytest.mark.parametrize("qp,bd", [(0, 8), (22, 8), (32, 8), (45, 8), (51, 8), (0, 10), (27, 10), (51, 10)])
+def test_p_encode_stage(pkg, qp, bd):
     torch = _torch()
+    say(f"hevc P coding qp={qp} bd={bd}")
     w, h = 192, 112
     (cy, cuv), (ry, ruv) = _frames(pkg, 91 + qp, w, h, 2, 4)
     hc, wc = cy.shape
@@ -154,20 +178,28 @@ def test_p_encode_stage(pkg, qp):
     rs = np.random.default_rng(qp)
     mv = rs.integers(-70, 71, (t * 4, 2)).astype(np.int32)
     mv[::5] = 0
+    if bd == 10:
+        cy, cuv, ry, ruv = (_ten(a, rs) for a in (cy, cuv, ry, ruv))
     cb, cr = orc.split_uv(cuv)
     fb, fr = orc.split_uv(ruv)
-    want = orc.hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, orc.HEVC_ROUND_INTER)
+    want = orc.hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, orc.HEVC_ROUND_INTER, bd)
     mv4 = np.zeros((t * 4, 4), np.int32)
     mv4[:, :2] = mv
-    d = [dev(torch, a) for a in (cy, cuv, ry, ruv, mv4)]
+    if bd == 10:
+        d = [_p010(torch, a) for a in (cy, cuv, ry, ruv)] + [dev(torch, mv4)]
+        pitch, back = wc * 2, _unp010
+    else:
+        d = [dev(torch, a) for a in (cy, cuv, ry, ruv, mv4)]
+        pitch, back = wc, host
     oy, ouv = torch.zeros_like(d[0]), torch.zeros_like(d[1])
     coef = torch.zeros(t * 384, dtype=torch.int16, device="cuda")
-    assert pkg.hip.bc250_hevc_launch_p_encode(P(d[0]), P(d[1]), wc, wc, P(d[2]), P(d[3]), wc, wc, P(oy), P(ouv), wc,
-                                              wc, wc, hc, qp, orc.HEVC_ROUND_INTER, P(d[4]), P(coef), None) == 0
+    assert pkg.hip.bc250_hevc_launch_p_encode(P(d[0]), P(d[1]), pitch, pitch, P(d[2]), P(d[3]), pitch, pitch, P(oy),
+                                              P(ouv), pitch, pitch, wc, hc, qp, orc.HEVC_ROUND_INTER, P(d[4]), P(coef),
+                                              bd, None) == 0
     torch.cuda.synchronize()
     assert np.array_equal(host(coef), want["coef"])
-    assert np.array_equal(host(oy), want["y"])
-    gb, gr = orc.split_uv(host(ouv))
+    assert np.array_equal(back(oy), want["y"])
+    gb, gr = orc.split_uv(back(ouv))
     assert np.array_equal(gb, want["cb"]) and np.array_equal(gr, want["cr"])
     assert (want["coef"] != 0).any()
 
@@ -305,14 +337,45 @@ def test_deblock(pkg, w, h, qp, intra, ns):
     gy[:, :w], guv[:, :w] = y, uv
     d = [dev(torch, a) for a in (gy, guv, cbf, mv4)]
     assert pkg.hip.bc250_hevc_launch_deblock(P(d[0]), P(d[1]), pitch, pitch, w, h, qp, intra, P(d[2]), P(d[3]), ns,
-                                             None) == 0
+                                             8, None) == 0
     torch.cuda.synchronize()
     oy, ouv = host(d[0]), host(d[1])
     assert np.array_equal(oy[:, :w], wy)
     assert np.array_equal(ouv[:, :w], wuv)
     assert (oy[:, w:] == 7).all() and (ouv[:, w:] == 9).all()
     assert pkg.hip.bc250_hevc_launch_deblock(P(d[0]), P(d[1]), pitch, pitch, w + 8, h, qp, intra, P(d[2]), P(d[3]),
-                                             ns, None) == -1
+                                             ns, 8, None) == -1
+
+
+@pytest.mark.parametrize("w,h,qp,intra,ns", [(64, 48, 37, 1, 1), (192, 112, 30, 0, 2), (320, 240, 45, 1, 3),
+                                             (128, 96, 22, 0, 1)])
+def test_deblock_p010(pkg, w, h, qp, intra, ns):
+    """10-bit deblocking on P010 planes (sample << 6) against the oracle's
+    10-bit filter on plain samples."""
+    torch = _torch()
+    say(f"hevc deblock p010 {w}x{h} qp={qp} intra={intra} slices={ns}")
+    rs = np.random.default_rng(w * 5 + qp)
+    y = _blocky(rs, h, w, 12).astype(np.uint16) * 4 + rs.integers(0, 4, (h, w)).astype(np.uint16)
+    uv = _blocky(rs, h // 2, w, 10).astype(np.uint16) * 4 + rs.integers(0, 4, (h // 2, w)).astype(np.uint16)
+    ncu = (w // 8) * (h // 8)
+    cbf = (rs.random(ncu) < 0.3).astype(np.uint8)
+    mv4 = np.zeros((ncu, 4), np.int32)
+    mv4[:, :2] = rs.integers(-6, 7, (ncu, 2))
+    mv4[::3, :2] = 0
+    wy, wuv = orc.hevc_deblock(y, uv, qp, intra, cbf, mv4, ns, 10)
+    assert not np.array_equal(wy, y)
+    pitch = w + 32          # samples; a margin that must stay untouched
+    gy = np.full((h, pitch), 7 << 6, np.uint16)
+    guv = np.full((h // 2, pitch), 9 << 6, np.uint16)
+    gy[:, :w], guv[:, :w] = y << 6, uv << 6
+    d = [dev(torch, a) for a in (gy.view(np.int16), guv.view(np.int16), cbf, mv4)]
+    assert pkg.hip.bc250_hevc_launch_deblock(P(d[0]), P(d[1]), pitch * 2, pitch * 2, w, h, qp, intra, P(d[2]),
+                                             P(d[3]), ns, 10, None) == 0
+    torch.cuda.synchronize()
+    oy, ouv = host(d[0]).view(np.uint16), host(d[1]).view(np.uint16)
+    assert np.array_equal(oy[:, :w] >> 6, wy) and np.array_equal(ouv[:, :w] >> 6, wuv)
+    assert not (oy[:, :w] & 63).any() and not (ouv[:, :w] & 63).any()
+    assert (oy[:, w:] == 7 << 6).all() and (ouv[:, w:] == 9 << 6).all()
 
 
 def test_cu_cbf(pkg):

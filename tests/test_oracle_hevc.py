@@ -477,35 +477,39 @@ def _blocky_picture(rs, h, w, step, noise):
 def test_deblock_matches_reference_decoder():
     """The oracle's picture deblocking equals the reference decoder's own
     loop filter (hevcd_filter_stage, hevc_filter.c:507, run by the harness
-    oracle/ref_hevc_deblock.c) on seeded blocky pictures: I and P, every QP
-    band, coded-luma flags and vectors that give bS 0/1/2, CTU-row slices."""
+    oracle/ref_hevc_deblock.c) on seeded blocky pictures: 8 and 10 bits, I and
+    P, every QP band, coded-luma flags and vectors that give bS 0/1/2, CTU-row
+    slices."""
     lib = C.CDLL(orc.REF_HEVC_DEBLOCK, mode=os.RTLD_LAZY)
-    lib.ref_hevc_deblock.argtypes = [P, P, I, I, I, I, P, P, I]
+    lib.ref_hevc_deblock.argtypes = [P, P, I, I, I, I, P, P, I, I]
     lib.ref_hevc_deblock.restype = I
     rs = np.random.default_rng(11)
     cases = 0
     changed = 0
-    for (w, h) in ((64, 48), (96, 64), (160, 96)):
+    for (w, h, bd) in ((64, 48, 8), (96, 64, 8), (160, 96, 8), (96, 64, 10), (128, 80, 10)):
         for qp in (17, 22, 27, 30, 34, 37, 41, 45, 51):
             for intra in (0, 1):
                 for ns in (1, 2, 3):
                     step = int(rs.integers(2, 20))
                     y = _blocky_picture(rs, h, w, step, int(rs.integers(0, 3)))
                     uv = _blocky_picture(rs, h // 2, w, step // 2 + 1, 1)
+                    if bd == 10:   # 10-bit pictures: 4x the 8-bit one plus a dither
+                        y = y.astype(np.uint16) * 4 + rs.integers(0, 4, y.shape).astype(np.uint16)
+                        uv = uv.astype(np.uint16) * 4 + rs.integers(0, 4, uv.shape).astype(np.uint16)
                     ncu = (w // 8) * (h // 8)
                     cbf = (rs.random(ncu) < 0.4).astype(np.uint8)
                     mv4 = np.zeros((ncu, 4), np.int32)
                     mv4[:, :2] = rs.integers(-8, 9, (ncu, 2))
                     mv4[rs.random(ncu) < 0.3, :2] = 0
-                    wy, wuv = orc.hevc_deblock(y, uv, qp, intra, cbf, mv4, ns)
+                    wy, wuv = orc.hevc_deblock(y, uv, qp, intra, cbf, mv4, ns, bd)
                     ry, ruv = y.copy(), uv.copy()
                     assert lib.ref_hevc_deblock(ry.ctypes.data, ruv.ctypes.data, w, h, qp, intra,
-                                                cbf.ctypes.data, mv4.ctypes.data, ns) == 0
-                    assert np.array_equal(wy, ry), (w, h, qp, intra, ns)
-                    assert np.array_equal(wuv, ruv), (w, h, qp, intra, ns)
+                                                cbf.ctypes.data, mv4.ctypes.data, ns, bd) == 0
+                    assert np.array_equal(wy, ry), (w, h, bd, qp, intra, ns)
+                    assert np.array_equal(wuv, ruv), (w, h, bd, qp, intra, ns)
                     cases += 1
                     changed += int(not np.array_equal(wy, y)) + int(not np.array_equal(wuv, uv))
-    assert cases == 162 and changed > 150
+    assert cases == 270 and changed > 250
 
 
 def _two_columns(left, right, w=32, h=16):
