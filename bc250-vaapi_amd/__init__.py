@@ -88,6 +88,7 @@ _sig(hip, "bc250_gpu_destroy", None, [_CTX])
 _sig(hip, "gpu_compute_create_image", _I, [_CTX, _I, _I, _I, C.POINTER(GpuImage), C.POINTER(GpuMemory)])
 _sig(hip, "gpu_compute_destroy_image", None, [_CTX, GpuImage, GpuMemory])
 _sig(hip, "gpu_compute_upload_nv12", _I, [_CTX, C.POINTER(GpuImage), GpuMemory, _P, _I, _P, _I, _I, _I])
+_sig(hip, "gpu_compute_upload_p010", _I, [_CTX, C.POINTER(GpuImage), GpuMemory, _P, _I, _P, _I, _I, _I])
 _sig(hip, "gpu_compute_download_nv12", _I, [_CTX, C.POINTER(GpuImage), GpuMemory, _P, _I, _P, _I, _I, _I])
 _sig(hip, "gpu_compute_begin_picture", _I, [_CTX, GpuImage])
 _sig(hip, "gpu_compute_dispatch_encode_ext", _I, [_CTX, GpuImage, _I, _I, _I, _I, _I, _I, _P])
@@ -137,11 +138,12 @@ _sig(hip, "bc250_hevc_mc_uni_batch", _I, [_P, _I, _I, _I, _P, _I, _I, _I, _I, _I
 _sig(hip, "bc250_hevc_launch_refine8", _I, [_P, _I, _P, _I, _I, _I, _P, _P, _I, _P])
 _sig(hip, "bc250_hevc_intra_work_bytes", C.c_size_t, [_I, _I])
 _sig(hip, "bc250_hevc_launch_i_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _U,
-                                            _P])
+                                            _I, _P])
 _sig(hip, "bc250_hevc_launch_p_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P,
                                             _P, _I, _P])
 _sig(hip, "bc250_hevc_launch_deblock", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P])
 _sig(hip, "bc250_hevc_cu_cbf", _I, [_P, _I, _P, _P])
+_sig(hip, "bc250_hevc_p010_to_8bit", _I, [_P, _I, _P, _I, _I, _I, _P])
 
 _sig(host, "bc250_gen_create", _P, [C.c_uint64, _I, _I, _I])
 _sig(host, "bc250_gen_destroy", None, [_P])
@@ -191,8 +193,9 @@ class FrameGen:
 
 
 def pad_coded(y, uv):
-    """Visible NV12 planes -> coded-size planes (16-aligned) with the last
-    row/column replicated, which is what an upload leaves on the device."""
+    """Visible NV12 planes (uint8, or uint16 10-bit samples) -> coded-size
+    planes (16-aligned) with the last row/column replicated, which is what an
+    upload leaves on the device."""
     h, w = y.shape
     wc, hc = (w + 15) // 16 * 16, (h + 15) // 16 * 16
     yc = np.pad(y, ((0, hc - h), (0, wc - w)), mode="edge")
@@ -200,7 +203,7 @@ def pad_coded(y, uv):
     v = uv[:, 1::2]
     u = np.pad(u, ((0, hc // 2 - u.shape[0]), (0, wc // 2 - u.shape[1])), mode="edge")
     v = np.pad(v, ((0, hc // 2 - v.shape[0]), (0, wc // 2 - v.shape[1])), mode="edge")
-    uvc = np.empty((hc // 2, wc), np.uint8)
+    uvc = np.empty((hc // 2, wc), y.dtype)
     uvc[:, 0::2] = u
     uvc[:, 1::2] = v
     return np.ascontiguousarray(yc), uvc
@@ -211,8 +214,12 @@ def pad_coded(y, uv):
 class Encoder:
     """One gpu_compute context: upload frames, encode, read staging back."""
 
-    def __init__(self, width, height, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0):
+    def __init__(self, width, height, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0,
+                 bit_depth=8):
+        """bit_depth 10: P010 surfaces (HEVC Main10); upload() then takes uint16
+        planes of 0..1023 samples and map_recon() returns them."""
         self.w, self.h = width, height
+        self.fmt = GPU_IMAGE_P010 if bit_depth == 10 else GPU_IMAGE_NV12
         self.wc, self.hc = (width + 15) // 16 * 16, (height + 15) // 16 * 16
         self.nmb = (self.wc // 16) * (self.hc // 16)
         self.ctx = GpuContext()
@@ -230,7 +237,7 @@ class Encoder:
 
     def new_image(self):
         img, mem = GpuImage(), GpuMemory()
-        if hip.gpu_compute_create_image(C.byref(self.ctx), self.w, self.h, GPU_IMAGE_NV12, C.byref(img),
+        if hip.gpu_compute_create_image(C.byref(self.ctx), self.w, self.h, self.fmt, C.byref(img),
                                         C.byref(mem)) != 0:
             raise RuntimeError("create_image failed")
         self.images.append((img, mem))
@@ -238,10 +245,16 @@ class Encoder:
 
     def upload(self, img_mem, y, uv):
         img, mem = img_mem
-        y = np.ascontiguousarray(y)
-        uv = np.ascontiguousarray(uv)
-        if hip.gpu_compute_upload_nv12(C.byref(self.ctx), C.byref(img), mem, _ptr(y), y.strides[0], _ptr(uv),
-                                       uv.strides[0], self.w, self.h) != 0:
+        if self.fmt == GPU_IMAGE_P010:
+            y = np.ascontiguousarray(y, np.uint16)
+            uv = np.ascontiguousarray(uv, np.uint16)
+            f = hip.gpu_compute_upload_p010
+        else:
+            y = np.ascontiguousarray(y)
+            uv = np.ascontiguousarray(uv)
+            f = hip.gpu_compute_upload_nv12
+        if f(C.byref(self.ctx), C.byref(img), mem, _ptr(y), y.strides[0], _ptr(uv), uv.strides[0], self.w,
+             self.h) != 0:
             raise RuntimeError("upload failed")
 
     def encode(self, img_mem, qp, is_intra, num_slices=1, cpu_mvs=None):
@@ -289,7 +302,7 @@ class Encoder:
         """Reads the newest reconstruction the way the reference's CPU ME route
         does (encoder_h264.c:2325-2378): cpu_read_safe, then map_surface on
         ctx.recon_image / ctx.recon_memory, then unmap.  Returns coded-size
-        (y, uv) copies."""
+        (y, uv) copies (P010: uint16 0..1023 samples)."""
         import numpy as np
         c = C.byref(self.ctx)
         if hip.gpu_compute_cpu_read_safe(c) != 1:
@@ -303,13 +316,19 @@ class Encoder:
             raise RuntimeError("map_surface failed")
         try:
             im = self.ctx.recon_image
+            bps = 2 if im.format == GPU_IMAGE_P010 else 1
             raw = np.ctypeslib.as_array((C.c_uint8 * lay.total_size).from_address(base))
             y = raw[lay.y_offset:lay.y_offset + lay.y_pitch * im.coded_height].reshape(
-                im.coded_height, lay.y_pitch)[:, :im.coded_width].copy()
+                im.coded_height, lay.y_pitch)[:, :im.coded_width * bps].copy()
             uv = raw[lay.uv_offset:lay.uv_offset + lay.uv_pitch * (im.coded_height // 2)].reshape(
-                im.coded_height // 2, lay.uv_pitch)[:, :im.coded_width].copy()
+                im.coded_height // 2, lay.uv_pitch)[:, :im.coded_width * bps].copy()
         finally:
             hip.gpu_compute_unmap_surface(c, mem, unmap)
+        if bps == 2:   # P010 words -> 0..1023 samples
+            y, uv = y.view(np.uint16), uv.view(np.uint16)
+            if (y & 63).any() or (uv & 63).any():
+                raise RuntimeError("P010 reconstruction has bits below the sample")
+            y, uv = y >> 6, uv >> 6
         return y, uv
 
     def stage_times(self, slot):

@@ -52,7 +52,10 @@ struct bc250_hip_state {
     hipStream_t stream = nullptr;
     // resolution-dependent state
     int w = 0, h = 0, wc = 0, hc = 0, mbw = 0, mbh = 0, nmb = 0;
+    int bps = 1;           // bytes per sample of the reconstructions: 1 (NV12) or 2 (P010)
     Plane2 recon[2];
+    uint8_t *me8 = nullptr;  // P010 pictures: 8-bit luma of the current and reference frames for the search
+    int me8_pitch = 0;
     int ref_idx = 0;
     bool has_recon = false;
     int16_t *levels = nullptr;
@@ -111,26 +114,34 @@ __global__ void pad_plane_kernel(uint8_t *p, int pitch, int w, int h, int wc, in
     p[(size_t)y * pitch + xb] = p[(size_t)sy * pitch + sx];
 }
 
-extern "C" int bc250_launch_pad_edges(uint8_t *y, uint8_t *uv, int py, int puv, int w, int h, int wc, int hc,
-                                      void *stream)
+// bps: bytes per sample (1 NV12, 2 P010); the replicated unit is one luma
+// sample and one Cb/Cr pair.
+static int pad_edges(uint8_t *y, uint8_t *uv, int py, int puv, int w, int h, int wc, int hc, int bps,
+                     hipStream_t st)
 {
-    hipStream_t st = (hipStream_t)stream;
     if (w == wc && h == hc) return 0;
     // columns first within visible rows, then whole rows: a single pass reads
     // a source that is either visible or itself padded on an earlier row, so
     // run rows in two launches: visible rows, then the rows below.
     dim3 b(256);
-    const int cwb = 2 * ((w + 1) / 2), ch = (h + 1) / 2;  // visible chroma bytes per row / rows
+    const int wb = w * bps, wcb = wc * bps;
+    const int cwb = 2 * ((w + 1) / 2) * bps, ch = (h + 1) / 2;  // visible chroma bytes per row / rows
     if (w < wc) {
-        pad_plane_kernel<<<dim3((wc + 255) / 256, h), b, 0, st>>>(y, py, w, h, wc, h, 1);
-        if (cwb < wc) pad_plane_kernel<<<dim3((wc + 255) / 256, ch), b, 0, st>>>(uv, puv, cwb, ch, wc, ch, 2);
+        pad_plane_kernel<<<dim3((wcb + 255) / 256, h), b, 0, st>>>(y, py, wb, h, wcb, h, bps);
+        if (cwb < wcb) pad_plane_kernel<<<dim3((wcb + 255) / 256, ch), b, 0, st>>>(uv, puv, cwb, ch, wcb, ch, 2 * bps);
     }
     if (h < hc) {
-        pad_plane_kernel<<<dim3((wc + 255) / 256, hc), b, 0, st>>>(y, py, wc, h, wc, hc, 1);
+        pad_plane_kernel<<<dim3((wcb + 255) / 256, hc), b, 0, st>>>(y, py, wcb, h, wcb, hc, bps);
         if (ch < hc / 2)
-            pad_plane_kernel<<<dim3((wc + 255) / 256, hc / 2), b, 0, st>>>(uv, puv, wc, ch, wc, hc / 2, 2);
+            pad_plane_kernel<<<dim3((wcb + 255) / 256, hc / 2), b, 0, st>>>(uv, puv, wcb, ch, wcb, hc / 2, 2 * bps);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_launch_pad_edges(uint8_t *y, uint8_t *uv, int py, int puv, int w, int h, int wc, int hc,
+                                      void *stream)
+{
+    return pad_edges(y, uv, py, puv, w, h, wc, hc, 1, (hipStream_t)stream);
 }
 
 // ------------------------------------------------------------ buffers
@@ -142,6 +153,8 @@ static void free_res(bc250_hip_state *s)
         if (r.base) hipFree(r.base);
         r = Plane2{};
     }
+    if (s->me8) hipFree(s->me8);
+    s->me8 = nullptr;
     void *dev[] = {s->levels, s->dc,     s->nz,     s->modes,  s->mvs,   s->mv8,
                    s->imv16,  s->imv8, s->sync_i, s->sync_d, s->sync_h, s->hmodes, s->hcbf};
     for (void *p : dev)
@@ -161,9 +174,9 @@ static void free_res(bc250_hip_state *s)
     s->has_recon = false;
 }
 
-static int alloc_plane2(Plane2 *p, int wc, int hc)
+static int alloc_plane2(Plane2 *p, int wc, int hc, int bps)
 {
-    p->py = align_up(wc, 256);
+    p->py = align_up(wc * bps, 256);
     p->puv = p->py;
     size_t ysz = (size_t)p->py * hc, uvsz = (size_t)p->puv * (hc / 2);
     if (!HIP_OK(hipMalloc(&p->base, ysz + uvsz))) return -1;
@@ -174,13 +187,15 @@ static int alloc_plane2(Plane2 *p, int wc, int hc)
 
 // Lazily (re)allocates everything for a resolution, like
 // allocate_encoding_buffers() at gpu_compute.c:233-532.
-static int ensure_res(bc250_hip_state *s, int w, int h)
+static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
 {
-    if (s->w == w && s->h == h && s->levels) return 0;
+    if (s->w == w && s->h == h && s->bps == bps && s->levels) return 0;
     free_res(s);
     const int wc = align_up(w, 16), hc = align_up(h, 16);
     const int mbw = wc / 16, mbh = hc / 16, nmb = mbw * mbh;
-    bool ok = alloc_plane2(&s->recon[0], wc, hc) == 0 && alloc_plane2(&s->recon[1], wc, hc) == 0;
+    bool ok = alloc_plane2(&s->recon[0], wc, hc, bps) == 0 && alloc_plane2(&s->recon[1], wc, hc, bps) == 0;
+    s->me8_pitch = align_up(wc, 256);
+    if (bps == 2) ok = ok && HIP_OK(hipMalloc(&s->me8, (size_t)s->me8_pitch * hc * 2));
     ok = ok && HIP_OK(hipMalloc(&s->levels, (size_t)nmb * 384 * 2)) && HIP_OK(hipMalloc(&s->dc, (size_t)nmb * 24 * 4)) &&
          HIP_OK(hipMalloc(&s->nz, (size_t)nmb * 24 * 4)) && HIP_OK(hipMalloc(&s->modes, (size_t)nmb * 4)) &&
          HIP_OK(hipMalloc(&s->mvs, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->mv8, (size_t)nmb * 64)) &&
@@ -204,7 +219,7 @@ static int ensure_res(bc250_hip_state *s, int w, int h)
         free_res(s);
         return -1;
     }
-    s->w = w; s->h = h; s->wc = wc; s->hc = hc; s->mbw = mbw; s->mbh = mbh; s->nmb = nmb;
+    s->w = w; s->h = h; s->wc = wc; s->hc = hc; s->mbw = mbw; s->mbh = mbh; s->nmb = nmb; s->bps = bps;
     s->has_recon = false;
     s->ref_idx = 0;
     return 0;
@@ -225,7 +240,7 @@ static void publish_recon(gpu_context_t *ctx, bc250_hip_state *s)
     im.y_pitch = (uint32_t)r.py;
     im.uv_pitch = (uint32_t)r.puv;
     im.current_layout = 0;
-    im.format = GPU_IMAGE_NV12;
+    im.format = s->bps == 2 ? GPU_IMAGE_P010 : GPU_IMAGE_NV12;
     ctx->recon_memory.memory = r.base;
     ctx->recon_memory.size = (size_t)r.py * s->hc + (size_t)r.puv * (s->hc / 2);
     ctx->recon_memory.mapped_ptr = nullptr;
@@ -364,8 +379,8 @@ extern "C" void gpu_compute_unmap_surface(gpu_context_t *ctx, gpu_memory_t memor
     if (!s || !unmap || !s->map_host || !memory.memory) return;
     hipMemcpyAsync(memory.memory, s->map_host, memory.size, hipMemcpyHostToDevice, s->stream);
     const gpu_image_t &im = s->map_image;
-    bc250_launch_pad_edges((uint8_t *)im.y_plane, (uint8_t *)im.uv_plane, im.y_pitch, im.uv_pitch, im.width,
-                           im.height, im.coded_width, im.coded_height, s->stream);
+    pad_edges((uint8_t *)im.y_plane, (uint8_t *)im.uv_plane, im.y_pitch, im.uv_pitch, im.width, im.height,
+              im.coded_width, im.coded_height, im.format == GPU_IMAGE_P010 ? 2 : 1, s->stream);
     hipStreamSynchronize(s->stream);
 }
 
@@ -382,10 +397,9 @@ extern "C" int gpu_compute_upload_nv12(gpu_context_t *ctx, gpu_image_t *image, g
               HIP_OK(hipMemcpy2DAsync(image->uv_plane, image->uv_pitch, uv_plane, uv_pitch,
                                       (size_t)((width + 1) / 2) * 2 * bpp, (height + 1) / 2, hipMemcpyHostToDevice,
                                       s->stream));
-    if (ok && bpp == 1)
-        ok = bc250_launch_pad_edges((uint8_t *)image->y_plane, (uint8_t *)image->uv_plane, image->y_pitch,
-                                    image->uv_pitch, width, height, image->coded_width, image->coded_height,
-                                    s->stream) == 0;
+    if (ok)
+        ok = pad_edges((uint8_t *)image->y_plane, (uint8_t *)image->uv_plane, image->y_pitch, image->uv_pitch, width,
+                       height, image->coded_width, image->coded_height, bpp, s->stream) == 0;
     ok = ok && HIP_OK(hipStreamSynchronize(s->stream));
     return ok ? 0 : -1;
 }
@@ -502,14 +516,28 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
     hipStream_t st = s->stream;
     const gpu_encode_params_t &p = s->params;
     const size_t n = (size_t)s->nmb;
+    const int bd = s->bps == 2 ? 10 : 8;
     int r = 0;
     hipEventRecord(sl.ev[0], st);
-    if (!is_intra)
-        r = bc250_launch_me_fullsearch(cur.y, cur.py, ref.y, ref.py, s->wc, s->hc, p.search_range, p.lambda8,
-                                       s->imv16, s->imv8, st);
+    if (!is_intra) {
+        // the integer search runs on 8-bit luma: at 10 bits the top 8 bits of each sample
+        const uint8_t *c8 = cur.y, *r8 = ref.y;
+        int cp8 = cur.py, rp8 = ref.py;
+        if (bd == 10) {
+            uint8_t *m0 = s->me8, *m1 = s->me8 + (size_t)s->me8_pitch * s->hc;
+            r = bc250_hevc_p010_to_8bit(cur.y, cur.py, m0, s->me8_pitch, s->wc, s->hc, st);
+            if (r == 0) r = bc250_hevc_p010_to_8bit(ref.y, ref.py, m1, s->me8_pitch, s->wc, s->hc, st);
+            c8 = m0;
+            r8 = m1;
+            cp8 = rp8 = s->me8_pitch;
+        }
+        if (r == 0)
+            r = bc250_launch_me_fullsearch(c8, cp8, r8, rp8, s->wc, s->hc, p.search_range, p.lambda8, s->imv16,
+                                           s->imv8, st);
+    }
     hipEventRecord(sl.ev[1], st);
     if (r == 0 && !is_intra)
-        r = bc250_hevc_launch_refine8(cur.y, cur.py, ref.y, ref.py, s->wc, s->hc, s->imv8, s->mv8, 8, st);
+        r = bc250_hevc_launch_refine8(cur.y, cur.py, ref.y, ref.py, s->wc, s->hc, s->imv8, s->mv8, bd, st);
     hipEventRecord(sl.ev[2], st);
     if (r == 0) {
         if (is_intra) {
@@ -517,12 +545,12 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
             if (r == 0)
                 r = bc250_hevc_launch_i_encode(cur.y, cur.uv, cur.py, cur.puv, out.y, out.uv, out.py, out.puv, s->wc,
                                                s->hc, qp, num_slices, 4 /* rounding of intra CUs in an I picture */,
-                                               s->hmodes, (int16_t *)s->levels, s->sync_h, kSpinLimit, st);
+                                               s->hmodes, (int16_t *)s->levels, s->sync_h, kSpinLimit, bd, st);
             if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, (int)n * 16, st);
         } else {
             r = bc250_hevc_launch_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv, out.y,
                                            out.uv, out.py, out.puv, s->wc, s->hc, qp, 2 /* inter rounding */, s->mv8,
-                                           (int16_t *)s->levels, 8, st);
+                                           (int16_t *)s->levels, bd, st);
             if (r == 0) r = bc250_launch_zero_i32((int *)s->hmodes, (int)n * 4, st);
         }
     }
@@ -533,7 +561,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
         if (!is_intra) r = bc250_hevc_cu_cbf((const int16_t *)s->levels, (int)n, s->hcbf, st);
         if (r == 0)
             r = bc250_hevc_launch_deblock(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->hcbf,
-                                          s->mv8, num_slices, 8, st);
+                                          s->mv8, num_slices, bd, st);
     }
     hipEventRecord(sl.ev[4], st);
     if (r == 0 && p.readback) {
@@ -557,8 +585,10 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     bc250_hip_state *s = st_of(ctx);
     if (!s || !rt.y_plane || qp < 0 || qp > 51) return -1;
     if (num_slices < 1) num_slices = 1;
-    if (rt.format != GPU_IMAGE_NV12) return -1;  // 8-bit pipeline
-    if (ensure_res(s, width, height) != 0) return -1;
+    const gpu_encode_params_t &p0 = s->params;
+    // 10-bit (P010) pictures: the HEVC Main10 pipeline only
+    if (rt.format != GPU_IMAGE_NV12 && !(rt.format == GPU_IMAGE_P010 && p0.codec == GPU_CODEC_HEVC)) return -1;
+    if (ensure_res(s, width, height, rt.format == GPU_IMAGE_P010 ? 2 : 1) != 0) return -1;
     if (!s->has_recon) {  // new resolution: no reconstruction yet
         ctx->recon_image = gpu_image_t{};
         ctx->recon_memory = gpu_memory_t{};
@@ -669,7 +699,7 @@ extern "C" int gpu_compute_dispatch_me_only(gpu_context_t *ctx, gpu_image_t rt, 
 {
     bc250_hip_state *s = st_of(ctx);
     if (!s || !rt.y_plane || rt.format != GPU_IMAGE_NV12) return -1;
-    if (ensure_res(s, width, height) != 0) return -1;
+    if (ensure_res(s, width, height, 1) != 0) return -1;
     if (!s->has_recon) {  // new resolution: no reconstruction yet
         ctx->recon_image = gpu_image_t{};
         ctx->recon_memory = gpu_memory_t{};

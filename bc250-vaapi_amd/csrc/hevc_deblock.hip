@@ -95,34 +95,6 @@ BC_DEV void chroma_line(int p1, int &p0, int &q0, int q1, int tc, int maxv)
     q0 = clampi(q0 - delta, 0, maxv);
 }
 
-// Four consecutive samples of a row: one dword (8-bit) or two (P010, where
-// the sample is word >> 6).
-template <typename Pix>
-BC_DEV void ld4(const uint8_t *row, int i, int v[4])
-{
-    if constexpr (sizeof(Pix) == 1) {
-        const uint32_t w = *(const uint32_t *)(row + i);
-#pragma unroll
-        for (int k = 0; k < 4; k++) v[k] = (w >> (8 * k)) & 255;
-    } else {
-        const uint2 w = *(const uint2 *)(row + 2 * i);
-        v[0] = (w.x & 0xffffu) >> 6;
-        v[1] = w.x >> 22;
-        v[2] = (w.y & 0xffffu) >> 6;
-        v[3] = w.y >> 22;
-    }
-}
-
-template <typename Pix>
-BC_DEV void st4(uint8_t *row, int i, const int v[4])
-{
-    if constexpr (sizeof(Pix) == 1) {
-        *(uint32_t *)(row + i) = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
-    } else {
-        *(uint2 *)(row + 2 * i) = make_uint2(((uint32_t)v[0] << 6) | ((uint32_t)v[1] << 22),
-                                             ((uint32_t)v[2] << 6) | ((uint32_t)v[3] << 22));
-    }
-}
 }  // namespace
 
 // VERT: vertical edges (filtered along x); otherwise horizontal edges.

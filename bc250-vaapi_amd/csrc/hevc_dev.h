@@ -95,4 +95,33 @@ BC_DEV void stpix(uint8_t *row, int x, int v)
     else ((uint16_t *)row)[x] = (uint16_t)(v << 6);
 }
 
+// Four consecutive samples x .. x+3 of a row (x a multiple of 4): one dword
+// (8-bit) or two (P010, where the sample is word >> 6).
+template <typename Pix>
+BC_DEV void ld4(const uint8_t *row, int x, int v[4])
+{
+    if constexpr (sizeof(Pix) == 1) {
+        const uint32_t w = *(const uint32_t *)(row + x);
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = (w >> (8 * k)) & 255;
+    } else {
+        const uint2 w = *(const uint2 *)(row + 2 * x);
+        v[0] = (w.x & 0xffffu) >> 6;
+        v[1] = w.x >> 22;
+        v[2] = (w.y & 0xffffu) >> 6;
+        v[3] = w.y >> 22;
+    }
+}
+
+template <typename Pix>
+BC_DEV void st4(uint8_t *row, int x, const int v[4])
+{
+    if constexpr (sizeof(Pix) == 1) {
+        *(uint32_t *)(row + x) = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
+    } else {
+        *(uint2 *)(row + 2 * x) = make_uint2(((uint32_t)v[0] << 6) | ((uint32_t)v[1] << 22),
+                                             ((uint32_t)v[2] << 6) | ((uint32_t)v[3] << 22));
+    }
+}
+
 }  // namespace hevc

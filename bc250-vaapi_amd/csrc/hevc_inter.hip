@@ -453,6 +453,28 @@ extern "C" int bc250_hevc_dqitn_batch(const int16_t *coef, int n, int log2n, int
 
 static int grid_xcd(int items) { return ((items + 7) / 8) * 8; }
 
+// P010 luma -> 8-bit luma holding the top 8 bits of each 10-bit sample (the
+// high byte of the P010 word), four samples per thread.
+__global__ __launch_bounds__(256) void hevc_p010_hi8_kernel(const uint8_t *__restrict__ src, int sp,
+                                                            uint8_t *__restrict__ dst, int dp, int w4, int h)
+{
+    const int x4 = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x4 >= w4 || y >= h) return;
+    const uint2 v = *(const uint2 *)(src + (size_t)y * sp + (size_t)x4 * 8);
+    *(uint32_t *)(dst + (size_t)y * dp + (size_t)x4 * 4) =
+        ((v.x >> 8) & 0xffu) | ((v.x >> 24) << 8) | (((v.y >> 8) & 0xffu) << 16) | ((v.y >> 24) << 24);
+}
+
+extern "C" int bc250_hevc_p010_to_8bit(const void *src, int spitch, uint8_t *dst, int dpitch, int w, int h,
+                                       void *stream)
+{
+    if (w <= 0 || h <= 0 || (w & 3) || (spitch & 7) || (dpitch & 3) || spitch < 2 * w || dpitch < w) return -1;
+    const int w4 = w / 4;
+    hevc_p010_hi8_kernel<<<dim3((w4 + 255) / 256, h), 256, 0, (hipStream_t)stream>>>((const uint8_t *)src, spitch, dst,
+                                                                                       dpitch, w4, h);
+    return launch_ok();
+}
+
 extern "C" int bc250_hevc_launch_refine8(const void *cur, int cpitch, const void *ref, int rpitch, int wc, int hc,
                                          const uint32_t *imv8, uint32_t *out, int bd, void *stream)
 {

@@ -16,11 +16,14 @@
 // lanes 16..31 / 32..47 the Cb / Cr block of the CU on its first step.  A
 // CTU needs the bottom rows of CTUs x-1, x, x+1 of the row above (corner,
 // top, top-right); each finished CTU hands its bottom luma / Cb / Cr row (32
-// bytes) to the wave below through an LDS ring, or to the next workgroup as
-// system-coherent 64-bit records {tag = x + 1, 4 samples} that the reader
-// polls until every tag matches (cleared before the launch).  Rows that
-// start a slice read nothing above and wait for nothing.  Waits are bounded
-// and report on *err.
+// samples) to the wave below through an LDS ring, or to the next workgroup as
+// system-coherent 64-bit records {tag = x + 1, one dword of samples} that the
+// reader polls until every tag matches (cleared before the launch).  Rows
+// that start a slice read nothing above and wait for nothing.  Waits are
+// bounded and report on *err.
+//
+// 8-bit NV12 or 10-bit P010 frames (the sample in the high 10 bits of each
+// 16-bit word); inside the kernel samples are plain values of type Pix.
 #include "bc250_hevc.h"
 #include "hevc_dev.h"
 
@@ -29,22 +32,23 @@ using namespace hevc;
 namespace {
 constexpr int WAVES = 4;
 constexpr int RING = 16;
-constexpr int REC = 8;   // dwords per CTU record: luma row 15 (16 B), Cb row 7 (8 B), Cr row 7 (8 B)
+constexpr int REC_MAX = 16;  // dwords per CTU record at 16 bits per sample
 // Tiles: row 0 = the row above (col 3 corner, then the CTU's width, then 4
 // top-right samples), rows 1.. = the CTU with col 3 the left neighbour column.
 constexpr int LP = 24;   // luma tile pitch (cols 3..23), 17 rows
 constexpr int CPT = 16;  // chroma tile pitch (cols 3..15), 9 rows
 
-// LDS bytes of one wave, addressed by offset so a lane can pick its plane.
+// LDS samples of one wave, addressed by offset so a lane can pick its plane.
 constexpr int O_TY = 0;                    // luma tile 17 x LP
 constexpr int O_TC = O_TY + 17 * LP;       // Cb, Cr tiles 9 x CPT each
 constexpr int O_AY = O_TC + 2 * 9 * CPT;   // records of the row above, luma: 4 CTU slots x 16
 constexpr int O_AC = O_AY + 4 * 16;        // records of the row above, Cb / Cr: 2 x 4 slots x 8
 constexpr int O_SRC = O_AC + 2 * 4 * 8;    // source CTU: luma 256, Cb 64, Cr 64
-constexpr int WAVE_BYTES = O_SRC + 384;
+constexpr int WAVE_SAMPLES = O_SRC + 384;
 
+template <typename Pix>
 struct alignas(16) WaveLds {
-    uint8_t b[WAVE_BYTES];
+    Pix b[WAVE_SAMPLES];
     int res[64], tmp[64];
 };
 
@@ -86,12 +90,17 @@ __global__ void hevc_clear_records_kernel(uint64_t *rec, int n)
     if (i < n) rec[i] = 0;
 }
 
+template <typename Pix, int BD>
 __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
     const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, uint8_t *__restrict__ oy,
     uint8_t *__restrict__ ouv, int opy, int opuv, int wc, int hc, int qp, int num_slices, int round_q12,
     uint8_t *__restrict__ modes, int16_t *__restrict__ coef, int *sync, uint64_t *ghand, unsigned spin_limit)
 {
-    __shared__ WaveLds wl[WAVES];
+    // samples per dword; record dwords: luma row (LD), Cb row, Cr row (CD each)
+    constexpr int SPD = 4 / (int)sizeof(Pix), LD = 16 / SPD, CD = 8 / SPD, REC = LD + 2 * CD;
+    constexpr int MAXV = (1 << BD) - 1, QOFF = 6 * (BD - 8);
+    constexpr int S1 = BD == 8 ? 1 : 3, QSH = BD - 3, S2INV = 20 - BD;
+    __shared__ WaveLds<Pix> wl[WAVES];
     __shared__ uint32_t ring[WAVES - 1][RING][REC];
     __shared__ int prog[WAVES], cons[WAVES];
     __shared__ int s_group;
@@ -105,7 +114,7 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
     const int yrow = s_group * WAVES + wv;
     if (yrow >= htu) return;
     volatile int *vprog = prog, *vcons = cons;
-    uint8_t *B = wl[wv].b;
+    Pix *B = wl[wv].b;
     int *RES = wl[wv].res, *TMP = wl[wv].tmp;
 
     const int r0 = slice_first_row(yrow, htu, num_slices);
@@ -131,26 +140,26 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
         mci[t] = M[t * 4 + pi];
         mcj[t] = M[t * 4 + pj];
     }
-    const int cqp = hevc::chroma_qp(qp);
-    const Quant QL = make_quant(clampi(qp, 0, 51), 5, round_q12), QC = make_quant(clampi(cqp, 0, 51), 5, round_q12);
-    const int q = luma ? qp : cqp;
+    const int qy = clampi(qp + QOFF, 0, BD == 8 ? 51 : 63), cqp = hevc::chroma_qp(qp) + QOFF;
+    const Quant QL = make_quant(qy, QSH, round_q12), QC = make_quant(clampi(cqp, 0, 63), QSH, round_q12);
+    const int q = luma ? qy : cqp;
 
-    // source prefetch: luma one dword per lane, chroma one dword per lane 0..31
+    // source prefetch: 4 luma samples per lane, 4 interleaved chroma samples per lane 0..31
     const int Y0r = yrow * 16;
-    uint32_t pre_y = 0, pre_c = 0;
+    int pre_y[4], pre_c[4] = {0, 0, 0, 0};
     auto prefetch = [&](int x) {
-        pre_y = *(const uint32_t *)(cy + (size_t)(Y0r + (lane >> 2)) * cpy + x * 16 + (lane & 3) * 4);
-        if (lane < 32) pre_c = *(const uint32_t *)(cuv + (size_t)(yrow * 8 + (lane >> 2)) * cpuv + x * 16 + (lane & 3) * 4);
+        ld4<Pix>(cy + (size_t)(Y0r + (lane >> 2)) * cpy, x * 16 + (lane & 3) * 4, pre_y);
+        if (lane < 32) ld4<Pix>(cuv + (size_t)(yrow * 8 + (lane >> 2)) * cpuv, x * 16 + (lane & 3) * 4, pre_c);
     };
     prefetch(0);
 
     // CTU-local sample (rx in -1..cs+3, ry in -1..cs-1) of this lane's plane
     auto sample = [&](int rx, int ry) -> int { return B[tile + (ry + 1) * tp + 4 + rx]; };
     auto take_record = [&](int j, uint32_t v, int d) {
-        // dword d of record j: 0..3 luma bytes 4d.., 4..5 Cb, 6..7 Cr
-        if (d < 4) *(uint32_t *)&B[O_AY + (j & 3) * 16 + d * 4] = v;
-        else if (d < 6) *(uint32_t *)&B[O_AC + (j & 3) * 8 + (d - 4) * 4] = v;
-        else *(uint32_t *)&B[O_AC + 32 + (j & 3) * 8 + (d - 6) * 4] = v;
+        // dword d of record j: luma row samples, then Cb, then Cr, SPD per dword
+        if (d < LD) *(uint32_t *)&B[O_AY + (j & 3) * 16 + d * SPD] = v;
+        else if (d < LD + CD) *(uint32_t *)&B[O_AC + (j & 3) * 8 + (d - LD) * SPD] = v;
+        else *(uint32_t *)&B[O_AC + 32 + (j & 3) * 8 + (d - LD - CD) * SPD] = v;
     };
     auto fetch_above = [&](int j) {  // record j of the row above into its slot
         uint32_t v = 0;
@@ -209,13 +218,14 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             wave_lds_sync();
         }
         // ---- source of CTU x into LDS; prefetch CTU x+1
-        *(uint32_t *)&B[O_SRC + lane * 4] = pre_y;
+#pragma unroll
+        for (int i = 0; i < 4; i++) B[O_SRC + lane * 4 + i] = (Pix)pre_y[i];
         if (lane < 32) {
             const int r = lane >> 2, c2 = (lane & 3) * 2;
-            B[O_SRC + 256 + r * 8 + c2] = (uint8_t)pre_c;
-            B[O_SRC + 320 + r * 8 + c2] = (uint8_t)(pre_c >> 8);
-            B[O_SRC + 256 + r * 8 + c2 + 1] = (uint8_t)(pre_c >> 16);
-            B[O_SRC + 320 + r * 8 + c2 + 1] = (uint8_t)(pre_c >> 24);
+            B[O_SRC + 256 + r * 8 + c2] = (Pix)pre_c[0];
+            B[O_SRC + 320 + r * 8 + c2] = (Pix)pre_c[1];
+            B[O_SRC + 256 + r * 8 + c2 + 1] = (Pix)pre_c[2];
+            B[O_SRC + 320 + r * 8 + c2 + 1] = (Pix)pre_c[3];
         }
         if (x + 1 < wtu) prefetch(x + 1);
         wave_lds_sync();
@@ -250,7 +260,7 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             v[9] = sample(rx + 4, ry - 1);
             const int vbl = sample(rx - 1, ry + 4 < cs ? ry + 4 : ry + 3);
             // first available in scan order fills what precedes it; later gaps copy the previous
-            int firstv = 128;
+            int firstv = 1 << (BD - 1);
 #pragma unroll
             for (int t = 9; t >= 0; t--)
                 if (a[t]) firstv = v[t];
@@ -274,8 +284,8 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
                 else if (pi == 0) p_dc = (Tj + 3 * dc + 2) >> 2;
                 else if (pj == 0) p_dc = (Li + 3 * dc + 2) >> 2;
             }
-            const int p_h = (luma && pi == 0) ? clampi(L0 + ((Tj - Cc) >> 1), 0, 255) : Li;
-            const int p_v = (luma && pj == 0) ? clampi(T0 + ((Li - Cc) >> 1), 0, 255) : Tj;
+            const int p_h = (luma && pi == 0) ? clampi(L0 + ((Tj - Cc) >> 1), 0, MAXV) : Li;
+            const int p_v = (luma && pj == 0) ? clampi(T0 + ((Li - Cc) >> 1), 0, MAXV) : Tj;
             const int s = B[srco + (by - cty0 + pi) * cs + (bx - ctx0 + pj)];
             // ---- SAD decision over the 16 lanes of the group (luma only)
             uint32_t sp = (uint32_t)iabs(s - p_pl), sd = (uint32_t)iabs(s - p_dc);
@@ -297,14 +307,14 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             int acc = 0;
 #pragma unroll
             for (int r = 0; r < 4; r++) acc += mri[r] * RES[g16 + r * 4 + pj];
-            TMP[lane] = sat16((acc + 1) >> 1);
+            TMP[lane] = sat16((acc + (1 << (S1 - 1))) >> S1);
             wave_lds_sync();
             acc = 0;
 #pragma unroll
             for (int c = 0; c < 4; c++) acc += mrj[c] * TMP[g16 + pi * 4 + c];
             const int lv = quant(sat16((acc + 128) >> 8), luma ? QL : QC);
             // ---- dequantisation, inverse, reconstruction
-            RES[lane] = dequant(lv, q, 5);
+            RES[lane] = dequant(lv, q, QSH);
             wave_lds_sync();
             acc = 0;
 #pragma unroll
@@ -314,9 +324,9 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             acc = 0;
 #pragma unroll
             for (int t = 0; t < 4; t++) acc += mcj[t] * TMP[g16 + pi * 4 + t];
-            const int rec = clampi(pred + sat16((acc + 2048) >> 12), 0, 255);
+            const int rec = clampi(pred + sat16((acc + (1 << (S2INV - 1))) >> S2INV), 0, MAXV);
             if (active) {
-                B[tile + (ry + pi + 1) * tp + 4 + (rx + pj)] = (uint8_t)rec;
+                B[tile + (ry + pi + 1) * tp + 4 + (rx + pj)] = (Pix)rec;
                 const size_t cb = (size_t)(yrow * wtu + x) * 384;
                 if (luma) coef[cb + k * 16 + k16] = (int16_t)lv;
                 else coef[cb + 256 + (plane - 1) * 64 + cu * 16 + k16] = (int16_t)lv;
@@ -327,9 +337,9 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
         // ---- hand the CTU's bottom rows down
         if (give_below) {
             uint32_t h = 0;
-            if (lane < 4) h = *(const uint32_t *)&B[O_TY + 16 * LP + 4 + lane * 4];
-            else if (lane < 6) h = *(const uint32_t *)&B[O_TC + 8 * CPT + 4 + (lane - 4) * 4];
-            else if (lane < 8) h = *(const uint32_t *)&B[O_TC + 9 * CPT + 8 * CPT + 4 + (lane - 6) * 4];
+            if (lane < LD) h = *(const uint32_t *)&B[O_TY + 16 * LP + 4 + lane * SPD];
+            else if (lane < LD + CD) h = *(const uint32_t *)&B[O_TC + 8 * CPT + 4 + (lane - LD) * SPD];
+            else if (lane < REC) h = *(const uint32_t *)&B[O_TC + 9 * CPT + 8 * CPT + 4 + (lane - LD - CD) * SPD];
             if (lds_below) {
                 unsigned n = 0;
                 while (vcons[wv + 1] < x + 1 - RING + 1) {
@@ -344,13 +354,16 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             }
         }
         // ---- the CTU's reconstruction to the frame
-        *(uint32_t *)(oy + (size_t)(Y0r + (lane >> 2)) * opy + X0 + (lane & 3) * 4) =
-            *(const uint32_t *)&B[O_TY + ((lane >> 2) + 1) * LP + 4 + (lane & 3) * 4];
+        {
+            const Pix *t = &B[O_TY + ((lane >> 2) + 1) * LP + 4 + (lane & 3) * 4];
+            const int v[4] = {t[0], t[1], t[2], t[3]};
+            st4<Pix>(oy + (size_t)(Y0r + (lane >> 2)) * opy, X0 + (lane & 3) * 4, v);
+        }
         if (lane < 32) {
             const int r = lane >> 2, c2 = (lane & 3) * 2;
-            const uint8_t *tb = &B[O_TC + (r + 1) * CPT + 4 + c2], *tr = &B[O_TC + 9 * CPT + (r + 1) * CPT + 4 + c2];
-            *(uint32_t *)(ouv + (size_t)(yrow * 8 + r) * opuv + X0 + (lane & 3) * 4) =
-                (uint32_t)tb[0] | ((uint32_t)tr[0] << 8) | ((uint32_t)tb[1] << 16) | ((uint32_t)tr[1] << 24);
+            const Pix *tb = &B[O_TC + (r + 1) * CPT + 4 + c2], *tr = &B[O_TC + 9 * CPT + (r + 1) * CPT + 4 + c2];
+            const int v[4] = {tb[0], tr[0], tb[1], tr[1]};
+            st4<Pix>(ouv + (size_t)(yrow * 8 + r) * opuv, X0 + (lane & 3) * 4, v);
         }
         wave_lds_sync();
         // ---- slide: this CTU's right column becomes the left neighbour column
@@ -366,24 +379,41 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
 extern "C" size_t bc250_hevc_intra_work_bytes(int wc, int hc)
 {
     const size_t nctu = (size_t)(wc / 16) * (hc / 16);
-    return 256 + nctu * REC * 8;
+    return 256 + nctu * REC_MAX * 8;
 }
 
-extern "C" int bc250_hevc_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, uint8_t *oy,
-                                          uint8_t *ouv, int opy, int opuv, int wc, int hc, int qp, int num_slices,
-                                          int round_q12, uint8_t *modes, int16_t *coef, int *sync,
-                                          unsigned spin_limit, void *stream)
+template <typename Pix, int BD>
+static void launch_i(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, uint8_t *oy, uint8_t *ouv, int opy,
+                     int opuv, int wc, int hc, int qp, int num_slices, int round_q12, uint8_t *modes, int16_t *coef,
+                     int *sync, unsigned spin_limit, hipStream_t st)
 {
-    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || num_slices < 1 || round_q12 < 0 ||
-        round_q12 > 12 || (cpy & 3) || (cpuv & 3) || (opy & 3) || (opuv & 3) || cpy < wc || cpuv < wc || opy < wc ||
-        opuv < wc)
-        return -1;
-    hipStream_t st = (hipStream_t)stream;
+    constexpr int REC = 32 / (4 / (int)sizeof(Pix));
     const int htu = hc / 16, nrec = (wc / 16) * htu * REC;
     uint64_t *ghand = (uint64_t *)((uint8_t *)sync + 256);
     hevc_clear_records_kernel<<<(nrec + 255) / 256, 256, 0, st>>>(ghand, nrec);
-    hevc_i_encode_kernel<<<(htu + WAVES - 1) / WAVES, 64 * WAVES, 0, st>>>(cy, cuv, cpy, cpuv, oy, ouv, opy, opuv, wc,
-                                                                            hc, qp, num_slices, round_q12, modes, coef,
-                                                                            sync, ghand, spin_limit);
+    hevc_i_encode_kernel<Pix, BD><<<(htu + WAVES - 1) / WAVES, 64 * WAVES, 0, st>>>(
+        cy, cuv, cpy, cpuv, oy, ouv, opy, opuv, wc, hc, qp, num_slices, round_q12, modes, coef, sync, ghand,
+        spin_limit);
+}
+
+extern "C" int bc250_hevc_launch_i_encode(const void *cy, const void *cuv, int cpy, int cpuv, void *oy, void *ouv,
+                                          int opy, int opuv, int wc, int hc, int qp, int num_slices, int round_q12,
+                                          uint8_t *modes, int16_t *coef, int *sync, unsigned spin_limit, int bd,
+                                          void *stream)
+{
+    const int b = bd == 8 ? 1 : 2, al = 4 * b - 1;
+    if ((bd != 8 && bd != 10) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 ||
+        num_slices < 1 || round_q12 < 0 || round_q12 > 12 || (cpy & al) || (cpuv & al) || (opy & al) ||
+        (opuv & al) || cpy < wc * b || cpuv < wc * b || opy < wc * b || opuv < wc * b)
+        return -1;
+    hipStream_t st = (hipStream_t)stream;
+    auto *C = (const uint8_t *)cy, *CU = (const uint8_t *)cuv;
+    auto *O = (uint8_t *)oy, *OU = (uint8_t *)ouv;
+    if (bd == 8)
+        launch_i<uint8_t, 8>(C, CU, cpy, cpuv, O, OU, opy, opuv, wc, hc, qp, num_slices, round_q12, modes, coef, sync,
+                             spin_limit, st);
+    else
+        launch_i<uint16_t, 10>(C, CU, cpy, cpuv, O, OU, opy, opuv, wc, hc, qp, num_slices, round_q12, modes, coef,
+                               sync, spin_limit, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

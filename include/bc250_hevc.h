@@ -67,6 +67,10 @@ int bc250_hevc_mc_uni_batch(const void *ref, int stride, int w_pic, int h_pic, c
 /* Frame-stage entry points.  Planes are 8-bit NV12 (bd 8) or P010 (bd 10:
  * the sample in the high 10 bits of each 16-bit word); pitches in bytes. */
 
+/* 8-bit luma for the integer search of a P010 picture: dst = the top 8 bits
+ * of each 10-bit sample (w a multiple of 4). */
+int bc250_hevc_p010_to_8bit(const void *src, int spitch, uint8_t *dst, int dpitch, int w, int h, void *stream);
+
 /* Quarter-sample refinement of every 8x8 CU: imv8[4*cu] = {dx, dy, cost,
  * sad} integer vectors (bc250_launch_me_fullsearch's 8x8 output); out[4*cu]
  * = {mvx, mvy (quarter samples), sad, 0}.  Centre, then 8 half-sample
@@ -86,15 +90,17 @@ int bc250_hevc_launch_p_encode(const void *cy, const void *cuv, int cpy, int cpu
 
 /* I picture: CTU16 -> 4 CU8 -> 4 PU4 intra (Planar/DC/H/V by SAD, DST),
  * chroma DC per CU (DCT, QpC), slices as CTU-row ranges
- * [floor(rows*s/ns), floor(rows*(s+1)/ns)).  One persistent wavefront
- * launch; `sync` is a workspace of bc250_hevc_intra_work_bytes() bytes whose
- * first 2 ints must be zero; sync[1] != 0 afterwards means a dependency wait
- * gave up after spin_limit polls.  modes: uint8 [nctu*16] (0 Planar, 1 DC,
- * 10 H, 26 V), PU z-order. */
+ * [floor(rows*s/ns), floor(rows*(s+1)/ns)), QP' = qp + 6 (bd - 8) (the
+ * reference's encode_ctu -> intra_trial, hevc_enc_template.c:502-594).  One
+ * persistent wavefront launch; `sync` is a workspace of
+ * bc250_hevc_intra_work_bytes() bytes whose first 2 ints must be zero;
+ * sync[1] != 0 afterwards means a dependency wait gave up after spin_limit
+ * polls.  modes: uint8 [nctu*16] (0 Planar, 1 DC, 10 H, 26 V), PU z-order.
+ * Pitches multiples of 4 (bd 8) or 8 (bd 10) bytes. */
 size_t bc250_hevc_intra_work_bytes(int wc, int hc);
-int bc250_hevc_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, uint8_t *oy, uint8_t *ouv,
-                               int opy, int opuv, int wc, int hc, int qp, int num_slices, int round_q12,
-                               uint8_t *modes, int16_t *coef, int *sync, unsigned spin_limit, void *stream);
+int bc250_hevc_launch_i_encode(const void *cy, const void *cuv, int cpy, int cpuv, void *oy, void *ouv, int opy,
+                               int opuv, int wc, int hc, int qp, int num_slices, int round_q12, uint8_t *modes,
+                               int16_t *coef, int *sync, unsigned spin_limit, int bd, void *stream);
 
 /* Per-CU coded-luma flags of a P picture's levels (bc250_hevc_launch_p_encode
  * layout): cbf[4*t + z] = 1 when any of CU z's 64 luma levels is non-zero,

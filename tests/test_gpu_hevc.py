@@ -204,50 +204,61 @@ def test_p_encode_stage(pkg, qp, bd):
     assert (want["coef"] != 0).any()
 
 
-@pytest.mark.parametrize("w,h,qp,ns", [(64, 48, 22, 1), (128, 96, 37, 2), (200, 120, 27, 3), (320, 240, 32, 4)])
-def test_i_encode_stage(pkg, w, h, qp, ns):
+@pytest.mark.parametrize("w,h,qp,ns,bd", [(64, 48, 22, 1, 8), (128, 96, 37, 2, 8), (200, 120, 27, 3, 8),
+                                           (320, 240, 32, 4, 8), (64, 48, 22, 1, 10), (128, 96, 45, 2, 10),
+                                           (208, 128, 27, 3, 10), (320, 240, 0, 4, 10)])
+def test_i_encode_stage(pkg, w, h, qp, ns, bd):
     torch = _torch()
-    say(f"hevc I {w}x{h} qp={qp} slices={ns}")
+    say(f"hevc I {w}x{h} qp={qp} slices={ns} bd={bd}")
     (cy, cuv), = _frames(pkg, 5 + qp, w, h, 1, 3)
     hc, wc = cy.shape
     t = (wc // 16) * (hc // 16)
+    if bd == 10:
+        rs = np.random.default_rng(qp + 1)
+        cy, cuv = _ten(cy, rs), _ten(cuv, rs)
     cb, cr = orc.split_uv(cuv)
-    want = orc.hevc_encode_i(cy, cb, cr, qp, ns, orc.HEVC_ROUND_INTRA_I)
-    d = [dev(torch, a) for a in (cy, cuv)]
+    want = orc.hevc_encode_i(cy, cb, cr, qp, ns, orc.HEVC_ROUND_INTRA_I, bd)
+    if bd == 10:
+        d, pitch, back = [_p010(torch, a) for a in (cy, cuv)], wc * 2, _unp010
+    else:
+        d, pitch, back = [dev(torch, a) for a in (cy, cuv)], wc, host
     oy, ouv = torch.zeros_like(d[0]), torch.zeros_like(d[1])
     modes = torch.zeros(t * 16, dtype=torch.uint8, device="cuda")
     coef = torch.zeros(t * 384, dtype=torch.int16, device="cuda")
     nwork = pkg.hip.bc250_hevc_intra_work_bytes(wc, hc)
     sync = torch.zeros((nwork + 3) // 4, dtype=torch.int32, device="cuda")
-    assert pkg.hip.bc250_hevc_launch_i_encode(P(d[0]), P(d[1]), wc, wc, P(oy), P(ouv), wc, wc, wc, hc, qp, ns,
-                                              orc.HEVC_ROUND_INTRA_I, P(modes), P(coef), P(sync), 1 << 22,
+    assert pkg.hip.bc250_hevc_launch_i_encode(P(d[0]), P(d[1]), pitch, pitch, P(oy), P(ouv), pitch, pitch, wc, hc, qp,
+                                              ns, orc.HEVC_ROUND_INTRA_I, P(modes), P(coef), P(sync), 1 << 22, bd,
                                               None) == 0
     torch.cuda.synchronize()
     assert host(sync)[1] == 0, "wavefront wait gave up"
     assert np.array_equal(host(modes), want["modes"])
     assert np.array_equal(host(coef), want["coef"])
-    assert np.array_equal(host(oy), want["y"])
-    gb, gr = orc.split_uv(host(ouv))
+    assert np.array_equal(back(oy), want["y"])
+    gb, gr = orc.split_uv(back(ouv))
     assert np.array_equal(gb, want["cb"]) and np.array_equal(gr, want["cr"])
 
 
-def _hevc_sequence(pkg, w, h, n, qp, ns, rng, lam, seed, nrect, gop=0, deblock=True):
-    say(f"hevc pipeline {w}x{h} frames={n} qp={qp} slices={ns} R={rng} deblock={deblock}")
-    fr = _frames(pkg, seed, w, h, n, nrect)
+def _hevc_sequence(pkg, w, h, n, qp, ns, rng, lam, seed, nrect, gop=0, deblock=True, bd=8):
+    say(f"hevc pipeline {w}x{h} frames={n} qp={qp} slices={ns} R={rng} deblock={deblock} bd={bd}")
     g = pkg.FrameGen(seed, w, h, nrect)
     enc = pkg.Encoder(w, h, search_range=rng, lambda8=lam, deblock_idc=0 if deblock else 1,
-                      codec=pkg.GPU_CODEC_HEVC)
+                      codec=pkg.GPU_CODEC_HEVC, bit_depth=bd)
+    rs = np.random.default_rng(seed)
     ref = None
     try:
         for i in range(n):
+            vy, vuv = g.frame(i)
+            if bd == 10:
+                vy, vuv = _ten(vy, rs), _ten(vuv, rs)
             im = enc.new_image()
-            enc.upload(im, *g.frame(i))
+            enc.upload(im, vy, vuv)
             intra = i == 0 or (gop and i % gop == 0)
             slot = enc.encode(im, qp, intra, ns)
             got = enc.results(slot)
-            cy, cuv = fr[i]
+            cy, cuv = pkg.pad_coded(vy, vuv)
             ry, ruv = (cy, cuv) if ref is None else ref
-            exp = orc.hevc_encode_frame(cy, cuv, ry, ruv, qp, intra, ns, rng, lam, deblock)
+            exp = orc.hevc_encode_frame(cy, cuv, ry, ruv, qp, intra, ns, rng, lam, deblock, bd)
             assert np.array_equal(got["mv8"], exp["mv8"]), f"frame {i} mv8"
             assert np.array_equal(got["hevc_mode"], exp["modes"]), f"frame {i} modes"
             assert np.array_equal(got["quant"], exp["coef"]), f"frame {i} coef"
@@ -275,6 +286,18 @@ def test_hevc_pipeline_no_deblock(pkg):
 def test_hevc_pipeline_odd_size_slices_gop(pkg):
     _torch()
     _hevc_sequence(pkg, 200, 120, 5, 37, 3, 32, 2, seed=32, nrect=2, gop=3)
+
+
+def test_hevc_pipeline_main10(pkg):
+    """10-bit P010 pictures (4 v + dither, as the 4K Main10 workload makes
+    them), odd size, slices, a GOP."""
+    _torch()
+    _hevc_sequence(pkg, 200, 120, 5, 27, 3, 16, 1, seed=41, nrect=2, gop=3, bd=10)
+
+
+def test_hevc_pipeline_main10_no_deblock(pkg):
+    _torch()
+    _hevc_sequence(pkg, 128, 96, 3, 37, 1, 16, 1, seed=42, nrect=3, deblock=False, bd=10)
 
 
 def test_hevc_pipeline_640x480_cqp32_4slices(pkg):
