@@ -160,6 +160,61 @@ def run_pipeline(pkg, torch, frames, codec, args):
         enc.close()
 
 
+def bench_main10_4k(pkg, torch, args):
+    """The 4K Main10 workload (not the headline): 3840x2160 P010 (10-bit
+    samples = 4 x the generator's 8-bit output + uniform [0, 3]), IPPP, CQP
+    27, 4 slices, +-64 full search, deblocking; one JSON line."""
+    w, h = 3840, 2160
+    n = args.warmup + args.steps
+    log(f"generating {n} frames {w}x{h} P010")
+    g = pkg.FrameGen(2160, w, h, 12)
+    rs = np.random.default_rng(2160)
+    frames = []
+    for i in range(n):
+        y, uv = g.frame(i)
+        frames.append((y.astype(np.uint16) * 4 + rs.integers(0, 4, y.shape, dtype=np.uint16),
+                       uv.astype(np.uint16) * 4 + rs.integers(0, 4, uv.shape, dtype=np.uint16)))
+    g.close()
+    enc = pkg.Encoder(w, h, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, me_8x8=1, readback=0,
+                      codec=pkg.GPU_CODEC_HEVC, bit_depth=10)
+    try:
+        images = []
+        for y, uv in frames:
+            im = enc.new_image()
+            enc.upload(im, y, uv)
+            images.append(im)
+        del frames
+        log("main10 4k: warmup")
+        enc.encode_sequence(images[:args.warmup], args.qp, gop=0, num_slices=4, pipelined=1)
+        enc.stage_stats(reset=True)
+        log(f"main10 4k: timed {args.steps} frames IPPP")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        frame_ms, _ = enc.encode_sequence(images[args.warmup:], args.qp, gop=0, num_slices=4, pipelined=1)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st, errs = enc.stage_stats(reset=True), enc.error_flags()
+    finally:
+        enc.close()
+    names = ["me_integer", "me_subpel", "encode", "deblock", "staging"]
+    res = {
+        "metric": "4K HEVC Main10 fps on 1 GPU", "value": round(args.steps / wall, 2), "unit": "fps", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
+        "config": {"workload": f"3840x2160 P010, {args.steps} frames IPPP, CQP {args.qp}, 4 slices: HEVC Main10 "
+                               f"pipeline (CTU16), +-{args.range} full search on the top 8 bits, 10-bit refine, "
+                               "CU coding, CTU intra wavefront, 8.7.2 deblocking",
+                   "codec": "hevc", "width": w, "height": h, "bit_depth": 10, "frames": args.steps, "qp": args.qp,
+                   "search_range": args.range, "num_slices": 4, "seed": 2160, "rectangles": 12},
+        "p50_frame_ms": round(float(np.percentile(frame_ms, 50)), 4),
+        "p99_frame_ms": round(float(np.percentile(frame_ms, 99)), 4),
+        "mpixel_per_s": round(args.steps / wall * w * h / 1e6, 1),
+        "stage_ms": {k: round(st["sum_ms"][i] / max(st["frames"], 1), 4) for i, k in enumerate(names)},
+        "wavefront_wait_errors": errs,
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +228,8 @@ def main():
     ap.add_argument("--single", action="store_true", help="skip the second codec's run")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle integer search")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("1080p", "4k-main10"), default="1080p",
+                    help="4k-main10: the 3840x2160 P010 HEVC Main10 workload instead of the 1080p headline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the pipeline shards by session (one GPU per group of sessions); "
@@ -181,6 +238,9 @@ def main():
 
     import torch
     pkg = load_pkg()
+    if args.workload == "4k-main10":
+        bench_main10_4k(pkg, torch, args)
+        return
     n = args.warmup + args.steps
     log(f"generating {n} frames {W}x{H}")
     g = pkg.FrameGen(1080, W, H, 12)

@@ -300,6 +300,66 @@ def test_hevc_pipeline_main10_no_deblock(pkg):
     _hevc_sequence(pkg, 128, 96, 3, 37, 1, 16, 1, seed=42, nrect=3, deblock=False, bd=10)
 
 
+def test_hevc_main10_4k_full_size(pkg):
+    """The 4K Main10 workload at its real shape: 3840x2160 P010, CQP 27, 4
+    slices, +-64 search, deblocking.  The I picture is compared whole with
+    the oracle.  For the P picture the oracle codes the frame with the GPU's
+    vectors (levels and deblocked reconstruction must match exactly), and a
+    spread sample of 64 MBs is searched and refined by the oracle from
+    scratch (vectors and SADs must match)."""
+    _torch()
+    w, h, qp, ns, rng = 3840, 2160, 27, 4, 64
+    say("hevc main10 4k: 2 frames")
+    g = pkg.FrameGen(2160, w, h, 12)
+    rs = np.random.default_rng(2160)
+    vis = [tuple(_ten(a, rs) for a in g.frame(i)) for i in range(2)]
+    g.close()
+    enc = pkg.Encoder(w, h, search_range=rng, lambda8=1, codec=pkg.GPU_CODEC_HEVC, bit_depth=10)
+    try:
+        cods = [pkg.pad_coded(*v) for v in vis]
+        im = enc.new_image()
+        enc.upload(im, *vis[0])
+        got = enc.results(enc.encode(im, qp, True, ns))
+        cy, cuv = cods[0]
+        exp = orc.hevc_encode_frame(cy, cuv, cy, cuv, qp, True, ns, rng, 1, True, 10)
+        assert np.array_equal(got["hevc_mode"], exp["modes"]) and np.array_equal(got["quant"], exp["coef"])
+        my, muv = enc.map_recon()
+        assert np.array_equal(my, exp["y"]) and np.array_equal(muv, exp["uv"]), "I recon"
+        ry, ruv = exp["y"], exp["uv"]
+        say("  I ok")
+        im = enc.new_image()
+        enc.upload(im, *vis[1])
+        got = enc.results(enc.encode(im, qp, False, ns))
+        assert enc.error_flags() == 0
+        cy, cuv = cods[1]
+        mv8 = got["mv8"]
+        mv = np.stack([mv8["mvx"], mv8["mvy"]], 1).astype(np.int32)
+        cb, cr = orc.split_uv(cuv)
+        fb, fr = orc.split_uv(ruv)
+        o = orc.hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, orc.HEVC_ROUND_INTER, 10)
+        assert np.array_equal(got["quant"], o["coef"]), "P levels"
+        uv = np.empty_like(cuv)
+        uv[:, 0::2], uv[:, 1::2] = o["cb"], o["cr"]
+        mv4 = np.zeros((mv.shape[0], 4), np.int32)
+        mv4[:, :2] = mv
+        wy, wuv = orc.hevc_deblock(o["y"], uv, qp, 0, orc.hevc_cu_cbf(o["coef"]), mv4, ns, 10)
+        my, muv = enc.map_recon()
+        assert np.array_equal(my, wy) and np.array_equal(muv, wuv), "P recon"
+        c8, r8 = (cy >> 2).astype(np.uint8), (ry >> 2).astype(np.uint8)
+        mbw, nmb = w // 16, (w // 16) * ((h + 15) // 16)
+        for m in np.linspace(0, nmb - 1, 64).astype(int):
+            _, i8 = orc.me_fullsearch_mb(c8, r8, rng, 1, int(m % mbw), int(m // mbw))
+            for z in range(4):
+                x0, y0 = (m % mbw) * 16 + (z & 1) * 8, (m // mbw) * 16 + (z >> 1) * 8
+                mx, my_, sad = orc.hevc_refine8(cy, ry, int(x0), int(y0), int(i8[z]["dx"]) * 4,
+                                                int(i8[z]["dy"]) * 4, 10)
+                cu = 4 * m + z
+                assert (mv8[cu]["mvx"], mv8[cu]["mvy"], mv8[cu]["sad"]) == (mx, my_, sad), (m, z)
+        say("  P ok")
+    finally:
+        enc.close()
+
+
 def test_hevc_pipeline_640x480_cqp32_4slices(pkg):
     """The 640x480 workload's shape: CQP 32, 4 slices, +-32 full search."""
     _torch()
