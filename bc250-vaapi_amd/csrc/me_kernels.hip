@@ -1,14 +1,15 @@
 // me_kernels.hip - +-R integer full-search motion estimation and the
 // half/quarter-pel refinement, hand-written for gfx950.
 //
-// Integer search (me_fullsearch_kernel): one workgroup per macroblock.  The
-// reference window of the MB (edge-replicated, (16+2R+7) x (20+2R) bytes) is
-// staged in LDS; the 16x16 current block is wave-uniform and lives in SGPRs.
-// Every lane owns a tile of 4 (dx) x 8 (dy) candidates: for each reference
-// row it reads 5 aligned dwords from LDS, builds the byte-shifted copies with
-// v_alignbyte_b32, and feeds v_sad_u8 with four 8x8-quadrant accumulators
-// per candidate (128 VGPRs), so the 16x16 cost and the four 8x8 costs come
-// from one pass.  Accumulators start at lambda8*|mv|_1, which makes
+// Integer search (me_fullsearch_kernel): one 256-thread workgroup per
+// macroblock.  The reference window of the MB (edge-replicated) is staged in
+// LDS; the 16x16 current block is wave-uniform and lives in SGPRs.  Every
+// lane owns a tile of 4 (dx) x 9 (dy) candidates (at R = 64: 33 x 15 = 495
+// tiles, two per thread, and 4 waves so that two workgroups give every SIMD
+// two waves): for each reference row it reads 5 aligned dwords from LDS,
+// builds the byte-shifted copies with v_alignbyte_b32, and feeds v_sad_u8
+// with four 8x8-quadrant accumulators per candidate (144 VGPRs), so the
+// 16x16 cost and the four 8x8 costs come from one pass.  Accumulators start at lambda8*|mv|_1, which makes
 //   cost8 = SAD8 + lambda8*|mv|_1      cost16 = SAD16 + 4*lambda8*|mv|_1.
 // The argmin key is cost*2^15 + raster index of (dx,dy) (ties -> lowest
 // raster index), reduced per lane, across the wave with DPP shuffles and
@@ -18,15 +19,16 @@
 
 namespace {
 
-constexpr int ME_THREADS = 192;
+constexpr int ME_THREADS = 256;
+constexpr int DY = 9;  // dy candidates per lane task
 
 template <int R>
 struct FsGeom {
     static constexpr int NC = 2 * R + 1;          // candidates per axis
     static constexpr int NG = (NC + 3) / 4;       // dx groups of 4
-    static constexpr int NGY = (NC + 7) / 8;      // dy groups of 8
+    static constexpr int NGY = (NC + DY - 1) / DY;  // dy groups of DY
     static constexpr int NT = NG * NGY;           // lane tasks per MB
-    static constexpr int NR = NGY * 8 + 15;       // window rows
+    static constexpr int NR = NGY * DY + 15;      // window rows
     static constexpr int WD = NG + 4;             // window dwords per row
     static constexpr int PITCH = WD | 1;          // odd dword pitch
     static constexpr int SLOTS = (NT + ME_THREADS - 1) / ME_THREADS;
@@ -38,14 +40,14 @@ struct FsGeom {
 
 // out16[m] = {dx, dy, cost, sad}; out8[4m+q] likewise for quadrant q.
 template <int R>
-__global__ __launch_bounds__(ME_THREADS) void me_fullsearch_kernel(
+__global__ __launch_bounds__(ME_THREADS, 2) void me_fullsearch_kernel(
     const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref, int rpitch,
     int wc, int hc, int mbw, int nmb, int lambda8, uint4 *__restrict__ out16,
     uint4 *__restrict__ out8)
 {
     using Gm = FsGeom<R>;
     __shared__ uint32_t win[Gm::NR * Gm::PITCH];
-    __shared__ uint32_t red[3][5];
+    __shared__ uint32_t red[ME_THREADS / 64][5];
 
     const int mb = xcd_item(blockIdx.x, nmb);
     if (mb >= nmb) return;
@@ -88,10 +90,10 @@ __global__ __launch_bounds__(ME_THREADS) void me_fullsearch_kernel(
         const int t = tid + slot * ME_THREADS;
         if (t < Gm::NT) {
             const int dxg = t % Gm::NG, dyg = t / Gm::NG;
-            const int dx0 = -R + 4 * dxg, dy0 = -R + 8 * dyg;
-            uint32_t acc[8][4][4];
+            const int dx0 = -R + 4 * dxg, dy0 = -R + DY * dyg;
+            uint32_t acc[DY][4][4];
 #pragma unroll
-            for (int j = 0; j < 8; j++)
+            for (int j = 0; j < DY; j++)
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     int dx = dx0 + k, dy = dy0 + j;
@@ -100,9 +102,9 @@ __global__ __launch_bounds__(ME_THREADS) void me_fullsearch_kernel(
 #pragma unroll
                     for (int q = 0; q < 4; q++) acc[j][k][q] = init;
                 }
-            const uint32_t *wrow = win + (dyg * 8) * Gm::PITCH + dxg;
+            const uint32_t *wrow = win + (dyg * DY) * Gm::PITCH + dxg;
 #pragma unroll
-            for (int r = 0; r < 23; r++) {
+            for (int r = 0; r < DY + 15; r++) {
                 uint32_t d[5];
 #pragma unroll
                 for (int i = 0; i < 5; i++) d[i] = wrow[r * Gm::PITCH + i];
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(ME_THREADS) void me_fullsearch_kernel(
                     s[3][c] = __builtin_amdgcn_alignbyte(d[c + 1], d[c], 3);
                 }
 #pragma unroll
-                for (int j = 0; j < 8; j++) {
+                for (int j = 0; j < DY; j++) {
                     const int cr = r - j;
                     if (cr < 0 || cr > 15) continue;
                     const int qr = (cr >> 3) << 1;
@@ -127,21 +129,22 @@ __global__ __launch_bounds__(ME_THREADS) void me_fullsearch_kernel(
                                 __builtin_amdgcn_sad_u8(s[k][c], cw[cr][c], acc[j][k][qr + (c >> 1)]);
                 }
             }
-            // per-lane argmin with a 5-bit local index (j-major = raster order)
+            // per-lane argmin with a 6-bit local index (j-major = raster order)
             uint32_t l16 = 0xffffffffu, lq[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
 #pragma unroll
-            for (int j = 0; j < 8; j++)
+            for (int j = 0; j < DY; j++)
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const uint32_t li = (uint32_t)(j * 4 + k);
                     uint32_t s16 = acc[j][k][0] + acc[j][k][1] + acc[j][k][2] + acc[j][k][3];
-                    l16 = min(l16, (s16 << 5) | li);
+                    l16 = min(l16, (s16 << 6) | li);
 #pragma unroll
-                    for (int q = 0; q < 4; q++) lq[q] = min(lq[q], (acc[j][k][q] << 5) | li);
+                    for (int q = 0; q < 4; q++) lq[q] = min(lq[q], (acc[j][k][q] << 6) | li);
                 }
-            // promote to global keys: cost*2^15 + raster index
+            // promote to global keys: cost*2^15 + raster index (every tile holds a
+            // candidate inside the range, so its minimum is one)
             auto gkey = [&](uint32_t lk) -> uint32_t {
-                uint32_t li = lk & 31u, cost = lk >> 5;
+                uint32_t li = lk & 63u, cost = lk >> 6;
                 int dx = dx0 + (int)(li & 3u), dy = dy0 + (int)(li >> 2);
                 uint32_t idx = (uint32_t)((dy + R) * Gm::NC + (dx + R));
                 return (cost << 15) | idx;
@@ -164,7 +167,9 @@ __global__ __launch_bounds__(ME_THREADS) void me_fullsearch_kernel(
     }
     __syncthreads();
     if (tid < 5) {
-        uint32_t k = min(min(red[0][tid], red[1][tid]), red[2][tid]);
+        uint32_t k = red[0][tid];
+#pragma unroll
+        for (int w = 1; w < ME_THREADS / 64; w++) k = min(k, red[w][tid]);
         uint32_t idx = k & 0x7fffu, cost = k >> 15;
         int dx = (int)(idx % Gm::NC) - R, dy = (int)(idx / Gm::NC) - R;
         uint32_t l1 = (uint32_t)(iabs(dx) + iabs(dy));
