@@ -248,32 +248,75 @@ __global__ __launch_bounds__(64) void hevc_mc_batch_kernel(const Pix *__restrict
 // ---------------------------------------------------------------------------
 // Quarter-sample refinement, one wave per 8x8 CU, 4 CUs (one CTU) per
 // workgroup.  Every candidate lies within 3 quarter samples of the integer
-// vector, so all taps fall in a 16x16 window at (x0+dx-4, y0+dy-4).  Pitches
+// vector (dx, dy), so all taps fall in a 16x16 window W at (x0+dx-4, y0+dy-4)
+// and a sample's integer position is window column pc + 3 or pc + 4 (row
+// likewise).  The horizontal 8-tap results for fx = 1..3 at window columns
+// 3..11 of every row are computed once (H, the int16 intermediates of
+// 8.5.3.3.3.1); a candidate then costs 8 vertical taps per sample.  Pitches
 // in bytes; Pix uint8_t (8-bit) or uint16_t (P010, BD 10).
+struct RefineLds8 {
+    int w[16 * 16];      // window samples
+    int h[3][16 * 9];    // fx = 1..3: horizontal sums >> (BD - 8) at columns 3..11
+};
+
 template <typename Pix, int BD>
 __global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__restrict__ cur, int cpitch,
                                                            const uint8_t *__restrict__ ref, int rpitch, int wc, int hc,
                                                            int nctu, const uint4 *__restrict__ imv8,
                                                            uint4 *__restrict__ out)
 {
-    __shared__ int win[4][16 * 16];
+    constexpr int DOWN = BD - 8, MAXV = (1 << BD) - 1, SH = 14 - BD, OFF = 1 << (13 - BD);
+    __shared__ RefineLds8 Ls[4];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ctu = xcd_item(blockIdx.x, nctu);
     if (ctu >= nctu) return;
+    RefineLds8 &L = Ls[wv];
     const int cu = ctu * 4 + wv, mbw = wc / 16;
     const int x0 = (ctu % mbw) * 16 + (wv & 1) * 8, y0 = (ctu / mbw) * 16 + (wv >> 1) * 8;
     const uint4 iv = imv8[cu];
     const int dx = (int)iv.x, dy = (int)iv.y;
     const int ox = x0 + dx - 4, oy = y0 + dy - 4;
-    int *W = win[wv];
     for (int t = lane; t < 256; t += 64)
-        W[t] = ldpix<Pix>(ref + (size_t)clampi(oy + (t >> 4), 0, hc - 1) * rpitch, clampi(ox + (t & 15), 0, wc - 1));
+        L.w[t] = ldpix<Pix>(ref + (size_t)clampi(oy + (t >> 4), 0, hc - 1) * rpitch, clampi(ox + (t & 15), 0, wc - 1));
     const int pr = lane >> 3, pc = lane & 7;
     const int cv = ldpix<Pix>(cur + (size_t)(y0 + pr) * cpitch, x0 + pc);
     wave_lds_sync();
+    // horizontal intermediates: 3 fractions x 16 rows x 9 columns
+    for (int t = lane; t < 3 * 144; t += 64) {
+        const int f = t / 144, e = t - f * 144, row = e / 9, col = e - row * 9 + 3;
+        const int8_t *fh = kQpel[f + 1];
+        int acc = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) acc += fh[k] * L.w[row * 16 + col - 3 + k];
+        L.h[f][e] = (int16_t)(acc >> DOWN);
+    }
+    wave_lds_sync();
     auto sad_of = [&](int mx, int my) {
-        const int wx = pc + (mx >> 2) - dx + 4, wy = pr + (my >> 2) - dy + 4;
-        const int pv = mc_sample<false>(W, 16, wx, wy, mx & 3, my & 3, BD);
+        const int fx = mx & 3, fy = my & 3;
+        const int wx = pc + (mx >> 2) - dx + 4, wy = pr + (my >> 2) - dy + 4;  // integer position in W
+        int pv;
+        if (!fx && !fy) {
+            pv = L.w[wy * 16 + wx];
+        } else {
+            int p14;
+            if (!fy) {
+                p14 = L.h[fx - 1][wy * 9 + wx - 3];
+            } else {
+                const int8_t *fv = kQpel[fy];
+                int acc = 0;
+                if (!fx) {
+#pragma unroll
+                    for (int k = 0; k < 8; k++) acc += fv[k] * L.w[(wy - 3 + k) * 16 + wx];
+                    p14 = (int16_t)(acc >> DOWN);
+                } else {
+                    const int *H = L.h[fx - 1];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) acc += fv[k] * H[(wy - 3 + k) * 9 + wx - 3];
+                    p14 = (int16_t)(acc >> 6);
+                }
+            }
+            pv = clampi((p14 + OFF) >> SH, 0, MAXV);
+        }
         return wave_sum_u32((uint32_t)iabs(cv - pv));
     };
     const int hx[8] = {2, -2, 0, 0, 2, 2, -2, -2}, hy[8] = {0, 0, 2, -2, 2, -2, 2, -2};
