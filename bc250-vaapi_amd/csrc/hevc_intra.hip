@@ -49,7 +49,6 @@ constexpr int WAVE_SAMPLES = O_SRC + 384;
 template <typename Pix>
 struct alignas(16) WaveLds {
     Pix b[WAVE_SAMPLES];
-    int res[64], tmp[64];
 };
 
 // z-scan order of the 4x4 blocks inside a CTU (hevc_intra.c:75-130): the
@@ -58,6 +57,36 @@ BC_DEV int morton4(int rx, int ry)
 {
     const int bx = rx >> 2, by = ry >> 2;
     return ((by >> 1) << 3) | ((bx >> 1) << 2) | ((by & 1) << 1) | (bx & 1);
+}
+
+// Cross-lane moves inside each row of 16 lanes (one 4x4 block, lane =
+// 4 * row + column): down_rows<t> gives every lane the value of the lane t
+// block rows above it (row_ror by 4t, wrapping), next_cols<t> the value of the
+// lane t columns to its right in the same block row (quad_perm rotation).
+template <int T>
+BC_DEV int down_rows(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, 0x120 + 4 * T, 0xF, 0xF, false);
+}
+
+template <int T>
+BC_DEV int next_cols(int v)
+{
+    constexpr int ctrl = T == 1 ? 0x39 : (T == 2 ? 0x4E : 0x93);  // [1,2,3,0], [2,3,0,1], [3,0,1,2]
+    return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xF, 0xF, false);
+}
+
+// One pass of a 4x4 transform over the block held by a row of 16 lanes, the
+// lane's coefficients already permuted to match the moved values: columns
+// (sum over block rows) or rows (sum over block columns).
+BC_DEV int col_pass(int v, const int c[4])
+{
+    return c[0] * v + c[1] * down_rows<1>(v) + c[2] * down_rows<2>(v) + c[3] * down_rows<3>(v);
+}
+
+BC_DEV int row_pass(int v, const int c[4])
+{
+    return c[0] * v + c[1] * next_cols<1>(v) + c[2] * next_cols<2>(v) + c[3] * next_cols<3>(v);
 }
 
 // Sum over each row of 16 lanes, result in every lane of the row: DPP
@@ -115,7 +144,6 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
     if (yrow >= htu) return;
     volatile int *vprog = prog, *vcons = cons;
     Pix *B = wl[wv].b;
-    int *RES = wl[wv].res, *TMP = wl[wv].tmp;
 
     const int r0 = slice_first_row(yrow, htu, num_slices);
     const bool need_above = yrow > r0;
@@ -129,16 +157,19 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
     const int cs = luma ? 16 : 8;
     const int tile = luma ? O_TY : O_TC + (plane - 1) * 9 * CPT, tp = luma ? LP : CPT;
     const int srco = luma ? O_SRC : O_SRC + 256 + (plane - 1) * 64;
-    // the matrix entries this lane multiplies by, kept in registers:
-    // row pi / row pj (forward passes), column pi / column pj (inverse passes)
-    int mri[4], mrj[4], mci[4], mcj[4];
+    // the matrix entries this lane multiplies by, in the order the moves of
+    // col_pass / row_pass deliver the block: forward columns M[pi][pi-t],
+    // forward rows M[pj][pj+t], inverse columns M[pi-t][pi], inverse rows
+    // M[pj+t][pj] (indices mod 4)
+    int fc[4], fr[4], ic[4], ir[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const int8_t *M = luma ? kDst4 : kDct4;
-        mri[t] = M[pi * 4 + t];
-        mrj[t] = M[pj * 4 + t];
-        mci[t] = M[t * 4 + pi];
-        mcj[t] = M[t * 4 + pj];
+        const int a = (pi - t) & 3, b = (pj + t) & 3;
+        fc[t] = M[pi * 4 + a];
+        fr[t] = M[pj * 4 + b];
+        ic[t] = M[a * 4 + pi];
+        ir[t] = M[b * 4 + pj];
     }
     const int qy = clampi(qp + QOFF, 0, BD == 8 ? 51 : 63), cqp = hevc::chroma_qp(qp) + QOFF;
     const Quant QL = make_quant(qy, QSH, round_q12), QC = make_quant(clampi(cqp, 0, 63), QSH, round_q12);
@@ -300,31 +331,12 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             if (sh < best) { best = sh; mode = 10; pred = p_h; }
             if (sv < best) { best = sv; mode = 26; pred = p_v; }
             if (!luma) { mode = 1; pred = dc; }
-            // ---- forward transform, quantisation
-            const int g16 = (lane >> 4) * 16;
-            RES[lane] = s - pred;
-            wave_lds_sync();
-            int acc = 0;
-#pragma unroll
-            for (int r = 0; r < 4; r++) acc += mri[r] * RES[g16 + r * 4 + pj];
-            TMP[lane] = sat16((acc + (1 << (S1 - 1))) >> S1);
-            wave_lds_sync();
-            acc = 0;
-#pragma unroll
-            for (int c = 0; c < 4; c++) acc += mrj[c] * TMP[g16 + pi * 4 + c];
-            const int lv = quant(sat16((acc + 128) >> 8), luma ? QL : QC);
+            // ---- forward transform, quantisation (cross-lane moves, no LDS)
+            const int t1 = sat16((col_pass(s - pred, fc) + (1 << (S1 - 1))) >> S1);
+            const int lv = quant(sat16((row_pass(t1, fr) + 128) >> 8), luma ? QL : QC);
             // ---- dequantisation, inverse, reconstruction
-            RES[lane] = dequant(lv, q, QSH);
-            wave_lds_sync();
-            acc = 0;
-#pragma unroll
-            for (int t = 0; t < 4; t++) acc += mci[t] * RES[g16 + t * 4 + pj];
-            TMP[lane] = sat16((acc + 64) >> 7);
-            wave_lds_sync();
-            acc = 0;
-#pragma unroll
-            for (int t = 0; t < 4; t++) acc += mcj[t] * TMP[g16 + pi * 4 + t];
-            const int rec = clampi(pred + sat16((acc + (1 << (S2INV - 1))) >> S2INV), 0, MAXV);
+            const int t2 = sat16((col_pass(dequant(lv, q, QSH), ic) + 64) >> 7);
+            const int rec = clampi(pred + sat16((row_pass(t2, ir) + (1 << (S2INV - 1))) >> S2INV), 0, MAXV);
             if (active) {
                 B[tile + (ry + pi + 1) * tp + 4 + (rx + pj)] = (Pix)rec;
                 const size_t cb = (size_t)(yrow * wtu + x) * 384;
