@@ -57,7 +57,14 @@ class GpuContext(C.Structure):
 
 class EncodeParams(C.Structure):
     _fields_ = [("search_range", C.c_int), ("lambda8", C.c_int), ("deblock_idc", C.c_int),
-                ("me_8x8", C.c_int), ("readback", C.c_int), ("codec", C.c_int)]
+                ("me_8x8", C.c_int), ("readback", C.c_int), ("codec", C.c_int), ("square_radius", C.c_int)]
+
+
+class CpuSimdMeConfig(C.Structure):
+    """cpu_simd_me_config_t (bc250_cpu_me.h)."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("width_in_mbs", C.c_uint32),
+                ("height_in_mbs", C.c_uint32), ("search_radius", C.c_uint32), ("num_threads", C.c_int),
+                ("core_ids", C.c_int * 2)]
 
 
 class StageStats(C.Structure):
@@ -123,6 +130,9 @@ _sig(hip, "bc250_launch_zero_i32", _I, [_P, _I, _P])
 _sig(hip, "bc250_deblock_work_bytes", C.c_size_t, [_I, _I])
 _sig(hip, "bc250_intra_work_bytes", C.c_size_t, [_I, _I])
 _sig(hip, "bc250_launch_pad_edges", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P])
+_sig(hip, "bc250_launch_square_me", _I, [_P, _I, _P, _I, _I, _I, _I, _P, _P])
+_sig(hip, "cpu_simd_sad_16x16", _U, [_P, _I, _P, _I])
+_sig(hip, "cpu_simd_me_search_frame", _I, [_P, _I, _P, _I, _U, _U, _P, _P])
 _sig(hip, "bc250_h264_pred4x4_batch", _I, [_P, _P, _I, _P, _P])
 _sig(hip, "bc250_h264_pred8x8_batch", _I, [_P, _P, _I, _P, _P])
 _sig(hip, "bc250_h264_tq8x8_batch", _I, [_P, _I, _I, _I, _P, _P])
@@ -215,7 +225,7 @@ class Encoder:
     """One gpu_compute context: upload frames, encode, read staging back."""
 
     def __init__(self, width, height, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0,
-                 bit_depth=8):
+                 bit_depth=8, square_radius=0):
         """bit_depth 10: P010 surfaces (HEVC Main10); upload() then takes uint16
         planes of 0..1023 samples and map_recon() returns them."""
         self.w, self.h = width, height
@@ -226,12 +236,13 @@ class Encoder:
         if hip.bc250_gpu_init(C.byref(self.ctx)) != 0:
             raise RuntimeError("bc250_gpu_init failed (no HIP device?)")
         self.codec = codec
-        self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback, codec)
+        self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius)
         self.images = []
 
-    def set_params(self, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0):
-        """codec: GPU_CODEC_H264 (0) or GPU_CODEC_HEVC (1)."""
-        p = EncodeParams(search_range, lambda8, deblock_idc, me_8x8, readback, codec)
+    def set_params(self, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0, square_radius=0):
+        """codec: GPU_CODEC_H264 (0) or GPU_CODEC_HEVC (1); square_radius 2..16
+        selects the reference's Tier-2 square search instead of the full one."""
+        p = EncodeParams(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius)
         if hip.gpu_compute_set_encode_params(C.byref(self.ctx), C.byref(p)) != 0:
             raise ValueError("invalid encode parameters")
 

@@ -67,7 +67,7 @@ struct bc250_hip_state {
     Slot slot[NSLOT];
     int cur = 0, submitted = 0;
     double last_latency_ms = 0.0;
-    gpu_encode_params_t params{64, 1, 0, 1, 1, GPU_CODEC_H264};
+    gpu_encode_params_t params{64, 1, 0, 1, 1, GPU_CODEC_H264, 0};
     int err_flags = 0;
     // per-stage GPU time accumulated at sync_slot: [stage] sums, frames seen
     double stage_sum[NSTAGE] = {0, 0, 0, 0, 0};
@@ -498,6 +498,14 @@ extern "C" int gpu_compute_begin_picture(gpu_context_t *ctx, gpu_image_t render_
 static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &ref, hipEvent_t mid)
 {
     const gpu_encode_params_t &p = s->params;
+    if (p.square_radius) {
+        // the reference's Tier-2 search: final quarter-sample vectors, no 8x8 ones
+        int r = bc250_launch_square_me((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->w, s->h,
+                                       p.square_radius, s->mvs, s->stream);
+        hipEventRecord(mid, s->stream);
+        if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, s->nmb * 16, s->stream);
+        return r;
+    }
     int r = bc250_launch_me_fullsearch((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc,
                                        p.search_range, p.lambda8, s->imv16, s->imv8, s->stream);
     hipEventRecord(mid, s->stream);
@@ -845,6 +853,7 @@ extern "C" int gpu_compute_set_encode_params(gpu_context_t *ctx, const gpu_encod
     if (p->search_range != 16 && p->search_range != 32 && p->search_range != 64) return -1;
     if (p->lambda8 < 0 || p->lambda8 > 15 || p->deblock_idc < 0 || p->deblock_idc > 2) return -1;
     if (p->codec != GPU_CODEC_H264 && p->codec != GPU_CODEC_HEVC) return -1;
+    if (p->square_radius != 0 && (p->square_radius < 2 || p->square_radius > 16)) return -1;
     s->params = *p;
     s->params.me_8x8 = p->me_8x8 ? 1 : 0;
     s->params.readback = p->readback ? 1 : 0;

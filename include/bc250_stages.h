@@ -95,6 +95,18 @@ int bc250_launch_zero_i32(int *p, int n, void *stream);
  * padding (w x h visible, wc x hc coded). */
 int bc250_launch_pad_edges(uint8_t *y, uint8_t *uv, int py, int puv, int w, int h, int wc, int hc, void *stream);
 
+/* The reference's Tier-2 motion search (src/cpu_simd_me.c:275-497,
+ * cpu_simd_me_search_frame) on the GPU, bit-exact: per MB in raster order the
+ * zero-vector SAD (<= 512 keeps it), the left neighbour's vector / 4 as a
+ * predictor (taken when it costs < 768), the 8-point square search with steps
+ * R/2 .. 1 and cost SAD + 5|mv|, then four half-sample candidates (rounded-up
+ * two-sample averages) when 128 < cost < 3000.  R = search_radius clamped to
+ * 2..16.  Planes MB-padded (pitch >= 16 * ceil(width / 16)); out_mvs:
+ * gpu_mv_t per MB {mvx, mvy (quarter samples), cost, 0}.  One wave per MB
+ * row (the rows are independent, MBs within a row are not). */
+int bc250_launch_square_me(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int width, int height,
+                           int search_radius, uint32_t *out_mvs, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -185,6 +185,9 @@ typedef struct {
     int me_8x8;        /* also refine the four 8x8 quadrant MVs (default 1) */
     int readback;      /* copy results into host staging at end_picture (default 1) */
     int codec;         /* GPU_CODEC_H264 (default) or GPU_CODEC_HEVC, see below */
+    int square_radius; /* H.264: 0 = the +-search_range full search (default); 2..16 = the reference's
+                        * Tier-2 square search (cpu_simd_me.c:275-497) on the GPU with this radius,
+                        * its vectors used as they come (no 8x8 vectors, no further refinement) */
 } gpu_encode_params_t;
 
 /* codec = GPU_CODEC_HEVC runs the HEVC stages of include/bc250_hevc.h

@@ -96,6 +96,19 @@ def me_fullsearch(cur, ref, rng, lambda8):
     return i16, i8
 
 
+_sig("orc_square_me", None, [_P, _I, _P, _I, _I, _I, _I, _P])
+
+
+def square_me(cur, ref, width, height, search_radius):
+    """The reference's Tier-2 square search over MB-padded planes; returns MV [nmb]."""
+    cur = np.ascontiguousarray(cur, np.uint8)
+    ref = np.ascontiguousarray(ref, np.uint8)
+    nmb = ((width + 15) // 16) * ((height + 15) // 16)
+    out = np.zeros(nmb, MV)
+    lib.orc_square_me(p(cur), cur.strides[0], p(ref), ref.strides[0], width, height, search_radius, p(out))
+    return out
+
+
 def me_fullsearch_mb(cur, ref, rng, lambda8, mbx, mby):
     cur = np.ascontiguousarray(cur, np.uint8)
     ref = np.ascontiguousarray(ref, np.uint8)

@@ -1115,3 +1115,125 @@ void orc_h264_dqit8x8(const int16_t lv[64], int qp, int16_t out[64])
     orc_h264_inv8x8(d, r);
     for (int p = 0; p < 64; p++) out[p] = (int16_t)r[p];
 }
+
+/* ---------------------------------------------------------------------------
+ * The reference's Tier-2 motion search, cpu_simd_me_search_frame
+ * (src/cpu_simd_me.c:275-497; the one the H.264 encoder runs on the CPU and
+ * hands to dispatch_encode_ext as cpu_mvs, encoder_h264.c:2360-2382).  Per
+ * 16x16 MB, in raster order (the left neighbour's result feeds the next MB):
+ *   1. SAD at (0,0); <= 512 -> (0,0).
+ *   2. Left neighbour's vector / 4 (C division, towards zero), if non-zero,
+ *      within +-R and inside the MB-padded frame: cost SAD + 5*|mv|; if it
+ *      beats the zero SAD and costs < 768, take it and stop.
+ *   3. Square search around the best, steps R/2, R/4, .. 1, the 8 offsets
+ *      (0,1) (0,-1) (1,0) (-1,0) (1,1) (-1,-1) (1,-1) (-1,1), cost SAD +
+ *      5*|mv|, strict '<'; stop early at cost 0.
+ *   4. If 128 < cost < 3000 and the block has a one-sample margin, the four
+ *      half-sample candidates (+-2, 0) (0, +-2) (quarter units) predicted by
+ *      the rounded-up average of two neighbouring integer samples, compared
+ *      by plain SAD against the step-3 cost; strict '<'.
+ * R = search_radius clamped to 2..16.  out: {mvx, mvy (quarter samples),
+ * cost, 0} per MB.  Planes hold at least the MB-padded frame.
+ * ------------------------------------------------------------------------- */
+static uint32_t sad16x16(const uint8_t *a, int ap, const uint8_t *b, int bp)
+{
+    uint32_t s = 0;
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++) s += (uint32_t)abs((int)a[r * ap + c] - (int)b[r * bp + c]);
+    return s;
+}
+
+void orc_square_me(const uint8_t *src, int sp, const uint8_t *ref, int rp, int width, int height, int search_radius,
+                   int32_t *out)
+{
+    static const int pat[8][2] = {{0, 1}, {0, -1}, {1, 0}, {-1, 0}, {1, 1}, {-1, -1}, {1, -1}, {-1, 1}};
+    const int wmb = (width + 15) / 16, hmb = (height + 15) / 16;
+    const int R = search_radius < 2 ? 2 : (search_radius > 16 ? 16 : search_radius);
+    const int max_x = wmb * 16, max_y = hmb * 16;
+    const uint32_t lambda = 5;
+    for (int mby = 0; mby < hmb; mby++)
+        for (int mbx = 0; mbx < wmb; mbx++) {
+            const int mb = mby * wmb + mbx, px = mbx * 16, py = mby * 16;
+            int32_t *o = out + 4 * mb;
+            const uint8_t *cur = src + (size_t)py * sp + px;
+            const uint32_t zero = sad16x16(cur, sp, ref + (size_t)py * rp + px, rp);
+            if (zero <= 512) {
+                o[0] = o[1] = 0;
+                o[2] = (int32_t)zero;
+                o[3] = 0;
+                continue;
+            }
+            int bx = 0, by = 0;
+            uint32_t best = zero;
+            if (mbx > 0) {
+                const int lx = o[-4] / 4, ly = o[-3] / 4;
+                if ((lx || ly) && abs(lx) <= R && abs(ly) <= R) {
+                    const int tx = px + lx, ty = py + ly;
+                    if (tx >= 0 && tx + 16 <= max_x && ty >= 0 && ty + 16 <= max_y) {
+                        const uint32_t c = sad16x16(cur, sp, ref + (size_t)ty * rp + tx, rp) +
+                                           lambda * (uint32_t)(abs(lx) + abs(ly));
+                        if (c < best) {
+                            best = c;
+                            bx = lx;
+                            by = ly;
+                        }
+                    }
+                }
+            }
+            if ((bx || by) && best < 768) {
+                o[0] = bx * 4;
+                o[1] = by * 4;
+                o[2] = (int32_t)best;
+                o[3] = 0;
+                continue;
+            }
+            for (int step = R / 2; step >= 1; step /= 2) {
+                const int cx = bx, cy = by;
+                for (int k = 0; k < 8; k++) {
+                    const int x = cx + pat[k][0] * step, y = cy + pat[k][1] * step;
+                    if (abs(x) > R || abs(y) > R) continue;
+                    const int tx = px + x, ty = py + y;
+                    if (tx < 0 || tx + 16 > max_x || ty < 0 || ty + 16 > max_y) continue;
+                    const uint32_t c = sad16x16(cur, sp, ref + (size_t)ty * rp + tx, rp) + lambda * (uint32_t)(abs(x) + abs(y));
+                    if (c < best) {
+                        best = c;
+                        bx = x;
+                        by = y;
+                    }
+                }
+                if (best == 0) break;
+            }
+            int qx = bx * 4, qy = by * 4;
+            if (best > 128 && best < 3000) {
+                const int ox = px + bx, oy = py + by;
+                if (ox > 0 && ox + 17 < max_x && oy > 0 && oy + 17 < max_y) {
+                    static const int hp[4][2] = {{2, 0}, {-2, 0}, {0, 2}, {0, -2}};
+                    int sx_best = 0, sy_best = 0;
+                    uint32_t low = best;
+                    for (int h = 0; h < 4; h++) {
+                        const int dx = hp[h][0] > 0 ? 1 : (hp[h][0] < 0 ? -1 : 0);
+                        const int dy = hp[h][1] > 0 ? 1 : (hp[h][1] < 0 ? -1 : 0);
+                        uint32_t s = 0;
+                        for (int r = 0; r < 16; r++)
+                            for (int c = 0; c < 16; c++) {
+                                const int a = ref[(size_t)(oy + r) * rp + ox + c];
+                                const int b = ref[(size_t)(oy + r + dy) * rp + ox + c + dx];
+                                s += (uint32_t)abs((int)cur[r * sp + c] - ((a + b + 1) >> 1));
+                            }
+                        if (s < low) {
+                            low = s;
+                            sx_best = hp[h][0];
+                            sy_best = hp[h][1];
+                        }
+                    }
+                    qx += sx_best;
+                    qy += sy_best;
+                    best = low;
+                }
+            }
+            o[0] = qx;
+            o[1] = qy;
+            o[2] = (int32_t)best;
+            o[3] = 0;
+        }
+}

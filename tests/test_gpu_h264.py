@@ -31,6 +31,10 @@ def host(t):
     return t.cpu().numpy()
 
 
+def P(t):
+    return C.c_void_p(t.data_ptr())
+
+
 def frames(pkg, seed, w, h, n, nrect):
     g = pkg.FrameGen(seed, w, h, nrect)
     out = [pkg.pad_coded(*g.frame(i)) for i in range(n)]
@@ -453,3 +457,92 @@ def test_t8x8_batch(pkg, qp):
             want = orc.h264_tq8x8(res[b], qp, rnd)
             assert np.array_equal(lv[b], want), (b, rnd)
             assert np.array_equal(back[b], orc.h264_dqit8x8(want, qp)), b
+
+
+# ------------------------------------------------- the reference's Tier-2 search
+
+@pytest.mark.parametrize("w,h,radius", [(128, 96, 2), (200, 120, 8), (320, 240, 16), (64, 64, 8)])
+def test_square_me_stage(pkg, w, h, radius):
+    torch = _torch()
+    say(f"square me {w}x{h} R={radius}")
+    fr = frames(pkg, 60 + radius, w, h, 2, 4)
+    (ry, _), (cy, _) = fr
+    want = orc.square_me(cy, ry, w, h, radius)
+    hc, wc = cy.shape
+    dcy, dry = dev(torch, cy), dev(torch, ry)
+    out = torch.zeros(want.shape[0] * 4, dtype=torch.int32, device="cuda")
+    assert pkg.hip.bc250_launch_square_me(P(dcy), wc, P(dry), wc, w, h, radius, P(out), None) == 0
+    torch.cuda.synchronize()
+    got = host(out).view(orc.MV)
+    np.testing.assert_array_equal(got, want)
+    assert w < 128 or (want["mvx"] != 0).sum() > 3
+    # a ramp moved by half a sample: zero-vector SAD 768, every half-sample
+    # branch of the search taken
+    xx = np.arange(wc)[None, :].repeat(hc, 0)
+    r2 = np.clip(20 + 6 * (xx % 32), 0, 255).astype(np.uint8)
+    c2 = ((r2.astype(int) + np.roll(r2, -1, 1) + 1) >> 1).astype(np.uint8)
+    want = orc.square_me(c2, r2, w, h, radius)
+    assert ((want["mvx"] % 4) != 0).sum() > 0
+    dcy, dry = dev(torch, c2), dev(torch, r2)
+    assert pkg.hip.bc250_launch_square_me(P(dcy), wc, P(dry), wc, w, h, radius, P(out), None) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(host(out).view(orc.MV), want)
+
+
+def test_square_me_reference_interface(pkg):
+    """cpu_simd_me_search_frame through the reference-named C entry points
+    (bc250_cpu_me.h), on the reference test's frame: one textured MB moved by
+    (+2, +2) is found at (8, 8) quarter samples; a flat +10 offset costs 2560."""
+    _torch()
+    cur = np.full((64, 64), 100, np.uint8)
+    ref = cur.copy()
+    yy, xx = np.mgrid[0:16, 0:16]
+    cur[16:32, 16:32] = (xx * 13 + yy * 7 + 40) % 256
+    ref[18:34, 18:34] = cur[16:32, 16:32]
+    cfg = pkg.CpuSimdMeConfig()
+    pkg.hip.cpu_simd_me_config_init.argtypes = [C.c_void_p, C.c_uint, C.c_uint]
+    pkg.hip.cpu_simd_me_config_init(C.addressof(cfg), 64, 64)
+    assert cfg.width_in_mbs == 4 and cfg.core_ids[0] == -1 and cfg.search_radius == 0
+    cfg.search_radius = 8
+    mvs = np.zeros(16, orc.MV)
+    assert pkg.hip.cpu_simd_me_search_frame(cur.ctypes.data, 64, ref.ctypes.data, 64, 64, 64, mvs.ctypes.data,
+                                            C.addressof(cfg)) == 0
+    assert (mvs[5]["mvx"], mvs[5]["mvy"]) == (8, 8) and (mvs[0]["mvx"], mvs[0]["mvy"]) == (0, 0)
+    np.testing.assert_array_equal(mvs, orc.square_me(cur, ref, 64, 64, 8))
+    a = np.full((64, 64), 128, np.uint8)
+    b = np.full((64, 64), 138, np.uint8)
+    assert pkg.hip.cpu_simd_sad_16x16(a.ctypes.data, 64, b.ctypes.data, 64) == 2560
+
+
+def test_pipeline_square_me(pkg):
+    """The H.264 pipeline with the Tier-2 search (square_radius 2, the radius
+    the reference's encoder runs with) in place of the full search."""
+    _torch()
+    w, h, qp, ns, R = 160, 96, 30, 2, 2
+    fr = frames(pkg, 23, w, h, 4, 3)
+    g = pkg.FrameGen(23, w, h, 3)
+    enc = pkg.Encoder(w, h, deblock_idc=0, square_radius=R)
+    ref = None
+    try:
+        for i in range(4):
+            im = enc.new_image()
+            enc.upload(im, *g.frame(i))
+            got = enc.results(enc.encode(im, qp, i == 0, ns))
+            cy, cuv = fr[i]
+            if i == 0:
+                e = orc.encode_i(cy, cuv, qp, ns)
+                mvs = np.zeros((w // 16) * (h // 16), orc.MV)
+            else:
+                mvs = orc.square_me(cy, ref[0], w, h, R)
+                e = orc.encode_p(cy, cuv, ref[0], ref[1], qp, mvs)
+            y, uv = orc.deblock(e["y"], e["uv"], qp, i == 0, e["nz"], mvs, ns, 0)
+            np.testing.assert_array_equal(got["mv"], mvs, err_msg=f"frame {i} mvs")
+            np.testing.assert_array_equal(got["quant"], e["levels"], err_msg=f"frame {i} levels")
+            my, muv = enc.map_recon()
+            np.testing.assert_array_equal(my, y, err_msg=f"frame {i} recon")
+            np.testing.assert_array_equal(muv, uv, err_msg=f"frame {i} recon uv")
+            ref = (y, uv)
+            say(f"  frame {i} ok")
+    finally:
+        enc.close()
+        g.close()

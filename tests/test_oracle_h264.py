@@ -588,3 +588,74 @@ def test_8x8_quantiser_inverts_the_decoder():
         lv = orc.h264_tq8x8(np.full(64, v, np.int16), 0, 6)
         assert np.count_nonzero(lv[1:]) == 0
         assert np.array_equal(orc.h264_dqit8x8(lv, 0), np.full(64, v, np.int16))
+
+
+# ------------------------------------------------- the reference's Tier-2 search
+
+def _sq_cost(cur, ref, x0, y0, mvx, mvy):
+    """Cost of a square-search result recomputed from the planes: SAD + 5|mv|
+    for an integer vector, plain SAD against the rounded-up average of the
+    two integer samples around a half-sample one."""
+    c = cur[y0:y0 + 16, x0:x0 + 16].astype(int)
+    bx, by = mvx // 4, mvy // 4
+    if mvx % 4 == 0 and mvy % 4 == 0:
+        r = ref[y0 + by:y0 + by + 16, x0 + bx:x0 + bx + 16].astype(int)
+        return int(np.abs(c - r).sum()) + 5 * (abs(bx) + abs(by))
+    hx, hy = (1, 0) if mvx % 4 else (0, 1)
+    a = ref[y0 + by:y0 + by + 16, x0 + bx:x0 + bx + 16].astype(int)
+    b = ref[y0 + by + hy:y0 + by + hy + 16, x0 + bx + hx:x0 + bx + hx + 16].astype(int)
+    return int(np.abs(c - ((a + b + 1) >> 1)).sum())
+
+
+def test_square_me_reference_fixtures():
+    """The expectations of the reference's own tests (tests/test_cpu_simd_me.c:
+    16-224), restated on the oracle."""
+    # flat frames: static below the threshold; a uniform +10 offset costs 2560
+    # everywhere, so (0,0) wins every strict comparison
+    a = np.full((64, 64), 128, np.uint8)
+    m = orc.square_me(a, a, 64, 64, 8)
+    assert (m["mvx"] == 0).all() and (m["sad"] == 0).all()
+    m = orc.square_me(a, np.full((64, 64), 138, np.uint8), 64, 64, 8)
+    assert (m["mvx"] == 0).all() and (m["mvy"] == 0).all() and (m["sad"] == 2560).all()
+    # one textured MB moved by (+2,+2): found at (8, 8) quarter samples
+    cur = np.full((64, 64), 100, np.uint8)
+    ref = cur.copy()
+    yy, xx = np.mgrid[0:16, 0:16]
+    cur[16:32, 16:32] = (xx * 13 + yy * 7 + 40) % 256
+    ref[18:34, 18:34] = cur[16:32, 16:32]
+    m = orc.square_me(cur, ref, 64, 64, 8)
+    assert (m[0]["mvx"], m[0]["mvy"]) == (0, 0)
+    assert (m[5]["mvx"], m[5]["mvy"]) == (8, 8)
+    # two neighbours moved alike: the second takes the left predictor
+    cur = np.full((64, 64), 120, np.uint8)
+    ref = cur.copy()
+    for mbx in (1, 2):
+        cur[16:32, mbx * 16:mbx * 16 + 16] = (xx * 13 + yy * 7 + mbx * 30) % 256
+        ref[18:34, mbx * 16 + 2:mbx * 16 + 18] = cur[16:32, mbx * 16:mbx * 16 + 16]
+    m = orc.square_me(cur, ref, 64, 64, 8)
+    assert (m[5]["mvx"], m[5]["mvy"]) == (8, 8) and (m[6]["mvx"], m[6]["mvy"]) == (8, 8) and m[6]["sad"] < 100
+    # a height that is not a multiple of 16 (MB-padded planes)
+    m = orc.square_me(np.zeros((64, 32), np.uint8), np.zeros((64, 32), np.uint8), 32, 50, 8)
+    assert m.shape == (8,) and (m["sad"] == 0).all()
+
+
+@pytest.mark.parametrize("radius", [2, 8, 16])
+def test_square_me_costs_recompute(pkg, radius):
+    """On generated frames every reported cost equals the cost recomputed at
+    the reported vector, vectors stay within the radius (+ a half sample),
+    and no result costs more than the zero vector."""
+    g = pkg.FrameGen(9, 160, 96, 4)
+    (cy, _), (ry, _) = [pkg.pad_coded(*g.frame(i)) for i in (1, 0)]
+    g.close()
+    m = orc.square_me(cy, ry, 160, 96, radius)
+    wmb = 10
+    moved = 0
+    for i, v in enumerate(m):
+        x0, y0 = (i % wmb) * 16, (i // wmb) * 16
+        mvx, mvy, sad = int(v["mvx"]), int(v["mvy"]), int(v["sad"])
+        assert abs(mvx) <= 4 * radius + 2 and abs(mvy) <= 4 * radius + 2
+        assert _sq_cost(cy, ry, x0, y0, mvx, mvy) == sad, (i, mvx, mvy, sad)
+        zero = int(np.abs(cy[y0:y0 + 16, x0:x0 + 16].astype(int) - ry[y0:y0 + 16, x0:x0 + 16]).sum())
+        assert sad <= zero
+        moved += (mvx, mvy) != (0, 0)
+    assert moved > 10
