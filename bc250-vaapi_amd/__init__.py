@@ -57,7 +57,8 @@ class GpuContext(C.Structure):
 
 class EncodeParams(C.Structure):
     _fields_ = [("search_range", C.c_int), ("lambda8", C.c_int), ("deblock_idc", C.c_int),
-                ("me_8x8", C.c_int), ("readback", C.c_int), ("codec", C.c_int), ("square_radius", C.c_int)]
+                ("me_8x8", C.c_int), ("readback", C.c_int), ("codec", C.c_int), ("square_radius", C.c_int),
+                ("diamond", C.c_int)]
 
 
 class CpuSimdMeConfig(C.Structure):
@@ -131,6 +132,7 @@ _sig(hip, "bc250_deblock_work_bytes", C.c_size_t, [_I, _I])
 _sig(hip, "bc250_intra_work_bytes", C.c_size_t, [_I, _I])
 _sig(hip, "bc250_launch_pad_edges", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P])
 _sig(hip, "bc250_launch_square_me", _I, [_P, _I, _P, _I, _I, _I, _I, _P, _P])
+_sig(hip, "bc250_launch_diamond_me", _I, [_P, _I, _P, _I, _I, _I, _I, _P, _P])
 _sig(hip, "cpu_simd_sad_16x16", _U, [_P, _I, _P, _I])
 _sig(hip, "cpu_simd_me_search_frame", _I, [_P, _I, _P, _I, _U, _U, _P, _P])
 _sig(hip, "bc250_h264_pred4x4_batch", _I, [_P, _P, _I, _P, _P])
@@ -225,7 +227,7 @@ class Encoder:
     """One gpu_compute context: upload frames, encode, read staging back."""
 
     def __init__(self, width, height, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0,
-                 bit_depth=8, square_radius=0):
+                 bit_depth=8, square_radius=0, diamond=0):
         """bit_depth 10: P010 surfaces (HEVC Main10); upload() then takes uint16
         planes of 0..1023 samples and map_recon() returns them."""
         self.w, self.h = width, height
@@ -236,13 +238,15 @@ class Encoder:
         if hip.bc250_gpu_init(C.byref(self.ctx)) != 0:
             raise RuntimeError("bc250_gpu_init failed (no HIP device?)")
         self.codec = codec
-        self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius)
+        self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius, diamond)
         self.images = []
 
-    def set_params(self, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0, square_radius=0):
+    def set_params(self, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0, square_radius=0,
+                   diamond=0):
         """codec: GPU_CODEC_H264 (0) or GPU_CODEC_HEVC (1); square_radius 2..16
-        selects the reference's Tier-2 square search instead of the full one."""
-        p = EncodeParams(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius)
+        selects the reference's Tier-2 square search, diamond 1 its GPU
+        diamond search, instead of the full one."""
+        p = EncodeParams(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius, diamond)
         if hip.gpu_compute_set_encode_params(C.byref(self.ctx), C.byref(p)) != 0:
             raise ValueError("invalid encode parameters")
 
@@ -268,7 +272,7 @@ class Encoder:
              self.h) != 0:
             raise RuntimeError("upload failed")
 
-    def encode(self, img_mem, qp, is_intra, num_slices=1, cpu_mvs=None):
+    def encode(self, img_mem, qp, is_intra, num_slices=1, cpu_mvs=None, me_mode=0):
         """Submit one frame and wait for it; returns the slot."""
         img, _ = img_mem
         c = C.byref(self.ctx)
@@ -278,7 +282,8 @@ class Encoder:
         if cpu_mvs is not None:
             cpu_mvs = np.ascontiguousarray(cpu_mvs, dtype=MV_DTYPE)
             mvp = _ptr(cpu_mvs)
-        if hip.gpu_compute_dispatch_encode_ext(c, img, self.w, self.h, qp, int(is_intra), num_slices, 0, mvp) != 0:
+        if hip.gpu_compute_dispatch_encode_ext(c, img, self.w, self.h, qp, int(is_intra), num_slices, me_mode,
+                                               mvp) != 0:
             raise RuntimeError("dispatch_encode_ext failed")
         if hip.gpu_compute_end_picture(c) != 0:
             raise RuntimeError("end_picture failed")

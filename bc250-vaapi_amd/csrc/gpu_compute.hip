@@ -67,7 +67,7 @@ struct bc250_hip_state {
     Slot slot[NSLOT];
     int cur = 0, submitted = 0;
     double last_latency_ms = 0.0;
-    gpu_encode_params_t params{64, 1, 0, 1, 1, GPU_CODEC_H264, 0};
+    gpu_encode_params_t params{64, 1, 0, 1, 1, GPU_CODEC_H264, 0, 0};
     int err_flags = 0;
     // per-stage GPU time accumulated at sync_slot: [stage] sums, frames seen
     double stage_sum[NSTAGE] = {0, 0, 0, 0, 0};
@@ -495,9 +495,17 @@ extern "C" int gpu_compute_begin_picture(gpu_context_t *ctx, gpu_image_t render_
     return 0;
 }
 
-static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &ref, hipEvent_t mid)
+static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &ref, int me_mode, hipEvent_t mid)
 {
     const gpu_encode_params_t &p = s->params;
+    if (p.diamond) {
+        // the reference's diamond search: final quarter-sample vectors, no 8x8 ones
+        int r = bc250_launch_diamond_me((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc,
+                                        me_mode == 1 ? 8 : 5, s->mvs, s->stream);
+        hipEventRecord(mid, s->stream);
+        if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, s->nmb * 16, s->stream);
+        return r;
+    }
     if (p.square_radius) {
         // the reference's Tier-2 search: final quarter-sample vectors, no 8x8 ones
         int r = bc250_launch_square_me((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->w, s->h,
@@ -589,7 +597,6 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
 extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t rt, int width, int height, int qp,
                                                int is_intra, int num_slices, int me_mode, const gpu_mv_t *cpu_mvs)
 {
-    (void)me_mode;
     bc250_hip_state *s = st_of(ctx);
     if (!s || !rt.y_plane || qp < 0 || qp > 51) return -1;
     if (num_slices < 1) num_slices = 1;
@@ -637,7 +644,7 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     int r = 0;
     hipEventRecord(sl.ev[0], st);
     if (!is_intra && !cpu_mvs) {
-        r = enqueue_me(s, rt, ref, sl.ev[1]);
+        r = enqueue_me(s, rt, ref, me_mode, sl.ev[1]);
     } else {
         hipEventRecord(sl.ev[1], st);
         if (!is_intra)
@@ -719,7 +726,7 @@ extern "C" int gpu_compute_dispatch_me_only(gpu_context_t *ctx, gpu_image_t rt, 
     cur_in.py = rt.y_pitch;
     const Plane2 &ref = s->has_recon ? s->recon[s->ref_idx] : cur_in;
     hipEventRecord(sl.ev[0], s->stream);
-    int r = enqueue_me(s, rt, ref, sl.ev[1]);
+    int r = enqueue_me(s, rt, ref, 0, sl.ev[1]);
     for (int i = 2; i < NEV - 1; i++) hipEventRecord(sl.ev[i], s->stream);
     if (r == 0 && s->params.readback) {
         bool ok = HIP_OK(hipMemcpyAsync(sl.mvs, s->mvs, (size_t)s->nmb * 16, hipMemcpyDeviceToHost, s->stream)) &&
@@ -854,6 +861,8 @@ extern "C" int gpu_compute_set_encode_params(gpu_context_t *ctx, const gpu_encod
     if (p->lambda8 < 0 || p->lambda8 > 15 || p->deblock_idc < 0 || p->deblock_idc > 2) return -1;
     if (p->codec != GPU_CODEC_H264 && p->codec != GPU_CODEC_HEVC) return -1;
     if (p->square_radius != 0 && (p->square_radius < 2 || p->square_radius > 16)) return -1;
+    if (p->diamond != 0 && p->diamond != 1) return -1;
+    if (p->diamond && p->square_radius) return -1;  // one search at a time
     s->params = *p;
     s->params.me_8x8 = p->me_8x8 ? 1 : 0;
     s->params.readback = p->readback ? 1 : 0;

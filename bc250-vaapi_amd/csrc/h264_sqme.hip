@@ -1,7 +1,12 @@
-// h264_sqme.hip - the reference's Tier-2 motion search (the CPU SIMD route
-// of src/cpu_simd_me.c:275-497, whose vectors the H.264 encoder hands to
-// dispatch_encode_ext as cpu_mvs) as a gfx950 kernel.
+// h264_sqme.hip - the reference's own two motion searches as gfx950 kernels,
+// beside the full search of me_kernels.hip:
+//   * the Tier-2 square search (the CPU SIMD route of src/cpu_simd_me.c:
+//     275-497, whose vectors the H.264 encoder hands to dispatch_encode_ext
+//     as cpu_mvs), bit-exact;
+//   * the diamond search of shaders/motion_estimation.comp:278-469 with exact
+//     integer SADs in place of the shader's float-normalised ones.
 //
+// Square search:
 // The search is serial along an MB row (each MB starts from its left
 // neighbour's vector), so one wave runs one MB row and rows run in parallel.
 // Per MB the wave stages the reference window the search can touch
@@ -196,6 +201,107 @@ extern "C" int bc250_launch_square_me(const uint8_t *cur, int cpitch, const uint
     const int R = search_radius < 2 ? 2 : (search_radius > RMAX ? RMAX : search_radius);
     h264_square_me_kernel<<<hmb, 64, 0, (hipStream_t)stream>>>(cur, cpitch, ref, rpitch, wmb, hmb, R,
                                                                (uint4 *)out_mvs);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------
+// Diamond search (motion_estimation.comp:278-469): one wave per MB, 4 MBs
+// per workgroup, every MB independent.  The window the search and its
+// sub-sample refinement can touch, (x0 - 20 .. x0 + 35)^2 with replicated
+// edges, is staged in LDS; lane l owns pixels (l / 4, 4 (l % 4) .. + 3).
+// Integer candidates: 2 x the SAD over pixels with even x + y (a byte mask
+// per row parity) + lambda |mv|; zero-vector check on the full SAD; strict
+// '<' throughout, in the shader's order.
+namespace {
+constexpr int DW = 56;  // diamond window side and pitch
+}
+
+__global__ __launch_bounds__(256) void h264_diamond_me_kernel(const uint8_t *__restrict__ cur, int cpitch,
+                                                              const uint8_t *__restrict__ ref, int rpitch, int wc,
+                                                              int hc, int nmb, int lambda, uint4 *__restrict__ out)
+{
+    __shared__ uint32_t wins[4][DW * DW / 4 + 2];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int mb = blockIdx.x * 4 + wv;
+    if (mb >= nmb) return;
+    const int mbw = wc / 16, x0 = (mb % mbw) * 16, y0 = (mb / mbw) * 16;
+    uint32_t *win = wins[wv];
+    uint8_t *wb = (uint8_t *)win;
+    for (int t = lane; t < DW * DW; t += 64) {
+        const int wy = t / DW, wx = t - wy * DW;
+        wb[t] = ref[(size_t)clampi(y0 - 20 + wy, 0, hc - 1) * rpitch + clampi(x0 - 20 + wx, 0, wc - 1)];
+    }
+    const int r = lane >> 2, c0 = (lane & 3) * 4;
+    const uint32_t cw = *(const uint32_t *)(cur + (size_t)(y0 + r) * cpitch + x0 + c0);
+    const uint32_t even = (r & 1) ? 0xFF00FF00u : 0x00FF00FFu;  // bytes with even x + y
+    wave_lds_sync();
+    auto ref4 = [&](int x, int y) -> uint32_t {
+        const int o = (r + 20 + y) * DW + c0 + 20 + x;
+        return __builtin_amdgcn_alignbyte(win[(o >> 2) + 1], win[o >> 2], (uint32_t)(o & 3));
+    };
+    const uint32_t zero = wave_sum_u32(__builtin_amdgcn_sad_u8(ref4(0, 0), cw, 0));
+    uint32_t best = zero;
+    int qx = 0, qy = 0;
+    if (zero > 512) {
+        int bx = 0, by = 0;
+        for (int step = 8; step >= 1; step /= 2) {
+            const int cx = bx, cy = by;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int x = cx + (k < 2 ? 0 : (k == 2 ? step : -step));
+                const int y = cy + (k == 0 ? step : (k == 1 ? -step : 0));
+                if (iabs(x) > 16 || iabs(y) > 16) continue;  // wave-uniform
+                const uint32_t s = 2u * wave_sum_u32(__builtin_amdgcn_sad_u8(ref4(x, y) & even, cw & even, 0));
+                const uint32_t cost = s + (uint32_t)lambda * (uint32_t)(iabs(x) + iabs(y));
+                if (cost < best) {
+                    best = cost;
+                    bx = x;
+                    by = y;
+                }
+            }
+            if (best < 768) break;
+        }
+        qx = bx * 4;
+        qy = by * 4;
+        // half, then quarter samples around the running winner
+        for (int stage = 0; stage < 2; stage++) {
+            const int ax = qx, ay = qy;
+            int sx = 0, sy = 0;
+#pragma unroll 1
+            for (int k = 0; k < 8; k++) {
+                const int hx = k < 4 ? (k == 0 ? 2 : (k == 1 ? -2 : 0)) : (k < 6 ? 2 : -2);
+                const int hy = k < 4 ? (k == 2 ? 2 : (k == 3 ? -2 : 0)) : ((k & 1) ? -2 : 2);
+                const int ox = stage ? hx / 2 : hx, oy = stage ? hy / 2 : hy;
+                const int vx = ax + ox, vy = ay + oy;
+                uint32_t d = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int p = qpel_win(wb, DW, c0 + i + 20 + (vx >> 2), r + 20 + (vy >> 2), vx & 3, vy & 3);
+                    d += (uint32_t)iabs((int)((cw >> (8 * i)) & 255u) - p);
+                }
+                const uint32_t s = wave_sum_u32(d);
+                if (s < best) {
+                    best = s;
+                    sx = ox;
+                    sy = oy;
+                }
+            }
+            qx = ax + sx;
+            qy = ay + sy;
+        }
+    }
+    if (lane == 0) out[mb] = make_uint4((uint32_t)qx, (uint32_t)qy, best, 0u);
+}
+
+extern "C" int bc250_launch_diamond_me(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
+                                       int hc, int lambda, uint32_t *out_mvs, void *stream)
+{
+    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || (cpitch & 3) || cpitch < wc || rpitch < wc || lambda < 0 ||
+        lambda > 255)
+        return -1;
+    const int nmb = (wc / 16) * (hc / 16);
+    h264_diamond_me_kernel<<<(nmb + 3) / 4, 256, 0, (hipStream_t)stream>>>(cur, cpitch, ref, rpitch, wc, hc, nmb, lambda,
+                                                                    (uint4 *)out_mvs);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -107,6 +107,16 @@ int bc250_launch_pad_edges(uint8_t *y, uint8_t *uv, int py, int puv, int w, int 
 int bc250_launch_square_me(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int width, int height,
                            int search_radius, uint32_t *out_mvs, void *stream);
 
+/* The reference's GPU motion search (shaders/motion_estimation.comp:
+ * 278-469) with exact integer SADs in place of its float-normalised ones:
+ * zero-vector SAD (<= 512 keeps it), diamond (0,+-1) (+-1,0) at steps 8, 4,
+ * 2, 1 within +-16 with cost 2 x even-parity SAD + lambda |mv| (stop after a
+ * step below 768), then 8 half- and 8 quarter-sample candidates by plain SAD
+ * of the 8.4.2.2.1 samples.  lambda 5, or 8 in the reference's fast mode.
+ * Coded-size planes (edges replicated); out_mvs: gpu_mv_t per MB. */
+int bc250_launch_diamond_me(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
+                            int lambda, uint32_t *out_mvs, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

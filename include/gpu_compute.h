@@ -188,6 +188,9 @@ typedef struct {
     int square_radius; /* H.264: 0 = the +-search_range full search (default); 2..16 = the reference's
                         * Tier-2 square search (cpu_simd_me.c:275-497) on the GPU with this radius,
                         * its vectors used as they come (no 8x8 vectors, no further refinement) */
+    int diamond;       /* H.264: 1 = the reference's diamond search (motion_estimation.comp) with integer
+                        * SADs in place of the full search: lambda 5, or 8 when dispatch_encode_ext's
+                        * me_mode is 1 (the reference's fast mode); no 8x8 vectors */
 } gpu_encode_params_t;
 
 /* codec = GPU_CODEC_HEVC runs the HEVC stages of include/bc250_hevc.h

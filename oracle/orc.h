@@ -57,6 +57,8 @@ void orc_pred_luma_block(const uint8_t *ref, int py, int wc, int hc,
 /* the reference's Tier-2 square search (cpu_simd_me_search_frame); out: 4 int32 per MB */
 void orc_square_me(const uint8_t *src, int sp, const uint8_t *ref, int rp, int width, int height, int search_radius,
                    int32_t *out);
+/* the reference's diamond search (motion_estimation.comp) with integer SADs; out: 4 int32 per MB */
+void orc_diamond_me(const uint8_t *cur, const uint8_t *ref, int py, int wc, int hc, int lambda, int32_t *out);
 void orc_me_fullsearch(const uint8_t *cur, const uint8_t *ref, int py,
                        int wc, int hc, int range, int lambda8,
                        orc_imv_t *int16, orc_imv_t *int8);

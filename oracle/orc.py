@@ -109,6 +109,19 @@ def square_me(cur, ref, width, height, search_radius):
     return out
 
 
+_sig("orc_diamond_me", None, [_P, _P, _I, _I, _I, _I, _P])
+
+
+def diamond_me(cur, ref, lam=5):
+    """The reference's diamond search with integer SADs over coded-size planes; MV [nmb]."""
+    cur = np.ascontiguousarray(cur, np.uint8)
+    ref = np.ascontiguousarray(ref, np.uint8)
+    hc, wc = cur.shape
+    out = np.zeros((wc // 16) * (hc // 16), MV)
+    lib.orc_diamond_me(p(cur), p(ref), wc, wc, hc, lam, p(out))
+    return out
+
+
 def me_fullsearch_mb(cur, ref, rng, lambda8, mbx, mby):
     cur = np.ascontiguousarray(cur, np.uint8)
     ref = np.ascontiguousarray(ref, np.uint8)

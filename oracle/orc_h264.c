@@ -1237,3 +1237,87 @@ void orc_square_me(const uint8_t *src, int sp, const uint8_t *ref, int rp, int w
             o[3] = 0;
         }
 }
+
+/* ---------------------------------------------------------------------------
+ * The reference's GPU motion search, motion_estimation.comp:278-469, with
+ * exact integer SADs in place of its float-normalised, truncated ones.  Per
+ * MB (independent of every other MB):
+ *   1. full SAD at (0,0) (clamped reference); <= 512 -> (0,0) with that SAD.
+ *   2. diamond (0,1) (0,-1) (1,0) (-1,0) at steps 8, 4, 2, 1 around the step's
+ *      centre, within +-16; cost = 2 x (SAD over the pixels with even x + y)
+ *      + lambda * |mv|, compared with the best cost so far (the zero SAD at
+ *      first), strict '<'; stop after a step whose best cost is < 768.
+ *   3. 8 half-sample candidates (+-2,0) (0,+-2) (2,2) (2,-2) (-2,2) (-2,-2)
+ *      around the integer winner, then 8 quarter-sample ones (+-1 ...) around
+ *      the half winner, plain SAD of the 8.4.2.2.1 samples, strict '<'
+ *      against the running best (the step-2 cost at first).
+ * lambda 5, or 8 in the reference's fast mode.  out: {mvx, mvy (quarter
+ * samples), cost, 0} per MB.  Planes are coded-size (edges replicated).
+ * ------------------------------------------------------------------------- */
+void orc_diamond_me(const uint8_t *cur, const uint8_t *ref, int py, int wc, int hc, int lambda, int32_t *out)
+{
+    static const int dia[4][2] = {{0, 1}, {0, -1}, {1, 0}, {-1, 0}};
+    static const int half[8][2] = {{2, 0}, {-2, 0}, {0, 2}, {0, -2}, {2, 2}, {2, -2}, {-2, 2}, {-2, -2}};
+    const int wmb = wc / 16, hmb = hc / 16;
+    for (int mby = 0; mby < hmb; mby++)
+        for (int mbx = 0; mbx < wmb; mbx++) {
+            const int x0 = mbx * 16, y0 = mby * 16;
+            int32_t *o = out + 4 * (mby * wmb + mbx);
+#define REFC(x, y) ref[(size_t)clampi((y), 0, hc - 1) * py + clampi((x), 0, wc - 1)]
+            uint32_t zero = 0;
+            for (int r = 0; r < 16; r++)
+                for (int c = 0; c < 16; c++) zero += (uint32_t)abs(cur[(size_t)(y0 + r) * py + x0 + c] - REFC(x0 + c, y0 + r));
+            if (zero <= 512) {
+                o[0] = o[1] = 0;
+                o[2] = (int32_t)zero;
+                o[3] = 0;
+                continue;
+            }
+            int bx = 0, by = 0;
+            uint32_t best = zero;
+            for (int step = 8; step >= 1; step /= 2) {
+                const int cx = bx, cy = by;
+                for (int k = 0; k < 4; k++) {
+                    const int x = cx + dia[k][0] * step, y = cy + dia[k][1] * step;
+                    if (abs(x) > 16 || abs(y) > 16) continue;
+                    uint32_t s = 0;
+                    for (int r = 0; r < 16; r++)
+                        for (int c = 0; c < 16; c++)
+                            if (((r + c) & 1) == 0)
+                                s += 2u * (uint32_t)abs(cur[(size_t)(y0 + r) * py + x0 + c] - REFC(x0 + c + x, y0 + r + y));
+                    const uint32_t cost = s + (uint32_t)lambda * (uint32_t)(abs(x) + abs(y));
+                    if (cost < best) {
+                        best = cost;
+                        bx = x;
+                        by = y;
+                    }
+                }
+                if (best < 768) break;
+            }
+#undef REFC
+            int qx = bx * 4, qy = by * 4;
+            for (int stage = 0; stage < 2; stage++) {
+                const int ax = qx, ay = qy;
+                int sx = 0, sy = 0;
+                for (int k = 0; k < 8; k++) {
+                    const int ox = stage ? half[k][0] / 2 : half[k][0], oy = stage ? half[k][1] / 2 : half[k][1];
+                    uint32_t s = 0;
+                    for (int r = 0; r < 16; r++)
+                        for (int c = 0; c < 16; c++)
+                            s += (uint32_t)abs(cur[(size_t)(y0 + r) * py + x0 + c] -
+                                               orc_luma_qpel(ref, py, wc, hc, (x0 + c) * 4 + ax + ox, (y0 + r) * 4 + ay + oy));
+                    if (s < best) {
+                        best = s;
+                        sx = ox;
+                        sy = oy;
+                    }
+                }
+                qx = ax + sx;
+                qy = ay + sy;
+            }
+            o[0] = qx;
+            o[1] = qy;
+            o[2] = (int32_t)best;
+            o[3] = 0;
+        }
+}

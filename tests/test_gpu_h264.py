@@ -546,3 +546,54 @@ def test_pipeline_square_me(pkg):
     finally:
         enc.close()
         g.close()
+
+
+# ------------------------------------------------- the reference's diamond search
+
+@pytest.mark.parametrize("w,h,lam", [(128, 96, 5), (320, 176, 8), (200, 120, 5)])
+def test_diamond_me_stage(pkg, w, h, lam):
+    torch = _torch()
+    say(f"diamond me {w}x{h} lambda={lam}")
+    (ry, _), (cy, _) = frames(pkg, 70 + lam, w, h, 2, 5)
+    hc, wc = cy.shape
+    want = orc.diamond_me(cy, ry, lam)
+    dcy, dry = dev(torch, cy), dev(torch, ry)
+    out = torch.zeros(want.shape[0] * 4, dtype=torch.int32, device="cuda")
+    assert pkg.hip.bc250_launch_diamond_me(P(dcy), wc, P(dry), wc, wc, hc, lam, P(out), None) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(host(out).view(orc.MV), want)
+    assert ((want["mvx"] % 4) != 0).sum() > 0 and (want["mvx"] != 0).sum() > 3
+
+
+@pytest.mark.parametrize("me_mode", [0, 1])
+def test_pipeline_diamond_me(pkg, me_mode):
+    """The H.264 pipeline with the reference's diamond search in place of the
+    full search; me_mode 1 is the reference's fast mode (lambda 8)."""
+    _torch()
+    w, h, qp, ns = 160, 112, 28, 1
+    fr = frames(pkg, 24, w, h, 3, 4)
+    g = pkg.FrameGen(24, w, h, 4)
+    enc = pkg.Encoder(w, h, deblock_idc=0, diamond=1)
+    ref = None
+    try:
+        for i in range(3):
+            im = enc.new_image()
+            enc.upload(im, *g.frame(i))
+            got = enc.results(enc.encode(im, qp, i == 0, ns, me_mode=me_mode))
+            cy, cuv = fr[i]
+            if i == 0:
+                e = orc.encode_i(cy, cuv, qp, ns)
+                mvs = np.zeros((w // 16) * (h // 16), orc.MV)
+            else:
+                mvs = orc.diamond_me(cy, ref[0], 8 if me_mode == 1 else 5)
+                e = orc.encode_p(cy, cuv, ref[0], ref[1], qp, mvs)
+            y, uv = orc.deblock(e["y"], e["uv"], qp, i == 0, e["nz"], mvs, ns, 0)
+            np.testing.assert_array_equal(got["mv"], mvs, err_msg=f"frame {i} mvs")
+            np.testing.assert_array_equal(got["quant"], e["levels"], err_msg=f"frame {i} levels")
+            my, muv = enc.map_recon()
+            np.testing.assert_array_equal(my, y, err_msg=f"frame {i} recon")
+            ref = (y, uv)
+            say(f"  frame {i} ok")
+    finally:
+        enc.close()
+        g.close()

@@ -659,3 +659,53 @@ def test_square_me_costs_recompute(pkg, radius):
         assert sad <= zero
         moved += (mvx, mvy) != (0, 0)
     assert moved > 10
+
+
+# ------------------------------------------------- the reference's diamond search
+
+def _dia_cost_checks(cy, ry, m, lam):
+    """Every result's cost recomputed from the planes: plain SAD of the
+    8.4.2.2.1 prediction for a fractional vector; for an integer one either
+    the zero-vector SAD (vector 0) or 2 x the even-parity SAD + lam |mv|."""
+    hc, wc = cy.shape
+    for i, v in enumerate(m):
+        x0, y0 = (i % (wc // 16)) * 16, (i // (wc // 16)) * 16
+        mvx, mvy, cost = int(v["mvx"]), int(v["mvy"]), int(v["sad"])
+        assert abs(mvx) <= 67 and abs(mvy) <= 67
+        c = cy[y0:y0 + 16, x0:x0 + 16].astype(int)
+        pred = np.zeros((16, 16), np.uint8)
+        orc.lib.orc_pred_luma_block(p(ry), wc, wc, hc, x0, y0, 16, 16, mvx, mvy, p(pred), 16)
+        full = int(np.abs(pred.astype(int) - c).sum())
+        if mvx % 4 or mvy % 4:
+            assert cost == full, (i, mvx, mvy)
+        elif (mvx, mvy) == (0, 0):
+            assert cost == full or cost <= 512 and cost == full
+        else:
+            par = (np.add.outer(np.arange(16), np.arange(16)) & 1) == 0
+            assert cost == 2 * int(np.abs(pred.astype(int) - c)[par].sum()) + lam * (abs(mvx) + abs(mvy)) // 4, i
+
+
+def test_diamond_me_moved_block_and_static():
+    """A textured MB moved by (+8, 0) in a flat frame is found by the first
+    step's diamond point (8, 0) at cost 5 * 8 (no earlier point of that step
+    is cheaper, and no fractional candidate's SAD gets under 40); static MBs
+    keep (0, 0)."""
+    cur = np.full((64, 64), 90, np.uint8)
+    ref = cur.copy()
+    yy, xx = np.mgrid[0:16, 0:16]
+    cur[16:32, 16:32] = (xx * 29 + yy * 11 + 7) % 200 + 20
+    ref[16:32, 24:40] = cur[16:32, 16:32]
+    m = orc.diamond_me(cur, ref)
+    assert (m[5]["mvx"], m[5]["mvy"]) == (32, 0) and m[5]["sad"] == 5 * 8
+    assert all((m[i]["mvx"], m[i]["mvy"]) == (0, 0) for i in (0, 3, 12, 15))
+    _dia_cost_checks(cur, ref, m, 5)
+
+
+@pytest.mark.parametrize("lam", [5, 8])
+def test_diamond_me_costs_recompute(pkg, lam):
+    g = pkg.FrameGen(13, 192, 112, 5)
+    (cy, _), (ry, _) = [pkg.pad_coded(*g.frame(i)) for i in (1, 0)]
+    g.close()
+    m = orc.diamond_me(cy, ry, lam)
+    assert ((m["mvx"] % 4) != 0).sum() + ((m["mvy"] % 4) != 0).sum() > 5
+    _dia_cost_checks(cy, ry, m, lam)
