@@ -15,8 +15,9 @@
 // The argmin key is cost*2^15 + raster index of (dx,dy) (ties -> lowest
 // raster index), reduced per lane, across the wave with DPP shuffles and
 // across waves through LDS.  Measured on MI355X at 1080p, +-64: 0.417 ms per
-// frame (this shape) against 0.425 ms for 32-bit accumulators and 0.544 ms for
-// 4x8 tiles on 3-wave workgroups, which leave SIMDs unevenly loaded.
+// frame (this shape; 0.406 ms with the one-instruction argmin keys below)
+// against 0.425 ms for 32-bit accumulators and 0.544 ms for 4x8 tiles on
+// 3-wave workgroups, which leave SIMDs unevenly loaded.
 #include "bc250_dev.h"
 #include "bc250_stages.h"
 
@@ -51,6 +52,21 @@ struct FsGeom {
     static_assert(NC * NC <= 32768, "raster index must fit 15 bits");
 };
 
+// Argmin key of a 16-bit cost half, cost * 64 + li, in one v_mad_u32_u16
+// (op_sel picks the high half); the compiler builds it from a mask or shift
+// and an or, two to three instructions.
+BC_DEV uint32_t mad_lo16(uint32_t x, uint32_t li)
+{
+    uint32_t d;
+    asm("v_mad_u32_u16 %0, %1, 64, %2" : "=v"(d) : "v"(x), "v"(li));
+    return d;
+}
+BC_DEV uint32_t mad_hi16(uint32_t x, uint32_t li)
+{
+    uint32_t d;
+    asm("v_mad_u32_u16 %0, %1, 64, %2 op_sel:[1,0,0,0]" : "=v"(d) : "v"(x), "v"(li));
+    return d;
+}
 }  // namespace
 
 // out16[m] = {dx, dy, cost, sad}; out8[4m+q] likewise for quadrant q.
@@ -165,10 +181,10 @@ __global__ __launch_bounds__(ME_THREADS, BC250_ME_WAVES_PER_SIMD) void me_fullse
                     // the four halves summed: |x.lo - 0| + |x.hi - 0| + c
                     const uint32_t s16 = __builtin_amdgcn_sad_u16(bot, 0u, __builtin_amdgcn_sad_u16(top, 0u, 0u));
                     l16 = min(l16, (s16 << 6) | li);
-                    lq[0] = min(lq[0], ((top & 0xffffu) << 6) | li);
-                    lq[1] = min(lq[1], ((top >> 16) << 6) | li);
-                    lq[2] = min(lq[2], ((bot & 0xffffu) << 6) | li);
-                    lq[3] = min(lq[3], ((bot >> 16) << 6) | li);
+                    lq[0] = min(lq[0], mad_lo16(top, li));
+                    lq[1] = min(lq[1], mad_hi16(top, li));
+                    lq[2] = min(lq[2], mad_lo16(bot, li));
+                    lq[3] = min(lq[3], mad_hi16(bot, li));
                 }
             // promote to global keys: cost*2^15 + raster index (every tile holds a
             // candidate inside the range, so its minimum is one)
