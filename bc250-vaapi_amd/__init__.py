@@ -17,6 +17,8 @@ HOST_LIB = os.path.join(_HERE, "libbc250_host.so")
 GPU_IMAGE_NV12 = 0
 GPU_CODEC_H264, GPU_CODEC_HEVC = 0, 1
 GPU_IMAGE_P010 = 1
+# forms of the 32x32 transform batches (include/bc250_hevc.h)
+T32_BUTTERFLY, T32_MFMA, T32_MATRIX = 0, 1, 2
 
 
 def _load(path):
@@ -146,6 +148,8 @@ _sig(hip, "bc250_hevc_tq8_batch", _I, [_P, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_dqit8_batch", _I, [_P, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_tqn_batch", _I, [_P, _I, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_dqitn_batch", _I, [_P, _I, _I, _I, _I, _P, _P])
+_sig(hip, "bc250_hevc_tq32_batch", _I, [_P, _I, _I, _I, _I, _I, _P, _P])
+_sig(hip, "bc250_hevc_dqit32_batch", _I, [_P, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_mc_uni_batch", _I, [_P, _I, _I, _I, _P, _I, _I, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_hevc_launch_refine8", _I, [_P, _I, _P, _I, _I, _I, _P, _P, _I, _P])
 _sig(hip, "bc250_hevc_intra_work_bytes", C.c_size_t, [_I, _I])

@@ -29,6 +29,32 @@ static __constant__ int8_t kEpel[8][4] = {{0, 0, 0, 0},    {-2, 58, 10, -2}, {-4
 
 BC_DEV int sat16(int v) { return v > 32767 ? 32767 : (v < -32768 ? -32768 : v); }
 
+// Entry (k, n) of the N-point (N = 4..32) DCT-II matrix of 8.6.4.2.  Row k of
+// the N-point matrix is row k * 32 / N of the 32-point one, whose entries are
+// +-64 sqrt(2) cos(a pi / 64) at the angle index a = (2n + 1) k mod 128 folded
+// into 0..32, rounded as the standard's table has them (row 0 is all 64).
+// constexpr so that unrolled butterflies fold each entry to a constant.
+BC_DEV constexpr int dct_coef(int k, int n, int N)
+{
+    const int kk = k * (32 / N);
+    if (kk == 0) return 64;
+    int a = ((2 * n + 1) * kk) & 127, sign = 1;
+    if (a > 64) a = 128 - a;
+    if (a > 32) {
+        a = 64 - a;
+        sign = -1;
+    }
+    const int odd[16] = {90, 90, 88, 85, 82, 78, 73, 67, 61, 54, 46, 38, 31, 22, 13, 4};
+    const int by2[8] = {90, 87, 80, 70, 57, 43, 25, 9};
+    const int by4[4] = {89, 75, 50, 18};
+    const int mag = (a & 1)   ? odd[a >> 1]
+                    : (a & 2) ? by2[a >> 2]
+                    : (a & 4) ? by4[a >> 3]
+                    : (a & 8) ? (a == 8 ? 83 : 36)
+                              : 64;
+    return sign * mag;
+}
+
 BC_DEV int level_scale(int r)
 {
     return r == 0 ? 40 : r == 1 ? 45 : r == 2 ? 51 : r == 3 ? 57 : r == 4 ? 64 : 72;
@@ -56,15 +82,16 @@ BC_DEV Quant make_quant(int q, int shift, int round_q12)
     Q.shift = shift;
     Q.d = (uint32_t)level_scale(q % 6);
     Q.off = (uint32_t)((((uint64_t)Q.d << Q.p) * (uint64_t)round_q12) / 12u);
-    Q.m = (uint32_t)(((1ull << 32) + Q.d - 1) / Q.d);
+    Q.m = 0xffffffffu / Q.d + 1u;  // ceil(2^32 / d) for d > 1, in 32-bit arithmetic
     return Q;
 }
 
 // floor(t / d) as a multiply-high: exact because t < 2^21 and m*d - 2^32 < d.
+// v is an int16 value and shift <= 10, so mag << shift is a 24-bit product.
 BC_DEV int quant(int v, const Quant &Q)
 {
     const uint32_t mag = (uint32_t)(v < 0 ? -v : v);
-    const uint32_t t = ((mag << Q.shift) + Q.off) >> Q.p;
+    const uint32_t t = (__umul24(mag, 1u << Q.shift) + Q.off) >> Q.p;
     uint32_t l = __umulhi(t, Q.m);
     l = l > 32767u ? 32767u : l;
     return v < 0 ? -(int)l : (int)l;
@@ -123,5 +150,11 @@ BC_DEV void st4(uint8_t *row, int x, const int v[4])
                                              ((uint32_t)v[2] << 6) | ((uint32_t)v[3] << 22));
     }
 }
+
+// The plain N-point matrix product of the batch transforms (one 256-thread
+// workgroup per block, the matrix and both passes in LDS), defined in
+// hevc_inter.hip.
+int launch_nxn_matrix(const int16_t *in, int n, int log2n, int q, int bd, int round_q12, bool inverse,
+                      int16_t *out, hipStream_t stream);
 
 }  // namespace hevc

@@ -170,19 +170,9 @@ __global__ __launch_bounds__(256) void hevc_dqit8_kernel(const int16_t *__restri
 // 9, log2n + 6), quant / dequant bdShift bd + log2n - 5, inverse (7, 20 - bd).
 BC_DEV void build_dct_matrix(int *M, int N)
 {
-    const int oddset[16] = {90, 90, 88, 85, 82, 78, 73, 67, 61, 54, 46, 38, 31, 22, 13, 4};
     for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
-        const int k = e / N, n = e - k * N, kk = k * (32 / N);
-        int a = ((2 * n + 1) * kk) & 127, sign = 1;
-        if (a > 64) a = 128 - a;
-        if (a > 32) { a = 64 - a; sign = -1; }
-        int mag;
-        if (a & 1) mag = oddset[a >> 1];
-        else if (a & 2) mag = ((const int[8]){90, 87, 80, 70, 57, 43, 25, 9})[a >> 2];
-        else if (a & 4) mag = ((const int[4]){89, 75, 50, 18})[a >> 3];
-        else if (a & 8) mag = a == 8 ? 83 : 36;
-        else mag = 64;
-        M[e] = kk == 0 ? 64 : sign * mag;
+        const int k = e / N;
+        M[e] = dct_coef(k, e - k * N, N);
     }
 }
 
@@ -476,13 +466,21 @@ extern "C" int bc250_hevc_mc_uni_batch(const void *ref, int stride, int w_pic, i
     return launch_ok();
 }
 
+int hevc::launch_nxn_matrix(const int16_t *in, int n, int log2n, int q, int bd, int round_q12, bool inverse,
+                            int16_t *out, hipStream_t stream)
+{
+    if (inverse) hevc_nxn_kernel<true><<<n, 256, 0, stream>>>(in, n, log2n, q, bd, 0, out);
+    else hevc_nxn_kernel<false><<<n, 256, 0, stream>>>(in, n, log2n, q, bd, round_q12, out);
+    return launch_ok();
+}
+
 extern "C" int bc250_hevc_tqn_batch(const int16_t *res, int n, int log2n, int qp, int bd, int round_q12,
                                     int16_t *out, void *stream)
 {
     if (n < 0 || log2n < 2 || log2n > 5 || !bd_ok(bd) || round_q12 < 0 || round_q12 > 12) return -1;
     if (n == 0) return 0;
-    hevc_nxn_kernel<false><<<n, 256, 0, (hipStream_t)stream>>>(res, n, log2n, qp, bd, round_q12, out);
-    return launch_ok();
+    if (log2n == 5) return bc250_hevc_tq32_batch(res, n, qp, bd, round_q12, BC250_T32_DEFAULT, out, stream);
+    return hevc::launch_nxn_matrix(res, n, log2n, qp, bd, round_q12, false, out, (hipStream_t)stream);
 }
 
 extern "C" int bc250_hevc_dqitn_batch(const int16_t *coef, int n, int log2n, int qp, int bd, int16_t *out,
@@ -490,8 +488,8 @@ extern "C" int bc250_hevc_dqitn_batch(const int16_t *coef, int n, int log2n, int
 {
     if (n < 0 || log2n < 2 || log2n > 5 || !bd_ok(bd)) return -1;
     if (n == 0) return 0;
-    hevc_nxn_kernel<true><<<n, 256, 0, (hipStream_t)stream>>>(coef, n, log2n, qp, bd, 0, out);
-    return launch_ok();
+    if (log2n == 5) return bc250_hevc_dqit32_batch(coef, n, qp, bd, BC250_T32_DEFAULT, out, stream);
+    return hevc::launch_nxn_matrix(coef, n, log2n, qp, bd, 0, true, out, (hipStream_t)stream);
 }
 
 static int grid_xcd(int items) { return ((items + 7) / 8) * 8; }

@@ -215,6 +215,61 @@ def bench_main10_4k(pkg, torch, args):
     print(json.dumps(res))
 
 
+def bench_t32(pkg, torch, args):
+    """The 32x32 transform forms compared (not the headline): 64800 blocks
+    (eight 3840x2160 luma planes as 32x32 TUs) of 10-bit residuals, forward
+    + quant and dequant + inverse at Qp' 39, each form launched --steps times
+    after --warmup launches, timed with HIP events on the launch stream.
+    Algorithmic bytes per block: 2048 read + 2048 written."""
+    import ctypes as C
+    n, bd, qp = 64800, 10, 39
+    rs = np.random.default_rng(32)
+    res = rs.integers(-1023, 1024, (n, 1024)).astype(np.int16)
+    res[1::2] //= 13
+    dres = torch.from_numpy(res).cuda()
+    forms = {"butterfly": pkg.T32_BUTTERFLY, "mfma": pkg.T32_MFMA, "matrix": pkg.T32_MATRIX}
+    outs, coefs, out = {}, {}, {}
+    for name, m in forms.items():
+        lv = torch.empty_like(dres)
+        back = torch.empty_like(dres)
+        fwd = lambda: pkg.hip.bc250_hevc_tq32_batch(C.c_void_p(dres.data_ptr()), n, qp, bd, 2, m,
+                                                    C.c_void_p(lv.data_ptr()), None)
+        inv = lambda: pkg.hip.bc250_hevc_dqit32_batch(C.c_void_p(lv.data_ptr()), n, qp, bd, m,
+                                                      C.c_void_p(back.data_ptr()), None)
+        row = {}
+        for d, f in (("fwd", fwd), ("inv", inv)):
+            log(f"t32 {name} {d}: {args.warmup} warmup + {args.steps} timed launches")
+            for _ in range(args.warmup):
+                if f() != 0:
+                    raise RuntimeError(f"{name} {d} launch failed")
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(args.steps):
+                f()
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / args.steps
+            gbs = n * 4096 / (us * 1e-6) / 1e9
+            row[d] = {"us_per_launch": round(us, 2), "blocks_per_s": round(n / (us * 1e-6)),
+                      "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        outs[name], coefs[name], out[name] = lv, back, row
+    same = all(torch.equal(outs[k], outs["matrix"]) and torch.equal(coefs[k], coefs["matrix"]) for k in forms)
+    best = min(forms, key=lambda k: out[k]["fwd"]["us_per_launch"])
+    fw = out[best]["fwd"]
+    print(json.dumps({
+        "metric": "HEVC 32x32 transform + quant blocks/s on 1 GPU", "value": fw["blocks_per_s"],
+        "unit": "blocks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(fw["us_per_launch"] / 1e3, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int16", "data": "synthetic",
+        "config": {"workload": f"{n} 32x32 blocks of 10-bit residuals (eight 3840x2160 luma planes), Qp' {qp}",
+                   "blocks": n, "bit_depth": bd, "qp": qp},
+        "forms": out, "fastest_forward": best, "forms_identical": bool(same),
+        "roofline": {"bound": "hbm", "achieved": fw["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": fw["hbm_frac"], "traffic": None},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,8 +283,9 @@ def main():
     ap.add_argument("--single", action="store_true", help="skip the second codec's run")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle integer search")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("1080p", "4k-main10"), default="1080p",
-                    help="4k-main10: the 3840x2160 P010 HEVC Main10 workload instead of the 1080p headline")
+    ap.add_argument("--workload", choices=("1080p", "4k-main10", "t32"), default="1080p",
+                    help="4k-main10: the 3840x2160 P010 HEVC Main10 workload instead of the 1080p headline; "
+                         "t32: the 32x32 transform forms compared")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the pipeline shards by session (one GPU per group of sessions); "
@@ -240,6 +296,9 @@ def main():
     pkg = load_pkg()
     if args.workload == "4k-main10":
         bench_main10_4k(pkg, torch, args)
+        return
+    if args.workload == "t32":
+        bench_t32(pkg, torch, args)
         return
     n = args.warmup + args.steps
     log(f"generating {n} frames {W}x{H}")

@@ -58,6 +58,23 @@ int bc250_hevc_tqn_batch(const int16_t *res, int n, int log2n, int qp, int bd, i
                          void *stream);
 int bc250_hevc_dqitn_batch(const int16_t *coef, int n, int log2n, int qp, int bd, int16_t *out, void *stream);
 
+/* The 32x32 pair of bc250_hevc_tqn_batch / bc250_hevc_dqitn_batch (log2n 5,
+ * identical results) in a chosen form: BC250_T32_BUTTERFLY, register partial
+ * butterflies on the vector ALUs, BC250_T32_MFMA, an exact split-int8
+ * product on the matrix cores (int16 operand = 256 * signed high byte +
+ * low byte, int32 accumulation), or BC250_T32_MATRIX.  No reference
+ * counterpart: the reference codes no TU above 8x8; the inverse is the
+ * decoder's hevcd_transform (hevc_transform.c:274) at nTbS 32. */
+#define BC250_T32_BUTTERFLY 0
+#define BC250_T32_MFMA 1
+/* The plain matrix product of the N-point batches, kept for comparison. */
+#define BC250_T32_MATRIX 2
+/* The form bc250_hevc_tqn_batch / bc250_hevc_dqitn_batch use at log2n 5. */
+#define BC250_T32_DEFAULT BC250_T32_MFMA
+int bc250_hevc_tq32_batch(const int16_t *res, int n, int qp, int bd, int round_q12, int method, int16_t *out,
+                          void *stream);
+int bc250_hevc_dqit32_batch(const int16_t *coef, int n, int qp, int bd, int method, int16_t *out, void *stream);
+
 /* n blocks of w x h (w, h <= 16) from one plane (uint8 when bd == 8, else
  * uint16; stride in samples): blk[4*i .. 4*i+3] = {x, y, mvx, mvy}, vectors
  * in quarter (luma) or eighth (chroma) samples.  dst: n * w * h samples. */

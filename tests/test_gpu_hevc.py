@@ -389,6 +389,42 @@ def test_tqn_dqitn_batch(pkg, log2n, bd):
             assert np.array_equal(back[b], orc.hevc_dqitn(want, log2n, qp, bd)), (b, qp)
 
 
+@pytest.mark.parametrize("method", ["butterfly", "mfma", "matrix"])
+@pytest.mark.parametrize("bd", [8, 10])
+def test_t32_forms_match_oracle(pkg, method, bd):
+    """The 32x32 forms (VALU butterflies, split-int8 matrix cores, LDS matrix
+    product) against the oracle's plain matrix product: residual-range and
+    full-int16 inputs (the saturating cases), odd batch sizes (the butterfly
+    pairs blocks)."""
+    torch = _torch()
+    m = {"butterfly": pkg.T32_BUTTERFLY, "mfma": pkg.T32_MFMA, "matrix": pkg.T32_MATRIX}[method]
+    rs = np.random.default_rng(3200 + bd)
+    vmax = (1 << bd) - 1
+    for n in (1, 3, 257):
+        res = rs.integers(-vmax, vmax + 1, (n, 1024)).astype(np.int16)
+        res[1::3] //= 9
+        if n > 3:
+            res[2] = rs.integers(-32768, 32768, 1024)     # saturating first pass
+            res[5] = 32767
+            res[6] = -32768
+        for qp in ([0, 27, 51] if bd == 8 else [12, 39, 63]):
+            say(f"t32 {method} bd={bd} n={n} qp={qp}")
+            dres, dout = dev(torch, res), torch.zeros((n, 1024), dtype=torch.int16, device="cuda")
+            assert pkg.hip.bc250_hevc_tq32_batch(P(dres), n, qp, bd, 2, m, P(dout), None) == 0
+            coef = rs.integers(-32768, 32768, (n, 1024)).astype(np.int16)
+            coef[::2] //= 311
+            dcoef, dback = dev(torch, coef), torch.zeros_like(dout)
+            assert pkg.hip.bc250_hevc_dqit32_batch(P(dcoef), n, qp, bd, m, P(dback), None) == 0
+            torch.cuda.synchronize()
+            lv, back = host(dout), host(dback)
+            for b in sorted({0, n - 1} | set(range(0, n, 16)) | ({2, 5, 6} if n > 3 else set())):
+                assert np.array_equal(lv[b], orc.hevc_tqn(res[b], 5, qp, bd, 2)), (b, qp)
+                assert np.array_equal(back[b], orc.hevc_dqitn(coef[b], 5, qp, bd)), (b, qp)
+    assert pkg.hip.bc250_hevc_tq32_batch(None, 0, 27, bd, 2, m, None, None) == 0
+    assert pkg.hip.bc250_hevc_tq32_batch(None, 4, 27, 9, 2, m, None, None) == -1
+    assert pkg.hip.bc250_hevc_dqit32_batch(None, 4, 27, bd, 7, None, None) == -1
+
+
 def _blocky(rs, h, w, step):
     """Smooth ramps with per-8x8 offsets (block edges the filter acts on) and
     light noise; plane values stay inside 0..255."""
