@@ -60,7 +60,7 @@ class GpuContext(C.Structure):
 class EncodeParams(C.Structure):
     _fields_ = [("search_range", C.c_int), ("lambda8", C.c_int), ("deblock_idc", C.c_int),
                 ("me_8x8", C.c_int), ("readback", C.c_int), ("codec", C.c_int), ("square_radius", C.c_int),
-                ("diamond", C.c_int)]
+                ("diamond", C.c_int), ("hevc_ctu_log2", C.c_int)]
 
 
 class CpuSimdMeConfig(C.Structure):
@@ -158,6 +158,9 @@ _sig(hip, "bc250_hevc_launch_i_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _I,
 _sig(hip, "bc250_hevc_launch_p_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P,
                                             _P, _I, _P])
 _sig(hip, "bc250_hevc_launch_deblock", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P])
+_sig(hip, "bc250_hevc_launch_deblock_ctu", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P])
+_sig(hip, "bc250_hevc_launch_i_encode_ctu", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P,
+                                                 _U, _I, _I, _P])
 _sig(hip, "bc250_hevc_cu_cbf", _I, [_P, _I, _P, _P])
 _sig(hip, "bc250_hevc_p010_to_8bit", _I, [_P, _I, _P, _I, _I, _I, _P])
 
@@ -231,9 +234,10 @@ class Encoder:
     """One gpu_compute context: upload frames, encode, read staging back."""
 
     def __init__(self, width, height, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0,
-                 bit_depth=8, square_radius=0, diamond=0):
+                 bit_depth=8, square_radius=0, diamond=0, ctu_log2=4):
         """bit_depth 10: P010 surfaces (HEVC Main10); upload() then takes uint16
-        planes of 0..1023 samples and map_recon() returns them."""
+        planes of 0..1023 samples and map_recon() returns them.  ctu_log2 5:
+        HEVC CTU32 (gpu_encode_params_t.hevc_ctu_log2)."""
         self.w, self.h = width, height
         self.fmt = GPU_IMAGE_P010 if bit_depth == 10 else GPU_IMAGE_NV12
         self.wc, self.hc = (width + 15) // 16 * 16, (height + 15) // 16 * 16
@@ -242,15 +246,15 @@ class Encoder:
         if hip.bc250_gpu_init(C.byref(self.ctx)) != 0:
             raise RuntimeError("bc250_gpu_init failed (no HIP device?)")
         self.codec = codec
-        self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius, diamond)
+        self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius, diamond, ctu_log2)
         self.images = []
 
     def set_params(self, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0, square_radius=0,
-                   diamond=0):
+                   diamond=0, ctu_log2=4):
         """codec: GPU_CODEC_H264 (0) or GPU_CODEC_HEVC (1); square_radius 2..16
         selects the reference's Tier-2 square search, diamond 1 its GPU
         diamond search, instead of the full one."""
-        p = EncodeParams(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius, diamond)
+        p = EncodeParams(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius, diamond, ctu_log2)
         if hip.gpu_compute_set_encode_params(C.byref(self.ctx), C.byref(p)) != 0:
             raise ValueError("invalid encode parameters")
 

@@ -67,7 +67,7 @@ struct bc250_hip_state {
     Slot slot[NSLOT];
     int cur = 0, submitted = 0;
     double last_latency_ms = 0.0;
-    gpu_encode_params_t params{64, 1, 0, 1, 1, GPU_CODEC_H264, 0, 0};
+    gpu_encode_params_t params{64, 1, 0, 1, 1, GPU_CODEC_H264, 0, 0, 4};
     int err_flags = 0;
     // per-stage GPU time accumulated at sync_slot: [stage] sums, frames seen
     double stage_sum[NSTAGE] = {0, 0, 0, 0, 0};
@@ -559,9 +559,11 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
         if (is_intra) {
             r = bc250_launch_zero_i32(s->sync_h, 2, st);
             if (r == 0)
-                r = bc250_hevc_launch_i_encode(cur.y, cur.uv, cur.py, cur.puv, out.y, out.uv, out.py, out.puv, s->wc,
-                                               s->hc, qp, num_slices, 4 /* rounding of intra CUs in an I picture */,
-                                               s->hmodes, (int16_t *)s->levels, s->sync_h, kSpinLimit, bd, st);
+                r = bc250_hevc_launch_i_encode_ctu(cur.y, cur.uv, cur.py, cur.puv, out.y, out.uv, out.py, out.puv,
+                                                   s->wc, s->hc, qp, num_slices,
+                                                   4 /* rounding of intra CUs in an I picture */, s->hmodes,
+                                                   (int16_t *)s->levels, s->sync_h, kSpinLimit, p.hevc_ctu_log2, bd,
+                                                   st);
             if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, (int)n * 16, st);
         } else {
             r = bc250_hevc_launch_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv, out.y,
@@ -576,8 +578,8 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
     if (r == 0 && p.deblock_idc != 1) {
         if (!is_intra) r = bc250_hevc_cu_cbf((const int16_t *)s->levels, (int)n, s->hcbf, st);
         if (r == 0)
-            r = bc250_hevc_launch_deblock(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->hcbf,
-                                          s->mv8, num_slices, bd, st);
+            r = bc250_hevc_launch_deblock_ctu(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->hcbf,
+                                              s->mv8, num_slices, p.hevc_ctu_log2, bd, st);
     }
     hipEventRecord(sl.ev[4], st);
     if (r == 0 && p.readback) {
@@ -863,7 +865,9 @@ extern "C" int gpu_compute_set_encode_params(gpu_context_t *ctx, const gpu_encod
     if (p->square_radius != 0 && (p->square_radius < 2 || p->square_radius > 16)) return -1;
     if (p->diamond != 0 && p->diamond != 1) return -1;
     if (p->diamond && p->square_radius) return -1;  // one search at a time
+    if (p->hevc_ctu_log2 != 0 && p->hevc_ctu_log2 != 4 && p->hevc_ctu_log2 != 5) return -1;
     s->params = *p;
+    if (!s->params.hevc_ctu_log2) s->params.hevc_ctu_log2 = 4;
     s->params.me_8x8 = p->me_8x8 ? 1 : 0;
     s->params.readback = p->readback ? 1 : 0;
     return 0;

@@ -37,17 +37,6 @@ BC_DEV int strength(int intra, const uint8_t *cbf, const uint4 *mv, int cp, int 
     return (iabs((int)a.x - (int)b.x) >= 4 || iabs((int)a.y - (int)b.y) >= 4) ? 1 : 0;
 }
 
-BC_DEV int slice_first_row(int row, int rows, int ns)
-{
-    ns = ns < 1 ? 1 : (ns > rows ? rows : ns);
-    int r0 = 0;
-    for (int s = 0; s < ns; s++) {
-        const int b = (int)(((long long)rows * s) / ns);
-        if (row >= b) r0 = b;
-    }
-    return r0;
-}
-
 // One luma segment: v[k][0..7] = p3 p2 p1 p0 q0 q1 q2 q3 of line k (8.7.2.5.3,
 // 8.7.2.5.6, 8.7.2.5.7); beta and tc already scaled to the bit depth.
 BC_DEV void luma_segment(int v[4][8], int beta, int tc, int maxv)
@@ -103,8 +92,8 @@ template <typename Pix, int BD, bool VERT>
 __global__ __launch_bounds__(256) void hevc_deblock_kernel(uint8_t *__restrict__ y, uint8_t *__restrict__ uv, int py,
                                                            int puv, int wc, int hc, int qp, int intra,
                                                            const uint8_t *__restrict__ cbf,
-                                                           const uint4 *__restrict__ mv, int num_slices, int nl,
-                                                           int nc)
+                                                           const uint4 *__restrict__ mv, int num_slices,
+                                                           int ctu_log2, int nl, int nc)
 {
     constexpr int MAXV = (1 << BD) - 1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -113,7 +102,7 @@ __global__ __launch_bounds__(256) void hevc_deblock_kernel(uint8_t *__restrict__
         // luma: edges at 8, 16, .. (VERT: columns), segments of 4 lines along the edge
         const int nseg = (VERT ? hc : wc) >> 2;
         const int e = 8 * (t / nseg + 1), s = 4 * (t % nseg);
-        if (!VERT && (e & 15) == 0 && slice_first_row(e >> 4, hctu, num_slices) == (e >> 4)) return;
+        if (!VERT && (e & 15) == 0 && slice_first_row16(e >> 4, hctu, num_slices, ctu_log2) == (e >> 4)) return;
         const int x = VERT ? e : s, yy = VERT ? s : e;
         const int cq = cu_at(x, yy, wctu), cp = cu_at(VERT ? x - 1 : x, VERT ? yy : yy - 1, wctu);
         const int bs = strength(intra, cbf, mv, cp, cq);
@@ -158,7 +147,7 @@ __global__ __launch_bounds__(256) void hevc_deblock_kernel(uint8_t *__restrict__
     // chroma: edges at chroma 8, 16, .. (luma multiples of 16), 4-line segments, bS 2 only
     const int nseg = (VERT ? hc : wc) >> 3;
     const int e = 8 * (c / nseg + 1), s = 4 * (c % nseg);
-    if (!VERT && (e & 7) == 0 && slice_first_row(e >> 3, hctu, num_slices) == (e >> 3)) return;
+    if (!VERT && (e & 7) == 0 && slice_first_row16(e >> 3, hctu, num_slices, ctu_log2) == (e >> 3)) return;
     const int cx = VERT ? e : s, cy = VERT ? s : e;
     const int lx = 2 * cx, ly = 2 * cy;
     const int cq = cu_at(lx, ly, wctu), cp = cu_at(VERT ? lx - 1 : lx, VERT ? ly : ly - 1, wctu);
@@ -219,37 +208,46 @@ extern "C" int bc250_hevc_cu_cbf(const int16_t *coef, int nctu, uint8_t *cbf, vo
 
 template <typename Pix, int BD>
 static void launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
-                           const uint8_t *cbf, const uint4 *mvs, int num_slices, hipStream_t st)
+                           const uint8_t *cbf, const uint4 *mvs, int num_slices, int ctu_log2, hipStream_t st)
 {
     // vertical pass
     int nl = (wc / 8 - 1) * (hc / 4), nc = (wc / 16 - 1) * (hc / 8);
     int n = nl + nc;
     if (n > 0)
         hevc_deblock_kernel<Pix, BD, true>
-            <<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, num_slices, nl, nc);
+            <<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, num_slices, ctu_log2,
+                                              nl, nc);
     // horizontal pass
     nl = (hc / 8 - 1) * (wc / 4);
     nc = (hc / 16 - 1) * (wc / 8);
     n = nl + nc;
     if (n > 0)
         hevc_deblock_kernel<Pix, BD, false>
-            <<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, num_slices, nl, nc);
+            <<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, num_slices, ctu_log2,
+                                              nl, nc);
+}
+
+extern "C" int bc250_hevc_launch_deblock_ctu(void *y, void *uv, int py, int puv, int wc, int hc, int qp,
+                                             int is_intra, const uint8_t *cbf, const uint32_t *mvs, int num_slices,
+                                             int ctu_log2, int bd, void *stream)
+{
+    const int bps = bd == 8 ? 1 : 2;
+    if ((bd != 8 && bd != 10) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || (py & 7) ||
+        (puv & 7) || py < wc * bps || puv < wc * bps || num_slices < 1 || (ctu_log2 != 4 && ctu_log2 != 5))
+        return -1;
+    hipStream_t st = (hipStream_t)stream;
+    if (bd == 8)
+        launch_deblock<uint8_t, 8>((uint8_t *)y, (uint8_t *)uv, py, puv, wc, hc, qp, is_intra, cbf,
+                                   (const uint4 *)mvs, num_slices, ctu_log2, st);
+    else
+        launch_deblock<uint16_t, 10>((uint8_t *)y, (uint8_t *)uv, py, puv, wc, hc, qp, is_intra, cbf,
+                                     (const uint4 *)mvs, num_slices, ctu_log2, st);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 extern "C" int bc250_hevc_launch_deblock(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
                                          const uint8_t *cbf, const uint32_t *mvs, int num_slices, int bd,
                                          void *stream)
 {
-    const int bps = bd == 8 ? 1 : 2;
-    if ((bd != 8 && bd != 10) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || (py & 7) ||
-        (puv & 7) || py < wc * bps || puv < wc * bps || num_slices < 1)
-        return -1;
-    hipStream_t st = (hipStream_t)stream;
-    if (bd == 8)
-        launch_deblock<uint8_t, 8>((uint8_t *)y, (uint8_t *)uv, py, puv, wc, hc, qp, is_intra, cbf,
-                                   (const uint4 *)mvs, num_slices, st);
-    else
-        launch_deblock<uint16_t, 10>((uint8_t *)y, (uint8_t *)uv, py, puv, wc, hc, qp, is_intra, cbf,
-                                     (const uint4 *)mvs, num_slices, st);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    return bc250_hevc_launch_deblock_ctu(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, num_slices, 4, bd, stream);
 }

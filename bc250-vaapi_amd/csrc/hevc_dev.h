@@ -151,6 +151,27 @@ BC_DEV void st4(uint8_t *row, int x, const int v[4])
     }
 }
 
+// First row of the slice holding row `row` when slices are the row ranges
+// [floor(rows * s / ns), floor(rows * (s + 1) / ns)).
+BC_DEV int slice_first_row(int row, int rows, int ns)
+{
+    ns = ns < 1 ? 1 : (ns > rows ? rows : ns);
+    int r0 = 0;
+    for (int s = 0; s < ns; s++) {
+        const int b = (int)(((long long)rows * s) / ns);
+        if (row >= b) r0 = b;
+    }
+    return r0;
+}
+
+// The same in CTU16 rows (16 luma rows) when the slices are ranges of CTU
+// rows of 1 << ctu_log2 luma rows (ctu_log2 4 or 5).
+BC_DEV int slice_first_row16(int row16, int rows16, int ns, int ctu_log2)
+{
+    const int s = ctu_log2 - 4;
+    return slice_first_row(row16 >> s, (rows16 + s) >> s, ns) << s;
+}
+
 // The plain N-point matrix product of the batch transforms (one 256-thread
 // workgroup per block, the matrix and both passes in LDS), defined in
 // hevc_inter.hip.

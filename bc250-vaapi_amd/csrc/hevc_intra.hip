@@ -101,16 +101,6 @@ BC_DEV uint32_t row16_sum(uint32_t v)
     return v;
 }
 
-BC_DEV int slice_first_row(int row, int rows, int ns)
-{
-    ns = ns < 1 ? 1 : (ns > rows ? rows : ns);
-    int r0 = 0;
-    for (int s = 0; s < ns; s++) {
-        const int b = (int)(((long long)rows * s) / ns);
-        if (row >= b) r0 = b;
-    }
-    return r0;
-}
 }  // namespace
 
 __global__ void hevc_clear_records_kernel(uint64_t *rec, int n)
@@ -123,7 +113,8 @@ template <typename Pix, int BD>
 __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
     const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, uint8_t *__restrict__ oy,
     uint8_t *__restrict__ ouv, int opy, int opuv, int wc, int hc, int qp, int num_slices, int round_q12,
-    uint8_t *__restrict__ modes, int16_t *__restrict__ coef, int *sync, uint64_t *ghand, unsigned spin_limit)
+    uint8_t *__restrict__ modes, int16_t *__restrict__ coef, int *sync, uint64_t *ghand, unsigned spin_limit,
+    int ctu_log2)
 {
     // samples per dword; record dwords: luma row (LD), Cb row, Cr row (CD each)
     constexpr int SPD = 4 / (int)sizeof(Pix), LD = 16 / SPD, CD = 8 / SPD, REC = LD + 2 * CD;
@@ -131,23 +122,30 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
     constexpr int S1 = BD == 8 ? 1 : 3, QSH = BD - 3, S2INV = 20 - BD;
     __shared__ WaveLds<Pix> wl[WAVES];
     __shared__ uint32_t ring[WAVES - 1][RING][REC];
-    __shared__ int prog[WAVES], cons[WAVES];
+    __shared__ int prog[WAVES], cons[WAVES], done[WAVES];
+    __shared__ int bls[WAVES][3];
     __shared__ int s_group;
 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     int *err = sync + 1;
     if (tid == 0) s_group = atomicAdd(sync, 1);
-    if (tid < WAVES) prog[tid] = cons[tid] = 0;
+    if (tid < WAVES) prog[tid] = cons[tid] = done[tid] = 0;
     __syncthreads();
     const int wtu = wc >> 4, htu = hc >> 4;
     const int yrow = s_group * WAVES + wv;
     if (yrow >= htu) return;
-    volatile int *vprog = prog, *vcons = cons;
+    volatile int *vprog = prog, *vcons = cons, *vdone = done;
     Pix *B = wl[wv].b;
 
-    const int r0 = slice_first_row(yrow, htu, num_slices);
+    const int r0 = slice_first_row16(yrow, htu, num_slices, ctu_log2);
     const bool need_above = yrow > r0;
-    const bool give_below = yrow + 1 < htu && slice_first_row(yrow + 1, htu, num_slices) != yrow + 1;
+    const bool give_below = yrow + 1 < htu && slice_first_row16(yrow + 1, htu, num_slices, ctu_log2) != yrow + 1;
+    // CTU32: rows pair up (2r, 2r+1) in one workgroup and the four 16x16
+    // quadrants of a CTU32 go top-left, top-right, bottom-left, bottom-right,
+    // so the upper row also waits for the lower one: the top-left quadrant of
+    // CTU32 j+1 has its bottom-left neighbour in the bottom-right one of j.
+    const bool c32 = ctu_log2 == 5, lower = c32 && (yrow & 1);
+    const bool has_lower = c32 && !(yrow & 1) && yrow + 1 < htu;
     const bool lds_above = wv > 0, lds_below = wv < WAVES - 1;
 
     // lane roles: group 0 luma PU samples, 1 Cb, 2 Cr (3: idle copy of Cr)
@@ -218,11 +216,26 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
         if (lane < REC) take_record(j, v, lane);
     };
 
+    int fetched = 0;  // records of the row above taken so far
     for (int x = 0; x < wtu; x++) {
         const int X0 = x * 16;
+        // bottom-right quadrant: its top-right neighbour (the next CTU32) comes later
+        const bool br = lower && (x & 1);
+        // top-left quadrant after a CTU32: bottom-left neighbours in the lower row
+        const bool tl_next = has_lower && !(x & 1) && x > 0;
+        int blv = 0;
+        if (tl_next) {
+            unsigned n = 0;
+            while (vdone[wv + 1] < x) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++n > spin_limit) { atomicOr(err, 1); break; }
+            }
+            wave_lds_sync();
+            blv = bls[wv + 1][plane];
+        }
         // ---- records of the row above up to CTU x+1
         if (need_above) {
-            const int need = x + 2 < wtu ? x + 2 : wtu;
+            const int need = br ? x + 1 : (x + 2 < wtu ? x + 2 : wtu);
             if (lds_above) {
                 unsigned n = 0;
                 while (vprog[wv - 1] < need) {
@@ -231,8 +244,9 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
                 }
                 wave_lds_sync();
             }
-            if (x == 0) fetch_above(0);
-            if (need > x + 1) fetch_above(x + 1);
+            // records 0 .. need-1 (a CTU32 bottom-right quadrant takes none: the
+            // next bottom-left one then takes two)
+            while (fetched < need) fetch_above(fetched++);
             wave_lds_sync();
             if (lds_above && lane == 0) vcons[wv] = need;
             // tile row 0: corner (record x-1), top (record x), top-right (record x+1)
@@ -269,12 +283,13 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             const int by = luma ? Y0r + (cu >> 1) * 8 + (pu >> 1) * 4 : cty0 + (cu >> 1) * 4;
             // ---- availability (z-scan inside the CTU, slices and edges outside)
             const int rx = bx - ctx0, ry = by - cty0, mz = morton4(rx, ry);
-            const bool leftC = x > 0, aboveC = need_above, aboveR = need_above && x + 1 < wtu;
+            const bool leftC = x > 0, aboveC = need_above, aboveR = need_above && x + 1 < wtu && !br;
             const bool aL = rx > 0 || leftC;
             const bool aT = ry > 0 || aboveC;
             const bool aC = rx > 0 ? (ry > 0 || aboveC) : (ry > 0 ? leftC : (leftC && aboveC));
             const bool aTR = ry == 0 ? (rx + 4 < cs ? aboveC : aboveR) : (rx + 4 < cs && morton4(rx + 4, ry - 4) < mz);
-            const bool aBL = ry + 4 < cs && (rx == 0 ? leftC : morton4(rx - 4, ry + 4) < mz);
+            const bool blx = tl_next && rx == 0 && ry + 4 == cs;  // from the lower row's last quadrant
+            const bool aBL = (ry + 4 < cs && (rx == 0 ? leftC : morton4(rx - 4, ry + 4) < mz)) || blx;
             // ---- neighbours with the substitution scan
             int v[10];
             bool a[10];
@@ -289,7 +304,7 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             v[4] = sample(rx - 1, ry - 1);
             a[9] = aTR;
             v[9] = sample(rx + 4, ry - 1);
-            const int vbl = sample(rx - 1, ry + 4 < cs ? ry + 4 : ry + 3);
+            const int vbl = blx ? blv : sample(rx - 1, ry + 4 < cs ? ry + 4 : ry + 3);
             // first available in scan order fills what precedes it; later gaps copy the previous
             int firstv = 1 << (BD - 1);
 #pragma unroll
@@ -346,6 +361,14 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             }
             wave_lds_sync();
         }
+        if (lower) {
+            // bottom-right quadrant: its top-right samples are the bottom-left
+            // neighbours of the next CTU32's top-left quadrant
+            if ((x & 1) && lane < 3)
+                bls[wv][lane] = lane == 0 ? B[O_TY + LP + 19] : B[O_TC + (lane - 1) * 9 * CPT + CPT + 11];
+            wave_lds_sync();
+            if (lane == 0) vdone[wv] = x + 1;
+        }
         // ---- hand the CTU's bottom rows down
         if (give_below) {
             uint32_t h = 0;
@@ -397,7 +420,7 @@ extern "C" size_t bc250_hevc_intra_work_bytes(int wc, int hc)
 template <typename Pix, int BD>
 static void launch_i(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, uint8_t *oy, uint8_t *ouv, int opy,
                      int opuv, int wc, int hc, int qp, int num_slices, int round_q12, uint8_t *modes, int16_t *coef,
-                     int *sync, unsigned spin_limit, hipStream_t st)
+                     int *sync, unsigned spin_limit, int ctu_log2, hipStream_t st)
 {
     constexpr int REC = 32 / (4 / (int)sizeof(Pix));
     const int htu = hc / 16, nrec = (wc / 16) * htu * REC;
@@ -405,16 +428,17 @@ static void launch_i(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, u
     hevc_clear_records_kernel<<<(nrec + 255) / 256, 256, 0, st>>>(ghand, nrec);
     hevc_i_encode_kernel<Pix, BD><<<(htu + WAVES - 1) / WAVES, 64 * WAVES, 0, st>>>(
         cy, cuv, cpy, cpuv, oy, ouv, opy, opuv, wc, hc, qp, num_slices, round_q12, modes, coef, sync, ghand,
-        spin_limit);
+        spin_limit, ctu_log2);
 }
 
-extern "C" int bc250_hevc_launch_i_encode(const void *cy, const void *cuv, int cpy, int cpuv, void *oy, void *ouv,
-                                          int opy, int opuv, int wc, int hc, int qp, int num_slices, int round_q12,
-                                          uint8_t *modes, int16_t *coef, int *sync, unsigned spin_limit, int bd,
-                                          void *stream)
+extern "C" int bc250_hevc_launch_i_encode_ctu(const void *cy, const void *cuv, int cpy, int cpuv, void *oy,
+                                              void *ouv, int opy, int opuv, int wc, int hc, int qp, int num_slices,
+                                              int round_q12, uint8_t *modes, int16_t *coef, int *sync,
+                                              unsigned spin_limit, int ctu_log2, int bd, void *stream)
 {
     const int b = bd == 8 ? 1 : 2, al = 4 * b - 1;
-    if ((bd != 8 && bd != 10) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 ||
+    if ((bd != 8 && bd != 10) || (ctu_log2 != 4 && ctu_log2 != 5) || wc <= 0 || hc <= 0 || (wc & 15) ||
+        (hc & 15) || qp < 0 || qp > 51 ||
         num_slices < 1 || round_q12 < 0 || round_q12 > 12 || (cpy & al) || (cpuv & al) || (opy & al) ||
         (opuv & al) || cpy < wc * b || cpuv < wc * b || opy < wc * b || opuv < wc * b)
         return -1;
@@ -423,9 +447,18 @@ extern "C" int bc250_hevc_launch_i_encode(const void *cy, const void *cuv, int c
     auto *O = (uint8_t *)oy, *OU = (uint8_t *)ouv;
     if (bd == 8)
         launch_i<uint8_t, 8>(C, CU, cpy, cpuv, O, OU, opy, opuv, wc, hc, qp, num_slices, round_q12, modes, coef, sync,
-                             spin_limit, st);
+                             spin_limit, ctu_log2, st);
     else
         launch_i<uint16_t, 10>(C, CU, cpy, cpuv, O, OU, opy, opuv, wc, hc, qp, num_slices, round_q12, modes, coef,
-                               sync, spin_limit, st);
+                               sync, spin_limit, ctu_log2, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_hevc_launch_i_encode(const void *cy, const void *cuv, int cpy, int cpuv, void *oy, void *ouv,
+                                          int opy, int opuv, int wc, int hc, int qp, int num_slices, int round_q12,
+                                          uint8_t *modes, int16_t *coef, int *sync, unsigned spin_limit, int bd,
+                                          void *stream)
+{
+    return bc250_hevc_launch_i_encode_ctu(cy, cuv, cpy, cpuv, oy, ouv, opy, opuv, wc, hc, qp, num_slices, round_q12,
+                                          modes, coef, sync, spin_limit, 4, bd, stream);
 }

@@ -162,8 +162,8 @@ def run_pipeline(pkg, torch, frames, codec, args):
 
 def bench_main10_4k(pkg, torch, args):
     """The 4K Main10 workload (not the headline): 3840x2160 P010 (10-bit
-    samples = 4 x the generator's 8-bit output + uniform [0, 3]), IPPP, CQP
-    27, 4 slices, +-64 full search, deblocking; one JSON line."""
+    samples = 4 x the generator's 8-bit output + uniform [0, 3]), CTU32,
+    IPPP, CQP 27, 4 slices, +-64 full search, deblocking; one JSON line."""
     w, h = 3840, 2160
     n = args.warmup + args.steps
     log(f"generating {n} frames {w}x{h} P010")
@@ -176,7 +176,7 @@ def bench_main10_4k(pkg, torch, args):
                        uv.astype(np.uint16) * 4 + rs.integers(0, 4, uv.shape, dtype=np.uint16)))
     g.close()
     enc = pkg.Encoder(w, h, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, me_8x8=1, readback=0,
-                      codec=pkg.GPU_CODEC_HEVC, bit_depth=10)
+                      codec=pkg.GPU_CODEC_HEVC, bit_depth=10, ctu_log2=5)
     try:
         images = []
         for y, uv in frames:
@@ -202,10 +202,10 @@ def bench_main10_4k(pkg, torch, args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
         "config": {"workload": f"3840x2160 P010, {args.steps} frames IPPP, CQP {args.qp}, 4 slices: HEVC Main10 "
-                               f"pipeline (CTU16), +-{args.range} full search on the top 8 bits, 10-bit refine, "
+                               f"pipeline (CTU32), +-{args.range} full search on the top 8 bits, 10-bit refine, "
                                "CU coding, CTU intra wavefront, 8.7.2 deblocking",
                    "codec": "hevc", "width": w, "height": h, "bit_depth": 10, "frames": args.steps, "qp": args.qp,
-                   "search_range": args.range, "num_slices": 4, "seed": 2160, "rectangles": 12},
+                   "search_range": args.range, "num_slices": 4, "ctu": 32, "seed": 2160, "rectangles": 12},
         "p50_frame_ms": round(float(np.percentile(frame_ms, 50)), 4),
         "p99_frame_ms": round(float(np.percentile(frame_ms, 99)), 4),
         "mpixel_per_s": round(args.steps / wall * w * h / 1e6, 1),

@@ -22,6 +22,8 @@
  *   bc250_hevc_intra_work_bytes /
  *   bc250_hevc_launch_i_encode               encode_ctu's I branch + intra_trial
  *                                            (hevc_enc_template.c:502-594,1409-1427)
+ *   bc250_hevc_launch_i_encode_ctu /         the same two at CTU32 (no reference
+ *   bc250_hevc_launch_deblock_ctu            counterpart)
  *
  * Picture layout: coded size (16-aligned), 8-bit, NV12 (luma plane, then
  * interleaved Cb/Cr), the same surfaces as the H.264 path.  CTUs are 16x16
@@ -119,6 +121,20 @@ int bc250_hevc_launch_i_encode(const void *cy, const void *cuv, int cpy, int cpu
                                int opuv, int wc, int hc, int qp, int num_slices, int round_q12, uint8_t *modes,
                                int16_t *coef, int *sync, unsigned spin_limit, int bd, void *stream);
 
+/* The same with the coding tree unit size 1 << ctu_log2: 4 is the call
+ * above; 5 codes CTU32s (no reference counterpart: the reference encoder
+ * codes CTU16 only): the four 16x16 quadrants of a CTU32 in z-order, each
+ * as above, neighbours available by the CTU32 z-scan order (6.5.2: a
+ * bottom-right quadrant's top-right neighbour is not yet coded, the next
+ * CTU32's top-left quadrant has its bottom-left neighbour), slices as
+ * CTU32-row ranges; outputs keep the per-16x16 layout.  Picture sizes as
+ * above (partial CTU32s at the right and bottom edges code the quadrants
+ * inside the picture). */
+int bc250_hevc_launch_i_encode_ctu(const void *cy, const void *cuv, int cpy, int cpuv, void *oy, void *ouv, int opy,
+                                   int opuv, int wc, int hc, int qp, int num_slices, int round_q12, uint8_t *modes,
+                                   int16_t *coef, int *sync, unsigned spin_limit, int ctu_log2, int bd,
+                                   void *stream);
+
 /* Per-CU coded-luma flags of a P picture's levels (bc250_hevc_launch_p_encode
  * layout): cbf[4*t + z] = 1 when any of CU z's 64 luma levels is non-zero,
  * the bS 1 condition of the deblocking filter. */
@@ -139,6 +155,11 @@ int bc250_hevc_cu_cbf(const int16_t *coef, int nctu, uint8_t *cbf, void *stream)
  * asked. */
 int bc250_hevc_launch_deblock(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
                               const uint8_t *cbf, const uint32_t *mvs, int num_slices, int bd, void *stream);
+/* The same with slices as ranges of CTU rows of 1 << ctu_log2 (4 or 5)
+ * luma rows (the CTU32 pictures of bc250_hevc_launch_i_encode_ctu). */
+int bc250_hevc_launch_deblock_ctu(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                                  const uint8_t *cbf, const uint32_t *mvs, int num_slices, int ctu_log2, int bd,
+                                  void *stream);
 
 #ifdef __cplusplus
 }

@@ -191,6 +191,10 @@ typedef struct {
     int diamond;       /* H.264: 1 = the reference's diamond search (motion_estimation.comp) with integer
                         * SADs in place of the full search: lambda 5, or 8 when dispatch_encode_ext's
                         * me_mode is 1 (the reference's fast mode); no 8x8 vectors */
+    int hevc_ctu_log2; /* HEVC coding tree unit: 0 or 4 = CTU16 (default, the reference's), 5 = CTU32:
+                        * I pictures code the four 16x16 quadrants of each CTU32 in z-order with the
+                        * CTU32 z-scan availability, slices are CTU32-row ranges (P pictures and the
+                        * coefficient / mode layouts, indexed by 16x16 area, are unchanged) */
 } gpu_encode_params_t;
 
 /* codec = GPU_CODEC_HEVC runs the HEVC stages of include/bc250_hevc.h

@@ -134,6 +134,13 @@ void orc_hevc_predict4(const void *plane, int bd, int stride, int width, int hei
 int  orc_hevc_choose_mode(const void *src, const void *recon, int bd, int stride, int width, int height,
                           int x0, int y0, int y_min, int32_t pred[16]);
 int  orc_hevc_slice_y_min(int ctu_row, int ctu_rows, int ns);
+int  orc_hevc_slice_y_min_ctu(int r16, int rows16, int ns, int ctu_log2);
+long long orc_hevc_zrank(int x, int y, int width, int is_luma, int ctu_log2);
+void orc_hevc_encode_i_ctu(const void *sy, const void *scb, const void *scr, void *ry, void *rcb, void *rcr,
+                           int bd, int wc, int hc, int qp, int num_slices, int round_q12, int ctu_log2,
+                           uint8_t *modes, int16_t *coef);
+void orc_hevc_deblock_ctu(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                          const uint8_t *cbf, const int32_t *mv4, int num_slices, int ctu_log2, int bd);
 void orc_hevc_encode_i(const void *sy, const void *scb, const void *scr, void *ry, void *rcb, void *rcr,
                        int bd, int wc, int hc, int qp, int num_slices, int round_q12, uint8_t *modes,
                        int16_t *coef);

@@ -211,6 +211,9 @@ _sig("orc_hevc_predict4", None, [_P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P])
 _sig("orc_hevc_choose_mode", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P])
 _sig("orc_hevc_slice_y_min", _I, [_I, _I, _I])
 _sig("orc_hevc_encode_i", None, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P])
+_sig("orc_hevc_encode_i_ctu", None, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P])
+_sig("orc_hevc_slice_y_min_ctu", _I, [_I, _I, _I, _I])
+_sig("orc_hevc_zrank", C.c_longlong, [_I, _I, _I, _I, _I])
 _sig("orc_hevc_encode_p", None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P])
 _sig("orc_hevc_refine8", C.c_uint32, [_P, _P, _I, _I, _I, _I, _I, C.POINTER(_I), C.POINTER(_I)])
 
@@ -275,15 +278,25 @@ def split_uv(uv):
     return np.ascontiguousarray(uv[:, 0::2]), np.ascontiguousarray(uv[:, 1::2])
 
 
-def hevc_encode_i(sy, scb, scr, qp, num_slices=1, round_q12=HEVC_ROUND_INTRA_I, bd=8):
+def hevc_encode_i(sy, scb, scr, qp, num_slices=1, round_q12=HEVC_ROUND_INTRA_I, bd=8, ctu_log2=4):
+    """ctu_log2 4: the reference's CTU16 order (orc_hevc_encode_i); 5: CTU32
+    (16x16 quadrants in z-order, CTU32-row slices)."""
     hc, wc = sy.shape
     t = (wc // 16) * (hc // 16)
     sy, scb, scr = (np.ascontiguousarray(a, _pix(bd)) for a in (sy, scb, scr))
     ry, rcb, rcr = np.zeros_like(sy), np.zeros_like(scb), np.zeros_like(scr)
     modes, coef = np.zeros(t * 16, np.uint8), np.zeros(t * 384, np.int16)
-    lib.orc_hevc_encode_i(p(sy), p(scb), p(scr), p(ry), p(rcb), p(rcr), bd, wc, hc, qp, num_slices, round_q12,
-                          p(modes), p(coef))
+    if ctu_log2 == 4:
+        lib.orc_hevc_encode_i(p(sy), p(scb), p(scr), p(ry), p(rcb), p(rcr), bd, wc, hc, qp, num_slices, round_q12,
+                              p(modes), p(coef))
+    else:
+        lib.orc_hevc_encode_i_ctu(p(sy), p(scb), p(scr), p(ry), p(rcb), p(rcr), bd, wc, hc, qp, num_slices,
+                                  round_q12, ctu_log2, p(modes), p(coef))
     return dict(modes=modes, coef=coef, y=ry, cb=rcb, cr=rcr)
+
+
+def hevc_zrank(x, y, width, is_luma, ctu_log2):
+    return lib.orc_hevc_zrank(x, y, width, int(is_luma), ctu_log2)
 
 
 def hevc_encode_p(sy, scb, scr, fy, fcb, fcr, qp, mv, round_q12=HEVC_ROUND_INTER, bd=8):
@@ -307,7 +320,8 @@ def hevc_refine8(cur, ref, x0, y0, mvx, mvy, bd=8):
     return mx.value, my.value, sad
 
 
-def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lambda8=1, deblock=True, bd=8):
+def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lambda8=1, deblock=True, bd=8,
+                      ctu_log2=4):
     """One picture of the HEVC pipeline in its stage order: I -> CTU intra
     coding (rounding 4); P -> +-rng full search on 8x8 blocks (at 10 bits on
     the top 8 bits of each sample, v >> 2), quarter-sample refine with the
@@ -315,14 +329,16 @@ def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lamb
     2); then, with deblock, the 8.7.2 filter (bS from intra / coded luma /
     vector difference).  Planes are coded-size NV12 (cuv / ruv interleaved;
     uint8 at bd 8, else uint16 samples); returns coef, modes (uint8
-    [nctu*16]), mv8 (MV [nctu*4]) and the recon as NV12 (y, uv)."""
+    [nctu*16]), mv8 (MV [nctu*4]) and the recon as NV12 (y, uv).  ctu_log2 5
+    codes CTU32s: the I-picture order and availability and the slice rows
+    change; outputs stay indexed by 16x16 area."""
     hc, wc = cy.shape
     t = (wc // 16) * (hc // 16)
     pix = _pix(bd)
     cb, cr = split_uv(cuv)
     mv8 = np.zeros(t * 4, MV)
     if is_intra:
-        o = hevc_encode_i(cy, cb, cr, qp, num_slices, HEVC_ROUND_INTRA_I, bd)
+        o = hevc_encode_i(cy, cb, cr, qp, num_slices, HEVC_ROUND_INTRA_I, bd, ctu_log2)
         modes = o["modes"]
     else:
         s8 = (lambda a: np.asarray(a, np.uint8)) if bd == 8 else (lambda a: (np.asarray(a) >> (bd - 8)).astype(np.uint8))
@@ -344,7 +360,7 @@ def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lamb
         cbf = hevc_cu_cbf(o["coef"]) if not is_intra else np.zeros(t * 4, np.uint8)
         mv4 = np.zeros((t * 4, 4), np.int32)
         mv4[:, 0], mv4[:, 1] = mv8["mvx"], mv8["mvy"]
-        y, uv = hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices, bd)
+        y, uv = hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices, bd, ctu_log2)
     return dict(coef=o["coef"], modes=modes, mv8=mv8, y=y, uv=uv)
 
 
@@ -423,9 +439,10 @@ _sig("orc_hevc_deblock", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I])
 
 
 _sig("orc_hevc_deblock_bd", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I])
+_sig("orc_hevc_deblock_ctu", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I])
 
 
-def hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices=1, bd=8):
+def hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices=1, bd=8, ctu_log2=4):
     """Returns deblocked copies of the NV12 planes (uint8 at bd 8, else
     uint16 holding the sample in the low bits); cbf uint8 [ncu], mv4 int32
     [ncu][4]."""
@@ -435,5 +452,6 @@ def hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices=1, bd=8):
     hc, wc = y.shape
     cbf = np.ascontiguousarray(cbf, np.uint8)
     mv4 = np.ascontiguousarray(mv4, np.int32).reshape(-1, 4)
-    lib.orc_hevc_deblock_bd(p(y), p(uv), wc, wc, wc, hc, qp, int(is_intra), p(cbf), p(mv4), num_slices, bd)
+    lib.orc_hevc_deblock_ctu(p(y), p(uv), wc, wc, wc, hc, qp, int(is_intra), p(cbf), p(mv4), num_slices,
+                             ctu_log2, bd)
     return y, uv

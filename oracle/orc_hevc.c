@@ -256,22 +256,25 @@ void orc_hevc_mc_uni(const void *ref, int stride, int w_pic, int h_pic, int x, i
 }
 
 /* --------------------------------------------------------- intra 4x4 */
-/* z-scan rank for the fixed coding shape CTU16 -> 4 CU8 -> 4 PU4 (luma) or
- * CTU 8x8 -> 4 CU 4x4 (chroma), CTUs in raster order. */
-static long long zrank(int x, int y, int width, int is_luma)
+/* z-scan rank of the 4x4 block holding sample (x, y) (6.5.2): CTUs of
+ * 1 << ctu_log2 luma samples (16 or 32; chroma CTUs half that) in raster
+ * order, inside a CTU the Morton order of the 4x4 blocks (y bit above x bit
+ * at every level).  ctu_log2 4 is the reference's fixed shape CTU16 -> 4 CU8
+ * -> 4 PU4 (hevc_intra.c:75-130). */
+long long orc_hevc_zrank(int x, int y, int width, int is_luma, int ctu_log2)
 {
-    const int cs = is_luma ? 16 : 8, cu = cs / 2;
+    const int lg = is_luma ? ctu_log2 : ctu_log2 - 1, cs = 1 << lg;
     const int wctu = (width + cs - 1) / cs;
-    const int rx = x % cs, ry = y % cs;
-    long long r = ((long long)(y / cs) * wctu + x / cs) * 4 + (ry / cu) * 2 + rx / cu;
-    if (is_luma) r = r * 4 + ((ry % cu) / 4) * 2 + (rx % cu) / 4;
+    const int rx = (x % cs) >> 2, ry = (y % cs) >> 2;
+    long long r = (long long)(y / cs) * wctu + x / cs;
+    for (int b = lg - 3; b >= 0; b--) r = r * 4 + ((ry >> b) & 1) * 2 + ((rx >> b) & 1);
     return r;
 }
 
-static int zavail(int nx, int ny, int width, int height, int is_luma, int y_min, long long cur)
+static int zavail(int nx, int ny, int width, int height, int is_luma, int y_min, long long cur, int ctu_log2)
 {
     if (nx < 0 || ny < y_min || nx >= width || ny >= height) return 0;
-    return zrank(nx, ny, width, is_luma) < cur;
+    return orc_hevc_zrank(nx, ny, width, is_luma, ctu_log2) < cur;
 }
 
 /* Neighbours of the 4x4 block at (x0, y0): left[0..4] = p[-1][0..4],
@@ -279,13 +282,13 @@ static int zavail(int nx, int ny, int width, int height, int is_luma, int y_min,
  * over (p[-1][3..0], p[-1][-1], p[0..3][-1], p[4][-1]) and p[-1][4] read
  * when available, else copied from left[3]. */
 static void gather(const void *plane, int bd, int stride, int width, int height, int x0, int y0, int is_luma,
-                   int y_min, int left[5], int top[5], int *corner)
+                   int y_min, int ctu_log2, int left[5], int top[5], int *corner)
 {
-    const long long cur = zrank(x0, y0, width, is_luma);
-    const int aL = zavail(x0 - 1, y0, width, height, is_luma, y_min, cur);
-    const int aT = zavail(x0, y0 - 1, width, height, is_luma, y_min, cur);
-    const int aC = zavail(x0 - 1, y0 - 1, width, height, is_luma, y_min, cur);
-    const int aTR = zavail(x0 + 4, y0 - 1, width, height, is_luma, y_min, cur);
+    const long long cur = orc_hevc_zrank(x0, y0, width, is_luma, ctu_log2);
+    const int aL = zavail(x0 - 1, y0, width, height, is_luma, y_min, cur, ctu_log2);
+    const int aT = zavail(x0, y0 - 1, width, height, is_luma, y_min, cur, ctu_log2);
+    const int aC = zavail(x0 - 1, y0 - 1, width, height, is_luma, y_min, cur, ctu_log2);
+    const int aTR = zavail(x0 + 4, y0 - 1, width, height, is_luma, y_min, cur, ctu_log2);
     int v[10], a[10];
     for (int i = 0; i < 4; i++) {
         a[i] = aL;
@@ -313,7 +316,7 @@ static void gather(const void *plane, int bd, int stride, int width, int height,
     }
     *corner = v[4];
     top[4] = v[9];
-    left[4] = zavail(x0 - 1, y0 + 4, width, height, is_luma, y_min, cur)
+    left[4] = zavail(x0 - 1, y0 + 4, width, height, is_luma, y_min, cur, ctu_log2)
                   ? px_at(plane, bd, stride, width, height, x0 - 1, y0 + 4)
                   : left[3];
 }
@@ -352,22 +355,28 @@ static void pred_refs(const int left[5], const int top[5], int corner, int mode,
     }
 }
 
-void orc_hevc_predict4(const void *plane, int bd, int stride, int width, int height, int x0, int y0, int mode,
-                       int is_luma, int y_min, int32_t pred[16])
+static void predict4(const void *plane, int bd, int stride, int width, int height, int x0, int y0, int mode,
+                     int is_luma, int y_min, int ctu_log2, int32_t pred[16])
 {
     int l[5], t[5], c, o[16];
-    gather(plane, bd, stride, width, height, x0, y0, is_luma, y_min, l, t, &c);
+    gather(plane, bd, stride, width, height, x0, y0, is_luma, y_min, ctu_log2, l, t, &c);
     pred_refs(l, t, c, mode, is_luma, bd, o);
     for (int i = 0; i < 16; i++) pred[i] = o[i];
 }
 
+void orc_hevc_predict4(const void *plane, int bd, int stride, int width, int height, int x0, int y0, int mode,
+                       int is_luma, int y_min, int32_t pred[16])
+{
+    predict4(plane, bd, stride, width, height, x0, y0, mode, is_luma, y_min, 4, pred);
+}
+
 /* SAD mode decision in the order Planar, DC, H, V with strict '<'. */
-int orc_hevc_choose_mode(const void *src, const void *recon, int bd, int stride, int width, int height, int x0,
-                         int y0, int y_min, int32_t pred[16])
+static int choose_mode(const void *src, const void *recon, int bd, int stride, int width, int height, int x0,
+                       int y0, int y_min, int ctu_log2, int32_t pred[16])
 {
     static const int modes[4] = {0, 1, 10, 26};
     int l[5], t[5], c;
-    gather(recon, bd, stride, width, height, x0, y0, 1, y_min, l, t, &c);
+    gather(recon, bd, stride, width, height, x0, y0, 1, y_min, ctu_log2, l, t, &c);
     int best = -1, best_sad = 0;
     for (int m = 0; m < 4; m++) {
         int o[16], sad = 0;
@@ -380,6 +389,12 @@ int orc_hevc_choose_mode(const void *src, const void *recon, int bd, int stride,
         }
     }
     return best;
+}
+
+int orc_hevc_choose_mode(const void *src, const void *recon, int bd, int stride, int width, int height, int x0,
+                         int y0, int y_min, int32_t pred[16])
+{
+    return choose_mode(src, recon, bd, stride, width, height, x0, y0, y_min, 4, pred);
 }
 
 static void put(void *p, int bd, int stride, int x, int y, int v)
@@ -406,49 +421,70 @@ int orc_hevc_slice_y_min(int ctu_row, int ctu_rows, int ns)
     return 0;
 }
 
-/* I picture: every CTU split into 4 CU8, every luma CU into 4 PU4 (intra,
- * DST, SAD mode choice), chroma one 4x4 DC block per CU and component
- * (DCT, QpC).  Outputs per CTU (z-order inside): modes[16] (PU z-order),
+/* First luma row of the slice containing CTU16 row r16 when slices are
+ * ranges of CTUs of 1 << ctu_log2 luma rows (16 or 32), split as above. */
+int orc_hevc_slice_y_min_ctu(int r16, int rows16, int ns, int ctu_log2)
+{
+    const int s = ctu_log2 - 4, rows = (rows16 + (1 << s) - 1) >> s;
+    return orc_hevc_slice_y_min(r16 >> s, rows, ns) << s;
+}
+
+/* I picture: CTUs of 1 << ctu_log2 (16 or 32) luma samples in raster order,
+ * a CTU32 split into its four 16x16 quadrants in z-order (those inside the
+ * picture), every 16x16 into 4 CU8, every luma CU into 4 PU4 (intra, DST, SAD
+ * mode choice), chroma one 4x4 DC block per CU and component (DCT, QpC),
+ * neighbours available by the z-scan order of that CTU size.  Outputs per
+ * 16x16 area in raster order (z-order inside): modes[16] (PU z-order),
  * coef[384] = 16 luma PUs x 16, then 4 Cb CUs x 16, then 4 Cr CUs x 16 (each
  * 4x4 row-major); recon planes written in place (start them as any value). */
-void orc_hevc_encode_i(const void *sy, const void *scb, const void *scr, void *ry, void *rcb, void *rcr, int bd,
-                       int wc, int hc, int qp, int num_slices, int round_q12, uint8_t *modes, int16_t *coef)
+void orc_hevc_encode_i_ctu(const void *sy, const void *scb, const void *scr, void *ry, void *rcb, void *rcr, int bd,
+                           int wc, int hc, int qp, int num_slices, int round_q12, int ctu_log2, uint8_t *modes,
+                           int16_t *coef)
 {
     static const int ox[4] = {0, 4, 0, 4}, oy[4] = {0, 0, 4, 4};
     const int wctu = wc / 16, hctu = hc / 16, cwc = wc / 2, chc = hc / 2, maxv = (1 << bd) - 1;
     const int qoff = bd == 8 ? 0 : 12;
     const int cqp = orc_hevc_chroma_qp(qp) + qoff;
-    for (int ty = 0; ty < hctu; ty++) {
-        const int y_min = orc_hevc_slice_y_min(ty, hctu, num_slices);
-        for (int tx = 0; tx < wctu; tx++) {
-            const int t = ty * wctu + tx;
-            for (int cu = 0; cu < 4; cu++) {
-                const int cx = tx * 16 + ox[cu] * 2, cy = ty * 16 + oy[cu] * 2;
-                for (int pu = 0; pu < 4; pu++) {
-                    const int px = cx + ox[pu], py = cy + oy[pu], k = cu * 4 + pu;
-                    int32_t pred[16];
-                    modes[t * 16 + k] = (uint8_t)orc_hevc_choose_mode(sy, ry, bd, wc, wc, hc, px, py, y_min, pred);
-                    int16_t res[16], rr[16], *lv = coef + (size_t)t * 384 + k * 16;
-                    for (int i = 0; i < 16; i++) res[i] = (int16_t)(get(sy, bd, wc, px + (i & 3), py + (i >> 2)) - pred[i]);
-                    orc_hevc_tq4(res, qp + qoff, 1, round_q12, bd, lv);
-                    orc_hevc_dqit4(lv, qp + qoff, 1, bd, rr);
-                    for (int i = 0; i < 16; i++) put(ry, bd, wc, px + (i & 3), py + (i >> 2), clampi(pred[i] + rr[i], 0, maxv));
-                }
-                const int ux = cx / 2, uy = cy / 2;
-                for (int comp = 0; comp < 2; comp++) {
-                    const void *s = comp ? scr : scb;
-                    void *r = comp ? rcr : rcb;
-                    int32_t pred[16];
-                    orc_hevc_predict4(r, bd, cwc, cwc, chc, ux, uy, 1, 0, y_min / 2, pred);
-                    int16_t res[16], rr[16], *lv = coef + (size_t)t * 384 + 256 + comp * 64 + cu * 16;
-                    for (int i = 0; i < 16; i++) res[i] = (int16_t)(get(s, bd, cwc, ux + (i & 3), uy + (i >> 2)) - pred[i]);
-                    orc_hevc_tq4(res, cqp, 0, round_q12, bd, lv);
-                    orc_hevc_dqit4(lv, cqp, 0, bd, rr);
-                    for (int i = 0; i < 16; i++) put(r, bd, cwc, ux + (i & 3), uy + (i >> 2), clampi(pred[i] + rr[i], 0, maxv));
+    const int sub = ctu_log2 == 5 ? 2 : 1;  /* 16x16 areas per CTU side */
+    for (int cty = 0; cty * sub < hctu; cty++)
+        for (int ctx = 0; ctx * sub < wctu; ctx++)
+            for (int q = 0; q < sub * sub; q++) {
+                const int tx = ctx * sub + (q & 1), ty = cty * sub + (q >> 1);
+                if (tx >= wctu || ty >= hctu) continue;
+                const int y_min = orc_hevc_slice_y_min_ctu(ty, hctu, num_slices, ctu_log2);
+                const int t = ty * wctu + tx;
+                for (int cu = 0; cu < 4; cu++) {
+                    const int cx = tx * 16 + ox[cu] * 2, cy = ty * 16 + oy[cu] * 2;
+                    for (int pu = 0; pu < 4; pu++) {
+                        const int px = cx + ox[pu], py = cy + oy[pu], k = cu * 4 + pu;
+                        int32_t pred[16];
+                        modes[t * 16 + k] = (uint8_t)choose_mode(sy, ry, bd, wc, wc, hc, px, py, y_min, ctu_log2, pred);
+                        int16_t res[16], rr[16], *lv = coef + (size_t)t * 384 + k * 16;
+                        for (int i = 0; i < 16; i++) res[i] = (int16_t)(get(sy, bd, wc, px + (i & 3), py + (i >> 2)) - pred[i]);
+                        orc_hevc_tq4(res, qp + qoff, 1, round_q12, bd, lv);
+                        orc_hevc_dqit4(lv, qp + qoff, 1, bd, rr);
+                        for (int i = 0; i < 16; i++) put(ry, bd, wc, px + (i & 3), py + (i >> 2), clampi(pred[i] + rr[i], 0, maxv));
+                    }
+                    const int ux = cx / 2, uy = cy / 2;
+                    for (int comp = 0; comp < 2; comp++) {
+                        const void *s = comp ? scr : scb;
+                        void *r = comp ? rcr : rcb;
+                        int32_t pred[16];
+                        predict4(r, bd, cwc, cwc, chc, ux, uy, 1, 0, y_min / 2, ctu_log2, pred);
+                        int16_t res[16], rr[16], *lv = coef + (size_t)t * 384 + 256 + comp * 64 + cu * 16;
+                        for (int i = 0; i < 16; i++) res[i] = (int16_t)(get(s, bd, cwc, ux + (i & 3), uy + (i >> 2)) - pred[i]);
+                        orc_hevc_tq4(res, cqp, 0, round_q12, bd, lv);
+                        orc_hevc_dqit4(lv, cqp, 0, bd, rr);
+                        for (int i = 0; i < 16; i++) put(r, bd, cwc, ux + (i & 3), uy + (i >> 2), clampi(pred[i] + rr[i], 0, maxv));
+                    }
                 }
             }
-        }
-    }
+}
+
+void orc_hevc_encode_i(const void *sy, const void *scb, const void *scr, void *ry, void *rcb, void *rcr, int bd,
+                       int wc, int hc, int qp, int num_slices, int round_q12, uint8_t *modes, int16_t *coef)
+{
+    orc_hevc_encode_i_ctu(sy, scb, scr, ry, rcb, rcr, bd, wc, hc, qp, num_slices, round_q12, 4, modes, coef);
 }
 
 /* P picture, every CU8 inter with the given vector (mv[cu] quarter luma
@@ -638,15 +674,15 @@ int orc_hevc_bs(int is_intra, const uint8_t *cbf, const int32_t *mv4, int cp, in
  * uint8 at bd 8, else uint16 with the sample in the low bits; mv4: {mvx,
  * mvy, ., .} per CU; cbf: per CU.  beta and tC scale by 2^(bd-8) (8.7.2.5.3,
  * 8.7.2.5.5). */
-void orc_hevc_deblock_bd(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
-                         const uint8_t *cbf, const int32_t *mv4, int num_slices, int bd)
+void orc_hevc_deblock_ctu(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                          const uint8_t *cbf, const int32_t *mv4, int num_slices, int ctu_log2, int bd)
 {
     const int wctu = wc / 16, hctu = hc / 16;
     const int beta = kBeta[clampi(qp, 0, 51)] << (bd - 8);
     const int qpc = orc_hevc_chroma_qp(qp);
     for (int dir = 0; dir < 2; dir++) {       /* 0: vertical edges, 1: horizontal edges */
         for (int e = 8; e < (dir ? hc : wc); e += 8) {
-            if (dir && (e & 15) == 0 && orc_hevc_slice_y_min(e >> 4, hctu, num_slices) == e) continue;
+            if (dir && (e & 15) == 0 && orc_hevc_slice_y_min_ctu(e >> 4, hctu, num_slices, ctu_log2) == e) continue;
             for (int s = 0; s < (dir ? wc : hc); s += 4) {   /* 4-sample segments along the edge */
                 const int x = dir ? s : e, yy = dir ? e : s;
                 const int cq = cu_at(x, yy, wctu), cp = cu_at(dir ? x : x - 1, dir ? yy - 1 : yy, wctu);
@@ -669,6 +705,12 @@ void orc_hevc_deblock_bd(void *y, void *uv, int py, int puv, int wc, int hc, int
             }
         }
     }
+}
+
+void orc_hevc_deblock_bd(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
+                         const uint8_t *cbf, const int32_t *mv4, int num_slices, int bd)
+{
+    orc_hevc_deblock_ctu(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mv4, num_slices, 4, bd);
 }
 
 void orc_hevc_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,

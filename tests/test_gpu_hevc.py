@@ -208,8 +208,21 @@ def test_p_encode_stage(pkg, qp, bd):
                                            (320, 240, 32, 4, 8), (64, 48, 22, 1, 10), (128, 96, 45, 2, 10),
                                            (208, 128, 27, 3, 10), (320, 240, 0, 4, 10)])
 def test_i_encode_stage(pkg, w, h, qp, ns, bd):
+    _i_encode_case(pkg, w, h, qp, ns, bd, 4)
+
+
+@pytest.mark.parametrize("w,h,qp,ns,bd", [(112, 80, 27, 1, 8), (128, 96, 37, 2, 8), (320, 240, 32, 4, 8),
+                                           (48, 16, 27, 1, 8), (208, 144, 22, 3, 10), (64, 48, 45, 1, 10)])
+def test_i_encode_ctu32_stage(pkg, w, h, qp, ns, bd):
+    """CTU32 order and availability: partial CTU32s at the right and bottom
+    edges (112x80 is 7x5 16x16 areas), a single 16-row picture (no lower
+    row), CTU32-row slices, 10 bit."""
+    _i_encode_case(pkg, w, h, qp, ns, bd, 5)
+
+
+def _i_encode_case(pkg, w, h, qp, ns, bd, ctu_log2):
     torch = _torch()
-    say(f"hevc I {w}x{h} qp={qp} slices={ns} bd={bd}")
+    say(f"hevc I {w}x{h} qp={qp} slices={ns} bd={bd} ctu_log2={ctu_log2}")
     (cy, cuv), = _frames(pkg, 5 + qp, w, h, 1, 3)
     hc, wc = cy.shape
     t = (wc // 16) * (hc // 16)
@@ -217,7 +230,7 @@ def test_i_encode_stage(pkg, w, h, qp, ns, bd):
         rs = np.random.default_rng(qp + 1)
         cy, cuv = _ten(cy, rs), _ten(cuv, rs)
     cb, cr = orc.split_uv(cuv)
-    want = orc.hevc_encode_i(cy, cb, cr, qp, ns, orc.HEVC_ROUND_INTRA_I, bd)
+    want = orc.hevc_encode_i(cy, cb, cr, qp, ns, orc.HEVC_ROUND_INTRA_I, bd, ctu_log2)
     if bd == 10:
         d, pitch, back = [_p010(torch, a) for a in (cy, cuv)], wc * 2, _unp010
     else:
@@ -227,9 +240,14 @@ def test_i_encode_stage(pkg, w, h, qp, ns, bd):
     coef = torch.zeros(t * 384, dtype=torch.int16, device="cuda")
     nwork = pkg.hip.bc250_hevc_intra_work_bytes(wc, hc)
     sync = torch.zeros((nwork + 3) // 4, dtype=torch.int32, device="cuda")
-    assert pkg.hip.bc250_hevc_launch_i_encode(P(d[0]), P(d[1]), pitch, pitch, P(oy), P(ouv), pitch, pitch, wc, hc, qp,
-                                              ns, orc.HEVC_ROUND_INTRA_I, P(modes), P(coef), P(sync), 1 << 22, bd,
-                                              None) == 0
+    if ctu_log2 == 4:
+        assert pkg.hip.bc250_hevc_launch_i_encode(P(d[0]), P(d[1]), pitch, pitch, P(oy), P(ouv), pitch, pitch, wc, hc,
+                                                  qp, ns, orc.HEVC_ROUND_INTRA_I, P(modes), P(coef), P(sync), 1 << 22,
+                                                  bd, None) == 0
+    else:
+        assert pkg.hip.bc250_hevc_launch_i_encode_ctu(P(d[0]), P(d[1]), pitch, pitch, P(oy), P(ouv), pitch, pitch,
+                                                      wc, hc, qp, ns, orc.HEVC_ROUND_INTRA_I, P(modes), P(coef),
+                                                      P(sync), 1 << 22, ctu_log2, bd, None) == 0
     torch.cuda.synchronize()
     assert host(sync)[1] == 0, "wavefront wait gave up"
     assert np.array_equal(host(modes), want["modes"])
@@ -239,11 +257,11 @@ def test_i_encode_stage(pkg, w, h, qp, ns, bd):
     assert np.array_equal(gb, want["cb"]) and np.array_equal(gr, want["cr"])
 
 
-def _hevc_sequence(pkg, w, h, n, qp, ns, rng, lam, seed, nrect, gop=0, deblock=True, bd=8):
-    say(f"hevc pipeline {w}x{h} frames={n} qp={qp} slices={ns} R={rng} deblock={deblock} bd={bd}")
+def _hevc_sequence(pkg, w, h, n, qp, ns, rng, lam, seed, nrect, gop=0, deblock=True, bd=8, ctu_log2=4):
+    say(f"hevc pipeline {w}x{h} frames={n} qp={qp} slices={ns} R={rng} deblock={deblock} bd={bd} ctu={ctu_log2}")
     g = pkg.FrameGen(seed, w, h, nrect)
     enc = pkg.Encoder(w, h, search_range=rng, lambda8=lam, deblock_idc=0 if deblock else 1,
-                      codec=pkg.GPU_CODEC_HEVC, bit_depth=bd)
+                      codec=pkg.GPU_CODEC_HEVC, bit_depth=bd, ctu_log2=ctu_log2)
     rs = np.random.default_rng(seed)
     ref = None
     try:
@@ -258,7 +276,7 @@ def _hevc_sequence(pkg, w, h, n, qp, ns, rng, lam, seed, nrect, gop=0, deblock=T
             got = enc.results(slot)
             cy, cuv = pkg.pad_coded(vy, vuv)
             ry, ruv = (cy, cuv) if ref is None else ref
-            exp = orc.hevc_encode_frame(cy, cuv, ry, ruv, qp, intra, ns, rng, lam, deblock, bd)
+            exp = orc.hevc_encode_frame(cy, cuv, ry, ruv, qp, intra, ns, rng, lam, deblock, bd, ctu_log2)
             assert np.array_equal(got["mv8"], exp["mv8"]), f"frame {i} mv8"
             assert np.array_equal(got["hevc_mode"], exp["modes"]), f"frame {i} modes"
             assert np.array_equal(got["quant"], exp["coef"]), f"frame {i} coef"
@@ -295,14 +313,25 @@ def test_hevc_pipeline_main10(pkg):
     _hevc_sequence(pkg, 200, 120, 5, 27, 3, 16, 1, seed=41, nrect=2, gop=3, bd=10)
 
 
+def test_hevc_pipeline_ctu32_slices_gop(pkg):
+    """CTU32: odd size (partial CTU32s), CTU32-row slices, I pictures in a GOP."""
+    _torch()
+    _hevc_sequence(pkg, 200, 120, 5, 32, 3, 16, 1, seed=43, nrect=2, gop=3, ctu_log2=5)
+
+
+def test_hevc_pipeline_ctu32_main10(pkg):
+    _torch()
+    _hevc_sequence(pkg, 240, 144, 4, 27, 2, 16, 1, seed=44, nrect=3, gop=2, bd=10, ctu_log2=5)
+
+
 def test_hevc_pipeline_main10_no_deblock(pkg):
     _torch()
     _hevc_sequence(pkg, 128, 96, 3, 37, 1, 16, 1, seed=42, nrect=3, deblock=False, bd=10)
 
 
 def test_hevc_main10_4k_full_size(pkg):
-    """The 4K Main10 workload at its real shape: 3840x2160 P010, CQP 27, 4
-    slices, +-64 search, deblocking.  The I picture is compared whole with
+    """The 4K Main10 workload at its real shape: 3840x2160 P010, CTU32, CQP
+    27, 4 slices, +-64 search, deblocking.  The I picture is compared whole with
     the oracle.  For the P picture the oracle codes the frame with the GPU's
     vectors (levels and deblocked reconstruction must match exactly), and a
     spread sample of 64 MBs is searched and refined by the oracle from
@@ -314,14 +343,14 @@ def test_hevc_main10_4k_full_size(pkg):
     rs = np.random.default_rng(2160)
     vis = [tuple(_ten(a, rs) for a in g.frame(i)) for i in range(2)]
     g.close()
-    enc = pkg.Encoder(w, h, search_range=rng, lambda8=1, codec=pkg.GPU_CODEC_HEVC, bit_depth=10)
+    enc = pkg.Encoder(w, h, search_range=rng, lambda8=1, codec=pkg.GPU_CODEC_HEVC, bit_depth=10, ctu_log2=5)
     try:
         cods = [pkg.pad_coded(*v) for v in vis]
         im = enc.new_image()
         enc.upload(im, *vis[0])
         got = enc.results(enc.encode(im, qp, True, ns))
         cy, cuv = cods[0]
-        exp = orc.hevc_encode_frame(cy, cuv, cy, cuv, qp, True, ns, rng, 1, True, 10)
+        exp = orc.hevc_encode_frame(cy, cuv, cy, cuv, qp, True, ns, rng, 1, True, 10, 5)
         assert np.array_equal(got["hevc_mode"], exp["modes"]) and np.array_equal(got["quant"], exp["coef"])
         my, muv = enc.map_recon()
         assert np.array_equal(my, exp["y"]) and np.array_equal(muv, exp["uv"]), "I recon"
@@ -342,7 +371,7 @@ def test_hevc_main10_4k_full_size(pkg):
         uv[:, 0::2], uv[:, 1::2] = o["cb"], o["cr"]
         mv4 = np.zeros((mv.shape[0], 4), np.int32)
         mv4[:, :2] = mv
-        wy, wuv = orc.hevc_deblock(o["y"], uv, qp, 0, orc.hevc_cu_cbf(o["coef"]), mv4, ns, 10)
+        wy, wuv = orc.hevc_deblock(o["y"], uv, qp, 0, orc.hevc_cu_cbf(o["coef"]), mv4, ns, 10, 5)
         my, muv = enc.map_recon()
         assert np.array_equal(my, wy) and np.array_equal(muv, wuv), "P recon"
         c8, r8 = (cy >> 2).astype(np.uint8), (ry >> 2).astype(np.uint8)
@@ -464,6 +493,30 @@ def test_deblock(pkg, w, h, qp, intra, ns):
     assert (oy[:, w:] == 7).all() and (ouv[:, w:] == 9).all()
     assert pkg.hip.bc250_hevc_launch_deblock(P(d[0]), P(d[1]), pitch, pitch, w + 8, h, qp, intra, P(d[2]), P(d[3]),
                                              ns, 8, None) == -1
+
+
+@pytest.mark.parametrize("w,h,qp,intra,ns", [(192, 112, 45, 1, 2), (320, 240, 27, 0, 3), (208, 144, 37, 1, 3)])
+def test_deblock_ctu32_slices(pkg, w, h, qp, intra, ns):
+    """Slices as CTU32-row ranges: no filtering across their borders only."""
+    torch = _torch()
+    say(f"hevc deblock ctu32 {w}x{h} qp={qp} intra={intra} slices={ns}")
+    rs = np.random.default_rng(w * 11 + qp)
+    y = _blocky(rs, h, w, 12)
+    uv = _blocky(rs, h // 2, w, 10)
+    ncu = (w // 8) * (h // 8)
+    cbf = (rs.random(ncu) < 0.3).astype(np.uint8)
+    mv4 = np.zeros((ncu, 4), np.int32)
+    mv4[:, :2] = rs.integers(-6, 7, (ncu, 2))
+    wy, wuv = orc.hevc_deblock(y, uv, qp, intra, cbf, mv4, ns, 8, 5)
+    assert not np.array_equal(wy, orc.hevc_deblock(y, uv, qp, intra, cbf, mv4, ns, 8, 4)[0])
+    d = [dev(torch, a) for a in (y, uv, cbf, mv4)]
+    assert pkg.hip.bc250_hevc_launch_deblock_ctu(P(d[0]), P(d[1]), w, w, w, h, qp, intra, P(d[2]), P(d[3]), ns, 5,
+                                                 8, None) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(host(d[0]), wy)
+    assert np.array_equal(host(d[1]), wuv)
+    assert pkg.hip.bc250_hevc_launch_deblock_ctu(P(d[0]), P(d[1]), w, w, w, h, qp, intra, P(d[2]), P(d[3]), ns, 6,
+                                                 8, None) == -1
 
 
 @pytest.mark.parametrize("w,h,qp,intra,ns", [(64, 48, 37, 1, 1), (192, 112, 30, 0, 2), (320, 240, 45, 1, 3),

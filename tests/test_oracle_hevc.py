@@ -16,6 +16,9 @@ the HIP kernels.
 3. Closed forms: DC-only round trips, the transforms' matrix products, the
    quantiser as the inverse of the dequantiser, refinement never worse than
    its start.
+4. CTU32 (no reference counterpart): the z-scan rank against the standard's
+   MinTbAddrZs formula (6.5.2), CTU32-row slices, and CTU32 coding equal to
+   CTU16 coding where both orders and neighbourhoods coincide.
 """
 import ctypes as C
 import os
@@ -323,6 +326,67 @@ def test_encode_i_matches_reference_order(bd, qp, ns):
     for k in ("modes", "coef", "y", "cb", "cr"):
         assert np.array_equal(got[k], want[k]), k
     assert len(set(got["modes"].tolist())) > 1  # the decision is exercised
+
+
+def _spec_min_tb_addr_zs(x, y, width, ctb_log2):
+    """6.5.2 (no tiles): MinTbAddrZs of the 4x4 block at sample (x, y):
+    CtbAddrRs << (2 (CtbLog2 - 2)) plus, for every bit i of the block's
+    coordinates inside the CTB, m*m for an x bit and 2*m*m for a y bit
+    (m = 1 << i)."""
+    pic_w_ctbs = -(-width // (1 << ctb_log2))
+    bx, by = x >> 2, y >> 2
+    tb_x, tb_y = (bx << 2) >> ctb_log2, (by << 2) >> ctb_log2
+    addr = (pic_w_ctbs * tb_y + tb_x) << ((ctb_log2 - 2) * 2)
+    for i in range(ctb_log2 - 2):
+        m = 1 << i
+        addr += (m * m if m & bx else 0) + (2 * m * m if m & by else 0)
+    return addr
+
+
+@pytest.mark.parametrize("w,h", [(48, 32), (112, 80), (64, 64)])
+def test_zscan_rank_is_6_5_2(w, h):
+    """The oracle's z-scan rank for CTU16 and CTU32 (luma, and chroma at half
+    the CTU size) equals the standard's MinTbAddrZs at every 4x4 block,
+    partial CTUs at the right and bottom edges included."""
+    for ctu_log2 in (4, 5):
+        for luma in (1, 0):
+            cw, ch, lg = (w, h, ctu_log2) if luma else (w // 2, h // 2, ctu_log2 - 1)
+            for y in range(0, ch, 4):
+                for x in range(0, cw, 4):
+                    assert orc.hevc_zrank(x, y, cw, luma, ctu_log2) == _spec_min_tb_addr_zs(x, y, cw, lg), \
+                        (ctu_log2, luma, x, y)
+
+
+def test_ctu32_slices_are_ctu32_rows():
+    lib = orc.lib
+    for rows16 in (1, 2, 5, 8, 9, 135):
+        rows32 = (rows16 + 1) // 2
+        for ns in (1, 2, 3, 4):
+            for r16 in range(rows16):
+                starts = [rows32 * s // min(ns, rows32) for s in range(min(ns, rows32))]
+                r0 = max(b for b in starts if r16 // 2 >= b)
+                assert lib.orc_hevc_slice_y_min_ctu(r16, rows16, ns, 5) == 32 * r0
+                assert lib.orc_hevc_slice_y_min_ctu(r16, rows16, ns, 4) == lib.orc_hevc_slice_y_min(r16, rows16, ns)
+
+
+@pytest.mark.parametrize("w,bd", [(32, 8), (16, 10)])
+def test_ctu32_equals_ctu16_when_one_ctu32_wide(w, bd):
+    """A picture one CTU32 wide (or half of one) is coded in the same order
+    with the same neighbours either way, so CTU32 and CTU16 coding agree;
+    a wider picture does not (the bottom-right quadrants lose their
+    top-right neighbours)."""
+    rs = np.random.default_rng(w)
+    sy, scb, scr = _frame(rs, w, 96, bd)
+    a = orc.hevc_encode_i(sy, scb, scr, 27, 1, orc.HEVC_ROUND_INTRA_I, bd, 4)
+    b = orc.hevc_encode_i(sy, scb, scr, 27, 1, orc.HEVC_ROUND_INTRA_I, bd, 5)
+    for k in ("modes", "coef", "y", "cb", "cr"):
+        assert np.array_equal(a[k], b[k]), k
+    sy, scb, scr = _frame(rs, 96, 64, bd)
+    a = orc.hevc_encode_i(sy, scb, scr, 27, 1, orc.HEVC_ROUND_INTRA_I, bd, 4)
+    b = orc.hevc_encode_i(sy, scb, scr, 27, 1, orc.HEVC_ROUND_INTRA_I, bd, 5)
+    # the first CTU32's first three quadrants see the same neighbours
+    assert np.array_equal(a["y"][:16, :32], b["y"][:16, :32]) and np.array_equal(a["y"][16:32, :16], b["y"][16:32, :16])
+    assert not np.array_equal(a["y"], b["y"])
 
 
 @need_ref
