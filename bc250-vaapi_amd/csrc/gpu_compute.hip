@@ -64,6 +64,7 @@ struct bc250_hip_state {
     uint32_t *mvs = nullptr, *mv8 = nullptr, *imv16 = nullptr, *imv8 = nullptr;
     int *sync_i = nullptr, *sync_d = nullptr, *sync_h = nullptr;
     uint8_t *hmodes = nullptr, *hcbf = nullptr;  // HEVC PU modes, per-CU coded-luma flags
+    uint8_t *hculog2 = nullptr;                   // HEVC P: log2 CU size per CU8 (CTU32 CU choice)
     Slot slot[NSLOT];
     int cur = 0, submitted = 0;
     double last_latency_ms = 0.0;
@@ -156,12 +157,12 @@ static void free_res(bc250_hip_state *s)
     if (s->me8) hipFree(s->me8);
     s->me8 = nullptr;
     void *dev[] = {s->levels, s->dc,     s->nz,     s->modes,  s->mvs,   s->mv8,
-                   s->imv16,  s->imv8, s->sync_i, s->sync_d, s->sync_h, s->hmodes, s->hcbf};
+                   s->imv16,  s->imv8, s->sync_i, s->sync_d, s->sync_h, s->hmodes, s->hcbf, s->hculog2};
     for (void *p : dev)
         if (p) hipFree(p);
     s->levels = nullptr; s->dc = nullptr; s->nz = nullptr; s->modes = nullptr; s->mvs = nullptr;
     s->mv8 = nullptr; s->imv16 = nullptr; s->imv8 = nullptr; s->sync_i = nullptr; s->sync_d = nullptr;
-    s->sync_h = nullptr; s->hmodes = nullptr; s->hcbf = nullptr;
+    s->sync_h = nullptr; s->hmodes = nullptr; s->hcbf = nullptr; s->hculog2 = nullptr;
     for (auto &sl : s->slot) {
         void *host[] = {sl.levels, sl.dc, sl.modes, sl.nz, sl.mvs, sl.mv8, sl.hmodes, sl.err};
         for (void *p : host)
@@ -203,7 +204,8 @@ static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
          HIP_OK(hipMalloc(&s->sync_i, bc250_intra_work_bytes(wc, hc))) &&
          HIP_OK(hipMalloc(&s->sync_d, bc250_deblock_work_bytes(wc, hc))) &&
          HIP_OK(hipMalloc(&s->sync_h, bc250_hevc_intra_work_bytes(wc, hc))) &&
-         HIP_OK(hipMalloc(&s->hmodes, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->hcbf, (size_t)nmb * 4));
+         HIP_OK(hipMalloc(&s->hmodes, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->hcbf, (size_t)nmb * 4)) &&
+         HIP_OK(hipMalloc(&s->hculog2, (size_t)nmb * 4));
     for (auto &sl : s->slot) {
         ok = ok && HIP_OK(hipHostMalloc((void **)&sl.levels, (size_t)nmb * 384 * 2, hipHostMallocDefault)) &&
              HIP_OK(hipHostMalloc((void **)&sl.dc, (size_t)nmb * 24 * 4, hipHostMallocDefault)) &&
@@ -566,20 +568,25 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
                                                    st);
             if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, (int)n * 16, st);
         } else {
-            r = bc250_hevc_launch_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv, out.y,
-                                           out.uv, out.py, out.puv, s->wc, s->hc, qp, 2 /* inter rounding */, s->mv8,
-                                           (int16_t *)s->levels, bd, st);
-            if (r == 0) r = bc250_launch_zero_i32((int *)s->hmodes, (int)n * 4, st);
+            // P modes: zero, or at CTU32 the log2 CU size of each CU8 at [16 t + z]
+            r = bc250_launch_zero_i32((int *)s->hmodes, (int)n * 4, st);
+            if (r == 0)
+                r = bc250_hevc_launch_p_encode_ctu(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv,
+                                                   out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp,
+                                                   2 /* inter rounding */, s->mv8, (int16_t *)s->levels,
+                                                   p.hevc_ctu_log2, s->hculog2,
+                                                   p.hevc_ctu_log2 == 5 ? s->hmodes : nullptr, bd, st);
         }
     }
     hipEventRecord(sl.ev[3], st);
     // 8.7.2 deblocking (the reference encoder's PPS turns it off: deblock_idc 1);
     // slices never filter across their borders, so idc 0 and 2 agree
     if (r == 0 && p.deblock_idc != 1) {
-        if (!is_intra) r = bc250_hevc_cu_cbf((const int16_t *)s->levels, (int)n, s->hcbf, st);
+        const uint8_t *cus = !is_intra && p.hevc_ctu_log2 == 5 ? s->hculog2 : nullptr;
+        if (!is_intra) r = bc250_hevc_cu_cbf_ctu((const int16_t *)s->levels, s->wc, s->hc, cus, s->hcbf, st);
         if (r == 0)
             r = bc250_hevc_launch_deblock_ctu(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->hcbf,
-                                              s->mv8, num_slices, p.hevc_ctu_log2, bd, st);
+                                              s->mv8, cus, num_slices, p.hevc_ctu_log2, bd, st);
     }
     hipEventRecord(sl.ev[4], st);
     if (r == 0 && p.readback) {

@@ -29,6 +29,17 @@ BC_DEV int cu_at(int x, int y, int wctu)
     return ((y >> 4) * wctu + (x >> 4)) * 4 + ((y >> 3) & 1) * 2 + ((x >> 3) & 1);
 }
 
+// Whether the 8x8-grid edge at coordinate e (x of a vertical edge, y of a
+// horizontal one) between CU8 p (before it) and q lies inside one CU16 /
+// CU32, i.e. is neither a TU nor a PU edge: both sides carry the same CU size
+// 2^L >= 16 and e is not a multiple of 2^L.
+BC_DEV bool inside_cu(const uint8_t *culog2, int cp, int cq, int e)
+{
+    if (!culog2) return false;
+    const int L = culog2[cq];
+    return L >= 4 && culog2[cp] == L && (e & ((1 << L) - 1)) != 0;
+}
+
 BC_DEV int strength(int intra, const uint8_t *cbf, const uint4 *mv, int cp, int cq)
 {
     if (intra) return 2;
@@ -92,7 +103,8 @@ template <typename Pix, int BD, bool VERT>
 __global__ __launch_bounds__(256) void hevc_deblock_kernel(uint8_t *__restrict__ y, uint8_t *__restrict__ uv, int py,
                                                            int puv, int wc, int hc, int qp, int intra,
                                                            const uint8_t *__restrict__ cbf,
-                                                           const uint4 *__restrict__ mv, int num_slices,
+                                                           const uint4 *__restrict__ mv,
+                                                           const uint8_t *__restrict__ culog2, int num_slices,
                                                            int ctu_log2, int nl, int nc)
 {
     constexpr int MAXV = (1 << BD) - 1;
@@ -105,7 +117,7 @@ __global__ __launch_bounds__(256) void hevc_deblock_kernel(uint8_t *__restrict__
         if (!VERT && (e & 15) == 0 && slice_first_row16(e >> 4, hctu, num_slices, ctu_log2) == (e >> 4)) return;
         const int x = VERT ? e : s, yy = VERT ? s : e;
         const int cq = cu_at(x, yy, wctu), cp = cu_at(VERT ? x - 1 : x, VERT ? yy : yy - 1, wctu);
-        const int bs = strength(intra, cbf, mv, cp, cq);
+        const int bs = inside_cu(culog2, cp, cq, e) ? 0 : strength(intra, cbf, mv, cp, cq);
         if (!bs) return;
         const int beta = kBetaT[clampi(qp, 0, 51)] << (BD - 8), tc = kTcT[clampi(qp + 2 * (bs - 1), 0, 53)] << (BD - 8);
         int v[4][8];
@@ -151,7 +163,7 @@ __global__ __launch_bounds__(256) void hevc_deblock_kernel(uint8_t *__restrict__
     const int cx = VERT ? e : s, cy = VERT ? s : e;
     const int lx = 2 * cx, ly = 2 * cy;
     const int cq = cu_at(lx, ly, wctu), cp = cu_at(VERT ? lx - 1 : lx, VERT ? ly : ly - 1, wctu);
-    if (strength(intra, cbf, mv, cp, cq) != 2) return;
+    if (inside_cu(culog2, cp, cq, 2 * e) || strength(intra, cbf, mv, cp, cq) != 2) return;
     const int tc = kTcT[clampi(hevc::chroma_qp(qp) + 2, 0, 53)] << (BD - 8);
     if (VERT) {
         // per line: interleaved Cb/Cr of chroma columns cx-2 .. cx+1 (8 samples)
@@ -198,6 +210,41 @@ __global__ __launch_bounds__(256) void hevc_cu_cbf_kernel(const int16_t *__restr
     if (lane == 0) cbf[cu] = any ? 1 : 0;
 }
 
+// The same with CU16 / CU32 (culog2 per CU8, the P layout of
+// bc250_hevc_launch_p_encode_ctu): the flag of a CU8 is that of the
+// transform block holding it, 64, 256 or 4 x 256 luma levels.
+__global__ __launch_bounds__(256) void hevc_cu_cbf_ctu_kernel(const int16_t *__restrict__ coef, int ncu, int wtu,
+                                                              const uint8_t *__restrict__ culog2,
+                                                              uint8_t *__restrict__ cbf)
+{
+    const int cu = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (cu >= ncu) return;
+    const int t = cu >> 2, L = culog2[cu];
+    bool nz = false;
+    if (L == 3) {
+        nz = coef[(size_t)t * 384 + (cu & 3) * 64 + lane] != 0;
+    } else {
+        const int tx = (t % wtu) & ~1, ty = (t / wtu) & ~1;
+        for (int a = 0; a < (L == 5 ? 4 : 1); a++) {
+            const int ta = L == 5 ? (ty + (a >> 1)) * wtu + tx + (a & 1) : t;
+#pragma unroll
+            for (int i = 0; i < 4; i++) nz = nz || coef[(size_t)ta * 384 + lane + 64 * i] != 0;
+        }
+    }
+    const bool any = __any(nz);
+    if (lane == 0) cbf[cu] = any ? 1 : 0;
+}
+
+extern "C" int bc250_hevc_cu_cbf_ctu(const int16_t *coef, int wc, int hc, const uint8_t *culog2, uint8_t *cbf,
+                                     void *stream)
+{
+    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15)) return -1;
+    const int wtu = wc / 16, nctu = wtu * (hc / 16);
+    if (!culog2) return bc250_hevc_cu_cbf(coef, nctu, cbf, stream);
+    hevc_cu_cbf_ctu_kernel<<<nctu, 256, 0, (hipStream_t)stream>>>(coef, nctu * 4, wtu, culog2, cbf);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 extern "C" int bc250_hevc_cu_cbf(const int16_t *coef, int nctu, uint8_t *cbf, void *stream)
 {
     if (nctu < 0) return -1;
@@ -208,28 +255,30 @@ extern "C" int bc250_hevc_cu_cbf(const int16_t *coef, int nctu, uint8_t *cbf, vo
 
 template <typename Pix, int BD>
 static void launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
-                           const uint8_t *cbf, const uint4 *mvs, int num_slices, int ctu_log2, hipStream_t st)
+                           const uint8_t *cbf, const uint4 *mvs, const uint8_t *culog2, int num_slices, int ctu_log2,
+                           hipStream_t st)
 {
     // vertical pass
     int nl = (wc / 8 - 1) * (hc / 4), nc = (wc / 16 - 1) * (hc / 8);
     int n = nl + nc;
     if (n > 0)
         hevc_deblock_kernel<Pix, BD, true>
-            <<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, num_slices, ctu_log2,
-                                              nl, nc);
+            <<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, culog2, num_slices,
+                                              ctu_log2, nl, nc);
     // horizontal pass
     nl = (hc / 8 - 1) * (wc / 4);
     nc = (hc / 16 - 1) * (wc / 8);
     n = nl + nc;
     if (n > 0)
         hevc_deblock_kernel<Pix, BD, false>
-            <<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, num_slices, ctu_log2,
-                                              nl, nc);
+            <<<(n + 255) / 256, 256, 0, st>>>(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, culog2, num_slices,
+                                              ctu_log2, nl, nc);
 }
 
 extern "C" int bc250_hevc_launch_deblock_ctu(void *y, void *uv, int py, int puv, int wc, int hc, int qp,
-                                             int is_intra, const uint8_t *cbf, const uint32_t *mvs, int num_slices,
-                                             int ctu_log2, int bd, void *stream)
+                                             int is_intra, const uint8_t *cbf, const uint32_t *mvs,
+                                             const uint8_t *culog2, int num_slices, int ctu_log2, int bd,
+                                             void *stream)
 {
     const int bps = bd == 8 ? 1 : 2;
     if ((bd != 8 && bd != 10) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || (py & 7) ||
@@ -238,10 +287,10 @@ extern "C" int bc250_hevc_launch_deblock_ctu(void *y, void *uv, int py, int puv,
     hipStream_t st = (hipStream_t)stream;
     if (bd == 8)
         launch_deblock<uint8_t, 8>((uint8_t *)y, (uint8_t *)uv, py, puv, wc, hc, qp, is_intra, cbf,
-                                   (const uint4 *)mvs, num_slices, ctu_log2, st);
+                                   (const uint4 *)mvs, culog2, num_slices, ctu_log2, st);
     else
         launch_deblock<uint16_t, 10>((uint8_t *)y, (uint8_t *)uv, py, puv, wc, hc, qp, is_intra, cbf,
-                                     (const uint4 *)mvs, num_slices, ctu_log2, st);
+                                     (const uint4 *)mvs, culog2, num_slices, ctu_log2, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -249,5 +298,6 @@ extern "C" int bc250_hevc_launch_deblock(void *y, void *uv, int py, int puv, int
                                          const uint8_t *cbf, const uint32_t *mvs, int num_slices, int bd,
                                          void *stream)
 {
-    return bc250_hevc_launch_deblock_ctu(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, num_slices, 4, bd, stream);
+    return bc250_hevc_launch_deblock_ctu(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mvs, nullptr, num_slices, 4, bd,
+                                         stream);
 }

@@ -410,6 +410,172 @@ __global__ __launch_bounds__(256) void hevc_p_encode_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// P picture at CTU32: CU16 / CU32 choice and coding, one 256-thread workgroup
+// per CTU32, after hevc_p_encode_kernel (whose CU8 levels and reconstruction
+// it replaces where it chooses a larger CU).  A 16x16 area whose four CU8
+// vectors are equal is one CU16; a CTU32 inside the picture whose four areas
+// are CU16s with one vector is one CU32.  A CU16 / CU32 codes one luma TU of
+// its size and one chroma TU of half its size per component (N-point DCT-II,
+// shifts (log2n + bd - 9, log2n + 6), quantiser shift bd + log2n - 5,
+// inverse (7, 20 - bd)).  Levels: area q of a CU32 (z-order) holds luma
+// levels [256 q, 256 q + 256) of the TU (row-major) and chroma levels
+// [64 q, 64 q + 64) per component, in the area's luma / Cb / Cr slots.
+// culog2[4 t + z] = log2 of the CU holding CU8 z of area t; modes (may be
+// null) gets the same at [16 t + z].
+struct LargeLds {
+    int w[39 * 39];     // reference window: luma (n + 7)^2, chroma (n/2 + 3)^2
+    int x[1024];        // residual, then dequantised levels
+    int t[1024];        // first-pass results
+    int m[1024];        // N-point matrix
+    int mv[16][2];
+    int cu16[4], all;
+};
+
+template <typename Pix, int BD>
+__global__ __launch_bounds__(256) void hevc_p_large_kernel(
+    const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, const uint8_t *__restrict__ ry,
+    const uint8_t *__restrict__ ruv, int rpy, int rpuv, uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy,
+    int opuv, int wc, int hc, int qp, int round_q12, const uint4 *__restrict__ mvs, int16_t *__restrict__ coef,
+    uint8_t *__restrict__ culog2, uint8_t *__restrict__ modes)
+{
+    constexpr int MAXV = (1 << BD) - 1, QOFF = 6 * (BD - 8);
+    __shared__ LargeLds L;
+    const int tid = threadIdx.x, wtu = wc >> 4, htu = hc >> 4, w32 = (wtu + 1) >> 1;
+    const int X = blockIdx.x % w32, Y = blockIdx.x / w32;
+    if (tid < 16) {
+        const int a = tid >> 2, z = tid & 3, tx = 2 * X + (a & 1), ty = 2 * Y + (a >> 1);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (tx < wtu && ty < htu) v = mvs[(ty * wtu + tx) * 4 + z];
+        L.mv[tid][0] = (int)v.x;
+        L.mv[tid][1] = (int)v.y;
+    }
+    __syncthreads();
+    if (tid < 4) {
+        const int tx = 2 * X + (tid & 1), ty = 2 * Y + (tid >> 1);
+        bool eq = tx < wtu && ty < htu;
+        for (int z = 1; z < 4; z++)
+            eq = eq && L.mv[tid * 4 + z][0] == L.mv[tid * 4][0] && L.mv[tid * 4 + z][1] == L.mv[tid * 4][1];
+        L.cu16[tid] = eq;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        bool all = L.cu16[0] && L.cu16[1] && L.cu16[2] && L.cu16[3];
+        for (int a = 1; a < 4; a++) all = all && L.mv[4 * a][0] == L.mv[0][0] && L.mv[4 * a][1] == L.mv[0][1];
+        L.all = all;
+    }
+    __syncthreads();
+    const bool all = L.all;
+    if (tid < 16) {
+        const int a = tid >> 2, z = tid & 3, tx = 2 * X + (a & 1), ty = 2 * Y + (a >> 1);
+        if (tx < wtu && ty < htu) {
+            const int t = ty * wtu + tx, lg = all ? 5 : (L.cu16[a] ? 4 : 3);
+            culog2[t * 4 + z] = (uint8_t)lg;
+            if (modes) modes[t * 16 + z] = (uint8_t)lg;
+        }
+    }
+    const int q = clampi(qp + QOFF, 0, 63), cq = hevc::chroma_qp(qp) + QOFF;
+    for (int a = 0; a < (all ? 1 : 4); a++) {
+        if (!all && !L.cu16[a]) continue;
+        const int log2n = all ? 5 : 4, n = 1 << log2n;
+        const int x0 = 32 * X + (all ? 0 : (a & 1) * 16), y0 = 32 * Y + (all ? 0 : (a >> 1) * 16);
+        const int mvx = L.mv[4 * a][0], mvy = L.mv[4 * a][1];
+        for (int comp = -1; comp < 2; comp++) {
+            const bool chroma = comp >= 0;
+            const int lg = chroma ? log2n - 1 : log2n, m = 1 << lg, mm = m * m;
+            const int P = chroma ? m + 3 : m + 7, back = chroma ? 1 : 3;
+            const int bx = chroma ? x0 >> 1 : x0, by = chroma ? y0 >> 1 : y0;
+            const int pw = chroma ? wc >> 1 : wc, ph = chroma ? hc >> 1 : hc;
+            const int wx0 = bx + (chroma ? mvx >> 3 : mvx >> 2) - back, wy0 = by + (chroma ? mvy >> 3 : mvy >> 2) - back;
+            const int fx = chroma ? mvx & 7 : mvx & 3, fy = chroma ? mvy & 7 : mvy & 3;
+            const uint8_t *ref = chroma ? ruv : ry;
+            const int rp = chroma ? rpuv : rpy;
+            for (int e = tid; e < P * P; e += 256) {
+                const int r = e / P, c = e - r * P;
+                const int sx = clampi(wx0 + c, 0, pw - 1);
+                L.w[e] = ldpix<Pix>(ref + (size_t)clampi(wy0 + r, 0, ph - 1) * rp, chroma ? 2 * sx + comp : sx);
+            }
+            for (int e = tid; e < mm; e += 256) {
+                const int k = e >> lg;
+                L.m[e] = dct_coef(k, e - (k << lg), m);
+            }
+            __syncthreads();
+            int pred[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int e = tid + 256 * i;
+                if (e < mm) {
+                    const int r = e >> lg, c = e & (m - 1);
+                    pred[i] = chroma ? mc_sample<true>(L.w, P, c + back, r + back, fx, fy, BD)
+                                     : mc_sample<false>(L.w, P, c + back, r + back, fx, fy, BD);
+                    const int cur = chroma ? ldpix<Pix>(cuv + (size_t)(by + r) * cpuv, 2 * (bx + c) + comp)
+                                           : ldpix<Pix>(cy + (size_t)(by + r) * cpy, bx + c);
+                    L.x[e] = cur - pred[i];
+                }
+            }
+            __syncthreads();
+            const int qq = chroma ? cq : q, bds = BD + lg - 5;
+            const int s1 = lg + BD - 9, s2 = lg + 6, s2i = 20 - BD;
+            for (int e = tid; e < mm; e += 256) {  // columns: T[i][c] = sum_r M[i][r] X[r][c]
+                const int i = e >> lg, c = e & (m - 1);
+                int acc = 0;
+                for (int r = 0; r < m; r++) acc += L.m[(i << lg) + r] * L.x[(r << lg) + c];
+                L.t[e] = sat16((acc + (1 << (s1 - 1))) >> s1);
+            }
+            __syncthreads();
+            const Quant Q = make_quant(qq, bds, round_q12);
+            const int per = mm >> (log2n == 5 ? 2 : 0), base = chroma ? 256 + comp * 64 : 0;
+            int lv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {  // rows: Y[i][j] = sum_c M[j][c] T[i][c]
+                const int e = tid + 256 * i;
+                if (e < mm) {
+                    const int ii = e >> lg, j = e & (m - 1);
+                    int acc = 0;
+                    for (int c = 0; c < m; c++) acc += L.m[(j << lg) + c] * L.t[(ii << lg) + c];
+                    lv[i] = quant(sat16((acc + (1 << (s2 - 1))) >> s2), Q);
+                    const int ar = e / per, tx = (x0 >> 4) + (ar & 1), ty = (y0 >> 4) + (ar >> 1);
+                    coef[(size_t)(ty * wtu + tx) * 384 + base + (e - ar * per)] = (int16_t)lv[i];
+                }
+            }
+            __syncthreads();  // T and X free
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int e = tid + 256 * i;
+                if (e < mm) L.x[e] = dequant(lv[i], qq, bds);
+            }
+            __syncthreads();
+            for (int e = tid; e < mm; e += 256) {  // inverse columns: T[i][c] = sum_r M[r][i] X[r][c]
+                const int i = e >> lg, c = e & (m - 1);
+                int acc = 0;
+                for (int r = 0; r < m; r++) acc += L.m[(r << lg) + i] * L.x[(r << lg) + c];
+                L.t[e] = sat16((acc + 64) >> 7);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 4; i++) {  // inverse rows, reconstruction
+                const int e = tid + 256 * i;
+                if (e < mm) {
+                    const int ii = e >> lg, j = e & (m - 1);
+                    int acc = 0;
+                    for (int c = 0; c < m; c++) acc += L.m[(c << lg) + j] * L.t[(ii << lg) + c];
+                    const int rv = clampi(pred[i] + sat16((acc + (1 << (s2i - 1))) >> s2i), 0, MAXV);
+                    if (chroma) stpix<Pix>(ouv + (size_t)(by + ii) * opuv, 2 * (bx + j) + comp, rv);
+                    else stpix<Pix>(oy + (size_t)(by + ii) * opy, bx + j, rv);
+                }
+            }
+            __syncthreads();  // window, matrix and buffers reused by the next block
+        }
+    }
+}
+
+// Fills n bytes with v.
+__global__ void fill_u8_kernel(uint8_t *p, int n, uint8_t v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
 static int launch_ok() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 static bool bd_ok(int bd) { return bd == 8 || bd == 10; }
 
@@ -556,5 +722,35 @@ extern "C" int bc250_hevc_launch_p_encode(const void *cy, const void *cuv, int c
         hevc_p_encode_kernel<uint16_t, 10><<<grid_xcd(nctu), 256, 0, st>>>(C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU,
                                                                            opy, opuv, wc, hc, nctu, qp, round_q12,
                                                                            (const uint4 *)mvs, coef);
+    return launch_ok();
+}
+
+extern "C" int bc250_hevc_launch_p_encode_ctu(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
+                                              const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy,
+                                              int opuv, int wc, int hc, int qp, int round_q12, const uint32_t *mvs,
+                                              int16_t *coef, int ctu_log2, uint8_t *culog2, uint8_t *modes, int bd,
+                                              void *stream)
+{
+    if ((ctu_log2 != 4 && ctu_log2 != 5) || !culog2) return -1;
+    const int r = bc250_hevc_launch_p_encode(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp,
+                                             round_q12, mvs, coef, bd, stream);
+    if (r != 0) return r;
+    hipStream_t st = (hipStream_t)stream;
+    const int wtu = wc / 16, htu = hc / 16, ncu = wtu * htu * 4;
+    if (ctu_log2 == 4) {
+        fill_u8_kernel<<<(ncu + 255) / 256, 256, 0, st>>>(culog2, ncu, 3);
+        return launch_ok();
+    }
+    const int n32 = ((wtu + 1) / 2) * ((htu + 1) / 2);
+    auto *C = (const uint8_t *)cy, *CU = (const uint8_t *)cuv, *R = (const uint8_t *)ry, *RU = (const uint8_t *)ruv;
+    auto *O = (uint8_t *)oy, *OU = (uint8_t *)ouv;
+    if (bd == 8)
+        hevc_p_large_kernel<uint8_t, 8><<<n32, 256, 0, st>>>(C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv, wc,
+                                                             hc, qp, round_q12, (const uint4 *)mvs, coef, culog2,
+                                                             modes);
+    else
+        hevc_p_large_kernel<uint16_t, 10><<<n32, 256, 0, st>>>(C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv,
+                                                               wc, hc, qp, round_q12, (const uint4 *)mvs, coef,
+                                                               culog2, modes);
     return launch_ok();
 }

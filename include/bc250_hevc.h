@@ -107,6 +107,23 @@ int bc250_hevc_launch_p_encode(const void *cy, const void *cuv, int cpy, int cpu
                                int rpy, int rpuv, void *oy, void *ouv, int opy, int opuv, int wc, int hc, int qp,
                                int round_q12, const uint32_t *mvs, int16_t *coef, int bd, void *stream);
 
+/* The same with the coding tree unit size 1 << ctu_log2 (no reference
+ * counterpart beyond 4, which is the call above).  At CTU32 (5) each CTU32
+ * chooses its CUs from the vectors: a 16x16 area whose four CU8 vectors are
+ * equal is one CU16, a CTU32 inside the picture whose four areas are CU16s
+ * with one vector is one CU32; a CU16 / CU32 codes one luma TU of its size
+ * and one chroma TU of half its size per component (the N-point transforms
+ * of bc250_hevc_tqn_batch / dqitn).  Levels stay in the per-16x16 layout:
+ * a CU16's area holds its 16x16 luma TU (256, row-major) and 8x8 Cb / Cr
+ * TUs (64 each); area q (z-order) of a CU32 holds levels [256 q, 256 q +
+ * 256) of its 32x32 luma TU and [64 q, 64 q + 64) of each 16x16 chroma TU.
+ * culog2 (nctu * 4 bytes, required) = log2 of the CU holding each CU8 (3,
+ * 4 or 5); modes (may be NULL) gets the same at [16 t + z]. */
+int bc250_hevc_launch_p_encode_ctu(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
+                                   const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy, int opuv, int wc,
+                                   int hc, int qp, int round_q12, const uint32_t *mvs, int16_t *coef, int ctu_log2,
+                                   uint8_t *culog2, uint8_t *modes, int bd, void *stream);
+
 /* I picture: CTU16 -> 4 CU8 -> 4 PU4 intra (Planar/DC/H/V by SAD, DST),
  * chroma DC per CU (DCT, QpC), slices as CTU-row ranges
  * [floor(rows*s/ns), floor(rows*(s+1)/ns)), QP' = qp + 6 (bd - 8) (the
@@ -139,6 +156,11 @@ int bc250_hevc_launch_i_encode_ctu(const void *cy, const void *cuv, int cpy, int
  * layout): cbf[4*t + z] = 1 when any of CU z's 64 luma levels is non-zero,
  * the bS 1 condition of the deblocking filter. */
 int bc250_hevc_cu_cbf(const int16_t *coef, int nctu, uint8_t *cbf, void *stream);
+/* The same for a P picture of bc250_hevc_launch_p_encode_ctu (coded size
+ * wc x hc): cbf[4 t + z] = whether the transform block holding CU8 z of area
+ * t (8x8, 16x16 or 32x32 luma, culog2[4 t + z] = 3, 4 or 5) has a non-zero
+ * level; culog2 NULL is bc250_hevc_cu_cbf. */
+int bc250_hevc_cu_cbf_ctu(const int16_t *coef, int wc, int hc, const uint8_t *culog2, uint8_t *cbf, void *stream);
 
 /* H.265 8.7.2 deblocking of a reconstructed picture in place (wc, hc
  * multiples of 16; pitches in bytes, multiples of 8): bd 8 on NV12 planes,
@@ -156,10 +178,13 @@ int bc250_hevc_cu_cbf(const int16_t *coef, int nctu, uint8_t *cbf, void *stream)
 int bc250_hevc_launch_deblock(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
                               const uint8_t *cbf, const uint32_t *mvs, int num_slices, int bd, void *stream);
 /* The same with slices as ranges of CTU rows of 1 << ctu_log2 (4 or 5)
- * luma rows (the CTU32 pictures of bc250_hevc_launch_i_encode_ctu). */
+ * luma rows (the CTU32 pictures of bc250_hevc_launch_i_encode_ctu), and,
+ * when culog2 (per CU8, see bc250_hevc_launch_p_encode_ctu) is not NULL, no
+ * filtering of the 8x8-grid edges inside a CU16 / CU32 (they are neither
+ * TU nor PU edges; cbf then per transform block: bc250_hevc_cu_cbf_ctu). */
 int bc250_hevc_launch_deblock_ctu(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
-                                  const uint8_t *cbf, const uint32_t *mvs, int num_slices, int ctu_log2, int bd,
-                                  void *stream);
+                                  const uint8_t *cbf, const uint32_t *mvs, const uint8_t *culog2, int num_slices,
+                                  int ctu_log2, int bd, void *stream);
 
 #ifdef __cplusplus
 }

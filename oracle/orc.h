@@ -140,7 +140,11 @@ void orc_hevc_encode_i_ctu(const void *sy, const void *scb, const void *scr, voi
                            int bd, int wc, int hc, int qp, int num_slices, int round_q12, int ctu_log2,
                            uint8_t *modes, int16_t *coef);
 void orc_hevc_deblock_ctu(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
-                          const uint8_t *cbf, const int32_t *mv4, int num_slices, int ctu_log2, int bd);
+                          const uint8_t *cbf, const int32_t *mv4, int num_slices, int ctu_log2, int bd,
+                          const uint8_t *culog2);
+void orc_hevc_encode_p_ctu(const void *sy, const void *scb, const void *scr, const void *fy, const void *fcb,
+                           const void *fcr, void *ry, void *rcb, void *rcr, int bd, int wc, int hc, int qp,
+                           int round_q12, const int32_t *mv, int ctu_log2, int16_t *coef, uint8_t *culog2);
 void orc_hevc_encode_i(const void *sy, const void *scb, const void *scr, void *ry, void *rcb, void *rcr,
                        int bd, int wc, int hc, int qp, int num_slices, int round_q12, uint8_t *modes,
                        int16_t *coef);

@@ -215,6 +215,7 @@ _sig("orc_hevc_encode_i_ctu", None, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I,
 _sig("orc_hevc_slice_y_min_ctu", _I, [_I, _I, _I, _I])
 _sig("orc_hevc_zrank", C.c_longlong, [_I, _I, _I, _I, _I])
 _sig("orc_hevc_encode_p", None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P])
+_sig("orc_hevc_encode_p_ctu", None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _I, _P, _P])
 _sig("orc_hevc_refine8", C.c_uint32, [_P, _P, _I, _I, _I, _I, _I, C.POINTER(_I), C.POINTER(_I)])
 
 HEVC_ROUND_INTER, HEVC_ROUND_INTRA_P, HEVC_ROUND_INTRA_I, HEVC_ROUND_NEAREST = 2, 3, 4, 6
@@ -299,17 +300,20 @@ def hevc_zrank(x, y, width, is_luma, ctu_log2):
     return lib.orc_hevc_zrank(x, y, width, int(is_luma), ctu_log2)
 
 
-def hevc_encode_p(sy, scb, scr, fy, fcb, fcr, qp, mv, round_q12=HEVC_ROUND_INTER, bd=8):
-    """mv: int32 [nctu*4][2] quarter-sample vectors, CUs in z-order per CTU."""
+def hevc_encode_p(sy, scb, scr, fy, fcb, fcr, qp, mv, round_q12=HEVC_ROUND_INTER, bd=8, ctu_log2=4):
+    """mv: int32 [nctu*4][2] quarter-sample vectors, CUs in z-order per CTU.
+    ctu_log2 5: CU16 / CU32 where the vectors agree (orc_hevc_encode_p_ctu);
+    culog2 (uint8 [nctu*4]) = log2 of the CU holding each CU8."""
     hc, wc = sy.shape
     t = (wc // 16) * (hc // 16)
     a = [np.ascontiguousarray(x, _pix(bd)) for x in (sy, scb, scr, fy, fcb, fcr)]
     ry, rcb, rcr = np.zeros_like(a[0]), np.zeros_like(a[1]), np.zeros_like(a[2])
     coef = np.zeros(t * 384, np.int16)
+    culog2 = np.zeros(t * 4, np.uint8)
     mv = np.ascontiguousarray(mv, np.int32).reshape(t * 4, 2)
-    lib.orc_hevc_encode_p(p(a[0]), p(a[1]), p(a[2]), p(a[3]), p(a[4]), p(a[5]), p(ry), p(rcb), p(rcr), bd, wc, hc,
-                          qp, round_q12, p(mv), p(coef))
-    return dict(coef=coef, y=ry, cb=rcb, cr=rcr)
+    lib.orc_hevc_encode_p_ctu(p(a[0]), p(a[1]), p(a[2]), p(a[3]), p(a[4]), p(a[5]), p(ry), p(rcb), p(rcr), bd, wc,
+                              hc, qp, round_q12, p(mv), ctu_log2, p(coef), p(culog2))
+    return dict(coef=coef, y=ry, cb=rcb, cr=rcr, culog2=culog2)
 
 
 def hevc_refine8(cur, ref, x0, y0, mvx, mvy, bd=8):
@@ -351,24 +355,42 @@ def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lamb
             mv[cu] = (mx, my)
             mv8[cu] = (mx, my, sad, 0)
         fb, fr = split_uv(ruv)
-        o = hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, HEVC_ROUND_INTER, bd)
+        o = hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, HEVC_ROUND_INTER, bd, ctu_log2)
         modes = np.zeros(t * 16, np.uint8)
+        if ctu_log2 == 5:
+            modes.reshape(t, 16)[:, :4] = o["culog2"].reshape(t, 4)
     uv = np.empty((hc // 2, wc), pix)
     uv[:, 0::2], uv[:, 1::2] = o["cb"], o["cr"]
     y = o["y"]
     if deblock:
-        cbf = hevc_cu_cbf(o["coef"]) if not is_intra else np.zeros(t * 4, np.uint8)
+        cus = o["culog2"] if not is_intra and ctu_log2 == 5 else None
+        cbf = hevc_cu_cbf(o["coef"], cus, wc) if not is_intra else np.zeros(t * 4, np.uint8)
         mv4 = np.zeros((t * 4, 4), np.int32)
         mv4[:, 0], mv4[:, 1] = mv8["mvx"], mv8["mvy"]
-        y, uv = hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices, bd, ctu_log2)
+        y, uv = hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices, bd, ctu_log2, cus)
     return dict(coef=o["coef"], modes=modes, mv8=mv8, y=y, uv=uv)
 
 
-def hevc_cu_cbf(coef):
-    """Per-CU coded-luma flag of a P picture's levels (CU z of CTU t holds
-    coef[384 t + 64 z : +64])."""
-    c = np.asarray(coef, np.int16).reshape(-1, 384)[:, :256].reshape(-1, 64)
-    return (c != 0).any(axis=1).astype(np.uint8)
+def hevc_cu_cbf(coef, culog2=None, wc=None):
+    """Per-CU8 coded-luma flag of a P picture's levels (CU z of CTU t holds
+    coef[384 t + 64 z : +64]); with culog2 (and the coded width wc) the flag
+    of the transform block holding the CU8 (16x16: the area's 256 levels,
+    32x32: the four areas' luma levels)."""
+    a = np.asarray(coef, np.int16).reshape(-1, 384)[:, :256]
+    cbf = (a.reshape(-1, 64) != 0).any(axis=1).astype(np.uint8)
+    if culog2 is None:
+        return cbf
+    wtu = wc // 16
+    area = (a != 0).any(axis=1)
+    for cu in range(cbf.size):
+        t, L = cu // 4, int(culog2[cu])
+        if L == 4:
+            cbf[cu] = area[t]
+        elif L == 5:
+            tx, ty = (t % wtu) & ~1, (t // wtu) & ~1
+            cbf[cu] = area[ty * wtu + tx] | area[ty * wtu + tx + 1] | area[(ty + 1) * wtu + tx] | \
+                area[(ty + 1) * wtu + tx + 1]
+    return cbf
 
 _sig("orc_hevc_dct_matrix", None, [_I, _P])
 _sig("orc_hevc_tqn", None, [_P, _I, _I, _I, _I, _P])
@@ -439,10 +461,10 @@ _sig("orc_hevc_deblock", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I])
 
 
 _sig("orc_hevc_deblock_bd", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I])
-_sig("orc_hevc_deblock_ctu", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I])
+_sig("orc_hevc_deblock_ctu", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P])
 
 
-def hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices=1, bd=8, ctu_log2=4):
+def hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices=1, bd=8, ctu_log2=4, culog2=None):
     """Returns deblocked copies of the NV12 planes (uint8 at bd 8, else
     uint16 holding the sample in the low bits); cbf uint8 [ncu], mv4 int32
     [ncu][4]."""
@@ -452,6 +474,7 @@ def hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices=1, bd=8, ctu_log2=4):
     hc, wc = y.shape
     cbf = np.ascontiguousarray(cbf, np.uint8)
     mv4 = np.ascontiguousarray(mv4, np.int32).reshape(-1, 4)
+    cu = None if culog2 is None else np.ascontiguousarray(culog2, np.uint8)
     lib.orc_hevc_deblock_ctu(p(y), p(uv), wc, wc, wc, hc, qp, int(is_intra), p(cbf), p(mv4), num_slices,
-                             ctu_log2, bd)
+                             ctu_log2, bd, None if cu is None else p(cu))
     return y, uv

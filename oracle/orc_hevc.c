@@ -535,6 +535,93 @@ void orc_hevc_encode_p(const void *sy, const void *scb, const void *scr, const v
         }
 }
 
+/* One inter CU of 16x16 or 32x32 luma (log2n 4 or 5) at (x0, y0), vector
+ * (mvx, mvy): one luma TU of its size and one chroma TU of half its size per
+ * component (N-point DCT-II, quant / dequant as orc_hevc_tqn / dqitn).  Its
+ * levels go to the 16x16 areas it covers (t0 = the first, wctu areas per
+ * row): area q (z-order inside a CU32) holds luma levels [256 q, 256 q + 256)
+ * of the TU (row-major) and chroma levels [64 q, 64 q + 64) of each chroma
+ * TU, in the per-area slots of orc_hevc_encode_p (luma 0..255, Cb 256..319,
+ * Cr 320..383). */
+static void code_large_cu(const void *sy, const void *scb, const void *scr, const void *fy, const void *fcb,
+                          const void *fcr, void *ry, void *rcb, void *rcr, int bd, int wc, int hc, int qp,
+                          int round_q12, int x0, int y0, int log2n, int mvx, int mvy, int16_t *coef)
+{
+    const int n = 1 << log2n, cn = n / 2, wctu = wc / 16, cwc = wc / 2, chc = hc / 2, maxv = (1 << bd) - 1;
+    const int qoff = bd == 8 ? 0 : 12, cqp = orc_hevc_chroma_qp(qp) + qoff;
+    static uint16_t pbuf[1024];
+    static int16_t res[1024], lv[1024], rr[1024];
+    for (int comp = -1; comp < 2; comp++) {
+        const int chroma = comp >= 0, m = chroma ? cn : n, lg = chroma ? log2n - 1 : log2n;
+        const int bx = chroma ? x0 / 2 : x0, by = chroma ? y0 / 2 : y0, pw = chroma ? cwc : wc;
+        const void *s = comp < 0 ? sy : (comp ? scr : scb), *f = comp < 0 ? fy : (comp ? fcr : fcb);
+        void *r = comp < 0 ? ry : (comp ? rcr : rcb);
+        orc_hevc_mc_uni(f, pw, pw, chroma ? chc : hc, bx, by, m, m, mvx, mvy, chroma, bd, pbuf, m);
+        for (int i = 0; i < m * m; i++) {
+            const int pv = bd == 8 ? ((uint8_t *)pbuf)[i] : pbuf[i];
+            res[i] = (int16_t)(get(s, bd, pw, bx + i % m, by + i / m) - pv);
+        }
+        const int q = chroma ? cqp : qp + qoff;
+        orc_hevc_tqn(res, lg, q, bd, round_q12, lv);
+        orc_hevc_dqitn(lv, lg, q, bd, rr);
+        for (int i = 0; i < m * m; i++) {
+            const int pv = bd == 8 ? ((uint8_t *)pbuf)[i] : pbuf[i];
+            put(r, bd, pw, bx + i % m, by + i / m, clampi(pv + rr[i], 0, maxv));
+        }
+        const int per = chroma ? 64 : 256, base = chroma ? 256 + comp * 64 : 0;
+        for (int a = 0; a < (log2n == 5 ? 4 : 1); a++) {
+            const int t = (y0 / 16 + (a >> 1)) * wctu + x0 / 16 + (a & 1);
+            memcpy(coef + (size_t)t * 384 + base, lv + a * per, (size_t)per * 2);
+        }
+    }
+}
+
+/* P picture at CTU16 (ctu_log2 4: orc_hevc_encode_p) or CTU32 (5).  At
+ * CTU32 each CTU32 chooses its CUs from the 8x8 vectors: a 16x16 area whose
+ * four CU8 vectors are equal is one CU16, a CTU32 inside the picture whose
+ * four areas are CU16s with one vector is one CU32; every other area stays
+ * four CU8s.  culog2[4 t + z] (may be NULL) = log2 of the CU holding CU8 z of
+ * area t (3, 4 or 5). */
+void orc_hevc_encode_p_ctu(const void *sy, const void *scb, const void *scr, const void *fy, const void *fcb,
+                           const void *fcr, void *ry, void *rcb, void *rcr, int bd, int wc, int hc, int qp,
+                           int round_q12, const int32_t *mv, int ctu_log2, int16_t *coef, uint8_t *culog2)
+{
+    const int wctu = wc / 16, hctu = hc / 16;
+    orc_hevc_encode_p(sy, scb, scr, fy, fcb, fcr, ry, rcb, rcr, bd, wc, hc, qp, round_q12, mv, coef);
+    if (culog2) memset(culog2, 3, (size_t)wctu * hctu * 4);
+    if (ctu_log2 != 5) return;
+    for (int Y = 0; 2 * Y < hctu; Y++)
+        for (int X = 0; 2 * X < wctu; X++) {
+            int cu16[4], all = 1;
+            for (int a = 0; a < 4; a++) {
+                const int tx = 2 * X + (a & 1), ty = 2 * Y + (a >> 1);
+                cu16[a] = 0;
+                if (tx < wctu && ty < hctu) {
+                    const int32_t *v = mv + (size_t)(ty * wctu + tx) * 8;
+                    cu16[a] = 1;
+                    for (int z = 1; z < 4; z++) cu16[a] &= v[2 * z] == v[0] && v[2 * z + 1] == v[1];
+                }
+                all &= cu16[a];
+            }
+            const int32_t *v0 = mv + (size_t)(2 * Y * wctu + 2 * X) * 8;
+            for (int a = 1; a < 4 && all; a++) {
+                const int32_t *v = mv + (size_t)((2 * Y + (a >> 1)) * wctu + 2 * X + (a & 1)) * 8;
+                all = v[0] == v0[0] && v[1] == v0[1];
+            }
+            for (int a = 0; a < 4; a++) {
+                const int tx = 2 * X + (a & 1), ty = 2 * Y + (a >> 1), t = ty * wctu + tx;
+                if (!cu16[a]) continue;
+                if (culog2) memset(culog2 + (size_t)t * 4, all ? 5 : 4, 4);
+                if (!all)
+                    code_large_cu(sy, scb, scr, fy, fcb, fcr, ry, rcb, rcr, bd, wc, hc, qp, round_q12, tx * 16, ty * 16,
+                                  4, mv[(size_t)t * 8], mv[(size_t)t * 8 + 1], coef);
+            }
+            if (all)
+                code_large_cu(sy, scb, scr, fy, fcb, fcr, ry, rcb, rcr, bd, wc, hc, qp, round_q12, X * 32, Y * 32, 5,
+                              v0[0], v0[1], coef);
+        }
+}
+
 /* Quarter-pel refinement of one 8x8 luma block with the HEVC interpolation:
  * centre = the integer vector, then 8 half-sample candidates in the order
  * (2,0) (-2,0) (0,2) (0,-2) (2,2) (2,-2) (-2,2) (-2,-2) around it, then the
@@ -674,8 +761,20 @@ int orc_hevc_bs(int is_intra, const uint8_t *cbf, const int32_t *mv4, int cp, in
  * uint8 at bd 8, else uint16 with the sample in the low bits; mv4: {mvx,
  * mvy, ., .} per CU; cbf: per CU.  beta and tC scale by 2^(bd-8) (8.7.2.5.3,
  * 8.7.2.5.5). */
+/* Whether the 8x8-grid edge at coordinate e between CU8 p and q lies inside
+ * one CU16 / CU32 (neither a TU nor a PU edge). */
+static int inside_cu(const uint8_t *culog2, int cp, int cq, int e)
+{
+    if (!culog2) return 0;
+    const int L = culog2[cq];
+    return L >= 4 && culog2[cp] == L && (e % (1 << L)) != 0;
+}
+
+/* culog2 (may be NULL): log2 CU size per CU8 (orc_hevc_encode_p_ctu); edges
+ * inside a CU16 / CU32 are not filtered. */
 void orc_hevc_deblock_ctu(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
-                          const uint8_t *cbf, const int32_t *mv4, int num_slices, int ctu_log2, int bd)
+                          const uint8_t *cbf, const int32_t *mv4, int num_slices, int ctu_log2, int bd,
+                          const uint8_t *culog2)
 {
     const int wctu = wc / 16, hctu = hc / 16;
     const int beta = kBeta[clampi(qp, 0, 51)] << (bd - 8);
@@ -686,7 +785,7 @@ void orc_hevc_deblock_ctu(void *y, void *uv, int py, int puv, int wc, int hc, in
             for (int s = 0; s < (dir ? wc : hc); s += 4) {   /* 4-sample segments along the edge */
                 const int x = dir ? s : e, yy = dir ? e : s;
                 const int cq = cu_at(x, yy, wctu), cp = cu_at(dir ? x : x - 1, dir ? yy - 1 : yy, wctu);
-                const int bs = orc_hevc_bs(is_intra, cbf, mv4, cp, cq);
+                const int bs = inside_cu(culog2, cp, cq, e) ? 0 : orc_hevc_bs(is_intra, cbf, mv4, cp, cq);
                 if (!bs) continue;
                 const int tc = kTc[clampi(qp + 2 * (bs - 1), 0, 53)] << (bd - 8);
                 luma_segment(y, bd, (ptrdiff_t)yy * py + x, dir ? 1 : py, dir ? py : 1, beta, tc);
@@ -698,7 +797,7 @@ void orc_hevc_deblock_ctu(void *y, void *uv, int py, int puv, int wc, int hc, in
                     const int cx = dir ? s : e / 2, cy = dir ? e / 2 : s;
                     const int lx = 2 * cx, ly = 2 * cy;
                     const int cq = cu_at(lx, ly, wctu), cp = cu_at(dir ? lx : lx - 1, dir ? ly - 1 : ly, wctu);
-                    if (orc_hevc_bs(is_intra, cbf, mv4, cp, cq) != 2) continue;
+                    if (inside_cu(culog2, cp, cq, e) || orc_hevc_bs(is_intra, cbf, mv4, cp, cq) != 2) continue;
                     for (int comp = 0; comp < 2; comp++)
                         chroma_segment(uv, bd, (ptrdiff_t)cy * puv + 2 * cx + comp, dir ? 2 : puv, dir ? puv : 2, ctc);
                 }
@@ -710,7 +809,7 @@ void orc_hevc_deblock_ctu(void *y, void *uv, int py, int puv, int wc, int hc, in
 void orc_hevc_deblock_bd(void *y, void *uv, int py, int puv, int wc, int hc, int qp, int is_intra,
                          const uint8_t *cbf, const int32_t *mv4, int num_slices, int bd)
 {
-    orc_hevc_deblock_ctu(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mv4, num_slices, 4, bd);
+    orc_hevc_deblock_ctu(y, uv, py, puv, wc, hc, qp, is_intra, cbf, mv4, num_slices, 4, bd, NULL);
 }
 
 void orc_hevc_deblock(uint8_t *y, uint8_t *uv, int py, int puv, int wc, int hc, int qp, int is_intra,

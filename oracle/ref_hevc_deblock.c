@@ -22,11 +22,16 @@
 
 #include "hevc_dec_internal.h"
 
-int ref_hevc_deblock(void *y, void *uv, int wc, int hc, int qp, int is_intra, const uint8_t *cbf,
-                     const int32_t *mv4, int num_slices, int bd)
+/* ctu_log2 4 or 5 (CTBs of 16 or 32, slices as CTB-row ranges); culog2 (may
+ * be NULL): log2 CU size per CU8 (CU16 / CU32 of a CTU32 P picture): the
+ * 8x8 cells inside such a CU carry no edge bits on their inner sides. */
+int ref_hevc_deblock_ctu(void *y, void *uv, int wc, int hc, int qp, int is_intra, const uint8_t *cbf,
+                         const int32_t *mv4, int num_slices, int bd, int ctu_log2, const uint8_t *culog2)
 {
     const size_t bps = bd > 8 ? 2 : 1;  /* bytes per sample; planes hold the sample in the low bits */
-    const int wctu = wc / 16, hctu = hc / 16, w4 = wc / 4, h4 = hc / 4, w8 = wc / 8, h8 = hc / 8;
+    const int csz = 1 << ctu_log2;
+    const int wctu = (wc + csz - 1) / csz, hctu = (hc + csz - 1) / csz, w4 = wc / 4, h4 = hc / 4, w8 = wc / 8,
+              h8 = hc / 8, w16 = wc / 16;
     hevc_sps_t *sps = calloc(1, sizeof *sps);
     hevc_pps_t *pps = calloc(1, sizeof *pps);
     hevcd_t *d = calloc(1, sizeof *d);
@@ -43,8 +48,8 @@ int ref_hevc_deblock(void *y, void *uv, int wc, int hc, int qp, int is_intra, co
     sps->height = hc;
     sps->bit_depth_luma = sps->bit_depth_chroma = bd;
     sps->log2_min_cb = 3;
-    sps->log2_ctb = 4;
-    sps->ctb_size = 16;
+    sps->log2_ctb = ctu_log2;
+    sps->ctb_size = csz;
     sps->ctb_width = wctu;
     sps->ctb_height = hctu;
     sps->ctb_count = wctu * hctu;
@@ -63,11 +68,19 @@ int ref_hevc_deblock(void *y, void *uv, int wc, int hc, int qp, int is_intra, co
         sf[k].across_slices = 0;
     }
     memset(edges, 1 | 2 | 4 | 8, (size_t)w8 * h8);
+    if (culog2)
+        for (int cy = 0; cy < h8; cy++)
+            for (int cx = 0; cx < w8; cx++) {
+                const int cu = ((cy >> 1) * w16 + (cx >> 1)) * 4 + (cy & 1) * 2 + (cx & 1), L = culog2[cu];
+                if (L < 4) continue;
+                if ((cx * 8) % (1 << L)) edges[cy * w8 + cx] &= (uint8_t)~(1 | 4);
+                if ((cy * 8) % (1 << L)) edges[cy * w8 + cx] &= (uint8_t)~(2 | 8);
+            }
     memset(qpm, qp, (size_t)w8 * h8);
     for (int j = 0; j < h4; j++)
         for (int i = 0; i < w4; i++) {
             const int x = 4 * i, yy = 4 * j;
-            const int cu = ((yy >> 4) * wctu + (x >> 4)) * 4 + ((yy >> 3) & 1) * 2 + ((x >> 3) & 1);
+            const int cu = ((yy >> 4) * w16 + (x >> 4)) * 4 + ((yy >> 3) & 1) * 2 + ((x >> 3) & 1);
             hevcd_mvf_t *m = &mvf[j * w4 + i];
             if (!is_intra) {
                 m->pred_flag = HEVCD_PF_L0;
@@ -122,4 +135,10 @@ out:
     free(sps); free(pps); free(d); free(mvf); free(cbf4); free(edges); free(qpm); free(slice_of); free(sf);
     free(cb); free(cr);
     return rc;
+}
+
+int ref_hevc_deblock(void *y, void *uv, int wc, int hc, int qp, int is_intra, const uint8_t *cbf,
+                     const int32_t *mv4, int num_slices, int bd)
+{
+    return ref_hevc_deblock_ctu(y, uv, wc, hc, qp, is_intra, cbf, mv4, num_slices, bd, 4, NULL);
 }

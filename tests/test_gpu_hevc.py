@@ -204,6 +204,76 @@ def test_p_encode_stage(pkg, qp, bd):
     assert (want["coef"] != 0).any()
 
 
+@pytest.mark.parametrize("w,h,qp,bd", [(112, 80, 22, 8), (256, 160, 37, 8), (112, 80, 27, 10), (192, 128, 4, 10)])
+def test_p_encode_ctu32_stage(pkg, w, h, qp, bd):
+    """CTU32 P coding: CU16 / CU32 where the CU8 vectors agree (whole CTU32s,
+    single areas, areas of partial CTU32s at the edges), CU8 elsewhere; the
+    per-area level layout, the CU-size map, the modes copy and the
+    per-transform-block cbf."""
+    torch = _torch()
+    say(f"hevc P ctu32 {w}x{h} qp={qp} bd={bd}")
+    (cy, cuv), (ry, ruv) = _frames(pkg, 191 + qp, w, h, 2, 4)
+    hc, wc = cy.shape
+    wtu, htu = wc // 16, hc // 16
+    t = wtu * htu
+    rs = np.random.default_rng(qp + w)
+    mv = rs.integers(-70, 71, (t * 4, 2)).astype(np.int32)
+    for Y in range((htu + 1) // 2):
+        for X in range((wtu + 1) // 2):
+            areas = [(2 * Y + (a >> 1)) * wtu + 2 * X + (a & 1) for a in range(4)
+                     if 2 * X + (a & 1) < wtu and 2 * Y + (a >> 1) < htu]
+            kind = int(rs.integers(0, 3))
+            v = rs.integers(-70, 71, 2)
+            for a in areas:
+                if kind == 2 or (kind == 1 and rs.random() < 0.5):
+                    mv[a * 4:a * 4 + 4] = v if kind == 2 else rs.integers(-70, 71, 2)
+    if bd == 10:
+        cy, cuv, ry, ruv = (_ten(a, rs) for a in (cy, cuv, ry, ruv))
+    cb, cr = orc.split_uv(cuv)
+    fb, fr = orc.split_uv(ruv)
+    want = orc.hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, orc.HEVC_ROUND_INTER, bd, 5)
+    assert (want["culog2"] == 5).any() and (want["culog2"] == 4).any() and (want["culog2"] == 3).any()
+    mv4 = np.zeros((t * 4, 4), np.int32)
+    mv4[:, :2] = mv
+    if bd == 10:
+        d = [_p010(torch, a) for a in (cy, cuv, ry, ruv)] + [dev(torch, mv4)]
+        pitch, back = wc * 2, _unp010
+    else:
+        d = [dev(torch, a) for a in (cy, cuv, ry, ruv, mv4)]
+        pitch, back = wc, host
+    oy, ouv = torch.zeros_like(d[0]), torch.zeros_like(d[1])
+    coef = torch.zeros(t * 384, dtype=torch.int16, device="cuda")
+    culog2 = torch.zeros(t * 4, dtype=torch.uint8, device="cuda")
+    modes = torch.zeros(t * 16, dtype=torch.uint8, device="cuda")
+    assert pkg.hip.bc250_hevc_launch_p_encode_ctu(P(d[0]), P(d[1]), pitch, pitch, P(d[2]), P(d[3]), pitch, pitch,
+                                                  P(oy), P(ouv), pitch, pitch, wc, hc, qp, orc.HEVC_ROUND_INTER,
+                                                  P(d[4]), P(coef), 5, P(culog2), P(modes), bd, None) == 0
+    cbf = torch.zeros(t * 4, dtype=torch.uint8, device="cuda")
+    assert pkg.hip.bc250_hevc_cu_cbf_ctu(P(coef), wc, hc, P(culog2), P(cbf), None) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(host(culog2), want["culog2"])
+    assert np.array_equal(host(modes).reshape(t, 16)[:, :4].reshape(-1), want["culog2"])
+    assert not host(modes).reshape(t, 16)[:, 4:].any()
+    assert np.array_equal(host(coef), want["coef"])
+    assert np.array_equal(back(oy), want["y"])
+    gb, gr = orc.split_uv(back(ouv))
+    assert np.array_equal(gb, want["cb"]) and np.array_equal(gr, want["cr"])
+    assert np.array_equal(host(cbf), orc.hevc_cu_cbf(want["coef"], want["culog2"], wc))
+    # deblocking with the CU-size map: no filtering inside a CU16 / CU32
+    uv = np.empty((hc // 2, wc), want["y"].dtype)
+    uv[:, 0::2], uv[:, 1::2] = want["cb"], want["cr"]
+    wcbf = orc.hevc_cu_cbf(want["coef"], want["culog2"], wc)
+    wy, wuv = orc.hevc_deblock(want["y"], uv, qp, 0, wcbf, mv4, 2, bd, 5, want["culog2"])
+    if bd == 10:
+        dy, duv = _p010(torch, want["y"]), _p010(torch, uv)
+    else:
+        dy, duv = dev(torch, want["y"]), dev(torch, uv)
+    assert pkg.hip.bc250_hevc_launch_deblock_ctu(P(dy), P(duv), pitch, pitch, wc, hc, qp, 0, P(cbf), P(d[4]),
+                                                 P(culog2), 2, 5, bd, None) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(back(dy), wy) and np.array_equal(back(duv), wuv)
+
+
 @pytest.mark.parametrize("w,h,qp,ns,bd", [(64, 48, 22, 1, 8), (128, 96, 37, 2, 8), (200, 120, 27, 3, 8),
                                            (320, 240, 32, 4, 8), (64, 48, 22, 1, 10), (128, 96, 45, 2, 10),
                                            (208, 128, 27, 3, 10), (320, 240, 0, 4, 10)])
@@ -365,13 +435,17 @@ def test_hevc_main10_4k_full_size(pkg):
         mv = np.stack([mv8["mvx"], mv8["mvy"]], 1).astype(np.int32)
         cb, cr = orc.split_uv(cuv)
         fb, fr = orc.split_uv(ruv)
-        o = orc.hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, orc.HEVC_ROUND_INTER, 10)
+        o = orc.hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, orc.HEVC_ROUND_INTER, 10, 5)
         assert np.array_equal(got["quant"], o["coef"]), "P levels"
+        nt = o["culog2"].size // 4
+        assert np.array_equal(got["hevc_mode"].reshape(nt, 16)[:, :4].reshape(-1), o["culog2"]), "P CU sizes"
+        say(f"  P CU8 / CU16 / CU32 share: {[int((o['culog2'] == L).sum()) for L in (3, 4, 5)]}")
         uv = np.empty_like(cuv)
         uv[:, 0::2], uv[:, 1::2] = o["cb"], o["cr"]
         mv4 = np.zeros((mv.shape[0], 4), np.int32)
         mv4[:, :2] = mv
-        wy, wuv = orc.hevc_deblock(o["y"], uv, qp, 0, orc.hevc_cu_cbf(o["coef"]), mv4, ns, 10, 5)
+        wy, wuv = orc.hevc_deblock(o["y"], uv, qp, 0, orc.hevc_cu_cbf(o["coef"], o["culog2"], w), mv4, ns, 10, 5,
+                                   o["culog2"])
         my, muv = enc.map_recon()
         assert np.array_equal(my, wy) and np.array_equal(muv, wuv), "P recon"
         c8, r8 = (cy >> 2).astype(np.uint8), (ry >> 2).astype(np.uint8)
@@ -510,13 +584,13 @@ def test_deblock_ctu32_slices(pkg, w, h, qp, intra, ns):
     wy, wuv = orc.hevc_deblock(y, uv, qp, intra, cbf, mv4, ns, 8, 5)
     assert not np.array_equal(wy, orc.hevc_deblock(y, uv, qp, intra, cbf, mv4, ns, 8, 4)[0])
     d = [dev(torch, a) for a in (y, uv, cbf, mv4)]
-    assert pkg.hip.bc250_hevc_launch_deblock_ctu(P(d[0]), P(d[1]), w, w, w, h, qp, intra, P(d[2]), P(d[3]), ns, 5,
-                                                 8, None) == 0
+    assert pkg.hip.bc250_hevc_launch_deblock_ctu(P(d[0]), P(d[1]), w, w, w, h, qp, intra, P(d[2]), P(d[3]), None,
+                                                 ns, 5, 8, None) == 0
     torch.cuda.synchronize()
     assert np.array_equal(host(d[0]), wy)
     assert np.array_equal(host(d[1]), wuv)
-    assert pkg.hip.bc250_hevc_launch_deblock_ctu(P(d[0]), P(d[1]), w, w, w, h, qp, intra, P(d[2]), P(d[3]), ns, 6,
-                                                 8, None) == -1
+    assert pkg.hip.bc250_hevc_launch_deblock_ctu(P(d[0]), P(d[1]), w, w, w, h, qp, intra, P(d[2]), P(d[3]), None,
+                                                 ns, 6, 8, None) == -1
 
 
 @pytest.mark.parametrize("w,h,qp,intra,ns", [(64, 48, 37, 1, 1), (192, 112, 30, 0, 2), (320, 240, 45, 1, 3),

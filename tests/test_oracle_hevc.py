@@ -435,6 +435,64 @@ def test_encode_p_matches_reference_order(bd, qp):
                                       np.clip(cp.astype(int) + crr, 0, vmax))
 
 
+@need_ref
+@pytest.mark.parametrize("bd,qp", [(8, 22), (10, 32)])
+def test_encode_p_ctu32_large_cus_decode_to_recon(bd, qp):
+    """CTU32 P coding: the CU choice follows the vectors, CU8 areas equal
+    the CTU16 coding, and every CU16 / CU32 reconstructs as the reference
+    decoder would from its levels: the reference's MC (hevc_mc_uni) plus
+    its inverse transform (hevcd_transform) of the flat-list dequantised
+    levels, gathered from the per-area layout."""
+    L = ref()
+    L.hevcd_transform.argtypes = [P, I, C.c_bool, I]
+    mc = L.hevc_mc_uni_8 if bd == 8 else L.hevc_mc_uni_10
+    rs = np.random.default_rng(qp)
+    w, h = 112, 80                          # 7 x 5 areas: partial CTU32s at the right and bottom
+    sy, scb, scr = _frame(rs, w, h, bd)
+    fy, fcb, fcr = _frame(np.random.default_rng(qp + 1), w, h, bd)
+    wtu, htu = w // 16, h // 16
+    t = wtu * htu
+    mv = rs.integers(-40, 41, (t * 4, 2)).astype(np.int32)
+    for a in (0, 1, wtu, wtu + 1):          # CTU32 (0, 0): one vector -> CU32
+        mv[a * 4:a * 4 + 4] = (13, -6)
+    mv[2 * 4:2 * 4 + 4] = (5, 7)            # CTU32 (1, 0): one CU16 (area 2), the rest CU8
+    mv[6 * 4:6 * 4 + 4] = (-9, 3)           # area 6: right edge, CTU32 (3, 0) partial -> CU16
+    mv[(4 * wtu + 2) * 4:(4 * wtu + 2) * 4 + 4] = (1, 1)   # bottom row (partial CTU32) -> CU16
+    want_lg = {0: 5, 1: 5, wtu: 5, wtu + 1: 5, 2: 4, 6: 4, 4 * wtu + 2: 4}
+    got = orc.hevc_encode_p(sy, scb, scr, fy, fcb, fcr, qp, mv, orc.HEVC_ROUND_INTER, bd, 5)
+    base = orc.hevc_encode_p(sy, scb, scr, fy, fcb, fcr, qp, mv, orc.HEVC_ROUND_INTER, bd, 4)
+    for a in range(t):
+        assert (got["culog2"][a * 4:a * 4 + 4] == want_lg.get(a, 3)).all(), a
+        if a not in want_lg:
+            ax, ay = (a % wtu) * 16, (a // wtu) * 16
+            assert np.array_equal(got["coef"][a * 384:(a + 1) * 384], base["coef"][a * 384:(a + 1) * 384])
+            assert np.array_equal(got["y"][ay:ay + 16, ax:ax + 16], base["y"][ay:ay + 16, ax:ax + 16])
+    qoff = 0 if bd == 8 else 12
+    cqp = L.hevc_chroma_qp_from_luma(qp) + qoff
+    vmax = (1 << bd) - 1
+    nonzero = 0
+    for (x0, y0, log2n, areas) in ((0, 0, 5, (0, 1, wtu, wtu + 1)), (32, 0, 4, (2,)), (96, 0, 4, (6,)),
+                                   (32, 64, 4, (4 * wtu + 2,))):
+        mx, my = (int(v) for v in mv[areas[0] * 4])
+        for comp, (plane, ref_p, key) in enumerate(((sy, fy, "y"), (scb, fcb, "cb"), (scr, fcr, "cr"))):
+            lg = log2n if comp == 0 else log2n - 1
+            n, chroma = 1 << lg, comp > 0
+            bx, by = (x0, y0) if not chroma else (x0 // 2, y0 // 2)
+            pw, ph = (w, h) if not chroma else (w // 2, h // 2)
+            pred = np.zeros(n * n, pix(bd))
+            mc(p(np.ascontiguousarray(ref_p)), pw, pw, ph, bx, by, n, n, mx, my, int(chroma), p(pred), n)
+            per, slot = (256, 0) if comp == 0 else (64, 256 + (comp - 1) * 64)
+            per = per if log2n == 5 else n * n
+            lv = np.concatenate([got["coef"][a * 384 + slot:][:per] for a in areas]).astype(np.int16)
+            d = np.zeros(n * n, np.int16)
+            orc.lib.orc_hevc_dequant(p(lv), n * n, cqp if chroma else qp + qoff, bd + lg - 5, p(d))
+            L.hevcd_transform(p(d), lg, False, bd)
+            rec = np.clip(pred.astype(int) + d, 0, vmax).reshape(n, n)
+            assert np.array_equal(got[key][by:by + n, bx:bx + n], rec), (x0, y0, key)
+            nonzero += int(lv.any())
+    assert nonzero >= 4  # levels are exercised
+
+
 def test_refine8_never_worse_and_consistent():
     rs = np.random.default_rng(12)
     h, w = 48, 64
@@ -574,6 +632,70 @@ def test_deblock_matches_reference_decoder():
                     cases += 1
                     changed += int(not np.array_equal(wy, y)) + int(not np.array_equal(wuv, uv))
     assert cases == 270 and changed > 250
+
+
+@pytest.mark.skipif(not os.path.exists(orc.REF_HEVC_DEBLOCK), reason="oracle/_ref not built (reference tree absent)")
+def test_deblock_ctu32_large_cus_matches_reference_decoder():
+    """CTU32 pictures through the reference decoder's loop filter: CTB 32
+    (slices as CTU32-row ranges) and, for P pictures, CU16 / CU32 whose inner
+    8x8 cells carry no edge bits; the oracle's CU-size map must skip exactly
+    those edges.  CU sizes drawn per CTU32 (partial CTU32s at the edges keep
+    CU16 at most), cbf per transform block."""
+    lib = C.CDLL(orc.REF_HEVC_DEBLOCK, mode=os.RTLD_LAZY)
+    lib.ref_hevc_deblock_ctu.argtypes = [P, P, I, I, I, I, P, P, I, I, I, P]
+    lib.ref_hevc_deblock_ctu.restype = I
+    rs = np.random.default_rng(12)
+    cases = 0
+    for (w, h, bd) in ((96, 64, 8), (112, 80, 8), (160, 96, 10)):
+        wtu, htu = w // 16, h // 16
+        for qp in (22, 30, 37, 45):
+            for intra in (0, 1):
+                for ns in (1, 2, 3):
+                    step = int(rs.integers(2, 20))
+                    y = _blocky_picture(rs, h, w, step, int(rs.integers(0, 3)))
+                    uv = _blocky_picture(rs, h // 2, w, step // 2 + 1, 1)
+                    if bd == 10:
+                        y = y.astype(np.uint16) * 4 + rs.integers(0, 4, y.shape).astype(np.uint16)
+                        uv = uv.astype(np.uint16) * 4 + rs.integers(0, 4, uv.shape).astype(np.uint16)
+                    ncu = wtu * htu * 4
+                    culog2 = np.full(ncu, 3, np.uint8)
+                    if not intra:
+                        for Y in range((htu + 1) // 2):
+                            for X in range((wtu + 1) // 2):
+                                areas = [(2 * Y + (a >> 1)) * wtu + 2 * X + (a & 1) for a in range(4)
+                                         if 2 * X + (a & 1) < wtu and 2 * Y + (a >> 1) < htu]
+                                kind = int(rs.integers(0, 3))
+                                if kind == 2 and len(areas) == 4:
+                                    for t in areas:
+                                        culog2[t * 4:t * 4 + 4] = 5
+                                elif kind >= 1:
+                                    for t in areas:
+                                        if rs.random() < 0.6:
+                                            culog2[t * 4:t * 4 + 4] = 4
+                    mv4 = np.zeros((ncu, 4), np.int32)
+                    mv4[:, :2] = rs.integers(-8, 9, (ncu, 2))
+                    for cu in range(ncu):   # one vector per CU
+                        L = int(culog2[cu])
+                        if L >= 4:
+                            t = cu // 4
+                            t0 = t if L == 4 else ((t // wtu) & ~1) * wtu + ((t % wtu) & ~1)
+                            mv4[cu] = mv4[t0 * 4]
+                    cbf = (rs.random(ncu) < 0.4).astype(np.uint8)
+                    for cu in range(ncu):
+                        L = int(culog2[cu])
+                        if L >= 4:
+                            t = cu // 4
+                            t0 = t if L == 4 else ((t // wtu) & ~1) * wtu + ((t % wtu) & ~1)
+                            cbf[cu] = cbf[t0 * 4]
+                    wy, wuv = orc.hevc_deblock(y, uv, qp, intra, cbf, mv4, ns, bd, 5, None if intra else culog2)
+                    ry, ruv = y.copy(), uv.copy()
+                    assert lib.ref_hevc_deblock_ctu(ry.ctypes.data, ruv.ctypes.data, w, h, qp, intra,
+                                                    cbf.ctypes.data, mv4.ctypes.data, ns, bd, 5,
+                                                    None if intra else culog2.ctypes.data) == 0
+                    assert np.array_equal(wy, ry), (w, h, bd, qp, intra, ns)
+                    assert np.array_equal(wuv, ruv), (w, h, bd, qp, intra, ns)
+                    cases += 1
+    assert cases == 72
 
 
 def _two_columns(left, right, w=32, h=16):
