@@ -79,14 +79,17 @@ BC_DEV int next_cols(int v)
 // One pass of a 4x4 transform over the block held by a row of 16 lanes, the
 // lane's coefficients already permuted to match the moved values: columns
 // (sum over block rows) or rows (sum over block columns).
+// Values are int16 and |c| <= 90, so 24-bit multiplies (full rate) are exact.
 BC_DEV int col_pass(int v, const int c[4])
 {
-    return c[0] * v + c[1] * down_rows<1>(v) + c[2] * down_rows<2>(v) + c[3] * down_rows<3>(v);
+    return __mul24(c[0], v) + __mul24(c[1], down_rows<1>(v)) + __mul24(c[2], down_rows<2>(v)) +
+           __mul24(c[3], down_rows<3>(v));
 }
 
 BC_DEV int row_pass(int v, const int c[4])
 {
-    return c[0] * v + c[1] * next_cols<1>(v) + c[2] * next_cols<2>(v) + c[3] * next_cols<3>(v);
+    return __mul24(c[0], v) + __mul24(c[1], next_cols<1>(v)) + __mul24(c[2], next_cols<2>(v)) +
+           __mul24(c[3], next_cols<3>(v));
 }
 
 // Sum over each row of 16 lanes, result in every lane of the row: DPP
@@ -171,6 +174,10 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
     }
     const int qy = clampi(qp + QOFF, 0, BD == 8 ? 51 : 63), cqp = hevc::chroma_qp(qp) + QOFF;
     const Quant QL = make_quant(qy, QSH, round_q12), QC = make_quant(clampi(cqp, 0, 63), QSH, round_q12);
+    // this lane's quantiser, computed here once (the empty asm keeps the
+    // compiler from recomputing the division and 64-bit products per PU)
+    Quant QP = luma ? QL : QC;
+    asm("" : "+v"(QP.p), "+v"(QP.shift), "+v"(QP.d), "+v"(QP.off), "+v"(QP.m));
     const int q = luma ? qy : cqp;
 
     // source prefetch: 4 luma samples per lane, 4 interleaved chroma samples per lane 0..31
@@ -322,7 +329,8 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             const int Li = pi == 0 ? L0 : pi == 1 ? L1 : pi == 2 ? L2 : L3;
             const int Tj = pj == 0 ? T0 : pj == 1 ? T1 : pj == 2 ? T2 : T3;
             // ---- the four predictions of this lane's sample
-            const int p_pl = ((3 - pj) * Li + (pj + 1) * T4 + (3 - pi) * Tj + (pi + 1) * L4 + 4) >> 3;
+            const int p_pl =
+                (__mul24(3 - pj, Li) + __mul24(pj + 1, T4) + __mul24(3 - pi, Tj) + __mul24(pi + 1, L4) + 4) >> 3;
             const int dc = (L0 + L1 + L2 + L3 + T0 + T1 + T2 + T3 + 4) >> 3;
             int p_dc = dc;
             if (luma) {
@@ -348,7 +356,7 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             if (!luma) { mode = 1; pred = dc; }
             // ---- forward transform, quantisation (cross-lane moves, no LDS)
             const int t1 = sat16((col_pass(s - pred, fc) + (1 << (S1 - 1))) >> S1);
-            const int lv = quant(sat16((row_pass(t1, fr) + 128) >> 8), luma ? QL : QC);
+            const int lv = quant(sat16((row_pass(t1, fr) + 128) >> 8), QP);
             // ---- dequantisation, inverse, reconstruction
             const int t2 = sat16((col_pass(dequant(lv, q, QSH), ic) + 64) >> 7);
             const int rec = clampi(pred + sat16((row_pass(t2, ir) + (1 << (S2INV - 1))) >> S2INV), 0, MAXV);
