@@ -46,7 +46,10 @@ __constant__ uint8_t kTc0[52][3] = {
     {4, 5, 7}, {4, 5, 8}, {4, 6, 9}, {5, 7, 10}, {6, 8, 11}, {6, 8, 13}, {7, 10, 14}, {8, 11, 16},
     {9, 12, 18}, {10, 13, 20}, {11, 15, 23}, {13, 17, 25}};
 
-constexpr int ROWS = 2;        // MB rows per workgroup
+#ifndef BC250_DBK_ROWS
+#define BC250_DBK_ROWS 2
+#endif
+constexpr int ROWS = BC250_DBK_ROWS;  // MB rows per workgroup (a power of two)
 constexpr int LP = 32;         // luma tile pitch: cols 0..15 MB x-1, 16..31 MB x
 constexpr int CP = 16;         // chroma tile pitch: cols 0..7 MB x-1, 8..15 MB x
 constexpr int RING = 16;       // LDS hand-off slots per producing wave

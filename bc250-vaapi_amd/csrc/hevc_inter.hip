@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__rest
         L.h[f][e] = (int16_t)(acc >> DOWN);
     }
     wave_lds_sync();
-    auto sad_of = [&](int mx, int my) {
+    auto diff_of = [&](int mx, int my) {
         const int fx = mx & 3, fy = my & 3;
         const int wx = pc + (mx >> 2) - dx + 4, wy = pr + (my >> 2) - dy + 4;  // integer position in W
         int pv;
@@ -307,22 +307,28 @@ __global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__rest
             }
             pv = clampi((p14 + OFF) >> SH, 0, MAXV);
         }
-        return wave_sum_u32((uint32_t)iabs(cv - pv));
+        return (uint32_t)iabs(cv - pv);
     };
     const int hx[8] = {2, -2, 0, 0, 2, 2, -2, -2}, hy[8] = {0, 0, 2, -2, 2, -2, 2, -2};
     int bx = dx * 4, by = dy * 4;
-    uint32_t best = sad_of(bx, by);
+    uint32_t best = wave_sum_u32(diff_of(bx, by));
 #pragma unroll 1
     for (int stage = 1; stage <= 2; stage++) {
         const int cx = bx, cy = by;
-#pragma unroll 1
+        // the stage's eight differences first, then their sums, then the
+        // choice in candidate order (strictly lower wins)
+        uint32_t d[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            d[k] = diff_of(cx + (stage == 1 ? hx[k] : hx[k] / 2), cy + (stage == 1 ? hy[k] : hy[k] / 2));
+#pragma unroll
+        for (int k = 0; k < 8; k++) d[k] = wave_sum_u32(d[k]);
+#pragma unroll
         for (int k = 0; k < 8; k++) {
-            const int mx = cx + (stage == 1 ? hx[k] : hx[k] / 2), my = cy + (stage == 1 ? hy[k] : hy[k] / 2);
-            const uint32_t s = sad_of(mx, my);
-            if (s < best) {
-                best = s;
-                bx = mx;
-                by = my;
+            if (d[k] < best) {
+                best = d[k];
+                bx = cx + (stage == 1 ? hx[k] : hx[k] / 2);
+                by = cy + (stage == 1 ? hy[k] : hy[k] / 2);
             }
         }
     }
