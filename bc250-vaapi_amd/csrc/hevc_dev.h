@@ -101,7 +101,7 @@ BC_DEV int quant(int v, const Quant &Q)
 BC_DEV int dequant(int c, int q, int bds)
 {
     const int a = q / 6 + 4;
-    const int prod = c * level_scale(q % 6);
+    const int prod = __mul24(c, level_scale(q % 6));  // int16 level x <= 72: 24-bit product is exact
     const int v = a >= bds ? prod * (1 << (a - bds)) : (prod + (1 << (bds - a - 1))) >> (bds - a);
     return sat16(v);
 }

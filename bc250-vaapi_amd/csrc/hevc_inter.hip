@@ -34,12 +34,15 @@ BC_DEV Depth depth_of(int bd)
 // One n x n forward or inverse pass for the coefficient this lane owns:
 // lane (i, j) -> sat16((sum_k M[i][k] * X[k][j] + rnd) >> s)  (forward rows of
 // M; `transposed` uses M[k][i] instead).  X row-major in LDS.
+// Every product in this file has an int8 matrix entry or filter tap and an
+// int16-range operand (samples, residuals, saturated intermediates), so the
+// full-rate 24-bit multiply (__mul24) is exact.
 template <int N>
 BC_DEV int pass(const int8_t *M, bool transposed, const int *X, int i, int j, int s)
 {
     int acc = 0;
 #pragma unroll
-    for (int k = 0; k < N; k++) acc += (int)(transposed ? M[k * N + i] : M[i * N + k]) * X[k * N + j];
+    for (int k = 0; k < N; k++) acc += __mul24((int)(transposed ? M[k * N + i] : M[i * N + k]), X[k * N + j]);
     return sat16((acc + (1 << (s - 1))) >> s);
 }
 
@@ -50,7 +53,7 @@ BC_DEV int pass_rows(const int8_t *M, bool transposed, const int *T, int i, int 
 {
     int acc = 0;
 #pragma unroll
-    for (int k = 0; k < N; k++) acc += (int)(transposed ? M[k * N + j] : M[j * N + k]) * T[i * N + k];
+    for (int k = 0; k < N; k++) acc += __mul24((int)(transposed ? M[k * N + j] : M[j * N + k]), T[i * N + k]);
     return sat16((acc + (1 << (s - 1))) >> s);
 }
 
@@ -69,12 +72,12 @@ BC_DEV int mc_sample(const int *W, int P, int wx, int wy, int fx, int fy, int bd
     if (!fy) {
         int s = 0;
 #pragma unroll
-        for (int k = 0; k < NT; k++) s += fh[k] * W[wy * P + wx + k - BACK];
+        for (int k = 0; k < NT; k++) s += __mul24((int)fh[k], W[wy * P + wx + k - BACK]);
         p14 = (int16_t)(s >> down);
     } else if (!fx) {
         int s = 0;
 #pragma unroll
-        for (int k = 0; k < NT; k++) s += fv[k] * W[(wy + k - BACK) * P + wx];
+        for (int k = 0; k < NT; k++) s += __mul24((int)fv[k], W[(wy + k - BACK) * P + wx]);
         p14 = (int16_t)(s >> down);
     } else {
         int s = 0;
@@ -82,8 +85,8 @@ BC_DEV int mc_sample(const int *W, int P, int wx, int wy, int fx, int fy, int bd
         for (int k = 0; k < NT; k++) {
             int t = 0;
 #pragma unroll
-            for (int j = 0; j < NT; j++) t += fh[j] * W[(wy + k - BACK) * P + wx + j - BACK];
-            s += fv[k] * (int)(int16_t)(t >> down);
+            for (int j = 0; j < NT; j++) t += __mul24((int)fh[j], W[(wy + k - BACK) * P + wx + j - BACK]);
+            s += __mul24((int)fv[k], (int)(int16_t)(t >> down));
         }
         p14 = (int16_t)(s >> 6);
     }
@@ -193,7 +196,7 @@ __global__ __launch_bounds__(256) void hevc_nxn_kernel(const int16_t *__restrict
     for (int e = threadIdx.x; e < N2; e += 256) {  // columns: T[i][c] = sum_r M'[i][r] X[r][c]
         const int i = e >> log2n, c = e & (N - 1);
         int acc = 0;
-        for (int r = 0; r < N; r++) acc += (INVERSE ? M[r * N + i] : M[i * N + r]) * X[r * N + c];
+        for (int r = 0; r < N; r++) acc += __mul24(INVERSE ? M[r * N + i] : M[i * N + r], X[r * N + c]);
         T[e] = sat16((acc + (1 << (s1 - 1))) >> s1);
     }
     __syncthreads();
@@ -201,7 +204,7 @@ __global__ __launch_bounds__(256) void hevc_nxn_kernel(const int16_t *__restrict
     for (int e = threadIdx.x; e < N2; e += 256) {  // rows: Y[i][j] = sum_c M'[j][c] T[i][c]
         const int i = e >> log2n, j = e & (N - 1);
         int acc = 0;
-        for (int c = 0; c < N; c++) acc += (INVERSE ? M[c * N + j] : M[j * N + c]) * T[i * N + c];
+        for (int c = 0; c < N; c++) acc += __mul24(INVERSE ? M[c * N + j] : M[j * N + c], T[i * N + c]);
         const int v = sat16((acc + (1 << (s2 - 1))) >> s2);
         out[(size_t)b * N2 + e] = (int16_t)(INVERSE ? v : quant(v, Q));
     }
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__rest
         const int8_t *fh = kQpel[f + 1];
         int acc = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) acc += fh[k] * L.w[row * 16 + col - 3 + k];
+        for (int k = 0; k < 8; k++) acc += __mul24((int)fh[k], L.w[row * 16 + col - 3 + k]);
         L.h[f][e] = (int16_t)(acc >> DOWN);
     }
     wave_lds_sync();
@@ -296,12 +299,12 @@ __global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__rest
                 int acc = 0;
                 if (!fx) {
 #pragma unroll
-                    for (int k = 0; k < 8; k++) acc += fv[k] * L.w[(wy - 3 + k) * 16 + wx];
+                    for (int k = 0; k < 8; k++) acc += __mul24((int)fv[k], L.w[(wy - 3 + k) * 16 + wx]);
                     p14 = (int16_t)(acc >> DOWN);
                 } else {
                     const int *H = L.h[fx - 1];
 #pragma unroll
-                    for (int k = 0; k < 8; k++) acc += fv[k] * H[(wy - 3 + k) * 9 + wx - 3];
+                    for (int k = 0; k < 8; k++) acc += __mul24((int)fv[k], H[(wy - 3 + k) * 9 + wx - 3]);
                     p14 = (int16_t)(acc >> 6);
                 }
             }
@@ -524,7 +527,7 @@ __global__ __launch_bounds__(256) void hevc_p_large_kernel(
             for (int e = tid; e < mm; e += 256) {  // columns: T[i][c] = sum_r M[i][r] X[r][c]
                 const int i = e >> lg, c = e & (m - 1);
                 int acc = 0;
-                for (int r = 0; r < m; r++) acc += L.m[(i << lg) + r] * L.x[(r << lg) + c];
+                for (int r = 0; r < m; r++) acc += __mul24(L.m[(i << lg) + r], L.x[(r << lg) + c]);
                 L.t[e] = sat16((acc + (1 << (s1 - 1))) >> s1);
             }
             __syncthreads();
@@ -537,7 +540,7 @@ __global__ __launch_bounds__(256) void hevc_p_large_kernel(
                 if (e < mm) {
                     const int ii = e >> lg, j = e & (m - 1);
                     int acc = 0;
-                    for (int c = 0; c < m; c++) acc += L.m[(j << lg) + c] * L.t[(ii << lg) + c];
+                    for (int c = 0; c < m; c++) acc += __mul24(L.m[(j << lg) + c], L.t[(ii << lg) + c]);
                     lv[i] = quant(sat16((acc + (1 << (s2 - 1))) >> s2), Q);
                     const int ar = e / per, tx = (x0 >> 4) + (ar & 1), ty = (y0 >> 4) + (ar >> 1);
                     coef[(size_t)(ty * wtu + tx) * 384 + base + (e - ar * per)] = (int16_t)lv[i];
@@ -553,7 +556,7 @@ __global__ __launch_bounds__(256) void hevc_p_large_kernel(
             for (int e = tid; e < mm; e += 256) {  // inverse columns: T[i][c] = sum_r M[r][i] X[r][c]
                 const int i = e >> lg, c = e & (m - 1);
                 int acc = 0;
-                for (int r = 0; r < m; r++) acc += L.m[(r << lg) + i] * L.x[(r << lg) + c];
+                for (int r = 0; r < m; r++) acc += __mul24(L.m[(r << lg) + i], L.x[(r << lg) + c]);
                 L.t[e] = sat16((acc + 64) >> 7);
             }
             __syncthreads();
@@ -563,7 +566,7 @@ __global__ __launch_bounds__(256) void hevc_p_large_kernel(
                 if (e < mm) {
                     const int ii = e >> lg, j = e & (m - 1);
                     int acc = 0;
-                    for (int c = 0; c < m; c++) acc += L.m[(c << lg) + j] * L.t[(ii << lg) + c];
+                    for (int c = 0; c < m; c++) acc += __mul24(L.m[(c << lg) + j], L.t[(ii << lg) + c]);
                     const int rv = clampi(pred[i] + sat16((acc + (1 << (s2i - 1))) >> s2i), 0, MAXV);
                     if (chroma) stpix<Pix>(ouv + (size_t)(by + ii) * opuv, 2 * (bx + j) + comp, rv);
                     else stpix<Pix>(oy + (size_t)(by + ii) * opy, bx + j, rv);
