@@ -570,12 +570,15 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
         } else {
             // P modes: zero, or at CTU32 the log2 CU size of each CU8 at [16 t + z]
             r = bc250_launch_zero_i32((int *)s->hmodes, (int)n * 4, st);
-            if (r == 0)
+            if (r == 0 && p.hevc_ctu_log2 == 5)
                 r = bc250_hevc_launch_p_encode_ctu(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv,
                                                    out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp,
-                                                   2 /* inter rounding */, s->mv8, (int16_t *)s->levels,
-                                                   p.hevc_ctu_log2, s->hculog2,
-                                                   p.hevc_ctu_log2 == 5 ? s->hmodes : nullptr, bd, st);
+                                                   2 /* inter rounding */, s->mv8, (int16_t *)s->levels, 5,
+                                                   s->hculog2, s->hmodes, bd, st);
+            else if (r == 0)
+                r = bc250_hevc_launch_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv, out.y,
+                                               out.uv, out.py, out.puv, s->wc, s->hc, qp, 2 /* inter rounding */,
+                                               s->mv8, (int16_t *)s->levels, bd, st);
         }
     }
     hipEventRecord(sl.ev[3], st);
