@@ -568,17 +568,14 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
                                                    st);
             if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, (int)n * 16, st);
         } else {
-            // P modes: zero, or at CTU32 the log2 CU size of each CU8 at [16 t + z]
-            r = bc250_launch_zero_i32((int *)s->hmodes, (int)n * 4, st);
-            if (r == 0 && p.hevc_ctu_log2 == 5)
-                r = bc250_hevc_launch_p_encode_ctu(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv,
-                                                   out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp,
-                                                   2 /* inter rounding */, s->mv8, (int16_t *)s->levels, 5,
-                                                   s->hculog2, s->hmodes, bd, st);
-            else if (r == 0)
-                r = bc250_hevc_launch_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv, out.y,
+            // P modes: zero, or at CTU32 the log2 CU size of each CU8 at [16 t + z];
+            // at CTU16 the coding kernel also writes the coded-luma flags
+            const bool c32 = p.hevc_ctu_log2 == 5;
+            r = bc250_hevc_launch_p_encode_ctu(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv, out.y,
                                                out.uv, out.py, out.puv, s->wc, s->hc, qp, 2 /* inter rounding */,
-                                               s->mv8, (int16_t *)s->levels, bd, st);
+                                               s->mv8, (int16_t *)s->levels, p.hevc_ctu_log2,
+                                               c32 ? s->hculog2 : nullptr, s->hmodes, c32 ? nullptr : s->hcbf, bd,
+                                               st);
         }
     }
     hipEventRecord(sl.ev[3], st);
@@ -586,7 +583,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
     // slices never filter across their borders, so idc 0 and 2 agree
     if (r == 0 && p.deblock_idc != 1) {
         const uint8_t *cus = !is_intra && p.hevc_ctu_log2 == 5 ? s->hculog2 : nullptr;
-        if (!is_intra) r = bc250_hevc_cu_cbf_ctu((const int16_t *)s->levels, s->wc, s->hc, cus, s->hcbf, st);
+        if (cus) r = bc250_hevc_cu_cbf_ctu((const int16_t *)s->levels, s->wc, s->hc, cus, s->hcbf, st);
         if (r == 0)
             r = bc250_hevc_launch_deblock_ctu(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->hcbf,
                                               s->mv8, cus, num_slices, p.hevc_ctu_log2, bd, st);

@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256) void hevc_p_encode_kernel(
     const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, const uint8_t *__restrict__ ry,
     const uint8_t *__restrict__ ruv, int rpy, int rpuv, uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy,
     int opuv, int wc, int hc, int nctu, int qp, int round_q12, const uint4 *__restrict__ mvs,
-    int16_t *__restrict__ coef)
+    int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes)
 {
     constexpr int MAXV = (1 << BD) - 1, QOFF = 6 * (BD - 8);
     constexpr int S1_8 = BD - 6, QSH8 = BD - 2, S1_4 = BD == 8 ? 1 : 3, QSH4 = BD - 3, S2INV = 20 - BD;
@@ -401,6 +401,12 @@ __global__ __launch_bounds__(256) void hevc_p_encode_kernel(
                        : 0;
     int16_t *co = coef + (size_t)ctu * 384;
     co[wv * 64 + lane] = (int16_t)lv;
+    // optional side outputs: the CU's coded-luma flag, and zero P modes
+    if (cbf) {
+        const bool any = __any(lv != 0);
+        if (lane == 0) cbf[cu] = any ? 1 : 0;
+    }
+    if (modes && lane < 4) modes[(size_t)ctu * 16 + wv * 4 + lane] = 0;
     if (cl) co[256 + comp * 64 + wv * 16 + ck] = (int16_t)clv;
 
     // dequantise, inverse: columns, then rows
@@ -709,10 +715,10 @@ extern "C" int bc250_hevc_launch_refine8(const void *cur, int cpitch, const void
     return launch_ok();
 }
 
-extern "C" int bc250_hevc_launch_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
-                                          const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy, int opuv,
-                                          int wc, int hc, int qp, int round_q12, const uint32_t *mvs, int16_t *coef,
-                                          int bd, void *stream)
+static int launch_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry, const void *ruv,
+                           int rpy, int rpuv, void *oy, void *ouv, int opy, int opuv, int wc, int hc, int qp,
+                           int round_q12, const uint32_t *mvs, int16_t *coef, uint8_t *cbf, uint8_t *modes, int bd,
+                           void *stream)
 {
     const int b = bd == 8 ? 1 : 2, mw = wc * b;
     if (!bd_ok(bd) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || round_q12 < 0 ||
@@ -726,28 +732,37 @@ extern "C" int bc250_hevc_launch_p_encode(const void *cy, const void *cuv, int c
     if (bd == 8)
         hevc_p_encode_kernel<uint8_t, 8><<<grid_xcd(nctu), 256, 0, st>>>(C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy,
                                                                          opuv, wc, hc, nctu, qp, round_q12,
-                                                                         (const uint4 *)mvs, coef);
+                                                                         (const uint4 *)mvs, coef, cbf, modes);
     else
         hevc_p_encode_kernel<uint16_t, 10><<<grid_xcd(nctu), 256, 0, st>>>(C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU,
                                                                            opy, opuv, wc, hc, nctu, qp, round_q12,
-                                                                           (const uint4 *)mvs, coef);
+                                                                           (const uint4 *)mvs, coef, cbf, modes);
     return launch_ok();
+}
+
+extern "C" int bc250_hevc_launch_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
+                                          const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy, int opuv,
+                                          int wc, int hc, int qp, int round_q12, const uint32_t *mvs, int16_t *coef,
+                                          int bd, void *stream)
+{
+    return launch_p_encode(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12, mvs,
+                           coef, nullptr, nullptr, bd, stream);
 }
 
 extern "C" int bc250_hevc_launch_p_encode_ctu(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
                                               const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy,
                                               int opuv, int wc, int hc, int qp, int round_q12, const uint32_t *mvs,
-                                              int16_t *coef, int ctu_log2, uint8_t *culog2, uint8_t *modes, int bd,
-                                              void *stream)
+                                              int16_t *coef, int ctu_log2, uint8_t *culog2, uint8_t *modes,
+                                              uint8_t *cbf, int bd, void *stream)
 {
-    if ((ctu_log2 != 4 && ctu_log2 != 5) || !culog2) return -1;
-    const int r = bc250_hevc_launch_p_encode(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp,
-                                             round_q12, mvs, coef, bd, stream);
+    if ((ctu_log2 != 4 && ctu_log2 != 5) || (ctu_log2 == 5 && !culog2)) return -1;
+    const int r = launch_p_encode(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12,
+                                  mvs, coef, cbf, modes, bd, stream);
     if (r != 0) return r;
     hipStream_t st = (hipStream_t)stream;
     const int wtu = wc / 16, htu = hc / 16, ncu = wtu * htu * 4;
     if (ctu_log2 == 4) {
-        fill_u8_kernel<<<(ncu + 255) / 256, 256, 0, st>>>(culog2, ncu, 3);
+        if (culog2) fill_u8_kernel<<<(ncu + 255) / 256, 256, 0, st>>>(culog2, ncu, 3);
         return launch_ok();
     }
     const int n32 = ((wtu + 1) / 2) * ((htu + 1) / 2);

@@ -117,12 +117,15 @@ int bc250_hevc_launch_p_encode(const void *cy, const void *cuv, int cpy, int cpu
  * a CU16's area holds its 16x16 luma TU (256, row-major) and 8x8 Cb / Cr
  * TUs (64 each); area q (z-order) of a CU32 holds levels [256 q, 256 q +
  * 256) of its 32x32 luma TU and [64 q, 64 q + 64) of each 16x16 chroma TU.
- * culog2 (nctu * 4 bytes, required) = log2 of the CU holding each CU8 (3,
- * 4 or 5); modes (may be NULL) gets the same at [16 t + z]. */
+ * culog2 (nctu * 4 bytes; required at CTU32, may be NULL at CTU16) = log2
+ * of the CU holding each CU8 (3, 4 or 5).  modes (may be NULL, nctu * 16
+ * bytes): zero, then at CTU32 the CU sizes at [16 t + z].  cbf (may be NULL,
+ * nctu * 4 bytes): the coded-luma flag of each CU8's 8x8 levels (the
+ * deblocking input at CTU16; at CTU32 use bc250_hevc_cu_cbf_ctu). */
 int bc250_hevc_launch_p_encode_ctu(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
                                    const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy, int opuv, int wc,
                                    int hc, int qp, int round_q12, const uint32_t *mvs, int16_t *coef, int ctu_log2,
-                                   uint8_t *culog2, uint8_t *modes, int bd, void *stream);
+                                   uint8_t *culog2, uint8_t *modes, uint8_t *cbf, int bd, void *stream);
 
 /* I picture: CTU16 -> 4 CU8 -> 4 PU4 intra (Planar/DC/H/V by SAD, DST),
  * chroma DC per CU (DCT, QpC), slices as CTU-row ranges

@@ -247,7 +247,7 @@ def test_p_encode_ctu32_stage(pkg, w, h, qp, bd):
     modes = torch.zeros(t * 16, dtype=torch.uint8, device="cuda")
     assert pkg.hip.bc250_hevc_launch_p_encode_ctu(P(d[0]), P(d[1]), pitch, pitch, P(d[2]), P(d[3]), pitch, pitch,
                                                   P(oy), P(ouv), pitch, pitch, wc, hc, qp, orc.HEVC_ROUND_INTER,
-                                                  P(d[4]), P(coef), 5, P(culog2), P(modes), bd, None) == 0
+                                                  P(d[4]), P(coef), 5, P(culog2), P(modes), None, bd, None) == 0
     cbf = torch.zeros(t * 4, dtype=torch.uint8, device="cuda")
     assert pkg.hip.bc250_hevc_cu_cbf_ctu(P(coef), wc, hc, P(culog2), P(cbf), None) == 0
     torch.cuda.synchronize()
