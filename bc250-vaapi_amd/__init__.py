@@ -162,6 +162,8 @@ _sig(hip, "bc250_hevc_launch_deblock_ctu", _I, [_P, _P, _I, _I, _I, _I, _I, _I, 
 _sig(hip, "bc250_hevc_cu_cbf_ctu", _I, [_P, _I, _I, _P, _P, _P])
 _sig(hip, "bc250_hevc_launch_p_encode_ctu", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P,
                                                  _P, _I, _P, _P, _P, _I, _P])
+_sig(hip, "bc250_hevc_launch_refine_p_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I,
+                                                    _P, _P, _P, _I, _P, _P, _P, _I, _P])
 _sig(hip, "bc250_hevc_launch_i_encode_ctu", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P,
                                                  _U, _I, _I, _P])
 _sig(hip, "bc250_hevc_cu_cbf", _I, [_P, _I, _P, _P])

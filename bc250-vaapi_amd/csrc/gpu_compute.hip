@@ -554,8 +554,8 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
                                            s->imv8, st);
     }
     hipEventRecord(sl.ev[1], st);
-    if (r == 0 && !is_intra)
-        r = bc250_hevc_launch_refine8(cur.y, cur.py, ref.y, ref.py, s->wc, s->hc, s->imv8, s->mv8, bd, st);
+    // P pictures: the quarter-sample refinement runs fused with the CU coding
+    // below (bc250_hevc_launch_refine_p_encode), so it is timed with the coding
     hipEventRecord(sl.ev[2], st);
     if (r == 0) {
         if (is_intra) {
@@ -571,11 +571,11 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
             // P modes: zero, or at CTU32 the log2 CU size of each CU8 at [16 t + z];
             // at CTU16 the coding kernel also writes the coded-luma flags
             const bool c32 = p.hevc_ctu_log2 == 5;
-            r = bc250_hevc_launch_p_encode_ctu(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv, out.y,
-                                               out.uv, out.py, out.puv, s->wc, s->hc, qp, 2 /* inter rounding */,
-                                               s->mv8, (int16_t *)s->levels, p.hevc_ctu_log2,
-                                               c32 ? s->hculog2 : nullptr, s->hmodes, c32 ? nullptr : s->hcbf, bd,
-                                               st);
+            r = bc250_hevc_launch_refine_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv,
+                                                  out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp,
+                                                  2 /* inter rounding */, s->imv8, s->mv8, (int16_t *)s->levels,
+                                                  p.hevc_ctu_log2, c32 ? s->hculog2 : nullptr, s->hmodes,
+                                                  c32 ? nullptr : s->hcbf, bd, st);
         }
     }
     hipEventRecord(sl.ev[3], st);

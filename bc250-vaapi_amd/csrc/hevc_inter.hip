@@ -252,21 +252,13 @@ struct RefineLds8 {
     int h[3][16 * 9];    // fx = 1..3: horizontal sums >> (BD - 8) at columns 3..11
 };
 
+// Refinement of the CU8 at (x0, y0) from its integer vector iv by one wave:
+// returns {mvx, mvy (quarter samples), sad, 0}.
 template <typename Pix, int BD>
-__global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__restrict__ cur, int cpitch,
-                                                           const uint8_t *__restrict__ ref, int rpitch, int wc, int hc,
-                                                           int nctu, const uint4 *__restrict__ imv8,
-                                                           uint4 *__restrict__ out)
+BC_DEV uint4 refine_cu(RefineLds8 &L, const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref,
+                       int rpitch, int wc, int hc, int x0, int y0, uint4 iv, int lane)
 {
     constexpr int DOWN = BD - 8, MAXV = (1 << BD) - 1, SH = 14 - BD, OFF = 1 << (13 - BD);
-    __shared__ RefineLds8 Ls[4];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ctu = xcd_item(blockIdx.x, nctu);
-    if (ctu >= nctu) return;
-    RefineLds8 &L = Ls[wv];
-    const int cu = ctu * 4 + wv, mbw = wc / 16;
-    const int x0 = (ctu % mbw) * 16 + (wv & 1) * 8, y0 = (ctu / mbw) * 16 + (wv >> 1) * 8;
-    const uint4 iv = imv8[cu];
     const int dx = (int)iv.x, dy = (int)iv.y;
     const int ox = x0 + dx - 4, oy = y0 + dy - 4;
     for (int t = lane; t < 256; t += 64)
@@ -335,7 +327,23 @@ __global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__rest
             }
         }
     }
-    if (lane == 0) out[cu] = make_uint4((uint32_t)bx, (uint32_t)by, best, 0u);
+    return make_uint4((uint32_t)bx, (uint32_t)by, best, 0u);
+}
+
+template <typename Pix, int BD>
+__global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__restrict__ cur, int cpitch,
+                                                           const uint8_t *__restrict__ ref, int rpitch, int wc, int hc,
+                                                           int nctu, const uint4 *__restrict__ imv8,
+                                                           uint4 *__restrict__ out)
+{
+    __shared__ RefineLds8 Ls[4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ctu = xcd_item(blockIdx.x, nctu);
+    if (ctu >= nctu) return;
+    const int cu = ctu * 4 + wv, mbw = wc / 16;
+    const int x0 = (ctu % mbw) * 16 + (wv & 1) * 8, y0 = (ctu / mbw) * 16 + (wv >> 1) * 8;
+    const uint4 v = refine_cu<Pix, BD>(Ls[wv], cur, cpitch, ref, rpitch, wc, hc, x0, y0, imv8[cu], lane);
+    if (lane == 0) out[cu] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -348,25 +356,19 @@ struct PcuLds {
     int ca[32], cb[32];
 };
 
+// Coding of CU8 wv of CTU ctu (origin (x0, y0)) with vector (mvx, mvy) by one
+// wave.
 template <typename Pix, int BD>
-__global__ __launch_bounds__(256) void hevc_p_encode_kernel(
-    const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, const uint8_t *__restrict__ ry,
-    const uint8_t *__restrict__ ruv, int rpy, int rpuv, uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy,
-    int opuv, int wc, int hc, int nctu, int qp, int round_q12, const uint4 *__restrict__ mvs,
-    int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes)
+BC_DEV void encode_cu(PcuLds &L, const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv,
+                      const uint8_t *__restrict__ ry, const uint8_t *__restrict__ ruv, int rpy, int rpuv,
+                      uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy, int opuv, int wc, int hc, int qp,
+                      int round_q12, int ctu, int wv, int lane, int x0, int y0, int mvx, int mvy,
+                      int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes)
 {
     constexpr int MAXV = (1 << BD) - 1, QOFF = 6 * (BD - 8);
     constexpr int S1_8 = BD - 6, QSH8 = BD - 2, S1_4 = BD == 8 ? 1 : 3, QSH4 = BD - 3, S2INV = 20 - BD;
-    __shared__ PcuLds Ls[4];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ctu = xcd_item(blockIdx.x, nctu);
-    if (ctu >= nctu) return;
-    PcuLds &L = Ls[wv];
-    const int cu = ctu * 4 + wv, mbw = wc / 16;
-    const int x0 = (ctu % mbw) * 16 + (wv & 1) * 8, y0 = (ctu / mbw) * 16 + (wv >> 1) * 8;
+    const int cu = ctu * 4 + wv;
     const int ux0 = x0 >> 1, uy0 = y0 >> 1, wcc = wc >> 1, hcc = hc >> 1;
-    const uint4 mv = mvs[cu];
-    const int mvx = (int)mv.x, mvy = (int)mv.y;
     const int lx = x0 + (mvx >> 2) - 3, ly = y0 + (mvy >> 2) - 3;
     const int cx = ux0 + (mvx >> 3) - 1, cyy = uy0 + (mvy >> 3) - 1;
     for (int t = lane; t < 225; t += 64)
@@ -422,6 +424,57 @@ __global__ __launch_bounds__(256) void hevc_p_encode_kernel(
         const int crr = pass_rows<4>(kDct4, true, &L.cb[comp * 16], cr_, cc_, S2INV);
         stpix<Pix>(ouv + (size_t)(uy0 + cr_) * opuv, 2 * (ux0 + cc_) + comp, clampi(cpred + crr, 0, MAXV));
     }
+}
+
+template <typename Pix, int BD>
+__global__ __launch_bounds__(256) void hevc_p_encode_kernel(
+    const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, const uint8_t *__restrict__ ry,
+    const uint8_t *__restrict__ ruv, int rpy, int rpuv, uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy,
+    int opuv, int wc, int hc, int nctu, int qp, int round_q12, const uint4 *__restrict__ mvs,
+    int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes)
+{
+    __shared__ PcuLds Ls[4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ctu = xcd_item(blockIdx.x, nctu);
+    if (ctu >= nctu) return;
+    const int mbw = wc / 16;
+    const int x0 = (ctu % mbw) * 16 + (wv & 1) * 8, y0 = (ctu / mbw) * 16 + (wv >> 1) * 8;
+    const uint4 mv = mvs[ctu * 4 + wv];
+    encode_cu<Pix, BD>(Ls[wv], cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12, ctu,
+                       wv, lane, x0, y0, (int)mv.x, (int)mv.y, coef, cbf, modes);
+}
+
+// Refinement and coding of each CU8 in one launch: the wave that refines a
+// CU's vector codes the CU with it (the same work and results as
+// hevc_refine8_kernel followed by hevc_p_encode_kernel).
+// At least 6 waves per SIMD: the refinement half needs the occupancy that
+// the coding half's registers would otherwise take (measured: 1080p P coding
+// 0.137 -> 0.128 ms per frame; 8 waves spill more and lose it again).
+#ifndef BC250_RPE_WAVES_PER_EU
+#define BC250_RPE_WAVES_PER_EU 6
+#endif
+template <typename Pix, int BD>
+__global__ __launch_bounds__(256, BC250_RPE_WAVES_PER_EU) void hevc_refine_p_encode_kernel(
+    const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, const uint8_t *__restrict__ ry,
+    const uint8_t *__restrict__ ruv, int rpy, int rpuv, uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy,
+    int opuv, int wc, int hc, int nctu, int qp, int round_q12, const uint4 *__restrict__ imv8,
+    uint4 *__restrict__ mv8, int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes)
+{
+    union CuLds {
+        RefineLds8 r;
+        PcuLds p;
+    };
+    __shared__ CuLds Ls[4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ctu = xcd_item(blockIdx.x, nctu);
+    if (ctu >= nctu) return;
+    const int cu = ctu * 4 + wv, mbw = wc / 16;
+    const int x0 = (ctu % mbw) * 16 + (wv & 1) * 8, y0 = (ctu / mbw) * 16 + (wv >> 1) * 8;
+    const uint4 v = refine_cu<Pix, BD>(Ls[wv].r, cy, cpy, ry, rpy, wc, hc, x0, y0, imv8[cu], lane);
+    if (lane == 0) mv8[cu] = v;
+    wave_lds_sync();  // the refinement's LDS is the coding's from here on
+    encode_cu<Pix, BD>(Ls[wv].p, cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12, ctu,
+                       wv, lane, x0, y0, (int)v.x, (int)v.y, coef, cbf, modes);
 }
 
 // ---------------------------------------------------------------------------
@@ -715,6 +768,25 @@ extern "C" int bc250_hevc_launch_refine8(const void *cur, int cpitch, const void
     return launch_ok();
 }
 
+// The CTU32 large-CU pass (hevc_p_large_kernel) over the refined vectors.
+static int launch_large(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry, const void *ruv, int rpy,
+                        int rpuv, void *oy, void *ouv, int opy, int opuv, int wc, int hc, int qp, int round_q12,
+                        const uint32_t *mvs, int16_t *coef, uint8_t *culog2, uint8_t *modes, int bd, hipStream_t st)
+{
+    const int wtu = wc / 16, htu = hc / 16, n32 = ((wtu + 1) / 2) * ((htu + 1) / 2);
+    auto *C = (const uint8_t *)cy, *CU = (const uint8_t *)cuv, *R = (const uint8_t *)ry, *RU = (const uint8_t *)ruv;
+    auto *O = (uint8_t *)oy, *OU = (uint8_t *)ouv;
+    if (bd == 8)
+        hevc_p_large_kernel<uint8_t, 8><<<n32, 256, 0, st>>>(C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv, wc,
+                                                             hc, qp, round_q12, (const uint4 *)mvs, coef, culog2,
+                                                             modes);
+    else
+        hevc_p_large_kernel<uint16_t, 10><<<n32, 256, 0, st>>>(C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv,
+                                                               wc, hc, qp, round_q12, (const uint4 *)mvs, coef,
+                                                               culog2, modes);
+    return launch_ok();
+}
+
 static int launch_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry, const void *ruv,
                            int rpy, int rpuv, void *oy, void *ouv, int opy, int opuv, int wc, int hc, int qp,
                            int round_q12, const uint32_t *mvs, int16_t *coef, uint8_t *cbf, uint8_t *modes, int bd,
@@ -765,16 +837,40 @@ extern "C" int bc250_hevc_launch_p_encode_ctu(const void *cy, const void *cuv, i
         if (culog2) fill_u8_kernel<<<(ncu + 255) / 256, 256, 0, st>>>(culog2, ncu, 3);
         return launch_ok();
     }
-    const int n32 = ((wtu + 1) / 2) * ((htu + 1) / 2);
+    return launch_large(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12, mvs, coef,
+                        culog2, modes, bd, st);
+}
+
+extern "C" int bc250_hevc_launch_refine_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
+                                                 const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy,
+                                                 int opuv, int wc, int hc, int qp, int round_q12,
+                                                 const uint32_t *imv8, uint32_t *mv8, int16_t *coef, int ctu_log2,
+                                                 uint8_t *culog2, uint8_t *modes, uint8_t *cbf, int bd, void *stream)
+{
+    const int b = bd == 8 ? 1 : 2, mw = wc * b;
+    if (!bd_ok(bd) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || round_q12 < 0 ||
+        round_q12 > 12 || cpy < mw || cpuv < mw || rpy < mw || rpuv < mw || opy < mw || opuv < mw ||
+        ((cpy | cpuv | rpy | rpuv | opy | opuv) % b) || (ctu_log2 != 4 && ctu_log2 != 5) ||
+        (ctu_log2 == 5 && !culog2) || !imv8 || !mv8)
+        return -1;
+    const int nctu = (wc / 16) * (hc / 16);
+    hipStream_t st = (hipStream_t)stream;
     auto *C = (const uint8_t *)cy, *CU = (const uint8_t *)cuv, *R = (const uint8_t *)ry, *RU = (const uint8_t *)ruv;
     auto *O = (uint8_t *)oy, *OU = (uint8_t *)ouv;
     if (bd == 8)
-        hevc_p_large_kernel<uint8_t, 8><<<n32, 256, 0, st>>>(C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv, wc,
-                                                             hc, qp, round_q12, (const uint4 *)mvs, coef, culog2,
-                                                             modes);
+        hevc_refine_p_encode_kernel<uint8_t, 8><<<grid_xcd(nctu), 256, 0, st>>>(
+            C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv, wc, hc, nctu, qp, round_q12, (const uint4 *)imv8,
+            (uint4 *)mv8, coef, cbf, modes);
     else
-        hevc_p_large_kernel<uint16_t, 10><<<n32, 256, 0, st>>>(C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv,
-                                                               wc, hc, qp, round_q12, (const uint4 *)mvs, coef,
-                                                               culog2, modes);
-    return launch_ok();
+        hevc_refine_p_encode_kernel<uint16_t, 10><<<grid_xcd(nctu), 256, 0, st>>>(
+            C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv, wc, hc, nctu, qp, round_q12, (const uint4 *)imv8,
+            (uint4 *)mv8, coef, cbf, modes);
+    if (launch_ok() != 0) return -1;
+    const int ncu = nctu * 4;
+    if (ctu_log2 == 4) {
+        if (culog2) fill_u8_kernel<<<(ncu + 255) / 256, 256, 0, st>>>(culog2, ncu, 3);
+        return launch_ok();
+    }
+    return launch_large(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12, mv8, coef,
+                        culog2, modes, bd, st);
 }

@@ -127,6 +127,17 @@ int bc250_hevc_launch_p_encode_ctu(const void *cy, const void *cuv, int cpy, int
                                    int hc, int qp, int round_q12, const uint32_t *mvs, int16_t *coef, int ctu_log2,
                                    uint8_t *culog2, uint8_t *modes, uint8_t *cbf, int bd, void *stream);
 
+/* bc250_hevc_launch_refine8 and bc250_hevc_launch_p_encode_ctu in one pass
+ * (identical results): the wave that refines a CU8's vector (imv8 -> mv8,
+ * the refine8 records) codes the CU with it, on the same planes (cy / ry at
+ * the full bit depth).  No reference counterpart: the fusion saves the
+ * launch and the vector round trip between the two stages. */
+int bc250_hevc_launch_refine_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
+                                      const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy, int opuv,
+                                      int wc, int hc, int qp, int round_q12, const uint32_t *imv8, uint32_t *mv8,
+                                      int16_t *coef, int ctu_log2, uint8_t *culog2, uint8_t *modes, uint8_t *cbf,
+                                      int bd, void *stream);
+
 /* I picture: CTU16 -> 4 CU8 -> 4 PU4 intra (Planar/DC/H/V by SAD, DST),
  * chroma DC per CU (DCT, QpC), slices as CTU-row ranges
  * [floor(rows*s/ns), floor(rows*(s+1)/ns)), QP' = qp + 6 (bd - 8) (the

@@ -204,6 +204,58 @@ def test_p_encode_stage(pkg, qp, bd):
     assert (want["coef"] != 0).any()
 
 
+@pytest.mark.parametrize("w,h,qp,bd,ctu", [(128, 96, 27, 8, 4), (112, 80, 37, 8, 5), (144, 96, 32, 10, 4),
+                                            (112, 80, 22, 10, 5)])
+def test_refine_p_encode_fused_equals_stages(pkg, w, h, qp, bd, ctu):
+    """The fused refinement + CU coding equals refine8 followed by
+    p_encode_ctu on the same inputs: vectors, levels, reconstruction, modes,
+    CU sizes and coded-luma flags."""
+    torch = _torch()
+    say(f"hevc refine+P fused {w}x{h} qp={qp} bd={bd} ctu_log2={ctu}")
+    (cy, cuv), (ry, ruv) = _frames(pkg, 301 + qp, w, h, 2, 4)
+    hc, wc = cy.shape
+    t = (wc // 16) * (hc // 16)
+    _, i8 = orc.me_fullsearch(ry, cy, 16, 1)
+    if bd == 10:
+        rs = np.random.default_rng(qp)
+        cy, cuv, ry, ruv = (_ten(a, rs) for a in (cy, cuv, ry, ruv))
+        d = [_p010(torch, a) for a in (cy, cuv, ry, ruv)]
+        pitch = wc * 2
+    else:
+        d = [dev(torch, a) for a in (cy, cuv, ry, ruv)]
+        pitch = wc
+    di8 = dev(torch, i8.view(np.int32))
+    outs = []
+    for fused in (False, True):
+        oy, ouv = torch.zeros_like(d[0]), torch.zeros_like(d[1])
+        mv8 = torch.zeros(t * 16, dtype=torch.int32, device="cuda")
+        coef = torch.zeros(t * 384, dtype=torch.int16, device="cuda")
+        cul = torch.zeros(t * 4, dtype=torch.uint8, device="cuda")
+        modes = torch.full((t * 16,), 9, dtype=torch.uint8, device="cuda")
+        cbf = torch.full((t * 4,), 7, dtype=torch.uint8, device="cuda")
+        cbfp = None if ctu == 5 else P(cbf)
+        if fused:
+            assert pkg.hip.bc250_hevc_launch_refine_p_encode(P(d[0]), P(d[1]), pitch, pitch, P(d[2]), P(d[3]), pitch,
+                                                             pitch, P(oy), P(ouv), pitch, pitch, wc, hc, qp,
+                                                             orc.HEVC_ROUND_INTER, P(di8), P(mv8), P(coef), ctu,
+                                                             P(cul), P(modes), cbfp, bd, None) == 0
+        else:
+            assert pkg.hip.bc250_hevc_launch_refine8(P(d[0]), pitch, P(d[2]), pitch, wc, hc, P(di8), P(mv8), bd,
+                                                     None) == 0
+            assert pkg.hip.bc250_hevc_launch_p_encode_ctu(P(d[0]), P(d[1]), pitch, pitch, P(d[2]), P(d[3]), pitch,
+                                                          pitch, P(oy), P(ouv), pitch, pitch, wc, hc, qp,
+                                                          orc.HEVC_ROUND_INTER, P(mv8), P(coef), ctu, P(cul),
+                                                          P(modes), cbfp, bd, None) == 0
+        torch.cuda.synchronize()
+        outs.append([host(x) for x in (mv8, coef, oy, ouv, cul, modes, cbf)])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    assert not outs[1][5].reshape(t, 16)[:, 4:].any()
+    if ctu == 4:
+        want = orc.hevc_cu_cbf(outs[1][1])
+        assert np.array_equal(outs[1][6], want) and want.any()
+
+
 @pytest.mark.parametrize("w,h,qp,bd", [(112, 80, 22, 8), (256, 160, 37, 8), (112, 80, 27, 10), (192, 128, 4, 10)])
 def test_p_encode_ctu32_stage(pkg, w, h, qp, bd):
     """CTU32 P coding: CU16 / CU32 where the CU8 vectors agree (whole CTU32s,
