@@ -286,17 +286,21 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
         for (int k = 0; k < 16; k++) {
             const int cu = k >> 2, pu = k & 3;
             const bool active = luma || (grp < 3 && pu == 0);
-            const int bx = luma ? X0 + (cu & 1) * 8 + (pu & 1) * 4 : ctx0 + (cu & 1) * 4;
-            const int by = luma ? Y0r + (cu >> 1) * 8 + (pu >> 1) * 4 : cty0 + (cu >> 1) * 4;
-            // ---- availability (z-scan inside the CTU, slices and edges outside)
-            const int rx = bx - ctx0, ry = by - cty0, mz = morton4(rx, ry);
-            const bool leftC = x > 0, aboveC = need_above, aboveR = need_above && x + 1 < wtu && !br;
-            const bool aL = rx > 0 || leftC;
-            const bool aT = ry > 0 || aboveC;
-            const bool aC = rx > 0 ? (ry > 0 || aboveC) : (ry > 0 ? leftC : (leftC && aboveC));
-            const bool aTR = ry == 0 ? (rx + 4 < cs ? aboveC : aboveR) : (rx + 4 < cs && morton4(rx + 4, ry - 4) < mz);
-            const bool blx = tl_next && rx == 0 && ry + 4 == cs;  // from the lower row's last quadrant
-            const bool aBL = (ry + 4 < cs && (rx == 0 ? leftC : morton4(rx - 4, ry + 4) < mz)) || blx;
+            // ---- availability (z-scan inside the CTU, slices and edges outside),
+            // written with bitwise logic and selects so that it stays branch-free
+            const int rx = luma ? (cu & 1) * 8 + (pu & 1) * 4 : (cu & 1) * 4;
+            const int ry = luma ? (cu >> 1) * 8 + (pu >> 1) * 4 : (cu >> 1) * 4;
+            const int bx = ctx0 + rx, by = cty0 + ry, mz = morton4(rx, ry);
+            const bool leftC = x > 0, aboveC = need_above, aboveR = need_above & (x + 1 < wtu) & !br;
+            const bool in_r = rx + 4 < cs, in_b = ry + 4 < cs;
+            // (the Morton codes of out-of-CTU positions are computed but never selected)
+            const bool mtr = morton4(rx + 4, ry - 4) < mz, mbl = morton4(rx - 4, ry + 4) < mz;
+            const bool aL = (rx > 0) | leftC;
+            const bool aT = (ry > 0) | aboveC;
+            const bool aC = rx > 0 ? ((ry > 0) | aboveC) : (ry > 0 ? leftC : (leftC & aboveC));
+            const bool aTR = ry == 0 ? (in_r ? aboveC : aboveR) : (in_r & mtr);
+            const bool blx = tl_next & (rx == 0) & (ry + 4 == cs);  // from the lower row's last quadrant
+            const bool aBL = (in_b & (rx == 0 ? leftC : mbl)) | blx;
             // ---- neighbours with the substitution scan
             int v[10];
             bool a[10];
@@ -311,7 +315,8 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             v[4] = sample(rx - 1, ry - 1);
             a[9] = aTR;
             v[9] = sample(rx + 4, ry - 1);
-            const int vbl = blx ? blv : sample(rx - 1, ry + 4 < cs ? ry + 4 : ry + 3);
+            const int vbs = sample(rx - 1, in_b ? ry + 4 : ry + 3);
+            const int vbl = blx ? blv : vbs;
             // first available in scan order fills what precedes it; later gaps copy the previous
             int firstv = 1 << (BD - 1);
 #pragma unroll
@@ -332,22 +337,19 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
             const int p_pl =
                 (__mul24(3 - pj, Li) + __mul24(pj + 1, T4) + __mul24(3 - pi, Tj) + __mul24(pi + 1, L4) + 4) >> 3;
             const int dc = (L0 + L1 + L2 + L3 + T0 + T1 + T2 + T3 + 4) >> 3;
-            int p_dc = dc;
-            if (luma) {
-                if (pi == 0 && pj == 0) p_dc = (L0 + 2 * dc + T0 + 2) >> 2;
-                else if (pi == 0) p_dc = (Tj + 3 * dc + 2) >> 2;
-                else if (pj == 0) p_dc = (Li + 3 * dc + 2) >> 2;
-            }
+            // luma DC edge filter: corner, top row, left column
+            const int dc_c = (L0 + 2 * dc + T0 + 2) >> 2, dc_t = (Tj + 3 * dc + 2) >> 2;
+            const int dc_l = (Li + 3 * dc + 2) >> 2;
+            const int dcf = pi == 0 ? (pj == 0 ? dc_c : dc_t) : (pj == 0 ? dc_l : dc);
+            const int p_dc = luma ? dcf : dc;
             const int p_h = (luma && pi == 0) ? clampi(L0 + ((Tj - Cc) >> 1), 0, MAXV) : Li;
             const int p_v = (luma && pj == 0) ? clampi(T0 + ((Li - Cc) >> 1), 0, MAXV) : Tj;
             const int s = B[srco + (by - cty0 + pi) * cs + (bx - ctx0 + pj)];
             // ---- SAD decision over the 16 lanes of the group (luma only)
-            uint32_t sp = (uint32_t)iabs(s - p_pl), sd = (uint32_t)iabs(s - p_dc);
-            uint32_t sh = (uint32_t)iabs(s - p_h), sv = (uint32_t)iabs(s - p_v);
-            sp = row16_sum(sp);
-            sd = row16_sum(sd);
-            sh = row16_sum(sh);
-            sv = row16_sum(sv);
+            // two SADs per word: 16 samples of at most 1023 stay below 2^16
+            const uint32_t spd = row16_sum((uint32_t)iabs(s - p_pl) | (uint32_t)iabs(s - p_dc) << 16);
+            const uint32_t shv = row16_sum((uint32_t)iabs(s - p_h) | (uint32_t)iabs(s - p_v) << 16);
+            const uint32_t sp = spd & 0xffffu, sd = spd >> 16, sh = shv & 0xffffu, sv = shv >> 16;
             int mode = 0, pred = p_pl;
             uint32_t best = sp;
             if (sd < best) { best = sd; mode = 1; pred = p_dc; }
