@@ -119,6 +119,19 @@ void orc_h264_encode_frame(const uint8_t *cy, const uint8_t *cuv,
                            int16_t *levels, int32_t *dc, uint32_t *nz,
                            uint32_t *modes, uint8_t *oy, uint8_t *ouv);
 
+/* H.264 CAVLC (orc_cavlc.c): one residual block (raster coefficients,
+ * zig-zag scanned from position `ac`), a chroma DC block, Exp-Golomb codes,
+ * the two motion vector predictors and the whole-frame slice writer. */
+int  orc_cavlc_block(const int16_t coef[16], int ac, int nc, uint8_t *buf, int cap, int *nbits);
+int  orc_cavlc_chroma_dc(const int32_t c[4], uint8_t *buf, int cap, int *nbits);
+int  orc_exp_golomb(int value, int is_signed, uint8_t *buf, int cap, int *nbits);
+int  orc_h264_mb_header(int kind, int a, int b, int c, int d, uint8_t *buf, int cap, int *nbits);
+void orc_h264_mv_pred(const orc_mv_t *mv, int wmb, int start, int mb, int *px, int *py);
+void orc_h264_skip_mv(const orc_mv_t *mv, int wmb, int start, int mb, int *px, int *py);
+int64_t orc_h264_cavlc_slices(const int16_t *levels, const int32_t *dc, const uint32_t *modes,
+                              const orc_mv_t *mv, const int32_t *cfg, uint8_t *out, int64_t cap, int64_t *off,
+                              int64_t *len);
+
 /* --- HEVC stages (orc_hevc.c) --------------------------------------------- */
 int  orc_hevc_chroma_qp(int qp_luma);
 void orc_hevc_quant(const int32_t *v, int n, int shift, int q, int round_q12, int16_t *out);

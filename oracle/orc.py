@@ -195,6 +195,92 @@ def encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lambda8=1
     return o
 
 
+# ------------------------------------------------------------ H.264 CAVLC
+REF_CAVLC = os.path.join(_HERE, "_ref", "libref_cavlc.so")
+_L64 = C.c_int64
+_sig("orc_cavlc_block", _I, [_P, _I, _I, _P, _I, _P])
+_sig("orc_cavlc_chroma_dc", _I, [_P, _P, _I, _P])
+_sig("orc_exp_golomb", _I, [_I, _I, _P, _I, _P])
+_sig("orc_h264_mb_header", _I, [_I, _I, _I, _I, _I, _P, _I, _P])
+_sig("orc_h264_mv_pred", None, [_P, _I, _I, _I, _P, _P])
+_sig("orc_h264_skip_mv", None, [_P, _I, _I, _I, _P, _P])
+_sig("orc_h264_cavlc_slices", _L64, [_P, _P, _P, _P, _P, _P, _L64, _P, _P])
+
+# slice configuration, in the order of bc250_h264_slice_cfg_t
+SLICE_CFG = ("width_in_mbs", "height_in_mbs", "qp", "is_idr", "num_slices", "pps_id", "pic_init_qp_minus26",
+             "frame_num", "log2_max_frame_num_minus4", "idr_pic_id", "poc", "log2_max_poc_lsb_minus4",
+             "deblock_idc", "relax_chroma_skip")
+
+
+def slice_cfg(wmb, hmb, qp, is_idr, num_slices=1, pps_id=0, pic_init_qp_minus26=0, frame_num=0,
+              log2_max_frame_num_minus4=0, idr_pic_id=0, poc=0, log2_max_poc_lsb_minus4=0, deblock_idc=0,
+              relax_chroma_skip=0):
+    """The slice configuration as the int32 array both writers take."""
+    v = dict(locals())
+    v["width_in_mbs"], v["height_in_mbs"] = wmb, hmb
+    return np.array([int(v[k]) for k in SLICE_CFG], np.int32)
+
+
+def _bits(buf, n):
+    return np.unpackbits(np.frombuffer(bytes(buf), np.uint8))[:n]
+
+
+def cavlc_block(coef, ac, nc):
+    """One 4x4 residual block (raster coefficients, ac = 1 skips the DC
+    position) -> (bit array, TotalCoeff)."""
+    c = np.ascontiguousarray(coef, np.int16).reshape(16)
+    buf, nb = np.zeros(256, np.uint8), C.c_int(0)
+    tc = lib.orc_cavlc_block(p(c), int(ac), int(nc), p(buf), 256, C.byref(nb))
+    return _bits(buf, nb.value), tc
+
+
+def cavlc_chroma_dc(c4):
+    c = c32(c4).reshape(4)
+    buf, nb = np.zeros(64, np.uint8), C.c_int(0)
+    tc = lib.orc_cavlc_chroma_dc(p(c), p(buf), 64, C.byref(nb))
+    return _bits(buf, nb.value), tc
+
+
+def exp_golomb(value, signed):
+    buf, nb = np.zeros(16, np.uint8), C.c_int(0)
+    lib.orc_exp_golomb(int(value), int(signed), p(buf), 16, C.byref(nb))
+    return _bits(buf, nb.value)
+
+
+def mb_header(kind, a, b, c, d):
+    """kind 0: I16x16 header (pred mode, chroma mode, cbp chroma, cbp luma),
+    1: P16x16 header (mvd x, mvd y, cbp, qp delta), 2: mb_skip_run a."""
+    buf, nb = np.zeros(32, np.uint8), C.c_int(0)
+    lib.orc_h264_mb_header(kind, a, b, c, d, p(buf), 32, C.byref(nb))
+    return _bits(buf, nb.value)
+
+
+def h264_mv_pred(mvs, wmb, start, mb, skip=False):
+    m = np.ascontiguousarray(mvs, MV)
+    x, y = C.c_int(0), C.c_int(0)
+    (lib.orc_h264_skip_mv if skip else lib.orc_h264_mv_pred)(p(m), wmb, start, mb, C.byref(x), C.byref(y))
+    return x.value, y.value
+
+
+def h264_cavlc_slices(levels, dc, modes, mvs, cfg):
+    """The frame's slice RBSPs (list of bytes) written by the CAVLC oracle.
+    levels [nmb, 24, 16] int16, dc [nmb, 24] int32, modes [nmb] uint32,
+    mvs [nmb] MV (ignored for IDR frames), cfg from slice_cfg()."""
+    cfg = np.ascontiguousarray(cfg, np.int32)
+    nmb, ns = int(cfg[0]) * int(cfg[1]), int(cfg[4])
+    lv = np.ascontiguousarray(levels, np.int16).reshape(nmb, 24, 16)
+    d = c32(dc).reshape(nmb, 24)
+    md = np.ascontiguousarray(modes, np.uint32).reshape(nmb)
+    mv = np.ascontiguousarray(mvs if mvs is not None else np.zeros(nmb, MV), MV).reshape(nmb)
+    cap = nmb * 2560 + 4096 * ns
+    out = np.zeros(cap, np.uint8)
+    off, ln = np.zeros(ns, np.int64), np.zeros(ns, np.int64)
+    r = lib.orc_h264_cavlc_slices(p(lv), p(d), p(md), p(mv), p(cfg), p(out), cap, p(off), p(ln))
+    if r < 0:
+        raise RuntimeError("CAVLC oracle: output capacity exceeded")
+    return [out[o:o + n].tobytes() for o, n in zip(off, ln)]
+
+
 # ------------------------------------------------------------------- HEVC
 REF_HEVC = os.path.join(_HERE, "_ref", "libref_hevc.so")
 REF_HEVC_DEBLOCK = os.path.join(_HERE, "_ref", "libref_hevc_deblock.so")
