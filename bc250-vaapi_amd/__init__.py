@@ -169,6 +169,10 @@ _sig(hip, "bc250_hevc_launch_i_encode_ctu", _I, [_P, _P, _I, _I, _P, _P, _I, _I,
 _sig(hip, "bc250_hevc_cu_cbf", _I, [_P, _I, _P, _P])
 _sig(hip, "bc250_hevc_p010_to_8bit", _I, [_P, _I, _P, _I, _I, _I, _P])
 
+# H.264 CAVLC slices (include/bc250_bitstream.h)
+_sig(hip, "bc250_h264_cavlc_work_size", C.c_size_t, [_I, _I])
+_sig(hip, "bc250_h264_cavlc_slices", _I, [_P, _P, _P, _P, _P, _P, C.c_size_t, _P, _P, C.c_size_t, _P])
+
 _sig(host, "bc250_gen_create", _P, [C.c_uint64, _I, _I, _I])
 _sig(host, "bc250_gen_destroy", None, [_P])
 _sig(host, "bc250_gen_frame", _I, [_P, _I, _P, _I, _P, _I])
@@ -183,6 +187,77 @@ _sig(host, "h264_chroma_qp", _I, [_I])
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------- CAVLC
+
+# bc250_h264_slice_cfg_t, field by field (all int32)
+SLICE_CFG_FIELDS = ("width_in_mbs", "height_in_mbs", "qp", "is_idr", "num_slices", "pps_id", "pic_init_qp_minus26",
+                    "frame_num", "log2_max_frame_num_minus4", "idr_pic_id", "poc", "log2_max_poc_lsb_minus4",
+                    "deblock_idc", "relax_chroma_skip")
+MAX_SLICES = 64
+
+
+def h264_slice_cfg(width_in_mbs, height_in_mbs, qp, is_idr, num_slices=1, **kw):
+    """A bc250_h264_slice_cfg_t as an int32 array; other fields by keyword
+    (default 0)."""
+    v = dict(width_in_mbs=width_in_mbs, height_in_mbs=height_in_mbs, qp=qp, is_idr=int(is_idr),
+             num_slices=num_slices)
+    unknown = set(kw) - set(SLICE_CFG_FIELDS)
+    if unknown:
+        raise TypeError(f"unknown slice fields {sorted(unknown)}")
+    v.update(kw)
+    return np.array([int(v.get(k, 0)) for k in SLICE_CFG_FIELDS], np.int32)
+
+
+class CavlcWriter:
+    """The GPU H.264 CAVLC slice writer (bc250_h264_cavlc_slices) for frames
+    of one size on the current device; owns its work and output buffers.
+    Inputs are device tensors in the staging layout of include/gpu_compute.h:
+    levels int16 [nmb*384], dc int32 [nmb*24], modes int32 [nmb], mvs int32
+    [nmb*4] (gpu_mv_t)."""
+
+    def __init__(self, width_in_mbs, height_in_mbs, cap_bytes=None):
+        import torch
+        self.torch = torch
+        self.wmb, self.hmb = width_in_mbs, height_in_mbs
+        nmb = width_in_mbs * height_in_mbs
+        self.work = torch.empty(int(hip.bc250_h264_cavlc_work_size(width_in_mbs, height_in_mbs)),
+                                dtype=torch.uint8, device="cuda")
+        cap = cap_bytes if cap_bytes is not None else nmb * 2560 + 4096 * MAX_SLICES
+        self.cap = (cap + 3) // 4 * 4
+        self.out = torch.empty(self.cap // 4, dtype=torch.int32, device="cuda")
+        self.info = torch.zeros(4 * MAX_SLICES + 2, dtype=torch.int32, device="cuda")
+
+    def launch(self, levels, dc, modes, mvs, cfg, stream=None):
+        """Enqueues the frame's slices on `stream` (a torch stream, or the
+        current one); raises on invalid arguments."""
+        cfg = np.ascontiguousarray(cfg, np.int32)
+        if cfg[0] != self.wmb or cfg[1] != self.hmb:
+            raise ValueError("cfg size differs from the writer's")
+        st = (stream or self.torch.cuda.current_stream()).cuda_stream
+
+        def ptr(t):
+            return None if t is None else C.c_void_p(t.data_ptr())
+        rc = hip.bc250_h264_cavlc_slices(ptr(levels), ptr(dc), ptr(modes), ptr(mvs), _ptr(cfg),
+                                         ptr(self.out), self.cap, ptr(self.info), ptr(self.work),
+                                         self.work.numel(), C.c_void_p(st))
+        if rc != 0:
+            raise ValueError("bc250_h264_cavlc_slices rejected its arguments")
+
+    def result(self, num_slices):
+        """Synchronizes and returns the slices' RBSPs (list of bytes)."""
+        self.torch.cuda.synchronize()
+        info = self.info.cpu().numpy().view(np.uint32)
+        if info[4 * num_slices + 1]:
+            raise RuntimeError(f"CAVLC output exceeded {self.cap} bytes")
+        total = int(info[4 * num_slices])
+        data = self.out.view(self.torch.uint8)[:total].cpu().numpy().tobytes()
+        return [data[info[4 * s]:info[4 * s] + info[4 * s + 1]] for s in range(num_slices)]
+
+    def slices(self, levels, dc, modes, mvs, cfg):
+        self.launch(levels, dc, modes, mvs, cfg)
+        return self.result(int(cfg[4]))
 
 
 # ---------------------------------------------------------------- frames

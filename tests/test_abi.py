@@ -44,7 +44,7 @@ def declared_functions(path):
 HEADERS = {
     "libbc250_hip.so": [os.path.join(ROOT, "include", h) for h in ("gpu_compute.h", "bc250_stages.h",
                                                                      "bc250_calib.h", "bc250_hevc.h",
-                                                                     "bc250_cpu_me.h")],
+                                                                     "bc250_cpu_me.h", "bc250_bitstream.h")],
     "libbc250_host.so": [os.path.join(PKG, "host", "bc250_host.h")],
 }
 

@@ -1,0 +1,184 @@
+"""The GPU H.264 CAVLC slice writer (include/bc250_bitstream.h) against the
+CPU oracle: every slice's RBSP byte for byte.  Synthetic staging data with
+escapes, all-skip and all-zero frames, up to 64 slices and one-MB-wide
+pictures; oracle-encoded frames; the real pipeline's 1080p staging, I then
+P; output bounds and argument checks."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import orc
+from conftest import have_gpu, load_pkg
+from test_oracle_cavlc import synth_frame
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    if not have_gpu():
+        pytest.skip("no GPU")
+    return load_pkg()
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _staging(torch, lv, dc, modes, mvs, n):
+    mv = np.zeros(n, orc.MV) if mvs is None else np.ascontiguousarray(mvs, orc.MV)
+    return (_dev(torch, np.ascontiguousarray(lv, np.int16).reshape(-1)),
+            _dev(torch, np.ascontiguousarray(dc, np.int32).reshape(-1)),
+            _dev(torch, np.ascontiguousarray(modes, np.uint32).view(np.int32)),
+            _dev(torch, mv.view(np.int32).reshape(-1)))
+
+
+def compare(pkg, lv, dc, modes, mvs, cfg, writer=None):
+    import torch
+    wmb, hmb, ns = int(cfg[0]), int(cfg[1]), int(cfg[4])
+    n = wmb * hmb
+    want = orc.h264_cavlc_slices(lv, dc, modes, mvs, cfg)
+    w = writer or pkg.CavlcWriter(wmb, hmb)
+    got = w.slices(*_staging(torch, lv, dc, modes, mvs, n), cfg)
+    assert len(got) == ns
+    for s in range(ns):
+        if got[s] != want[s]:
+            a, b = np.frombuffer(got[s], np.uint8), np.frombuffer(want[s], np.uint8)
+            k = min(len(a), len(b))
+            d = np.nonzero(a[:k] != b[:k])[0]
+            raise AssertionError(f"slice {s}: {len(a)} vs {len(b)} bytes, first difference at "
+                                 f"{d[0] if len(d) else k}")
+    return got
+
+
+@pytest.mark.parametrize("wmb,hmb,ns,idr", [(5, 4, 1, 1), (5, 4, 1, 0), (7, 3, 2, 1), (7, 3, 2, 0),
+                                            (6, 5, 3, 0), (9, 4, 4, 1), (8, 6, 5, 0), (4, 4, 16, 0),
+                                            (1, 7, 2, 0), (1, 5, 1, 1), (13, 9, 64, 0), (13, 9, 64, 1)])
+def test_synthetic_frames(pkg, wmb, hmb, ns, idr):
+    say(f"synthetic {wmb}x{hmb} MBs, {ns} slices, idr {idr}")
+    rs = np.random.default_rng(wmb * 100 + hmb * 10 + ns + idr)
+    lv, dc, modes, mvs = synth_frame(rs, wmb, hmb, idr)
+    cfg = orc.slice_cfg(wmb, hmb, 27, idr, ns, pps_id=1, pic_init_qp_minus26=-3, frame_num=5,
+                        log2_max_frame_num_minus4=2, idr_pic_id=3, poc=14, log2_max_poc_lsb_minus4=1,
+                        deblock_idc=ns % 3, relax_chroma_skip=ns & 1)
+    compare(pkg, lv, dc, modes, mvs, cfg)
+
+
+@pytest.mark.parametrize("qp,idr", [(0, 1), (12, 0), (22, 1), (37, 0), (51, 1)])
+def test_escapes_and_qps(pkg, qp, idr):
+    say(f"big levels qp {qp} idr {idr}")
+    rs = np.random.default_rng(qp + 100 * idr)
+    lv, dc, modes, mvs = synth_frame(rs, 11, 6, idr, density=0.6, big=True)
+    lv[3] = rs.integers(-32767, 32768, (24, 16))  # every coefficient of one MB at the int16 extremes
+    compare(pkg, lv, dc, modes, mvs, orc.slice_cfg(11, 6, qp, idr, 3))
+
+
+def test_all_skip_and_all_zero(pkg):
+    say("all-skip P frame, all-zero I frame")
+    wmb, hmb = 12, 7
+    n = wmb * hmb
+    z = np.zeros((n, 24, 16), np.int16)
+    compare(pkg, z, np.zeros((n, 24), np.int32), np.zeros(n, np.uint32), np.zeros(n, orc.MV),
+            orc.slice_cfg(wmb, hmb, 30, 0, 2))
+    compare(pkg, z, np.zeros((n, 24), np.int32), np.full(n, 15, np.uint32), None, orc.slice_cfg(wmb, hmb, 30, 1, 2))
+
+
+def test_oracle_encoded_frames_one_writer(pkg):
+    """I then two P frames of a moving picture through one writer (the
+    buffers carry the previous frame's bytes)."""
+    say("oracle-encoded 320x192 IPP")
+    rs = np.random.default_rng(23)
+    w, h = 320, 192
+    y = rs.integers(30, 220, (h, w)).astype(np.uint8)
+    y = ((y.astype(int) + np.roll(y, 1, 1) + np.roll(y, 1, 0) + np.roll(y, 1, (0, 1))) // 4).astype(np.uint8)
+    uv = rs.integers(90, 160, (h // 2, w)).astype(np.uint8)
+    writer = pkg.CavlcWriter(w // 16, h // 16)
+    f = orc.encode_frame(y, uv, y, uv, 24, True, 3)
+    compare(pkg, f["levels"], f["dc"], f["modes"], None, orc.slice_cfg(w // 16, h // 16, 24, 1, 3), writer)
+    ry, ruv = f["y"], f["uv"]
+    for k in (1, 2):
+        y2, uv2 = np.roll(y, (k, 2 * k), (0, 1)), np.roll(uv, (0, 2 * k), (0, 1))
+        f = orc.encode_frame(y2, uv2, ry, ruv, 24, False, 3, rng=16)
+        compare(pkg, f["levels"], f["dc"], f["modes"], f["mvs"],
+                orc.slice_cfg(w // 16, h // 16, 24, 0, 3, frame_num=k, poc=2 * k), writer)
+        ry, ruv = f["y"], f["uv"]
+
+
+def test_pipeline_1080p_staging(pkg):
+    """The H.264 pipeline's own staging of a 1080p I and P frame (+-64)."""
+    import torch
+    say("1080p pipeline staging")
+    w, h, qp = 1920, 1080, 27
+    g = pkg.FrameGen(1080, w, h, 12)
+    enc = pkg.Encoder(w, h, search_range=64, lambda8=1, deblock_idc=0)
+    try:
+        ims = []
+        for f in range(2):
+            im = enc.new_image()
+            enc.upload(im, *g.frame(f))
+            ims.append(im)
+        wmb, hmb = 120, 68
+        writer = pkg.CavlcWriter(wmb, hmb)
+        for f, im in enumerate(ims):
+            slot = enc.encode(im, qp, f == 0, 4)
+            r = enc.results(slot)
+            cfg = orc.slice_cfg(wmb, hmb, qp, int(f == 0), 4, frame_num=f, poc=2 * f)
+            got = compare(pkg, r["quant"], r["dc"], r["pred_mode"], r["mv"], cfg, writer)
+            say(f"  frame {f}: {sum(len(x) for x in got)} bytes")
+    finally:
+        enc.close()
+        g.close()
+
+
+def test_output_capacity_is_respected(pkg):
+    """A buffer too small for the frame: the overflow flag is set and no
+    byte past the capacity changes."""
+    import torch
+    say("capacity")
+    rs = np.random.default_rng(31)
+    wmb, hmb = 8, 6
+    n = wmb * hmb
+    lv, dc, modes, mvs = synth_frame(rs, wmb, hmb, 1, density=0.5, big=True)
+    cfg = orc.slice_cfg(wmb, hmb, 20, 1, 2)
+    want = orc.h264_cavlc_slices(lv, dc, modes, mvs, cfg)
+    total = sum(len(x) for x in want)
+    cap = (total // 2) // 4 * 4
+    out = torch.full((total + 4096,), 0x5A, dtype=torch.uint8, device="cuda")
+    info = torch.zeros(4 * 2 + 2, dtype=torch.int32, device="cuda")
+    work = torch.empty(int(pkg.hip.bc250_h264_cavlc_work_size(wmb, hmb)), dtype=torch.uint8, device="cuda")
+    st = _staging(torch, lv, dc, modes, mvs, n)
+    cfg_c = np.ascontiguousarray(cfg, np.int32)
+    p = lambda t: C.c_void_p(t.data_ptr())
+    rc = pkg.hip.bc250_h264_cavlc_slices(p(st[0]), p(st[1]), p(st[2]), p(st[3]), cfg_c.ctypes.data_as(C.c_void_p),
+                                         p(out), cap, p(info), p(work), work.numel(), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    i = info.cpu().numpy().view(np.uint32)
+    assert i[8] == total and i[9] == 1
+    assert (out[cap:].cpu().numpy() == 0x5A).all()
+
+
+def test_invalid_arguments_are_rejected(pkg):
+    import torch
+    say("argument checks")
+    t = torch.zeros(4096, dtype=torch.int32, device="cuda")
+    w = torch.zeros(1 << 20, dtype=torch.uint8, device="cuda")
+    p = C.c_void_p(t.data_ptr())
+    for bad in (dict(qp=52), dict(num_slices=0), dict(num_slices=65), dict(deblock_idc=3),
+                dict(log2_max_frame_num_minus4=13), dict(is_idr=2)):
+        c = dict(width_in_mbs=2, height_in_mbs=2, qp=26, is_idr=0, num_slices=1)
+        c.update(bad)
+        cfg = pkg.h264_slice_cfg(**c)
+        rc = pkg.hip.bc250_h264_cavlc_slices(p, p, p, p, cfg.ctypes.data_as(C.c_void_p), p, 4096, p,
+                                             C.c_void_p(w.data_ptr()), w.numel(), None)
+        assert rc == -1, bad
+    cfg = pkg.h264_slice_cfg(2, 2, 26, 0, 1)
+    assert pkg.hip.bc250_h264_cavlc_slices(p, p, p, None, cfg.ctypes.data_as(C.c_void_p), p, 4096, p,
+                                           C.c_void_p(w.data_ptr()), w.numel(), None) == -1  # P frame without mvs
+    assert pkg.hip.bc250_h264_cavlc_slices(p, p, p, p, cfg.ctypes.data_as(C.c_void_p), p, 4096, p,
+                                           C.c_void_p(w.data_ptr()), 16, None) == -1  # work too small
