@@ -270,6 +270,97 @@ def bench_t32(pkg, torch, args):
     }))
 
 
+def bench_cavlc(pkg, torch, args):
+    """The GPU H.264 CAVLC slice writer (not the headline): the 1080p H.264
+    pipeline's own staging (levels, DC terms, modes, vectors; +-64, CQP
+    --qp, 4 slices) of an I frame and 7 P frames, resident in HBM; each
+    frame's slices are checked byte for byte against the oracle, then
+    --steps writer calls cycling over the 8 frames are timed with HIP events
+    on the launch stream.  Algorithmic bytes per frame: the staging read once
+    (nmb x (768 + 96 + 16 + 4)) plus the RBSP written."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    wmb, hmb, ns, nf = (W + 15) // 16, (H + 15) // 16, 4, 8
+    nmb = wmb * hmb
+    keys = ("quant", "dc", "pred_mode", "mv")
+    frames = []
+    if args.staging and os.path.exists(args.staging):
+        log(f"cavlc: staging of {nf} frames from {args.staging}")
+        z = np.load(args.staging)
+        for f in range(nf):
+            r = {k: z[f"{k}{f}"] for k in keys}
+            r["mv"] = r["mv"].view(pkg.MV_DTYPE).reshape(-1)
+            frames.append((r, pkg.h264_slice_cfg(wmb, hmb, args.qp, f == 0, ns, frame_num=f, poc=2 * f)))
+    else:
+        log(f"cavlc: encoding {nf} frames {W}x{H} for their staging")
+        g = pkg.FrameGen(1080, W, H, 12)
+        enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, readback=1)
+        try:
+            for f in range(nf):
+                im = enc.new_image()
+                enc.upload(im, *g.frame(f))
+                r = enc.results(enc.encode(im, args.qp, f == 0, ns))
+                cfg = pkg.h264_slice_cfg(wmb, hmb, args.qp, f == 0, ns, frame_num=f, poc=2 * f)
+                frames.append((r, cfg))
+        finally:
+            enc.close()
+            g.close()
+        if args.staging:
+            np.savez(args.staging, **{f"{k}{f}": (r[k].view(np.int32) if k == "mv" else r[k])
+                                      for f, (r, _) in enumerate(frames) for k in keys})
+    dev = [tuple(torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in
+                 (r["quant"], r["dc"], r["pred_mode"].view(np.int32), r["mv"].view(np.int32).reshape(-1)))
+           for r, _ in frames]
+    writer = pkg.CavlcWriter(wmb, hmb)
+    sizes = []
+    for (r, cfg), d in zip(frames, dev):
+        got = writer.slices(*d, cfg)
+        want = orc.h264_cavlc_slices(r["quant"], r["dc"], r["pred_mode"], r["mv"], cfg)
+        if got != want:
+            raise RuntimeError("GPU CAVLC slices differ from the oracle's")
+        sizes.append(sum(len(x) for x in got))
+    log(f"cavlc: bit-exact on {nf} frames ({', '.join(str(x) for x in sizes)} bytes); "
+        f"{args.warmup} warmup + {args.steps} timed calls")
+    st = torch.cuda.current_stream()
+    for i in range(args.warmup):
+        writer.launch(*dev[i % nf], frames[i % nf][1])
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for i in range(args.steps):
+        writer.launch(*dev[i % nf], frames[i % nf][1])
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    out_bytes = sum(sizes[i % nf] for i in range(args.steps)) / args.steps
+    alg = nmb * (768 + 96 + 16 + 4) + out_bytes
+    gbs = alg / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        log("cavlc: timing the oracle writer on the host")
+        t0, k = time.perf_counter(), 0
+        while time.perf_counter() - t0 < min(args.cpu_budget, 10.0):
+            r, cfg = frames[k % nf]
+            orc.h264_cavlc_slices(r["quant"], r["dc"], r["pred_mode"], r["mv"], cfg)
+            k += 1
+        cpu_s = (time.perf_counter() - t0) / k
+        cpu = {"value": round(1.0 / cpu_s, 2), "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"oracle CAVLC writer, 1 thread, {k} calls over the same {nf} staged 1080p frames"}
+    print(json.dumps({
+        "metric": "1080p H.264 CAVLC slices written per second on 1 GPU (staging resident in HBM)",
+        "value": round(1e3 / ms, 1), "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int16", "data": "synthetic",
+        "config": {"workload": f"CAVLC slices of the 1080p H.264 pipeline's staging, I + 7 P frames, CQP {args.qp}, "
+                               f"{ns} slices, +-{args.range}", "frames": nf, "slices": ns, "qp": args.qp,
+                   "mean_frame_bytes": round(out_bytes)},
+        "roofline": {"bound": "hbm", "kernel": "cavlc writer (5 launches)", "achieved": round(gbs, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "algorithmic_bytes": round(alg)},
+        "cpu_baseline": cpu,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,9 +374,11 @@ def main():
     ap.add_argument("--single", action="store_true", help="skip the second codec's run")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle integer search")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("1080p", "4k-main10", "t32"), default="1080p",
+    ap.add_argument("--staging", default=None,
+                    help="cavlc: .npz of the frames' staging, written by a run that encodes them, read when present")
+    ap.add_argument("--workload", choices=("1080p", "4k-main10", "t32", "cavlc"), default="1080p",
                     help="4k-main10: the 3840x2160 P010 HEVC Main10 workload instead of the 1080p headline; "
-                         "t32: the 32x32 transform forms compared")
+                         "t32: the 32x32 transform forms compared; cavlc: the H.264 CAVLC slice writer")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the pipeline shards by session (one GPU per group of sessions); "
@@ -299,6 +392,9 @@ def main():
         return
     if args.workload == "t32":
         bench_t32(pkg, torch, args)
+        return
+    if args.workload == "cavlc":
+        bench_cavlc(pkg, torch, args)
         return
     n = args.warmup + args.steps
     log(f"generating {n} frames {W}x{H}")

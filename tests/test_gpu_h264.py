@@ -299,6 +299,20 @@ def test_pipeline_640x480_cqp32_4slices(pkg):
 
 # ------------------------------------------------------------------ full size
 
+def test_pipeline_640x368_r64_chain(pkg):
+    """+-64 over an I + 4 P chain, 4 slices: every stage against the oracle."""
+    _torch()
+    _run_sequence(pkg, 640, 368, 5, 27, 4, 64, 1, seed=64, nrect=6)
+
+
+def test_pipeline_1080p_ipppp_chain(pkg):
+    """1920x1080, I then 5 P frames, 4 slices, +-16: every stage of every
+    frame against the oracle's own chain (each P frame predicts from the
+    previous reconstruction)."""
+    _torch()
+    _run_sequence(pkg, 1920, 1080, 6, 27, 4, 16, 1, seed=1080, nrect=12)
+
+
 def test_1080p_stages_full_frame(pkg):
     """1920x1080 at +-64: I frame, then a P frame; every stage output is
     compared with the oracle at full size except the integer search, which
