@@ -483,6 +483,16 @@ extern "C" int gpu_compute_cpu_read_safe(gpu_context_t *ctx)
 
 // ------------------------------------------------------------ encode
 
+// Waits for a frame's last event by polling it: a blocking wait can sleep the
+// host thread past the frame still queued behind it, which leaves the GPU
+// idle between frames.
+static hipError_t wait_event(hipEvent_t ev)
+{
+    hipError_t e;
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady) __builtin_ia32_pause();
+    return e;
+}
+
 extern "C" int gpu_compute_begin_picture(gpu_context_t *ctx, gpu_image_t render_target)
 {
     (void)render_target;
@@ -491,7 +501,7 @@ extern "C" int gpu_compute_begin_picture(gpu_context_t *ctx, gpu_image_t render_
     Slot &sl = s->slot[s->cur];
     if (sl.pending) {
         // the slot is about to be reused: its previous frame must be done
-        if (!HIP_OK(hipEventSynchronize(sl.ev[NEV - 1]))) return -1;
+        if (!HIP_OK(wait_event(sl.ev[NEV - 1]))) return -1;
         sl.pending = false;
     }
     return 0;
@@ -779,7 +789,7 @@ extern "C" int gpu_compute_sync_slot(gpu_context_t *ctx, int slot)
     if (!s || slot < 0 || slot >= NSLOT) return -1;
     Slot &sl = s->slot[slot];
     if (!sl.recorded) return -1;
-    if (!HIP_OK(hipEventSynchronize(sl.ev[NEV - 1]))) {
+    if (!HIP_OK(wait_event(sl.ev[NEV - 1]))) {
         sl.have_data = false;
         return -1;
     }
