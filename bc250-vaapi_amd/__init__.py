@@ -60,7 +60,7 @@ class GpuContext(C.Structure):
 class EncodeParams(C.Structure):
     _fields_ = [("search_range", C.c_int), ("lambda8", C.c_int), ("deblock_idc", C.c_int),
                 ("me_8x8", C.c_int), ("readback", C.c_int), ("codec", C.c_int), ("square_radius", C.c_int),
-                ("diamond", C.c_int), ("hevc_ctu_log2", C.c_int)]
+                ("diamond", C.c_int), ("hevc_ctu_log2", C.c_int), ("entropy", C.c_int)]
 
 
 class CpuSimdMeConfig(C.Structure):
@@ -172,6 +172,10 @@ _sig(hip, "bc250_hevc_p010_to_8bit", _I, [_P, _I, _P, _I, _I, _I, _P])
 # H.264 CAVLC slices (include/bc250_bitstream.h)
 _sig(hip, "bc250_h264_cavlc_work_size", C.c_size_t, [_I, _I])
 _sig(hip, "bc250_h264_cavlc_slices", _I, [_P, _P, _P, _P, _P, _P, C.c_size_t, _P, _P, C.c_size_t, _P])
+_sig(hip, "bc250_h264_cavlc_copy_out", _I, [_P, _P, _I, _P, C.c_size_t, _P, _P])
+_sig(hip, "gpu_compute_set_h264_slice_header", _I, [_CTX, _P])
+_sig(hip, "gpu_compute_get_slices_slot", _I, [_CTX, _I, C.POINTER(_P), C.POINTER(C.POINTER(C.c_uint32)),
+                                              C.POINTER(_I)])
 
 _sig(host, "bc250_gen_create", _P, [C.c_uint64, _I, _I, _I])
 _sig(host, "bc250_gen_destroy", None, [_P])
@@ -314,10 +318,11 @@ class Encoder:
     """One gpu_compute context: upload frames, encode, read staging back."""
 
     def __init__(self, width, height, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0,
-                 bit_depth=8, square_radius=0, diamond=0, ctu_log2=4):
+                 bit_depth=8, square_radius=0, diamond=0, ctu_log2=4, entropy=0):
         """bit_depth 10: P010 surfaces (HEVC Main10); upload() then takes uint16
         planes of 0..1023 samples and map_recon() returns them.  ctu_log2 5:
-        HEVC CTU32 (gpu_encode_params_t.hevc_ctu_log2)."""
+        HEVC CTU32 (gpu_encode_params_t.hevc_ctu_log2).  entropy 1: H.264
+        frames end with their CAVLC slices, read with slices()."""
         self.w, self.h = width, height
         self.fmt = GPU_IMAGE_P010 if bit_depth == 10 else GPU_IMAGE_NV12
         self.wc, self.hc = (width + 15) // 16 * 16, (height + 15) // 16 * 16
@@ -326,15 +331,17 @@ class Encoder:
         if hip.bc250_gpu_init(C.byref(self.ctx)) != 0:
             raise RuntimeError("bc250_gpu_init failed (no HIP device?)")
         self.codec = codec
-        self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius, diamond, ctu_log2)
+        self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius, diamond, ctu_log2,
+                        entropy)
         self.images = []
 
     def set_params(self, search_range=64, lambda8=1, deblock_idc=0, me_8x8=1, readback=1, codec=0, square_radius=0,
-                   diamond=0, ctu_log2=4):
+                   diamond=0, ctu_log2=4, entropy=0):
         """codec: GPU_CODEC_H264 (0) or GPU_CODEC_HEVC (1); square_radius 2..16
         selects the reference's Tier-2 square search, diamond 1 its GPU
         diamond search, instead of the full one."""
-        p = EncodeParams(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius, diamond, ctu_log2)
+        p = EncodeParams(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius, diamond, ctu_log2,
+                         entropy)
         if hip.gpu_compute_set_encode_params(C.byref(self.ctx), C.byref(p)) != 0:
             raise ValueError("invalid encode parameters")
 
@@ -389,6 +396,23 @@ class Encoder:
               "mv8": MV_DTYPE, "hevc_mode": np.uint8}[name]
         buf = (C.c_uint8 * n.value).from_address(d.value)
         return np.frombuffer(buf, dtype=dt).copy()
+
+    def slices(self, slot):
+        """The CAVLC slices (list of bytes) of the frame in `slot` (entropy 1)."""
+        d, info, n = C.c_void_p(), C.POINTER(C.c_uint32)(), C.c_int()
+        if hip.gpu_compute_get_slices_slot(C.byref(self.ctx), slot, C.byref(d), C.byref(info), C.byref(n)) != 0:
+            raise RuntimeError(f"no slices in slot {slot}")
+        ns = n.value
+        total = info[4 * ns]
+        data = C.string_at(d.value, total)
+        return [data[info[4 * s]:info[4 * s] + info[4 * s + 1]] for s in range(ns)]
+
+    def set_slice_header(self, **kw):
+        """gpu_compute_set_h264_slice_header: pps_id, pic_init_qp_minus26,
+        log2_max_frame_num_minus4, log2_max_poc_lsb_minus4, relax_chroma_skip."""
+        cfg = h264_slice_cfg(1, 1, 26, 0, 1, **kw)
+        if hip.gpu_compute_set_h264_slice_header(C.byref(self.ctx), _ptr(cfg)) != 0:
+            raise ValueError("invalid slice header fields")
 
     def results(self, slot):
         if self.codec == GPU_CODEC_HEVC:

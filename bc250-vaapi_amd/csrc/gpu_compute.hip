@@ -15,6 +15,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "bc250_bitstream.h"
 #include "bc250_dev.h"
 #include "bc250_hevc.h"
 #include "bc250_stages.h"
@@ -42,6 +43,9 @@ struct Slot {
     gpu_mv_t *mvs = nullptr, *mv8 = nullptr;
     uint8_t *hmodes = nullptr;  // HEVC PU modes
     int *err = nullptr;  // 2 ints: intra wait error, deblock wait error
+    uint8_t *rbsp = nullptr;     // CAVLC slices of the frame (entropy = 1), pinned
+    uint32_t *rbsp_info = nullptr;
+    int slices = 0;              // their number; 0 = no slices for this frame
     hipEvent_t ev[NEV] = {};
     bool pending = false, recorded = false, have_data = false, was_intra = false;
 };
@@ -68,7 +72,14 @@ struct bc250_hip_state {
     Slot slot[NSLOT];
     int cur = 0, submitted = 0;
     double last_latency_ms = 0.0;
-    gpu_encode_params_t params{64, 1, 0, 1, 1, GPU_CODEC_H264, 0, 0, 4};
+    gpu_encode_params_t params{64, 1, 0, 1, 1, GPU_CODEC_H264, 0, 0, 4, 0};
+    // CAVLC slices on the GPU (entropy = 1), allocated at the first such frame
+    uint8_t *cavlc_out = nullptr;
+    uint32_t *cavlc_info = nullptr;
+    void *cavlc_work = nullptr;
+    size_t cavlc_cap = 0, cavlc_work_size = 0;
+    bc250_h264_slice_cfg_t hdr{};  // static header fields (gpu_compute_set_h264_slice_header)
+    int frame_num = 0, idr_pic_id = -1;
     int err_flags = 0;
     // per-stage GPU time accumulated at sync_slot: [stage] sums, frames seen
     double stage_sum[NSTAGE] = {0, 0, 0, 0, 0};
@@ -163,7 +174,14 @@ static void free_res(bc250_hip_state *s)
     s->levels = nullptr; s->dc = nullptr; s->nz = nullptr; s->modes = nullptr; s->mvs = nullptr;
     s->mv8 = nullptr; s->imv16 = nullptr; s->imv8 = nullptr; s->sync_i = nullptr; s->sync_d = nullptr;
     s->sync_h = nullptr; s->hmodes = nullptr; s->hcbf = nullptr; s->hculog2 = nullptr;
+    for (void *p : {(void *)s->cavlc_out, (void *)s->cavlc_info, s->cavlc_work})
+        if (p) hipFree(p);
+    s->cavlc_out = nullptr; s->cavlc_info = nullptr; s->cavlc_work = nullptr;
+    s->cavlc_cap = s->cavlc_work_size = 0;
     for (auto &sl : s->slot) {
+        for (void *p : {(void *)sl.rbsp, (void *)sl.rbsp_info})
+            if (p) hipHostFree(p);
+        sl.rbsp = nullptr; sl.rbsp_info = nullptr; sl.slices = 0;
         void *host[] = {sl.levels, sl.dc, sl.modes, sl.nz, sl.mvs, sl.mv8, sl.hmodes, sl.err};
         for (void *p : host)
             if (p) hipHostFree(p);
@@ -224,6 +242,27 @@ static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
     s->w = w; s->h = h; s->wc = wc; s->hc = hc; s->mbw = mbw; s->mbh = mbh; s->nmb = nmb; s->bps = bps;
     s->has_recon = false;
     s->ref_idx = 0;
+    return 0;
+}
+
+// Device and pinned host buffers of the CAVLC writer: room for 2560 bytes
+// per MB (the reference's CAVLC allowance) and 4 KB per slice.
+static int ensure_cavlc(bc250_hip_state *s)
+{
+    if (s->cavlc_out) return 0;
+    const size_t cap = align_up((int)((size_t)s->nmb * 2560 + 4096 * 64), 256);
+    const size_t work = bc250_h264_cavlc_work_size(s->mbw, s->mbh);
+    bool ok = work && HIP_OK(hipMalloc((void **)&s->cavlc_out, cap)) &&
+              HIP_OK(hipMalloc((void **)&s->cavlc_info, (4 * 64 + 2) * 4)) && HIP_OK(hipMalloc(&s->cavlc_work, work));
+    for (auto &sl : s->slot)
+        ok = ok && HIP_OK(hipHostMalloc((void **)&sl.rbsp, cap, hipHostMallocDefault)) &&
+             HIP_OK(hipHostMalloc((void **)&sl.rbsp_info, (4 * 64 + 2) * 4, hipHostMallocDefault));
+    if (!ok) {
+        fprintf(stderr, "[bc250-hip] CAVLC buffer allocation failed\n");
+        return -1;
+    }
+    s->cavlc_cap = cap;
+    s->cavlc_work_size = work;
     return 0;
 }
 
@@ -642,7 +681,12 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     const int out_idx = s->has_recon ? (s->ref_idx ^ 1) : 0;
     Plane2 &out = s->recon[out_idx];
 
+    sl.slices = 0;
     if (p.codec == GPU_CODEC_HEVC) {
+        if (p.entropy) {
+            fprintf(stderr, "[bc250-hip] entropy = 1 writes H.264 CAVLC slices; HEVC has no slice writer\n");
+            return -1;
+        }
         if (cpu_mvs) {
             fprintf(stderr, "[bc250-hip] cpu_mvs is an H.264 route; HEVC searches on the GPU\n");
             return -1;
@@ -693,6 +737,35 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
                                      num_slices, p.deblock_idc, s->sync_d, kSpinLimit, st);
     }
     hipEventRecord(sl.ev[4], st);
+    if (r == 0 && p.entropy) {
+        // CAVLC slices of the frame, numbered the IPPP way (IDR at intra frames)
+        if (num_slices < 1 || num_slices > 64 || num_slices > s->nmb || ensure_cavlc(s) != 0) {
+            r = -1;
+        } else {
+            bc250_h264_slice_cfg_t c = s->hdr;
+            if (is_intra) {
+                s->frame_num = 0;
+                s->idr_pic_id = (s->idr_pic_id + 1) & 0xffff;
+            } else {
+                s->frame_num = (s->frame_num + 1) & ((1 << (c.log2_max_frame_num_minus4 + 4)) - 1);
+            }
+            c.width_in_mbs = s->mbw;
+            c.height_in_mbs = s->mbh;
+            c.qp = qp;
+            c.is_idr = is_intra ? 1 : 0;
+            c.num_slices = num_slices;
+            c.frame_num = s->frame_num;
+            c.idr_pic_id = s->idr_pic_id < 0 ? 0 : s->idr_pic_id;
+            c.poc = 2 * s->frame_num;
+            c.deblock_idc = p.deblock_idc;
+            r = bc250_h264_cavlc_slices(s->levels, s->dc, s->modes, (const gpu_mv_t *)s->mvs, &c, s->cavlc_out,
+                                        s->cavlc_cap, s->cavlc_info, s->cavlc_work, s->cavlc_work_size, st);
+            if (r == 0)
+                r = bc250_h264_cavlc_copy_out(s->cavlc_out, s->cavlc_info, num_slices, sl.rbsp, s->cavlc_cap,
+                                              sl.rbsp_info, st);
+            if (r == 0) sl.slices = num_slices;
+        }
+    }
     if (r == 0 && p.readback) {
         const size_t n = (size_t)s->nmb;
         bool ok = HIP_OK(hipMemcpyAsync(sl.levels, s->levels, n * 384 * 2, hipMemcpyDeviceToHost, st)) &&
@@ -755,6 +828,7 @@ extern "C" int gpu_compute_dispatch_me_only(gpu_context_t *ctx, gpu_image_t rt, 
     if (sl.err) sl.err[0] = sl.err[1] = 0;
     sl.have_data = false;
     sl.recorded = false;
+    sl.slices = 0;
     return r;
 }
 
@@ -883,10 +957,42 @@ extern "C" int gpu_compute_set_encode_params(gpu_context_t *ctx, const gpu_encod
     if (p->diamond != 0 && p->diamond != 1) return -1;
     if (p->diamond && p->square_radius) return -1;  // one search at a time
     if (p->hevc_ctu_log2 != 0 && p->hevc_ctu_log2 != 4 && p->hevc_ctu_log2 != 5) return -1;
+    if (p->entropy != 0 && p->entropy != 1) return -1;
     s->params = *p;
     if (!s->params.hevc_ctu_log2) s->params.hevc_ctu_log2 = 4;
     s->params.me_8x8 = p->me_8x8 ? 1 : 0;
     s->params.readback = p->readback ? 1 : 0;
+    return 0;
+}
+
+extern "C" int gpu_compute_set_h264_slice_header(gpu_context_t *ctx, const bc250_h264_slice_cfg_t *cfg)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !cfg) return -1;
+    if (cfg->pps_id < 0 || cfg->pps_id > 255 || cfg->pic_init_qp_minus26 < -26 || cfg->pic_init_qp_minus26 > 25 ||
+        cfg->log2_max_frame_num_minus4 < 0 || cfg->log2_max_frame_num_minus4 > 12 ||
+        cfg->log2_max_poc_lsb_minus4 < 0 || cfg->log2_max_poc_lsb_minus4 > 12 ||
+        (cfg->relax_chroma_skip != 0 && cfg->relax_chroma_skip != 1))
+        return -1;
+    s->hdr.pps_id = cfg->pps_id;
+    s->hdr.pic_init_qp_minus26 = cfg->pic_init_qp_minus26;
+    s->hdr.log2_max_frame_num_minus4 = cfg->log2_max_frame_num_minus4;
+    s->hdr.log2_max_poc_lsb_minus4 = cfg->log2_max_poc_lsb_minus4;
+    s->hdr.relax_chroma_skip = cfg->relax_chroma_skip;
+    return 0;
+}
+
+extern "C" int gpu_compute_get_slices_slot(gpu_context_t *ctx, int slot, const uint8_t **rbsp, const uint32_t **info,
+                                           int *num_slices)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !rbsp || !info || !num_slices || slot < 0 || slot >= NSLOT) return -1;
+    const Slot &sl = s->slot[slot];
+    if (!sl.slices || sl.pending || !sl.rbsp) return -1;
+    if (sl.rbsp_info[4 * sl.slices + 1] || sl.rbsp_info[4 * sl.slices] > s->cavlc_cap) return -1;
+    *rbsp = sl.rbsp;
+    *info = sl.rbsp_info;
+    *num_slices = sl.slices;
     return 0;
 }
 

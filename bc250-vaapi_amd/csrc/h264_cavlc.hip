@@ -758,7 +758,31 @@ __global__ __launch_bounds__(256) void cavlc_write_kernel(const int16_t *__restr
     s.finish();
 }
 
+// copies slice_info and the slices' bytes (as many as dst_cap allows) to dst
+__global__ __launch_bounds__(256) void cavlc_copy_out_kernel(const uint32_t *__restrict__ src,
+                                                             const uint32_t *__restrict__ info, int ns,
+                                                             uint32_t *__restrict__ dst, uint32_t dst_words,
+                                                             uint32_t *__restrict__ dst_info)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t < (uint32_t)(4 * ns + 2)) dst_info[t] = info[t];
+    const uint32_t words = min((info[4 * ns] + 3) >> 2, dst_words);
+    for (uint32_t i = t; i < words; i += gridDim.x * 256) dst[i] = src[i];
+}
+
 }  // namespace
+
+extern "C" int bc250_h264_cavlc_copy_out(const uint8_t *out, const uint32_t *slice_info, int num_slices,
+                                         uint8_t *dst, size_t dst_cap, uint32_t *dst_info, void *stream)
+{
+    if (!out || !slice_info || !dst || !dst_info || num_slices < 1 || num_slices > 64 || ((uintptr_t)out & 3) ||
+        ((uintptr_t)dst & 3))
+        return -1;
+    const uint32_t words = (uint32_t)std::min<size_t>(dst_cap / 4, (size_t)1 << 27);
+    cavlc_copy_out_kernel<<<64, 256, 0, (hipStream_t)stream>>>((const uint32_t *)out, slice_info, num_slices,
+                                                               (uint32_t *)dst, words, dst_info);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 extern "C" size_t bc250_h264_cavlc_work_size(int width_in_mbs, int height_in_mbs)
 {

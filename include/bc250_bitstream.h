@@ -62,6 +62,37 @@ int bc250_h264_cavlc_slices(const int16_t *levels, const int32_t *dc, const uint
                             const bc250_h264_slice_cfg_t *cfg, uint8_t *out, size_t out_cap, uint32_t *slice_info,
                             void *work, size_t work_size, void *stream);
 
+/* Copies slice_info (4 * num_slices + 2 words) and the slices' bytes that
+ * bc250_h264_cavlc_slices wrote (slice_info[4 n] of them, at most dst_cap) to
+ * dst / dst_info -- any memory the device can write, e.g. hipHostMalloc'd
+ * pinned host buffers -- so that a readback moves exactly the coded bytes
+ * without the host knowing their number.  Enqueued on `stream`. */
+int bc250_h264_cavlc_copy_out(const uint8_t *out, const uint32_t *slice_info, int num_slices, uint8_t *dst,
+                              size_t dst_cap, uint32_t *dst_info, void *stream);
+
+/* ---- in the pipeline (gpu_compute.h): gpu_encode_params_t.entropy = 1 ----
+ * Every H.264 frame then ends with its CAVLC slices, written on the GPU from
+ * the frame's staging and copied to the slot's pinned host buffers.  The
+ * context numbers the frames the usual IPPP way: an intra frame is an IDR
+ * (frame_num 0, idr_pic_id + 1), each P frame increments frame_num, and poc
+ * is 2 * frame_num.  Not available for HEVC (the dispatch fails). */
+
+/* Sets the slice-header fields the pipeline does not derive: pps_id,
+ * pic_init_qp_minus26, log2_max_frame_num_minus4, log2_max_poc_lsb_minus4
+ * and relax_chroma_skip.  The other fields of cfg are ignored: sizes, QP,
+ * slices and the frame numbering come from the dispatch, deblock_idc from
+ * the encode params.  Defaults are all 0.
+ * Returns 0, or -1 for out-of-range values. */
+int gpu_compute_set_h264_slice_header(gpu_context_t *ctx, const bc250_h264_slice_cfg_t *cfg);
+
+/* The slices of the frame in `slot`, after gpu_compute_sync_slot: *rbsp
+ * points at the RBSPs back to back, *info at slice_info as above (pinned host
+ * memory owned by the context, valid until the slot is reused), *num_slices
+ * at their number.  Returns -1 when the frame was coded without entropy = 1
+ * or its slices did not fit the buffers. */
+int gpu_compute_get_slices_slot(gpu_context_t *ctx, int slot, const uint8_t **rbsp, const uint32_t **info,
+                                int *num_slices);
+
 #ifdef __cplusplus
 }
 #endif

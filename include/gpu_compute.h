@@ -195,6 +195,8 @@ typedef struct {
                         * I pictures code the four 16x16 quadrants of each CTU32 in z-order with the
                         * CTU32 z-scan availability, slices are CTU32-row ranges (P pictures and the
                         * coefficient / mode layouts, indexed by 16x16 area, are unchanged) */
+    int entropy;       /* H.264: 1 = write each frame's CAVLC slices on the GPU after deblocking and copy
+                        * them to the slot (include/bc250_bitstream.h); 0 = staging only */
 } gpu_encode_params_t;
 
 /* codec = GPU_CODEC_HEVC runs the HEVC stages of include/bc250_hevc.h

@@ -182,3 +182,31 @@ def test_invalid_arguments_are_rejected(pkg):
                                            C.c_void_p(w.data_ptr()), w.numel(), None) == -1  # P frame without mvs
     assert pkg.hip.bc250_h264_cavlc_slices(p, p, p, p, cfg.ctypes.data_as(C.c_void_p), p, 4096, p,
                                            C.c_void_p(w.data_ptr()), 16, None) == -1  # work too small
+
+
+@pytest.mark.parametrize("w,h,ns,rng", [(320, 192, 3, 16), (1920, 1080, 4, 64)])
+def test_pipeline_entropy_slices(pkg, w, h, ns, rng):
+    """gpu_encode_params_t.entropy = 1: each frame's slices, numbered IPPP
+    with a second IDR, equal the oracle writer's on the frame's staging."""
+    say(f"pipeline slices {w}x{h}, {ns} slices, +-{rng}")
+    qp, wmb, hmb = 27, (w + 15) // 16, (h + 15) // 16
+    g = pkg.FrameGen(7, w, h, 6)
+    enc = pkg.Encoder(w, h, search_range=rng, lambda8=1, deblock_idc=0, readback=1, entropy=1)
+    enc.set_slice_header(pps_id=2, pic_init_qp_minus26=1, log2_max_frame_num_minus4=0, log2_max_poc_lsb_minus4=1)
+    try:
+        fn, idr = 0, -1
+        for i in range(7):
+            intra = i in (0, 5)
+            im = enc.new_image()
+            enc.upload(im, *g.frame(i))
+            slot = enc.encode(im, qp, intra, ns)
+            got, r = enc.slices(slot), enc.results(slot)
+            fn, idr = (0, idr + 1) if intra else ((fn + 1) % 16, idr)
+            cfg = orc.slice_cfg(wmb, hmb, qp, int(intra), ns, pps_id=2, pic_init_qp_minus26=1, frame_num=fn,
+                                idr_pic_id=idr, poc=2 * fn, log2_max_poc_lsb_minus4=1)
+            want = orc.h264_cavlc_slices(r["quant"], r["dc"], r["pred_mode"], r["mv"], cfg)
+            assert got == want, f"frame {i}"
+            say(f"  frame {i}: {sum(len(x) for x in got)} bytes")
+    finally:
+        enc.close()
+        g.close()
