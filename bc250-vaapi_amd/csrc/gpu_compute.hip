@@ -777,9 +777,13 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
         r = ok ? 0 : -1;
     }
     if (r == 0) {
-        // wait-error flags of the two wavefront kernels (sync[1])
-        bool ok = HIP_OK(hipMemcpyAsync(&sl.err[0], s->sync_i + 1, sizeof(int), hipMemcpyDeviceToHost, st)) &&
-                  HIP_OK(hipMemcpyAsync(&sl.err[1], s->sync_d + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        // wait-error flags of the two wavefront kernels (sync[1]); with
+        // deblocking off the deblock workspace is never cleared or written
+        bool ok = HIP_OK(hipMemcpyAsync(&sl.err[0], s->sync_i + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        if (p.deblock_idc != 1)
+            ok = ok && HIP_OK(hipMemcpyAsync(&sl.err[1], s->sync_d + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        else
+            sl.err[1] = 0;
         r = ok ? 0 : -1;
     }
     sl.have_data = r == 0 && p.readback;

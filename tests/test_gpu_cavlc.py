@@ -210,3 +210,36 @@ def test_pipeline_entropy_slices(pkg, w, h, ns, rng):
     finally:
         enc.close()
         g.close()
+
+
+def test_pipeline_slices_decode_to_recon(pkg):
+    """The pipeline's own slices (entropy = 1, deblocking off), parsed and
+    decoded with the reference decoder's prediction / MC / inverse
+    transforms, rebuild the pipeline's reconstruction exactly, IPPPP."""
+    import os
+    if not os.path.exists(orc.REF_H264):
+        pytest.skip("oracle/_ref not built")
+    from h264_cavlc_parse import SliceParser
+    from h264_decode import decode_frame
+    w, h, qp, ns = 320, 192, 28, 2
+    say(f"pipeline slices decode == recon {w}x{h}")
+    g = pkg.FrameGen(11, w, h, 4)
+    enc = pkg.Encoder(w, h, search_range=32, lambda8=1, deblock_idc=1, readback=0, entropy=1)
+    try:
+        ry = ruv = None
+        for i in range(5):
+            im = enc.new_image()
+            enc.upload(im, *g.frame(i))
+            slot = enc.encode(im, qp, i == 0, ns)
+            ps = SliceParser(w // 16, h // 16, ns)
+            for s, rb in enumerate(enc.slices(slot)):
+                ps.parse(s, rb)
+            dy, duv = decode_frame(ps, qp, ry, ruv)
+            gy, guv = enc.map_recon()
+            np.testing.assert_array_equal(dy, gy, err_msg=f"frame {i} luma")
+            np.testing.assert_array_equal(duv, guv, err_msg=f"frame {i} chroma")
+            ry, ruv = gy, guv
+            say(f"  frame {i} decodes to the reconstruction")
+    finally:
+        enc.close()
+        g.close()

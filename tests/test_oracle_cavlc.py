@@ -419,3 +419,36 @@ def test_skip_vector_is_8_4_1_1():
                     or (mv[m - wmb]["mvx"], mv[m - wmb]["mvy"]) == (0, 0))
             want = (0, 0) if zero else orc.h264_mv_pred(mv, wmb, start, m)
             assert orc.h264_mv_pred(mv, wmb, start, m, skip=True) == want
+
+
+@need_ref
+def test_slices_decode_to_the_reconstruction():
+    """End to end on the CPU: the oracle's I and P frames (deblocking off),
+    written as slices, parsed, and decoded with the reference decoder's own
+    prediction / MC / inverse transforms, equal the oracle's reconstruction."""
+    if not os.path.exists(orc.REF_H264):
+        pytest.skip("oracle/_ref not built")
+    from h264_decode import decode_frame
+    rs = np.random.default_rng(41)
+    w, h, qp, ns = 96, 64, 26, 2
+    yy, xx = np.mgrid[0:h, 0:w]
+    y0 = np.clip(128 + 50 * np.sin(xx / 6.0) * np.cos(yy / 8.0) + rs.normal(0, 4, (h, w)), 0, 255).astype(np.uint8)
+    uv0 = rs.integers(100, 150, (h // 2, w)).astype(np.uint8)
+    y1, uv1 = np.roll(y0, (1, 3), (0, 1)), np.roll(uv0, (0, 2), (0, 1))
+    fi = orc.encode_frame(y0, uv0, y0, uv0, qp, True, ns, 16, 1, 1)
+    ps = SliceParser(w // 16, h // 16, ns)
+    for s, rb in enumerate(orc.h264_cavlc_slices(fi["levels"], fi["dc"], fi["modes"], None,
+                                                 orc.slice_cfg(w // 16, h // 16, qp, 1, ns, deblock_idc=1))):
+        ps.parse(s, rb)
+    dy, duv = decode_frame(ps, qp)
+    np.testing.assert_array_equal(dy, fi["y"])
+    np.testing.assert_array_equal(duv, fi["uv"])
+    fp = orc.encode_frame(y1, uv1, fi["y"], fi["uv"], qp, False, ns, 16, 1, 1)
+    ps = SliceParser(w // 16, h // 16, ns)
+    for s, rb in enumerate(orc.h264_cavlc_slices(fp["levels"], fp["dc"], fp["modes"], fp["mvs"],
+                                                 orc.slice_cfg(w // 16, h // 16, qp, 0, ns, frame_num=1, poc=2,
+                                                               deblock_idc=1))):
+        ps.parse(s, rb)
+    dy, duv = decode_frame(ps, qp, fi["y"], fi["uv"])
+    np.testing.assert_array_equal(dy, fp["y"])
+    np.testing.assert_array_equal(duv, fp["uv"])
