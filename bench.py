@@ -336,6 +336,11 @@ def bench_cavlc(pkg, torch, args):
     out_bytes = sum(sizes[i % nf] for i in range(args.steps)) / args.steps
     alg = nmb * (768 + 96 + 16 + 4) + out_bytes
     gbs = alg / (ms * 1e-3) / 1e9
+    traffic = None  # HBM bytes per frame from the FETCH_SIZE / WRITE_SIZE profile
+    tfile = os.path.join(ROOT, "profiles", "cavlc_1080p_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f)["hbm_bytes_per_frame"]
     cpu = None
     if not args.no_cpu:
         log("cavlc: timing the oracle writer on the host")
@@ -356,7 +361,7 @@ def bench_cavlc(pkg, torch, args):
                                f"{ns} slices, +-{args.range}", "frames": nf, "slices": ns, "qp": args.qp,
                    "mean_frame_bytes": round(out_bytes)},
         "roofline": {"bound": "hbm", "kernel": "cavlc writer (5 launches)", "achieved": round(gbs, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes": round(alg)},
         "cpu_baseline": cpu,
     }))
