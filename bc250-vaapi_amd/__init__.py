@@ -187,6 +187,32 @@ _sig(host, "h264_intra16_luma_dc_transform", None, [_P, _I, _P, _P])
 _sig(host, "h264_chroma_dc_levels", None, [_P, _I, _P])
 _sig(host, "h264_frame_dc_levels", None, [_P, _I, _I, _I, _P, _P])
 _sig(host, "h264_chroma_qp", _I, [_I])
+_sig(host, "bc250_rbsp_to_ebsp", C.c_size_t, [_P, C.c_size_t, _P, C.c_size_t])
+_sig(host, "bc250_h264_frame_nals", C.c_size_t, [_P, _P, _I, _I, _P, C.c_size_t])
+
+
+def rbsp_to_ebsp(rbsp):
+    """Emulation prevention of one RBSP (host/h264_host.c)."""
+    src = np.frombuffer(bytes(rbsp), np.uint8)
+    dst = np.zeros(len(src) * 3 // 2 + 2, np.uint8)
+    n = host.bc250_rbsp_to_ebsp(_ptr(dst), dst.size, _ptr(src) if src.size else None, src.size)
+    return dst[:n].tobytes()
+
+
+def h264_frame_nals(slices, is_idr):
+    """Annex B NAL units of a frame's slice RBSPs (list of bytes)."""
+    data = b"".join(slices)
+    info = np.zeros(4 * len(slices) + 2, np.uint32)
+    off = 0
+    for s, x in enumerate(slices):
+        info[4 * s], info[4 * s + 1] = off, len(x)
+        off += len(x)
+    src = np.frombuffer(data, np.uint8) if data else np.zeros(1, np.uint8)
+    dst = np.zeros(2 * len(data) + 8 * len(slices) + 16, np.uint8)
+    n = host.bc250_h264_frame_nals(_ptr(src), _ptr(info), len(slices), int(is_idr), _ptr(dst), dst.size)
+    if n == 0:
+        raise RuntimeError("NAL buffer too small")
+    return dst[:n].tobytes()
 
 
 def _ptr(a):

@@ -43,6 +43,19 @@ void h264_chroma_dc_levels(const int dc_in[4], int qpc, int out[4]);
  * frames only, zero otherwise) and chroma [nmb*8] (Cb 4 then Cr 4). */
 void h264_frame_dc_levels(const int32_t *dc, int nmb, int qp, int is_intra, int16_t *luma_dc, int16_t *chroma_dc);
 
+/* NAL packaging of the GPU-written slices (the reference's
+ * encoder_h264.c:3440-3470 and bitstream.c:232-250).
+ * bc250_rbsp_to_ebsp: emulation prevention (7.4.1): after two zero bytes, a
+ * byte <= 3 is preceded by 0x03; returns the bytes written (stops at cap).
+ * bc250_h264_frame_nals: every slice of a frame (RBSPs back to back as
+ * slice_info describes them, see bc250_bitstream.h) as an Annex B NAL unit:
+ * 00 00 00 01, header 0x65 (IDR, nal_ref_idc 3) or 0x41 (P, nal_ref_idc 2),
+ * the EBSP.  Returns the bytes written, or 0 when cap (2 * RBSP + 5 bytes
+ * per slice is always enough) is too small. */
+size_t bc250_rbsp_to_ebsp(uint8_t *dst, size_t cap, const uint8_t *src, size_t n);
+size_t bc250_h264_frame_nals(const uint8_t *rbsp, const uint32_t *slice_info, int num_slices, int is_idr,
+                             uint8_t *dst, size_t cap);
+
 /* ---- sequence driver ------------------------------------------------------ */
 typedef struct {
     int qp;          /* CQP */

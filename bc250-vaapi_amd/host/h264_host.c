@@ -155,3 +155,37 @@ int bc250_encode_sequence(gpu_context_t *ctx, const gpu_image_t *frames, int n, 
     if (total_ms) *total_ms = now_ms() - t0;
     return rc;
 }
+
+size_t bc250_rbsp_to_ebsp(uint8_t *dst, size_t cap, const uint8_t *src, size_t n)
+{
+    size_t o = 0;
+    int zeros = 0;
+    for (size_t i = 0; i < n && o < cap; i++) {
+        const uint8_t b = src[i];
+        if (zeros >= 2 && b <= 3) {
+            dst[o++] = 3;
+            zeros = 0;
+            if (o >= cap) break;
+        }
+        dst[o++] = b;
+        zeros = b == 0 ? zeros + 1 : 0;
+    }
+    return o;
+}
+
+size_t bc250_h264_frame_nals(const uint8_t *rbsp, const uint32_t *slice_info, int num_slices, int is_idr,
+                             uint8_t *dst, size_t cap)
+{
+    if (!rbsp || !slice_info || !dst || num_slices < 1) return 0;
+    size_t o = 0;
+    for (int s = 0; s < num_slices; s++) {
+        const size_t len = slice_info[4 * s + 1];
+        /* worst case: one 0x03 per two input bytes */
+        if (o + 5 + len + len / 2 + 1 > cap) return 0;
+        dst[o] = dst[o + 1] = dst[o + 2] = 0;
+        dst[o + 3] = 1;
+        dst[o + 4] = is_idr ? (3 << 5 | 5) : (2 << 5 | 1);
+        o += 5 + bc250_rbsp_to_ebsp(dst + o + 5, cap - o - 5, rbsp + slice_info[4 * s], len);
+    }
+    return o;
+}
