@@ -78,6 +78,32 @@ def test_escapes_and_qps(pkg, qp, idr):
     compare(pkg, lv, dc, modes, mvs, orc.slice_cfg(11, 6, qp, idr, 3))
 
 
+@pytest.mark.parametrize("ns,idr", [(4, 0), (16, 1)])
+def test_4k_frames(pkg, ns, idr):
+    """3840x2160 (240 x 135 MBs): the scan's threads hold 32 MBs each."""
+    say(f"4K synthetic, {ns} slices, idr {idr}")
+    rs = np.random.default_rng(2160 + ns)
+    lv, dc, modes, mvs = synth_frame(rs, 240, 135, idr, density=0.05)
+    compare(pkg, lv, dc, modes, mvs, orc.slice_cfg(240, 135, 30, idr, ns))
+
+
+def test_largest_frame_and_the_limit(pkg):
+    """2^17 MBs (512 x 256) is accepted -- an all-skip P frame, 128 MBs per
+    scan thread -- and one more MB row is refused."""
+    import torch
+    say("2^17 MBs")
+    wmb, hmb = 512, 256
+    n = wmb * hmb
+    z = np.zeros((n, 24, 16), np.int16)
+    compare(pkg, z, np.zeros((n, 24), np.int32), np.zeros(n, np.uint32), np.zeros(n, orc.MV),
+            orc.slice_cfg(wmb, hmb, 30, 0, 3))
+    t = torch.zeros(4096, dtype=torch.int32, device="cuda")
+    p = C.c_void_p(t.data_ptr())
+    cfg = pkg.h264_slice_cfg(wmb, hmb + 1, 30, 0, 1)
+    assert pkg.hip.bc250_h264_cavlc_slices(p, p, p, p, cfg.ctypes.data_as(C.c_void_p), p, 4096, p, p, 1 << 40,
+                                           None) == -1
+
+
 def test_all_skip_and_all_zero(pkg):
     say("all-skip P frame, all-zero I frame")
     wmb, hmb = 12, 7
