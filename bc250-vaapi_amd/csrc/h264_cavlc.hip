@@ -804,7 +804,11 @@ extern "C" int bc250_h264_cavlc_slices(const int16_t *levels, const int32_t *dc,
     if (c.num_slices < 1 || c.num_slices > 64 || c.num_slices > total) return -1;
     if (c.log2_max_frame_num_minus4 < 0 || c.log2_max_frame_num_minus4 > 12) return -1;
     if (c.log2_max_poc_lsb_minus4 < 0 || c.log2_max_poc_lsb_minus4 > 12) return -1;
-    if (c.deblock_idc < 0 || c.deblock_idc > 2 || c.pps_id < 0 || c.frame_num < 0 || c.idr_pic_id < 0) return -1;
+    // PPS / slice header ranges of 7.4.2.2 and 7.4.3
+    if (c.deblock_idc < 0 || c.deblock_idc > 2 || c.pps_id < 0 || c.pps_id > 255 || c.frame_num < 0 ||
+        c.idr_pic_id < 0 || c.idr_pic_id > 65535 || c.pic_init_qp_minus26 < -26 || c.pic_init_qp_minus26 > 25 ||
+        c.poc < 0 || (c.relax_chroma_skip != 0 && c.relax_chroma_skip != 1))
+        return -1;
     if (c.is_idr ? !modes : !mvs) return -1;
     if (((uintptr_t)out & 3) || out_cap < 4) return -1;
     if (work_size < work_bytes((int)total)) return -1;

@@ -196,7 +196,9 @@ def test_invalid_arguments_are_rejected(pkg):
     w = torch.zeros(1 << 20, dtype=torch.uint8, device="cuda")
     p = C.c_void_p(t.data_ptr())
     for bad in (dict(qp=52), dict(num_slices=0), dict(num_slices=65), dict(deblock_idc=3),
-                dict(log2_max_frame_num_minus4=13), dict(is_idr=2)):
+                dict(log2_max_frame_num_minus4=13), dict(is_idr=2), dict(pps_id=256), dict(idr_pic_id=65536),
+                dict(pic_init_qp_minus26=26), dict(pic_init_qp_minus26=-27), dict(relax_chroma_skip=2),
+                dict(poc=-1)):
         c = dict(width_in_mbs=2, height_in_mbs=2, qp=26, is_idr=0, num_slices=1)
         c.update(bad)
         cfg = pkg.h264_slice_cfg(**c)
