@@ -53,6 +53,27 @@ void h264_frame_dc_levels(const int32_t *dc, int nmb, int qp, int is_intra, int1
  * the EBSP.  Returns the bytes written, or 0 when cap (2 * RBSP + 5 bytes
  * per slice is always enough) is too small. */
 size_t bc250_rbsp_to_ebsp(uint8_t *dst, size_t cap, const uint8_t *src, size_t n);
+
+/* Sequence and picture parameter sets, with the fields the reference's
+ * h264_sps_t / h264_pps_t carry (bitstream.h:94-150; all int32 here), in the
+ * syntax of 7.3.2.1.1 / 7.3.2.2 that its bs_write_sps / bs_write_pps emit
+ * (bitstream.c:301-417): constraint flags 0, no scaling lists, VUI with an
+ * extended SAR and optional timing, one slice group.  Written as Annex B NAL
+ * units (nal_ref_idc 3); return the bytes written, 0 when cap is too small. */
+typedef struct bc250_h264_sps {
+    int32_t profile_idc, level_idc, sps_id, chroma_format_idc, bit_depth_luma, bit_depth_chroma;
+    int32_t log2_max_frame_num_minus4, pic_order_cnt_type, log2_max_poc_lsb_minus4, max_num_ref_frames;
+    int32_t width_in_mbs, height_in_mbs, frame_mbs_only, direct_8x8_inference;
+    int32_t frame_cropping, crop_left, crop_right, crop_top, crop_bottom;
+    int32_t vui_present, sar_width, sar_height, timing_info_present, num_units_in_tick, time_scale;
+} bc250_h264_sps_t;
+typedef struct bc250_h264_pps {
+    int32_t pps_id, sps_id, entropy_coding_mode, pic_order_present, num_ref_idx_l0_default,
+        num_ref_idx_l1_default, weighted_pred, weighted_bipred_idc, pic_init_qp_minus26, chroma_qp_offset,
+        second_chroma_qp_offset, deblocking_filter_control, constrained_intra_pred, transform_8x8_mode;
+} bc250_h264_pps_t;
+size_t bc250_h264_write_sps(const bc250_h264_sps_t *sps, uint8_t *dst, size_t cap);
+size_t bc250_h264_write_pps(const bc250_h264_pps_t *pps, uint8_t *dst, size_t cap);
 size_t bc250_h264_frame_nals(const uint8_t *rbsp, const uint32_t *slice_info, int num_slices, int is_idr,
                              uint8_t *dst, size_t cap);
 

@@ -189,3 +189,120 @@ size_t bc250_h264_frame_nals(const uint8_t *rbsp, const uint32_t *slice_info, in
     }
     return o;
 }
+
+/* ---- parameter sets: a list of syntax elements, then one bit writer ---- */
+enum { EL_U, EL_UE, EL_SE };
+typedef struct { int kind, bits; int64_t v; } ps_el;
+typedef struct { ps_el e[64]; int n; } ps_list;
+
+static void el(ps_list *l, int kind, int bits, int64_t v)
+{
+    if (l->n < 64) l->e[l->n++] = (ps_el){kind, bits, v};
+}
+
+/* the RBSP of the element list (with rbsp_trailing_bits), then the NAL */
+static size_t ps_nal(const ps_list *l, int nal_type, uint8_t *dst, size_t cap)
+{
+    uint8_t rbsp[256] = {0};
+    size_t bit = 0;
+#define PUT(nb, val)                                                                   \
+    for (int b_ = (nb) - 1; b_ >= 0; b_--, bit++)                                     \
+        if (((uint64_t)(val) >> b_) & 1) rbsp[bit >> 3] |= (uint8_t)(0x80 >> (bit & 7));
+    for (int i = 0; i < l->n; i++) {
+        const ps_el *x = &l->e[i];
+        if (x->kind == EL_U) {
+            PUT(x->bits, x->v);
+        } else {
+            const uint64_t k = x->kind == EL_UE ? (uint64_t)x->v : (x->v > 0 ? (uint64_t)(2 * x->v - 1) : (uint64_t)(-2 * x->v));
+            const uint64_t cn = k + 1;
+            int z = 0;
+            while ((cn >> (z + 1)) != 0) z++;
+            PUT(z, 0);
+            PUT(z + 1, cn);
+        }
+    }
+    PUT(1, 1);
+#undef PUT
+    const size_t n = (bit + 7) >> 3;
+    if (cap < 5 + n + n / 2 + 1) return 0;
+    dst[0] = dst[1] = dst[2] = 0;
+    dst[3] = 1;
+    dst[4] = (uint8_t)(3 << 5 | nal_type);
+    return 5 + bc250_rbsp_to_ebsp(dst + 5, cap - 5, rbsp, n);
+}
+
+size_t bc250_h264_write_sps(const bc250_h264_sps_t *p, uint8_t *dst, size_t cap)
+{
+    if (!p || !dst) return 0;
+    ps_list l = {.n = 0};
+    el(&l, EL_U, 8, p->profile_idc);
+    el(&l, EL_U, 8, 0); /* constraint_set0..5_flag, reserved_zero_2bits */
+    el(&l, EL_U, 8, p->level_idc);
+    el(&l, EL_UE, 0, p->sps_id);
+    if (p->profile_idc == 100) {
+        el(&l, EL_UE, 0, p->chroma_format_idc);
+        el(&l, EL_UE, 0, p->bit_depth_luma - 8);
+        el(&l, EL_UE, 0, p->bit_depth_chroma - 8);
+        el(&l, EL_U, 2, 0); /* qpprime_y_zero_transform_bypass, seq_scaling_matrix_present */
+    }
+    el(&l, EL_UE, 0, p->log2_max_frame_num_minus4);
+    el(&l, EL_UE, 0, p->pic_order_cnt_type);
+    if (p->pic_order_cnt_type == 0) el(&l, EL_UE, 0, p->log2_max_poc_lsb_minus4);
+    el(&l, EL_UE, 0, p->max_num_ref_frames);
+    el(&l, EL_U, 1, 0); /* gaps_in_frame_num_value_allowed */
+    el(&l, EL_UE, 0, p->width_in_mbs - 1);
+    el(&l, EL_UE, 0, p->height_in_mbs - 1);
+    el(&l, EL_U, 1, p->frame_mbs_only != 0);
+    if (!p->frame_mbs_only) el(&l, EL_U, 1, 0); /* mb_adaptive_frame_field */
+    el(&l, EL_U, 1, p->direct_8x8_inference != 0);
+    el(&l, EL_U, 1, p->frame_cropping != 0);
+    if (p->frame_cropping) {
+        el(&l, EL_UE, 0, p->crop_left);
+        el(&l, EL_UE, 0, p->crop_right);
+        el(&l, EL_UE, 0, p->crop_top);
+        el(&l, EL_UE, 0, p->crop_bottom);
+    }
+    el(&l, EL_U, 1, p->vui_present != 0);
+    if (p->vui_present) {
+        el(&l, EL_U, 1, 1);   /* aspect_ratio_info_present */
+        el(&l, EL_U, 8, 255); /* Extended_SAR */
+        el(&l, EL_U, 16, p->sar_width);
+        el(&l, EL_U, 16, p->sar_height);
+        el(&l, EL_U, 3, 0); /* overscan, video_signal_type, chroma_loc_info present flags */
+        el(&l, EL_U, 1, p->timing_info_present != 0);
+        if (p->timing_info_present) {
+            el(&l, EL_U, 32, (uint32_t)p->num_units_in_tick);
+            el(&l, EL_U, 32, (uint32_t)p->time_scale);
+            el(&l, EL_U, 1, 0); /* fixed_frame_rate */
+        }
+        el(&l, EL_U, 4, 0); /* nal / vcl hrd, pic_struct, bitstream_restriction flags */
+    }
+    return ps_nal(&l, 7, dst, cap);
+}
+
+size_t bc250_h264_write_pps(const bc250_h264_pps_t *p, uint8_t *dst, size_t cap)
+{
+    if (!p || !dst) return 0;
+    ps_list l = {.n = 0};
+    el(&l, EL_UE, 0, p->pps_id);
+    el(&l, EL_UE, 0, p->sps_id);
+    el(&l, EL_U, 1, p->entropy_coding_mode != 0);
+    el(&l, EL_U, 1, p->pic_order_present != 0);
+    el(&l, EL_UE, 0, 0); /* num_slice_groups_minus1 */
+    el(&l, EL_UE, 0, p->num_ref_idx_l0_default);
+    el(&l, EL_UE, 0, p->num_ref_idx_l1_default);
+    el(&l, EL_U, 1, p->weighted_pred != 0);
+    el(&l, EL_U, 2, p->weighted_bipred_idc);
+    el(&l, EL_SE, 0, p->pic_init_qp_minus26);
+    el(&l, EL_SE, 0, 0); /* pic_init_qs_minus26 */
+    el(&l, EL_SE, 0, p->chroma_qp_offset);
+    el(&l, EL_U, 1, p->deblocking_filter_control != 0);
+    el(&l, EL_U, 1, p->constrained_intra_pred != 0);
+    el(&l, EL_U, 1, 0); /* redundant_pic_cnt_present */
+    if (p->transform_8x8_mode) {
+        el(&l, EL_U, 1, 1);
+        el(&l, EL_U, 1, 0); /* pic_scaling_matrix_present */
+        el(&l, EL_SE, 0, p->second_chroma_qp_offset);
+    }
+    return ps_nal(&l, 8, dst, cap);
+}

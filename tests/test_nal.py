@@ -77,3 +77,75 @@ def test_frame_nals_layout(pkg):
         for p, s in zip(parts[1:], slices):
             assert p[0] == (0x65 if idr else 0x41)
             assert unescape(p[1:]) == s
+
+
+class _RefSps(C.Structure):  # the reference's h264_sps_t (bitstream.h), natural alignment
+    _fields_ = [(n, C.c_uint8) for n in ("profile_idc", "level_idc", "sps_id", "chroma_format_idc", "bit_depth_luma",
+                                         "bit_depth_chroma", "log2_max_frame_num", "pic_order_cnt_type",
+                                         "log2_max_poc_lsb")] + \
+               [(n, C.c_uint32) for n in ("max_num_ref_frames", "pic_width_in_mbs", "pic_height_in_mbs")] + \
+               [(n, C.c_bool) for n in ("frame_mbs_only", "direct_8x8_inference", "frame_cropping")] + \
+               [(n, C.c_uint32) for n in ("crop_left", "crop_right", "crop_top", "crop_bottom")] + \
+               [("vui_present", C.c_bool), ("sar_width", C.c_uint32), ("sar_height", C.c_uint32),
+                ("timing_info_present", C.c_bool), ("num_units_in_tick", C.c_uint32), ("time_scale", C.c_uint32)]
+
+
+class _RefPps(C.Structure):  # the reference's h264_pps_t
+    _fields_ = [("pps_id", C.c_uint8), ("sps_id", C.c_uint8), ("entropy_coding_mode", C.c_bool),
+                ("pic_order_present", C.c_bool), ("num_ref_idx_l0_default", C.c_uint8),
+                ("num_ref_idx_l1_default", C.c_uint8), ("weighted_pred", C.c_bool), ("weighted_bipred_idc", C.c_uint8),
+                ("pic_init_qp", C.c_int8), ("chroma_qp_offset", C.c_int8), ("second_chroma_qp_offset", C.c_int8),
+                ("deblocking_filter_control", C.c_bool), ("constrained_intra_pred", C.c_bool),
+                ("transform_8x8_mode", C.c_bool)]
+
+
+SPS_FIELDS = ("profile_idc", "level_idc", "sps_id", "chroma_format_idc", "bit_depth_luma", "bit_depth_chroma",
+              "log2_max_frame_num", "pic_order_cnt_type", "log2_max_poc_lsb", "max_num_ref_frames", "pic_width_in_mbs",
+              "pic_height_in_mbs", "frame_mbs_only", "direct_8x8_inference", "frame_cropping", "crop_left",
+              "crop_right", "crop_top", "crop_bottom", "vui_present", "sar_width", "sar_height",
+              "timing_info_present", "num_units_in_tick", "time_scale")
+PPS_FIELDS = ("pps_id", "sps_id", "entropy_coding_mode", "pic_order_present", "num_ref_idx_l0_default",
+              "num_ref_idx_l1_default", "weighted_pred", "weighted_bipred_idc", "pic_init_qp", "chroma_qp_offset",
+              "second_chroma_qp_offset", "deblocking_filter_control", "constrained_intra_pred", "transform_8x8_mode")
+
+
+@pytest.mark.skipif(not os.path.exists(orc.REF_CAVLC), reason="oracle/_ref not built (reference tree absent)")
+def test_parameter_sets_match_reference(pkg):
+    ref = C.CDLL(orc.REF_CAVLC)
+    ref.h264_sps_default.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint8]
+    ref.h264_pps_default.argtypes = [C.c_void_p, C.c_uint8, C.c_bool, C.c_int32]
+    for f in ("bs_write_sps", "bs_write_pps"):
+        getattr(ref, f).restype = C.c_size_t
+        getattr(ref, f).argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
+    for f in ("bc250_h264_write_sps", "bc250_h264_write_pps"):
+        getattr(pkg.host, f).restype = C.c_size_t
+        getattr(pkg.host, f).argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    a, b = np.zeros(512, np.uint8), np.zeros(512, np.uint8)
+    pa, pb = a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p)
+    cases = 0
+    for (w, h) in ((1920, 1080), (1280, 720), (640, 480), (200, 120), (3840, 2160)):
+        for fps in (30, 60):
+            for prof in (66, 77, 100):
+                for vui in (1, 0):
+                    s = _RefSps()
+                    ref.h264_sps_default(C.byref(s), w, h, fps, prof)
+                    s.vui_present = bool(vui)
+                    s.timing_info_present = fps == 60 or not vui
+                    mine = np.array([int(getattr(s, k)) for k in SPS_FIELDS], np.int32)
+                    n1 = ref.bs_write_sps(pa, 512, C.byref(s))
+                    n2 = pkg.host.bc250_h264_write_sps(mine.ctypes.data_as(C.c_void_p), pb, 512)
+                    assert n1 == n2 and (a[:n1] == b[:n2]).all(), (w, h, fps, prof, vui)
+                    cases += 1
+    for cabac in (False, True):
+        for qp in (0, 22, 26, 51):
+            for t8 in (0, 1):
+                s = _RefPps()
+                ref.h264_pps_default(C.byref(s), 0, cabac, qp)
+                s.transform_8x8_mode = bool(t8)
+                s.second_chroma_qp_offset = -2 if t8 else 0
+                mine = np.array([int(getattr(s, k)) for k in PPS_FIELDS], np.int32)
+                n1 = ref.bs_write_pps(pa, 512, C.byref(s))
+                n2 = pkg.host.bc250_h264_write_pps(mine.ctypes.data_as(C.c_void_p), pb, 512)
+                assert n1 == n2 and (a[:n1] == b[:n2]).all(), (cabac, qp, t8)
+                cases += 1
+    assert cases == 76
