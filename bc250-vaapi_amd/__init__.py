@@ -189,6 +189,32 @@ _sig(host, "h264_frame_dc_levels", None, [_P, _I, _I, _I, _P, _P])
 _sig(host, "h264_chroma_qp", _I, [_I])
 _sig(host, "bc250_rbsp_to_ebsp", C.c_size_t, [_P, C.c_size_t, _P, C.c_size_t])
 _sig(host, "bc250_h264_frame_nals", C.c_size_t, [_P, _P, _I, _I, _P, C.c_size_t])
+_sig(host, "bc250_h264_write_sps", C.c_size_t, [_P, _P, C.c_size_t])
+_sig(host, "bc250_h264_write_pps", C.c_size_t, [_P, _P, C.c_size_t])
+
+# bc250_h264_sps_t / bc250_h264_pps_t, field by field (all int32)
+SPS_FIELDS = ("profile_idc", "level_idc", "sps_id", "chroma_format_idc", "bit_depth_luma", "bit_depth_chroma",
+              "log2_max_frame_num_minus4", "pic_order_cnt_type", "log2_max_poc_lsb_minus4", "max_num_ref_frames",
+              "width_in_mbs", "height_in_mbs", "frame_mbs_only", "direct_8x8_inference", "frame_cropping",
+              "crop_left", "crop_right", "crop_top", "crop_bottom", "vui_present", "sar_width", "sar_height",
+              "timing_info_present", "num_units_in_tick", "time_scale")
+PPS_FIELDS = ("pps_id", "sps_id", "entropy_coding_mode", "pic_order_present", "num_ref_idx_l0_default",
+              "num_ref_idx_l1_default", "weighted_pred", "weighted_bipred_idc", "pic_init_qp_minus26",
+              "chroma_qp_offset", "second_chroma_qp_offset", "deblocking_filter_control", "constrained_intra_pred",
+              "transform_8x8_mode")
+
+
+def h264_parameter_sets(sps, pps):
+    """SPS and PPS NAL units (Annex B) from field dicts (missing fields 0)."""
+    out = b""
+    for fields, vals, f in ((SPS_FIELDS, sps, host.bc250_h264_write_sps), (PPS_FIELDS, pps, host.bc250_h264_write_pps)):
+        a = np.array([int(vals.get(k, 0)) for k in fields], np.int32)
+        buf = np.zeros(512, np.uint8)
+        n = f(_ptr(a), _ptr(buf), buf.size)
+        if n == 0:
+            raise RuntimeError("parameter set does not fit")
+        out += buf[:n].tobytes()
+    return out
 
 
 def rbsp_to_ebsp(rbsp):

@@ -271,3 +271,51 @@ def test_pipeline_slices_decode_to_recon(pkg):
     finally:
         enc.close()
         g.close()
+
+
+def test_pipeline_annexb_stream_decodes(pkg):
+    """SPS + PPS + the pipeline's slice NAL units of an IPPPP sequence form
+    one Annex B stream; split at its start codes, un-escaped, parsed and
+    decoded with the reference decoder's primitives, every frame equals the
+    pipeline's reconstruction (deblocking off)."""
+    import os
+    if not os.path.exists(orc.REF_H264):
+        pytest.skip("oracle/_ref not built")
+    from h264_cavlc_parse import SliceParser
+    from h264_decode import decode_frame
+    from test_nal import unescape
+    w, h, qp, ns = 320, 192, 28, 2
+    wmb, hmb = w // 16, h // 16
+    say("Annex B stream")
+    g = pkg.FrameGen(13, w, h, 4)
+    enc = pkg.Encoder(w, h, search_range=32, lambda8=1, deblock_idc=1, readback=0, entropy=1)
+    enc.set_slice_header(log2_max_frame_num_minus4=4, log2_max_poc_lsb_minus4=4, pic_init_qp_minus26=qp - 26)
+    stream = pkg.h264_parameter_sets(
+        dict(profile_idc=66, level_idc=40, chroma_format_idc=1, bit_depth_luma=8, bit_depth_chroma=8,
+             log2_max_frame_num_minus4=4, log2_max_poc_lsb_minus4=4, max_num_ref_frames=1, width_in_mbs=wmb,
+             height_in_mbs=hmb, frame_mbs_only=1, direct_8x8_inference=1, vui_present=1, sar_width=1, sar_height=1,
+             timing_info_present=1, num_units_in_tick=1, time_scale=60),
+        dict(pic_init_qp_minus26=qp - 26, deblocking_filter_control=1))
+    recons = []
+    try:
+        for i in range(5):
+            im = enc.new_image()
+            enc.upload(im, *g.frame(i))
+            slot = enc.encode(im, qp, i == 0, ns)
+            stream += pkg.h264_frame_nals(enc.slices(slot), i == 0)
+            recons.append(enc.map_recon())
+    finally:
+        enc.close()
+        g.close()
+    nals = stream.split(b"\x00\x00\x00\x01")[1:]
+    assert [n[0] & 31 for n in nals] == [7, 8] + [5] * ns + [1] * (4 * ns)
+    ry = ruv = None
+    for i in range(5):
+        ps = SliceParser(wmb, hmb, ns, 4, 4)
+        for s in range(ns):
+            ps.parse(s, unescape(nals[2 + i * ns + s][1:]))
+        dy, duv = decode_frame(ps, qp, ry, ruv)
+        np.testing.assert_array_equal(dy, recons[i][0], err_msg=f"frame {i} luma")
+        np.testing.assert_array_equal(duv, recons[i][1], err_msg=f"frame {i} chroma")
+        ry, ruv = recons[i]
+    say(f"  {len(stream)} bytes, 5 frames decode to the reconstruction")
