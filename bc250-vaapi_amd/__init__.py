@@ -183,6 +183,19 @@ _sig(host, "bc250_gen_frame", _I, [_P, _I, _P, _I, _P, _I])
 _sig(host, "bc250_gen_offset_qpel", None, [_P, _I, C.POINTER(_I), C.POINTER(_I)])
 _sig(host, "bc250_encode_sequence", _I, [_CTX, C.POINTER(GpuImage), _I, _I, _I, C.POINTER(SeqCfg),
                                         C.POINTER(C.c_double), C.POINTER(C.c_double)])
+# per-frame consumer of bc250_encode_sequence_cb: (user, frame, slot, is_intra) -> int
+FRAME_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int)
+_sig(host, "bc250_encode_sequence_cb", _I, [_CTX, C.POINTER(GpuImage), _I, _I, _I, C.POINTER(SeqCfg),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p, C.c_void_p])
+
+
+class NalSink(C.Structure):
+    """bc250_nal_sink_t (host/bc250_host.h)."""
+    _fields_ = [("ctx", C.c_void_p), ("buf", C.c_void_p), ("cap", C.c_size_t), ("used", C.c_size_t),
+                ("frame_bytes", C.c_void_p)]
+
+
+NAL_SINK_FRAME = C.cast(host.bc250_nal_sink_frame, C.c_void_p)
 _sig(host, "h264_intra16_luma_dc_transform", None, [_P, _I, _P, _P])
 _sig(host, "h264_chroma_dc_levels", None, [_P, _I, _P])
 _sig(host, "h264_frame_dc_levels", None, [_P, _I, _I, _I, _P, _P])
@@ -383,6 +396,7 @@ class Encoder:
         if hip.bc250_gpu_init(C.byref(self.ctx)) != 0:
             raise RuntimeError("bc250_gpu_init failed (no HIP device?)")
         self.codec = codec
+        self.entropy = entropy
         self.set_params(search_range, lambda8, deblock_idc, me_8x8, readback, codec, square_radius, diamond, ctu_log2,
                         entropy)
         self.images = []
@@ -396,6 +410,7 @@ class Encoder:
                          entropy)
         if hip.gpu_compute_set_encode_params(C.byref(self.ctx), C.byref(p)) != 0:
             raise ValueError("invalid encode parameters")
+        self.codec, self.entropy = codec, entropy
 
     def new_image(self):
         img, mem = GpuImage(), GpuMemory()
@@ -527,16 +542,43 @@ class Encoder:
     def error_flags(self):
         return hip.gpu_compute_take_error_flags(C.byref(self.ctx))
 
-    def encode_sequence(self, images, qp, gop=0, num_slices=1, pipelined=1):
+    def encode_sequence(self, images, qp, gop=0, num_slices=1, pipelined=1, collect=False, nal_sink=None):
+        """Runs the host sequence driver (bc250_encode_sequence_cb) over
+        `images`; returns (frame_ms, total_ms), plus, with collect, the list
+        of every frame's outputs read in the driver's per-frame consumer
+        (results() and, with entropy 1, "slices" and "is_intra").  nal_sink (a
+        NalSink) makes the C NAL writer the consumer instead."""
         n = len(images)
         arr = (GpuImage * n)(*[im for im, _ in images])
         cfg = SeqCfg(qp, gop, num_slices, pipelined)
         fms = (C.c_double * n)()
         tot = C.c_double()
-        rc = host.bc250_encode_sequence(C.byref(self.ctx), arr, n, self.w, self.h, C.byref(cfg), fms, C.byref(tot))
+        got = [None] * n
+        cb, user = None, None
+        if collect:
+            def on_frame(_user, frame, slot, is_intra):
+                try:
+                    r = self.results(slot)
+                    if self.entropy:
+                        r["slices"] = self.slices(slot)
+                    r["is_intra"] = bool(is_intra)
+                    got[frame] = r
+                    return 0
+                except Exception as e:  # reported by the driver's return code
+                    got[frame] = e
+                    return 1
+            cb = FRAME_CB(on_frame)
+        elif nal_sink is not None:
+            nal_sink.ctx = C.addressof(self.ctx)
+            cb, user = NAL_SINK_FRAME, C.c_void_p(C.addressof(nal_sink))
+        cbp = C.cast(cb, C.c_void_p) if cb is not None else None
+        rc = host.bc250_encode_sequence_cb(C.byref(self.ctx), arr, n, self.w, self.h, C.byref(cfg), fms, C.byref(tot),
+                                          cbp, user)
         if rc != 0:
-            raise RuntimeError("encode_sequence failed")
-        return np.array(fms[:], dtype=np.float64), tot.value
+            bad = [g for g in got if isinstance(g, Exception)]
+            raise RuntimeError("encode_sequence failed" + (f": {bad[0]}" if bad else ""))
+        out = np.array(fms[:], dtype=np.float64), tot.value
+        return out + (got,) if collect else out
 
     def close(self):
         if self.ctx.hip:

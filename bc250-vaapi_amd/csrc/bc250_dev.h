@@ -291,6 +291,34 @@ BC_DEV void st_sys64(uint64_t *p, uint64_t v) { __hip_atomic_store(p, v, __ATOMI
 // Waits until every store this wave issued has completed.
 BC_DEV void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// Launch epochs of the wavefront kernels.  Every hand-off record and
+// progress word a wavefront kernel publishes carries the 16-bit epoch of its
+// launch in its high half, (epoch << 16) | step, so a word left in memory by
+// an earlier launch -- the previous frame's hand-off at the same position --
+// never matches what a reader of this launch waits for, whatever the order
+// in which earlier clears and stores reach memory.
+BC_DEV uint32_t wf_tag(uint32_t epoch, int step) { return (epoch << 16) | (uint32_t)step; }
+
+namespace {
+// Runs on the launch stream just before a wavefront kernel: zeroes its
+// ticket, error flag and progress words (`n` ints at `zero`) and advances
+// the launch epoch at *epoch (1..65535, never 0), all with system-scope
+// stores, so that no plain store of these words is left in an L2 of one XCD
+// while workgroups on another XCD poll them.
+__global__ void wavefront_begin_kernel(int *zero, int n, uint32_t *epoch)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        __hip_atomic_store(zero + i, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (threadIdx.x == 0) {
+        uint32_t e = (__hip_atomic_load(epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) + 1) & 0xffffu;
+        __hip_atomic_store(epoch, e ? e : 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+}  // namespace
+// The epoch a wavefront kernel tags its words with (written by
+// wavefront_begin_kernel, which completed before the kernel started).
+BC_DEV uint32_t wf_epoch(const uint32_t *epoch) { return ld_sys(epoch); }
+
 // Slice of MB `mb` for start(s) = floor(s*total/ns).
 BC_DEV int slice_of(int mb, int total, int ns)
 {

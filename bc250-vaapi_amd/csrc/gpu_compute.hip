@@ -2,12 +2,16 @@
 // (include/gpu_compute.h), the drop-in for the reference's Vulkan
 // orchestration (approach1-compute-encoder/src/gpu_compute.c).
 //
-// One context = one HIP stream (default priority) + per-resolution device
-// buffers + two staging slots of pinned host memory.  dispatch_encode_ext
-// enqueues the stages of a frame on the stream; end_picture "submits" the
-// slot (records its completion event) and toggles the slot, as the
-// reference's vkQueueSubmit + slot flip (gpu_compute.c:3523-3619);
-// sync_slot waits on that event (vkWaitForFences, gpu_compute.c:3640-3712).
+// One context = a compute stream and a copy stream (both default priority)
+// + per-resolution device buffers + two slots.  A slot owns the device
+// buffers its frame's outputs are written to and the pinned host staging
+// they are copied into.  dispatch_encode_ext enqueues the stages of a frame
+// on the compute stream and the staging copies on the copy stream (after
+// the frame's stages), so the copies of frame N overlap the stages of frame
+// N+1; end_picture "submits" the slot (records its completion event on the
+// copy stream) and toggles the slot, as the reference's vkQueueSubmit + slot
+// flip (gpu_compute.c:3523-3619); sync_slot waits on that event
+// (vkWaitForFences, gpu_compute.c:3640-3712).
 #include <hip/hip_runtime.h>
 
 #include <stddef.h>
@@ -36,6 +40,14 @@ struct Plane2 {
     void *base = nullptr;
 };
 
+// Device buffers of one slot's frame outputs (what the host reads back).
+struct DevOut {
+    int16_t *levels = nullptr;
+    int32_t *dc = nullptr;
+    uint32_t *nz = nullptr, *modes = nullptr, *mvs = nullptr, *mv8 = nullptr;
+    uint8_t *hmodes = nullptr;
+};
+
 struct Slot {
     int16_t *levels = nullptr;
     int32_t *dc = nullptr;
@@ -47,13 +59,18 @@ struct Slot {
     uint32_t *rbsp_info = nullptr;
     int slices = 0;              // their number; 0 = no slices for this frame
     hipEvent_t ev[NEV] = {};
+    hipEvent_t cdone = nullptr;  // the frame's stages done (compute stream): its copies may start
+    DevOut dev;
     bool pending = false, recorded = false, have_data = false, was_intra = false;
 };
 
 }  // namespace
 
 struct bc250_hip_state {
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // stages
+    hipStream_t cstream = nullptr;  // staging copies
+    void *xfer = nullptr;           // pinned bounce buffer of uploads and downloads
+    size_t xfer_size = 0;
     // resolution-dependent state
     int w = 0, h = 0, wc = 0, hc = 0, mbw = 0, mbh = 0, nmb = 0;
     int bps = 1;           // bytes per sample of the reconstructions: 1 (NV12) or 2 (P010)
@@ -62,12 +79,15 @@ struct bc250_hip_state {
     int me8_pitch = 0;
     int ref_idx = 0;
     bool has_recon = false;
+    // outputs of the frame being dispatched: the current slot's DevOut (bind_slot)
     int16_t *levels = nullptr;
     int32_t *dc = nullptr;
     uint32_t *nz = nullptr, *modes = nullptr;
-    uint32_t *mvs = nullptr, *mv8 = nullptr, *imv16 = nullptr, *imv8 = nullptr;
+    uint32_t *mvs = nullptr, *mv8 = nullptr;
+    uint8_t *hmodes = nullptr;  // HEVC PU modes
+    uint32_t *imv16 = nullptr, *imv8 = nullptr;
     int *sync_i = nullptr, *sync_d = nullptr, *sync_h = nullptr;
-    uint8_t *hmodes = nullptr, *hcbf = nullptr;  // HEVC PU modes, per-CU coded-luma flags
+    uint8_t *hcbf = nullptr;     // per-CU coded-luma flags
     uint8_t *hculog2 = nullptr;                   // HEVC P: log2 CU size per CU8 (CTU32 CU choice)
     Slot slot[NSLOT];
     int cur = 0, submitted = 0;
@@ -79,7 +99,10 @@ struct bc250_hip_state {
     void *cavlc_work = nullptr;
     size_t cavlc_cap = 0, cavlc_work_size = 0;
     bc250_h264_slice_cfg_t hdr{};  // static header fields (gpu_compute_set_h264_slice_header)
-    int frame_num = 0, idr_pic_id = -1;
+    // the next frame's frame_num and POC (0 at an IDR, +1 / +2 per coded frame, written
+    // modulo 2^log2_max_*; encoder_h264.c:2258-2260,3526-3527) and the last idr_pic_id
+    unsigned frame_num = 0, poc = 0;
+    int idr_pic_id = -1;
     int err_flags = 0;
     // per-stage GPU time accumulated at sync_slot: [stage] sums, frames seen
     double stage_sum[NSTAGE] = {0, 0, 0, 0, 0};
@@ -161,16 +184,23 @@ extern "C" int bc250_launch_pad_edges(uint8_t *y, uint8_t *uv, int py, int puv, 
 static void free_res(bc250_hip_state *s)
 {
     if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->cstream) hipStreamSynchronize(s->cstream);
     for (auto &r : s->recon) {
         if (r.base) hipFree(r.base);
         r = Plane2{};
     }
     if (s->me8) hipFree(s->me8);
     s->me8 = nullptr;
-    void *dev[] = {s->levels, s->dc,     s->nz,     s->modes,  s->mvs,   s->mv8,
-                   s->imv16,  s->imv8, s->sync_i, s->sync_d, s->sync_h, s->hmodes, s->hcbf, s->hculog2};
+    void *dev[] = {s->imv16, s->imv8, s->sync_i, s->sync_d, s->sync_h, s->hcbf, s->hculog2};
     for (void *p : dev)
         if (p) hipFree(p);
+    for (auto &sl : s->slot) {
+        DevOut &o = sl.dev;
+        void *odev[] = {o.levels, o.dc, o.nz, o.modes, o.mvs, o.mv8, o.hmodes};
+        for (void *p : odev)
+            if (p) hipFree(p);
+        o = DevOut{};
+    }
     s->levels = nullptr; s->dc = nullptr; s->nz = nullptr; s->modes = nullptr; s->mvs = nullptr;
     s->mv8 = nullptr; s->imv16 = nullptr; s->imv8 = nullptr; s->sync_i = nullptr; s->sync_d = nullptr;
     s->sync_h = nullptr; s->hmodes = nullptr; s->hcbf = nullptr; s->hculog2 = nullptr;
@@ -215,14 +245,18 @@ static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
     bool ok = alloc_plane2(&s->recon[0], wc, hc, bps) == 0 && alloc_plane2(&s->recon[1], wc, hc, bps) == 0;
     s->me8_pitch = align_up(wc, 256);
     if (bps == 2) ok = ok && HIP_OK(hipMalloc(&s->me8, (size_t)s->me8_pitch * hc * 2));
-    ok = ok && HIP_OK(hipMalloc(&s->levels, (size_t)nmb * 384 * 2)) && HIP_OK(hipMalloc(&s->dc, (size_t)nmb * 24 * 4)) &&
-         HIP_OK(hipMalloc(&s->nz, (size_t)nmb * 24 * 4)) && HIP_OK(hipMalloc(&s->modes, (size_t)nmb * 4)) &&
-         HIP_OK(hipMalloc(&s->mvs, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->mv8, (size_t)nmb * 64)) &&
-         HIP_OK(hipMalloc(&s->imv16, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->imv8, (size_t)nmb * 64)) &&
+    for (auto &sl : s->slot) {
+        DevOut &o = sl.dev;
+        ok = ok && HIP_OK(hipMalloc(&o.levels, (size_t)nmb * 384 * 2)) && HIP_OK(hipMalloc(&o.dc, (size_t)nmb * 24 * 4)) &&
+             HIP_OK(hipMalloc(&o.nz, (size_t)nmb * 24 * 4)) && HIP_OK(hipMalloc(&o.modes, (size_t)nmb * 4)) &&
+             HIP_OK(hipMalloc(&o.mvs, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&o.mv8, (size_t)nmb * 64)) &&
+             HIP_OK(hipMalloc(&o.hmodes, (size_t)nmb * 16));
+    }
+    ok = ok && HIP_OK(hipMalloc(&s->imv16, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->imv8, (size_t)nmb * 64)) &&
          HIP_OK(hipMalloc(&s->sync_i, bc250_intra_work_bytes(wc, hc))) &&
          HIP_OK(hipMalloc(&s->sync_d, bc250_deblock_work_bytes(wc, hc))) &&
          HIP_OK(hipMalloc(&s->sync_h, bc250_hevc_intra_work_bytes(wc, hc))) &&
-         HIP_OK(hipMalloc(&s->hmodes, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->hcbf, (size_t)nmb * 4)) &&
+         HIP_OK(hipMalloc(&s->hcbf, (size_t)nmb * 4)) &&
          HIP_OK(hipMalloc(&s->hculog2, (size_t)nmb * 4));
     for (auto &sl : s->slot) {
         ok = ok && HIP_OK(hipHostMalloc((void **)&sl.levels, (size_t)nmb * 384 * 2, hipHostMallocDefault)) &&
@@ -234,6 +268,11 @@ static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
              HIP_OK(hipHostMalloc((void **)&sl.hmodes, (size_t)nmb * 16, hipHostMallocDefault)) &&
              HIP_OK(hipHostMalloc((void **)&sl.err, 2 * sizeof(int), hipHostMallocDefault));
     }
+    // wavefront workspaces start zeroed: no hand-off record or epoch holds a
+    // value from whatever used the memory before
+    ok = ok && HIP_OK(hipMemset(s->sync_i, 0, bc250_intra_work_bytes(wc, hc))) &&
+         HIP_OK(hipMemset(s->sync_d, 0, bc250_deblock_work_bytes(wc, hc))) &&
+         HIP_OK(hipMemset(s->sync_h, 0, bc250_hevc_intra_work_bytes(wc, hc)));
     if (!ok) {
         fprintf(stderr, "[bc250-hip] buffer allocation failed for %dx%d\n", w, h);
         free_res(s);
@@ -243,6 +282,45 @@ static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
     s->has_recon = false;
     s->ref_idx = 0;
     return 0;
+}
+
+// Points the context's output pointers at slot `k`'s device buffers.
+static void bind_slot(bc250_hip_state *s, int k)
+{
+    const DevOut &o = s->slot[k].dev;
+    s->levels = o.levels;
+    s->dc = o.dc;
+    s->nz = o.nz;
+    s->modes = o.modes;
+    s->mvs = o.mvs;
+    s->mv8 = o.mv8;
+    s->hmodes = o.hmodes;
+}
+
+// Grows the pinned bounce buffer of uploads / downloads to `bytes`.
+static void *ensure_xfer(bc250_hip_state *s, size_t bytes)
+{
+    if (s->xfer_size >= bytes) return s->xfer;
+    if (s->xfer) hipHostFree(s->xfer);
+    s->xfer = nullptr;
+    s->xfer_size = 0;
+    if (!HIP_OK(hipHostMalloc(&s->xfer, bytes, hipHostMallocDefault))) return nullptr;
+    s->xfer_size = bytes;
+    return s->xfer;
+}
+
+// The wavefront kernels' wait-error flags of the frame ([0] intra, [1]
+// deblock; a null source reads as 0), stored straight into the slot's
+// pinned flag words by the compute stream at the end of the frame.
+__global__ void collect_flags_kernel(const int *e0, const int *e1, int *dst)
+{
+    if (threadIdx.x == 0) {
+        const int a = e0 ? __hip_atomic_load(e0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        const int b = e1 ? __hip_atomic_load(e1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        __hip_atomic_store(dst, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(dst + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
+    }
 }
 
 // Device and pinned host buffers of the CAVLC writer: room for 2560 bytes
@@ -305,13 +383,27 @@ extern "C" int bc250_gpu_init(bc250_gpu_context_t *ctx)
         delete s;
         return -1;
     }
-    for (auto &sl : s->slot)
-        for (auto &e : sl.ev)
-            if (!HIP_OK(hipEventCreate(&e))) {
-                hipStreamDestroy(s->stream);
-                delete s;
-                return -1;
-            }
+    if (!HIP_OK(hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking))) {
+        hipStreamDestroy(s->stream);
+        delete s;
+        return -1;
+    }
+    bool ev_ok = true;
+    for (auto &sl : s->slot) {
+        for (auto &e : sl.ev) ev_ok = ev_ok && HIP_OK(hipEventCreate(&e));
+        ev_ok = ev_ok && HIP_OK(hipEventCreateWithFlags(&sl.cdone, hipEventDisableTiming));
+    }
+    if (!ev_ok) {
+        for (auto &sl : s->slot) {
+            for (auto &e : sl.ev)
+                if (e) hipEventDestroy(e);
+            if (sl.cdone) hipEventDestroy(sl.cdone);
+        }
+        hipStreamDestroy(s->cstream);
+        hipStreamDestroy(s->stream);
+        delete s;
+        return -1;
+    }
     ctx->hip = s;
     ctx->frames_submitted = 0;
     ctx->has_recon_frame = false;
@@ -325,9 +417,13 @@ extern "C" void bc250_gpu_destroy(bc250_gpu_context_t *ctx)
     if (!s) return;
     free_res(s);
     if (s->map_host) hipHostFree(s->map_host);
-    for (auto &sl : s->slot)
+    if (s->xfer) hipHostFree(s->xfer);
+    for (auto &sl : s->slot) {
         for (auto &e : sl.ev)
             if (e) hipEventDestroy(e);
+        if (sl.cdone) hipEventDestroy(sl.cdone);
+    }
+    hipStreamDestroy(s->cstream);
     hipStreamDestroy(s->stream);
     delete s;
     ctx->hip = nullptr;
@@ -377,7 +473,10 @@ extern "C" int gpu_compute_create_image(gpu_context_t *ctx, int width, int heigh
 extern "C" void gpu_compute_destroy_image(gpu_context_t *ctx, gpu_image_t image, gpu_memory_t memory)
 {
     (void)image;
-    if (ctx && ctx->hip) hipStreamSynchronize(ctx->hip->stream);
+    if (ctx && ctx->hip) {
+        hipStreamSynchronize(ctx->hip->stream);
+        hipStreamSynchronize(ctx->hip->cstream);
+    }
     if (memory.memory) hipFree(memory.memory);
 }
 
@@ -433,11 +532,18 @@ extern "C" int gpu_compute_upload_nv12(gpu_context_t *ctx, gpu_image_t *image, g
     if (!s || !image || !memory.memory || !y_plane || !uv_plane) return -1;
     const int bpp = image->format == GPU_IMAGE_P010 ? 2 : 1;
     if (width <= 0 || height <= 0 || (uint32_t)width > image->width || (uint32_t)height > image->height) return -1;
-    bool ok = HIP_OK(hipMemcpy2DAsync(image->y_plane, image->y_pitch, y_plane, y_pitch, (size_t)width * bpp, height,
-                                      hipMemcpyHostToDevice, s->stream)) &&
-              HIP_OK(hipMemcpy2DAsync(image->uv_plane, image->uv_pitch, uv_plane, uv_pitch,
-                                      (size_t)((width + 1) / 2) * 2 * bpp, (height + 1) / 2, hipMemcpyHostToDevice,
-                                      s->stream));
+    // the caller's (pageable) rows are packed into the context's pinned
+    // buffer on the host, then copied by DMA in stream order
+    const size_t yb = (size_t)width * bpp, uvb = (size_t)((width + 1) / 2) * 2 * bpp;
+    const int ch = (height + 1) / 2;
+    uint8_t *x = (uint8_t *)ensure_xfer(s, yb * height + uvb * ch);
+    if (!x) return -1;
+    for (int r = 0; r < height; r++) memcpy(x + yb * r, y_plane + (size_t)y_pitch * r, yb);
+    for (int r = 0; r < ch; r++) memcpy(x + yb * height + uvb * r, uv_plane + (size_t)uv_pitch * r, uvb);
+    bool ok = HIP_OK(hipMemcpy2DAsync(image->y_plane, image->y_pitch, x, yb, yb, height, hipMemcpyHostToDevice,
+                                      s->stream)) &&
+              HIP_OK(hipMemcpy2DAsync(image->uv_plane, image->uv_pitch, x + yb * height, uvb, uvb, ch,
+                                      hipMemcpyHostToDevice, s->stream));
     if (ok)
         ok = pad_edges((uint8_t *)image->y_plane, (uint8_t *)image->uv_plane, image->y_pitch, image->uv_pitch, width,
                        height, image->coded_width, image->coded_height, bpp, s->stream) == 0;
@@ -480,6 +586,24 @@ extern "C" int gpu_compute_video_proc(gpu_context_t *, gpu_image_t *, const int 
 
 extern "C" void gpu_compute_copy_from_wc(void *dst, const void *src, size_t n) { memcpy(dst, src, n); }
 
+// Device planes -> the caller's rows, through the pinned bounce buffer.
+static int download_planes(bc250_hip_state *s, const uint8_t *dy, int dpy, const uint8_t *duv, int dpuv,
+                           uint8_t *y_plane, int y_pitch, uint8_t *uv_plane, int uv_pitch, size_t row_bytes,
+                           size_t uvb, int height)
+{
+    const int ch = (height + 1) / 2;
+    uint8_t *x = (uint8_t *)ensure_xfer(s, row_bytes * height + uvb * ch);
+    if (!x) return -1;
+    bool ok = HIP_OK(hipMemcpy2DAsync(x, row_bytes, dy, dpy, row_bytes, height, hipMemcpyDeviceToHost, s->stream)) &&
+              HIP_OK(hipMemcpy2DAsync(x + row_bytes * height, uvb, duv, dpuv, uvb, ch, hipMemcpyDeviceToHost,
+                                      s->stream)) &&
+              HIP_OK(hipStreamSynchronize(s->stream));
+    if (!ok) return -1;
+    for (int r = 0; r < height; r++) memcpy(y_plane + (size_t)y_pitch * r, x + row_bytes * r, row_bytes);
+    for (int r = 0; r < ch; r++) memcpy(uv_plane + (size_t)uv_pitch * r, x + row_bytes * height + uvb * r, uvb);
+    return 0;
+}
+
 extern "C" int gpu_compute_download_nv12(gpu_context_t *ctx, gpu_image_t *image, gpu_memory_t memory,
                                          uint8_t *y_plane, int y_pitch, uint8_t *uv_plane, int uv_pitch, int width,
                                          int height)
@@ -487,13 +611,10 @@ extern "C" int gpu_compute_download_nv12(gpu_context_t *ctx, gpu_image_t *image,
     bc250_hip_state *s = st_of(ctx);
     if (!s || !image || !memory.memory || !y_plane || !uv_plane || width <= 0 || height <= 0) return -1;
     const int bpp = image->format == GPU_IMAGE_P010 ? 2 : 1;
-    bool ok = HIP_OK(hipMemcpy2DAsync(y_plane, y_pitch, image->y_plane, image->y_pitch, (size_t)width * bpp, height,
-                                      hipMemcpyDeviceToHost, s->stream)) &&
-              HIP_OK(hipMemcpy2DAsync(uv_plane, uv_pitch, image->uv_plane, image->uv_pitch,
-                                      (size_t)((width + 1) / 2) * 2 * bpp, (height + 1) / 2, hipMemcpyDeviceToHost,
-                                      s->stream)) &&
-              HIP_OK(hipStreamSynchronize(s->stream));
-    return ok ? 0 : -1;
+    if ((uint32_t)width > image->coded_width || (uint32_t)height > image->coded_height) return -1;
+    return download_planes(s, (const uint8_t *)image->y_plane, image->y_pitch, (const uint8_t *)image->uv_plane,
+                           image->uv_pitch, y_plane, y_pitch, uv_plane, uv_pitch, (size_t)width * bpp,
+                           (size_t)((width + 1) / 2) * 2 * bpp, height);
 }
 
 extern "C" int gpu_compute_export_nv12_dmabuf(gpu_context_t *, gpu_memory_t, int *) { return -1; }
@@ -544,6 +665,43 @@ extern "C" int gpu_compute_begin_picture(gpu_context_t *ctx, gpu_image_t render_
         sl.pending = false;
     }
     return 0;
+}
+
+// Starts a frame in the current slot: the compute stream first waits for the
+// slot's previous copies (its device outputs are about to be rewritten), then
+// the slot's device buffers become the frame's outputs.
+static void start_frame(bc250_hip_state *s)
+{
+    Slot &sl = s->slot[s->cur];
+    if (sl.recorded) hipStreamWaitEvent(s->stream, sl.ev[NEV - 1], 0);
+    bind_slot(s, s->cur);
+}
+
+// Ends a frame's stages: the wavefront error flags (e0 intra, e1 deblock;
+// null = the stage did not run) go to the slot's pinned flag words, and,
+// with readback, the copy stream copies the listed outputs to the slot's
+// pinned staging once the stages are done.
+enum { RB_LEVELS = 1, RB_DC = 2, RB_MODES = 4, RB_MVS = 8, RB_NZ = 16, RB_MV8 = 32, RB_HMODES = 64 };
+static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *e1, int what)
+{
+    const size_t n = (size_t)s->nmb;
+    collect_flags_kernel<<<1, 64, 0, s->stream>>>(e0, e1, sl.err);
+    if (!HIP_OK(hipGetLastError()) || !HIP_OK(hipEventRecord(sl.cdone, s->stream)) ||
+        !HIP_OK(hipStreamWaitEvent(s->cstream, sl.cdone, 0)))
+        return -1;
+    if (!s->params.readback) return 0;
+    const DevOut &o = sl.dev;
+    hipStream_t cs = s->cstream;
+    const hipMemcpyKind k = hipMemcpyDeviceToHost;
+    bool ok = true;
+    if (what & RB_LEVELS) ok = ok && HIP_OK(hipMemcpyAsync(sl.levels, o.levels, n * 384 * 2, k, cs));
+    if (what & RB_DC) ok = ok && HIP_OK(hipMemcpyAsync(sl.dc, o.dc, n * 24 * 4, k, cs));
+    if (what & RB_MODES) ok = ok && HIP_OK(hipMemcpyAsync(sl.modes, o.modes, n * 4, k, cs));
+    if (what & RB_MVS) ok = ok && HIP_OK(hipMemcpyAsync(sl.mvs, o.mvs, n * 16, k, cs));
+    if (what & RB_NZ) ok = ok && HIP_OK(hipMemcpyAsync(sl.nz, o.nz, n * 24 * 4, k, cs));
+    if (what & RB_MV8) ok = ok && HIP_OK(hipMemcpyAsync(sl.mv8, o.mv8, n * 64, k, cs));
+    if (what & RB_HMODES) ok = ok && HIP_OK(hipMemcpyAsync(sl.hmodes, o.hmodes, n * 16, k, cs));
+    return ok ? 0 : -1;
 }
 
 static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &ref, int me_mode, hipEvent_t mid)
@@ -608,9 +766,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
     hipEventRecord(sl.ev[2], st);
     if (r == 0) {
         if (is_intra) {
-            r = bc250_launch_zero_i32(s->sync_h, 2, st);
-            if (r == 0)
-                r = bc250_hevc_launch_i_encode_ctu(cur.y, cur.uv, cur.py, cur.puv, out.y, out.uv, out.py, out.puv,
+            r = bc250_hevc_launch_i_encode_ctu(cur.y, cur.uv, cur.py, cur.puv, out.y, out.uv, out.py, out.puv,
                                                    s->wc, s->hc, qp, num_slices,
                                                    4 /* rounding of intra CUs in an I picture */, s->hmodes,
                                                    (int16_t *)s->levels, s->sync_h, kSpinLimit, p.hevc_ctu_log2, bd,
@@ -638,17 +794,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
                                               s->mv8, cus, num_slices, p.hevc_ctu_log2, bd, st);
     }
     hipEventRecord(sl.ev[4], st);
-    if (r == 0 && p.readback) {
-        bool ok = HIP_OK(hipMemcpyAsync(sl.levels, s->levels, n * 384 * 2, hipMemcpyDeviceToHost, st)) &&
-                  HIP_OK(hipMemcpyAsync(sl.hmodes, s->hmodes, n * 16, hipMemcpyDeviceToHost, st)) &&
-                  HIP_OK(hipMemcpyAsync(sl.mv8, s->mv8, n * 64, hipMemcpyDeviceToHost, st));
-        r = ok ? 0 : -1;
-    }
-    if (r == 0) {
-        bool ok = HIP_OK(hipMemcpyAsync(&sl.err[0], s->sync_h + 1, sizeof(int), hipMemcpyDeviceToHost, st));
-        sl.err[1] = 0;
-        r = ok ? 0 : -1;
-    }
+    if (r == 0) r = finish_frame(s, sl, is_intra ? s->sync_h + 1 : nullptr, nullptr, RB_LEVELS | RB_HMODES | RB_MV8);
     return r;
 }
 
@@ -691,6 +837,7 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
             fprintf(stderr, "[bc250-hip] cpu_mvs is an H.264 route; HEVC searches on the GPU\n");
             return -1;
         }
+        start_frame(s);
         if (enqueue_hevc(s, sl, cur_in, ref, out, qp, is_intra, num_slices) != 0) {
             fprintf(stderr, "[bc250-hip] HEVC dispatch failed (%dx%d qp %d intra %d)\n", width, height, qp, is_intra);
             return -1;
@@ -704,6 +851,8 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
         return 0;
     }
 
+    if (p.entropy && (num_slices > 64 || num_slices > s->nmb)) return -1;
+    start_frame(s);
     int r = 0;
     hipEventRecord(sl.ev[0], st);
     if (!is_intra && !cpu_mvs) {
@@ -716,9 +865,7 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     hipEventRecord(sl.ev[2], st);
     if (r == 0) {
         if (is_intra) {
-            r = bc250_launch_zero_i32(s->sync_i, s->mbh + 2, st);
-            if (r == 0)
-                r = bc250_launch_i_encode(cur_in.y, cur_in.uv, cur_in.py, cur_in.puv, out.y, out.uv, out.py, out.puv,
+            r = bc250_launch_i_encode(cur_in.y, cur_in.uv, cur_in.py, cur_in.puv, out.y, out.uv, out.py, out.puv,
                                           s->wc, s->hc, qp, num_slices, s->levels, s->dc, s->nz, s->modes, s->sync_i,
                                           kSpinLimit, st);
             if (r == 0) r = bc250_launch_zero_i32((int *)s->mvs, s->nmb * 4, st);
@@ -731,32 +878,28 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     }
     hipEventRecord(sl.ev[3], st);
     if (r == 0 && p.deblock_idc != 1) {
-        r = bc250_launch_zero_i32(s->sync_d, 2, st);
-        if (r == 0)
-            r = bc250_launch_deblock(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->nz, s->mvs,
+        r = bc250_launch_deblock(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->nz, s->mvs,
                                      num_slices, p.deblock_idc, s->sync_d, kSpinLimit, st);
     }
     hipEventRecord(sl.ev[4], st);
+    // frame_num / POC / idr_pic_id of this frame, committed once it is enqueued
+    const unsigned fn = is_intra ? 0u : s->frame_num, poc = is_intra ? 0u : s->poc;
+    const int idr_id = is_intra ? (s->idr_pic_id + 1) & 0xffff : (s->idr_pic_id < 0 ? 0 : s->idr_pic_id);
     if (r == 0 && p.entropy) {
         // CAVLC slices of the frame, numbered the IPPP way (IDR at intra frames)
-        if (num_slices < 1 || num_slices > 64 || num_slices > s->nmb || ensure_cavlc(s) != 0) {
+        if (ensure_cavlc(s) != 0) {
             r = -1;
         } else {
             bc250_h264_slice_cfg_t c = s->hdr;
-            if (is_intra) {
-                s->frame_num = 0;
-                s->idr_pic_id = (s->idr_pic_id + 1) & 0xffff;
-            } else {
-                s->frame_num = (s->frame_num + 1) & ((1 << (c.log2_max_frame_num_minus4 + 4)) - 1);
-            }
             c.width_in_mbs = s->mbw;
             c.height_in_mbs = s->mbh;
             c.qp = qp;
             c.is_idr = is_intra ? 1 : 0;
             c.num_slices = num_slices;
-            c.frame_num = s->frame_num;
-            c.idr_pic_id = s->idr_pic_id < 0 ? 0 : s->idr_pic_id;
-            c.poc = 2 * s->frame_num;
+            // written modulo 2^log2_max (encoder_h264.c:3102,3108)
+            c.frame_num = (int)(fn & ((1u << (c.log2_max_frame_num_minus4 + 4)) - 1));
+            c.idr_pic_id = idr_id;
+            c.poc = (int)(poc & ((1u << (c.log2_max_poc_lsb_minus4 + 4)) - 1));
             c.deblock_idc = p.deblock_idc;
             r = bc250_h264_cavlc_slices(s->levels, s->dc, s->modes, (const gpu_mv_t *)s->mvs, &c, s->cavlc_out,
                                         s->cavlc_cap, s->cavlc_info, s->cavlc_work, s->cavlc_work_size, st);
@@ -766,26 +909,11 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
             if (r == 0) sl.slices = num_slices;
         }
     }
-    if (r == 0 && p.readback) {
-        const size_t n = (size_t)s->nmb;
-        bool ok = HIP_OK(hipMemcpyAsync(sl.levels, s->levels, n * 384 * 2, hipMemcpyDeviceToHost, st)) &&
-                  HIP_OK(hipMemcpyAsync(sl.dc, s->dc, n * 24 * 4, hipMemcpyDeviceToHost, st)) &&
-                  HIP_OK(hipMemcpyAsync(sl.modes, s->modes, n * 4, hipMemcpyDeviceToHost, st)) &&
-                  HIP_OK(hipMemcpyAsync(sl.mvs, s->mvs, n * 16, hipMemcpyDeviceToHost, st)) &&
-                  HIP_OK(hipMemcpyAsync(sl.nz, s->nz, n * 24 * 4, hipMemcpyDeviceToHost, st)) &&
-                  HIP_OK(hipMemcpyAsync(sl.mv8, s->mv8, n * 64, hipMemcpyDeviceToHost, st));
-        r = ok ? 0 : -1;
-    }
-    if (r == 0) {
-        // wait-error flags of the two wavefront kernels (sync[1]); with
-        // deblocking off the deblock workspace is never cleared or written
-        bool ok = HIP_OK(hipMemcpyAsync(&sl.err[0], s->sync_i + 1, sizeof(int), hipMemcpyDeviceToHost, st));
-        if (p.deblock_idc != 1)
-            ok = ok && HIP_OK(hipMemcpyAsync(&sl.err[1], s->sync_d + 1, sizeof(int), hipMemcpyDeviceToHost, st));
-        else
-            sl.err[1] = 0;
-        r = ok ? 0 : -1;
-    }
+    // wait-error flags of the wavefront kernels this frame ran (sync[1]);
+    // with deblocking off the deblock workspace is never written
+    if (r == 0)
+        r = finish_frame(s, sl, is_intra ? s->sync_i + 1 : nullptr, p.deblock_idc != 1 ? s->sync_d + 1 : nullptr,
+                         RB_LEVELS | RB_DC | RB_MODES | RB_MVS | RB_NZ | RB_MV8);
     sl.have_data = r == 0 && p.readback;
     sl.was_intra = is_intra != 0;
     sl.recorded = false;
@@ -795,6 +923,9 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     }
     s->ref_idx = out_idx;
     s->has_recon = true;
+    if (is_intra) s->idr_pic_id = idr_id;
+    s->frame_num = fn + 1;
+    s->poc = poc + 2;
     publish_recon(ctx, s);
     return 0;
 }
@@ -821,15 +952,11 @@ extern "C" int gpu_compute_dispatch_me_only(gpu_context_t *ctx, gpu_image_t rt, 
     cur_in.y = (uint8_t *)rt.y_plane;
     cur_in.py = rt.y_pitch;
     const Plane2 &ref = s->has_recon ? s->recon[s->ref_idx] : cur_in;
+    start_frame(s);
     hipEventRecord(sl.ev[0], s->stream);
     int r = enqueue_me(s, rt, ref, 0, sl.ev[1]);
     for (int i = 2; i < NEV - 1; i++) hipEventRecord(sl.ev[i], s->stream);
-    if (r == 0 && s->params.readback) {
-        bool ok = HIP_OK(hipMemcpyAsync(sl.mvs, s->mvs, (size_t)s->nmb * 16, hipMemcpyDeviceToHost, s->stream)) &&
-                  HIP_OK(hipMemcpyAsync(sl.mv8, s->mv8, (size_t)s->nmb * 64, hipMemcpyDeviceToHost, s->stream));
-        r = ok ? 0 : -1;
-    }
-    if (sl.err) sl.err[0] = sl.err[1] = 0;
+    if (r == 0) r = finish_frame(s, sl, nullptr, nullptr, RB_MVS | RB_MV8);
     sl.have_data = false;
     sl.recorded = false;
     sl.slices = 0;
@@ -846,7 +973,8 @@ extern "C" int gpu_compute_end_picture(gpu_context_t *ctx)
     bc250_hip_state *s = st_of(ctx);
     if (!s) return -1;
     Slot &sl = s->slot[s->cur];
-    if (!HIP_OK(hipEventRecord(sl.ev[NEV - 1], s->stream))) return -1;
+    // the copy stream has waited for the frame's stages: its last event ends the frame
+    if (!HIP_OK(hipEventRecord(sl.ev[NEV - 1], s->cstream))) return -1;
     sl.pending = true;
     sl.recorded = true;
     s->submitted = s->cur;
@@ -993,6 +1121,8 @@ extern "C" int gpu_compute_get_slices_slot(gpu_context_t *ctx, int slot, const u
     if (!s || !rbsp || !info || !num_slices || slot < 0 || slot >= NSLOT) return -1;
     const Slot &sl = s->slot[slot];
     if (!sl.slices || sl.pending || !sl.rbsp) return -1;
+    // a wavefront wait that gave up leaves levels built from stale neighbours
+    if (sl.err && (sl.err[0] | sl.err[1])) return -1;
     if (sl.rbsp_info[4 * sl.slices + 1] || sl.rbsp_info[4 * sl.slices] > s->cavlc_cap) return -1;
     *rbsp = sl.rbsp;
     *info = sl.rbsp_info;
@@ -1014,11 +1144,8 @@ extern "C" int gpu_compute_download_recon(gpu_context_t *ctx, uint8_t *y_plane, 
     bc250_hip_state *s = st_of(ctx);
     if (!s || !s->has_recon || !y_plane || !uv_plane) return -1;
     const Plane2 &r = s->recon[s->ref_idx];
-    bool ok = HIP_OK(hipMemcpy2DAsync(y_plane, y_pitch, r.y, r.py, s->w, s->h, hipMemcpyDeviceToHost, s->stream)) &&
-              HIP_OK(hipMemcpy2DAsync(uv_plane, uv_pitch, r.uv, r.puv, (size_t)((s->w + 1) / 2) * 2, (s->h + 1) / 2,
-                                      hipMemcpyDeviceToHost, s->stream)) &&
-              HIP_OK(hipStreamSynchronize(s->stream));
-    return ok ? 0 : -1;
+    return download_planes(s, r.y, r.py, r.uv, r.puv, y_plane, y_pitch, uv_plane, uv_pitch, (size_t)s->w * s->bps,
+                           (size_t)((s->w + 1) / 2) * 2 * s->bps, s->h);
 }
 
 extern "C" int gpu_compute_get_stage_times(gpu_context_t *ctx, int slot, float ms[6])

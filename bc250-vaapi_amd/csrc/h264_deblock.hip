@@ -19,13 +19,15 @@
 //   - hands the bottom rows that the next row's top edge reads and partly
 //     rewrites (luma 12..15, chroma 6..7) to the wave below: through an LDS
 //     ring inside the workgroup, or to the next workgroup as system-coherent
-//     64-bit records {tag = x + 1, 4 samples} that the reader polls until
-//     every tag matches, so the writer never waits for its stores,
+//     64-bit records {tag = (epoch << 16) | (x + 1), 4 samples} that the
+//     reader polls until every tag matches, so the writer never waits for
+//     its stores; the launch epoch makes the previous frame's record at the
+//     same place a mismatch,
 //   - stores the other rows to the frame; the wave below stores the rows it
 //     rewrote (luma 13..15, chroma 7), so each frame byte has one writer.
-// The records are cleared by the boundary-strength kernel before each
-// frame.  No cache writeback or invalidate is issued.  Waits are bounded and
-// report on *err.
+// The records are also cleared, by system-scope stores of the
+// boundary-strength kernel, before each frame.  No cache writeback or
+// invalidate is issued.  Waits are bounded and report on *err.
 #include "bc250_dev.h"
 #include "bc250_stages.h"
 
@@ -143,7 +145,7 @@ __global__ void h264_bs_kernel(int mbw, int mbh, int is_intra, const uint32_t *_
     const int nmb = mbw * mbh;
     if (m >= nmb) return;
     const int x = m % mbw, y = m / mbw;
-    if (j < GHAND) ghand[(size_t)m * GHAND + j] = 0;  // no valid tag
+    if (j < GHAND) st_sys64(ghand + (size_t)m * GHAND + j, 0);  // no valid tag
     const int dir = j >> 4, e = (j >> 2) & 3, i = j & 3;
     int b = 0;
     bool exists = true;
@@ -181,7 +183,7 @@ __global__ void h264_bs_kernel(int mbw, int mbh, int is_intra, const uint32_t *_
 template <bool INTRA>
 __global__ __launch_bounds__(128 * ROWS) void h264_deblock_kernel(
     uint8_t *__restrict__ y_, uint8_t *__restrict__ uv, int py, int puv, int mbw, int mbh, int qp,
-    const uint4 *__restrict__ bsw, int *sync /* [0] ticket, [1] err */,
+    const uint4 *__restrict__ bsw, int *sync /* [0] ticket, [1] err, [2] epoch */,
     uint64_t *ghand /* [mbh][mbw][GHAND] {tag, value} */, unsigned spin_limit)
 {
     __shared__ LumaLds ll[ROWS];
@@ -194,6 +196,7 @@ __global__ __launch_bounds__(128 * ROWS) void h264_deblock_kernel(
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int chroma = wv >= ROWS, r4 = wv & (ROWS - 1);
     int *err = sync + 1;
+    const uint32_t ep = wf_epoch((const uint32_t *)sync + 2);
     if (tid == 0) s_group = atomicAdd(sync, 1);
     if (tid < ROWS) prog[0][tid] = prog[1][tid] = cons[0][tid] = cons[1][tid] = 0;
     __syncthreads();
@@ -254,7 +257,7 @@ __global__ __launch_bounds__(128 * ROWS) void h264_deblock_kernel(
                     if (lane == 0) vcons[r4] = x + 1;
                 } else {
                     const uint64_t *src = gh + ((size_t)(yrow - 1) * mbw + x) * GHAND + lane;
-                    const uint32_t tag = (uint32_t)x + 1;
+                    const uint32_t tag = wf_tag(ep, x + 1);
                     unsigned n = 0;
                     uint64_t q = 0;
                     for (;;) {
@@ -380,7 +383,7 @@ __global__ __launch_bounds__(128 * ROWS) void h264_deblock_kernel(
                 wave_lds_sync();
                 if (lane == 0) vprog[r4] = x;
             } else if (lane < nhand) {
-                st_sys64(gh + ((size_t)yrow * mbw + x - 1) * GHAND + lane, ((uint64_t)x << 32) | h);
+                st_sys64(gh + ((size_t)yrow * mbw + x - 1) * GHAND + lane, ((uint64_t)wf_tag(ep, x) << 32) | h);
             }
         }
         // ---- frame stores: MB x-1 (rows this wave owns) and the rewritten rows of MB (x, y-1)
@@ -435,14 +438,15 @@ extern "C" int bc250_launch_deblock(uint8_t *y, uint8_t *uv, int py, int puv, in
                                     int *sync, unsigned spin_limit, void *stream)
 {
     if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || (py & 3) || (puv & 3) ||
-        num_slices < 1 || idc < 0 || idc > 2)
+        num_slices < 1 || idc < 0 || idc > 2 || wc / 16 >= 65535)
         return -1;
     if (idc == 1) return 0;
     hipStream_t st = (hipStream_t)stream;
     const int mbw = wc / 16, mbh = hc / 16, nmb = mbw * mbh;
-    // workspace: 256 B of flags | hand-off records | bS words
+    // workspace: 256 B of flags (ticket, error, epoch) | hand-off records | bS words
     uint64_t *ghand = (uint64_t *)((uint8_t *)sync + 256);
     uint4 *bsw = (uint4 *)(ghand + (size_t)nmb * GHAND);
+    wavefront_begin_kernel<<<1, 64, 0, st>>>(sync, 2, (uint32_t *)sync + 2);
     h264_bs_kernel<<<(nmb + 7) / 8, 256, 0, st>>>(mbw, mbh, is_intra, nz, (const uint4 *)mvs, num_slices, idc, bsw,
                                                   ghand);
     const int groups = (mbh + ROWS - 1) / ROWS;

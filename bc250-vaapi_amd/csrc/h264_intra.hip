@@ -9,7 +9,9 @@
 // reconstructed MB x it hands that MB's bottom luma row and bottom chroma
 // row (32 bytes) to the wave below: through an LDS ring inside the
 // workgroup, or through a system-coherent record in global memory to the
-// next workgroup, published by a progress counter after an s_waitcnt.  No
+// next workgroup, published by a progress counter after an s_waitcnt.  The
+// counter carries the launch epoch in its high half, so a counter left at
+// "row done" by the previous I picture never releases a reader early.  No
 // cache writeback or invalidate is needed and the frame itself is only
 // written, never read back, during the kernel.
 // Per MB: I16x16 SAD mode decision (DC, V, H, Plane, strict <), residual,
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(256) void h264_i_encode_kernel(
     const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv,
     uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy, int opuv, int mbw, int mbh, int qp,
     int num_slices, int16_t *__restrict__ levels, int32_t *__restrict__ dc, uint32_t *__restrict__ nz,
-    uint32_t *__restrict__ modes, int *sync /* [0] ticket, [1] err, [2..2+mbh) progress */,
+    uint32_t *__restrict__ modes, int *sync /* [0] ticket, [1] err, [2..2+mbh) progress, [2+mbh] epoch */,
     uint32_t *ghand /* [mbh][mbw][HAND] */, unsigned spin_limit)
 {
     __shared__ IntraLds wl[WAVES];
@@ -82,6 +84,7 @@ __global__ __launch_bounds__(256) void h264_i_encode_kernel(
 
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     int *err = sync + 1, *gprog = sync + 2;
+    const uint32_t ep = wf_epoch((const uint32_t *)sync + 2 + mbh);
     if (tid == 0) s_group = atomicAdd(sync, 1);
     if (tid < WAVES) { lprog[tid] = 0; lcons[tid] = 0; }
     __syncthreads();
@@ -135,8 +138,11 @@ __global__ __launch_bounds__(256) void h264_i_encode_kernel(
                 if (lane == 0) vlcons[w] = x + 1;
             } else {
                 if (lane == 0) {
+                    // progress words of this launch are (ep << 16) | (MBs done)
                     unsigned n = 0;
-                    while ((int)ld_sys((const uint32_t *)&gprog[y - 1]) < x + 1) {
+                    for (;;) {
+                        const uint32_t v = ld_sys((const uint32_t *)&gprog[y - 1]);
+                        if ((v >> 16) == ep && (int)(v & 0xffffu) >= x + 1) break;
                         __builtin_amdgcn_s_sleep(1);
                         if (++n > spin_limit) { atomicOr(err, 1); break; }
                     }
@@ -348,16 +354,20 @@ __global__ __launch_bounds__(256) void h264_i_encode_kernel(
                 if (lane < HAND) st_sys(ghand + ((size_t)y * mbw + x) * HAND + lane, h);
                 wait_stores();
                 __builtin_amdgcn_wave_barrier();
-                if (lane == 0) st_sys((uint32_t *)&gprog[y], (uint32_t)(x + 1));
+                if (lane == 0) st_sys((uint32_t *)&gprog[y], wf_tag(ep, x + 1));
             }
         }
     }
 }
 
+// Workspace: a 256-byte-aligned header of mbh + 3 ints (ticket, error flag,
+// one progress word per MB row, launch epoch), then the hand-off records.
+static size_t intra_header_bytes(int mbh) { return ((size_t)(mbh + 3) * sizeof(int) + 255) & ~(size_t)255; }
+
 extern "C" size_t bc250_intra_work_bytes(int wc, int hc)
 {
     const int mbw = wc / 16, mbh = hc / 16;
-    return (((size_t)(mbh + 2) * sizeof(int) + 255) & ~(size_t)255) + (size_t)mbh * mbw * HAND * 4;
+    return intra_header_bytes(mbh) + (size_t)mbh * mbw * HAND * 4;
 }
 
 extern "C" int bc250_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, uint8_t *oy,
@@ -366,11 +376,12 @@ extern "C" int bc250_launch_i_encode(const uint8_t *cy, const uint8_t *cuv, int 
                                      unsigned spin_limit, void *stream)
 {
     if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || (opy & 3) || (opuv & 7) ||
-        (cpy & 3) || (cpuv & 7) || num_slices < 1)
+        (cpy & 3) || (cpuv & 7) || num_slices < 1 || wc / 16 >= 65535)
         return -1;
     const int mbw = wc / 16, mbh = hc / 16;
-    uint32_t *ghand = (uint32_t *)((uint8_t *)sync + (((size_t)(mbh + 2) * sizeof(int) + 255) & ~(size_t)255));
+    uint32_t *ghand = (uint32_t *)((uint8_t *)sync + intra_header_bytes(mbh));
     const int groups = (mbh + WAVES - 1) / WAVES;
+    wavefront_begin_kernel<<<1, 256, 0, (hipStream_t)stream>>>(sync, mbh + 2, (uint32_t *)sync + 2 + mbh);
     h264_i_encode_kernel<<<groups, 64 * WAVES, 0, (hipStream_t)stream>>>(cy, cuv, cpy, cpuv, oy, ouv, opy, opuv,
                                                                           mbw, mbh, qp, num_slices, levels, dc, nz,
                                                                           modes, sync, ghand, spin_limit);

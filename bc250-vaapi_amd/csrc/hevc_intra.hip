@@ -17,8 +17,10 @@
 // CTU needs the bottom rows of CTUs x-1, x, x+1 of the row above (corner,
 // top, top-right); each finished CTU hands its bottom luma / Cb / Cr row (32
 // samples) to the wave below through an LDS ring, or to the next workgroup as
-// system-coherent 64-bit records {tag = x + 1, one dword of samples} that the
-// reader polls until every tag matches (cleared before the launch).  Rows
+// system-coherent 64-bit records {tag = (epoch << 16) | (x + 1), one dword
+// of samples} that the reader polls until every tag matches (cleared by
+// system-scope stores before the launch; the launch epoch keeps the previous
+// picture's record at the same place from ever matching).  Rows
 // that start a slice read nothing above and wait for nothing.  Waits are
 // bounded and report on *err.
 //
@@ -109,7 +111,7 @@ BC_DEV uint32_t row16_sum(uint32_t v)
 __global__ void hevc_clear_records_kernel(uint64_t *rec, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) rec[i] = 0;
+    if (i < n) st_sys64(rec + i, 0);
 }
 
 template <typename Pix, int BD>
@@ -131,6 +133,7 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
 
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     int *err = sync + 1;
+    const uint32_t ep = wf_epoch((const uint32_t *)sync + 2);
     if (tid == 0) s_group = atomicAdd(sync, 1);
     if (tid < WAVES) prog[tid] = cons[tid] = done[tid] = 0;
     __syncthreads();
@@ -209,7 +212,7 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
                 bool ok = true;
                 if (lane < REC) {
                     qv = ld_sys64(src);
-                    ok = (uint32_t)(qv >> 32) == (uint32_t)j + 1;
+                    ok = (uint32_t)(qv >> 32) == wf_tag(ep, j + 1);
                 }
                 if (__all(ok)) break;
                 __builtin_amdgcn_s_sleep(1);
@@ -395,7 +398,7 @@ __global__ __launch_bounds__(256) void hevc_i_encode_kernel(
                 wave_lds_sync();
                 if (lane == 0) vprog[wv] = x + 1;
             } else if (lane < REC) {
-                st_sys64(ghand + ((size_t)yrow * wtu + x) * REC + lane, ((uint64_t)(x + 1) << 32) | h);
+                st_sys64(ghand + ((size_t)yrow * wtu + x) * REC + lane, ((uint64_t)wf_tag(ep, x + 1) << 32) | h);
             }
         }
         // ---- the CTU's reconstruction to the frame
@@ -434,7 +437,8 @@ static void launch_i(const uint8_t *cy, const uint8_t *cuv, int cpy, int cpuv, u
 {
     constexpr int REC = 32 / (4 / (int)sizeof(Pix));
     const int htu = hc / 16, nrec = (wc / 16) * htu * REC;
-    uint64_t *ghand = (uint64_t *)((uint8_t *)sync + 256);
+    uint64_t *ghand = (uint64_t *)((uint8_t *)sync + 256);  // after 256 B of flags: ticket, error, epoch
+    wavefront_begin_kernel<<<1, 64, 0, st>>>(sync, 2, (uint32_t *)sync + 2);
     hevc_clear_records_kernel<<<(nrec + 255) / 256, 256, 0, st>>>(ghand, nrec);
     hevc_i_encode_kernel<Pix, BD><<<(htu + WAVES - 1) / WAVES, 64 * WAVES, 0, st>>>(
         cy, cuv, cpy, cpuv, oy, ouv, opy, opuv, wc, hc, qp, num_slices, round_q12, modes, coef, sync, ghand,
@@ -448,7 +452,7 @@ extern "C" int bc250_hevc_launch_i_encode_ctu(const void *cy, const void *cuv, i
 {
     const int b = bd == 8 ? 1 : 2, al = 4 * b - 1;
     if ((bd != 8 && bd != 10) || (ctu_log2 != 4 && ctu_log2 != 5) || wc <= 0 || hc <= 0 || (wc & 15) ||
-        (hc & 15) || qp < 0 || qp > 51 ||
+        (hc & 15) || qp < 0 || qp > 51 || wc / 16 >= 65535 ||
         num_slices < 1 || round_q12 < 0 || round_q12 > 12 || (cpy & al) || (cpuv & al) || (opy & al) ||
         (opuv & al) || cpy < wc * b || cpuv < wc * b || opy < wc * b || opuv < wc * b)
         return -1;

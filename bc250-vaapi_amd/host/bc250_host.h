@@ -92,6 +92,29 @@ typedef struct {
 int bc250_encode_sequence(gpu_context_t *ctx, const gpu_image_t *frames, int n, int width, int height,
                           const bc250_seq_cfg_t *cfg, double *frame_ms, double *total_ms);
 
+/* Per-frame consumer of the sequence driver, called right after frame
+ * `frame` synced in `slot`: its staging (readback 1) and slices (entropy 1)
+ * stay valid until the driver submits the frame after next into the same
+ * slot.  A non-zero return stops the run (the driver then returns -1). */
+typedef int (*bc250_frame_cb)(void *user, int frame, int slot, int is_intra);
+/* bc250_encode_sequence with a consumer (on_frame may be NULL); a frame
+ * whose sync fails stops the run before its consumer is called. */
+int bc250_encode_sequence_cb(gpu_context_t *ctx, const gpu_image_t *frames, int n, int width, int height,
+                             const bc250_seq_cfg_t *cfg, double *frame_ms, double *total_ms, bc250_frame_cb on_frame,
+                             void *user);
+
+/* A consumer that writes the H.264 elementary stream: each frame's CAVLC
+ * slices (gpu_compute_get_slices_slot) as Annex B NAL units, appended to
+ * buf.  frame_bytes[frame] (optional, n entries) receives each frame's size.
+ * Stops the run when the slices are missing or buf is full. */
+typedef struct {
+    gpu_context_t *ctx;
+    uint8_t *buf;
+    size_t cap, used;
+    size_t *frame_bytes;
+} bc250_nal_sink_t;
+int bc250_nal_sink_frame(void *user, int frame, int slot, int is_intra);
+
 #ifdef __cplusplus
 }
 #endif

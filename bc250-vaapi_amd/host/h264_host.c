@@ -11,6 +11,7 @@
 #include <string.h>
 #include <time.h>
 
+#include "bc250_bitstream.h"
 #include "bc250_host.h"
 
 static const int DC_LS0[6] = {160, 176, 208, 224, 256, 288};
@@ -120,7 +121,29 @@ static double now_ms(void)
 int bc250_encode_sequence(gpu_context_t *ctx, const gpu_image_t *frames, int n, int width, int height,
                           const bc250_seq_cfg_t *cfg, double *frame_ms, double *total_ms)
 {
+    return bc250_encode_sequence_cb(ctx, frames, n, width, height, cfg, frame_ms, total_ms, NULL, NULL);
+}
+
+int bc250_nal_sink_frame(void *user, int frame, int slot, int is_intra)
+{
+    bc250_nal_sink_t *k = (bc250_nal_sink_t *)user;
+    const uint8_t *rbsp = NULL;
+    const uint32_t *info = NULL;
+    int ns = 0;
+    if (!k || !k->buf || gpu_compute_get_slices_slot(k->ctx, slot, &rbsp, &info, &ns) != 0) return -1;
+    const size_t n = bc250_h264_frame_nals(rbsp, info, ns, is_intra, k->buf + k->used, k->cap - k->used);
+    if (n == 0) return -1;
+    k->used += n;
+    if (k->frame_bytes) k->frame_bytes[frame] = n;
+    return 0;
+}
+
+int bc250_encode_sequence_cb(gpu_context_t *ctx, const gpu_image_t *frames, int n, int width, int height,
+                             const bc250_seq_cfg_t *cfg, double *frame_ms, double *total_ms, bc250_frame_cb on_frame,
+                             void *user)
+{
     if (!ctx || !frames || n <= 0 || !cfg) return -1;
+    int slot_intra[2] = {0, 0};
     double t_submit[2] = {0, 0};
     int slot_frame[2] = {-1, -1};
     const double t0 = now_ms();
@@ -138,20 +161,27 @@ int bc250_encode_sequence(gpu_context_t *ctx, const gpu_image_t *frames, int n, 
         const int slot = gpu_compute_submitted_slot(ctx);
         t_submit[slot] = ts;
         slot_frame[slot] = i;
+        slot_intra[slot] = is_intra;
         /* pipelined: finish the frame submitted before this one; else this one */
         const int fin = cfg->pipelined ? slot ^ 1 : slot;
         if (slot_frame[fin] >= 0) {
             if (gpu_compute_sync_slot(ctx, fin) != 0) rc = -1;
             if (frame_ms) frame_ms[slot_frame[fin]] = now_ms() - t_submit[fin];
+            if (rc == 0 && on_frame && on_frame(user, slot_frame[fin], fin, slot_intra[fin]) != 0) rc = -1;
             slot_frame[fin] = -1;
         }
     }
-    for (int s = 0; s < 2 && rc == 0; s++)
-        if (slot_frame[s] >= 0) {
-            if (gpu_compute_sync_slot(ctx, s) != 0) rc = -1;
-            if (frame_ms) frame_ms[slot_frame[s]] = now_ms() - t_submit[s];
-            slot_frame[s] = -1;
-        }
+    /* the last frames, oldest first */
+    for (int k = 0; k < 2 && rc == 0; k++) {
+        int s = -1;
+        for (int t = 0; t < 2; t++)
+            if (slot_frame[t] >= 0 && (s < 0 || slot_frame[t] < slot_frame[s])) s = t;
+        if (s < 0) break;
+        if (gpu_compute_sync_slot(ctx, s) != 0) rc = -1;
+        if (frame_ms) frame_ms[slot_frame[s]] = now_ms() - t_submit[s];
+        if (rc == 0 && on_frame && on_frame(user, slot_frame[s], s, slot_intra[s]) != 0) rc = -1;
+        slot_frame[s] = -1;
+    }
     if (total_ms) *total_ms = now_ms() - t0;
     return rc;
 }

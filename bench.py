@@ -10,13 +10,20 @@ intra wavefront, then 8.7.2 deblocking.  The H.264 pipeline (the reference's gpu
 with deblocking) runs on the same frames and is reported beside it.  The
 timed region is an IPPP sequence of --steps frames driven by the host C
 sequence driver (host/h264_host.c) through the gpu_compute_* ABI, pipelined
-over the two staging slots.
+over the two slots, with readback on: every frame's outputs (levels, modes,
+vectors) reach the slot's pinned host staging, as the reference's
+gpu_compute.c:3328-3353 copies them, before the frame counts as done.  The
+H.264 line with slices instead ends each frame with its CAVLC slices written
+on the GPU and packed as Annex B NAL units on the host by the driver's C
+consumer, so its output is the elementary stream.
 
 Prints ONE JSON line.  Besides the contract fields it carries the per-stage
-GPU times from HIP events recorded on the pipeline's own stream during the
-timed region, the roofline of the dominant kernel (the integer full search),
-and a CPU baseline: the oracle timed on a bounded sample of the same
-workload on this host.
+GPU times from HIP events recorded on the pipeline's own streams during the
+timed region, the same sequence at CQP 22 / 27 / 32 / 37, the roofline of
+the dominant kernel (the integer full search), and a CPU baseline: the
+oracle timed on a bounded sample of the same workload on this host, in the
+same leg that checks a short run of every timed pipeline, frame by frame,
+against the oracle (the run fails if any frame differs).
 """
 import argparse
 import importlib.util
@@ -133,32 +140,82 @@ def cpu_baseline(frames_coded, qp, rng, lam, budget_s, codec):
             "frame_s": round(frame_s, 2)}
 
 
-def run_pipeline(pkg, torch, frames, codec, args, entropy=0):
-    """Uploads the frames to a fresh context, runs the warm-up and the timed
-    IPPP sequence; returns fps, per-frame latencies and the stage statistics
-    from the pipeline's own HIP events.  entropy 1 (H.264): every frame also
-    writes its CAVLC slices and copies them to the host."""
-    enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, me_8x8=1, readback=0,
+def run_pipeline(pkg, torch, frames, codec, args, entropy=0, qps=None):
+    """Uploads the frames to a fresh context and, for each QP, runs the
+    warm-up and the timed IPPP sequence; returns, per QP, fps, per-frame
+    latencies and the stage statistics from the pipeline's own HIP events.
+    entropy 0: readback on (the staging reaches the host); entropy 1
+    (H.264): every frame writes its CAVLC slices, which the driver's C
+    consumer packs into an Annex B stream in host memory."""
+    enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, me_8x8=1,
+                      readback=0 if entropy else 1,
                       codec=pkg.GPU_CODEC_HEVC if codec == "hevc" else pkg.GPU_CODEC_H264, entropy=entropy)
+    out = {}
     try:
         images = []
         for y, uv in frames:
             im = enc.new_image()
             enc.upload(im, y, uv)
             images.append(im)
-        log(f"{codec}: warmup")
-        enc.encode_sequence(images[:args.warmup], args.qp, gop=0, pipelined=1)
-        enc.stage_stats(reset=True)
-        log(f"{codec}: timed {args.steps} frames IPPP")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        frame_ms, _ = enc.encode_sequence(images[args.warmup:], args.qp, gop=0, pipelined=1)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        return dict(wall=wall, fps=args.steps / wall, frame_ms=frame_ms, stats=enc.stage_stats(reset=True),
-                    errs=enc.error_flags())
+        stream = np.zeros(64 << 20, np.uint8) if entropy else None
+        sizes = np.zeros(args.steps, np.uint64)
+        for qp in qps or [args.qp]:
+            def seq(imgs):
+                if not entropy:
+                    return enc.encode_sequence(imgs, qp, gop=0, pipelined=1)
+                sink = pkg.NalSink(0, stream.ctypes.data, stream.size, 0, sizes.ctypes.data)
+                r = enc.encode_sequence(imgs, qp, gop=0, pipelined=1, nal_sink=sink)
+                seq.bytes = sink.used
+                return r
+            log(f"{codec}{' + slices' if entropy else ''} qp {qp}: warmup")
+            seq(images[:args.warmup])
+            enc.stage_stats(reset=True)
+            log(f"{codec}{' + slices' if entropy else ''} qp {qp}: timed {args.steps} frames IPPP")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            frame_ms, _ = seq(images[args.warmup:])
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            out[qp] = dict(wall=wall, fps=args.steps / wall, frame_ms=frame_ms, stats=enc.stage_stats(reset=True),
+                           errs=enc.error_flags())
+            if entropy:
+                out[qp]["stream_bytes"] = int(seq.bytes)
+        return out
     finally:
         enc.close()
+
+
+def verify_pipelines(pkg, frames, args, codecs, n=4):
+    """The checker half of the CPU leg: a fresh context per timed pipeline
+    encodes the first n frames exactly as timed (sequence driver, pipelined,
+    the same parameters) with a consumer that reads every frame's outputs,
+    and tests/seqcheck.py compares them with the oracle frame by frame.
+    Raises on the first difference."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import seqcheck
+    coded = [pkg.pad_coded(*f) for f in frames[:n]]
+    done = []
+    for codec, entropy in codecs:
+        enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, me_8x8=1, readback=1,
+                          codec=pkg.GPU_CODEC_HEVC if codec == "hevc" else pkg.GPU_CODEC_H264, entropy=entropy)
+        try:
+            images = []
+            for y, uv in frames[:n]:
+                im = enc.new_image()
+                enc.upload(im, y, uv)
+                images.append(im)
+            _, _, got = enc.encode_sequence(images, args.qp, gop=0, pipelined=1, collect=True)
+        finally:
+            enc.close()
+        log(f"checking {codec}{' + slices' if entropy else ''} ({n} frames) against the oracle")
+        if codec == "hevc":
+            seqcheck.check_hevc(coded, got, args.qp, 1, args.range, args.lambda8, sample=32)
+        else:
+            seqcheck.check_h264(coded, got, args.qp, 1, args.range, args.lambda8, sample=32)
+        done.append(codec + (" with CAVLC slices" if entropy else ""))
+    return (f"{', '.join(done)}: {n} frames (I + {n - 1} P) each through the timed path, every frame's outputs "
+            "equal to the oracle's (whole-frame coding and slices; the +-64 search on 32 sampled MBs + the corners)")
 
 
 def bench_main10_4k(pkg, torch, args):
@@ -176,7 +233,7 @@ def bench_main10_4k(pkg, torch, args):
         frames.append((y.astype(np.uint16) * 4 + rs.integers(0, 4, y.shape, dtype=np.uint16),
                        uv.astype(np.uint16) * 4 + rs.integers(0, 4, uv.shape, dtype=np.uint16)))
     g.close()
-    enc = pkg.Encoder(w, h, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, me_8x8=1, readback=0,
+    enc = pkg.Encoder(w, h, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, me_8x8=1, readback=1,
                       codec=pkg.GPU_CODEC_HEVC, bit_depth=10, ctu_log2=5)
     try:
         images = []
@@ -280,35 +337,30 @@ def bench_cavlc(pkg, torch, args):
     on the launch stream.  Algorithmic bytes per frame: the staging read once
     (nmb x (768 + 96 + 16 + 4)) plus the RBSP written."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
+    import seqcheck
     wmb, hmb, ns, nf = (W + 15) // 16, (H + 15) // 16, 4, 8
     nmb = wmb * hmb
-    keys = ("quant", "dc", "pred_mode", "mv")
-    frames = []
-    if args.staging and os.path.exists(args.staging):
-        log(f"cavlc: staging of {nf} frames from {args.staging}")
-        z = np.load(args.staging)
-        for f in range(nf):
-            r = {k: z[f"{k}{f}"] for k in keys}
-            r["mv"] = r["mv"].view(pkg.MV_DTYPE).reshape(-1)
-            frames.append((r, pkg.h264_slice_cfg(wmb, hmb, args.qp, f == 0, ns, frame_num=f, poc=2 * f)))
-    else:
-        log(f"cavlc: encoding {nf} frames {W}x{H} for their staging")
-        g = pkg.FrameGen(1080, W, H, 12)
-        enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, readback=1)
-        try:
-            for f in range(nf):
-                im = enc.new_image()
-                enc.upload(im, *g.frame(f))
-                r = enc.results(enc.encode(im, args.qp, f == 0, ns))
-                cfg = pkg.h264_slice_cfg(wmb, hmb, args.qp, f == 0, ns, frame_num=f, poc=2 * f)
-                frames.append((r, cfg))
-        finally:
-            enc.close()
-            g.close()
-        if args.staging:
-            np.savez(args.staging, **{f"{k}{f}": (r[k].view(np.int32) if k == "mv" else r[k])
-                                      for f, (r, _) in enumerate(frames) for k in keys})
+    log(f"cavlc: encoding {nf} frames {W}x{H} for their staging")
+    g = pkg.FrameGen(1080, W, H, 12)
+    pics = [g.frame(f) for f in range(nf)]
+    g.close()
+    enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, readback=1)
+    try:
+        images = []
+        for y, uv in pics:
+            im = enc.new_image()
+            enc.upload(im, y, uv)
+            images.append(im)
+        _, _, outs = enc.encode_sequence(images, args.qp, gop=0, num_slices=ns, pipelined=1, collect=True)
+    finally:
+        enc.close()
+    # the staging must be the pipeline's true output before the writer is timed on it
+    log("cavlc: checking the staging against the oracle")
+    seqcheck.check_h264([pkg.pad_coded(*f) for f in pics], outs, args.qp, ns, args.range, args.lambda8, sample=32)
+    frames = [(r, pkg.h264_slice_cfg(wmb, hmb, args.qp, f == 0, ns, frame_num=f, poc=2 * f))
+              for f, r in enumerate(outs)]
     dev = [tuple(torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in
                  (r["quant"], r["dc"], r["pred_mode"].view(np.int32), r["mv"].view(np.int32).reshape(-1)))
            for r, _ in frames]
@@ -372,7 +424,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=8)
-    ap.add_argument("--qp", type=int, default=27)
+    ap.add_argument("--qp", type=int, default=27, help="QP of the headline value")
+    ap.add_argument("--qps", default="22,27,32,37", help="QPs of the qp_sweep object (comma list; '' for none)")
     ap.add_argument("--range", type=int, default=64)
     ap.add_argument("--lambda8", type=int, default=1)
     ap.add_argument("--codec", choices=("hevc", "h264"), default="hevc",
@@ -380,8 +433,6 @@ def main():
     ap.add_argument("--single", action="store_true", help="skip the second codec's run")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle integer search")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--staging", default=None,
-                    help="cavlc: .npz of the frames' staging, written by a run that encodes them, read when present")
     ap.add_argument("--workload", choices=("1080p", "4k-main10", "t32", "cavlc"), default="1080p",
                     help="4k-main10: the 3840x2160 P010 HEVC Main10 workload instead of the 1080p headline; "
                          "t32: the 32x32 transform forms compared; cavlc: the H.264 CAVLC slice writer")
@@ -408,10 +459,12 @@ def main():
     frames = [g.frame(i) for i in range(n)]
     g.close()
     other = "h264" if args.codec == "hevc" else "hevc"
-    runs = {args.codec: run_pipeline(pkg, torch, frames, args.codec, args)}
+    qps = sorted(set([args.qp] + [int(q) for q in args.qps.split(",") if q]))
+    sweeps = {args.codec: run_pipeline(pkg, torch, frames, args.codec, args, qps=qps)}
     if not args.single:
-        runs[other] = run_pipeline(pkg, torch, frames, other, args)
-        runs["h264_cavlc"] = run_pipeline(pkg, torch, frames, "h264", args, entropy=1)
+        sweeps[other] = run_pipeline(pkg, torch, frames, other, args, qps=qps)
+        sweeps["h264_cavlc"] = run_pipeline(pkg, torch, frames, "h264", args, entropy=1, qps=qps)
+    runs = {k: v[args.qp] for k, v in sweeps.items()}
     head = runs[args.codec]
     st = head["stats"]
 
@@ -464,8 +517,16 @@ def main():
         "p50_frame_ms": round(float(np.percentile(head["frame_ms"], 50)), 4),
         "p99_frame_ms": round(float(np.percentile(head["frame_ms"], 99)), 4),
         "mpixel_per_s": round(head["fps"] * W * H / 1e6, 1),
+        "readback": "on: every frame's levels, modes and vectors copied to pinned host staging inside the "
+                    "timed region (HEVC has no entropy coder here; the H.264 line with slices ends in the "
+                    "Annex B stream in host memory)",
         "stage_ms": stage_ms(st),
         "wavefront_wait_errors": head["errs"],
+        "qp_sweep": {name: {str(q): {"fps": round(r["fps"], 2),
+                                     "p99_frame_ms": round(float(np.percentile(r["frame_ms"], 99)), 4),
+                                     **({"stream_bytes_per_frame": round(r["stream_bytes"] / args.steps)}
+                                        if "stream_bytes" in r else {})}
+                             for q, r in sw.items()} for name, sw in sweeps.items()},
         "roofline": {
             "bound": "hbm",
             "kernel": "me_fullsearch_kernel<64>",
@@ -494,16 +555,22 @@ def main():
                                     "stage_ms": stage_ms(o["stats"]), "wavefront_wait_errors": o["errs"]}
     if "h264_cavlc" in runs:
         # the H.264 pipeline ending in its CAVLC slices (entropy = 1): the
-        # "staging" stage is the slice writer plus the copy of the slices
+        # "staging" stage is the slice writer plus the copy of the slices;
+        # the host packs them into the Annex B stream inside the timed region
         o = runs["h264_cavlc"]
         res["h264_pipeline_with_cavlc"] = {"fps": round(o["fps"], 2),
                                            "ms_per_step": round(o["wall"] * 1e3 / args.steps, 4),
                                            "p99_frame_ms": round(float(np.percentile(o["frame_ms"], 99)), 4),
+                                           "stream_bytes_per_frame": round(o["stream_bytes"] / args.steps),
                                            "stage_ms": stage_ms(o["stats"]), "wavefront_wait_errors": o["errs"]}
     if not args.no_cpu:
+        log("cpu leg: checking the timed pipelines against the oracle")
+        pipes = [(args.codec, 0)] + ([(other, 0), ("h264", 1)] if not args.single else [])
+        checked = verify_pipelines(pkg, frames, args, pipes)
         log("cpu baseline (oracle)")
         coded = [pkg.pad_coded(*frames[args.warmup]), pkg.pad_coded(*frames[args.warmup + 1])]
         res["cpu_baseline"] = cpu_baseline(coded, args.qp, args.range, args.lambda8, args.cpu_budget, args.codec)
+        res["checked_against_oracle"] = checked
     else:
         res["cpu_baseline"] = None
     print(json.dumps(res))

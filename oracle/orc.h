@@ -68,6 +68,8 @@ void orc_me_fullsearch_mb(const uint8_t *cur, const uint8_t *ref, int py,
 void orc_me_refine(const uint8_t *cur, const uint8_t *ref, int py, int wc,
                    int hc, const orc_imv_t *int16, const orc_imv_t *int8,
                    orc_mv_t *mv16, orc_mv_t *mv8);
+void orc_me_refine_mb(const uint8_t *cur, const uint8_t *ref, int py, int wc, int hc, int mbx, int mby,
+                      const orc_imv_t *int16, const orc_imv_t *int8, orc_mv_t *mv16, orc_mv_t *mv8);
 
 /* --- macroblock encode ----------------------------------------------------- */
 void orc_h264_encode_p(const uint8_t *cy, const uint8_t *cuv,

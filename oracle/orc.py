@@ -48,6 +48,7 @@ _sig("orc_pred_luma_block", None, [_P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _
 _sig("orc_me_fullsearch", None, [_P, _P, _I, _I, _I, _I, _I, _P, _P])
 _sig("orc_me_fullsearch_mb", None, [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P])
 _sig("orc_me_refine", None, [_P, _P, _I, _I, _I, _P, _P, _P, _P])
+_sig("orc_me_refine_mb", None, [_P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P])
 _sig("orc_h264_encode_p", None, [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P])
 _sig("orc_h264_encode_i", None, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P])
 _sig("orc_slice_of", _I, [_I, _I, _I])
@@ -140,6 +141,18 @@ def me_refine(cur, ref, i16, i8):
     lib.orc_me_refine(p(cur), p(ref), cur.strides[0], wc, hc, p(np.ascontiguousarray(i16)),
                       p(np.ascontiguousarray(i8)), p(m16), p(m8))
     return m16, m8
+
+
+def me_refine_mb(cur, ref, mbx, mby, i16, i8):
+    """me_refine of one MB: i16 one IMV, i8 its four 8x8 IMVs -> (MV, MV[4])."""
+    cur = np.ascontiguousarray(cur, np.uint8)
+    ref = np.ascontiguousarray(ref, np.uint8)
+    hc, wc = cur.shape
+    a = np.array([i16], IMV)
+    b = np.ascontiguousarray(i8, IMV)
+    m16, m8 = np.zeros(1, MV), np.zeros(4, MV)
+    lib.orc_me_refine_mb(p(cur), p(ref), cur.strides[0], wc, hc, mbx, mby, p(a), p(b), p(m16), p(m8))
+    return m16[0], m8
 
 
 def _outs(wc, hc):
@@ -440,11 +453,7 @@ def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lamb
             mx, my, sad = hevc_refine8(cy, ry, x0, y0, int(i8[cu]["dx"]) * 4, int(i8[cu]["dy"]) * 4, bd)
             mv[cu] = (mx, my)
             mv8[cu] = (mx, my, sad, 0)
-        fb, fr = split_uv(ruv)
-        o = hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, HEVC_ROUND_INTER, bd, ctu_log2)
-        modes = np.zeros(t * 16, np.uint8)
-        if ctu_log2 == 5:
-            modes.reshape(t, 16)[:, :4] = o["culog2"].reshape(t, 4)
+        return hevc_code_p(cy, cuv, ry, ruv, qp, mv8, num_slices, deblock, bd, ctu_log2)
     uv = np.empty((hc // 2, wc), pix)
     uv[:, 0::2], uv[:, 1::2] = o["cb"], o["cr"]
     y = o["y"]
@@ -454,6 +463,33 @@ def hevc_encode_frame(cy, cuv, ry, ruv, qp, is_intra, num_slices=1, rng=16, lamb
         mv4 = np.zeros((t * 4, 4), np.int32)
         mv4[:, 0], mv4[:, 1] = mv8["mvx"], mv8["mvy"]
         y, uv = hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices, bd, ctu_log2, cus)
+    return dict(coef=o["coef"], modes=modes, mv8=mv8, y=y, uv=uv)
+
+
+def hevc_code_p(cy, cuv, ry, ruv, qp, mv8, num_slices=1, deblock=True, bd=8, ctu_log2=4):
+    """The P picture of hevc_encode_frame after its search, from given CU
+    vectors mv8 (MV [nctu*4], quarter samples): CU coding (rounding 2) and,
+    with deblock, the 8.7.2 filter; returns the same dict."""
+    hc, wc = cy.shape
+    t = (wc // 16) * (hc // 16)
+    pix = _pix(bd)
+    mv8 = np.ascontiguousarray(mv8, MV)
+    mv = np.stack([mv8["mvx"], mv8["mvy"]], axis=1).astype(np.int32)
+    cb, cr = split_uv(cuv)
+    fb, fr = split_uv(ruv)
+    o = hevc_encode_p(cy, cb, cr, ry, fb, fr, qp, mv, HEVC_ROUND_INTER, bd, ctu_log2)
+    modes = np.zeros(t * 16, np.uint8)
+    if ctu_log2 == 5:
+        modes.reshape(t, 16)[:, :4] = o["culog2"].reshape(t, 4)
+    uv = np.empty((hc // 2, wc), pix)
+    uv[:, 0::2], uv[:, 1::2] = o["cb"], o["cr"]
+    y = o["y"]
+    if deblock:
+        cus = o["culog2"] if ctu_log2 == 5 else None
+        cbf = hevc_cu_cbf(o["coef"], cus, wc)
+        mv4 = np.zeros((t * 4, 4), np.int32)
+        mv4[:, :2] = mv
+        y, uv = hevc_deblock(y, uv, qp, False, cbf, mv4, num_slices, bd, ctu_log2, cus)
     return dict(coef=o["coef"], modes=modes, mv8=mv8, y=y, uv=uv)
 
 

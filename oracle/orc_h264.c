@@ -435,6 +435,17 @@ void orc_me_refine(const uint8_t *cur, const uint8_t *ref, int py, int wc, int h
         }
 }
 
+/* The same refinement for one MB (its 16x16 result and its four 8x8 ones),
+ * so a checker can refine a sample of MBs of a large frame. */
+void orc_me_refine_mb(const uint8_t *cur, const uint8_t *ref, int py, int wc, int hc, int mbx, int mby,
+                      const orc_imv_t *int16, const orc_imv_t *int8, orc_mv_t *mv16, orc_mv_t *mv8)
+{
+    refine_block(cur, ref, py, wc, hc, mbx * 16, mby * 16, 16, 16, int16->dx, int16->dy, mv16);
+    for (int b = 0; b < 4; b++)
+        refine_block(cur, ref, py, wc, hc, mbx * 16 + (b & 1) * 8, mby * 16 + (b >> 1) * 8, 8, 8, int8[b].dx,
+                     int8[b].dy, &mv8[b]);
+}
+
 /* ------------------------------------------------------- P16x16 encode */
 
 /* One P macroblock: MC prediction (residual_predict.comp:417-434 luma,

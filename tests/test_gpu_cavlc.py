@@ -240,6 +240,32 @@ def test_pipeline_entropy_slices(pkg, w, h, ns, rng):
         g.close()
 
 
+def test_pipeline_frame_num_and_poc_counters(pkg):
+    """frame_num and POC are separate counters as in the reference (0 at an
+    IDR, +1 and +2 per frame, written modulo 2^log2_max: encoder_h264.c:
+    3102,3108,3526-3527), so with 4 frame_num bits and 8 POC bits the POC
+    keeps rising after frame_num wraps at 16; 22 frames through the
+    sequence driver, every frame checked against the oracle."""
+    import seqcheck
+    w, h, qp, n = 96, 64, 30, 22
+    hdr = dict(log2_max_frame_num_minus4=0, log2_max_poc_lsb_minus4=4)
+    g = pkg.FrameGen(21, w, h, 2)
+    frames = [g.frame(i) for i in range(n)]
+    g.close()
+    enc = pkg.Encoder(w, h, search_range=16, lambda8=1, deblock_idc=0, readback=1, entropy=1)
+    try:
+        enc.set_slice_header(**hdr)
+        images = []
+        for y, uv in frames:
+            im = enc.new_image()
+            enc.upload(im, y, uv)
+            images.append(im)
+        _, _, got = enc.encode_sequence(images, qp, gop=0, pipelined=1, collect=True)
+    finally:
+        enc.close()
+    seqcheck.check_h264([pkg.pad_coded(*f) for f in frames], got, qp, 1, 16, 1, sample=1000, slice_hdr=hdr)
+
+
 def test_pipeline_slices_decode_to_recon(pkg):
     """The pipeline's own slices (entropy = 1, deblocking off), parsed and
     decoded with the reference decoder's prediction / MC / inverse
