@@ -315,6 +315,30 @@ def test_mf_v_design_products():
             assert abs(mf * v16 / 2 ** 21 - targets[cls]) < 2e-3, (qp, cls, mf, v16)
 
 
+def test_ac_rounding_offset_is_a_third_intra_and_a_sixth_inter():
+    """Known answers of quantize.comp:105-124: level k first appears at
+    |c| = ceil((k 2^qbits - f) / MF) with f = 2^qbits / 3 in intra blocks and
+    2^qbits / 6 in inter ones (qbits = 15 + qp/6).  MF is the standard's
+    forward table; swapping the two offsets, or using another fraction, moves
+    every threshold."""
+    mf_table = [(13107, 5243, 8066), (11916, 4660, 7490), (10082, 4194, 6554),
+                (9362, 3647, 5825), (8192, 3355, 5243), (7282, 2893, 4559)]
+    pos_of_class = {0: 0, 1: 5, 2: 1}
+    for qp in range(0, 52, 3):
+        qbits = 15 + qp // 6
+        for cls, pos in pos_of_class.items():
+            mf = mf_table[qp % 6][cls]
+            for intra, frac in ((1, 3), (0, 6)):
+                f = (1 << qbits) // frac
+                for k in (1, 2, 7):
+                    c = -(-(k * (1 << qbits) - f) // mf)  # ceil
+                    assert orc.lib.orc_quant_ac(c, pos, qp, intra) == k, (qp, pos, intra, k)
+                    assert orc.lib.orc_quant_ac(c - 1, pos, qp, intra) == k - 1, (qp, pos, intra, k)
+                    assert orc.lib.orc_quant_ac(-c, pos, qp, intra) == -k
+    # one value where the two offsets disagree: qp 0, |c| = 2, MF 13107
+    assert orc.lib.orc_quant_ac(2, 0, 0, 1) == 1 and orc.lib.orc_quant_ac(2, 0, 0, 0) == 0
+
+
 def test_dequant_shift_doubles_every_six_qp():
     for qp in range(24, 46):
         for pos in (0, 1, 5):
@@ -365,6 +389,58 @@ def test_host_dc_stage_matches_oracle(pkg):
         assert pkg.host.h264_chroma_qp(qp) == qpc
 
 
+@need_ref
+def test_i16_decision_is_the_first_strict_argmin():
+    """The I16x16 choice (intra_wavefront.comp:485-498): SAD of each available
+    predictor against the source, tried in the order DC, V, H, Plane with
+    strict <; V needs the top MB, H the left one, Plane top, left and (the
+    standard's condition, deviation D3) top-left, all in the same slice.
+    Recomputed here for every MB from the oracle's own (unfiltered)
+    reconstruction with the reference decoder's 16x16 predictors, on frames
+    made to tie: flat ones (every mode equal: DC), stripes (V or H ties
+    Plane), two-level noise and ordinary texture."""
+    lib = ref()
+    rs = np.random.default_rng(16)
+    w, h = 64, 48
+    yy, xx = np.mgrid[0:h, 0:w]
+    pics = [_frames(w, h, 1)[0][0], np.full((h, w), 128, np.uint8),
+            (120 + 8 * ((xx // 4) % 2)).astype(np.uint8), (90 + 30 * ((yy // 3) % 2)).astype(np.uint8),
+            (100 + rs.integers(0, 2, (h, w))).astype(np.uint8)]
+    pics[1][16:32, 16:48] = 140
+    uv = np.full((h // 2, w), 128, np.uint8)
+    order = (2, 0, 1, 3)  # DC, V, H, Plane
+    chosen = set()
+    for cy in pics:
+        for ns in (1, 2):
+            for qp in (10, 30):
+                e = orc.encode_i(cy, uv, qp, ns)
+                mbw, total = w // 16, (w // 16) * (h // 16)
+                for m in range(total):
+                    mbx, mby = m % mbw, m // mbw
+                    sl = orc.lib.orc_slice_of(m, total, ns)
+                    ta = mby > 0 and orc.lib.orc_slice_of(m - mbw, total, ns) == sl
+                    la = mbx > 0 and orc.lib.orc_slice_of(m - 1, total, ns) == sl
+                    ca = mbx > 0 and mby > 0 and orc.lib.orc_slice_of(m - mbw - 1, total, ns) == sl
+                    y0, x0 = mby * 16, mbx * 16
+                    rec = e["y"]
+                    top = rec[y0 - 1, x0:x0 + 16].copy() if mby > 0 else np.full(16, 128, np.uint8)
+                    left = rec[y0:y0 + 16, x0 - 1].copy() if mbx > 0 else np.full(16, 128, np.uint8)
+                    corner = int(rec[y0 - 1, x0 - 1]) if (mbx > 0 and mby > 0) else 128
+                    avail = {2: True, 0: ta, 1: la, 3: ta and la and ca}
+                    best, mode = None, None
+                    for md in order:
+                        if not avail[md]:
+                            continue
+                        pred = np.zeros((16, 16), np.uint8)
+                        lib.h264d_pred16x16(p(pred), 16, md, p(top), p(left), corner, ta, la)
+                        sad = int(np.abs(pred.astype(int) - cy[y0:y0 + 16, x0:x0 + 16]).sum())
+                        if best is None or sad < best:
+                            best, mode = sad, md
+                    assert int(e["modes"][m]) & 3 == mode, (m, ns, qp)
+                    chosen.add(mode)
+    assert chosen == {0, 1, 2, 3}  # every mode is exercised
+
+
 def test_fullsearch_brute_force_tiny():
     rs = np.random.default_rng(7)
     for trial in range(6):
@@ -389,6 +465,81 @@ def test_fullsearch_brute_force_tiny():
                 c8 = np.abs(sw[:, :, qy:qy + 8, qx:qx + 8] - blk[qy:qy + 8, qx:qx + 8]).sum(axis=(2, 3)) + lam * l1
                 k8 = int(np.argmin(c8))
                 assert (i8[4 * m + q]["dy"], i8[4 * m + q]["dx"]) == (k8 // (2 * R + 1) - R, k8 % (2 * R + 1) - R)
+
+
+# the two candidate rings of motion_estimation.comp:395-459, in its order
+HALF_RING = ((2, 0), (-2, 0), (0, 2), (0, -2), (2, 2), (2, -2), (-2, 2), (-2, -2))
+QTR_RING = tuple((a // 2, b // 2) for a, b in HALF_RING)
+
+
+def _ref_block_sad(lib, cur, refp, x0, y0, n, mvx, mvy):
+    """SAD of an n x n block against the reference decoder's 8.4.2.2.1
+    prediction at quarter-sample vector (mvx, mvy)."""
+    h, w = refp.shape
+    buf = np.zeros((n + 6) * (n + 6), np.uint8)
+    out = np.zeros((n, n), np.uint8)
+    ss = C.c_int()
+    src = lib.h264d_mc_fetch_luma(p(buf), C.byref(ss), p(refp), refp.strides[0], w, h, x0 + (mvx >> 2),
+                                  y0 + (mvy >> 2), n, n)
+    lib.h264d_mc_luma(p(out), n, src, ss.value, n, n, mvx & 3, mvy & 3)
+    return int(np.abs(out.astype(int) - cur[y0:y0 + n, x0:x0 + n]).sum())
+
+
+def _brute_refine(lib, cur, refp, x0, y0, n, dx, dy):
+    cx, cy = 4 * dx, 4 * dy
+    best = _ref_block_sad(lib, cur, refp, x0, y0, n, cx, cy)
+    for ring in (HALF_RING, QTR_RING):
+        bx = by = 0
+        for ox, oy in ring:
+            sad = _ref_block_sad(lib, cur, refp, x0, y0, n, cx + ox, cy + oy)
+            if sad < best:
+                best, bx, by = sad, ox, oy
+        cx, cy = cx + bx, cy + by
+    return cx, cy, best
+
+
+@need_ref
+def test_refine_rings_brute_force():
+    """Sub-pel refinement by brute force: from the integer winner, the
+    shader's 8 half-sample candidates in its order, then its 8 quarter-sample
+    ones around the half winner, each taken only when its plain SAD (against
+    the reference decoder's interpolation) is strictly lower.  Low-contrast
+    two-level content makes ties frequent, so ring order and the strict
+    comparison both matter; every MB and every 8x8 block is compared."""
+    lib = ref()
+    rs = np.random.default_rng(23)
+    h, w = 48, 64
+    yy, xx = np.mgrid[0:h, 0:w]
+    cases = []
+    for trial in range(4):
+        if trial % 2:
+            base = (100 + rs.integers(0, 2, (h, w))).astype(np.uint8)
+        else:
+            base = np.clip(128 + 60 * np.sin(xx / (2.5 + trial)) * np.cos(yy / 3.7) + rs.normal(0, 2, (h, w)),
+                           0, 255).astype(np.uint8)
+        cur = np.roll(base, (trial - 1, 2 - trial), axis=(0, 1))
+        cases.append((cur, base))
+        # between two integer positions: half- and quarter-sample winners
+        a = base.astype(int)
+        half = (a + np.roll(a, 1, axis=1) + 1) >> 1
+        cases.append((half.astype(np.uint8), base))
+        cases.append((((3 * a + np.roll(a, -1, axis=0) + 2) >> 2).astype(np.uint8), base))
+    cases.append((np.full((h, w), 90, np.uint8), np.full((h, w), 90, np.uint8)))
+    moved = 0
+    for cur, refp in cases:
+        i16, i8 = orc.me_fullsearch(cur, refp, 4, 1)
+        m16, m8 = orc.me_refine(cur, refp, i16, i8)
+        for m in range(12):
+            x0, y0 = (m % 4) * 16, (m // 4) * 16
+            want = _brute_refine(lib, cur, refp, x0, y0, 16, int(i16[m]["dx"]), int(i16[m]["dy"]))
+            assert (int(m16[m]["mvx"]), int(m16[m]["mvy"]), int(m16[m]["sad"])) == want, m
+            moved += want[:2] != (4 * int(i16[m]["dx"]), 4 * int(i16[m]["dy"]))
+            for b in range(4):
+                bx, by = x0 + (b & 1) * 8, y0 + (b >> 1) * 8
+                k = 4 * m + b
+                want = _brute_refine(lib, cur, refp, bx, by, 8, int(i8[k]["dx"]), int(i8[k]["dy"]))
+                assert (int(m8[k]["mvx"]), int(m8[k]["mvy"]), int(m8[k]["sad"])) == want, (m, b)
+    assert moved > 0  # fractional vectors occur
 
 
 def test_refine_never_worse_than_integer():

@@ -493,6 +493,56 @@ def test_encode_p_ctu32_large_cus_decode_to_recon(bd, qp):
     assert nonzero >= 4  # levels are exercised
 
 
+@need_ref
+@pytest.mark.parametrize("bd", [8, 10])
+def test_refine8_rings_brute_force(bd):
+    """The HEVC 8x8 refinement by brute force: the integer start, then the
+    8 half-sample candidates (2,0) (-2,0) (0,2) (0,-2) (2,2) (2,-2) (-2,2)
+    (-2,-2) around it, then the same offsets halved around the half winner,
+    each taken only when its SAD against the reference's own uni-prediction
+    (hevc_mc_uni_8 / _10) is strictly lower.  Two-level and flat content
+    make ties frequent."""
+    L = ref()
+    f = L.hevc_mc_uni_8 if bd == 8 else L.hevc_mc_uni_10
+    rs = np.random.default_rng(40 + bd)
+    h, w = 40, 48
+    yy, xx = np.mgrid[0:h, 0:w]
+    sc = 1 << (bd - 8)
+    planes = [(100 * sc + rs.integers(0, 2, (h, w))).astype(pix(bd)),
+              np.clip(128 * sc + 50 * sc * np.sin(xx / 2.7) * np.cos(yy / 3.1), 0, 255 * sc).astype(pix(bd)),
+              np.full((h, w), 77 * sc, pix(bd))]
+    ring = [(2, 0), (-2, 0), (0, 2), (0, -2), (2, 2), (2, -2), (-2, 2), (-2, -2)]
+
+    def sad(cur, refp, x0, y0, mx, my):
+        pr = np.zeros((8, 8), pix(bd))
+        f(p(refp), w, w, h, x0, y0, 8, 8, mx, my, 0, p(pr), 8)
+        return int(np.abs(pr.astype(int) - cur[y0:y0 + 8, x0:x0 + 8]).sum())
+
+    # p(x, y) = g(x + y): a half-sample move in x predicts exactly what one in
+    # y does, so (2,0) and (0,2) tie and the ring order decides
+    diag = np.clip(128 * sc + 60 * sc * np.sin((xx + yy) / 2.3), 0, 255 * sc).astype(pix(bd))
+    half = np.zeros((h, w), pix(bd))
+    f(p(diag), w, w, h, 0, 0, w, h, 2, 0, 0, p(half), w)
+    moved = 0
+    pairs = [(np.ascontiguousarray(np.roll(r, (1, -1), axis=(0, 1))), r) for r in planes] + [(half, diag)]
+    for cur, refp in pairs:
+        for (x0, y0) in ((0, 0), (8, 16), (24, 8), (40, 32)):
+            for (dx, dy) in ((0, 0), (-1, 0), (1, -1)):
+                cx, cy = 4 * dx, 4 * dy
+                best = sad(cur, refp, x0, y0, cx, cy)
+                for div in (1, 2):
+                    bx = by = 0
+                    for ox, oy in ring:
+                        s_ = sad(cur, refp, x0, y0, cx + ox // div, cy + oy // div)
+                        if s_ < best:
+                            best, bx, by = s_, ox // div, oy // div
+                    cx, cy = cx + bx, cy + by
+                got = orc.hevc_refine8(cur, refp, x0, y0, 4 * dx, 4 * dy, bd)
+                assert got == (cx, cy, best), (bd, x0, y0, dx, dy)
+                moved += (cx, cy) != (4 * dx, 4 * dy)
+    assert moved > 0
+
+
 def test_refine8_never_worse_and_consistent():
     rs = np.random.default_rng(12)
     h, w = 48, 64
