@@ -14,10 +14,14 @@
 // (vkWaitForFences, gpu_compute.c:3640-3712).
 #include <hip/hip_runtime.h>
 
+#include <errno.h>
+#include <fcntl.h>
 #include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "bc250_bitstream.h"
 #include "bc250_dev.h"
@@ -1071,6 +1075,35 @@ extern "C" int gpu_compute_get_nz_staging_data(gpu_context_t *c, void **d, size_
 // The reference's generic staging getter returns the levels of the last frame.
 extern "C" int gpu_compute_get_staging_data(gpu_context_t *c, void **d, size_t *z) { return staging(c, gpu_compute_submitted_slot(c), 0, d, z); }
 extern "C" int gpu_compute_release_staging_data(gpu_context_t *ctx) { return ctx ? 0 : -1; }
+
+extern "C" FILE *bc250_debug_dump_open(const char *name, const char *what)
+{
+    if (!name || !*name || strchr(name, '/')) return nullptr;
+    char dir[4096];
+    const char *d = getenv("BC250_DUMP_DIR");
+    if (d && *d) {
+        snprintf(dir, sizeof dir, "%s", d);
+    } else {
+        const char *rt = getenv("XDG_RUNTIME_DIR");
+        if (!rt || !*rt) {
+            fprintf(stderr, "[bc250-hip] %s: no BC250_DUMP_DIR or XDG_RUNTIME_DIR, not dumping\n",
+                    what ? what : "dump");
+            return nullptr;
+        }
+        snprintf(dir, sizeof dir, "%s/bc250_dump_frames", rt);
+        mkdir(dir, 0700);
+    }
+    char path[4352];
+    snprintf(path, sizeof path, "%s/%s", dir, name);
+    const int fd = open(path, O_WRONLY | O_CREAT | O_TRUNC | O_NOFOLLOW | O_CLOEXEC, 0600);
+    if (fd < 0) {
+        fprintf(stderr, "[bc250-hip] %s: cannot create %s: %s\n", what ? what : "dump", path, strerror(errno));
+        return nullptr;
+    }
+    FILE *f = fdopen(fd, "wb");
+    if (!f) close(fd);
+    return f;
+}
 
 extern "C" void bc250_debug_dump_nv12_frame(const uint8_t *, int, const uint8_t *, int, int, int) {}
 extern "C" void gpu_compute_debug_dump_recon(gpu_context_t *, int, int) {}

@@ -22,9 +22,18 @@
 #include <stdbool.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
+#endif
+
+/* The reference's callers test the surface handles of gpu_image_t /
+ * gpu_memory_t against VK_NULL_HANDLE (encoder_h264.c:2293,2314-2316,2424;
+ * encoder_h265.c:1798,1842).  Those handles are plain pointers here, so the
+ * null handle is a null pointer; defined only when no other header did. */
+#ifndef VK_NULL_HANDLE
+#define VK_NULL_HANDLE 0
 #endif
 
 /* reference gpu_compute.h:23-40 (gpu_image_t, passed by value) */
@@ -166,6 +175,13 @@ int gpu_compute_get_dc_staging_data(gpu_context_t *ctx, void **data, size_t *siz
 int gpu_compute_get_pred_mode_staging_data(gpu_context_t *ctx, void **data, size_t *size);
 int gpu_compute_get_mv_staging_data(gpu_context_t *ctx, void **data, size_t *size);
 int gpu_compute_get_nz_staging_data(gpu_context_t *ctx, void **data, size_t *size);
+
+/* reference gpu_compute.h:559: creates a diagnostic dump file (mode 0600, a
+ * symbolic link at the path is refused) in $BC250_DUMP_DIR, else
+ * $XDG_RUNTIME_DIR/bc250_dump_frames (created 0700); `what` names the switch
+ * that asked, for the messages.  NULL on failure.  The encoders' own dump
+ * switches call it (encoder_h264.c:2815,2821; encoder_h265.c:1745). */
+FILE *bc250_debug_dump_open(const char *name, const char *what);
 
 /* ---- debug hooks: reference gpu_compute.h:556-667 (no-ops here) ---------- */
 void bc250_debug_dump_nv12_frame(const uint8_t *y_plane, int y_pitch, const uint8_t *uv_plane, int uv_pitch,
