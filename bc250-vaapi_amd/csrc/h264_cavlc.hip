@@ -21,6 +21,8 @@
 #include "bc250_bitstream.h"
 
 #include <algorithm>
+#include <mutex>
+#include <stdio.h>
 
 namespace {
 
@@ -823,4 +825,66 @@ extern "C" int bc250_h264_cavlc_slices(const int16_t *levels, const int32_t *dc,
     cavlc_write_kernel<<<blocks + 1, 256, 0, st>>>(levels, dc, c, w, (uint32_t *)out, cap_words, slice_info,
                                                   blocks);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free entry point.  Its work buffer is `scratch`, a
+// file-scope device array of BC250_CAVLC_SCRATCH_BYTES that every GPU the
+// library runs on holds until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, once per
+// device with that device current, and is kept per device.  The writer's
+// kernels write every work word they later read, so nothing in the array
+// needs clearing between calls; calls that overlap on one device share the
+// array and race.
+static __device__ __align__(256) uint8_t scratch[BC250_CAVLC_SCRATCH_BYTES];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+extern "C" size_t bc250_h264_cavlc_noalloc_max_mbs(void)
+{
+    // the largest frame whose work fits: work_bytes grows with the MB count
+    size_t lo = 0, hi = (size_t)1 << 17;
+    while (lo < hi) {
+        const size_t mid = (lo + hi + 1) / 2;
+        if (work_bytes((int)mid) <= BC250_CAVLC_SCRATCH_BYTES) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+extern "C" int bc250_h264_cavlc_slices_noalloc(const int16_t *levels, const int32_t *dc, const uint32_t *modes,
+                                               const gpu_mv_t *mvs, const bc250_h264_slice_cfg_t *cfg, uint8_t *out,
+                                               size_t out_cap, uint32_t *slice_info, void *stream)
+{
+    if (!cfg || cfg->width_in_mbs <= 0 || cfg->height_in_mbs <= 0) return -1;
+    const int64_t total = (int64_t)cfg->width_in_mbs * cfg->height_in_mbs;
+    if (total > (int64_t)bc250_h264_cavlc_noalloc_max_mbs()) {
+        fprintf(stderr, "[bc250-hip] bc250_h264_cavlc_slices_noalloc: %lld MBs exceed the %zu-MB limit of its "
+                        "%d-byte scratch\n", (long long)total, bc250_h264_cavlc_noalloc_max_mbs(),
+                BC250_CAVLC_SCRATCH_BYTES);
+        return -1;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -1;
+    void *ptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void *p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                fprintf(stderr, "[bc250-hip] hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+                        hipGetErrorString(e));
+                return -1;
+            }
+            g_scratch_ptr[dev] = p;
+        }
+        ptr = g_scratch_ptr[dev];
+    }
+    return bc250_h264_cavlc_slices(levels, dc, modes, mvs, cfg, out, out_cap, slice_info, ptr,
+                                   BC250_CAVLC_SCRATCH_BYTES, stream);
 }

@@ -62,6 +62,23 @@ int bc250_h264_cavlc_slices(const int16_t *levels, const int32_t *dc, const uint
                             const bc250_h264_slice_cfg_t *cfg, uint8_t *out, size_t out_cap, uint32_t *slice_info,
                             void *work, size_t work_size, void *stream);
 
+/* The same writer with no work argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Its
+ * work buffer is a file-scope device array of BC250_CAVLC_SCRATCH_BYTES
+ * (4 MiB) held by each GPU the library runs on until the process ends, which
+ * bounds the frame to bc250_h264_cavlc_noalloc_max_mbs() MBs (34,944: 4K UHD
+ * is 32,400); larger frames are refused (-1, with a message).  It calls no
+ * allocator.  Every call on a device shares that device's array: calls that
+ * overlap on one device (different streams, or threads) race and must be
+ * serialised by the caller.  Output, arguments and return value are those of
+ * bc250_h264_cavlc_slices. */
+#define BC250_CAVLC_SCRATCH_BYTES (4 << 20)
+int bc250_h264_cavlc_slices_noalloc(const int16_t *levels, const int32_t *dc, const uint32_t *modes,
+                                    const gpu_mv_t *mvs, const bc250_h264_slice_cfg_t *cfg, uint8_t *out,
+                                    size_t out_cap, uint32_t *slice_info, void *stream);
+size_t bc250_h264_cavlc_noalloc_max_mbs(void);
+
 /* Copies slice_info (4 * num_slices + 2 words) and the slices' bytes that
  * bc250_h264_cavlc_slices wrote (slice_info[4 n] of them, at most dst_cap) to
  * dst / dst_info -- any memory the device can write, e.g. hipHostMalloc'd
