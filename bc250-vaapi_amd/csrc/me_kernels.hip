@@ -69,18 +69,29 @@ BC_DEV uint32_t mad_hi16(uint32_t x, uint32_t li)
 }
 }  // namespace
 
-// out16[m] = {dx, dy, cost, sad}; out8[4m+q] likewise for quadrant q.
+// out16[m] = {dx, dy, cost, sad}; out8[4m+q] likewise for quadrant q.  With
+// `list`, only the MBs list[0 .. *count) are searched (count is read on the
+// device, so the grid is sized for the largest list).
 template <int R>
 __global__ __launch_bounds__(ME_THREADS, BC250_ME_WAVES_PER_SIMD) void me_fullsearch_kernel(
     const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref, int rpitch,
     int wc, int hc, int mbw, int nmb, int lambda8, uint4 *__restrict__ out16,
-    uint4 *__restrict__ out8)
+    uint4 *__restrict__ out8, const int *__restrict__ list, const int *__restrict__ count)
 {
     using Gm = FsGeom<R>;
     __shared__ uint32_t win[Gm::NR * Gm::PITCH];
     __shared__ uint32_t red[ME_THREADS / 64][5];
 
-    const int mb = xcd_item(blockIdx.x, nmb);
+    int mb;
+    if (list) {
+        const int n = min(*count, nmb), per = (n + 7) >> 3;
+        if ((int)(blockIdx.x >> 3) >= per) return;
+        const int i = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+        if (i >= n) return;
+        mb = list[i];
+    } else {
+        mb = xcd_item(blockIdx.x, nmb);
+    }
     if (mb >= nmb) return;
     const int mbx = mb % mbw, mby = mb / mbw;
     const int x0 = mbx * 16, y0 = mby * 16;
@@ -366,32 +377,41 @@ __global__ __launch_bounds__(64) void me_refine_kernel(
 }
 
 // Host-side launchers (declared in bc250_stages.h).
-extern "C" int bc250_launch_me_fullsearch(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch,
-                                          int wc, int hc, int range, int lambda8, uint32_t *out16_,
-                                          uint32_t *out8_, void *stream)
+extern "C" int bc250_launch_me_fullsearch_list(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch,
+                                               int wc, int hc, int range, int lambda8, uint32_t *out16_,
+                                               uint32_t *out8_, const int *list, const int *count, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
     uint4 *out16 = (uint4 *)out16_, *out8 = (uint4 *)out8_;
     if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || (cpitch & 3) || lambda8 < 0 || lambda8 > 15) return -1;
+    if ((list == nullptr) != (count == nullptr)) return -1;
     const int mbw = wc / 16, nmb = mbw * (hc / 16);
     const int grid = ((nmb + 7) / 8) * 8;
     switch (range) {
     case 16:
         me_fullsearch_kernel<16><<<grid, ME_THREADS, 0, st>>>(cur, cpitch, ref, rpitch, wc, hc, mbw, nmb,
-                                                             lambda8, out16, out8);
+                                                             lambda8, out16, out8, list, count);
         break;
     case 32:
         me_fullsearch_kernel<32><<<grid, ME_THREADS, 0, st>>>(cur, cpitch, ref, rpitch, wc, hc, mbw, nmb,
-                                                             lambda8, out16, out8);
+                                                             lambda8, out16, out8, list, count);
         break;
     case 64:
         me_fullsearch_kernel<64><<<grid, ME_THREADS, 0, st>>>(cur, cpitch, ref, rpitch, wc, hc, mbw, nmb,
-                                                             lambda8, out16, out8);
+                                                             lambda8, out16, out8, list, count);
         break;
     default:
         return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_launch_me_fullsearch(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch,
+                                          int wc, int hc, int range, int lambda8, uint32_t *out16,
+                                          uint32_t *out8, void *stream)
+{
+    return bc250_launch_me_fullsearch_list(cur, cpitch, ref, rpitch, wc, hc, range, lambda8, out16, out8, nullptr,
+                                           nullptr, stream);
 }
 
 extern "C" int bc250_launch_me_refine(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,

@@ -1,0 +1,223 @@
+// me_learned.hip - the integer motion search of the learned form: the
+// network's proposals refined in +-W windows, with the plain full search as
+// the fallback, for gfx950.
+//
+//   1. cnn_mvin_kernel       : the head's neighbour-vector input of every
+//                              block: the previous picture's integer 16x16
+//                              vectors at MB offsets -1..2 in x and y
+//   2. cnn_candidates_kernel : per MB the network's hypotheses ordered by
+//                              confidence (ties: lower index), rounded to
+//                              whole pels and clamped to the search range
+//   3. cand_search_kernel    : per MB the (2W+1)^2 positions around each of
+//                              the first n candidates, with the full
+//                              search's costs and argmin keys (cost * 2^15 +
+//                              raster index in the +-R square): wherever the
+//                              full search's winner lies in a window, the
+//                              result is the full search's, bit for bit
+//   4. fallback_select_kernel: MBs whose 16x16 SAD exceeds a bound from
+//                              their neighbours' SADs (and every MB above an
+//                              absolute ceiling) go to a list, which
+//                              me_fullsearch_kernel then searches in full
+#include "bc250_dev.h"
+#include "bc250_learned.h"
+
+namespace {
+
+constexpr int CW = 4;  // window radius of the candidate search (pels)
+constexpr int WS = 16 + 2 * CW;       // window side (24)
+constexpr int WP = WS / 4 + 1;        // window pitch in dwords (7, odd)
+
+__global__ void cnn_mvin_kernel(const uint4 *__restrict__ prev16, int mbw, int mbh, int tw, int ntiles,
+                                float *__restrict__ mvin)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;  // (tile, block)
+    if (b >= ntiles * 16) return;
+    const int tile = b / 16, j = b % 16;
+    const int mx = (tile % tw) * 4 + (j & 3), my = (tile / tw) * 4 + (j >> 2);
+    float *o = mvin + (size_t)b * 32;
+    for (int k = 0; k < 16; k++) {
+        const int nx = clampi(mx + (k & 3) - 1, 0, mbw - 1), ny = clampi(my + (k >> 2) - 1, 0, mbh - 1);
+        const uint4 v = prev16 ? prev16[ny * mbw + nx] : make_uint4(0, 0, 0, 0);
+        o[2 * k] = (float)(int)v.x * (1.f / 16.f);
+        o[2 * k + 1] = (float)(int)v.y * (1.f / 16.f);
+    }
+}
+
+__global__ void cnn_candidates_kernel(const float *__restrict__ z, int mbw, int mbh, int tw, int range,
+                                      int2 *__restrict__ cand)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= mbw * mbh) return;
+    const int mx = m % mbw, my = m / mbw;
+    const int tile = (my >> 2) * tw + (mx >> 2), j = (my & 3) * 4 + (mx & 3);
+    const float *zb = z + ((size_t)tile * 16 + j) * BC250_CNN_NOUT;
+    int ord[4] = {0, 1, 2, 3};
+    // stable selection by confidence, highest first
+    for (int a = 0; a < 4; a++)
+        for (int b = a + 1; b < 4; b++)
+            if (zb[8 + ord[b]] > zb[8 + ord[a]]) { const int t = ord[a]; ord[a] = ord[b]; ord[b] = t; }
+    for (int k = 0; k < 4; k++) {
+        const int h = ord[k];
+        const int dx = clampi((int)rintf(zb[2 * h] * 16.f), -range, range);
+        const int dy = clampi((int)rintf(zb[2 * h + 1] * 16.f), -range, range);
+        cand[(size_t)m * 4 + k] = make_int2(dx, dy);
+    }
+}
+
+// One wave per MB.  The current block is wave-uniform; each candidate's
+// window (edge-replicated like the full search's) is staged in LDS and a
+// lane evaluates positions lane and lane + 64 of the (2W+1)^2 around it.
+__global__ __launch_bounds__(64) void cand_search_kernel(const uint8_t *__restrict__ cur, int cpitch,
+                                                         const uint8_t *__restrict__ ref, int rpitch, int wc, int hc,
+                                                         int mbw, int nmb, int range, int lambda8,
+                                                         const int2 *__restrict__ cand, int ncand,
+                                                         uint4 *__restrict__ out16, uint4 *__restrict__ out8)
+{
+    __shared__ uint32_t win[WS * WP];
+    const int mb = xcd_item(blockIdx.x, nmb);
+    if (mb >= nmb) return;
+    const int lane = threadIdx.x;
+    const int x0 = (mb % mbw) * 16, y0 = (mb / mbw) * 16;
+    const int nc = 2 * range + 1;
+    uint32_t cw[16][4];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t *p = (const uint32_t *)(cur + (size_t)(y0 + r) * cpitch + x0);
+#pragma unroll
+        for (int c = 0; c < 4; c++) cw[r][c] = __builtin_amdgcn_readfirstlane(p[c]);
+    }
+    uint32_t b16 = 0xffffffffu, bq[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+    for (int k = 0; k < ncand; k++) {
+        const int2 c = cand[(size_t)mb * 4 + k];
+        // window origin: the block at vector (c.x - W, c.y - W)
+        const int wx0 = x0 + c.x - CW, wy0 = y0 + c.y - CW;
+        __syncthreads();
+        for (int i = lane; i < WS * (WS / 4); i += 64) {
+            const int r = i / (WS / 4), d = i % (WS / 4);
+            const uint8_t *row = ref + (size_t)clampi(wy0 + r, 0, hc - 1) * rpitch;
+            const int gx = wx0 + 4 * d;
+            uint32_t v;
+            if (gx >= 0 && gx + 3 < wc) {
+                v = (uint32_t)row[gx] | ((uint32_t)row[gx + 1] << 8) | ((uint32_t)row[gx + 2] << 16) |
+                    ((uint32_t)row[gx + 3] << 24);
+            } else {
+                v = (uint32_t)row[clampi(gx, 0, wc - 1)] | ((uint32_t)row[clampi(gx + 1, 0, wc - 1)] << 8) |
+                    ((uint32_t)row[clampi(gx + 2, 0, wc - 1)] << 16) |
+                    ((uint32_t)row[clampi(gx + 3, 0, wc - 1)] << 24);
+            }
+            win[r * WP + d] = v;
+        }
+        if (lane < WS) win[lane * WP + WS / 4] = 0;  // the pad dword read past a row's end
+        __syncthreads();
+        for (int pi = lane; pi < (2 * CW + 1) * (2 * CW + 1); pi += 64) {
+            const int ox = pi % (2 * CW + 1), oy = pi / (2 * CW + 1);
+            const int dx = c.x - CW + ox, dy = c.y - CW + oy;
+            if (dx < -range || dx > range || dy < -range || dy > range) continue;
+            const int sh = ox & 3, d0 = ox >> 2;
+            uint32_t q[4] = {0, 0, 0, 0};
+            for (int r = 0; r < 16; r++) {
+                const uint32_t *wr = win + (oy + r) * WP + d0;
+                uint32_t d[5];
+#pragma unroll
+                for (int i = 0; i < 5; i++) d[i] = wr[i];
+                uint32_t s[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) s[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+                const int h = (r >> 3) * 2;
+                q[h] = __builtin_amdgcn_sad_u8(s[1], cw[r][1], __builtin_amdgcn_sad_u8(s[0], cw[r][0], q[h]));
+                q[h + 1] = __builtin_amdgcn_sad_u8(s[3], cw[r][3], __builtin_amdgcn_sad_u8(s[2], cw[r][2], q[h + 1]));
+            }
+            const uint32_t l1 = (uint32_t)(iabs(dx) + iabs(dy));
+            const uint32_t idx = (uint32_t)((dy + range) * nc + (dx + range));
+            const uint32_t sad16 = q[0] + q[1] + q[2] + q[3];
+            b16 = min(b16, ((sad16 + 4u * lambda8 * l1) << 15) | idx);
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) bq[qq] = min(bq[qq], ((q[qq] + (uint32_t)lambda8 * l1) << 15) | idx);
+        }
+    }
+    b16 = wave_min_u32(b16);
+#pragma unroll
+    for (int qq = 0; qq < 4; qq++) bq[qq] = wave_min_u32(bq[qq]);
+    if (lane < 5) {
+        const uint32_t key = lane == 0 ? b16 : bq[lane - 1];
+        const uint32_t idx = key & 0x7fffu, cost = key >> 15;
+        const int dx = (int)(idx % nc) - range, dy = (int)(idx / nc) - range;
+        const uint32_t l1 = (uint32_t)(iabs(dx) + iabs(dy));
+        const uint32_t sad = cost - (lane == 0 ? 4u : 1u) * (uint32_t)lambda8 * l1;
+        const uint4 o = make_uint4((uint32_t)dx, (uint32_t)dy, cost, sad);
+        if (lane == 0) out16[mb] = o;
+        else out8[4 * mb + lane - 1] = o;
+    }
+}
+
+__global__ void zero1_kernel(int *p) { *p = 0; }
+
+// The fallback rule: an MB goes to the full search when its 16x16 SAD
+// exceeds (alpha_q4 / 16) * m + beta, m the median (lower middle) of its
+// left, right, top and bottom neighbours' SADs, or exceeds t_high.
+__global__ void fallback_select_kernel(const uint4 *__restrict__ r16, int mbw, int mbh, int alpha_q4, int beta,
+                                       int t_high, int *__restrict__ list, int *__restrict__ count)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= mbw * mbh) return;
+    const int mx = m % mbw, my = m / mbw;
+    uint32_t nb[4];
+    int n = 0;
+    if (mx > 0) nb[n++] = r16[m - 1].w;
+    if (mx + 1 < mbw) nb[n++] = r16[m + 1].w;
+    if (my > 0) nb[n++] = r16[m - mbw].w;
+    if (my + 1 < mbh) nb[n++] = r16[m + mbw].w;
+    for (int a = 0; a < n; a++)
+        for (int b = a + 1; b < n; b++)
+            if (nb[b] < nb[a]) { const uint32_t t = nb[a]; nb[a] = nb[b]; nb[b] = t; }
+    const uint32_t med = n ? nb[(n - 1) / 2] : 0u;
+    const uint32_t sad = r16[m].w;
+    const uint64_t bound = ((uint64_t)alpha_q4 * med >> 4) + (uint64_t)beta;
+    if (sad > bound || sad > (uint32_t)t_high) list[atomicAdd(count, 1)] = m;
+}
+
+}  // namespace
+
+extern "C" int bc250_cnn_mvin(const uint32_t *prev16, int mbw, int mbh, float *mvin, void *stream)
+{
+    if (mbw <= 0 || mbh <= 0 || !mvin) return -1;
+    const int tw = (mbw + 3) / 4, nt = tw * ((mbh + 3) / 4);
+    cnn_mvin_kernel<<<(nt * 16 + 127) / 128, 128, 0, (hipStream_t)stream>>>((const uint4 *)prev16, mbw, mbh, tw, nt,
+                                                                           mvin);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_cnn_candidates(const float *z, int mbw, int mbh, int range, int *cand, void *stream)
+{
+    if (mbw <= 0 || mbh <= 0 || !z || !cand || range < CW || range > 64) return -1;
+    const int nmb = mbw * mbh;
+    cnn_candidates_kernel<<<(nmb + 127) / 128, 128, 0, (hipStream_t)stream>>>(z, mbw, mbh, (mbw + 3) / 4, range,
+                                                                             (int2 *)cand);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_launch_cand_search(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
+                                        int hc, int range, int lambda8, const int *cand, int ncand, uint32_t *out16,
+                                        uint32_t *out8, void *stream)
+{
+    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || (cpitch & 3) || lambda8 < 0 || lambda8 > 15 ||
+        ncand < 1 || ncand > 4 || !cand || range < CW || range > 64 || (2 * range + 1) * (2 * range + 1) > 32768)
+        return -1;
+    const int mbw = wc / 16, nmb = mbw * (hc / 16);
+    cand_search_kernel<<<((nmb + 7) / 8) * 8, 64, 0, (hipStream_t)stream>>>(
+        cur, cpitch, ref, rpitch, wc, hc, mbw, nmb, range, lambda8, (const int2 *)cand, ncand, (uint4 *)out16,
+        (uint4 *)out8);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_fallback_select(const uint32_t *r16, int mbw, int mbh, int alpha_q4, int beta, int t_high,
+                                     int *list, int *count, void *stream)
+{
+    if (mbw <= 0 || mbh <= 0 || !r16 || !list || !count || alpha_q4 < 0 || beta < 0 || t_high < 0) return -1;
+    const int nmb = mbw * mbh;
+    hipStream_t st = (hipStream_t)stream;
+    zero1_kernel<<<1, 1, 0, st>>>(count);
+    fallback_select_kernel<<<(nmb + 127) / 128, 128, 0, st>>>((const uint4 *)r16, mbw, mbh, alpha_q4, beta, t_high,
+                                                              list, count);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -29,6 +29,11 @@ extern "C" {
  * integer stage (shaders/motion_estimation.comp:278-374). */
 int bc250_launch_me_fullsearch(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
                                int range, int lambda8, uint32_t *out16, uint32_t *out8, void *stream);
+/* The same search restricted to the MBs list[0 .. *count), count read on the
+ * device (the other MBs' records are left as they are). */
+int bc250_launch_me_fullsearch_list(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
+                                    int range, int lambda8, uint32_t *out16, uint32_t *out8, const int *list,
+                                    const int *count, void *stream);
 
 /* Half- then quarter-pel refinement of the integer winners
  * (shaders/motion_estimation.comp:376-469). do8 = also the 8x8 quadrants. */

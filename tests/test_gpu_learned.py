@@ -1,0 +1,263 @@
+"""The learned proposer's gfx950 kernels against the CPU oracle
+(oracle/orc_cnn.py): each bf16 MFMA convolution on the GPU's own input
+(equal to the oracle's bf16 rounding of a float64 sum, up to one bf16 step
+where the fp32 sum lands near a rounding boundary), the block sums and head
+in fp32, the backward pass (input gradients with flipped weights, weight
+gradients, head and loss) against the oracle's adjoint formulation, and the
+candidate search against a brute force of the full search's costs."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import orc
+import orc_cnn as oc
+from conftest import have_gpu, load_pkg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    if not have_gpu():
+        pytest.skip("no GPU")
+    return load_pkg()
+
+
+@pytest.fixture(scope="module")
+def learned(pkg):
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location("bc250_learned_py", os.path.join(os.path.dirname(pkg.__file__),
+                                                                                    "learned.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def bf(t):
+    """int16 device tensor of bf16 bits -> float32 numpy."""
+    a = t.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
+    return a.view(np.float32)
+
+
+def _setup(pkg, learned, n, seed, train=False):
+    import torch
+    rs = np.random.default_rng(seed)
+    p = learned.init_params(seed)
+    p += rs.normal(0, 0.02, p.shape).astype(np.float32)
+    for l, (cin, cout) in enumerate(learned.LAYERS):   # padding columns stay zero
+        kp = learned.kpad(cin)
+        m = p[learned.OFFSETS[l]:learned.OFFSETS[l] + cout * (kp + 1)].reshape(cout, kp + 1)
+        m[:, 9 * cin:kp] = 0
+    net = learned.Net(pkg, n, p, train=train)
+    h, w = 128, 192
+    yy, xx = np.mgrid[0:h, 0:w]
+    cur = np.clip(128 + 60 * np.sin(xx / 5.0) * np.cos(yy / 7.0) + rs.normal(0, 8, (h, w)), 0, 255).astype(np.uint8)
+    ref = np.roll(cur, (2, -3), axis=(0, 1))
+    dc, dr = torch.from_numpy(cur).cuda(), torch.from_numpy(ref).cuda()
+    tiles = np.array([(0, i % 3, i // 3 % 2) for i in range(n)], np.int32)
+    net.set_input(dc, dr, w, w, h, tiles)
+    mvin = rs.normal(0, 0.5, (n, 16, 32)).astype(np.float32)
+    dmv = torch.from_numpy(mvin).cuda()
+    x = oc.input_tiles(cur, ref, [(int(t[1]), int(t[2])) for t in tiles])
+    return net, p, x, mvin, dmv
+
+
+def _close_bf16(got, want, what, frac=0.01):
+    """Equal, except a small fraction of entries one bf16 step apart (plus
+    the fp32 summation error, 1e-5 of the layer's largest value, for values
+    that cancel to near zero)."""
+    got, want = np.asarray(got, np.float32), np.asarray(want, np.float32)
+    diff = got != want
+    step = np.abs(want) * 2 ** -7 + 1e-5 * np.abs(want).max()
+    assert (np.abs(got - want)[diff] <= 1.01 * step[diff]).all(), f"{what}: a difference above one bf16 step"
+    assert diff.mean() <= frac, f"{what}: {diff.mean():.4f} of entries differ"
+
+
+def test_input_tiles_match_oracle(pkg, learned):
+    net, p, x, _, _ = _setup(pkg, learned, 6, 1)
+    got = bf(net.act[0])[:, 1:65, 1:65, :]
+    np.testing.assert_array_equal(got, x)
+    assert not bf(net.act[0])[:, 0, :, :].any()  # the border stays zero
+
+
+def test_forward_layers_match_oracle(pkg, learned):
+    say("learned: forward, 6 tiles")
+    net, p, x, mvin, dmv = _setup(pkg, learned, 6, 2)
+    net.forward(6, dmv)
+    import torch
+    torch.cuda.synchronize()
+    prev = x
+    for l in range(4):
+        got = bf(net.act[l + 1])[:, 1:65, 1:65, :]
+        w, b = oc.layer_params(p, l)
+        w = oc.bf16(w.astype(np.float32)).astype(np.float64)
+        want = oc.bf16(np.maximum(oc.conv(prev, w, b), 0).astype(np.float32))
+        _close_bf16(got, want, f"conv{l + 1}")
+        assert (got > 0).mean() > 0.2, f"conv{l + 1} mostly dead"
+        prev = got
+    pool = net.pool.cpu().numpy()
+    a4 = prev.astype(np.float64)
+    want_pool = a4.reshape(6, 4, 16, 4, 16, 32).sum(axis=(2, 4)).reshape(6, 16, 32)
+    np.testing.assert_allclose(pool, want_pool, rtol=1e-5, atol=1e-4)
+    wh, bh = oc.head_params(p)
+    want_z = np.concatenate([want_pool / 256, mvin], axis=2) @ wh.T + bh
+    np.testing.assert_allclose(net.z.cpu().numpy(), want_z, rtol=1e-4, atol=1e-4)
+
+
+def test_backward_matches_oracle(pkg, learned):
+    import torch
+    say("learned: backward, 4 tiles")
+    n = 4
+    net, p, x, mvin, dmv = _setup(pkg, learned, n, 3, train=True)
+    net.forward(n, dmv)
+    rs = np.random.default_rng(9)
+    target = rs.integers(-24, 25, (n, 16, 2)).astype(np.float32)
+    scale = 1.0 / (n * 16)
+    loss = net.backward(n, dmv, torch.from_numpy(target).cuda(), scale).cpu().numpy()
+    torch.cuda.synchronize()
+    acts = [x] + [bf(net.act[l])[:, 1:65, 1:65, :] for l in range(1, 5)]
+    pool = net.pool.cpu().numpy().astype(np.float64)
+    wh, bh = oc.head_params(p)
+    z = np.concatenate([pool / 256, mvin], axis=2) @ wh.T + bh
+    lb, dzh = oc.loss_and_dz(z, target)
+    np.testing.assert_allclose(loss, lb.sum(axis=1), rtol=1e-4, atol=1e-4)
+    g, dzs = oc.backward(p, acts, pool, mvin, dzh * scale, rounding="bf16")
+    gg = net.grad.cpu().numpy().astype(np.float64)
+    # the pre-activation gradients the kernels stored
+    for l in (4, 3, 2, 1):
+        got = bf(net.dz[l])[:, 1:65, 1:65, :]
+        want = dzs[l]
+        cos = (got * want).sum() / (np.linalg.norm(got) * np.linalg.norm(want) + 1e-30)
+        assert cos > 0.999, f"dz{l}: cosine {cos}"
+    for l in range(5):
+        lo, hi = oc.OFFSETS[l], (oc.OFFSETS[l + 1] if l < 4 else oc.NPARAM)
+        a, b = gg[lo:hi], g[lo:hi]
+        rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+        assert rel < 2e-2, f"layer {l + 1} gradient: relative error {rel}"
+
+
+def test_training_steps_reduce_the_loss(pkg, learned):
+    import torch
+    say("learned: 60 Adam steps on a fixed batch")
+    n = 6
+    net, p, x, mvin, dmv = _setup(pkg, learned, n, 4, train=True)
+    tv = np.array([-3.0, 2.0], np.float32)
+    target = torch.from_numpy(np.tile(tv[None, :], (n, 16, 1))).cuda()
+    losses = []
+    for _ in range(60):
+        net.forward(n, dmv)
+        losses.append(float(net.backward(n, dmv, target, 1.0 / (n * 16)).sum()))
+        net.adam(2e-2)
+    z = net.forward(n, dmv).cpu().numpy()
+    err = np.abs(z[..., 0:8].reshape(n, 16, 4, 2) * 16 - tv).max(axis=-1).min(axis=-1)
+    assert losses[-1] < 0.6 * losses[0], losses   # the floor at the start is mostly log(4) per block
+    assert (err < 0.5).mean() > 0.9, err          # the nearest hypothesis within half a pel
+
+
+def _keys(cur, ref, mbx, mby, dx, dy, R, lam):
+    """The full search's 16x16 key and four 8x8 keys at one vector."""
+    h, w = ref.shape
+    x0, y0 = mbx * 16, mby * 16
+    ys = np.clip(np.arange(y0 + dy, y0 + dy + 16), 0, h - 1)
+    xs = np.clip(np.arange(x0 + dx, x0 + dx + 16), 0, w - 1)
+    d = np.abs(ref[np.ix_(ys, xs)].astype(int) - cur[y0:y0 + 16, x0:x0 + 16])
+    q = [d[:8, :8].sum(), d[:8, 8:].sum(), d[8:, :8].sum(), d[8:, 8:].sum()]
+    l1 = abs(dx) + abs(dy)
+    idx = (dy + R) * (2 * R + 1) + (dx + R)
+    return ((sum(q) + 4 * lam * l1) << 15) | idx, [((v + lam * l1) << 15) | idx for v in q]
+
+
+def test_candidate_search_brute_force_and_full_search(pkg):
+    """Every MB's result is the minimum key over the +-4 windows of its
+    first ncand candidates (positions outside +-R skipped), and when a
+    candidate is the full search's own winner, the result is the full
+    search's record, bit for bit."""
+    import torch
+    rs = np.random.default_rng(12)
+    w, h, R, lam = 96, 64, 16, 1
+    ref = rs.integers(0, 256, (h, w)).astype(np.uint8)
+    cur = np.clip(np.roll(ref, (3, -5), axis=(0, 1)).astype(int) + rs.integers(-4, 5, (h, w)), 0, 255).astype(np.uint8)
+    nmb = (w // 16) * (h // 16)
+    i16, i8 = orc.me_fullsearch(cur, ref, R, lam)
+    cand = rs.integers(-R - 2, R + 3, (nmb, 4, 2)).astype(np.int32)
+    cand = np.clip(cand, -R, R)
+    cand[::2, 1] = np.stack([i16["dx"], i16["dy"]], axis=1)[::2]     # the true winner on every other MB
+    dcur, dref = torch.from_numpy(cur).cuda(), torch.from_numpy(ref).cuda()
+    dc = torch.from_numpy(cand.reshape(-1)).cuda()
+    for ncand in (2, 4):
+        o16 = torch.zeros(nmb * 4, dtype=torch.int32, device="cuda")
+        o8 = torch.zeros(nmb * 16, dtype=torch.int32, device="cuda")
+        assert pkg.hip.bc250_launch_cand_search(C.c_void_p(dcur.data_ptr()), w, C.c_void_p(dref.data_ptr()), w, w, h,
+                                                R, lam, C.c_void_p(dc.data_ptr()), ncand, C.c_void_p(o16.data_ptr()),
+                                                C.c_void_p(o8.data_ptr()), None) == 0
+        g16 = o16.cpu().numpy().view(orc.IMV)
+        g8 = o8.cpu().numpy().view(orc.IMV)
+        for m in range(nmb):
+            mbx, mby = m % (w // 16), m // (w // 16)
+            best16, bq = None, [None] * 4
+            for k in range(ncand):
+                cx, cy = cand[m, k]
+                for dy in range(cy - 4, cy + 5):
+                    for dx in range(cx - 4, cx + 5):
+                        if abs(dx) > R or abs(dy) > R:
+                            continue
+                        k16, k8 = _keys(cur, ref, mbx, mby, dx, dy, R, lam)
+                        best16 = k16 if best16 is None else min(best16, k16)
+                        bq = [v if b is None else min(b, v) for b, v in zip(bq, k8)]
+            for key, rec in [(best16, g16[m])] + list(zip(bq, g8[4 * m:4 * m + 4])):
+                idx = key & 0x7FFF
+                assert (int(rec["dx"]), int(rec["dy"]), int(rec["cost"])) == (idx % (2 * R + 1) - R,
+                                                                               idx // (2 * R + 1) - R, key >> 15), m
+            if m % 2 == 0 and ncand >= 2:
+                assert g16[m] == i16[m]
+
+
+def test_fallback_select_and_list_search(pkg):
+    import torch
+    rs = np.random.default_rng(13)
+    mbw, mbh = 9, 5
+    nmb = mbw * mbh
+    r16 = np.zeros(nmb, orc.IMV)
+    r16["sad"] = rs.integers(200, 3000, nmb)
+    r16["sad"][[7, 20, 33]] = [20000, 9000, 60000]
+    d = torch.from_numpy(r16.view(np.int32).reshape(-1).copy()).cuda()
+    lst = torch.full((nmb,), -1, dtype=torch.int32, device="cuda")
+    cnt = torch.full((1,), 77, dtype=torch.int32, device="cuda")
+    alpha, beta, high = 24, 300, 8000
+    assert pkg.hip.bc250_fallback_select(C.c_void_p(d.data_ptr()), mbw, mbh, alpha, beta, high,
+                                         C.c_void_p(lst.data_ptr()), C.c_void_p(cnt.data_ptr()), None) == 0
+    n = int(cnt.item())
+    got = sorted(lst.cpu().numpy()[:n].tolist())
+    want = []
+    for m in range(nmb):
+        mx, my = m % mbw, m // mbw
+        nb = sorted(int(r16["sad"][k]) for k, ok in ((m - 1, mx > 0), (m + 1, mx + 1 < mbw), (m - mbw, my > 0),
+                                                     (m + mbw, my + 1 < mbh)) if ok)
+        med = nb[(len(nb) - 1) // 2]
+        s = int(r16["sad"][m])
+        if s > (alpha * med >> 4) + beta or s > high:
+            want.append(m)
+    assert got == want and {7, 20, 33} <= set(got)
+    # the full search on the list rewrites exactly those MBs
+    w, h = mbw * 16, mbh * 16
+    ref = rs.integers(0, 256, (h, w)).astype(np.uint8)
+    cur = np.roll(ref, (1, 2), axis=(0, 1))
+    i16, i8 = orc.me_fullsearch(cur, ref, 16, 1)
+    o16 = torch.full((nmb * 4,), -7, dtype=torch.int32, device="cuda")
+    o8 = torch.full((nmb * 16,), -7, dtype=torch.int32, device="cuda")
+    dcur, dref = torch.from_numpy(cur).cuda(), torch.from_numpy(ref).cuda()
+    assert pkg.hip.bc250_launch_me_fullsearch_list(C.c_void_p(dcur.data_ptr()), w, C.c_void_p(dref.data_ptr()), w, w,
+                                                   h, 16, 1, C.c_void_p(o16.data_ptr()), C.c_void_p(o8.data_ptr()),
+                                                   C.c_void_p(lst.data_ptr()), C.c_void_p(cnt.data_ptr()), None) == 0
+    g16 = o16.cpu().numpy().view(orc.IMV)
+    for m in range(nmb):
+        if m in got:
+            assert g16[m] == i16[m]
+        else:
+            assert g16[m]["dx"] == -7
