@@ -127,7 +127,7 @@ _sig(hip, "bc250_launch_me_fullsearch_list", _I, [_P, _I, _P, _I, _I, _I, _I, _I
 # the learned proposer (include/bc250_learned.h)
 _F = C.c_float
 _sig(hip, "bc250_cnn_param_count", C.c_size_t, [])
-_sig(hip, "bc250_cnn_input", _I, [_P, _P, _I, C.c_size_t, _I, _I, _P, _I, _P, _P])
+_sig(hip, "bc250_cnn_input", _I, [_P, _I, _P, _I, C.c_size_t, _I, _I, _P, _I, _P, _P])
 _sig(hip, "bc250_cnn_conv", _I, [_I, _I, _P, _P, _P, _P, _P, _P, _I, _P])
 _sig(hip, "bc250_cnn_pool", _I, [_P, _P, _I, _P])
 _sig(hip, "bc250_cnn_wgrad", _I, [_I, _P, _P, _P, _P, _I, _P])
@@ -140,6 +140,27 @@ _sig(hip, "bc250_cnn_mvin", _I, [_P, _I, _I, _P, _P])
 _sig(hip, "bc250_cnn_candidates", _I, [_P, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_launch_cand_search", _I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _I, _P, _P, _P])
 _sig(hip, "bc250_fallback_select", _I, [_P, _I, _I, _I, _I, _I, _P, _P, _P])
+
+
+class ProposerCfg(C.Structure):
+    """bc250_proposer_cfg_t (include/bc250_learned.h)."""
+    _fields_ = [("ncand", C.c_int32), ("alpha_q4", C.c_int32), ("beta", C.c_int32), ("t_high", C.c_int32)]
+
+
+_sig(hip, "gpu_compute_set_me_proposer", _I, [_CTX, _P, C.POINTER(ProposerCfg)])
+_sig(hip, "gpu_compute_get_me_fallbacks", _I, [_CTX, _I])
+# the default search of the learned form: 4 candidates, fallback above 1.5x
+# the neighbours' median SAD + 256 or above 6000
+PROPOSER_DEFAULT = dict(ncand=4, alpha_q4=24, beta=256, t_high=6000)
+WEIGHTS = os.path.join(_HERE, "weights", "me_proposer.f32")
+
+
+def load_proposer_weights(path=WEIGHTS):
+    """The trained proposer's parameters (raw little-endian float32)."""
+    w = np.fromfile(path, dtype="<f4")
+    if w.size != hip.bc250_cnn_param_count():
+        raise ValueError(f"{path}: {w.size} parameters, expected {hip.bc250_cnn_param_count()}")
+    return w
 _sig(hip, "bc250_launch_me_refine", _I, [_P, _I, _P, _I, _I, _I, _P, _P, _P, _P, _I, _P])
 _sig(hip, "bc250_launch_p_encode", _I, [_P, _P, _I, _I, _P, _P, _I, _I, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P,
                                        _P, _P, _P])
@@ -273,6 +294,20 @@ def h264_frame_nals(slices, is_idr):
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+_HIPRT = None
+
+
+def _hiprt():
+    """The HIP runtime (already loaded by libbc250_hip.so)."""
+    global _HIPRT
+    if _HIPRT is None:
+        _HIPRT = C.CDLL("libamdhip64.so", mode=C.RTLD_GLOBAL)
+        _HIPRT.hipMemcpy2D.restype = C.c_int
+        _HIPRT.hipMemcpy2D.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t,
+                                       C.c_int]
+    return _HIPRT
 
 
 # ---------------------------------------------------------------- CAVLC
@@ -490,6 +525,35 @@ class Encoder:
         total = info[4 * ns]
         data = C.string_at(d.value, total)
         return [data[info[4 * s]:info[4 * s] + info[4 * s + 1]] for s in range(ns)]
+
+    def set_proposer(self, params, ncand=4, alpha_q4=24, beta=256, t_high=6000):
+        """The learned integer search (gpu_compute_set_me_proposer); params
+        None turns it off."""
+        if params is None:
+            rc = hip.gpu_compute_set_me_proposer(C.byref(self.ctx), None, None)
+        else:
+            self._prop = np.ascontiguousarray(params, np.float32)
+            cfg = ProposerCfg(ncand, alpha_q4, beta, t_high)
+            rc = hip.gpu_compute_set_me_proposer(C.byref(self.ctx), _ptr(self._prop), C.byref(cfg))
+        if rc != 0:
+            raise ValueError("invalid proposer settings")
+
+    def fallbacks(self, slot):
+        """MBs the frame in `slot` searched in full after the learned search
+        (-1: the frame did not use it)."""
+        return hip.gpu_compute_get_me_fallbacks(C.byref(self.ctx), slot)
+
+    def recon_luma_device(self, torch):
+        """A device copy (torch uint8 [coded h, coded w]) of the newest
+        reconstruction's luma (8-bit pictures)."""
+        im = self.ctx.recon_image
+        out = torch.empty((im.coded_height, im.coded_width), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        if _hiprt().hipMemcpy2D(C.c_void_p(out.data_ptr()), C.c_size_t(im.coded_width), C.c_void_p(im.y_plane),
+                                C.c_size_t(im.y_pitch), C.c_size_t(im.coded_width), C.c_size_t(im.coded_height),
+                                3) != 0:
+            raise RuntimeError("hipMemcpy2D failed")
+        return out
 
     def set_slice_header(self, **kw):
         """gpu_compute_set_h264_slice_header: pps_id, pic_init_qp_minus26,

@@ -26,6 +26,7 @@
 #include "bc250_bitstream.h"
 #include "bc250_dev.h"
 #include "bc250_hevc.h"
+#include "bc250_learned.h"
 #include "bc250_stages.h"
 #include "gpu_compute.h"
 
@@ -58,7 +59,7 @@ struct Slot {
     uint32_t *modes = nullptr, *nz = nullptr;
     gpu_mv_t *mvs = nullptr, *mv8 = nullptr;
     uint8_t *hmodes = nullptr;  // HEVC PU modes
-    int *err = nullptr;  // 2 ints: intra wait error, deblock wait error
+    int *err = nullptr;  // 4 ints: intra wait error, deblock wait error, learned-search fallback MBs (-1: none)
     uint8_t *rbsp = nullptr;     // CAVLC slices of the frame (entropy = 1), pinned
     uint32_t *rbsp_info = nullptr;
     int slices = 0;              // their number; 0 = no slices for this frame
@@ -66,6 +67,23 @@ struct Slot {
     hipEvent_t cdone = nullptr;  // the frame's stages done (compute stream): its copies may start
     DevOut dev;
     bool pending = false, recorded = false, have_data = false, was_intra = false;
+};
+
+// The learned proposer's weights and per-resolution buffers
+// (gpu_compute_set_me_proposer).
+struct Proposer {
+    bool on = false;
+    bc250_proposer_cfg_t cfg{};
+    float *params = nullptr;     // fp32 [BC250_CNN_NPARAM]
+    uint16_t *packed = nullptr;  // bf16 [BC250_CNN_NPACKED]
+    int ntiles = 0, tw = 0;
+    int *tiles = nullptr;        // {0, tx, ty} per tile
+    uint16_t *a0 = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr;
+    float *rows = nullptr, *pool = nullptr, *mvin = nullptr, *z = nullptr;
+    int *cand = nullptr, *list = nullptr, *count = nullptr;
+    uint32_t *prev16 = nullptr;  // the previous picture's integer 16x16 vectors
+    bool has_prev = false;       // prev16 holds the picture before the current one
+    bool ran = false;            // the current picture ran the learned search
 };
 
 }  // namespace
@@ -113,6 +131,8 @@ struct bc250_hip_state {
     double total_sum = 0;
     int stage_frames = 0, stage_p_frames = 0;
     double me_int_sum_p = 0;
+    Proposer prop;
+    const int *fb_frame = nullptr;  // the frame being dispatched used the learned search: its fallback count
     // map_surface mirror
     void *map_host = nullptr;
     size_t map_size = 0;
@@ -210,6 +230,19 @@ static void free_res(bc250_hip_state *s)
     s->sync_h = nullptr; s->hmodes = nullptr; s->hcbf = nullptr; s->hculog2 = nullptr;
     for (void *p : {(void *)s->cavlc_out, (void *)s->cavlc_info, s->cavlc_work})
         if (p) hipFree(p);
+    {
+        Proposer &q = s->prop;
+        for (void *p : {(void *)q.tiles, (void *)q.a0, (void *)q.a1, (void *)q.a2, (void *)q.a3, (void *)q.rows,
+                        (void *)q.pool, (void *)q.mvin, (void *)q.z, (void *)q.cand, (void *)q.list, (void *)q.count,
+                        (void *)q.prev16})
+            if (p) hipFree(p);
+        q.tiles = q.cand = q.list = q.count = nullptr;
+        q.a0 = q.a1 = q.a2 = q.a3 = nullptr;
+        q.rows = q.pool = q.mvin = q.z = nullptr;
+        q.prev16 = nullptr;
+        q.ntiles = q.tw = 0;
+        q.has_prev = false;
+    }
     s->cavlc_out = nullptr; s->cavlc_info = nullptr; s->cavlc_work = nullptr;
     s->cavlc_cap = s->cavlc_work_size = 0;
     for (auto &sl : s->slot) {
@@ -270,7 +303,7 @@ static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
              HIP_OK(hipHostMalloc((void **)&sl.mvs, (size_t)nmb * 16, hipHostMallocDefault)) &&
              HIP_OK(hipHostMalloc((void **)&sl.mv8, (size_t)nmb * 64, hipHostMallocDefault)) &&
              HIP_OK(hipHostMalloc((void **)&sl.hmodes, (size_t)nmb * 16, hipHostMallocDefault)) &&
-             HIP_OK(hipHostMalloc((void **)&sl.err, 2 * sizeof(int), hipHostMallocDefault));
+             HIP_OK(hipHostMalloc((void **)&sl.err, 4 * sizeof(int), hipHostMallocDefault));
     }
     // wavefront workspaces start zeroed: no hand-off record or epoch holds a
     // value from whatever used the memory before
@@ -316,13 +349,15 @@ static void *ensure_xfer(bc250_hip_state *s, size_t bytes)
 // The wavefront kernels' wait-error flags of the frame ([0] intra, [1]
 // deblock; a null source reads as 0), stored straight into the slot's
 // pinned flag words by the compute stream at the end of the frame.
-__global__ void collect_flags_kernel(const int *e0, const int *e1, int *dst)
+__global__ void collect_flags_kernel(const int *e0, const int *e1, const int *fb, int *dst)
 {
     if (threadIdx.x == 0) {
         const int a = e0 ? __hip_atomic_load(e0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
         const int b = e1 ? __hip_atomic_load(e1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        const int c = fb ? __hip_atomic_load(fb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : -1;
         __hip_atomic_store(dst, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(dst + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(dst + 2, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __threadfence_system();
     }
 }
@@ -346,6 +381,79 @@ static int ensure_cavlc(bc250_hip_state *s)
     s->cavlc_cap = cap;
     s->cavlc_work_size = work;
     return 0;
+}
+
+// The proposer's per-resolution buffers (network activations of every 64x64
+// tile of the picture, candidates, fallback list), allocated at its first
+// frame at a resolution; borders of the activations start (and stay) zero.
+static int ensure_prop(bc250_hip_state *s)
+{
+    Proposer &q = s->prop;
+    if (q.a0) return 0;
+    const int tw = (s->mbw + 3) / 4, th = (s->mbh + 3) / 4, nt = tw * th;
+    const size_t t2 = bc250_cnn_tile_bytes(2), t16 = bc250_cnn_tile_bytes(16), t32 = bc250_cnn_tile_bytes(32);
+    bool ok = HIP_OK(hipMalloc(&q.tiles, (size_t)nt * 12)) && HIP_OK(hipMalloc(&q.a0, nt * t2)) &&
+              HIP_OK(hipMalloc(&q.a1, nt * t16)) && HIP_OK(hipMalloc(&q.a2, nt * t32)) &&
+              HIP_OK(hipMalloc(&q.a3, nt * t32)) && HIP_OK(hipMalloc(&q.rows, (size_t)nt * 64 * 4 * 32 * 4)) &&
+              HIP_OK(hipMalloc(&q.pool, (size_t)nt * 16 * 32 * 4)) &&
+              HIP_OK(hipMalloc(&q.mvin, (size_t)nt * 16 * 32 * 4)) &&
+              HIP_OK(hipMalloc(&q.z, (size_t)nt * 16 * BC250_CNN_NOUT * 4)) &&
+              HIP_OK(hipMalloc(&q.cand, (size_t)s->nmb * 4 * 8)) && HIP_OK(hipMalloc(&q.list, (size_t)s->nmb * 4)) &&
+              HIP_OK(hipMalloc(&q.count, 4)) && HIP_OK(hipMalloc(&q.prev16, (size_t)s->nmb * 16));
+    ok = ok && HIP_OK(hipMemset(q.a0, 0, nt * t2)) && HIP_OK(hipMemset(q.a1, 0, nt * t16)) &&
+         HIP_OK(hipMemset(q.a2, 0, nt * t32)) && HIP_OK(hipMemset(q.a3, 0, nt * t32)) &&
+         HIP_OK(hipMemset(q.count, 0, 4));
+    if (ok) {
+        int *h = (int *)malloc((size_t)nt * 12);
+        if (!h) ok = false;
+        for (int i = 0; ok && i < nt; i++) { h[3 * i] = 0; h[3 * i + 1] = i % tw; h[3 * i + 2] = i / tw; }
+        ok = ok && HIP_OK(hipMemcpy(q.tiles, h, (size_t)nt * 12, hipMemcpyHostToDevice));
+        free(h);
+    }
+    if (!ok) {
+        fprintf(stderr, "[bc250-hip] proposer buffer allocation failed\n");
+        return -1;
+    }
+    q.ntiles = nt;
+    q.tw = tw;
+    q.has_prev = false;
+    return 0;
+}
+
+// The learned integer search of one P picture: the network's candidates
+// refined in +-4 windows, the fallback list searched in full; the result is
+// out16 / out8 in the full search's format.  The 16x16 vectors are kept as
+// the next picture's neighbour input.
+static int enqueue_learned(bc250_hip_state *s, const uint8_t *c8, int cp, const uint8_t *r8, int rp, uint32_t *out16,
+                           uint32_t *out8, hipStream_t st)
+{
+    Proposer &q = s->prop;
+    if (ensure_prop(s) != 0) return -1;
+    const gpu_encode_params_t &p = s->params;
+    const int nt = q.ntiles;
+    auto wf = [&](int l) { return q.packed + bc250_cnn_packed_offset(l - 1, 0); };
+    auto bias = [&](int l, int cin) { return q.params + bc250_cnn_w_offset(l - 1) + (9 * cin + 31) / 32 * 32; };
+    int r = bc250_cnn_input(c8, cp, r8, rp, 0, s->wc, s->hc, q.tiles, nt, q.a0, st);
+    if (r == 0) r = bc250_cnn_conv(1, 0, q.a0, wf(1), bias(1, 2), q.a1, nullptr, nullptr, nt, st);
+    if (r == 0) r = bc250_cnn_conv(2, 0, q.a1, wf(2), bias(2, 16), q.a2, nullptr, nullptr, nt, st);
+    if (r == 0) r = bc250_cnn_conv(3, 0, q.a2, wf(3), bias(3, 32), q.a3, nullptr, nullptr, nt, st);
+    if (r == 0) r = bc250_cnn_conv(4, 2, q.a3, wf(4), bias(4, 32), nullptr, nullptr, q.rows, nt, st);
+    if (r == 0) r = bc250_cnn_pool(q.rows, q.pool, nt, st);
+    if (r == 0) r = bc250_cnn_mvin(q.has_prev ? q.prev16 : nullptr, s->mbw, s->mbh, q.mvin, st);
+    if (r == 0) r = bc250_cnn_head(q.params + bc250_cnn_w_offset(4), q.pool, q.mvin, q.z, nt, st);
+    if (r == 0) r = bc250_cnn_candidates(q.z, s->mbw, s->mbh, p.search_range, q.cand, st);
+    if (r == 0)
+        r = bc250_launch_cand_search(c8, cp, r8, rp, s->wc, s->hc, p.search_range, p.lambda8, q.cand, q.cfg.ncand,
+                                     out16, out8, st);
+    if (r == 0)
+        r = bc250_fallback_select(out16, s->mbw, s->mbh, q.cfg.alpha_q4, q.cfg.beta, q.cfg.t_high, q.list, q.count,
+                                  st);
+    if (r == 0)
+        r = bc250_launch_me_fullsearch_list(c8, cp, r8, rp, s->wc, s->hc, p.search_range, p.lambda8, out16, out8,
+                                            q.list, q.count, st);
+    if (r == 0) r = HIP_OK(hipMemcpyAsync(q.prev16, out16, (size_t)s->nmb * 16, hipMemcpyDeviceToDevice, st)) ? 0 : -1;
+    q.ran = r == 0;
+    return r;
 }
 
 static bc250_hip_state *st_of(gpu_context_t *ctx) { return ctx ? ctx->hip : nullptr; }
@@ -422,6 +530,8 @@ extern "C" void bc250_gpu_destroy(bc250_gpu_context_t *ctx)
     free_res(s);
     if (s->map_host) hipHostFree(s->map_host);
     if (s->xfer) hipHostFree(s->xfer);
+    if (s->prop.params) hipFree(s->prop.params);
+    if (s->prop.packed) hipFree(s->prop.packed);
     for (auto &sl : s->slot) {
         for (auto &e : sl.ev)
             if (e) hipEventDestroy(e);
@@ -679,17 +789,23 @@ static void start_frame(bc250_hip_state *s)
     Slot &sl = s->slot[s->cur];
     if (sl.recorded) hipStreamWaitEvent(s->stream, sl.ev[NEV - 1], 0);
     bind_slot(s, s->cur);
+    // the learned search's neighbour input is the previous picture's vectors
+    // only when that picture ran it (not after an I picture)
+    s->prop.has_prev = s->prop.ran;
+    s->prop.ran = false;
+    s->fb_frame = nullptr;
 }
 
 // Ends a frame's stages: the wavefront error flags (e0 intra, e1 deblock;
-// null = the stage did not run) go to the slot's pinned flag words, and,
+// null = the stage did not run) and the learned search's fallback count
+// (fb, null = not used) go to the slot's pinned flag words, and,
 // with readback, the copy stream copies the listed outputs to the slot's
 // pinned staging once the stages are done.
 enum { RB_LEVELS = 1, RB_DC = 2, RB_MODES = 4, RB_MVS = 8, RB_NZ = 16, RB_MV8 = 32, RB_HMODES = 64 };
-static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *e1, int what)
+static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *e1, const int *fb, int what)
 {
     const size_t n = (size_t)s->nmb;
-    collect_flags_kernel<<<1, 64, 0, s->stream>>>(e0, e1, sl.err);
+    collect_flags_kernel<<<1, 64, 0, s->stream>>>(e0, e1, fb, sl.err);
     if (!HIP_OK(hipGetLastError()) || !HIP_OK(hipEventRecord(sl.cdone, s->stream)) ||
         !HIP_OK(hipStreamWaitEvent(s->cstream, sl.cdone, 0)))
         return -1;
@@ -727,8 +843,14 @@ static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &r
         if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, s->nmb * 16, s->stream);
         return r;
     }
-    int r = bc250_launch_me_fullsearch((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc,
+    int r;
+    if (s->prop.on) {
+        r = enqueue_learned(s, (const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->imv16, s->imv8, s->stream);
+        s->fb_frame = s->prop.count;
+    } else {
+        r = bc250_launch_me_fullsearch((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc,
                                        p.search_range, p.lambda8, s->imv16, s->imv8, s->stream);
+    }
     hipEventRecord(mid, s->stream);
     if (r == 0)
         r = bc250_launch_me_refine((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc, s->imv16,
@@ -760,9 +882,13 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
             r8 = m1;
             cp8 = rp8 = s->me8_pitch;
         }
-        if (r == 0)
+        if (r == 0 && s->prop.on) {
+            r = enqueue_learned(s, c8, cp8, r8, rp8, s->imv16, s->imv8, st);
+            s->fb_frame = s->prop.count;
+        } else if (r == 0) {
             r = bc250_launch_me_fullsearch(c8, cp8, r8, rp8, s->wc, s->hc, p.search_range, p.lambda8, s->imv16,
                                            s->imv8, st);
+        }
     }
     hipEventRecord(sl.ev[1], st);
     // P pictures: the quarter-sample refinement runs fused with the CU coding
@@ -798,7 +924,9 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
                                               s->mv8, cus, num_slices, p.hevc_ctu_log2, bd, st);
     }
     hipEventRecord(sl.ev[4], st);
-    if (r == 0) r = finish_frame(s, sl, is_intra ? s->sync_h + 1 : nullptr, nullptr, RB_LEVELS | RB_HMODES | RB_MV8);
+    if (r == 0)
+        r = finish_frame(s, sl, is_intra ? s->sync_h + 1 : nullptr, nullptr, s->fb_frame,
+                         RB_LEVELS | RB_HMODES | RB_MV8);
     return r;
 }
 
@@ -917,6 +1045,7 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     // with deblocking off the deblock workspace is never written
     if (r == 0)
         r = finish_frame(s, sl, is_intra ? s->sync_i + 1 : nullptr, p.deblock_idc != 1 ? s->sync_d + 1 : nullptr,
+                         s->fb_frame,
                          RB_LEVELS | RB_DC | RB_MODES | RB_MVS | RB_NZ | RB_MV8);
     sl.have_data = r == 0 && p.readback;
     sl.was_intra = is_intra != 0;
@@ -960,7 +1089,7 @@ extern "C" int gpu_compute_dispatch_me_only(gpu_context_t *ctx, gpu_image_t rt, 
     hipEventRecord(sl.ev[0], s->stream);
     int r = enqueue_me(s, rt, ref, 0, sl.ev[1]);
     for (int i = 2; i < NEV - 1; i++) hipEventRecord(sl.ev[i], s->stream);
-    if (r == 0) r = finish_frame(s, sl, nullptr, nullptr, RB_MVS | RB_MV8);
+    if (r == 0) r = finish_frame(s, sl, nullptr, nullptr, s->fb_frame, RB_MVS | RB_MV8);
     sl.have_data = false;
     sl.recorded = false;
     sl.slices = 0;
@@ -1161,6 +1290,37 @@ extern "C" int gpu_compute_get_slices_slot(gpu_context_t *ctx, int slot, const u
     *info = sl.rbsp_info;
     *num_slices = sl.slices;
     return 0;
+}
+
+extern "C" int gpu_compute_set_me_proposer(gpu_context_t *ctx, const float *params, const bc250_proposer_cfg_t *cfg)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s) return -1;
+    Proposer &q = s->prop;
+    if (!params || !cfg) {
+        q.on = false;
+        return 0;
+    }
+    if (cfg->ncand < 1 || cfg->ncand > 4 || cfg->alpha_q4 < 0 || cfg->beta < 0 || cfg->t_high < 0) return -1;
+    if (!q.params && (!HIP_OK(hipMalloc(&q.params, BC250_CNN_NPARAM * sizeof(float))) ||
+                      !HIP_OK(hipMalloc(&q.packed, BC250_CNN_NPACKED * sizeof(uint16_t)))))
+        return -1;
+    if (!HIP_OK(hipStreamSynchronize(s->stream)) ||
+        !HIP_OK(hipMemcpy(q.params, params, BC250_CNN_NPARAM * sizeof(float), hipMemcpyHostToDevice)) ||
+        bc250_cnn_pack(q.params, q.packed, s->stream) != 0 || !HIP_OK(hipStreamSynchronize(s->stream)))
+        return -1;
+    q.cfg = *cfg;
+    q.on = true;
+    return 0;
+}
+
+extern "C" int gpu_compute_get_me_fallbacks(gpu_context_t *ctx, int slot)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || slot < 0 || slot >= NSLOT) return -1;
+    const Slot &sl = s->slot[slot];
+    if (!sl.err || sl.pending || !sl.recorded) return -1;
+    return sl.err[2];
 }
 
 extern "C" int gpu_compute_get_encode_params(gpu_context_t *ctx, gpu_encode_params_t *p)

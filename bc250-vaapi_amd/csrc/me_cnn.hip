@@ -258,9 +258,9 @@ __global__ void sum_rows_kernel(const float *__restrict__ in, int n, int m, floa
 // Input tiles: (cur - 128) / 64 and (ref - 128) / 64, bf16, into the
 // interior of [tile][66][66][2]; samples beyond the picture are the nearest
 // edge sample.  tiles[i] = {picture slot, tile x, tile y}; the picture pair
-// of slot s is cur + s * cur_stride, ref + s * ref_stride.
-__global__ void cnn_input_kernel(const uint8_t *__restrict__ cur, const uint8_t *__restrict__ ref, int pitch,
-                                 size_t slot_stride, int w, int h, const int3 *__restrict__ tiles,
+// of slot s is cur + s * slot_stride, ref + s * slot_stride.
+__global__ void cnn_input_kernel(const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref,
+                                 int rpitch, size_t slot_stride, int w, int h, const int3 *__restrict__ tiles,
                                  uint16_t *__restrict__ in, int ntiles)
 {
     const int i = blockIdx.x, t = threadIdx.x;  // 256 threads: 16 pixels each
@@ -270,8 +270,8 @@ __global__ void cnn_input_kernel(const uint8_t *__restrict__ cur, const uint8_t 
     for (int p = t; p < T * T; p += 256) {
         const int yy = p / T, xx = p % T;
         const int gy = min(tl.z * T + yy, h - 1), gx = min(tl.y * T + xx, w - 1);
-        const float a = ((int)c[(size_t)gy * pitch + gx] - 128) * (1.f / 64.f);
-        const float b = ((int)r[(size_t)gy * pitch + gx] - 128) * (1.f / 64.f);
+        const float a = ((int)c[(size_t)gy * cpitch + gx] - 128) * (1.f / 64.f);
+        const float b = ((int)r[(size_t)gy * rpitch + gx] - 128) * (1.f / 64.f);
         *(uint32_t *)(in + (((size_t)i * TP + yy + 1) * TP + xx + 1) * 2) = (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
     }
 }
@@ -426,12 +426,12 @@ extern "C" size_t bc250_cnn_param_count(void) { return BC250_CNN_NPARAM; }
 
 extern "C" size_t bc250_cnn_tile_bytes(int channels) { return (size_t)TP * TP * channels * 2; }
 
-extern "C" int bc250_cnn_input(const uint8_t *cur, const uint8_t *ref, int pitch, size_t slot_stride, int w, int h,
-                               const int *tiles, int ntiles, uint16_t *in, void *stream)
+extern "C" int bc250_cnn_input(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, size_t slot_stride,
+                               int w, int h, const int *tiles, int ntiles, uint16_t *in, void *stream)
 {
-    if (!cur || !ref || !tiles || !in || ntiles <= 0 || w <= 0 || h <= 0 || pitch < w) return -1;
-    cnn_input_kernel<<<ntiles, 256, 0, (hipStream_t)stream>>>(cur, ref, pitch, slot_stride, w, h, (const int3 *)tiles,
-                                                            in, ntiles);
+    if (!cur || !ref || !tiles || !in || ntiles <= 0 || w <= 0 || h <= 0 || cpitch < w || rpitch < w) return -1;
+    cnn_input_kernel<<<ntiles, 256, 0, (hipStream_t)stream>>>(cur, cpitch, ref, rpitch, slot_stride, w, h,
+                                                            (const int3 *)tiles, in, ntiles);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

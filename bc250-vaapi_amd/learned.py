@@ -107,7 +107,7 @@ class Net:
         t = self.torch.as_tensor(np.ascontiguousarray(tiles, np.int32).reshape(-1)).cuda()
         n = t.numel() // 3
         ptr = lambda a: C.c_void_p(a if isinstance(a, int) else a.data_ptr())
-        self._ok(self.hip.bc250_cnn_input(ptr(cur), ptr(ref), pitch, slot_stride, w, h, self._p(t), n,
+        self._ok(self.hip.bc250_cnn_input(ptr(cur), pitch, ptr(ref), pitch, slot_stride, w, h, self._p(t), n,
                                           self._p(self.act[0]), None), "input")
         return n
 

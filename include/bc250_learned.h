@@ -51,10 +51,10 @@ size_t bc250_cnn_param_count(void);
 size_t bc250_cnn_tile_bytes(int channels);
 
 /* Input tiles from 8-bit luma pictures: tiles[3 i .. 3 i + 2] = {picture
- * slot, tile x, tile y}; slot s reads cur / ref + s * slot_stride (pitch
- * `pitch`, w x h samples, edge samples repeated past the picture). */
-int bc250_cnn_input(const uint8_t *cur, const uint8_t *ref, int pitch, size_t slot_stride, int w, int h,
-                    const int *tiles, int ntiles, uint16_t *in, void *stream);
+ * slot, tile x, tile y}; slot s reads cur / ref + s * slot_stride (pitches
+ * cpitch / rpitch, w x h samples, edge samples repeated past the picture). */
+int bc250_cnn_input(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, size_t slot_stride, int w,
+                    int h, const int *tiles, int ntiles, uint16_t *in, void *stream);
 
 /* One layer.  mode 0: out = relu(conv(in, w) + b); mode 2 (layer 4 only):
  * the same, plus the bf16 outputs summed over each 16-pixel row segment into
@@ -126,6 +126,32 @@ int bc250_launch_cand_search(const uint8_t *cur, int cpitch, const uint8_t *ref,
  * order).  *count is cleared on the stream first. */
 int bc250_fallback_select(const uint32_t *r16, int mbw, int mbh, int alpha_q4, int beta, int t_high, int *list,
                           int *count, void *stream);
+
+/* ---- in the pipeline ------------------------------------------------------
+ * With a proposer set, the integer search of every P picture (H.264 and
+ * HEVC; +-search_range as set by gpu_compute_set_encode_params) is the
+ * learned one: the network over the picture's 64x64 tiles (current luma and
+ * the reference reconstruction; at 10 bits their top 8 bits), the first
+ * ncand candidates of every MB searched in +-4 windows, and the MBs whose
+ * SAD exceeds the neighbour bound (alpha_q4, beta, t_high as in
+ * bc250_fallback_select) searched in full.  Everything after the integer
+ * search is unchanged. */
+typedef struct {
+    int32_t ncand;     /* 1..4 */
+    int32_t alpha_q4;  /* fallback bound: alpha_q4 / 16 * median of the neighbours' SADs ... */
+    int32_t beta;      /* ... + beta */
+    int32_t t_high;    /* and every MB above this SAD */
+} bc250_proposer_cfg_t;
+
+/* params: BC250_CNN_NPARAM host floats (copied), cfg: the search; either
+ * NULL turns the proposer off (the plain full search again).  Returns 0, or
+ * -1 for out-of-range fields or a failed allocation. */
+struct bc250_gpu_context;
+int gpu_compute_set_me_proposer(struct bc250_gpu_context *ctx, const float *params, const bc250_proposer_cfg_t *cfg);
+
+/* MBs the frame in `slot` searched in full after the learned search
+ * (after gpu_compute_sync_slot), -1 when the frame did not use it. */
+int gpu_compute_get_me_fallbacks(struct bc250_gpu_context *ctx, int slot);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,10 @@ def _same(what, frame, got, want):
 
 
 def _sample_mbs(nmb, mbw, n, seed):
+    """A seeded sample of n MBs plus the four corners; none when n is 0 (the
+    caller checks the search itself)."""
+    if n == 0:
+        return []
     mbh = nmb // mbw
     corners = [0, mbw - 1, (mbh - 1) * mbw, nmb - 1]
     rs = np.random.default_rng(seed)

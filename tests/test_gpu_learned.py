@@ -261,3 +261,125 @@ def test_fallback_select_and_list_search(pkg):
             assert g16[m] == i16[m]
         else:
             assert g16[m]["dx"] == -7
+
+
+def _fixed_candidate_params(learned, cands):
+    """Parameters whose head ignores its input: every block's hypotheses are
+    `cands` (pel) and the confidences fall in their listed order."""
+    p = np.zeros(learned.NPARAM, np.float32)
+    h = p[learned.OFFSETS[4]:].reshape(learned.NOUT, learned.NFEAT + 1)
+    for k, (dx, dy) in enumerate(cands):
+        h[2 * k, -1], h[2 * k + 1, -1] = dx / 16.0, dy / 16.0
+        h[8 + k, -1] = 4.0 - k
+    return p
+
+
+def _learned_int_search(cur, ref, R, lam, cands, ncand, alpha, beta, high):
+    """CPU restatement of the learned integer search with fixed candidates:
+    per MB the best keys over the +-4 windows, then the fallback rule on the
+    16x16 SADs, the oracle's full search for the listed MBs."""
+    h, w = cur.shape
+    mbw, mbh = w // 16, h // 16
+    nmb = mbw * mbh
+    r16 = np.zeros(nmb, orc.IMV)
+    r8 = np.zeros(nmb * 4, orc.IMV)
+    for m in range(nmb):
+        mbx, mby = m % mbw, m // mbw
+        b16, bq = None, [None] * 4
+        for cx, cy in cands[:ncand]:
+            for dy in range(cy - 4, cy + 5):
+                for dx in range(cx - 4, cx + 5):
+                    if abs(dx) > R or abs(dy) > R:
+                        continue
+                    k16, k8 = _keys(cur, ref, mbx, mby, dx, dy, R, lam)
+                    b16 = k16 if b16 is None else min(b16, k16)
+                    bq = [v if b is None else min(b, v) for b, v in zip(bq, k8)]
+        for rec, key, mul in [(r16[m:m + 1], b16, 4)] + [(r8[4 * m + q:4 * m + q + 1], bq[q], 1) for q in range(4)]:
+            idx, cost = key & 0x7FFF, key >> 15
+            dx, dy = idx % (2 * R + 1) - R, idx // (2 * R + 1) - R
+            rec[0] = (dx, dy, cost, cost - mul * lam * (abs(dx) + abs(dy)))
+    fall = []
+    for m in range(nmb):
+        mx, my = m % mbw, m // mbw
+        nb = sorted(int(r16["sad"][k]) for k, ok in ((m - 1, mx > 0), (m + 1, mx + 1 < mbw), (m - mbw, my > 0),
+                                                     (m + mbw, my + 1 < mbh)) if ok)
+        med = nb[(len(nb) - 1) // 2] if nb else 0
+        s = int(r16["sad"][m])
+        if s > (alpha * med >> 4) + beta or s > high:
+            fall.append(m)
+    for m in fall:
+        a, b = orc.me_fullsearch_mb(cur, ref, R, lam, m % mbw, m // mbw)
+        r16[m], r8[4 * m:4 * m + 4] = a, b
+    return r16, r8, fall
+
+
+@pytest.mark.parametrize("codec", ["h264", "hevc"])
+def test_pipeline_learned_search_with_fixed_candidates(pkg, learned, codec):
+    """The learned integer search inside the pipeline (set_proposer): with
+    the network's output pinned to four known candidates, every P frame's
+    vectors are the CPU restatement's (windows, fallback rule, full search
+    of the fallback MBs, then the codec's own sub-pel refinement), the
+    fallback count reported per frame is the restatement's, and the coding
+    after it is the oracle's (tests/seqcheck.py with the GPU's vectors)."""
+    import seqcheck
+    w, h, R, lam, qp = 128, 96, 16, 1, 30
+    cands = [(0, 0), (5, -3), (-9, 6), (12, 12)]
+    cfg = dict(ncand=3, alpha_q4=20, beta=200, t_high=3000)
+    g = pkg.FrameGen(33, w, h, 3)
+    frames = [g.frame(i) for i in range(4)]
+    g.close()
+    c = pkg.GPU_CODEC_HEVC if codec == "hevc" else pkg.GPU_CODEC_H264
+    enc = pkg.Encoder(w, h, search_range=R, lambda8=lam, deblock_idc=0, readback=1, codec=c)
+    fb_counts = []
+    try:
+        enc.set_proposer(_fixed_candidate_params(learned, cands), **cfg)
+        images = []
+        for y, uv in frames:
+            im = enc.new_image()
+            enc.upload(im, y, uv)
+            images.append(im)
+        got = []
+        for i, im in enumerate(images):
+            slot = enc.encode(im, qp, i == 0, 1)
+            r = enc.results(slot)
+            r["is_intra"] = i == 0
+            got.append(r)
+            fb_counts.append(enc.fallbacks(slot))
+    finally:
+        enc.close()
+    coded = [pkg.pad_coded(*f) for f in frames]
+    assert fb_counts[0] == -1  # the I frame has no search
+    ref_y, ref_uv = coded[0]
+    # the oracle chain: the reconstruction each P frame searched is the
+    # previous frame's as the coding check rebuilds it
+    recon = []
+    if codec == "hevc":
+        seqcheck.check_hevc(coded[:1], got[:1], qp, 1, R, lam)
+    for i in range(1, len(frames)):
+        cy, cuv = coded[i]
+        if codec == "hevc":
+            ry, ruv = seqcheck.check_hevc(coded[:i], got[:i], qp, 1, R, lam, sample=0)
+        else:
+            ry, ruv = seqcheck.check_h264(coded[:i], got[:i], qp, 1, R, lam, sample=0)
+        r16, r8, fall = _learned_int_search(cy, ry, R, lam, cands, cfg["ncand"], cfg["alpha_q4"], cfg["beta"],
+                                            cfg["t_high"])
+        assert fb_counts[i] == len(fall), (i, fb_counts[i], len(fall))
+        mbw = w // 16
+        if codec == "hevc":
+            for t in range((w // 16) * (h // 16)):
+                for z in range(4):
+                    x0, y0 = (t % mbw) * 16 + (z & 1) * 8, (t // mbw) * 16 + (z >> 1) * 8
+                    mx, my, sad = orc.hevc_refine8(cy, ry, x0, y0, int(r8[4 * t + z]["dx"]) * 4,
+                                                   int(r8[4 * t + z]["dy"]) * 4)
+                    gv = got[i]["mv8"][4 * t + z]
+                    assert (int(gv["mvx"]), int(gv["mvy"]), int(gv["sad"])) == (mx, my, sad), (i, t, z)
+        else:
+            for m in range((w // 16) * (h // 16)):
+                m16, m8 = orc.me_refine_mb(cy, ry, m % mbw, m // mbw, r16[m], r8[4 * m:4 * m + 4])
+                assert got[i]["mv"][m] == m16, (i, m)
+                assert (got[i]["mv8"][4 * m:4 * m + 4] == m8).all(), (i, m)
+    # and the coding of every frame with those vectors
+    if codec == "hevc":
+        seqcheck.check_hevc(coded, got, qp, 1, R, lam, sample=0)
+    else:
+        seqcheck.check_h264(coded, got, qp, 1, R, lam, sample=0)
