@@ -140,18 +140,21 @@ def cpu_baseline(frames_coded, qp, rng, lam, budget_s, codec):
             "frame_s": round(frame_s, 2)}
 
 
-def run_pipeline(pkg, torch, frames, codec, args, entropy=0, qps=None):
+def run_pipeline(pkg, torch, frames, codec, args, entropy=0, qps=None, proposer=None):
     """Uploads the frames to a fresh context and, for each QP, runs the
     warm-up and the timed IPPP sequence; returns, per QP, fps, per-frame
     latencies and the stage statistics from the pipeline's own HIP events.
     entropy 0: readback on (the staging reaches the host); entropy 1
     (H.264): every frame writes its CAVLC slices, which the driver's C
-    consumer packs into an Annex B stream in host memory."""
+    consumer packs into an Annex B stream in host memory.  proposer: the
+    learned integer search's weights (None: the plain full search)."""
     enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, me_8x8=1,
                       readback=0 if entropy else 1,
                       codec=pkg.GPU_CODEC_HEVC if codec == "hevc" else pkg.GPU_CODEC_H264, entropy=entropy)
     out = {}
     try:
+        if proposer is not None:
+            enc.set_proposer(proposer, **pkg.PROPOSER_DEFAULT)
         images = []
         for y, uv in frames:
             im = enc.new_image()
@@ -183,6 +186,69 @@ def run_pipeline(pkg, torch, frames, codec, args, entropy=0, qps=None):
         return out
     finally:
         enc.close()
+
+
+def bd_rate(r0, p0, r1, p1):
+    """Bjontegaard rate difference (%) of curve 1 against curve 0: cubic fits
+    of log10(rate) over PSNR, averaged over the common PSNR interval."""
+    c0, c1 = np.polyfit(p0, np.log10(r0), 3), np.polyfit(p1, np.log10(r1), 3)
+    lo, hi = max(min(p0), min(p1)), min(max(p0), max(p1))
+    i0, i1 = np.polyint(c0), np.polyint(c1)
+    a0 = (np.polyval(i0, hi) - np.polyval(i0, lo)) / (hi - lo)
+    a1 = (np.polyval(i1, hi) - np.polyval(i1, lo)) / (hi - lo)
+    return (10 ** (a1 - a0) - 1) * 100
+
+
+def learned_quality(pkg, frames, args, weights, qps=(22, 27, 32, 37), n=24):
+    """Rate and quality of the learned integer search against the plain one
+    on the bench's frames (seed 1080, outside the training and held-out
+    seeds): the H.264 pipeline with its CAVLC slices (the stream's bytes;
+    HEVC has no entropy coder here), PSNR-Y of every frame's reconstruction,
+    I + (n-1) P frames per QP; plus the fraction of MBs the learned form
+    searched in full and of 16x16 vectors equal to the plain form's."""
+    res = {"plain": {}, "learned": {}}
+    vec_same, fb = [], []
+    for qp in qps:
+        mvs = {}
+        for mode in ("plain", "learned"):
+            enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, readback=1,
+                              entropy=1)
+            try:
+                if mode == "learned":
+                    enc.set_proposer(weights, **pkg.PROPOSER_DEFAULT)
+                nbytes, psnr, mv = 0, [], []
+                for i, (y, uv) in enumerate(frames[:n]):
+                    im = enc.new_image()
+                    enc.upload(im, y, uv)
+                    slot = enc.encode(im, qp, i == 0, 1)
+                    nbytes += len(pkg.h264_frame_nals(enc.slices(slot), i == 0))
+                    if mode == "learned" and i > 0:
+                        fb.append(enc.fallbacks(slot) / enc.nmb)
+                    mv.append(enc.staging(slot, "mv"))
+                    ry, _ = enc.recon()
+                    mse = np.mean((ry.astype(np.float64) - y) ** 2)
+                    psnr.append(10 * np.log10(255.0 ** 2 / max(mse, 1e-10)))
+                    enc.images.remove(im)
+                    pkg.hip.gpu_compute_destroy_image(pkg.C.byref(enc.ctx), im[0], im[1])
+            finally:
+                enc.close()
+            res[mode][qp] = {"bytes": nbytes, "psnr_y": float(np.mean(psnr))}
+            mvs[mode] = mv
+        vec_same += [float(np.mean((a["mvx"] == b["mvx"]) & (a["mvy"] == b["mvy"])))
+                     for a, b in zip(mvs["plain"][1:], mvs["learned"][1:])]
+        log(f"learned quality qp {qp}: plain {res['plain'][qp]}, learned {res['learned'][qp]}")
+    r0 = [res["plain"][q]["bytes"] for q in qps]
+    p0 = [res["plain"][q]["psnr_y"] for q in qps]
+    r1 = [res["learned"][q]["bytes"] for q in qps]
+    p1 = [res["learned"][q]["psnr_y"] for q in qps]
+    return {"bd_rate_pct": round(bd_rate(r0, p0, r1, p1), 3),
+            "bd_rate_basis": f"H.264 CAVLC stream bytes and PSNR-Y, CQP {'/'.join(map(str, qps))}, "
+                             f"{n} frames IPPP of the bench's 1080p sequence",
+            "per_qp": {str(q): {"plain_bytes": res["plain"][q]["bytes"], "learned_bytes": res["learned"][q]["bytes"],
+                                "plain_psnr_y": round(res["plain"][q]["psnr_y"], 3),
+                                "learned_psnr_y": round(res["learned"][q]["psnr_y"], 3)} for q in qps},
+            "fallback_mb_frac": round(float(np.mean(fb)), 4),
+            "vectors_equal_to_plain_frac": round(float(np.mean(vec_same)), 4)}
 
 
 def verify_pipelines(pkg, frames, args, codecs, n=4):
@@ -433,6 +499,7 @@ def main():
     ap.add_argument("--single", action="store_true", help="skip the second codec's run")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle integer search")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-ME line")
     ap.add_argument("--workload", choices=("1080p", "4k-main10", "t32", "cavlc"), default="1080p",
                     help="4k-main10: the 3840x2160 P010 HEVC Main10 workload instead of the 1080p headline; "
                          "t32: the 32x32 transform forms compared; cavlc: the H.264 CAVLC slice writer")
@@ -464,6 +531,10 @@ def main():
     if not args.single:
         sweeps[other] = run_pipeline(pkg, torch, frames, other, args, qps=qps)
         sweeps["h264_cavlc"] = run_pipeline(pkg, torch, frames, "h264", args, entropy=1, qps=qps)
+    weights = None
+    if not args.no_learned and os.path.exists(pkg.WEIGHTS):
+        weights = pkg.load_proposer_weights()
+        sweeps["hevc_learned"] = run_pipeline(pkg, torch, frames, "hevc", args, qps=[args.qp], proposer=weights)
     runs = {k: v[args.qp] for k, v in sweeps.items()}
     head = runs[args.codec]
     st = head["stats"]
@@ -563,6 +634,19 @@ def main():
                                            "p99_frame_ms": round(float(np.percentile(o["frame_ms"], 99)), 4),
                                            "stream_bytes_per_frame": round(o["stream_bytes"] / args.steps),
                                            "stage_ms": stage_ms(o["stats"]), "wavefront_wait_errors": o["errs"]}
+    if weights is not None:
+        lr, pl = runs["hevc_learned"], runs["hevc"]
+        res["learned_me"] = {
+            "weights": os.path.relpath(pkg.WEIGHTS, ROOT),
+            "hevc_fps": round(lr["fps"], 2), "plain_hevc_fps": round(pl["fps"], 2),
+            "speedup": round(lr["fps"] / pl["fps"], 3),
+            "p99_frame_ms": round(float(np.percentile(lr["frame_ms"], 99)), 4),
+            "stage_ms": stage_ms(lr["stats"]),
+            "search": "network over the picture's 64x64 tiles, 4 candidates per MB searched in +-4 windows, "
+                      "MBs above the neighbour SAD bound searched in full (+-64)",
+        }
+        log("learned ME: rate and quality against plain")
+        res["learned_me"].update(learned_quality(pkg, frames, args, weights))
     if not args.no_cpu:
         log("cpu leg: checking the timed pipelines against the oracle")
         pipes = [(args.codec, 0)] + ([(other, 0), ("h264", 1)] if not args.single else [])

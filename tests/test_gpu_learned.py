@@ -383,3 +383,25 @@ def test_pipeline_learned_search_with_fixed_candidates(pkg, learned, codec):
         seqcheck.check_hevc(coded, got, qp, 1, R, lam, sample=0)
     else:
         seqcheck.check_h264(coded, got, qp, 1, R, lam, sample=0)
+
+
+def test_trained_network_gains_over_no_network(pkg, learned):
+    """The committed weights on held-out sequences (seeds 9100.., not seen
+    in training): the share of MBs whose candidate search alone (before any
+    fallback) finds exactly the +-64 full search's 16x16 result is higher
+    with the network's candidates than with the same search's no-network
+    candidates (zero, the previous picture's co-located, left and top
+    vectors), and misses fall by at least a third."""
+    import importlib.util
+    import os
+    import torch
+    here = os.path.dirname(pkg.__file__)
+    spec = importlib.util.spec_from_file_location("bc250_train_py", os.path.join(here, "train_proposer.py"))
+    tp = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(tp)
+    say("learned: held-out hit rates, 2 sequences x 7 frames at 1080p")
+    ds = tp.Dataset(pkg, torch, [9100, 9101], 7, seed_offset=1)
+    hits = tp.hit_rates(ds, pkg, learned, torch, pkg.load_proposer_weights())
+    say(f"learned: {hits}")
+    assert hits["network"] > hits["no_network"]
+    assert (1 - hits["network"]) < (2 / 3) * (1 - hits["no_network"]), hits
