@@ -290,13 +290,33 @@ __device__ __forceinline__ void head_block(const float *__restrict__ wh, const f
     }
 }
 
-// Inference head: every block's 12 outputs into z [tile][16][12].
+// Inference head: every block's 12 outputs into z [tile][16][12], one
+// thread per (block, output) with the block's 64 inputs read as float4s
+// (the same sums in the same order as head_block).
 __global__ void cnn_head_kernel(const float *__restrict__ wh, const float *__restrict__ pool,
                                 const float *__restrict__ mvin, float *__restrict__ z, int nblocks)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblocks) return;
-    head_block(wh, pool + (size_t)b * 32, mvin + (size_t)b * 32, z + (size_t)b * BC250_CNN_NOUT);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nblocks * BC250_CNN_NOUT) return;
+    const int b = i / BC250_CNN_NOUT, o = i % BC250_CNN_NOUT;
+    const float *row = wh + o * (BC250_CNN_NFEAT + 1);
+    const float4 *pb = (const float4 *)(pool + (size_t)b * 32), *mb = (const float4 *)(mvin + (size_t)b * 32);
+    float s = row[BC250_CNN_NFEAT];
+    for (int q = 0; q < 8; q++) {
+        const float4 v = pb[q];
+        s += row[4 * q] * (v.x * (1.f / 256.f));
+        s += row[4 * q + 1] * (v.y * (1.f / 256.f));
+        s += row[4 * q + 2] * (v.z * (1.f / 256.f));
+        s += row[4 * q + 3] * (v.w * (1.f / 256.f));
+    }
+    for (int q = 0; q < 8; q++) {
+        const float4 v = mb[q];
+        s += row[32 + 4 * q] * v.x;
+        s += row[32 + 4 * q + 1] * v.y;
+        s += row[32 + 4 * q + 2] * v.z;
+        s += row[32 + 4 * q + 3] * v.w;
+    }
+    z[i] = s;
 }
 
 __device__ __forceinline__ float huber(float d, float delta, float *g)
@@ -510,7 +530,7 @@ extern "C" int bc250_cnn_head(const float *wh, const float *pool, const float *m
 {
     if (ntiles <= 0 || !wh || !pool || !mvin || !z) return -1;
     const int nb = ntiles * 16;
-    cnn_head_kernel<<<(nb + 127) / 128, 128, 0, (hipStream_t)stream>>>(wh, pool, mvin, z, nb);
+    cnn_head_kernel<<<(nb * BC250_CNN_NOUT + 255) / 256, 256, 0, (hipStream_t)stream>>>(wh, pool, mvin, z, nb);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
