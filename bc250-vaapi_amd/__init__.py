@@ -148,6 +148,125 @@ class ProposerCfg(C.Structure):
 
 
 _sig(hip, "gpu_compute_set_me_proposer", _I, [_CTX, _P, C.POINTER(ProposerCfg)])
+_sig(hip, "bc250_cnn_set_grid_cap", _I, [_I])
+
+
+# ---------------------------------------------------------------- serving (include/bc250_serve.h)
+
+class ServerCfg(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", C.c_int32), ("learned", C.c_int32),
+                ("search_range", C.c_int32), ("lambda8", C.c_int32), ("proposer", ProposerCfg)]
+
+
+class SessionCfg(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("codec", C.c_int32), ("qp", C.c_int32),
+                ("num_slices", C.c_int32), ("gop", C.c_int32), ("ring", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("session", C.c_int32), ("frame", C.c_int32), ("source", C.c_int32), ("pad", C.c_int32),
+                ("t_arrival", C.c_double)]
+
+
+class FrameResult(C.Structure):
+    _fields_ = [("session", C.c_int32), ("frame", C.c_int32), ("batch", C.c_int32), ("fallbacks", C.c_int32),
+                ("t_arrival", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+                ("checksum", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("batches", C.c_int32), ("frames", C.c_int32), ("max_batch_seen", C.c_int32),
+                ("sessions", C.c_int32), ("mean_batch", C.c_double), ("conv_grid_cap", C.c_int32),
+                ("autotune_us", C.c_double * 8), ("hbm_used", C.c_size_t)]
+
+
+_sig(hip, "bc250_server_create", _P, [C.POINTER(ServerCfg), _P])
+_sig(hip, "bc250_server_destroy", None, [_P])
+_sig(hip, "bc250_server_session_bytes", C.c_size_t, [_P, C.POINTER(SessionCfg)])
+_sig(hip, "bc250_server_open", _I, [_P, C.POINTER(SessionCfg)])
+_sig(hip, "bc250_server_close", _I, [_P, _I])
+_sig(hip, "bc250_server_upload", _I, [_P, _I, _I, _P, _I, _P, _I])
+_sig(hip, "bc250_server_run", _I, [_P, C.POINTER(Request), _I, _I, C.POINTER(FrameResult), C.POINTER(C.c_double)])
+_sig(hip, "bc250_server_autotune", _I, [_P])
+_sig(hip, "bc250_server_stats", _I, [_P, C.POINTER(ServerStats)])
+_sig(hip, "bc250_fnv1a64", C.c_uint64, [C.c_uint64, _P, C.c_size_t])
+
+
+class Server:
+    """Many sessions on the GPU (include/bc250_serve.h)."""
+
+    def __init__(self, learned=False, max_batch=32, hbm_budget=0, search_range=64, lambda8=1, params=None,
+                 proposer=None):
+        pc = ProposerCfg(**(proposer or PROPOSER_DEFAULT))
+        cfg = ServerCfg(hbm_budget, max_batch, int(learned), search_range, lambda8, pc)
+        self._params = None
+        if learned:
+            self._params = np.ascontiguousarray(load_proposer_weights() if params is None else params, np.float32)
+        self.h = hip.bc250_server_create(C.byref(cfg), _ptr(self._params) if learned else None)
+        if not self.h:
+            raise RuntimeError("bc250_server_create failed")
+
+    def session_bytes(self, **kw):
+        return hip.bc250_server_session_bytes(self.h, C.byref(SessionCfg(**kw)))
+
+    def open(self, width, height, codec=GPU_CODEC_HEVC, qp=27, num_slices=1, gop=0, ring=1):
+        """Session id, or -1 when its footprint does not fit the budget."""
+        return hip.bc250_server_open(self.h, C.byref(SessionCfg(width, height, codec, qp, num_slices, gop, ring)))
+
+    def close_session(self, sid):
+        if hip.bc250_server_close(self.h, sid) != 0:
+            raise RuntimeError("close failed")
+
+    def upload(self, sid, source, y, uv):
+        y, uv = np.ascontiguousarray(y), np.ascontiguousarray(uv)
+        if hip.bc250_server_upload(self.h, sid, source, _ptr(y), y.strides[0], _ptr(uv), uv.strides[0]) != 0:
+            raise RuntimeError("upload failed")
+
+    def run(self, requests, checksums=False):
+        """requests: (session, frame, source, t_arrival) sorted by arrival;
+        returns (results as a structured numpy array, wall seconds)."""
+        n = len(requests)
+        reqs = (Request * n)(*[Request(s, f, src, 0, t) for s, f, src, t in requests])
+        out = (FrameResult * n)()
+        wall = C.c_double()
+        if hip.bc250_server_run(self.h, reqs, n, int(checksums), out, C.byref(wall)) != 0:
+            raise RuntimeError("bc250_server_run failed")
+        dt = np.dtype([("session", "<i4"), ("frame", "<i4"), ("batch", "<i4"), ("fallbacks", "<i4"),
+                       ("t_arrival", "<f8"), ("t_start", "<f8"), ("t_done", "<f8"), ("checksum", "<u8")])
+        return np.frombuffer(bytes(out), dt).copy(), wall.value
+
+    def autotune(self):
+        if hip.bc250_server_autotune(self.h) != 0:
+            raise RuntimeError("autotune failed")
+
+    def stats(self):
+        st = ServerStats()
+        hip.bc250_server_stats(self.h, C.byref(st))
+        return {"batches": st.batches, "frames": st.frames, "max_batch": st.max_batch_seen, "sessions": st.sessions,
+                "mean_batch": st.mean_batch, "conv_grid_cap": st.conv_grid_cap,
+                "autotune_us": list(st.autotune_us)[:5], "hbm_used": st.hbm_used}
+
+    def close(self):
+        if self.h:
+            hip.bc250_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def staging_checksum(results, codec):
+    """The server's FNV-1a over a frame's staging, computed from Encoder
+    results (the same arrays in the same order)."""
+    h = 0xcbf29ce484222325
+    keys = ("quant", "mv8", "hevc_mode") if codec == GPU_CODEC_HEVC else ("quant", "mv8", "dc", "pred_mode", "mv")
+    for k in keys:
+        a = np.ascontiguousarray(results[k]).view(np.uint8)
+        h = hip.bc250_fnv1a64(h, _ptr(a), a.size)
+    return h
 _sig(hip, "gpu_compute_get_me_fallbacks", _I, [_CTX, _I])
 # the default search of the learned form: 4 candidates, fallback above 1.5x
 # the neighbours' median SAD + 256 or above 6000

@@ -132,6 +132,9 @@ struct bc250_hip_state {
     int stage_frames = 0, stage_p_frames = 0;
     double me_int_sum_p = 0;
     Proposer prop;
+    // integer search results written by someone else (a batched launch): P
+    // pictures then skip their own integer search (gpu_compute_set_external_int_me)
+    const uint32_t *ext16 = nullptr, *ext8 = nullptr;
     const int *fb_frame = nullptr;  // the frame being dispatched used the learned search: its fallback count
     // map_surface mirror
     void *map_host = nullptr;
@@ -843,8 +846,12 @@ static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &r
         if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, s->nmb * 16, s->stream);
         return r;
     }
-    int r;
-    if (s->prop.on) {
+    int r = 0;
+    const uint32_t *i16 = s->imv16, *i8 = s->imv8;
+    if (s->ext16) {
+        i16 = s->ext16;
+        i8 = s->ext8;
+    } else if (s->prop.on) {
         r = enqueue_learned(s, (const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->imv16, s->imv8, s->stream);
         s->fb_frame = s->prop.count;
     } else {
@@ -853,8 +860,8 @@ static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &r
     }
     hipEventRecord(mid, s->stream);
     if (r == 0)
-        r = bc250_launch_me_refine((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc, s->imv16,
-                                   s->imv8, s->mvs, s->mv8, p.me_8x8, s->stream);
+        r = bc250_launch_me_refine((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc, i16, i8,
+                                   s->mvs, s->mv8, p.me_8x8, s->stream);
     return r;
 }
 
@@ -882,7 +889,9 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
             r8 = m1;
             cp8 = rp8 = s->me8_pitch;
         }
-        if (r == 0 && s->prop.on) {
+        if (r == 0 && s->ext16) {
+            // the integer search ran elsewhere
+        } else if (r == 0 && s->prop.on) {
             r = enqueue_learned(s, c8, cp8, r8, rp8, s->imv16, s->imv8, st);
             s->fb_frame = s->prop.count;
         } else if (r == 0) {
@@ -908,7 +917,8 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
             const bool c32 = p.hevc_ctu_log2 == 5;
             r = bc250_hevc_launch_refine_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv,
                                                   out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp,
-                                                  2 /* inter rounding */, s->imv8, s->mv8, (int16_t *)s->levels,
+                                                  2 /* inter rounding */, s->ext8 ? s->ext8 : s->imv8, s->mv8,
+                                                  (int16_t *)s->levels,
                                                   p.hevc_ctu_log2, c32 ? s->hculog2 : nullptr, s->hmodes,
                                                   c32 ? nullptr : s->hcbf, bd, st);
         }
@@ -1312,6 +1322,39 @@ extern "C" int gpu_compute_set_me_proposer(gpu_context_t *ctx, const float *para
     q.cfg = *cfg;
     q.on = true;
     return 0;
+}
+
+extern "C" int gpu_compute_set_external_int_me(gpu_context_t *ctx, const uint32_t *imv16, const uint32_t *imv8)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || (imv16 == nullptr) != (imv8 == nullptr)) return -1;
+    s->ext16 = imv16;
+    s->ext8 = imv8;
+    return 0;
+}
+
+extern "C" int gpu_compute_get_reference_luma(gpu_context_t *ctx, const uint8_t **y, int *pitch)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !y || !pitch || !s->has_recon) return -1;
+    *y = s->bps == 2 ? nullptr : s->recon[s->ref_idx].y;
+    *pitch = s->recon[s->ref_idx].py;
+    return *y ? 0 : -1;
+}
+
+extern "C" int gpu_compute_stream_wait_event(gpu_context_t *ctx, void *event)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || !event) return -1;
+    return HIP_OK(hipStreamWaitEvent(s->stream, (hipEvent_t)event, 0)) ? 0 : -1;
+}
+
+extern "C" int gpu_compute_slot_done(gpu_context_t *ctx, int slot)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s || slot < 0 || slot >= NSLOT || !s->slot[slot].recorded) return -1;
+    const hipError_t e = hipEventQuery(s->slot[slot].ev[NEV - 1]);
+    return e == hipSuccess ? 1 : (e == hipErrorNotReady ? 0 : -1);
 }
 
 extern "C" int gpu_compute_get_me_fallbacks(gpu_context_t *ctx, int slot)

@@ -276,6 +276,23 @@ __global__ void cnn_input_kernel(const uint8_t *__restrict__ cur, int cpitch, co
     }
 }
 
+// Input tiles of a batch of pictures: blockIdx = batch-wide tile.
+__global__ void cnn_input_batch_kernel(const bc250_me_batch_t b, uint16_t *__restrict__ in)
+{
+    const int i = blockIdx.x, t = threadIdx.x;
+    if (i >= b.ntile_total) return;
+    const int f = batch_frame_of_tile(b, i);
+    const bc250_batch_frame_t &fr = b.f[f];
+    const int tw = (fr.wc / 16 + 3) / 4, lt = i - fr.tile_base, tx = lt % tw, ty = lt / tw;
+    for (int p = t; p < T * T; p += 256) {
+        const int yy = p / T, xx = p % T;
+        const int gy = min(ty * T + yy, fr.hc - 1), gx = min(tx * T + xx, fr.wc - 1);
+        const float a = ((int)fr.cur[(size_t)gy * fr.cpitch + gx] - 128) * (1.f / 64.f);
+        const float c = ((int)fr.ref[(size_t)gy * fr.rpitch + gx] - 128) * (1.f / 64.f);
+        *(uint32_t *)(in + (((size_t)i * TP + yy + 1) * TP + xx + 1) * 2) = (uint32_t)f2bf(a) | ((uint32_t)f2bf(c) << 16);
+    }
+}
+
 // Head of one 16x16 block: z = Wh [pool/256 ; mv] + bh (12 outputs).
 __device__ __forceinline__ void head_block(const float *__restrict__ wh, const float *__restrict__ pool,
                                            const float *__restrict__ mv, float *z)
@@ -455,12 +472,20 @@ extern "C" int bc250_cnn_input(const uint8_t *cur, int cpitch, const uint8_t *re
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Workgroups per convolution launch: enough to fill the chip, never more
+// than there are rows; the cap is a tunable (bc250_cnn_set_grid_cap).
+static int g_conv_grid_cap = 4096;
+extern "C" int bc250_cnn_set_grid_cap(int cap)
+{
+    if (cap < 64 || cap > 65536) return -1;
+    g_conv_grid_cap = cap;
+    return 0;
+}
 static int conv_grid(int ntiles)
 {
-    // enough waves to fill the chip, never more than there are rows
     const int rows = ntiles * T;
     const int g = (rows + 3) / 4;
-    return g < 4096 ? g : 4096;
+    return g < g_conv_grid_cap ? g : g_conv_grid_cap;
 }
 
 extern "C" int bc250_cnn_conv(int layer, int mode, const uint16_t *in, const uint16_t *w, const float *bias,
@@ -574,5 +599,12 @@ extern "C" int bc250_cnn_pack(const float *params, uint16_t *packed, void *strea
                                                               packed + bc250_cnn_packed_offset(l, 0),
                                                               l > 0 ? packed + bc250_cnn_packed_offset(l, 1) : nullptr);
     }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_cnn_input_batch(const bc250_me_batch_t *b, uint16_t *in, void *stream)
+{
+    if (!b || !in || b->n < 1 || b->n > BC250_BATCH_MAX || b->ntile_total <= 0) return -1;
+    cnn_input_batch_kernel<<<b->ntile_total, 256, 0, (hipStream_t)stream>>>(*b, in);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

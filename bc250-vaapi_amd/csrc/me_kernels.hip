@@ -19,6 +19,7 @@
 // against 0.425 ms for 32-bit accumulators and 0.544 ms for 4x8 tiles on
 // 3-wave workgroups, which leave SIMDs unevenly loaded.
 #include "bc250_dev.h"
+#include "bc250_learned.h"
 #include "bc250_stages.h"
 
 namespace {
@@ -69,30 +70,15 @@ BC_DEV uint32_t mad_hi16(uint32_t x, uint32_t li)
 }
 }  // namespace
 
-// out16[m] = {dx, dy, cost, sad}; out8[4m+q] likewise for quadrant q.  With
-// `list`, only the MBs list[0 .. *count) are searched (count is read on the
-// device, so the grid is sized for the largest list).
+// The full search of one MB (mb of a frame mbw MBs wide) by one workgroup:
+// out16[mb] = {dx, dy, cost, sad}; out8[4 mb + q] likewise for quadrant q.
 template <int R>
-__global__ __launch_bounds__(ME_THREADS, BC250_ME_WAVES_PER_SIMD) void me_fullsearch_kernel(
-    const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref, int rpitch,
-    int wc, int hc, int mbw, int nmb, int lambda8, uint4 *__restrict__ out16,
-    uint4 *__restrict__ out8, const int *__restrict__ list, const int *__restrict__ count)
+__device__ __forceinline__ void fullsearch_mb(const uint8_t *__restrict__ cur, int cpitch,
+                                              const uint8_t *__restrict__ ref, int rpitch, int wc, int hc, int mbw,
+                                              int mb, int lambda8, uint4 *__restrict__ out16,
+                                              uint4 *__restrict__ out8, uint32_t *win, uint32_t (*red)[5])
 {
     using Gm = FsGeom<R>;
-    __shared__ uint32_t win[Gm::NR * Gm::PITCH];
-    __shared__ uint32_t red[ME_THREADS / 64][5];
-
-    int mb;
-    if (list) {
-        const int n = min(*count, nmb), per = (n + 7) >> 3;
-        if ((int)(blockIdx.x >> 3) >= per) return;
-        const int i = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-        if (i >= n) return;
-        mb = list[i];
-    } else {
-        mb = xcd_item(blockIdx.x, nmb);
-    }
-    if (mb >= nmb) return;
     const int mbx = mb % mbw, mby = mb / mbw;
     const int x0 = mbx * 16, y0 = mby * 16;
     const int tid = threadIdx.x;
@@ -234,6 +220,57 @@ __global__ __launch_bounds__(ME_THREADS, BC250_ME_WAVES_PER_SIMD) void me_fullse
         if (tid == 0) out16[mb] = o;
         else out8[4 * mb + tid - 1] = o;
     }
+}
+
+// With `list`, only the MBs list[0 .. *count) are searched (count is read on
+// the device, so the grid is sized for the largest list).
+template <int R>
+__global__ __launch_bounds__(ME_THREADS, BC250_ME_WAVES_PER_SIMD) void me_fullsearch_kernel(
+    const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref, int rpitch,
+    int wc, int hc, int mbw, int nmb, int lambda8, uint4 *__restrict__ out16,
+    uint4 *__restrict__ out8, const int *__restrict__ list, const int *__restrict__ count)
+{
+    using Gm = FsGeom<R>;
+    __shared__ uint32_t win[Gm::NR * Gm::PITCH];
+    __shared__ uint32_t red[ME_THREADS / 64][5];
+    int mb;
+    if (list) {
+        const int n = min(*count, nmb), per = (n + 7) >> 3;
+        if ((int)(blockIdx.x >> 3) >= per) return;
+        const int i = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+        if (i >= n) return;
+        mb = list[i];
+    } else {
+        mb = xcd_item(blockIdx.x, nmb);
+    }
+    if (mb >= nmb) return;
+    fullsearch_mb<R>(cur, cpitch, ref, rpitch, wc, hc, mbw, mb, lambda8, out16, out8, win, red);
+}
+
+// The same search over the MBs of a batch of pictures (one workgroup per MB
+// of every picture, or per listed MB: list holds batch-wide MB indices).
+template <int R>
+__global__ __launch_bounds__(ME_THREADS, BC250_ME_WAVES_PER_SIMD) void me_fullsearch_batch_kernel(
+    const bc250_me_batch_t b, const int *__restrict__ list, const int *__restrict__ count)
+{
+    using Gm = FsGeom<R>;
+    __shared__ uint32_t win[Gm::NR * Gm::PITCH];
+    __shared__ uint32_t red[ME_THREADS / 64][5];
+    int g;
+    if (list) {
+        const int n = min(*count, b.nmb_total), per = (n + 7) >> 3;
+        if ((int)(blockIdx.x >> 3) >= per) return;
+        const int i = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+        if (i >= n) return;
+        g = list[i];
+    } else {
+        g = xcd_item(blockIdx.x, b.nmb_total);
+    }
+    if (g >= b.nmb_total) return;
+    const int f = batch_frame_of_mb(b, g);
+    const bc250_batch_frame_t &fr = b.f[f];
+    fullsearch_mb<R>(fr.cur, fr.cpitch, fr.ref, fr.rpitch, fr.wc, fr.hc, fr.wc / 16, g - fr.mb_base, b.lambda8,
+                     (uint4 *)fr.out16, (uint4 *)fr.out8, win, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -412,6 +449,23 @@ extern "C" int bc250_launch_me_fullsearch(const uint8_t *cur, int cpitch, const 
 {
     return bc250_launch_me_fullsearch_list(cur, cpitch, ref, rpitch, wc, hc, range, lambda8, out16, out8, nullptr,
                                            nullptr, stream);
+}
+
+extern "C" int bc250_launch_me_fullsearch_batch(const bc250_me_batch_t *b, const int *list, const int *count,
+                                                void *stream)
+{
+    if (!b || b->n < 1 || b->n > BC250_BATCH_MAX || b->nmb_total <= 0 || b->lambda8 < 0 || b->lambda8 > 15 ||
+        (list == nullptr) != (count == nullptr))
+        return -1;
+    const int grid = ((b->nmb_total + 7) / 8) * 8;
+    hipStream_t st = (hipStream_t)stream;
+    switch (b->range) {
+    case 16: me_fullsearch_batch_kernel<16><<<grid, ME_THREADS, 0, st>>>(*b, list, count); break;
+    case 32: me_fullsearch_batch_kernel<32><<<grid, ME_THREADS, 0, st>>>(*b, list, count); break;
+    case 64: me_fullsearch_batch_kernel<64><<<grid, ME_THREADS, 0, st>>>(*b, list, count); break;
+    default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 extern "C" int bc250_launch_me_refine(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,

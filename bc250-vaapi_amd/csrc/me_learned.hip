@@ -27,14 +27,10 @@ constexpr int CW = 4;  // window radius of the candidate search (pels)
 constexpr int WS = 16 + 2 * CW;       // window side (24)
 constexpr int WP = WS / 4 + 1;        // window pitch in dwords (7, odd)
 
-__global__ void cnn_mvin_kernel(const uint4 *__restrict__ prev16, int mbw, int mbh, int tw, int ntiles,
-                                float *__restrict__ mvin)
+__device__ __forceinline__ void mvin_block(const uint4 *__restrict__ prev16, int mbw, int mbh, int tw, int tile,
+                                           int j, float *__restrict__ o)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;  // (tile, block)
-    if (b >= ntiles * 16) return;
-    const int tile = b / 16, j = b % 16;
     const int mx = (tile % tw) * 4 + (j & 3), my = (tile / tw) * 4 + (j >> 2);
-    float *o = mvin + (size_t)b * 32;
     for (int k = 0; k < 16; k++) {
         const int nx = clampi(mx + (k & 3) - 1, 0, mbw - 1), ny = clampi(my + (k >> 2) - 1, 0, mbh - 1);
         const uint4 v = prev16 ? prev16[ny * mbw + nx] : make_uint4(0, 0, 0, 0);
@@ -43,11 +39,18 @@ __global__ void cnn_mvin_kernel(const uint4 *__restrict__ prev16, int mbw, int m
     }
 }
 
-__global__ void cnn_candidates_kernel(const float *__restrict__ z, int mbw, int mbh, int tw, int range,
-                                      int2 *__restrict__ cand)
+__global__ void cnn_mvin_kernel(const uint4 *__restrict__ prev16, int mbw, int mbh, int tw, int ntiles,
+                                float *__restrict__ mvin)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= mbw * mbh) return;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;  // (tile, block)
+    if (b >= ntiles * 16) return;
+    mvin_block(prev16, mbw, mbh, tw, b / 16, b % 16, mvin + (size_t)b * 32);
+}
+
+// Candidates of MB m of a picture whose tiles start at z (tw tiles per row).
+__device__ __forceinline__ void candidates_mb(const float *__restrict__ z, int mbw, int tw, int range, int m,
+                                              int2 *__restrict__ cand)
+{
     const int mx = m % mbw, my = m / mbw;
     const int tile = (my >> 2) * tw + (mx >> 2), j = (my & 3) * 4 + (mx & 3);
     const float *zb = z + ((size_t)tile * 16 + j) * BC250_CNN_NOUT;
@@ -64,18 +67,23 @@ __global__ void cnn_candidates_kernel(const float *__restrict__ z, int mbw, int 
     }
 }
 
+__global__ void cnn_candidates_kernel(const float *__restrict__ z, int mbw, int mbh, int tw, int range,
+                                      int2 *__restrict__ cand)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= mbw * mbh) return;
+    candidates_mb(z, mbw, tw, range, m, cand);
+}
+
 // One wave per MB.  The current block is wave-uniform; each candidate's
 // window (edge-replicated like the full search's) is staged in LDS and a
 // lane evaluates positions lane and lane + 64 of the (2W+1)^2 around it.
-__global__ __launch_bounds__(64) void cand_search_kernel(const uint8_t *__restrict__ cur, int cpitch,
-                                                         const uint8_t *__restrict__ ref, int rpitch, int wc, int hc,
-                                                         int mbw, int nmb, int range, int lambda8,
-                                                         const int2 *__restrict__ cand, int ncand,
-                                                         uint4 *__restrict__ out16, uint4 *__restrict__ out8)
+__device__ __forceinline__ void cand_search_mb(const uint8_t *__restrict__ cur, int cpitch,
+                                               const uint8_t *__restrict__ ref, int rpitch, int wc, int hc, int mbw,
+                                               int mb, int range, int lambda8, const int2 *__restrict__ cand,
+                                               int ncand, uint4 *__restrict__ out16, uint4 *__restrict__ out8,
+                                               uint32_t *win)
 {
-    __shared__ uint32_t win[WS * WP];
-    const int mb = xcd_item(blockIdx.x, nmb);
-    if (mb >= nmb) return;
     const int lane = threadIdx.x;
     const int x0 = (mb % mbw) * 16, y0 = (mb / mbw) * 16;
     const int nc = 2 * range + 1;
@@ -150,16 +158,26 @@ __global__ __launch_bounds__(64) void cand_search_kernel(const uint8_t *__restri
     }
 }
 
+__global__ __launch_bounds__(64) void cand_search_kernel(const uint8_t *__restrict__ cur, int cpitch,
+                                                         const uint8_t *__restrict__ ref, int rpitch, int wc, int hc,
+                                                         int mbw, int nmb, int range, int lambda8,
+                                                         const int2 *__restrict__ cand, int ncand,
+                                                         uint4 *__restrict__ out16, uint4 *__restrict__ out8)
+{
+    __shared__ uint32_t win[WS * WP];
+    const int mb = xcd_item(blockIdx.x, nmb);
+    if (mb >= nmb) return;
+    cand_search_mb(cur, cpitch, ref, rpitch, wc, hc, mbw, mb, range, lambda8, cand, ncand, out16, out8, win);
+}
+
 __global__ void zero1_kernel(int *p) { *p = 0; }
 
 // The fallback rule: an MB goes to the full search when its 16x16 SAD
 // exceeds (alpha_q4 / 16) * m + beta, m the median (lower middle) of its
 // left, right, top and bottom neighbours' SADs, or exceeds t_high.
-__global__ void fallback_select_kernel(const uint4 *__restrict__ r16, int mbw, int mbh, int alpha_q4, int beta,
-                                       int t_high, int *__restrict__ list, int *__restrict__ count)
+__device__ __forceinline__ bool fallback_mb(const uint4 *__restrict__ r16, int mbw, int mbh, int alpha_q4, int beta,
+                                            int t_high, int m)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= mbw * mbh) return;
     const int mx = m % mbw, my = m / mbw;
     uint32_t nb[4];
     int n = 0;
@@ -173,7 +191,60 @@ __global__ void fallback_select_kernel(const uint4 *__restrict__ r16, int mbw, i
     const uint32_t med = n ? nb[(n - 1) / 2] : 0u;
     const uint32_t sad = r16[m].w;
     const uint64_t bound = ((uint64_t)alpha_q4 * med >> 4) + (uint64_t)beta;
-    if (sad > bound || sad > (uint32_t)t_high) list[atomicAdd(count, 1)] = m;
+    return sad > bound || sad > (uint32_t)t_high;
+}
+
+__global__ void fallback_select_kernel(const uint4 *__restrict__ r16, int mbw, int mbh, int alpha_q4, int beta,
+                                       int t_high, int *__restrict__ list, int *__restrict__ count)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= mbw * mbh) return;
+    if (fallback_mb(r16, mbw, mbh, alpha_q4, beta, t_high, m)) list[atomicAdd(count, 1)] = m;
+}
+
+// ---- batch forms: the picture of each batch-wide tile / MB from the descriptor
+
+__global__ void cnn_mvin_batch_kernel(const bc250_me_batch_t b, float *__restrict__ mvin)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (batch tile, block)
+    if (i >= b.ntile_total * 16) return;
+    const int t = i / 16, f = batch_frame_of_tile(b, t);
+    const bc250_batch_frame_t &fr = b.f[f];
+    const int mbw = fr.wc / 16, mbh = fr.hc / 16;
+    mvin_block((const uint4 *)fr.prev16, mbw, mbh, (mbw + 3) / 4, t - fr.tile_base, i % 16, mvin + (size_t)i * 32);
+}
+
+__global__ void cnn_candidates_batch_kernel(const bc250_me_batch_t b, const float *__restrict__ z)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.nmb_total) return;
+    const int f = batch_frame_of_mb(b, g);
+    const bc250_batch_frame_t &fr = b.f[f];
+    const int mbw = fr.wc / 16;
+    candidates_mb(z + (size_t)fr.tile_base * 16 * BC250_CNN_NOUT, mbw, (mbw + 3) / 4, b.range, g - fr.mb_base,
+                  (int2 *)fr.cand);
+}
+
+__global__ __launch_bounds__(64) void cand_search_batch_kernel(const bc250_me_batch_t b, int ncand)
+{
+    __shared__ uint32_t win[WS * WP];
+    const int g = xcd_item(blockIdx.x, b.nmb_total);
+    if (g >= b.nmb_total) return;
+    const int f = batch_frame_of_mb(b, g);
+    const bc250_batch_frame_t &fr = b.f[f];
+    cand_search_mb(fr.cur, fr.cpitch, fr.ref, fr.rpitch, fr.wc, fr.hc, fr.wc / 16, g - fr.mb_base, b.range, b.lambda8,
+                   (const int2 *)fr.cand, ncand, (uint4 *)fr.out16, (uint4 *)fr.out8, win);
+}
+
+__global__ void fallback_select_batch_kernel(const bc250_me_batch_t b, int alpha_q4, int beta, int t_high,
+                                             int *__restrict__ list, int *__restrict__ count)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.nmb_total) return;
+    const int f = batch_frame_of_mb(b, g);
+    const bc250_batch_frame_t &fr = b.f[f];
+    if (fallback_mb((const uint4 *)fr.out16, fr.wc / 16, fr.hc / 16, alpha_q4, beta, t_high, g - fr.mb_base))
+        list[atomicAdd(count, 1)] = g;
 }
 
 }  // namespace
@@ -220,4 +291,87 @@ extern "C" int bc250_fallback_select(const uint32_t *r16, int mbw, int mbh, int 
     fallback_select_kernel<<<(nmb + 127) / 128, 128, 0, st>>>((const uint4 *)r16, mbw, mbh, alpha_q4, beta, t_high,
                                                               list, count);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------
+// The batched learned search.  Work layout (each part 256-byte aligned): input
+// tiles, conv1..3 activations, conv4 row sums, block sums, neighbour inputs,
+// head outputs, the fallback list, its count.
+namespace {
+struct LearnedWork {
+    uint16_t *a0, *a1, *a2, *a3;
+    float *rows, *pool, *mvin, *z;
+    int *list, *count;
+    size_t bytes;
+};
+size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+LearnedWork carve_learned(void *base, int nt, int nmb)
+{
+    uint8_t *p = (uint8_t *)base;
+    LearnedWork w;
+    size_t o = 0;
+    auto take = [&](size_t n) { uint8_t *q = p ? p + o : nullptr; o += al256(n); return q; };
+    w.a0 = (uint16_t *)take((size_t)nt * bc250_cnn_tile_bytes(2));
+    w.a1 = (uint16_t *)take((size_t)nt * bc250_cnn_tile_bytes(16));
+    w.a2 = (uint16_t *)take((size_t)nt * bc250_cnn_tile_bytes(32));
+    w.a3 = (uint16_t *)take((size_t)nt * bc250_cnn_tile_bytes(32));
+    w.rows = (float *)take((size_t)nt * 64 * 4 * 32 * 4);
+    w.pool = (float *)take((size_t)nt * 16 * 32 * 4);
+    w.mvin = (float *)take((size_t)nt * 16 * 32 * 4);
+    w.z = (float *)take((size_t)nt * 16 * BC250_CNN_NOUT * 4);
+    w.list = (int *)take((size_t)nmb * 4);
+    w.count = (int *)take(4);
+    w.bytes = o;
+    return w;
+}
+}  // namespace
+
+extern "C" size_t bc250_learned_work_bytes(int ntiles, int nmb)
+{
+    if (ntiles <= 0 || nmb <= 0) return 0;
+    return carve_learned(nullptr, ntiles, nmb).bytes;
+}
+
+extern "C" const int *bc250_learned_fallback_count(const void *work, int ntiles, int nmb)
+{
+    if (!work || ntiles <= 0 || nmb <= 0) return nullptr;
+    return carve_learned((void *)work, ntiles, nmb).count;
+}
+
+extern "C" int bc250_learned_search_batch(const bc250_me_batch_t *b, const float *params, const uint16_t *packed,
+                                          int ncand, int alpha_q4, int beta, int t_high, void *work, int work_tiles,
+                                          int work_mbs, void *stream)
+{
+    if (!b || b->n < 1 || b->n > BC250_BATCH_MAX || b->nmb_total <= 0 || b->ntile_total <= 0 || !params ||
+        !packed || !work || ncand < 1 || ncand > 4 || b->range < CW || b->range > 64 ||
+        b->ntile_total > work_tiles || b->nmb_total > work_mbs)
+        return -1;
+    // the layout is the work buffer's (its capacity), whatever the batch's size,
+    // so every layer's tile borders stay where they were zeroed
+    const LearnedWork w = carve_learned(work, work_tiles, work_mbs);
+    hipStream_t st = (hipStream_t)stream;
+    const int nt = b->ntile_total;
+    auto wf = [&](int l) { return packed + bc250_cnn_packed_offset(l - 1, 0); };
+    auto bias = [&](int l, int cin) { return params + bc250_cnn_w_offset(l - 1) + (9 * cin + 31) / 32 * 32; };
+    int r = bc250_cnn_input_batch(b, w.a0, st);
+    if (r == 0) r = bc250_cnn_conv(1, 0, w.a0, wf(1), bias(1, 2), w.a1, nullptr, nullptr, nt, st);
+    if (r == 0) r = bc250_cnn_conv(2, 0, w.a1, wf(2), bias(2, 16), w.a2, nullptr, nullptr, nt, st);
+    if (r == 0) r = bc250_cnn_conv(3, 0, w.a2, wf(3), bias(3, 32), w.a3, nullptr, nullptr, nt, st);
+    if (r == 0) r = bc250_cnn_conv(4, 2, w.a3, wf(4), bias(4, 32), nullptr, nullptr, w.rows, nt, st);
+    if (r == 0) r = bc250_cnn_pool(w.rows, w.pool, nt, st);
+    if (r == 0) {
+        cnn_mvin_batch_kernel<<<(nt * 16 + 127) / 128, 128, 0, st>>>(*b, w.mvin);
+        r = hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+    if (r == 0) r = bc250_cnn_head(params + bc250_cnn_w_offset(4), w.pool, w.mvin, w.z, nt, st);
+    if (r == 0) {
+        cnn_candidates_batch_kernel<<<(b->nmb_total + 127) / 128, 128, 0, st>>>(*b, w.z);
+        cand_search_batch_kernel<<<((b->nmb_total + 7) / 8) * 8, 64, 0, st>>>(*b, ncand);
+        zero1_kernel<<<1, 1, 0, st>>>(w.count);
+        fallback_select_batch_kernel<<<(b->nmb_total + 127) / 128, 128, 0, st>>>(*b, alpha_q4, beta, t_high, w.list,
+                                                                                 w.count);
+        r = hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+    if (r == 0) r = bc250_launch_me_fullsearch_batch(b, w.list, w.count, st);
+    return r;
 }

@@ -92,6 +92,10 @@ int bc250_cnn_pool_backward(const float *dpool, const uint16_t *a4, uint16_t *dz
 /* Adam step t (>= 1), beta 0.9 / 0.999, eps 1e-8, on n parameters. */
 int bc250_cnn_adam(float *p, const float *g, float *m, float *v, int n, float lr, int t, void *stream);
 
+/* Workgroups per convolution launch at most (a launch shape the serving
+ * autotuner chooses; default 4096, 64..65536). */
+int bc250_cnn_set_grid_cap(int cap);
+
 /* fp32 parameters -> the bf16 packed weights (BC250_CNN_NPACKED). */
 int bc250_cnn_pack(const float *params, uint16_t *packed, void *stream);
 
@@ -127,6 +131,71 @@ int bc250_launch_cand_search(const uint8_t *cur, int cpitch, const uint8_t *ref,
 int bc250_fallback_select(const uint32_t *r16, int mbw, int mbh, int alpha_q4, int beta, int t_high, int *list,
                           int *count, void *stream);
 
+/* ---- batches of pictures ---------------------------------------------------
+ * One launch per stage for the integer searches of several pictures (of
+ * different sessions and sizes): the batch travels by value as a kernel
+ * argument.  Batch-wide MB index g belongs to the picture f with
+ * f[k].mb_base <= g < f[k+1].mb_base (local MB g - mb_base); tiles likewise
+ * with tile_base (local tiles ceil(mbw/4) per row, as above). */
+#define BC250_BATCH_MAX 32
+typedef struct {
+    const uint8_t *cur, *ref;     /* 8-bit luma of the picture and of its reference (coded size) */
+    int32_t cpitch, rpitch, wc, hc;
+    uint32_t *out16, *out8;       /* its integer search records [nmb], [4 nmb] */
+    const uint32_t *prev16;       /* the previous picture's 16x16 records (NULL: none) */
+    int32_t *cand;                /* [nmb][4] {dx, dy} candidates */
+    int32_t mb_base, tile_base;
+} bc250_batch_frame_t;
+typedef struct {
+    int32_t n, range, lambda8, nmb_total, ntile_total, pad;
+    bc250_batch_frame_t f[BC250_BATCH_MAX];
+} bc250_me_batch_t;
+
+#if defined(__HIPCC__) || defined(__HIP__)
+/* The picture of batch-wide index g (MB or tile), by binary search. */
+static __device__ __forceinline__ int batch_frame_of_mb(const bc250_me_batch_t &b, int g)
+{
+    int lo = 0, hi = b.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.f[mid].mb_base <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+static __device__ __forceinline__ int batch_frame_of_tile(const bc250_me_batch_t &b, int t)
+{
+    int lo = 0, hi = b.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.f[mid].tile_base <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+#endif
+
+/* The plain full search of every MB of the batch (list / count: only the
+ * listed batch-wide MBs, count read on the device). */
+int bc250_launch_me_fullsearch_batch(const bc250_me_batch_t *b, const int *list, const int *count, void *stream);
+
+/* The learned search of every picture of the batch, one launch per stage:
+ * input tiles, the four layers, block sums, neighbour inputs, head,
+ * candidates, candidate search, fallback list (batch-wide MB indices) and
+ * the full search of the listed MBs.  work: bc250_learned_work_bytes(
+ * work_tiles, work_mbs) bytes, zeroed once before its first use and laid out
+ * by those capacities (any batch up to them reuses it); params / packed as
+ * for the single-picture path. */
+size_t bc250_learned_work_bytes(int ntiles, int nmb);
+int bc250_learned_search_batch(const bc250_me_batch_t *b, const float *params, const uint16_t *packed, int ncand,
+                               int alpha_q4, int beta, int t_high, void *work, int work_tiles, int work_mbs,
+                               void *stream);
+/* The device int holding the fallback count of the last
+ * bc250_learned_search_batch on `work` (of these capacities). */
+const int *bc250_learned_fallback_count(const void *work, int work_tiles, int work_mbs);
+/* Input tiles of every picture of the batch (batch-wide tile order). */
+int bc250_cnn_input_batch(const bc250_me_batch_t *b, uint16_t *in, void *stream);
+
 /* ---- in the pipeline ------------------------------------------------------
  * With a proposer set, the integer search of every P picture (H.264 and
  * HEVC; +-search_range as set by gpu_compute_set_encode_params) is the
@@ -152,6 +221,20 @@ int gpu_compute_set_me_proposer(struct bc250_gpu_context *ctx, const float *para
 /* MBs the frame in `slot` searched in full after the learned search
  * (after gpu_compute_sync_slot), -1 when the frame did not use it. */
 int gpu_compute_get_me_fallbacks(struct bc250_gpu_context *ctx, int slot);
+
+/* ---- for a batching caller (bc250_serve.h) ---------------------------------
+ * imv16 / imv8 (device, [nmb] / [4 nmb] records) non-NULL: P pictures skip
+ * their own integer search and refine these instead (the caller wrote them,
+ * e.g. with a batched search, and orders that before the dispatch with
+ * gpu_compute_stream_wait_event); NULL restores the context's own search. */
+int gpu_compute_set_external_int_me(struct bc250_gpu_context *ctx, const uint32_t *imv16, const uint32_t *imv8);
+/* The 8-bit luma the next P picture refers to (device) and its pitch. */
+int gpu_compute_get_reference_luma(struct bc250_gpu_context *ctx, const uint8_t **y, int *pitch);
+/* The context's compute stream waits for a hipEvent_t (device-side wait). */
+int gpu_compute_stream_wait_event(struct bc250_gpu_context *ctx, void *event);
+/* 1 when the frame submitted in `slot` has finished (stages and copies), 0
+ * while it runs, -1 for an unused slot; never blocks. */
+int gpu_compute_slot_done(struct bc250_gpu_context *ctx, int slot);
 
 #ifdef __cplusplus
 }
