@@ -130,6 +130,8 @@ _sig(hip, "bc250_cnn_param_count", C.c_size_t, [])
 _sig(hip, "bc250_cnn_input", _I, [_P, _I, _P, _I, C.c_size_t, _I, _I, _P, _I, _P, _P])
 _sig(hip, "bc250_cnn_conv", _I, [_I, _I, _P, _P, _P, _P, _P, _P, _I, _P])
 _sig(hip, "bc250_cnn_pool", _I, [_P, _P, _I, _P])
+_sig(hip, "bc250_cnn_forward", _I, [_P, _P, _P, _P, _I, _P])
+_sig(hip, "bc250_cnn_set_forward_variant", _I, [_I])
 _sig(hip, "bc250_cnn_wgrad", _I, [_I, _P, _P, _P, _P, _I, _P])
 _sig(hip, "bc250_cnn_head", _I, [_P, _P, _P, _P, _I, _P])
 _sig(hip, "bc250_cnn_head_train", _I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, _F, _P])
@@ -176,7 +178,7 @@ class FrameResult(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("batches", C.c_int32), ("frames", C.c_int32), ("max_batch_seen", C.c_int32),
-                ("sessions", C.c_int32), ("mean_batch", C.c_double), ("conv_grid_cap", C.c_int32),
+                ("sessions", C.c_int32), ("mean_batch", C.c_double), ("forward_variant", C.c_int32),
                 ("autotune_us", C.c_double * 8), ("hbm_used", C.c_size_t)]
 
 
@@ -243,8 +245,8 @@ class Server:
         st = ServerStats()
         hip.bc250_server_stats(self.h, C.byref(st))
         return {"batches": st.batches, "frames": st.frames, "max_batch": st.max_batch_seen, "sessions": st.sessions,
-                "mean_batch": st.mean_batch, "conv_grid_cap": st.conv_grid_cap,
-                "autotune_us": list(st.autotune_us)[:5], "hbm_used": st.hbm_used}
+                "mean_batch": st.mean_batch, "forward_variant": st.forward_variant,
+                "autotune_us": list(st.autotune_us)[:3], "hbm_used": st.hbm_used}
 
     def close(self):
         if self.h:

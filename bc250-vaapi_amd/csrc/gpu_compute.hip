@@ -78,8 +78,8 @@ struct Proposer {
     uint16_t *packed = nullptr;  // bf16 [BC250_CNN_NPACKED]
     int ntiles = 0, tw = 0;
     int *tiles = nullptr;        // {0, tx, ty} per tile
-    uint16_t *a0 = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr;
-    float *rows = nullptr, *pool = nullptr, *mvin = nullptr, *z = nullptr;
+    uint16_t *a0 = nullptr;      // input tiles [tile][66][66][2]
+    float *pool = nullptr, *mvin = nullptr, *z = nullptr;
     int *cand = nullptr, *list = nullptr, *count = nullptr;
     uint32_t *prev16 = nullptr;  // the previous picture's integer 16x16 vectors
     bool has_prev = false;       // prev16 holds the picture before the current one
@@ -235,13 +235,12 @@ static void free_res(bc250_hip_state *s)
         if (p) hipFree(p);
     {
         Proposer &q = s->prop;
-        for (void *p : {(void *)q.tiles, (void *)q.a0, (void *)q.a1, (void *)q.a2, (void *)q.a3, (void *)q.rows,
-                        (void *)q.pool, (void *)q.mvin, (void *)q.z, (void *)q.cand, (void *)q.list, (void *)q.count,
-                        (void *)q.prev16})
+        for (void *p : {(void *)q.tiles, (void *)q.a0, (void *)q.pool, (void *)q.mvin, (void *)q.z, (void *)q.cand,
+                        (void *)q.list, (void *)q.count, (void *)q.prev16})
             if (p) hipFree(p);
         q.tiles = q.cand = q.list = q.count = nullptr;
-        q.a0 = q.a1 = q.a2 = q.a3 = nullptr;
-        q.rows = q.pool = q.mvin = q.z = nullptr;
+        q.a0 = nullptr;
+        q.pool = q.mvin = q.z = nullptr;
         q.prev16 = nullptr;
         q.ntiles = q.tw = 0;
         q.has_prev = false;
@@ -394,18 +393,14 @@ static int ensure_prop(bc250_hip_state *s)
     Proposer &q = s->prop;
     if (q.a0) return 0;
     const int tw = (s->mbw + 3) / 4, th = (s->mbh + 3) / 4, nt = tw * th;
-    const size_t t2 = bc250_cnn_tile_bytes(2), t16 = bc250_cnn_tile_bytes(16), t32 = bc250_cnn_tile_bytes(32);
+    const size_t t2 = bc250_cnn_tile_bytes(2);
     bool ok = HIP_OK(hipMalloc(&q.tiles, (size_t)nt * 12)) && HIP_OK(hipMalloc(&q.a0, nt * t2)) &&
-              HIP_OK(hipMalloc(&q.a1, nt * t16)) && HIP_OK(hipMalloc(&q.a2, nt * t32)) &&
-              HIP_OK(hipMalloc(&q.a3, nt * t32)) && HIP_OK(hipMalloc(&q.rows, (size_t)nt * 64 * 4 * 32 * 4)) &&
               HIP_OK(hipMalloc(&q.pool, (size_t)nt * 16 * 32 * 4)) &&
               HIP_OK(hipMalloc(&q.mvin, (size_t)nt * 16 * 32 * 4)) &&
               HIP_OK(hipMalloc(&q.z, (size_t)nt * 16 * BC250_CNN_NOUT * 4)) &&
               HIP_OK(hipMalloc(&q.cand, (size_t)s->nmb * 4 * 8)) && HIP_OK(hipMalloc(&q.list, (size_t)s->nmb * 4)) &&
               HIP_OK(hipMalloc(&q.count, 4)) && HIP_OK(hipMalloc(&q.prev16, (size_t)s->nmb * 16));
-    ok = ok && HIP_OK(hipMemset(q.a0, 0, nt * t2)) && HIP_OK(hipMemset(q.a1, 0, nt * t16)) &&
-         HIP_OK(hipMemset(q.a2, 0, nt * t32)) && HIP_OK(hipMemset(q.a3, 0, nt * t32)) &&
-         HIP_OK(hipMemset(q.count, 0, 4));
+    ok = ok && HIP_OK(hipMemset(q.a0, 0, nt * t2)) && HIP_OK(hipMemset(q.count, 0, 4));
     if (ok) {
         int *h = (int *)malloc((size_t)nt * 12);
         if (!h) ok = false;
@@ -434,14 +429,8 @@ static int enqueue_learned(bc250_hip_state *s, const uint8_t *c8, int cp, const 
     if (ensure_prop(s) != 0) return -1;
     const gpu_encode_params_t &p = s->params;
     const int nt = q.ntiles;
-    auto wf = [&](int l) { return q.packed + bc250_cnn_packed_offset(l - 1, 0); };
-    auto bias = [&](int l, int cin) { return q.params + bc250_cnn_w_offset(l - 1) + (9 * cin + 31) / 32 * 32; };
     int r = bc250_cnn_input(c8, cp, r8, rp, 0, s->wc, s->hc, q.tiles, nt, q.a0, st);
-    if (r == 0) r = bc250_cnn_conv(1, 0, q.a0, wf(1), bias(1, 2), q.a1, nullptr, nullptr, nt, st);
-    if (r == 0) r = bc250_cnn_conv(2, 0, q.a1, wf(2), bias(2, 16), q.a2, nullptr, nullptr, nt, st);
-    if (r == 0) r = bc250_cnn_conv(3, 0, q.a2, wf(3), bias(3, 32), q.a3, nullptr, nullptr, nt, st);
-    if (r == 0) r = bc250_cnn_conv(4, 2, q.a3, wf(4), bias(4, 32), nullptr, nullptr, q.rows, nt, st);
-    if (r == 0) r = bc250_cnn_pool(q.rows, q.pool, nt, st);
+    if (r == 0) r = bc250_cnn_forward(q.a0, q.packed, q.params, q.pool, nt, st);
     if (r == 0) r = bc250_cnn_mvin(q.has_prev ? q.prev16 : nullptr, s->mbw, s->mbh, q.mvin, st);
     if (r == 0) r = bc250_cnn_head(q.params + bc250_cnn_w_offset(4), q.pool, q.mvin, q.z, nt, st);
     if (r == 0) r = bc250_cnn_candidates(q.z, s->mbw, s->mbh, p.search_range, q.cand, st);

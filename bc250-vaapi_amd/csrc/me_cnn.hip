@@ -39,14 +39,27 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int T = 64;        // tile side
 constexpr int TP = T + 2;    // padded side
 
-__device__ __forceinline__ uint16_t f2bf(float f)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// fp32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t pack2bf(float a, float b)
 {
-    // round to nearest even (inputs are finite)
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    const bf16x2 v = {(__bf16)a, (__bf16)b};
+    return __builtin_bit_cast(uint32_t, v);
 }
+__device__ __forceinline__ uint16_t f2bf(float f) { return (uint16_t)pack2bf(f, 0.f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Sum over the 16 lanes of a DPP row, complete in the row's last lane:
+// ((v15 + v14) + (v13 + v12)) + ... as a balanced tree (row_shr scan).
+__device__ __forceinline__ float row16_sum(float v)
+{
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xf, 0xf, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xf, 0xf, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xf, 0xf, true));
+    return v;
+}
 
 constexpr int kpad(int cin) { return (9 * cin + 31) / 32 * 32; }
 
@@ -115,7 +128,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t *__restrict
         for (int x0 = 0; x0 < T; x0 += 16) {
             f32x4 acc[NT];
 #pragma unroll
-            for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int t = 0; t < NT; t++) acc[t] = f32x4{bv[t][0], bv[t][1], bv[t][2], bv[t][3]};
 #pragma unroll
             for (int s = 0; s < KS; s++) {
                 const bf16x8 b = im2col8<CIN>(tin, y, x0 + lr, s * 32 + 8 * lg);
@@ -127,10 +140,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t *__restrict
             for (int t = 0; t < NT; t++) {
                 float v[4];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    v[r] = acc[t][r] + bv[t][r];
-                    if (MODE != MODE_MASK) v[r] = fmaxf(v[r], 0.f);
-                }
+                for (int r = 0; r < 4; r++) v[r] = MODE != MODE_MASK ? fmaxf(acc[t][r], 0.f) : acc[t][r];
                 const int c0 = t * 16 + 4 * lg;
                 if (MODE == MODE_MASK) {
                     const uint2 m = *(const uint2 *)(mask + opix * COUT + c0);
@@ -139,28 +149,278 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t *__restrict
                     for (int r = 0; r < 4; r++)
                         if (bf2f((uint16_t)(mm[r >> 1] >> (16 * (r & 1)))) <= 0.f) v[r] = 0.f;
                 }
-                if (out) {
-                    uint2 o;
-                    o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-                    o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
-                    *(uint2 *)(out + opix * COUT + c0) = o;
-                }
+                const uint2 o = make_uint2(pack2bf(v[0], v[1]), pack2bf(v[2], v[3]));
+                if (out) *(uint2 *)(out + opix * COUT + c0) = o;
                 if (MODE == MODE_RELU_POOL) {
                     // the 16 pixels of this segment are in one 16x16 block:
                     // sum the bf16-rounded values over the 16 lanes of a row
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        float s_ = bf2f(f2bf(v[r]));
-#pragma unroll
-                        for (int o = 8; o > 0; o >>= 1) s_ += __shfl_xor(s_, o, 16);
-                        v[r] = s_;
-                    }
-                    if (lr == 0)
+                    v[0] = row16_sum(__uint_as_float(o.x << 16));
+                    v[1] = row16_sum(__uint_as_float(o.x & 0xffff0000u));
+                    v[2] = row16_sum(__uint_as_float(o.y << 16));
+                    v[3] = row16_sum(__uint_as_float(o.y & 0xffff0000u));
+                    if (lr == 15)
                         *(float4 *)(pool + (((size_t)tile * T + y) * 4 + (x0 >> 4)) * COUT + c0) =
                             make_float4(v[0], v[1], v[2], v[3]);
                 }
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference: the four convolutions and the block sums of one band of
+// FB = 16 output rows (one row of 16x16 blocks) of one tile per workgroup,
+// every intermediate activation held in LDS.  conv4's band needs conv3 rows
+// -1 .. 16, conv2 rows -2 .. 17, conv1 rows -3 .. 18 and input rows -4 .. 19
+// (band-relative); rows outside the tile are the zero padding, so they are
+// stored as zeros and not computed.  Two LDS regions alternate:
+//   Q: input (24 x 66 x 2)  -> P: conv1 (22 x 66 x 16) -> Q: conv2 (20 x 66 x 32)
+//   -> P: conv3 (18 x 66 x 32) -> Q: conv4 row sums (16 x 4 x 32 fp32)
+// (160,512 bytes: one workgroup per CU).  Each stored row keeps the zero
+// columns 0 and 65.  The MFMA operands, their K order and the epilogue are
+// the unfused kernels' (conv3x3_kernel, pool_rows_kernel), so the block sums
+// are the same bits.  conv3 and conv4 reuse each loaded input row for R
+// output rows: a task walks input rows i = 0 .. R+1 and kx = 0 .. 2, and
+// output row j takes the step at ky = i - j, so every accumulator still
+// receives its K steps in the order (ky, kx) = (0,0), (0,1), .. (2,2).
+constexpr int FB = 16;
+constexpr int FP_ELEMS = (FB + 2) * TP * 32;   // conv1 / conv3 output
+constexpr int FQ_ELEMS = (FB + 4) * TP * 32;   // input / conv2 output / conv4 row sums
+static_assert((FB + 6) * TP * 16 <= FP_ELEMS, "conv1 rows fit P");
+static_assert((FB + 8) * TP * 2 <= FQ_ELEMS, "input rows fit Q");
+static_assert(FB * 4 * 32 * 2 <= FQ_ELEMS, "row sums fit Q");
+constexpr int FWAVES = 8;
+
+// LDS placement of the 8-channel chunks of a 16- or 32-channel pixel.  The
+// B operand's 16 lanes of one lane group read one chunk of 16 consecutive
+// pixels; unswizzled, pixels 64 (or 32) bytes apart fall on the same banks.
+// SW 1 XORs the chunk index with the pixel bits above the ones that already
+// spread a 16-lane read over 8 banks-groups of 16 bytes (32 banks), SW 2
+// over 16 (64 banks); SW 0 is the plain NHWC order.
+template <int C, int SW>
+__device__ __forceinline__ int chunk_off(int row, int x, int chunk)
+{
+    constexpr int C8 = C / 8, LC = C8 == 4 ? 2 : 1;
+    constexpr int SH = SW == 0 ? 0 : (SW == 1 ? 3 : 4) - LC;
+    const int sw = SW == 0 ? 0 : (x >> SH) & (C8 - 1);
+    return ((row * TP + x) * C8 + (chunk ^ sw)) * 8;
+}
+
+// One layer of the fused band: `nrows` output rows in groups of R, dst row r
+// reads src rows r .. r+2; tile row of dst row 0 is `trow0`.  dst rows
+// outside the tile are zero rows.  rowsum (conv4 only): the bf16 values'
+// 16-pixel segment sums, [row][segment][COUT] fp32.
+// The operands one task reads, for each of its SEGS 16-pixel segments:
+// conv3 / conv4 the (R + 2) x 3 input chunks of its rows, conv2 its 5
+// im2col K steps, conv1 its one K step.
+template <int CIN, int R, int SEGS>
+struct TaskB {
+    static constexpr int PER = CIN == 32 ? (R + 2) * 3 : kpad(CIN) / 32;
+    bf16x8 v[SEGS][PER];
+};
+
+template <int CIN, int R, int SEGS, int SW>
+__device__ __forceinline__ void load_task(const uint16_t *__restrict__ src, int r0, int x0, int lr, int lg,
+                                          TaskB<CIN, R, SEGS> &b)
+{
+#pragma unroll
+    for (int g = 0; g < SEGS; g++) {
+        const int x = x0 + 16 * g + lr;
+        if constexpr (CIN == 32) {
+#pragma unroll
+            for (int i = 0; i < R + 2; i++)
+#pragma unroll
+                for (int kx = 0; kx < 3; kx++)
+                    b.v[g][i * 3 + kx] = __builtin_bit_cast(bf16x8, *(const uint4 *)(src + chunk_off<32, SW>(r0 + i, x + kx, lg)));
+        } else if constexpr (CIN == 16) {
+            static_assert(R == 1, "conv2 walks single rows");
+            // the last K step holds position 8 in lane groups 0, 1 and zero
+            // padding in 2, 3: read position 8 there too and clear it
+#pragma unroll
+            for (int s = 0; s < TaskB<CIN, R, SEGS>::PER; s++) {
+                const int pos = 2 * s + (lg >> 1), pc = pos < 9 ? pos : 8;
+                uint4 q = *(const uint4 *)(src + chunk_off<16, SW>(r0 + pc / 3, x + pc % 3, lg & 1));
+                if (pos >= 9) q = make_uint4(0, 0, 0, 0);
+                b.v[g][s] = __builtin_bit_cast(bf16x8, q);
+            }
+        } else {
+            static_assert(CIN == 2 && R == 1, "conv1: one K step");
+            // k = 8 lg .. 8 lg + 7 are positions 4 lg .. 4 lg + 3, two channels each
+            uint32_t e[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int pos = 4 * lg + q, pc = pos < 9 ? pos : 8;
+                e[q] = *(const uint32_t *)(src + ((r0 + pc / 3) * TP + x + pc % 3) * 2);
+                if (pos >= 9) e[q] = 0u;
+            }
+            b.v[g][0] = __builtin_bit_cast(bf16x8, make_uint4(e[0], e[1], e[2], e[3]));
+        }
+    }
+}
+
+template <int CIN, int COUT>
+struct LayerW {
+    static constexpr int KP = kpad(CIN), KS = KP / 32, NT = COUT / 16;
+    bf16x8 a[NT][KS];
+    float b[NT][4];
+};
+
+// Each lane's share of a layer's weights (the MFMA A operand) and biases.
+template <int CIN, int COUT>
+__device__ __forceinline__ void load_w(const uint16_t *__restrict__ w, const float *__restrict__ bias, LayerW<CIN, COUT> &lw)
+{
+    using LW = LayerW<CIN, COUT>;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < LW::NT; t++)
+#pragma unroll
+        for (int s = 0; s < LW::KS; s++)
+            lw.a[t][s] = __builtin_bit_cast(bf16x8, *(const uint4 *)(w + (size_t)(t * 16 + lr) * LW::KP + s * 32 + 8 * lg));
+#pragma unroll
+    for (int t = 0; t < LW::NT; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) lw.b[t][r] = bias[(size_t)(t * 16 + 4 * lg + r) * (LW::KP + 1)];
+}
+
+// One layer of the fused band: `nrows` output rows, a task being R rows by
+// SEGS 16-pixel segments; dst row r reads src rows r .. r+2; tile row of dst
+// row 0 is `trow0`.  dst rows outside the tile are zero rows.  rowsum (conv4
+// only): the bf16 values' 16-pixel segment sums, [row][segment][COUT] fp32.
+// A wave loads the next task's operands before the current task's
+// epilogue, so their LDS latency overlaps the conversions and stores.
+template <int CIN, int COUT, int R, int SEGS, int SW>
+__device__ __forceinline__ void fused_layer(const uint16_t *__restrict__ src, uint16_t *__restrict__ dst, int nrows,
+                                            int trow0, const LayerW<CIN, COUT> &lw, float *__restrict__ rowsum)
+{
+    constexpr int NT = COUT / 16, TPR = 4 / SEGS;  // tasks per row group
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lg = lane >> 4;
+    const int ntask = nrows / R * TPR;
+    TaskB<CIN, R, SEGS> b;
+    if (wave < ntask) load_task<CIN, R, SEGS, SW>(src, wave / TPR * R, wave % TPR * SEGS * 16, lr, lg, b);
+    for (int task = wave; task < ntask; task += FWAVES) {
+        const int r0 = task / TPR * R, x0 = task % TPR * SEGS * 16;
+        f32x4 acc[SEGS][R][NT];
+#pragma unroll
+        for (int g = 0; g < SEGS; g++)
+#pragma unroll
+            for (int j = 0; j < R; j++)
+#pragma unroll
+                for (int t = 0; t < NT; t++) acc[g][j][t] = f32x4{lw.b[t][0], lw.b[t][1], lw.b[t][2], lw.b[t][3]};
+#pragma unroll
+        for (int g = 0; g < SEGS; g++) {
+            if constexpr (CIN == 32) {
+#pragma unroll
+                for (int i = 0; i < R + 2; i++)
+#pragma unroll
+                    for (int kx = 0; kx < 3; kx++)
+#pragma unroll
+                        for (int j = 0; j < R; j++) {
+                            const int ky = i - j;
+                            if (ky >= 0 && ky <= 2)
+#pragma unroll
+                                for (int t = 0; t < NT; t++)
+                                    acc[g][j][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                        lw.a[t][3 * ky + kx], b.v[g][i * 3 + kx], acc[g][j][t], 0, 0, 0);
+                        }
+            } else {
+#pragma unroll
+                for (int s = 0; s < TaskB<CIN, R, SEGS>::PER; s++)
+#pragma unroll
+                    for (int t = 0; t < NT; t++)
+                        acc[g][0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lw.a[t][s], b.v[g][s], acc[g][0][t], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const int nx = task + FWAVES;
+        if (nx < ntask) load_task<CIN, R, SEGS, SW>(src, nx / TPR * R, nx % TPR * SEGS * 16, lr, lg, b);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int g = 0; g < SEGS; g++) {
+            const int xs = x0 + 16 * g;
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                const int row = r0 + j, tr = trow0 + row;
+                const bool inside = tr >= 0 && tr < T;
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    float v[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) v[r] = inside ? fmaxf(acc[g][j][t][r], 0.f) : 0.f;
+                    const int c0 = t * 16 + 4 * lg;
+                    const uint2 o = make_uint2(pack2bf(v[0], v[1]), pack2bf(v[2], v[3]));
+                    if (dst) {
+                        const int ch = c0 >> 3, half = c0 & 7;
+                        *(uint2 *)(dst + chunk_off<COUT, SW>(row, xs + lr + 1, ch) + half) = o;
+                        if (xs == 0 && lr == 0) *(uint2 *)(dst + chunk_off<COUT, SW>(row, 0, ch) + half) = make_uint2(0, 0);
+                        if (xs == T - 16 && lr == 15)
+                            *(uint2 *)(dst + chunk_off<COUT, SW>(row, TP - 1, ch) + half) = make_uint2(0, 0);
+                    }
+                    if (rowsum) {
+                        v[0] = row16_sum(__uint_as_float(o.x << 16));
+                        v[1] = row16_sum(__uint_as_float(o.x & 0xffff0000u));
+                        v[2] = row16_sum(__uint_as_float(o.y << 16));
+                        v[3] = row16_sum(__uint_as_float(o.y & 0xffff0000u));
+                        if (lr == 15)
+                            *(float4 *)(rowsum + ((row * 4) + (xs >> 4)) * COUT + c0) = make_float4(v[0], v[1], v[2], v[3]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// in: input tiles [tile][66][66][2]; packed: forward weights; params: fp32
+// parameters (biases); pool: [tile][16][32] block sums of conv4.
+struct FusedWeights {
+    const uint16_t *w[4];   // forward weights [COUT][KPAD] per layer
+    const float *b[4];      // biases, channel c at b[c * (KPAD + 1)]
+};
+
+template <int SW>
+__global__ __launch_bounds__(FWAVES * 64) void cnn_forward_fused_kernel(const uint16_t *__restrict__ in,
+                                                                        const FusedWeights fw,
+                                                                        float *__restrict__ pool, int ntiles)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t sp[FP_ELEMS];
+    __shared__ __attribute__((aligned(16))) uint16_t sq[FQ_ELEMS];
+    const int tile = blockIdx.x >> 2, band = blockIdx.x & 3, tid = threadIdx.x;
+    if (tile >= ntiles) return;
+    const int y0 = band * FB;  // tile row of the band's first output row
+    // the weights of conv1..conv3 are requested first (their latency hides
+    // under the input staging), conv4's at the end of conv3, into conv3's
+    // registers
+    LayerW<2, 16> w1;
+    LayerW<16, 32> w2;
+    LayerW<32, 32> w3;
+    load_w(fw.w[0], fw.b[0], w1);
+    load_w(fw.w[1], fw.b[1], w2);
+    load_w(fw.w[2], fw.b[2], w3);
+    // input rows y0-4 .. y0+FB+3: padded global row = tile row + 1
+    {
+        const uint32_t *g = (const uint32_t *)(in + (size_t)tile * TP * TP * 2);
+        uint32_t *q = (uint32_t *)sq;
+        for (int i = tid; i < (FB + 8) * TP; i += FWAVES * 64) {
+            const int r = i / TP, x = i % TP, tr = y0 - 4 + r;
+            q[i] = (tr >= -1 && tr <= T) ? g[(tr + 1) * TP + x] : 0u;
+        }
+    }
+    __syncthreads();
+    fused_layer<2, 16, 1, 4, SW>(sq, sp, FB + 6, y0 - 3, w1, nullptr);
+    __syncthreads();
+    fused_layer<16, 32, 1, 2, SW>(sp, sq, FB + 4, y0 - 2, w2, nullptr);
+    __syncthreads();
+    fused_layer<32, 32, 3, 1, SW>(sq, sp, FB + 2, y0 - 1, w3, nullptr);
+    load_w(fw.w[3], fw.b[3], w3);
+    __syncthreads();
+    float *rows = (float *)sq;
+    fused_layer<32, 32, 4, 1, SW>(sp, nullptr, FB, y0, w3, rows);
+    __syncthreads();
+    if (tid < 4 * 32) {
+        const int bx = tid >> 5, c = tid & 31;
+        float s = 0.f;
+        for (int y = 0; y < FB; y++) s += rows[(y * 4 + bx) * 32 + c];
+        pool[((size_t)tile * 16 + band * 4 + bx) * 32 + c] = s;
     }
 }
 
@@ -522,6 +782,34 @@ extern "C" int bc250_cnn_conv(int layer, int mode, const uint16_t *in, const uin
         return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+static int g_forward_variant = 1;
+extern "C" int bc250_cnn_forward(const uint16_t *in, const uint16_t *packed, const float *params, float *pool,
+                                 int ntiles, void *stream)
+{
+    if (!in || !packed || !params || !pool || ntiles <= 0 || ntiles > (1 << 28)) return -1;
+    const int cin[4] = {2, 16, 32, 32};
+    FusedWeights fw;
+    for (int l = 0; l < 4; l++) {
+        fw.w[l] = packed + bc250_cnn_packed_offset(l, 0);
+        fw.b[l] = params + bc250_cnn_w_offset(l) + kpad(cin[l]);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    switch (g_forward_variant) {
+    case 0: cnn_forward_fused_kernel<0><<<ntiles * 4, FWAVES * 64, 0, st>>>(in, fw, pool, ntiles); break;
+    case 1: cnn_forward_fused_kernel<1><<<ntiles * 4, FWAVES * 64, 0, st>>>(in, fw, pool, ntiles); break;
+    default: cnn_forward_fused_kernel<2><<<ntiles * 4, FWAVES * 64, 0, st>>>(in, fw, pool, ntiles); break;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// The fused forward's LDS swizzle (0, 1, 2: see chunk_off); a tunable.
+extern "C" int bc250_cnn_set_forward_variant(int v)
+{
+    if (v < 0 || v > 2) return -1;
+    g_forward_variant = v;
+    return 0;
 }
 
 extern "C" int bc250_cnn_pool(const float *rows, float *pool, int ntiles, void *stream)

@@ -299,8 +299,8 @@ extern "C" int bc250_fallback_select(const uint32_t *r16, int mbw, int mbh, int 
 // head outputs, the fallback list, its count.
 namespace {
 struct LearnedWork {
-    uint16_t *a0, *a1, *a2, *a3;
-    float *rows, *pool, *mvin, *z;
+    uint16_t *a0;
+    float *pool, *mvin, *z;
     int *list, *count;
     size_t bytes;
 };
@@ -312,10 +312,6 @@ LearnedWork carve_learned(void *base, int nt, int nmb)
     size_t o = 0;
     auto take = [&](size_t n) { uint8_t *q = p ? p + o : nullptr; o += al256(n); return q; };
     w.a0 = (uint16_t *)take((size_t)nt * bc250_cnn_tile_bytes(2));
-    w.a1 = (uint16_t *)take((size_t)nt * bc250_cnn_tile_bytes(16));
-    w.a2 = (uint16_t *)take((size_t)nt * bc250_cnn_tile_bytes(32));
-    w.a3 = (uint16_t *)take((size_t)nt * bc250_cnn_tile_bytes(32));
-    w.rows = (float *)take((size_t)nt * 64 * 4 * 32 * 4);
     w.pool = (float *)take((size_t)nt * 16 * 32 * 4);
     w.mvin = (float *)take((size_t)nt * 16 * 32 * 4);
     w.z = (float *)take((size_t)nt * 16 * BC250_CNN_NOUT * 4);
@@ -351,14 +347,8 @@ extern "C" int bc250_learned_search_batch(const bc250_me_batch_t *b, const float
     const LearnedWork w = carve_learned(work, work_tiles, work_mbs);
     hipStream_t st = (hipStream_t)stream;
     const int nt = b->ntile_total;
-    auto wf = [&](int l) { return packed + bc250_cnn_packed_offset(l - 1, 0); };
-    auto bias = [&](int l, int cin) { return params + bc250_cnn_w_offset(l - 1) + (9 * cin + 31) / 32 * 32; };
     int r = bc250_cnn_input_batch(b, w.a0, st);
-    if (r == 0) r = bc250_cnn_conv(1, 0, w.a0, wf(1), bias(1, 2), w.a1, nullptr, nullptr, nt, st);
-    if (r == 0) r = bc250_cnn_conv(2, 0, w.a1, wf(2), bias(2, 16), w.a2, nullptr, nullptr, nt, st);
-    if (r == 0) r = bc250_cnn_conv(3, 0, w.a2, wf(3), bias(3, 32), w.a3, nullptr, nullptr, nt, st);
-    if (r == 0) r = bc250_cnn_conv(4, 2, w.a3, wf(4), bias(4, 32), nullptr, nullptr, w.rows, nt, st);
-    if (r == 0) r = bc250_cnn_pool(w.rows, w.pool, nt, st);
+    if (r == 0) r = bc250_cnn_forward(w.a0, packed, params, w.pool, nt, st);
     if (r == 0) {
         cnn_mvin_batch_kernel<<<(nt * 16 + 127) / 128, 128, 0, st>>>(*b, w.mvin);
         r = hipGetLastError() == hipSuccess ? 0 : -1;

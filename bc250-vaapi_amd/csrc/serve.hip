@@ -71,7 +71,7 @@ struct bc250_server {
     int *fb_host = nullptr;         // pinned fallback counts of the last FB_RING batches, by batch number
     // statistics
     int batches = 0, frames = 0, max_batch_seen = 0;
-    int conv_cap = 0;
+    int forward_variant = -1;
     double tune_us[8] = {0};
 };
 
@@ -475,14 +475,13 @@ extern "C" int bc250_server_autotune(bc250_server_t *srv)
         b.ntile_total += s.ntiles;
     }
     if (b.n == 0) return -1;
-    const int caps[5] = {1024, 2048, 4096, 8192, 16384};
     hipEvent_t e0, e1;
     if (!HIP_OK(hipEventCreate(&e0)) || !HIP_OK(hipEventCreate(&e1))) return -1;
     const bc250_proposer_cfg_t &q = srv->cfg.proposer;
     double best = 1e30;
-    int best_cap = 4096, rc = 0;
-    for (int i = 0; i < 5 && rc == 0; i++) {
-        bc250_cnn_set_grid_cap(caps[i]);
+    int best_v = 1, rc = 0;
+    for (int i = 0; i < 3 && rc == 0; i++) {
+        bc250_cnn_set_forward_variant(i);
         float ms = 0.f, tot = 0.f;
         for (int rep = 0; rep < 4 && rc == 0; rep++) {
             hipEventRecord(e0, srv->stream);
@@ -496,13 +495,13 @@ extern "C" int bc250_server_autotune(bc250_server_t *srv)
         srv->tune_us[i] = tot / 3 * 1e3;
         if (srv->tune_us[i] < best) {
             best = srv->tune_us[i];
-            best_cap = caps[i];
+            best_v = i;
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    bc250_cnn_set_grid_cap(best_cap);
-    srv->conv_cap = best_cap;
+    bc250_cnn_set_forward_variant(best_v);
+    srv->forward_variant = best_v;
     return rc;
 }
 
@@ -515,7 +514,7 @@ extern "C" int bc250_server_stats(const bc250_server_t *srv, bc250_server_stats_
     st->max_batch_seen = srv->max_batch_seen;
     st->mean_batch = srv->batches ? (double)srv->frames / srv->batches : 0.0;
     for (auto &s : srv->ses) st->sessions += s.open ? 1 : 0;
-    st->conv_grid_cap = srv->conv_cap;
+    st->forward_variant = srv->forward_variant;
     for (int i = 0; i < 8; i++) st->autotune_us[i] = srv->tune_us[i];
     st->hbm_used = srv->used;
     return 0;

@@ -124,6 +124,15 @@ class Net:
                  "head")
         return self.z[:n]
 
+    def forward_fused(self, n, mvin):
+        """Inference in one launch (layers 1..4 and the block sums on chip),
+        then the head; the activations act[1..4] are not written."""
+        self._ok(self.hip.bc250_cnn_forward(self._p(self.act[0]), self._p(self.packed), self._p(self.params),
+                                            self._p(self.pool), n, None), "fused forward")
+        self._ok(self.hip.bc250_cnn_head(self.head_w(), self._p(self.pool), self._p(mvin), self._p(self.z), n, None),
+                 "head")
+        return self.z[:n]
+
     def backward(self, n, mvin, target, scale):
         """Loss per tile and the parameter gradient (times scale) of the last
         forward(keep=True) on n tiles; target [n,16,2] float pel (device)."""

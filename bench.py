@@ -321,8 +321,8 @@ def bench_serve(pkg, args, seconds=2.0, ring=4, sat_frames=40):
             "hbm_bytes": st["hbm_used"],
         }
         if mode == "learned":
-            out[mode]["autotune"] = {"conv_grid_cap": st["conv_grid_cap"],
-                                     "us_by_cap_1024_2048_4096_8192_16384": [round(x, 1) for x in st["autotune_us"]]}
+            out[mode]["autotune"] = {"forward_variant": st["forward_variant"],
+                                     "us_by_variant_0_1_2": [round(x, 1) for x in st["autotune_us"]]}
             fb = res["fallbacks"][res["fallbacks"] >= 0]
             out[mode]["fallback_mbs_per_batch"] = round(float(fb.mean()), 1) if fb.size else None
         log(f"serve {mode}: {out[mode]['paced']}")

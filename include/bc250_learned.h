@@ -69,6 +69,20 @@ int bc250_cnn_conv(int layer, int mode, const uint16_t *in, const uint16_t *w, c
 /* Block sums from the row-segment sums, in a fixed order. */
 int bc250_cnn_pool(const float *rows, float *pool, int ntiles, void *stream);
 
+/* Inference in one launch: conv1..conv4 and the block sums [tile][16][32]
+ * of the input tiles `in` ([tile][66][66][2], zero borders), every
+ * intermediate activation kept on chip; packed: the forward weights
+ * (bc250_cnn_pack), params: the fp32 parameters (biases).  The same bits as
+ * bc250_cnn_conv layers 1..4 (mode 2 for layer 4) followed by
+ * bc250_cnn_pool. */
+int bc250_cnn_forward(const uint16_t *in, const uint16_t *packed, const float *params, float *pool, int ntiles,
+                      void *stream);
+
+/* The fused launch's LDS placement of activation channels: 0 plain NHWC,
+ * 1 or 2 channel chunks XOR-swizzled by pixel for 32- or 64-bank reads.
+ * Process-wide; every variant gives the same bits. */
+int bc250_cnn_set_forward_variant(int variant);
+
 /* Weight and bias gradient of `layer` from its input activation a and its
  * pre-activation gradient dz, summed over the tiles in a fixed order into
  * grad ([COUT][KPAD + 1], the layer's parameter layout); part is a work

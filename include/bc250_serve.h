@@ -64,7 +64,7 @@ typedef struct {
 typedef struct {
     int32_t batches, frames, max_batch_seen, sessions;
     double mean_batch;
-    int32_t conv_grid_cap;   /* the autotuner's choice (0: not tuned) */
+    int32_t forward_variant; /* the autotuner's choice of the network's LDS layout (-1: not tuned) */
     double autotune_us[8];   /* its measurements, per candidate in bc250_server_autotune's order */
     size_t hbm_used;
 } bc250_server_stats_t;
@@ -100,8 +100,9 @@ int bc250_server_upload(bc250_server_t *srv, int session, int source, const uint
 int bc250_server_run(bc250_server_t *srv, const bc250_request_t *reqs, int n, int checksums,
                      bc250_frame_result_t *out, double *wall_s);
 
-/* Times the learned search's convolution launch shapes on a batch of every
- * open session and keeps the fastest (a no-op for the plain search). */
+/* Times the learned search on a batch of every open session with each
+ * LDS layout of the fused network launch (bc250_cnn_set_forward_variant)
+ * and keeps the fastest (a no-op for the plain search). */
 int bc250_server_autotune(bc250_server_t *srv);
 
 int bc250_server_stats(const bc250_server_t *srv, bc250_server_stats_t *st);

@@ -110,6 +110,33 @@ def test_forward_layers_match_oracle(pkg, learned):
     np.testing.assert_allclose(net.z.cpu().numpy(), want_z, rtol=1e-4, atol=1e-4)
 
 
+def test_fused_forward_equals_the_layers(pkg, learned):
+    """The one-launch inference (bands of 16 rows, activations in LDS, input
+    rows reused across output rows) gives the block sums and head outputs of
+    the four separate layer launches bit for bit: random weights on 7 tiles
+    at the picture's edges, then the trained weights."""
+    import torch
+    say("learned: fused forward against the layer launches, 7 tiles")
+    for trained in (False, True):
+        net, p, x, mvin, dmv = _setup(pkg, learned, 7, 11)
+        if trained:
+            net.params.copy_(torch.from_numpy(pkg.load_proposer_weights()))
+            net.pack()
+        want_z = net.forward(7, dmv).clone()
+        want_pool = net.pool[:7].clone()
+        for variant in (0, 1, 2):   # every LDS layout
+            assert pkg.hip.bc250_cnn_set_forward_variant(variant) == 0
+            net.pool.zero_()
+            net.z.zero_()
+            got_z = net.forward_fused(7, dmv)
+            torch.cuda.synchronize()
+            assert torch.equal(net.pool[:7], want_pool), f"block sums differ (trained={trained}, variant {variant})"
+            assert torch.equal(got_z, want_z), f"head outputs differ (trained={trained}, variant {variant})"
+        assert pkg.hip.bc250_cnn_set_forward_variant(1) == 0
+        assert pkg.hip.bc250_cnn_set_forward_variant(3) == -1
+        assert (want_pool > 0).float().mean() > 0.2
+
+
 def test_backward_matches_oracle(pkg, learned):
     import torch
     say("learned: backward, 4 tiles")
