@@ -81,6 +81,7 @@ struct Proposer {
     uint16_t *a0 = nullptr;      // input tiles [tile][66][66][2]
     float *pool = nullptr, *mvin = nullptr, *z = nullptr;
     int *cand = nullptr, *list = nullptr, *count = nullptr;
+    uint32_t *keys = nullptr;    // the split fallback search's keys
     uint32_t *prev16 = nullptr;  // the previous picture's integer 16x16 vectors
     bool has_prev = false;       // prev16 holds the picture before the current one
     bool ran = false;            // the current picture ran the learned search
@@ -236,9 +237,10 @@ static void free_res(bc250_hip_state *s)
     {
         Proposer &q = s->prop;
         for (void *p : {(void *)q.tiles, (void *)q.a0, (void *)q.pool, (void *)q.mvin, (void *)q.z, (void *)q.cand,
-                        (void *)q.list, (void *)q.count, (void *)q.prev16})
+                        (void *)q.list, (void *)q.count, (void *)q.prev16, (void *)q.keys})
             if (p) hipFree(p);
         q.tiles = q.cand = q.list = q.count = nullptr;
+        q.keys = nullptr;
         q.a0 = nullptr;
         q.pool = q.mvin = q.z = nullptr;
         q.prev16 = nullptr;
@@ -399,7 +401,8 @@ static int ensure_prop(bc250_hip_state *s)
               HIP_OK(hipMalloc(&q.mvin, (size_t)nt * 16 * 32 * 4)) &&
               HIP_OK(hipMalloc(&q.z, (size_t)nt * 16 * BC250_CNN_NOUT * 4)) &&
               HIP_OK(hipMalloc(&q.cand, (size_t)s->nmb * 4 * 8)) && HIP_OK(hipMalloc(&q.list, (size_t)s->nmb * 4)) &&
-              HIP_OK(hipMalloc(&q.count, 4)) && HIP_OK(hipMalloc(&q.prev16, (size_t)s->nmb * 16));
+              HIP_OK(hipMalloc(&q.count, 4)) && HIP_OK(hipMalloc(&q.prev16, (size_t)s->nmb * 16)) &&
+              HIP_OK(hipMalloc(&q.keys, (size_t)s->nmb * 5 * 4));
     ok = ok && HIP_OK(hipMemset(q.a0, 0, nt * t2)) && HIP_OK(hipMemset(q.count, 0, 4));
     if (ok) {
         int *h = (int *)malloc((size_t)nt * 12);
@@ -441,8 +444,8 @@ static int enqueue_learned(bc250_hip_state *s, const uint8_t *c8, int cp, const 
         r = bc250_fallback_select(out16, s->mbw, s->mbh, q.cfg.alpha_q4, q.cfg.beta, q.cfg.t_high, q.list, q.count,
                                   st);
     if (r == 0)
-        r = bc250_launch_me_fullsearch_list(c8, cp, r8, rp, s->wc, s->hc, p.search_range, p.lambda8, out16, out8,
-                                            q.list, q.count, st);
+        r = bc250_launch_me_fullsearch_list_split(c8, cp, r8, rp, s->wc, s->hc, p.search_range, p.lambda8, out16,
+                                                  out8, q.list, q.count, q.keys, st);
     if (r == 0) r = HIP_OK(hipMemcpyAsync(q.prev16, out16, (size_t)s->nmb * 16, hipMemcpyDeviceToDevice, st)) ? 0 : -1;
     q.ran = r == 0;
     return r;

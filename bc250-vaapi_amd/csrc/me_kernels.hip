@@ -70,6 +70,92 @@ BC_DEV uint32_t mad_hi16(uint32_t x, uint32_t li)
 }
 }  // namespace
 
+// One lane task of the full search (the split list search's copy of the
+// loop body in fullsearch_mb, which keeps its own inline so that its
+// register allocation stays spill-free): the 4 (dx) x DY (dy) candidates
+// from (dx0, dy0), window rows from wrow (the task's first row and dword),
+// keys folded into best16 / bq.
+template <int R>
+__device__ __forceinline__ void fs_task(const uint32_t *__restrict__ wrow, const uint32_t (&cw)[16][4], int lambda8,
+                                        int dx0, int dy0, uint32_t &best16, uint32_t (&bq)[4])
+{
+    using Gm = FsGeom<R>;
+    // Two 8x8-quadrant costs per register: the left quadrant in bits 0..15
+    // (v_sad_u8), the right one in bits 16..31 (v_sad_hi_u8); acc[j][k][0]
+    // rows 0-7, [1] rows 8-15.  Both start at lambda8*|dx| + lambda8*|dy|,
+    // plus 20480 per axis outside the range: a valid quadrant costs at most
+    // 16320 + 15*128 = 18240 < 20480, and no half exceeds 2*20480 + 2055 +
+    // 16320 < 2^16, so the halves never carry into each other.
+    uint32_t ax[4], by[DY];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        ax[k] = (uint32_t)(lambda8 * iabs(dx0 + k)) + (dx0 + k > R ? 20480u : 0u);
+#pragma unroll
+    for (int j = 0; j < DY; j++)
+        by[j] = (uint32_t)(lambda8 * iabs(dy0 + j)) + (dy0 + j > R ? 20480u : 0u);
+    uint32_t acc[DY][4][2];
+#pragma unroll
+    for (int j = 0; j < DY; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[j][k][0] = acc[j][k][1] = (ax[k] + by[j]) * 0x10001u;
+#pragma unroll
+    for (int r = 0; r < DY + 15; r++) {
+        uint32_t d[5];
+#pragma unroll
+        for (int i = 0; i < 5; i++) d[i] = wrow[r * Gm::PITCH + i];
+        uint32_t s[4][4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            s[0][c] = d[c];
+            s[1][c] = __builtin_amdgcn_alignbyte(d[c + 1], d[c], 1);
+            s[2][c] = __builtin_amdgcn_alignbyte(d[c + 1], d[c], 2);
+            s[3][c] = __builtin_amdgcn_alignbyte(d[c + 1], d[c], 3);
+        }
+#pragma unroll
+        for (int j = 0; j < DY; j++) {
+            const int cr = r - j;
+            if (cr < 0 || cr > 15) continue;
+            const int h = cr >> 3;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                uint32_t a = acc[j][k][h];
+                a = __builtin_amdgcn_sad_u8(s[k][0], cw[cr][0], a);
+                a = __builtin_amdgcn_sad_u8(s[k][1], cw[cr][1], a);
+                a = __builtin_amdgcn_sad_hi_u8(s[k][2], cw[cr][2], a);
+                a = __builtin_amdgcn_sad_hi_u8(s[k][3], cw[cr][3], a);
+                acc[j][k][h] = a;
+            }
+        }
+    }
+    // per-lane argmin with a 6-bit local index (j-major = raster order)
+    uint32_t l16 = 0xffffffffu, lq[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+#pragma unroll
+    for (int j = 0; j < DY; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t li = (uint32_t)(j * 4 + k);
+            const uint32_t top = acc[j][k][0], bot = acc[j][k][1];
+            // the four halves summed: |x.lo - 0| + |x.hi - 0| + c
+            const uint32_t s16 = __builtin_amdgcn_sad_u16(bot, 0u, __builtin_amdgcn_sad_u16(top, 0u, 0u));
+            l16 = min(l16, (s16 << 6) | li);
+            lq[0] = min(lq[0], mad_lo16(top, li));
+            lq[1] = min(lq[1], mad_hi16(top, li));
+            lq[2] = min(lq[2], mad_lo16(bot, li));
+            lq[3] = min(lq[3], mad_hi16(bot, li));
+        }
+    // promote to global keys: cost*2^15 + raster index (every tile holds a
+    // candidate inside the range, so its minimum is one)
+    auto gkey = [&](uint32_t lk) -> uint32_t {
+        uint32_t li = lk & 63u, cost = lk >> 6;
+        int dx = dx0 + (int)(li & 3u), dy = dy0 + (int)(li >> 2);
+        uint32_t idx = (uint32_t)((dy + R) * Gm::NC + (dx + R));
+        return (cost << 15) | idx;
+    };
+    best16 = min(best16, gkey(l16));
+#pragma unroll
+    for (int q = 0; q < 4; q++) bq[q] = min(bq[q], gkey(lq[q]));
+}
+
 // The full search of one MB (mb of a frame mbw MBs wide) by one workgroup:
 // out16[mb] = {dx, dy, cost, sad}; out8[4 mb + q] likewise for quadrant q.
 template <int R>
@@ -273,6 +359,151 @@ __global__ __launch_bounds__(ME_THREADS, BC250_ME_WAVES_PER_SIMD) void me_fullse
                      (uint4 *)fr.out16, (uint4 *)fr.out8, win, red);
 }
 
+// ---- The full search of listed MBs with each MB split over NPART
+// workgroups by dy groups (GPP groups of DY candidate rows each), so that a
+// short list (the learned search's fallback) still fills the chip.  Every
+// part's workgroup minimum of the five keys is folded into keys[5 i ..
+// 5 i + 4] (i = list position) with atomicMin, which gives the unsplit
+// search's keys whatever the order; fs_finish turns keys into records.
+template <int R>
+struct FsSplit {
+    using Gm = FsGeom<R>;
+    static constexpr int GPP = 3;
+    static constexpr int NPART = (Gm::NGY + GPP - 1) / GPP;
+    static constexpr int NTH = (Gm::NG * GPP + 63) / 64 * 64;
+    static constexpr int NR = GPP * DY + 15;
+};
+
+template <int R>
+__device__ __forceinline__ void fullsearch_part(const uint8_t *__restrict__ cur, int cpitch,
+                                                const uint8_t *__restrict__ ref, int rpitch, int wc, int hc, int mbw,
+                                                int mb, int lambda8, int part, uint32_t *win, uint32_t (*red)[5],
+                                                uint32_t *__restrict__ keys)
+{
+    using Gm = FsGeom<R>;
+    using Sp = FsSplit<R>;
+    const int x0 = (mb % mbw) * 16, y0 = (mb / mbw) * 16, tid = threadIdx.x;
+    const int dyg0 = part * Sp::GPP, ng = min(Sp::GPP, Gm::NGY - dyg0);
+    // the window rows of dy groups dyg0 .. dyg0 + ng - 1, edge-replicated
+    const int wx0 = x0 - R, wy0 = y0 - R + dyg0 * DY, nrows = ng * DY + 15;
+    for (int i = tid; i < nrows * Gm::WD; i += Sp::NTH) {
+        const int r = i / Gm::WD, d = i - r * Gm::WD;
+        const uint8_t *row = ref + (size_t)clampi(wy0 + r, 0, hc - 1) * rpitch;
+        const int gx = wx0 + 4 * d;
+        uint32_t v;
+        if (gx >= 0 && gx + 3 < wc) {
+            v = *(const uint32_t *)(row + gx);
+        } else {
+            v = (uint32_t)row[clampi(gx, 0, wc - 1)] | ((uint32_t)row[clampi(gx + 1, 0, wc - 1)] << 8) |
+                ((uint32_t)row[clampi(gx + 2, 0, wc - 1)] << 16) | ((uint32_t)row[clampi(gx + 3, 0, wc - 1)] << 24);
+        }
+        win[r * Gm::PITCH + d] = v;
+    }
+    uint32_t cw[16][4];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t *p = (const uint32_t *)(cur + (size_t)(y0 + r) * cpitch + x0);
+#pragma unroll
+        for (int c = 0; c < 4; c++) cw[r][c] = __builtin_amdgcn_readfirstlane(p[c]);
+    }
+    __syncthreads();
+    uint32_t best16 = 0xffffffffu, bq[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+    if (tid < Gm::NG * ng) {
+        const int dxg = tid % Gm::NG, dyl = tid / Gm::NG;
+        fs_task<R>(win + (dyl * DY) * Gm::PITCH + dxg, cw, lambda8, -R + 4 * dxg, -R + DY * (dyg0 + dyl), best16, bq);
+    }
+    best16 = wave_min_u32(best16);
+#pragma unroll
+    for (int q = 0; q < 4; q++) bq[q] = wave_min_u32(bq[q]);
+    const int wave = tid >> 6, lane = tid & 63;
+    if (lane == 0) {
+        red[wave][0] = best16;
+#pragma unroll
+        for (int q = 0; q < 4; q++) red[wave][1 + q] = bq[q];
+    }
+    __syncthreads();
+    if (tid < 5) {
+        uint32_t k = red[0][tid];
+#pragma unroll
+        for (int w = 1; w < Sp::NTH / 64; w++) k = min(k, red[w][tid]);
+        atomicMin(keys + tid, k);
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(FsSplit<R>::NTH) void me_fullsearch_split_kernel(
+    const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref, int rpitch, int wc, int hc,
+    int mbw, int nmb, int lambda8, const int *__restrict__ list, const int *__restrict__ count,
+    uint32_t *__restrict__ keys)
+{
+    __shared__ uint32_t win[FsSplit<R>::NR * FsGeom<R>::PITCH];
+    __shared__ uint32_t red[FsSplit<R>::NTH / 64][5];
+    const int n = min(*count, nmb);
+    const int i = (int)blockIdx.x / FsSplit<R>::NPART, part = (int)blockIdx.x % FsSplit<R>::NPART;
+    if (i >= n) return;
+    fullsearch_part<R>(cur, cpitch, ref, rpitch, wc, hc, mbw, list[i], lambda8, part, win, red, keys + 5 * i);
+}
+
+template <int R>
+__global__ __launch_bounds__(FsSplit<R>::NTH) void me_fullsearch_split_batch_kernel(
+    const bc250_me_batch_t b, const int *__restrict__ list, const int *__restrict__ count,
+    uint32_t *__restrict__ keys)
+{
+    __shared__ uint32_t win[FsSplit<R>::NR * FsGeom<R>::PITCH];
+    __shared__ uint32_t red[FsSplit<R>::NTH / 64][5];
+    const int n = min(*count, b.nmb_total);
+    const int i = (int)blockIdx.x / FsSplit<R>::NPART, part = (int)blockIdx.x % FsSplit<R>::NPART;
+    if (i >= n) return;
+    const int g = list[i];
+    if (g < 0 || g >= b.nmb_total) return;
+    const bc250_batch_frame_t &fr = b.f[batch_frame_of_mb(b, g)];
+    fullsearch_part<R>(fr.cur, fr.cpitch, fr.ref, fr.rpitch, fr.wc, fr.hc, fr.wc / 16, g - fr.mb_base, b.lambda8,
+                       part, win, red, keys + 5 * i);
+}
+
+__global__ void fs_keys_init_kernel(const int *__restrict__ count, int cap, uint32_t *__restrict__ keys)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 5 * min(*count, cap)) keys[i] = 0xffffffffu;
+}
+
+// Record of key q (0: 16x16, 1..4: 8x8 quadrants) of a search of range R.
+__device__ __forceinline__ uint4 fs_record(uint32_t k, int q, int range, int lambda8)
+{
+    const int nc = 2 * range + 1;
+    const uint32_t idx = k & 0x7fffu, cost = k >> 15;
+    const int dx = (int)(idx % nc) - range, dy = (int)(idx / nc) - range;
+    const uint32_t l1 = (uint32_t)(iabs(dx) + iabs(dy));
+    const uint32_t sad = cost - (q == 0 ? 4u : 1u) * (uint32_t)lambda8 * l1;
+    return make_uint4((uint32_t)dx, (uint32_t)dy, cost, sad);
+}
+
+__global__ void fs_finish_kernel(const int *__restrict__ list, const int *__restrict__ count, int cap,
+                                 const uint32_t *__restrict__ keys, int range, int lambda8, uint4 *__restrict__ out16,
+                                 uint4 *__restrict__ out8)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 5 * min(*count, cap)) return;
+    const int e = i / 5, q = i % 5, mb = list[e];
+    const uint4 o = fs_record(keys[i], q, range, lambda8);
+    if (q == 0) out16[mb] = o;
+    else out8[4 * mb + q - 1] = o;
+}
+
+__global__ void fs_finish_batch_kernel(const bc250_me_batch_t b, const int *__restrict__ list,
+                                       const int *__restrict__ count, const uint32_t *__restrict__ keys)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 5 * min(*count, b.nmb_total)) return;
+    const int e = i / 5, q = i % 5, g = list[e];
+    if (g < 0 || g >= b.nmb_total) return;
+    const bc250_batch_frame_t &fr = b.f[batch_frame_of_mb(b, g)];
+    const int mb = g - fr.mb_base;
+    const uint4 o = fs_record(keys[i], q, b.range, b.lambda8);
+    if (q == 0) ((uint4 *)fr.out16)[mb] = o;
+    else ((uint4 *)fr.out8)[4 * mb + q - 1] = o;
+}
+
 // ---------------------------------------------------------------------------
 // Sub-pel refinement: 16x16 then each 8x8 quadrant, one wave per MB.  For a
 // block with integer winner (dx,dy): SAD at the integer position, then the 8
@@ -440,6 +671,61 @@ extern "C" int bc250_launch_me_fullsearch_list(const uint8_t *cur, int cpitch, c
     default:
         return -1;
     }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_launch_me_fullsearch_list_split(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch,
+                                                     int wc, int hc, int range, int lambda8, uint32_t *out16,
+                                                     uint32_t *out8, const int *list, const int *count,
+                                                     uint32_t *keys, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || (cpitch & 3) || lambda8 < 0 || lambda8 > 15 || !list ||
+        !count || !keys || !out16 || !out8)
+        return -1;
+    const int mbw = wc / 16, nmb = mbw * (hc / 16);
+    const int kb = (5 * nmb + 255) / 256;
+    fs_keys_init_kernel<<<kb, 256, 0, st>>>(count, nmb, keys);
+    switch (range) {
+#define BC250_SPLIT(RR)                                                                                               \
+    case RR:                                                                                                          \
+        me_fullsearch_split_kernel<RR><<<FsSplit<RR>::NPART * nmb, FsSplit<RR>::NTH, 0, st>>>(                        \
+            cur, cpitch, ref, rpitch, wc, hc, mbw, nmb, lambda8, list, count, keys);                                 \
+        break;
+        BC250_SPLIT(16)
+        BC250_SPLIT(32)
+        BC250_SPLIT(64)
+#undef BC250_SPLIT
+    default:
+        return -1;
+    }
+    fs_finish_kernel<<<kb, 256, 0, st>>>(list, count, nmb, keys, range, lambda8, (uint4 *)out16, (uint4 *)out8);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_launch_me_fullsearch_batch_split(const bc250_me_batch_t *b, const int *list, const int *count,
+                                                      uint32_t *keys, void *stream)
+{
+    if (!b || b->n < 1 || b->n > BC250_BATCH_MAX || b->nmb_total <= 0 || b->lambda8 < 0 || b->lambda8 > 15 ||
+        !list || !count || !keys)
+        return -1;
+    hipStream_t st = (hipStream_t)stream;
+    const int kb = (5 * b->nmb_total + 255) / 256;
+    fs_keys_init_kernel<<<kb, 256, 0, st>>>(count, b->nmb_total, keys);
+    switch (b->range) {
+#define BC250_SPLIT(RR)                                                                                               \
+    case RR:                                                                                                          \
+        me_fullsearch_split_batch_kernel<RR><<<FsSplit<RR>::NPART * b->nmb_total, FsSplit<RR>::NTH, 0, st>>>(        \
+            *b, list, count, keys);                                                                                   \
+        break;
+        BC250_SPLIT(16)
+        BC250_SPLIT(32)
+        BC250_SPLIT(64)
+#undef BC250_SPLIT
+    default:
+        return -1;
+    }
+    fs_finish_batch_kernel<<<kb, 256, 0, st>>>(*b, list, count, keys);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

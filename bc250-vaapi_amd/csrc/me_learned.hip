@@ -75,15 +75,16 @@ __global__ void cnn_candidates_kernel(const float *__restrict__ z, int mbw, int 
     candidates_mb(z, mbw, tw, range, m, cand);
 }
 
-// One wave per MB.  The current block is wave-uniform; each candidate's
-// window (edge-replicated like the full search's) is staged in LDS and a
-// lane evaluates positions lane and lane + 64 of the (2W+1)^2 around it.
+// One wave per MB.  The current block is wave-uniform; the windows of all
+// candidates (edge-replicated like the full search's) are staged in LDS at
+// once, then a lane evaluates every 64th of the ncand x (2W+1)^2 positions.
 __device__ __forceinline__ void cand_search_mb(const uint8_t *__restrict__ cur, int cpitch,
                                                const uint8_t *__restrict__ ref, int rpitch, int wc, int hc, int mbw,
                                                int mb, int range, int lambda8, const int2 *__restrict__ cand,
                                                int ncand, uint4 *__restrict__ out16, uint4 *__restrict__ out8,
                                                uint32_t *win)
 {
+    constexpr int NP = (2 * CW + 1) * (2 * CW + 1), WIN = WS * WP;
     const int lane = threadIdx.x;
     const int x0 = (mb % mbw) * 16, y0 = (mb / mbw) * 16;
     const int nc = 2 * range + 1;
@@ -94,54 +95,55 @@ __device__ __forceinline__ void cand_search_mb(const uint8_t *__restrict__ cur, 
 #pragma unroll
         for (int c = 0; c < 4; c++) cw[r][c] = __builtin_amdgcn_readfirstlane(p[c]);
     }
+    int2 cv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) cv[k] = k < ncand ? cand[(size_t)mb * 4 + k] : make_int2(0, 0);
+    // window k's origin: the block at vector (c.x - W, c.y - W)
+    for (int i = lane; i < ncand * WS * (WS / 4); i += 64) {
+        const int k = i / (WS * (WS / 4)), e = i % (WS * (WS / 4));
+        const int r = e / (WS / 4), d = e % (WS / 4);
+        const int2 c = k == 0 ? cv[0] : k == 1 ? cv[1] : k == 2 ? cv[2] : cv[3];
+        const uint8_t *row = ref + (size_t)clampi(y0 + c.y - CW + r, 0, hc - 1) * rpitch;
+        const int gx = x0 + c.x - CW + 4 * d;
+        uint32_t v;
+        if (gx >= 0 && gx + 3 < wc) {
+            v = (uint32_t)row[gx] | ((uint32_t)row[gx + 1] << 8) | ((uint32_t)row[gx + 2] << 16) |
+                ((uint32_t)row[gx + 3] << 24);
+        } else {
+            v = (uint32_t)row[clampi(gx, 0, wc - 1)] | ((uint32_t)row[clampi(gx + 1, 0, wc - 1)] << 8) |
+                ((uint32_t)row[clampi(gx + 2, 0, wc - 1)] << 16) | ((uint32_t)row[clampi(gx + 3, 0, wc - 1)] << 24);
+        }
+        win[k * WIN + r * WP + d] = v;
+    }
+    for (int i = lane; i < ncand * WS; i += 64) win[(i / WS) * WIN + (i % WS) * WP + WS / 4] = 0;  // pad dwords
+    __syncthreads();
     uint32_t b16 = 0xffffffffu, bq[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
-    for (int k = 0; k < ncand; k++) {
-        const int2 c = cand[(size_t)mb * 4 + k];
-        // window origin: the block at vector (c.x - W, c.y - W)
-        const int wx0 = x0 + c.x - CW, wy0 = y0 + c.y - CW;
-        __syncthreads();
-        for (int i = lane; i < WS * (WS / 4); i += 64) {
-            const int r = i / (WS / 4), d = i % (WS / 4);
-            const uint8_t *row = ref + (size_t)clampi(wy0 + r, 0, hc - 1) * rpitch;
-            const int gx = wx0 + 4 * d;
-            uint32_t v;
-            if (gx >= 0 && gx + 3 < wc) {
-                v = (uint32_t)row[gx] | ((uint32_t)row[gx + 1] << 8) | ((uint32_t)row[gx + 2] << 16) |
-                    ((uint32_t)row[gx + 3] << 24);
-            } else {
-                v = (uint32_t)row[clampi(gx, 0, wc - 1)] | ((uint32_t)row[clampi(gx + 1, 0, wc - 1)] << 8) |
-                    ((uint32_t)row[clampi(gx + 2, 0, wc - 1)] << 16) |
-                    ((uint32_t)row[clampi(gx + 3, 0, wc - 1)] << 24);
-            }
-            win[r * WP + d] = v;
+    for (int pi = lane; pi < ncand * NP; pi += 64) {
+        const int k = pi / NP, pp = pi % NP;
+        const int2 c = k == 0 ? cv[0] : k == 1 ? cv[1] : k == 2 ? cv[2] : cv[3];
+        const int ox = pp % (2 * CW + 1), oy = pp / (2 * CW + 1);
+        const int dx = c.x - CW + ox, dy = c.y - CW + oy;
+        if (dx < -range || dx > range || dy < -range || dy > range) continue;
+        const int sh = ox & 3, d0 = ox >> 2;
+        uint32_t q[4] = {0, 0, 0, 0};
+        for (int r = 0; r < 16; r++) {
+            const uint32_t *wr = win + k * WIN + (oy + r) * WP + d0;
+            uint32_t d[5];
+#pragma unroll
+            for (int i = 0; i < 5; i++) d[i] = wr[i];
+            uint32_t s[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) s[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+            const int h = (r >> 3) * 2;
+            q[h] = __builtin_amdgcn_sad_u8(s[1], cw[r][1], __builtin_amdgcn_sad_u8(s[0], cw[r][0], q[h]));
+            q[h + 1] = __builtin_amdgcn_sad_u8(s[3], cw[r][3], __builtin_amdgcn_sad_u8(s[2], cw[r][2], q[h + 1]));
         }
-        if (lane < WS) win[lane * WP + WS / 4] = 0;  // the pad dword read past a row's end
-        __syncthreads();
-        for (int pi = lane; pi < (2 * CW + 1) * (2 * CW + 1); pi += 64) {
-            const int ox = pi % (2 * CW + 1), oy = pi / (2 * CW + 1);
-            const int dx = c.x - CW + ox, dy = c.y - CW + oy;
-            if (dx < -range || dx > range || dy < -range || dy > range) continue;
-            const int sh = ox & 3, d0 = ox >> 2;
-            uint32_t q[4] = {0, 0, 0, 0};
-            for (int r = 0; r < 16; r++) {
-                const uint32_t *wr = win + (oy + r) * WP + d0;
-                uint32_t d[5];
+        const uint32_t l1 = (uint32_t)(iabs(dx) + iabs(dy));
+        const uint32_t idx = (uint32_t)((dy + range) * nc + (dx + range));
+        const uint32_t sad16 = q[0] + q[1] + q[2] + q[3];
+        b16 = min(b16, ((sad16 + 4u * lambda8 * l1) << 15) | idx);
 #pragma unroll
-                for (int i = 0; i < 5; i++) d[i] = wr[i];
-                uint32_t s[4];
-#pragma unroll
-                for (int i = 0; i < 4; i++) s[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
-                const int h = (r >> 3) * 2;
-                q[h] = __builtin_amdgcn_sad_u8(s[1], cw[r][1], __builtin_amdgcn_sad_u8(s[0], cw[r][0], q[h]));
-                q[h + 1] = __builtin_amdgcn_sad_u8(s[3], cw[r][3], __builtin_amdgcn_sad_u8(s[2], cw[r][2], q[h + 1]));
-            }
-            const uint32_t l1 = (uint32_t)(iabs(dx) + iabs(dy));
-            const uint32_t idx = (uint32_t)((dy + range) * nc + (dx + range));
-            const uint32_t sad16 = q[0] + q[1] + q[2] + q[3];
-            b16 = min(b16, ((sad16 + 4u * lambda8 * l1) << 15) | idx);
-#pragma unroll
-            for (int qq = 0; qq < 4; qq++) bq[qq] = min(bq[qq], ((q[qq] + (uint32_t)lambda8 * l1) << 15) | idx);
-        }
+        for (int qq = 0; qq < 4; qq++) bq[qq] = min(bq[qq], ((q[qq] + (uint32_t)lambda8 * l1) << 15) | idx);
     }
     b16 = wave_min_u32(b16);
 #pragma unroll
@@ -164,7 +166,7 @@ __global__ __launch_bounds__(64) void cand_search_kernel(const uint8_t *__restri
                                                          const int2 *__restrict__ cand, int ncand,
                                                          uint4 *__restrict__ out16, uint4 *__restrict__ out8)
 {
-    __shared__ uint32_t win[WS * WP];
+    __shared__ uint32_t win[4 * WS * WP];
     const int mb = xcd_item(blockIdx.x, nmb);
     if (mb >= nmb) return;
     cand_search_mb(cur, cpitch, ref, rpitch, wc, hc, mbw, mb, range, lambda8, cand, ncand, out16, out8, win);
@@ -227,7 +229,7 @@ __global__ void cnn_candidates_batch_kernel(const bc250_me_batch_t b, const floa
 
 __global__ __launch_bounds__(64) void cand_search_batch_kernel(const bc250_me_batch_t b, int ncand)
 {
-    __shared__ uint32_t win[WS * WP];
+    __shared__ uint32_t win[4 * WS * WP];
     const int g = xcd_item(blockIdx.x, b.nmb_total);
     if (g >= b.nmb_total) return;
     const int f = batch_frame_of_mb(b, g);
@@ -302,6 +304,7 @@ struct LearnedWork {
     uint16_t *a0;
     float *pool, *mvin, *z;
     int *list, *count;
+    uint32_t *keys;
     size_t bytes;
 };
 size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -317,6 +320,7 @@ LearnedWork carve_learned(void *base, int nt, int nmb)
     w.z = (float *)take((size_t)nt * 16 * BC250_CNN_NOUT * 4);
     w.list = (int *)take((size_t)nmb * 4);
     w.count = (int *)take(4);
+    w.keys = (uint32_t *)take((size_t)nmb * 5 * 4);
     w.bytes = o;
     return w;
 }
@@ -362,6 +366,6 @@ extern "C" int bc250_learned_search_batch(const bc250_me_batch_t *b, const float
                                                                                  w.count);
         r = hipGetLastError() == hipSuccess ? 0 : -1;
     }
-    if (r == 0) r = bc250_launch_me_fullsearch_batch(b, w.list, w.count, st);
+    if (r == 0) r = bc250_launch_me_fullsearch_batch_split(b, w.list, w.count, w.keys, st);
     return r;
 }

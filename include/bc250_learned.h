@@ -192,6 +192,10 @@ static __device__ __forceinline__ int batch_frame_of_tile(const bc250_me_batch_t
 /* The plain full search of every MB of the batch (list / count: only the
  * listed batch-wide MBs, count read on the device). */
 int bc250_launch_me_fullsearch_batch(const bc250_me_batch_t *b, const int *list, const int *count, void *stream);
+/* The listed MBs' search split over several workgroups per MB (keys: 5 *
+ * nmb_total uint32 of work space); the same records. */
+int bc250_launch_me_fullsearch_batch_split(const bc250_me_batch_t *b, const int *list, const int *count,
+                                           uint32_t *keys, void *stream);
 
 /* The learned search of every picture of the batch, one launch per stage:
  * input tiles, the four layers, block sums, neighbour inputs, head,

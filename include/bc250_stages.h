@@ -34,6 +34,12 @@ int bc250_launch_me_fullsearch(const uint8_t *cur, int cpitch, const uint8_t *re
 int bc250_launch_me_fullsearch_list(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
                                     int range, int lambda8, uint32_t *out16, uint32_t *out8, const int *list,
                                     const int *count, void *stream);
+/* The same records for a short list: each listed MB's candidate rows are
+ * split over several workgroups whose keys meet in keys (5 * MBs of the
+ * picture uint32 of work space), so a few hundred MBs still fill the GPU. */
+int bc250_launch_me_fullsearch_list_split(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
+                                          int hc, int range, int lambda8, uint32_t *out16, uint32_t *out8,
+                                          const int *list, const int *count, uint32_t *keys, void *stream);
 
 /* Half- then quarter-pel refinement of the integer winners
  * (shaders/motion_estimation.comp:376-469). do8 = also the 8x8 quadrants. */

@@ -290,6 +290,44 @@ def test_fallback_select_and_list_search(pkg):
             assert g16[m]["dx"] == -7
 
 
+def test_split_list_search_equals_full_search(pkg):
+    """The fallback's split search (each listed MB's candidate rows over
+    several workgroups, keys met with atomicMin) writes the oracle full
+    search's 16x16 and 8x8 records for exactly the listed MBs, at every
+    range, for lists in any order, an empty list and the whole picture."""
+    import torch
+    rs = np.random.default_rng(29)
+    mbw, mbh = 12, 8
+    nmb, w, h = mbw * mbh, mbw * 16, mbh * 16
+    ref = rs.integers(0, 256, (h, w)).astype(np.uint8)
+    ref[40:90, 50:140] = 128 + (np.arange(50, 140)[None, :] % 9) * 3   # a periodic patch: ties
+    cur = np.roll(ref, (3, -5), axis=(0, 1))
+    dcur, dref = torch.from_numpy(cur).cuda(), torch.from_numpy(ref).cuda()
+    keys = torch.zeros(nmb * 5, dtype=torch.int32, device="cuda")
+    P = lambda t: C.c_void_p(t.data_ptr())
+    for R in (16, 32, 64):
+        say(f"learned: split list search, range {R}")
+        i16, i8 = orc.me_fullsearch(cur, ref, R, 1)
+        sub = [int(m) for m in rs.permutation(nmb)[:nmb // 3]] + [0, mbw - 1, nmb - mbw, nmb - 1]
+        sub = list(dict.fromkeys(sub))
+        for lst_py in (sub, [], list(range(nmb))):
+            lst = torch.tensor(lst_py + [-1] * (nmb - len(lst_py)), dtype=torch.int32, device="cuda")
+            cnt = torch.tensor([len(lst_py)], dtype=torch.int32, device="cuda")
+            o16 = torch.full((nmb * 4,), -7, dtype=torch.int32, device="cuda")
+            o8 = torch.full((nmb * 16,), -7, dtype=torch.int32, device="cuda")
+            assert pkg.hip.bc250_launch_me_fullsearch_list_split(P(dcur), w, P(dref), w, w, h, R, 1, P(o16), P(o8),
+                                                                 P(lst), P(cnt), P(keys), None) == 0
+            g16 = o16.cpu().numpy().view(orc.IMV)
+            g8 = o8.cpu().numpy().view(orc.IMV).reshape(nmb, 4)
+            on = set(lst_py)
+            for m in range(nmb):
+                if m in on:
+                    assert g16[m] == i16[m], (R, m)
+                    assert (g8[m] == i8.reshape(nmb, 4)[m]).all(), (R, m)
+                else:
+                    assert g16[m]["dx"] == -7 and (g8[m]["dx"] == -7).all()
+
+
 def _fixed_candidate_params(learned, cands):
     """Parameters whose head ignores its input: every block's hypotheses are
     `cands` (pel) and the confidences fall in their listed order."""
