@@ -59,7 +59,9 @@ size_t bc250_rbsp_to_ebsp(uint8_t *dst, size_t cap, const uint8_t *src, size_t n
  * syntax of 7.3.2.1.1 / 7.3.2.2 that its bs_write_sps / bs_write_pps emit
  * (bitstream.c:301-417): constraint flags 0, no scaling lists, VUI with an
  * extended SAR and optional timing, one slice group.  Written as Annex B NAL
- * units (nal_ref_idc 3); return the bytes written, 0 when cap is too small. */
+ * units (nal_ref_idc 3); return the bytes written, 0 when cap is too small
+ * or a field is outside its range in 7.4.2.1.1 / 7.4.2.2 (pic_order_cnt_type 1,
+ * whose offset cycle is not written, included). */
 typedef struct bc250_h264_sps {
     int32_t profile_idc, level_idc, sps_id, chroma_format_idc, bit_depth_luma, bit_depth_chroma;
     int32_t log2_max_frame_num_minus4, pic_order_cnt_type, log2_max_poc_lsb_minus4, max_num_ref_frames;

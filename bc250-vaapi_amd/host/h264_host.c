@@ -261,9 +261,53 @@ static size_t ps_nal(const ps_list *l, int nal_type, uint8_t *dst, size_t cap)
     return 5 + bc250_rbsp_to_ebsp(dst + 5, cap - 5, rbsp, n);
 }
 
+static int in_range(int32_t v, int32_t lo, int32_t hi) { return v >= lo && v <= hi; }
+
+/* The value ranges of 7.4.2.1.1 for the fields this writer emits (and
+ * pic_order_cnt_type 1, whose offsets it does not write, refused). */
+static int sps_fields_ok(const bc250_h264_sps_t *p)
+{
+    if (!in_range(p->profile_idc, 0, 255) || !in_range(p->level_idc, 0, 255) || !in_range(p->sps_id, 0, 31)) return 0;
+    const int cf = p->profile_idc == 100 ? p->chroma_format_idc : 1;
+    if (p->profile_idc == 100 &&
+        (!in_range(cf, 0, 3) || !in_range(p->bit_depth_luma, 8, 14) || !in_range(p->bit_depth_chroma, 8, 14)))
+        return 0;
+    if (!in_range(p->log2_max_frame_num_minus4, 0, 12) || !in_range(p->pic_order_cnt_type, 0, 2) ||
+        p->pic_order_cnt_type == 1)
+        return 0;
+    if (p->pic_order_cnt_type == 0 && !in_range(p->log2_max_poc_lsb_minus4, 0, 12)) return 0;
+    if (!in_range(p->max_num_ref_frames, 0, 16) || !in_range(p->width_in_mbs, 1, 1 << 16) ||
+        !in_range(p->height_in_mbs, 1, 1 << 16))
+        return 0;
+    if (p->frame_cropping) {
+        /* CropUnitX / CropUnitY of 7.4.2.1.1 (equations 7-19 .. 7-22) */
+        const int fm = p->frame_mbs_only ? 1 : 2;
+        const int ux = cf == 1 || cf == 2 ? 2 : 1, uy = (cf == 1 ? 2 : 1) * fm;
+        if (p->crop_left < 0 || p->crop_right < 0 || p->crop_top < 0 || p->crop_bottom < 0) return 0;
+        if ((int64_t)ux * ((int64_t)p->crop_left + p->crop_right) >= 16 * (int64_t)p->width_in_mbs ||
+            (int64_t)uy * ((int64_t)p->crop_top + p->crop_bottom) >= 16 * (int64_t)fm * p->height_in_mbs)
+            return 0;
+    }
+    if (p->vui_present) {
+        if (!in_range(p->sar_width, 0, 65535) || !in_range(p->sar_height, 0, 65535)) return 0;
+        if (p->timing_info_present && (p->num_units_in_tick <= 0 || p->time_scale <= 0)) return 0;
+    }
+    return 1;
+}
+
+/* The value ranges of 7.4.2.2 (pic_init_qp_minus26 down to -(26 + 36), the
+ * deepest QpBdOffsetY). */
+static int pps_fields_ok(const bc250_h264_pps_t *p)
+{
+    return in_range(p->pps_id, 0, 255) && in_range(p->sps_id, 0, 31) && in_range(p->num_ref_idx_l0_default, 0, 31) &&
+           in_range(p->num_ref_idx_l1_default, 0, 31) && in_range(p->weighted_bipred_idc, 0, 2) &&
+           in_range(p->pic_init_qp_minus26, -62, 25) && in_range(p->chroma_qp_offset, -12, 12) &&
+           (!p->transform_8x8_mode || in_range(p->second_chroma_qp_offset, -12, 12));
+}
+
 size_t bc250_h264_write_sps(const bc250_h264_sps_t *p, uint8_t *dst, size_t cap)
 {
-    if (!p || !dst) return 0;
+    if (!p || !dst || !sps_fields_ok(p)) return 0;
     ps_list l = {.n = 0};
     el(&l, EL_U, 8, p->profile_idc);
     el(&l, EL_U, 8, 0); /* constraint_set0..5_flag, reserved_zero_2bits */
@@ -312,7 +356,7 @@ size_t bc250_h264_write_sps(const bc250_h264_sps_t *p, uint8_t *dst, size_t cap)
 
 size_t bc250_h264_write_pps(const bc250_h264_pps_t *p, uint8_t *dst, size_t cap)
 {
-    if (!p || !dst) return 0;
+    if (!p || !dst || !pps_fields_ok(p)) return 0;
     ps_list l = {.n = 0};
     el(&l, EL_UE, 0, p->pps_id);
     el(&l, EL_UE, 0, p->sps_id);

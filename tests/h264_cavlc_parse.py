@@ -144,6 +144,75 @@ def residual_block(bs, nc, maxc):
     return coef, tc
 
 
+def parse_sps(rbsp):
+    """seq_parameter_set_data (7.3.2.1.1) for the subset the writer emits:
+    no scaling matrices, pic_order_cnt_type 0 or 2, VUI with an aspect
+    ratio and optional timing."""
+    bs = Bits(rbsp)
+    d = {"profile_idc": bs.u(8), "constraint_flags": bs.u(8), "level_idc": bs.u(8), "sps_id": bs.ue()}
+    if d["profile_idc"] in (100, 110, 122, 244, 44, 83, 86, 118, 128):
+        d["chroma_format_idc"] = bs.ue()
+        if d["chroma_format_idc"] == 3:
+            d["separate_colour_plane"] = bs.u(1)
+        d["bit_depth_luma"] = bs.ue() + 8
+        d["bit_depth_chroma"] = bs.ue() + 8
+        d["transform_bypass"] = bs.u(1)
+        if bs.u(1):
+            raise ValueError("seq_scaling_matrix_present: not emitted by the writer")
+    else:
+        d["chroma_format_idc"], d["bit_depth_luma"], d["bit_depth_chroma"] = 1, 8, 8
+    d["log2_max_frame_num_minus4"] = bs.ue()
+    d["pic_order_cnt_type"] = bs.ue()
+    if d["pic_order_cnt_type"] == 0:
+        d["log2_max_poc_lsb_minus4"] = bs.ue()
+    elif d["pic_order_cnt_type"] == 1:
+        raise ValueError("pic_order_cnt_type 1: not emitted by the writer")
+    d["max_num_ref_frames"] = bs.ue()
+    d["gaps_allowed"] = bs.u(1)
+    d["width_in_mbs"] = bs.ue() + 1
+    d["height_in_map_units"] = bs.ue() + 1
+    d["frame_mbs_only"] = bs.u(1)
+    if not d["frame_mbs_only"]:
+        d["mb_adaptive_frame_field"] = bs.u(1)
+    d["direct_8x8_inference"] = bs.u(1)
+    d["crop"] = [0, 0, 0, 0]
+    if bs.u(1):
+        d["crop"] = [bs.ue() for _ in range(4)]
+    d["vui_present"] = bs.u(1)
+    if d["vui_present"]:
+        if bs.u(1):
+            idc = bs.u(8)
+            d["sar"] = (bs.u(16), bs.u(16)) if idc == 255 else idc
+        if bs.u(1) or bs.u(1) or bs.u(1):
+            raise ValueError("overscan / video signal / chroma location info: not emitted by the writer")
+        if bs.u(1):
+            d["num_units_in_tick"], d["time_scale"], d["fixed_frame_rate"] = bs.u(32), bs.u(32), bs.u(1)
+        if bs.u(1) or bs.u(1) or bs.u(1) or bs.u(1):
+            raise ValueError("HRD / pic_struct / bitstream restriction: not emitted by the writer")
+    if bs.u(1) != 1:
+        raise ValueError("SPS: no rbsp stop bit")
+    return d
+
+
+def parse_pps(rbsp):
+    """pic_parameter_set_rbsp (7.3.2.2), one slice group."""
+    bs = Bits(rbsp)
+    d = {"pps_id": bs.ue(), "sps_id": bs.ue(), "entropy_coding_mode": bs.u(1), "pic_order_present": bs.u(1)}
+    if bs.ue() != 0:
+        raise ValueError("slice groups: not emitted by the writer")
+    d["num_ref_idx_l0_default_minus1"] = bs.ue()
+    d["num_ref_idx_l1_default_minus1"] = bs.ue()
+    d["weighted_pred"] = bs.u(1)
+    d["weighted_bipred_idc"] = bs.u(2)
+    d["pic_init_qp_minus26"] = bs.se()
+    d["pic_init_qs_minus26"] = bs.se()
+    d["chroma_qp_offset"] = bs.se()
+    d["deblocking_filter_control"] = bs.u(1)
+    d["constrained_intra_pred"] = bs.u(1)
+    d["redundant_pic_cnt_present"] = bs.u(1)
+    return d
+
+
 def _median(a, b, c):
     return sorted((a, b, c))[1]
 

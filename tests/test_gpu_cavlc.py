@@ -307,7 +307,7 @@ def test_pipeline_annexb_stream_decodes(pkg):
     import os
     if not os.path.exists(orc.REF_H264):
         pytest.skip("oracle/_ref not built")
-    from h264_cavlc_parse import SliceParser
+    from h264_cavlc_parse import SliceParser, parse_pps, parse_sps
     from h264_decode import decode_frame
     from test_nal import unescape
     w, h, qp, ns = 320, 192, 28, 2
@@ -335,12 +335,25 @@ def test_pipeline_annexb_stream_decodes(pkg):
         g.close()
     nals = stream.split(b"\x00\x00\x00\x01")[1:]
     assert [n[0] & 31 for n in nals] == [7, 8] + [5] * ns + [1] * (4 * ns)
+    # the parameter sets agree with the encoder's settings, and the slices
+    # are parsed and decoded with what they say
+    sps, pps = parse_sps(unescape(nals[0][1:])), parse_pps(unescape(nals[1][1:]))
+    assert (sps["width_in_mbs"], sps["height_in_map_units"], sps["frame_mbs_only"]) == (wmb, hmb, 1)
+    assert (sps["log2_max_frame_num_minus4"], sps["pic_order_cnt_type"], sps["log2_max_poc_lsb_minus4"]) == (4, 0, 4)
+    assert (sps["profile_idc"], sps["level_idc"], sps["time_scale"], sps["sar"]) == (66, 40, 60, (1, 1))
+    assert (pps["sps_id"], pps["entropy_coding_mode"], pps["pic_init_qp_minus26"]) == (sps["sps_id"], 0, qp - 26)
+    assert pps["deblocking_filter_control"] == 1
     ry = ruv = None
     for i in range(5):
-        ps = SliceParser(wmb, hmb, ns, 4, 4)
+        ps = SliceParser(sps["width_in_mbs"], sps["height_in_map_units"], ns, sps["log2_max_frame_num_minus4"],
+                         sps["log2_max_poc_lsb_minus4"])
         for s in range(ns):
             ps.parse(s, unescape(nals[2 + i * ns + s][1:]))
-        dy, duv = decode_frame(ps, qp, ry, ruv)
+        slice_qps = {26 + pps["pic_init_qp_minus26"] + hd["slice_qp_delta"] for hd in ps.headers}
+        assert slice_qps == {qp}, slice_qps
+        assert all(hd["pps_id"] == pps["pps_id"] for hd in ps.headers)
+        assert all(hd["frame_num"] == i and hd["poc_lsb"] == 2 * i for hd in ps.headers)
+        dy, duv = decode_frame(ps, slice_qps.pop(), ry, ruv)
         np.testing.assert_array_equal(dy, recons[i][0], err_msg=f"frame {i} luma")
         np.testing.assert_array_equal(duv, recons[i][1], err_msg=f"frame {i} chroma")
         ry, ruv = recons[i]

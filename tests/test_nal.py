@@ -149,3 +149,50 @@ def test_parameter_sets_match_reference(pkg):
                 assert n1 == n2 and (a[:n1] == b[:n2]).all(), (cabac, qp, t8)
                 cases += 1
     assert cases == 76
+
+
+def _ps_len(pkg, fields, vals, which):
+    f = pkg.host.bc250_h264_write_sps if which == "sps" else pkg.host.bc250_h264_write_pps
+    f.restype = C.c_size_t
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    a = np.array([int(vals.get(k, 0)) for k in fields], np.int32)
+    buf = np.zeros(512, np.uint8)
+    return f(a.ctypes.data_as(C.c_void_p), buf.ctypes.data_as(C.c_void_p), 512)
+
+
+def test_parameter_sets_refuse_out_of_range_fields(pkg):
+    """Fields outside their 7.4.2.1.1 / 7.4.2.2 ranges give no parameter set
+    (0 bytes) instead of an oversized Exp-Golomb code; the range ends are
+    accepted."""
+    base_sps = dict(profile_idc=100, level_idc=40, sps_id=0, chroma_format_idc=1, bit_depth_luma=8,
+                    bit_depth_chroma=8, log2_max_frame_num_minus4=0, pic_order_cnt_type=0,
+                    log2_max_poc_lsb_minus4=2, max_num_ref_frames=1, width_in_mbs=120, height_in_mbs=68,
+                    frame_mbs_only=1, direct_8x8_inference=1, frame_cropping=1, crop_bottom=4, vui_present=1,
+                    sar_width=1, sar_height=1, timing_info_present=1, num_units_in_tick=1, time_scale=60)
+    assert _ps_len(pkg, pkg.SPS_FIELDS, base_sps, "sps") > 0
+    good = [("sps_id", 31), ("bit_depth_luma", 14), ("log2_max_frame_num_minus4", 12), ("pic_order_cnt_type", 2),
+            ("log2_max_poc_lsb_minus4", 12), ("max_num_ref_frames", 16), ("width_in_mbs", 1), ("crop_bottom", 0),
+            ("sar_width", 65535), ("chroma_format_idc", 3)]
+    bad = [("sps_id", 32), ("sps_id", -1), ("bit_depth_luma", 7), ("bit_depth_chroma", 15),
+           ("chroma_format_idc", 4), ("log2_max_frame_num_minus4", 13), ("log2_max_frame_num_minus4", -1),
+           ("pic_order_cnt_type", 1), ("pic_order_cnt_type", 3), ("log2_max_poc_lsb_minus4", 13),
+           ("max_num_ref_frames", 17), ("width_in_mbs", 0), ("height_in_mbs", 0), ("crop_bottom", -1),
+           ("crop_bottom", 544), ("crop_right", 960), ("sar_width", 65536), ("num_units_in_tick", 0),
+           ("time_scale", -5), ("profile_idc", 256), ("level_idc", -1)]
+    for k, v in good:
+        assert _ps_len(pkg, pkg.SPS_FIELDS, dict(base_sps, **{k: v}), "sps") > 0, (k, v)
+    for k, v in bad:
+        assert _ps_len(pkg, pkg.SPS_FIELDS, dict(base_sps, **{k: v}), "sps") == 0, (k, v)
+    # fields a profile does not write are not checked
+    assert _ps_len(pkg, pkg.SPS_FIELDS, dict(base_sps, profile_idc=66, bit_depth_luma=0, chroma_format_idc=9), "sps") > 0
+    base_pps = dict(pps_id=0, sps_id=0, num_ref_idx_l0_default=0, pic_init_qp_minus26=0, deblocking_filter_control=1,
+                    transform_8x8_mode=1, second_chroma_qp_offset=0)
+    assert _ps_len(pkg, pkg.PPS_FIELDS, base_pps, "pps") > 0
+    for k, v in [("pps_id", 255), ("num_ref_idx_l0_default", 31), ("pic_init_qp_minus26", -26),
+                 ("pic_init_qp_minus26", 25), ("chroma_qp_offset", -12), ("second_chroma_qp_offset", 12),
+                 ("weighted_bipred_idc", 2)]:
+        assert _ps_len(pkg, pkg.PPS_FIELDS, dict(base_pps, **{k: v}), "pps") > 0, (k, v)
+    for k, v in [("pps_id", 256), ("sps_id", 32), ("num_ref_idx_l0_default", 32), ("num_ref_idx_l1_default", -1),
+                 ("weighted_bipred_idc", 3), ("pic_init_qp_minus26", 26), ("pic_init_qp_minus26", -63),
+                 ("chroma_qp_offset", 13), ("second_chroma_qp_offset", -13)]:
+        assert _ps_len(pkg, pkg.PPS_FIELDS, dict(base_pps, **{k: v}), "pps") == 0, (k, v)
