@@ -358,6 +358,8 @@ class NalSink(C.Structure):
 
 
 NAL_SINK_FRAME = C.cast(host.bc250_nal_sink_frame, C.c_void_p)
+_sig(host, "bc250_encode_sequence_rc", _I, [_CTX, C.POINTER(GpuImage), _I, _I, _I, C.POINTER(SeqCfg), _P, _P, _P,
+                                           _P, C.POINTER(C.c_double), C.POINTER(C.c_double)])
 _sig(host, "h264_intra16_luma_dc_transform", None, [_P, _I, _P, _P])
 _sig(host, "h264_chroma_dc_levels", None, [_P, _I, _P])
 _sig(host, "h264_frame_dc_levels", None, [_P, _I, _I, _I, _P, _P])
@@ -366,6 +368,61 @@ _sig(host, "bc250_rbsp_to_ebsp", C.c_size_t, [_P, C.c_size_t, _P, C.c_size_t])
 _sig(host, "bc250_h264_frame_nals", C.c_size_t, [_P, _P, _I, _I, _P, C.c_size_t])
 _sig(host, "bc250_h264_write_sps", C.c_size_t, [_P, _P, C.c_size_t])
 _sig(host, "bc250_h264_write_pps", C.c_size_t, [_P, _P, C.c_size_t])
+
+
+class RcState(C.Structure):
+    """bc250_rc_t (host/bc250_rc.h)."""
+    _fields_ = [("mode", C.c_int32), ("target_bitrate", C.c_uint32), ("max_bitrate", C.c_uint32),
+                ("qp_min", C.c_int32), ("qp_max", C.c_int32), ("framerate", C.c_double),
+                ("target_bits_per_frame", C.c_uint32), ("buffer_fullness", C.c_int64), ("buffer_size", C.c_int64),
+                ("base_qp", C.c_int32), ("current_qp", C.c_int32), ("prev_frame_sad", C.c_uint64),
+                ("error_integral", C.c_int64), ("max_frame_bits", C.c_uint32), ("quality_level", C.c_uint32),
+                ("wallclock_drain", C.c_int32), ("last_frame_ns", C.c_uint64), ("measured_fps", C.c_double),
+                ("model", C.c_int32), ("cplx", C.c_double * 2), ("debt", C.c_double), ("model_last_p_qp", C.c_int32),
+                ("cplx_still", C.c_double), ("still_last_qp", C.c_int32), ("still_share", C.c_double),
+                ("pixels", C.c_double)]
+
+
+RC_CBR, RC_VBR, RC_LOW_LATENCY, RC_CQP = 0, 1, 2, 3
+_sig(host, "bc250_rc_init", None, [_P, _I, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32])
+_sig(host, "bc250_rc_update_bitrate", None, [_P, C.c_uint32, C.c_uint32, C.c_uint32])
+_sig(host, "bc250_rc_get_frame_qp", _I, [_P, C.c_uint64])
+_sig(host, "bc250_rc_update_stats", None, [_P, _I])
+_sig(host, "bc250_rc_model_frame_qp", _I, [_P, _I, _I])
+_sig(host, "bc250_rc_model_frame_coded", None, [_P, _I, _I, _I, _I])
+_sig(host, "bc250_rc_set_max_frame_size", None, [_P, C.c_uint32])
+_sig(host, "bc250_rc_set_quality_level", None, [_P, C.c_uint32])
+_sig(host, "bc250_rc_set_wallclock_drain", None, [_P, _I])
+
+
+class RateControl:
+    """The host rate controller (host/bc250_rc.c): the reference's
+    rate_control.c interface, method for function."""
+
+    def __init__(self, mode, bitrate, fps, width, height, model=False, current_qp=0):
+        self.st = RcState()
+        self.st.current_qp = current_qp   # the QP a CQP controller keeps
+        host.bc250_rc_init(C.byref(self.st), mode, bitrate, fps, width, height)
+        self.st.model = int(bool(model))
+
+    def frame_qp(self, est_sad=0, intra=False, still=False):
+        if self.st.model:
+            return host.bc250_rc_model_frame_qp(C.byref(self.st), int(intra), int(still))
+        return host.bc250_rc_get_frame_qp(C.byref(self.st), est_sad)
+
+    def coded(self, bits, qp=0, intra=False, still=False):
+        host.bc250_rc_update_stats(C.byref(self.st), int(bits))
+        if self.st.model:
+            host.bc250_rc_model_frame_coded(C.byref(self.st), int(intra), int(still), qp, int(bits))
+
+    def update_bitrate(self, bitrate, width, height):
+        host.bc250_rc_update_bitrate(C.byref(self.st), bitrate, width, height)
+
+    def set_max_frame_size(self, bits):
+        host.bc250_rc_set_max_frame_size(C.byref(self.st), bits)
+
+    def set_quality_level(self, level):
+        host.bc250_rc_set_quality_level(C.byref(self.st), level)
 
 # bc250_h264_sps_t / bc250_h264_pps_t, field by field (all int32)
 SPS_FIELDS = ("profile_idc", "level_idc", "sps_id", "chroma_format_idc", "bit_depth_luma", "bit_depth_chroma",
@@ -784,6 +841,27 @@ class Encoder:
             raise RuntimeError("encode_sequence failed" + (f": {bad[0]}" if bad else ""))
         out = np.array(fms[:], dtype=np.float64), tot.value
         return out + (got,) if collect else out
+
+    def encode_sequence_rc(self, images, rc, gop=0, num_slices=1, pipelined=1, cap=None):
+        """Rate-controlled H.264 sequence (bc250_encode_sequence_rc; the
+        encoder needs entropy 1): rc is a RateControl.  Returns (stream bytes,
+        per-frame QPs, per-frame bits, frame_ms, total_ms)."""
+        if not self.entropy:
+            raise ValueError("rate control needs the CAVLC slices (entropy=1)")
+        n = len(images)
+        arr = (GpuImage * n)(*[im for im, _ in images])
+        cfg = SeqCfg(0, gop, num_slices, pipelined)
+        buf = np.zeros(cap or (self.w * self.h * 2 * n + 4096), np.uint8)
+        fbytes = np.zeros(n, np.uint64)
+        sink = NalSink(C.addressof(self.ctx), buf.ctypes.data, buf.size, 0, fbytes.ctypes.data)
+        qps, bits = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        fms = (C.c_double * n)()
+        tot = C.c_double()
+        r = host.bc250_encode_sequence_rc(C.byref(self.ctx), arr, n, self.w, self.h, C.byref(cfg), C.byref(rc.st),
+                                          C.byref(sink), _ptr(qps), _ptr(bits), fms, C.byref(tot))
+        if r != 0:
+            raise RuntimeError("encode_sequence_rc failed")
+        return buf[:sink.used].tobytes(), qps, bits, np.array(fms[:], np.float64), tot.value
 
     def close(self):
         if self.ctx.hip:

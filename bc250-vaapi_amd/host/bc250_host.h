@@ -8,6 +8,7 @@
 #define BC250_HOST_H
 
 #include <stdint.h>
+#include "bc250_rc.h"
 
 #include "gpu_compute.h"
 
@@ -116,6 +117,19 @@ typedef struct {
     size_t *frame_bytes;
 } bc250_nal_sink_t;
 int bc250_nal_sink_frame(void *user, int frame, int slot, int is_intra);
+
+/* A rate-controlled sequence (H.264, the context created with entropy 1):
+ * each picture's QP comes from rc at its submission (bc250_rc_get_frame_qp,
+ * or bc250_rc_model_frame_qp when rc->model; clamped to [rc->qp_min, 51] as
+ * encoder_h264.c:2265-2269 does), and once it has synced, its Annex B bytes
+ * (written to sink) x 8 go back through bc250_rc_update_stats (and
+ * bc250_rc_model_frame_coded).  Pipelined, a picture's QP sees the pictures
+ * finished before its submission, like the reference's submit / finish
+ * split (encoder_h264.c:2244-2392, 2513-3542).  qp_out[i] (required) and
+ * bits_out[i] (optional) receive each picture's QP and bits. */
+int bc250_encode_sequence_rc(gpu_context_t *ctx, const gpu_image_t *frames, int n, int width, int height,
+                             const bc250_seq_cfg_t *cfg, bc250_rc_t *rc, bc250_nal_sink_t *sink, int *qp_out,
+                             int *bits_out, double *frame_ms, double *total_ms);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,7 @@
 
 #include "bc250_bitstream.h"
 #include "bc250_host.h"
+#include "bc250_rc.h"
 
 static const int DC_LS0[6] = {160, 176, 208, 224, 256, 288};
 
@@ -138,9 +139,12 @@ int bc250_nal_sink_frame(void *user, int frame, int slot, int is_intra)
     return 0;
 }
 
-int bc250_encode_sequence_cb(gpu_context_t *ctx, const gpu_image_t *frames, int n, int width, int height,
-                             const bc250_seq_cfg_t *cfg, double *frame_ms, double *total_ms, bc250_frame_cb on_frame,
-                             void *user)
+/* The driver loop; qp_fn (optional) picks each picture's QP at its
+ * submission, else cfg->qp. */
+typedef int (*seq_qp_fn)(void *user, int frame, int is_intra);
+static int seq_run(gpu_context_t *ctx, const gpu_image_t *frames, int n, int width, int height,
+                   const bc250_seq_cfg_t *cfg, double *frame_ms, double *total_ms, bc250_frame_cb on_frame,
+                   void *user, seq_qp_fn qp_fn, void *qp_user)
 {
     if (!ctx || !frames || n <= 0 || !cfg) return -1;
     int slot_intra[2] = {0, 0};
@@ -152,7 +156,8 @@ int bc250_encode_sequence_cb(gpu_context_t *ctx, const gpu_image_t *frames, int 
         const int is_intra = i == 0 || (cfg->gop > 0 && i % cfg->gop == 0);
         if (gpu_compute_begin_picture(ctx, frames[i]) != 0) { rc = -1; break; }
         const double ts = now_ms();
-        if (gpu_compute_dispatch_encode_ext(ctx, frames[i], width, height, cfg->qp, is_intra, cfg->num_slices, 0,
+        const int qp = qp_fn ? qp_fn(qp_user, i, is_intra) : cfg->qp;
+        if (gpu_compute_dispatch_encode_ext(ctx, frames[i], width, height, qp, is_intra, cfg->num_slices, 0,
                                             NULL) != 0 ||
             gpu_compute_end_picture(ctx) != 0) {
             rc = -1;
@@ -184,6 +189,54 @@ int bc250_encode_sequence_cb(gpu_context_t *ctx, const gpu_image_t *frames, int 
     }
     if (total_ms) *total_ms = now_ms() - t0;
     return rc;
+}
+
+int bc250_encode_sequence_cb(gpu_context_t *ctx, const gpu_image_t *frames, int n, int width, int height,
+                             const bc250_seq_cfg_t *cfg, double *frame_ms, double *total_ms, bc250_frame_cb on_frame,
+                             void *user)
+{
+    return seq_run(ctx, frames, n, width, height, cfg, frame_ms, total_ms, on_frame, user, NULL, NULL);
+}
+
+/* ---- rate-controlled sequences */
+
+typedef struct {
+    bc250_nal_sink_t *sink;
+    bc250_rc_t *rc;
+    int *qp;
+    int *bits;
+} rc_run_t;
+
+static int rc_pick_qp(void *u, int frame, int is_intra)
+{
+    rc_run_t *r = (rc_run_t *)u;
+    int qp = r->rc->model ? bc250_rc_model_frame_qp(r->rc, is_intra, 0) : bc250_rc_get_frame_qp(r->rc, 0);
+    /* the encoder's own clamp (encoder_h264.c:2265-2269) */
+    if (qp < r->rc->qp_min) qp = r->rc->qp_min;
+    if (qp > 51) qp = 51;
+    r->qp[frame] = qp;
+    return qp;
+}
+
+static int rc_frame_done(void *u, int frame, int slot, int is_intra)
+{
+    rc_run_t *r = (rc_run_t *)u;
+    const size_t before = r->sink->used;
+    if (bc250_nal_sink_frame(r->sink, frame, slot, is_intra) != 0) return -1;
+    const int bits = (int)((r->sink->used - before) * 8);
+    if (r->bits) r->bits[frame] = bits;
+    bc250_rc_update_stats(r->rc, bits);
+    if (r->rc->model) bc250_rc_model_frame_coded(r->rc, is_intra, 0, r->qp[frame], bits);
+    return 0;
+}
+
+int bc250_encode_sequence_rc(gpu_context_t *ctx, const gpu_image_t *frames, int n, int width, int height,
+                             const bc250_seq_cfg_t *cfg, bc250_rc_t *rc, bc250_nal_sink_t *sink, int *qp_out,
+                             int *bits_out, double *frame_ms, double *total_ms)
+{
+    if (!rc || !sink || !qp_out) return -1;
+    rc_run_t r = {sink, rc, qp_out, bits_out};
+    return seq_run(ctx, frames, n, width, height, cfg, frame_ms, total_ms, rc_frame_done, &r, rc_pick_qp, &r);
 }
 
 size_t bc250_rbsp_to_ebsp(uint8_t *dst, size_t cap, const uint8_t *src, size_t n)
