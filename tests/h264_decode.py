@@ -1,8 +1,9 @@
 """Decodes a frame parsed by h264_cavlc_parse.SliceParser with the reference
 decoder's own prediction, motion compensation, DC transforms and inverse
 transforms (oracle/_ref/libref_h264.so, built from src/h264_pred.c,
-h264_mc.c, h264_recon.c) -- test helper, not a test module.  No loop filter:
-compare with reconstructions coded with deblocking off (idc 1)."""
+h264_mc.c, h264_recon.c) -- test helper, not a test module.  decode_frame
+stops before the loop filter; loop_filter then applies clause 8.7
+(tests/h264_loopfilter.py) with the parsed slices' settings."""
 import ctypes as C
 
 import numpy as np
@@ -78,4 +79,32 @@ def decode_frame(ps, qp, ref_y=None, ref_uv=None):
                 duv[mby * 8:mby * 8 + 8, x0 + comp:x0 + 16:2] = cp
             _decode_mb_residual(lib, dy, duv, mbx, mby, lv, qp, False, None)
         _chroma_from_levels(lib, duv, mbx, mby, lv, ps.cdc[m], qp)
+    return dy, duv
+
+
+def loop_filter(ps, qp, dy, duv):
+    """Clause 8.7 over a decoded frame in place (dy coded-size luma, duv NV12
+    chroma): per MB intra or not, QP_Y = qp, the non-zero 4x4 luma blocks
+    from the parsed TotalCoeff (blkIdx order -> raster), one vector per
+    inter MB, one reference picture; each slice's
+    disable_deblocking_filter_idc and offsets (slice_alpha_c0_offset_div2,
+    slice_beta_offset_div2, doubled) from its header."""
+    import h264_loopfilter as lf
+    n = ps.wmb * ps.hmb
+    nnz = []
+    for m in range(n):
+        r = [0] * 16
+        for k in range(16):
+            x, y = (k & 1) | (k & 4) >> 1, (k & 2) >> 1 | (k & 8) >> 2
+            r[y * 4 + x] = int(ps.tcl[m][k])
+        nnz.append(r)
+    intra = [ps.kind[m] == "I" for m in range(n)]
+    mv = [[(0, 0) if intra[m] else (int(ps.mv[m][0]), int(ps.mv[m][1]))] * 16 for m in range(n)]
+    hd = ps.headers
+    pic = lf.Picture(ps.wmb, ps.hmb, intra=intra, qp=[qp] * n, nnz=nnz, mv=mv, ref=[0] * n,
+                     slice_of=[int(v) for v in ps.slice_of], idc=[h["deblock_idc"] for h in hd],
+                     off_a=[2 * h.get("alpha", 0) for h in hd], off_b=[2 * h.get("beta", 0) for h in hd])
+    cb, cr = np.ascontiguousarray(duv[:, 0::2]), np.ascontiguousarray(duv[:, 1::2])
+    lf.deblock(pic, dy, cb, cr)
+    duv[:, 0::2], duv[:, 1::2] = cb, cr
     return dy, duv

@@ -299,6 +299,48 @@ def test_pipeline_slices_decode_to_recon(pkg):
         g.close()
 
 
+@pytest.mark.parametrize("qp,idc", [(24, 0), (36, 0), (40, 2)])
+def test_pipeline_slices_decode_to_recon_with_loop_filter(pkg, qp, idc):
+    """Deblocking on (idc 0: across slice edges; 2: not): the pipeline's
+    slices parsed, decoded with the reference decoder's primitives and
+    filtered by a literal reading of clause 8.7 give the pipeline's
+    (GPU-deblocked) reconstruction exactly, IPPPP with two slices; the
+    filter moves samples, so the check is not vacuous."""
+    import os
+    if not os.path.exists(orc.REF_H264):
+        pytest.skip("oracle/_ref not built")
+    from h264_cavlc_parse import SliceParser
+    from h264_decode import decode_frame, loop_filter
+    w, h, ns = 320, 192, 2
+    say(f"decode == recon with the loop filter, {w}x{h}, QP {qp}, idc {idc}")
+    g = pkg.FrameGen(31, w, h, 4)
+    enc = pkg.Encoder(w, h, search_range=32, lambda8=1, deblock_idc=idc, readback=0, entropy=1)
+    try:
+        ry = ruv = None
+        moved = 0
+        for i in range(5):
+            im = enc.new_image()
+            enc.upload(im, *g.frame(i))
+            slot = enc.encode(im, qp, i == 0, ns)
+            ps = SliceParser(w // 16, h // 16, ns)
+            for s, rb in enumerate(enc.slices(slot)):
+                ps.parse(s, rb)
+            assert {hd["deblock_idc"] for hd in ps.headers} == {idc}
+            dy, duv = decode_frame(ps, qp, ry, ruv)
+            uy = dy.copy()
+            loop_filter(ps, qp, dy, duv)
+            moved += int(np.sum(uy != dy))
+            gy, guv = enc.map_recon()
+            np.testing.assert_array_equal(dy, gy, err_msg=f"frame {i} luma")
+            np.testing.assert_array_equal(duv, guv, err_msg=f"frame {i} chroma")
+            ry, ruv = gy, guv
+        assert moved > 1000, moved
+        say(f"  5 frames decode to the deblocked reconstruction ({moved} luma samples moved by the filter)")
+    finally:
+        enc.close()
+        g.close()
+
+
 def test_pipeline_annexb_stream_decodes(pkg):
     """SPS + PPS + the pipeline's slice NAL units of an IPPPP sequence form
     one Annex B stream; split at its start codes, un-escaped, parsed and
