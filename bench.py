@@ -88,6 +88,55 @@ def copy_calibration(pkg, torch, nbytes=64 << 20, reps=3):
     return nbytes
 
 
+def _cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline_hevc(pkg, args):
+    """The repo's CPU encoder (oracle/orc_baseline.c: the oracle's HEVC
+    stages, AVX2 integer search, search and refine over OpenMP threads) on
+    the host cores, timed per picture: the 640x480 CPU configuration (HEVC
+    Main 8-bit, IPPP, CQP 32, 4 slices, +-32, 6 moving rectangles) and the
+    1080p headline shape (+-64, CQP of the headline), each on all the
+    threads the box gives this process (OMP_NUM_THREADS, else the CPU count,
+    at most 16) and on one thread.  value: 1080p fps on all threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    model, ncpu = _cpu_info()
+    nt = max(1, min(int(os.environ.get("OMP_NUM_THREADS", ncpu or 1)), ncpu or 1, 16))
+
+    def seq(w, h, nrect, n, rng, qp, threads):
+        g = pkg.FrameGen(480 if w == 640 else 1080, w, h, nrect)
+        fr = [pkg.pad_coded(*g.frame(i)) for i in range(n)]
+        g.close()
+        ms, _, _ = orc.baseline_hevc_seq(np.stack([f[0] for f in fr]), np.stack([f[1] for f in fr]), qp, rng,
+                                         args.lambda8, 4, threads)
+        p = ms[1:]   # the P pictures
+        return {"threads": threads, "frames": n, "fps": round(n / (ms.sum() / 1e3), 3),
+                "mpixel_per_s": round(n * w * h / 1e6 / (ms.sum() / 1e3), 2),
+                "p50_frame_ms": round(float(np.percentile(p, 50)), 2),
+                "p99_frame_ms": round(float(np.percentile(p, 99)), 2), "i_frame_ms": round(float(ms[0]), 2)}
+
+    log(f"cpu baseline: {model}, {ncpu} CPUs, {nt} threads")
+    small = {"all": seq(640, 480, 6, 32, 32, 32, nt), "one": seq(640, 480, 6, 8, 32, 32, 1)}
+    big = {"all": seq(W, H, 12, 4, args.range, args.qp, nt), "one": seq(W, H, 12, 2, args.range, args.qp, 1)}
+    return {"value": big["all"]["fps"], "unit": "fps", "cores": nt, "kind": "port",
+            "sample": f"the repo's CPU encoder (oracle stages, AVX2 +-{args.range} search over {nt} OpenMP threads) "
+                      f"on 4 pictures (I + 3 P) of the 1080p sequence; 640x480 CPU configuration and 1-thread runs "
+                      f"beside it",
+            "cpu_model": model, "nproc": ncpu, "avx2": orc.baseline_has_avx2(),
+            "hevc_640x480_ippp_cqp32_4slices_r32": small, "hevc_1080p": big}
+
+
 def cpu_baseline(frames_coded, qp, rng, lam, budget_s, codec):
     """Oracle (single thread) on a bounded sample: the integer search of a
     spread sample of macroblocks, scaled to the frame, plus the rest of a
@@ -746,7 +795,10 @@ def main():
         checked = verify_pipelines(pkg, frames, args, pipes)
         log("cpu baseline (oracle)")
         coded = [pkg.pad_coded(*frames[args.warmup]), pkg.pad_coded(*frames[args.warmup + 1])]
-        res["cpu_baseline"] = cpu_baseline(coded, args.qp, args.range, args.lambda8, args.cpu_budget, args.codec)
+        if args.codec == "hevc":
+            res["cpu_baseline"] = cpu_baseline_hevc(pkg, args)
+        else:
+            res["cpu_baseline"] = cpu_baseline(coded, args.qp, args.range, args.lambda8, args.cpu_budget, args.codec)
         res["checked_against_oracle"] = checked
     else:
         res["cpu_baseline"] = None

@@ -600,3 +600,50 @@ def hevc_deblock(y, uv, qp, is_intra, cbf, mv4, num_slices=1, bd=8, ctu_log2=4, 
     lib.orc_hevc_deblock_ctu(p(y), p(uv), wc, wc, wc, hc, qp, int(is_intra), p(cbf), p(mv4), num_slices,
                              ctu_log2, bd, None if cu is None else p(cu))
     return y, uv
+
+
+# ---- the CPU baseline encoder (orc_baseline.c, liborc_mt.so): the oracle's
+# stages with an AVX2 search over OpenMP threads; measurement, not a checker
+LIB_MT = os.path.join(_HERE, "liborc_mt.so")
+_mt = None
+
+
+def _mtlib():
+    global _mt
+    if _mt is None:
+        _mt = C.CDLL(LIB_MT)
+        _mt.orc_baseline_has_avx2.restype = C.c_int
+        _mt.orc_baseline_fullsearch.argtypes = [_P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P]
+        _mt.orc_baseline_fullsearch.restype = None
+        _mt.orc_baseline_hevc_seq.argtypes = [_P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P]
+        _mt.orc_baseline_hevc_seq.restype = _I
+    return _mt
+
+
+def baseline_fullsearch(cur, ref, rng, lambda8, nthreads=1, simd=1):
+    cur = np.ascontiguousarray(cur, np.uint8)
+    ref = np.ascontiguousarray(ref, np.uint8)
+    hc, wc = cur.shape
+    nmb = (wc // 16) * (hc // 16)
+    i16, i8 = np.zeros(nmb, IMV), np.zeros(nmb * 4, IMV)
+    _mtlib().orc_baseline_fullsearch(p(cur), p(ref), wc, wc, hc, rng, lambda8, nthreads, simd, p(i16), p(i8))
+    return i16, i8
+
+
+def baseline_hevc_seq(ys, uvs, qp, rng, lambda8, num_slices, nthreads, simd=1):
+    """ys [n, hc, wc], uvs [n, hc/2, wc] NV12 pictures, IPPP; returns
+    (per-picture ms, last reconstruction y, uv)."""
+    ys = np.ascontiguousarray(ys, np.uint8)
+    uvs = np.ascontiguousarray(uvs, np.uint8)
+    n, hc, wc = ys.shape
+    ms = np.zeros(n, np.float64)
+    oy, ouv = np.zeros((hc, wc), np.uint8), np.zeros((hc // 2, wc), np.uint8)
+    rc = _mtlib().orc_baseline_hevc_seq(p(ys), p(uvs), n, wc, hc, qp, rng, lambda8, num_slices, nthreads, simd,
+                                        p(ms), p(oy), p(ouv))
+    if rc != 0:
+        raise MemoryError("baseline buffers")
+    return ms, oy, ouv
+
+
+def baseline_has_avx2():
+    return bool(_mtlib().orc_baseline_has_avx2())
