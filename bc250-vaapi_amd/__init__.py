@@ -124,7 +124,8 @@ _sig(hip, "gpu_compute_unmap_surface", None, [_CTX, GpuMemory, C.c_bool])
 _U = C.c_uint
 _sig(hip, "bc250_launch_me_fullsearch", _I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P, _P])
 _sig(hip, "bc250_launch_me_fullsearch_list", _I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P])
-_sig(hip, "bc250_launch_me_fullsearch_list_split", _I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P])
+_sig(hip, "bc250_launch_me_fullsearch_list_split", _I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _I,
+                                                        _P])
 # the learned proposer (include/bc250_learned.h)
 _F = C.c_float
 _sig(hip, "bc250_cnn_param_count", C.c_size_t, [])
@@ -132,6 +133,7 @@ _sig(hip, "bc250_cnn_input", _I, [_P, _I, _P, _I, C.c_size_t, _I, _I, _P, _I, _P
 _sig(hip, "bc250_cnn_conv", _I, [_I, _I, _P, _P, _P, _P, _P, _P, _I, _P])
 _sig(hip, "bc250_cnn_pool", _I, [_P, _P, _I, _P])
 _sig(hip, "bc250_cnn_forward", _I, [_P, _P, _P, _P, _I, _P])
+_sig(hip, "bc250_cnn_forward_picture", _I, [_P, _I, _P, _I, _I, _I, _P, _P, _P, _P])
 _sig(hip, "bc250_cnn_set_forward_variant", _I, [_I])
 _sig(hip, "bc250_cnn_wgrad", _I, [_I, _P, _P, _P, _P, _I, _P])
 _sig(hip, "bc250_cnn_head", _I, [_P, _P, _P, _P, _I, _P])
@@ -143,6 +145,9 @@ _sig(hip, "bc250_cnn_mvin", _I, [_P, _I, _I, _P, _P])
 _sig(hip, "bc250_cnn_candidates", _I, [_P, _I, _I, _I, _P, _P])
 _sig(hip, "bc250_launch_cand_search", _I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _I, _P, _P, _P])
 _sig(hip, "bc250_fallback_select", _I, [_P, _I, _I, _I, _I, _I, _P, _P, _P])
+_sig(hip, "bc250_cnn_propose", _I, [_P, _P, _P, _I, _I, _I, _P, _P])
+_sig(hip, "bc250_learned_search_picture", _I, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P,
+                                               _P, _P, _P, _P, _P, _P])
 
 
 class ProposerCfg(C.Structure):

@@ -76,10 +76,7 @@ struct Proposer {
     bc250_proposer_cfg_t cfg{};
     float *params = nullptr;     // fp32 [BC250_CNN_NPARAM]
     uint16_t *packed = nullptr;  // bf16 [BC250_CNN_NPACKED]
-    int ntiles = 0, tw = 0;
-    int *tiles = nullptr;        // {0, tx, ty} per tile
-    uint16_t *a0 = nullptr;      // input tiles [tile][66][66][2]
-    float *pool = nullptr, *mvin = nullptr, *z = nullptr;
+    float *pool = nullptr;       // block sums [tile][16][32] of the picture's 64x64 tiles
     int *cand = nullptr, *list = nullptr, *count = nullptr;
     uint32_t *keys = nullptr;    // the split fallback search's keys
     uint32_t *prev16 = nullptr;  // the previous picture's integer 16x16 vectors
@@ -236,15 +233,13 @@ static void free_res(bc250_hip_state *s)
         if (p) hipFree(p);
     {
         Proposer &q = s->prop;
-        for (void *p : {(void *)q.tiles, (void *)q.a0, (void *)q.pool, (void *)q.mvin, (void *)q.z, (void *)q.cand,
-                        (void *)q.list, (void *)q.count, (void *)q.prev16, (void *)q.keys})
+        for (void *p : {(void *)q.pool, (void *)q.cand, (void *)q.list, (void *)q.count, (void *)q.prev16,
+                        (void *)q.keys})
             if (p) hipFree(p);
-        q.tiles = q.cand = q.list = q.count = nullptr;
+        q.cand = q.list = q.count = nullptr;
         q.keys = nullptr;
-        q.a0 = nullptr;
-        q.pool = q.mvin = q.z = nullptr;
+        q.pool = nullptr;
         q.prev16 = nullptr;
-        q.ntiles = q.tw = 0;
         q.has_prev = false;
     }
     s->cavlc_out = nullptr; s->cavlc_info = nullptr; s->cavlc_work = nullptr;
@@ -387,36 +382,22 @@ static int ensure_cavlc(bc250_hip_state *s)
     return 0;
 }
 
-// The proposer's per-resolution buffers (network activations of every 64x64
-// tile of the picture, candidates, fallback list), allocated at its first
-// frame at a resolution; borders of the activations start (and stay) zero.
+// The proposer's per-resolution buffers (block sums of every 64x64 tile of
+// the picture, candidates, fallback list, split-search keys), allocated at
+// its first frame at a resolution.
 static int ensure_prop(bc250_hip_state *s)
 {
     Proposer &q = s->prop;
-    if (q.a0) return 0;
-    const int tw = (s->mbw + 3) / 4, th = (s->mbh + 3) / 4, nt = tw * th;
-    const size_t t2 = bc250_cnn_tile_bytes(2);
-    bool ok = HIP_OK(hipMalloc(&q.tiles, (size_t)nt * 12)) && HIP_OK(hipMalloc(&q.a0, nt * t2)) &&
-              HIP_OK(hipMalloc(&q.pool, (size_t)nt * 16 * 32 * 4)) &&
-              HIP_OK(hipMalloc(&q.mvin, (size_t)nt * 16 * 32 * 4)) &&
-              HIP_OK(hipMalloc(&q.z, (size_t)nt * 16 * BC250_CNN_NOUT * 4)) &&
+    if (q.pool) return 0;
+    const int nt = ((s->mbw + 3) / 4) * ((s->mbh + 3) / 4);
+    bool ok = HIP_OK(hipMalloc(&q.pool, (size_t)nt * 16 * 32 * 4)) &&
               HIP_OK(hipMalloc(&q.cand, (size_t)s->nmb * 4 * 8)) && HIP_OK(hipMalloc(&q.list, (size_t)s->nmb * 4)) &&
               HIP_OK(hipMalloc(&q.count, 4)) && HIP_OK(hipMalloc(&q.prev16, (size_t)s->nmb * 16)) &&
               HIP_OK(hipMalloc(&q.keys, (size_t)s->nmb * 5 * 4));
-    ok = ok && HIP_OK(hipMemset(q.a0, 0, nt * t2)) && HIP_OK(hipMemset(q.count, 0, 4));
-    if (ok) {
-        int *h = (int *)malloc((size_t)nt * 12);
-        if (!h) ok = false;
-        for (int i = 0; ok && i < nt; i++) { h[3 * i] = 0; h[3 * i + 1] = i % tw; h[3 * i + 2] = i / tw; }
-        ok = ok && HIP_OK(hipMemcpy(q.tiles, h, (size_t)nt * 12, hipMemcpyHostToDevice));
-        free(h);
-    }
     if (!ok) {
         fprintf(stderr, "[bc250-hip] proposer buffer allocation failed\n");
         return -1;
     }
-    q.ntiles = nt;
-    q.tw = tw;
     q.has_prev = false;
     return 0;
 }
@@ -431,21 +412,9 @@ static int enqueue_learned(bc250_hip_state *s, const uint8_t *c8, int cp, const 
     Proposer &q = s->prop;
     if (ensure_prop(s) != 0) return -1;
     const gpu_encode_params_t &p = s->params;
-    const int nt = q.ntiles;
-    int r = bc250_cnn_input(c8, cp, r8, rp, 0, s->wc, s->hc, q.tiles, nt, q.a0, st);
-    if (r == 0) r = bc250_cnn_forward(q.a0, q.packed, q.params, q.pool, nt, st);
-    if (r == 0) r = bc250_cnn_mvin(q.has_prev ? q.prev16 : nullptr, s->mbw, s->mbh, q.mvin, st);
-    if (r == 0) r = bc250_cnn_head(q.params + bc250_cnn_w_offset(4), q.pool, q.mvin, q.z, nt, st);
-    if (r == 0) r = bc250_cnn_candidates(q.z, s->mbw, s->mbh, p.search_range, q.cand, st);
-    if (r == 0)
-        r = bc250_launch_cand_search(c8, cp, r8, rp, s->wc, s->hc, p.search_range, p.lambda8, q.cand, q.cfg.ncand,
-                                     out16, out8, st);
-    if (r == 0)
-        r = bc250_fallback_select(out16, s->mbw, s->mbh, q.cfg.alpha_q4, q.cfg.beta, q.cfg.t_high, q.list, q.count,
-                                  st);
-    if (r == 0)
-        r = bc250_launch_me_fullsearch_list_split(c8, cp, r8, rp, s->wc, s->hc, p.search_range, p.lambda8, out16,
-                                                  out8, q.list, q.count, q.keys, st);
+    int r = bc250_learned_search_picture(c8, cp, r8, rp, s->wc, s->hc, p.search_range, p.lambda8, q.params, q.packed,
+                                         q.has_prev ? q.prev16 : nullptr, q.cfg.ncand, q.cfg.alpha_q4, q.cfg.beta,
+                                         q.cfg.t_high, q.pool, q.cand, q.list, q.count, q.keys, out16, out8, st);
     if (r == 0) r = HIP_OK(hipMemcpyAsync(q.prev16, out16, (size_t)s->nmb * 16, hipMemcpyDeviceToDevice, st)) ? 0 : -1;
     q.ran = r == 0;
     return r;

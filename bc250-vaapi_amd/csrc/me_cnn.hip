@@ -377,15 +377,42 @@ struct FusedWeights {
     const float *b[4];      // biases, channel c at b[c * (KPAD + 1)]
 };
 
-template <int SW>
-__global__ __launch_bounds__(FWAVES * 64) void cnn_forward_fused_kernel(const uint16_t *__restrict__ in,
-                                                                        const FusedWeights fw,
-                                                                        float *__restrict__ pool, int ntiles)
+// Where a band's input comes from: the input tiles [tile][66][66][2]
+// (FROM_TILES), one picture's 8-bit luma pair (FROM_PICTURE: tile i is
+// (i % tw, i / tw) of the picture), or a batch of pictures (FROM_BATCH).
+// Staged values are the input tiles' own: (sample - 128) / 64 as bf16,
+// samples past the picture repeating its last row / column, zero outside
+// the tile.
+enum { FROM_TILES = 0, FROM_PICTURE = 1, FROM_BATCH = 2 };
+struct FusedInput {
+    const uint16_t *tiles;
+    const uint8_t *cur, *ref;
+    int cpitch, rpitch, w, h, tw;
+};
+
+__device__ __forceinline__ void stage_picture_rows(uint32_t *__restrict__ q, const uint8_t *__restrict__ cur,
+                                                   int cpitch, const uint8_t *__restrict__ ref, int rpitch, int w,
+                                                   int h, int tx, int ty, int y0)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t sp[FP_ELEMS];
-    __shared__ __attribute__((aligned(16))) uint16_t sq[FQ_ELEMS];
-    const int tile = blockIdx.x >> 2, band = blockIdx.x & 3, tid = threadIdx.x;
-    if (tile >= ntiles) return;
+    for (int i = threadIdx.x; i < (FB + 8) * TP; i += FWAVES * 64) {
+        const int r = i / TP, x = i % TP, tr = y0 - 4 + r, tc = x - 1;
+        uint32_t v = 0u;
+        if (tr >= 0 && tr < T && tc >= 0 && tc < T) {
+            const int gy = min(ty * T + tr, h - 1), gx = min(tx * T + tc, w - 1);
+            const float a = ((int)cur[(size_t)gy * cpitch + gx] - 128) * (1.f / 64.f);
+            const float c = ((int)ref[(size_t)gy * rpitch + gx] - 128) * (1.f / 64.f);
+            v = (uint32_t)f2bf(a) | ((uint32_t)f2bf(c) << 16);
+        }
+        q[i] = v;
+    }
+}
+
+template <int SW, int FROM>
+__device__ __forceinline__ void cnn_forward_band(const FusedInput &src, const bc250_me_batch_t *bt,
+                                                 const FusedWeights &fw, float *__restrict__ pool, int tile,
+                                                 int band, uint16_t *sp, uint16_t *sq)
+{
+    const int tid = threadIdx.x;
     const int y0 = band * FB;  // tile row of the band's first output row
     // the weights of conv1..conv3 are requested first (their latency hides
     // under the input staging), conv4's at the end of conv3, into conv3's
@@ -396,14 +423,22 @@ __global__ __launch_bounds__(FWAVES * 64) void cnn_forward_fused_kernel(const ui
     load_w(fw.w[0], fw.b[0], w1);
     load_w(fw.w[1], fw.b[1], w2);
     load_w(fw.w[2], fw.b[2], w3);
-    // input rows y0-4 .. y0+FB+3: padded global row = tile row + 1
-    {
-        const uint32_t *g = (const uint32_t *)(in + (size_t)tile * TP * TP * 2);
+    // input rows y0-4 .. y0+FB+3
+    if constexpr (FROM == FROM_TILES) {
+        // padded global row = tile row + 1
+        const uint32_t *g = (const uint32_t *)(src.tiles + (size_t)tile * TP * TP * 2);
         uint32_t *q = (uint32_t *)sq;
         for (int i = tid; i < (FB + 8) * TP; i += FWAVES * 64) {
             const int r = i / TP, x = i % TP, tr = y0 - 4 + r;
             q[i] = (tr >= -1 && tr <= T) ? g[(tr + 1) * TP + x] : 0u;
         }
+    } else if constexpr (FROM == FROM_PICTURE) {
+        stage_picture_rows((uint32_t *)sq, src.cur, src.cpitch, src.ref, src.rpitch, src.w, src.h, tile % src.tw,
+                           tile / src.tw, y0);
+    } else {
+        const bc250_batch_frame_t &fr = bt->f[batch_frame_of_tile(*bt, tile)];
+        const int tw = (fr.wc / 16 + 3) / 4, lt = tile - fr.tile_base;
+        stage_picture_rows((uint32_t *)sq, fr.cur, fr.cpitch, fr.ref, fr.rpitch, fr.wc, fr.hc, lt % tw, lt / tw, y0);
     }
     __syncthreads();
     fused_layer<2, 16, 1, 4, SW>(sq, sp, FB + 6, y0 - 3, w1, nullptr);
@@ -422,6 +457,43 @@ __global__ __launch_bounds__(FWAVES * 64) void cnn_forward_fused_kernel(const ui
         for (int y = 0; y < FB; y++) s += rows[(y * 4 + bx) * 32 + c];
         pool[((size_t)tile * 16 + band * 4 + bx) * 32 + c] = s;
     }
+}
+
+template <int SW>
+__global__ __launch_bounds__(FWAVES * 64) void cnn_forward_fused_kernel(const uint16_t *__restrict__ in,
+                                                                        const FusedWeights fw,
+                                                                        float *__restrict__ pool, int ntiles)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t sp[FP_ELEMS];
+    __shared__ __attribute__((aligned(16))) uint16_t sq[FQ_ELEMS];
+    const int tile = blockIdx.x >> 2;
+    if (tile >= ntiles) return;
+    FusedInput src{};
+    src.tiles = in;
+    cnn_forward_band<SW, FROM_TILES>(src, nullptr, fw, pool, tile, blockIdx.x & 3, sp, sq);
+}
+
+template <int SW>
+__global__ __launch_bounds__(FWAVES * 64) void cnn_forward_picture_kernel(const FusedInput src, const FusedWeights fw,
+                                                                          float *__restrict__ pool, int ntiles)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t sp[FP_ELEMS];
+    __shared__ __attribute__((aligned(16))) uint16_t sq[FQ_ELEMS];
+    const int tile = blockIdx.x >> 2;
+    if (tile >= ntiles) return;
+    cnn_forward_band<SW, FROM_PICTURE>(src, nullptr, fw, pool, tile, blockIdx.x & 3, sp, sq);
+}
+
+template <int SW>
+__global__ __launch_bounds__(FWAVES * 64) void cnn_forward_batch_kernel(const bc250_me_batch_t b, const FusedWeights fw,
+                                                                        float *__restrict__ pool)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t sp[FP_ELEMS];
+    __shared__ __attribute__((aligned(16))) uint16_t sq[FQ_ELEMS];
+    const int tile = blockIdx.x >> 2;
+    if (tile >= b.ntile_total) return;
+    FusedInput src{};
+    cnn_forward_band<SW, FROM_BATCH>(src, &b, fw, pool, tile, blockIdx.x & 3, sp, sq);
 }
 
 // Weight (and bias) gradients of one layer: dW[co][k] = sum over pixels of
@@ -569,7 +641,9 @@ __device__ __forceinline__ void head_block(const float *__restrict__ wh, const f
 
 // Inference head: every block's 12 outputs into z [tile][16][12], one
 // thread per (block, output) with the block's 64 inputs read as float4s
-// (the same sums in the same order as head_block).
+// (the same sums in the same order as head_block, each one fused
+// multiply-add, so that the fused proposer of the learned search, which
+// evaluates the head per macroblock, gives the same z bit for bit).
 __global__ void cnn_head_kernel(const float *__restrict__ wh, const float *__restrict__ pool,
                                 const float *__restrict__ mvin, float *__restrict__ z, int nblocks)
 {
@@ -581,17 +655,17 @@ __global__ void cnn_head_kernel(const float *__restrict__ wh, const float *__res
     float s = row[BC250_CNN_NFEAT];
     for (int q = 0; q < 8; q++) {
         const float4 v = pb[q];
-        s += row[4 * q] * (v.x * (1.f / 256.f));
-        s += row[4 * q + 1] * (v.y * (1.f / 256.f));
-        s += row[4 * q + 2] * (v.z * (1.f / 256.f));
-        s += row[4 * q + 3] * (v.w * (1.f / 256.f));
+        s = fmaf(row[4 * q], v.x * (1.f / 256.f), s);
+        s = fmaf(row[4 * q + 1], v.y * (1.f / 256.f), s);
+        s = fmaf(row[4 * q + 2], v.z * (1.f / 256.f), s);
+        s = fmaf(row[4 * q + 3], v.w * (1.f / 256.f), s);
     }
     for (int q = 0; q < 8; q++) {
         const float4 v = mb[q];
-        s += row[32 + 4 * q] * v.x;
-        s += row[32 + 4 * q + 1] * v.y;
-        s += row[32 + 4 * q + 2] * v.z;
-        s += row[32 + 4 * q + 3] * v.w;
+        s = fmaf(row[32 + 4 * q], v.x, s);
+        s = fmaf(row[32 + 4 * q + 1], v.y, s);
+        s = fmaf(row[32 + 4 * q + 2], v.z, s);
+        s = fmaf(row[32 + 4 * q + 3], v.w, s);
     }
     z[i] = s;
 }
@@ -800,6 +874,58 @@ extern "C" int bc250_cnn_forward(const uint16_t *in, const uint16_t *packed, con
     case 0: cnn_forward_fused_kernel<0><<<ntiles * 4, FWAVES * 64, 0, st>>>(in, fw, pool, ntiles); break;
     case 1: cnn_forward_fused_kernel<1><<<ntiles * 4, FWAVES * 64, 0, st>>>(in, fw, pool, ntiles); break;
     default: cnn_forward_fused_kernel<2><<<ntiles * 4, FWAVES * 64, 0, st>>>(in, fw, pool, ntiles); break;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+static FusedWeights fused_weights(const uint16_t *packed, const float *params)
+{
+    const int cin[4] = {2, 16, 32, 32};
+    FusedWeights fw;
+    for (int l = 0; l < 4; l++) {
+        fw.w[l] = packed + bc250_cnn_packed_offset(l, 0);
+        fw.b[l] = params + bc250_cnn_w_offset(l) + kpad(cin[l]);
+    }
+    return fw;
+}
+
+extern "C" int bc250_cnn_forward_picture(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
+                                         int hc, const uint16_t *packed, const float *params, float *pool,
+                                         void *stream)
+{
+    if (!cur || !ref || !packed || !params || !pool || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) ||
+        cpitch < wc || rpitch < wc)
+        return -1;
+    FusedInput src{};
+    src.cur = cur;
+    src.ref = ref;
+    src.cpitch = cpitch;
+    src.rpitch = rpitch;
+    src.w = wc;
+    src.h = hc;
+    src.tw = (wc / 16 + 3) / 4;
+    const int ntiles = src.tw * ((hc / 16 + 3) / 4);
+    const FusedWeights fw = fused_weights(packed, params);
+    hipStream_t st = (hipStream_t)stream;
+    switch (g_forward_variant) {
+    case 0: cnn_forward_picture_kernel<0><<<ntiles * 4, FWAVES * 64, 0, st>>>(src, fw, pool, ntiles); break;
+    case 1: cnn_forward_picture_kernel<1><<<ntiles * 4, FWAVES * 64, 0, st>>>(src, fw, pool, ntiles); break;
+    default: cnn_forward_picture_kernel<2><<<ntiles * 4, FWAVES * 64, 0, st>>>(src, fw, pool, ntiles); break;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_cnn_forward_batch(const bc250_me_batch_t *b, const uint16_t *packed, const float *params,
+                                       float *pool, void *stream)
+{
+    if (!b || b->n < 1 || b->n > BC250_BATCH_MAX || b->ntile_total <= 0 || !packed || !params || !pool) return -1;
+    const FusedWeights fw = fused_weights(packed, params);
+    hipStream_t st = (hipStream_t)stream;
+    const int g = b->ntile_total * 4;
+    switch (g_forward_variant) {
+    case 0: cnn_forward_batch_kernel<0><<<g, FWAVES * 64, 0, st>>>(*b, fw, pool); break;
+    case 1: cnn_forward_batch_kernel<1><<<g, FWAVES * 64, 0, st>>>(*b, fw, pool); break;
+    default: cnn_forward_batch_kernel<2><<<g, FWAVES * 64, 0, st>>>(*b, fw, pool); break;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -677,7 +677,7 @@ extern "C" int bc250_launch_me_fullsearch_list(const uint8_t *cur, int cpitch, c
 extern "C" int bc250_launch_me_fullsearch_list_split(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch,
                                                      int wc, int hc, int range, int lambda8, uint32_t *out16,
                                                      uint32_t *out8, const int *list, const int *count,
-                                                     uint32_t *keys, void *stream)
+                                                     uint32_t *keys, int keys_ready, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
     if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || (cpitch & 3) || lambda8 < 0 || lambda8 > 15 || !list ||
@@ -685,7 +685,7 @@ extern "C" int bc250_launch_me_fullsearch_list_split(const uint8_t *cur, int cpi
         return -1;
     const int mbw = wc / 16, nmb = mbw * (hc / 16);
     const int kb = (5 * nmb + 255) / 256;
-    fs_keys_init_kernel<<<kb, 256, 0, st>>>(count, nmb, keys);
+    if (!keys_ready) fs_keys_init_kernel<<<kb, 256, 0, st>>>(count, nmb, keys);
     switch (range) {
 #define BC250_SPLIT(RR)                                                                                               \
     case RR:                                                                                                          \
@@ -704,14 +704,14 @@ extern "C" int bc250_launch_me_fullsearch_list_split(const uint8_t *cur, int cpi
 }
 
 extern "C" int bc250_launch_me_fullsearch_batch_split(const bc250_me_batch_t *b, const int *list, const int *count,
-                                                      uint32_t *keys, void *stream)
+                                                      uint32_t *keys, int keys_ready, void *stream)
 {
     if (!b || b->n < 1 || b->n > BC250_BATCH_MAX || b->nmb_total <= 0 || b->lambda8 < 0 || b->lambda8 > 15 ||
         !list || !count || !keys)
         return -1;
     hipStream_t st = (hipStream_t)stream;
     const int kb = (5 * b->nmb_total + 255) / 256;
-    fs_keys_init_kernel<<<kb, 256, 0, st>>>(count, b->nmb_total, keys);
+    if (!keys_ready) fs_keys_init_kernel<<<kb, 256, 0, st>>>(count, b->nmb_total, keys);
     switch (b->range) {
 #define BC250_SPLIT(RR)                                                                                               \
     case RR:                                                                                                          \

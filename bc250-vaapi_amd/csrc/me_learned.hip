@@ -20,6 +20,7 @@
 //                              me_fullsearch_kernel then searches in full
 #include "bc250_dev.h"
 #include "bc250_learned.h"
+#include "bc250_stages.h"
 
 namespace {
 
@@ -47,13 +48,10 @@ __global__ void cnn_mvin_kernel(const uint4 *__restrict__ prev16, int mbw, int m
     mvin_block(prev16, mbw, mbh, tw, b / 16, b % 16, mvin + (size_t)b * 32);
 }
 
-// Candidates of MB m of a picture whose tiles start at z (tw tiles per row).
-__device__ __forceinline__ void candidates_mb(const float *__restrict__ z, int mbw, int tw, int range, int m,
-                                              int2 *__restrict__ cand)
+// The 4 hypotheses of one block's head outputs zb ordered by confidence
+// (ties: lower index), in whole pels, clamped to the range, into cand[m].
+__device__ __forceinline__ void cand_from_z(const float *zb, int range, int m, int2 *__restrict__ cand)
 {
-    const int mx = m % mbw, my = m / mbw;
-    const int tile = (my >> 2) * tw + (mx >> 2), j = (my & 3) * 4 + (mx & 3);
-    const float *zb = z + ((size_t)tile * 16 + j) * BC250_CNN_NOUT;
     int ord[4] = {0, 1, 2, 3};
     // stable selection by confidence, highest first
     for (int a = 0; a < 4; a++)
@@ -65,6 +63,15 @@ __device__ __forceinline__ void candidates_mb(const float *__restrict__ z, int m
         const int dy = clampi((int)rintf(zb[2 * h + 1] * 16.f), -range, range);
         cand[(size_t)m * 4 + k] = make_int2(dx, dy);
     }
+}
+
+// Candidates of MB m of a picture whose tiles start at z (tw tiles per row).
+__device__ __forceinline__ void candidates_mb(const float *__restrict__ z, int mbw, int tw, int range, int m,
+                                              int2 *__restrict__ cand)
+{
+    const int mx = m % mbw, my = m / mbw;
+    const int tile = (my >> 2) * tw + (mx >> 2), j = (my & 3) * 4 + (mx & 3);
+    cand_from_z(z + ((size_t)tile * 16 + j) * BC250_CNN_NOUT, range, m, cand);
 }
 
 __global__ void cnn_candidates_kernel(const float *__restrict__ z, int mbw, int mbh, int tw, int range,
@@ -164,11 +171,15 @@ __global__ __launch_bounds__(64) void cand_search_kernel(const uint8_t *__restri
                                                          const uint8_t *__restrict__ ref, int rpitch, int wc, int hc,
                                                          int mbw, int nmb, int range, int lambda8,
                                                          const int2 *__restrict__ cand, int ncand,
-                                                         uint4 *__restrict__ out16, uint4 *__restrict__ out8)
+                                                         uint4 *__restrict__ out16, uint4 *__restrict__ out8,
+                                                         int *__restrict__ clear_count)
 {
     __shared__ uint32_t win[4 * WS * WP];
     const int mb = xcd_item(blockIdx.x, nmb);
     if (mb >= nmb) return;
+    // the fallback list of this picture starts empty (its previous use
+    // ended before this launch; its next starts after it)
+    if (clear_count && mb == 0 && threadIdx.x == 0) *clear_count = 0;
     cand_search_mb(cur, cpitch, ref, rpitch, wc, hc, mbw, mb, range, lambda8, cand, ncand, out16, out8, win);
 }
 
@@ -196,42 +207,37 @@ __device__ __forceinline__ bool fallback_mb(const uint4 *__restrict__ r16, int m
     return sad > bound || sad > (uint32_t)t_high;
 }
 
+// keys (optional): the split full search's keys of each listed MB, set to
+// the empty key as the MB is listed.
+__device__ __forceinline__ void list_mb(int *__restrict__ list, int *__restrict__ count, uint32_t *__restrict__ keys,
+                                        int m)
+{
+    const int i = atomicAdd(count, 1);
+    list[i] = m;
+    if (keys) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) keys[5 * i + q] = 0xffffffffu;
+    }
+}
+
 __global__ void fallback_select_kernel(const uint4 *__restrict__ r16, int mbw, int mbh, int alpha_q4, int beta,
-                                       int t_high, int *__restrict__ list, int *__restrict__ count)
+                                       int t_high, int *__restrict__ list, int *__restrict__ count,
+                                       uint32_t *__restrict__ keys)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= mbw * mbh) return;
-    if (fallback_mb(r16, mbw, mbh, alpha_q4, beta, t_high, m)) list[atomicAdd(count, 1)] = m;
+    if (fallback_mb(r16, mbw, mbh, alpha_q4, beta, t_high, m)) list_mb(list, count, keys, m);
 }
 
 // ---- batch forms: the picture of each batch-wide tile / MB from the descriptor
 
-__global__ void cnn_mvin_batch_kernel(const bc250_me_batch_t b, float *__restrict__ mvin)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (batch tile, block)
-    if (i >= b.ntile_total * 16) return;
-    const int t = i / 16, f = batch_frame_of_tile(b, t);
-    const bc250_batch_frame_t &fr = b.f[f];
-    const int mbw = fr.wc / 16, mbh = fr.hc / 16;
-    mvin_block((const uint4 *)fr.prev16, mbw, mbh, (mbw + 3) / 4, t - fr.tile_base, i % 16, mvin + (size_t)i * 32);
-}
-
-__global__ void cnn_candidates_batch_kernel(const bc250_me_batch_t b, const float *__restrict__ z)
-{
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= b.nmb_total) return;
-    const int f = batch_frame_of_mb(b, g);
-    const bc250_batch_frame_t &fr = b.f[f];
-    const int mbw = fr.wc / 16;
-    candidates_mb(z + (size_t)fr.tile_base * 16 * BC250_CNN_NOUT, mbw, (mbw + 3) / 4, b.range, g - fr.mb_base,
-                  (int2 *)fr.cand);
-}
-
-__global__ __launch_bounds__(64) void cand_search_batch_kernel(const bc250_me_batch_t b, int ncand)
+__global__ __launch_bounds__(64) void cand_search_batch_kernel(const bc250_me_batch_t b, int ncand,
+                                                               int *__restrict__ clear_count)
 {
     __shared__ uint32_t win[4 * WS * WP];
     const int g = xcd_item(blockIdx.x, b.nmb_total);
     if (g >= b.nmb_total) return;
+    if (clear_count && g == 0 && threadIdx.x == 0) *clear_count = 0;
     const int f = batch_frame_of_mb(b, g);
     const bc250_batch_frame_t &fr = b.f[f];
     cand_search_mb(fr.cur, fr.cpitch, fr.ref, fr.rpitch, fr.wc, fr.hc, fr.wc / 16, g - fr.mb_base, b.range, b.lambda8,
@@ -239,14 +245,58 @@ __global__ __launch_bounds__(64) void cand_search_batch_kernel(const bc250_me_ba
 }
 
 __global__ void fallback_select_batch_kernel(const bc250_me_batch_t b, int alpha_q4, int beta, int t_high,
-                                             int *__restrict__ list, int *__restrict__ count)
+                                             int *__restrict__ list, int *__restrict__ count,
+                                             uint32_t *__restrict__ keys)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= b.nmb_total) return;
     const int f = batch_frame_of_mb(b, g);
     const bc250_batch_frame_t &fr = b.f[f];
     if (fallback_mb((const uint4 *)fr.out16, fr.wc / 16, fr.hc / 16, alpha_q4, beta, t_high, g - fr.mb_base))
-        list[atomicAdd(count, 1)] = g;
+        list_mb(list, count, keys, g);
+}
+
+// ---- proposals in one launch: per MB its neighbour-vector features, the
+// head's 12 outputs (the order and fused multiply-adds of cnn_head_kernel)
+// and the candidates (cand_from_z); pool / prev16 / cand of one picture.
+__device__ __forceinline__ void propose_mb(const float *__restrict__ pool, const float *__restrict__ wh,
+                                           const uint4 *__restrict__ prev16, int mbw, int mbh, int range, int m,
+                                           int2 *__restrict__ cand)
+{
+    const int tw = (mbw + 3) / 4, mx = m % mbw, my = m / mbw;
+    const int tile = (my >> 2) * tw + (mx >> 2), j = (my & 3) * 4 + (mx & 3);
+    float mv[32];
+    mvin_block(prev16, mbw, mbh, tw, tile, j, mv);
+    const float *pb = pool + ((size_t)tile * 16 + j) * 32;
+    float z[BC250_CNN_NOUT];
+#pragma unroll
+    for (int o = 0; o < BC250_CNN_NOUT; o++) {
+        const float *row = wh + o * (BC250_CNN_NFEAT + 1);
+        float s = row[BC250_CNN_NFEAT];
+        for (int i = 0; i < 32; i++) s = fmaf(row[i], pb[i] * (1.f / 256.f), s);
+        for (int i = 0; i < 32; i++) s = fmaf(row[32 + i], mv[i], s);
+        z[o] = s;
+    }
+    cand_from_z(z, range, m, cand);
+}
+
+__global__ void cnn_propose_kernel(const float *__restrict__ pool, const float *__restrict__ wh,
+                                   const uint4 *__restrict__ prev16, int mbw, int mbh, int range,
+                                   int2 *__restrict__ cand)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= mbw * mbh) return;
+    propose_mb(pool, wh, prev16, mbw, mbh, range, m, cand);
+}
+
+__global__ void cnn_propose_batch_kernel(const bc250_me_batch_t b, const float *__restrict__ pool,
+                                         const float *__restrict__ wh)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.nmb_total) return;
+    const bc250_batch_frame_t &fr = b.f[batch_frame_of_mb(b, g)];
+    propose_mb(pool + (size_t)fr.tile_base * 16 * 32, wh, (const uint4 *)fr.prev16, fr.wc / 16, fr.hc / 16, b.range,
+               g - fr.mb_base, (int2 *)fr.cand);
 }
 
 }  // namespace
@@ -279,7 +329,7 @@ extern "C" int bc250_launch_cand_search(const uint8_t *cur, int cpitch, const ui
     const int mbw = wc / 16, nmb = mbw * (hc / 16);
     cand_search_kernel<<<((nmb + 7) / 8) * 8, 64, 0, (hipStream_t)stream>>>(
         cur, cpitch, ref, rpitch, wc, hc, mbw, nmb, range, lambda8, (const int2 *)cand, ncand, (uint4 *)out16,
-        (uint4 *)out8);
+        (uint4 *)out8, nullptr);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -291,18 +341,54 @@ extern "C" int bc250_fallback_select(const uint32_t *r16, int mbw, int mbh, int 
     hipStream_t st = (hipStream_t)stream;
     zero1_kernel<<<1, 1, 0, st>>>(count);
     fallback_select_kernel<<<(nmb + 127) / 128, 128, 0, st>>>((const uint4 *)r16, mbw, mbh, alpha_q4, beta, t_high,
-                                                              list, count);
+                                                              list, count, nullptr);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+extern "C" int bc250_cnn_propose(const float *pool, const float *params, const uint32_t *prev16, int mbw, int mbh,
+                                 int range, int *cand, void *stream)
+{
+    if (!pool || !params || !cand || mbw <= 0 || mbh <= 0 || range < CW || range > 64) return -1;
+    const int nmb = mbw * mbh;
+    cnn_propose_kernel<<<(nmb + 127) / 128, 128, 0, (hipStream_t)stream>>>(
+        pool, params + bc250_cnn_w_offset(4), (const uint4 *)prev16, mbw, mbh, range, (int2 *)cand);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_learned_search_picture(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
+                                            int hc, int range, int lambda8, const float *params,
+                                            const uint16_t *packed, const uint32_t *prev16, int ncand, int alpha_q4,
+                                            int beta, int t_high, float *pool, int *cand, int *list, int *count,
+                                            uint32_t *keys, uint32_t *out16, uint32_t *out8, void *stream)
+{
+    if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || (cpitch & 3) || lambda8 < 0 || lambda8 > 15 ||
+        ncand < 1 || ncand > 4 || range < CW || range > 64 || !params || !packed || !pool || !cand || !list ||
+        !count || !keys || !out16 || !out8 || alpha_q4 < 0 || beta < 0 || t_high < 0)
+        return -1;
+    hipStream_t st = (hipStream_t)stream;
+    const int mbw = wc / 16, mbh = hc / 16, nmb = mbw * mbh;
+    int r = bc250_cnn_forward_picture(cur, cpitch, ref, rpitch, wc, hc, packed, params, pool, st);
+    if (r == 0) r = bc250_cnn_propose(pool, params, prev16, mbw, mbh, range, cand, st);
+    if (r == 0) {
+        cand_search_kernel<<<((nmb + 7) / 8) * 8, 64, 0, st>>>(cur, cpitch, ref, rpitch, wc, hc, mbw, nmb, range,
+                                                               lambda8, (const int2 *)cand, ncand, (uint4 *)out16,
+                                                               (uint4 *)out8, count);
+        fallback_select_kernel<<<(nmb + 127) / 128, 128, 0, st>>>((const uint4 *)out16, mbw, mbh, alpha_q4, beta,
+                                                                  t_high, list, count, keys);
+        r = hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+    if (r == 0)
+        r = bc250_launch_me_fullsearch_list_split(cur, cpitch, ref, rpitch, wc, hc, range, lambda8, out16, out8, list,
+                                                  count, keys, 1, st);
+    return r;
+}
+
 // ---------------------------------------------------------------------------
-// The batched learned search.  Work layout (each part 256-byte aligned): input
-// tiles, conv1..3 activations, conv4 row sums, block sums, neighbour inputs,
-// head outputs, the fallback list, its count.
+// The batched learned search.  Work layout (each part 256-byte aligned):
+// block sums, the fallback list, its count, the split search's keys.
 namespace {
 struct LearnedWork {
-    uint16_t *a0;
-    float *pool, *mvin, *z;
+    float *pool;
     int *list, *count;
     uint32_t *keys;
     size_t bytes;
@@ -314,10 +400,7 @@ LearnedWork carve_learned(void *base, int nt, int nmb)
     LearnedWork w;
     size_t o = 0;
     auto take = [&](size_t n) { uint8_t *q = p ? p + o : nullptr; o += al256(n); return q; };
-    w.a0 = (uint16_t *)take((size_t)nt * bc250_cnn_tile_bytes(2));
     w.pool = (float *)take((size_t)nt * 16 * 32 * 4);
-    w.mvin = (float *)take((size_t)nt * 16 * 32 * 4);
-    w.z = (float *)take((size_t)nt * 16 * BC250_CNN_NOUT * 4);
     w.list = (int *)take((size_t)nmb * 4);
     w.count = (int *)take(4);
     w.keys = (uint32_t *)take((size_t)nmb * 5 * 4);
@@ -346,26 +429,16 @@ extern "C" int bc250_learned_search_batch(const bc250_me_batch_t *b, const float
         !packed || !work || ncand < 1 || ncand > 4 || b->range < CW || b->range > 64 ||
         b->ntile_total > work_tiles || b->nmb_total > work_mbs)
         return -1;
-    // the layout is the work buffer's (its capacity), whatever the batch's size,
-    // so every layer's tile borders stay where they were zeroed
     const LearnedWork w = carve_learned(work, work_tiles, work_mbs);
     hipStream_t st = (hipStream_t)stream;
-    const int nt = b->ntile_total;
-    int r = bc250_cnn_input_batch(b, w.a0, st);
-    if (r == 0) r = bc250_cnn_forward(w.a0, packed, params, w.pool, nt, st);
+    int r = bc250_cnn_forward_batch(b, packed, params, w.pool, st);
     if (r == 0) {
-        cnn_mvin_batch_kernel<<<(nt * 16 + 127) / 128, 128, 0, st>>>(*b, w.mvin);
-        r = hipGetLastError() == hipSuccess ? 0 : -1;
-    }
-    if (r == 0) r = bc250_cnn_head(params + bc250_cnn_w_offset(4), w.pool, w.mvin, w.z, nt, st);
-    if (r == 0) {
-        cnn_candidates_batch_kernel<<<(b->nmb_total + 127) / 128, 128, 0, st>>>(*b, w.z);
-        cand_search_batch_kernel<<<((b->nmb_total + 7) / 8) * 8, 64, 0, st>>>(*b, ncand);
-        zero1_kernel<<<1, 1, 0, st>>>(w.count);
+        cnn_propose_batch_kernel<<<(b->nmb_total + 127) / 128, 128, 0, st>>>(*b, w.pool, params + bc250_cnn_w_offset(4));
+        cand_search_batch_kernel<<<((b->nmb_total + 7) / 8) * 8, 64, 0, st>>>(*b, ncand, w.count);
         fallback_select_batch_kernel<<<(b->nmb_total + 127) / 128, 128, 0, st>>>(*b, alpha_q4, beta, t_high, w.list,
-                                                                                 w.count);
+                                                                                 w.count, w.keys);
         r = hipGetLastError() == hipSuccess ? 0 : -1;
     }
-    if (r == 0) r = bc250_launch_me_fullsearch_batch_split(b, w.list, w.count, w.keys, st);
+    if (r == 0) r = bc250_launch_me_fullsearch_batch_split(b, w.list, w.count, w.keys, 1, st);
     return r;
 }

@@ -129,6 +129,12 @@ int bc250_cnn_mvin(const uint32_t *prev16, int mbw, int mbh, float *mvin, void *
  * nearest, clamped to +-range. */
 int bc250_cnn_candidates(const float *z, int mbw, int mbh, int range, int *cand, void *stream);
 
+/* bc250_cnn_mvin -> bc250_cnn_head -> bc250_cnn_candidates in one launch,
+ * one thread per MB, from the block sums pool [tile][16][32]: the same
+ * candidates, bit for bit, without mvin and z in memory. */
+int bc250_cnn_propose(const float *pool, const float *params, const uint32_t *prev16, int mbw, int mbh, int range,
+                      int *cand, void *stream);
+
 /* Integer search over the +-4 windows around the first ncand candidates of
  * every MB (positions outside +-range skipped) with the full search's
  * costs (SAD + lambda8 |mv|_1 per 8x8 block, SAD + 4 lambda8 |mv|_1 per
@@ -144,6 +150,19 @@ int bc250_launch_cand_search(const uint8_t *cur, int cpitch, const uint8_t *ref,
  * order).  *count is cleared on the stream first. */
 int bc250_fallback_select(const uint32_t *r16, int mbw, int mbh, int alpha_q4, int beta, int t_high, int *list,
                           int *count, void *stream);
+
+/* The learned search of one picture in five launches: the network from the
+ * pictures (bc250_cnn_forward_picture), the proposer (bc250_cnn_propose),
+ * the candidate search (which also clears *count), the fallback list (which
+ * also sets the split search's keys) and the split full search of the
+ * listed MBs (bc250_launch_me_fullsearch_list_split).  Work space: pool
+ * [ntiles][16][32] float, cand [nmb][4] int2, list [nmb], count, keys
+ * [5 nmb]; the records out16 / out8 are bc250_launch_me_fullsearch's. */
+int bc250_learned_search_picture(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
+                                 int range, int lambda8, const float *params, const uint16_t *packed,
+                                 const uint32_t *prev16, int ncand, int alpha_q4, int beta, int t_high, float *pool,
+                                 int *cand, int *list, int *count, uint32_t *keys, uint32_t *out16, uint32_t *out8,
+                                 void *stream);
 
 /* ---- batches of pictures ---------------------------------------------------
  * One launch per stage for the integer searches of several pictures (of
@@ -189,18 +208,29 @@ static __device__ __forceinline__ int batch_frame_of_tile(const bc250_me_batch_t
 }
 #endif
 
+/* The same launch reading one picture pair directly (8-bit luma, coded
+ * size wc x hc, multiples of 16; the tiles in raster order, (wc/16 + 3)/4
+ * per row), or every picture of a batch (tiles in batch order): the block
+ * sums bc250_cnn_input / bc250_cnn_input_batch + bc250_cnn_forward give,
+ * without the input tiles in memory. */
+int bc250_cnn_forward_picture(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
+                              const uint16_t *packed, const float *params, float *pool, void *stream);
+int bc250_cnn_forward_batch(const bc250_me_batch_t *b, const uint16_t *packed, const float *params, float *pool,
+                            void *stream);
+
 /* The plain full search of every MB of the batch (list / count: only the
  * listed batch-wide MBs, count read on the device). */
 int bc250_launch_me_fullsearch_batch(const bc250_me_batch_t *b, const int *list, const int *count, void *stream);
 /* The listed MBs' search split over several workgroups per MB (keys: 5 *
- * nmb_total uint32 of work space); the same records. */
+ * nmb_total uint32 of work space, set on the stream first unless keys_ready
+ * says the fallback list already set them); the same records. */
 int bc250_launch_me_fullsearch_batch_split(const bc250_me_batch_t *b, const int *list, const int *count,
-                                           uint32_t *keys, void *stream);
+                                           uint32_t *keys, int keys_ready, void *stream);
 
 /* The learned search of every picture of the batch, one launch per stage:
- * input tiles, the four layers, block sums, neighbour inputs, head,
- * candidates, candidate search, fallback list (batch-wide MB indices) and
- * the full search of the listed MBs.  work: bc250_learned_work_bytes(
+ * the network from the pictures (block sums), the proposer, the candidate
+ * search, the fallback list (batch-wide MB indices) and the split full
+ * search of the listed MBs.  work: bc250_learned_work_bytes(
  * work_tiles, work_mbs) bytes, zeroed once before its first use and laid out
  * by those capacities (any batch up to them reuses it); params / packed as
  * for the single-picture path. */

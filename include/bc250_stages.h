@@ -36,10 +36,13 @@ int bc250_launch_me_fullsearch_list(const uint8_t *cur, int cpitch, const uint8_
                                     const int *count, void *stream);
 /* The same records for a short list: each listed MB's candidate rows are
  * split over several workgroups whose keys meet in keys (5 * MBs of the
- * picture uint32 of work space), so a few hundred MBs still fill the GPU. */
+ * picture uint32 of work space, set on the stream first unless keys_ready:
+ * the learned search's fallback list sets them itself), so a few hundred
+ * MBs still fill the GPU. */
 int bc250_launch_me_fullsearch_list_split(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
                                           int hc, int range, int lambda8, uint32_t *out16, uint32_t *out8,
-                                          const int *list, const int *count, uint32_t *keys, void *stream);
+                                          const int *list, const int *count, uint32_t *keys, int keys_ready,
+                                          void *stream);
 
 /* Half- then quarter-pel refinement of the integer winners
  * (shaders/motion_estimation.comp:376-469). do8 = also the 8x8 quadrants. */

@@ -316,7 +316,7 @@ def test_split_list_search_equals_full_search(pkg):
             o16 = torch.full((nmb * 4,), -7, dtype=torch.int32, device="cuda")
             o8 = torch.full((nmb * 16,), -7, dtype=torch.int32, device="cuda")
             assert pkg.hip.bc250_launch_me_fullsearch_list_split(P(dcur), w, P(dref), w, w, h, R, 1, P(o16), P(o8),
-                                                                 P(lst), P(cnt), P(keys), None) == 0
+                                                                 P(lst), P(cnt), P(keys), 0, None) == 0
             g16 = o16.cpu().numpy().view(orc.IMV)
             g8 = o8.cpu().numpy().view(orc.IMV).reshape(nmb, 4)
             on = set(lst_py)
@@ -326,6 +326,94 @@ def test_split_list_search_equals_full_search(pkg):
                     assert (g8[m] == i8.reshape(nmb, 4)[m]).all(), (R, m)
                 else:
                     assert g16[m]["dx"] == -7 and (g8[m]["dx"] == -7).all()
+
+
+def test_picture_chain_equals_the_stages(pkg, learned):
+    """The five-launch learned search of one picture gives, bit for bit,
+    what the separate stages give: the network read straight from the
+    pictures (pitches above the width, partial 64x64 tiles at the right and
+    bottom) equals input tiles + the fused forward in every LDS layout; the
+    one-launch proposer equals neighbour inputs -> head -> candidates with
+    and without a previous picture's vectors; the candidate search, fallback
+    list and split search equal the separate launches (records, count and
+    list), on two calls in a row with different inputs."""
+    import torch
+    say("learned: picture chain against the stages, 208x144")
+    rs = np.random.default_rng(41)
+    w, h, R, lam = 208, 144, 32, 1
+    mbw, mbh = w // 16, h // 16
+    nmb, tw = mbw * mbh, (mbw + 3) // 4
+    nt = tw * ((mbh + 3) // 4)
+    P = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+    net = learned.Net(pkg, nt, pkg.load_proposer_weights())
+    tiles = np.array([(0, i % tw, i // tw) for i in range(nt)], np.int32)
+    cfg = (4, 18, 128, 3000)
+    cand_a = torch.zeros(nmb * 8, dtype=torch.int32, device="cuda")
+    cand_b = torch.zeros_like(cand_a)
+    mvin = torch.zeros((nt, 16, 32), dtype=torch.float32, device="cuda")
+    z = torch.zeros((nt, 16, learned.NOUT), dtype=torch.float32, device="cuda")
+    pool_b = torch.zeros((nt, 16, 32), dtype=torch.float32, device="cuda")
+    lists = [torch.full((nmb,), -1, dtype=torch.int32, device="cuda") for _ in range(2)]
+    counts = [torch.full((1,), 77, dtype=torch.int32, device="cuda") for _ in range(2)]
+    keys = [torch.zeros(nmb * 5, dtype=torch.int32, device="cuda") for _ in range(2)]
+    outs = [(torch.zeros(nmb * 4, dtype=torch.int32, device="cuda"), torch.zeros(nmb * 16, dtype=torch.int32,
+                                                                                   device="cuda")) for _ in range(2)]
+    wh = C.c_void_p(net.params.data_ptr() + 4 * learned.OFFSETS[4])
+    for call in range(2):
+        yy, xx = np.mgrid[0:h, 0:w]
+        cur = np.clip(128 + 50 * np.sin(xx / (4.0 + call)) * np.cos(yy / 6.0) + rs.normal(0, 6, (h, w)), 0,
+                      255).astype(np.uint8)
+        cur[40:80, 96:160] = rs.integers(0, 256, (40, 64))          # noise: MBs for the fallback
+        ref = np.roll(cur, (3 + call, -6), axis=(0, 1))
+        cpad = np.zeros((h, 224), np.uint8)
+        rpad = np.zeros((h, 256), np.uint8)
+        cpad[:, :w], rpad[:, :w] = cur, ref
+        dc, dr = torch.from_numpy(cur).cuda(), torch.from_numpy(ref).cuda()
+        dcp, drp = torch.from_numpy(cpad).cuda(), torch.from_numpy(rpad).cuda()
+        prev = None
+        if call == 1:
+            v = np.zeros((nmb, 4), np.int32)
+            v[:, 0], v[:, 1] = rs.integers(-R, R + 1, nmb), rs.integers(-R, R + 1, nmb)
+            prev = torch.from_numpy(v.reshape(-1)).cuda()
+        # the network: input tiles + fused forward against the picture form
+        net.set_input(dc, dr, w, w, h, tiles)
+        assert net.hip.bc250_cnn_forward(P(net.act[0]), P(net.packed), P(net.params), P(net.pool), nt, None) == 0
+        for variant in (0, 1, 2):
+            assert pkg.hip.bc250_cnn_set_forward_variant(variant) == 0
+            pool_b.fill_(-1.0)
+            assert pkg.hip.bc250_cnn_forward_picture(P(dcp), 224, P(drp), 256, w, h, P(net.packed), P(net.params),
+                                                     P(pool_b), None) == 0
+            torch.cuda.synchronize()
+            assert torch.equal(pool_b, net.pool), (call, variant)
+        assert pkg.hip.bc250_cnn_set_forward_variant(1) == 0
+        # the proposer: mvin -> head -> candidates against one launch
+        assert pkg.hip.bc250_cnn_mvin(P(prev), mbw, mbh, P(mvin), None) == 0
+        assert pkg.hip.bc250_cnn_head(wh, P(net.pool), P(mvin), P(z), nt, None) == 0
+        assert pkg.hip.bc250_cnn_candidates(P(z), mbw, mbh, R, P(cand_a), None) == 0
+        assert pkg.hip.bc250_cnn_propose(P(net.pool), P(net.params), P(prev), mbw, mbh, R, P(cand_b), None) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(cand_a, cand_b), call
+        # the search: separate stages (0) against the chain (1)
+        (a16, a8), (b16, b8) = outs
+        assert pkg.hip.bc250_launch_cand_search(P(dc), w, P(dr), w, w, h, R, lam, P(cand_a), cfg[0], P(a16), P(a8),
+                                                None) == 0
+        assert pkg.hip.bc250_fallback_select(P(a16), mbw, mbh, cfg[1], cfg[2], cfg[3], P(lists[0]), P(counts[0]),
+                                             None) == 0
+        assert pkg.hip.bc250_launch_me_fullsearch_list_split(P(dc), w, P(dr), w, w, h, R, lam, P(a16), P(a8),
+                                                             P(lists[0]), P(counts[0]), P(keys[0]), 0, None) == 0
+        assert pkg.hip.bc250_learned_search_picture(P(dcp), 224, P(drp), 256, w, h, R, lam, P(net.params),
+                                                    P(net.packed), P(prev), *cfg, P(pool_b), P(cand_b),
+                                                    P(lists[1]), P(counts[1]), P(keys[1]), P(b16), P(b8), None) == 0
+        torch.cuda.synchronize()
+        n = int(counts[0].item())
+        assert int(counts[1].item()) == n and 0 < n < nmb, (call, n)
+        assert sorted(lists[0][:n].tolist()) == sorted(lists[1][:n].tolist())
+        assert torch.equal(a16, b16) and torch.equal(a8, b8), call
+        # and the records of the listed MBs are the oracle's full search
+        i16, _ = orc.me_fullsearch(cur, ref, R, lam)
+        g16 = b16.cpu().numpy().view(orc.IMV)
+        for m in lists[1][:n].tolist():
+            assert g16[m] == i16[m], (call, m)
 
 
 def _fixed_candidate_params(learned, cands):
