@@ -375,6 +375,57 @@ _sig(host, "bc250_h264_write_sps", C.c_size_t, [_P, _P, C.c_size_t])
 _sig(host, "bc250_h264_write_pps", C.c_size_t, [_P, _P, C.c_size_t])
 
 
+class HevcStreamCfg(C.Structure):
+    """bc250_hevc_stream_cfg_t (include/bc250_hevc_stream.h)."""
+    _fields_ = [(k, C.c_int32) for k in ("width", "height", "crop_width", "crop_height", "bit_depth", "ctu_log2",
+                                         "num_slices", "deblock", "init_qp")]
+
+
+_sig(host, "bc250_hevc_write_parameter_sets", C.c_size_t, [C.POINTER(HevcStreamCfg), _P, C.c_size_t])
+_sig(host, "bc250_hevc_write_picture", C.c_size_t, [C.POINTER(HevcStreamCfg), _I, _I, _I, _P, _P, _P, _P,
+                                                    C.c_size_t])
+
+
+class HevcStream:
+    """The HEVC bitstream of pipeline pictures (include/bc250_hevc_stream.h):
+    headers() gives VPS / SPS / PPS, picture() one picture's slice NAL units
+    from its staging (levels int16, modes uint8, 8x8 CU vectors uint32 x 4),
+    both in Annex B form."""
+
+    def __init__(self, width, height, bit_depth=8, ctu_log2=4, num_slices=1, deblock=True, init_qp=26,
+                 crop_width=0, crop_height=0):
+        self.cfg = HevcStreamCfg(width, height, crop_width, crop_height, bit_depth, ctu_log2, num_slices,
+                                 int(bool(deblock)), init_qp)
+        self.w, self.h = width, height
+
+    def headers(self):
+        buf = np.empty(1024, np.uint8)
+        n = host.bc250_hevc_write_parameter_sets(C.byref(self.cfg), _ptr(buf), buf.size)
+        if not n:
+            raise ValueError("invalid HEVC stream configuration")
+        return buf[:n].tobytes()
+
+    def picture(self, is_idr, poc, qp, coef, modes, mv8=None):
+        coef = np.ascontiguousarray(coef, np.int16)
+        modes = np.ascontiguousarray(modes, np.uint8)
+        t = (self.w // 16) * (self.h // 16)
+        if coef.size != t * 384 or modes.size != t * 16:
+            raise ValueError("staging does not match the picture size")
+        mvp = None
+        if not is_idr:
+            mv = np.ascontiguousarray(mv8).view(np.uint32).reshape(-1)
+            if mv.size != t * 16:
+                raise ValueError("vectors do not match the picture size")
+            mvp = _ptr(mv)
+        cap = self.w * self.h * 8 + (1 << 16)
+        buf = np.empty(cap, np.uint8)
+        n = host.bc250_hevc_write_picture(C.byref(self.cfg), int(bool(is_idr)), poc, qp, _ptr(coef), _ptr(modes), mvp,
+                                          _ptr(buf), cap)
+        if not n:
+            raise RuntimeError("HEVC picture writer failed")
+        return buf[:n].tobytes()
+
+
 class RcState(C.Structure):
     """bc250_rc_t (host/bc250_rc.h)."""
     _fields_ = [("mode", C.c_int32), ("target_bitrate", C.c_uint32), ("max_bitrate", C.c_uint32),
