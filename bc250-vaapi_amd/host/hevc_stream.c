@@ -302,9 +302,18 @@ static inline void cab_bypass(cab_t *c, int bin)
     cab_byte(c);
 }
 
+/* n bypass bins, the bits of v from the most significant: up to eight at
+ * a time, each group one multiply-add into low (the queue then holds at
+ * most 7 + 8 bits below the carry, within 32). */
 static void cab_bypass_bits(cab_t *c, uint32_t v, int n)
 {
-    for (int i = n - 1; i >= 0; i--) cab_bypass(c, (int)((v >> i) & 1));
+    while (n > 0) {
+        const int k = n > 8 ? 8 : n;
+        n -= k;
+        c->low = (c->low << k) + c->range * ((v >> n) & ((1u << k) - 1));
+        c->queue += k;
+        cab_byte(c);
+    }
 }
 
 /* end_of_slice_segment_flag.  With bin 1 the coder is flushed (H.264
@@ -405,10 +414,8 @@ static void last_prefix(cab_t *c, int v, int log2, int cidx, int base, int *sfx,
 static void cab_remaining(cab_t *c, uint32_t v, int k)
 {
     if (v < (3u << k)) {
-        const uint32_t q = v >> k;
-        for (uint32_t i = 0; i < q; i++) cab_bypass(c, 1);
-        cab_bypass(c, 0);
-        cab_bypass_bits(c, v & ((1u << k) - 1), k);
+        const uint32_t q = v >> k; /* q ones, a zero, k bits */
+        cab_bypass_bits(c, ((((1u << q) - 1) << 1) << k) | (v & ((1u << k) - 1)), (int)q + 1 + k);
         return;
     }
     uint32_t r = v - (3u << k);
@@ -417,8 +424,8 @@ static void cab_remaining(cab_t *c, uint32_t v, int k)
         r -= 1u << len;
         len++;
     }
-    for (int i = 0; i < 3 + len - k; i++) cab_bypass(c, 1);
-    cab_bypass(c, 0);
+    const int ones = 3 + len - k;
+    cab_bypass_bits(c, ((1u << ones) - 1) << 1, ones + 1);
     cab_bypass_bits(c, r, len);
 }
 
@@ -540,8 +547,10 @@ static void residual(cab_t *c, const int16_t *co, int log2, int cidx, int scan)
             g2[first_g1] = a > 2;
             cab_bin(c, C_G2 + ctx_set + (cidx ? 4 : 0), g2[first_g1]);
         }
+        uint32_t signs = 0;
         for (int k = 15; k >= 0; k--)
-            if (sig[k]) cab_bypass(c, lv[k] < 0);
+            if (sig[k]) signs = signs << 1 | (lv[k] < 0);
+        cab_bypass_bits(c, signs, nsig);
         int nseen = 0, rice = 0;
         for (int k = 15; k >= 0; k--) {
             if (!sig[k]) continue;
@@ -566,31 +575,39 @@ typedef struct {
     uint8_t *depth8;     /* CtDepth per 8x8 */
     uint8_t *mode4;      /* IntraPredModeY per 4x4 */
     int32_t *mv;         /* per 8x8: x, y (quarter samples) */
+    uint8_t *skip8;      /* cu_skip_flag per 8x8 */
+    uint8_t zs4[8][8];   /* MinTbAddrZs inside a CTB by (y, x) at 4x4 granularity */
     const int16_t *coef;
     const uint8_t *modes;
     const uint32_t *mv8;
     cab_t c;
 } pic_t;
 
-/* MinTbAddrZs order inside a CTB at 4x4 granularity (6.5.2). */
-static int zs(int x, int y, int L)
+/* MinTbAddrZs order inside a CTB at 4x4 granularity (6.5.2): the bits of
+ * the 4x4 column and row interleaved. */
+static void make_zs(pic_t *p)
 {
-    const int m = (1 << L) - 1, xi = (x & m) >> 2, yi = (y & m) >> 2;
-    int r = 0;
-    for (int b = 0; b < L - 2; b++) r |= ((xi >> b) & 1) << (2 * b) | ((yi >> b) & 1) << (2 * b + 1);
-    return r;
+    for (int yi = 0; yi < 8; yi++)
+        for (int xi = 0; xi < 8; xi++) {
+            int r = 0;
+            for (int b = 0; b < 3; b++) r |= ((xi >> b) & 1) << (2 * b) | ((yi >> b) & 1) << (2 * b + 1);
+            p->zs4[yi][xi] = (uint8_t)r;
+        }
 }
 
 /* 6.4.1: whether the block at (xn, yn) is available to the one at (xc, yc):
  * inside the picture, in the same slice, and earlier in decoding order. */
-static int avail(const pic_t *p, int xc, int yc, int xn, int yn)
+static inline int avail(const pic_t *p, int xc, int yc, int xn, int yn)
 {
     if (xn < 0 || yn < 0 || xn >= p->w || yn >= p->h) return 0;
-    const int rn = yn >> p->L, rc = yc >> p->L;
+    const int L = p->L, rn = yn >> L, rc = yc >> L;
+    const int cn = xn >> L, cc = xc >> L;
+    if (rn == rc && cn == cc) {
+        const int m = (1 << L) - 1;
+        return p->zs4[(yn & m) >> 2][(xn & m) >> 2] < p->zs4[(yc & m) >> 2][(xc & m) >> 2];
+    }
     if (p->slice_of_row[rn] != p->slice_of_row[rc]) return 0;
-    const int an = rn * p->wctb + (xn >> p->L), ac = rc * p->wctb + (xc >> p->L);
-    if (an != ac) return an < ac;
-    return zs(xn, yn, p->L) < zs(xc, yc, p->L);
+    return rn < rc || (rn == rc && cn < cc);
 }
 
 /* The staging offset of the 16x16 area holding (x, y). */
@@ -612,10 +629,16 @@ static int split_ctx(const pic_t *p, int x0, int y0, int depth)
     return l + a;
 }
 
+/* n a multiple of 16 */
 static int any_nz(const int16_t *v, int n)
 {
-    for (int i = 0; i < n; i++)
-        if (v[i]) return 1;
+    uint64_t acc = 0;
+    for (int i = 0; i < n; i += 16) {
+        uint64_t w[4];
+        memcpy(w, v + i, sizeof w);
+        acc |= w[0] | w[1] | w[2] | w[3];
+        if (acc) return 1;
+    }
     return 0;
 }
 
@@ -805,24 +828,52 @@ static void amvp(const pic_t *p, int x, int y, int s, int out[2][2])
     }
 }
 
-/* A P picture's inter CU of side 1 << log2 at (x0, y0): 2Nx2N AMVP, one
- * luma TU of the CU's size and one chroma TU per component of half it. */
+/* 8.5.3.2.2 - 8.5.3.2.5: the five merge candidates of the 2Nx2N PU at
+ * (x, y), side s, in a P slice with one reference picture, every CU inter
+ * and no temporal candidate: the spatial ones A1, B1, B0, A0, B2 with the
+ * standard's pruning, then zero vectors. */
+static void merge_list(const pic_t *p, int x, int y, int s, int out[5][2])
+{
+    const int w8 = p->w >> 3;
+    const int nx[5] = {x - 1, x + s - 1, x + s, x - 1, x - 1}; /* A1 B1 B0 A0 B2 */
+    const int ny[5] = {y + s - 1, y - 1, y - 1, y + s, y - 1};
+    int av[5], m[5][2];
+    for (int k = 0; k < 5; k++) {
+        av[k] = avail(p, x, y, nx[k], ny[k]);
+        if (av[k]) {
+            const int32_t *v = p->mv + 2 * ((size_t)(ny[k] >> 3) * w8 + (nx[k] >> 3));
+            m[k][0] = v[0];
+            m[k][1] = v[1];
+        }
+    }
+#define SAME(a, b) (av[a] && m[a][0] == m[b][0] && m[a][1] == m[b][1])
+    int fl[5];
+    fl[0] = av[0];
+    fl[1] = av[1] && !SAME(0, 1);
+    fl[2] = av[2] && !SAME(1, 2);
+    fl[3] = av[3] && !SAME(0, 3);
+    fl[4] = av[4] && !SAME(0, 4) && !SAME(1, 4) && fl[0] + fl[1] + fl[2] + fl[3] != 4;
+#undef SAME
+    int n = 0;
+    for (int k = 0; k < 5; k++)
+        if (fl[k]) {
+            out[n][0] = m[k][0];
+            out[n][1] = m[k][1];
+            n++;
+        }
+    for (; n < 5; n++) out[n][0] = out[n][1] = 0;
+}
+
+/* A P picture's inter CU of side 1 << log2 at (x0, y0), 2Nx2N with one
+ * luma TU of the CU's size and one chroma TU per component of half it.
+ * Its vector is sent as a merge index when it is one of the merge
+ * candidates (a skip when it has no levels either), else by AMVP: the
+ * same prediction and residual either way, so the same picture. */
 static void inter_cu(pic_t *p, int x0, int y0, int log2)
 {
     cab_t *c = &p->c;
     const int s = 1 << log2, w8 = p->w >> 3;
     const int32_t *mv = p->mv + 2 * ((size_t)(y0 >> 3) * w8 + (x0 >> 3));
-    cab_bin(c, C_SKIP + 0, 0);     /* no neighbour is skipped: ctxInc 0 */
-    cab_bin(c, C_PRED_MODE, 0);    /* MODE_INTER */
-    cab_bin(c, C_PART_MODE, 1);    /* PART_2Nx2N */
-    cab_bin(c, C_MERGE_FLAG, 0);
-    int cand[2][2];
-    amvp(p, x0, y0, s, cand);
-    const int c0 = mvd_cost(mv[0] - cand[0][0]) + mvd_cost(mv[1] - cand[0][1]);
-    const int c1 = mvd_cost(mv[0] - cand[1][0]) + mvd_cost(mv[1] - cand[1][1]);
-    const int idx = c1 < c0;
-    mvd_code(c, mv[0] - cand[idx][0], mv[1] - cand[idx][1]);
-    cab_bin(c, C_MVP, idx);
 
     /* the levels: a CU8 in its area's z-slot; a CU16 the area; a CU32 the four areas' chunks */
     int16_t lbuf[1024], cbbuf[256], crbuf[256];
@@ -844,7 +895,36 @@ static void inter_cu(pic_t *p, int x0, int y0, int log2)
     }
     const int cbf_y = any_nz(lbuf, nl), cbf_cb = any_nz(cbbuf, nc), cbf_cr = any_nz(crbuf, nc);
     const int root = cbf_y | cbf_cb | cbf_cr;
-    cab_bin(c, C_ROOT_CBF, root);
+
+    int mc[5][2], midx = -1;
+    merge_list(p, x0, y0, s, mc);
+    for (int k = 0; k < 5 && midx < 0; k++)
+        if (mc[k][0] == mv[0] && mc[k][1] == mv[1]) midx = k;
+    /* cu_skip_flag's context: the left and above CUs' skip flags (9.3.4.2.2) */
+    const int sl = avail(p, x0, y0, x0 - 1, y0) && p->skip8[(y0 >> 3) * w8 + ((x0 - 1) >> 3)];
+    const int sa = avail(p, x0, y0, x0, y0 - 1) && p->skip8[((y0 - 1) >> 3) * w8 + (x0 >> 3)];
+    const int skip = midx >= 0 && !root;
+    cab_bin(c, C_SKIP + sl + sa, skip);
+    for (int j = 0; j < (s >> 3); j++)
+        for (int i = 0; i < (s >> 3); i++) p->skip8[((y0 >> 3) + j) * w8 + (x0 >> 3) + i] = (uint8_t)skip;
+    if (!skip) {
+        cab_bin(c, C_PRED_MODE, 0); /* MODE_INTER */
+        cab_bin(c, C_PART_MODE, 1); /* PART_2Nx2N */
+        cab_bin(c, C_MERGE_FLAG, midx >= 0);
+    }
+    if (midx >= 0) { /* merge_idx: TR, cMax 4, first bin in context */
+        cab_bin(c, C_MERGE_IDX, midx > 0);
+        for (int k = 1; k < 4 && k <= midx; k++) cab_bypass(c, midx > k);
+    } else {
+        int cand[2][2];
+        amvp(p, x0, y0, s, cand);
+        const int c0 = mvd_cost(mv[0] - cand[0][0]) + mvd_cost(mv[1] - cand[0][1]);
+        const int c1 = mvd_cost(mv[0] - cand[1][0]) + mvd_cost(mv[1] - cand[1][1]);
+        const int idx = c1 < c0;
+        mvd_code(c, mv[0] - cand[idx][0], mv[1] - cand[idx][1]);
+        cab_bin(c, C_MVP, idx);
+        cab_bin(c, C_ROOT_CBF, root); /* not sent for a 2Nx2N merge: there it is 1 */
+    }
     if (root) {
         cab_bin(c, C_CBF_CHROMA + 0, cbf_cb);
         cab_bin(c, C_CBF_CHROMA + 0, cbf_cr);
@@ -1070,6 +1150,7 @@ size_t bc250_hevc_write_picture(const bc250_hevc_stream_cfg_t *cfg, int is_idr, 
     p.wctb = (p.w + (1 << p.L) - 1) >> p.L;
     p.hctb = (p.h + (1 << p.L) - 1) >> p.L;
     p.a16w = p.w >> 4;
+    make_zs(&p);
     p.coef = coef;
     p.modes = modes;
     p.mv8 = mv8;
@@ -1078,11 +1159,12 @@ size_t bc250_hevc_write_picture(const bc250_hevc_stream_cfg_t *cfg, int is_idr, 
     p.depth8 = (uint8_t *)calloc((size_t)w8 * h8, 1);
     p.mode4 = (uint8_t *)malloc((size_t)(p.w >> 2) * (p.h >> 2));
     p.mv = (int32_t *)calloc((size_t)w8 * h8 * 2, sizeof(int32_t));
+    p.skip8 = (uint8_t *)calloc((size_t)w8 * h8, 1);
     /* the slice data of one slice is at most a few bits per sample */
     const size_t rcap = (size_t)p.w * p.h * 4 + 4096;
     uint8_t *rb = (uint8_t *)malloc(rcap);
     size_t at = 0;
-    int ok = p.slice_of_row && p.depth8 && p.mode4 && p.mv && rb;
+    int ok = p.slice_of_row && p.depth8 && p.mode4 && p.mv && p.skip8 && rb;
     if (ok) {
         memset(p.mode4, 1, (size_t)(p.w >> 2) * (p.h >> 2));
         const int ns = cfg->num_slices;
@@ -1134,6 +1216,7 @@ size_t bc250_hevc_write_picture(const bc250_hevc_stream_cfg_t *cfg, int is_idr, 
     free(p.depth8);
     free(p.mode4);
     free(p.mv);
+    free(p.skip8);
     free(rb);
     return ok ? at : 0;
 }

@@ -151,16 +151,21 @@ def test_writer_refuses_invalid_configurations(pkg):
 @pytest.mark.parametrize("bd", [8, 10])
 def test_still_picture_codes_cu32(pkg, bd):
     """A repeated picture: one vector everywhere, so the CTU32 P pictures
-    are CU32s (with CU16s and CU8s at the partial CTUs of the edges)."""
+    are CU32s (with CU16s and CU8s at the partial CTUs of the edges), and
+    their CUs go as merges and skips: the P pictures cost a few percent of
+    the I picture."""
     w, h = 144, 112
     f0 = _frames(pkg, w, h, 1, bd, 9)[0]
     st = pkg.HevcStream(w, h, bd, 5, 2, True, init_qp=26)
-    s, recs, ry, ruv = st.headers(), [], None, None
+    s, recs, ry, ruv, nbytes = st.headers(), [], None, None, []
     for i in range(3):
         o = orc.hevc_encode_frame(f0[0], f0[1], ry, ruv, 24, i == 0, 2, 16, 1, True, bd, 5)
-        s += st.picture(i == 0, i, 24, o["coef"], o["modes"], o["mv8"])
+        pic = st.picture(i == 0, i, 24, o["coef"], o["modes"], o["mv8"])
+        s += pic
+        nbytes.append(len(pic))
         ry, ruv = o["y"], o["uv"]
         recs.append(o)
     sizes = set(recs[1]["modes"].reshape(-1, 16)[:, :4].ravel().tolist())
     assert 5 in sizes and 4 in sizes, sizes
+    assert max(nbytes[1:]) < nbytes[0] / 10, nbytes
     _assert_decodes_to(s, recs, w, h, bd)
