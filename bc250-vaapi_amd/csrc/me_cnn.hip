@@ -2,17 +2,23 @@
 // over 64x64 luma tiles (current picture + co-located reference) that
 // proposes candidate vectors per 16x16 block, with its training kernels.
 //
-// Network (per 64x64 tile, "same" 3x3 convolutions with a zero border):
-//   input  2 ch  : (cur - 128) / 64, (ref - 128) / 64
-//   conv1  2->16, conv2 16->32, conv3 32->32, conv4 32->32, ReLU after each
-//   head   per 16x16 block: the block mean of conv4 (32) and the 16
-//          neighbour vectors (32 numbers, pel / 16) -> 12 outputs: 4
+// Network (per 64x64 tile, 3x3 convolutions with a one-pixel zero border):
+//   input  2 ch  64x64 : (cur - 128) / 64, (ref - 128) / 64
+//   conv1  2->16,  stride 2 -> 32x32
+//   conv2  16->32, stride 2 -> 16x16
+//   conv3  32->32, stride 1 -> 16x16
+//   conv4  32->32, stride 1 -> 16x16, ReLU after each layer
+//   head   per 16x16 block (a 4x4 cell of conv4): the cell's mean (32) and
+//          the 16 neighbour vectors (32 numbers, pel / 16) -> 12 outputs: 4
 //          hypotheses (dx, dy) in units of 16 pel, then 4 confidence logits
-// Activations and weights are bf16, every product accumulates in fp32.
+// Output pixel (y, x) of a layer of stride S reads input pixels S*y - 1 ..
+// S*y + 1 (rows) and S*x - 1 .. S*x + 1 (columns).  Activations and weights
+// are bf16, every product accumulates in fp32.
 //
-// Layout: activations NHWC per tile with a one-pixel zero border,
-// [tile][66][66][C] bf16; only the 64x64 interior is ever written, so the
-// border stays the zero padding.  A convolution is an implicit GEMM,
+// Layout: activations NHWC per tile with a one-pixel zero border: the input
+// [tile][66][66][2], conv1 [tile][34][34][16], conv2..conv4
+// [tile][18][18][32] bf16; only the interior is ever written by a layer, so
+// the border stays the zero padding.  A convolution is an implicit GEMM,
 // out^T (Cout x 16 pixels) = W (Cout x K) * im2col^T (K x 16 pixels), K =
 // 9*Cin ordered (ky, kx, ci) and zero-padded to a multiple of 32, one
 // v_mfma_f32_16x16x32_bf16 per 16 output channels per 32 K:
@@ -21,11 +27,13 @@
 //   D            lane l: col px = l%16, rows co = 4*(l/16) .. +3
 // so a lane ends with 4 consecutive channels of one pixel (one 8-byte
 // store).  A wave keeps its whole weight tile in VGPRs and walks 16-pixel
-// row segments.  The same kernel computes input gradients (Cin and Cout
-// swapped, weights flipped) with the ReLU mask of the layer below fused into
-// the epilogue.  Weight gradients are a second MFMA kernel over pixel
-// chunks staged in LDS, reduced per tile into a partial buffer and then
-// across tiles in a fixed order, so training is deterministic.
+// row segments.  Input gradients of the stride-1 layers are the same kernel
+// with Cin and Cout swapped and the weights flipped, the ReLU mask of the
+// layer below fused into the epilogue; conv2's (stride 2) is that
+// convolution over the zero-upsampled gradient, read in place.  Weight
+// gradients are a second MFMA kernel over pixel chunks staged in LDS,
+// reduced per tile into a partial buffer and then across tiles in a fixed
+// order, so training is deterministic.
 #include "bc250_dev.h"
 #include "bc250_learned.h"
 
@@ -36,8 +44,11 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int T = 64;        // tile side
+constexpr int T = 64;        // input tile side
 constexpr int TP = T + 2;    // padded side
+constexpr int F1 = 32, F1P = F1 + 2;   // conv1 output side, padded
+constexpr int F2 = 16, F2P = F2 + 2;   // conv2..conv4 output side, padded
+constexpr float POOL_MEAN = BC250_CNN_POOL_SCALE;  // 1 / (4 x 4) cell pixels
 
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
@@ -50,14 +61,12 @@ __device__ __forceinline__ uint32_t pack2bf(float a, float b)
 __device__ __forceinline__ uint16_t f2bf(float f) { return (uint16_t)pack2bf(f, 0.f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Sum over the 16 lanes of a DPP row, complete in the row's last lane:
-// ((v15 + v14) + (v13 + v12)) + ... as a balanced tree (row_shr scan).
-__device__ __forceinline__ float row16_sum(float v)
+// Sum over each group of 4 lanes of a DPP row, complete in the group's last
+// lane (3, 7, 11, 15): (v3 + v2) + (v1 + v0), a row_shr scan.
+__device__ __forceinline__ float row4_sum(float v)
 {
     v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, true));
     v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xf, 0xf, true));
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xf, 0xf, true));
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xf, 0xf, true));
     return v;
 }
 
@@ -65,27 +74,36 @@ constexpr int kpad(int cin) { return (9 * cin + 31) / 32 * 32; }
 
 enum { MODE_RELU = 0, MODE_MASK = 1, MODE_RELU_POOL = 2 };
 
-// im2col operand of one lane: pixel (y, x) of tile `in` (padded
-// coordinates of the output pixel are (y+1, x+1)), k = k0 .. k0+7.
-template <int CIN>
+// im2col operand of one lane: output pixel (y, x) of a layer of stride S,
+// k = k0 .. k0+7, from the padded input `in` of side INP (input element (S*y
+// + ky, S*x + kx) in padded coordinates).  UP = 2: `in` is a gradient of
+// padded side INP/2 + 1 read as its zero-upsampled view of side INP:
+// element (j, i) is in[(j+1)/2][(i+1)/2] when j and i are both odd, else 0.
+template <int CIN, int S, int INP, int UP>
 __device__ __forceinline__ bf16x8 im2col8(const uint16_t *__restrict__ in, int y, int x, int k0)
 {
+    constexpr int BP = UP == 2 ? INP / 2 + 1 : INP;
     bf16x8 v;
     if constexpr (CIN % 8 == 0) {
         const int pos = k0 / CIN, ci = k0 % CIN;
+        uint4 q = make_uint4(0, 0, 0, 0);
         if (pos < 9) {
-            const int ky = pos / 3, kx = pos % 3;
-            const uint4 q = *(const uint4 *)(in + ((size_t)(y + ky) * TP + (x + kx)) * CIN + ci);
-            v = __builtin_bit_cast(bf16x8, q);
-        } else {
-            v = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+            const int jy = S * y + pos / 3, jx = S * x + pos % 3;
+            if constexpr (UP == 2) {
+                if ((jy & 1) && (jx & 1))
+                    q = *(const uint4 *)(in + ((size_t)((jy + 1) >> 1) * BP + ((jx + 1) >> 1)) * CIN + ci);
+            } else {
+                q = *(const uint4 *)(in + ((size_t)jy * INP + jx) * CIN + ci);
+            }
         }
+        v = __builtin_bit_cast(bf16x8, q);
     } else {
+        static_assert(UP == 1, "only whole-chunk channels are read upsampled");
         uint16_t e[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int k = k0 + j, pos = k / CIN, ci = k % CIN;
-            e[j] = pos < 9 ? in[((size_t)(y + pos / 3) * TP + (x + pos % 3)) * CIN + ci] : (uint16_t)0;
+            e[j] = pos < 9 ? in[((size_t)(S * y + pos / 3) * INP + (S * x + pos % 3)) * CIN + ci] : (uint16_t)0;
         }
         v = __builtin_bit_cast(bf16x8, make_uint4(e[0] | (uint32_t)e[1] << 16, e[2] | (uint32_t)e[3] << 16,
                                                   e[4] | (uint32_t)e[5] << 16, e[6] | (uint32_t)e[7] << 16));
@@ -93,19 +111,24 @@ __device__ __forceinline__ bf16x8 im2col8(const uint16_t *__restrict__ in, int y
     return v;
 }
 
-// One 3x3 convolution layer over `ntiles` tiles.  w: [COUT][KPAD] bf16, bias
-// fp32, channel c at bias[c * bstride] (null: none).  MODE_RELU: out = relu(conv + b).  MODE_MASK:
-// out = conv * (mask > 0) (input gradients through the ReLU below).
-// MODE_RELU_POOL: relu(conv + b), its bf16 values summed over each 16-pixel
-// row segment into rows [tile][64 rows][4 segments][COUT] fp32 (`pool`), out
-// optional; pool_rows_kernel then sums 16 rows per block in a fixed order.
-template <int CIN, int COUT, int MODE>
+// One 3x3 convolution layer over `ntiles` tiles: output side OUT (padded
+// OUTP), stride S over an input of side IN (padded INP; UP as im2col8).
+// w: [COUT][KPAD] bf16, bias fp32, channel c at bias[c * bstride] (null:
+// none).  MODE_RELU: out = relu(conv + b).  MODE_MASK: out = conv * (mask >
+// 0) (input gradients through the ReLU below).  MODE_RELU_POOL (conv4):
+// relu(conv + b), its bf16 values summed over each 4-pixel row group into
+// rows [tile][16 rows][4 groups][COUT] fp32 (`pool`), out optional;
+// pool_rows_kernel then sums 4 rows per cell in a fixed order.
+template <int CIN, int COUT, int MODE, int S, int IN, int OUT, int UP>
 __global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
                                                       const float *__restrict__ bias, uint16_t *__restrict__ out,
                                                       const uint16_t *__restrict__ mask, float *__restrict__ pool,
                                                       int ntiles, int bstride)
 {
     constexpr int KS = kpad(CIN) / 32, NT = COUT / 16;
+    constexpr int INP = IN + 2, OUTP = OUT + 2;
+    constexpr size_t IN_TILE = UP == 2 ? (size_t)(INP / 2 + 1) * (INP / 2 + 1) * CIN : (size_t)INP * INP * CIN;
+    static_assert(OUT % 16 == 0 && (MODE != MODE_RELU_POOL || OUT == F2), "row segments of 16 pixels");
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = gridDim.x * 4;
     const int lr = lane & 15, lg = lane >> 4;
@@ -121,21 +144,21 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t *__restrict
 #pragma unroll
         for (int r = 0; r < 4; r++) bv[t][r] = bias ? bias[(size_t)(t * 16 + 4 * lg + r) * bstride] : 0.f;
 
-    // a task is one 64-pixel row of one tile
-    for (int task = wave; task < ntiles * T; task += nwaves) {
-        const int tile = task / T, y = task % T;
-        const uint16_t *tin = in + (size_t)tile * TP * TP * CIN;
-        for (int x0 = 0; x0 < T; x0 += 16) {
+    // a task is one output row of one tile
+    for (int task = wave; task < ntiles * OUT; task += nwaves) {
+        const int tile = task / OUT, y = task % OUT;
+        const uint16_t *tin = in + (size_t)tile * IN_TILE;
+        for (int x0 = 0; x0 < OUT; x0 += 16) {
             f32x4 acc[NT];
 #pragma unroll
             for (int t = 0; t < NT; t++) acc[t] = f32x4{bv[t][0], bv[t][1], bv[t][2], bv[t][3]};
 #pragma unroll
             for (int s = 0; s < KS; s++) {
-                const bf16x8 b = im2col8<CIN>(tin, y, x0 + lr, s * 32 + 8 * lg);
+                const bf16x8 b = im2col8<CIN, S, INP, UP>(tin, y, x0 + lr, s * 32 + 8 * lg);
 #pragma unroll
                 for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][s], b, acc[t], 0, 0, 0);
             }
-            const size_t opix = ((size_t)tile * TP + (y + 1)) * TP + (x0 + lr + 1);
+            const size_t opix = ((size_t)tile * OUTP + (y + 1)) * OUTP + (x0 + lr + 1);
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 float v[4];
@@ -152,14 +175,14 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t *__restrict
                 const uint2 o = make_uint2(pack2bf(v[0], v[1]), pack2bf(v[2], v[3]));
                 if (out) *(uint2 *)(out + opix * COUT + c0) = o;
                 if (MODE == MODE_RELU_POOL) {
-                    // the 16 pixels of this segment are in one 16x16 block:
-                    // sum the bf16-rounded values over the 16 lanes of a row
-                    v[0] = row16_sum(__uint_as_float(o.x << 16));
-                    v[1] = row16_sum(__uint_as_float(o.x & 0xffff0000u));
-                    v[2] = row16_sum(__uint_as_float(o.y << 16));
-                    v[3] = row16_sum(__uint_as_float(o.y & 0xffff0000u));
-                    if (lr == 15)
-                        *(float4 *)(pool + (((size_t)tile * T + y) * 4 + (x0 >> 4)) * COUT + c0) =
+                    // each 4 pixels of this row lie in one cell: sum the
+                    // bf16-rounded values over the 4 lanes of a group
+                    v[0] = row4_sum(__uint_as_float(o.x << 16));
+                    v[1] = row4_sum(__uint_as_float(o.x & 0xffff0000u));
+                    v[2] = row4_sum(__uint_as_float(o.y << 16));
+                    v[3] = row4_sum(__uint_as_float(o.y & 0xffff0000u));
+                    if ((lr & 3) == 3)
+                        *(float4 *)(pool + (((size_t)tile * F2 + y) * 4 + (lr >> 2)) * COUT + c0) =
                             make_float4(v[0], v[1], v[2], v[3]);
                 }
             }
@@ -168,93 +191,86 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const uint16_t *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: the four convolutions and the block sums of one band of
-// FB = 16 output rows (one row of 16x16 blocks) of one tile per workgroup,
-// every intermediate activation held in LDS.  conv4's band needs conv3 rows
-// -1 .. 16, conv2 rows -2 .. 17, conv1 rows -3 .. 18 and input rows -4 .. 19
-// (band-relative); rows outside the tile are the zero padding, so they are
-// stored as zeros and not computed.  Two LDS regions alternate:
-//   Q: input (24 x 66 x 2)  -> P: conv1 (22 x 66 x 16) -> Q: conv2 (20 x 66 x 32)
-//   -> P: conv3 (18 x 66 x 32) -> Q: conv4 row sums (16 x 4 x 32 fp32)
-// (160,512 bytes: one workgroup per CU).  Each stored row keeps the zero
-// columns 0 and 65.  The MFMA operands, their K order and the epilogue are
-// the unfused kernels' (conv3x3_kernel, pool_rows_kernel), so the block sums
-// are the same bits.  conv3 and conv4 reuse each loaded input row for R
-// output rows: a task walks input rows i = 0 .. R+1 and kx = 0 .. 2, and
-// output row j takes the step at ky = i - j, so every accumulator still
-// receives its K steps in the order (ky, kx) = (0,0), (0,1), .. (2,2).
-constexpr int FB = 16;
-constexpr int FP_ELEMS = (FB + 2) * TP * 32;   // conv1 / conv3 output
-constexpr int FQ_ELEMS = (FB + 4) * TP * 32;   // input / conv2 output / conv4 row sums
-static_assert((FB + 6) * TP * 16 <= FP_ELEMS, "conv1 rows fit P");
-static_assert((FB + 8) * TP * 2 <= FQ_ELEMS, "input rows fit Q");
-static_assert(FB * 4 * 32 * 2 <= FQ_ELEMS, "row sums fit Q");
-constexpr int FWAVES = 8;
+// Fused inference: the four convolutions and the cell sums of one tile per
+// workgroup, every intermediate activation held in LDS.  Two regions
+// alternate:
+//   A: input (66 x 66 x 2)  -> B: conv1 (34 x 34 x 16) -> A: conv2 (18 x 18 x 32)
+//   -> B: conv3 (18 x 18 x 32) -> A: conv4 row sums (16 x 4 x 32 fp32)
+// (57,728 bytes: two workgroups per CU).  A layer writes its output's zero
+// border itself (the region held another layout before).  The MFMA
+// operands, their K order and the epilogue are the layer kernels'
+// (conv3x3_kernel, pool_rows_kernel), so the cell sums are the same bits.
+// conv3 and conv4 reuse each loaded input row for R output rows: a task
+// walks input rows i = 0 .. R+1 and kx = 0 .. 2, and output row j takes the
+// step at ky = i - j, so every accumulator still receives its K steps in the
+// order (ky, kx) = (0,0), (0,1), .. (2,2).
+constexpr int FA_ELEMS = F2P * F2P * 32;   // input / conv2 output / conv4 row sums
+constexpr int FB_ELEMS = F1P * F1P * 16;   // conv1 / conv3 output
+static_assert(TP * TP * 2 <= FA_ELEMS, "input fits A");
+static_assert(F2 * 4 * 32 * 2 <= FA_ELEMS, "row sums fit A");
+static_assert(F2P * F2P * 32 <= FB_ELEMS, "conv3 output fits B");
 
-// LDS placement of the 8-channel chunks of a 16- or 32-channel pixel.  The
-// B operand's 16 lanes of one lane group read one chunk of 16 consecutive
-// pixels; unswizzled, pixels 64 (or 32) bytes apart fall on the same banks.
-// SW 1 XORs the chunk index with the pixel bits above the ones that already
-// spread a 16-lane read over 8 banks-groups of 16 bytes (32 banks), SW 2
-// over 16 (64 banks); SW 0 is the plain NHWC order.
-template <int C, int SW>
+// LDS placement of the 8-channel chunks of a 16- or 32-channel pixel in a
+// region of row pitch PITCH pixels.  The B operand's 16 lanes of one lane
+// group read one chunk of 16 pixels S apart; unswizzled, pixels 64 bytes
+// apart fall on the same banks.  SW 1 XORs the chunk index with the pixel
+// bits above the ones that already spread a 16-lane read over 8 bank groups
+// of 16 bytes (32 banks), SW 2 over 16 (64 banks); SW 0 is the plain NHWC
+// order.
+template <int C, int SW, int PITCH>
 __device__ __forceinline__ int chunk_off(int row, int x, int chunk)
 {
     constexpr int C8 = C / 8, LC = C8 == 4 ? 2 : 1;
     constexpr int SH = SW == 0 ? 0 : (SW == 1 ? 3 : 4) - LC;
     const int sw = SW == 0 ? 0 : (x >> SH) & (C8 - 1);
-    return ((row * TP + x) * C8 + (chunk ^ sw)) * 8;
+    return ((row * PITCH + x) * C8 + (chunk ^ sw)) * 8;
 }
 
-// One layer of the fused band: `nrows` output rows in groups of R, dst row r
-// reads src rows r .. r+2; tile row of dst row 0 is `trow0`.  dst rows
-// outside the tile are zero rows.  rowsum (conv4 only): the bf16 values'
-// 16-pixel segment sums, [row][segment][COUT] fp32.
-// The operands one task reads, for each of its SEGS 16-pixel segments:
-// conv3 / conv4 the (R + 2) x 3 input chunks of its rows, conv2 its 5
-// im2col K steps, conv1 its one K step.
-template <int CIN, int R, int SEGS>
+// The operands one task reads, for its 16-pixel segment: conv3 / conv4 the
+// (R + 2) x 3 input chunks of its rows, conv2 its 5 im2col K steps, conv1
+// its one K step.
+template <int CIN, int R>
 struct TaskB {
     static constexpr int PER = CIN == 32 ? (R + 2) * 3 : kpad(CIN) / 32;
-    bf16x8 v[SEGS][PER];
+    bf16x8 v[PER];
 };
 
-template <int CIN, int R, int SEGS, int SW>
+// Operands of the task at output rows r0 .. r0+R-1, pixels x0 .. x0+15 (this
+// lane: pixel x0 + lr, lane group lg) from the padded source of side SRCP.
+template <int CIN, int S, int SRCP, int R, int SW>
 __device__ __forceinline__ void load_task(const uint16_t *__restrict__ src, int r0, int x0, int lr, int lg,
-                                          TaskB<CIN, R, SEGS> &b)
+                                          TaskB<CIN, R> &b)
 {
+    const int x = x0 + lr;
+    if constexpr (CIN == 32) {
+        static_assert(S == 1, "conv3 / conv4");
 #pragma unroll
-    for (int g = 0; g < SEGS; g++) {
-        const int x = x0 + 16 * g + lr;
-        if constexpr (CIN == 32) {
+        for (int i = 0; i < R + 2; i++)
 #pragma unroll
-            for (int i = 0; i < R + 2; i++)
+            for (int kx = 0; kx < 3; kx++)
+                b.v[i * 3 + kx] = __builtin_bit_cast(bf16x8, *(const uint4 *)(src + chunk_off<32, SW, SRCP>(r0 + i, x + kx, lg)));
+    } else if constexpr (CIN == 16) {
+        static_assert(R == 1 && S == 2, "conv2 walks single rows at stride 2");
+        // the last K step holds position 8 in lane groups 0, 1 and zero
+        // padding in 2, 3: read position 8 there too and clear it
 #pragma unroll
-                for (int kx = 0; kx < 3; kx++)
-                    b.v[g][i * 3 + kx] = __builtin_bit_cast(bf16x8, *(const uint4 *)(src + chunk_off<32, SW>(r0 + i, x + kx, lg)));
-        } else if constexpr (CIN == 16) {
-            static_assert(R == 1, "conv2 walks single rows");
-            // the last K step holds position 8 in lane groups 0, 1 and zero
-            // padding in 2, 3: read position 8 there too and clear it
-#pragma unroll
-            for (int s = 0; s < TaskB<CIN, R, SEGS>::PER; s++) {
-                const int pos = 2 * s + (lg >> 1), pc = pos < 9 ? pos : 8;
-                uint4 q = *(const uint4 *)(src + chunk_off<16, SW>(r0 + pc / 3, x + pc % 3, lg & 1));
-                if (pos >= 9) q = make_uint4(0, 0, 0, 0);
-                b.v[g][s] = __builtin_bit_cast(bf16x8, q);
-            }
-        } else {
-            static_assert(CIN == 2 && R == 1, "conv1: one K step");
-            // k = 8 lg .. 8 lg + 7 are positions 4 lg .. 4 lg + 3, two channels each
-            uint32_t e[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int pos = 4 * lg + q, pc = pos < 9 ? pos : 8;
-                e[q] = *(const uint32_t *)(src + ((r0 + pc / 3) * TP + x + pc % 3) * 2);
-                if (pos >= 9) e[q] = 0u;
-            }
-            b.v[g][0] = __builtin_bit_cast(bf16x8, make_uint4(e[0], e[1], e[2], e[3]));
+        for (int s = 0; s < TaskB<CIN, R>::PER; s++) {
+            const int pos = 2 * s + (lg >> 1), pc = pos < 9 ? pos : 8;
+            uint4 q = *(const uint4 *)(src + chunk_off<16, SW, SRCP>(2 * r0 + pc / 3, 2 * x + pc % 3, lg & 1));
+            if (pos >= 9) q = make_uint4(0, 0, 0, 0);
+            b.v[s] = __builtin_bit_cast(bf16x8, q);
         }
+    } else {
+        static_assert(CIN == 2 && R == 1 && S == 2, "conv1: one K step at stride 2");
+        // k = 8 lg .. 8 lg + 7 are positions 4 lg .. 4 lg + 3, two channels each
+        uint32_t e[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int pos = 4 * lg + q, pc = pos < 9 ? pos : 8;
+            e[q] = *(const uint32_t *)(src + ((2 * r0 + pc / 3) * SRCP + 2 * x + pc % 3) * 2);
+            if (pos >= 9) e[q] = 0u;
+        }
+        b.v[0] = __builtin_bit_cast(bf16x8, make_uint4(e[0], e[1], e[2], e[3]));
     }
 }
 
@@ -282,88 +298,90 @@ __device__ __forceinline__ void load_w(const uint16_t *__restrict__ w, const flo
         for (int r = 0; r < 4; r++) lw.b[t][r] = bias[(size_t)(t * 16 + 4 * lg + r) * (LW::KP + 1)];
 }
 
-// One layer of the fused band: `nrows` output rows, a task being R rows by
-// SEGS 16-pixel segments; dst row r reads src rows r .. r+2; tile row of dst
-// row 0 is `trow0`.  dst rows outside the tile are zero rows.  rowsum (conv4
-// only): the bf16 values' 16-pixel segment sums, [row][segment][COUT] fp32.
-// A wave loads the next task's operands before the current task's
-// epilogue, so their LDS latency overlaps the conversions and stores.
-template <int CIN, int COUT, int R, int SEGS, int SW>
-__device__ __forceinline__ void fused_layer(const uint16_t *__restrict__ src, uint16_t *__restrict__ dst, int nrows,
-                                            int trow0, const LayerW<CIN, COUT> &lw, float *__restrict__ rowsum)
+// Zero border of a padded C-channel region of side P (rows 0 and P-1, columns
+// 0 and P-1), all threads.
+template <int C, int SW, int P>
+__device__ __forceinline__ void zero_border(uint16_t *__restrict__ dst, int nthreads)
 {
-    constexpr int NT = COUT / 16, TPR = 4 / SEGS;  // tasks per row group
+    constexpr int C8 = C / 8, NB = 4 * (P - 1);
+    for (int i = threadIdx.x; i < NB * C8; i += nthreads) {
+        const int b = i / C8, ch = i % C8;
+        int row, x;
+        if (b < P) { row = 0; x = b; }
+        else if (b < 2 * P) { row = P - 1; x = b - P; }
+        else if (b < 3 * P - 2) { row = b - 2 * P + 1; x = 0; }
+        else { row = b - (3 * P - 2) + 1; x = P - 1; }
+        *(uint4 *)(dst + chunk_off<C, SW, P>(row, x, ch)) = make_uint4(0, 0, 0, 0);
+    }
+}
+
+// One layer of the fused tile: output side OUT (padded OUTP), tasks of R
+// rows by one 16-pixel segment over NW waves; the source region is padded
+// side SRCP.  rowsum (conv4 only): the bf16 values' 4-pixel group sums,
+// [row][group][COUT] fp32.  A wave loads the next task's operands before the
+// current task's epilogue, so their LDS latency overlaps the conversions
+// and stores.
+template <int CIN, int COUT, int S, int SRCP, int OUTP, int R, int NW, int SW>
+__device__ __forceinline__ void fused_layer(const uint16_t *__restrict__ src, uint16_t *__restrict__ dst,
+                                            const LayerW<CIN, COUT> &lw, float *__restrict__ rowsum)
+{
+    constexpr int OUT = OUTP - 2, NT = COUT / 16, SPR = OUT / 16;  // segments per row
+    constexpr int NTASK = OUT / R * SPR;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lg = lane >> 4;
-    const int ntask = nrows / R * TPR;
-    TaskB<CIN, R, SEGS> b;
-    if (wave < ntask) load_task<CIN, R, SEGS, SW>(src, wave / TPR * R, wave % TPR * SEGS * 16, lr, lg, b);
-    for (int task = wave; task < ntask; task += FWAVES) {
-        const int r0 = task / TPR * R, x0 = task % TPR * SEGS * 16;
-        f32x4 acc[SEGS][R][NT];
+    TaskB<CIN, R> b;
+    if (wave < NTASK) load_task<CIN, S, SRCP, R, SW>(src, wave / SPR * R, wave % SPR * 16, lr, lg, b);
+    for (int task = wave; task < NTASK; task += NW) {
+        const int r0 = task / SPR * R, x0 = task % SPR * 16;
+        f32x4 acc[R][NT];
 #pragma unroll
-        for (int g = 0; g < SEGS; g++)
+        for (int j = 0; j < R; j++)
 #pragma unroll
-            for (int j = 0; j < R; j++)
+            for (int t = 0; t < NT; t++) acc[j][t] = f32x4{lw.b[t][0], lw.b[t][1], lw.b[t][2], lw.b[t][3]};
+        if constexpr (CIN == 32) {
 #pragma unroll
-                for (int t = 0; t < NT; t++) acc[g][j][t] = f32x4{lw.b[t][0], lw.b[t][1], lw.b[t][2], lw.b[t][3]};
+            for (int i = 0; i < R + 2; i++)
 #pragma unroll
-        for (int g = 0; g < SEGS; g++) {
-            if constexpr (CIN == 32) {
+                for (int kx = 0; kx < 3; kx++)
 #pragma unroll
-                for (int i = 0; i < R + 2; i++)
+                    for (int j = 0; j < R; j++) {
+                        const int ky = i - j;
+                        if (ky >= 0 && ky <= 2)
 #pragma unroll
-                    for (int kx = 0; kx < 3; kx++)
+                            for (int t = 0; t < NT; t++)
+                                acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lw.a[t][3 * ky + kx], b.v[i * 3 + kx],
+                                                                                    acc[j][t], 0, 0, 0);
+                    }
+        } else {
 #pragma unroll
-                        for (int j = 0; j < R; j++) {
-                            const int ky = i - j;
-                            if (ky >= 0 && ky <= 2)
+            for (int s = 0; s < TaskB<CIN, R>::PER; s++)
 #pragma unroll
-                                for (int t = 0; t < NT; t++)
-                                    acc[g][j][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                        lw.a[t][3 * ky + kx], b.v[g][i * 3 + kx], acc[g][j][t], 0, 0, 0);
-                        }
-            } else {
-#pragma unroll
-                for (int s = 0; s < TaskB<CIN, R, SEGS>::PER; s++)
-#pragma unroll
-                    for (int t = 0; t < NT; t++)
-                        acc[g][0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lw.a[t][s], b.v[g][s], acc[g][0][t], 0, 0, 0);
-            }
+                for (int t = 0; t < NT; t++)
+                    acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lw.a[t][s], b.v[s], acc[0][t], 0, 0, 0);
         }
         __builtin_amdgcn_sched_barrier(0);
-        const int nx = task + FWAVES;
-        if (nx < ntask) load_task<CIN, R, SEGS, SW>(src, nx / TPR * R, nx % TPR * SEGS * 16, lr, lg, b);
+        const int nx = task + NW;
+        if (nx < NTASK) load_task<CIN, S, SRCP, R, SW>(src, nx / SPR * R, nx % SPR * 16, lr, lg, b);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int g = 0; g < SEGS; g++) {
-            const int xs = x0 + 16 * g;
+        for (int j = 0; j < R; j++) {
+            const int row = r0 + j;
 #pragma unroll
-            for (int j = 0; j < R; j++) {
-                const int row = r0 + j, tr = trow0 + row;
-                const bool inside = tr >= 0 && tr < T;
+            for (int t = 0; t < NT; t++) {
+                float v[4];
 #pragma unroll
-                for (int t = 0; t < NT; t++) {
-                    float v[4];
-#pragma unroll
-                    for (int r = 0; r < 4; r++) v[r] = inside ? fmaxf(acc[g][j][t][r], 0.f) : 0.f;
-                    const int c0 = t * 16 + 4 * lg;
-                    const uint2 o = make_uint2(pack2bf(v[0], v[1]), pack2bf(v[2], v[3]));
-                    if (dst) {
-                        const int ch = c0 >> 3, half = c0 & 7;
-                        *(uint2 *)(dst + chunk_off<COUT, SW>(row, xs + lr + 1, ch) + half) = o;
-                        if (xs == 0 && lr == 0) *(uint2 *)(dst + chunk_off<COUT, SW>(row, 0, ch) + half) = make_uint2(0, 0);
-                        if (xs == T - 16 && lr == 15)
-                            *(uint2 *)(dst + chunk_off<COUT, SW>(row, TP - 1, ch) + half) = make_uint2(0, 0);
-                    }
-                    if (rowsum) {
-                        v[0] = row16_sum(__uint_as_float(o.x << 16));
-                        v[1] = row16_sum(__uint_as_float(o.x & 0xffff0000u));
-                        v[2] = row16_sum(__uint_as_float(o.y << 16));
-                        v[3] = row16_sum(__uint_as_float(o.y & 0xffff0000u));
-                        if (lr == 15)
-                            *(float4 *)(rowsum + ((row * 4) + (xs >> 4)) * COUT + c0) = make_float4(v[0], v[1], v[2], v[3]);
-                    }
+                for (int r = 0; r < 4; r++) v[r] = fmaxf(acc[j][t][r], 0.f);
+                const int c0 = t * 16 + 4 * lg;
+                const uint2 o = make_uint2(pack2bf(v[0], v[1]), pack2bf(v[2], v[3]));
+                if (dst) *(uint2 *)(dst + chunk_off<COUT, SW, OUTP>(row + 1, x0 + lr + 1, c0 >> 3) + (c0 & 7)) = o;
+                if (rowsum) {
+                    v[0] = row4_sum(__uint_as_float(o.x << 16));
+                    v[1] = row4_sum(__uint_as_float(o.x & 0xffff0000u));
+                    v[2] = row4_sum(__uint_as_float(o.y << 16));
+                    v[3] = row4_sum(__uint_as_float(o.y & 0xffff0000u));
+                    if ((lr & 3) == 3)
+                        *(float4 *)(rowsum + ((row * 4) + ((x0 + lr) >> 2)) * COUT + c0) =
+                            make_float4(v[0], v[1], v[2], v[3]);
                 }
             }
         }
@@ -371,18 +389,18 @@ __device__ __forceinline__ void fused_layer(const uint16_t *__restrict__ src, ui
 }
 
 // in: input tiles [tile][66][66][2]; packed: forward weights; params: fp32
-// parameters (biases); pool: [tile][16][32] block sums of conv4.
+// parameters (biases); pool: [tile][16][32] cell sums of conv4.
 struct FusedWeights {
     const uint16_t *w[4];   // forward weights [COUT][KPAD] per layer
     const float *b[4];      // biases, channel c at b[c * (KPAD + 1)]
 };
 
-// Where a band's input comes from: the input tiles [tile][66][66][2]
+// Where a tile's input comes from: the input tiles [tile][66][66][2]
 // (FROM_TILES), one picture's 8-bit luma pair (FROM_PICTURE: tile i is
 // (i % tw, i / tw) of the picture), or a batch of pictures (FROM_BATCH).
 // Staged values are the input tiles' own: (sample - 128) / 64 as bf16,
-// samples past the picture repeating its last row / column, zero outside
-// the tile.
+// samples past the picture repeating its last row / column, zero on the
+// border.
 enum { FROM_TILES = 0, FROM_PICTURE = 1, FROM_BATCH = 2 };
 struct FusedInput {
     const uint16_t *tiles;
@@ -390,12 +408,12 @@ struct FusedInput {
     int cpitch, rpitch, w, h, tw;
 };
 
-__device__ __forceinline__ void stage_picture_rows(uint32_t *__restrict__ q, const uint8_t *__restrict__ cur,
-                                                   int cpitch, const uint8_t *__restrict__ ref, int rpitch, int w,
-                                                   int h, int tx, int ty, int y0)
+__device__ __forceinline__ void stage_picture(uint32_t *__restrict__ q, const uint8_t *__restrict__ cur, int cpitch,
+                                              const uint8_t *__restrict__ ref, int rpitch, int w, int h, int tx,
+                                              int ty, int nthreads)
 {
-    for (int i = threadIdx.x; i < (FB + 8) * TP; i += FWAVES * 64) {
-        const int r = i / TP, x = i % TP, tr = y0 - 4 + r, tc = x - 1;
+    for (int i = threadIdx.x; i < TP * TP; i += nthreads) {
+        const int tr = i / TP - 1, tc = i % TP - 1;
         uint32_t v = 0u;
         if (tr >= 0 && tr < T && tc >= 0 && tc < T) {
             const int gy = min(ty * T + tr, h - 1), gx = min(tx * T + tc, w - 1);
@@ -407,13 +425,13 @@ __device__ __forceinline__ void stage_picture_rows(uint32_t *__restrict__ q, con
     }
 }
 
-template <int SW, int FROM>
-__device__ __forceinline__ void cnn_forward_band(const FusedInput &src, const bc250_me_batch_t *bt,
+template <int NW, int R, int SW, int FROM>
+__device__ __forceinline__ void cnn_forward_tile(const FusedInput &src, const bc250_me_batch_t *bt,
                                                  const FusedWeights &fw, float *__restrict__ pool, int tile,
-                                                 int band, uint16_t *sp, uint16_t *sq)
+                                                 uint16_t *sa, uint16_t *sb)
 {
+    constexpr int NTH = NW * 64;
     const int tid = threadIdx.x;
-    const int y0 = band * FB;  // tile row of the band's first output row
     // the weights of conv1..conv3 are requested first (their latency hides
     // under the input staging), conv4's at the end of conv3, into conv3's
     // registers
@@ -423,109 +441,123 @@ __device__ __forceinline__ void cnn_forward_band(const FusedInput &src, const bc
     load_w(fw.w[0], fw.b[0], w1);
     load_w(fw.w[1], fw.b[1], w2);
     load_w(fw.w[2], fw.b[2], w3);
-    // input rows y0-4 .. y0+FB+3
     if constexpr (FROM == FROM_TILES) {
-        // padded global row = tile row + 1
         const uint32_t *g = (const uint32_t *)(src.tiles + (size_t)tile * TP * TP * 2);
-        uint32_t *q = (uint32_t *)sq;
-        for (int i = tid; i < (FB + 8) * TP; i += FWAVES * 64) {
-            const int r = i / TP, x = i % TP, tr = y0 - 4 + r;
-            q[i] = (tr >= -1 && tr <= T) ? g[(tr + 1) * TP + x] : 0u;
-        }
+        uint32_t *q = (uint32_t *)sa;
+        for (int i = tid; i < TP * TP; i += NTH) q[i] = g[i];
     } else if constexpr (FROM == FROM_PICTURE) {
-        stage_picture_rows((uint32_t *)sq, src.cur, src.cpitch, src.ref, src.rpitch, src.w, src.h, tile % src.tw,
-                           tile / src.tw, y0);
+        stage_picture((uint32_t *)sa, src.cur, src.cpitch, src.ref, src.rpitch, src.w, src.h, tile % src.tw,
+                      tile / src.tw, NTH);
     } else {
         const bc250_batch_frame_t &fr = bt->f[batch_frame_of_tile(*bt, tile)];
         const int tw = (fr.wc / 16 + 3) / 4, lt = tile - fr.tile_base;
-        stage_picture_rows((uint32_t *)sq, fr.cur, fr.cpitch, fr.ref, fr.rpitch, fr.wc, fr.hc, lt % tw, lt / tw, y0);
+        stage_picture((uint32_t *)sa, fr.cur, fr.cpitch, fr.ref, fr.rpitch, fr.wc, fr.hc, lt % tw, lt / tw, NTH);
     }
     __syncthreads();
-    fused_layer<2, 16, 1, 4, SW>(sq, sp, FB + 6, y0 - 3, w1, nullptr);
+    zero_border<16, SW, F1P>(sb, NTH);
+    fused_layer<2, 16, 2, TP, F1P, 1, NW, SW>(sa, sb, w1, nullptr);
     __syncthreads();
-    fused_layer<16, 32, 1, 2, SW>(sp, sq, FB + 4, y0 - 2, w2, nullptr);
+    zero_border<32, SW, F2P>(sa, NTH);
+    fused_layer<16, 32, 2, F1P, F2P, 1, NW, SW>(sb, sa, w2, nullptr);
     __syncthreads();
-    fused_layer<32, 32, 3, 1, SW>(sq, sp, FB + 2, y0 - 1, w3, nullptr);
+    zero_border<32, SW, F2P>(sb, NTH);
+    fused_layer<32, 32, 1, F2P, F2P, R, NW, SW>(sa, sb, w3, nullptr);
     load_w(fw.w[3], fw.b[3], w3);
     __syncthreads();
-    float *rows = (float *)sq;
-    fused_layer<32, 32, 4, 1, SW>(sp, nullptr, FB, y0, w3, rows);
+    float *rows = (float *)sa;
+    fused_layer<32, 32, 1, F2P, F2P, R, NW, SW>(sb, nullptr, w3, rows);
     __syncthreads();
-    if (tid < 4 * 32) {
-        const int bx = tid >> 5, c = tid & 31;
+    for (int i = tid; i < 16 * 32; i += NTH) {
+        const int cell = i >> 5, c = i & 31, cy = cell >> 2, cx = cell & 3;
         float s = 0.f;
-        for (int y = 0; y < FB; y++) s += rows[(y * 4 + bx) * 32 + c];
-        pool[((size_t)tile * 16 + band * 4 + bx) * 32 + c] = s;
+        for (int y = 4 * cy; y < 4 * cy + 4; y++) s += rows[(y * 4 + cx) * 32 + c];
+        pool[((size_t)tile * 16 + cell) * 32 + c] = s;
     }
 }
 
-template <int SW>
-__global__ __launch_bounds__(FWAVES * 64) void cnn_forward_fused_kernel(const uint16_t *__restrict__ in,
-                                                                        const FusedWeights fw,
-                                                                        float *__restrict__ pool, int ntiles)
+// The launch shapes of the fused forward (the serving autotuner's choice):
+// variant 0: 4 waves, 4-row tasks, plain LDS order; 1: 4 waves, 4-row tasks,
+// 32-bank swizzle; 2: 8 waves, 2-row tasks, 64-bank swizzle.
+template <int V> struct FwdShape;
+template <> struct FwdShape<0> { static constexpr int NW = 4, R = 4, SW = 0; };
+template <> struct FwdShape<1> { static constexpr int NW = 4, R = 4, SW = 1; };
+template <> struct FwdShape<2> { static constexpr int NW = 8, R = 2, SW = 2; };
+
+template <int V>
+__global__ __launch_bounds__(FwdShape<V>::NW * 64) void cnn_forward_fused_kernel(const uint16_t *__restrict__ in,
+                                                                                const FusedWeights fw,
+                                                                                float *__restrict__ pool, int ntiles)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t sp[FP_ELEMS];
-    __shared__ __attribute__((aligned(16))) uint16_t sq[FQ_ELEMS];
-    const int tile = blockIdx.x >> 2;
+    using FS = FwdShape<V>;
+    __shared__ __attribute__((aligned(16))) uint16_t sa[FA_ELEMS];
+    __shared__ __attribute__((aligned(16))) uint16_t sb[FB_ELEMS];
+    const int tile = blockIdx.x;
     if (tile >= ntiles) return;
     FusedInput src{};
     src.tiles = in;
-    cnn_forward_band<SW, FROM_TILES>(src, nullptr, fw, pool, tile, blockIdx.x & 3, sp, sq);
+    cnn_forward_tile<FS::NW, FS::R, FS::SW, FROM_TILES>(src, nullptr, fw, pool, tile, sa, sb);
 }
 
-template <int SW>
-__global__ __launch_bounds__(FWAVES * 64) void cnn_forward_picture_kernel(const FusedInput src, const FusedWeights fw,
-                                                                          float *__restrict__ pool, int ntiles)
+template <int V>
+__global__ __launch_bounds__(FwdShape<V>::NW * 64) void cnn_forward_picture_kernel(const FusedInput src,
+                                                                                  const FusedWeights fw,
+                                                                                  float *__restrict__ pool, int ntiles)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t sp[FP_ELEMS];
-    __shared__ __attribute__((aligned(16))) uint16_t sq[FQ_ELEMS];
-    const int tile = blockIdx.x >> 2;
+    using FS = FwdShape<V>;
+    __shared__ __attribute__((aligned(16))) uint16_t sa[FA_ELEMS];
+    __shared__ __attribute__((aligned(16))) uint16_t sb[FB_ELEMS];
+    const int tile = blockIdx.x;
     if (tile >= ntiles) return;
-    cnn_forward_band<SW, FROM_PICTURE>(src, nullptr, fw, pool, tile, blockIdx.x & 3, sp, sq);
+    cnn_forward_tile<FS::NW, FS::R, FS::SW, FROM_PICTURE>(src, nullptr, fw, pool, tile, sa, sb);
 }
 
-template <int SW>
-__global__ __launch_bounds__(FWAVES * 64) void cnn_forward_batch_kernel(const bc250_me_batch_t b, const FusedWeights fw,
-                                                                        float *__restrict__ pool)
+template <int V>
+__global__ __launch_bounds__(FwdShape<V>::NW * 64) void cnn_forward_batch_kernel(const bc250_me_batch_t b,
+                                                                                const FusedWeights fw,
+                                                                                float *__restrict__ pool)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t sp[FP_ELEMS];
-    __shared__ __attribute__((aligned(16))) uint16_t sq[FQ_ELEMS];
-    const int tile = blockIdx.x >> 2;
+    using FS = FwdShape<V>;
+    __shared__ __attribute__((aligned(16))) uint16_t sa[FA_ELEMS];
+    __shared__ __attribute__((aligned(16))) uint16_t sb[FB_ELEMS];
+    const int tile = blockIdx.x;
     if (tile >= b.ntile_total) return;
     FusedInput src{};
-    cnn_forward_band<SW, FROM_BATCH>(src, &b, fw, pool, tile, blockIdx.x & 3, sp, sq);
+    cnn_forward_tile<FS::NW, FS::R, FS::SW, FROM_BATCH>(src, &b, fw, pool, tile, sa, sb);
 }
 
-// Weight (and bias) gradients of one layer: dW[co][k] = sum over pixels of
-// dz[p][co] * im2col(a)[p][k], db[co] = sum dz[p][co].  One workgroup per
-// tile, pixel chunks of 32 staged in LDS; part[tile] = [COUT][KPAD + 1] fp32
-// (the last column is db).
-template <int CIN, int COUT>
+// Weight (and bias) gradients of one layer (stride S, input side IN, output
+// side OUT): dW[co][k] = sum over output pixels p of dz[p][co] *
+// im2col(a)[p][k], db[co] = sum dz[p][co].  One workgroup per tile, chunks of
+// 32 output pixels staged in LDS; part[tile] = [COUT][KPAD + 1] fp32 (the
+// last column is db).
+template <int CIN, int COUT, int S, int IN, int OUT>
 __global__ __launch_bounds__(256) void conv_wgrad_kernel(const uint16_t *__restrict__ a, const uint16_t *__restrict__ dz,
                                                          float *__restrict__ part)
 {
     constexpr int KP = kpad(CIN), NT = COUT / 16, NK = KP / 16, NOUT = NT * NK;
     constexpr int PER_WAVE = (NOUT + 3) / 4;
+    constexpr int INP = IN + 2, OUTP = OUT + 2;
     __shared__ uint16_t sa[32][KP + 8];   // im2col rows of 32 pixels
     __shared__ uint16_t sd[32][COUT + 8]; // dz of the same pixels
     const int tile = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 15, lg = lane >> 4;
-    const uint16_t *ta = a + (size_t)tile * TP * TP * CIN;
-    const uint16_t *td = dz + (size_t)tile * TP * TP * COUT;
+    const uint16_t *ta = a + (size_t)tile * INP * INP * CIN;
+    const uint16_t *td = dz + (size_t)tile * OUTP * OUTP * COUT;
     f32x4 acc[PER_WAVE];
 #pragma unroll
     for (int i = 0; i < PER_WAVE; i++) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
     float dbs = 0.f;
-    for (int chunk = 0; chunk < T * T / 32; chunk++) {
-        const int y = chunk / 2, x0 = (chunk & 1) * 32;
+    for (int chunk = 0; chunk < OUT * OUT / 32; chunk++) {
         __syncthreads();
         for (int i = tid; i < 32 * KP; i += 256) {
             const int p = i / KP, k = i % KP, pos = k / CIN, ci = k % CIN;
-            sa[p][k] = pos < 9 ? ta[((size_t)(y + pos / 3) * TP + (x0 + p + pos % 3)) * CIN + ci] : (uint16_t)0;
+            const int y = (chunk * 32 + p) / OUT, x = (chunk * 32 + p) % OUT;
+            sa[p][k] = pos < 9 ? ta[((size_t)(S * y + pos / 3) * INP + (S * x + pos % 3)) * CIN + ci] : (uint16_t)0;
         }
         for (int i = tid; i < 32 * COUT; i += 256) {
             const int p = i / COUT, c = i % COUT;
-            sd[p][c] = td[((size_t)(y + 1) * TP + (x0 + p + 1)) * COUT + c];
+            const int y = (chunk * 32 + p) / OUT, x = (chunk * 32 + p) % OUT;
+            sd[p][c] = td[((size_t)(y + 1) * OUTP + (x + 1)) * COUT + c];
         }
         __syncthreads();
         if (tid < COUT)
@@ -564,16 +596,16 @@ __global__ __launch_bounds__(256) void conv_wgrad_kernel(const uint16_t *__restr
     if (tid < COUT) pt[(size_t)tid * (KP + 1) + KP] = dbs;
 }
 
-// Block sums of conv4 from its row-segment sums: pool[tile][by*4+bx][c] =
-// sum over y in 16 by .. 16 by + 15 of rows[tile][y][bx][c], in order.
+// Cell sums of conv4 from its row-group sums: pool[tile][cy*4+cx][c] =
+// sum over y in 4 cy .. 4 cy + 3 of rows[tile][y][cx][c], in order.
 __global__ void pool_rows_kernel(const float *__restrict__ rows, float *__restrict__ pool, int ntiles)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (tile, block, c)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (tile, cell, c)
     if (i >= ntiles * 16 * 32) return;
-    const int c = i % 32, blk = (i / 32) % 16, tile = i / 512;
-    const int by = blk >> 2, bx = blk & 3;
+    const int c = i % 32, cell = (i / 32) % 16, tile = i / 512;
+    const int cy = cell >> 2, cx = cell & 3;
     float s = 0.f;
-    for (int y = 16 * by; y < 16 * by + 16; y++) s += rows[(((size_t)tile * T + y) * 4 + bx) * 32 + c];
+    for (int y = 4 * cy; y < 4 * cy + 4; y++) s += rows[(((size_t)tile * F2 + y) * 4 + cx) * 32 + c];
     pool[i] = s;
 }
 
@@ -625,7 +657,8 @@ __global__ void cnn_input_batch_kernel(const bc250_me_batch_t b, uint16_t *__res
     }
 }
 
-// Head of one 16x16 block: z = Wh [pool/256 ; mv] + bh (12 outputs).
+// Head of one 16x16 block: z = Wh [pool/16 ; mv] + bh (12 outputs): pool is the
+// sum over the block's 4x4 cell of conv4, pool/16 its mean.
 __device__ __forceinline__ void head_block(const float *__restrict__ wh, const float *__restrict__ pool,
                                            const float *__restrict__ mv, float *z)
 {
@@ -633,7 +666,7 @@ __device__ __forceinline__ void head_block(const float *__restrict__ wh, const f
     for (int o = 0; o < BC250_CNN_NOUT; o++) {
         const float *row = wh + o * (BC250_CNN_NFEAT + 1);
         float s = row[BC250_CNN_NFEAT];
-        for (int i = 0; i < 32; i++) s += row[i] * (pool[i] * (1.f / 256.f));
+        for (int i = 0; i < 32; i++) s += row[i] * (pool[i] * POOL_MEAN);
         for (int i = 0; i < 32; i++) s += row[32 + i] * mv[i];
         z[o] = s;
     }
@@ -655,10 +688,10 @@ __global__ void cnn_head_kernel(const float *__restrict__ wh, const float *__res
     float s = row[BC250_CNN_NFEAT];
     for (int q = 0; q < 8; q++) {
         const float4 v = pb[q];
-        s = fmaf(row[4 * q], v.x * (1.f / 256.f), s);
-        s = fmaf(row[4 * q + 1], v.y * (1.f / 256.f), s);
-        s = fmaf(row[4 * q + 2], v.z * (1.f / 256.f), s);
-        s = fmaf(row[4 * q + 3], v.w * (1.f / 256.f), s);
+        s = fmaf(row[4 * q], v.x * POOL_MEAN, s);
+        s = fmaf(row[4 * q + 1], v.y * POOL_MEAN, s);
+        s = fmaf(row[4 * q + 2], v.z * POOL_MEAN, s);
+        s = fmaf(row[4 * q + 3], v.w * POOL_MEAN, s);
     }
     for (int q = 0; q < 8; q++) {
         const float4 v = mb[q];
@@ -728,7 +761,7 @@ __global__ void cnn_head_train_kernel(const float *__restrict__ wh, const float 
         total += lb;
         for (int o = 0; o < BC250_CNN_NOUT; o++) dz[o] *= scale;
         float feat[BC250_CNN_NFEAT];
-        for (int i = 0; i < 32; i++) { feat[i] = pb[i] * (1.f / 256.f); feat[32 + i] = mb[i]; }
+        for (int i = 0; i < 32; i++) { feat[i] = pb[i] * POOL_MEAN; feat[32 + i] = mb[i]; }
         for (int o = 0; o < BC250_CNN_NOUT; o++) {
             float *row = pt + o * NIN;
             for (int i = 0; i < BC250_CNN_NFEAT; i++) row[i] += dz[o] * feat[i];
@@ -737,25 +770,25 @@ __global__ void cnn_head_train_kernel(const float *__restrict__ wh, const float 
         for (int i = 0; i < 32; i++) {
             float s = 0.f;
             for (int o = 0; o < BC250_CNN_NOUT; o++) s += wh[o * NIN + i] * dz[o];
-            dpool[b * 32 + i] = s * (1.f / 256.f);
+            dpool[b * 32 + i] = s * POOL_MEAN;
         }
     }
     loss_out[tile] = total;
 }
 
-// d conv4 output -> its pre-activation gradient: dz4[p][c] = dpool[block(p)][c]
+// d conv4 output -> its pre-activation gradient: dz4[p][c] = dpool[cell(p)][c]
 // where conv4's output a4[p][c] > 0, else 0 (bf16, padded layout).
 __global__ void cnn_pool_backward_kernel(const float *__restrict__ dpool, const uint16_t *__restrict__ a4,
                                          uint16_t *__restrict__ dz4, int ntiles)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t n = (size_t)ntiles * T * T * 32;
+    const size_t n = (size_t)ntiles * F2 * F2 * 32;
     if (i >= n) return;
     const int c = (int)(i % 32);
     const size_t p = i / 32;
-    const int tile = (int)(p / (T * T)), yy = (int)(p % (T * T)) / T, xx = (int)(p % T);
-    const size_t o = (((size_t)tile * TP + yy + 1) * TP + xx + 1) * 32 + c;
-    const float g = dpool[((size_t)tile * 16 + (yy >> 4) * 4 + (xx >> 4)) * 32 + c];
+    const int tile = (int)(p / (F2 * F2)), yy = (int)(p % (F2 * F2)) / F2, xx = (int)(p % F2);
+    const size_t o = (((size_t)tile * F2P + yy + 1) * F2P + xx + 1) * 32 + c;
+    const float g = dpool[((size_t)tile * 16 + (yy >> 2) * 4 + (xx >> 2)) * 32 + c];
     dz4[o] = bf2f(a4[o]) > 0.f ? f2bf(g) : (uint16_t)0;
 }
 
@@ -795,7 +828,15 @@ __global__ void pack_weights_kernel(const float *__restrict__ w, int cin, int co
 
 extern "C" size_t bc250_cnn_param_count(void) { return BC250_CNN_NPARAM; }
 
-extern "C" size_t bc250_cnn_tile_bytes(int channels) { return (size_t)TP * TP * channels * 2; }
+extern "C" size_t bc250_cnn_tile_bytes(int layer)
+{
+    switch (layer) {
+    case 0: return (size_t)TP * TP * 2 * 2;
+    case 1: return (size_t)F1P * F1P * 16 * 2;
+    case 2: case 3: case 4: return (size_t)F2P * F2P * 32 * 2;
+    default: return 0;
+    }
+}
 
 extern "C" int bc250_cnn_input(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, size_t slot_stride,
                                int w, int h, const int *tiles, int ntiles, uint16_t *in, void *stream)
@@ -815,9 +856,9 @@ extern "C" int bc250_cnn_set_grid_cap(int cap)
     g_conv_grid_cap = cap;
     return 0;
 }
-static int conv_grid(int ntiles)
+static int conv_grid(int ntiles, int out_side)
 {
-    const int rows = ntiles * T;
+    const int rows = ntiles * out_side;
     const int g = (rows + 3) / 4;
     return g < g_conv_grid_cap ? g : g_conv_grid_cap;
 }
@@ -829,27 +870,43 @@ extern "C" int bc250_cnn_conv(int layer, int mode, const uint16_t *in, const uin
     // the bias is the last column of the layer's [COUT][KPAD + 1] parameter matrix
     const int bstride = kpad(layer == 1 ? 2 : layer == 2 ? 16 : 32) + 1;
     hipStream_t st = (hipStream_t)stream;
-    const int g = conv_grid(ntiles);
-    // forward: mode 0 (ReLU) or 2 (ReLU + block sums, layer 4 only);
+    // forward: mode 0 (ReLU) or 2 (ReLU + cell sums, layer 4 only);
     // input gradients: mode 1, layer = the layer whose input is differentiated
+    // (layer 2's stride-2 input gradient reads its dz zero-upsampled)
     if (mode == MODE_RELU) {
         if (!out) return -1;
         switch (layer) {
-        case 1: conv3x3_kernel<2, 16, MODE_RELU><<<g, 256, 0, st>>>(in, w, bias, out, nullptr, nullptr, ntiles, bstride); break;
-        case 2: conv3x3_kernel<16, 32, MODE_RELU><<<g, 256, 0, st>>>(in, w, bias, out, nullptr, nullptr, ntiles, bstride); break;
+        case 1:
+            conv3x3_kernel<2, 16, MODE_RELU, 2, T, F1, 1><<<conv_grid(ntiles, F1), 256, 0, st>>>(
+                in, w, bias, out, nullptr, nullptr, ntiles, bstride);
+            break;
+        case 2:
+            conv3x3_kernel<16, 32, MODE_RELU, 2, F1, F2, 1><<<conv_grid(ntiles, F2), 256, 0, st>>>(
+                in, w, bias, out, nullptr, nullptr, ntiles, bstride);
+            break;
         case 3:
-        case 4: conv3x3_kernel<32, 32, MODE_RELU><<<g, 256, 0, st>>>(in, w, bias, out, nullptr, nullptr, ntiles, bstride); break;
+        case 4:
+            conv3x3_kernel<32, 32, MODE_RELU, 1, F2, F2, 1><<<conv_grid(ntiles, F2), 256, 0, st>>>(
+                in, w, bias, out, nullptr, nullptr, ntiles, bstride);
+            break;
         default: return -1;
         }
     } else if (mode == MODE_RELU_POOL) {
         if (layer != 4 || !pool) return -1;
-        conv3x3_kernel<32, 32, MODE_RELU_POOL><<<g, 256, 0, st>>>(in, w, bias, out, nullptr, pool, ntiles, bstride);
+        conv3x3_kernel<32, 32, MODE_RELU_POOL, 1, F2, F2, 1><<<conv_grid(ntiles, F2), 256, 0, st>>>(
+            in, w, bias, out, nullptr, pool, ntiles, bstride);
     } else if (mode == MODE_MASK) {
         if (!out || !mask) return -1;
         switch (layer) {
-        case 2: conv3x3_kernel<32, 16, MODE_MASK><<<g, 256, 0, st>>>(in, w, nullptr, out, mask, nullptr, ntiles, 0); break;
+        case 2:
+            conv3x3_kernel<32, 16, MODE_MASK, 1, F1, F1, 2><<<conv_grid(ntiles, F1), 256, 0, st>>>(
+                in, w, nullptr, out, mask, nullptr, ntiles, 0);
+            break;
         case 3:
-        case 4: conv3x3_kernel<32, 32, MODE_MASK><<<g, 256, 0, st>>>(in, w, nullptr, out, mask, nullptr, ntiles, 0); break;
+        case 4:
+            conv3x3_kernel<32, 32, MODE_MASK, 1, F2, F2, 1><<<conv_grid(ntiles, F2), 256, 0, st>>>(
+                in, w, nullptr, out, mask, nullptr, ntiles, 0);
+            break;
         default: return -1;
         }
     } else {
@@ -871,9 +928,9 @@ extern "C" int bc250_cnn_forward(const uint16_t *in, const uint16_t *packed, con
     }
     hipStream_t st = (hipStream_t)stream;
     switch (g_forward_variant) {
-    case 0: cnn_forward_fused_kernel<0><<<ntiles * 4, FWAVES * 64, 0, st>>>(in, fw, pool, ntiles); break;
-    case 1: cnn_forward_fused_kernel<1><<<ntiles * 4, FWAVES * 64, 0, st>>>(in, fw, pool, ntiles); break;
-    default: cnn_forward_fused_kernel<2><<<ntiles * 4, FWAVES * 64, 0, st>>>(in, fw, pool, ntiles); break;
+    case 0: cnn_forward_fused_kernel<0><<<ntiles, FwdShape<0>::NW * 64, 0, st>>>(in, fw, pool, ntiles); break;
+    case 1: cnn_forward_fused_kernel<1><<<ntiles, FwdShape<1>::NW * 64, 0, st>>>(in, fw, pool, ntiles); break;
+    default: cnn_forward_fused_kernel<2><<<ntiles, FwdShape<2>::NW * 64, 0, st>>>(in, fw, pool, ntiles); break;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -908,9 +965,9 @@ extern "C" int bc250_cnn_forward_picture(const uint8_t *cur, int cpitch, const u
     const FusedWeights fw = fused_weights(packed, params);
     hipStream_t st = (hipStream_t)stream;
     switch (g_forward_variant) {
-    case 0: cnn_forward_picture_kernel<0><<<ntiles * 4, FWAVES * 64, 0, st>>>(src, fw, pool, ntiles); break;
-    case 1: cnn_forward_picture_kernel<1><<<ntiles * 4, FWAVES * 64, 0, st>>>(src, fw, pool, ntiles); break;
-    default: cnn_forward_picture_kernel<2><<<ntiles * 4, FWAVES * 64, 0, st>>>(src, fw, pool, ntiles); break;
+    case 0: cnn_forward_picture_kernel<0><<<ntiles, FwdShape<0>::NW * 64, 0, st>>>(src, fw, pool, ntiles); break;
+    case 1: cnn_forward_picture_kernel<1><<<ntiles, FwdShape<1>::NW * 64, 0, st>>>(src, fw, pool, ntiles); break;
+    default: cnn_forward_picture_kernel<2><<<ntiles, FwdShape<2>::NW * 64, 0, st>>>(src, fw, pool, ntiles); break;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -921,16 +978,16 @@ extern "C" int bc250_cnn_forward_batch(const bc250_me_batch_t *b, const uint16_t
     if (!b || b->n < 1 || b->n > BC250_BATCH_MAX || b->ntile_total <= 0 || !packed || !params || !pool) return -1;
     const FusedWeights fw = fused_weights(packed, params);
     hipStream_t st = (hipStream_t)stream;
-    const int g = b->ntile_total * 4;
+    const int g = b->ntile_total;
     switch (g_forward_variant) {
-    case 0: cnn_forward_batch_kernel<0><<<g, FWAVES * 64, 0, st>>>(*b, fw, pool); break;
-    case 1: cnn_forward_batch_kernel<1><<<g, FWAVES * 64, 0, st>>>(*b, fw, pool); break;
-    default: cnn_forward_batch_kernel<2><<<g, FWAVES * 64, 0, st>>>(*b, fw, pool); break;
+    case 0: cnn_forward_batch_kernel<0><<<g, FwdShape<0>::NW * 64, 0, st>>>(*b, fw, pool); break;
+    case 1: cnn_forward_batch_kernel<1><<<g, FwdShape<1>::NW * 64, 0, st>>>(*b, fw, pool); break;
+    default: cnn_forward_batch_kernel<2><<<g, FwdShape<2>::NW * 64, 0, st>>>(*b, fw, pool); break;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// The fused forward's LDS swizzle (0, 1, 2: see chunk_off); a tunable.
+// The fused forward's launch shape (0, 1, 2: see FwdShape); a tunable.
 extern "C" int bc250_cnn_set_forward_variant(int v)
 {
     if (v < 0 || v > 2) return -1;
@@ -953,10 +1010,10 @@ extern "C" int bc250_cnn_wgrad(int layer, const uint16_t *a, const uint16_t *dz,
     hipStream_t st = (hipStream_t)stream;
     int cin, cout;
     switch (layer) {
-    case 1: conv_wgrad_kernel<2, 16><<<ntiles, 256, 0, st>>>(a, dz, part); cin = 2; cout = 16; break;
-    case 2: conv_wgrad_kernel<16, 32><<<ntiles, 256, 0, st>>>(a, dz, part); cin = 16; cout = 32; break;
+    case 1: conv_wgrad_kernel<2, 16, 2, T, F1><<<ntiles, 256, 0, st>>>(a, dz, part); cin = 2; cout = 16; break;
+    case 2: conv_wgrad_kernel<16, 32, 2, F1, F2><<<ntiles, 256, 0, st>>>(a, dz, part); cin = 16; cout = 32; break;
     case 3:
-    case 4: conv_wgrad_kernel<32, 32><<<ntiles, 256, 0, st>>>(a, dz, part); cin = 32; cout = 32; break;
+    case 4: conv_wgrad_kernel<32, 32, 1, F2, F2><<<ntiles, 256, 0, st>>>(a, dz, part); cin = 32; cout = 32; break;
     default: return -1;
     }
     const int m = cout * (kpad(cin) + 1);
@@ -989,7 +1046,7 @@ extern "C" int bc250_cnn_pool_backward(const float *dpool, const uint16_t *a4, u
                                        void *stream)
 {
     if (ntiles <= 0 || !dpool || !a4 || !dz4) return -1;
-    const size_t n = (size_t)ntiles * T * T * 32;
+    const size_t n = (size_t)ntiles * F2 * F2 * 32;
     cnn_pool_backward_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(dpool, a4, dz4, ntiles);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

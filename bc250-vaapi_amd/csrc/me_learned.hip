@@ -273,7 +273,7 @@ __device__ __forceinline__ void propose_mb(const float *__restrict__ pool, const
     for (int o = 0; o < BC250_CNN_NOUT; o++) {
         const float *row = wh + o * (BC250_CNN_NFEAT + 1);
         float s = row[BC250_CNN_NFEAT];
-        for (int i = 0; i < 32; i++) s = fmaf(row[i], pb[i] * (1.f / 256.f), s);
+        for (int i = 0; i < 32; i++) s = fmaf(row[i], pb[i] * BC250_CNN_POOL_SCALE, s);
         for (int i = 0; i < 32; i++) s = fmaf(row[32 + i], mv[i], s);
         z[o] = s;
     }

@@ -4,16 +4,18 @@ gfx950 kernels of csrc/me_cnn.hip / csrc/me_learned.hip through the C ABI
 own inference path is native (gpu_compute_set_me_proposer); this module
 trains the weights it loads.
 
-Layer l (1..4) of the network: conv 3x3 (2->16, 16->32, 32->32, 32->32) +
-ReLU on 64x64 tiles, then a linear head per 16x16 block over the block mean
-of conv4 and 16 neighbour vectors -> 4 hypotheses and their confidences."""
+Layer l (1..4) of the network: conv 3x3 (2->16 and 16->32 at stride 2,
+32->32 twice at stride 1) + ReLU on 64x64 tiles, so conv4 is 16x16; then a
+linear head per 16x16 block of the picture over the mean of its 4x4 cell of
+conv4 and 16 neighbour vectors -> 4 hypotheses and their confidences."""
 import ctypes as C
 
 import numpy as np
 
 LAYERS = ((2, 16), (16, 32), (32, 32), (32, 32))
 NOUT, NFEAT = 12, 64
-TP = 66
+TP = 66                      # padded side of the input tile
+SIDES = (66, 34, 18, 18, 18)   # padded side of activation 0 (input) .. 4
 
 
 def kpad(cin):
@@ -60,14 +62,14 @@ class Net:
             raise ValueError("parameter vector has the wrong size")
         self.params = torch.from_numpy(p.copy()).to(dev)
         self.packed = torch.zeros(NPACKED, dtype=torch.int16, device=dev)
-        z16 = lambda c: torch.zeros((ntiles, TP, TP, c), dtype=torch.int16, device=dev)
-        self.act = [z16(2)] + [z16(c) for _, c in LAYERS]
-        self.rows = torch.zeros((ntiles, 64, 4, 32), dtype=torch.float32, device=dev)
+        z16 = lambda l, c: torch.zeros((ntiles, SIDES[l], SIDES[l], c), dtype=torch.int16, device=dev)
+        self.act = [z16(0, 2)] + [z16(l + 1, c) for l, (_, c) in enumerate(LAYERS)]
+        self.rows = torch.zeros((ntiles, 16, 4, 32), dtype=torch.float32, device=dev)
         self.pool = torch.zeros((ntiles, 16, 32), dtype=torch.float32, device=dev)
         self.z = torch.zeros((ntiles, 16, NOUT), dtype=torch.float32, device=dev)
         self.train = train
         if train:
-            self.dz = [None] + [z16(c) for _, c in LAYERS]
+            self.dz = [None] + [z16(l + 1, c) for l, (_, c) in enumerate(LAYERS)]
             self.grad = torch.zeros(NPARAM, dtype=torch.float32, device=dev)
             self.m = torch.zeros(NPARAM, dtype=torch.float32, device=dev)
             self.v = torch.zeros(NPARAM, dtype=torch.float32, device=dev)
@@ -112,7 +114,7 @@ class Net:
         return n
 
     def forward(self, n, mvin, keep=True):
-        """Layers 1..4 on the first n tiles, block sums and head outputs."""
+        """Layers 1..4 on the first n tiles, cell sums and head outputs."""
         for l in (1, 2, 3):
             self._ok(self.hip.bc250_cnn_conv(l, 0, self._p(self.act[l - 1]), self.wf(l), self.bias(l),
                                              self._p(self.act[l]), None, None, n, None), f"conv{l}")

@@ -251,53 +251,76 @@ def bd_rate(r0, p0, r1, p1):
 def learned_quality(pkg, frames, args, weights, qps=(22, 27, 32, 37), n=24):
     """Rate and quality of the learned integer search against the plain one
     on the bench's frames (seed 1080, outside the training and held-out
-    seeds): the H.264 pipeline with its CAVLC slices (the stream's bytes;
-    HEVC has no entropy coder here), PSNR-Y of every frame's reconstruction,
-    I + (n-1) P frames per QP; plus the fraction of MBs the learned form
-    searched in full and of 16x16 vectors equal to the plain form's."""
-    res = {"plain": {}, "learned": {}}
-    vec_same, fb = [], []
-    for qp in qps:
-        mvs = {}
-        for mode in ("plain", "learned"):
-            enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0, readback=1,
-                              entropy=1)
-            try:
-                if mode == "learned":
-                    enc.set_proposer(weights, **pkg.PROPOSER_DEFAULT)
-                nbytes, psnr, mv = 0, [], []
-                for i, (y, uv) in enumerate(frames[:n]):
-                    im = enc.new_image()
-                    enc.upload(im, y, uv)
-                    slot = enc.encode(im, qp, i == 0, 1)
-                    nbytes += len(pkg.h264_frame_nals(enc.slices(slot), i == 0))
-                    if mode == "learned" and i > 0:
-                        fb.append(enc.fallbacks(slot) / enc.nmb)
-                    mv.append(enc.staging(slot, "mv"))
-                    ry, _ = enc.recon()
-                    mse = np.mean((ry.astype(np.float64) - y) ** 2)
-                    psnr.append(10 * np.log10(255.0 ** 2 / max(mse, 1e-10)))
-                    enc.images.remove(im)
-                    pkg.hip.gpu_compute_destroy_image(pkg.C.byref(enc.ctx), im[0], im[1])
-            finally:
-                enc.close()
-            res[mode][qp] = {"bytes": nbytes, "psnr_y": float(np.mean(psnr))}
-            mvs[mode] = mv
-        vec_same += [float(np.mean((a["mvx"] == b["mvx"]) & (a["mvy"] == b["mvy"])))
-                     for a, b in zip(mvs["plain"][1:], mvs["learned"][1:])]
-        log(f"learned quality qp {qp}: plain {res['plain'][qp]}, learned {res['learned'][qp]}")
-    r0 = [res["plain"][q]["bytes"] for q in qps]
-    p0 = [res["plain"][q]["psnr_y"] for q in qps]
-    r1 = [res["learned"][q]["bytes"] for q in qps]
-    p1 = [res["learned"][q]["psnr_y"] for q in qps]
-    return {"bd_rate_pct": round(bd_rate(r0, p0, r1, p1), 3),
-            "bd_rate_basis": f"H.264 CAVLC stream bytes and PSNR-Y, CQP {'/'.join(map(str, qps))}, "
-                             f"{n} frames IPPP of the bench's 1080p sequence",
+    seeds), I + (n-1) P frames per QP: the HEVC pipeline's pictures written
+    by the host HEVC writer (the stream's bytes, VPS/SPS/PPS included) and
+    PSNR-Y of every reconstruction -- the BD-rate of the metric -- and the
+    same on the H.264 pipeline's CAVLC slices beside it; plus the fraction of
+    MBs the learned form searched in full and of 16x16 vectors equal to the
+    plain form's (the final vectors: HEVC 8x8 CUs, H.264 16x16 MBs)."""
+    out = {}
+    for codec in ("hevc", "h264"):
+        res = {"plain": {}, "learned": {}}
+        vec_same, fb = [], []
+        for qp in qps:
+            mvs = {}
+            for mode in ("plain", "learned"):
+                if codec == "hevc":
+                    enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, readback=1,
+                                      codec=pkg.GPU_CODEC_HEVC)
+                    st = pkg.HevcStream(enc.wc, enc.hc, 8, 4, 1, True, init_qp=26,
+                                        crop_width=W if W != enc.wc else 0, crop_height=H if H != enc.hc else 0)
+                else:
+                    enc = pkg.Encoder(W, H, search_range=args.range, lambda8=args.lambda8, deblock_idc=0,
+                                      readback=1, entropy=1)
+                try:
+                    if mode == "learned":
+                        enc.set_proposer(weights, **pkg.PROPOSER_DEFAULT)
+                    nbytes = len(st.headers()) if codec == "hevc" else 0
+                    psnr, mv = [], []
+                    for i, (y, uv) in enumerate(frames[:n]):
+                        im = enc.new_image()
+                        enc.upload(im, y, uv)
+                        slot = enc.encode(im, qp, i == 0, 1)
+                        if codec == "hevc":
+                            r = enc.results(slot)
+                            nbytes += len(st.picture(i == 0, i, qp, r["quant"], r["hevc_mode"], r["mv8"]))
+                        else:
+                            nbytes += len(pkg.h264_frame_nals(enc.slices(slot), i == 0))
+                        if mode == "learned" and i > 0:
+                            fb.append(enc.fallbacks(slot) / enc.nmb)
+                        mv.append(enc.staging(slot, "mv8" if codec == "hevc" else "mv"))
+                        ry, _ = enc.recon()
+                        mse = np.mean((ry.astype(np.float64) - y) ** 2)
+                        psnr.append(10 * np.log10(255.0 ** 2 / max(mse, 1e-10)))
+                        enc.images.remove(im)
+                        pkg.hip.gpu_compute_destroy_image(pkg.C.byref(enc.ctx), im[0], im[1])
+                finally:
+                    enc.close()
+                res[mode][qp] = {"bytes": nbytes, "psnr_y": float(np.mean(psnr))}
+                mvs[mode] = mv
+            vec_same += [float(np.mean((a["mvx"] == b["mvx"]) & (a["mvy"] == b["mvy"])))
+                         for a, b in zip(mvs["plain"][1:], mvs["learned"][1:])]
+            log(f"learned quality {codec} qp {qp}: plain {res['plain'][qp]}, learned {res['learned'][qp]}")
+        r0 = [res["plain"][q]["bytes"] for q in qps]
+        p0 = [res["plain"][q]["psnr_y"] for q in qps]
+        r1 = [res["learned"][q]["bytes"] for q in qps]
+        p1 = [res["learned"][q]["psnr_y"] for q in qps]
+        out[codec] = {
+            "bd_rate_pct": round(bd_rate(r0, p0, r1, p1), 3),
             "per_qp": {str(q): {"plain_bytes": res["plain"][q]["bytes"], "learned_bytes": res["learned"][q]["bytes"],
                                 "plain_psnr_y": round(res["plain"][q]["psnr_y"], 3),
                                 "learned_psnr_y": round(res["learned"][q]["psnr_y"], 3)} for q in qps},
             "fallback_mb_frac": round(float(np.mean(fb)), 4),
             "vectors_equal_to_plain_frac": round(float(np.mean(vec_same)), 4)}
+    h = out["hevc"]
+    return {"bd_rate_pct": h["bd_rate_pct"],
+            "bd_rate_basis": f"HEVC Main stream bytes (the host writer over the pipeline's staging, parameter sets "
+                             f"included) and PSNR-Y, CQP {'/'.join(map(str, qps))}, {n} frames IPPP of the bench's "
+                             f"1080p sequence",
+            "per_qp": h["per_qp"], "fallback_mb_frac": h["fallback_mb_frac"],
+            "vectors_equal_to_plain_frac": h["vectors_equal_to_plain_frac"],
+            "h264_cavlc": {"bd_rate_pct": out["h264"]["bd_rate_pct"], "per_qp": out["h264"]["per_qp"],
+                           "basis": "the H.264 pipeline's CAVLC slices, same frames, QPs and search"}}
 
 
 def bench_rate_control(pkg, frames, args, fps=60.0, rate=20_000_000):

@@ -22,19 +22,24 @@ extern "C" {
 #endif
 
 /* ---- the network --------------------------------------------------------
- * Tiles of 64x64 luma; activations bf16 [tile][66][66][C] (a zero border
- * around the 64x64 interior, which only the kernels write); see
- * csrc/me_cnn.hip for the layer list.  Parameters are one fp32 vector of
- * BC250_CNN_NPARAM: per conv layer l (1..4) a [COUT][KPAD + 1] matrix
- * (weights with K = (ky, kx, ci) zero-padded to KPAD, the bias last), then
- * the head [12][65] (32 block-mean features, 32 neighbour-vector inputs,
- * bias).  bf16 copies for the kernels ("packed"): per layer the forward
- * [COUT][KPAD] and, for layers 2..4, the flipped [CIN][KPAD(COUT)]. */
+ * Tiles of 64x64 luma.  conv1 (2->16) and conv2 (16->32) have stride 2,
+ * conv3 and conv4 (32->32) stride 1, so a tile's activations are bf16
+ * [tile][66][66][2] (input), [tile][34][34][16] (conv1) and
+ * [tile][18][18][32] (conv2..conv4), each with a zero border around the
+ * interior, which only the kernels write; a 16x16 block of the picture is a
+ * 4x4 cell of conv4.  See csrc/me_cnn.hip for the layer list.  Parameters
+ * are one fp32 vector of BC250_CNN_NPARAM: per conv layer l (1..4) a
+ * [COUT][KPAD + 1] matrix (weights with K = (ky, kx, ci) zero-padded to
+ * KPAD, the bias last), then the head [12][65] (32 cell-mean features, 32
+ * neighbour-vector inputs, bias).  bf16 copies for the kernels ("packed"):
+ * per layer the forward [COUT][KPAD] and, for layers 2..4, the flipped
+ * [CIN][KPAD(COUT)]. */
 #define BC250_CNN_NOUT 12
 #define BC250_CNN_NFEAT 64
 #define BC250_CNN_EPS 0.02f
 #define BC250_CNN_NPARAM 24956
 #define BC250_CNN_NPACKED 47104
+#define BC250_CNN_POOL_SCALE 0.0625f /* cell sum -> cell mean (4 x 4 pixels of conv4) */
 
 static inline size_t bc250_cnn_w_offset(int l) /* l = 0..3 conv layers, 4 = head */
 {
@@ -48,7 +53,8 @@ static inline size_t bc250_cnn_packed_offset(int l, int flipped)
 }
 
 size_t bc250_cnn_param_count(void);
-size_t bc250_cnn_tile_bytes(int channels);
+/* Bytes of one tile's activation of layer 0 (input) .. 4 (0: bad layer). */
+size_t bc250_cnn_tile_bytes(int layer);
 
 /* Input tiles from 8-bit luma pictures: tiles[3 i .. 3 i + 2] = {picture
  * slot, tile x, tile y}; slot s reads cur / ref + s * slot_stride (pitches
@@ -57,30 +63,33 @@ int bc250_cnn_input(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpit
                     int h, const int *tiles, int ntiles, uint16_t *in, void *stream);
 
 /* One layer.  mode 0: out = relu(conv(in, w) + b); mode 2 (layer 4 only):
- * the same, plus the bf16 outputs summed over each 16-pixel row segment into
- * `pool` = rows [tile][64][4][32] fp32 (out may be NULL), which
- * bc250_cnn_pool reduces to the block sums [tile][16][32]; mode 1: input
- * gradient of `layer` -- out = conv(in = dz of layer, w = its flipped
- * weights) where mask (that layer's input activation) > 0, else 0.
- * bias: the layer's parameter matrix base + KPAD (stride KPAD + 1). */
+ * the same, plus the bf16 outputs summed over each 4-pixel row group into
+ * `pool` = rows [tile][16][4][32] fp32 (out may be NULL), which
+ * bc250_cnn_pool reduces to the cell sums [tile][16][32]; mode 1 (layers
+ * 2..4): input gradient of `layer` -- out = conv(in = dz of layer, w = its
+ * flipped weights) where mask (that layer's input activation) > 0, else 0;
+ * for layer 2 (stride 2) the convolution runs over dz zero-upsampled to the
+ * input's size (dz at the odd padded positions).  bias: the layer's
+ * parameter matrix base + KPAD (stride KPAD + 1). */
 int bc250_cnn_conv(int layer, int mode, const uint16_t *in, const uint16_t *w, const float *bias, uint16_t *out,
                    const uint16_t *mask, float *pool, int ntiles, void *stream);
 
-/* Block sums from the row-segment sums, in a fixed order. */
+/* Cell sums from the row-group sums, in a fixed order. */
 int bc250_cnn_pool(const float *rows, float *pool, int ntiles, void *stream);
 
-/* Inference in one launch: conv1..conv4 and the block sums [tile][16][32]
- * of the input tiles `in` ([tile][66][66][2], zero borders), every
- * intermediate activation kept on chip; packed: the forward weights
+/* Inference in one launch: conv1..conv4 and the cell sums [tile][16][32]
+ * of the input tiles `in` ([tile][66][66][2], zero borders), one tile per
+ * workgroup, every intermediate activation kept on chip; packed: the forward weights
  * (bc250_cnn_pack), params: the fp32 parameters (biases).  The same bits as
  * bc250_cnn_conv layers 1..4 (mode 2 for layer 4) followed by
  * bc250_cnn_pool. */
 int bc250_cnn_forward(const uint16_t *in, const uint16_t *packed, const float *params, float *pool, int ntiles,
                       void *stream);
 
-/* The fused launch's LDS placement of activation channels: 0 plain NHWC,
- * 1 or 2 channel chunks XOR-swizzled by pixel for 32- or 64-bank reads.
- * Process-wide; every variant gives the same bits. */
+/* The fused launch's shape: 0 four waves, 4-row tasks, plain NHWC in LDS;
+ * 1 the same with channel chunks XOR-swizzled by pixel for 32-bank reads;
+ * 2 eight waves, 2-row tasks, swizzled for 64 banks.  Process-wide; every
+ * variant gives the same bits. */
 int bc250_cnn_set_forward_variant(int variant);
 
 /* Weight and bias gradient of `layer` from its input activation a and its
@@ -90,7 +99,7 @@ int bc250_cnn_set_forward_variant(int variant);
 int bc250_cnn_wgrad(int layer, const uint16_t *a, const uint16_t *dz, float *part, float *grad, int ntiles,
                     void *stream);
 
-/* Head: z [tile][16][12] from the block sums of conv4 and mvin [tile][16][32]. */
+/* Head: z [tile][16][12] from the cell sums of conv4 and mvin [tile][16][32]. */
 int bc250_cnn_head(const float *wh, const float *pool, const float *mvin, float *z, int ntiles, void *stream);
 
 /* Training head: forward, loss (target [tile][16][2] in pel), backward.
@@ -100,7 +109,7 @@ int bc250_cnn_head(const float *wh, const float *pool, const float *mvin, float 
 int bc250_cnn_head_train(const float *wh, const float *pool, const float *mvin, const float *target, float *loss,
                          float *dpool, float *part, float *grad, int ntiles, float scale, void *stream);
 
-/* dz of conv4's pre-activation from dpool: dpool[block] where a4 > 0. */
+/* dz of conv4's pre-activation from dpool: dpool[cell] where a4 > 0. */
 int bc250_cnn_pool_backward(const float *dpool, const uint16_t *a4, uint16_t *dz4, int ntiles, void *stream);
 
 /* Adam step t (>= 1), beta 0.9 / 0.999, eps 1e-8, on n parameters. */

@@ -23,7 +23,9 @@ import numpy as np
 
 NOUT, NFEAT, EPS = 12, 64, 0.02
 LAYERS = ((2, 16), (16, 32), (32, 32), (32, 32))
+STRIDES = (2, 2, 1, 1)   # conv1 64 -> 32, conv2 32 -> 16, conv3 / conv4 at 16x16
 T = 64
+CELL = 4                 # a 16x16 block of the picture is a 4x4 cell of conv4
 
 
 def kpad(cin):
@@ -90,55 +92,62 @@ def input_tiles(cur, ref, tiles):
     return bf16(out)
 
 
-def _patches(x):
-    """x [n, 64, 64, c] -> [n, 64, 64, 3, 3, c] with a zero border."""
+def _patches(x, stride=1):
+    """x [n, s, s, c] -> [n, s/stride, s/stride, 3, 3, c]: the 3x3 window of
+    every output pixel, output (y, x) reading input rows stride*y - 1 ..
+    stride*y + 1 and columns likewise, zero outside."""
     xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
-    n, c = x.shape[0], x.shape[3]
-    out = np.empty((n, T, T, 3, 3, c), x.dtype)
+    n, s, c = x.shape[0], x.shape[1], x.shape[3]
+    o = s // stride
+    out = np.empty((n, o, o, 3, 3, c), x.dtype)
     for ky in range(3):
         for kx in range(3):
-            out[:, :, :, ky, kx, :] = xp[:, ky:ky + T, kx:kx + T, :]
+            out[:, :, :, ky, kx, :] = xp[:, ky:ky + stride * o:stride, kx:kx + stride * o:stride, :]
     return out
 
 
-def conv(x, w, b):
-    """3x3 'same' convolution: x [n,64,64,cin], w [cout,3,3,cin] -> [n,64,64,cout]."""
-    pt = _patches(np.asarray(x, np.float64))
+def conv(x, w, b, stride=1):
+    """3x3 convolution with a zero border: x [n,s,s,cin], w [cout,3,3,cin]
+    -> [n,s/stride,s/stride,cout]."""
+    pt = _patches(np.asarray(x, np.float64), stride)
     return np.einsum("nyxabc,oabc->nyxo", pt, w, optimize=True) + b
 
 
-def conv_input_grad(dz, w):
+def conv_input_grad(dz, w, stride=1):
     """Adjoint of conv with respect to its input: scatter dz * w into the
-    padded input and crop."""
-    n = dz.shape[0]
+    padded input (output pixel (y, x) onto the window it read) and crop."""
+    n, o = dz.shape[0], dz.shape[1]
+    s = o * stride
     cin = w.shape[3]
-    dxp = np.zeros((n, T + 2, T + 2, cin))
+    dxp = np.zeros((n, s + 2, s + 2, cin))
     for ky in range(3):
         for kx in range(3):
-            dxp[:, ky:ky + T, kx:kx + T, :] += np.einsum("nyxo,oc->nyxc", dz, w[:, ky, kx, :], optimize=True)
-    return dxp[:, 1:T + 1, 1:T + 1, :]
+            dxp[:, ky:ky + stride * o:stride, kx:kx + stride * o:stride, :] += np.einsum(
+                "nyxo,oc->nyxc", dz, w[:, ky, kx, :], optimize=True)
+    return dxp[:, 1:s + 1, 1:s + 1, :]
 
 
-def conv_weight_grad(x, dz):
-    pt = _patches(np.asarray(x, np.float64))
+def conv_weight_grad(x, dz, stride=1):
+    pt = _patches(np.asarray(x, np.float64), stride)
     return np.einsum("nyxabc,nyxo->oabc", pt, dz, optimize=True), dz.sum(axis=(0, 1, 2))
 
 
 def forward(p, x, mvin, rounding="bf16"):
-    """Activations a0..a4, block sums of a4 [n,16,32] and head outputs
-    z [n,16,12]."""
+    """Activations a0..a4 ([n,64,64,2], [n,32,32,16], [n,16,16,32] x 3), the
+    4x4 cell sums of a4 [n,16,32] (cell j = 16x16 block j of the tile) and
+    head outputs z [n,16,12] from the cell means."""
     r = bf16 if rounding == "bf16" else (lambda v: np.asarray(v, np.float64))
     acts = [r(x)]
     for l in range(4):
         w, b = layer_params(p, l)
         if rounding == "bf16":
             w = bf16(w.astype(np.float32)).astype(np.float64)
-        acts.append(r(np.maximum(conv(acts[-1], w, b), 0.0).astype(np.float32 if rounding else np.float64)))
+        acts.append(r(np.maximum(conv(acts[-1], w, b, STRIDES[l]), 0.0).astype(np.float32 if rounding else np.float64)))
     a4 = np.asarray(acts[4], np.float64)
     n = a4.shape[0]
-    pool = a4.reshape(n, 4, 16, 4, 16, 32).sum(axis=(2, 4)).reshape(n, 16, 32)
+    pool = a4.reshape(n, 4, CELL, 4, CELL, 32).sum(axis=(2, 4)).reshape(n, 16, 32)
     wh, bh = head_params(p)
-    feat = np.concatenate([pool / 256.0, np.asarray(mvin, np.float64)], axis=2)
+    feat = np.concatenate([pool / CELL ** 2, np.asarray(mvin, np.float64)], axis=2)
     z = feat @ wh.T + bh
     return acts, pool, z
 
@@ -182,12 +191,12 @@ def backward(p, acts, pool, mvin, dz_head, rounding="bf16"):
     g = np.zeros(NPARAM, np.float64)
     wh, bh = head_params(p)
     n = pool.shape[0]
-    feat = np.concatenate([pool / 256.0, np.asarray(mvin, np.float64)], axis=2)
+    feat = np.concatenate([pool / CELL ** 2, np.asarray(mvin, np.float64)], axis=2)
     gh = g[OFFSETS[4]:OFFSETS[4] + NOUT * (NFEAT + 1)].reshape(NOUT, NFEAT + 1)
     gh[:, :NFEAT] = np.einsum("nbo,nbi->oi", dz_head, feat)
     gh[:, NFEAT] = dz_head.sum(axis=(0, 1))
-    dpool = (dz_head @ wh[:, :32]) / 256.0                        # [n,16,32]
-    da = np.repeat(np.repeat(dpool.reshape(n, 4, 4, 32), 16, axis=1), 16, axis=2)
+    dpool = (dz_head @ wh[:, :32]) / CELL ** 2                    # [n,16,32]
+    da = np.repeat(np.repeat(dpool.reshape(n, 4, 4, 32), CELL, axis=1), CELL, axis=2)
     dzs = [None] * 5
     for l in range(3, -1, -1):
         a_out = np.asarray(acts[l + 1], np.float64)
@@ -197,7 +206,7 @@ def backward(p, acts, pool, mvin, dz_head, rounding="bf16"):
         dzs[l + 1] = dz
         cin, cout = LAYERS[l]
         kp = kpad(cin)
-        gw, gb = conv_weight_grad(acts[l], dz)
+        gw, gb = conv_weight_grad(acts[l], dz, STRIDES[l])
         gm = g[OFFSETS[l]:OFFSETS[l] + cout * (kp + 1)].reshape(cout, kp + 1)
         gm[:, :9 * cin] = gw.reshape(cout, 9 * cin)
         gm[:, kp] = gb
@@ -205,7 +214,7 @@ def backward(p, acts, pool, mvin, dz_head, rounding="bf16"):
             w, _ = layer_params(p, l)
             if rounding == "bf16":
                 w = bf16(w.astype(np.float32)).astype(np.float64)
-            da = conv_input_grad(dz, w)
+            da = conv_input_grad(dz, w, STRIDES[l])
     return g, dzs
 
 

@@ -94,25 +94,26 @@ def test_forward_layers_match_oracle(pkg, learned):
     torch.cuda.synchronize()
     prev = x
     for l in range(4):
-        got = bf(net.act[l + 1])[:, 1:65, 1:65, :]
+        got = bf(net.act[l + 1])[:, 1:-1, 1:-1, :]
         w, b = oc.layer_params(p, l)
         w = oc.bf16(w.astype(np.float32)).astype(np.float64)
-        want = oc.bf16(np.maximum(oc.conv(prev, w, b), 0).astype(np.float32))
+        want = oc.bf16(np.maximum(oc.conv(prev, w, b, oc.STRIDES[l]), 0).astype(np.float32))
+        assert not bf(net.act[l + 1])[:, [0, -1]].any() and not bf(net.act[l + 1])[:, :, [0, -1]].any()
         _close_bf16(got, want, f"conv{l + 1}")
         assert (got > 0).mean() > 0.2, f"conv{l + 1} mostly dead"
         prev = got
     pool = net.pool.cpu().numpy()
     a4 = prev.astype(np.float64)
-    want_pool = a4.reshape(6, 4, 16, 4, 16, 32).sum(axis=(2, 4)).reshape(6, 16, 32)
+    want_pool = a4.reshape(6, 4, 4, 4, 4, 32).sum(axis=(2, 4)).reshape(6, 16, 32)
     np.testing.assert_allclose(pool, want_pool, rtol=1e-5, atol=1e-4)
     wh, bh = oc.head_params(p)
-    want_z = np.concatenate([want_pool / 256, mvin], axis=2) @ wh.T + bh
+    want_z = np.concatenate([want_pool / 16, mvin], axis=2) @ wh.T + bh
     np.testing.assert_allclose(net.z.cpu().numpy(), want_z, rtol=1e-4, atol=1e-4)
 
 
 def test_fused_forward_equals_the_layers(pkg, learned):
-    """The one-launch inference (bands of 16 rows, activations in LDS, input
-    rows reused across output rows) gives the block sums and head outputs of
+    """The one-launch inference (a tile per workgroup, activations in LDS,
+    input rows reused across output rows) gives the cell sums and head outputs of
     the four separate layer launches bit for bit: random weights on 7 tiles
     at the picture's edges, then the trained weights."""
     import torch
@@ -148,17 +149,17 @@ def test_backward_matches_oracle(pkg, learned):
     scale = 1.0 / (n * 16)
     loss = net.backward(n, dmv, torch.from_numpy(target).cuda(), scale).cpu().numpy()
     torch.cuda.synchronize()
-    acts = [x] + [bf(net.act[l])[:, 1:65, 1:65, :] for l in range(1, 5)]
+    acts = [x] + [bf(net.act[l])[:, 1:-1, 1:-1, :] for l in range(1, 5)]
     pool = net.pool.cpu().numpy().astype(np.float64)
     wh, bh = oc.head_params(p)
-    z = np.concatenate([pool / 256, mvin], axis=2) @ wh.T + bh
+    z = np.concatenate([pool / 16, mvin], axis=2) @ wh.T + bh
     lb, dzh = oc.loss_and_dz(z, target)
     np.testing.assert_allclose(loss, lb.sum(axis=1), rtol=1e-4, atol=1e-4)
     g, dzs = oc.backward(p, acts, pool, mvin, dzh * scale, rounding="bf16")
     gg = net.grad.cpu().numpy().astype(np.float64)
     # the pre-activation gradients the kernels stored
     for l in (4, 3, 2, 1):
-        got = bf(net.dz[l])[:, 1:65, 1:65, :]
+        got = bf(net.dz[l])[:, 1:-1, 1:-1, :]
         want = dzs[l]
         cos = (got * want).sum() / (np.linalg.norm(got) * np.linalg.norm(want) + 1e-30)
         assert cos > 0.999, f"dz{l}: cosine {cos}"

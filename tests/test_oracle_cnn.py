@@ -69,6 +69,36 @@ def test_conv_is_the_direct_sum():
         assert abs(s - y[0, yy, xx, o]) < 1e-9
 
 
+def test_strided_conv_is_the_direct_sum():
+    """Stride 2: output (y, x) sums the input window centred on (2y, 2x)."""
+    rs = np.random.default_rng(8)
+    x = rs.normal(size=(1, 32, 32, 3))
+    w = rs.normal(size=(4, 3, 3, 3))
+    b = rs.normal(size=4)
+    y = oc.conv(x, w, b, stride=2)
+    assert y.shape == (1, 16, 16, 4)
+    for (yy, xx, o) in ((0, 0, 0), (15, 15, 3), (7, 0, 1), (3, 12, 2)):
+        s = b[o]
+        for ky in range(3):
+            for kx in range(3):
+                sy, sx = 2 * yy + ky - 1, 2 * xx + kx - 1
+                if 0 <= sy < 32 and 0 <= sx < 32:
+                    s += (w[o, ky, kx] * x[0, sy, sx]).sum()
+        assert abs(s - y[0, yy, xx, o]) < 1e-9
+
+
+def test_network_shapes_and_cells():
+    """conv1 and conv2 halve the side, the cell sums are the 4x4 cells of
+    conv4 in raster order (cell j covers the tile's 16x16 block j)."""
+    p, x, mvin, _ = _case(9)
+    acts, pool, _ = oc.forward(p, x, mvin, rounding=None)
+    assert [a.shape[1:] for a in acts] == [(64, 64, 2), (32, 32, 16), (16, 16, 32), (16, 16, 32), (16, 16, 32)]
+    a4 = acts[4]
+    for j in (0, 5, 15):
+        cy, cx = divmod(j, 4)
+        np.testing.assert_allclose(pool[0, j], a4[0, 4 * cy:4 * cy + 4, 4 * cx:4 * cx + 4].sum(axis=(0, 1)))
+
+
 def test_bf16_mode_stays_near_float64():
     p, x, mvin, _ = _case(7)
     _, _, z64 = oc.forward(p, x, mvin, rounding=None)
