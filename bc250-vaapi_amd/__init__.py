@@ -111,6 +111,7 @@ for _n in ("quant", "dc", "pred_mode", "mv", "nz", "mv8", "hevc_mode"):
 _sig(hip, "gpu_compute_set_encode_params", _I, [_CTX, C.POINTER(EncodeParams)])
 _sig(hip, "gpu_compute_download_recon", _I, [_CTX, _P, _I, _P, _I])
 _sig(hip, "gpu_compute_get_stage_times", _I, [_CTX, _I, C.POINTER(C.c_float)])
+_sig(hip, "gpu_compute_set_stage_timing", _I, [_CTX, _I])
 _sig(hip, "gpu_compute_take_error_flags", _I, [_CTX])
 _sig(hip, "gpu_compute_get_stage_stats", _I, [_CTX, C.POINTER(StageStats), _I])
 _sig(hip, "gpu_compute_get_last_latency_ms", C.c_double, [_CTX])
@@ -843,6 +844,11 @@ class Encoder:
                 raise RuntimeError("P010 reconstruction has bits below the sample")
             y, uv = y >> 6, uv >> 6
         return y, uv
+
+    def set_stage_timing(self, on=True):
+        """Stage events on the context's stream (stage_times / stage_stats)."""
+        if hip.gpu_compute_set_stage_timing(C.byref(self.ctx), int(bool(on))) != 0:
+            raise RuntimeError("set_stage_timing failed")
 
     def stage_times(self, slot):
         ms = (C.c_float * 6)()

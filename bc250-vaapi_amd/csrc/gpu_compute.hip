@@ -67,6 +67,7 @@ struct Slot {
     hipEvent_t cdone = nullptr;  // the frame's stages done (compute stream): its copies may start
     DevOut dev;
     bool pending = false, recorded = false, have_data = false, was_intra = false;
+    bool timed = false;          // the stage events ev[0 .. NEV-2] were recorded for this frame
 };
 
 // The learned proposer's weights and per-resolution buffers
@@ -79,8 +80,7 @@ struct Proposer {
     float *pool = nullptr;       // block sums [tile][16][32] of the picture's 64x64 tiles
     int *cand = nullptr, *list = nullptr, *count = nullptr;
     uint32_t *keys = nullptr;    // the split fallback search's keys
-    uint32_t *prev16 = nullptr;  // the previous picture's integer 16x16 vectors
-    bool has_prev = false;       // prev16 holds the picture before the current one
+    bool has_prev = false;       // the integer records hold the picture before the current one
     bool ran = false;            // the current picture ran the learned search
 };
 
@@ -112,6 +112,9 @@ struct bc250_hip_state {
     Slot slot[NSLOT];
     int cur = 0, submitted = 0;
     double last_latency_ms = 0.0;
+    // stage events on the compute stream (gpu_compute_set_stage_timing; on
+    // when BC250_PERF_STATS is set, as the reference's timestamp queries)
+    bool timing = false;
     gpu_encode_params_t params{64, 1, 0, 1, 1, GPU_CODEC_H264, 0, 0, 4, 0};
     // CAVLC slices on the GPU (entropy = 1), allocated at the first such frame
     uint8_t *cavlc_out = nullptr;
@@ -233,13 +236,12 @@ static void free_res(bc250_hip_state *s)
         if (p) hipFree(p);
     {
         Proposer &q = s->prop;
-        for (void *p : {(void *)q.pool, (void *)q.cand, (void *)q.list, (void *)q.count, (void *)q.prev16,
+        for (void *p : {(void *)q.pool, (void *)q.cand, (void *)q.list, (void *)q.count,
                         (void *)q.keys})
             if (p) hipFree(p);
         q.cand = q.list = q.count = nullptr;
         q.keys = nullptr;
         q.pool = nullptr;
-        q.prev16 = nullptr;
         q.has_prev = false;
     }
     s->cavlc_out = nullptr; s->cavlc_info = nullptr; s->cavlc_work = nullptr;
@@ -392,7 +394,7 @@ static int ensure_prop(bc250_hip_state *s)
     const int nt = ((s->mbw + 3) / 4) * ((s->mbh + 3) / 4);
     bool ok = HIP_OK(hipMalloc(&q.pool, (size_t)nt * 16 * 32 * 4)) &&
               HIP_OK(hipMalloc(&q.cand, (size_t)s->nmb * 4 * 8)) && HIP_OK(hipMalloc(&q.list, (size_t)s->nmb * 4)) &&
-              HIP_OK(hipMalloc(&q.count, 4)) && HIP_OK(hipMalloc(&q.prev16, (size_t)s->nmb * 16)) &&
+              HIP_OK(hipMalloc(&q.count, 4)) &&
               HIP_OK(hipMalloc(&q.keys, (size_t)s->nmb * 5 * 4));
     if (!ok) {
         fprintf(stderr, "[bc250-hip] proposer buffer allocation failed\n");
@@ -404,18 +406,20 @@ static int ensure_prop(bc250_hip_state *s)
 
 // The learned integer search of one P picture: the network's candidates
 // refined in +-4 windows, the fallback list searched in full; the result is
-// out16 / out8 in the full search's format.  The 16x16 vectors are kept as
-// the next picture's neighbour input.
+// out16 / out8 in the full search's format, which keeps the 16x16 vectors
+// as the next picture's neighbour input.
 static int enqueue_learned(bc250_hip_state *s, const uint8_t *c8, int cp, const uint8_t *r8, int rp, uint32_t *out16,
                            uint32_t *out8, hipStream_t st)
 {
     Proposer &q = s->prop;
     if (ensure_prop(s) != 0) return -1;
     const gpu_encode_params_t &p = s->params;
+    // out16 still holds the previous picture's records when this picture's
+    // proposer reads them: the candidate search rewrites it only after that
+    // launch on the same stream
     int r = bc250_learned_search_picture(c8, cp, r8, rp, s->wc, s->hc, p.search_range, p.lambda8, q.params, q.packed,
-                                         q.has_prev ? q.prev16 : nullptr, q.cfg.ncand, q.cfg.alpha_q4, q.cfg.beta,
+                                         q.has_prev ? out16 : nullptr, q.cfg.ncand, q.cfg.alpha_q4, q.cfg.beta,
                                          q.cfg.t_high, q.pool, q.cand, q.list, q.count, q.keys, out16, out8, st);
-    if (r == 0) r = HIP_OK(hipMemcpyAsync(q.prev16, out16, (size_t)s->nmb * 16, hipMemcpyDeviceToDevice, st)) ? 0 : -1;
     q.ran = r == 0;
     return r;
 }
@@ -455,6 +459,7 @@ extern "C" int bc250_gpu_init(bc250_gpu_context_t *ctx)
     }
     if (!HIP_OK(hipSetDevice(ctx->device))) return -1;
     bc250_hip_state *s = new bc250_hip_state();
+    s->timing = getenv("BC250_PERF_STATS") != nullptr;
     if (!HIP_OK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking))) {
         delete s;
         return -1;
@@ -745,6 +750,13 @@ extern "C" int gpu_compute_begin_picture(gpu_context_t *ctx, gpu_image_t render_
     return 0;
 }
 
+// A stage boundary: its event, recorded only while stage timing is on (an
+// event with a timestamp costs the stream a few microseconds).
+static void mark(const bc250_hip_state *s, hipEvent_t e, hipStream_t st)
+{
+    if (s->timing) hipEventRecord(e, st);
+}
+
 // Starts a frame in the current slot: the compute stream first waits for the
 // slot's previous copies (its device outputs are about to be rewritten), then
 // the slot's device buffers become the frame's outputs.
@@ -752,6 +764,7 @@ static void start_frame(bc250_hip_state *s)
 {
     Slot &sl = s->slot[s->cur];
     if (sl.recorded) hipStreamWaitEvent(s->stream, sl.ev[NEV - 1], 0);
+    sl.timed = s->timing;
     bind_slot(s, s->cur);
     // the learned search's neighbour input is the previous picture's vectors
     // only when that picture ran it (not after an I picture)
@@ -795,7 +808,7 @@ static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &r
         // the reference's diamond search: final quarter-sample vectors, no 8x8 ones
         int r = bc250_launch_diamond_me((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc,
                                         me_mode == 1 ? 8 : 5, s->mvs, s->stream);
-        hipEventRecord(mid, s->stream);
+        mark(s, mid, s->stream);
         if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, s->nmb * 16, s->stream);
         return r;
     }
@@ -803,7 +816,7 @@ static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &r
         // the reference's Tier-2 search: final quarter-sample vectors, no 8x8 ones
         int r = bc250_launch_square_me((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->w, s->h,
                                        p.square_radius, s->mvs, s->stream);
-        hipEventRecord(mid, s->stream);
+        mark(s, mid, s->stream);
         if (r == 0) r = bc250_launch_zero_i32((int *)s->mv8, s->nmb * 16, s->stream);
         return r;
     }
@@ -819,7 +832,7 @@ static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &r
         r = bc250_launch_me_fullsearch((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc,
                                        p.search_range, p.lambda8, s->imv16, s->imv8, s->stream);
     }
-    hipEventRecord(mid, s->stream);
+    mark(s, mid, s->stream);
     if (r == 0)
         r = bc250_launch_me_refine((const uint8_t *)rt.y_plane, rt.y_pitch, ref.y, ref.py, s->wc, s->hc, i16, i8,
                                    s->mvs, s->mv8, p.me_8x8, s->stream);
@@ -837,7 +850,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
     const size_t n = (size_t)s->nmb;
     const int bd = s->bps == 2 ? 10 : 8;
     int r = 0;
-    hipEventRecord(sl.ev[0], st);
+    mark(s, sl.ev[0], st);
     if (!is_intra) {
         // the integer search runs on 8-bit luma: at 10 bits the top 8 bits of each sample
         const uint8_t *c8 = cur.y, *r8 = ref.y;
@@ -860,10 +873,10 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
                                            s->imv8, st);
         }
     }
-    hipEventRecord(sl.ev[1], st);
+    mark(s, sl.ev[1], st);
     // P pictures: the quarter-sample refinement runs fused with the CU coding
     // below (bc250_hevc_launch_refine_p_encode), so it is timed with the coding
-    hipEventRecord(sl.ev[2], st);
+    mark(s, sl.ev[2], st);
     if (r == 0) {
         if (is_intra) {
             r = bc250_hevc_launch_i_encode_ctu(cur.y, cur.uv, cur.py, cur.puv, out.y, out.uv, out.py, out.puv,
@@ -884,7 +897,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
                                                   c32 ? nullptr : s->hcbf, bd, st);
         }
     }
-    hipEventRecord(sl.ev[3], st);
+    mark(s, sl.ev[3], st);
     // 8.7.2 deblocking (the reference encoder's PPS turns it off: deblock_idc 1);
     // slices never filter across their borders, so idc 0 and 2 agree
     if (r == 0 && p.deblock_idc != 1) {
@@ -894,7 +907,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
             r = bc250_hevc_launch_deblock_ctu(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->hcbf,
                                               s->mv8, cus, num_slices, p.hevc_ctu_log2, bd, st);
     }
-    hipEventRecord(sl.ev[4], st);
+    mark(s, sl.ev[4], st);
     if (r == 0)
         r = finish_frame(s, sl, is_intra ? s->sync_h + 1 : nullptr, nullptr, s->fb_frame,
                          RB_LEVELS | RB_HMODES | RB_MV8);
@@ -957,15 +970,15 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     if (p.entropy && (num_slices > 64 || num_slices > s->nmb)) return -1;
     start_frame(s);
     int r = 0;
-    hipEventRecord(sl.ev[0], st);
+    mark(s, sl.ev[0], st);
     if (!is_intra && !cpu_mvs) {
         r = enqueue_me(s, rt, ref, me_mode, sl.ev[1]);
     } else {
-        hipEventRecord(sl.ev[1], st);
+        mark(s, sl.ev[1], st);
         if (!is_intra)
             r = HIP_OK(hipMemcpyAsync(s->mvs, cpu_mvs, (size_t)s->nmb * 16, hipMemcpyHostToDevice, st)) ? 0 : -1;
     }
-    hipEventRecord(sl.ev[2], st);
+    mark(s, sl.ev[2], st);
     if (r == 0) {
         if (is_intra) {
             r = bc250_launch_i_encode(cur_in.y, cur_in.uv, cur_in.py, cur_in.puv, out.y, out.uv, out.py, out.puv,
@@ -979,12 +992,12 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
                                       s->nz, s->modes, st);
         }
     }
-    hipEventRecord(sl.ev[3], st);
+    mark(s, sl.ev[3], st);
     if (r == 0 && p.deblock_idc != 1) {
         r = bc250_launch_deblock(out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp, is_intra, s->nz, s->mvs,
                                      num_slices, p.deblock_idc, s->sync_d, kSpinLimit, st);
     }
-    hipEventRecord(sl.ev[4], st);
+    mark(s, sl.ev[4], st);
     // frame_num / POC / idr_pic_id of this frame, committed once it is enqueued
     const unsigned fn = is_intra ? 0u : s->frame_num, poc = is_intra ? 0u : s->poc;
     const int idr_id = is_intra ? (s->idr_pic_id + 1) & 0xffff : (s->idr_pic_id < 0 ? 0 : s->idr_pic_id);
@@ -1057,9 +1070,9 @@ extern "C" int gpu_compute_dispatch_me_only(gpu_context_t *ctx, gpu_image_t rt, 
     cur_in.py = rt.y_pitch;
     const Plane2 &ref = s->has_recon ? s->recon[s->ref_idx] : cur_in;
     start_frame(s);
-    hipEventRecord(sl.ev[0], s->stream);
+    mark(s, sl.ev[0], s->stream);
     int r = enqueue_me(s, rt, ref, 0, sl.ev[1]);
-    for (int i = 2; i < NEV - 1; i++) hipEventRecord(sl.ev[i], s->stream);
+    for (int i = 2; i < NEV - 1; i++) mark(s, sl.ev[i], s->stream);
     if (r == 0) r = finish_frame(s, sl, nullptr, nullptr, s->fb_frame, RB_MVS | RB_MV8);
     sl.have_data = false;
     sl.recorded = false;
@@ -1105,7 +1118,7 @@ extern "C" int gpu_compute_sync_slot(gpu_context_t *ctx, int slot)
     }
     sl.pending = false;
     float ms = 0.f;
-    if (HIP_OK(hipEventElapsedTime(&ms, sl.ev[0], sl.ev[NEV - 1]))) {
+    if (sl.timed && HIP_OK(hipEventElapsedTime(&ms, sl.ev[0], sl.ev[NEV - 1]))) {
         s->last_latency_ms = ms;
         s->total_sum += ms;
         for (int i = 0; i < NSTAGE; i++) {
@@ -1348,12 +1361,20 @@ extern "C" int gpu_compute_download_recon(gpu_context_t *ctx, uint8_t *y_plane, 
 extern "C" int gpu_compute_get_stage_times(gpu_context_t *ctx, int slot, float ms[6])
 {
     bc250_hip_state *s = st_of(ctx);
-    if (!s || !ms || slot < 0 || slot >= NSLOT || !s->slot[slot].recorded) return -1;
+    if (!s || !ms || slot < 0 || slot >= NSLOT || !s->slot[slot].recorded || !s->slot[slot].timed) return -1;
     Slot &sl = s->slot[slot];
     if (!HIP_OK(hipEventSynchronize(sl.ev[NEV - 1]))) return -1;
     for (int i = 0; i < NSTAGE; i++)
         if (!HIP_OK(hipEventElapsedTime(&ms[i], sl.ev[i], sl.ev[i + 1]))) return -1;
     return HIP_OK(hipEventElapsedTime(&ms[NSTAGE], sl.ev[0], sl.ev[NEV - 1])) ? 0 : -1;
+}
+
+extern "C" int gpu_compute_set_stage_timing(gpu_context_t *ctx, int on)
+{
+    bc250_hip_state *s = st_of(ctx);
+    if (!s) return -1;
+    s->timing = on != 0;
+    return 0;
 }
 
 extern "C" int gpu_compute_get_stage_stats(gpu_context_t *ctx, gpu_stage_stats_t *st, int reset)

@@ -408,18 +408,53 @@ struct FusedInput {
     int cpitch, rpitch, w, h, tw;
 };
 
-__device__ __forceinline__ void stage_picture(uint32_t *__restrict__ q, const uint8_t *__restrict__ cur, int cpitch,
-                                              const uint8_t *__restrict__ ref, int rpitch, int w, int h, int tx,
-                                              int ty, int nthreads)
+__device__ __forceinline__ uint32_t input_pair(int c, int r)
 {
-    for (int i = threadIdx.x; i < TP * TP; i += nthreads) {
+    return (uint32_t)f2bf((c - 128) * (1.f / 64.f)) | ((uint32_t)f2bf((r - 128) * (1.f / 64.f)) << 16);
+}
+
+// The input tile (tx, ty) of a picture pair into q ([66][66] dwords, the
+// border zero).  A tile inside the picture with 4-byte aligned rows reads
+// both pictures as dwords, every thread's loads issued before their
+// conversions; an edge tile repeats the picture's last row / column sample
+// by sample.
+template <int NTH>
+__device__ __forceinline__ void stage_picture(uint32_t *__restrict__ q, const uint8_t *__restrict__ cur, int cpitch,
+                                              const uint8_t *__restrict__ ref, int rpitch, int w, int h, int tx, int ty)
+{
+    const int tid = threadIdx.x;
+    const bool inside = tx * T + T <= w && ty * T + T <= h &&
+                        !(((uintptr_t)cur | (uintptr_t)ref | (uintptr_t)cpitch | (uintptr_t)rpitch) & 3);
+    if (inside) {
+        constexpr int PER = T * T / 4 / NTH;
+        static_assert(PER * NTH * 4 == T * T, "whole dwords per thread");
+        uint32_t a[PER], c[PER];
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int i = tid + j * NTH, r = i >> 4, d = i & 15;
+            a[j] = *(const uint32_t *)(cur + (size_t)(ty * T + r) * cpitch + tx * T + 4 * d);
+            c[j] = *(const uint32_t *)(ref + (size_t)(ty * T + r) * rpitch + tx * T + 4 * d);
+        }
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int i = tid + j * NTH, r = i >> 4, d = i & 15;
+            uint32_t *o = q + (r + 1) * TP + 1 + 4 * d;
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = input_pair((a[j] >> (8 * k)) & 255, (c[j] >> (8 * k)) & 255);
+        }
+        for (int i = tid; i < 4 * (TP - 1); i += NTH) {
+            const int row = i < TP ? 0 : i < 2 * TP ? TP - 1 : i < 3 * TP - 2 ? i - 2 * TP + 1 : i - (3 * TP - 2) + 1;
+            const int col = i < TP ? i : i < 2 * TP ? i - TP : i < 3 * TP - 2 ? 0 : TP - 1;
+            q[row * TP + col] = 0u;
+        }
+        return;
+    }
+    for (int i = tid; i < TP * TP; i += NTH) {
         const int tr = i / TP - 1, tc = i % TP - 1;
         uint32_t v = 0u;
         if (tr >= 0 && tr < T && tc >= 0 && tc < T) {
             const int gy = min(ty * T + tr, h - 1), gx = min(tx * T + tc, w - 1);
-            const float a = ((int)cur[(size_t)gy * cpitch + gx] - 128) * (1.f / 64.f);
-            const float c = ((int)ref[(size_t)gy * rpitch + gx] - 128) * (1.f / 64.f);
-            v = (uint32_t)f2bf(a) | ((uint32_t)f2bf(c) << 16);
+            v = input_pair(cur[(size_t)gy * cpitch + gx], ref[(size_t)gy * rpitch + gx]);
         }
         q[i] = v;
     }
@@ -446,12 +481,12 @@ __device__ __forceinline__ void cnn_forward_tile(const FusedInput &src, const bc
         uint32_t *q = (uint32_t *)sa;
         for (int i = tid; i < TP * TP; i += NTH) q[i] = g[i];
     } else if constexpr (FROM == FROM_PICTURE) {
-        stage_picture((uint32_t *)sa, src.cur, src.cpitch, src.ref, src.rpitch, src.w, src.h, tile % src.tw,
-                      tile / src.tw, NTH);
+        stage_picture<NTH>((uint32_t *)sa, src.cur, src.cpitch, src.ref, src.rpitch, src.w, src.h, tile % src.tw,
+                           tile / src.tw);
     } else {
         const bc250_batch_frame_t &fr = bt->f[batch_frame_of_tile(*bt, tile)];
         const int tw = (fr.wc / 16 + 3) / 4, lt = tile - fr.tile_base;
-        stage_picture((uint32_t *)sa, fr.cur, fr.cpitch, fr.ref, fr.rpitch, fr.wc, fr.hc, lt % tw, lt / tw, NTH);
+        stage_picture<NTH>((uint32_t *)sa, fr.cur, fr.cpitch, fr.ref, fr.rpitch, fr.wc, fr.hc, lt % tw, lt / tw);
     }
     __syncthreads();
     zero_border<16, SW, F1P>(sb, NTH);

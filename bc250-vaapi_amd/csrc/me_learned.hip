@@ -82,16 +82,21 @@ __global__ void cnn_candidates_kernel(const float *__restrict__ z, int mbw, int 
     candidates_mb(z, mbw, tw, range, m, cand);
 }
 
-// One wave per MB.  The current block is wave-uniform; the windows of all
-// candidates (edge-replicated like the full search's) are staged in LDS at
-// once, then a lane evaluates every 64th of the ncand x (2W+1)^2 positions.
+// One wave per MB.  The current block is wave-uniform (scalar registers);
+// the windows of all candidates (edge-replicated like the full search's)
+// are staged in LDS at once: a window inside the picture as aligned dwords,
+// every lane's loads issued before its stores, one touching the edge sample
+// by sample.  Lane 9 k + ox (k < ncand, ox = 0 .. 8) then owns the nine
+// positions (ox, oy = 0 .. 8) of window k: it reads each window row once and
+// adds it to every position whose block covers that row, so a row's LDS
+// reads and byte alignment serve up to nine positions.
 __device__ __forceinline__ void cand_search_mb(const uint8_t *__restrict__ cur, int cpitch,
                                                const uint8_t *__restrict__ ref, int rpitch, int wc, int hc, int mbw,
                                                int mb, int range, int lambda8, const int2 *__restrict__ cand,
                                                int ncand, uint4 *__restrict__ out16, uint4 *__restrict__ out8,
                                                uint32_t *win)
 {
-    constexpr int NP = (2 * CW + 1) * (2 * CW + 1), WIN = WS * WP;
+    constexpr int NO = 2 * CW + 1, WIN = WS * WP, NROW = 4 * WS;
     const int lane = threadIdx.x;
     const int x0 = (mb % mbw) * 16, y0 = (mb / mbw) * 16;
     const int nc = 2 * range + 1;
@@ -105,52 +110,90 @@ __device__ __forceinline__ void cand_search_mb(const uint8_t *__restrict__ cur, 
     int2 cv[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) cv[k] = k < ncand ? cand[(size_t)mb * 4 + k] : make_int2(0, 0);
-    // window k's origin: the block at vector (c.x - W, c.y - W)
-    for (int i = lane; i < ncand * WS * (WS / 4); i += 64) {
-        const int k = i / (WS * (WS / 4)), e = i % (WS * (WS / 4));
-        const int r = e / (WS / 4), d = e % (WS / 4);
+    // window k's origin: the block at vector (c.x - W, c.y - W); rows of all
+    // windows, two per lane
+    const bool aligned = !(((uintptr_t)ref | (uintptr_t)rpitch) & 3);
+    uint32_t raw[2][7];
+    int rsh[2];
+    bool fast[2];
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+        const int i = lane + 64 * it, k = i / WS, r = i % WS;
         const int2 c = k == 0 ? cv[0] : k == 1 ? cv[1] : k == 2 ? cv[2] : cv[3];
-        const uint8_t *row = ref + (size_t)clampi(y0 + c.y - CW + r, 0, hc - 1) * rpitch;
-        const int gx = x0 + c.x - CW + 4 * d;
-        uint32_t v;
-        if (gx >= 0 && gx + 3 < wc) {
-            v = (uint32_t)row[gx] | ((uint32_t)row[gx + 1] << 8) | ((uint32_t)row[gx + 2] << 16) |
-                ((uint32_t)row[gx + 3] << 24);
-        } else {
-            v = (uint32_t)row[clampi(gx, 0, wc - 1)] | ((uint32_t)row[clampi(gx + 1, 0, wc - 1)] << 8) |
-                ((uint32_t)row[clampi(gx + 2, 0, wc - 1)] << 16) | ((uint32_t)row[clampi(gx + 3, 0, wc - 1)] << 24);
+        const int gx = x0 + c.x - CW, gy = y0 + c.y - CW + r;
+        fast[it] = i < ncand * WS && aligned && gx >= 0 && gx + WS <= wc && gy >= 0 && gy < hc;
+        rsh[it] = gx & 3;
+        if (fast[it]) {
+            const uint32_t *src = (const uint32_t *)(ref + (size_t)gy * rpitch + (gx & ~3));
+#pragma unroll
+            for (int d = 0; d < 6; d++) raw[it][d] = src[d];
+            raw[it][6] = rsh[it] ? src[6] : 0u;  // only when the row reaches into it
         }
-        win[k * WIN + r * WP + d] = v;
     }
-    for (int i = lane; i < ncand * WS; i += 64) win[(i / WS) * WIN + (i % WS) * WP + WS / 4] = 0;  // pad dwords
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+        const int i = lane + 64 * it, k = i / WS, r = i % WS;
+        if (i >= ncand * WS || i >= NROW) continue;
+        uint32_t *dst = win + k * WIN + r * WP;
+        if (fast[it]) {
+#pragma unroll
+            for (int d = 0; d < 6; d++) dst[d] = __builtin_amdgcn_alignbyte(raw[it][d + 1], raw[it][d], rsh[it]);
+        } else {
+            const int2 c = k == 0 ? cv[0] : k == 1 ? cv[1] : k == 2 ? cv[2] : cv[3];
+            const uint8_t *row = ref + (size_t)clampi(y0 + c.y - CW + r, 0, hc - 1) * rpitch;
+#pragma unroll
+            for (int d = 0; d < 6; d++) {
+                const int gx = x0 + c.x - CW + 4 * d;
+                dst[d] = (uint32_t)row[clampi(gx, 0, wc - 1)] | ((uint32_t)row[clampi(gx + 1, 0, wc - 1)] << 8) |
+                         ((uint32_t)row[clampi(gx + 2, 0, wc - 1)] << 16) |
+                         ((uint32_t)row[clampi(gx + 3, 0, wc - 1)] << 24);
+            }
+        }
+        dst[WS / 4] = 0;  // pad dword
+    }
     __syncthreads();
     uint32_t b16 = 0xffffffffu, bq[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
-    for (int pi = lane; pi < ncand * NP; pi += 64) {
-        const int k = pi / NP, pp = pi % NP;
+    if (lane < ncand * NO) {
+        const int k = lane / NO, ox = lane % NO;
         const int2 c = k == 0 ? cv[0] : k == 1 ? cv[1] : k == 2 ? cv[2] : cv[3];
-        const int ox = pp % (2 * CW + 1), oy = pp / (2 * CW + 1);
-        const int dx = c.x - CW + ox, dy = c.y - CW + oy;
-        if (dx < -range || dx > range || dy < -range || dy > range) continue;
         const int sh = ox & 3, d0 = ox >> 2;
-        uint32_t q[4] = {0, 0, 0, 0};
-        for (int r = 0; r < 16; r++) {
-            const uint32_t *wr = win + k * WIN + (oy + r) * WP + d0;
+        uint32_t q[NO][4];
+#pragma unroll
+        for (int oy = 0; oy < NO; oy++)
+#pragma unroll
+            for (int h = 0; h < 4; h++) q[oy][h] = 0;
+        const uint32_t *wk = win + k * WIN + d0;
+#pragma unroll
+        for (int r = 0; r < WS; r++) {
             uint32_t d[5];
 #pragma unroll
-            for (int i = 0; i < 5; i++) d[i] = wr[i];
+            for (int i = 0; i < 5; i++) d[i] = wk[r * WP + i];
             uint32_t s[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) s[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
-            const int h = (r >> 3) * 2;
-            q[h] = __builtin_amdgcn_sad_u8(s[1], cw[r][1], __builtin_amdgcn_sad_u8(s[0], cw[r][0], q[h]));
-            q[h + 1] = __builtin_amdgcn_sad_u8(s[3], cw[r][3], __builtin_amdgcn_sad_u8(s[2], cw[r][2], q[h + 1]));
-        }
-        const uint32_t l1 = (uint32_t)(iabs(dx) + iabs(dy));
-        const uint32_t idx = (uint32_t)((dy + range) * nc + (dx + range));
-        const uint32_t sad16 = q[0] + q[1] + q[2] + q[3];
-        b16 = min(b16, ((sad16 + 4u * lambda8 * l1) << 15) | idx);
 #pragma unroll
-        for (int qq = 0; qq < 4; qq++) bq[qq] = min(bq[qq], ((q[qq] + (uint32_t)lambda8 * l1) << 15) | idx);
+            for (int oy = 0; oy < NO; oy++) {
+                const int cr = r - oy;  // the block row this window row meets at offset oy
+                if (cr >= 0 && cr < 16) {
+                    const int h = (cr >> 3) * 2;
+                    q[oy][h] = __builtin_amdgcn_sad_u8(s[1], cw[cr][1], __builtin_amdgcn_sad_u8(s[0], cw[cr][0], q[oy][h]));
+                    q[oy][h + 1] =
+                        __builtin_amdgcn_sad_u8(s[3], cw[cr][3], __builtin_amdgcn_sad_u8(s[2], cw[cr][2], q[oy][h + 1]));
+                }
+            }
+        }
+        const int dx = c.x - CW + ox;
+#pragma unroll
+        for (int oy = 0; oy < NO; oy++) {
+            const int dy = c.y - CW + oy;
+            if (dx < -range || dx > range || dy < -range || dy > range) continue;
+            const uint32_t l1 = (uint32_t)(iabs(dx) + iabs(dy));
+            const uint32_t idx = (uint32_t)((dy + range) * nc + (dx + range));
+            const uint32_t sad16 = q[oy][0] + q[oy][1] + q[oy][2] + q[oy][3];
+            b16 = min(b16, ((sad16 + 4u * lambda8 * l1) << 15) | idx);
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) bq[qq] = min(bq[qq], ((q[oy][qq] + (uint32_t)lambda8 * l1) << 15) | idx);
+        }
     }
     b16 = wave_min_u32(b16);
 #pragma unroll
@@ -258,45 +301,86 @@ __global__ void fallback_select_batch_kernel(const bc250_me_batch_t b, int alpha
 
 // ---- proposals in one launch: per MB its neighbour-vector features, the
 // head's 12 outputs (the order and fused multiply-adds of cnn_head_kernel)
-// and the candidates (cand_from_z); pool / prev16 / cand of one picture.
-__device__ __forceinline__ void propose_mb(const float *__restrict__ pool, const float *__restrict__ wh,
-                                           const uint4 *__restrict__ prev16, int mbw, int mbh, int range, int m,
-                                           int2 *__restrict__ cand)
+// and the candidates (cand_from_z).  A workgroup takes PMB consecutive MBs
+// (batch-wide for a batch), one thread per (MB, neighbour) for the features,
+// per (MB, output) for the head and per MB for the candidates.
+constexpr int PMB = 16;
+
+struct ProposeSrc {       // one picture's inputs and candidate array
+    const float *pool;    // [tile][16][32] of the picture
+    const uint4 *prev16;
+    int2 *cand;
+    int mbw, mbh, m;      // m: the local MB, -1 for none
+};
+
+template <typename GetSrc>
+__device__ __forceinline__ void propose_group(const float *__restrict__ wh, int range, int g0, GetSrc src_of)
 {
-    const int tw = (mbw + 3) / 4, mx = m % mbw, my = m / mbw;
-    const int tile = (my >> 2) * tw + (mx >> 2), j = (my & 3) * 4 + (mx & 3);
-    float mv[32];
-    mvin_block(prev16, mbw, mbh, tw, tile, j, mv);
-    const float *pb = pool + ((size_t)tile * 16 + j) * 32;
-    float z[BC250_CNN_NOUT];
-#pragma unroll
-    for (int o = 0; o < BC250_CNN_NOUT; o++) {
-        const float *row = wh + o * (BC250_CNN_NFEAT + 1);
-        float s = row[BC250_CNN_NFEAT];
-        for (int i = 0; i < 32; i++) s = fmaf(row[i], pb[i] * BC250_CNN_POOL_SCALE, s);
-        for (int i = 0; i < 32; i++) s = fmaf(row[32 + i], mv[i], s);
-        z[o] = s;
+    __shared__ float smv[PMB][32];
+    __shared__ float sz[PMB][BC250_CNN_NOUT];
+    const int t = threadIdx.x;
+    for (int i = t; i < PMB * 16; i += PMB * BC250_CNN_NOUT) {
+        const int j = i >> 4, k = i & 15;
+        const ProposeSrc s = src_of(g0 + j);
+        if (s.m >= 0) {
+            const int mx = s.m % s.mbw, my = s.m / s.mbw;
+            const int nx = clampi(mx + (k & 3) - 1, 0, s.mbw - 1), ny = clampi(my + (k >> 2) - 1, 0, s.mbh - 1);
+            const uint4 v = s.prev16 ? s.prev16[ny * s.mbw + nx] : make_uint4(0, 0, 0, 0);
+            smv[j][2 * k] = (float)(int)v.x * (1.f / 16.f);
+            smv[j][2 * k + 1] = (float)(int)v.y * (1.f / 16.f);
+        }
     }
-    cand_from_z(z, range, m, cand);
+    __syncthreads();
+    if (t < PMB * BC250_CNN_NOUT) {
+        const int j = t / BC250_CNN_NOUT, o = t % BC250_CNN_NOUT;
+        const ProposeSrc s = src_of(g0 + j);
+        if (s.m >= 0) {
+            const int tw = (s.mbw + 3) / 4, mx = s.m % s.mbw, my = s.m / s.mbw;
+            const int tile = (my >> 2) * tw + (mx >> 2), jb = (my & 3) * 4 + (mx & 3);
+            const float *row = wh + o * (BC250_CNN_NFEAT + 1);
+            const float4 *pb = (const float4 *)(s.pool + ((size_t)tile * 16 + jb) * 32);
+            float sum = row[BC250_CNN_NFEAT];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const float4 v = pb[q];
+                sum = fmaf(row[4 * q], v.x * BC250_CNN_POOL_SCALE, sum);
+                sum = fmaf(row[4 * q + 1], v.y * BC250_CNN_POOL_SCALE, sum);
+                sum = fmaf(row[4 * q + 2], v.z * BC250_CNN_POOL_SCALE, sum);
+                sum = fmaf(row[4 * q + 3], v.w * BC250_CNN_POOL_SCALE, sum);
+            }
+#pragma unroll
+            for (int i = 0; i < 32; i++) sum = fmaf(row[32 + i], smv[j][i], sum);
+            sz[j][o] = sum;
+        }
+    }
+    __syncthreads();
+    if (t < PMB) {
+        const ProposeSrc s = src_of(g0 + t);
+        if (s.m >= 0) cand_from_z(sz[t], range, s.m, s.cand);
+    }
 }
 
-__global__ void cnn_propose_kernel(const float *__restrict__ pool, const float *__restrict__ wh,
-                                   const uint4 *__restrict__ prev16, int mbw, int mbh, int range,
-                                   int2 *__restrict__ cand)
+__global__ __launch_bounds__(PMB * BC250_CNN_NOUT) void cnn_propose_kernel(const float *__restrict__ pool,
+                                                                          const float *__restrict__ wh,
+                                                                          const uint4 *__restrict__ prev16, int mbw,
+                                                                          int mbh, int range, int2 *__restrict__ cand)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= mbw * mbh) return;
-    propose_mb(pool, wh, prev16, mbw, mbh, range, m, cand);
+    const int nmb = mbw * mbh;
+    propose_group(wh, range, blockIdx.x * PMB, [&](int m) {
+        return ProposeSrc{pool, prev16, cand, mbw, mbh, m < nmb ? m : -1};
+    });
 }
 
-__global__ void cnn_propose_batch_kernel(const bc250_me_batch_t b, const float *__restrict__ pool,
-                                         const float *__restrict__ wh)
+__global__ __launch_bounds__(PMB * BC250_CNN_NOUT) void cnn_propose_batch_kernel(const bc250_me_batch_t b,
+                                                                                const float *__restrict__ pool,
+                                                                                const float *__restrict__ wh)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= b.nmb_total) return;
-    const bc250_batch_frame_t &fr = b.f[batch_frame_of_mb(b, g)];
-    propose_mb(pool + (size_t)fr.tile_base * 16 * 32, wh, (const uint4 *)fr.prev16, fr.wc / 16, fr.hc / 16, b.range,
-               g - fr.mb_base, (int2 *)fr.cand);
+    propose_group(wh, b.range, blockIdx.x * PMB, [&](int g) {
+        if (g >= b.nmb_total) return ProposeSrc{nullptr, nullptr, nullptr, 1, 1, -1};
+        const bc250_batch_frame_t &fr = b.f[batch_frame_of_mb(b, g)];
+        return ProposeSrc{pool + (size_t)fr.tile_base * 16 * 32, (const uint4 *)fr.prev16, (int2 *)fr.cand,
+                          fr.wc / 16, fr.hc / 16, g - fr.mb_base};
+    });
 }
 
 }  // namespace
@@ -350,7 +434,7 @@ extern "C" int bc250_cnn_propose(const float *pool, const float *params, const u
 {
     if (!pool || !params || !cand || mbw <= 0 || mbh <= 0 || range < CW || range > 64) return -1;
     const int nmb = mbw * mbh;
-    cnn_propose_kernel<<<(nmb + 127) / 128, 128, 0, (hipStream_t)stream>>>(
+    cnn_propose_kernel<<<(nmb + PMB - 1) / PMB, PMB * BC250_CNN_NOUT, 0, (hipStream_t)stream>>>(
         pool, params + bc250_cnn_w_offset(4), (const uint4 *)prev16, mbw, mbh, range, (int2 *)cand);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -433,7 +517,8 @@ extern "C" int bc250_learned_search_batch(const bc250_me_batch_t *b, const float
     hipStream_t st = (hipStream_t)stream;
     int r = bc250_cnn_forward_batch(b, packed, params, w.pool, st);
     if (r == 0) {
-        cnn_propose_batch_kernel<<<(b->nmb_total + 127) / 128, 128, 0, st>>>(*b, w.pool, params + bc250_cnn_w_offset(4));
+        cnn_propose_batch_kernel<<<(b->nmb_total + PMB - 1) / PMB, PMB * BC250_CNN_NOUT, 0, st>>>(
+            *b, w.pool, params + bc250_cnn_w_offset(4));
         cand_search_batch_kernel<<<((b->nmb_total + 7) / 8) * 8, 64, 0, st>>>(*b, ncand, w.count);
         fallback_select_batch_kernel<<<(b->nmb_total + 127) / 128, 128, 0, st>>>(*b, alpha_q4, beta, t_high, w.list,
                                                                                  w.count, w.keys);

@@ -244,9 +244,17 @@ int gpu_compute_get_hevc_mode_staging_data_slot(gpu_context_t *ctx, int slot, vo
 int gpu_compute_download_recon(gpu_context_t *ctx, uint8_t *y_plane, int y_pitch, uint8_t *uv_plane,
                                int uv_pitch);
 
+/* Stage timing: with it on, every frame records a HIP event at each stage
+ * boundary (a few microseconds of stream time each); off (the default,
+ * unless the environment sets BC250_PERF_STATS when the context is created,
+ * as the reference gates its timestamp queries, gpu_compute.c:3684-3708),
+ * frames record none and the two getters below report nothing new. */
+int gpu_compute_set_stage_timing(gpu_context_t *ctx, int on);
+
 /* GPU time of the stages of the frame in `slot` (ms), from HIP events on
  * the context's stream: [0] integer search, [1] sub-pel refine, [2] intra
- * or inter encode, [3] deblock, [4] staging copies, [5] total. */
+ * or inter encode, [3] deblock, [4] staging copies, [5] total; -1 when the
+ * frame ran without stage timing. */
 int gpu_compute_get_stage_times(gpu_context_t *ctx, int slot, float ms[6]);
 
 /* Stage times summed over every frame synced since the last reset (the
