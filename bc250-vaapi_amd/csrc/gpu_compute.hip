@@ -51,6 +51,12 @@ struct DevOut {
     int32_t *dc = nullptr;
     uint32_t *nz = nullptr, *modes = nullptr, *mvs = nullptr, *mv8 = nullptr;
     uint8_t *hmodes = nullptr;
+    // the levels of a P frame packed for the readback: every 16-level group
+    // holding a nonzero level (lvdata, 32 bytes each) with its group index
+    // (lvidx), in no particular order, and their count
+    uint32_t *lvidx = nullptr;
+    uint4 *lvdata = nullptr;
+    int *lvcount = nullptr;
 };
 
 struct Slot {
@@ -68,6 +74,18 @@ struct Slot {
     DevOut dev;
     bool pending = false, recorded = false, have_data = false, was_intra = false;
     bool timed = false;          // the stage events ev[0 .. NEV-2] were recorded for this frame
+    // packed levels in host memory (pinned): the copies of DevOut's, the
+    // first lv_cap groups of them; `levels` is rebuilt from them when read
+    uint32_t *lvidx = nullptr;
+    int16_t *lvdata = nullptr;
+    int *lvcount = nullptr;
+    int lv_cap = 0;
+    bool lv_packed = false;        // this frame's levels arrived packed
+    bool lv_settled = false;       // ... their count was checked against lv_cap
+    bool lv_built = false;         // ... and `levels` holds them again
+    uint32_t *built_idx = nullptr; // groups `levels` holds nonzero (host heap)
+    int built_n = 0;
+    bool built_all = true;         // `levels` may hold nonzero levels anywhere
 };
 
 // The learned proposer's weights and per-resolution buffers
@@ -111,6 +129,7 @@ struct bc250_hip_state {
     uint8_t *hculog2 = nullptr;                   // HEVC P: log2 CU size per CU8 (CTU32 CU choice)
     Slot slot[NSLOT];
     int cur = 0, submitted = 0;
+    int lv_cap_next = 0;   // packed level groups the next P frame copies back
     double last_latency_ms = 0.0;
     // stage events on the compute stream (gpu_compute_set_stage_timing; on
     // when BC250_PERF_STATS is set, as the reference's timestamp queries)
@@ -224,7 +243,7 @@ static void free_res(bc250_hip_state *s)
         if (p) hipFree(p);
     for (auto &sl : s->slot) {
         DevOut &o = sl.dev;
-        void *odev[] = {o.levels, o.dc, o.nz, o.modes, o.mvs, o.mv8, o.hmodes};
+        void *odev[] = {o.levels, o.dc, o.nz, o.modes, o.mvs, o.mv8, o.hmodes, o.lvidx, o.lvdata, o.lvcount};
         for (void *p : odev)
             if (p) hipFree(p);
         o = DevOut{};
@@ -250,11 +269,17 @@ static void free_res(bc250_hip_state *s)
         for (void *p : {(void *)sl.rbsp, (void *)sl.rbsp_info})
             if (p) hipHostFree(p);
         sl.rbsp = nullptr; sl.rbsp_info = nullptr; sl.slices = 0;
-        void *host[] = {sl.levels, sl.dc, sl.modes, sl.nz, sl.mvs, sl.mv8, sl.hmodes, sl.err};
+        void *host[] = {sl.levels, sl.dc, sl.modes, sl.nz, sl.mvs, sl.mv8, sl.hmodes, sl.err, sl.lvidx, sl.lvdata,
+                        sl.lvcount};
         for (void *p : host)
             if (p) hipHostFree(p);
+        free(sl.built_idx);
         sl.levels = nullptr; sl.dc = nullptr; sl.modes = nullptr; sl.nz = nullptr; sl.mvs = nullptr;
         sl.mv8 = nullptr; sl.hmodes = nullptr; sl.err = nullptr;
+        sl.lvidx = nullptr; sl.lvdata = nullptr; sl.lvcount = nullptr; sl.built_idx = nullptr;
+        sl.lv_packed = sl.lv_built = false;
+        sl.built_n = 0;
+        sl.built_all = true;
         sl.pending = sl.have_data = false;
     }
     s->w = s->h = s->wc = s->hc = s->mbw = s->mbh = s->nmb = 0;
@@ -288,7 +313,9 @@ static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
         ok = ok && HIP_OK(hipMalloc(&o.levels, (size_t)nmb * 384 * 2)) && HIP_OK(hipMalloc(&o.dc, (size_t)nmb * 24 * 4)) &&
              HIP_OK(hipMalloc(&o.nz, (size_t)nmb * 24 * 4)) && HIP_OK(hipMalloc(&o.modes, (size_t)nmb * 4)) &&
              HIP_OK(hipMalloc(&o.mvs, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&o.mv8, (size_t)nmb * 64)) &&
-             HIP_OK(hipMalloc(&o.hmodes, (size_t)nmb * 16));
+             HIP_OK(hipMalloc(&o.hmodes, (size_t)nmb * 16)) &&
+             HIP_OK(hipMalloc(&o.lvidx, (size_t)nmb * 24 * 4)) && HIP_OK(hipMalloc(&o.lvdata, (size_t)nmb * 24 * 32)) &&
+             HIP_OK(hipMalloc(&o.lvcount, 4));
     }
     ok = ok && HIP_OK(hipMalloc(&s->imv16, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->imv8, (size_t)nmb * 64)) &&
          HIP_OK(hipMalloc(&s->sync_i, bc250_intra_work_bytes(wc, hc))) &&
@@ -304,8 +331,17 @@ static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
              HIP_OK(hipHostMalloc((void **)&sl.mvs, (size_t)nmb * 16, hipHostMallocDefault)) &&
              HIP_OK(hipHostMalloc((void **)&sl.mv8, (size_t)nmb * 64, hipHostMallocDefault)) &&
              HIP_OK(hipHostMalloc((void **)&sl.hmodes, (size_t)nmb * 16, hipHostMallocDefault)) &&
-             HIP_OK(hipHostMalloc((void **)&sl.err, 4 * sizeof(int), hipHostMallocDefault));
+             HIP_OK(hipHostMalloc((void **)&sl.err, 4 * sizeof(int), hipHostMallocDefault)) &&
+             HIP_OK(hipHostMalloc((void **)&sl.lvidx, (size_t)nmb * 24 * 4, hipHostMallocDefault)) &&
+             HIP_OK(hipHostMalloc((void **)&sl.lvdata, (size_t)nmb * 24 * 32, hipHostMallocDefault)) &&
+             HIP_OK(hipHostMalloc((void **)&sl.lvcount, sizeof(int), hipHostMallocDefault));
+        sl.built_idx = ok ? (uint32_t *)malloc((size_t)nmb * 24 * 4) : nullptr;
+        ok = ok && sl.built_idx;
+        sl.built_n = 0;
+        sl.built_all = true;
+        sl.lv_packed = sl.lv_built = false;
     }
+    s->lv_cap_next = nmb * 24 / 16;
     // wavefront workspaces start zeroed: no hand-off record or epoch holds a
     // value from whatever used the memory before
     ok = ok && HIP_OK(hipMemset(s->sync_i, 0, bc250_intra_work_bytes(wc, hc))) &&
@@ -345,6 +381,35 @@ static void *ensure_xfer(bc250_hip_state *s, size_t bytes)
     if (!HIP_OK(hipHostMalloc(&s->xfer, bytes, hipHostMallocDefault))) return nullptr;
     s->xfer_size = bytes;
     return s->xfer;
+}
+
+// Packs a P frame's levels for the readback: each thread one 16-level group
+// (32 bytes); the groups holding a nonzero level go to idx / data at slots
+// taken one atomic add per wave.  *count starts at zero.
+__global__ __launch_bounds__(256) void pack_levels_kernel(const int16_t *__restrict__ levels, int ngroups,
+                                                          uint32_t *__restrict__ idx, uint4 *__restrict__ data,
+                                                          int *__restrict__ count)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
+    uint4 a = make_uint4(0, 0, 0, 0), b = a;
+    if (g < ngroups) {
+        const uint4 *p = (const uint4 *)(levels + (size_t)g * 16);
+        a = p[0];
+        b = p[1];
+    }
+    const bool nz = (a.x | a.y | a.z | a.w | b.x | b.y | b.z | b.w) != 0;
+    const uint64_t m = __ballot(nz);
+    if (!m) return;
+    const int first = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == first) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, first);
+    if (nz) {
+        const int k = base + __popcll(m & ((1ull << lane) - 1));
+        idx[k] = (uint32_t)g;
+        data[2 * k] = a;
+        data[2 * k + 1] = b;
+    }
 }
 
 // The wavefront kernels' wait-error flags of the frame ([0] intra, [1]
@@ -778,10 +843,21 @@ static void start_frame(bc250_hip_state *s)
 // (fb, null = not used) go to the slot's pinned flag words, and,
 // with readback, the copy stream copies the listed outputs to the slot's
 // pinned staging once the stages are done.
-enum { RB_LEVELS = 1, RB_DC = 2, RB_MODES = 4, RB_MVS = 8, RB_NZ = 16, RB_MV8 = 32, RB_HMODES = 64 };
+// RB_PACK (P frames): the levels travel packed (pack_levels_kernel), the
+// first lv_cap_next groups of them; sync_slot fetches all the levels when
+// more groups than that were packed, and the staging getter rebuilds the
+// full levels on the host from the packed ones.
+enum { RB_LEVELS = 1, RB_DC = 2, RB_MODES = 4, RB_MVS = 8, RB_NZ = 16, RB_MV8 = 32, RB_HMODES = 64, RB_PACK = 128 };
 static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *e1, const int *fb, int what)
 {
     const size_t n = (size_t)s->nmb;
+    const bool pack = s->params.readback && (what & RB_LEVELS) && (what & RB_PACK);
+    if (pack) {
+        const int ng = s->nmb * 24;
+        zero_i32_kernel<<<1, 64, 0, s->stream>>>(sl.dev.lvcount, 1);
+        pack_levels_kernel<<<(ng + 255) / 256, 256, 0, s->stream>>>(sl.dev.levels, ng, sl.dev.lvidx, sl.dev.lvdata,
+                                                                     sl.dev.lvcount);
+    }
     collect_flags_kernel<<<1, 64, 0, s->stream>>>(e0, e1, fb, sl.err);
     if (!HIP_OK(hipGetLastError()) || !HIP_OK(hipEventRecord(sl.cdone, s->stream)) ||
         !HIP_OK(hipStreamWaitEvent(s->cstream, sl.cdone, 0)))
@@ -791,6 +867,17 @@ static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *
     hipStream_t cs = s->cstream;
     const hipMemcpyKind k = hipMemcpyDeviceToHost;
     bool ok = true;
+    sl.lv_packed = pack;
+    sl.lv_settled = sl.lv_built = false;
+    if (pack) {
+        sl.lv_cap = s->lv_cap_next;
+        ok = HIP_OK(hipMemcpyAsync(sl.lvcount, o.lvcount, sizeof(int), k, cs)) &&
+             HIP_OK(hipMemcpyAsync(sl.lvidx, o.lvidx, (size_t)sl.lv_cap * 4, k, cs)) &&
+             HIP_OK(hipMemcpyAsync(sl.lvdata, o.lvdata, (size_t)sl.lv_cap * 32, k, cs));
+        what &= ~RB_LEVELS;
+    } else if (what & RB_LEVELS) {
+        sl.built_all = true;  // the whole array arrives: any group may be nonzero
+    }
     if (what & RB_LEVELS) ok = ok && HIP_OK(hipMemcpyAsync(sl.levels, o.levels, n * 384 * 2, k, cs));
     if (what & RB_DC) ok = ok && HIP_OK(hipMemcpyAsync(sl.dc, o.dc, n * 24 * 4, k, cs));
     if (what & RB_MODES) ok = ok && HIP_OK(hipMemcpyAsync(sl.modes, o.modes, n * 4, k, cs));
@@ -910,7 +997,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
     mark(s, sl.ev[4], st);
     if (r == 0)
         r = finish_frame(s, sl, is_intra ? s->sync_h + 1 : nullptr, nullptr, s->fb_frame,
-                         RB_LEVELS | RB_HMODES | RB_MV8);
+                         RB_LEVELS | RB_HMODES | RB_MV8 | (is_intra ? 0 : RB_PACK));
     return r;
 }
 
@@ -1030,7 +1117,7 @@ extern "C" int gpu_compute_dispatch_encode_ext(gpu_context_t *ctx, gpu_image_t r
     if (r == 0)
         r = finish_frame(s, sl, is_intra ? s->sync_i + 1 : nullptr, p.deblock_idc != 1 ? s->sync_d + 1 : nullptr,
                          s->fb_frame,
-                         RB_LEVELS | RB_DC | RB_MODES | RB_MVS | RB_NZ | RB_MV8);
+                         RB_LEVELS | RB_DC | RB_MODES | RB_MVS | RB_NZ | RB_MV8 | (is_intra ? 0 : RB_PACK));
     sl.have_data = r == 0 && p.readback;
     sl.was_intra = is_intra != 0;
     sl.recorded = false;
@@ -1106,6 +1193,8 @@ extern "C" int gpu_compute_submitted_slot(gpu_context_t *ctx)
     return s ? s->submitted : 0;
 }
 
+static int settle_levels(bc250_hip_state *s, Slot &sl);
+
 extern "C" int gpu_compute_sync_slot(gpu_context_t *ctx, int slot)
 {
     bc250_hip_state *s = st_of(ctx);
@@ -1117,6 +1206,10 @@ extern "C" int gpu_compute_sync_slot(gpu_context_t *ctx, int slot)
         return -1;
     }
     sl.pending = false;
+    if (settle_levels(s, sl) != 0) {
+        sl.have_data = false;
+        return -1;
+    }
     float ms = 0.f;
     if (sl.timed && HIP_OK(hipEventElapsedTime(&ms, sl.ev[0], sl.ev[NEV - 1]))) {
         s->last_latency_ms = ms;
@@ -1145,6 +1238,50 @@ extern "C" int gpu_compute_sync(gpu_context_t *ctx)
     return gpu_compute_sync_slot(ctx, gpu_compute_submitted_slot(ctx));
 }
 
+// A finished frame whose levels arrived packed: when more groups were
+// packed than copied, the whole levels are fetched instead (the slot's
+// device buffers stay until its next frame is submitted); the next P frame
+// copies twice this one's count (and a margin).
+static int settle_levels(bc250_hip_state *s, Slot &sl)
+{
+    if (!sl.lv_packed || sl.lv_settled) return 0;
+    const int ng = s->nmb * 24, cnt = *sl.lvcount;
+    if (cnt < 0 || cnt > ng) return -1;
+    if (cnt > sl.lv_cap) {
+        if (!HIP_OK(hipMemcpy(sl.levels, sl.dev.levels, (size_t)ng * 32, hipMemcpyDeviceToHost))) return -1;
+        sl.lv_packed = false;
+        sl.built_all = true;
+    }
+    const int want = 2 * cnt + 1024, lo = ng / 64;
+    s->lv_cap_next = want < lo ? lo : want > ng ? ng : want;
+    sl.lv_settled = true;
+    return 0;
+}
+
+// The full levels of a slot whose frame arrived packed: the groups the
+// array held nonzero before are cleared, the packed groups written.
+static void build_levels(bc250_hip_state *s, Slot &sl)
+{
+    if (settle_levels(s, sl) != 0 || !sl.lv_packed || sl.lv_built) return;
+    const size_t ng = (size_t)s->nmb * 24;
+    if (sl.built_all) {
+        memset(sl.levels, 0, ng * 32);
+    } else {
+        for (int i = 0; i < sl.built_n; i++) memset(sl.levels + (size_t)sl.built_idx[i] * 16, 0, 32);
+    }
+    const int cnt = *sl.lvcount;
+    int kept = 0;
+    for (int k = 0; k < cnt; k++) {
+        const uint32_t g = sl.lvidx[k];
+        if (g >= ng) continue;
+        memcpy(sl.levels + (size_t)g * 16, sl.lvdata + (size_t)k * 16, 32);
+        sl.built_idx[kept++] = g;
+    }
+    sl.built_n = kept;
+    sl.built_all = false;
+    sl.lv_built = true;
+}
+
 static int staging(gpu_context_t *ctx, int slot, int which, void **data, size_t *size)
 {
     bc250_hip_state *s = st_of(ctx);
@@ -1158,7 +1295,11 @@ static int staging(gpu_context_t *ctx, int slot, int which, void **data, size_t 
         return -1;
     }
     switch (which) {
-    case 0: *data = sl.levels; if (size) *size = n * 384 * 2; break;
+    case 0:
+        build_levels(s, sl);
+        *data = sl.levels;
+        if (size) *size = n * 384 * 2;
+        break;
     case 1: *data = sl.dc; if (size) *size = n * 24 * 4; break;
     case 2: *data = sl.modes; if (size) *size = n * 4; break;
     case 3: *data = sl.mvs; if (size) *size = n * 16; break;
