@@ -315,7 +315,7 @@ static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
              HIP_OK(hipMalloc(&o.mvs, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&o.mv8, (size_t)nmb * 64)) &&
              HIP_OK(hipMalloc(&o.hmodes, (size_t)nmb * 16)) &&
              HIP_OK(hipMalloc(&o.lvidx, (size_t)nmb * 24 * 4)) && HIP_OK(hipMalloc(&o.lvdata, (size_t)nmb * 24 * 32)) &&
-             HIP_OK(hipMalloc(&o.lvcount, 4));
+             HIP_OK(hipMalloc(&o.lvcount, 4)) && HIP_OK(hipMemset(o.lvcount, 0, 4));
     }
     ok = ok && HIP_OK(hipMalloc(&s->imv16, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->imv8, (size_t)nmb * 64)) &&
          HIP_OK(hipMalloc(&s->sync_i, bc250_intra_work_bytes(wc, hc))) &&
@@ -385,7 +385,8 @@ static void *ensure_xfer(bc250_hip_state *s, size_t bytes)
 
 // Packs a P frame's levels for the readback: each thread one 16-level group
 // (32 bytes); the groups holding a nonzero level go to idx / data at slots
-// taken one atomic add per wave.  *count starts at zero.
+// taken one atomic add per wave.  *count starts at zero: it is cleared at
+// allocation and, on the copy stream, after each readback of it.
 __global__ __launch_bounds__(256) void pack_levels_kernel(const int16_t *__restrict__ levels, int ngroups,
                                                           uint32_t *__restrict__ idx, uint4 *__restrict__ data,
                                                           int *__restrict__ count)
@@ -854,7 +855,6 @@ static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *
     const bool pack = s->params.readback && (what & RB_LEVELS) && (what & RB_PACK);
     if (pack) {
         const int ng = s->nmb * 24;
-        zero_i32_kernel<<<1, 64, 0, s->stream>>>(sl.dev.lvcount, 1);
         pack_levels_kernel<<<(ng + 255) / 256, 256, 0, s->stream>>>(sl.dev.levels, ng, sl.dev.lvidx, sl.dev.lvdata,
                                                                      sl.dev.lvcount);
     }
@@ -874,6 +874,10 @@ static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *
         ok = HIP_OK(hipMemcpyAsync(sl.lvcount, o.lvcount, sizeof(int), k, cs)) &&
              HIP_OK(hipMemcpyAsync(sl.lvidx, o.lvidx, (size_t)sl.lv_cap * 4, k, cs)) &&
              HIP_OK(hipMemcpyAsync(sl.lvdata, o.lvdata, (size_t)sl.lv_cap * 32, k, cs));
+        // the slot's next frame packs from zero (its stages wait for this
+        // stream's last event of this frame)
+        zero_i32_kernel<<<1, 64, 0, cs>>>(o.lvcount, 1);
+        ok = ok && HIP_OK(hipGetLastError());
         what &= ~RB_LEVELS;
     } else if (what & RB_LEVELS) {
         sl.built_all = true;  // the whole array arrives: any group may be nonzero
