@@ -804,9 +804,10 @@ def main():
         "p50_frame_ms": round(float(np.percentile(head["frame_ms"], 50)), 4),
         "p99_frame_ms": round(float(np.percentile(head["frame_ms"], 99)), 4),
         "mpixel_per_s": round(head["fps"] * W * H / 1e6, 1),
-        "readback": "on: every frame's levels, modes and vectors copied to pinned host staging inside the "
-                    "timed region (HEVC has no entropy coder here; the H.264 line with slices ends in the "
-                    "Annex B stream in host memory)",
+        "readback": "on: every frame's levels (P frames packed on the GPU: the 16-level groups holding a nonzero "
+                    "level), modes and vectors copied to pinned host staging inside the timed region; the HEVC "
+                    "writer (host CABAC) turns them into the stream outside it; the H.264 line with slices ends "
+                    "in the Annex B stream in host memory",
         "stage_ms": stage_ms(st),
         "stage_events": "stage_ms and the roofline's kernel time come from HIP events at every stage boundary on "
                         "the encoder's stream, recorded in a second pass over the same timed frames; the first "
