@@ -253,10 +253,12 @@ struct RefineLds8 {
 };
 
 // Refinement of the CU8 at (x0, y0) from its integer vector iv by one wave:
-// returns {mvx, mvy (quarter samples), sad, 0}.
+// returns {mvx, mvy (quarter samples), sad, 0}; with pred non-null, also
+// this lane's luma prediction sample at the returned vector (the value the
+// CU coding's motion compensation gives, from the same window).
 template <typename Pix, int BD>
 BC_DEV uint4 refine_cu(RefineLds8 &L, const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref,
-                       int rpitch, int wc, int hc, int x0, int y0, uint4 iv, int lane)
+                       int rpitch, int wc, int hc, int x0, int y0, uint4 iv, int lane, int *pred = nullptr)
 {
     constexpr int DOWN = BD - 8, MAXV = (1 << BD) - 1, SH = 14 - BD, OFF = 1 << (13 - BD);
     const int dx = (int)iv.x, dy = (int)iv.y;
@@ -276,7 +278,7 @@ BC_DEV uint4 refine_cu(RefineLds8 &L, const uint8_t *__restrict__ cur, int cpitc
         L.h[f][e] = (int16_t)(acc >> DOWN);
     }
     wave_lds_sync();
-    auto diff_of = [&](int mx, int my) {
+    auto pred_of = [&](int mx, int my) {
         const int fx = mx & 3, fy = my & 3;
         const int wx = pc + (mx >> 2) - dx + 4, wy = pr + (my >> 2) - dy + 4;  // integer position in W
         int pv;
@@ -302,8 +304,9 @@ BC_DEV uint4 refine_cu(RefineLds8 &L, const uint8_t *__restrict__ cur, int cpitc
             }
             pv = clampi((p14 + OFF) >> SH, 0, MAXV);
         }
-        return (uint32_t)iabs(cv - pv);
+        return pv;
     };
+    auto diff_of = [&](int mx, int my) { return (uint32_t)iabs(cv - pred_of(mx, my)); };
     const int hx[8] = {2, -2, 0, 0, 2, 2, -2, -2}, hy[8] = {0, 0, 2, -2, 2, -2, 2, -2};
     int bx = dx * 4, by = dy * 4;
     uint32_t best = wave_sum_u32(diff_of(bx, by));
@@ -327,6 +330,7 @@ BC_DEV uint4 refine_cu(RefineLds8 &L, const uint8_t *__restrict__ cur, int cpitc
             }
         }
     }
+    if (pred) *pred = pred_of(bx, by);
     return make_uint4((uint32_t)bx, (uint32_t)by, best, 0u);
 }
 
@@ -357,13 +361,15 @@ struct PcuLds {
 };
 
 // Coding of CU8 wv of CTU ctu (origin (x0, y0)) with vector (mvx, mvy) by one
-// wave.
-template <typename Pix, int BD>
+// wave.  HAVE_PRED: the lane's luma prediction sample is `pred_in` (the
+// refinement's), so no luma window is staged.
+template <typename Pix, int BD, bool HAVE_PRED = false>
 BC_DEV void encode_cu(PcuLds &L, const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv,
                       const uint8_t *__restrict__ ry, const uint8_t *__restrict__ ruv, int rpy, int rpuv,
                       uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy, int opuv, int wc, int hc, int qp,
                       int round_q12, int ctu, int wv, int lane, int x0, int y0, int mvx, int mvy,
-                      int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes)
+                      int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes,
+                      int pred_in = 0)
 {
     constexpr int MAXV = (1 << BD) - 1, QOFF = 6 * (BD - 8);
     constexpr int S1_8 = BD - 6, QSH8 = BD - 2, S1_4 = BD == 8 ? 1 : 3, QSH4 = BD - 3, S2INV = 20 - BD;
@@ -371,8 +377,9 @@ BC_DEV void encode_cu(PcuLds &L, const uint8_t *__restrict__ cy, const uint8_t *
     const int ux0 = x0 >> 1, uy0 = y0 >> 1, wcc = wc >> 1, hcc = hc >> 1;
     const int lx = x0 + (mvx >> 2) - 3, ly = y0 + (mvy >> 2) - 3;
     const int cx = ux0 + (mvx >> 3) - 1, cyy = uy0 + (mvy >> 3) - 1;
-    for (int t = lane; t < 225; t += 64)
-        L.wl[t] = ldpix<Pix>(ry + (size_t)clampi(ly + t / 15, 0, hc - 1) * rpy, clampi(lx + t % 15, 0, wc - 1));
+    if (!HAVE_PRED)
+        for (int t = lane; t < 225; t += 64)
+            L.wl[t] = ldpix<Pix>(ry + (size_t)clampi(ly + t / 15, 0, hc - 1) * rpy, clampi(lx + t % 15, 0, wc - 1));
     for (int t = lane; t < 98; t += 64) {
         const int comp = t >= 49, k = t - comp * 49;
         L.wcr[comp][k] =
@@ -386,7 +393,7 @@ BC_DEV void encode_cu(PcuLds &L, const uint8_t *__restrict__ cy, const uint8_t *
     wave_lds_sync();
 
     // prediction and residual
-    const int pred = mc_sample<false>(L.wl, 15, pc + 3, pr + 3, mvx & 3, mvy & 3, BD);
+    const int pred = HAVE_PRED ? pred_in : mc_sample<false>(L.wl, 15, pc + 3, pr + 3, mvx & 3, mvy & 3, BD);
     const int cpred = cl ? mc_sample<true>(L.wcr[comp], 7, cc_ + 1, cr_ + 1, mvx & 7, mvy & 7, BD) : 0;
     L.a[lane] = cur - pred;
     if (cl) L.ca[lane] = ccur - cpred;
@@ -470,11 +477,12 @@ __global__ __launch_bounds__(256, BC250_RPE_WAVES_PER_EU) void hevc_refine_p_enc
     if (ctu >= nctu) return;
     const int cu = ctu * 4 + wv, mbw = wc / 16;
     const int x0 = (ctu % mbw) * 16 + (wv & 1) * 8, y0 = (ctu / mbw) * 16 + (wv >> 1) * 8;
-    const uint4 v = refine_cu<Pix, BD>(Ls[wv].r, cy, cpy, ry, rpy, wc, hc, x0, y0, imv8[cu], lane);
+    int pred;
+    const uint4 v = refine_cu<Pix, BD>(Ls[wv].r, cy, cpy, ry, rpy, wc, hc, x0, y0, imv8[cu], lane, &pred);
     if (lane == 0) mv8[cu] = v;
     wave_lds_sync();  // the refinement's LDS is the coding's from here on
-    encode_cu<Pix, BD>(Ls[wv].p, cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12, ctu,
-                       wv, lane, x0, y0, (int)v.x, (int)v.y, coef, cbf, modes);
+    encode_cu<Pix, BD, true>(Ls[wv].p, cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp,
+                             round_q12, ctu, wv, lane, x0, y0, (int)v.x, (int)v.y, coef, cbf, modes, pred);
 }
 
 // ---------------------------------------------------------------------------
