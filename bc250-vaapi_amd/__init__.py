@@ -135,6 +135,7 @@ _sig(hip, "bc250_cnn_conv", _I, [_I, _I, _P, _P, _P, _P, _P, _P, _I, _P])
 _sig(hip, "bc250_cnn_pool", _I, [_P, _P, _I, _P])
 _sig(hip, "bc250_cnn_forward", _I, [_P, _P, _P, _P, _I, _P])
 _sig(hip, "bc250_cnn_forward_picture", _I, [_P, _I, _P, _I, _I, _I, _P, _P, _P, _P])
+_sig(hip, "bc250_cnn_forward_propose_picture", _I, [_P, _I, _P, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P])
 _sig(hip, "bc250_cnn_set_forward_variant", _I, [_I])
 _sig(hip, "bc250_cnn_wgrad", _I, [_I, _P, _P, _P, _P, _I, _P])
 _sig(hip, "bc250_cnn_head", _I, [_P, _P, _P, _P, _I, _P])

@@ -413,22 +413,6 @@ __global__ __launch_bounds__(256) void pack_levels_kernel(const int16_t *__restr
     }
 }
 
-// The wavefront kernels' wait-error flags of the frame ([0] intra, [1]
-// deblock; a null source reads as 0), stored straight into the slot's
-// pinned flag words by the compute stream at the end of the frame.
-__global__ void collect_flags_kernel(const int *e0, const int *e1, const int *fb, int *dst)
-{
-    if (threadIdx.x == 0) {
-        const int a = e0 ? __hip_atomic_load(e0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-        const int b = e1 ? __hip_atomic_load(e1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-        const int c = fb ? __hip_atomic_load(fb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : -1;
-        __hip_atomic_store(dst, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __hip_atomic_store(dst + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __hip_atomic_store(dst + 2, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __threadfence_system();
-    }
-}
-
 // Device and pinned host buffers of the CAVLC writer: room for 2560 bytes
 // per MB (the reference's CAVLC allowance) and 4 KB per slice.
 static int ensure_cavlc(bc250_hip_state *s)
@@ -858,10 +842,20 @@ static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *
         pack_levels_kernel<<<(ng + 255) / 256, 256, 0, s->stream>>>(sl.dev.levels, ng, sl.dev.lvidx, sl.dev.lvdata,
                                                                      sl.dev.lvcount);
     }
-    collect_flags_kernel<<<1, 64, 0, s->stream>>>(e0, e1, fb, sl.err);
+    // the slot's flag words: those of the kernels that ran are copied on the
+    // copy stream once the stages are done, the others hold their default
+    // (the slot's previous frame is finished: the host waited for it)
+    sl.err[0] = 0;
+    sl.err[1] = 0;
+    sl.err[2] = -1;
     if (!HIP_OK(hipGetLastError()) || !HIP_OK(hipEventRecord(sl.cdone, s->stream)) ||
         !HIP_OK(hipStreamWaitEvent(s->cstream, sl.cdone, 0)))
         return -1;
+    const int *flag_src[3] = {e0, e1, fb};
+    for (int i = 0; i < 3; i++)
+        if (flag_src[i] && !HIP_OK(hipMemcpyAsync(sl.err + i, flag_src[i], sizeof(int), hipMemcpyDeviceToHost,
+                                                  s->cstream)))
+            return -1;
     if (!s->params.readback) return 0;
     const DevOut &o = sl.dev;
     hipStream_t cs = s->cstream;

@@ -36,6 +36,7 @@
 // order, so training is deterministic.
 #include "bc250_dev.h"
 #include "bc250_learned.h"
+#include "learned_dev.h"
 
 #include <hip/hip_bf16.h>
 
@@ -460,10 +461,20 @@ __device__ __forceinline__ void stage_picture(uint32_t *__restrict__ q, const ui
     }
 }
 
-template <int NW, int R, int SW, int FROM>
+// The proposer run at the end of the tile's launch (PROPOSE): the head's
+// weights, and for a single picture its previous 16x16 records, candidate
+// array and range (a batch takes them from its descriptor).
+struct ProposeArgs {
+    const float *wh;
+    const uint4 *prev16;
+    int2 *cand;
+    int range;
+};
+
+template <int NW, int R, int SW, int FROM, bool PROPOSE>
 __device__ __forceinline__ void cnn_forward_tile(const FusedInput &src, const bc250_me_batch_t *bt,
                                                  const FusedWeights &fw, float *__restrict__ pool, int tile,
-                                                 uint16_t *sa, uint16_t *sb)
+                                                 uint16_t *sa, uint16_t *sb, const ProposeArgs &pa)
 {
     constexpr int NTH = NW * 64;
     const int tid = threadIdx.x;
@@ -502,11 +513,27 @@ __device__ __forceinline__ void cnn_forward_tile(const FusedInput &src, const bc
     float *rows = (float *)sa;
     fused_layer<32, 32, 1, F2P, F2P, R, NW, SW>(sb, nullptr, w3, rows);
     __syncthreads();
+    float *spool = (float *)sb;  // conv3's region is free again
     for (int i = tid; i < 16 * 32; i += NTH) {
         const int cell = i >> 5, c = i & 31, cy = cell >> 2, cx = cell & 3;
         float s = 0.f;
         for (int y = 4 * cy; y < 4 * cy + 4; y++) s += rows[(y * 4 + cx) * 32 + c];
         pool[((size_t)tile * 16 + cell) * 32 + c] = s;
+        if constexpr (PROPOSE) spool[i] = s;
+    }
+    if constexpr (PROPOSE) {
+        __syncthreads();
+        float(*smv)[32] = (float(*)[32])(spool + 16 * 32);
+        float(*sz)[BC250_CNN_NOUT] = (float(*)[BC250_CNN_NOUT])(spool + 2 * 16 * 32);
+        if constexpr (FROM == FROM_BATCH) {
+            const bc250_batch_frame_t &fr = bt->f[batch_frame_of_tile(*bt, tile)];
+            const int mbw = fr.wc / 16, tw = (mbw + 3) / 4, lt = tile - fr.tile_base;
+            propose_tile(spool, pa.wh, (const uint4 *)fr.prev16, mbw, fr.hc / 16, lt % tw, lt / tw, bt->range,
+                         (int2 *)fr.cand, smv, sz, NTH);
+        } else {
+            propose_tile(spool, pa.wh, pa.prev16, src.w / 16, src.h / 16, tile % src.tw, tile / src.tw, pa.range,
+                         pa.cand, smv, sz, NTH);
+        }
     }
 }
 
@@ -530,26 +557,28 @@ __global__ __launch_bounds__(FwdShape<V>::NW * 64) void cnn_forward_fused_kernel
     if (tile >= ntiles) return;
     FusedInput src{};
     src.tiles = in;
-    cnn_forward_tile<FS::NW, FS::R, FS::SW, FROM_TILES>(src, nullptr, fw, pool, tile, sa, sb);
+    cnn_forward_tile<FS::NW, FS::R, FS::SW, FROM_TILES, false>(src, nullptr, fw, pool, tile, sa, sb, ProposeArgs{});
 }
 
-template <int V>
+template <int V, bool PROPOSE>
 __global__ __launch_bounds__(FwdShape<V>::NW * 64) void cnn_forward_picture_kernel(const FusedInput src,
                                                                                   const FusedWeights fw,
-                                                                                  float *__restrict__ pool, int ntiles)
+                                                                                  float *__restrict__ pool, int ntiles,
+                                                                                  const ProposeArgs pa)
 {
     using FS = FwdShape<V>;
     __shared__ __attribute__((aligned(16))) uint16_t sa[FA_ELEMS];
     __shared__ __attribute__((aligned(16))) uint16_t sb[FB_ELEMS];
     const int tile = blockIdx.x;
     if (tile >= ntiles) return;
-    cnn_forward_tile<FS::NW, FS::R, FS::SW, FROM_PICTURE>(src, nullptr, fw, pool, tile, sa, sb);
+    cnn_forward_tile<FS::NW, FS::R, FS::SW, FROM_PICTURE, PROPOSE>(src, nullptr, fw, pool, tile, sa, sb, pa);
 }
 
-template <int V>
+template <int V, bool PROPOSE>
 __global__ __launch_bounds__(FwdShape<V>::NW * 64) void cnn_forward_batch_kernel(const bc250_me_batch_t b,
                                                                                 const FusedWeights fw,
-                                                                                float *__restrict__ pool)
+                                                                                float *__restrict__ pool,
+                                                                                const ProposeArgs pa)
 {
     using FS = FwdShape<V>;
     __shared__ __attribute__((aligned(16))) uint16_t sa[FA_ELEMS];
@@ -557,7 +586,7 @@ __global__ __launch_bounds__(FwdShape<V>::NW * 64) void cnn_forward_batch_kernel
     const int tile = blockIdx.x;
     if (tile >= b.ntile_total) return;
     FusedInput src{};
-    cnn_forward_tile<FS::NW, FS::R, FS::SW, FROM_BATCH>(src, &b, fw, pool, tile, sa, sb);
+    cnn_forward_tile<FS::NW, FS::R, FS::SW, FROM_BATCH, PROPOSE>(src, &b, fw, pool, tile, sa, sb, pa);
 }
 
 // Weight (and bias) gradients of one layer (stride S, input side IN, output
@@ -981,9 +1010,10 @@ static FusedWeights fused_weights(const uint16_t *packed, const float *params)
     return fw;
 }
 
-extern "C" int bc250_cnn_forward_picture(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
-                                         int hc, const uint16_t *packed, const float *params, float *pool,
-                                         void *stream)
+template <bool PROPOSE>
+static int forward_picture(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
+                           const uint16_t *packed, const float *params, float *pool, const ProposeArgs &pa,
+                           void *stream)
 {
     if (!cur || !ref || !packed || !params || !pool || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) ||
         cpitch < wc || rpitch < wc)
@@ -1000,9 +1030,50 @@ extern "C" int bc250_cnn_forward_picture(const uint8_t *cur, int cpitch, const u
     const FusedWeights fw = fused_weights(packed, params);
     hipStream_t st = (hipStream_t)stream;
     switch (g_forward_variant) {
-    case 0: cnn_forward_picture_kernel<0><<<ntiles, FwdShape<0>::NW * 64, 0, st>>>(src, fw, pool, ntiles); break;
-    case 1: cnn_forward_picture_kernel<1><<<ntiles, FwdShape<1>::NW * 64, 0, st>>>(src, fw, pool, ntiles); break;
-    default: cnn_forward_picture_kernel<2><<<ntiles, FwdShape<2>::NW * 64, 0, st>>>(src, fw, pool, ntiles); break;
+    case 0:
+        cnn_forward_picture_kernel<0, PROPOSE><<<ntiles, FwdShape<0>::NW * 64, 0, st>>>(src, fw, pool, ntiles, pa);
+        break;
+    case 1:
+        cnn_forward_picture_kernel<1, PROPOSE><<<ntiles, FwdShape<1>::NW * 64, 0, st>>>(src, fw, pool, ntiles, pa);
+        break;
+    default:
+        cnn_forward_picture_kernel<2, PROPOSE><<<ntiles, FwdShape<2>::NW * 64, 0, st>>>(src, fw, pool, ntiles, pa);
+        break;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int bc250_cnn_forward_picture(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
+                                         int hc, const uint16_t *packed, const float *params, float *pool,
+                                         void *stream)
+{
+    return forward_picture<false>(cur, cpitch, ref, rpitch, wc, hc, packed, params, pool, ProposeArgs{}, stream);
+}
+
+extern "C" int bc250_cnn_forward_propose_picture(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch,
+                                                 int wc, int hc, const uint16_t *packed, const float *params,
+                                                 float *pool, const uint32_t *prev16, int range, int *cand,
+                                                 void *stream)
+{
+    if (!cand || range < 4 || range > 64) return -1;
+    const ProposeArgs pa{params ? params + bc250_cnn_w_offset(4) : nullptr, (const uint4 *)prev16, (int2 *)cand,
+                         range};
+    return forward_picture<true>(cur, cpitch, ref, rpitch, wc, hc, packed, params, pool, pa, stream);
+}
+
+template <bool PROPOSE>
+static int forward_batch(const bc250_me_batch_t *b, const uint16_t *packed, const float *params, float *pool,
+                         void *stream)
+{
+    if (!b || b->n < 1 || b->n > BC250_BATCH_MAX || b->ntile_total <= 0 || !packed || !params || !pool) return -1;
+    const FusedWeights fw = fused_weights(packed, params);
+    const ProposeArgs pa{params + bc250_cnn_w_offset(4), nullptr, nullptr, b->range};
+    hipStream_t st = (hipStream_t)stream;
+    const int g = b->ntile_total;
+    switch (g_forward_variant) {
+    case 0: cnn_forward_batch_kernel<0, PROPOSE><<<g, FwdShape<0>::NW * 64, 0, st>>>(*b, fw, pool, pa); break;
+    case 1: cnn_forward_batch_kernel<1, PROPOSE><<<g, FwdShape<1>::NW * 64, 0, st>>>(*b, fw, pool, pa); break;
+    default: cnn_forward_batch_kernel<2, PROPOSE><<<g, FwdShape<2>::NW * 64, 0, st>>>(*b, fw, pool, pa); break;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -1010,16 +1081,14 @@ extern "C" int bc250_cnn_forward_picture(const uint8_t *cur, int cpitch, const u
 extern "C" int bc250_cnn_forward_batch(const bc250_me_batch_t *b, const uint16_t *packed, const float *params,
                                        float *pool, void *stream)
 {
-    if (!b || b->n < 1 || b->n > BC250_BATCH_MAX || b->ntile_total <= 0 || !packed || !params || !pool) return -1;
-    const FusedWeights fw = fused_weights(packed, params);
-    hipStream_t st = (hipStream_t)stream;
-    const int g = b->ntile_total;
-    switch (g_forward_variant) {
-    case 0: cnn_forward_batch_kernel<0><<<g, FwdShape<0>::NW * 64, 0, st>>>(*b, fw, pool); break;
-    case 1: cnn_forward_batch_kernel<1><<<g, FwdShape<1>::NW * 64, 0, st>>>(*b, fw, pool); break;
-    default: cnn_forward_batch_kernel<2><<<g, FwdShape<2>::NW * 64, 0, st>>>(*b, fw, pool); break;
-    }
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    return forward_batch<false>(b, packed, params, pool, stream);
+}
+
+extern "C" int bc250_cnn_forward_propose_batch(const bc250_me_batch_t *b, const uint16_t *packed,
+                                               const float *params, float *pool, void *stream)
+{
+    if (b && (b->range < 4 || b->range > 64)) return -1;
+    return forward_batch<true>(b, packed, params, pool, stream);
 }
 
 // The fused forward's launch shape (0, 1, 2: see FwdShape); a tunable.

@@ -21,6 +21,7 @@
 #include "bc250_dev.h"
 #include "bc250_learned.h"
 #include "bc250_stages.h"
+#include "learned_dev.h"
 
 namespace {
 
@@ -46,23 +47,6 @@ __global__ void cnn_mvin_kernel(const uint4 *__restrict__ prev16, int mbw, int m
     const int b = blockIdx.x * blockDim.x + threadIdx.x;  // (tile, block)
     if (b >= ntiles * 16) return;
     mvin_block(prev16, mbw, mbh, tw, b / 16, b % 16, mvin + (size_t)b * 32);
-}
-
-// The 4 hypotheses of one block's head outputs zb ordered by confidence
-// (ties: lower index), in whole pels, clamped to the range, into cand[m].
-__device__ __forceinline__ void cand_from_z(const float *zb, int range, int m, int2 *__restrict__ cand)
-{
-    int ord[4] = {0, 1, 2, 3};
-    // stable selection by confidence, highest first
-    for (int a = 0; a < 4; a++)
-        for (int b = a + 1; b < 4; b++)
-            if (zb[8 + ord[b]] > zb[8 + ord[a]]) { const int t = ord[a]; ord[a] = ord[b]; ord[b] = t; }
-    for (int k = 0; k < 4; k++) {
-        const int h = ord[k];
-        const int dx = clampi((int)rintf(zb[2 * h] * 16.f), -range, range);
-        const int dy = clampi((int)rintf(zb[2 * h + 1] * 16.f), -range, range);
-        cand[(size_t)m * 4 + k] = make_int2(dx, dy);
-    }
 }
 
 // Candidates of MB m of a picture whose tiles start at z (tw tiles per row).
@@ -322,13 +306,7 @@ __device__ __forceinline__ void propose_group(const float *__restrict__ wh, int 
     for (int i = t; i < PMB * 16; i += PMB * BC250_CNN_NOUT) {
         const int j = i >> 4, k = i & 15;
         const ProposeSrc s = src_of(g0 + j);
-        if (s.m >= 0) {
-            const int mx = s.m % s.mbw, my = s.m / s.mbw;
-            const int nx = clampi(mx + (k & 3) - 1, 0, s.mbw - 1), ny = clampi(my + (k >> 2) - 1, 0, s.mbh - 1);
-            const uint4 v = s.prev16 ? s.prev16[ny * s.mbw + nx] : make_uint4(0, 0, 0, 0);
-            smv[j][2 * k] = (float)(int)v.x * (1.f / 16.f);
-            smv[j][2 * k + 1] = (float)(int)v.y * (1.f / 16.f);
-        }
+        if (s.m >= 0) neighbour_input(s.prev16, s.mbw, s.mbh, s.m % s.mbw, s.m / s.mbw, k, &smv[j][2 * k]);
     }
     __syncthreads();
     if (t < PMB * BC250_CNN_NOUT) {
@@ -337,20 +315,7 @@ __device__ __forceinline__ void propose_group(const float *__restrict__ wh, int 
         if (s.m >= 0) {
             const int tw = (s.mbw + 3) / 4, mx = s.m % s.mbw, my = s.m / s.mbw;
             const int tile = (my >> 2) * tw + (mx >> 2), jb = (my & 3) * 4 + (mx & 3);
-            const float *row = wh + o * (BC250_CNN_NFEAT + 1);
-            const float4 *pb = (const float4 *)(s.pool + ((size_t)tile * 16 + jb) * 32);
-            float sum = row[BC250_CNN_NFEAT];
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const float4 v = pb[q];
-                sum = fmaf(row[4 * q], v.x * BC250_CNN_POOL_SCALE, sum);
-                sum = fmaf(row[4 * q + 1], v.y * BC250_CNN_POOL_SCALE, sum);
-                sum = fmaf(row[4 * q + 2], v.z * BC250_CNN_POOL_SCALE, sum);
-                sum = fmaf(row[4 * q + 3], v.w * BC250_CNN_POOL_SCALE, sum);
-            }
-#pragma unroll
-            for (int i = 0; i < 32; i++) sum = fmaf(row[32 + i], smv[j][i], sum);
-            sz[j][o] = sum;
+            sz[j][o] = head_output(wh, o, s.pool + ((size_t)tile * 16 + jb) * 32, smv[j]);
         }
     }
     __syncthreads();
@@ -368,18 +333,6 @@ __global__ __launch_bounds__(PMB * BC250_CNN_NOUT) void cnn_propose_kernel(const
     const int nmb = mbw * mbh;
     propose_group(wh, range, blockIdx.x * PMB, [&](int m) {
         return ProposeSrc{pool, prev16, cand, mbw, mbh, m < nmb ? m : -1};
-    });
-}
-
-__global__ __launch_bounds__(PMB * BC250_CNN_NOUT) void cnn_propose_batch_kernel(const bc250_me_batch_t b,
-                                                                                const float *__restrict__ pool,
-                                                                                const float *__restrict__ wh)
-{
-    propose_group(wh, b.range, blockIdx.x * PMB, [&](int g) {
-        if (g >= b.nmb_total) return ProposeSrc{nullptr, nullptr, nullptr, 1, 1, -1};
-        const bc250_batch_frame_t &fr = b.f[batch_frame_of_mb(b, g)];
-        return ProposeSrc{pool + (size_t)fr.tile_base * 16 * 32, (const uint4 *)fr.prev16, (int2 *)fr.cand,
-                          fr.wc / 16, fr.hc / 16, g - fr.mb_base};
     });
 }
 
@@ -451,8 +404,8 @@ extern "C" int bc250_learned_search_picture(const uint8_t *cur, int cpitch, cons
         return -1;
     hipStream_t st = (hipStream_t)stream;
     const int mbw = wc / 16, mbh = hc / 16, nmb = mbw * mbh;
-    int r = bc250_cnn_forward_picture(cur, cpitch, ref, rpitch, wc, hc, packed, params, pool, st);
-    if (r == 0) r = bc250_cnn_propose(pool, params, prev16, mbw, mbh, range, cand, st);
+    int r = bc250_cnn_forward_propose_picture(cur, cpitch, ref, rpitch, wc, hc, packed, params, pool, prev16, range,
+                                              cand, st);
     if (r == 0) {
         cand_search_kernel<<<((nmb + 7) / 8) * 8, 64, 0, st>>>(cur, cpitch, ref, rpitch, wc, hc, mbw, nmb, range,
                                                                lambda8, (const int2 *)cand, ncand, (uint4 *)out16,
@@ -515,10 +468,8 @@ extern "C" int bc250_learned_search_batch(const bc250_me_batch_t *b, const float
         return -1;
     const LearnedWork w = carve_learned(work, work_tiles, work_mbs);
     hipStream_t st = (hipStream_t)stream;
-    int r = bc250_cnn_forward_batch(b, packed, params, w.pool, st);
+    int r = bc250_cnn_forward_propose_batch(b, packed, params, w.pool, st);
     if (r == 0) {
-        cnn_propose_batch_kernel<<<(b->nmb_total + PMB - 1) / PMB, PMB * BC250_CNN_NOUT, 0, st>>>(
-            *b, w.pool, params + bc250_cnn_w_offset(4));
         cand_search_batch_kernel<<<((b->nmb_total + 7) / 8) * 8, 64, 0, st>>>(*b, ncand, w.count);
         fallback_select_batch_kernel<<<(b->nmb_total + 127) / 128, 128, 0, st>>>(*b, alpha_q4, beta, t_high, w.list,
                                                                                  w.count, w.keys);

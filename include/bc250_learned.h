@@ -160,9 +160,9 @@ int bc250_launch_cand_search(const uint8_t *cur, int cpitch, const uint8_t *ref,
 int bc250_fallback_select(const uint32_t *r16, int mbw, int mbh, int alpha_q4, int beta, int t_high, int *list,
                           int *count, void *stream);
 
-/* The learned search of one picture in five launches: the network from the
- * pictures (bc250_cnn_forward_picture), the proposer (bc250_cnn_propose),
- * the candidate search (which also clears *count), the fallback list (which
+/* The learned search of one picture in four steps: the network and the
+ * proposer from the pictures (bc250_cnn_forward_propose_picture), the
+ * candidate search (which also clears *count), the fallback list (which
  * also sets the split search's keys) and the split full search of the
  * listed MBs (bc250_launch_me_fullsearch_list_split).  Work space: pool
  * [ntiles][16][32] float, cand [nmb][4] int2, list [nmb], count, keys
@@ -226,6 +226,16 @@ int bc250_cnn_forward_picture(const uint8_t *cur, int cpitch, const uint8_t *ref
                               const uint16_t *packed, const float *params, float *pool, void *stream);
 int bc250_cnn_forward_batch(const bc250_me_batch_t *b, const uint16_t *packed, const float *params, float *pool,
                             void *stream);
+/* The same launches ending with the proposer: each tile's workgroup also
+ * runs bc250_cnn_propose for its 16 MBs from the cell sums it holds, so cand
+ * [nmb][4] {dx, dy} gets the same candidates bit for bit (prev16: the
+ * previous picture's 16x16 records or NULL; range 4..64).  The batch form
+ * takes prev16 and cand of each picture, and the range, from the batch. */
+int bc250_cnn_forward_propose_picture(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
+                                      const uint16_t *packed, const float *params, float *pool,
+                                      const uint32_t *prev16, int range, int *cand, void *stream);
+int bc250_cnn_forward_propose_batch(const bc250_me_batch_t *b, const uint16_t *packed, const float *params,
+                                    float *pool, void *stream);
 
 /* The plain full search of every MB of the batch (list / count: only the
  * listed batch-wide MBs, count read on the device). */
@@ -237,10 +247,10 @@ int bc250_launch_me_fullsearch_batch_split(const bc250_me_batch_t *b, const int 
                                            uint32_t *keys, int keys_ready, void *stream);
 
 /* The learned search of every picture of the batch, one launch per stage:
- * the network from the pictures (block sums), the proposer, the candidate
- * search, the fallback list (batch-wide MB indices) and the split full
- * search of the listed MBs.  work: bc250_learned_work_bytes(
- * work_tiles, work_mbs) bytes, zeroed once before its first use and laid out
+ * the network and the proposer from the pictures
+ * (bc250_cnn_forward_propose_batch), the candidate search, the fallback list
+ * (batch-wide MB indices) and the split full search of the listed MBs.
+ * work: bc250_learned_work_bytes(work_tiles, work_mbs) bytes, zeroed once before its first use and laid out
  * by those capacities (any batch up to them reuses it); params / packed as
  * for the single-picture path. */
 size_t bc250_learned_work_bytes(int ntiles, int nmb);

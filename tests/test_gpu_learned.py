@@ -330,14 +330,15 @@ def test_split_list_search_equals_full_search(pkg):
 
 
 def test_picture_chain_equals_the_stages(pkg, learned):
-    """The five-launch learned search of one picture gives, bit for bit,
-    what the separate stages give: the network read straight from the
-    pictures (pitches above the width, partial 64x64 tiles at the right and
-    bottom) equals input tiles + the fused forward in every LDS layout; the
+    """The learned search of one picture gives, bit for bit, what the
+    separate stages give: the network read straight from the pictures
+    (pitches above the width, partial 64x64 tiles at the right and bottom)
+    equals input tiles + the fused forward in every launch shape; the
     one-launch proposer equals neighbour inputs -> head -> candidates with
-    and without a previous picture's vectors; the candidate search, fallback
-    list and split search equal the separate launches (records, count and
-    list), on two calls in a row with different inputs."""
+    and without a previous picture's vectors, and so does the proposer run
+    inside the network's launch; the candidate search, fallback list and
+    split search equal the separate launches (records, count and list), on
+    two calls in a row with different inputs."""
     import torch
     say("learned: picture chain against the stages, 208x144")
     rs = np.random.default_rng(41)
@@ -406,6 +407,8 @@ def test_picture_chain_equals_the_stages(pkg, learned):
                                                     P(net.packed), P(prev), *cfg, P(pool_b), P(cand_b),
                                                     P(lists[1]), P(counts[1]), P(keys[1]), P(b16), P(b8), None) == 0
         torch.cuda.synchronize()
+        # the chain's proposer runs inside the network's launch: the same candidates
+        assert torch.equal(cand_a, cand_b), call
         n = int(counts[0].item())
         assert int(counts[1].item()) == n and 0 < n < nmb, (call, n)
         assert sorted(lists[0][:n].tolist()) == sorted(lists[1][:n].tolist())
