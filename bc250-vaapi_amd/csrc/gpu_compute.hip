@@ -832,12 +832,16 @@ static void start_frame(bc250_hip_state *s)
 // first lv_cap_next groups of them; sync_slot fetches all the levels when
 // more groups than that were packed, and the staging getter rebuilds the
 // full levels on the host from the packed ones.
-enum { RB_LEVELS = 1, RB_DC = 2, RB_MODES = 4, RB_MVS = 8, RB_NZ = 16, RB_MV8 = 32, RB_HMODES = 64, RB_PACK = 128 };
+// RB_PACKED: the stages packed the levels themselves (no packing launch).
+enum {
+    RB_LEVELS = 1, RB_DC = 2, RB_MODES = 4, RB_MVS = 8, RB_NZ = 16, RB_MV8 = 32, RB_HMODES = 64, RB_PACK = 128,
+    RB_PACKED = 256
+};
 static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *e1, const int *fb, int what)
 {
     const size_t n = (size_t)s->nmb;
     const bool pack = s->params.readback && (what & RB_LEVELS) && (what & RB_PACK);
-    if (pack) {
+    if (pack && !(what & RB_PACKED)) {
         const int ng = s->nmb * 24;
         pack_levels_kernel<<<(ng + 255) / 256, 256, 0, s->stream>>>(sl.dev.levels, ng, sl.dev.lvidx, sl.dev.lvdata,
                                                                      sl.dev.lvcount);
@@ -935,6 +939,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
     const size_t n = (size_t)s->nmb;
     const int bd = s->bps == 2 ? 10 : 8;
     int r = 0;
+    bool packed = false;  // the coding kernel packed the levels (CTU16 P with readback)
     mark(s, sl.ev[0], st);
     if (!is_intra) {
         // the integer search runs on 8-bit luma: at 10 bits the top 8 bits of each sample
@@ -974,12 +979,23 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
             // P modes: zero, or at CTU32 the log2 CU size of each CU8 at [16 t + z];
             // at CTU16 the coding kernel also writes the coded-luma flags
             const bool c32 = p.hevc_ctu_log2 == 5;
-            r = bc250_hevc_launch_refine_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv,
-                                                  out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp,
-                                                  2 /* inter rounding */, s->ext8 ? s->ext8 : s->imv8, s->mv8,
-                                                  (int16_t *)s->levels,
-                                                  p.hevc_ctu_log2, c32 ? s->hculog2 : nullptr, s->hmodes,
-                                                  c32 ? nullptr : s->hcbf, bd, st);
+            if (!c32 && p.readback) {
+                // CTU16: the coding kernel packs the levels for the readback itself
+                const DevOut &o = sl.dev;
+                r = bc250_hevc_launch_refine_p_encode_pack(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py,
+                                                           ref.puv, out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp,
+                                                           2 /* inter rounding */, s->ext8 ? s->ext8 : s->imv8,
+                                                           s->mv8, (int16_t *)s->levels, s->hmodes, s->hcbf, bd,
+                                                           o.lvidx, (int16_t *)o.lvdata, o.lvcount, st);
+                packed = r == 0;
+            } else {
+                r = bc250_hevc_launch_refine_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv,
+                                                      out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp,
+                                                      2 /* inter rounding */, s->ext8 ? s->ext8 : s->imv8, s->mv8,
+                                                      (int16_t *)s->levels,
+                                                      p.hevc_ctu_log2, c32 ? s->hculog2 : nullptr, s->hmodes,
+                                                      c32 ? nullptr : s->hcbf, bd, st);
+            }
         }
     }
     mark(s, sl.ev[3], st);
@@ -995,7 +1011,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
     mark(s, sl.ev[4], st);
     if (r == 0)
         r = finish_frame(s, sl, is_intra ? s->sync_h + 1 : nullptr, nullptr, s->fb_frame,
-                         RB_LEVELS | RB_HMODES | RB_MV8 | (is_intra ? 0 : RB_PACK));
+                         RB_LEVELS | RB_HMODES | RB_MV8 | (is_intra ? 0 : RB_PACK) | (packed ? RB_PACKED : 0));
     return r;
 }
 

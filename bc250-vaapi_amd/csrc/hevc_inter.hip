@@ -360,6 +360,16 @@ struct PcuLds {
     int ca[32], cb[32];
 };
 
+// Packed levels for the readback (CTU16 P pictures): every 16-level group of
+// the levels array that holds a nonzero level, with its group index (the
+// 24 groups of a CTU16: 16 luma, 4 Cb, 4 Cr), at slots taken with one atomic
+// add per CU; *count starts at zero.  idx null: no packing.
+struct PackOut {
+    uint32_t *idx;
+    int16_t *data;
+    int *count;
+};
+
 // Coding of CU8 wv of CTU ctu (origin (x0, y0)) with vector (mvx, mvy) by one
 // wave.  HAVE_PRED: the lane's luma prediction sample is `pred_in` (the
 // refinement's), so no luma window is staged.
@@ -369,7 +379,7 @@ BC_DEV void encode_cu(PcuLds &L, const uint8_t *__restrict__ cy, const uint8_t *
                       uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy, int opuv, int wc, int hc, int qp,
                       int round_q12, int ctu, int wv, int lane, int x0, int y0, int mvx, int mvy,
                       int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes,
-                      int pred_in = 0)
+                      int pred_in = 0, const PackOut pk = PackOut{nullptr, nullptr, nullptr})
 {
     constexpr int MAXV = (1 << BD) - 1, QOFF = 6 * (BD - 8);
     constexpr int S1_8 = BD - 6, QSH8 = BD - 2, S1_4 = BD == 8 ? 1 : 3, QSH4 = BD - 3, S2INV = 20 - BD;
@@ -417,6 +427,31 @@ BC_DEV void encode_cu(PcuLds &L, const uint8_t *__restrict__ cy, const uint8_t *
     }
     if (modes && lane < 4) modes[(size_t)ctu * 16 + wv * 4 + lane] = 0;
     if (cl) co[256 + comp * 64 + wv * 16 + ck] = (int16_t)clv;
+    if (pk.idx) {
+        // the CU's six groups: luma rows 2i, 2i+1 (lanes 16i..), Cb, Cr
+        const uint64_t ml = __ballot(lv != 0), mc = __ballot(cl && clv != 0);
+        uint32_t gm = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) gm |= ((ml >> (16 * i)) & 0xffffull) ? 1u << i : 0u;
+        gm |= (mc & 0xffffull) ? 16u : 0u;
+        gm |= ((mc >> 16) & 0xffffull) ? 32u : 0u;
+        if (gm) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(pk.count, __popc(gm));
+            base = __shfl(base, 0);
+            const int gi = lane >> 4, j = lane & 15;
+            if ((gm >> gi) & 1u) {
+                const int k = base + __popc(gm & ((1u << gi) - 1u));
+                pk.data[(size_t)k * 16 + j] = (int16_t)lv;
+                if (j == 0) pk.idx[k] = (uint32_t)(ctu * 24 + wv * 4 + gi);
+            }
+            if (cl && ((gm >> (4 + comp)) & 1u)) {
+                const int k = base + __popc(gm & ((1u << (4 + comp)) - 1u));
+                pk.data[(size_t)k * 16 + j] = (int16_t)clv;
+                if (j == 0) pk.idx[k] = (uint32_t)(ctu * 24 + 16 + comp * 4 + wv);
+            }
+        }
+    }
 
     // dequantise, inverse: columns, then rows
     L.a[lane] = dequant(lv, q, QSH8);
@@ -465,7 +500,8 @@ __global__ __launch_bounds__(256, BC250_RPE_WAVES_PER_EU) void hevc_refine_p_enc
     const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, const uint8_t *__restrict__ ry,
     const uint8_t *__restrict__ ruv, int rpy, int rpuv, uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy,
     int opuv, int wc, int hc, int nctu, int qp, int round_q12, const uint4 *__restrict__ imv8,
-    uint4 *__restrict__ mv8, int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes)
+    uint4 *__restrict__ mv8, int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes,
+    const PackOut pk)
 {
     union CuLds {
         RefineLds8 r;
@@ -482,7 +518,7 @@ __global__ __launch_bounds__(256, BC250_RPE_WAVES_PER_EU) void hevc_refine_p_enc
     if (lane == 0) mv8[cu] = v;
     wave_lds_sync();  // the refinement's LDS is the coding's from here on
     encode_cu<Pix, BD, true>(Ls[wv].p, cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp,
-                             round_q12, ctu, wv, lane, x0, y0, (int)v.x, (int)v.y, coef, cbf, modes, pred);
+                             round_q12, ctu, wv, lane, x0, y0, (int)v.x, (int)v.y, coef, cbf, modes, pred, pk);
 }
 
 // ---------------------------------------------------------------------------
@@ -849,17 +885,16 @@ extern "C" int bc250_hevc_launch_p_encode_ctu(const void *cy, const void *cuv, i
                         culog2, modes, bd, st);
 }
 
-extern "C" int bc250_hevc_launch_refine_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
-                                                 const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy,
-                                                 int opuv, int wc, int hc, int qp, int round_q12,
-                                                 const uint32_t *imv8, uint32_t *mv8, int16_t *coef, int ctu_log2,
-                                                 uint8_t *culog2, uint8_t *modes, uint8_t *cbf, int bd, void *stream)
+static int refine_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry, const void *ruv, int rpy,
+                           int rpuv, void *oy, void *ouv, int opy, int opuv, int wc, int hc, int qp, int round_q12,
+                           const uint32_t *imv8, uint32_t *mv8, int16_t *coef, int ctu_log2, uint8_t *culog2,
+                           uint8_t *modes, uint8_t *cbf, int bd, const PackOut &pk, void *stream)
 {
     const int b = bd == 8 ? 1 : 2, mw = wc * b;
     if (!bd_ok(bd) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || round_q12 < 0 ||
         round_q12 > 12 || cpy < mw || cpuv < mw || rpy < mw || rpuv < mw || opy < mw || opuv < mw ||
         ((cpy | cpuv | rpy | rpuv | opy | opuv) % b) || (ctu_log2 != 4 && ctu_log2 != 5) ||
-        (ctu_log2 == 5 && !culog2) || !imv8 || !mv8)
+        (ctu_log2 == 5 && !culog2) || !imv8 || !mv8 || (pk.idx && (ctu_log2 != 4 || !pk.data || !pk.count)))
         return -1;
     const int nctu = (wc / 16) * (hc / 16);
     hipStream_t st = (hipStream_t)stream;
@@ -868,11 +903,11 @@ extern "C" int bc250_hevc_launch_refine_p_encode(const void *cy, const void *cuv
     if (bd == 8)
         hevc_refine_p_encode_kernel<uint8_t, 8><<<grid_xcd(nctu), 256, 0, st>>>(
             C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv, wc, hc, nctu, qp, round_q12, (const uint4 *)imv8,
-            (uint4 *)mv8, coef, cbf, modes);
+            (uint4 *)mv8, coef, cbf, modes, pk);
     else
         hevc_refine_p_encode_kernel<uint16_t, 10><<<grid_xcd(nctu), 256, 0, st>>>(
             C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv, wc, hc, nctu, qp, round_q12, (const uint4 *)imv8,
-            (uint4 *)mv8, coef, cbf, modes);
+            (uint4 *)mv8, coef, cbf, modes, pk);
     if (launch_ok() != 0) return -1;
     const int ncu = nctu * 4;
     if (ctu_log2 == 4) {
@@ -881,4 +916,27 @@ extern "C" int bc250_hevc_launch_refine_p_encode(const void *cy, const void *cuv
     }
     return launch_large(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12, mv8, coef,
                         culog2, modes, bd, st);
+}
+
+extern "C" int bc250_hevc_launch_refine_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
+                                                 const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy,
+                                                 int opuv, int wc, int hc, int qp, int round_q12,
+                                                 const uint32_t *imv8, uint32_t *mv8, int16_t *coef, int ctu_log2,
+                                                 uint8_t *culog2, uint8_t *modes, uint8_t *cbf, int bd, void *stream)
+{
+    return refine_p_encode(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12, imv8, mv8,
+                           coef, ctu_log2, culog2, modes, cbf, bd, PackOut{nullptr, nullptr, nullptr}, stream);
+}
+
+extern "C" int bc250_hevc_launch_refine_p_encode_pack(const void *cy, const void *cuv, int cpy, int cpuv,
+                                                      const void *ry, const void *ruv, int rpy, int rpuv, void *oy,
+                                                      void *ouv, int opy, int opuv, int wc, int hc, int qp,
+                                                      int round_q12, const uint32_t *imv8, uint32_t *mv8,
+                                                      int16_t *coef, uint8_t *modes, uint8_t *cbf, int bd,
+                                                      uint32_t *pack_idx, int16_t *pack_data, int *pack_count,
+                                                      void *stream)
+{
+    if (!pack_idx || !pack_data || !pack_count) return -1;
+    return refine_p_encode(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12, imv8, mv8,
+                           coef, 4, nullptr, modes, cbf, bd, PackOut{pack_idx, pack_data, pack_count}, stream);
 }

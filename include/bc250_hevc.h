@@ -138,6 +138,18 @@ int bc250_hevc_launch_refine_p_encode(const void *cy, const void *cuv, int cpy, 
                                       int16_t *coef, int ctu_log2, uint8_t *culog2, uint8_t *modes, uint8_t *cbf,
                                       int bd, void *stream);
 
+/* bc250_hevc_launch_refine_p_encode at CTU16 that also packs the levels for
+ * a readback: each 16-level group of coef (24 per CTU16: 16 luma, 4 Cb, 4
+ * Cr) holding a nonzero level goes to pack_data[k][16] with its group index
+ * in pack_idx[k], k taken from *pack_count (which must start at zero) by one
+ * atomic add per CU; the order of the groups is not fixed.  No reference
+ * counterpart. */
+int bc250_hevc_launch_refine_p_encode_pack(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
+                                           const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy, int opuv,
+                                           int wc, int hc, int qp, int round_q12, const uint32_t *imv8, uint32_t *mv8,
+                                           int16_t *coef, uint8_t *modes, uint8_t *cbf, int bd, uint32_t *pack_idx,
+                                           int16_t *pack_data, int *pack_count, void *stream);
+
 /* I picture: CTU16 -> 4 CU8 -> 4 PU4 intra (Planar/DC/H/V by SAD, DST),
  * chroma DC per CU (DCT, QpC), slices as CTU-row ranges
  * [floor(rows*s/ns), floor(rows*(s+1)/ns)), QP' = qp + 6 (bd - 8) (the
