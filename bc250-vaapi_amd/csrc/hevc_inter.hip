@@ -31,6 +31,24 @@ BC_DEV Depth depth_of(int bd)
     return d;
 }
 
+// 24-bit multiply(-add)s, always the full-rate v_mul_i32_i24 / v_mad_i32_i24
+// (a plain or __mul24 product may become the quarter-rate v_mul_lo_u32).
+// Every operand here is a sample, a tap or an int16-range intermediate, so
+// the 24-bit products are exact.
+template <int C>
+BC_DEV int mad24c(int v, int acc)
+{
+    int d;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(v), "i"(C), "v"(acc));
+    return d;
+}
+BC_DEV int mad24(int a, int b, int acc)
+{
+    int d;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(acc));
+    return d;
+}
+
 // One n x n forward or inverse pass for the coefficient this lane owns:
 // lane (i, j) -> sat16((sum_k M[i][k] * X[k][j] + rnd) >> s)  (forward rows of
 // M; `transposed` uses M[k][i] instead).  X row-major in LDS.
@@ -72,12 +90,12 @@ BC_DEV int mc_sample(const int *W, int P, int wx, int wy, int fx, int fy, int bd
     if (!fy) {
         int s = 0;
 #pragma unroll
-        for (int k = 0; k < NT; k++) s += __mul24((int)fh[k], W[wy * P + wx + k - BACK]);
+        for (int k = 0; k < NT; k++) s = mad24((int)fh[k], W[wy * P + wx + k - BACK], s);
         p14 = (int16_t)(s >> down);
     } else if (!fx) {
         int s = 0;
 #pragma unroll
-        for (int k = 0; k < NT; k++) s += __mul24((int)fv[k], W[(wy + k - BACK) * P + wx]);
+        for (int k = 0; k < NT; k++) s = mad24((int)fv[k], W[(wy + k - BACK) * P + wx], s);
         p14 = (int16_t)(s >> down);
     } else {
         int s = 0;
@@ -85,8 +103,8 @@ BC_DEV int mc_sample(const int *W, int P, int wx, int wy, int fx, int fy, int bd
         for (int k = 0; k < NT; k++) {
             int t = 0;
 #pragma unroll
-            for (int j = 0; j < NT; j++) t += __mul24((int)fh[j], W[(wy + k - BACK) * P + wx + j - BACK]);
-            s += __mul24((int)fv[k], (int)(int16_t)(t >> down));
+            for (int j = 0; j < NT; j++) t = mad24((int)fh[j], W[(wy + k - BACK) * P + wx + j - BACK], t);
+            s = mad24((int)fv[k], (int)(int16_t)(t >> down), s);
         }
         p14 = (int16_t)(s >> 6);
     }
@@ -249,62 +267,79 @@ __global__ __launch_bounds__(64) void hevc_mc_batch_kernel(const Pix *__restrict
 // in bytes; Pix uint8_t (8-bit) or uint16_t (P010, BD 10).
 struct RefineLds8 {
     int w[16 * 16];      // window samples
-    int h[3][16 * 9];    // fx = 1..3: horizontal sums >> (BD - 8) at columns 3..11
+    int h[3][16 * 16];   // fx = 1..3: horizontal sums >> (BD - 8) at columns 3..11 (pitch 16)
 };
 
-// Refinement of the CU8 at (x0, y0) from its integer vector iv by one wave:
-// returns {mvx, mvy (quarter samples), sad, 0}; with pred non-null, also
-// this lane's luma prediction sample at the returned vector (the value the
-// CU coding's motion compensation gives, from the same window).
+// The luma interpolation sum of fraction F (1..3) over the eight samples
+// v(0..7), taps as immediates (8.5.3.3.3.1; the same sums as kQpel[F]).
+template <int F, typename G>
+BC_DEV int qpel_sum(G v)
+{
+    if constexpr (F == 1)
+        return mad24c<-5>(v(5), mad24c<17>(v(4), mad24c<58>(v(3), mad24c<-10>(v(2), 4 * v(1) - v(0) + v(6)))));
+    else if constexpr (F == 2)
+        return mad24c<40>(v(3) + v(4), mad24c<-11>(v(2) + v(5), 4 * (v(1) + v(6)) - v(0) - v(7)));
+    else
+        return mad24c<-10>(v(5), mad24c<58>(v(4), mad24c<17>(v(3), mad24c<-5>(v(2), v(1) + 4 * v(6) - v(7)))));
+}
+
+// qpel_sum of a fraction f known to be the same in every lane of the wave.
+template <typename G>
+BC_DEV int qpel_sum_u(int f, G v)
+{
+    if (f == 1) return qpel_sum<1>(v);
+    if (f == 2) return qpel_sum<2>(v);
+    return qpel_sum<3>(v);
+}
+
+// Refinement of the CU8 at (x0, y0) from its integer vector iv (the same in
+// every lane) by one wave: returns {mvx, mvy (quarter samples), sad, 0}, the
+// same in every lane; with pred non-null, also this lane's luma prediction
+// sample at the returned vector (the value the CU coding's motion
+// compensation gives, from the same window).
 template <typename Pix, int BD>
 BC_DEV uint4 refine_cu(RefineLds8 &L, const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref,
                        int rpitch, int wc, int hc, int x0, int y0, uint4 iv, int lane, int *pred = nullptr)
 {
     constexpr int DOWN = BD - 8, MAXV = (1 << BD) - 1, SH = 14 - BD, OFF = 1 << (13 - BD);
-    const int dx = (int)iv.x, dy = (int)iv.y;
+    const int dx = __builtin_amdgcn_readfirstlane((int)iv.x), dy = __builtin_amdgcn_readfirstlane((int)iv.y);
     const int ox = x0 + dx - 4, oy = y0 + dy - 4;
     for (int t = lane; t < 256; t += 64)
         L.w[t] = ldpix<Pix>(ref + (size_t)clampi(oy + (t >> 4), 0, hc - 1) * rpitch, clampi(ox + (t & 15), 0, wc - 1));
     const int pr = lane >> 3, pc = lane & 7;
     const int cv = ldpix<Pix>(cur + (size_t)(y0 + pr) * cpitch, x0 + pc);
     wave_lds_sync();
-    // horizontal intermediates: 3 fractions x 16 rows x 9 columns
-    for (int t = lane; t < 3 * 144; t += 64) {
-        const int f = t / 144, e = t - f * 144, row = e / 9, col = e - row * 9 + 3;
-        const int8_t *fh = kQpel[f + 1];
-        int acc = 0;
+    // horizontal intermediates of the three fractions at 16 rows x columns
+    // 3..11: a lane per position, its eight samples read once
+    for (int t = lane; t < 144; t += 64) {
+        const int row = t / 9, col = t - row * 9 + 3;
+        const int *w = &L.w[row * 16 + col - 3];
+        int sm[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) acc += __mul24((int)fh[k], L.w[row * 16 + col - 3 + k]);
-        L.h[f][e] = (int16_t)(acc >> DOWN);
+        for (int k = 0; k < 8; k++) sm[k] = w[k];
+        auto v = [&](int k) { return sm[k]; };
+        const int e = row * 16 + col - 3;
+        L.h[0][e] = (int16_t)(qpel_sum<1>(v) >> DOWN);
+        L.h[1][e] = (int16_t)(qpel_sum<2>(v) >> DOWN);
+        L.h[2][e] = (int16_t)(qpel_sum<3>(v) >> DOWN);
     }
     wave_lds_sync();
+    // (mx, my) is the same in every lane, so each branch below is uniform
     auto pred_of = [&](int mx, int my) {
         const int fx = mx & 3, fy = my & 3;
         const int wx = pc + (mx >> 2) - dx + 4, wy = pr + (my >> 2) - dy + 4;  // integer position in W
-        int pv;
-        if (!fx && !fy) {
-            pv = L.w[wy * 16 + wx];
+        if (!fx && !fy) return L.w[wy * 16 + wx];
+        int p14;
+        if (!fy) {
+            p14 = L.h[fx - 1][wy * 16 + wx - 3];
+        } else if (!fx) {
+            const int *c = &L.w[(wy - 3) * 16 + wx];
+            p14 = (int16_t)(qpel_sum_u(fy, [&](int k) { return c[k * 16]; }) >> DOWN);
         } else {
-            int p14;
-            if (!fy) {
-                p14 = L.h[fx - 1][wy * 9 + wx - 3];
-            } else {
-                const int8_t *fv = kQpel[fy];
-                int acc = 0;
-                if (!fx) {
-#pragma unroll
-                    for (int k = 0; k < 8; k++) acc += __mul24((int)fv[k], L.w[(wy - 3 + k) * 16 + wx]);
-                    p14 = (int16_t)(acc >> DOWN);
-                } else {
-                    const int *H = L.h[fx - 1];
-#pragma unroll
-                    for (int k = 0; k < 8; k++) acc += __mul24((int)fv[k], H[(wy - 3 + k) * 9 + wx - 3]);
-                    p14 = (int16_t)(acc >> 6);
-                }
-            }
-            pv = clampi((p14 + OFF) >> SH, 0, MAXV);
+            const int *c = &L.h[fx - 1][(wy - 3) * 16 + wx - 3];
+            p14 = (int16_t)(qpel_sum_u(fy, [&](int k) { return c[k * 16]; }) >> 6);
         }
-        return pv;
+        return clampi((p14 + OFF) >> SH, 0, MAXV);
     };
     auto diff_of = [&](int mx, int my) { return (uint32_t)iabs(cv - pred_of(mx, my)); };
     const int hx[8] = {2, -2, 0, 0, 2, 2, -2, -2}, hy[8] = {0, 0, 2, -2, 2, -2, 2, -2};
@@ -313,14 +348,19 @@ BC_DEV uint4 refine_cu(RefineLds8 &L, const uint8_t *__restrict__ cur, int cpitc
 #pragma unroll 1
     for (int stage = 1; stage <= 2; stage++) {
         const int cx = bx, cy = by;
-        // the stage's eight differences first, then their sums, then the
-        // choice in candidate order (strictly lower wins)
+        // the stage's eight differences first, two to a word (64 lanes of at
+        // most 1023 sum below 2^16), then their sums, then the choice in
+        // candidate order (strictly lower wins)
         uint32_t d[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++)
-            d[k] = diff_of(cx + (stage == 1 ? hx[k] : hx[k] / 2), cy + (stage == 1 ? hy[k] : hy[k] / 2));
-#pragma unroll
-        for (int k = 0; k < 8; k++) d[k] = wave_sum_u32(d[k]);
+        for (int k = 0; k < 8; k += 2) {
+            const int e = stage == 1 ? 1 : 2;
+            const uint32_t lo = diff_of(cx + hx[k] / e, cy + hy[k] / e);
+            const uint32_t hi = diff_of(cx + hx[k + 1] / e, cy + hy[k + 1] / e);
+            const uint32_t sum = wave_sum_u32(lo | hi << 16);
+            d[k] = sum & 0xffffu;
+            d[k + 1] = sum >> 16;
+        }
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             if (d[k] < best) {
@@ -341,7 +381,7 @@ __global__ __launch_bounds__(256) void hevc_refine8_kernel(const uint8_t *__rest
                                                            uint4 *__restrict__ out)
 {
     __shared__ RefineLds8 Ls[4];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int ctu = xcd_item(blockIdx.x, nctu);
     if (ctu >= nctu) return;
     const int cu = ctu * 4 + wv, mbw = wc / 16;
@@ -476,7 +516,7 @@ __global__ __launch_bounds__(256) void hevc_p_encode_kernel(
     int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes)
 {
     __shared__ PcuLds Ls[4];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int ctu = xcd_item(blockIdx.x, nctu);
     if (ctu >= nctu) return;
     const int mbw = wc / 16;
@@ -508,7 +548,7 @@ __global__ __launch_bounds__(256, BC250_RPE_WAVES_PER_EU) void hevc_refine_p_enc
         PcuLds p;
     };
     __shared__ CuLds Ls[4];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int ctu = xcd_item(blockIdx.x, nctu);
     if (ctu >= nctu) return;
     const int cu = ctu * 4 + wv, mbw = wc / 16;
