@@ -57,6 +57,7 @@ struct DevOut {
     uint32_t *lvidx = nullptr;
     uint4 *lvdata = nullptr;
     int *lvcount = nullptr;
+    unsigned *rbdone = nullptr;  // the readback kernel's finished blocks (0 between launches)
 };
 
 struct Slot {
@@ -71,17 +72,19 @@ struct Slot {
     int slices = 0;              // their number; 0 = no slices for this frame
     hipEvent_t ev[NEV] = {};
     hipEvent_t cdone = nullptr;  // the frame's stages done (compute stream): its copies may start
+    hipEvent_t cearly = nullptr; // the frame's outputs final before its last stages (see finish_frame)
     DevOut dev;
     bool pending = false, recorded = false, have_data = false, was_intra = false;
+    bool mv8_half = false;     // mv8 arrived as {x | y << 16, sad} per CU8 in its first half
+    bool hmodes_zero = false;  // the frame's PU modes are all zero and did not travel
     bool timed = false;          // the stage events ev[0 .. NEV-2] were recorded for this frame
-    // packed levels in host memory (pinned): the copies of DevOut's, the
-    // first lv_cap groups of them; `levels` is rebuilt from them when read
+    // packed levels in host memory (pinned): the copies of DevOut's, every
+    // packed group; `levels` is rebuilt from them when read
     uint32_t *lvidx = nullptr;
     int16_t *lvdata = nullptr;
     int *lvcount = nullptr;
-    int lv_cap = 0;
     bool lv_packed = false;        // this frame's levels arrived packed
-    bool lv_settled = false;       // ... their count was checked against lv_cap
+    bool lv_settled = false;       // ... their count was checked
     bool lv_built = false;         // ... and `levels` holds them again
     uint32_t *built_idx = nullptr; // groups `levels` holds nonzero (host heap)
     int built_n = 0;
@@ -129,7 +132,6 @@ struct bc250_hip_state {
     uint8_t *hculog2 = nullptr;                   // HEVC P: log2 CU size per CU8 (CTU32 CU choice)
     Slot slot[NSLOT];
     int cur = 0, submitted = 0;
-    int lv_cap_next = 0;   // packed level groups the next P frame copies back
     double last_latency_ms = 0.0;
     // stage events on the compute stream (gpu_compute_set_stage_timing; on
     // when BC250_PERF_STATS is set, as the reference's timestamp queries)
@@ -243,7 +245,8 @@ static void free_res(bc250_hip_state *s)
         if (p) hipFree(p);
     for (auto &sl : s->slot) {
         DevOut &o = sl.dev;
-        void *odev[] = {o.levels, o.dc, o.nz, o.modes, o.mvs, o.mv8, o.hmodes, o.lvidx, o.lvdata, o.lvcount};
+        void *odev[] = {o.levels, o.dc, o.nz, o.modes, o.mvs, o.mv8, o.hmodes, o.lvidx, o.lvdata, o.lvcount,
+                        o.rbdone};
         for (void *p : odev)
             if (p) hipFree(p);
         o = DevOut{};
@@ -315,7 +318,8 @@ static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
              HIP_OK(hipMalloc(&o.mvs, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&o.mv8, (size_t)nmb * 64)) &&
              HIP_OK(hipMalloc(&o.hmodes, (size_t)nmb * 16)) &&
              HIP_OK(hipMalloc(&o.lvidx, (size_t)nmb * 24 * 4)) && HIP_OK(hipMalloc(&o.lvdata, (size_t)nmb * 24 * 32)) &&
-             HIP_OK(hipMalloc(&o.lvcount, 4)) && HIP_OK(hipMemset(o.lvcount, 0, 4));
+             HIP_OK(hipMalloc(&o.lvcount, 4)) && HIP_OK(hipMemset(o.lvcount, 0, 4)) &&
+             HIP_OK(hipMalloc(&o.rbdone, 4)) && HIP_OK(hipMemset(o.rbdone, 0, 4));
     }
     ok = ok && HIP_OK(hipMalloc(&s->imv16, (size_t)nmb * 16)) && HIP_OK(hipMalloc(&s->imv8, (size_t)nmb * 64)) &&
          HIP_OK(hipMalloc(&s->sync_i, bc250_intra_work_bytes(wc, hc))) &&
@@ -341,7 +345,6 @@ static int ensure_res(bc250_hip_state *s, int w, int h, int bps)
         sl.built_all = true;
         sl.lv_packed = sl.lv_built = false;
     }
-    s->lv_cap_next = nmb * 24 / 16;
     // wavefront workspaces start zeroed: no hand-off record or epoch holds a
     // value from whatever used the memory before
     ok = ok && HIP_OK(hipMemset(s->sync_i, 0, bc250_intra_work_bytes(wc, hc))) &&
@@ -386,7 +389,7 @@ static void *ensure_xfer(bc250_hip_state *s, size_t bytes)
 // Packs a P frame's levels for the readback: each thread one 16-level group
 // (32 bytes); the groups holding a nonzero level go to idx / data at slots
 // taken one atomic add per wave.  *count starts at zero: it is cleared at
-// allocation and, on the copy stream, after each readback of it.
+// allocation and by each readback of it.
 __global__ __launch_bounds__(256) void pack_levels_kernel(const int16_t *__restrict__ levels, int ngroups,
                                                           uint32_t *__restrict__ idx, uint4 *__restrict__ data,
                                                           int *__restrict__ count)
@@ -410,6 +413,80 @@ __global__ __launch_bounds__(256) void pack_levels_kernel(const int16_t *__restr
         idx[k] = (uint32_t)g;
         data[2 * k] = a;
         data[2 * k + 1] = b;
+    }
+}
+
+// The readback of a finished frame, one launch on the copy stream: the
+// kernel writes the frame's outputs straight into the slot's pinned staging
+// (host memory that the device addresses): the listed arrays whole, the
+// packed levels as many groups as were packed (the count read here, on the
+// device), and the three flag words.  The last block to finish clears the
+// packed count for the slot's next frame.
+constexpr int RB_SPANS = 8, RB_BLOCKS = 32;
+struct RbSpan {
+    const uint32_t *src;
+    uint32_t *dst;
+    uint32_t words;
+};
+struct RbArgs {
+    RbSpan sp[RB_SPANS];
+    int nsp;
+    const uint4 *mv8;        // non-null: n8 records {x, y, sad, 0} go to mv8_dst as {x | y << 16, sad}
+    uint2 *mv8_dst;
+    int n8;
+    const int *flag_src[3];  // null: the flag takes flag_def
+    int flag_def[3];
+    int *flag_dst;
+    const int *lvcount;      // null: no packed levels
+    int maxrec;
+    const uint32_t *lvidx;
+    const uint4 *lvdata;
+    int *lvcount_dst;
+    uint32_t *lvidx_dst;
+    uint4 *lvdata_dst;
+    unsigned *done;
+};
+
+__global__ __launch_bounds__(256) void readback_kernel(const RbArgs a)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
+    const int cnt = a.lvcount ? min(max(*a.lvcount, 0), a.maxrec) : 0;
+    for (int k = 0; k < a.nsp; k++) {
+        const RbSpan sp = a.sp[k];
+        if ((sp.words & 3) == 0) {
+            const uint4 *s4 = (const uint4 *)sp.src;
+            uint4 *d4 = (uint4 *)sp.dst;
+            for (uint32_t i = tid; i < sp.words / 4; i += nth) d4[i] = s4[i];
+        } else {
+            for (uint32_t i = tid; i < sp.words; i += nth) sp.dst[i] = sp.src[i];
+        }
+    }
+    for (int i = tid; i < a.n8; i += nth) {
+        const uint4 v = a.mv8[i];
+        a.mv8_dst[i] = make_uint2((v.x & 0xffffu) | (v.y << 16), v.z);
+    }
+    if (a.lvcount) {
+        for (int i = tid; i < cnt; i += nth) a.lvidx_dst[i] = a.lvidx[i];
+        for (int i = tid; i < 2 * cnt; i += nth) a.lvdata_dst[i] = a.lvdata[i];
+    }
+    if (tid == 0) {
+        for (int i = 0; i < 3; i++) a.flag_dst[i] = a.flag_src[i] ? *a.flag_src[i] : a.flag_def[i];
+        if (a.lvcount) *a.lvcount_dst = cnt;
+    }
+    if (a.lvcount) {
+        // every thread of a block has read the count before the block is
+        // counted, so the last block counted clears it after all reads
+        __shared__ bool last;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();
+            last = atomicAdd(a.done, 1u) == gridDim.x - 1;
+        }
+        __syncthreads();
+        if (last && threadIdx.x == 0) {
+            atomicExch(const_cast<int *>(a.lvcount), 0);
+            atomicExch(a.done, 0u);
+        }
     }
 }
 
@@ -522,13 +599,15 @@ extern "C" int bc250_gpu_init(bc250_gpu_context_t *ctx)
     bool ev_ok = true;
     for (auto &sl : s->slot) {
         for (auto &e : sl.ev) ev_ok = ev_ok && HIP_OK(hipEventCreate(&e));
-        ev_ok = ev_ok && HIP_OK(hipEventCreateWithFlags(&sl.cdone, hipEventDisableTiming));
+        ev_ok = ev_ok && HIP_OK(hipEventCreateWithFlags(&sl.cdone, hipEventDisableTiming)) &&
+                HIP_OK(hipEventCreateWithFlags(&sl.cearly, hipEventDisableTiming));
     }
     if (!ev_ok) {
         for (auto &sl : s->slot) {
             for (auto &e : sl.ev)
                 if (e) hipEventDestroy(e);
             if (sl.cdone) hipEventDestroy(sl.cdone);
+            if (sl.cearly) hipEventDestroy(sl.cearly);
         }
         hipStreamDestroy(s->cstream);
         hipStreamDestroy(s->stream);
@@ -555,6 +634,7 @@ extern "C" void bc250_gpu_destroy(bc250_gpu_context_t *ctx)
         for (auto &e : sl.ev)
             if (e) hipEventDestroy(e);
         if (sl.cdone) hipEventDestroy(sl.cdone);
+        if (sl.cearly) hipEventDestroy(sl.cearly);
     }
     hipStreamDestroy(s->cstream);
     hipStreamDestroy(s->stream);
@@ -825,19 +905,25 @@ static void start_frame(bc250_hip_state *s)
 
 // Ends a frame's stages: the wavefront error flags (e0 intra, e1 deblock;
 // null = the stage did not run) and the learned search's fallback count
-// (fb, null = not used) go to the slot's pinned flag words, and,
-// with readback, the copy stream copies the listed outputs to the slot's
-// pinned staging once the stages are done.
-// RB_PACK (P frames): the levels travel packed (pack_levels_kernel), the
-// first lv_cap_next groups of them; sync_slot fetches all the levels when
-// more groups than that were packed, and the staging getter rebuilds the
-// full levels on the host from the packed ones.
+// (fb, null = not used) go to the slot's pinned flag words, and, with
+// readback, the listed outputs to the slot's pinned staging, by one
+// readback_kernel on the copy stream once the stages are done.
+// RB_PACK (P frames): the levels travel packed (pack_levels_kernel first),
+// and the staging getter rebuilds the full levels on the host from them.
 // RB_PACKED: the stages packed the levels themselves (no packing launch).
+// RB_MV8_HALF: the 8x8 vectors travel as 8 bytes each (quarter-sample
+// components within int16) and the staging getter widens them.
+// RB_HMODES_ZERO: the PU modes are all zero; the getter clears the staging.
+// early: the outputs and flags were final at sl.cearly, recorded by the
+// caller before the frame's last stages (deblocking, which changes only the
+// reconstruction): the readback overlaps those stages, and the copy stream
+// still waits for the whole frame before the frame's last event.
 enum {
     RB_LEVELS = 1, RB_DC = 2, RB_MODES = 4, RB_MVS = 8, RB_NZ = 16, RB_MV8 = 32, RB_HMODES = 64, RB_PACK = 128,
-    RB_PACKED = 256
+    RB_PACKED = 256, RB_MV8_HALF = 512, RB_HMODES_ZERO = 1024
 };
-static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *e1, const int *fb, int what)
+static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *e1, const int *fb, int what,
+                        bool early = false)
 {
     const size_t n = (size_t)s->nmb;
     const bool pack = s->params.readback && (what & RB_LEVELS) && (what & RB_PACK);
@@ -846,48 +932,54 @@ static int finish_frame(bc250_hip_state *s, Slot &sl, const int *e0, const int *
         pack_levels_kernel<<<(ng + 255) / 256, 256, 0, s->stream>>>(sl.dev.levels, ng, sl.dev.lvidx, sl.dev.lvdata,
                                                                      sl.dev.lvcount);
     }
-    // the slot's flag words: those of the kernels that ran are copied on the
-    // copy stream once the stages are done, the others hold their default
-    // (the slot's previous frame is finished: the host waited for it)
-    sl.err[0] = 0;
-    sl.err[1] = 0;
-    sl.err[2] = -1;
     if (!HIP_OK(hipGetLastError()) || !HIP_OK(hipEventRecord(sl.cdone, s->stream)) ||
-        !HIP_OK(hipStreamWaitEvent(s->cstream, sl.cdone, 0)))
+        !HIP_OK(hipStreamWaitEvent(s->cstream, early ? sl.cearly : sl.cdone, 0)))
         return -1;
-    const int *flag_src[3] = {e0, e1, fb};
-    for (int i = 0; i < 3; i++)
-        if (flag_src[i] && !HIP_OK(hipMemcpyAsync(sl.err + i, flag_src[i], sizeof(int), hipMemcpyDeviceToHost,
-                                                  s->cstream)))
-            return -1;
-    if (!s->params.readback) return 0;
     const DevOut &o = sl.dev;
-    hipStream_t cs = s->cstream;
-    const hipMemcpyKind k = hipMemcpyDeviceToHost;
-    bool ok = true;
-    sl.lv_packed = pack;
-    sl.lv_settled = sl.lv_built = false;
-    if (pack) {
-        sl.lv_cap = s->lv_cap_next;
-        ok = HIP_OK(hipMemcpyAsync(sl.lvcount, o.lvcount, sizeof(int), k, cs)) &&
-             HIP_OK(hipMemcpyAsync(sl.lvidx, o.lvidx, (size_t)sl.lv_cap * 4, k, cs)) &&
-             HIP_OK(hipMemcpyAsync(sl.lvdata, o.lvdata, (size_t)sl.lv_cap * 32, k, cs));
-        // the slot's next frame packs from zero (its stages wait for this
-        // stream's last event of this frame)
-        zero_i32_kernel<<<1, 64, 0, cs>>>(o.lvcount, 1);
-        ok = ok && HIP_OK(hipGetLastError());
-        what &= ~RB_LEVELS;
-    } else if (what & RB_LEVELS) {
-        sl.built_all = true;  // the whole array arrives: any group may be nonzero
+    RbArgs a{};
+    a.flag_src[0] = e0;
+    a.flag_src[1] = e1;
+    a.flag_src[2] = fb;
+    a.flag_def[2] = -1;
+    a.flag_dst = sl.err;
+    a.done = o.rbdone;
+    if (s->params.readback) {
+        sl.lv_packed = pack;
+        sl.lv_settled = sl.lv_built = false;
+        if (pack) {
+            a.lvcount = o.lvcount;
+            a.maxrec = s->nmb * 24;
+            a.lvidx = o.lvidx;
+            a.lvdata = o.lvdata;
+            a.lvcount_dst = sl.lvcount;
+            a.lvidx_dst = sl.lvidx;
+            a.lvdata_dst = (uint4 *)sl.lvdata;
+            what &= ~RB_LEVELS;
+        } else if (what & RB_LEVELS) {
+            sl.built_all = true;  // the whole array arrives: any group may be nonzero
+        }
+        auto add = [&](const void *src, void *dst, size_t bytes) {
+            a.sp[a.nsp++] = RbSpan{(const uint32_t *)src, (uint32_t *)dst, (uint32_t)(bytes / 4)};
+        };
+        if (what & RB_LEVELS) add(o.levels, sl.levels, n * 384 * 2);
+        if (what & RB_DC) add(o.dc, sl.dc, n * 24 * 4);
+        if (what & RB_MODES) add(o.modes, sl.modes, n * 4);
+        if (what & RB_MVS) add(o.mvs, sl.mvs, n * 16);
+        if (what & RB_NZ) add(o.nz, sl.nz, n * 24 * 4);
+        sl.mv8_half = (what & RB_MV8) && (what & RB_MV8_HALF);
+        sl.hmodes_zero = (what & RB_HMODES) && (what & RB_HMODES_ZERO);
+        if (sl.mv8_half) {
+            a.mv8 = (const uint4 *)o.mv8;
+            a.mv8_dst = (uint2 *)sl.mv8;
+            a.n8 = (int)n * 4;
+        } else if (what & RB_MV8) {
+            add(o.mv8, sl.mv8, n * 64);
+        }
+        if ((what & RB_HMODES) && !sl.hmodes_zero) add(o.hmodes, sl.hmodes, n * 16);
     }
-    if (what & RB_LEVELS) ok = ok && HIP_OK(hipMemcpyAsync(sl.levels, o.levels, n * 384 * 2, k, cs));
-    if (what & RB_DC) ok = ok && HIP_OK(hipMemcpyAsync(sl.dc, o.dc, n * 24 * 4, k, cs));
-    if (what & RB_MODES) ok = ok && HIP_OK(hipMemcpyAsync(sl.modes, o.modes, n * 4, k, cs));
-    if (what & RB_MVS) ok = ok && HIP_OK(hipMemcpyAsync(sl.mvs, o.mvs, n * 16, k, cs));
-    if (what & RB_NZ) ok = ok && HIP_OK(hipMemcpyAsync(sl.nz, o.nz, n * 24 * 4, k, cs));
-    if (what & RB_MV8) ok = ok && HIP_OK(hipMemcpyAsync(sl.mv8, o.mv8, n * 64, k, cs));
-    if (what & RB_HMODES) ok = ok && HIP_OK(hipMemcpyAsync(sl.hmodes, o.hmodes, n * 16, k, cs));
-    return ok ? 0 : -1;
+    readback_kernel<<<RB_BLOCKS, 256, 0, s->cstream>>>(a);
+    if (early && !HIP_OK(hipStreamWaitEvent(s->cstream, sl.cdone, 0))) return -1;
+    return HIP_OK(hipGetLastError()) ? 0 : -1;
 }
 
 static int enqueue_me(bc250_hip_state *s, const gpu_image_t &rt, const Plane2 &ref, int me_mode, hipEvent_t mid)
@@ -987,7 +1079,7 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
                                                            2 /* inter rounding */, s->ext8 ? s->ext8 : s->imv8,
                                                            s->mv8, (int16_t *)s->levels, s->hmodes, s->hcbf, bd,
                                                            o.lvidx, (int16_t *)o.lvdata, o.lvcount, st);
-                packed = r == 0;
+                packed = r == 0 && HIP_OK(hipEventRecord(sl.cearly, st));
             } else {
                 r = bc250_hevc_launch_refine_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv,
                                                       out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp,
@@ -1011,7 +1103,8 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
     mark(s, sl.ev[4], st);
     if (r == 0)
         r = finish_frame(s, sl, is_intra ? s->sync_h + 1 : nullptr, nullptr, s->fb_frame,
-                         RB_LEVELS | RB_HMODES | RB_MV8 | (is_intra ? 0 : RB_PACK) | (packed ? RB_PACKED : 0));
+                         RB_LEVELS | RB_HMODES | RB_MV8 | (is_intra ? 0 : RB_PACK) |
+                             (packed ? RB_PACKED | RB_HMODES_ZERO : 0) | RB_MV8_HALF);
     return r;
 }
 
@@ -1252,22 +1345,12 @@ extern "C" int gpu_compute_sync(gpu_context_t *ctx)
     return gpu_compute_sync_slot(ctx, gpu_compute_submitted_slot(ctx));
 }
 
-// A finished frame whose levels arrived packed: when more groups were
-// packed than copied, the whole levels are fetched instead (the slot's
-// device buffers stay until its next frame is submitted); the next P frame
-// copies twice this one's count (and a margin).
+// A finished frame whose levels arrived packed: its count is checked.
 static int settle_levels(bc250_hip_state *s, Slot &sl)
 {
     if (!sl.lv_packed || sl.lv_settled) return 0;
-    const int ng = s->nmb * 24, cnt = *sl.lvcount;
-    if (cnt < 0 || cnt > ng) return -1;
-    if (cnt > sl.lv_cap) {
-        if (!HIP_OK(hipMemcpy(sl.levels, sl.dev.levels, (size_t)ng * 32, hipMemcpyDeviceToHost))) return -1;
-        sl.lv_packed = false;
-        sl.built_all = true;
-    }
-    const int want = 2 * cnt + 1024, lo = ng / 64;
-    s->lv_cap_next = want < lo ? lo : want > ng ? ng : want;
+    const int cnt = *sl.lvcount;
+    if (cnt < 0 || cnt > s->nmb * 24) return -1;
     sl.lv_settled = true;
     return 0;
 }
@@ -1318,8 +1401,28 @@ static int staging(gpu_context_t *ctx, int slot, int which, void **data, size_t 
     case 2: *data = sl.modes; if (size) *size = n * 4; break;
     case 3: *data = sl.mvs; if (size) *size = n * 16; break;
     case 4: *data = sl.nz; if (size) *size = n * 24 * 4; break;
-    case 5: *data = sl.mv8; if (size) *size = n * 64; break;
-    default: *data = sl.hmodes; if (size) *size = n * 16; break;
+    case 5:
+        if (sl.mv8_half) {
+            // widen in place from the last record down: record i's 16 bytes
+            // cover half records 2i and 2i + 1, read already (or now, at i = 0)
+            const uint2 *h = (const uint2 *)sl.mv8;
+            for (size_t i = n * 4; i-- > 0;) {
+                const uint2 v = h[i];
+                sl.mv8[i] = gpu_mv_t{(int16_t)(v.x & 0xffffu), (int16_t)(v.x >> 16), v.y, 0u};
+            }
+            sl.mv8_half = false;
+        }
+        *data = sl.mv8;
+        if (size) *size = n * 64;
+        break;
+    default:
+        if (sl.hmodes_zero) {
+            memset(sl.hmodes, 0, n * 16);
+            sl.hmodes_zero = false;
+        }
+        *data = sl.hmodes;
+        if (size) *size = n * 16;
+        break;
     }
     return 0;
 }
