@@ -535,8 +535,11 @@ __global__ __launch_bounds__(256) void hevc_p_encode_kernel(
 #ifndef BC250_RPE_WAVES_PER_EU
 #define BC250_RPE_WAVES_PER_EU 6
 #endif
+// One wave per workgroup (a CU8 each, the CTU's four on one XCD): nothing is
+// shared between a CTU's CU8s, and single-wave workgroups measured faster
+// than four-wave ones.
 template <typename Pix, int BD>
-__global__ __launch_bounds__(256, BC250_RPE_WAVES_PER_EU) void hevc_refine_p_encode_kernel(
+__global__ __launch_bounds__(64, BC250_RPE_WAVES_PER_EU) void hevc_refine_p_encode_kernel(
     const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cuv, int cpy, int cpuv, const uint8_t *__restrict__ ry,
     const uint8_t *__restrict__ ruv, int rpy, int rpuv, uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy,
     int opuv, int wc, int hc, int nctu, int qp, int round_q12, const uint4 *__restrict__ imv8,
@@ -547,17 +550,17 @@ __global__ __launch_bounds__(256, BC250_RPE_WAVES_PER_EU) void hevc_refine_p_enc
         RefineLds8 r;
         PcuLds p;
     };
-    __shared__ CuLds Ls[4];
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int ctu = xcd_item(blockIdx.x, nctu);
-    if (ctu >= nctu) return;
-    const int cu = ctu * 4 + wv, mbw = wc / 16;
+    __shared__ CuLds Ls;
+    const int lane = threadIdx.x;
+    const int cu = xcd_item(blockIdx.x, 4 * nctu);
+    if (cu >= 4 * nctu) return;
+    const int ctu = cu >> 2, wv = cu & 3, mbw = wc / 16;
     const int x0 = (ctu % mbw) * 16 + (wv & 1) * 8, y0 = (ctu / mbw) * 16 + (wv >> 1) * 8;
     int pred;
-    const uint4 v = refine_cu<Pix, BD>(Ls[wv].r, cy, cpy, ry, rpy, wc, hc, x0, y0, imv8[cu], lane, &pred);
+    const uint4 v = refine_cu<Pix, BD>(Ls.r, cy, cpy, ry, rpy, wc, hc, x0, y0, imv8[cu], lane, &pred);
     if (lane == 0) mv8[cu] = v;
     wave_lds_sync();  // the refinement's LDS is the coding's from here on
-    encode_cu<Pix, BD, true>(Ls[wv].p, cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp,
+    encode_cu<Pix, BD, true>(Ls.p, cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp,
                              round_q12, ctu, wv, lane, x0, y0, (int)v.x, (int)v.y, coef, cbf, modes, pred, pk);
 }
 
@@ -941,11 +944,11 @@ static int refine_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, c
     auto *C = (const uint8_t *)cy, *CU = (const uint8_t *)cuv, *R = (const uint8_t *)ry, *RU = (const uint8_t *)ruv;
     auto *O = (uint8_t *)oy, *OU = (uint8_t *)ouv;
     if (bd == 8)
-        hevc_refine_p_encode_kernel<uint8_t, 8><<<grid_xcd(nctu), 256, 0, st>>>(
+        hevc_refine_p_encode_kernel<uint8_t, 8><<<grid_xcd(4 * nctu), 64, 0, st>>>(
             C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv, wc, hc, nctu, qp, round_q12, (const uint4 *)imv8,
             (uint4 *)mv8, coef, cbf, modes, pk);
     else
-        hevc_refine_p_encode_kernel<uint16_t, 10><<<grid_xcd(nctu), 256, 0, st>>>(
+        hevc_refine_p_encode_kernel<uint16_t, 10><<<grid_xcd(4 * nctu), 64, 0, st>>>(
             C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv, wc, hc, nctu, qp, round_q12, (const uint4 *)imv8,
             (uint4 *)mv8, coef, cbf, modes, pk);
     if (launch_ok() != 0) return -1;
