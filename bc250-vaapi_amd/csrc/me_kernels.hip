@@ -553,6 +553,84 @@ __device__ __forceinline__ int sample_at(const RefineLds &L, int xi, int yi, int
 }
 }  // namespace
 
+// One block of the refinement (BW = 16: the MB, 8: quadrant job - 1), its
+// block size a constant so that the sample loops index by shifts.
+template <int BW>
+__device__ __forceinline__ void refine_job(RefineLds &L, const uint8_t *__restrict__ ref, int rpitch, int wc,
+                                           int hc, int mb, int X0, int Y0, int job, uint4 iv,
+                                           uint4 *__restrict__ mv16, uint4 *__restrict__ mv8, int lane)
+{
+    const int HO[8][2] = {{2, 0}, {-2, 0}, {0, 2}, {0, -2}, {2, 2}, {2, -2}, {-2, 2}, {-2, -2}};
+    const int QO[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+    constexpr int LB = BW == 16 ? 4 : 3;
+    const int bx0 = job == 0 ? 0 : ((job - 1) & 1) * 8, by0 = job == 0 ? 0 : ((job - 1) >> 1) * 8;
+    const int dx = __builtin_amdgcn_readfirstlane((int)iv.x), dy = __builtin_amdgcn_readfirstlane((int)iv.y);
+    constexpr int ww = BW + 8, np = BW + 2;
+    wave_lds_sync();
+    // window with replicated edges
+    for (int i = lane; i < ww * ww; i += 64) {
+        const int yy = i / ww, xx = i - yy * ww;
+        const int gx = clampi(X0 + bx0 + dx - 4 + xx, 0, wc - 1);
+        const int gy = clampi(Y0 + by0 + dy - 4 + yy, 0, hc - 1);
+        L.win[yy * RW + xx] = ref[(size_t)gy * rpitch + gx];
+    }
+    wave_lds_sync();
+    // unrounded horizontal intermediates at rows 1..ww-1, cols 3..3+np-1
+    for (int i = lane; i < (ww - 1) * np; i += 64) {
+        const int r = i / np, k = i - r * np;
+        const uint8_t *p = &L.win[(1 + r) * RW + 3 + k];
+        L.hr[r * RP + k] = (int16_t)tap6(p[-2], p[-1], p[0], p[1], p[2], p[3]);
+    }
+    wave_lds_sync();
+    for (int i = lane; i < np * np; i += 64) {
+        const int r = i / np, k = i - r * np;
+        // b at window (3+k, 3+r) is hr row (3+r)-1
+        L.B[r * RP + k] = (uint8_t)clip255((L.hr[(2 + r) * RP + k] + 16) >> 5);
+        const uint8_t *q = &L.win[(3 + r) * RW + 3 + k];
+        L.Hh[r * RP + k] = (uint8_t)clip255((tap6(q[-2 * RW], q[-RW], q[0], q[RW], q[2 * RW], q[3 * RW]) + 16) >> 5);
+        const int16_t *t = &L.hr[r * RP + k];  // rows (3+r)-2 .. (3+r)+3 are hr rows r .. r+5
+        const int j1 = tap6(t[0], t[RP], t[2 * RP], t[3 * RP], t[4 * RP], t[5 * RP]);
+        L.J[r * RP + k] = (uint8_t)clip255((j1 + 512) >> 10);
+    }
+    wave_lds_sync();
+    // (ox, oy) is the same in every lane: sample_at's switch is uniform
+    auto sad_at = [&](int ox, int oy) -> uint32_t {
+        uint32_t s = 0;
+        const int fx = ox & 3, fy = oy & 3;
+#pragma unroll
+        for (int p = lane; p < BW * BW; p += 64) {
+            const int px = p & (BW - 1), pyy = p >> LB;
+            const int xi = 4 + px + (ox >> 2), yi = 4 + pyy + (oy >> 2);
+            s += (uint32_t)iabs((int)L.cb[(by0 + pyy) * 16 + bx0 + px] - sample_at(L, xi, yi, fx, fy));
+        }
+        return s;
+    };
+    uint32_t best = wave_sum_u32(sad_at(0, 0));
+    uint32_t hs[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) hs[c] = sad_at(HO[c][0], HO[c][1]);
+#pragma unroll
+    for (int c = 0; c < 8; c++) hs[c] = wave_sum_u32(hs[c]);
+    int hx = 0, hy = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+        if (hs[c] < best) { best = hs[c]; hx = HO[c][0]; hy = HO[c][1]; }
+    uint32_t qs[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) qs[c] = sad_at(hx + QO[c][0], hy + QO[c][1]);
+#pragma unroll
+    for (int c = 0; c < 8; c++) qs[c] = wave_sum_u32(qs[c]);
+    int qx = 0, qy = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+        if (qs[c] < best) { best = qs[c]; qx = QO[c][0]; qy = QO[c][1]; }
+    if (lane == 0) {
+        uint4 o = make_uint4((uint32_t)(4 * dx + hx + qx), (uint32_t)(4 * dy + hy + qy), best, 0u);
+        if (job == 0) mv16[mb] = o;
+        else mv8[4 * mb + job - 1] = o;
+    }
+}
+
 __global__ __launch_bounds__(64) void me_refine_kernel(
     const uint8_t *__restrict__ cur, int cpitch, const uint8_t *__restrict__ ref, int rpitch,
     int wc, int hc, int mbw, int nmb, const uint4 *__restrict__ in16, const uint4 *__restrict__ in8,
@@ -564,84 +642,14 @@ __global__ __launch_bounds__(64) void me_refine_kernel(
     const int lane = threadIdx.x;
     const int mbx = mb % mbw, mby = mb / mbw;
     const int X0 = mbx * 16, Y0 = mby * 16;
-    const int HO[8][2] = {{2, 0}, {-2, 0}, {0, 2}, {0, -2}, {2, 2}, {2, -2}, {-2, 2}, {-2, -2}};
-    const int QO[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
-
     for (int i = lane; i < 64; i += 64) {
         const uint32_t v = *(const uint32_t *)(cur + (size_t)(Y0 + (i >> 2)) * cpitch + X0 + (i & 3) * 4);
         *(uint32_t *)&L.cb[(i >> 2) * 16 + (i & 3) * 4] = v;
     }
-
-    const int njobs = do8 ? 5 : 1;
-    for (int job = 0; job < njobs; job++) {
-        const int bw = job == 0 ? 16 : 8;
-        const int bx0 = job == 0 ? 0 : ((job - 1) & 1) * 8, by0 = job == 0 ? 0 : ((job - 1) >> 1) * 8;
-        const uint4 iv = job == 0 ? in16[mb] : in8[4 * mb + job - 1];
-        const int dx = (int)iv.x, dy = (int)iv.y;
-        const int ww = bw + 8, np = bw + 2;
-        wave_lds_sync();
-        // window with replicated edges
-        for (int i = lane; i < ww * ww; i += 64) {
-            const int yy = i / ww, xx = i - yy * ww;
-            const int gx = clampi(X0 + bx0 + dx - 4 + xx, 0, wc - 1);
-            const int gy = clampi(Y0 + by0 + dy - 4 + yy, 0, hc - 1);
-            L.win[yy * RW + xx] = ref[(size_t)gy * rpitch + gx];
-        }
-        wave_lds_sync();
-        // unrounded horizontal intermediates at rows 1..ww-1, cols 3..3+np-1
-        for (int i = lane; i < (ww - 1) * np; i += 64) {
-            const int r = i / np, k = i - r * np;
-            const uint8_t *p = &L.win[(1 + r) * RW + 3 + k];
-            L.hr[r * RP + k] = (int16_t)tap6(p[-2], p[-1], p[0], p[1], p[2], p[3]);
-        }
-        wave_lds_sync();
-        for (int i = lane; i < np * np; i += 64) {
-            const int r = i / np, k = i - r * np;
-            // b at window (3+k, 3+r) is hr row (3+r)-1
-            L.B[r * RP + k] = (uint8_t)clip255((L.hr[(2 + r) * RP + k] + 16) >> 5);
-            const uint8_t *q = &L.win[(3 + r) * RW + 3 + k];
-            L.Hh[r * RP + k] = (uint8_t)clip255((tap6(q[-2 * RW], q[-RW], q[0], q[RW], q[2 * RW], q[3 * RW]) + 16) >> 5);
-            const int16_t *t = &L.hr[r * RP + k];  // rows (3+r)-2 .. (3+r)+3 are hr rows r .. r+5
-            const int j1 = tap6(t[0], t[RP], t[2 * RP], t[3 * RP], t[4 * RP], t[5 * RP]);
-            L.J[r * RP + k] = (uint8_t)clip255((j1 + 512) >> 10);
-        }
-        wave_lds_sync();
-        const int npx = bw * bw;
-        auto sad_at = [&](int ox, int oy) -> uint32_t {
-            uint32_t s = 0;
-            const int fx = ox & 3, fy = oy & 3;
-            for (int p = lane; p < npx; p += 64) {
-                const int px = p % bw, pyy = p / bw;
-                const int xi = 4 + px + (ox >> 2), yi = 4 + pyy + (oy >> 2);
-                s += (uint32_t)iabs((int)L.cb[(by0 + pyy) * 16 + bx0 + px] - sample_at(L, xi, yi, fx, fy));
-            }
-            return s;
-        };
-        uint32_t best = wave_sum_u32(sad_at(0, 0));
-        uint32_t hs[8];
-#pragma unroll
-        for (int c = 0; c < 8; c++) hs[c] = sad_at(HO[c][0], HO[c][1]);
-#pragma unroll
-        for (int c = 0; c < 8; c++) hs[c] = wave_sum_u32(hs[c]);
-        int hx = 0, hy = 0;
-#pragma unroll
-        for (int c = 0; c < 8; c++)
-            if (hs[c] < best) { best = hs[c]; hx = HO[c][0]; hy = HO[c][1]; }
-        uint32_t qs[8];
-#pragma unroll
-        for (int c = 0; c < 8; c++) qs[c] = sad_at(hx + QO[c][0], hy + QO[c][1]);
-#pragma unroll
-        for (int c = 0; c < 8; c++) qs[c] = wave_sum_u32(qs[c]);
-        int qx = 0, qy = 0;
-#pragma unroll
-        for (int c = 0; c < 8; c++)
-            if (qs[c] < best) { best = qs[c]; qx = QO[c][0]; qy = QO[c][1]; }
-        if (lane == 0) {
-            uint4 o = make_uint4((uint32_t)(4 * dx + hx + qx), (uint32_t)(4 * dy + hy + qy), best, 0u);
-            if (job == 0) mv16[mb] = o;
-            else mv8[4 * mb + job - 1] = o;
-        }
-    }
+    refine_job<16>(L, ref, rpitch, wc, hc, mb, X0, Y0, 0, in16[mb], mv16, mv8, lane);
+    if (do8)
+        for (int job = 1; job < 5; job++)
+            refine_job<8>(L, ref, rpitch, wc, hc, mb, X0, Y0, job, in8[4 * mb + job - 1], mv16, mv8, lane);
 }
 
 // Host-side launchers (declared in bc250_stages.h).
