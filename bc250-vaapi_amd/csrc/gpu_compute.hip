@@ -154,6 +154,7 @@ struct bc250_hip_state {
     int stage_frames = 0, stage_p_frames = 0;
     double me_int_sum_p = 0;
     Proposer prop;
+    const uint32_t *fs_keys = nullptr;  // this picture's learned search left its full searches as keys
     // integer search results written by someone else (a batched launch): P
     // pictures then skip their own integer search (gpu_compute_set_external_int_me)
     const uint32_t *ext16 = nullptr, *ext8 = nullptr;
@@ -536,7 +537,7 @@ static int ensure_prop(bc250_hip_state *s)
 // out16 / out8 in the full search's format, which keeps the 16x16 vectors
 // as the next picture's neighbour input.
 static int enqueue_learned(bc250_hip_state *s, const uint8_t *c8, int cp, const uint8_t *r8, int rp, uint32_t *out16,
-                           uint32_t *out8, hipStream_t st)
+                           uint32_t *out8, hipStream_t st, bool keys = false)
 {
     Proposer &q = s->prop;
     if (ensure_prop(s) != 0) return -1;
@@ -544,10 +545,13 @@ static int enqueue_learned(bc250_hip_state *s, const uint8_t *c8, int cp, const 
     // out16 still holds the previous picture's records when this picture's
     // proposer reads them: the candidate search rewrites it only after that
     // launch on the same stream
-    int r = bc250_learned_search_picture(c8, cp, r8, rp, s->wc, s->hc, p.search_range, p.lambda8, q.params, q.packed,
-                                         q.has_prev ? out16 : nullptr, q.cfg.ncand, q.cfg.alpha_q4, q.cfg.beta,
-                                         q.cfg.t_high, q.pool, q.cand, q.list, q.count, q.keys, out16, out8, st);
+    // keys: the CTU16 HEVC coding makes the records of the MBs searched in
+    // full from the keys itself (one launch fewer)
+    int r = (keys ? bc250_learned_search_picture_keys : bc250_learned_search_picture)(
+        c8, cp, r8, rp, s->wc, s->hc, p.search_range, p.lambda8, q.params, q.packed, q.has_prev ? out16 : nullptr,
+        q.cfg.ncand, q.cfg.alpha_q4, q.cfg.beta, q.cfg.t_high, q.pool, q.cand, q.list, q.count, q.keys, out16, out8, st);
     q.ran = r == 0;
+    s->fs_keys = r == 0 && keys ? q.keys : nullptr;
     return r;
 }
 
@@ -901,6 +905,7 @@ static void start_frame(bc250_hip_state *s)
     s->prop.has_prev = s->prop.ran;
     s->prop.ran = false;
     s->fb_frame = nullptr;
+    s->fs_keys = nullptr;
 }
 
 // Ends a frame's stages: the wavefront error flags (e0 intra, e1 deblock;
@@ -1048,7 +1053,8 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
         if (r == 0 && s->ext16) {
             // the integer search ran elsewhere
         } else if (r == 0 && s->prop.on) {
-            r = enqueue_learned(s, c8, cp8, r8, rp8, s->imv16, s->imv8, st);
+            r = enqueue_learned(s, c8, cp8, r8, rp8, s->imv16, s->imv8, st,
+                                !is_intra && p.hevc_ctu_log2 == 4 && p.readback);
             s->fb_frame = s->prop.count;
         } else if (r == 0) {
             r = bc250_launch_me_fullsearch(c8, cp8, r8, rp8, s->wc, s->hc, p.search_range, p.lambda8, s->imv16,
@@ -1074,11 +1080,17 @@ static int enqueue_hevc(bc250_hip_state *s, Slot &sl, const Plane2 &cur, const P
             if (!c32 && p.readback) {
                 // CTU16: the coding kernel packs the levels for the readback itself
                 const DevOut &o = sl.dev;
-                r = bc250_hevc_launch_refine_p_encode_pack(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py,
-                                                           ref.puv, out.y, out.uv, out.py, out.puv, s->wc, s->hc, qp,
-                                                           2 /* inter rounding */, s->ext8 ? s->ext8 : s->imv8,
-                                                           s->mv8, (int16_t *)s->levels, s->hmodes, s->hcbf, bd,
-                                                           o.lvidx, (int16_t *)o.lvdata, o.lvcount, st);
+                if (s->fs_keys && !s->ext8)
+                    r = bc250_hevc_launch_refine_p_encode_pack_keys(
+                        cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv, out.y, out.uv, out.py, out.puv,
+                        s->wc, s->hc, qp, 2 /* inter rounding */, s->imv8, s->mv8, (int16_t *)s->levels, s->hmodes,
+                        s->hcbf, bd, o.lvidx, (int16_t *)o.lvdata, o.lvcount, s->fs_keys, s->imv16, p.search_range,
+                        p.lambda8, st);
+                else
+                    r = bc250_hevc_launch_refine_p_encode_pack(
+                        cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv, out.y, out.uv, out.py, out.puv,
+                        s->wc, s->hc, qp, 2 /* inter rounding */, s->ext8 ? s->ext8 : s->imv8, s->mv8,
+                        (int16_t *)s->levels, s->hmodes, s->hcbf, bd, o.lvidx, (int16_t *)o.lvdata, o.lvcount, st);
                 packed = r == 0 && HIP_OK(hipEventRecord(sl.cearly, st));
             } else {
                 r = bc250_hevc_launch_refine_p_encode(cur.y, cur.uv, cur.py, cur.puv, ref.y, ref.uv, ref.py, ref.puv,

@@ -9,6 +9,7 @@
 // saturation the reference applies after every pass.
 #include "bc250_hevc.h"
 #include "hevc_dev.h"
+#include "learned_dev.h"
 
 using namespace hevc;
 
@@ -410,6 +411,16 @@ struct PackOut {
     int *count;
 };
 
+// The learned search's full-search keys, 5 per MB (bc250_learned_search_
+// picture_keys): an MB whose 16x16 key is set was searched in full, and its
+// records are made here from the keys (its 8x8 one for the CU8, the 16x16
+// one to out16 for the next picture's proposer).  keys null: none.
+struct FsKeys {
+    const uint32_t *keys;
+    uint4 *out16;
+    int range, lambda8;
+};
+
 // Coding of CU8 wv of CTU ctu (origin (x0, y0)) with vector (mvx, mvy) by one
 // wave.  HAVE_PRED: the lane's luma prediction sample is `pred_in` (the
 // refinement's), so no luma window is staged.
@@ -544,7 +555,7 @@ __global__ __launch_bounds__(64, BC250_RPE_WAVES_PER_EU) void hevc_refine_p_enco
     const uint8_t *__restrict__ ruv, int rpy, int rpuv, uint8_t *__restrict__ oy, uint8_t *__restrict__ ouv, int opy,
     int opuv, int wc, int hc, int nctu, int qp, int round_q12, const uint4 *__restrict__ imv8,
     uint4 *__restrict__ mv8, int16_t *__restrict__ coef, uint8_t *__restrict__ cbf, uint8_t *__restrict__ modes,
-    const PackOut pk)
+    const PackOut pk, const FsKeys fk)
 {
     union CuLds {
         RefineLds8 r;
@@ -557,7 +568,15 @@ __global__ __launch_bounds__(64, BC250_RPE_WAVES_PER_EU) void hevc_refine_p_enco
     const int ctu = cu >> 2, wv = cu & 3, mbw = wc / 16;
     const int x0 = (ctu % mbw) * 16 + (wv & 1) * 8, y0 = (ctu / mbw) * 16 + (wv >> 1) * 8;
     int pred;
-    const uint4 v = refine_cu<Pix, BD>(Ls.r, cy, cpy, ry, rpy, wc, hc, x0, y0, imv8[cu], lane, &pred);
+    uint4 iv = imv8[cu];
+    if (fk.keys) {
+        const uint32_t k16 = fk.keys[5 * ctu], k8 = fk.keys[5 * ctu + 1 + wv];
+        if (k16 != 0xffffffffu) {
+            iv = fs_record(k8, 1 + wv, fk.range, fk.lambda8);
+            if (wv == 0 && lane == 0) fk.out16[ctu] = fs_record(k16, 0, fk.range, fk.lambda8);
+        }
+    }
+    const uint4 v = refine_cu<Pix, BD>(Ls.r, cy, cpy, ry, rpy, wc, hc, x0, y0, iv, lane, &pred);
     if (lane == 0) mv8[cu] = v;
     wave_lds_sync();  // the refinement's LDS is the coding's from here on
     encode_cu<Pix, BD, true>(Ls.p, cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp,
@@ -931,7 +950,8 @@ extern "C" int bc250_hevc_launch_p_encode_ctu(const void *cy, const void *cuv, i
 static int refine_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry, const void *ruv, int rpy,
                            int rpuv, void *oy, void *ouv, int opy, int opuv, int wc, int hc, int qp, int round_q12,
                            const uint32_t *imv8, uint32_t *mv8, int16_t *coef, int ctu_log2, uint8_t *culog2,
-                           uint8_t *modes, uint8_t *cbf, int bd, const PackOut &pk, void *stream)
+                           uint8_t *modes, uint8_t *cbf, int bd, const PackOut &pk, void *stream,
+                           const FsKeys &fk = FsKeys{nullptr, nullptr, 0, 0})
 {
     const int b = bd == 8 ? 1 : 2, mw = wc * b;
     if (!bd_ok(bd) || wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || qp < 0 || qp > 51 || round_q12 < 0 ||
@@ -946,11 +966,11 @@ static int refine_p_encode(const void *cy, const void *cuv, int cpy, int cpuv, c
     if (bd == 8)
         hevc_refine_p_encode_kernel<uint8_t, 8><<<grid_xcd(4 * nctu), 64, 0, st>>>(
             C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv, wc, hc, nctu, qp, round_q12, (const uint4 *)imv8,
-            (uint4 *)mv8, coef, cbf, modes, pk);
+            (uint4 *)mv8, coef, cbf, modes, pk, fk);
     else
         hevc_refine_p_encode_kernel<uint16_t, 10><<<grid_xcd(4 * nctu), 64, 0, st>>>(
             C, CU, cpy, cpuv, R, RU, rpy, rpuv, O, OU, opy, opuv, wc, hc, nctu, qp, round_q12, (const uint4 *)imv8,
-            (uint4 *)mv8, coef, cbf, modes, pk);
+            (uint4 *)mv8, coef, cbf, modes, pk, fk);
     if (launch_ok() != 0) return -1;
     const int ncu = nctu * 4;
     if (ctu_log2 == 4) {
@@ -982,4 +1002,18 @@ extern "C" int bc250_hevc_launch_refine_p_encode_pack(const void *cy, const void
     if (!pack_idx || !pack_data || !pack_count) return -1;
     return refine_p_encode(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12, imv8, mv8,
                            coef, 4, nullptr, modes, cbf, bd, PackOut{pack_idx, pack_data, pack_count}, stream);
+}
+
+extern "C" int bc250_hevc_launch_refine_p_encode_pack_keys(
+    const void *cy, const void *cuv, int cpy, int cpuv, const void *ry, const void *ruv, int rpy, int rpuv, void *oy,
+    void *ouv, int opy, int opuv, int wc, int hc, int qp, int round_q12, const uint32_t *imv8, uint32_t *mv8,
+    int16_t *coef, uint8_t *modes, uint8_t *cbf, int bd, uint32_t *pack_idx, int16_t *pack_data, int *pack_count,
+    const uint32_t *fs_keys, uint32_t *out16, int range, int lambda8, void *stream)
+{
+    if (!pack_idx || !pack_data || !pack_count || !fs_keys || !out16 || range < 1 || range > 64 || lambda8 < 0 ||
+        lambda8 > 15)
+        return -1;
+    return refine_p_encode(cy, cuv, cpy, cpuv, ry, ruv, rpy, rpuv, oy, ouv, opy, opuv, wc, hc, qp, round_q12, imv8, mv8,
+                           coef, 4, nullptr, modes, cbf, bd, PackOut{pack_idx, pack_data, pack_count}, stream,
+                           FsKeys{fs_keys, (uint4 *)out16, range, lambda8});
 }

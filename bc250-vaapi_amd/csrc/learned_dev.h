@@ -81,3 +81,15 @@ __device__ __forceinline__ void propose_tile(const float *spool, const float *__
         if (mx < mbw && my < mbh) cand_from_z(sz[t], range, my * mbw + mx, cand);
     }
 }
+
+// The search record of key k of the full search (q 0: 16x16, 1..4: the 8x8
+// quadrants) at range `range`: the key is cost * 2^15 + raster index.
+__device__ __forceinline__ uint4 fs_record(uint32_t k, int q, int range, int lambda8)
+{
+    const int nc = 2 * range + 1;
+    const uint32_t idx = k & 0x7fffu, cost = k >> 15;
+    const int dx = (int)(idx % nc) - range, dy = (int)(idx / nc) - range;
+    const uint32_t l1 = (uint32_t)(iabs(dx) + iabs(dy));
+    const uint32_t sad = cost - (q == 0 ? 4u : 1u) * (uint32_t)lambda8 * l1;
+    return make_uint4((uint32_t)dx, (uint32_t)dy, cost, sad);
+}

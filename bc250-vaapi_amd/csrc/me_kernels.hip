@@ -21,6 +21,7 @@
 #include "bc250_dev.h"
 #include "bc250_learned.h"
 #include "bc250_stages.h"
+#include "learned_dev.h"
 
 namespace {
 
@@ -441,7 +442,8 @@ __global__ __launch_bounds__(FsSplit<R>::NTH) void me_fullsearch_split_kernel(
     const int n = min(*count, nmb);
     const int i = (int)blockIdx.x / FsSplit<R>::NPART, part = (int)blockIdx.x % FsSplit<R>::NPART;
     if (i >= n) return;
-    fullsearch_part<R>(cur, cpitch, ref, rpitch, wc, hc, mbw, list[i], lambda8, part, win, red, keys + 5 * i);
+    const int mb = list[i];
+    fullsearch_part<R>(cur, cpitch, ref, rpitch, wc, hc, mbw, mb, lambda8, part, win, red, keys + 5 * mb);
 }
 
 template <int R>
@@ -458,25 +460,16 @@ __global__ __launch_bounds__(FsSplit<R>::NTH) void me_fullsearch_split_batch_ker
     if (g < 0 || g >= b.nmb_total) return;
     const bc250_batch_frame_t &fr = b.f[batch_frame_of_mb(b, g)];
     fullsearch_part<R>(fr.cur, fr.cpitch, fr.ref, fr.rpitch, fr.wc, fr.hc, fr.wc / 16, g - fr.mb_base, b.lambda8,
-                       part, win, red, keys + 5 * i);
+                       part, win, red, keys + 5 * g);
 }
 
-__global__ void fs_keys_init_kernel(const int *__restrict__ count, int cap, uint32_t *__restrict__ keys)
+__global__ void fs_keys_init_kernel(const int *__restrict__ list, const int *__restrict__ count, int cap,
+                                    uint32_t *__restrict__ keys)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 5 * min(*count, cap)) keys[i] = 0xffffffffu;
+    if (i < 5 * min(*count, cap)) keys[5 * list[i / 5] + i % 5] = 0xffffffffu;
 }
 
-// Record of key q (0: 16x16, 1..4: 8x8 quadrants) of a search of range R.
-__device__ __forceinline__ uint4 fs_record(uint32_t k, int q, int range, int lambda8)
-{
-    const int nc = 2 * range + 1;
-    const uint32_t idx = k & 0x7fffu, cost = k >> 15;
-    const int dx = (int)(idx % nc) - range, dy = (int)(idx / nc) - range;
-    const uint32_t l1 = (uint32_t)(iabs(dx) + iabs(dy));
-    const uint32_t sad = cost - (q == 0 ? 4u : 1u) * (uint32_t)lambda8 * l1;
-    return make_uint4((uint32_t)dx, (uint32_t)dy, cost, sad);
-}
 
 __global__ void fs_finish_kernel(const int *__restrict__ list, const int *__restrict__ count, int cap,
                                  const uint32_t *__restrict__ keys, int range, int lambda8, uint4 *__restrict__ out16,
@@ -485,7 +478,7 @@ __global__ void fs_finish_kernel(const int *__restrict__ list, const int *__rest
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 5 * min(*count, cap)) return;
     const int e = i / 5, q = i % 5, mb = list[e];
-    const uint4 o = fs_record(keys[i], q, range, lambda8);
+    const uint4 o = fs_record(keys[5 * mb + q], q, range, lambda8);
     if (q == 0) out16[mb] = o;
     else out8[4 * mb + q - 1] = o;
 }
@@ -499,7 +492,7 @@ __global__ void fs_finish_batch_kernel(const bc250_me_batch_t b, const int *__re
     if (g < 0 || g >= b.nmb_total) return;
     const bc250_batch_frame_t &fr = b.f[batch_frame_of_mb(b, g)];
     const int mb = g - fr.mb_base;
-    const uint4 o = fs_record(keys[i], q, b.range, b.lambda8);
+    const uint4 o = fs_record(keys[5 * g + q], q, b.range, b.lambda8);
     if (q == 0) ((uint4 *)fr.out16)[mb] = o;
     else ((uint4 *)fr.out8)[4 * mb + q - 1] = o;
 }
@@ -693,7 +686,9 @@ extern "C" int bc250_launch_me_fullsearch_list_split(const uint8_t *cur, int cpi
         return -1;
     const int mbw = wc / 16, nmb = mbw * (hc / 16);
     const int kb = (5 * nmb + 255) / 256;
-    if (!keys_ready) fs_keys_init_kernel<<<kb, 256, 0, st>>>(count, nmb, keys);
+    // keys_ready bit 0: the list's keys are set already; bit 1: no records
+    // (the caller turns the keys into records itself)
+    if (!(keys_ready & 1)) fs_keys_init_kernel<<<kb, 256, 0, st>>>(list, count, nmb, keys);
     switch (range) {
 #define BC250_SPLIT(RR)                                                                                               \
     case RR:                                                                                                          \
@@ -707,7 +702,8 @@ extern "C" int bc250_launch_me_fullsearch_list_split(const uint8_t *cur, int cpi
     default:
         return -1;
     }
-    fs_finish_kernel<<<kb, 256, 0, st>>>(list, count, nmb, keys, range, lambda8, (uint4 *)out16, (uint4 *)out8);
+    if (!(keys_ready & 2))
+        fs_finish_kernel<<<kb, 256, 0, st>>>(list, count, nmb, keys, range, lambda8, (uint4 *)out16, (uint4 *)out8);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -719,7 +715,7 @@ extern "C" int bc250_launch_me_fullsearch_batch_split(const bc250_me_batch_t *b,
         return -1;
     hipStream_t st = (hipStream_t)stream;
     const int kb = (5 * b->nmb_total + 255) / 256;
-    if (!keys_ready) fs_keys_init_kernel<<<kb, 256, 0, st>>>(count, b->nmb_total, keys);
+    if (!keys_ready) fs_keys_init_kernel<<<kb, 256, 0, st>>>(list, count, b->nmb_total, keys);
     switch (b->range) {
 #define BC250_SPLIT(RR)                                                                                               \
     case RR:                                                                                                          \

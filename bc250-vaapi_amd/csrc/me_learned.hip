@@ -199,7 +199,8 @@ __global__ __launch_bounds__(64) void cand_search_kernel(const uint8_t *__restri
                                                          int mbw, int nmb, int range, int lambda8,
                                                          const int2 *__restrict__ cand, int ncand,
                                                          uint4 *__restrict__ out16, uint4 *__restrict__ out8,
-                                                         int *__restrict__ clear_count)
+                                                         int *__restrict__ clear_count,
+                                                         uint32_t *__restrict__ clear_keys)
 {
     __shared__ uint32_t win[4 * WS * WP];
     const int mb = xcd_item(blockIdx.x, nmb);
@@ -207,6 +208,9 @@ __global__ __launch_bounds__(64) void cand_search_kernel(const uint8_t *__restri
     // the fallback list of this picture starts empty (its previous use
     // ended before this launch; its next starts after it)
     if (clear_count && mb == 0 && threadIdx.x == 0) *clear_count = 0;
+    // clear_keys: every MB's split-search keys start empty, so that a set
+    // key after the search marks an MB searched in full
+    if (clear_keys && threadIdx.x < 5) clear_keys[5 * mb + threadIdx.x] = 0xffffffffu;
     cand_search_mb(cur, cpitch, ref, rpitch, wc, hc, mbw, mb, range, lambda8, cand, ncand, out16, out8, win);
 }
 
@@ -234,8 +238,8 @@ __device__ __forceinline__ bool fallback_mb(const uint4 *__restrict__ r16, int m
     return sad > bound || sad > (uint32_t)t_high;
 }
 
-// keys (optional): the split full search's keys of each listed MB, set to
-// the empty key as the MB is listed.
+// keys (optional): the split full search's keys, 5 per MB of the picture
+// (batch), those of a listed MB set to the empty key as it is listed.
 __device__ __forceinline__ void list_mb(int *__restrict__ list, int *__restrict__ count, uint32_t *__restrict__ keys,
                                         int m)
 {
@@ -243,7 +247,7 @@ __device__ __forceinline__ void list_mb(int *__restrict__ list, int *__restrict_
     list[i] = m;
     if (keys) {
 #pragma unroll
-        for (int q = 0; q < 5; q++) keys[5 * i + q] = 0xffffffffu;
+        for (int q = 0; q < 5; q++) keys[5 * m + q] = 0xffffffffu;
     }
 }
 
@@ -366,7 +370,7 @@ extern "C" int bc250_launch_cand_search(const uint8_t *cur, int cpitch, const ui
     const int mbw = wc / 16, nmb = mbw * (hc / 16);
     cand_search_kernel<<<((nmb + 7) / 8) * 8, 64, 0, (hipStream_t)stream>>>(
         cur, cpitch, ref, rpitch, wc, hc, mbw, nmb, range, lambda8, (const int2 *)cand, ncand, (uint4 *)out16,
-        (uint4 *)out8, nullptr);
+        (uint4 *)out8, nullptr, nullptr);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -392,11 +396,10 @@ extern "C" int bc250_cnn_propose(const float *pool, const float *params, const u
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-extern "C" int bc250_learned_search_picture(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
-                                            int hc, int range, int lambda8, const float *params,
-                                            const uint16_t *packed, const uint32_t *prev16, int ncand, int alpha_q4,
-                                            int beta, int t_high, float *pool, int *cand, int *list, int *count,
-                                            uint32_t *keys, uint32_t *out16, uint32_t *out8, void *stream)
+static int learned_search(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc, int range,
+                          int lambda8, const float *params, const uint16_t *packed, const uint32_t *prev16, int ncand,
+                          int alpha_q4, int beta, int t_high, float *pool, int *cand, int *list, int *count,
+                          uint32_t *keys, uint32_t *out16, uint32_t *out8, void *stream, bool records)
 {
     if (wc <= 0 || hc <= 0 || (wc & 15) || (hc & 15) || (cpitch & 3) || lambda8 < 0 || lambda8 > 15 ||
         ncand < 1 || ncand > 4 || range < CW || range > 64 || !params || !packed || !pool || !cand || !list ||
@@ -409,15 +412,36 @@ extern "C" int bc250_learned_search_picture(const uint8_t *cur, int cpitch, cons
     if (r == 0) {
         cand_search_kernel<<<((nmb + 7) / 8) * 8, 64, 0, st>>>(cur, cpitch, ref, rpitch, wc, hc, mbw, nmb, range,
                                                                lambda8, (const int2 *)cand, ncand, (uint4 *)out16,
-                                                               (uint4 *)out8, count);
+                                                               (uint4 *)out8, count, records ? nullptr : keys);
         fallback_select_kernel<<<(nmb + 127) / 128, 128, 0, st>>>((const uint4 *)out16, mbw, mbh, alpha_q4, beta,
                                                                   t_high, list, count, keys);
         r = hipGetLastError() == hipSuccess ? 0 : -1;
     }
     if (r == 0)
         r = bc250_launch_me_fullsearch_list_split(cur, cpitch, ref, rpitch, wc, hc, range, lambda8, out16, out8, list,
-                                                  count, keys, 1, st);
+                                                  count, keys, records ? 1 : 3, st);
     return r;
+}
+
+extern "C" int bc250_learned_search_picture(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc,
+                                            int hc, int range, int lambda8, const float *params,
+                                            const uint16_t *packed, const uint32_t *prev16, int ncand, int alpha_q4,
+                                            int beta, int t_high, float *pool, int *cand, int *list, int *count,
+                                            uint32_t *keys, uint32_t *out16, uint32_t *out8, void *stream)
+{
+    return learned_search(cur, cpitch, ref, rpitch, wc, hc, range, lambda8, params, packed, prev16, ncand, alpha_q4,
+                          beta, t_high, pool, cand, list, count, keys, out16, out8, stream, true);
+}
+
+extern "C" int bc250_learned_search_picture_keys(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch,
+                                                 int wc, int hc, int range, int lambda8, const float *params,
+                                                 const uint16_t *packed, const uint32_t *prev16, int ncand,
+                                                 int alpha_q4, int beta, int t_high, float *pool, int *cand,
+                                                 int *list, int *count, uint32_t *keys, uint32_t *out16,
+                                                 uint32_t *out8, void *stream)
+{
+    return learned_search(cur, cpitch, ref, rpitch, wc, hc, range, lambda8, params, packed, prev16, ncand, alpha_q4,
+                          beta, t_high, pool, cand, list, count, keys, out16, out8, stream, false);
 }
 
 // ---------------------------------------------------------------------------

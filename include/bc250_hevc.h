@@ -149,6 +149,17 @@ int bc250_hevc_launch_refine_p_encode_pack(const void *cy, const void *cuv, int 
                                            int wc, int hc, int qp, int round_q12, const uint32_t *imv8, uint32_t *mv8,
                                            int16_t *coef, uint8_t *modes, uint8_t *cbf, int bd, uint32_t *pack_idx,
                                            int16_t *pack_data, int *pack_count, void *stream);
+/* The same with the learned search's full-search keys (bc250_learned_search_
+ * picture_keys: 5 per MB, the 16x16 key set only for MBs searched in full):
+ * such an MB's 8x8 records come from its keys instead of imv8, and its 16x16
+ * record is written to out16 (the next picture's proposer input). */
+int bc250_hevc_launch_refine_p_encode_pack_keys(const void *cy, const void *cuv, int cpy, int cpuv, const void *ry,
+                                                const void *ruv, int rpy, int rpuv, void *oy, void *ouv, int opy,
+                                                int opuv, int wc, int hc, int qp, int round_q12, const uint32_t *imv8,
+                                                uint32_t *mv8, int16_t *coef, uint8_t *modes, uint8_t *cbf, int bd,
+                                                uint32_t *pack_idx, int16_t *pack_data, int *pack_count,
+                                                const uint32_t *fs_keys, uint32_t *out16, int range, int lambda8,
+                                                void *stream);
 
 /* I picture: CTU16 -> 4 CU8 -> 4 PU4 intra (Planar/DC/H/V by SAD, DST),
  * chroma DC per CU (DCT, QpC), slices as CTU-row ranges

@@ -172,6 +172,15 @@ int bc250_learned_search_picture(const uint8_t *cur, int cpitch, const uint8_t *
                                  const uint32_t *prev16, int ncand, int alpha_q4, int beta, int t_high, float *pool,
                                  int *cand, int *list, int *count, uint32_t *keys, uint32_t *out16, uint32_t *out8,
                                  void *stream);
+/* The same without the records of the MBs searched in full: every MB's keys
+ * (5 per MB) start empty and those MBs' keys hold their search (out16 /
+ * out8 keep the candidate search's records for them); the HEVC coding makes
+ * the records from the keys (bc250_hevc_launch_refine_p_encode_pack_keys). */
+int bc250_learned_search_picture_keys(const uint8_t *cur, int cpitch, const uint8_t *ref, int rpitch, int wc, int hc,
+                                      int range, int lambda8, const float *params, const uint16_t *packed,
+                                      const uint32_t *prev16, int ncand, int alpha_q4, int beta, int t_high,
+                                      float *pool, int *cand, int *list, int *count, uint32_t *keys, uint32_t *out16,
+                                      uint32_t *out8, void *stream);
 
 /* ---- batches of pictures ---------------------------------------------------
  * One launch per stage for the integer searches of several pictures (of
