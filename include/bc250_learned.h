@@ -88,8 +88,8 @@ int bc250_cnn_forward(const uint16_t *in, const uint16_t *packed, const float *p
 
 /* The fused launch's shape: 0 four waves, 4-row tasks, plain NHWC in LDS;
  * 1 the same with channel chunks XOR-swizzled by pixel for 32-bank reads;
- * 2 eight waves, 2-row tasks, swizzled for 64 banks.  Process-wide; every
- * variant gives the same bits. */
+ * 2 eight waves, 2-row tasks, swizzled for 64 banks.  Process-wide (0 until
+ * set); every variant gives the same bits. */
 int bc250_cnn_set_forward_variant(int variant);
 
 /* Weight and bias gradient of `layer` from its input activation a and its

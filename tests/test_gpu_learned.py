@@ -133,7 +133,7 @@ def test_fused_forward_equals_the_layers(pkg, learned):
             torch.cuda.synchronize()
             assert torch.equal(net.pool[:7], want_pool), f"block sums differ (trained={trained}, variant {variant})"
             assert torch.equal(got_z, want_z), f"head outputs differ (trained={trained}, variant {variant})"
-        assert pkg.hip.bc250_cnn_set_forward_variant(1) == 0
+        assert pkg.hip.bc250_cnn_set_forward_variant(0) == 0
         assert pkg.hip.bc250_cnn_set_forward_variant(3) == -1
         assert (want_pool > 0).float().mean() > 0.2
 
@@ -387,7 +387,7 @@ def test_picture_chain_equals_the_stages(pkg, learned):
                                                      P(pool_b), None) == 0
             torch.cuda.synchronize()
             assert torch.equal(pool_b, net.pool), (call, variant)
-        assert pkg.hip.bc250_cnn_set_forward_variant(1) == 0
+        assert pkg.hip.bc250_cnn_set_forward_variant(0) == 0
         # the proposer: mvin -> head -> candidates against one launch
         assert pkg.hip.bc250_cnn_mvin(P(prev), mbw, mbh, P(mvin), None) == 0
         assert pkg.hip.bc250_cnn_head(wh, P(net.pool), P(mvin), P(z), nt, None) == 0
